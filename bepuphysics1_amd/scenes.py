@@ -1,0 +1,166 @@
+"""Seeded synthetic scenes for the step path.
+
+A scene is a numpy structured array, one row per body, in add order:
+
+    shape     int32    BEPU_SHAPE_BOX (0) / SPHERE (1) / CONVEX_HULL (2) / CAPSULE (3)
+    dims      f32[3]   box (width, height, length); sphere (radius, 0, 0)
+    mass      f32      <= 0 makes a kinematic body (Entity(shape, mass), EntityBase.cs:632-645)
+    state     f32[13]  MotionState: position xyz, orientation xyzw, linear v, angular v
+    material  f32[3]   static friction, kinetic friction, bounciness
+    flags     uint32   BEPU_FLAG_*
+
+The same array feeds the GPU library (through ``Space``) and the oracle, so
+both run on identical inputs.  Scenes follow the reference's demo scenes
+(BEPUphysicsDemos/Demos/PyramidDemo.cs:17-37, BoxBoxTestDemo.cs:18-51) at the
+sizes of the benchmark workloads.
+"""
+import numpy as np
+
+SHAPE_BOX = 0
+SHAPE_SPHERE = 1
+SHAPE_CONVEX_HULL = 2
+SHAPE_CAPSULE = 3
+FLAG_ALWAYS_ACTIVE = 1
+FLAG_NO_STABILIZATION = 2
+
+BODY_DTYPE = np.dtype([
+    ("shape", np.int32),
+    ("dims", np.float32, 3),
+    ("mass", np.float32),
+    ("state", np.float32, 13),
+    ("material", np.float32, 3),
+    ("flags", np.uint32),
+])
+
+IDENTITY = (0.0, 0.0, 0.0, 1.0)
+
+
+def _row(shape, dims, mass, pos, quat=IDENTITY, vel=(0, 0, 0), angvel=(0, 0, 0),
+         material=(1.0, 0.8, 0.0), flags=FLAG_ALWAYS_ACTIVE):
+    r = np.zeros((), BODY_DTYPE)
+    r["shape"] = shape
+    r["dims"] = dims
+    r["mass"] = mass
+    r["state"] = list(pos) + list(quat) + list(vel) + list(angvel)
+    r["material"] = material
+    r["flags"] = flags
+    return r
+
+
+def make(rows):
+    out = np.zeros(len(rows), BODY_DTYPE)
+    for i, r in enumerate(rows):
+        out[i] = r
+    return out
+
+
+def ground(width=50.0, height=1.0, length=50.0, top=0.0, center=(0.0, 0.0), material=(1.0, 1.0, 0.0)):
+    """Static (kinematic) ground box whose top face sits at y = top."""
+    return _row(SHAPE_BOX, (width, height, length), 0.0, (center[0], top - height / 2, center[1]),
+                material=material, flags=0)
+
+
+def pyramid(rows=10, gap=0.01, friction=0.5):
+    """Pyramid of unit boxes, mass 1, on a 50x1x50 static ground.
+
+    Row r holds rows - r boxes; interaction friction 0.5 (static and kinetic)
+    and bounciness 0 come from box material (0.5, 0.5, 0) times ground material
+    (1, 1, 0) under the product blend (MaterialManager.cs:77-85).
+    """
+    out = [ground()]
+    for r in range(rows):
+        n = rows - r
+        for j in range(n):
+            x = (j - (n - 1) / 2.0) * (1.0 + gap)
+            out.append(_row(SHAPE_BOX, (1.0, 1.0, 1.0), 1.0, (x, 0.5 + r, 0.0),
+                            material=(friction, friction, 0.0)))
+    return make(out)
+
+
+def towers(nx=64, nz=64, height=20, spacing=2.0, jitter=0.01, half_extent=0.5, seed=0):
+    """nx x nz grid of box towers (height boxes each) on one static ground.
+
+    Every box has half extent 0.5, mass 1 and a seeded horizontal jitter in
+    [-jitter, jitter].  At spacing 2.0 no two towers touch, so every tower
+    is an independent 20-deep contact chain.
+    """
+    rng = np.random.default_rng(seed)
+    size = max(nx, nz) * spacing + 10.0
+    n = nx * nz * height
+    out = np.zeros(n + 1, BODY_DTYPE)
+    out[0] = ground(size, 1.0, size)
+    ix, iz, lv = np.meshgrid(np.arange(nx), np.arange(nz), np.arange(height), indexing="ij")
+    ix, iz, lv = ix.ravel(), iz.ravel(), lv.ravel()
+    jx = rng.uniform(-jitter, jitter, n).astype(np.float32)
+    jz = rng.uniform(-jitter, jitter, n).astype(np.float32)
+    b = out[1:]
+    b["shape"] = SHAPE_BOX
+    b["dims"] = 2 * half_extent
+    b["mass"] = 1.0
+    st = np.zeros((n, 13), np.float32)
+    st[:, 0] = (ix - (nx - 1) / 2.0) * spacing + jx
+    st[:, 1] = half_extent + lv * (2 * half_extent)
+    st[:, 2] = (iz - (nz - 1) / 2.0) * spacing + jz
+    st[:, 6] = 1.0
+    b["state"] = st
+    b["material"] = (1.0, 0.8, 0.0)
+    b["flags"] = FLAG_ALWAYS_ACTIVE
+    return out
+
+
+def random_quaternions(rng, n):
+    """Uniform random unit quaternions (Shoemake's method), xyzw."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1 - u1), np.sqrt(u1)
+    q = np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                  b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)], axis=1)
+    return q.astype(np.float32)
+
+
+def random_pile(n, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE), drop=(1.0, 10.0), with_ground=True):
+    """A random pile of n bodies dropped onto a ground, mixed shapes.
+
+    Boxes have half extents in [0.25, 1], spheres radius in [0.25, 1]; bodies
+    are spread over a footprint that grows with sqrt(n) and dropped from
+    heights uniform in `drop`.
+    """
+    rng = np.random.default_rng(seed)
+    foot = 2.0 * np.sqrt(max(n, 1)) + 2.0
+    out = np.zeros(n + (1 if with_ground else 0), BODY_DTYPE)
+    k = 0
+    if with_ground:
+        out[0] = ground(foot + 20.0, 1.0, foot + 20.0)
+        k = 1
+    kinds = rng.choice(np.array(shapes), size=n)
+    q = random_quaternions(rng, n)
+    for i in range(n):
+        s = int(kinds[i])
+        if s == SHAPE_BOX:
+            dims = 2 * rng.uniform(0.25, 1.0, 3)
+        else:
+            dims = (rng.uniform(0.25, 1.0), 0.0, 0.0)
+        pos = (rng.uniform(-foot / 2, foot / 2), rng.uniform(*drop), rng.uniform(-foot / 2, foot / 2))
+        out[k + i] = _row(s, dims, 1.0, pos, quat=q[i])
+    return out
+
+
+def stress(n=32768, edge=None, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE)):
+    """Bodies placed uniformly at random in a cube of side `edge` with random
+    orientations, no ground.  Box half extents and sphere radii in [0.25, 1].
+    With the default edge the mean AABB degree is about 4 for this mix."""
+    rng = np.random.default_rng(seed)
+    if edge is None:
+        edge = 100.0 * (n / 32768.0) ** (1.0 / 3.0) * 0.74
+    out = np.zeros(n, BODY_DTYPE)
+    kinds = np.array(shapes)[np.arange(n) % len(shapes)]
+    rng.shuffle(kinds)
+    q = random_quaternions(rng, n)
+    pos = rng.uniform(0.0, edge, (n, 3)).astype(np.float32)
+    for i in range(n):
+        s = int(kinds[i])
+        if s == SHAPE_BOX:
+            dims = 2 * rng.uniform(0.25, 1.0, 3)
+        else:
+            dims = (rng.uniform(0.25, 1.0), 0.0, 0.0)
+        out[i] = _row(s, dims, 1.0, pos[i], quat=q[i])
+    return out

@@ -1,0 +1,214 @@
+/*
+ * bepu_gpu.h -- C ABI of the MI355X (gfx950) rigid-body step library.
+ *
+ * The library replaces the contiguous block of BEPUphysics v1 stages that
+ * Space.DoTimeStep runs between the deactivation candidacy update and the
+ * position update (BEPUphysics/Space.cs:496-521):
+ *
+ *   ForceUpdater -> BoundingBoxUpdater -> BroadPhase -> NarrowPhase
+ *   -> Solver -> PositionUpdater
+ *
+ * for convex entities (Box, Sphere, ConvexHull, Capsule) with the default
+ * collision rules and material blending.  All state of a world (bodies,
+ * pair cache, manifolds, accumulated impulses, friction memory) lives on the
+ * device between calls; host buffers are only touched by the add/get calls.
+ *
+ * Conventions
+ *  - Status codes replace the reference's exceptions (see bepu_status).
+ *  - Buffers passed in are caller-owned host memory.
+ *  - A world handle is not thread safe; like Space.Update it is driven by a
+ *    single caller (BEPUphysics/Space.cs:527-531).
+ *  - Body ids are dense uint32 in the order of the add calls.
+ *  - Pair keys are (min(idA,idB) << 32) | max(idA,idB).
+ *  - Quaternions are stored x, y, z, w (BEPUutilities/Quaternion.cs).
+ */
+#ifndef BEPU_GPU_H
+#define BEPU_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEPU_ABI_VERSION 1
+
+/* Maps 1:1 onto the exceptions the reference throws on the same misuse:
+ * ArgumentException on double add / foreign remove (Space.cs:243-244,366-367),
+ * InvalidOperationException for a bad mass (Entities/EntityBase.cs:879-880). */
+typedef enum bepu_status {
+    BEPU_OK = 0,
+    BEPU_E_INVALID_ARG = 1,
+    BEPU_E_ALREADY_ADDED = 2,
+    BEPU_E_NOT_FOUND = 3,
+    BEPU_E_OUT_OF_MEMORY = 4,
+    BEPU_E_DEVICE = 5,
+    BEPU_E_NONFINITE = 6,
+    BEPU_E_CAPACITY = 7,
+    BEPU_E_UNSUPPORTED = 8
+} bepu_status;
+
+/* One POD block per world (or per batch of worlds).  Every field defaults to
+ * the reference's static default; bepu_settings_default fills them in.
+ * Sources: Settings/CollisionDetectionSettings.cs:11-87,
+ * Settings/CollisionResponseSettings.cs:12-43, Constraints/SolverSettings.cs:62,70,
+ * Constraints/Solver.cs:15, DeactivationManagement/DeactivationManager.cs:18-20,60,
+ * Entities/EntityBase.cs:1164-1165, Materials/MaterialManager.cs:29-39,
+ * BEPUutilities/Toolbox.cs:20-25. */
+typedef struct bepu_settings {
+    float gravity[3];                  /* ForceUpdater.Gravity; demos use (0,-9.81,0) */
+    float dt;                          /* TimeStepSettings.TimeStepDuration (1/60) */
+    int32_t iteration_limit;           /* Solver.IterationLimit (10) */
+    int32_t min_iteration_count;       /* SolverSettings.DefaultMinimumIterationCount (1) */
+    float min_impulse;                 /* SolverSettings.DefaultMinimumImpulse (0.001) */
+    float allowed_penetration;         /* CollisionDetectionSettings.AllowedPenetration (0.01) */
+    float default_margin;              /* CollisionDetectionSettings.DefaultMargin (0.04) */
+    float max_contact_distance;        /* CollisionDetectionSettings.maximumContactDistance (0.1) */
+    float contact_invalidation_length_sq;   /* (0.01) */
+    float contact_min_separation_sq;        /* (0.0009) */
+    float bounce_velocity_threshold;   /* CollisionResponseSettings.BouncinessVelocityThreshold (1) */
+    float max_penetration_recovery_speed;   /* (2) */
+    float penetration_recovery_stiffness;   /* (0.2) */
+    float static_friction_velocity_threshold; /* (0.2) */
+    float twist_friction_factor;       /* (1) */
+    float softness;                    /* (0.001) */
+    int32_t use_stabilization;         /* DeactivationManager.UseStabilization (1) */
+    float velocity_lower_limit;        /* DeactivationManager.VelocityLowerLimit (0.26) */
+    float low_velocity_time_minimum;   /* DeactivationManager.LowVelocityTimeMinimum (1) */
+    float linear_damping;              /* Entity default LinearDamping (0.03) */
+    float angular_damping;             /* Entity default AngularDamping (0.15) */
+} bepu_settings;
+
+/* BEPUphysics/EntityStateManagement/MotionState.cs:10-55 */
+typedef struct bepu_motion_state {
+    float position[3];
+    float orientation[4];  /* x, y, z, w */
+    float linear_velocity[3];
+    float angular_velocity[3];
+} bepu_motion_state;
+
+/* BEPUphysics/Materials/Material.cs:10,29,49; pair values are the product
+ * of the two bodies' values (MaterialManager.DefaultMaterialBlender). */
+typedef struct bepu_material {
+    float static_friction;   /* default 1.0 */
+    float kinetic_friction;  /* default 0.8 */
+    float bounciness;        /* default 0.0 */
+} bepu_material;
+
+/* Per-body flags. */
+#define BEPU_FLAG_ALWAYS_ACTIVE 1u      /* SimulationIslandMember.IsAlwaysActive */
+#define BEPU_FLAG_NO_STABILIZATION 2u   /* SimulationIslandMember.AllowStabilization = false */
+
+/* Shape kinds. */
+#define BEPU_SHAPE_BOX 0
+#define BEPU_SHAPE_SPHERE 1
+#define BEPU_SHAPE_CONVEX_HULL 2
+#define BEPU_SHAPE_CAPSULE 3
+
+/* One contact of a manifold, in the manifold's contact-list order. */
+typedef struct bepu_contact {
+    float position[3];
+    float normal[3];
+    float penetration_depth;
+    int32_t id;
+} bepu_contact;
+
+/* One pair's manifold (only pairs with at least one contact are reported). */
+typedef struct bepu_manifold {
+    uint64_t key;
+    uint32_t body_a, body_b;       /* A/B as the pair handler orders them */
+    int32_t count;
+    int32_t kind;                  /* BEPU_PAIR_* */
+    bepu_contact contacts[4];
+} bepu_manifold;
+
+#define BEPU_PAIR_BOX_BOX 0
+#define BEPU_PAIR_BOX_SPHERE 1
+#define BEPU_PAIR_SPHERE_SPHERE 2
+#define BEPU_PAIR_CONVEX 3
+
+/* Accumulated impulses of one contact constraint after a step, listed in
+ * solve order (colour, then key).  `normal[i]` is the i-th penetration
+ * constraint (ContactPenetrationConstraint.accumulatedImpulse);
+ * convex pairs fill `sliding` (SlidingFrictionTwoAxis) and `twist`
+ * (TwistFrictionConstraint); non-convex pairs (sphere-sphere, box-sphere)
+ * fill `friction[i]` (ContactFrictionConstraint). */
+typedef struct bepu_impulses {
+    uint64_t key;
+    int32_t color;
+    int32_t count;
+    float normal[4];
+    float friction[4];
+    float sliding[2];
+    float twist;
+    float pad;
+} bepu_impulses;
+
+typedef struct bepu_step_stats {
+    uint32_t bodies;
+    uint32_t pairs;           /* broadphase overlaps */
+    uint32_t manifolds;       /* pairs with >= 1 contact (solver constraints) */
+    uint32_t contacts;
+    uint32_t colors;
+    uint32_t steps;           /* steps run since creation */
+    float max_penetration;    /* deepest contact of the last step */
+    float pad;
+} bepu_step_stats;
+
+typedef struct bepu_world_s* bepu_world;
+
+/* Fills every field with the reference defaults listed above
+ * (BEPUphysicsDemos/ConfigurationHelper.cs:39-68 documents them). */
+void bepu_settings_default(bepu_settings* out);
+
+/* Space() constructor analogue (Space.cs:199-233).  `capacity_bodies` sizes
+ * the device arrays; adding more bodies than that returns BEPU_E_CAPACITY. */
+bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_bodies,
+                              bepu_world* out);
+bepu_status bepu_world_destroy(bepu_world world);
+
+/* Space.Add(new Box(motionState, width, height, length, mass)) -- Space.cs:241-358,
+ * Entities/Prefabs/Box.cs:60-64.  mass <= 0 makes a kinematic body (BecomeKinematic),
+ * a NaN/inf mass returns BEPU_E_INVALID_ARG (EntityBase.cs:879-880). */
+bepu_status bepu_add_box(bepu_world world, const bepu_motion_state* state, float width,
+                         float height, float length, float mass,
+                         const bepu_material* material, uint32_t flags, uint32_t* id_out);
+/* Space.Add(new Sphere(motionState, radius, mass)) -- Entities/Prefabs/Sphere.cs:70-74. */
+bepu_status bepu_add_sphere(bepu_world world, const bepu_motion_state* state, float radius,
+                            float mass, const bepu_material* material, uint32_t flags,
+                            uint32_t* id_out);
+
+/* Space.Update() x n_steps (Space.cs:527-531). */
+bepu_status bepu_step(bepu_world world, int32_t n_steps);
+
+/* Entity.MotionState for every body, in id order. */
+bepu_status bepu_get_motion_states(bepu_world world, bepu_motion_state* out, uint32_t capacity,
+                                   uint32_t* count_out);
+/* BroadPhase.Overlaps of the last step as a sorted set of pair keys
+ * (BroadPhaseSystems/BroadPhase.cs:34-41). */
+bepu_status bepu_get_pairs(bepu_world world, uint64_t* keys, uint32_t capacity,
+                           uint32_t* count_out);
+/* NarrowPhase pair manifolds with >= 1 contact, sorted by key. */
+bepu_status bepu_get_manifolds(bepu_world world, bepu_manifold* out, uint32_t capacity,
+                               uint32_t* count_out);
+/* Solver constraints of the last step in solve order. */
+bepu_status bepu_get_impulses(bepu_world world, bepu_impulses* out, uint32_t capacity,
+                              uint32_t* count_out);
+bepu_status bepu_get_stats(bepu_world world, bepu_step_stats* out);
+
+/* Per-stage device time of the last bepu_step call, in milliseconds, measured
+ * with HIP events on the world's stream.  `names` receives a pointer to a
+ * static, comma separated list of stage names matching `ms`. */
+bepu_status bepu_get_stage_times(bepu_world world, float* ms, uint32_t capacity,
+                                 uint32_t* count_out, const char** names);
+
+/* Library-wide information: 1 if a HIP device is usable. */
+int32_t bepu_device_available(void);
+const char* bepu_status_string(bepu_status status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEPU_GPU_H */

@@ -1,0 +1,261 @@
+"""ctypes binding of the oracle library (liboracle_refcpu.so).
+
+TEST INFRASTRUCTURE ONLY: this module is the checker.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import it; the
+product package (bepuphysics1_amd) never does.
+
+The oracle restates BEPUphysics v1's per-step path on the CPU, single
+threaded (BEPUphysics/Space.cs:496-521 and the files cited in the C++
+sources).  Its API mirrors include/bepu_gpu.h with a ``refcpu_`` prefix.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_refcpu.so")
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [
+        ("gravity", ctypes.c_float * 3),
+        ("dt", ctypes.c_float),
+        ("iteration_limit", ctypes.c_int32),
+        ("min_iteration_count", ctypes.c_int32),
+        ("min_impulse", ctypes.c_float),
+        ("allowed_penetration", ctypes.c_float),
+        ("default_margin", ctypes.c_float),
+        ("max_contact_distance", ctypes.c_float),
+        ("contact_invalidation_length_sq", ctypes.c_float),
+        ("contact_min_separation_sq", ctypes.c_float),
+        ("bounce_velocity_threshold", ctypes.c_float),
+        ("max_penetration_recovery_speed", ctypes.c_float),
+        ("penetration_recovery_stiffness", ctypes.c_float),
+        ("static_friction_velocity_threshold", ctypes.c_float),
+        ("twist_friction_factor", ctypes.c_float),
+        ("softness", ctypes.c_float),
+        ("use_stabilization", ctypes.c_int32),
+        ("velocity_lower_limit", ctypes.c_float),
+        ("low_velocity_time_minimum", ctypes.c_float),
+        ("linear_damping", ctypes.c_float),
+        ("angular_damping", ctypes.c_float),
+    ]
+
+
+class MotionState(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_float * 3),
+        ("orientation", ctypes.c_float * 4),
+        ("linear_velocity", ctypes.c_float * 3),
+        ("angular_velocity", ctypes.c_float * 3),
+    ]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("static_friction", ctypes.c_float), ("kinetic_friction", ctypes.c_float),
+                ("bounciness", ctypes.c_float)]
+
+
+class Contact(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("normal", ctypes.c_float * 3),
+                ("penetration_depth", ctypes.c_float), ("id", ctypes.c_int32)]
+
+
+class Manifold(ctypes.Structure):
+    _fields_ = [("key", ctypes.c_uint64), ("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
+                ("count", ctypes.c_int32), ("kind", ctypes.c_int32), ("contacts", Contact * 4)]
+
+
+class Impulses(ctypes.Structure):
+    _fields_ = [("key", ctypes.c_uint64), ("color", ctypes.c_int32), ("count", ctypes.c_int32),
+                ("normal", ctypes.c_float * 4), ("friction", ctypes.c_float * 4),
+                ("sliding", ctypes.c_float * 2), ("twist", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("bodies", ctypes.c_uint32), ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32),
+                ("contacts", ctypes.c_uint32), ("colors", ctypes.c_uint32), ("steps", ctypes.c_uint32),
+                ("max_penetration", ctypes.c_float), ("pad", ctypes.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run `make -C oracle` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.refcpu_create.restype = vp
+        L.refcpu_create.argtypes = [ctypes.POINTER(Settings)]
+        L.refcpu_destroy.argtypes = [vp]
+        L.refcpu_set_broadphase.argtypes = [vp, ctypes.c_int]
+        L.refcpu_settings_default.argtypes = [ctypes.POINTER(Settings)]
+        L.refcpu_add_box.argtypes = [vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.POINTER(Material),
+                                     ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_add_sphere.argtypes = [vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
+                                        ctypes.POINTER(Material), ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_step.argtypes = [vp, ctypes.c_int]
+        L.refcpu_broadphase_only.argtypes = [vp]
+        for name, typ in (("refcpu_get_motion_states", MotionState), ("refcpu_get_manifolds", Manifold),
+                          ("refcpu_get_impulses", Impulses)):
+            getattr(L, name).argtypes = [vp, ctypes.POINTER(typ), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_get_pairs.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_get_aabbs.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_get_stats.argtypes = [vp, ctypes.POINTER(StepStats)]
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.refcpu_box_box.argtypes = [fp, fp, fp, fp, fp, fp, fp, fp, fp, ctypes.POINTER(ctypes.c_int)]
+        L.refcpu_sphere_sphere.argtypes = [ctypes.c_float, ctypes.c_float, fp, fp, fp]
+        L.refcpu_box_sphere.argtypes = [fp, fp, fp, ctypes.c_float, fp, fp]
+        L.refcpu_mix64.restype = ctypes.c_uint64
+        L.refcpu_mix64.argtypes = [ctypes.c_uint64]
+        _lib = L
+    return _lib
+
+
+def default_settings():
+    s = Settings()
+    lib().refcpu_settings_default(ctypes.byref(s))
+    return s
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class World:
+    """One oracle world; mirrors bepu_world of include/bepu_gpu.h."""
+
+    def __init__(self, settings=None):
+        self._s = settings if settings is not None else default_settings()
+        self.h = lib().refcpu_create(ctypes.byref(self._s))
+
+    def close(self):
+        if self.h:
+            lib().refcpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bruteforce_broadphase(self, on=True):
+        lib().refcpu_set_broadphase(self.h, 1 if on else 0)
+
+    def add_body(self, shape, dims, mass, state, material, flags=0):
+        ms = MotionState()
+        ms.position[:] = [float(x) for x in state[0:3]]
+        ms.orientation[:] = [float(x) for x in state[3:7]]
+        ms.linear_velocity[:] = [float(x) for x in state[7:10]]
+        ms.angular_velocity[:] = [float(x) for x in state[10:13]]
+        mat = Material(*[float(x) for x in material])
+        bid = ctypes.c_uint32()
+        if shape == 0:
+            rc = lib().refcpu_add_box(self.h, ctypes.byref(ms), float(dims[0]), float(dims[1]), float(dims[2]),
+                                      float(mass), ctypes.byref(mat), int(flags), ctypes.byref(bid))
+        elif shape == 1:
+            rc = lib().refcpu_add_sphere(self.h, ctypes.byref(ms), float(dims[0]), float(mass),
+                                         ctypes.byref(mat), int(flags), ctypes.byref(bid))
+        else:
+            raise NotImplementedError("shape %d" % shape)
+        if rc != 0:
+            raise ValueError("refcpu add failed with status %d" % rc)
+        return bid.value
+
+    def add_scene(self, scene):
+        for i in range(len(scene)):
+            r = scene[i]
+            self.add_body(int(r["shape"]), r["dims"], r["mass"], r["state"], r["material"], int(r["flags"]))
+
+    def step(self, n=1):
+        lib().refcpu_step(self.h, int(n))
+
+    def broadphase_only(self):
+        lib().refcpu_broadphase_only(self.h)
+
+    def _count(self, fn):
+        n = ctypes.c_uint32()
+        fn(self.h, None, 0, ctypes.byref(n))
+        return n.value
+
+    def motion_states(self):
+        n = self._count(lib().refcpu_get_motion_states)
+        buf = (MotionState * max(n, 1))()
+        lib().refcpu_get_motion_states(self.h, buf, n, ctypes.byref(ctypes.c_uint32()))
+        arr = np.frombuffer(buf, dtype=np.float32).reshape(-1, 13)[:n]
+        return arr.copy()
+
+    def aabbs(self):
+        n = self._count(lib().refcpu_get_aabbs)
+        out = np.zeros((max(n, 1), 6), np.float32)
+        lib().refcpu_get_aabbs(self.h, _fptr(out), n, ctypes.byref(ctypes.c_uint32()))
+        return out[:n]
+
+    def pairs(self):
+        n = self._count(lib().refcpu_get_pairs)
+        out = np.zeros(max(n, 1), np.uint64)
+        lib().refcpu_get_pairs(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+                               ctypes.byref(ctypes.c_uint32()))
+        return out[:n]
+
+    def manifolds(self):
+        n = self._count(lib().refcpu_get_manifolds)
+        buf = (Manifold * max(n, 1))()
+        lib().refcpu_get_manifolds(self.h, buf, n, ctypes.byref(ctypes.c_uint32()))
+        return list(buf)[:n]
+
+    def impulses(self):
+        n = self._count(lib().refcpu_get_impulses)
+        buf = (Impulses * max(n, 1))()
+        lib().refcpu_get_impulses(self.h, buf, n, ctypes.byref(ctypes.c_uint32()))
+        return list(buf)[:n]
+
+    def stats(self):
+        s = StepStats()
+        lib().refcpu_get_stats(self.h, ctypes.byref(s))
+        return s
+
+
+def box_box(half_a, pos_a, q_a, half_b, pos_b, q_b):
+    """BoxBoxCollider.AreBoxesColliding on one pair: returns (hit, axis, distance, contacts[n,4], ids[n])."""
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    ha, pa, qa, hb, pb, qb = map(f, (half_a, pos_a, q_a, half_b, pos_b, q_b))
+    axis = np.zeros(3, np.float32)
+    dist = np.zeros(1, np.float32)
+    cont = np.zeros((8, 4), np.float32)
+    ids = np.zeros(8, np.int32)
+    n = lib().refcpu_box_box(_fptr(ha), _fptr(pa), _fptr(qa), _fptr(hb), _fptr(pb), _fptr(qb), _fptr(axis),
+                             _fptr(dist), _fptr(cont), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if n < 0:
+        return False, axis, float(dist[0]), cont[:0], ids[:0]
+    return True, axis, float(dist[0]), cont[:n].copy(), ids[:n].copy()
+
+
+def sphere_sphere(ra, rb, pa, pb):
+    out = np.zeros(7, np.float32)
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    pa, pb = f(pa), f(pb)
+    hit = lib().refcpu_sphere_sphere(float(ra), float(rb), _fptr(pa), _fptr(pb), _fptr(out))
+    return bool(hit), out
+
+
+def box_sphere(half, bp, bq, r, sp):
+    out = np.zeros(7, np.float32)
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    half, bp, bq, sp = map(f, (half, bp, bq, sp))
+    hit = lib().refcpu_box_sphere(_fptr(half), _fptr(bp), _fptr(bq), float(r), _fptr(sp), _fptr(out))
+    return bool(hit), out
+
+
+def mix64(k):
+    return int(lib().refcpu_mix64(int(k)))

@@ -1,0 +1,1354 @@
+// refcpu_world.cpp -- oracle restatement of one BEPUphysics v1 time step
+// (Space.DoTimeStep, BEPUphysics/Space.cs:496-521) for convex bodies, run
+// single-threaded, plus its C API for the tests (ctypes).
+//
+// TEST INFRASTRUCTURE ONLY.  This is the checker the GPU library is compared
+// against; it is never linked into the product.
+//
+// Deliberate, documented deviations from the reference (all parity tests
+// state them):
+//  * Solve order.  The reference visits solver updateables in the permuted
+//    order of PermutationMapper over the Solver.Add history (Solver.cs:239-263).
+//    Both the oracle and the GPU library use the deterministic colour order
+//    defined in colour_constraints() below (the north-star comparison is made
+//    "once the reference's constraint order is permuted to the colour order").
+//  * Pair orientation.  The reference takes A/B from DynamicHierarchy traversal
+//    order; here A is the lower body id, except box-sphere where A is the box
+//    as BoxSpherePairHandler.Initialize enforces (BoxSpherePairHandler.cs:78-97).
+//  * Deactivation.  Sleeping is out of scope: every dynamic body behaves as
+//    IsAlwaysActive; the per-body stabilization state is kept
+//    (SimulationIslandMember.cs:53-70, EntityBase.cs:927-936).
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "../include/bepu_gpu.h"
+#include "refcpu.h"
+
+namespace ref {
+
+struct Body {
+    int shape;       // BEPU_SHAPE_*
+    V3 half;         // box half extents
+    float radius;    // sphere radius (= collision margin)
+    bool dynamic;
+    float mass, inv_mass;
+    M3 local_inertia_inv;
+    M3 inertia_inv;  // world space, refreshed by the force update
+    V3 pos;
+    Q4 q;
+    M3 R;            // orientationMatrix
+    V3 v, w;
+    float lin_damp, ang_damp, lin_boost, ang_boost;
+    float prev_velocity;
+    bool is_slowing;
+    float time_below;
+    bool allow_stabilization;
+    bepu_material mat;
+    V3 bmin, bmax;
+};
+
+struct ContactObj {
+    V3 pos, normal;
+    float depth;
+    int id;
+};
+
+struct PenC {  // ContactPenetrationConstraint
+    int slot;
+    float acc;
+    float lin[3], angA[3], angB[3];
+    float softness, bias, v2i;
+    V3 ra, rb;
+    bool aDyn, bDyn;
+    bool ais;  // isActiveInSolver
+    int cur, zero;
+};
+struct FricC {  // ContactFrictionConstraint
+    float acc;
+    float lin[3], angA[3], angB[3];
+    float friction, v2i;
+    bool aDyn, bDyn, ais;
+    int cur, zero;
+};
+struct Sliding {  // SlidingFrictionTwoAxis
+    bool active, ais;
+    int cur, zero;
+    float acc[2];
+    float lin[6], angA[6], angB[6];  // Matrix2x3 M11 M12 M13 M21 M22 M23
+    float v2i[4];                    // Matrix2x2 M11 M12 M21 M22
+    float friction;
+    int count;
+    V3 center, relvel, ra, rb;
+    bool aDyn, bDyn;
+};
+struct Twist {  // TwistFrictionConstraint
+    bool active, ais;
+    int cur, zero;
+    float acc;
+    float ang[3];
+    float v2i, friction;
+    float lever[4];
+    int count;
+    bool aDyn, bDyn;
+};
+
+struct Pair {
+    uint32_t a, b;
+    int kind;
+    float kinetic, static_f, bounce;
+    ContactObj slot[4];
+    bool used[4];
+    int clist[4];
+    int ccount;
+    PenC pen[4];
+    int npen;
+    FricC fric[4];
+    int nfric;
+    Sliding sl;
+    Twist tw;
+    bool prev_colliding;
+    bool group_ais;
+    int group_cur, group_zero;
+    int color;
+};
+
+struct Settings {
+    bepu_settings s;
+};
+
+}  // namespace ref
+
+using namespace ref;
+
+struct refcpu_world {
+    bepu_settings s;
+    std::vector<Body> bodies;
+    std::map<uint64_t, Pair> pairs;
+    std::vector<uint64_t> overlaps;
+    std::vector<uint64_t> order;  // solve order of the last step (pair keys)
+    int ncolors = 0;
+    uint32_t steps = 0;
+    float max_pen = 0;
+    int threads = 1;
+    int broadphase_mode = 0;  // 0: sort and sweep, 1: brute force
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Entity construction (host side, once per body)
+
+void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass, const bepu_material* m,
+               uint32_t flags, const M3& volume_distribution) {
+    // Entity.Initialize(collidable, mass) -> BecomeDynamic(mass, VolumeDistribution * (mass * 2.5))
+    // (EntityBase.cs:632-645, 867-905; InertiaHelper.InertiaTensorScale = 2.5)
+    if (mass > 0) {
+        b->dynamic = true;
+        M3 inertia = m3_scale(volume_distribution, mass * 2.5f);
+        b->local_inertia_inv = m3_adaptive_invert(inertia);
+        b->mass = mass;
+        b->inv_mass = 1 / mass;
+    } else {
+        // BecomeKinematic: infinite mass, zero inverse inertia.
+        b->dynamic = false;
+        b->local_inertia_inv = M3{0, 0, 0, 0, 0, 0, 0, 0, 0};
+        b->mass = 0;
+        b->inv_mass = 0;
+    }
+    b->pos = V3{st->position[0], st->position[1], st->position[2]};
+    // Orientation setter normalises (EntityBase.cs:68-84)
+    b->q = qnormalize(Q4{st->orientation[0], st->orientation[1], st->orientation[2], st->orientation[3]});
+    b->R = m3_from_quat(b->q);
+    b->inertia_inv = m3_mul(m3_mul_transposed(b->R, b->local_inertia_inv), b->R);
+    b->v = V3{st->linear_velocity[0], st->linear_velocity[1], st->linear_velocity[2]};
+    b->w = V3{st->angular_velocity[0], st->angular_velocity[1], st->angular_velocity[2]};
+    b->lin_damp = w->s.linear_damping;
+    b->ang_damp = w->s.angular_damping;
+    b->lin_boost = b->ang_boost = 0;
+    b->prev_velocity = 0;
+    b->is_slowing = false;
+    b->time_below = 0;
+    b->allow_stabilization = (flags & BEPU_FLAG_NO_STABILIZATION) == 0;
+    b->mat = m ? *m : bepu_material{1.0f, 0.8f, 0.0f};
+}
+
+// ---------------------------------------------------------------------------
+// Body stages
+
+// SimulationIslandMember.UpdateDeactivationCandidacy, dynamic and active branch
+// (SimulationIslandMember.cs:53-70)
+void candidacy(refcpu_world* w, Body& b) {
+    if (!b.dynamic) return;
+    float velocity = length_sq(b.v) + length_sq(b.w);
+    b.is_slowing = velocity <= b.prev_velocity;
+    float lim = w->s.velocity_lower_limit;
+    if (velocity < lim * lim) b.time_below += w->s.dt;
+    else b.time_below = 0;
+    b.prev_velocity = velocity;
+}
+
+// Entity.UpdateForForces (EntityBase.cs:911-977); ForceUpdater.gravityDt (ForceUpdater.cs:85-92)
+void force_update(refcpu_world* w, Body& b) {
+    if (!b.dynamic) return;
+    const bepu_settings& s = w->s;
+    V3 gravityDt = mul(V3{s.gravity[0], s.gravity[1], s.gravity[2]}, s.dt);
+    b.v = add(gravityDt, b.v);
+    float lim = s.velocity_lower_limit;
+    if (s.use_stabilization && b.allow_stabilization &&
+        (b.is_slowing || b.time_below > s.low_velocity_time_minimum)) {
+        float energy = length_sq(b.v) + length_sq(b.w);
+        if (energy < lim * lim) {
+            float boost = 1 - (float)(std::sqrt((double)energy) / (double)(2.0f * lim));
+            // ModifyAngularDamping / ModifyLinearDamping (EntityBase.cs:1205-1221)
+            float total = b.ang_damp + b.ang_boost;
+            float rem = 1 - total;
+            b.ang_boost += boost * rem;
+            total = b.lin_damp + b.lin_boost;
+            rem = 1 - total;
+            b.lin_boost += boost * rem;
+        }
+    }
+    float linear = b.lin_damp + b.lin_boost;
+    if (linear > 0) b.v = mul(b.v, (float)std::pow((double)clampf(1 - linear, 0, 1), (double)s.dt));
+    float angular = b.ang_damp + b.ang_boost;
+    if (angular > 0) b.w = mul(b.w, (float)std::pow((double)clampf(1 - angular, 0, 1), (double)s.dt));
+    b.lin_boost = 0;
+    b.ang_boost = 0;
+    b.inertia_inv = m3_mul(m3_mul_transposed(b.R, b.local_inertia_inv), b.R);
+}
+
+// EntityCollidable.UpdateBoundingBox -> Shape.GetBoundingBox -> ExpandBoundingBox
+// (EntityCollidable.cs:140-148,195-257; BoxShape.cs:149-174; SphereShape.cs:75-86)
+void bounding_box(refcpu_world* w, Body& b) {
+    if (b.shape == BEPU_SHAPE_BOX) {
+        M3 o = m3_from_quat(b.q);
+        V3 right{signf_int(o.m11) * b.half.x, signf_int(o.m21) * b.half.y, signf_int(o.m31) * b.half.z};
+        V3 up{signf_int(o.m12) * b.half.x, signf_int(o.m22) * b.half.y, signf_int(o.m32) * b.half.z};
+        V3 back{signf_int(o.m13) * b.half.x, signf_int(o.m23) * b.half.y, signf_int(o.m33) * b.half.z};
+        V3 off;
+        off.x = right.x * o.m11 + right.y * o.m21 + right.z * o.m31;
+        off.y = up.x * o.m12 + up.y * o.m22 + up.z * o.m32;
+        off.z = back.x * o.m13 + back.y * o.m23 + back.z * o.m33;
+        b.bmax = add(b.pos, off);
+        b.bmin = sub(b.pos, off);
+    } else {
+        float r = b.radius;
+        b.bmin = V3{b.pos.x - r, b.pos.y - r, b.pos.z - r};
+        b.bmax = V3{b.pos.x + r, b.pos.y + r, b.pos.z + r};
+    }
+    float dt = w->s.dt;
+    if (dt > 0) {
+        if (b.v.x > 0) b.bmax.x += b.v.x * dt * 1.0f; else b.bmin.x += b.v.x * dt * 1.0f;
+        if (b.v.y > 0) b.bmax.y += b.v.y * dt * 1.0f; else b.bmin.y += b.v.y * dt * 1.0f;
+        if (b.v.z > 0) b.bmax.z += b.v.z * dt * 1.0f; else b.bmin.z += b.v.z * dt * 1.0f;
+    }
+}
+
+// Entity.PreUpdatePosition (EntityBase.cs:1124-1155)
+void integrate(refcpu_world* w, Body& b) {
+    float dt = w->s.dt;
+    V3 inc = mul(b.w, dt * .5f);
+    Q4 m{inc.x, inc.y, inc.z, 0};
+    m = qmul(m, b.q);
+    b.q = qadd(b.q, m);
+    b.q = qnormalize(b.q);
+    b.R = m3_from_quat(b.q);
+    inc = mul(b.v, dt);
+    b.pos = add(b.pos, inc);
+}
+
+// ---------------------------------------------------------------------------
+// Broadphase: the pair set of DynamicHierarchy equals the closed-interval
+// AABB overlap set (BoundingBox.Intersects, BoundingBox.cs:58-69) filtered
+// by the default collision rules: kinematic-kinematic pairs are NoBroadPhase
+// (CollisionRules.cs:175-178) and one entry must be active (BroadPhase.cs:97-102).
+
+inline bool intersects(const Body& a, const Body& b) {
+    if (b.bmin.x > a.bmax.x || b.bmin.y > a.bmax.y || b.bmin.z > a.bmax.z) return false;
+    if (a.bmin.x > b.bmax.x || a.bmin.y > b.bmax.y || a.bmin.z > b.bmax.z) return false;
+    return true;
+}
+inline uint64_t pair_key(uint32_t a, uint32_t b) {
+    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return ((uint64_t)lo << 32) | hi;
+}
+
+void broadphase_bruteforce(refcpu_world* w, std::vector<uint64_t>* out) {
+    out->clear();
+    size_t n = w->bodies.size();
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = i + 1; j < n; ++j) {
+            const Body& a = w->bodies[i];
+            const Body& b = w->bodies[j];
+            if (!a.dynamic && !b.dynamic) continue;
+            if (intersects(a, b)) out->push_back(pair_key((uint32_t)i, (uint32_t)j));
+        }
+    std::sort(out->begin(), out->end());
+}
+
+// SortAndSweep1D-style sweep along x (SortAndSweep/SortAndSweep1D.cs:140-172); same set.
+void broadphase_sweep(refcpu_world* w, std::vector<uint64_t>* out) {
+    out->clear();
+    size_t n = w->bodies.size();
+    std::vector<uint32_t> idx(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = (uint32_t)i;
+    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        float fa = w->bodies[a].bmin.x, fb = w->bodies[b].bmin.x;
+        return fa < fb || (fa == fb && a < b);
+    });
+    for (size_t i = 0; i < n; ++i) {
+        const Body& a = w->bodies[idx[i]];
+        for (size_t j = i + 1; j < n; ++j) {
+            const Body& b = w->bodies[idx[j]];
+            if (b.bmin.x > a.bmax.x) break;
+            if (!a.dynamic && !b.dynamic) continue;
+            if (intersects(a, b)) out->push_back(pair_key(idx[i], idx[j]));
+        }
+    }
+    std::sort(out->begin(), out->end());
+}
+
+// ---------------------------------------------------------------------------
+// Contact constraint bookkeeping (ConvexContactManifoldConstraint.cs:117-155,
+// NonConvexContactManifoldConstraint.cs:98-139, ContactManifold.cs)
+
+inline bool convex_kind(int kind) { return kind == BEPU_PAIR_BOX_BOX || kind == BEPU_PAIR_CONVEX; }
+
+void constraint_add_contact(Pair& p, int slot) {
+    PenC& pc = p.pen[p.npen++];
+    std::memset(&pc, 0, sizeof(pc));
+    pc.slot = slot;
+    pc.acc = 0;
+    if (convex_kind(p.kind)) {
+        if (!p.tw.active) {
+            // TwistFrictionConstraint.Setup / SlidingFrictionTwoAxis.Setup
+            p.tw.active = true;
+            p.sl.active = true;
+            for (int k = 0; k < 6; ++k) p.sl.lin[k] = 0;
+        }
+    } else {
+        FricC& f = p.fric[p.nfric++];
+        std::memset(&f, 0, sizeof(f));
+        f.acc = 0;
+        f.lin[0] = f.lin[1] = f.lin[2] = 0;
+    }
+}
+
+void constraint_remove_contact(Pair& p, int slot) {
+    int idx = -1;
+    for (int i = 0; i < p.npen; ++i)
+        if (p.pen[i].slot == slot) { idx = i; break; }
+    if (idx >= 0) {
+        for (int k = idx; k < p.npen - 1; ++k) p.pen[k] = p.pen[k + 1];
+        p.npen--;
+    }
+    if (convex_kind(p.kind)) {
+        if (p.npen == 0) {
+            p.tw.active = false;
+            p.tw.acc = 0;
+            p.sl.active = false;
+            p.sl.acc[0] = p.sl.acc[1] = 0;
+        }
+    } else if (idx >= 0) {
+        // The friction constraint added with the removed penetration constraint
+        // sits at the same list position.
+        for (int k = idx; k < p.nfric - 1; ++k) p.fric[k] = p.fric[k + 1];
+        p.nfric--;
+    }
+}
+
+// ContactManifold.Remove (FastRemoveAt on the contact list, ContactManifold.cs)
+void manifold_remove(Pair& p, int index) {
+    int slot = p.clist[index];
+    p.ccount--;
+    if (index < p.ccount) p.clist[index] = p.clist[p.ccount];
+    p.used[slot] = false;
+    constraint_remove_contact(p, slot);
+}
+// ContactManifold.Add
+void manifold_add(Pair& p, const ContactData& c) {
+    int slot = 0;
+    while (p.used[slot]) ++slot;
+    p.used[slot] = true;
+    p.slot[slot].pos = c.pos;
+    p.slot[slot].normal = c.normal;
+    p.slot[slot].depth = c.depth;
+    p.slot[slot].id = c.id;
+    p.clist[p.ccount++] = slot;
+    constraint_add_contact(p, slot);
+}
+
+// BoxContactManifold.Update, non-ALLOWUNSAFE (BoxContactManifold.cs:125-206)
+void update_box_box(refcpu_world* w, Pair& p) {
+    const Body& A = w->bodies[p.a];
+    const Body& B = w->bodies[p.b];
+    float distance;
+    V3 axis;
+    BoxContactList m;
+    if (box_box_colliding(A.half, B.half, A.pos, A.q, B.pos, B.q, &distance, &axis, &m)) {
+        axis = neg(axis);
+        int toRemove[4];
+        int nrem = 0;
+        for (int i = 0; i < p.ccount; i++) {
+            bool found = false;
+            ContactObj& c = p.slot[p.clist[i]];
+            for (int j = m.n - 1; j >= 0; j--) {
+                if (c.id == m.e[j].id) {
+                    found = true;
+                    c.pos = m.e[j].pos;
+                    c.depth = -m.e[j].depth;
+                    c.normal = axis;
+                    m.remove_at(j);
+                    break;
+                }
+            }
+            if (!found) toRemove[nrem++] = i;
+        }
+        for (int i = nrem - 1; i >= 0; i--) manifold_remove(p, toRemove[i]);
+        for (int i = 0; i < m.n; i++) {
+            ContactData nc{m.e[i].pos, axis, -m.e[i].depth, m.e[i].id};
+            manifold_add(p, nc);
+        }
+    } else {
+        for (int i = p.ccount - 1; i >= 0; i--) manifold_remove(p, i);
+    }
+}
+
+// SphereContactManifold.Update / BoxSphereContactManifold.Update
+// (SphereContactManifold.cs:55-81, BoxSphereContactManifold.cs:55-81)
+void update_single_contact(refcpu_world* w, Pair& p) {
+    const Body& A = w->bodies[p.a];
+    const Body& B = w->bodies[p.b];
+    ContactData cd{};
+    bool hit;
+    if (p.kind == BEPU_PAIR_SPHERE_SPHERE)
+        hit = sphere_sphere_colliding(A.radius, B.radius, A.pos, B.pos, w->s.max_contact_distance, &cd);
+    else
+        hit = box_sphere_colliding(A.half, A.pos, A.q, B.radius, B.pos, w->s.max_contact_distance, &cd);
+    bool colliding = false;
+    if (hit) {
+        if (!p.prev_colliding && cd.depth >= 0) {
+            cd.id = 0;
+            manifold_add(p, cd);
+            colliding = true;
+        } else if (p.prev_colliding) {
+            ContactObj& c = p.slot[p.clist[0]];
+            c.normal = cd.normal;
+            c.depth = cd.depth;
+            c.pos = cd.pos;
+            colliding = true;
+        }
+    } else if (p.prev_colliding) {
+        // SphereContactManifold.Remove uses an ordered RemoveAt on a 1-element list.
+        manifold_remove(p, 0);
+    }
+    p.prev_colliding = colliding;
+}
+
+Pair make_pair_state(refcpu_world* w, uint32_t i, uint32_t j) {
+    Pair p;
+    std::memset(&p, 0, sizeof(p));
+    const Body& bi = w->bodies[i];
+    const Body& bj = w->bodies[j];
+    int si = bi.shape, sj = bj.shape;
+    if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) { p.kind = BEPU_PAIR_BOX_BOX; p.a = i; p.b = j; }
+    else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_SPHERE) { p.kind = BEPU_PAIR_SPHERE_SPHERE; p.a = i; p.b = j; }
+    else if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_SPHERE) { p.kind = BEPU_PAIR_BOX_SPHERE; p.a = i; p.b = j; }
+    else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_BOX) { p.kind = BEPU_PAIR_BOX_SPHERE; p.a = j; p.b = i; }
+    else { p.kind = BEPU_PAIR_CONVEX; p.a = i; p.b = j; }
+    // MaterialManager.DefaultMaterialBlender (MaterialManager.cs:77-85)
+    const bepu_material& ma = w->bodies[p.a].mat;
+    const bepu_material& mb = w->bodies[p.b].mat;
+    p.bounce = ma.bounciness * mb.bounciness;
+    p.kinetic = ma.kinetic_friction * mb.kinetic_friction;
+    p.static_f = ma.static_friction * mb.static_friction;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Solve order.  Constraints conflict when they share a dynamic body.  The
+// colour of a constraint is the smallest colour not used by its conflicting
+// neighbours of higher priority, processing constraints in decreasing
+// priority; priority(key) = mix64(key), a bijection.  Within a colour no two
+// constraints share a dynamic body, so any order gives identical results; the
+// canonical order is ascending key.  The solve order is (colour, key).
+
+uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void colour_constraints(refcpu_world* w, std::vector<Pair*>& cons) {
+    std::vector<std::pair<uint64_t, Pair*>> pri;
+    pri.reserve(cons.size());
+    for (Pair* p : cons) pri.push_back({mix64(pair_key(p->a, p->b)), p});
+    std::sort(pri.begin(), pri.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    // Per dynamic body: bitmask (and overflow list) of colours already taken.
+    std::vector<std::vector<int>> used(w->bodies.size());
+    int ncol = 0;
+    for (auto& e : pri) {
+        Pair* p = e.second;
+        std::vector<char> taken(64, 0);
+        auto mark = [&](uint32_t bi) {
+            if (!w->bodies[bi].dynamic) return;
+            for (int c : used[bi]) {
+                if ((size_t)c >= taken.size()) taken.resize(c + 1, 0);
+                taken[c] = 1;
+            }
+        };
+        mark(p->a);
+        mark(p->b);
+        int c = 0;
+        while ((size_t)c < taken.size() && taken[c]) ++c;
+        p->color = c;
+        if (c + 1 > ncol) ncol = c + 1;
+        if (w->bodies[p->a].dynamic) used[p->a].push_back(c);
+        if (w->bodies[p->b].dynamic) used[p->b].push_back(c);
+    }
+    std::sort(cons.begin(), cons.end(), [](Pair* x, Pair* y) {
+        if (x->color != y->color) return x->color < y->color;
+        return pair_key(x->a, x->b) < pair_key(y->a, y->b);
+    });
+    w->ncolors = ncol;
+}
+
+// ---------------------------------------------------------------------------
+// Solver rows
+
+inline void apply_linear(Body& b, const float l[3]) {  // Entity.ApplyLinearImpulse (EntityBase.cs:754-760)
+    b.v.x += l[0] * b.inv_mass;
+    b.v.y += l[1] * b.inv_mass;
+    b.v.z += l[2] * b.inv_mass;
+}
+inline void apply_angular(Body& b, const float i[3]) {  // Entity.ApplyAngularImpulse (EntityBase.cs:769-788)
+    const M3& t = b.inertia_inv;
+    b.w.x += i[0] * t.m11 + i[1] * t.m21 + i[2] * t.m31;
+    b.w.y += i[0] * t.m12 + i[1] * t.m22 + i[2] * t.m32;
+    b.w.z += i[0] * t.m13 + i[1] * t.m23 + i[2] * t.m33;
+}
+
+// ContactPenetrationConstraint.RelativeVelocity (ContactPenetrationConstraint.cs:86-104)
+inline float pen_relvel(const PenC& c, const Body& A, const Body& B) {
+    float lambda = A.v.x * c.lin[0] + A.v.y * c.lin[1] + A.v.z * c.lin[2] + A.w.x * c.angA[0] + A.w.y * c.angA[1] +
+                   A.w.z * c.angA[2];
+    lambda += -B.v.x * c.lin[0] - B.v.y * c.lin[1] - B.v.z * c.lin[2] + B.w.x * c.angB[0] + B.w.y * c.angB[1] +
+              B.w.z * c.angB[2];
+    return lambda;
+}
+
+// ContactPenetrationConstraint.Update (ContactPenetrationConstraint.cs:114-233)
+void pen_update(refcpu_world* w, Pair& p, PenC& c, Body& A, Body& B) {
+    const bepu_settings& s = w->s;
+    const ContactObj& ct = p.slot[c.slot];
+    c.aDyn = A.dynamic;
+    c.bDyn = B.dynamic;
+    c.lin[0] = -ct.normal.x;
+    c.lin[1] = -ct.normal.y;
+    c.lin[2] = -ct.normal.z;
+    c.ra = sub(ct.pos, A.pos);
+    c.angA[0] = (c.ra.y * c.lin[2]) - (c.ra.z * c.lin[1]);
+    c.angA[1] = (c.ra.z * c.lin[0]) - (c.ra.x * c.lin[2]);
+    c.angA[2] = (c.ra.x * c.lin[1]) - (c.ra.y * c.lin[0]);
+    c.rb = sub(ct.pos, B.pos);
+    c.angB[0] = (c.lin[1] * c.rb.z) - (c.lin[2] * c.rb.y);
+    c.angB[1] = (c.lin[2] * c.rb.x) - (c.lin[0] * c.rb.z);
+    c.angB[2] = (c.lin[0] * c.rb.y) - (c.lin[1] * c.rb.x);
+    float entryA, entryB, tX, tY, tZ;
+    if (c.aDyn) {
+        const M3& I = A.inertia_inv;
+        tX = c.angA[0] * I.m11 + c.angA[1] * I.m21 + c.angA[2] * I.m31;
+        tY = c.angA[0] * I.m12 + c.angA[1] * I.m22 + c.angA[2] * I.m32;
+        tZ = c.angA[0] * I.m13 + c.angA[1] * I.m23 + c.angA[2] * I.m33;
+        entryA = tX * c.angA[0] + tY * c.angA[1] + tZ * c.angA[2] + A.inv_mass;
+    } else entryA = 0;
+    if (c.bDyn) {
+        const M3& I = B.inertia_inv;
+        tX = c.angB[0] * I.m11 + c.angB[1] * I.m21 + c.angB[2] * I.m31;
+        tY = c.angB[0] * I.m12 + c.angB[1] * I.m22 + c.angB[2] * I.m32;
+        tZ = c.angB[0] * I.m13 + c.angB[1] * I.m23 + c.angB[2] * I.m33;
+        entryB = tX * c.angB[0] + tY * c.angB[1] + tZ * c.angB[2] + B.inv_mass;
+    } else entryB = 0;
+    float emi = entryA + entryB;
+    float updateRate = 1 / s.dt;
+    c.softness = s.softness * emi * updateRate;
+    c.v2i = -1 / (c.softness + emi);
+    if (ct.depth >= 0) {
+        c.bias = minf(maxf(0, ct.depth - s.allowed_penetration) * s.penetration_recovery_stiffness * updateRate,
+                      s.max_penetration_recovery_speed);
+        if (p.bounce > 0) {
+            float bounceVelocity = -pen_relvel(c, A, B);
+            if (bounceVelocity > 0) {
+                float lowThreshold = s.bounce_velocity_threshold * 0.3f;
+                float velocityFraction = clampf(
+                    (bounceVelocity - lowThreshold) / (s.bounce_velocity_threshold - lowThreshold + 1e-7f), 0, 1);
+                float bouncinessVelocity = velocityFraction * bounceVelocity * p.bounce;
+                c.bias = maxf(bouncinessVelocity, c.bias);
+            }
+        }
+    } else {
+        c.bias = ct.depth * updateRate;
+    }
+}
+
+// ContactPenetrationConstraint.ExclusiveUpdate (ContactPenetrationConstraint.cs:240-271)
+void pen_warm(const PenC& c, Body& A, Body& B) {
+    float lin[3] = {c.acc * c.lin[0], c.acc * c.lin[1], c.acc * c.lin[2]};
+    float ang[3];
+    if (c.aDyn) {
+        ang[0] = c.acc * c.angA[0]; ang[1] = c.acc * c.angA[1]; ang[2] = c.acc * c.angA[2];
+        apply_linear(A, lin);
+        apply_angular(A, ang);
+    }
+    if (c.bDyn) {
+        lin[0] = -lin[0]; lin[1] = -lin[1]; lin[2] = -lin[2];
+        ang[0] = c.acc * c.angB[0]; ang[1] = c.acc * c.angB[1]; ang[2] = c.acc * c.angB[2];
+        apply_linear(B, lin);
+        apply_angular(B, ang);
+    }
+}
+
+// ContactPenetrationConstraint.SolveIteration (ContactPenetrationConstraint.cs:278-321)
+float pen_solve(PenC& c, Body& A, Body& B) {
+    float lambda = (pen_relvel(c, A, B) - c.bias + c.softness * c.acc) * c.v2i;
+    float prev = c.acc;
+    c.acc = maxf(0, c.acc + lambda);
+    lambda = c.acc - prev;
+    float lin[3] = {lambda * c.lin[0], lambda * c.lin[1], lambda * c.lin[2]};
+    float ang[3];
+    if (c.aDyn) {
+        ang[0] = lambda * c.angA[0]; ang[1] = lambda * c.angA[1]; ang[2] = lambda * c.angA[2];
+        apply_linear(A, lin);
+        apply_angular(A, ang);
+    }
+    if (c.bDyn) {
+        lin[0] = -lin[0]; lin[1] = -lin[1]; lin[2] = -lin[2];
+        ang[0] = lambda * c.angB[0]; ang[1] = lambda * c.angB[1]; ang[2] = lambda * c.angB[2];
+        apply_linear(B, lin);
+        apply_angular(B, ang);
+    }
+    return std::fabs(lambda);
+}
+
+// SlidingFrictionTwoAxis.Update (SlidingFrictionTwoAxis.cs:242-440)
+void sliding_update(refcpu_world* w, Pair& p, Body& A, Body& B) {
+    Sliding& sl = p.sl;
+    sl.aDyn = A.dynamic;
+    sl.bDyn = B.dynamic;
+    sl.count = p.npen;
+    const V3& c0 = p.slot[p.pen[0].slot].pos;
+    switch (sl.count) {
+    case 1: sl.center = c0; break;
+    case 2:
+        sl.center = add(c0, p.slot[p.pen[1].slot].pos);
+        sl.center.x *= .5f; sl.center.y *= .5f; sl.center.z *= .5f;
+        break;
+    case 3:
+        sl.center = add(c0, p.slot[p.pen[1].slot].pos);
+        sl.center = add(p.slot[p.pen[2].slot].pos, sl.center);
+        sl.center.x *= .333333333f; sl.center.y *= .333333333f; sl.center.z *= .333333333f;
+        break;
+    case 4:
+        sl.center = add(c0, p.slot[p.pen[1].slot].pos);
+        sl.center = add(p.slot[p.pen[2].slot].pos, sl.center);
+        sl.center = add(p.slot[p.pen[3].slot].pos, sl.center);
+        sl.center.x *= .25f; sl.center.y *= .25f; sl.center.z *= .25f;
+        break;
+    default: sl.center = V3{-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f}; break;
+    }
+    sl.ra = sub(sl.center, A.pos);
+    V3 velA = add(cross(A.w, sl.ra), A.v);
+    sl.rb = sub(sl.center, B.pos);
+    V3 velB = add(cross(B.w, sl.rb), B.v);
+    sl.relvel = sub(velA, velB);
+    V3 normal = p.slot[p.pen[0].slot].normal;
+    float nvs = normal.x * sl.relvel.x + normal.y * sl.relvel.y + normal.z * sl.relvel.z;
+    sl.relvel.x -= nvs * normal.x;
+    sl.relvel.y -= nvs * normal.y;
+    sl.relvel.z -= nvs * normal.z;
+    float length = length_sq(sl.relvel);
+    const bepu_settings& s = w->s;
+    if (length > 1e-7f) {
+        length = std::sqrt(length);
+        float inv = 1 / length;
+        sl.lin[0] = sl.relvel.x * inv;
+        sl.lin[1] = sl.relvel.y * inv;
+        sl.lin[2] = sl.relvel.z * inv;
+        sl.friction = length > s.static_friction_velocity_threshold ? p.kinetic : p.static_f;
+    } else {
+        sl.friction = p.static_f;
+        if (!(sl.lin[0] != 0 || sl.lin[1] != 0 || sl.lin[2] != 0)) {
+            V3 axis1 = cross(normal, V3{1, 0, 0});
+            length = length_sq(axis1);
+            if (length > 1e-7f) {
+                length = std::sqrt(length);
+                float inv = 1 / length;
+                sl.lin[0] = axis1.x * inv;
+                sl.lin[1] = axis1.y * inv;
+                sl.lin[2] = axis1.z * inv;
+            } else {
+                axis1 = normalize(cross(normal, V3{0, 1, 0}));
+                sl.lin[0] = axis1.x;
+                sl.lin[1] = axis1.y;
+                sl.lin[2] = axis1.z;
+            }
+        }
+    }
+    float* L = sl.lin;
+    L[3] = (L[1] * normal.z) - (L[2] * normal.y);
+    L[4] = (L[2] * normal.x) - (L[0] * normal.z);
+    L[5] = (L[0] * normal.y) - (L[1] * normal.x);
+    const V3& ra = sl.ra;
+    const V3& rb = sl.rb;
+    float* aA = sl.angA;
+    float* aB = sl.angB;
+    aA[0] = (ra.y * L[2]) - (ra.z * L[1]);
+    aA[1] = (ra.z * L[0]) - (ra.x * L[2]);
+    aA[2] = (ra.x * L[1]) - (ra.y * L[0]);
+    aA[3] = (ra.y * L[5]) - (ra.z * L[4]);
+    aA[4] = (ra.z * L[3]) - (ra.x * L[5]);
+    aA[5] = (ra.x * L[4]) - (ra.y * L[3]);
+    aB[0] = (L[1] * rb.z) - (L[2] * rb.y);
+    aB[1] = (L[2] * rb.x) - (L[0] * rb.z);
+    aB[2] = (L[0] * rb.y) - (L[1] * rb.x);
+    aB[3] = (L[4] * rb.z) - (L[5] * rb.y);
+    aB[4] = (L[5] * rb.x) - (L[3] * rb.z);
+    aB[5] = (L[3] * rb.y) - (L[4] * rb.x);
+    // entry = J * I^-1 * J^T (+ m^-1 on the diagonal): Matrix2x3.Multiply, Transpose,
+    // Matrix2x2.Multiply (Matrix2x3.cs:94-111,225-235; Matrix2x2.cs:245-252)
+    auto entry = [](const float* J, const M3& I, float invMass, float e[4]) {
+        float t[6];
+        t[0] = J[0] * I.m11 + J[1] * I.m21 + J[2] * I.m31;
+        t[1] = J[0] * I.m12 + J[1] * I.m22 + J[2] * I.m32;
+        t[2] = J[0] * I.m13 + J[1] * I.m23 + J[2] * I.m33;
+        t[3] = J[3] * I.m11 + J[4] * I.m21 + J[5] * I.m31;
+        t[4] = J[3] * I.m12 + J[4] * I.m22 + J[5] * I.m32;
+        t[5] = J[3] * I.m13 + J[4] * I.m23 + J[5] * I.m33;
+        // transpose of J: M11=J11 M12=J21 M21=J12 M22=J22 M31=J13 M32=J23
+        e[0] = t[0] * J[0] + t[1] * J[1] + t[2] * J[2];
+        e[1] = t[0] * J[3] + t[1] * J[4] + t[2] * J[5];
+        e[2] = t[3] * J[0] + t[4] * J[1] + t[5] * J[2];
+        e[3] = t[3] * J[3] + t[4] * J[4] + t[5] * J[5];
+        e[0] += invMass;
+        e[3] += invMass;
+    };
+    float eA[4] = {0, 0, 0, 0}, eB[4] = {0, 0, 0, 0};
+    if (sl.aDyn) entry(aA, A.inertia_inv, A.inv_mass, eA);
+    if (sl.bDyn) entry(aB, B.inertia_inv, B.inv_mass, eB);
+    float m11 = -eA[0] - eB[0];
+    float m12 = -eA[1] - eB[1];
+    float m21 = -eA[2] - eB[2];
+    float m22 = -eA[3] - eB[3];
+    // Matrix2x2.Invert (Matrix2x2.cs:160-174)
+    float di = 1 / (m11 * m22 - m12 * m21);
+    sl.v2i[0] = m22 * di;
+    sl.v2i[1] = -m12 * di;
+    sl.v2i[2] = -m21 * di;
+    sl.v2i[3] = m11 * di;
+}
+
+inline void sliding_apply(const Sliding& sl, float lx, float ly, Body& A, Body& B) {
+    const float* L = sl.lin;
+    float lin[3] = {lx * L[0] + ly * L[3], lx * L[1] + ly * L[4], lx * L[2] + ly * L[5]};
+    float ang[3];
+    if (sl.aDyn) {
+        ang[0] = lx * sl.angA[0] + ly * sl.angA[3];
+        ang[1] = lx * sl.angA[1] + ly * sl.angA[4];
+        ang[2] = lx * sl.angA[2] + ly * sl.angA[5];
+        apply_linear(A, lin);
+        apply_angular(A, ang);
+    }
+    if (sl.bDyn) {
+        lin[0] = -lin[0]; lin[1] = -lin[1]; lin[2] = -lin[2];
+        ang[0] = lx * sl.angB[0] + ly * sl.angB[3];
+        ang[1] = lx * sl.angB[1] + ly * sl.angB[4];
+        ang[2] = lx * sl.angB[2] + ly * sl.angB[5];
+        apply_linear(B, lin);
+        apply_angular(B, ang);
+    }
+}
+
+// SlidingFrictionTwoAxis.SolveIteration (SlidingFrictionTwoAxis.cs:155-234)
+float sliding_solve(Pair& p, Body& A, Body& B) {
+    Sliding& sl = p.sl;
+    float dvx = A.v.x + (A.w.y * sl.ra.z) - (A.w.z * sl.ra.y);
+    float dvy = A.v.y + (A.w.z * sl.ra.x) - (A.w.x * sl.ra.z);
+    float dvz = A.v.z + (A.w.x * sl.ra.y) - (A.w.y * sl.ra.x);
+    dvx += -B.v.x - (B.w.y * sl.rb.z) + (B.w.z * sl.rb.y);
+    dvy += -B.v.y - (B.w.z * sl.rb.x) + (B.w.x * sl.rb.z);
+    dvz += -B.v.z - (B.w.x * sl.rb.y) + (B.w.y * sl.rb.x);
+    const float* L = sl.lin;
+    float lx = dvx * L[0] + dvy * L[1] + dvz * L[2];
+    float ly = dvx * L[3] + dvy * L[4] + dvz * L[5];
+    float x = lx;
+    lx = x * sl.v2i[0] + ly * sl.v2i[2];
+    ly = x * sl.v2i[1] + ly * sl.v2i[3];
+    float px = sl.acc[0], py = sl.acc[1];
+    sl.acc[0] += lx;
+    sl.acc[1] += ly;
+    float length = sl.acc[0] * sl.acc[0] + sl.acc[1] * sl.acc[1];  // Vector2.LengthSquared
+    float maxF = 0;
+    for (int i = 0; i < sl.count; i++) maxF += p.pen[i].acc;
+    maxF *= sl.friction;
+    if (length > maxF * maxF) {
+        length = maxF / std::sqrt(length);
+        sl.acc[0] *= length;
+        sl.acc[1] *= length;
+    }
+    lx = sl.acc[0] - px;
+    ly = sl.acc[1] - py;
+    sliding_apply(sl, lx, ly, A, B);
+    return std::fabs(lx) + std::fabs(ly);
+}
+
+// TwistFrictionConstraint.RelativeVelocity (TwistFrictionConstraint.cs:46-56)
+inline float twist_relvel(const Twist& t, const Body& A, const Body& B) {
+    float lambda = A.w.x * t.ang[0] + A.w.y * t.ang[1] + A.w.z * t.ang[2];
+    lambda -= B.w.x * t.ang[0] + B.w.y * t.ang[1] + B.w.z * t.ang[2];
+    return lambda;
+}
+
+// TwistFrictionConstraint.Update (TwistFrictionConstraint.cs:114-175)
+void twist_update(refcpu_world* w, Pair& p, Body& A, Body& B) {
+    Twist& t = p.tw;
+    t.aDyn = A.dynamic;
+    t.bDyn = B.dynamic;
+    V3 normal = p.slot[p.pen[0].slot].normal;
+    t.ang[0] = normal.x; t.ang[1] = normal.y; t.ang[2] = normal.z;
+    float entryA, entryB, tX, tY, tZ;
+    if (t.aDyn) {
+        const M3& I = A.inertia_inv;
+        tX = t.ang[0] * I.m11 + t.ang[1] * I.m21 + t.ang[2] * I.m31;
+        tY = t.ang[0] * I.m12 + t.ang[1] * I.m22 + t.ang[2] * I.m32;
+        tZ = t.ang[0] * I.m13 + t.ang[1] * I.m23 + t.ang[2] * I.m33;
+        entryA = tX * t.ang[0] + tY * t.ang[1] + tZ * t.ang[2] + A.inv_mass;
+    } else entryA = 0;
+    if (t.bDyn) {
+        const M3& I = B.inertia_inv;
+        tX = t.ang[0] * I.m11 + t.ang[1] * I.m21 + t.ang[2] * I.m31;
+        tY = t.ang[0] * I.m12 + t.ang[1] * I.m22 + t.ang[2] * I.m32;
+        tZ = t.ang[0] * I.m13 + t.ang[1] * I.m23 + t.ang[2] * I.m33;
+        entryB = tX * t.ang[0] + tY * t.ang[1] + tZ * t.ang[2] + B.inv_mass;
+    } else entryB = 0;
+    t.v2i = -1 / (entryA + entryB);
+    float rav = twist_relvel(t, A, B);
+    const V3& rs = p.sl.relvel;
+    const bepu_settings& s = w->s;
+    t.friction = std::fabs(rav) > s.static_friction_velocity_threshold ||
+                         std::fabs(rs.x) + std::fabs(rs.y) + std::fabs(rs.z) > s.static_friction_velocity_threshold
+                     ? p.kinetic
+                     : p.static_f;
+    t.friction *= s.twist_friction_factor;
+    t.count = p.npen;
+    for (int i = 0; i < t.count; i++) t.lever[i] = length(sub(p.slot[p.pen[i].slot].pos, p.sl.center));
+}
+
+inline void twist_apply(const Twist& t, float lambda, Body& A, Body& B) {
+    float ang[3] = {lambda * t.ang[0], lambda * t.ang[1], lambda * t.ang[2]};
+    if (t.aDyn) apply_angular(A, ang);
+    if (t.bDyn) {
+        ang[0] = -ang[0]; ang[1] = -ang[1]; ang[2] = -ang[2];
+        apply_angular(B, ang);
+    }
+}
+
+// TwistFrictionConstraint.SolveIteration (TwistFrictionConstraint.cs:65-107)
+float twist_solve(Pair& p, Body& A, Body& B) {
+    Twist& t = p.tw;
+    float lambda = twist_relvel(t, A, B);
+    lambda *= t.v2i;
+    float prev = t.acc;
+    float maxF = 0;
+    for (int i = 0; i < t.count; i++) maxF += t.lever[i] * p.pen[i].acc;
+    maxF *= t.friction;
+    t.acc = clampf(t.acc + lambda, -maxF, maxF);
+    lambda = t.acc - prev;
+    twist_apply(t, lambda, A, B);
+    return std::fabs(lambda);
+}
+
+// ContactFrictionConstraint.RelativeVelocity (ContactFrictionConstraint.cs:114-126)
+inline float fric_relvel(const FricC& f, const Body& A, const Body& B) {
+    float velocity = 0;
+    velocity += A.v.x * f.lin[0] + A.v.y * f.lin[1] + A.v.z * f.lin[2] + A.w.x * f.angA[0] + A.w.y * f.angA[1] +
+                A.w.z * f.angA[2];
+    velocity += -B.v.x * f.lin[0] - B.v.y * f.lin[1] - B.v.z * f.lin[2] + B.w.x * f.angB[0] + B.w.y * f.angB[1] +
+                B.w.z * f.angB[2];
+    return velocity;
+}
+
+// ContactFrictionConstraint.Update (ContactFrictionConstraint.cs:180-287)
+void fric_update(refcpu_world* w, Pair& p, FricC& f, const PenC& pc, Body& A, Body& B) {
+    f.aDyn = A.dynamic;
+    f.bDyn = B.dynamic;
+    V3 ra = pc.ra, rb = pc.rb;
+    V3 velA = add(cross(A.w, ra), A.v);
+    V3 velB = add(cross(B.w, rb), B.v);
+    V3 rv = sub(velA, velB);
+    V3 normal = p.slot[pc.slot].normal;
+    float nvs = normal.x * rv.x + normal.y * rv.y + normal.z * rv.z;
+    rv.x -= nvs * normal.x;
+    rv.y -= nvs * normal.y;
+    rv.z -= nvs * normal.z;
+    float length = length_sq(rv);
+    if (length > 1e-7f) {
+        length = std::sqrt(length);
+        f.lin[0] = rv.x / length;
+        f.lin[1] = rv.y / length;
+        f.lin[2] = rv.z / length;
+        f.friction = length > w->s.static_friction_velocity_threshold ? p.kinetic : p.static_f;
+    } else {
+        if (f.lin[0] != 0 || f.lin[1] != 0 || f.lin[2] != 0) {
+            f.friction = p.static_f;
+        } else {
+            f.ais = false;
+            return;
+        }
+    }
+    f.angA[0] = (ra.y * f.lin[2]) - (ra.z * f.lin[1]);
+    f.angA[1] = (ra.z * f.lin[0]) - (ra.x * f.lin[2]);
+    f.angA[2] = (ra.x * f.lin[1]) - (ra.y * f.lin[0]);
+    f.angB[0] = (f.lin[1] * rb.z) - (f.lin[2] * rb.y);
+    f.angB[1] = (f.lin[2] * rb.x) - (f.lin[0] * rb.z);
+    f.angB[2] = (f.lin[0] * rb.y) - (f.lin[1] * rb.x);
+    float entryA, entryB, tX, tY, tZ;
+    if (f.aDyn) {
+        const M3& I = A.inertia_inv;
+        tX = f.angA[0] * I.m11 + f.angA[1] * I.m21 + f.angA[2] * I.m31;
+        tY = f.angA[0] * I.m12 + f.angA[1] * I.m22 + f.angA[2] * I.m32;
+        tZ = f.angA[0] * I.m13 + f.angA[1] * I.m23 + f.angA[2] * I.m33;
+        entryA = tX * f.angA[0] + tY * f.angA[1] + tZ * f.angA[2] + A.inv_mass;
+    } else entryA = 0;
+    if (f.bDyn) {
+        const M3& I = B.inertia_inv;
+        tX = f.angB[0] * I.m11 + f.angB[1] * I.m21 + f.angB[2] * I.m31;
+        tY = f.angB[0] * I.m12 + f.angB[1] * I.m22 + f.angB[2] * I.m32;
+        tZ = f.angB[0] * I.m13 + f.angB[1] * I.m23 + f.angB[2] * I.m33;
+        entryB = tX * f.angB[0] + tY * f.angB[1] + tZ * f.angB[2] + B.inv_mass;
+    } else entryB = 0;
+    f.v2i = -1 / (entryA + entryB);
+}
+
+inline void fric_apply(const FricC& f, float lambda, Body& A, Body& B) {
+    float lin[3] = {lambda * f.lin[0], lambda * f.lin[1], lambda * f.lin[2]};
+    float ang[3];
+    if (f.aDyn) {
+        ang[0] = lambda * f.angA[0]; ang[1] = lambda * f.angA[1]; ang[2] = lambda * f.angA[2];
+        apply_linear(A, lin);
+        apply_angular(A, ang);
+    }
+    if (f.bDyn) {
+        lin[0] = -lin[0]; lin[1] = -lin[1]; lin[2] = -lin[2];
+        ang[0] = lambda * f.angB[0]; ang[1] = lambda * f.angB[1]; ang[2] = lambda * f.angB[2];
+        apply_linear(B, lin);
+        apply_angular(B, ang);
+    }
+}
+
+// ContactFrictionConstraint.SolveIteration (ContactFrictionConstraint.cs:131-174)
+float fric_solve(FricC& f, const PenC& pc, Body& A, Body& B) {
+    float lambda = fric_relvel(f, A, B) * f.v2i;
+    float prev = f.acc;
+    float maxForce = f.friction * pc.acc;
+    f.acc = clampf(f.acc + lambda, -maxForce, maxForce);
+    lambda = f.acc - prev;
+    fric_apply(f, lambda, A, B);
+    return std::fabs(lambda);
+}
+
+// Solver.UnsafePrestep -> UpdateSolverActivity -> group Update
+// (Solver.cs:265-275; ContactManifoldConstraint.cs:128-145; SolverGroup.cs:70-76;
+//  ConvexContactManifoldConstraint.cs:171-177; NonConvexContactManifoldConstraint.cs:180-186)
+void prestep(refcpu_world* w, Pair& p) {
+    Body& A = w->bodies[p.a];
+    Body& B = w->bodies[p.b];
+    p.group_ais = A.dynamic || B.dynamic;  // dynamic bodies are always active here
+    if (!p.group_ais) return;
+    p.group_cur = 0;
+    p.group_zero = 0;
+    for (int i = 0; i < p.npen; ++i) {
+        PenC& c = p.pen[i];
+        c.ais = true;
+        c.cur = 0;
+        c.zero = 0;
+        pen_update(w, p, c, A, B);
+    }
+    if (convex_kind(p.kind)) {
+        p.sl.ais = p.sl.active;
+        p.sl.cur = p.sl.zero = 0;
+        if (p.sl.ais) sliding_update(w, p, A, B);
+        p.tw.ais = p.tw.active;
+        p.tw.cur = p.tw.zero = 0;
+        if (p.tw.ais) twist_update(w, p, A, B);
+    } else {
+        for (int i = 0; i < p.nfric; ++i) {
+            FricC& f = p.fric[i];
+            f.ais = true;
+            f.cur = 0;
+            f.zero = 0;
+            fric_update(w, p, f, p.pen[i], A, B);
+        }
+    }
+}
+
+// Solver.UnsafeExclusiveUpdate -> group ExclusiveUpdate (Solver.cs:277-283)
+void warm_start(refcpu_world* w, Pair& p) {
+    if (!p.group_ais) return;
+    Body& A = w->bodies[p.a];
+    Body& B = w->bodies[p.b];
+    for (int i = 0; i < p.npen; ++i)
+        if (p.pen[i].ais) pen_warm(p.pen[i], A, B);
+    if (convex_kind(p.kind)) {
+        if (p.sl.ais) sliding_apply(p.sl, p.sl.acc[0], p.sl.acc[1], A, B);
+        if (p.tw.ais) twist_apply(p.tw, p.tw.acc, A, B);
+    } else {
+        for (int i = 0; i < p.nfric; ++i) {
+            const FricC& f = p.fric[i];
+            if (f.ais) fric_apply(f, f.acc, A, B);
+        }
+    }
+}
+
+// SolverGroup.SolveUpdateable (SolverGroup.cs:117-150)
+template <typename F>
+inline void solve_sub(refcpu_world* w, bool& ais, int& cur, int& zero, int& active, F solve) {
+    if (!ais) return;
+    cur++;
+    if (cur <= w->s.iteration_limit) {
+        if (solve() < w->s.min_impulse) {
+            zero++;
+            if (zero > w->s.min_iteration_count) ais = false;
+            else active++;
+        } else {
+            zero = 0;
+            active++;
+        }
+    } else {
+        ais = false;
+    }
+}
+
+// Solver.UnsafeSolveIteration -> ContactManifoldConstraint.SolveIteration
+// (Solver.cs:285-314; ConvexContactManifoldConstraint.cs:199-212; NonConvex...:201-212)
+void solve_iteration(refcpu_world* w, Pair& p) {
+    if (!p.group_ais) return;
+    p.group_cur++;
+    if (p.group_cur > w->s.iteration_limit) {
+        p.group_ais = false;
+        return;
+    }
+    Body& A = w->bodies[p.a];
+    Body& B = w->bodies[p.b];
+    int active = 0;
+    for (int i = 0; i < p.npen; ++i) {
+        PenC& c = p.pen[i];
+        solve_sub(w, c.ais, c.cur, c.zero, active, [&] { return pen_solve(c, A, B); });
+    }
+    if (convex_kind(p.kind)) {
+        solve_sub(w, p.sl.ais, p.sl.cur, p.sl.zero, active, [&] { return sliding_solve(p, A, B); });
+        solve_sub(w, p.tw.ais, p.tw.cur, p.tw.zero, active, [&] { return twist_solve(p, A, B); });
+    } else {
+        for (int i = 0; i < p.nfric; ++i) {
+            FricC& f = p.fric[i];
+            solve_sub(w, f.ais, f.cur, f.zero, active, [&] { return fric_solve(f, p.pen[i], A, B); });
+        }
+    }
+    p.group_ais = active > 0;
+    // The group returns minimumImpulse + 1, so its own zero-impulse counter resets.
+    p.group_zero = 0;
+}
+
+// ---------------------------------------------------------------------------
+// One time step (Space.DoTimeStep, Space.cs:496-521)
+
+void step_once(refcpu_world* w) {
+    for (Body& b : w->bodies) candidacy(w, b);
+    for (Body& b : w->bodies) force_update(w, b);
+    for (Body& b : w->bodies) bounding_box(w, b);
+    if (w->broadphase_mode == 1) broadphase_bruteforce(w, &w->overlaps);
+    else broadphase_sweep(w, &w->overlaps);
+
+    // NarrowPhase.UpdateSingleThreaded: update (or create) one pair per overlap,
+    // then drop pairs whose overlap vanished (NarrowPhase.cs:254-340).
+    std::map<uint64_t, Pair> next;
+    for (uint64_t key : w->overlaps) {
+        auto it = w->pairs.find(key);
+        Pair p;
+        if (it != w->pairs.end()) p = it->second;
+        else p = make_pair_state(w, (uint32_t)(key >> 32), (uint32_t)(key & 0xffffffffu));
+        if (p.kind == BEPU_PAIR_BOX_BOX) update_box_box(w, p);
+        else update_single_contact(w, p);
+        next.emplace(key, p);
+    }
+    w->pairs.swap(next);
+
+    std::vector<Pair*> cons;
+    float maxpen = 0;
+    for (auto& kv : w->pairs) {
+        Pair& p = kv.second;
+        if (p.ccount > 0) cons.push_back(&p);
+        for (int i = 0; i < p.ccount; ++i) maxpen = std::max(maxpen, p.slot[p.clist[i]].depth);
+    }
+    w->max_pen = maxpen;
+    colour_constraints(w, cons);
+    w->order.clear();
+    for (Pair* p : cons) w->order.push_back(pair_key(p->a, p->b));
+
+    // Solver.UpdateSingleThreaded (Solver.cs:239-263) in colour order.
+    for (Pair* p : cons) prestep(w, *p);
+    for (Pair* p : cons) warm_start(w, *p);
+    for (int it = 0; it < w->s.iteration_limit; ++it)
+        for (Pair* p : cons) solve_iteration(w, *p);
+
+    // ContinuousPositionUpdater.UpdateSingleThreaded, discrete path (ContinuousPositionUpdater.cs:134-154)
+    for (Body& b : w->bodies) integrate(w, b);
+    w->steps++;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C API used by the tests
+
+extern "C" {
+
+void refcpu_settings_default(bepu_settings* s) {
+    s->gravity[0] = 0; s->gravity[1] = -9.81f; s->gravity[2] = 0;
+    s->dt = 1.0f / 60.0f;
+    s->iteration_limit = 10;
+    s->min_iteration_count = 1;
+    s->min_impulse = .001f;
+    s->allowed_penetration = .01f;
+    s->default_margin = .04f;
+    s->max_contact_distance = .1f;
+    s->contact_invalidation_length_sq = .01f;
+    s->contact_min_separation_sq = .0009f;
+    s->bounce_velocity_threshold = 1;
+    s->max_penetration_recovery_speed = 2;
+    s->penetration_recovery_stiffness = .2f;
+    s->static_friction_velocity_threshold = .2f;
+    s->twist_friction_factor = 1;
+    s->softness = .001f;
+    s->use_stabilization = 1;
+    s->velocity_lower_limit = .26f;
+    s->low_velocity_time_minimum = 1;
+    s->linear_damping = .03f;
+    s->angular_damping = .15f;
+}
+
+refcpu_world* refcpu_create(const bepu_settings* s) {
+    refcpu_world* w = new refcpu_world();
+    if (s) w->s = *s;
+    else refcpu_settings_default(&w->s);
+    return w;
+}
+void refcpu_destroy(refcpu_world* w) { delete w; }
+void refcpu_set_broadphase(refcpu_world* w, int mode) { w->broadphase_mode = mode; }
+
+static int finite3(const float* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return 0;
+    return 1;
+}
+
+int refcpu_add_box(refcpu_world* w, const bepu_motion_state* st, float width, float height, float length,
+                   float mass, const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!st || !(width > 0) || !(height > 0) || !(length > 0) || std::isnan(mass) || std::isinf(mass))
+        return BEPU_E_INVALID_ARG;
+    if (!finite3(st->position, 3) || !finite3(st->orientation, 4)) return BEPU_E_INVALID_ARG;
+    Body b;
+    std::memset(&b, 0, sizeof(b));
+    b.shape = BEPU_SHAPE_BOX;
+    // BoxShape(width, height, length) (BoxShape.cs:56-62) and ComputeDescription (:118-138)
+    b.half = V3{width * 0.5f, height * 0.5f, length * 0.5f};
+    float w2 = width * width, h2 = height * height, l2 = length * length;
+    const float inv12 = 1 / 12.0f;
+    M3 vd{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    vd.m11 = (h2 + l2) * inv12;
+    vd.m22 = (w2 + l2) * inv12;
+    vd.m33 = (w2 + h2) * inv12;
+    init_body(w, &b, st, mass, m, flags, vd);
+    if (id) *id = (uint32_t)w->bodies.size();
+    w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+int refcpu_add_sphere(refcpu_world* w, const bepu_motion_state* st, float radius, float mass,
+                      const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!st || !(radius > 0) || std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
+    if (!finite3(st->position, 3) || !finite3(st->orientation, 4)) return BEPU_E_INVALID_ARG;
+    Body b;
+    std::memset(&b, 0, sizeof(b));
+    b.shape = BEPU_SHAPE_SPHERE;
+    b.radius = radius;
+    // SphereShape.ComputeDescription (SphereShape.cs:52-67)
+    float diag = ((2.0f / 5.0f) * radius * radius);
+    M3 vd{diag, 0, 0, 0, diag, 0, 0, 0, diag};
+    init_body(w, &b, st, mass, m, flags, vd);
+    if (id) *id = (uint32_t)w->bodies.size();
+    w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+int refcpu_step(refcpu_world* w, int n) {
+    for (int i = 0; i < n; ++i) step_once(w);
+    return BEPU_OK;
+}
+
+// Runs only the body stages and the broadphase of one step (no narrowphase,
+// solver or integration), for AABB/pair-set checks on a frozen scene.
+int refcpu_broadphase_only(refcpu_world* w) {
+    for (Body& b : w->bodies) candidacy(w, b);
+    for (Body& b : w->bodies) force_update(w, b);
+    for (Body& b : w->bodies) bounding_box(w, b);
+    if (w->broadphase_mode == 1) broadphase_bruteforce(w, &w->overlaps);
+    else broadphase_sweep(w, &w->overlaps);
+    return BEPU_OK;
+}
+
+int refcpu_get_motion_states(refcpu_world* w, bepu_motion_state* out, uint32_t cap, uint32_t* n) {
+    uint32_t cnt = (uint32_t)w->bodies.size();
+    if (n) *n = cnt;
+    for (uint32_t i = 0; i < cnt && i < cap; ++i) {
+        const Body& b = w->bodies[i];
+        bepu_motion_state& s = out[i];
+        s.position[0] = b.pos.x; s.position[1] = b.pos.y; s.position[2] = b.pos.z;
+        s.orientation[0] = b.q.x; s.orientation[1] = b.q.y; s.orientation[2] = b.q.z; s.orientation[3] = b.q.w;
+        s.linear_velocity[0] = b.v.x; s.linear_velocity[1] = b.v.y; s.linear_velocity[2] = b.v.z;
+        s.angular_velocity[0] = b.w.x; s.angular_velocity[1] = b.w.y; s.angular_velocity[2] = b.w.z;
+    }
+    return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+}
+
+int refcpu_get_aabbs(refcpu_world* w, float* out, uint32_t cap, uint32_t* n) {
+    uint32_t cnt = (uint32_t)w->bodies.size();
+    if (n) *n = cnt;
+    for (uint32_t i = 0; i < cnt && i < cap; ++i) {
+        const Body& b = w->bodies[i];
+        float* o = out + 6 * i;
+        o[0] = b.bmin.x; o[1] = b.bmin.y; o[2] = b.bmin.z;
+        o[3] = b.bmax.x; o[4] = b.bmax.y; o[5] = b.bmax.z;
+    }
+    return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+}
+
+int refcpu_get_pairs(refcpu_world* w, uint64_t* keys, uint32_t cap, uint32_t* n) {
+    uint32_t cnt = (uint32_t)w->overlaps.size();
+    if (n) *n = cnt;
+    for (uint32_t i = 0; i < cnt && i < cap; ++i) keys[i] = w->overlaps[i];
+    return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+}
+
+int refcpu_get_manifolds(refcpu_world* w, bepu_manifold* out, uint32_t cap, uint32_t* n) {
+    uint32_t cnt = 0;
+    for (auto& kv : w->pairs) {
+        const Pair& p = kv.second;
+        if (p.ccount == 0) continue;
+        if (cnt < cap) {
+            bepu_manifold& m = out[cnt];
+            std::memset(&m, 0, sizeof(m));
+            m.key = kv.first;
+            m.body_a = p.a;
+            m.body_b = p.b;
+            m.count = p.ccount;
+            m.kind = p.kind;
+            for (int i = 0; i < p.ccount; ++i) {
+                const ContactObj& c = p.slot[p.clist[i]];
+                bepu_contact& o = m.contacts[i];
+                o.position[0] = c.pos.x; o.position[1] = c.pos.y; o.position[2] = c.pos.z;
+                o.normal[0] = c.normal.x; o.normal[1] = c.normal.y; o.normal[2] = c.normal.z;
+                o.penetration_depth = c.depth;
+                o.id = c.id;
+            }
+        }
+        cnt++;
+    }
+    if (n) *n = cnt;
+    return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+}
+
+int refcpu_get_impulses(refcpu_world* w, bepu_impulses* out, uint32_t cap, uint32_t* n) {
+    uint32_t cnt = (uint32_t)w->order.size();
+    if (n) *n = cnt;
+    for (uint32_t i = 0; i < cnt && i < cap; ++i) {
+        const Pair& p = w->pairs.at(w->order[i]);
+        bepu_impulses& o = out[i];
+        std::memset(&o, 0, sizeof(o));
+        o.key = w->order[i];
+        o.color = p.color;
+        o.count = p.npen;
+        for (int k = 0; k < p.npen; ++k) o.normal[k] = p.pen[k].acc;
+        for (int k = 0; k < p.nfric; ++k) o.friction[k] = p.fric[k].acc;
+        o.sliding[0] = p.sl.acc[0];
+        o.sliding[1] = p.sl.acc[1];
+        o.twist = p.tw.acc;
+    }
+    return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+}
+
+int refcpu_get_stats(refcpu_world* w, bepu_step_stats* s) {
+    std::memset(s, 0, sizeof(*s));
+    s->bodies = (uint32_t)w->bodies.size();
+    s->pairs = (uint32_t)w->overlaps.size();
+    uint32_t m = 0, c = 0;
+    for (auto& kv : w->pairs)
+        if (kv.second.ccount > 0) { m++; c += kv.second.ccount; }
+    s->manifolds = m;
+    s->contacts = c;
+    s->colors = (uint32_t)w->ncolors;
+    s->steps = w->steps;
+    s->max_penetration = w->max_pen;
+    return BEPU_OK;
+}
+
+// Stand-alone collision tests for closed-form checks.
+int refcpu_box_box(const float* halfA, const float* posA, const float* qA, const float* halfB, const float* posB,
+                   const float* qB, float* axis_out, float* distance_out, float* contacts_out /* 8 x (x,y,z,depth) */,
+                   int* ids_out) {
+    float d;
+    V3 ax;
+    BoxContactList m;
+    bool hit = box_box_colliding(V3{halfA[0], halfA[1], halfA[2]}, V3{halfB[0], halfB[1], halfB[2]},
+                                 V3{posA[0], posA[1], posA[2]}, Q4{qA[0], qA[1], qA[2], qA[3]},
+                                 V3{posB[0], posB[1], posB[2]}, Q4{qB[0], qB[1], qB[2], qB[3]}, &d, &ax, &m);
+    axis_out[0] = ax.x; axis_out[1] = ax.y; axis_out[2] = ax.z;
+    *distance_out = d;
+    if (!hit) return -1;
+    for (int i = 0; i < m.n; ++i) {
+        contacts_out[4 * i + 0] = m.e[i].pos.x;
+        contacts_out[4 * i + 1] = m.e[i].pos.y;
+        contacts_out[4 * i + 2] = m.e[i].pos.z;
+        contacts_out[4 * i + 3] = m.e[i].depth;
+        ids_out[i] = m.e[i].id;
+    }
+    return m.n;
+}
+
+int refcpu_sphere_sphere(float ra, float rb, const float* pa, const float* pb, float* out7) {
+    ContactData c{};
+    if (!sphere_sphere_colliding(ra, rb, V3{pa[0], pa[1], pa[2]}, V3{pb[0], pb[1], pb[2]}, .1f, &c)) return 0;
+    out7[0] = c.pos.x; out7[1] = c.pos.y; out7[2] = c.pos.z;
+    out7[3] = c.normal.x; out7[4] = c.normal.y; out7[5] = c.normal.z; out7[6] = c.depth;
+    return 1;
+}
+
+int refcpu_box_sphere(const float* half, const float* bp, const float* bq, float r, const float* sp, float* out7) {
+    ContactData c{};
+    if (!box_sphere_colliding(V3{half[0], half[1], half[2]}, V3{bp[0], bp[1], bp[2]}, Q4{bq[0], bq[1], bq[2], bq[3]},
+                              r, V3{sp[0], sp[1], sp[2]}, .1f, &c))
+        return 0;
+    out7[0] = c.pos.x; out7[1] = c.pos.y; out7[2] = c.pos.z;
+    out7[3] = c.normal.x; out7[4] = c.normal.y; out7[5] = c.normal.z; out7[6] = c.depth;
+    return 1;
+}
+
+uint64_t refcpu_mix64(uint64_t k) { return mix64(k); }
+
+}  // extern "C"
