@@ -1,0 +1,176 @@
+// k_bodies.hip -- per-body stages of the step: deactivation candidacy,
+// force update, bounding box (one fused pass), and position integration.
+//
+// HBM-bound elementwise kernels: one thread per body, float4 SoA loads.
+#include "world.h"
+
+namespace bepu {
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// One pass per body, in the order Space.DoTimeStep runs the stages
+// (Space.cs:505-508): SimulationIslandMember.UpdateDeactivationCandidacy
+// (SimulationIslandMember.cs:53-70), Entity.UpdateForForces
+// (EntityBase.cs:911-977), EntityCollidable.UpdateBoundingBox
+// (EntityCollidable.cs:140-148,195-257 with BoxShape.cs:149-174 /
+// SphereShape.cs:75-86).  Also folds the AABB centres into the scene bounds
+// used to quantise Morton codes.
+__global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bepu_settings& s = w.s;
+    float cx = 0, cy = 0, cz = 0;
+    bool valid = i < w.nbodies;
+    if (valid) {
+        bool dyn = (w.bflags[i] & 1u) != 0;
+        float4 P = w.pos[i];
+        float4 Q = w.quat[i];
+        float4 V = w.linv[i];
+        float4 W = w.angv[i];
+        f3 v = f3_of(V), av = f3_of(W);
+        q4 q{Q.x, Q.y, Q.z, Q.w};
+        if (dyn) {
+            float4 st = w.stab[i];
+            float lim = s.velocity_lower_limit;
+            // candidacy
+            float velocity = lensq3(v) + lensq3(av);
+            float slowing = velocity <= st.x ? 1.0f : 0.0f;
+            float tb = st.y;
+            if (velocity < lim * lim) tb += s.dt;
+            else tb = 0;
+            // force update: gravity, stabilization boost, damping, world inertia
+            f3 gdt = f3{s.gravity[0], s.gravity[1], s.gravity[2]} * s.dt;
+            v = gdt + v;
+            float2 dm = w.damp[i];
+            float lin_boost = 0, ang_boost = 0;
+            if (s.use_stabilization && st.w != 0.0f && (slowing != 0.0f || tb > s.low_velocity_time_minimum)) {
+                float energy = lensq3(v) + lensq3(av);
+                if (energy < lim * lim) {
+                    float boost = 1 - (float)(sqrt((double)energy) / (double)(2.0f * lim));
+                    float total = dm.y + ang_boost;
+                    float rem = 1 - total;
+                    ang_boost += boost * rem;
+                    total = dm.x + lin_boost;
+                    rem = 1 - total;
+                    lin_boost += boost * rem;
+                }
+            }
+            float linear = dm.x + lin_boost;
+            if (linear > 0) v = v * (float)pow((double)clampf_ref(1 - linear, 0, 1), (double)s.dt);
+            float angular = dm.y + ang_boost;
+            if (angular > 0) av = av * (float)pow((double)clampf_ref(1 - angular, 0, 1), (double)s.dt);
+            m3 R = mat_from_quat(q);
+            float4 l0 = w.li0[i], l1 = w.li1[i], l2 = w.li2[i];
+            m3 L{l0.x, l0.y, l0.z, l1.x, l1.y, l1.z, l2.x, l2.y, l2.z};
+            m3 I = rotate_inertia(R, L);
+            float im = w.ii0[i].w;
+            w.ii0[i] = make_float4(I.m11, I.m12, I.m13, im);
+            w.ii1[i] = make_float4(I.m21, I.m22, I.m23, 0.0f);
+            w.ii2[i] = make_float4(I.m31, I.m32, I.m33, 0.0f);
+            w.linv[i] = f4_of(v, 0.0f);
+            w.angv[i] = f4_of(av, 0.0f);
+            w.stab[i] = make_float4(velocity, tb, slowing, st.w);
+        }
+        // bounding box
+        float4 sh = w.shape[i];
+        f3 mn, mx;
+        if (sh.w == (float)BEPU_SHAPE_BOX) {
+            m3 o = mat_from_quat(q);
+            f3 right{sign_ref(o.m11) * sh.x, sign_ref(o.m21) * sh.y, sign_ref(o.m31) * sh.z};
+            f3 up{sign_ref(o.m12) * sh.x, sign_ref(o.m22) * sh.y, sign_ref(o.m32) * sh.z};
+            f3 back{sign_ref(o.m13) * sh.x, sign_ref(o.m23) * sh.y, sign_ref(o.m33) * sh.z};
+            f3 off;
+            off.x = right.x * o.m11 + right.y * o.m21 + right.z * o.m31;
+            off.y = up.x * o.m12 + up.y * o.m22 + up.z * o.m32;
+            off.z = back.x * o.m13 + back.y * o.m23 + back.z * o.m33;
+            f3 p = f3_of(P);
+            mx = p + off;
+            mn = p - off;
+        } else {
+            float r = sh.x;
+            mn = f3{P.x - r, P.y - r, P.z - r};
+            mx = f3{P.x + r, P.y + r, P.z + r};
+        }
+        float dt = s.dt;
+        if (dt > 0) {
+            if (v.x > 0) mx.x += v.x * dt * 1.0f; else mn.x += v.x * dt * 1.0f;
+            if (v.y > 0) mx.y += v.y * dt * 1.0f; else mn.y += v.y * dt * 1.0f;
+            if (v.z > 0) mx.z += v.z * dt * 1.0f; else mn.z += v.z * dt * 1.0f;
+        }
+        w.bmin[i] = f4_of(mn, 0.0f);
+        w.bmax[i] = f4_of(mx, 0.0f);
+        cx = 0.5f * (mn.x + mx.x);
+        cy = 0.5f * (mn.y + mx.y);
+        cz = 0.5f * (mn.z + mx.z);
+    }
+    // Scene bounds of AABB centres: wave reduction, then one atomic per wave.
+    uint32_t lo[3] = {valid ? f2ord(cx) : 0xffffffffu, valid ? f2ord(cy) : 0xffffffffu,
+                      valid ? f2ord(cz) : 0xffffffffu};
+    uint32_t hi[3] = {valid ? f2ord(cx) : 0u, valid ? f2ord(cy) : 0u, valid ? f2ord(cz) : 0u};
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off));
+            hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&w.bounds[k], lo[k]);
+            atomicMax(&w.bounds[3 + k], hi[k]);
+        }
+}
+
+// Entity.PreUpdatePosition (EntityBase.cs:1124-1155), discrete mode.
+__global__ void __launch_bounds__(256) k_integrate(DeviceWorld w) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.nbodies) return;
+    float dt = w.s.dt;
+    float4 Q = w.quat[i];
+    float4 P = w.pos[i];
+    f3 v = f3_of(w.linv[i]);
+    f3 av = f3_of(w.angv[i]);
+    q4 q{Q.x, Q.y, Q.z, Q.w};
+    f3 inc = av * (dt * .5f);
+    q4 m{inc.x, inc.y, inc.z, 0};
+    m = qmul(m, q);
+    q = q4{q.x + m.x, q.y + m.y, q.z + m.z, q.w + m.w};
+    q = qnormalize(q);
+    inc = v * dt;
+    f3 p = f3_of(P) + inc;
+    w.quat[i] = make_float4(q.x, q.y, q.z, q.w);
+    w.pos[i] = f4_of(p, 0.0f);
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+}
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+}
+
+void launch_body_pre(const DeviceWorld& w, hipStream_t st) {
+    // bounds[0..2] = +inf (as ordered ints: max), bounds[3..5] = -inf
+    launch_fill_u32(w.bounds, 0xffffffffu, 3, st);
+    launch_fill_u32(w.bounds + 3, 0u, 3, st);
+    if (w.nbodies == 0) return;
+    hipLaunchKernelGGL(k_body_pre, dim3((w.nbodies + 255) / 256), dim3(256), 0, st, w);
+}
+
+void launch_integrate(const DeviceWorld& w, hipStream_t st) {
+    if (w.nbodies == 0) return;
+    hipLaunchKernelGGL(k_integrate, dim3((w.nbodies + 255) / 256), dim3(256), 0, st, w);
+}
+
+}  // namespace bepu

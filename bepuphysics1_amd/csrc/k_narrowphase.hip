@@ -1,0 +1,713 @@
+// k_narrowphase.hip -- per-pair contact generation and persistent manifold
+// update on gfx950.
+//
+// One thread per broadphase pair.  The pair's previous state is found in last
+// step's key -> slot table; the updated state is written to this step's slot
+// and table.  Box-box candidate contacts (up to 8 before pruning) live in LDS,
+// one column per thread, so the clipper never spills to scratch.
+//
+// Colliders (paths relative to the reference repository root):
+//   box-box     BoxBoxCollider.AreBoxesColliding (BoxBoxCollider.cs:1494-2386)
+//   box-sphere  BoxSphereTester.AreShapesColliding (BoxSphereTester.cs:23-85)
+//   sphere      SphereTester.AreSpheresColliding (SphereTester.cs:23-56)
+// Manifolds: BoxContactManifold.Update (BoxContactManifold.cs:125-206),
+// SphereContactManifold / BoxSphereContactManifold.Update (…Manifold.cs:55-81).
+#include "world.h"
+
+namespace bepu {
+
+constexpr int kNpBlock = 128;
+constexpr float kEps = 1e-7f;      // Toolbox.Epsilon
+constexpr float kBigEps = 1e-5f;   // Toolbox.BigEpsilon
+
+// Candidate contact list of one thread, stored in LDS columns:
+// entry k of thread t is pd[k * kNpBlock + t] (position xyz, depth) and id[...].
+struct LdsList {
+    float4* pd;
+    int* id;
+    int n;
+    __device__ float4& P(int k) { return pd[k * kNpBlock]; }
+    __device__ int& I(int k) { return id[k * kNpBlock]; }
+    // TinyStructList semantics: Add beyond 8 entries is ignored, RemoveAt shifts.
+    __device__ void add(float4 p, int i) {
+        if (n < 8) { P(n) = p; I(n) = i; ++n; }
+    }
+    __device__ void remove_at(int i) {
+        for (int k = i; k < n - 1; ++k) { P(k) = P(k + 1); I(k) = I(k + 1); }
+        --n;
+    }
+    __device__ void copy_from(LdsList& o) {
+        for (int k = 0; k < o.n; ++k) { P(k) = o.P(k); I(k) = o.I(k); }
+        n = o.n;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Box-box: edge-edge contacts (BoxBoxCollider.cs:2392-2698)
+
+__device__ __forceinline__ int edge_id(int a, int b) { return (a + 1) * 571 + (b + 1) * 577; }
+
+// GetEdgeData (BoxBoxCollider.cs:2700-2864) as sign tables.
+__device__ __forceinline__ void edge_data(int index, int axis, f3 h, f3* s, f3* e, int* sid, int* eid) {
+    // For axis 0 the edge runs along x; index bits pick the signs of the other two axes.
+    float u = (index & 2) ? 1.0f : -1.0f;  // first of the remaining axes
+    float v = (index & 1) ? 1.0f : -1.0f;  // second
+    if (axis == 0) {
+        *s = f3{-h.x, u * h.y, v * h.z}; *e = f3{h.x, u * h.y, v * h.z};
+        *sid = index; *eid = 4 + index;
+    } else if (axis == 1) {
+        *s = f3{u * h.x, -h.y, v * h.z}; *e = f3{u * h.x, h.y, v * h.z};
+        *sid = ((index & 2) << 1) | (index & 1); *eid = *sid + 2;
+    } else {
+        *s = f3{u * h.x, v * h.y, -h.z}; *e = f3{u * h.x, v * h.y, h.z};
+        *sid = (index << 1); *eid = *sid + 1;
+    }
+}
+
+__device__ void pick_edges(f3 m, f3 h, f3* s1, f3* e1, int* s1id, int* e1id, f3* s2, f3* e2, int* s2id, int* e2id) {
+    float d[4];
+    int axis;
+    if (fabsf(m.x) < kEps) {
+        d[0] = -h.y * m.y - h.z * m.z; d[1] = -h.y * m.y + h.z * m.z;
+        d[2] = h.y * m.y - h.z * m.z;  d[3] = h.y * m.y + h.z * m.z;
+        axis = 0;
+    } else if (fabsf(m.y) < kEps) {
+        d[0] = -h.x * m.x - h.z * m.z; d[1] = -h.x * m.x + h.z * m.z;
+        d[2] = h.x * m.x - h.z * m.z;  d[3] = h.x * m.x + h.z * m.z;
+        axis = 1;
+    } else {
+        d[0] = -h.x * m.x - h.y * m.y; d[1] = -h.x * m.x + h.y * m.y;
+        d[2] = h.x * m.x - h.y * m.y;  d[3] = h.x * m.x + h.y * m.y;
+        axis = 2;
+    }
+    // FindHighestIndices (BoxBoxCollider.cs:2866-2890)
+    int hi = 0;
+    float hv = d[0];
+    for (int i = 1; i < 4; i++)
+        if (d[i] > hv) { hi = i; hv = d[i]; }
+    int se = 0;
+    float sv = -3.402823466e+38f;
+    for (int i = 0; i < 4; i++)
+        if (i != hi && d[i] > sv) { se = i; sv = d[i]; }
+    edge_data(hi, axis, h, s1, e1, s1id, e1id);
+    edge_data(se, axis, h, s2, e2, s2id, e2id);
+}
+
+// GetClosestPointsBetweenSegments (BoxBoxCollider.cs:2902-2990)
+__device__ bool seg_closest(f3 p1, f3 q1, f3 p2, f3 q2, f3* c1, f3* c2) {
+    f3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+    float a = lensq3(d1), e = lensq3(d2), f = dot3(d2, r);
+    float s, t;
+    if (a <= kEps && e <= kEps) return false;
+    if (a <= kEps) {
+        s = 0.0f;
+        t = f / e;
+        if (t < 0 || t > 1) return false;
+    } else {
+        float c = dot3(d1, r);
+        if (e <= kEps) {
+            t = 0.0f;
+            s = clampf_ref(-c / a, 0.0f, 1.0f);
+        } else {
+            float b = dot3(d1, d2);
+            float denom = a * e - b * b;
+            if (denom != 0.0f) {
+                s = (b * f - c * e) / denom;
+                if (s < 0 || s > 1) return false;
+            } else {
+                s = .5f;
+            }
+            t = (b * s + f) / e;
+            if (t < 0 || t > 1) return false;
+        }
+    }
+    *c1 = d1 * s + p1;
+    *c2 = d2 * t + p2;
+    return true;
+}
+
+__device__ void edge_edge(f3 ha, f3 hb, f3 pA, const m3& oA, f3 pB, const m3& oB, f3 mtd, LdsList& out) {
+    f3 mA = mat_transform_t(-mtd, oA);
+    f3 mB = mat_transform_t(mtd, oB);
+    f3 as[2], ae[2], bs[2], be[2];
+    int asid[2], aeid[2], bsid[2], beid[2];
+    pick_edges(mA, ha, &as[0], &ae[0], &asid[0], &aeid[0], &as[1], &ae[1], &asid[1], &aeid[1]);
+    pick_edges(mB, hb, &bs[0], &be[0], &bsid[0], &beid[0], &bs[1], &be[1], &bsid[1], &beid[1]);
+    for (int k = 0; k < 2; ++k) {
+        as[k] = mat_transform(as[k], oA) + pA;
+        ae[k] = mat_transform(ae[k], oA) + pA;
+        bs[k] = mat_transform(bs[k], oB) + pB;
+        be[k] = mat_transform(be[k], oB) + pB;
+    }
+    out.n = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            f3 onA, onB;
+            if (seg_closest(as[i], ae[i], bs[j], be[j], &onA, &onB)) {
+                float d = dot3(onA - onB, mtd);
+                if (d < 0) out.add(f4_of(onA, d), edge_id(asid[i], aeid[i]) * 2549 + edge_id(bsid[j], beid[j]) * 2857);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Box-box: face contacts (BoxBoxCollider.cs:3459-3487, 4517-5278, 6007-6289)
+
+struct Face {
+    int id[4];
+    f3 v[4];
+    f3 n;
+    float width, height;
+};
+
+__device__ __forceinline__ f3 xform(f3 c, const m3& o, f3 p) {  // Matrix.Transform (Matrix.cs:717-725)
+    return f3{c.x * o.m11 + c.y * o.m21 + c.z * o.m31 + p.x, c.x * o.m12 + c.y * o.m22 + c.z * o.m32 + p.y,
+              c.x * o.m13 + c.y * o.m23 + c.z * o.m33 + p.z};
+}
+
+// GetNearestFace (BoxBoxCollider.cs:6042-6177)
+__device__ void nearest_face(f3 p, const m3& o, f3 mtd, f3 h, Face& f) {
+    f.id[0] = f.id[1] = f.id[2] = f.id[3] = 0;
+    f.v[0] = f.v[1] = f.v[2] = f.v[3] = f3{0, 0, 0};
+    f.n = f3{0, 0, 0};
+    f.width = f.height = 0;
+    float xd = o.m11 * mtd.x + o.m12 * mtd.y + o.m13 * mtd.z;
+    float yd = o.m21 * mtd.x + o.m22 * mtd.y + o.m23 * mtd.z;
+    float zd = o.m31 * mtd.x + o.m32 * mtd.y + o.m33 * mtd.z;
+    float ax = fabsf(xd), ay = fabsf(yd), az = fabsf(zd);
+    float hw = h.x, hh = h.y, hl = h.z;
+    int bit;
+    if (ax > ay && ax > az) {
+        if (xd < 0) { hw = -hw; bit = 0; } else bit = 1;
+        f.v[0] = xform(f3{hw, hh, hl}, o, p); f.v[1] = xform(f3{hw, -hh, hl}, o, p);
+        f.v[2] = xform(f3{hw, -hh, -hl}, o, p); f.v[3] = xform(f3{hw, hh, -hl}, o, p);
+        f.n = xd < 0 ? f3{-o.m11, -o.m12, -o.m13} : f3{o.m11, o.m12, o.m13};
+        f.width = hh * 2; f.height = hl * 2;
+        f.id[0] = bit + 2 + 4; f.id[1] = bit + 4; f.id[2] = bit + 2; f.id[3] = bit;
+    } else if (ay > ax && ay > az) {
+        if (yd < 0) { hh = -hh; bit = 0; } else bit = 2;
+        f.v[0] = xform(f3{hw, hh, hl}, o, p); f.v[1] = xform(f3{-hw, hh, hl}, o, p);
+        f.v[2] = xform(f3{-hw, hh, -hl}, o, p); f.v[3] = xform(f3{hw, hh, -hl}, o, p);
+        f.n = yd < 0 ? f3{-o.m21, -o.m22, -o.m23} : f3{o.m21, o.m22, o.m23};
+        f.width = hw * 2; f.height = hl * 2;
+        f.id[0] = 1 + bit + 4; f.id[1] = bit + 4; f.id[2] = 1 + bit; f.id[3] = bit;
+    } else if (az > ax && az > ay) {
+        if (zd < 0) { hl = -hl; bit = 0; } else bit = 4;
+        f.v[0] = xform(f3{hw, hh, hl}, o, p); f.v[1] = xform(f3{-hw, hh, hl}, o, p);
+        f.v[2] = xform(f3{-hw, -hh, hl}, o, p); f.v[3] = xform(f3{hw, -hh, hl}, o, p);
+        f.n = zd < 0 ? f3{-o.m31, -o.m32, -o.m33} : f3{o.m31, o.m32, o.m33};
+        f.width = hw * 2; f.height = hh * 2;
+        f.id[0] = 1 + 2 + bit; f.id[1] = 2 + bit; f.id[2] = 1 + bit; f.id[3] = bit;
+    }
+}
+
+struct ClipEdge {
+    f3 a, b, perp;
+    int id;
+};
+
+// BoxFace.GetEdge (BoxBoxCollider.cs:6220-6268)
+__device__ ClipEdge clip_edge(const Face& f, int i) {
+    ClipEdge e;
+    int i1 = (i + 1) & 3, i2 = (i + 2) & 3;
+    e.a = f.v[i];
+    e.b = f.v[i1];
+    e.id = edge_id(f.id[i], f.id[i1]);
+    f3 dir = normalize3(e.b - e.a);
+    e.perp = cross3(dir, f.n);
+    if (dot3(e.perp, f.v[i2] - e.a) > 0) e.perp = -e.perp;
+    return e;
+}
+
+// ComputeIntersection (BoxBoxCollider.cs:6007-6040)
+__device__ bool intersect_edge(f3 a1, f3 a2, const ClipEdge& ce, f3* out) {
+    f3 off = ce.a - a1;
+    f3 dir = a2 - a1;
+    float t = dot3(off, ce.perp) / dot3(dir, ce.perp);
+    if (t < 0 || t > 1) return false;
+    *out = dir * t + a1;
+    off = *out - ce.a;
+    dir = ce.b - ce.a;
+    t = dot3(dir, off);
+    if (t < 0 || t > lensq3(dir)) return false;
+    return true;
+}
+
+// Keeps entries [from, n) of `l` whose depth against the clip face is <= 0,
+// recomputing their depth (the three "Compute depths" passes of ClipFacesDirect).
+__device__ void refilter_depths(LdsList& l, LdsList& tmp, int from, f3 mtd, float clipFaceDot) {
+    tmp.copy_from(l);
+    l.n = from;
+    for (int i = from; i < tmp.n; i++) {
+        float4 it = tmp.P(i);
+        float depth = dot3(f3_of(it), mtd) - clipFaceDot;
+        if (depth <= 0) l.add(make_float4(it.x, it.y, it.z, depth), tmp.I(i));
+    }
+}
+
+// ClipFacesDirect, non-ALLOWUNSAFE (BoxBoxCollider.cs:4517-5278)
+__device__ void clip_faces(const Face& cf, const Face& face, f3 mtd, LdsList& out, LdsList& tmp) {
+    out.n = 0;
+    float icw = 1 / cf.width, ich = 1 / cf.height;
+    float icw2 = icw * icw, ich2 = ich * ich;
+    f3 clipX = (cf.v[3] - cf.v[2]) * icw2;
+    f3 clipY = (cf.v[1] - cf.v[2]) * ich2;
+    float ifw = 1 / face.width, ifh = 1 / face.height;
+    float ifw2 = ifw * ifw, ifh2 = ifh * ifh;
+    f3 faceX = (face.v[3] - face.v[2]) * ifw2;
+    f3 faceY = (face.v[1] - face.v[2]) * ifh2;
+    f3 cc = cf.v[0] + cf.v[2];
+    float ccx = dot3(cc, clipX) * .5f, ccy = dot3(cc, clipY) * .5f;
+    f3 fc = face.v[0] + face.v[2];
+    float fcx = dot3(fc, faceX) * .5f, fcy = dot3(fc, faceY) * .5f;
+    float ex = .5f + .01f * icw, ey = .5f + .01f * ich;
+    float cMaxX = ccx + ex, cMaxY = ccy + ey, cMinX = ccx - ex, cMinY = ccy - ey;
+    ex = .5f + .01f * ifw;
+    ey = .5f + .01f * ifh;
+    float fMaxX = fcx + ex, fMaxY = fcy + ey, fMinX = fcx - ex, fMinY = fcy - ey;
+    // Bit k of each mask: face vertex k (resp. clip vertex k) is inside that bound.
+    unsigned maxX = 0, minX = 0, maxY = 0, minY = 0, cin = 0;
+    for (int i = 0; i < 4; ++i) {
+        float dx = dot3(clipX, face.v[i]);
+        float dy = dot3(clipY, face.v[i]);
+        maxX |= (dx < cMaxX) << i; minX |= (dx > cMinX) << i;
+        maxY |= (dy < cMaxY) << i; minY |= (dy > cMinY) << i;
+        float ex2 = dot3(faceX, cf.v[i]);
+        float ey2 = dot3(faceY, cf.v[i]);
+        cin |= ((ex2 < fMaxX) && (ex2 > fMinX) && (ey2 < fMaxY) && (ey2 > fMinY)) << i;
+    }
+    unsigned fin = maxX & minX & maxY & minY;
+    for (int i = 0; i < 4; ++i)
+        if (fin & (1u << i)) out.add(f4_of(face.v[i], 0.0f), face.id[i]);
+    float clipFaceDot = dot3(cf.v[0], mtd);
+    refilter_depths(out, tmp, 0, mtd, clipFaceDot);
+    int prev = out.n;
+    if (prev >= 4) return;
+    float b = dot3(face.v[0], face.n);
+    for (int i = 0; i < 4; ++i)
+        if (cin & (1u << i)) {
+            float a = dot3(cf.v[i], face.n);
+            out.add(f4_of(cf.v[i] - face.n * (a - b), 0.0f), cf.id[i] + 8);
+        }
+    refilter_depths(out, tmp, prev, mtd, clipFaceDot);
+    prev = out.n;
+    if (prev >= 4) return;
+    // Clip edge k is bounded by: 0 -> maxY, 1 -> minX, 2 -> minY, 3 -> maxX.
+    const unsigned F[4] = {maxY, minX, minY, maxX};
+    for (int k = 0; k < 4; ++k) {
+        ClipEdge ce = clip_edge(cf, k);
+        unsigned fl = F[k];
+        for (int i = 0; i < 4; ++i) {
+            if (fl & (1u << i)) continue;
+            // neighbours of vertex i in the reference's order (vertex 4 on clip edge 2 is reversed)
+            int n1 = (i == 0) ? 1 : (i == 1) ? 0 : (i == 2) ? 1 : (k == 2 ? 2 : 0);
+            int n2 = (i == 0) ? 3 : (i == 1) ? 2 : (i == 2) ? 3 : (k == 2 ? 0 : 2);
+            for (int q = 0; q < 2; ++q) {
+                int j = q == 0 ? n1 : n2;
+                if (!(fl & (1u << j)) || out.n >= 8) continue;
+                int lo = min(i, j), hi = max(i, j);
+                int s = (lo == 0 && hi == 3) ? 3 : lo;
+                int e = (lo == 0 && hi == 3) ? 0 : hi;
+                f3 v;
+                if (intersect_edge(face.v[s], face.v[e], ce, &v))
+                    out.add(f4_of(v, 0.0f), edge_id(face.id[s], face.id[e]) * 2549 + ce.id * 2857);
+            }
+        }
+    }
+    refilter_depths(out, tmp, prev, mtd, clipFaceDot);
+}
+
+// PruneContactsMaxDistance, non-ALLOWUNSAFE (BoxBoxCollider.cs:3612-3741)
+__device__ void prune4(f3 mtd, LdsList& in, LdsList& out) {
+    int n = in.n;
+    int deepest = 0;
+    for (int i = 1; i < n; i++)
+        if (in.P(i).w > in.P(deepest).w) deepest = i;
+    f3 dp = f3_of(in.P(deepest));
+    int furthest = 0;
+    float fd = distsq3(dp, f3_of(in.P(0)));
+    for (int i = 1; i < n; i++) {
+        float d = distsq3(dp, f3_of(in.P(i)));
+        if (d > fd) { fd = d; furthest = i; }
+    }
+    f3 yAxis = cross3(mtd, f3_of(in.P(furthest)) - dp);
+    int mn = 0, mx = 0;
+    float minY = dot3(f3_of(in.P(0)), yAxis), maxY = minY;
+    for (int i = 1; i < n; i++) {
+        float d = dot3(yAxis, f3_of(in.P(i)));
+        if (d < minY) { minY = d; mn = i; }
+        else if (d > maxY) { maxY = d; mx = i; }
+    }
+    int pick[4] = {deepest, furthest, mn, mx};
+    out.n = 0;
+    for (int k = 0; k < 4; ++k) out.add(in.P(pick[k]), in.I(pick[k]));
+}
+
+// Separating-axis tables for the nine edge axes A.i x B.j (BoxBoxCollider.cs:1795-2352):
+// rarb = sum of h[h_k] * |R[p_k]| (left to right), tl = t[tu]*R[q1] - t[tv]*R[q2].
+// Separation is only tested when |R[j][i]| < 1 (the reference reads the transposed entry).
+struct EdgeAxis {
+    int8_t h[4];
+    int8_t p[4];
+    int8_t tu, q1, tv, q2;
+};
+__constant__ EdgeAxis c_edge_axes[9] = {
+    {{1, 2, 4, 5}, {6, 3, 2, 1}, 2, 3, 1, 6},  // A.X x B.X
+    {{1, 2, 3, 5}, {7, 4, 2, 0}, 2, 4, 1, 7},  // A.X x B.Y
+    {{1, 2, 3, 4}, {8, 5, 1, 0}, 2, 5, 1, 8},  // A.X x B.Z
+    {{0, 2, 4, 5}, {6, 0, 5, 4}, 0, 6, 2, 0},  // A.Y x B.X
+    {{0, 2, 3, 5}, {7, 1, 5, 3}, 0, 7, 2, 1},  // A.Y x B.Y
+    {{0, 2, 3, 4}, {8, 2, 4, 3}, 0, 8, 2, 2},  // A.Y x B.Z
+    {{0, 1, 4, 5}, {3, 0, 8, 7}, 1, 0, 0, 3},  // A.Z x B.X
+    {{0, 1, 3, 5}, {4, 1, 8, 6}, 1, 1, 0, 4},  // A.Z x B.Y
+    {{0, 1, 3, 4}, {5, 2, 7, 6}, 1, 2, 0, 5},  // A.Z x B.Z
+};
+
+__device__ __forceinline__ float m3_at(const m3& m, int k) {
+    switch (k) {
+    case 0: return m.m11; case 1: return m.m12; case 2: return m.m13;
+    case 3: return m.m21; case 4: return m.m22; case 5: return m.m23;
+    case 6: return m.m31; case 7: return m.m32; default: return m.m33;
+    }
+}
+
+// BoxBoxCollider.AreBoxesColliding (BoxBoxCollider.cs:1494-2386).  Fills `out`
+// (and uses `tmp`) with up to 4 contacts; returns whether the boxes collide.
+__device__ bool box_box(f3 ha, f3 hb, f3 pA, q4 qA, f3 pB, q4 qB, f3* axis_out, LdsList& out, LdsList& tmp) {
+    m3 aO = mat_from_quat(qA);
+    m3 bO = mat_from_quat(qB);
+    f3 t = pB - pA;
+    float minD = -3.402823466e+38f;
+    f3 minAxis{0, 0, 0};
+    int feature = 2;
+    m3 R, aR;
+    R.m11 = aO.m11 * bO.m11 + aO.m12 * bO.m12 + aO.m13 * bO.m13;
+    R.m12 = aO.m11 * bO.m21 + aO.m12 * bO.m22 + aO.m13 * bO.m23;
+    R.m13 = aO.m11 * bO.m31 + aO.m12 * bO.m32 + aO.m13 * bO.m33;
+    aR.m11 = fabsf(R.m11) + kEps; aR.m12 = fabsf(R.m12) + kEps; aR.m13 = fabsf(R.m13) + kEps;
+    float tX = t.x;
+    t.x = t.x * aO.m11 + t.y * aO.m12 + t.z * aO.m13;
+    // Face test: returns false on separation, else folds the candidate in.
+    auto face = [&](float tv, float rarb, f3 r, int feat) -> bool {
+        if (tv > rarb || tv < -rarb) return false;
+        float td;
+        f3 ax;
+        if (tv > 0) { td = tv - rarb; ax = -r; }
+        else { td = -tv - rarb; ax = r; }
+        if (td > minD) { minD = td; minAxis = ax; feature = feat; }
+        return true;
+    };
+    if (!face(t.x, ha.x + hb.x * aR.m11 + hb.y * aR.m12 + hb.z * aR.m13, mat_row(aO, 0), 0)) return false;
+    R.m21 = aO.m21 * bO.m11 + aO.m22 * bO.m12 + aO.m23 * bO.m13;
+    R.m22 = aO.m21 * bO.m21 + aO.m22 * bO.m22 + aO.m23 * bO.m23;
+    R.m23 = aO.m21 * bO.m31 + aO.m22 * bO.m32 + aO.m23 * bO.m33;
+    aR.m21 = fabsf(R.m21) + kEps; aR.m22 = fabsf(R.m22) + kEps; aR.m23 = fabsf(R.m23) + kEps;
+    float tY = t.y;
+    t.y = tX * aO.m21 + t.y * aO.m22 + t.z * aO.m23;
+    if (!face(t.y, ha.y + hb.x * aR.m21 + hb.y * aR.m22 + hb.z * aR.m23, mat_row(aO, 1), 0)) return false;
+    R.m31 = aO.m31 * bO.m11 + aO.m32 * bO.m12 + aO.m33 * bO.m13;
+    R.m32 = aO.m31 * bO.m21 + aO.m32 * bO.m22 + aO.m33 * bO.m23;
+    R.m33 = aO.m31 * bO.m31 + aO.m32 * bO.m32 + aO.m33 * bO.m33;
+    aR.m31 = fabsf(R.m31) + kEps; aR.m32 = fabsf(R.m32) + kEps; aR.m33 = fabsf(R.m33) + kEps;
+    t.z = tX * aO.m31 + tY * aO.m32 + t.z * aO.m33;
+    if (!face(t.z, ha.z + hb.x * aR.m31 + hb.y * aR.m32 + hb.z * aR.m33, mat_row(aO, 2), 0)) return false;
+    const float antiB = .01f;
+    minD += antiB;
+    if (!face(t.x * R.m11 + t.y * R.m21 + t.z * R.m31, hb.x + ha.x * aR.m11 + ha.y * aR.m21 + ha.z * aR.m31,
+              mat_row(bO, 0), 1)) return false;
+    if (!face(t.x * R.m12 + t.y * R.m22 + t.z * R.m32, hb.y + ha.x * aR.m12 + ha.y * aR.m22 + ha.z * aR.m32,
+              mat_row(bO, 1), 1)) return false;
+    if (!face(t.x * R.m13 + t.y * R.m23 + t.z * R.m33, hb.z + ha.x * aR.m13 + ha.y * aR.m23 + ha.z * aR.m33,
+              mat_row(bO, 2), 1)) return false;
+    if (feature != 1) minD -= antiB;
+    const float antiEdge = .01f;
+    minD += antiEdge;
+    const float hs[6] = {ha.x, ha.y, ha.z, hb.x, hb.y, hb.z};
+    const float tt[3] = {t.x, t.y, t.z};
+    for (int e = 0; e < 9; ++e) {
+        const EdgeAxis& s = c_edge_axes[e];
+        int i = e / 3, j = e % 3;
+        float rarb = hs[s.h[0]] * m3_at(aR, s.p[0]) + hs[s.h[1]] * m3_at(aR, s.p[1]) +
+                     hs[s.h[2]] * m3_at(aR, s.p[2]) + hs[s.h[3]] * m3_at(aR, s.p[3]);
+        float tl = tt[s.tu] * m3_at(R, s.q1) - tt[s.tv] * m3_at(R, s.q2);
+        if (m3_at(aR, j * 3 + i) < 1 && (tl > rarb || tl < -rarb)) return false;
+        f3 a = mat_row(aO, i), b = mat_row(bO, j);
+        f3 ax;
+        float td;
+        if (tl > 0) {
+            ax = f3{b.y * a.z - b.z * a.y, b.z * a.x - b.x * a.z, b.x * a.y - b.y * a.x};
+            float inv = 1 / len3(ax);
+            td = (tl - rarb) * inv;
+            if (td > minD) { feature = 2; minD = td; minAxis = f3{ax.x * inv, ax.y * inv, ax.z * inv}; }
+        } else {
+            ax = f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+            float inv = 1 / len3(ax);
+            td = (-tl - rarb) * inv;
+            if (td > minD) { feature = 2; minD = td; minAxis = f3{ax.x * inv, ax.y * inv, ax.z * inv}; }
+        }
+    }
+    if (feature == 2) {
+        edge_edge(ha, hb, pA, aO, pB, bO, minAxis, out);
+    } else {
+        Face fa, fb;
+        nearest_face(pA, aO, -minAxis, ha, fa);
+        nearest_face(pB, bO, minAxis, hb, fb);
+        if (feature == 0) clip_faces(fa, fb, -minAxis, out, tmp);
+        else clip_faces(fb, fa, minAxis, out, tmp);
+        if (out.n > 4) {
+            tmp.copy_from(out);
+            prune4(minAxis, tmp, out);
+        }
+    }
+    *axis_out = minAxis;
+    return true;
+}
+
+// SphereTester.AreSpheresColliding (SphereTester.cs:23-56)
+__device__ bool sphere_sphere(float ra, float rb, f3 pa, f3 pb, float maxd, f3* pos, f3* nrm, float* depth) {
+    float rs = ra + rb;
+    f3 d = pb - pa;
+    float cd = lensq3(d);
+    if (!(cd < (rs + maxd) * (rs + maxd))) return false;
+    *pos = (rs > kEps ? d * (ra / rs) : f3{0, 0, 0}) + pa;
+    cd = sqrtf(cd);
+    *nrm = cd > kBigEps ? divide3(d, cd) : f3{0, 1, 0};
+    *depth = rs - cd;
+    return true;
+}
+
+// BoxSphereTester.AreShapesColliding (BoxSphereTester.cs:23-85)
+__device__ bool box_sphere(f3 h, f3 bp, q4 bq, float r, f3 sp, float maxd, f3* pos, f3* nrm, float* depth) {
+    f3 lp = qrotate(sp - bp, qconj(bq));
+    f3 lc{clampf_ref(lp.x, -h.x, h.x), clampf_ref(lp.y, -h.y, h.y), clampf_ref(lp.z, -h.z, h.z)};
+    *pos = qrotate(lc, bq) + bp;
+    f3 off = sp - *pos;
+    float ol = lensq3(off);
+    if (ol > (r + maxd) * (r + maxd)) return false;
+    if (ol > kEps) {
+        ol = sqrtf(ol);
+        *nrm = divide3(off, ol);
+        *depth = r - ol;
+    } else {
+        f3 pd{lc.x < 0 ? lc.x + h.x : h.x - lc.x, lc.y < 0 ? lc.y + h.y : h.y - lc.y,
+              lc.z < 0 ? lc.z + h.z : h.z - lc.z};
+        f3 n;
+        if (pd.x < pd.y && pd.x < pd.z) { n = lc.x > 0 ? f3{1, 0, 0} : f3{-1, 0, 0}; *depth = pd.x; }
+        else if (pd.y < pd.z) { n = lc.y > 0 ? f3{0, 1, 0} : f3{0, -1, 0}; *depth = pd.y; }
+        else { n = lc.z > 0 ? f3{0, 0, 1} : f3{0, 0, -1}; *depth = pd.z; }
+        *depth += r;
+        *nrm = qrotate(n, bq);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Manifold / contact-constraint bookkeeping on a PairState held in registers.
+
+__device__ __forceinline__ bool convex_kind(int kind) { return kind == BEPU_PAIR_BOX_BOX || kind == BEPU_PAIR_CONVEX; }
+
+// ConvexContactManifoldConstraint / NonConvexContactManifoldConstraint.AddContact
+__device__ void add_contact(PairState& s, f3 pos, f3 nrm, float depth, int id) {
+    int slot = 0;
+    while (s.flags & (16u << slot)) ++slot;
+    s.flags |= 16u << slot;
+    s.cpos[slot] = f4_of(pos, depth);
+    s.cnrm[slot] = f4_of(nrm, __int_as_float(id));
+    s.clist[s.ccount++] = (int8_t)slot;
+    int k = s.npen++;
+    s.pslot[k] = (int8_t)slot;
+    s.penacc[k] = 0;
+    if (convex_kind(s.kind)) {
+        if (!(s.flags & PF_FRICTION_ACTIVE)) {
+            s.flags |= PF_FRICTION_ACTIVE;
+            for (int q = 0; q < 6; ++q) s.sllin[q] = 0;
+        }
+    } else {
+        s.fracc[k] = 0;
+        s.frlin[k] = make_float4(0, 0, 0, 0);
+    }
+}
+
+// ContactManifold.Remove (FastRemoveAt) -> RemoveContact (ordered RemoveAt of the rows)
+__device__ void remove_contact(PairState& s, int index) {
+    int slot = s.clist[index];
+    s.ccount--;
+    if (index < s.ccount) s.clist[index] = s.clist[s.ccount];
+    s.flags &= ~(16u << slot);
+    int idx = -1;
+    for (int i = 0; i < s.npen; ++i)
+        if (s.pslot[i] == slot) { idx = i; break; }
+    if (idx < 0) return;
+    for (int k = idx; k < s.npen - 1; ++k) {
+        s.pslot[k] = s.pslot[k + 1];
+        s.penacc[k] = s.penacc[k + 1];
+        s.fracc[k] = s.fracc[k + 1];
+        s.frlin[k] = s.frlin[k + 1];
+    }
+    s.npen--;
+    if (convex_kind(s.kind) && s.npen == 0) {
+        s.flags &= ~PF_FRICTION_ACTIVE;
+        s.twacc = 0;
+        s.slacc[0] = s.slacc[1] = 0;
+    }
+}
+
+__device__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m, LdsList& tmp) {
+    float4 PA = w.pos[s.a], PB = w.pos[s.b], QA = w.quat[s.a], QB = w.quat[s.b];
+    float4 SA = w.shape[s.a], SB = w.shape[s.b];
+    f3 axis;
+    m.n = 0;
+    if (box_box(f3_of(SA), f3_of(SB), f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}, f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w},
+                &axis, m, tmp)) {
+        axis = -axis;
+        int rem[4], nrem = 0;
+        for (int i = 0; i < s.ccount; i++) {
+            int slot = s.clist[i];
+            int id = __float_as_int(s.cnrm[slot].w);
+            bool found = false;
+            for (int j = m.n - 1; j >= 0; j--) {
+                if (m.I(j) == id) {
+                    found = true;
+                    float4 c = m.P(j);
+                    s.cpos[slot] = make_float4(c.x, c.y, c.z, -c.w);
+                    s.cnrm[slot] = f4_of(axis, __int_as_float(id));
+                    m.remove_at(j);
+                    break;
+                }
+            }
+            if (!found) rem[nrem++] = i;
+        }
+        for (int i = nrem - 1; i >= 0; i--) remove_contact(s, rem[i]);
+        for (int i = 0; i < m.n; i++) {
+            float4 c = m.P(i);
+            add_contact(s, f3_of(c), axis, -c.w, m.I(i));
+        }
+    } else {
+        for (int i = s.ccount - 1; i >= 0; i--) remove_contact(s, i);
+    }
+}
+
+__device__ void update_single_pair(const DeviceWorld& w, PairState& s) {
+    float4 PA = w.pos[s.a], PB = w.pos[s.b];
+    float4 SA = w.shape[s.a], SB = w.shape[s.b];
+    f3 pos, nrm;
+    float depth;
+    bool hit;
+    if (s.kind == BEPU_PAIR_SPHERE_SPHERE) {
+        hit = sphere_sphere(SA.x, SB.x, f3_of(PA), f3_of(PB), w.s.max_contact_distance, &pos, &nrm, &depth);
+    } else {
+        float4 QA = w.quat[s.a];
+        hit = box_sphere(f3_of(SA), f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}, SB.x, f3_of(PB), w.s.max_contact_distance,
+                         &pos, &nrm, &depth);
+    }
+    bool prev = (s.flags & PF_PREV_COLLIDING) != 0;
+    bool colliding = false;
+    if (hit) {
+        if (!prev && depth >= 0) {
+            add_contact(s, pos, nrm, depth, 0);
+            colliding = true;
+        } else if (prev) {
+            int slot = s.clist[0];
+            s.cpos[slot] = f4_of(pos, depth);
+            s.cnrm[slot] = f4_of(nrm, s.cnrm[slot].w);
+            colliding = true;
+        }
+    } else if (prev) {
+        remove_contact(s, 0);
+    }
+    s.flags = colliding ? (s.flags | PF_PREV_COLLIDING) : (s.flags & ~PF_PREV_COLLIDING);
+}
+
+__device__ void init_pair(const DeviceWorld& w, uint64_t key, PairState& s) {
+    uint32_t i = (uint32_t)(key >> 32), j = (uint32_t)(key & 0xffffffffu);
+    int si = (int)w.shape[i].w, sj = (int)w.shape[j].w;
+    s.key = key;
+    s.ccount = 0;
+    s.npen = 0;
+    s.flags = 0;
+    for (int k = 0; k < 4; ++k) {
+        s.clist[k] = 0; s.pslot[k] = 0; s.penacc[k] = 0; s.fracc[k] = 0;
+        s.cpos[k] = make_float4(0, 0, 0, 0); s.cnrm[k] = make_float4(0, 0, 0, 0); s.frlin[k] = make_float4(0, 0, 0, 0);
+    }
+    s.slacc[0] = s.slacc[1] = 0;
+    s.twacc = 0;
+    s.pad0 = 0;
+    for (int k = 0; k < 6; ++k) s.sllin[k] = 0;
+    s.pad1[0] = s.pad1[1] = 0;
+    s.a = i;
+    s.b = j;
+    if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) s.kind = BEPU_PAIR_BOX_BOX;
+    else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_SPHERE) s.kind = BEPU_PAIR_SPHERE_SPHERE;
+    else if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_SPHERE) s.kind = BEPU_PAIR_BOX_SPHERE;
+    else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_BOX) { s.kind = BEPU_PAIR_BOX_SPHERE; s.a = j; s.b = i; }
+    else s.kind = BEPU_PAIR_CONVEX;
+    // MaterialManager.DefaultMaterialBlender (MaterialManager.cs:77-85)
+    float4 ma = w.mat[s.a], mb = w.mat[s.b];
+    s.bounce = ma.z * mb.z;
+    s.kinetic = ma.y * mb.y;
+    s.static_f = ma.x * mb.x;
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint64_t key, uint32_t cap) { return (uint32_t)mix64(key) & (cap - 1); }
+
+__device__ int hash_find(const uint64_t* keys, const uint32_t* vals, uint32_t cap, uint64_t key) {
+    uint32_t h = hash_slot(key, cap);
+    for (uint32_t probe = 0; probe < cap; ++probe) {
+        uint64_t k = keys[h];
+        if (k == key) return (int)vals[h];
+        if (k == ~0ull) return -1;
+        h = (h + 1) & (cap - 1);
+    }
+    return -1;
+}
+
+__device__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64_t key, uint32_t val) {
+    uint32_t h = hash_slot(key, cap);
+    for (uint32_t probe = 0; probe < cap; ++probe) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&keys[h], ~0ull, (unsigned long long)key);
+        if (prev == ~0ull || prev == key) {
+            vals[h] = val;
+            return true;
+        }
+        h = (h + 1) & (cap - 1);
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
+    __shared__ float4 lds_pd[2][8 * kNpBlock];
+    __shared__ int lds_id[2][8 * kNpBlock];
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t npairs = min(w.counters[CTR_PAIRS], w.pair_cap);
+    if (t >= npairs) return;
+    uint64_t key = w.pair_keys[t];
+    int old = hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
+    PairState s;
+    if (old >= 0) s = w.pairs_old[old];
+    else init_pair(w, key, s);
+    if (s.kind == BEPU_PAIR_BOX_BOX) {
+        LdsList m{&lds_pd[0][threadIdx.x], &lds_id[0][threadIdx.x], 0};
+        LdsList tmp{&lds_pd[1][threadIdx.x], &lds_id[1][threadIdx.x], 0};
+        update_box_pair(w, s, m, tmp);
+    } else {
+        update_single_pair(w, s);
+    }
+    w.pairs_cur[t] = s;
+    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[CTR_OVERFLOW] = 3;
+    if (s.ccount > 0) {
+        uint32_t c = atomicAdd(&w.counters[CTR_CONS], 1u);
+        w.cons[c] = t;
+        atomicAdd(&w.counters[CTR_CONTACTS], (uint32_t)s.ccount);
+        float mp = 0;
+        for (int i = 0; i < s.ccount; ++i) mp = fmaxf(mp, s.cpos[s.clist[i]].w);
+        atomicMax(&w.counters[CTR_MAXPEN], __float_as_uint(mp));
+    }
+}
+
+void launch_narrowphase(const DeviceWorld& w, hipStream_t st) {
+    if (w.pair_cap == 0) return;
+    // The grid covers the pair capacity; threads past this step's pair count exit.
+    hipLaunchKernelGGL(k_narrowphase, dim3((w.pair_cap + kNpBlock - 1) / kNpBlock), dim3(kNpBlock), 0, st, w);
+}
+
+}  // namespace bepu

@@ -1,0 +1,489 @@
+// world.cpp -- C ABI of the gfx950 step library (include/bepu_gpu.h):
+// world lifetime, body registration, the per-step launch sequence and
+// readback of pairs, manifolds, impulses and stats.
+//
+// A step is a fixed sequence of kernel launches on the world's stream with no
+// host synchronisation inside it: every kernel reads this step's counts from
+// device memory, so bepu_step(n) queues n steps back to back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/bepu_gpu.h"
+#include "bodies_host.h"
+#include "world.h"
+
+using namespace bepu;
+
+namespace {
+
+enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_SOLVER, ST_INTEGRATE, ST_COUNT };
+const char* kStageNames = "body_pre,broadphase,narrowphase,solver,integrate";
+
+struct HostBody {
+    float4 pos, quat, linv, angv, ii0, ii1, ii2, li0, li1, li2, shape, stab, mat;
+    float2 damp;
+    uint32_t flags;
+};
+
+template <typename T>
+bepu_status dalloc(T** p, size_t n) {
+    if (n == 0) n = 1;
+    if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
+    if (hipMemset(*p, 0, n * sizeof(T)) != hipSuccess) return BEPU_E_DEVICE;
+    return BEPU_OK;
+}
+
+uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return (uint32_t)p;
+}
+
+}  // namespace
+
+struct bepu_world_s {
+    DeviceWorld d{};
+    hipStream_t stream = nullptr;
+    std::vector<HostBody> staged;   // bodies not yet uploaded
+    uint32_t uploaded = 0;
+    void* sort_temp = nullptr;
+    size_t sort_bytes = 0;
+    void* scan_temp = nullptr;
+    size_t scan_bytes = 0;
+    hipEvent_t ev[ST_COUNT + 1] = {};
+    float stage_ms[ST_COUNT] = {};
+    uint32_t steps = 0;
+    std::vector<void*> allocations;
+};
+
+static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
+
+extern "C" {
+
+void bepu_settings_default(bepu_settings* s) {
+    if (!s) return;
+    s->gravity[0] = 0;
+    s->gravity[1] = -9.81f;
+    s->gravity[2] = 0;
+    s->dt = 1.0f / 60.0f;
+    s->iteration_limit = 10;
+    s->min_iteration_count = 1;
+    s->min_impulse = .001f;
+    s->allowed_penetration = .01f;
+    s->default_margin = .04f;
+    s->max_contact_distance = .1f;
+    s->contact_invalidation_length_sq = .01f;
+    s->contact_min_separation_sq = .0009f;
+    s->bounce_velocity_threshold = 1;
+    s->max_penetration_recovery_speed = 2;
+    s->penetration_recovery_stiffness = .2f;
+    s->static_friction_velocity_threshold = .2f;
+    s->twist_friction_factor = 1;
+    s->softness = .001f;
+    s->use_stabilization = 1;
+    s->velocity_lower_limit = .26f;
+    s->low_velocity_time_minimum = 1;
+    s->linear_damping = .03f;
+    s->angular_damping = .15f;
+}
+
+int32_t bepu_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0 ? 1 : 0;
+}
+
+const char* bepu_status_string(bepu_status s) {
+    switch (s) {
+    case BEPU_OK: return "ok";
+    case BEPU_E_INVALID_ARG: return "invalid argument";
+    case BEPU_E_ALREADY_ADDED: return "already added";
+    case BEPU_E_NOT_FOUND: return "not found";
+    case BEPU_E_OUT_OF_MEMORY: return "out of device memory";
+    case BEPU_E_DEVICE: return "device error";
+    case BEPU_E_NONFINITE: return "non-finite state";
+    case BEPU_E_CAPACITY: return "capacity exceeded";
+    case BEPU_E_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown";
+}
+
+bepu_status bepu_world_destroy(bepu_world w) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    for (void* p : w->allocations) (void)hipFree(p);
+    for (auto& e : w->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (w->stream) (void)hipStreamDestroy(w->stream);
+    delete w;
+    return BEPU_OK;
+}
+
+bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_bodies, bepu_world* out) {
+    if (!out || capacity_bodies == 0) return BEPU_E_INVALID_ARG;
+    *out = nullptr;
+    if (!bepu_device_available()) return BEPU_E_DEVICE;
+    bepu_world w = new bepu_world_s();
+    if (settings) w->d.s = *settings;
+    else bepu_settings_default(&w->d.s);
+    if (w->d.s.min_iteration_count > 6 || w->d.s.iteration_limit < 0 || !(w->d.s.dt > 0)) {
+        delete w;
+        return BEPU_E_INVALID_ARG;
+    }
+    DeviceWorld& d = w->d;
+    uint32_t n = capacity_bodies;
+    d.body_cap = n;
+    d.pair_cap = std::max<uint32_t>(64, 8 * n);
+    d.hash_cap = next_pow2(2ull * d.pair_cap);
+    bepu_status st = BEPU_OK;
+    auto A = [&](auto** p, size_t cnt) {
+        if (st != BEPU_OK) return;
+        st = dalloc(p, cnt);
+        if (st == BEPU_OK) w->allocations.push_back((void*)*p);
+    };
+    A(&d.pos, n); A(&d.quat, n); A(&d.linv, n); A(&d.angv, n);
+    A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
+    A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
+    A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
+    A(&d.node_child, n); A(&d.node_parent, 2 * n); A(&d.node_min, n); A(&d.node_max, n); A(&d.node_visit, n);
+    A(&d.bounds, 8);
+    uint32_t P = d.pair_cap;
+    A(&d.pair_keys, P); A(&d.counters, CTR_COUNT);
+    A(&d.pairs_cur, P); A(&d.pairs_old, P);
+    A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
+    A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
+    A(&d.cons, P); A(&d.color, P); A(&d.prio, P);
+    A(&d.inc_count, n + 1); A(&d.inc_offset, n + 1); A(&d.inc_list, 2 * (size_t)P);
+    A(&d.color_count, kMaxColors); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
+    A(&d.ordered, P);
+    SolverRows& r = d.rows;
+    A(&r.bodies, P); A(&r.meta, P); A(&r.pair, P); A(&r.act, P); A(&r.zero, P);
+    A(&r.pen, 12 * (size_t)P); A(&r.penacc, 4 * (size_t)P);
+    A(&r.sl, 9 * (size_t)P); A(&r.slacc, P); A(&r.tw, 3 * (size_t)P); A(&r.twacc, P);
+    A(&r.fr, 12 * (size_t)P); A(&r.fracc, 4 * (size_t)P);
+    if (st != BEPU_OK) {
+        bepu_world_destroy(w);
+        return st;
+    }
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+        bepu_world_destroy(w);
+        return BEPU_E_DEVICE;
+    }
+    for (auto& e : w->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            bepu_world_destroy(w);
+            return BEPU_E_DEVICE;
+        }
+    w->sort_bytes = broadphase_sort_temp_bytes(n);
+    w->scan_bytes = solver_scan_temp_bytes(n + 1);
+    if (hipMalloc(&w->sort_temp, std::max<size_t>(w->sort_bytes, 16)) != hipSuccess ||
+        hipMalloc(&w->scan_temp, std::max<size_t>(w->scan_bytes, 16)) != hipSuccess) {
+        bepu_world_destroy(w);
+        return BEPU_E_OUT_OF_MEMORY;
+    }
+    w->allocations.push_back(w->sort_temp);
+    w->allocations.push_back(w->scan_temp);
+    launch_fill_u64(d.hash_keys_old, ~0ull, d.hash_cap, w->stream);
+    launch_fill_u64(d.hash_keys_cur, ~0ull, d.hash_cap, w->stream);
+    if (hipStreamSynchronize(w->stream) != hipSuccess) {
+        bepu_world_destroy(w);
+        return BEPU_E_DEVICE;
+    }
+    *out = w;
+    return BEPU_OK;
+}
+
+}  // extern "C"
+
+static bool finite_n(const float* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+// Entity construction: Entity.Initialize(collidable, mass) -> BecomeDynamic /
+// BecomeKinematic (EntityBase.cs:632-645, 867-905), Orientation setter
+// normalisation (EntityBase.cs:68-84).
+static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape, f3 dims, const m3& vd, float mass,
+                            const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!w || !s) return BEPU_E_INVALID_ARG;
+    if (std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
+    if (!finite_n(s->position, 3) || !finite_n(s->orientation, 4) || !finite_n(s->linear_velocity, 3) ||
+        !finite_n(s->angular_velocity, 3))
+        return BEPU_E_INVALID_ARG;
+    uint32_t n = w->uploaded + (uint32_t)w->staged.size();
+    if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
+    HostBody b{};
+    bool dyn = mass > 0;
+    m3 li{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float im = 0;
+    if (dyn) {
+        li = adaptive_invert(scale3(vd, mass * 2.5f));  // InertiaHelper.InertiaTensorScale = 2.5
+        im = 1 / mass;
+    }
+    q4 q = qnormalize(q4{s->orientation[0], s->orientation[1], s->orientation[2], s->orientation[3]});
+    m3 R = mat_from_quat(q);
+    m3 I = rotate_inertia(R, li);
+    b.pos = make_float4(s->position[0], s->position[1], s->position[2], 0);
+    b.quat = make_float4(q.x, q.y, q.z, q.w);
+    b.linv = make_float4(s->linear_velocity[0], s->linear_velocity[1], s->linear_velocity[2], 0);
+    b.angv = make_float4(s->angular_velocity[0], s->angular_velocity[1], s->angular_velocity[2], 0);
+    b.ii0 = make_float4(I.m11, I.m12, I.m13, im);
+    b.ii1 = make_float4(I.m21, I.m22, I.m23, 0);
+    b.ii2 = make_float4(I.m31, I.m32, I.m33, 0);
+    b.li0 = make_float4(li.m11, li.m12, li.m13, 0);
+    b.li1 = make_float4(li.m21, li.m22, li.m23, 0);
+    b.li2 = make_float4(li.m31, li.m32, li.m33, 0);
+    b.shape = make_float4(dims.x, dims.y, dims.z, (float)shape);
+    b.stab = make_float4(0, 0, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
+    b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
+    b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
+    b.flags = dyn ? 1u : 0u;
+    w->staged.push_back(b);
+    if (id) *id = n;
+    return BEPU_OK;
+}
+
+static bepu_status upload(bepu_world w) {
+    if (w->staged.empty()) return BEPU_OK;
+    size_t k = w->staged.size();
+    uint32_t o = w->uploaded;
+    std::vector<float4> tmp(k);
+    std::vector<float2> tmp2(k);
+    std::vector<uint32_t> tmpu(k);
+    DeviceWorld& d = w->d;
+    auto put4 = [&](float4* dst, float4 HostBody::*f) -> bool {
+        for (size_t i = 0; i < k; ++i) tmp[i] = w->staged[i].*f;
+        return hipMemcpy(dst + o, tmp.data(), k * sizeof(float4), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = put4(d.pos, &HostBody::pos) && put4(d.quat, &HostBody::quat) && put4(d.linv, &HostBody::linv) &&
+              put4(d.angv, &HostBody::angv) && put4(d.ii0, &HostBody::ii0) && put4(d.ii1, &HostBody::ii1) &&
+              put4(d.ii2, &HostBody::ii2) && put4(d.li0, &HostBody::li0) && put4(d.li1, &HostBody::li1) &&
+              put4(d.li2, &HostBody::li2) && put4(d.shape, &HostBody::shape) && put4(d.stab, &HostBody::stab) &&
+              put4(d.mat, &HostBody::mat);
+    for (size_t i = 0; i < k; ++i) tmp2[i] = w->staged[i].damp;
+    ok = ok && hipMemcpy(d.damp + o, tmp2.data(), k * sizeof(float2), hipMemcpyHostToDevice) == hipSuccess;
+    for (size_t i = 0; i < k; ++i) tmpu[i] = w->staged[i].flags;
+    ok = ok && hipMemcpy(d.bflags + o, tmpu.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) return BEPU_E_DEVICE;
+    w->uploaded += (uint32_t)k;
+    d.nbodies = w->uploaded;
+    w->staged.clear();
+    return BEPU_OK;
+}
+
+extern "C" {
+
+bepu_status bepu_add_box(bepu_world w, const bepu_motion_state* s, float width, float height, float length,
+                         float mass, const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!(width > 0) || !(height > 0) || !(length > 0)) return BEPU_E_INVALID_ARG;
+    // BoxShape(width, height, length): half extents (BoxShape.cs:56-62)
+    return add_body(w, s, BEPU_SHAPE_BOX, f3{width * 0.5f, height * 0.5f, length * 0.5f},
+                    box_volume_distribution(width, height, length), mass, m, flags, id);
+}
+
+bepu_status bepu_add_sphere(bepu_world w, const bepu_motion_state* s, float radius, float mass,
+                            const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!(radius > 0)) return BEPU_E_INVALID_ARG;
+    return add_body(w, s, BEPU_SHAPE_SPHERE, f3{radius, 0, 0}, sphere_volume_distribution(radius), mass, m, flags, id);
+}
+
+bepu_status bepu_step(bepu_world w, int32_t n_steps) {
+    if (!w || n_steps < 0) return BEPU_E_INVALID_ARG;
+    bepu_status st = upload(w);
+    if (st != BEPU_OK) return st;
+    DeviceWorld& d = w->d;
+    hipStream_t s = w->stream;
+    for (int k = 0; k < n_steps; ++k) {
+        bool timed = (k == n_steps - 1);
+        launch_fill_u32(d.counters, 0u, CTR_COUNT, s);
+        launch_fill_u64(d.hash_keys_cur, ~0ull, d.hash_cap, s);
+        if (timed) (void)hipEventRecord(w->ev[0], s);
+        launch_body_pre(d, s);
+        if (timed) (void)hipEventRecord(w->ev[1], s);
+        launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
+        if (timed) (void)hipEventRecord(w->ev[2], s);
+        launch_narrowphase(d, s);
+        if (timed) (void)hipEventRecord(w->ev[3], s);
+        launch_solver_full(d, w->scan_temp, w->scan_bytes, s);
+        if (timed) (void)hipEventRecord(w->ev[4], s);
+        launch_integrate(d, s);
+        if (timed) (void)hipEventRecord(w->ev[5], s);
+        // This step's pair states and table become next step's history.
+        std::swap(d.pairs_cur, d.pairs_old);
+        std::swap(d.hash_keys_cur, d.hash_keys_old);
+        std::swap(d.hash_vals_cur, d.hash_vals_old);
+        w->steps++;
+    }
+    if (hipGetLastError() != hipSuccess) return BEPU_E_DEVICE;
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_stage_times(bepu_world w, float* ms, uint32_t cap, uint32_t* count, const char** names) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    if (count) *count = ST_COUNT;
+    if (names) *names = kStageNames;
+    for (uint32_t i = 0; i < ST_COUNT && i < cap; ++i) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, w->ev[i], w->ev[i + 1]) != hipSuccess) t = -1;
+        ms[i] = t;
+    }
+    return BEPU_OK;
+}
+
+static bepu_status read_counters(bepu_world w, uint32_t* c) {
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    if (hipMemcpy(c, w->d.counters, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return BEPU_E_DEVICE;
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_motion_states(bepu_world w, bepu_motion_state* out, uint32_t cap, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    uint32_t n = w->d.nbodies;
+    if (count) *count = n;
+    if (!out || cap == 0) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
+    uint32_t k = std::min(n, cap);
+    std::vector<float4> p(k), q(k), v(k), a(k);
+    bool ok = hipMemcpy(p.data(), w->d.pos, k * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(q.data(), w->d.quat, k * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(v.data(), w->d.linv, k * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(a.data(), w->d.angv, k * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return BEPU_E_DEVICE;
+    for (uint32_t i = 0; i < k; ++i) {
+        bepu_motion_state& m = out[i];
+        m.position[0] = p[i].x; m.position[1] = p[i].y; m.position[2] = p[i].z;
+        m.orientation[0] = q[i].x; m.orientation[1] = q[i].y; m.orientation[2] = q[i].z; m.orientation[3] = q[i].w;
+        m.linear_velocity[0] = v[i].x; m.linear_velocity[1] = v[i].y; m.linear_velocity[2] = v[i].z;
+        m.angular_velocity[0] = a[i].x; m.angular_velocity[1] = a[i].y; m.angular_velocity[2] = a[i].z;
+    }
+    return n <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+}
+
+bepu_status bepu_get_pairs(bepu_world w, uint64_t* keys, uint32_t cap, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    uint32_t c[CTR_COUNT];
+    if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
+    if (c[CTR_OVERFLOW]) return BEPU_E_CAPACITY;
+    uint32_t n = std::min(c[CTR_PAIRS], w->d.pair_cap);
+    if (count) *count = n;
+    if (!keys || cap < n) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
+    if (n && hipMemcpy(keys, w->d.pair_keys, n * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return BEPU_E_DEVICE;
+    std::sort(keys, keys + n);
+    return BEPU_OK;
+}
+
+// The last step's pair states sit in pairs_old after the end-of-step swap.
+static bepu_status read_pairs(bepu_world w, std::vector<PairState>& ps, uint32_t* c) {
+    if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
+    uint32_t n = w->steps ? std::min(c[CTR_PAIRS], w->d.pair_cap) : 0;
+    ps.resize(n);
+    if (n && hipMemcpy(ps.data(), w->d.pairs_old, n * sizeof(PairState), hipMemcpyDeviceToHost) != hipSuccess)
+        return BEPU_E_DEVICE;
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_manifolds(bepu_world w, bepu_manifold* out, uint32_t cap, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    std::vector<PairState> ps;
+    uint32_t c[CTR_COUNT];
+    if (read_pairs(w, ps, c) != BEPU_OK) return BEPU_E_DEVICE;
+    std::vector<const PairState*> live;
+    for (const PairState& s : ps)
+        if (s.ccount > 0) live.push_back(&s);
+    std::sort(live.begin(), live.end(), [](const PairState* a, const PairState* b) { return a->key < b->key; });
+    uint32_t n = (uint32_t)live.size();
+    if (count) *count = n;
+    if (!out || cap < n) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
+    for (uint32_t i = 0; i < n; ++i) {
+        const PairState& s = *live[i];
+        bepu_manifold& m = out[i];
+        std::memset(&m, 0, sizeof(m));
+        m.key = s.key;
+        m.body_a = s.a;
+        m.body_b = s.b;
+        m.count = s.ccount;
+        m.kind = s.kind;
+        for (int k = 0; k < s.ccount; ++k) {
+            int slot = s.clist[k];
+            bepu_contact& o = m.contacts[k];
+            o.position[0] = s.cpos[slot].x; o.position[1] = s.cpos[slot].y; o.position[2] = s.cpos[slot].z;
+            o.penetration_depth = s.cpos[slot].w;
+            o.normal[0] = s.cnrm[slot].x; o.normal[1] = s.cnrm[slot].y; o.normal[2] = s.cnrm[slot].z;
+            float idf = s.cnrm[slot].w;
+            std::memcpy(&o.id, &idf, 4);
+        }
+    }
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_impulses(bepu_world w, bepu_impulses* out, uint32_t cap, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    std::vector<PairState> ps;
+    uint32_t c[CTR_COUNT];
+    if (read_pairs(w, ps, c) != BEPU_OK) return BEPU_E_DEVICE;
+    uint32_t m = w->steps ? std::min(c[CTR_CONS], w->d.pair_cap) : 0;
+    if (count) *count = m;
+    if (!out || cap < m) return m == 0 ? BEPU_OK : BEPU_E_CAPACITY;
+    std::vector<int> pair(m);
+    std::vector<uint32_t> start(kMaxColors + 1);
+    if (m && (hipMemcpy(pair.data(), w->d.rows.pair, m * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(start.data(), w->d.color_start, (kMaxColors + 1) * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost) != hipSuccess))
+        return BEPU_E_DEVICE;
+    std::vector<bepu_impulses> v(m);
+    int col = 0;
+    for (uint32_t p = 0; p < m; ++p) {
+        while (col < kMaxColors && p >= start[col + 1]) ++col;
+        const PairState& s = ps[pair[p]];
+        bepu_impulses& o = v[p];
+        std::memset(&o, 0, sizeof(o));
+        o.key = s.key;
+        o.color = col;
+        o.count = s.npen;
+        for (int k = 0; k < s.npen; ++k) o.normal[k] = s.penacc[k];
+        bool convex = s.kind == BEPU_PAIR_BOX_BOX || s.kind == BEPU_PAIR_CONVEX;
+        if (convex) {
+            o.sliding[0] = s.slacc[0];
+            o.sliding[1] = s.slacc[1];
+            o.twist = s.twacc;
+        } else {
+            for (int k = 0; k < s.npen; ++k) o.friction[k] = s.fracc[k];
+        }
+    }
+    std::sort(v.begin(), v.end(), [](const bepu_impulses& a, const bepu_impulses& b) {
+        return a.color != b.color ? a.color < b.color : a.key < b.key;
+    });
+    std::copy(v.begin(), v.end(), out);
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
+    if (!w || !out) return BEPU_E_INVALID_ARG;
+    uint32_t c[CTR_COUNT];
+    if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
+    std::memset(out, 0, sizeof(*out));
+    out->bodies = w->d.nbodies;
+    bool any = w->steps > 0;
+    out->pairs = any ? std::min(c[CTR_PAIRS], w->d.pair_cap) : 0;
+    out->manifolds = any ? c[CTR_CONS] : 0;
+    out->contacts = any ? c[CTR_CONTACTS] : 0;
+    out->colors = any ? c[CTR_COLORS] : 0;
+    out->steps = w->steps;
+    float mp;
+    std::memcpy(&mp, &c[CTR_MAXPEN], 4);
+    out->max_penetration = any ? mp : 0.0f;
+    return c[CTR_OVERFLOW] ? BEPU_E_CAPACITY : BEPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+"""Host-side mirror of the reference's Space / Entity / SolverSettings API for
+the step path, driving the gfx950 library through its C ABI.
+
+Names and argument meaning follow BEPUphysics v1 (paths relative to the
+reference repository root):
+  Space()                      BEPUphysics/Space.cs:199-233
+  Space.Add(entity)            Space.cs:241-358  (ArgumentException on a double add -> ValueError)
+  Space.Update()               Space.cs:527-531
+  Box(position, w, h, l, mass) Entities/Prefabs/Box.cs:33-37 (mass omitted: kinematic)
+  Sphere(position, r, mass)    Entities/Prefabs/Sphere.cs:47-51
+  Entity.Position/Orientation/LinearVelocity/AngularVelocity, Material, IsAlwaysActive
+  Space.Solver.IterationLimit  Constraints/Solver.cs:15
+  Space.ForceUpdater.Gravity   OtherSpaceStages/ForceUpdater.cs
+  Space.TimeStepSettings.TimeStepDuration  TimeStepSettings.cs:19
+Entities added before the first Update are uploaded to the device in one
+batch; state lives on the device afterwards and entity properties read it back.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import BepuError, check
+
+SHAPE_BOX, SHAPE_SPHERE = 0, 1
+
+
+class Material:
+    """BEPUphysics/Materials/Material.cs (defaults 1.0 / 0.8 / 0.0)."""
+
+    def __init__(self, static_friction=1.0, kinetic_friction=0.8, bounciness=0.0):
+        self.StaticFriction = float(static_friction)
+        self.KineticFriction = float(kinetic_friction)
+        self.Bounciness = float(bounciness)
+
+
+class Entity:
+    shape = None
+
+    def __init__(self, position=(0, 0, 0), mass=None):
+        self._space = None
+        self._id = None
+        self._pending = {
+            "position": tuple(float(x) for x in position),
+            "orientation": (0.0, 0.0, 0.0, 1.0),
+            "linear_velocity": (0.0, 0.0, 0.0),
+            "angular_velocity": (0.0, 0.0, 0.0),
+        }
+        if mass is not None:
+            m = float(mass)
+            if m != m or m in (float("inf"), float("-inf")):
+                # EntityBase.BecomeDynamic throws InvalidOperationException (EntityBase.cs:879-880)
+                raise ValueError("Cannot use a mass of %r for a dynamic entity." % mass)
+        self.mass = 0.0 if mass is None else float(mass)
+        self.Material = Material()
+        self.IsAlwaysActive = True
+        self.AllowStabilization = True
+
+    @property
+    def IsDynamic(self):
+        return self.mass > 0
+
+    @property
+    def Space(self):
+        return self._space
+
+    @property
+    def Id(self):
+        return self._id
+
+    def _state(self):
+        if self._space is None or self._id is None or not self._space._uploaded(self._id):
+            return None
+        return self._space.motion_states()[self._id]
+
+    def _get(self, key, sl):
+        st = self._state()
+        return tuple(float(x) for x in st[sl]) if st is not None else self._pending[key]
+
+    def _set(self, key, value):
+        if self._space is not None and self._space._uploaded(self._id):
+            raise NotImplementedError("changing the state of an entity already on the device is not supported")
+        self._pending[key] = tuple(float(x) for x in value)
+
+    Position = property(lambda s: s._get("position", slice(0, 3)), lambda s, v: s._set("position", v))
+    Orientation = property(lambda s: s._get("orientation", slice(3, 7)), lambda s, v: s._set("orientation", v))
+    LinearVelocity = property(lambda s: s._get("linear_velocity", slice(7, 10)),
+                              lambda s, v: s._set("linear_velocity", v))
+    AngularVelocity = property(lambda s: s._get("angular_velocity", slice(10, 13)),
+                               lambda s, v: s._set("angular_velocity", v))
+
+    def _motion_state(self):
+        ms = _lib.MotionState()
+        ms.position[:] = self._pending["position"]
+        ms.orientation[:] = self._pending["orientation"]
+        ms.linear_velocity[:] = self._pending["linear_velocity"]
+        ms.angular_velocity[:] = self._pending["angular_velocity"]
+        return ms
+
+    def _flags(self):
+        f = 1 if self.IsAlwaysActive else 0
+        if not self.AllowStabilization:
+            f |= 2
+        return f
+
+    def _material(self):
+        m = self.Material
+        return _lib.Material(m.StaticFriction, m.KineticFriction, m.Bounciness)
+
+
+class Box(Entity):
+    """Entities/Prefabs/Box.cs: width/height/length are full extents."""
+    shape = SHAPE_BOX
+
+    def __init__(self, position, width, height, length, mass=None):
+        super().__init__(position, mass)
+        self.Width, self.Height, self.Length = float(width), float(height), float(length)
+
+    def _add(self, lib, h, out):
+        return lib.bepu_add_box(h, ctypes.byref(self._motion_state()), self.Width, self.Height, self.Length,
+                                self.mass, ctypes.byref(self._material()), self._flags(), ctypes.byref(out))
+
+
+class Sphere(Entity):
+    """Entities/Prefabs/Sphere.cs."""
+    shape = SHAPE_SPHERE
+
+    def __init__(self, position, radius, mass=None):
+        super().__init__(position, mass)
+        self.Radius = float(radius)
+
+    def _add(self, lib, h, out):
+        return lib.bepu_add_sphere(h, ctypes.byref(self._motion_state()), self.Radius, self.mass,
+                                   ctypes.byref(self._material()), self._flags(), ctypes.byref(out))
+
+
+class _Solver:
+    def __init__(self, space):
+        self._space = space
+
+    @property
+    def IterationLimit(self):
+        return self._space._settings.iteration_limit
+
+    @IterationLimit.setter
+    def IterationLimit(self, v):
+        self._space._configure("iteration_limit", max(int(v), 0))
+
+
+class _ForceUpdater:
+    def __init__(self, space):
+        self._space = space
+
+    @property
+    def Gravity(self):
+        return tuple(self._space._settings.gravity)
+
+    @Gravity.setter
+    def Gravity(self, g):
+        self._space._configure("gravity", tuple(float(x) for x in g))
+
+
+class _TimeStepSettings:
+    def __init__(self, space):
+        self._space = space
+
+    @property
+    def TimeStepDuration(self):
+        return self._space._settings.dt
+
+    @TimeStepDuration.setter
+    def TimeStepDuration(self, v):
+        self._space._configure("dt", float(v))
+
+
+class Space:
+    """A simulation world whose step runs on the GPU."""
+
+    def __init__(self, capacity=1024, settings=None):
+        self._lib = _lib.load()
+        self._settings = settings if settings is not None else _lib.default_settings()
+        self._capacity = int(capacity)
+        self._h = None
+        self._entities = []
+        self._n_uploaded = 0
+        self._ms_cache = None
+        self.Solver = _Solver(self)
+        self.ForceUpdater = _ForceUpdater(self)
+        self.TimeStepSettings = _TimeStepSettings(self)
+
+    # -- lifetime ----------------------------------------------------------
+    def _handle(self):
+        if self._h is None:
+            h = ctypes.c_void_p()
+            check(self._lib.bepu_world_create(ctypes.byref(self._settings), self._capacity, ctypes.byref(h)),
+                  "bepu_world_create")
+            self._h = h
+        return self._h
+
+    def _configure(self, field, value):
+        if self._h is not None:
+            raise NotImplementedError("settings are fixed once the world exists on the device")
+        setattr(self._settings, field, value)
+
+    def close(self):
+        if self._h is not None:
+            self._lib.bepu_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- Space.Add / Update -------------------------------------------------
+    def Add(self, entity):
+        if entity._space is not None:
+            raise ValueError("The object belongs to a Space already; cannot add it again.")
+        out = ctypes.c_uint32()
+        st = entity._add(self._lib, self._handle(), out)
+        if st != 0:
+            raise BepuError(st, "Space.Add")
+        entity._space = self
+        entity._id = out.value
+        self._entities.append(entity)
+        self._ms_cache = None
+        return entity
+
+    def Update(self, steps=1):
+        check(self._lib.bepu_step(self._handle(), int(steps)), "Space.Update")
+        self._n_uploaded = len(self._entities)
+        self._ms_cache = None
+
+    add, update = Add, Update
+
+    @property
+    def Entities(self):
+        return list(self._entities)
+
+    def _uploaded(self, idx):
+        return idx is not None and idx < self._n_uploaded
+
+    def add_scene(self, scene):
+        """Adds every row of a bepuphysics1_amd.scenes array; returns the ids."""
+        lib, h = self._lib, self._handle()
+        ids = np.empty(len(scene), np.uint32)
+        ms = _lib.MotionState()
+        mat = _lib.Material()
+        out = ctypes.c_uint32()
+        for i in range(len(scene)):
+            r = scene[i]
+            s = r["state"]
+            ms.position[:] = s[0:3].tolist()
+            ms.orientation[:] = s[3:7].tolist()
+            ms.linear_velocity[:] = s[7:10].tolist()
+            ms.angular_velocity[:] = s[10:13].tolist()
+            mat.static_friction, mat.kinetic_friction, mat.bounciness = [float(x) for x in r["material"]]
+            d = r["dims"]
+            if int(r["shape"]) == SHAPE_BOX:
+                st = lib.bepu_add_box(h, ctypes.byref(ms), float(d[0]), float(d[1]), float(d[2]), float(r["mass"]),
+                                      ctypes.byref(mat), int(r["flags"]), ctypes.byref(out))
+            elif int(r["shape"]) == SHAPE_SPHERE:
+                st = lib.bepu_add_sphere(h, ctypes.byref(ms), float(d[0]), float(r["mass"]), ctypes.byref(mat),
+                                         int(r["flags"]), ctypes.byref(out))
+            else:
+                raise NotImplementedError("shape %d" % int(r["shape"]))
+            if st != 0:
+                raise BepuError(st, "Space.Add")
+            ids[i] = out.value
+            self._entities.append(None)
+        self._ms_cache = None
+        return ids
+
+    # -- readback ---------------------------------------------------------
+    def _count(self, fn):
+        n = ctypes.c_uint32()
+        fn(self._handle(), None, 0, ctypes.byref(n))
+        return n.value
+
+    def motion_states(self):
+        """MotionState of every body as an [N, 13] float32 array (position, orientation xyzw, v, w)."""
+        if self._ms_cache is None:
+            n = self._count(self._lib.bepu_get_motion_states)
+            buf = (_lib.MotionState * max(n, 1))()
+            check(self._lib.bepu_get_motion_states(self._handle(), buf, n, ctypes.byref(ctypes.c_uint32())),
+                  "get_motion_states")
+            self._ms_cache = np.frombuffer(buf, np.float32).reshape(-1, 13)[:n].copy()
+        return self._ms_cache
+
+    def pairs(self):
+        """Sorted pair keys of the last step's broadphase overlaps."""
+        n = self._count(self._lib.bepu_get_pairs)
+        out = np.zeros(max(n, 1), np.uint64)
+        check(self._lib.bepu_get_pairs(self._handle(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n,
+                                       ctypes.byref(ctypes.c_uint32())), "get_pairs")
+        return out[:n]
+
+    def manifolds(self):
+        n = self._count(self._lib.bepu_get_manifolds)
+        buf = (_lib.Manifold * max(n, 1))()
+        check(self._lib.bepu_get_manifolds(self._handle(), buf, n, ctypes.byref(ctypes.c_uint32())),
+              "get_manifolds")
+        return list(buf)[:n]
+
+    def impulses(self):
+        n = self._count(self._lib.bepu_get_impulses)
+        buf = (_lib.Impulses * max(n, 1))()
+        check(self._lib.bepu_get_impulses(self._handle(), buf, n, ctypes.byref(ctypes.c_uint32())),
+              "get_impulses")
+        return list(buf)[:n]
+
+    def stats(self):
+        s = _lib.StepStats()
+        check(self._lib.bepu_get_stats(self._handle(), ctypes.byref(s)), "get_stats")
+        return s
+
+    def stage_times(self):
+        ms = (ctypes.c_float * 16)()
+        n = ctypes.c_uint32()
+        names = ctypes.c_char_p()
+        check(self._lib.bepu_get_stage_times(self._handle(), ms, 16, ctypes.byref(n), ctypes.byref(names)),
+              "get_stage_times")
+        return dict(zip(names.value.decode().split(","), list(ms)[:n.value]))

@@ -1,0 +1,51 @@
+"""Shared helpers for the parity tests: run one scene on the GPU library and
+on the oracle, and compare their outputs."""
+import numpy as np
+
+
+def f32bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def run_gpu(scene, steps, settings=None, capacity=None):
+    from bepuphysics1_amd import Space
+    sp = Space(capacity=capacity or max(len(scene), 2), settings=settings)
+    sp.add_scene(scene)
+    if steps:
+        sp.Update(steps)
+    return sp
+
+
+def run_oracle(scene, steps, settings=None):
+    from oracle import refcpu
+    w = refcpu.World(settings)
+    w.add_scene(scene)
+    if steps:
+        w.step(steps)
+    return w
+
+
+def manifold_table(manifolds):
+    """{key: (count, [(id, pos3, normal3, depth) sorted by id])}"""
+    out = {}
+    for m in manifolds:
+        cs = []
+        for i in range(m.count):
+            c = m.contacts[i]
+            cs.append((int(c.id), tuple(c.position), tuple(c.normal), float(c.penetration_depth)))
+        out[int(m.key)] = (m.count, int(m.body_a), int(m.body_b), cs)
+    return out
+
+
+def impulse_table(imps):
+    out = {}
+    for r in imps:
+        out[int(r.key)] = (int(r.color), int(r.count), list(r.normal)[:r.count], list(r.friction)[:r.count],
+                           list(r.sliding), float(r.twist))
+    return out
+
+
+def rel_err(a, b, floor=1e-3):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)

@@ -1,0 +1,112 @@
+"""GPU library vs oracle on identical seeded scenes (run on an MI355X).
+
+Bar (from the task's parity contract):
+  * broadphase pair sets: bit-exact;
+  * contact counts: exact; positions, normals, depths: within 1e-4 relative;
+  * accumulated impulses per step: within 1e-4 relative, constraints
+    compared in the colour order both sides use;
+  * stacks stay stable.
+The library is expected to be bit-identical to the oracle for boxes and
+spheres (same operation order, no FMA contraction, IEEE division/sqrt), so
+most checks below are exact; the tolerance is the stated fallback.
+"""
+import numpy as np
+import pytest
+
+from bepuphysics1_amd import scenes
+from helpers import f32bits, impulse_table, manifold_table, rel_err, run_gpu, run_oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def _compare_manifolds(g, o):
+    mg, mo = manifold_table(g.manifolds()), manifold_table(o.manifolds())
+    assert set(mg) == set(mo), "manifold key sets differ: %d vs %d" % (len(mg), len(mo))
+    worst = 0.0
+    for k in mo:
+        cg, ag, bg, lg = mg[k]
+        co, ao, bo, lo = mo[k]
+        assert cg == co, "contact count differs for pair %x" % k
+        assert (ag, bg) == (ao, bo)
+        ids_g = sorted(c[0] for c in lg)
+        ids_o = sorted(c[0] for c in lo)
+        assert ids_g == ids_o
+        for cgc, coc in zip(sorted(lg), sorted(lo)):
+            worst = max(worst, rel_err(cgc[1], coc[1]).max(), rel_err(cgc[2], coc[2]).max(),
+                        rel_err(cgc[3], coc[3]).max())
+    assert worst <= TOL, worst
+    return worst
+
+
+def _compare_impulses(g, o):
+    ig, io = impulse_table(g.impulses()), impulse_table(o.impulses())
+    assert set(ig) == set(io)
+    worst = 0.0
+    for k in io:
+        col_g, n_g, pen_g, fr_g, sl_g, tw_g = ig[k]
+        col_o, n_o, pen_o, fr_o, sl_o, tw_o = io[k]
+        assert col_g == col_o, "colour differs for %x" % k
+        assert n_g == n_o
+        worst = max(worst, rel_err(pen_g, pen_o).max(initial=0), rel_err(fr_g, fr_o).max(initial=0),
+                    rel_err(sl_g, sl_o).max(), rel_err([tw_g], [tw_o]).max())
+    assert worst <= TOL, worst
+    return worst
+
+
+def test_pyramid_first_step_exact():
+    sc = scenes.pyramid()
+    g, o = run_gpu(sc, 1), run_oracle(sc, 1)
+    assert np.array_equal(g.pairs(), o.pairs())
+    _compare_manifolds(g, o)
+    _compare_impulses(g, o)
+    assert np.array_equal(f32bits(g.motion_states()), f32bits(o.motion_states()))
+
+
+def test_pyramid_300_steps():
+    """C1: 55-box pyramid, 300 steps, 10 iterations, warm starting."""
+    sc = scenes.pyramid()
+    g, o = run_gpu(sc, 0), run_oracle(sc, 0)
+    for chunk in range(6):
+        g.Update(50)
+        o.step(50)
+        assert np.array_equal(g.pairs(), o.pairs()), "pairs differ after %d steps" % (50 * (chunk + 1))
+        _compare_impulses(g, o)
+    msg, mso = g.motion_states(), o.motion_states()
+    assert np.abs(msg - mso).max() < 1e-4
+    st = g.stats()
+    assert st.max_penetration < 0.05
+    # every box still stands where it started (within 5 cm)
+    assert np.abs(msg[1:, 0:3] - sc["state"][1:, 0:3]).max() < 0.2
+
+
+def test_box_sphere_pile_steps():
+    sc = scenes.random_pile(120, seed=3)
+    g, o = run_gpu(sc, 0), run_oracle(sc, 0)
+    for _ in range(4):
+        g.Update(30)
+        o.step(30)
+        assert np.array_equal(g.pairs(), o.pairs())
+    _compare_manifolds(g, o)
+    _compare_impulses(g, o)
+
+
+def test_stress_broadphase_exact():
+    sc = scenes.stress(4096, seed=11)
+    g, o = run_gpu(sc, 1), run_oracle(sc, 1)
+    pg, po = g.pairs(), o.pairs()
+    assert len(po) > 1000
+    assert np.array_equal(pg, po)
+    _compare_manifolds(g, o)
+
+
+def test_towers_small():
+    sc = scenes.towers(nx=8, nz=8, height=20, seed=1)
+    g, o = run_gpu(sc, 0), run_oracle(sc, 0)
+    g.Update(60)
+    o.step(60)
+    assert np.array_equal(g.pairs(), o.pairs())
+    _compare_manifolds(g, o)
+    _compare_impulses(g, o)
+    ms = g.motion_states()
+    assert np.abs(ms[1:, 0] - sc["state"][1:, 0]).max() < 0.1
