@@ -1,0 +1,239 @@
+"""Benchmark of the gfx950 step path: body-steps/s on one MI355X.
+
+Workload (N=1): the large stacking scene -- a 64 x 64 grid of 20-high box
+towers (81,920 dynamic boxes, half extent 0.5, 0.01 seeded jitter) on a
+static ground, dt 1/60, 10 solver iterations, FP32, warm starting on.
+Inputs are resident on the device before the timed region; the timed region
+is K full steps (force, AABB, broadphase, narrowphase, colouring, prestep,
+warm start + iterations, integration) bracketed by device synchronisation.
+
+The JSON line also carries
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average duration (HIP events on the world's stream), against
+                the MI355X HBM peak of 8 TB/s;
+  cpu_baseline  the oracle (this repository's single-threaded C++
+                restatement of the reference path) timed on the host on a
+                bounded sample of the same scene;
+  parity        a small scene stepped on both sides: pair sets and impulses;
+  latency       p50 / p99 of single synchronised steps.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit of work, per stage (FP32, each element counted
+# once per stage, no cache-reuse credit):
+#   body_pre     force/inertia 160 + AABB 80 per body
+#   broadphase   Morton 32 + radix sort 64 + LBVH build 88 + traversal leaf read 24 per body
+#   narrowphase  312 per broadphase pair
+#   color        64 per constraint (pair read 24, incidence 16, colour/priority 16, order 8)
+#   prestep      332 + 72 c per constraint (c = its contact count)
+#   solve        warm start 190 + 70 c, plus per iteration 316 + 44 c, per constraint
+#   integrate    80 per body
+def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters):
+    if stage == "body_pre":
+        return 240 * n_body
+    if stage == "broadphase":
+        return 208 * n_body
+    if stage == "narrowphase":
+        return 312 * n_pair
+    if stage == "color":
+        return 64 * n_cons
+    if stage == "prestep":
+        return 332 * n_cons + 72 * n_contacts
+    if stage == "solve":
+        return (190 * n_cons + 70 * n_contacts) + iters * (316 * n_cons + 44 * n_contacts)
+    if stage == "integrate":
+        return 80 * n_body
+    return 0
+
+
+def step_bytes(n_body, n_pair, n_cons, n_contacts, iters):
+    """Whole-step algorithmic bytes: B_body*N + B_pair*P + sum over manifolds."""
+    return 528 * n_body + 312 * n_pair + (332 + 190) * n_cons + (72 + 70) * n_contacts + \
+        iters * (316 * n_cons + 44 * n_contacts)
+
+
+def parity_check():
+    """Small towers scene stepped on the GPU and on the oracle."""
+    from bepuphysics1_amd import Space, scenes
+    from oracle import refcpu
+    sc = scenes.towers(nx=8, nz=8, height=20, seed=1)
+    sp = Space(capacity=len(sc))
+    sp.add_scene(sc)
+    sp.Update(60)
+    w = refcpu.World()
+    w.add_scene(sc)
+    w.step(60)
+    pairs_exact = bool(np.array_equal(sp.pairs(), w.pairs()))
+    ig = {int(r.key): r for r in sp.impulses()}
+    io = {int(r.key): r for r in w.impulses()}
+    err = 0.0
+    same_keys = set(ig) == set(io)
+    if same_keys:
+        for k, ro in io.items():
+            rg = ig[k]
+            a = np.array(list(rg.normal) + list(rg.friction) + list(rg.sliding) + [rg.twist], np.float64)
+            b = np.array(list(ro.normal) + list(ro.friction) + list(ro.sliding) + [ro.twist], np.float64)
+            err = max(err, float((np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-3)).max()))
+    motion = float(np.abs(sp.motion_states() - w.motion_states()).max())
+    sp.close()
+    return {"scene": "towers 8x8x20, 60 steps", "pairs_bit_exact": pairs_exact, "constraint_sets_equal": same_keys,
+            "max_impulse_rel_err": err, "max_motion_abs_diff": motion}
+
+
+def cpu_baseline(scene, steps):
+    """Oracle (single-threaded C++ restatement) on the same scene; one warm-up step excluded."""
+    from oracle import refcpu
+    w = refcpu.World()
+    w.add_scene(scene)
+    w.step(1)
+    t0 = time.perf_counter()
+    w.step(steps)
+    dt = time.perf_counter() - t0
+    dyn = int((scene["mass"] > 0).sum())
+    w.close()
+    return dyn * steps / dt, dt
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """Plain device copy of a known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--height", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--latency-steps", type=int, default=50)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(independent worlds shard across GPUs without communication; see DESIGN.md)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
+
+    from bepuphysics1_amd import Space, scenes
+
+    scene = scenes.towers(nx=a.nx, nz=a.nz, height=a.height, seed=0)
+    n_dyn = int((scene["mass"] > 0).sum())
+    sp = Space(capacity=len(scene))
+    sp.add_scene(scene)
+    iters = sp.Solver.IterationLimit
+    if a.warmup:
+        sp.Update(a.warmup)
+    sp.stats()  # synchronise
+
+    sp.set_timing(True)
+    t0 = time.perf_counter()
+    sp.Update(a.steps)
+    st = sp.stats()  # synchronises the world's stream
+    elapsed = time.perf_counter() - t0
+    stages = sp.stage_times()
+    sp.set_timing(False)
+
+    value = n_dyn * a.steps / elapsed
+    ms_per_step = 1e3 * elapsed / a.steps
+
+    counts = dict(n_body=st.bodies, n_pair=st.pairs, n_cons=st.manifolds, n_contacts=st.contacts, iters=iters)
+    per_stage = {}
+    for name, ms in stages.items():
+        b = algorithmic_bytes(name, **counts)
+        per_stage[name] = {"ms": round(ms, 5), "alg_bytes": int(b),
+                           "GBps": round(b / (ms * 1e-3) / 1e9, 2) if ms > 0 else None}
+    dominant = max(stages, key=lambda k: stages[k])
+    dom_bytes = algorithmic_bytes(dominant, **counts)
+    achieved = dom_bytes / (stages[dominant] * 1e-3) / 1e9
+    step_alg = step_bytes(**counts)
+
+    # latency of single synchronised steps
+    lat = []
+    for _ in range(a.latency_steps):
+        t = time.perf_counter()
+        sp.Update(1)
+        sp.stats()
+        lat.append(1e3 * (time.perf_counter() - t))
+    ms_state = sp.motion_states()
+    top = ms_state[1:].reshape(a.nx * a.nz, a.height, 13)[:, -1, :]
+    start_top = scene["state"][1:].reshape(a.nx * a.nz, a.height, 13)[:, -1, :]
+    # A tower stands while its top box has drifted less than half a box sideways
+    # and sits within one box height of where it started (contacts are soft:
+    # 20 stacked contacts compress by about 20 x allowed penetration).
+    standing = int(((np.abs(top[:, 0:3:2] - start_top[:, 0:3:2]).max(axis=1) < 0.5) &
+                    (np.abs(top[:, 1] - start_top[:, 1]) < 1.0)).sum())
+    sp.close()
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        v_cpu, dt_cpu = cpu_baseline(scene, a.cpu_steps)
+        cpu = {"value": round(v_cpu, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
+               "sample": "oracle (single-threaded C++ restatement of the reference path), same 81,920-box "
+                         "towers scene, %d steps after 1 untimed step, %.1f s" % (a.cpu_steps, dt_cpu)}
+
+    parity = None if a.no_parity else parity_check()
+
+    line = {
+        "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "towers %dx%dx%d boxes (plain single-world GPU path), dt 1/60, %d iterations"
+                   % (a.nx, a.nz, a.height, iters),
+                   "bodies": n_dyn, "pairs": st.pairs, "manifolds": st.manifolds, "contacts": st.contacts,
+                   "colors": st.colors},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
+                     "kernel_ms": round(stages[dominant], 5), "alg_bytes_per_launch": int(dom_bytes)},
+        "step_roofline": {"alg_bytes_per_step": int(step_alg),
+                          "achieved_GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 2),
+                          "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
+        "stages": per_stage,
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "parity": parity,
+        "towers_standing": standing,
+        "max_penetration": round(float(st.max_penetration), 5),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

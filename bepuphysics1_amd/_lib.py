@@ -105,6 +105,7 @@ EXPORTS = {
     "bepu_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(StepStats)]),
     "bepu_get_stage_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, _u32p,
                                             ctypes.POINTER(ctypes.c_char_p)]),
+    "bepu_set_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -105,20 +105,51 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
         cy = 0.5f * (mn.y + mx.y);
         cz = 0.5f * (mn.z + mx.z);
     }
-    // Scene bounds of AABB centres: wave reduction, then one atomic per wave.
-    uint32_t lo[3] = {valid ? f2ord(cx) : 0xffffffffu, valid ? f2ord(cy) : 0xffffffffu,
-                      valid ? f2ord(cz) : 0xffffffffu};
-    uint32_t hi[3] = {valid ? f2ord(cx) : 0u, valid ? f2ord(cy) : 0u, valid ? f2ord(cz) : 0u};
+    // Scene bounds of AABB centres: wave then block reduction; one partial per
+    // block, folded by k_bounds_reduce (no global atomics).
+    __shared__ uint32_t part[4][6];
+    uint32_t v6[6] = {valid ? f2ord(cx) : 0xffffffffu, valid ? f2ord(cy) : 0xffffffffu,
+                      valid ? f2ord(cz) : 0xffffffffu, valid ? f2ord(cx) : 0u, valid ? f2ord(cy) : 0u,
+                      valid ? f2ord(cz) : 0u};
     for (int off = 32; off > 0; off >>= 1)
         for (int k = 0; k < 3; ++k) {
-            lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off));
-            hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off));
+            v6[k] = min(v6[k], (uint32_t)__shfl_xor((int)v6[k], off));
+            v6[3 + k] = max(v6[3 + k], (uint32_t)__shfl_xor((int)v6[3 + k], off));
         }
+    int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; ++k) part[wv][k] = v6[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        uint32_t r = part[0][k];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = k < 3 ? min(r, part[q][k]) : max(r, part[q][k]);
+        w.bounds_part[blockIdx.x * 6 + k] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bounds_reduce(DeviceWorld w, uint32_t nparts) {
+    __shared__ uint32_t red[6][256];
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0, 0, 0};
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x)
         for (int k = 0; k < 3; ++k) {
-            atomicMin(&w.bounds[k], lo[k]);
-            atomicMax(&w.bounds[3 + k], hi[k]);
+            lo[k] = min(lo[k], w.bounds_part[p * 6 + k]);
+            hi[k] = max(hi[k], w.bounds_part[p * 6 + 3 + k]);
         }
+    for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = lo[k];
+        red[3 + k][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) {
+                red[k][threadIdx.x] = min(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+                red[3 + k][threadIdx.x] = max(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) w.bounds[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Entity.PreUpdatePosition (EntityBase.cs:1124-1155), discrete mode.
@@ -161,11 +192,10 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
 }
 
 void launch_body_pre(const DeviceWorld& w, hipStream_t st) {
-    // bounds[0..2] = +inf (as ordered ints: max), bounds[3..5] = -inf
-    launch_fill_u32(w.bounds, 0xffffffffu, 3, st);
-    launch_fill_u32(w.bounds + 3, 0u, 3, st);
     if (w.nbodies == 0) return;
-    hipLaunchKernelGGL(k_body_pre, dim3((w.nbodies + 255) / 256), dim3(256), 0, st, w);
+    uint32_t blocks = (w.nbodies + 255) / 256;
+    hipLaunchKernelGGL(k_body_pre, dim3(blocks), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_bounds_reduce, dim3(1), dim3(256), 0, st, w, blocks);
 }
 
 void launch_integrate(const DeviceWorld& w, hipStream_t st) {

@@ -1,6 +1,6 @@
 // k_broadphase.hip -- broadphase overlap finding: Morton codes of AABB
-// centres, radix sort, Karras LBVH build, bottom-up refit and a stack
-// traversal per leaf whose hits are compacted with wavefront ballots.
+// centres, radix sort, an implicit balanced BVH over the Morton order and a
+// stack traversal per leaf whose hits are compacted with wavefront ballots.
 //
 // Replaces DynamicHierarchy.UpdateSingleThreaded (DynamicHierarchy.cs:194-214).
 // The pair SET is what the reference produces: leaf boxes are the bodies'
@@ -8,11 +8,24 @@
 // interval test of BoundingBox.Intersects (BoundingBox.cs:58-69) and pairs
 // pass the default collision rule (kinematic-kinematic is NoBroadPhase,
 // CollisionRules.cs:175-178; dynamic bodies are always active here).
+//
+// Tree: level 0 holds the leaves in Morton order; node i of level L covers
+// leaves [i * 2^L, (i + 1) * 2^L).  Levels 1..8 are reduced inside the
+// workgroup that loaded the leaves (LDS), the few upper levels by one
+// workgroup, so the build needs no cross-workgroup synchronisation.  Each
+// node carries the largest leaf size below it and whether any leaf below is
+// dynamic.  A pair is reported by the leaf with the smaller (size, index);
+// subtrees that cannot hold a leaf ranked above the query are pruned, so a
+// large static body (a ground) does not walk the tree.
 #include <hipcub/hipcub.hpp>
 
 #include "world.h"
 
 namespace bepu {
+
+constexpr int kTreeBlock = 256;   // leaves per low-level build block (levels 1..8)
+constexpr int kTreeLow = 8;
+constexpr int kStack = 32;
 
 __device__ __forceinline__ float ord2f(uint32_t u) {
     u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
@@ -45,63 +58,67 @@ __global__ void __launch_bounds__(256) k_morton(DeviceWorld w) {
     w.leaf_ids_tmp[i] = i;
 }
 
-// Karras 2012 ("Maximizing parallelism in the construction of BVHs, octrees
-// and k-d trees"): internal node i covers a contiguous range of sorted leaves.
-__device__ __forceinline__ int delta(const uint32_t* k, int n, int i, int j) {
-    if (j < 0 || j >= n) return -1;
-    uint32_t a = k[i], b = k[j];
-    if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clz(a ^ b);
+// Node encoding: min.xyz + w = largest leaf size below; max.xyz + w = 1 if a
+// dynamic leaf is below.  Leaf size = sum of the AABB's edge lengths.
+__device__ __forceinline__ void merge(float4& amin, float4& amax, float4 bmin, float4 bmax) {
+    amin = make_float4(fminf(amin.x, bmin.x), fminf(amin.y, bmin.y), fminf(amin.z, bmin.z), fmaxf(amin.w, bmin.w));
+    amax = make_float4(fmaxf(amax.x, bmax.x), fmaxf(amax.y, bmax.y), fmaxf(amax.z, bmax.z), fmaxf(amax.w, bmax.w));
 }
 
-__global__ void __launch_bounds__(256) k_lbvh_build(DeviceWorld w) {
-    int n = (int)w.nbodies;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    const uint32_t* k = w.morton_sorted;
-    int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
-    int dmin = delta(k, n, i, i - d);
-    int lmax = 2;
-    while (delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
-    int j = i + l * d;
-    int dnode = delta(k, n, i, j);
-    int s = 0;
-    for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
-        if (delta(k, n, i, i + (s + t) * d) > dnode) s += t;
-        if (t == 1) break;
+__global__ void __launch_bounds__(kTreeBlock) k_tree_low(DeviceWorld w) {
+    __shared__ float4 smin[kTreeBlock], smax[kTreeBlock];
+    uint32_t n = w.nbodies;
+    uint32_t t = threadIdx.x;
+    uint32_t j = blockIdx.x * kTreeBlock + t;
+    if (j < n) {
+        uint32_t b = w.leaf_ids[j];
+        float4 mn = w.bmin[b], mx = w.bmax[b];
+        float ext = (mx.x - mn.x) + (mx.y - mn.y) + (mx.z - mn.z);
+        float dyn = (w.bflags[b] & 1u) ? 1.0f : 0.0f;
+        mn.w = ext;
+        mx.w = dyn;
+        w.node_min[j] = mn;
+        w.node_max[j] = mx;
+        smin[t] = mn;
+        smax[t] = mx;
     }
-    int gamma = i + s * d + min(d, 0);
-    int lo = min(i, j), hi = max(i, j);
-    int left = (lo == gamma) ? ~gamma : gamma;             // leaf when the range is a single leaf
-    int right = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
-    w.node_child[i] = make_int2(left, right);
-    w.node_parent[left >= 0 ? left : (n - 1) + ~left] = i;
-    w.node_parent[right >= 0 ? right : (n - 1) + ~right] = i;
-    if (i == 0) w.node_parent[0] = -1;
+    __syncthreads();
+    for (int L = 1; L <= kTreeLow && L < (int)w.tree_levels; ++L) {
+        uint32_t cnt_prev = w.tree_cnt[L - 1], cnt = w.tree_cnt[L];
+        uint32_t per = kTreeBlock >> L;              // nodes of level L owned by this block
+        uint32_t gi = blockIdx.x * per + t;          // global index at level L
+        float4 mn, mx;
+        bool ok = t < per && gi < cnt;
+        if (ok) {
+            uint32_t c0 = 2 * gi;
+            mn = smin[2 * t];
+            mx = smax[2 * t];
+            if (c0 + 1 < cnt_prev) merge(mn, mx, smin[2 * t + 1], smax[2 * t + 1]);
+        }
+        __syncthreads();
+        if (ok) {
+            smin[t] = mn;
+            smax[t] = mx;
+            w.node_min[w.tree_off[L] + gi] = mn;
+            w.node_max[w.tree_off[L] + gi] = mx;
+        }
+        __syncthreads();
+    }
 }
 
-// Bottom-up refit: the second child to arrive at a node writes its box.
-__global__ void __launch_bounds__(256) k_lbvh_refit(DeviceWorld w) {
-    int n = (int)w.nbodies;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    int node = w.node_parent[(n - 1) + j];
-    while (node >= 0) {
-        __threadfence();
-        if (atomicAdd(&w.node_visit[node], 1u) == 0) return;
-        __threadfence();
-        int2 c = w.node_child[node];
-        float4 amin, amax, bmin, bmax;
-        if (c.x >= 0) { amin = w.node_min[c.x]; amax = w.node_max[c.x]; }
-        else { uint32_t b = w.leaf_ids[~c.x]; amin = w.bmin[b]; amax = w.bmax[b]; }
-        if (c.y >= 0) { bmin = w.node_min[c.y]; bmax = w.node_max[c.y]; }
-        else { uint32_t b = w.leaf_ids[~c.y]; bmin = w.bmin[b]; bmax = w.bmax[b]; }
-        w.node_min[node] = make_float4(fminf(amin.x, bmin.x), fminf(amin.y, bmin.y), fminf(amin.z, bmin.z), 0.0f);
-        w.node_max[node] = make_float4(fmaxf(amax.x, bmax.x), fmaxf(amax.y, bmax.y), fmaxf(amax.z, bmax.z), 0.0f);
-        node = w.node_parent[node];
+// Levels above kTreeLow: one workgroup, level by level.
+__global__ void __launch_bounds__(1024) k_tree_high(DeviceWorld w) {
+    for (int L = kTreeLow + 1; L < (int)w.tree_levels; ++L) {
+        uint32_t cnt_prev = w.tree_cnt[L - 1], cnt = w.tree_cnt[L];
+        const float4* cmin = w.node_min + w.tree_off[L - 1];
+        const float4* cmax = w.node_max + w.tree_off[L - 1];
+        for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+            float4 mn = cmin[2 * i], mx = cmax[2 * i];
+            if (2 * i + 1 < cnt_prev) merge(mn, mx, cmin[2 * i + 1], cmax[2 * i + 1]);
+            w.node_min[w.tree_off[L] + i] = mn;
+            w.node_max[w.tree_off[L] + i] = mx;
+        }
+        __syncthreads();
     }
 }
 
@@ -111,74 +128,128 @@ __device__ __forceinline__ bool overlap(float4 amin, float4 amax, float4 bmin, f
     return true;
 }
 
-// Appends `key` for every lane with `found` set; one atomic per wavefront.
-__device__ __forceinline__ void wave_append(DeviceWorld& w, bool found, uint64_t key) {
-    uint64_t active = __ballot(1);
-    uint64_t mask = __ballot(found);
-    if (mask == 0) return;
-    int lane = __lane_id();
-    int leader = __ffsll((unsigned long long)active) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&w.counters[CTR_PAIRS], (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (found) {
-        uint32_t idx = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        if (idx < w.pair_cap) w.pair_keys[idx] = key;
+constexpr int kPairBuf = 1024;  // pairs buffered in LDS per workgroup before one global append
+
+// Flushes the workgroup's buffered pairs with a single global atomic.
+__device__ __forceinline__ void flush_pairs(DeviceWorld& w, uint64_t* buf, uint32_t n, uint32_t* base_slot) {
+    if (threadIdx.x == 0) *base_slot = n ? atomicAdd(&w.counters[CTR_PAIRS], n) : 0;
+    __syncthreads();
+    uint32_t base = *base_slot;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        uint32_t idx = base + i;
+        if (idx < w.pair_cap) w.pair_keys[idx] = buf[i];
         else w.counters[CTR_OVERFLOW] = 1;
     }
 }
 
-constexpr int kStack = 64;
-
-// One thread per sorted leaf j; reports leaves j' > j whose boxes overlap.
+// One thread per sorted leaf j; reports overlapping leaves ranked above it.
+// Hits are appended to an LDS buffer (wave ballot + one LDS atomic per wave)
+// and written out with one global atomic per workgroup.
 __global__ void __launch_bounds__(256) k_traverse(DeviceWorld w) {
-    __shared__ int stack_mem[kStack][256];
-    int n = (int)w.nbodies;
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t stack_mem[kStack][256];
+    __shared__ uint64_t buf[kPairBuf];
+    __shared__ uint32_t buf_n, base_slot;
+    uint32_t n = w.nbodies;
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) buf_n = 0;
+    __syncthreads();
     bool alive = j < n;
+    float4 qmin = alive ? w.node_min[j] : make_float4(0, 0, 0, 0);
+    float4 qmax = alive ? w.node_max[j] : make_float4(0, 0, 0, 0);
     uint32_t body = alive ? w.leaf_ids[j] : 0;
-    float4 qmin = alive ? w.bmin[body] : make_float4(0, 0, 0, 0);
-    float4 qmax = alive ? w.bmax[body] : make_float4(0, 0, 0, 0);
-    bool qdyn = alive ? ((w.bflags[body] & 1u) != 0) : false;
+    float qext = qmin.w;
+    bool qdyn = qmax.w != 0.0f;
+    int top = (int)w.tree_levels - 1;
     int sp = 0;
-    if (alive && n > 1) stack_mem[sp++][threadIdx.x] = 0;
-    // All lanes of the wave iterate together so that ballots see every lane.
-    while (__ballot(sp > 0) != 0) {
-        int node = -1;
-        if (sp > 0) node = stack_mem[--sp][threadIdx.x];
-        int2 c = node >= 0 ? w.node_child[node] : make_int2(0, 0);
+    if (alive && n > 1) stack_mem[sp++][threadIdx.x] = ((uint32_t)top << 27) | 0u;
+    int lane = __lane_id();
+    // The loop runs until every wave of the block is done so that buffer
+    // flushes (block barriers) are reached by all threads.
+    while (true) {
+        bool busy = sp > 0;
+        int L = -1;
+        uint32_t idx = 0;
+        if (busy) {
+            uint32_t e = stack_mem[--sp][threadIdx.x];
+            L = (int)(e >> 27);
+            idx = e & 0x7ffffffu;
+        }
         for (int side = 0; side < 2; ++side) {
             bool found = false;
             uint64_t key = 0;
-            if (node >= 0) {
-                int ch = side == 0 ? c.x : c.y;
-                if (ch >= 0) {
-                    if (overlap(qmin, qmax, w.node_min[ch], w.node_max[ch])) {
-                        if (sp < kStack) stack_mem[sp++][threadIdx.x] = ch;
-                        else w.counters[CTR_OVERFLOW] = 2;
-                    }
-                } else {
-                    int leaf = ~ch;
-                    if (leaf > j) {
-                        uint32_t other = w.leaf_ids[leaf];
-                        bool odyn = (w.bflags[other] & 1u) != 0;
-                        if ((qdyn || odyn) && overlap(qmin, qmax, w.bmin[other], w.bmax[other])) {
+            if (L > 0) {
+                int cl = L - 1;
+                uint32_t ci = 2 * idx + side;
+                if (ci < w.tree_cnt[cl]) {
+                    float4 cmin = w.node_min[w.tree_off[cl] + ci];
+                    float4 cmax = w.node_max[w.tree_off[cl] + ci];
+                    if (cl == 0) {
+                        float cext = cmin.w;
+                        bool mine = cext > qext || (cext == qext && ci > j);
+                        bool cdyn = cmax.w != 0.0f;
+                        if (mine && (qdyn || cdyn) && overlap(qmin, qmax, cmin, cmax)) {
                             found = true;
-                            key = pair_key(body, other);
+                            key = pair_key(body, w.leaf_ids[ci]);
+                        }
+                    } else {
+                        uint32_t hi_leaf = min(n, (ci + 1) << cl) - 1;
+                        bool ranked = cmin.w > qext || (cmin.w == qext && hi_leaf > j);
+                        if (ranked && (qdyn || cmax.w != 0.0f) && overlap(qmin, qmax, cmin, cmax)) {
+                            if (sp < kStack) stack_mem[sp++][threadIdx.x] = ((uint32_t)cl << 27) | ci;
+                            else w.counters[CTR_OVERFLOW] = 2;
                         }
                     }
                 }
             }
-            wave_append(w, found, key);
+            uint64_t mask = __ballot(found);
+            if (mask) {
+                uint32_t wbase = 0;
+                int leader = __ffsll((unsigned long long)mask) - 1;
+                if (lane == leader) wbase = atomicAdd(&buf_n, (uint32_t)__popcll(mask));
+                wbase = __shfl(wbase, leader);
+                if (found) {
+                    uint32_t at = wbase + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                    if (at < kPairBuf) buf[at] = key;
+                    else {  // buffer full: this pair goes straight out
+                        uint32_t idx2 = atomicAdd(&w.counters[CTR_PAIRS], 1u);
+                        if (idx2 < w.pair_cap) w.pair_keys[idx2] = key;
+                        else w.counters[CTR_OVERFLOW] = 1;
+                    }
+                }
+            }
+        }
+        // Block-level continuation test and buffer flush when nearly full.
+        if (__syncthreads_or(sp > 0) == 0) break;
+        if (buf_n >= kPairBuf - 2 * 256) {
+            uint32_t cnt = min(buf_n, (uint32_t)kPairBuf);
+            flush_pairs(w, buf, cnt, &base_slot);
+            __syncthreads();
+            if (threadIdx.x == 0) buf_n = 0;
+            __syncthreads();
         }
     }
+    __syncthreads();
+    flush_pairs(w, buf, min(buf_n, (uint32_t)kPairBuf), &base_slot);
 }
 
 size_t broadphase_sort_temp_bytes(uint32_t n) {
     size_t bytes = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
     return bytes;
+}
+
+void tree_layout(uint32_t n, uint32_t* levels, uint32_t* off, uint32_t* cnt) {
+    uint32_t L = 0, o = 0, c = n;
+    while (true) {
+        off[L] = o;
+        cnt[L] = c;
+        o += c;
+        ++L;
+        if (c <= 1 || L >= kMaxTreeLevels) break;
+        c = (c + 1) / 2;
+    }
+    *levels = L;
 }
 
 void launch_broadphase(const DeviceWorld& w, void* temp, size_t temp_bytes, hipStream_t st) {
@@ -186,11 +257,10 @@ void launch_broadphase(const DeviceWorld& w, void* temp, size_t temp_bytes, hipS
     if (n < 2) return;
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, w);
     size_t bytes = temp_bytes;
-    hipcub::DeviceRadixSort::SortPairs(temp, bytes, w.morton, w.morton_sorted, w.leaf_ids_tmp, w.leaf_ids, (int)n, 0,
-                                       30, st);
-    hipLaunchKernelGGL(k_lbvh_build, dim3((n - 1 + 255) / 256), dim3(256), 0, st, w);
-    launch_fill_u32(w.node_visit, 0u, n - 1, st);
-    hipLaunchKernelGGL(k_lbvh_refit, dim3((n + 255) / 256), dim3(256), 0, st, w);
+    (void)hipcub::DeviceRadixSort::SortPairs(temp, bytes, w.morton, w.morton_sorted, w.leaf_ids_tmp, w.leaf_ids,
+                                             (int)n, 0, 30, st);
+    hipLaunchKernelGGL(k_tree_low, dim3((n + kTreeBlock - 1) / kTreeBlock), dim3(kTreeBlock), 0, st, w);
+    if (w.tree_levels > kTreeLow + 1) hipLaunchKernelGGL(k_tree_high, dim3(1), dim3(1024), 0, st, w);
     hipLaunchKernelGGL(k_traverse, dim3((n + 255) / 256), dim3(256), 0, st, w);
 }
 

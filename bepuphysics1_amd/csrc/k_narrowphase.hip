@@ -631,7 +631,7 @@ __device__ void init_pair(const DeviceWorld& w, uint64_t key, PairState& s) {
     }
     s.slacc[0] = s.slacc[1] = 0;
     s.twacc = 0;
-    s.pad0 = 0;
+    s.color = -1;
     for (int k = 0; k < 6; ++k) s.sllin[k] = 0;
     s.pad1[0] = s.pad1[1] = 0;
     s.a = i;

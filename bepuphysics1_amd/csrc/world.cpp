@@ -15,14 +15,14 @@
 
 #include "../../include/bepu_gpu.h"
 #include "bodies_host.h"
-#include "world.h"
+#include "solver_rows.h"
 
 using namespace bepu;
 
 namespace {
 
-enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_SOLVER, ST_INTEGRATE, ST_COUNT };
-const char* kStageNames = "body_pre,broadphase,narrowphase,solver,integrate";
+enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE, ST_COUNT };
+const char* kStageNames = "body_pre,broadphase,narrowphase,islands,island_solve,global_solve,integrate";
 
 struct HostBody {
     float4 pos, quat, linv, angv, ii0, ii1, ii2, li0, li1, li2, shape, stab, mat;
@@ -55,8 +55,9 @@ struct bepu_world_s {
     size_t sort_bytes = 0;
     void* scan_temp = nullptr;
     size_t scan_bytes = 0;
-    hipEvent_t ev[ST_COUNT + 1] = {};
-    float stage_ms[ST_COUNT] = {};
+    bool timing = false;
+    std::vector<hipEvent_t> ev;     // (ST_COUNT + 1) per timed step of the last bepu_step call
+    uint32_t timed_steps = 0;
     uint32_t steps = 0;
     std::vector<void*> allocations;
 };
@@ -138,7 +139,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     DeviceWorld& d = w->d;
     uint32_t n = capacity_bodies;
     d.body_cap = n;
-    d.pair_cap = std::max<uint32_t>(64, 8 * n);
+    d.pair_cap = std::max<uint32_t>(1024, 4 * n);
     d.hash_cap = next_pow2(2ull * d.pair_cap);
     bepu_status st = BEPU_OK;
     auto A = [&](auto** p, size_t cnt) {
@@ -150,22 +151,22 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
-    A(&d.node_child, n); A(&d.node_parent, 2 * n); A(&d.node_min, n); A(&d.node_max, n); A(&d.node_visit, n);
-    A(&d.bounds, 8);
+    A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
+    A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
     uint32_t P = d.pair_cap;
     A(&d.pair_keys, P); A(&d.counters, CTR_COUNT);
     A(&d.pairs_cur, P); A(&d.pairs_old, P);
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
     A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
-    A(&d.cons, P); A(&d.color, P); A(&d.prio, P);
+    A(&d.cons, P);
+    A(&d.uf_parent, n); A(&d.isl_root, n); A(&d.isl_ccount, n + 1); A(&d.isl_cstart, n + 1);
+    A(&d.isl_bcount, n + 1); A(&d.isl_bstart, n + 1); A(&d.isl_cons, P); A(&d.isl_bodies, n);
+    A(&d.body_local, n); A(&d.isl_tier, n); A(&d.isl_small, n); A(&d.isl_medium, n); A(&d.isl_large, n);
+    A(&d.gcons, P); A(&d.color, P); A(&d.prio, P);
     A(&d.inc_count, n + 1); A(&d.inc_offset, n + 1); A(&d.inc_list, 2 * (size_t)P);
     A(&d.color_count, kMaxColors); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
     A(&d.ordered, P);
-    SolverRows& r = d.rows;
-    A(&r.bodies, P); A(&r.meta, P); A(&r.pair, P); A(&r.act, P); A(&r.zero, P);
-    A(&r.pen, 12 * (size_t)P); A(&r.penacc, 4 * (size_t)P);
-    A(&r.sl, 9 * (size_t)P); A(&r.slacc, P); A(&r.tw, 3 * (size_t)P); A(&r.twacc, P);
-    A(&r.fr, 12 * (size_t)P); A(&r.fracc, 4 * (size_t)P);
+    A(&d.grow_c, P); A(&d.grow_v, P); A(&d.grow_pair, P);
     if (st != BEPU_OK) {
         bepu_world_destroy(w);
         return st;
@@ -174,13 +175,8 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
         bepu_world_destroy(w);
         return BEPU_E_DEVICE;
     }
-    for (auto& e : w->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            bepu_world_destroy(w);
-            return BEPU_E_DEVICE;
-        }
     w->sort_bytes = broadphase_sort_temp_bytes(n);
-    w->scan_bytes = solver_scan_temp_bytes(n + 1);
+    w->scan_bytes = std::max(solver_scan_temp_bytes(n + 1), island_scan_temp_bytes(n + 1));
     if (hipMalloc(&w->sort_temp, std::max<size_t>(w->sort_bytes, 16)) != hipSuccess ||
         hipMalloc(&w->scan_temp, std::max<size_t>(w->scan_bytes, 16)) != hipSuccess) {
         bepu_world_destroy(w);
@@ -273,6 +269,7 @@ static bepu_status upload(bepu_world w) {
     if (!ok) return BEPU_E_DEVICE;
     w->uploaded += (uint32_t)k;
     d.nbodies = w->uploaded;
+    tree_layout(d.nbodies, &d.tree_levels, d.tree_off, d.tree_cnt);
     w->staged.clear();
     return BEPU_OK;
 }
@@ -293,27 +290,48 @@ bepu_status bepu_add_sphere(bepu_world w, const bepu_motion_state* s, float radi
     return add_body(w, s, BEPU_SHAPE_SPHERE, f3{radius, 0, 0}, sphere_volume_distribution(radius), mass, m, flags, id);
 }
 
+bepu_status bepu_set_timing(bepu_world w, int32_t enable) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    w->timing = enable != 0;
+    return BEPU_OK;
+}
+
 bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (!w || n_steps < 0) return BEPU_E_INVALID_ARG;
     bepu_status st = upload(w);
     if (st != BEPU_OK) return st;
     DeviceWorld& d = w->d;
     hipStream_t s = w->stream;
+    // Stage timers: every step of this call when timing is on, else the last step only.
+    uint32_t timed = w->timing ? (uint32_t)n_steps : (n_steps > 0 ? 1u : 0u);
+    size_t need = (size_t)timed * (ST_COUNT + 1);
+    if (need > 0 && hipStreamSynchronize(s) != hipSuccess) return BEPU_E_DEVICE;
+    while (w->ev.size() < need) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return BEPU_E_DEVICE;
+        w->ev.push_back(e);
+    }
+    if (timed) w->timed_steps = timed;
     for (int k = 0; k < n_steps; ++k) {
-        bool timed = (k == n_steps - 1);
+        int slot = (int)timed - (n_steps - k);  // >= 0 for the timed steps
+        hipEvent_t* e = slot >= 0 ? &w->ev[(size_t)slot * (ST_COUNT + 1)] : nullptr;
         launch_fill_u32(d.counters, 0u, CTR_COUNT, s);
         launch_fill_u64(d.hash_keys_cur, ~0ull, d.hash_cap, s);
-        if (timed) (void)hipEventRecord(w->ev[0], s);
+        if (e) (void)hipEventRecord(e[0], s);
         launch_body_pre(d, s);
-        if (timed) (void)hipEventRecord(w->ev[1], s);
+        if (e) (void)hipEventRecord(e[1], s);
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
-        if (timed) (void)hipEventRecord(w->ev[2], s);
+        if (e) (void)hipEventRecord(e[2], s);
         launch_narrowphase(d, s);
-        if (timed) (void)hipEventRecord(w->ev[3], s);
-        launch_solver_full(d, w->scan_temp, w->scan_bytes, s);
-        if (timed) (void)hipEventRecord(w->ev[4], s);
+        if (e) (void)hipEventRecord(e[3], s);
+        launch_islands(d, w->scan_temp, w->scan_bytes, s);
+        if (e) (void)hipEventRecord(e[4], s);
+        launch_island_solve(d, s);
+        if (e) (void)hipEventRecord(e[5], s);
+        launch_global_solver(d, w->scan_temp, w->scan_bytes, s);
+        if (e) (void)hipEventRecord(e[6], s);
         launch_integrate(d, s);
-        if (timed) (void)hipEventRecord(w->ev[5], s);
+        if (e) (void)hipEventRecord(e[7], s);
         // This step's pair states and table become next step's history.
         std::swap(d.pairs_cur, d.pairs_old);
         std::swap(d.hash_keys_cur, d.hash_keys_old);
@@ -324,15 +342,21 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     return BEPU_OK;
 }
 
+// Average device time per stage over the timed steps of the last bepu_step call.
 bepu_status bepu_get_stage_times(bepu_world w, float* ms, uint32_t cap, uint32_t* count, const char** names) {
     if (!w) return BEPU_E_INVALID_ARG;
     if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
     if (count) *count = ST_COUNT;
     if (names) *names = kStageNames;
     for (uint32_t i = 0; i < ST_COUNT && i < cap; ++i) {
-        float t = 0;
-        if (hipEventElapsedTime(&t, w->ev[i], w->ev[i + 1]) != hipSuccess) t = -1;
-        ms[i] = t;
+        double sum = 0;
+        for (uint32_t k = 0; k < w->timed_steps; ++k) {
+            float t = 0;
+            const hipEvent_t* e = &w->ev[(size_t)k * (ST_COUNT + 1)];
+            if (hipEventElapsedTime(&t, e[i], e[i + 1]) != hipSuccess) t = 0;
+            sum += t;
+        }
+        ms[i] = w->timed_steps ? (float)(sum / w->timed_steps) : 0.0f;
     }
     return BEPU_OK;
 }
@@ -432,24 +456,14 @@ bepu_status bepu_get_impulses(bepu_world w, bepu_impulses* out, uint32_t cap, ui
     std::vector<PairState> ps;
     uint32_t c[CTR_COUNT];
     if (read_pairs(w, ps, c) != BEPU_OK) return BEPU_E_DEVICE;
-    uint32_t m = w->steps ? std::min(c[CTR_CONS], w->d.pair_cap) : 0;
-    if (count) *count = m;
-    if (!out || cap < m) return m == 0 ? BEPU_OK : BEPU_E_CAPACITY;
-    std::vector<int> pair(m);
-    std::vector<uint32_t> start(kMaxColors + 1);
-    if (m && (hipMemcpy(pair.data(), w->d.rows.pair, m * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
-              hipMemcpy(start.data(), w->d.color_start, (kMaxColors + 1) * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost) != hipSuccess))
-        return BEPU_E_DEVICE;
-    std::vector<bepu_impulses> v(m);
-    int col = 0;
-    for (uint32_t p = 0; p < m; ++p) {
-        while (col < kMaxColors && p >= start[col + 1]) ++col;
-        const PairState& s = ps[pair[p]];
-        bepu_impulses& o = v[p];
+    std::vector<bepu_impulses> v;
+    for (const PairState& s : ps) {
+        if (s.ccount <= 0) continue;
+        v.emplace_back();
+        bepu_impulses& o = v.back();
         std::memset(&o, 0, sizeof(o));
         o.key = s.key;
-        o.color = col;
+        o.color = s.color;
         o.count = s.npen;
         for (int k = 0; k < s.npen; ++k) o.normal[k] = s.penacc[k];
         bool convex = s.kind == BEPU_PAIR_BOX_BOX || s.kind == BEPU_PAIR_CONVEX;
@@ -464,6 +478,9 @@ bepu_status bepu_get_impulses(bepu_world w, bepu_impulses* out, uint32_t cap, ui
     std::sort(v.begin(), v.end(), [](const bepu_impulses& a, const bepu_impulses& b) {
         return a.color != b.color ? a.color < b.color : a.key < b.key;
     });
+    uint32_t m = (uint32_t)v.size();
+    if (count) *count = m;
+    if (!out || cap < m) return m == 0 ? BEPU_OK : BEPU_E_CAPACITY;
     std::copy(v.begin(), v.end(), out);
     return BEPU_OK;
 }

@@ -42,29 +42,15 @@ struct PairState {
     float4 frlin[4];      // its remembered direction (xyz)
     float slacc[2];       // SlidingFrictionTwoAxis accumulated impulse
     float twacc;          // TwistFrictionConstraint accumulated impulse
-    float pad0;
+    int32_t color;        // colour of the constraint in the last solve (-1 before)
     float sllin[6];       // SlidingFrictionTwoAxis linearA (remembered across steps)
     float pad1[2];
 };
 
 enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u };
 
-// Solver rows of all constraints in solve order; index = position in order.
-struct SolverRows {
-    int2* bodies;       // A, B
-    int* meta;          // kind | npen << 4 | aDyn << 8 | bDyn << 9
-    int* pair;          // pair slot
-    uint32_t* act;      // bit k (k < 10): sub-constraint k active in solver; bit 31: group active
-    uint32_t* zero;     // 3-bit zero-impulse iteration counter per sub-constraint
-    float4* pen;        // [k][3][cap]: (lin.xyz, softness) (angA.xyz, bias) (angB.xyz, v2i)
-    float* penacc;      // [k][cap]
-    float4* sl;         // [9][cap]: lin0(xyz,friction) lin1 angA0 angA1 angB0 angB1 v2i(4) ra rb
-    float2* slacc;
-    float4* tw;         // [3][cap]: (ang.xyz, v2i) (lever0..3) (friction, count, -, -)
-    float* twacc;
-    float4* fr;         // [k][3][cap]: (lin.xyz, friction) (angA.xyz, v2i) (angB.xyz, -)
-    float* fracc;       // [k][cap]
-};
+struct RowConst;
+struct RowVar;
 
 struct DeviceWorld {
     bepu_settings s;
@@ -87,11 +73,12 @@ struct DeviceWorld {
     uint32_t* morton_sorted;
     uint32_t* leaf_ids;        // body id per sorted leaf
     uint32_t* leaf_ids_tmp;
-    int2* node_child;          // internal node children (>=0 internal, <0 leaf ~idx)
-    int* node_parent;          // parent for internal + leaf nodes (leaves offset by n-1)
-    float4 *node_min, *node_max;
-    uint32_t* node_visit;
+    float4 *node_min, *node_max;  // implicit tree, levels stored back to back (see tree_layout)
+    uint32_t tree_levels;
+    uint32_t tree_off[32];
+    uint32_t tree_cnt[32];
     uint32_t* bounds;          // ordered-int encoded scene bounds (6)
+    uint32_t* bounds_part;     // per-block partial bounds (6 per block of 256 bodies)
     // pairs
     uint64_t* pair_keys;       // this step's overlap list
     uint32_t* counters;        // see CTR_*
@@ -101,21 +88,39 @@ struct DeviceWorld {
     uint32_t* hash_vals_cur;
     uint64_t* hash_keys_old;
     uint32_t* hash_vals_old;
-    // solver
+    // islands (connected components of the constraint graph over dynamic bodies)
     uint32_t* cons;            // constraint -> pair slot (unordered)
-    int* color;                // per constraint
-    uint64_t* prio;            // per constraint
+    uint32_t* uf_parent;       // union-find forest over bodies
+    uint32_t* isl_root;        // island (smallest dynamic body id) of every dynamic body
+    uint32_t* isl_ccount;      // per root: constraints (body_cap + 1, zero between steps)
+    uint32_t* isl_cstart;
+    uint32_t* isl_bcount;      // per root: dynamic bodies
+    uint32_t* isl_bstart;
+    uint32_t* isl_cons;        // island-contiguous constraint list (pair slots)
+    uint32_t* isl_bodies;      // island-contiguous body list
+    uint32_t* body_local;      // index of a body inside its island's body list
+    uint32_t* isl_tier;        // per root: 1 small, 2 medium, 3 large
+    uint32_t* isl_small;       // island roots per tier
+    uint32_t* isl_medium;
+    uint32_t* isl_large;
+    // global path (large islands)
+    uint32_t* gcons;           // pair slots of constraints in large islands
+    int* color;                // per gcons entry
+    uint64_t* prio;
     uint32_t* inc_count;       // body incidence counts (body_cap + 1)
     uint32_t* inc_offset;
     uint32_t* inc_list;        // 2 * pair_cap
     uint32_t* color_count;     // kMaxColors
     uint32_t* color_start;     // kMaxColors + 1
     uint32_t* color_cursor;
-    uint32_t* ordered;         // solve order -> constraint index
-    SolverRows rows;
+    uint32_t* ordered;         // solve order -> gcons index
+    RowConst* grow_c;          // rows in solve order
+    RowVar* grow_v;
+    uint32_t* grow_pair;
 };
 
 constexpr int kMaxColors = 128;
+constexpr uint32_t kMaxTreeLevels = 32;
 
 // counters[] indices
 enum {
@@ -127,6 +132,10 @@ enum {
     CTR_OVERFLOW = 5,     // pair/hash capacity overflow flag
     CTR_ROUND = 6,        // colouring rounds
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
+    CTR_GCONS = 11,       // constraints on the global path
+    CTR_ISL_SMALL = 12,
+    CTR_ISL_MEDIUM = 13,
+    CTR_ISL_LARGE = 14,
     CTR_COUNT = 16
 };
 
@@ -134,10 +143,15 @@ enum {
 void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
 void launch_narrowphase(const DeviceWorld& w, hipStream_t st);
-void launch_solver_full(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
+void launch_islands(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
+void launch_island_solve(const DeviceWorld& w, hipStream_t st);
+void launch_global_solver(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
 size_t solver_scan_temp_bytes(uint32_t n);
+size_t island_scan_temp_bytes(uint32_t n);
 void launch_integrate(const DeviceWorld& w, hipStream_t st);
 size_t broadphase_sort_temp_bytes(uint32_t n);
+// Level offsets/counts of the implicit tree over n leaves (level 0 = leaves).
+void tree_layout(uint32_t n, uint32_t* levels, uint32_t* off, uint32_t* cnt);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 

@@ -321,6 +321,9 @@ class Space:
         check(self._lib.bepu_get_stats(self._handle(), ctypes.byref(s)), "get_stats")
         return s
 
+    def set_timing(self, enable=True):
+        check(self._lib.bepu_set_timing(self._handle(), 1 if enable else 0), "set_timing")
+
     def stage_times(self):
         ms = (ctypes.c_float * 16)()
         n = ctypes.c_uint32()

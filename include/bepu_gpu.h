@@ -197,9 +197,14 @@ bepu_status bepu_get_impulses(bepu_world world, bepu_impulses* out, uint32_t cap
                               uint32_t* count_out);
 bepu_status bepu_get_stats(bepu_world world, bepu_step_stats* out);
 
-/* Per-stage device time of the last bepu_step call, in milliseconds, measured
- * with HIP events on the world's stream.  `names` receives a pointer to a
- * static, comma separated list of stage names matching `ms`. */
+/* Stage timers.  With timing enabled every step of a bepu_step call records
+ * HIP events between its stages on the world's stream; otherwise only the
+ * last step does.  bepu_get_stage_times returns the average device time per
+ * stage (milliseconds) over those steps; `names` receives a pointer to a
+ * static, comma separated list of stage names matching `ms`.  No reference
+ * counterpart: it plays the role of the reference's PROFILE stage timers
+ * (ProcessingStage.cs:26-59, Space.cs:486-520). */
+bepu_status bepu_set_timing(bepu_world world, int32_t enable);
 bepu_status bepu_get_stage_times(bepu_world world, float* ms, uint32_t capacity,
                                  uint32_t* count_out, const char** names);
 
