@@ -33,33 +33,44 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per unit of work, per stage (FP32, each element counted
-# once per stage, no cache-reuse credit):
-#   body_pre     force/inertia 160 + AABB 80 per body
-#   broadphase   Morton 32 + radix sort 64 + LBVH build 88 + traversal leaf read 24 per body
-#   narrowphase  312 per broadphase pair
-#   color        64 per constraint (pair read 24, incidence 16, colour/priority 16, order 8)
-#   prestep      332 + 72 c per constraint (c = its contact count)
-#   solve        warm start 190 + 70 c, plus per iteration 316 + 44 c, per constraint
-#   integrate    80 per body
-def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters):
+# once per stage and once per solver iteration, no cache-reuse credit):
+#   body_pre      force/inertia 160 + AABB 80 per body
+#   broadphase    Morton 32 + radix sort 64 + tree build 88 + traversal leaf read 24 per body
+#   narrowphase   312 per broadphase pair
+#   islands       union-find and island bucketing: 40 per body (parent init 4,
+#                 compress 12, count 4, body list 8, local index 4, tier list 8)
+#                 + 32 per manifold (union reads 16, count 8, constraint list 8)
+#   island_solve  prestep 332 + 72 c, warm start 190 + 70 c and per iteration
+#                 316 + 44 c, per manifold of an island solved in one workgroup
+#   global_solve  the same per manifold of a large island, plus 64 for its
+#                 colouring (pair read 24, incidence 16, colour/priority 16, order 8)
+#   integrate     80 per body
+# c = contacts of the manifold (the scene's mean is used per tier).
+def manifold_bytes(m, contacts, iters):
+    return (332 + 190) * m + (72 + 70) * contacts + iters * (316 * m + 44 * contacts)
+
+
+def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global=0):
+    c_mean = n_contacts / n_cons if n_cons else 0.0
+    m_isl = n_cons - n_global
     if stage == "body_pre":
         return 240 * n_body
     if stage == "broadphase":
         return 208 * n_body
     if stage == "narrowphase":
         return 312 * n_pair
-    if stage == "color":
-        return 64 * n_cons
-    if stage == "prestep":
-        return 332 * n_cons + 72 * n_contacts
-    if stage == "solve":
-        return (190 * n_cons + 70 * n_contacts) + iters * (316 * n_cons + 44 * n_contacts)
+    if stage == "islands":
+        return 40 * n_body + 32 * n_cons
+    if stage == "island_solve":
+        return manifold_bytes(m_isl, c_mean * m_isl, iters)
+    if stage == "global_solve":
+        return 64 * n_global + manifold_bytes(n_global, c_mean * n_global, iters)
     if stage == "integrate":
         return 80 * n_body
     return 0
 
 
-def step_bytes(n_body, n_pair, n_cons, n_contacts, iters):
+def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0):
     """Whole-step algorithmic bytes: B_body*N + B_pair*P + sum over manifolds."""
     return 528 * n_body + 312 * n_pair + (332 + 190) * n_cons + (72 + 70) * n_contacts + \
         iters * (316 * n_cons + 44 * n_contacts)
@@ -164,7 +175,8 @@ def main():
     value = n_dyn * a.steps / elapsed
     ms_per_step = 1e3 * elapsed / a.steps
 
-    counts = dict(n_body=st.bodies, n_pair=st.pairs, n_cons=st.manifolds, n_contacts=st.contacts, iters=iters)
+    counts = dict(n_body=st.bodies, n_pair=st.pairs, n_cons=st.manifolds, n_contacts=st.contacts, iters=iters,
+                  n_global=st.global_manifolds)
     per_stage = {}
     for name, ms in stages.items():
         b = algorithmic_bytes(name, **counts)

@@ -82,7 +82,7 @@ class Impulses(ctypes.Structure):
 class StepStats(ctypes.Structure):
     _fields_ = [("bodies", ctypes.c_uint32), ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32),
                 ("contacts", ctypes.c_uint32), ("colors", ctypes.c_uint32), ("steps", ctypes.c_uint32),
-                ("max_penetration", ctypes.c_float), ("pad", ctypes.c_float)]
+                ("max_penetration", ctypes.c_float), ("global_manifolds", ctypes.c_uint32)]
 
 
 # Every entry point of include/bepu_gpu.h: name -> (restype, argtypes)
@@ -106,6 +106,7 @@ EXPORTS = {
     "bepu_get_stage_times": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, _u32p,
                                             ctypes.POINTER(ctypes.c_char_p)]),
     "bepu_set_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "bepu_set_broadphase": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

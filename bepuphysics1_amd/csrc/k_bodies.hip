@@ -21,9 +21,10 @@ __device__ __forceinline__ uint32_t f2ord(float f) {
 __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bepu_settings& s = w.s;
-    float cx = 0, cy = 0, cz = 0;
+    float cx = 0, cy = 0, cz = 0, side = 0;
     bool valid = i < w.nbodies;
     if (valid) {
+        w.uf_parent[i] = i;  // union-find forest of this step's islands
         bool dyn = (w.bflags[i] & 1u) != 0;
         float4 P = w.pos[i];
         float4 Q = w.quat[i];
@@ -104,52 +105,77 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
         cx = 0.5f * (mn.x + mx.x);
         cy = 0.5f * (mn.y + mx.y);
         cz = 0.5f * (mn.z + mx.z);
+        side = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
+        if (!(side >= 0.0f && side < 3.0e38f)) side = 0.0f;  // a non-finite box does not steer the grid size
     }
-    // Scene bounds of AABB centres: wave then block reduction; one partial per
-    // block, folded by k_bounds_reduce (no global atomics).
+    // Scene bounds of AABB centres and the sum of largest sides: wave then
+    // block reduction; one partial per block, folded by k_bounds_reduce (no
+    // global atomics, fixed summation order).
     __shared__ uint32_t part[4][6];
+    __shared__ float pside[4];
     uint32_t v6[6] = {valid ? f2ord(cx) : 0xffffffffu, valid ? f2ord(cy) : 0xffffffffu,
                       valid ? f2ord(cz) : 0xffffffffu, valid ? f2ord(cx) : 0u, valid ? f2ord(cy) : 0u,
                       valid ? f2ord(cz) : 0u};
-    for (int off = 32; off > 0; off >>= 1)
+    for (int off = 32; off > 0; off >>= 1) {
         for (int k = 0; k < 3; ++k) {
             v6[k] = min(v6[k], (uint32_t)__shfl_xor((int)v6[k], off));
             v6[3 + k] = max(v6[3 + k], (uint32_t)__shfl_xor((int)v6[3 + k], off));
         }
+        side += __shfl_xor(side, off);
+    }
     int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 0) {
         for (int k = 0; k < 6; ++k) part[wv][k] = v6[k];
+        pside[wv] = side;
+    }
     __syncthreads();
     if (threadIdx.x < 6) {
         int k = threadIdx.x;
         uint32_t r = part[0][k];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = k < 3 ? min(r, part[q][k]) : max(r, part[q][k]);
-        w.bounds_part[blockIdx.x * 6 + k] = r;
+        w.bounds_part[blockIdx.x * 7 + k] = r;
+    } else if (threadIdx.x == 6) {
+        float t = pside[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) t += pside[q];
+        w.bounds_part[blockIdx.x * 7 + 6] = __float_as_uint(t);
     }
 }
 
 __global__ void __launch_bounds__(256) k_bounds_reduce(DeviceWorld w, uint32_t nparts) {
     __shared__ uint32_t red[6][256];
+    __shared__ float rs[256];
     uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0, 0, 0};
-    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x)
+    float sum = 0;
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) {
         for (int k = 0; k < 3; ++k) {
-            lo[k] = min(lo[k], w.bounds_part[p * 6 + k]);
-            hi[k] = max(hi[k], w.bounds_part[p * 6 + 3 + k]);
+            lo[k] = min(lo[k], w.bounds_part[p * 7 + k]);
+            hi[k] = max(hi[k], w.bounds_part[p * 7 + 3 + k]);
         }
+        sum += __uint_as_float(w.bounds_part[p * 7 + 6]);
+    }
     for (int k = 0; k < 3; ++k) {
         red[k][threadIdx.x] = lo[k];
         red[3 + k][threadIdx.x] = hi[k];
     }
+    rs[threadIdx.x] = sum;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
+        if ((int)threadIdx.x < s) {
             for (int k = 0; k < 3; ++k) {
                 red[k][threadIdx.x] = min(red[k][threadIdx.x], red[k][threadIdx.x + s]);
                 red[3 + k][threadIdx.x] = max(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
             }
+            rs[threadIdx.x] += rs[threadIdx.x + s];
+        }
         __syncthreads();
     }
     if (threadIdx.x < 6) w.bounds[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 6) {
+        // Grid base cell edge: twice the mean largest side (1 when degenerate).
+        float e = 2.0f * (rs[0] / (float)w.nbodies);
+        if (!(e > 1e-6f && e < 1e30f)) e = 1.0f;
+        w.bounds[6] = __float_as_uint(e);
+    }
 }
 
 // Entity.PreUpdatePosition (EntityBase.cs:1124-1155), discrete mode.
@@ -180,6 +206,19 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Start of a step: clears the counters and this step's pair table.
+__global__ void __launch_bounds__(256) k_step_reset(DeviceWorld w) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < w.hash_cap) w.hash_keys_cur[i] = ~0ull;
+    if (i < w.gt_cap) w.gt_key[i] = ~0ull;
+    if (i < CTR_COUNT) w.counters[ctr(i)] = 0;
+}
+
+void launch_step_reset(const DeviceWorld& w, hipStream_t st) {
+    uint32_t n = std::max<uint32_t>(std::max<uint32_t>(w.hash_cap, w.gt_cap), CTR_COUNT);
+    hipLaunchKernelGGL(k_step_reset, dim3((n + 255) / 256), dim3(256), 0, st, w);
 }
 
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st) {

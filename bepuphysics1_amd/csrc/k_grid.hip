@@ -1,0 +1,246 @@
+// k_grid.hip -- hierarchical hash-grid broadphase.
+//
+// Produces the same pair SET as the tree broadphase and the reference's
+// DynamicHierarchy (DynamicHierarchy.cs:194-214): closed-interval AABB
+// overlap (BoundingBox.cs:58-69) of two bodies, at least one dynamic
+// (kinematic-kinematic is NoBroadPhase, CollisionRules.cs:175-178).
+//
+// Levels l = 0..14 have cubic cells of edge e_l = s0 * 4^l, s0 = twice the
+// mean largest AABB side of the step.  A body lives at the finest level with
+// largest side <= 0.99 * e_l, in the cell holding its AABB min corner.  If two
+// boxes overlap, their min corners differ by at most the larger of their
+// sides per axis, so:
+//   * two bodies of one level sit in cells at most one apart per axis;
+//   * a body overlaps a body of a coarser level l' only if the latter's cell
+//     is within one cell (at level l') of the cell holding the finer body's
+//     min corner.
+// A body therefore finds all its pairs by probing its own cell and the 13
+// "forward" neighbour cells at its level (a same-level pair is reported once:
+// through the forward neighbour, or by the smaller sorted index inside one
+// cell) and the 27 cells around its min corner at every coarser non-empty
+// level (a cross-level pair is reported by the finer body).  Cell indices are
+// computed in double precision, so the one-cell bound holds for coordinates
+// up to ~1e13 cell edges from the origin.
+//
+// Bodies are sorted by a 32-bit hash of their cell key (radix sort); a hash
+// table maps each cell key to its hash run in the sorted order.  Candidates
+// whose own cell key differs from the probed one (hash collisions, or cells
+// that alias after the 20-bit wrap of the packed coordinates) are skipped, so
+// no pair is tested twice.  Every probe is independent of the others, which
+// keeps the chain of dependent loads per body at three (table, key, box).
+#include "pairbuf.h"
+
+namespace bepu {
+
+constexpr int kGridLevels = 15;      // level fits in 4 bits and a key is never ~0
+constexpr float kFit = 0.99f;
+constexpr uint64_t kEmptyCell = ~0ull;
+constexpr int kQueryBlock = 256;     // queries per workgroup (one pair-list atomic per workgroup)
+constexpr int kQueryBuf = 2048;      // LDS pair buffer; overflow goes straight to the global list
+
+__device__ __forceinline__ int grid_level(float side, float s0) {
+    float lim = s0 * kFit;
+    int l = 0;
+    while (l < kGridLevels - 1 && !(side <= lim)) {
+        lim *= 4.0f;
+        ++l;
+    }
+    return l;
+}
+
+__device__ __forceinline__ double level_edge(float s0, int l) { return (double)s0 * (double)(1ull << (2 * l)); }
+
+__device__ __forceinline__ int64_t cell_of(float m, double edge) {
+    double q = floor((double)m / edge);
+    q = fmin(fmax(q, -1e15), 1e15);
+    return (int64_t)q;
+}
+
+__device__ __forceinline__ uint64_t cell_key(int l, uint64_t x, uint64_t y, uint64_t z) {
+    return ((uint64_t)l << 60) | ((x & 0xFFFFFull) << 40) | ((y & 0xFFFFFull) << 20) | (z & 0xFFFFFull);
+}
+
+__device__ __forceinline__ uint32_t cell_hash(uint64_t key) { return (uint32_t)(mix64(key) >> 32); }
+
+__device__ __forceinline__ float max_side(float4 mn, float4 mx) {
+    return fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
+}
+
+// Level and cell of every body; the sort key is the cell hash.
+__global__ void __launch_bounds__(256) k_grid_cells(DeviceWorld w) {
+    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    bool alive = b < w.nbodies;
+    uint32_t lmask = 0;
+    if (alive) {
+        float4 mn = w.bmin[b], mx = w.bmax[b];
+        float s0 = __uint_as_float(w.bounds[6]);
+        int l = grid_level(max_side(mn, mx), s0);
+        double e = level_edge(s0, l);
+        uint64_t key = cell_key(l, (uint64_t)cell_of(mn.x, e), (uint64_t)cell_of(mn.y, e), (uint64_t)cell_of(mn.z, e));
+        w.cell_key[b] = key;
+        w.morton[b] = cell_hash(key);
+        w.leaf_ids_tmp[b] = b;
+        lmask = 1u << l;
+    }
+    __shared__ uint32_t s_mask;
+    if (threadIdx.x == 0) s_mask = 0;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) lmask |= __shfl_xor(lmask, off);
+    if (__lane_id() == 0 && lmask) atomicOr(&s_mask, lmask);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_mask) atomicOr(&w.counters[ctr(CTR_LEVELS)], s_mask);
+}
+
+__device__ __forceinline__ void table_insert(DeviceWorld& w, uint64_t key, uint64_t val) {
+    uint32_t mask = w.gt_cap - 1, s = (uint32_t)mix64(key) & mask;
+    for (uint32_t p = 0; p < w.gt_cap; ++p) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&w.gt_key[s], kEmptyCell, key);
+        if (prev == kEmptyCell) {
+            w.gt_val[s] = val;
+            return;
+        }
+        if (prev == key) return;
+        s = (s + 1) & mask;
+    }
+    w.counters[ctr(CTR_OVERFLOW)] = 6;
+}
+
+// Sorted leaves (box, body id and dynamic flag in max.w, cell key) and the
+// cell table: the first index of every hash run inserts the run's keys.
+__global__ void __launch_bounds__(256) k_grid_gather(DeviceWorld w) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = w.nbodies;
+    if (j >= n) return;
+    uint32_t b = w.leaf_ids[j];
+    float4 mn = w.bmin[b], mx = w.bmax[b];
+    bool dyn = (w.bflags[b] & 1u) != 0;
+    mx.w = __uint_as_float(b | (dyn ? 0x80000000u : 0u));
+    w.node_min[j] = mn;
+    w.node_max[j] = mx;
+    w.grid_key[j] = w.cell_key[b];
+    uint32_t h = w.morton_sorted[j];
+    if (j == 0 || w.morton_sorted[j - 1] != h) {
+        uint32_t e = j + 1;
+        while (e < n && w.morton_sorted[e] == h) ++e;
+        uint64_t val = ((uint64_t)j << 32) | (uint64_t)(e - j);
+        for (uint32_t t = j; t < e; ++t) table_insert(w, w.cell_key[w.leaf_ids[t]], val);
+    }
+}
+
+__device__ __forceinline__ bool table_find(const DeviceWorld& w, uint64_t key, uint64_t first, uint32_t slot,
+                                           uint32_t& start, uint32_t& count) {
+    uint32_t mask = w.gt_cap - 1, s = slot;
+    uint64_t k = first;
+    for (uint32_t p = 0; p < w.gt_cap; ++p) {
+        if (k == key) {
+            uint64_t v = w.gt_val[s];
+            start = (uint32_t)(v >> 32);
+            count = (uint32_t)v;
+            return true;
+        }
+        if (k == kEmptyCell) return false;
+        s = (s + 1) & mask;
+        k = w.gt_key[s];
+    }
+    return false;
+}
+
+// Own cell first, then the 13 neighbours that are "forward" in (z, y, x) order.
+__constant__ int8_t c_fwd[14][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},
+                                    {-1, -1, 1}, {0, -1, 1}, {1, -1, 1}, {-1, 0, 1}, {0, 0, 1},
+                                    {1, 0, 1},   {-1, 1, 1}, {0, 1, 1},  {1, 1, 1}};
+
+struct GridQuery {
+    float4 qmin, qmax;
+    uint32_t j, body;
+    bool dyn;
+};
+
+// Tests the candidates of one probed cell; `own` restricts to larger sorted
+// indices (the query's own cell at its own level).
+__device__ __forceinline__ void probe_cell(DeviceWorld& w, const GridQuery& q, uint64_t key, uint64_t first,
+                                           uint32_t slot, bool own, uint64_t* buf, uint32_t* buf_n) {
+    uint32_t start = 0, count = 0;
+    if (!table_find(w, key, first, slot, start, count)) return;
+    for (uint32_t t = start; t < start + count; ++t) {
+        if (own && t <= q.j) continue;
+        if (w.grid_key[t] != key) continue;
+        float4 cmin = w.node_min[t], cmax = w.node_max[t];
+        uint32_t cb = __float_as_uint(cmax.w);
+        bool hit = (q.dyn || (cb >> 31)) && overlap(q.qmin, q.qmax, cmin, cmax);
+        if (hit) buf_push(w, buf, buf_n, kQueryBuf, true, pair_key(q.body, cb & 0x7fffffffu));
+    }
+}
+
+__global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
+    __shared__ uint64_t buf[kQueryBuf];
+    __shared__ uint32_t buf_n, base_slot;
+    if (threadIdx.x == 0) buf_n = 0;
+    __syncthreads();
+    const uint32_t j = blockIdx.x * kQueryBlock + threadIdx.x;
+    if (j < w.nbodies) {
+        GridQuery q;
+        q.j = j;
+        q.qmin = w.node_min[j];
+        q.qmax = w.node_max[j];
+        uint32_t qb = __float_as_uint(q.qmax.w);
+        q.body = qb & 0x7fffffffu;
+        q.dyn = (qb >> 31) != 0;
+        const uint64_t qkey = w.grid_key[j];
+        const int l = (int)(qkey >> 60);
+        const uint32_t mask = w.gt_cap - 1;
+        // Own level: own cell and forward neighbours (coordinates wrap with the key).
+        {
+            const uint64_t cx = (qkey >> 40) & 0xFFFFFull, cy = (qkey >> 20) & 0xFFFFFull, cz = qkey & 0xFFFFFull;
+            uint64_t pk[14], first[14];
+            uint32_t ps[14];
+#pragma unroll
+            for (int o = 0; o < 14; ++o) {
+                pk[o] = cell_key(l, cx + (uint64_t)(int64_t)c_fwd[o][0], cy + (uint64_t)(int64_t)c_fwd[o][1],
+                                 cz + (uint64_t)(int64_t)c_fwd[o][2]);
+                ps[o] = (uint32_t)mix64(pk[o]) & mask;
+                first[o] = w.gt_key[ps[o]];
+            }
+#pragma unroll
+            for (int o = 0; o < 14; ++o) probe_cell(w, q, pk[o], first[o], ps[o], o == 0, buf, &buf_n);
+        }
+        // Coarser non-empty levels: the 27 cells around the min corner.
+        uint32_t coarse = w.counters[ctr(CTR_LEVELS)] & ~((2u << l) - 1u);
+        const float s0 = __uint_as_float(w.bounds[6]);
+        while (coarse) {
+            const int l2 = __ffs(coarse) - 1;
+            coarse &= coarse - 1;
+            const double e = level_edge(s0, l2);
+            const uint64_t cx = (uint64_t)cell_of(q.qmin.x, e), cy = (uint64_t)cell_of(q.qmin.y, e),
+                           cz = (uint64_t)cell_of(q.qmin.z, e);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    uint64_t pk[3], first[3];
+                    uint32_t ps[3];
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        pk[dx + 1] = cell_key(l2, cx + (uint64_t)(int64_t)dx, cy + (uint64_t)(int64_t)dy,
+                                              cz + (uint64_t)(int64_t)dz);
+                        ps[dx + 1] = (uint32_t)mix64(pk[dx + 1]) & mask;
+                        first[dx + 1] = w.gt_key[ps[dx + 1]];
+                    }
+#pragma unroll
+                    for (int o = 0; o < 3; ++o) probe_cell(w, q, pk[o], first[o], ps[o], false, buf, &buf_n);
+                }
+        }
+    }
+    buf_flush<kQueryBlock>(w, buf, &buf_n, kQueryBuf, &base_slot);
+}
+
+void launch_grid_cells(const DeviceWorld& w, hipStream_t st) {
+    uint32_t n = w.nbodies;
+    hipLaunchKernelGGL(k_grid_cells, dim3((n + 255) / 256), dim3(256), 0, st, w);
+}
+
+void launch_grid_after_sort(const DeviceWorld& w, hipStream_t st) {
+    uint32_t n = w.nbodies;
+    hipLaunchKernelGGL(k_grid_gather, dim3((n + 255) / 256), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_grid_query, dim3((n + kQueryBlock - 1) / kQueryBlock), dim3(kQueryBlock), 0, st, w);
+}
+
+}  // namespace bepu

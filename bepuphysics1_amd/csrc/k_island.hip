@@ -4,297 +4,353 @@
 // Constraints only couple through dynamic bodies, so the connected components
 // of the constraint graph over dynamic bodies (simulation islands,
 // DeactivationManagement/SimulationIsland.cs) can be solved independently.
-// Small and medium islands are each solved by one wavefront / workgroup:
-// colouring, prestep, warm start and all iterations run with the island's
-// body velocities and inverse inertias in LDS and each constraint's rows in
-// registers; colours are separated by workgroup barriers.  Islands too large
-// for LDS go to the global path (k_solver.hip).  Both paths colour with the
-// same rule and solve colour by colour, so the result does not depend on
-// which path an island takes.
+//   * small islands (<= 64 constraints) are packed several to a wavefront:
+//     an island of nc constraints belongs to class k = min(64, 64 / nc) and
+//     a bundle holds k islands of one class, 64 / k lanes each;
+//   * medium islands (<= 256 constraints) take one 256-thread workgroup;
+//   * larger islands go to the global path (k_solver.hip).
+// Inside a bundle or workgroup, colouring, prestep, warm start and all
+// iterations run with the bodies' velocities and inverse inertias in LDS and
+// each constraint's rows in registers; colours are separated by workgroup
+// barriers.  Every path colours with the same rule and solves colour by
+// colour, so the result does not depend on the path an island takes or on
+// which islands share a bundle.
 #include <hipcub/hipcub.hpp>
 
 #include "solver_rows.h"
+#include "wave.h"
 
 namespace bepu {
 
-constexpr int kSmallBlock = 64, kSmallBodies = 64;
-constexpr int kMediumBlock = 256, kMediumBodies = 512;
+constexpr int kPackLanes = 64;                  // one wavefront per bundle
+constexpr int kPackDyn = 128;                   // dynamic slots: sum of nb <= 64 + islands <= 128
+constexpr int kPackSlots = kPackDyn + kPackLanes;
+constexpr int kMediumBlock = 256;
+constexpr int kMediumDyn = 264;                 // nb <= nc + 1 <= 257
+constexpr int kMediumSlots = kMediumDyn + kMediumBlock;
+constexpr uint64_t kConsOne = 1ull << 32;       // constraint count lives in the high word
 
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
+
+// Tier of an island with nc constraints and nb dynamic bodies.  A connected
+// island has nb <= nc + 1; the check keeps the slot layout safe regardless.
+__device__ __forceinline__ int island_tier(uint32_t nc, uint32_t nb) {
+    if (nc == 0) return 0;
+    if (nb > nc + 1) return 3;
+    if (nc <= (uint32_t)kPackLanes) return 1;
+    if (nc <= (uint32_t)kMediumBlock) return 2;
+    return 3;
 }
 
-// Union-find with hooking of the larger root under the smaller one, so every
-// island's representative is its smallest dynamic body id.
-__device__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
-    while (true) {
-        uint32_t p = ld_agent(&parent[x]);
-        if (p == x) return x;
-        uint32_t g = ld_agent(&parent[p]);
-        if (g != p) __hip_atomic_store(&parent[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = p;
-    }
-}
+// Islands per bundle for a small island (class index = result - 1).
+__device__ __forceinline__ uint32_t pack_class(uint32_t nc) { return min(64u, 64u / nc); }
 
-__global__ void k_uf_init(DeviceWorld w) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < w.nbodies) w.uf_parent[b] = b;
-}
-
-__global__ void k_uf_union(DeviceWorld w) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t m = min(w.counters[CTR_CONS], w.pair_cap);
-    if (c >= m) return;
-    const PairState& s = w.pairs_cur[w.cons[c]];
-    if (!(w.bflags[s.a] & 1u) || !(w.bflags[s.b] & 1u)) return;
-    uint32_t a = s.a, b = s.b;
-    while (true) {
-        a = uf_find(w.uf_parent, a);
-        b = uf_find(w.uf_parent, b);
-        if (a == b) return;
-        if (a < b) { uint32_t t = a; a = b; b = t; }
-        uint32_t expected = a;
-        if (__hip_atomic_compare_exchange_strong(&w.uf_parent[a], &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return;
-    }
-}
-
-// Final island id of every dynamic body; counts bodies and constraints per island.
-__global__ void k_uf_compress(DeviceWorld w) {
-    uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= w.nbodies) return;
-    if (!(w.bflags[b] & 1u)) return;
-    uint32_t r = uf_find(w.uf_parent, b);
-    w.isl_root[b] = r;
-    atomicAdd(&w.isl_bcount[r], 1u);
-}
-
-__device__ __forceinline__ uint32_t cons_island(const DeviceWorld& w, const PairState& s) {
-    return (w.bflags[s.a] & 1u) ? w.isl_root[s.a] : w.isl_root[s.b];
-}
-
-__global__ void k_isl_count_cons(DeviceWorld w) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t m = min(w.counters[CTR_CONS], w.pair_cap);
-    if (c >= m) return;
-    atomicAdd(&w.isl_ccount[cons_island(w, w.pairs_cur[w.cons[c]])], 1u);
-}
-
-// Scatter constraints and bodies into island-contiguous lists (counting down
-// leaves the counts at zero for the next step).
-__global__ void k_isl_fill(DeviceWorld w) {
+// Island of every dynamic body and every constraint (a pair with contacts);
+// counts both per island in one packed word (constraints high, bodies low).
+__global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t m = min(w.counters[CTR_CONS], w.pair_cap);
-    if (i < m) {
-        uint32_t slot = w.cons[i];
-        uint32_t r = cons_island(w, w.pairs_cur[slot]);
-        w.isl_cons[w.isl_cstart[r] + atomicSub(&w.isl_ccount[r], 1u) - 1] = slot;
+    uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
+    bool body = i < w.nbodies && (w.bflags[i] & 1u);
+    uint32_t rb = body ? uf_find(w.uf_parent, i) : 0;
+    if (body) w.isl_root[i] = rb;
+    (void)wave_atomic_add<unsigned long long>((unsigned long long*)w.isl_count, rb, body, 1ull);
+    bool cons = false;
+    uint32_t rc = 0;
+    if (i < npairs) {
+        const PairState& s = w.pairs_cur[i];
+        cons = s.ccount > 0;
+        if (cons) {
+            rc = uf_find(w.uf_parent, (w.bflags[s.a] & 1u) ? s.a : s.b);
+            w.cons_root[i] = rc;
+        }
     }
-    if (i < w.nbodies && (w.bflags[i] & 1u)) {
-        uint32_t r = w.isl_root[i];
-        uint32_t pos = atomicSub(&w.isl_bcount[r], 1u) - 1;
-        w.isl_bodies[w.isl_bstart[r] + pos] = i;
+    (void)wave_atomic_add<unsigned long long>((unsigned long long*)w.isl_count, rc, cons, kConsOne);
+}
+
+// Scatters constraints and bodies into island-contiguous lists (counting down
+// leaves the packed counts at zero for the next step), classifies islands
+// into tiers, lists the global path's constraints and the medium islands,
+// and appends small islands to their packing class's list (one global
+// atomic per workgroup and class).
+__global__ void __launch_bounds__(256) k_isl_fill(DeviceWorld w) {
+    __shared__ uint32_t s_tmp[256 / 64 + 1];
+    __shared__ uint32_t s_cls[64], s_base[64];
+    if (threadIdx.x < 64) s_cls[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
+    unsigned long long* cnt = (unsigned long long*)w.isl_count;
+
+    bool cons = i < npairs && w.pairs_cur[i].ccount > 0;
+    uint32_t rc = cons ? w.cons_root[i] : 0;
+    unsigned long long oc = wave_atomic_add<unsigned long long>(cnt, rc, cons, 0ull - kConsOne);
+    bool global = false;
+    if (cons) {
+        uint64_t s0 = w.isl_start[rc], s1 = w.isl_start[rc + 1];
+        w.isl_cons[hi32(s0) + hi32(oc) - 1] = i;
+        global = island_tier(hi32(s1) - hi32(s0), lo32(s1) - lo32(s0)) == 3;
+    }
+    block_push<256>(w.gcons, &w.counters[ctr(CTR_GCONS)], global, i, s_tmp);
+
+    bool body = i < w.nbodies && (w.bflags[i] & 1u);
+    uint32_t rb = body ? w.isl_root[i] : 0;
+    unsigned long long ob = wave_atomic_add<unsigned long long>(cnt, rb, body, ~0ull);
+    bool medium = false, small = false;
+    uint32_t k = 0, rank = 0;
+    if (body) {
+        uint64_t s0 = w.isl_start[rb], s1 = w.isl_start[rb + 1];
+        uint32_t pos = lo32(ob) - 1;
+        w.isl_bodies[lo32(s0) + pos] = i;
         w.body_local[i] = pos;
+        if (rb == i) {
+            uint32_t nc = hi32(s1) - hi32(s0);
+            int tier = island_tier(nc, lo32(s1) - lo32(s0));
+            small = tier == 1;
+            medium = tier == 2;
+            if (small) {
+                k = pack_class(nc) - 1;
+                rank = atomicAdd(&s_cls[k], 1u);
+            }
+        }
     }
-}
-
-__device__ __forceinline__ void wave_push(uint32_t* list, uint32_t* counter, bool take, uint32_t value) {
-    uint64_t active = __ballot(1);
-    uint64_t mask = __ballot(take);
-    if (!mask) return;
-    int lane = __lane_id();
-    int leader = __ffsll((unsigned long long)active) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
-
-// Classifies islands by size: small (wavefront), medium (workgroup), large (global path).
-__global__ void k_isl_list(DeviceWorld w) {
-    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    bool root = r < w.nbodies && (w.bflags[r] & 1u) && w.isl_root[r] == r;
-    uint32_t nc = root ? w.isl_cstart[r + 1] - w.isl_cstart[r] : 0;
-    uint32_t nb = root ? w.isl_bstart[r + 1] - w.isl_bstart[r] : 0;
-    bool small = nc > 0 && nc <= (uint32_t)kSmallBlock && nb <= (uint32_t)kSmallBodies;
-    bool medium = nc > 0 && !small && nc <= (uint32_t)kMediumBlock && nb <= (uint32_t)kMediumBodies;
-    bool large = nc > 0 && !small && !medium;
-    if (r < w.nbodies) w.isl_tier[r] = small ? 1 : (medium ? 2 : (large ? 3 : 0));
-    wave_push(w.isl_small, &w.counters[CTR_ISL_SMALL], small, r);
-    wave_push(w.isl_medium, &w.counters[CTR_ISL_MEDIUM], medium, r);
-    wave_push(w.isl_large, &w.counters[CTR_ISL_LARGE], large, r);
-}
-
-// Constraints of large islands form the global path's list.
-__global__ void k_isl_large_cons(DeviceWorld w) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t m = min(w.counters[CTR_CONS], w.pair_cap);
-    bool take = false;
-    uint32_t slot = 0;
-    if (c < m) {
-        slot = w.cons[c];
-        take = w.isl_tier[cons_island(w, w.pairs_cur[slot])] == 3;
+    block_push<256>(w.isl_medium, &w.counters[ctr(CTR_ISL_MEDIUM)], medium, i, s_tmp);
+    if (threadIdx.x < 64) {
+        uint32_t c = s_cls[threadIdx.x];
+        s_base[threadIdx.x] = c ? atomicAdd(&w.counters[ctr(CTR_CLS0 + threadIdx.x)], c) : 0;
     }
-    wave_push(w.gcons, &w.counters[CTR_GCONS], take, slot);
+    __syncthreads();
+    if (small) w.cls_list[(size_t)k * w.cls_cap + s_base[k] + rank] = i;
 }
 
 // ---------------------------------------------------------------------------
-// Island solver: one island per workgroup (BLOCK threads, one constraint per
-// thread), persistent over the island list of its tier.
+// Island solver core shared by the packed and the medium kernels.
 
-template <int BLOCK, int MAXB>
-__global__ void __launch_bounds__(BLOCK) k_island(DeviceWorld w, int tier) {
-    // Body slots: [0, nb) the island's dynamic bodies (written back at the
-    // end), then one read-only slot per constraint that touches a kinematic body.
-    __shared__ float4 s_v[MAXB + BLOCK], s_w[MAXB + BLOCK], s_i0[MAXB + BLOCK], s_i1[MAXB + BLOCK],
-        s_i2[MAXB + BLOCK];
-    __shared__ int s_color[BLOCK];
-    __shared__ int s_la[BLOCK], s_lb[BLOCK];
-    __shared__ uint64_t s_prio[BLOCK];
-    const uint32_t* list = tier == 1 ? w.isl_small : w.isl_medium;
-    uint32_t count = w.counters[tier == 1 ? CTR_ISL_SMALL : CTR_ISL_MEDIUM];
+template <int BLOCK, int NSLOT>
+struct IslandLds {
+    float4 v[NSLOT], w[NSLOT], i0[NSLOT], i1[NSLOT], i2[NSLOT];
+    int da[BLOCK], db[BLOCK];   // dynamic body slot of each lane's constraint, -1 if kinematic
+    int color[BLOCK];
+    uint64_t prio[BLOCK];
+    int red[BLOCK / 64];
+};
+
+template <int BLOCK, int NSLOT>
+__device__ __forceinline__ void lds_body_load(IslandLds<BLOCK, NSLOT>& L, const DeviceWorld& w, int l, uint32_t b) {
+    L.v[l] = w.linv[b];
+    L.w[l] = w.angv[b];
+    L.i0[l] = w.ii0[b];
+    L.i1[l] = w.ii1[b];
+    L.i2[l] = w.ii2[b];
+}
+
+// Solves the constraints held by the lanes of one workgroup.  On entry the
+// dynamic bodies are in slots [0, DYN), each lane's kinematic body (if any)
+// in slot DYN + lane, and L.da / L.db / L.prio are written for every lane;
+// the caller has synchronised.  A lane's neighbours (constraints sharing a
+// dynamic body) are among lanes [clo, chi).
+template <int BLOCK, int DYN, int NSLOT>
+__device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSLOT>& L, bool have, uint32_t slot,
+                                            uint32_t ga, uint32_t gb, int sa, int sb, int clo, int chi) {
     const int tid = threadIdx.x;
     const bepu_settings& st = w.s;
-    for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
-        uint32_t r = list[it];
-        uint32_t cs = w.isl_cstart[r], nc = w.isl_cstart[r + 1] - cs;
-        uint32_t bs = w.isl_bstart[r], nb = w.isl_bstart[r + 1] - bs;
-        for (uint32_t t = tid; t < nb; t += BLOCK) {
-            uint32_t b = w.isl_bodies[bs + t];
-            s_v[t] = w.linv[b];
-            s_w[t] = w.angv[b];
-            s_i0[t] = w.ii0[b];
-            s_i1[t] = w.ii1[b];
-            s_i2[t] = w.ii2[b];
-        }
-        bool have = (uint32_t)tid < nc;
-        uint32_t slot = have ? w.isl_cons[cs + tid] : 0;
-        uint32_t ga = 0, gb = 0;
-        int la = -1, lb = -1;   // island-local index of a dynamic body, -1 if kinematic
-        int sa = 0, sb = 0;     // LDS body slot used by the solve phases
-        if (have) {
-            const PairState& s = w.pairs_cur[slot];
-            ga = s.a;
-            gb = s.b;
-            if (w.bflags[ga] & 1u) la = (int)w.body_local[ga];
-            if (w.bflags[gb] & 1u) lb = (int)w.body_local[gb];
-            s_prio[tid] = mix64(s.key);
-            // A constraint has at most one kinematic body; it gets slot nb + tid.
-            int ext = (int)nb + tid;
-            sa = la >= 0 ? la : ext;
-            sb = lb >= 0 ? lb : ext;
-            if (la < 0 || lb < 0) {
-                uint32_t g = la < 0 ? ga : gb;
-                s_v[ext] = w.linv[g];
-                s_w[ext] = w.angv[g];
-                s_i0[ext] = w.ii0[g];
-                s_i1[ext] = w.ii1[g];
-                s_i2[ext] = w.ii2[g];
+    // Colouring: smallest colour unused by higher-priority neighbours; rounds
+    // until every constraint is set (each round sets the highest-priority
+    // uncoloured constraint of every island at least).
+    int color = -1;
+    const int la = have ? L.da[tid] : -1, lb = have ? L.db[tid] : -1;
+    const uint64_t prio = have ? L.prio[tid] : 0;
+    for (int round = 0; round <= BLOCK; ++round) {
+        bool need = have && color < 0;
+        if (need) {
+            uint64_t m0 = 0, m1 = 0;
+            bool ready = true;
+            for (int k = clo; k < chi; ++k) {
+                if (k == tid) continue;
+                int ka = L.da[k], kb = L.db[k];
+                bool nb_ = (ka >= 0 && (ka == la || ka == lb)) || (kb >= 0 && (kb == la || kb == lb));
+                if (!nb_ || L.prio[k] < prio) continue;
+                int ck = __hip_atomic_load(&L.color[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (ck < 0) { ready = false; break; }
+                if (ck < 64) m0 |= 1ull << ck;
+                else m1 |= 1ull << (ck - 64);
+            }
+            if (ready) {
+                if (~m0) color = __ffsll((unsigned long long)~m0) - 1;
+                else if (~m1) color = 64 + __ffsll((unsigned long long)~m1) - 1;
+                else { color = kMaxColors - 1; w.counters[ctr(CTR_OVERFLOW)] = 4; }
             }
         }
-        s_la[tid] = la;
-        s_lb[tid] = lb;
-        s_color[tid] = -1;
         __syncthreads();
-        // Colouring: smallest colour unused by higher-priority neighbours
-        // (constraints sharing a dynamic body); rounds until every one is set.
-        int color = -1;
-        uint64_t prio = have ? s_prio[tid] : 0;
-        for (int round = 0; round <= (int)nc; ++round) {
-            bool need = have && color < 0;
-            if (need) {
-                uint64_t m0 = 0, m1 = 0;
-                bool ready = true;
-                for (uint32_t k = 0; k < nc; ++k) {
-                    if ((int)k == tid) continue;
-                    int ka = s_la[k], kb = s_lb[k];
-                    bool nb_ = (ka >= 0 && (ka == la || ka == lb)) || (kb >= 0 && (kb == la || kb == lb));
-                    if (!nb_ || s_prio[k] < prio) continue;
-                    int ck = __hip_atomic_load(&s_color[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (ck < 0) { ready = false; break; }
-                    if (ck < 64) m0 |= 1ull << ck;
-                    else m1 |= 1ull << (ck - 64);
-                }
-                if (ready) {
-                    if (~m0) color = __ffsll((unsigned long long)~m0) - 1;
-                    else if (~m1) color = 64 + __ffsll((unsigned long long)~m1) - 1;
-                    else { color = kMaxColors - 1; w.counters[CTR_OVERFLOW] = 4; }
-                }
-            }
-            __syncthreads();
-            if (need && color >= 0) __hip_atomic_store(&s_color[tid], color, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (!__syncthreads_or(have && color < 0)) break;
+        if (need && color >= 0) __hip_atomic_store(&L.color[tid], color, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (!__syncthreads_or(have && color < 0)) break;
+    }
+    // Colours used = max colour + 1 over the workgroup.
+    int ncol = have ? color + 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) ncol = max(ncol, __shfl_xor(ncol, off));
+    if (BLOCK > 64) {
+        if ((tid & 63) == 0) L.red[tid >> 6] = ncol;
+        __syncthreads();
+        ncol = 0;
+        for (int k = 0; k < BLOCK / 64; ++k) ncol = max(ncol, L.red[k]);
+    }
+
+    RowConst row;
+    RowVar var;
+    BodyV A, B;
+    auto load = [&](int l, BodyV& x) {
+        float4 i0 = L.i0[l], i1 = L.i1[l], i2 = L.i2[l];
+        x.v = f3_of(L.v[l]);
+        x.w = f3_of(L.w[l]);
+        x.I = m3{i0.x, i0.y, i0.z, i1.x, i1.y, i1.z, i2.x, i2.y, i2.z};
+        x.im = i0.w;
+        x.dyn = l < DYN;
+    };
+    auto store = [&](int l, const BodyV& x) {
+        if (l < DYN) {
+            L.v[l] = f4_of(x.v, 0.0f);
+            L.w[l] = f4_of(x.w, 0.0f);
         }
-        int ncol = __syncthreads_or(0) ? 0 : 0;
-        {
-            // number of colours of this island = max colour + 1
-            int mine = have ? color + 1 : 0;
-            for (int off = 32; off > 0; off >>= 1) mine = max(mine, __shfl_xor(mine, off));
-            if (BLOCK > 64) {
-                if ((tid & 63) == 0) s_color[tid >> 6] = mine;
-                __syncthreads();
-                mine = 0;
-                for (int k = 0; k < BLOCK / 64; ++k) mine = max(mine, s_color[k]);
-            }
-            ncol = mine;
-        }
-        if (tid == 0) atomicMax(&w.counters[CTR_COLORS], (uint32_t)ncol);
-        // Prestep (reads velocities; no body is modified before every thread is done).
-        RowConst row;
-        RowVar var;
-        BodyV A, B;
-        const int nbi = (int)nb;
-        auto load = [&](int l, BodyV& x) {
-            float4 i0 = s_i0[l], i1 = s_i1[l], i2 = s_i2[l];
-            x.v = f3_of(s_v[l]);
-            x.w = f3_of(s_w[l]);
-            x.I = m3{i0.x, i0.y, i0.z, i1.x, i1.y, i1.z, i2.x, i2.y, i2.z};
-            x.im = i0.w;
-            x.dyn = l < nbi;
-        };
-        auto store = [&](int l, const BodyV& x) {
-            if (l < nbi) {
-                s_v[l] = f4_of(x.v, 0.0f);
-                s_w[l] = f4_of(x.w, 0.0f);
-            }
-        };
-        if (have) {
-            PairState& s = w.pairs_cur[slot];
+    };
+    // Prestep reads velocities only; no body changes before every lane is done.
+    if (have) {
+        PairState& s = w.pairs_cur[slot];
+        load(sa, A);
+        load(sb, B);
+        prestep_row(st, s, f3_of(w.pos[ga]), f3_of(w.pos[gb]), A, B, row, var);
+        s.color = color;
+    }
+    __syncthreads();
+    for (int col = 0; col < ncol; ++col) {
+        if (have && color == col) {
             load(sa, A);
             load(sb, B);
-            prestep_row(st, s, f3_of(w.pos[ga]), f3_of(w.pos[gb]), A, B, row, var);
-            s.color = color;
+            warm_row(row, var, A, B);
+            store(sa, A);
+            store(sb, B);
         }
         __syncthreads();
+    }
+    const float mi = st.min_impulse;
+    const int mic = st.min_iteration_count;
+    for (int iter = 0; iter < st.iteration_limit; ++iter)
         for (int col = 0; col < ncol; ++col) {
             if (have && color == col) {
                 load(sa, A);
                 load(sb, B);
-                warm_row(row, var, A, B);
+                solve_row(row, var, A, B, mi, mic);
                 store(sa, A);
                 store(sb, B);
             }
             __syncthreads();
         }
-        const float mi = st.min_impulse;
-        const int mic = st.min_iteration_count;
-        for (int iter = 0; iter < st.iteration_limit; ++iter)
-            for (int col = 0; col < ncol; ++col) {
-                if (have && color == col) {
-                    load(sa, A);
-                    load(sb, B);
-                    solve_row(row, var, A, B, mi, mic);
-                    store(sa, A);
-                    store(sb, B);
-                }
-                __syncthreads();
-            }
-        if (have) writeback_row(w.pairs_cur[slot], var);
-        for (uint32_t t = tid; t < nb; t += BLOCK) {
+    if (have) writeback_row(w.pairs_cur[slot], var);
+}
+
+// Per-lane constraint set-up: global bodies, LDS slots (dynamic: base +
+// island-local index; kinematic: DYN + lane) and the neighbour-test entries.
+template <int BLOCK, int DYN, int NSLOT>
+__device__ __forceinline__ void setup_lane(const DeviceWorld& w, IslandLds<BLOCK, NSLOT>& L, bool have, uint32_t slot,
+                                           int dbase, uint32_t& ga, uint32_t& gb, int& sa, int& sb) {
+    const int tid = threadIdx.x;
+    int la = -1, lb = -1;
+    ga = gb = 0;
+    sa = sb = 0;
+    if (have) {
+        const PairState& s = w.pairs_cur[slot];
+        ga = s.a;
+        gb = s.b;
+        if (w.bflags[ga] & 1u) la = dbase + (int)w.body_local[ga];
+        if (w.bflags[gb] & 1u) lb = dbase + (int)w.body_local[gb];
+        L.prio[tid] = mix64(s.key);
+        // A constraint has at most one kinematic body; it gets slot DYN + lane.
+        int ext = DYN + tid;
+        sa = la >= 0 ? la : ext;
+        sb = lb >= 0 ? lb : ext;
+        if (la < 0 || lb < 0) lds_body_load(L, w, ext, la < 0 ? ga : gb);
+    }
+    L.da[tid] = la;
+    L.db[tid] = lb;
+    L.color[tid] = -1;
+}
+
+// Small islands: one bundle of same-class islands per wavefront, persistent
+// over the bundles.
+__global__ void __launch_bounds__(kPackLanes) __attribute__((amdgpu_waves_per_eu(2))) k_island_pack(DeviceWorld w) {
+    __shared__ IslandLds<kPackLanes, kPackSlots> L;
+    __shared__ uint32_t s_cnt[64], s_bund[65];
+    const int tid = threadIdx.x;
+    {
+        // Bundles per class (class t packs t + 1 islands) and their prefix.
+        uint32_t c = w.counters[ctr(CTR_CLS0 + tid)];
+        uint32_t nb = (c + (uint32_t)tid) / ((uint32_t)tid + 1);
+        uint32_t a = nb;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t x = __shfl_up(a, off);
+            if (tid >= off) a += x;
+        }
+        s_cnt[tid] = c;
+        s_bund[tid] = a - nb;
+        if (tid == 63) s_bund[64] = a;
+    }
+    __syncthreads();
+    const uint32_t total = s_bund[64];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        // Class of bundle g: the last class whose first bundle is <= g.
+        int lo = 0, hi = 64;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (s_bund[mid] <= g) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t per = (uint32_t)lo + 1;
+        const uint32_t lanes = 64u / per;               // >= nc of every island of the class
+        const uint32_t first = (g - s_bund[lo]) * per;
+        const uint32_t p = (uint32_t)tid / lanes, q = (uint32_t)tid % lanes;
+        const bool isl = p < per && first + p < s_cnt[lo];
+        uint32_t r = isl ? w.cls_list[(size_t)lo * w.cls_cap + first + p] : 0;
+        uint64_t s0 = isl ? w.isl_start[r] : 0, s1 = isl ? w.isl_start[r + 1] : 0;
+        const uint32_t cs = hi32(s0), nc = hi32(s1) - cs, bs = lo32(s0), nb = lo32(s1) - bs;
+        const int dbase = (int)(p * (lanes + 1));       // dynamic slots [dbase, dbase + nb), nb <= lanes + 1
+        for (uint32_t t = q; isl && t < nb; t += lanes) lds_body_load(L, w, dbase + (int)t, w.isl_bodies[bs + t]);
+        const bool have = isl && q < nc;
+        const uint32_t slot = have ? w.isl_cons[cs + q] : 0;
+        uint32_t ga, gb;
+        int sa, sb;
+        setup_lane<kPackLanes, kPackDyn, kPackSlots>(w, L, have, slot, dbase, ga, gb, sa, sb);
+        __syncthreads();
+        const int clo = (int)(p * lanes), chi = clo + (int)(isl ? nc : 0);
+        solve_lanes<kPackLanes, kPackDyn, kPackSlots>(w, L, have, slot, ga, gb, sa, sb, clo, chi);
+        for (uint32_t t = q; isl && t < nb; t += lanes) {
             uint32_t b = w.isl_bodies[bs + t];
-            w.linv[b] = s_v[t];
-            w.angv[b] = s_w[t];
+            w.linv[b] = L.v[dbase + (int)t];
+            w.angv[b] = L.w[dbase + (int)t];
+        }
+        __syncthreads();
+    }
+}
+
+// Medium islands: one island per workgroup, persistent over the list.
+__global__ void __launch_bounds__(kMediumBlock) k_island_medium(DeviceWorld w) {
+    __shared__ IslandLds<kMediumBlock, kMediumSlots> L;
+    const int tid = threadIdx.x;
+    const uint32_t count = w.counters[ctr(CTR_ISL_MEDIUM)];
+    for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
+        uint32_t r = w.isl_medium[it];
+        uint64_t s0 = w.isl_start[r], s1 = w.isl_start[r + 1];
+        const uint32_t cs = hi32(s0), nc = hi32(s1) - cs, bs = lo32(s0), nb = lo32(s1) - bs;
+        for (uint32_t t = tid; t < nb; t += kMediumBlock) lds_body_load(L, w, (int)t, w.isl_bodies[bs + t]);
+        const bool have = (uint32_t)tid < nc;
+        const uint32_t slot = have ? w.isl_cons[cs + tid] : 0;
+        uint32_t ga, gb;
+        int sa, sb;
+        setup_lane<kMediumBlock, kMediumDyn, kMediumSlots>(w, L, have, slot, 0, ga, gb, sa, sb);
+        __syncthreads();
+        solve_lanes<kMediumBlock, kMediumDyn, kMediumSlots>(w, L, have, slot, ga, gb, sa, sb, 0, (int)nc);
+        for (uint32_t t = tid; t < nb; t += kMediumBlock) {
+            uint32_t b = w.isl_bodies[bs + t];
+            w.linv[b] = L.v[t];
+            w.angv[b] = L.w[t];
         }
         __syncthreads();
     }
@@ -302,7 +358,8 @@ __global__ void __launch_bounds__(BLOCK) k_island(DeviceWorld w, int tier) {
 
 size_t island_scan_temp_bytes(uint32_t n) {
     size_t bytes = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const unsigned long long*)nullptr,
+                                           (unsigned long long*)nullptr, (int)n);
     return bytes;
 }
 
@@ -319,27 +376,20 @@ static int persistent_grid(const void* kernel, int block) {
 
 void launch_islands(const DeviceWorld& w, void* temp, size_t temp_bytes, hipStream_t st) {
     uint32_t n = w.nbodies, cap = w.pair_cap;
-    dim3 gb((n + 255) / 256), gc((cap + 255) / 256), b(256);
-    hipLaunchKernelGGL(k_uf_init, gb, b, 0, st, w);
-    hipLaunchKernelGGL(k_uf_union, gc, b, 0, st, w);
-    hipLaunchKernelGGL(k_uf_compress, gb, b, 0, st, w);
-    hipLaunchKernelGGL(k_isl_count_cons, gc, b, 0, st, w);
+    dim3 b(256), gm((std::max(n, cap) + 255) / 256);
+    hipLaunchKernelGGL(k_uf_roots, gm, b, 0, st, w);
     size_t bytes = temp_bytes;
-    (void)hipcub::DeviceScan::ExclusiveSum(temp, bytes, w.isl_ccount, w.isl_cstart, (int)(n + 1), st);
-    bytes = temp_bytes;
-    (void)hipcub::DeviceScan::ExclusiveSum(temp, bytes, w.isl_bcount, w.isl_bstart, (int)(n + 1), st);
-    dim3 gf((std::max(n, cap) + 255) / 256);
-    hipLaunchKernelGGL(k_isl_fill, gf, b, 0, st, w);
-    hipLaunchKernelGGL(k_isl_list, gb, b, 0, st, w);
-    hipLaunchKernelGGL(k_isl_large_cons, gc, b, 0, st, w);
+    (void)hipcub::DeviceScan::ExclusiveSum(temp, bytes, (unsigned long long*)w.isl_count,
+                                           (unsigned long long*)w.isl_start, (int)(n + 1), st);
+    hipLaunchKernelGGL(k_isl_fill, gm, b, 0, st, w);
 }
 
 void launch_island_solve(const DeviceWorld& w, hipStream_t st) {
-    static int g_small = 0, g_medium = 0;
-    if (!g_small) g_small = persistent_grid((const void*)k_island<kSmallBlock, kSmallBodies>, kSmallBlock);
-    if (!g_medium) g_medium = persistent_grid((const void*)k_island<kMediumBlock, kMediumBodies>, kMediumBlock);
-    hipLaunchKernelGGL((k_island<kSmallBlock, kSmallBodies>), dim3(g_small), dim3(kSmallBlock), 0, st, w, 1);
-    hipLaunchKernelGGL((k_island<kMediumBlock, kMediumBodies>), dim3(g_medium), dim3(kMediumBlock), 0, st, w, 2);
+    static int g_pack = 0, g_medium = 0;
+    if (!g_pack) g_pack = persistent_grid((const void*)k_island_pack, kPackLanes);
+    if (!g_medium) g_medium = persistent_grid((const void*)k_island_medium, kMediumBlock);
+    hipLaunchKernelGGL(k_island_pack, dim3(g_pack), dim3(kPackLanes), 0, st, w);
+    hipLaunchKernelGGL(k_island_medium, dim3(g_medium), dim3(kMediumBlock), 0, st, w);
 }
 
 }  // namespace bepu

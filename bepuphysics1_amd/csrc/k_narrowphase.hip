@@ -13,6 +13,7 @@
 // Manifolds: BoxContactManifold.Update (BoxContactManifold.cs:125-206),
 // SphereContactManifold / BoxSphereContactManifold.Update (…Manifold.cs:55-81).
 #include "world.h"
+#include "wave.h"
 
 namespace bepu {
 
@@ -678,7 +679,7 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     __shared__ float4 lds_pd[2][8 * kNpBlock];
     __shared__ int lds_id[2][8 * kNpBlock];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t npairs = min(w.counters[CTR_PAIRS], w.pair_cap);
+    uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= npairs) return;
     uint64_t key = w.pair_keys[t];
     int old = hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
@@ -693,15 +694,64 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
         update_single_pair(w, s);
     }
     w.pairs_cur[t] = s;
-    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[CTR_OVERFLOW] = 3;
-    if (s.ccount > 0) {
-        uint32_t c = atomicAdd(&w.counters[CTR_CONS], 1u);
-        w.cons[c] = t;
-        atomicAdd(&w.counters[CTR_CONTACTS], (uint32_t)s.ccount);
-        float mp = 0;
-        for (int i = 0; i < s.ccount; ++i) mp = fmaxf(mp, s.cpos[s.clist[i]].w);
-        atomicMax(&w.counters[CTR_MAXPEN], __float_as_uint(mp));
+    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
+    // A constraint (a pair with contacts) joins the islands of its two bodies
+    // when both are dynamic; the union-find forest was reset by this step's
+    // body pass.
+    if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
+}
+
+// Statistics of the last step, computed on request from its pair states
+// (kept out of the step: counters shared by every workgroup serialise):
+// constraints, contacts, deepest contact and colours used.
+__global__ void __launch_bounds__(256) k_stats(DeviceWorld w, uint32_t npairs) {
+    __shared__ uint32_t s_red[4][4];
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t cons = 0, contacts = 0, colors = 0;
+    float mp = 0;
+    if (t < npairs) {
+        const PairState& s = w.pairs_old[t];
+        if (s.ccount > 0) {
+            cons = 1;
+            contacts = (uint32_t)s.ccount;
+            for (int i = 0; i < s.ccount; ++i) mp = fmaxf(mp, s.cpos[s.clist[i]].w);
+            colors = (uint32_t)max(s.color + 1, 0);
+        }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        cons += __shfl_xor(cons, off);
+        contacts += __shfl_xor(contacts, off);
+        colors = max(colors, (uint32_t)__shfl_xor((int)colors, off));
+        mp = fmaxf(mp, __shfl_xor(mp, off));
+    }
+    int wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        s_red[wv][0] = cons;
+        s_red[wv][1] = contacts;
+        s_red[wv][2] = colors;
+        s_red[wv][3] = __float_as_uint(mp);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) {
+            cons += s_red[k][0];
+            contacts += s_red[k][1];
+            colors = max(colors, s_red[k][2]);
+            mp = fmaxf(mp, __uint_as_float(s_red[k][3]));
+        }
+        if (cons) atomicAdd(&w.counters[ctr(CTR_CONS)], cons);
+        if (contacts) atomicAdd(&w.counters[ctr(CTR_CONTACTS)], contacts);
+        if (colors) atomicMax(&w.counters[ctr(CTR_COLORS)], colors);
+        if (mp > 0) atomicMax(&w.counters[ctr(CTR_MAXPEN)], __float_as_uint(mp));
+    }
+}
+
+void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st) {
+    launch_fill_u32(w.counters + ctr(CTR_CONS), 0u, 1, st);
+    launch_fill_u32(w.counters + ctr(CTR_CONTACTS), 0u, 1, st);
+    launch_fill_u32(w.counters + ctr(CTR_COLORS), 0u, 1, st);
+    launch_fill_u32(w.counters + ctr(CTR_MAXPEN), 0u, 1, st);
+    if (npairs) hipLaunchKernelGGL(k_stats, dim3((npairs + 255) / 256), dim3(256), 0, st, w, npairs);
 }
 
 void launch_narrowphase(const DeviceWorld& w, hipStream_t st) {

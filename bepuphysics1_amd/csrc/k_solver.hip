@@ -24,7 +24,7 @@ namespace bepu {
 
 constexpr int kSolveBlock = 256;
 
-__device__ __forceinline__ uint32_t gcount(const DeviceWorld& w) { return min(w.counters[CTR_GCONS], w.pair_cap); }
+__device__ __forceinline__ uint32_t gcount(const DeviceWorld& w) { return min(w.counters[ctr(CTR_GCONS)], w.pair_cap); }
 
 __global__ void k_inc_count(DeviceWorld w) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -76,25 +76,25 @@ __device__ int try_color(const DeviceWorld& w, uint32_t c) {
 __global__ void __launch_bounds__(kSolveBlock) k_color(DeviceWorld w) {
     cg::grid_group grid = cg::this_grid();
     uint32_t m = gcount(w);
+    if (m == 0) return;  // uniform across the grid: no barrier is skipped by a subset
     uint32_t gsize = gridDim.x * blockDim.x;
     uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     // Each round colours at least the highest-priority uncoloured constraint,
     // so m + 1 rounds always suffice.  Three rotating counters keep a round's
     // count stable until every thread has read it.
     for (uint32_t round = 0; round <= m; ++round) {
-        uint32_t* remaining = &w.counters[CTR_UNCOLORED0 + (round % 3)];
-        if (tid == 0) w.counters[CTR_UNCOLORED0 + ((round + 1) % 3)] = 0;
+        uint32_t* remaining = &w.counters[ctr(CTR_UNCOLORED0 + (round % 3))];
+        if (tid == 0) w.counters[ctr(CTR_UNCOLORED0 + ((round + 1) % 3))] = 0;
         uint32_t left = 0;
         for (uint32_t c = tid; c < m; c += gsize) {
             if (w.color[c] >= 0) continue;
             int col = try_color(w, c);
             if (col == -2) {
-                w.counters[CTR_OVERFLOW] = 4;
+                w.counters[ctr(CTR_OVERFLOW)] = 4;
                 col = kMaxColors - 1;
             }
             if (col >= 0) {
                 __hip_atomic_store(&w.color[c], col, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                atomicMax(&w.counters[CTR_COLORS], (uint32_t)(col + 1));
             } else {
                 ++left;
             }
@@ -117,16 +117,26 @@ __global__ void __launch_bounds__(kSolveBlock) k_color_hist(DeviceWorld w) {
         if (h[k]) atomicAdd(&w.color_count[k], h[k]);
 }
 
-__global__ void k_color_scan(DeviceWorld w) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t acc = 0;
-    for (int k = 0; k < kMaxColors; ++k) {
-        w.color_start[k] = acc;
-        w.color_cursor[k] = 0;
-        acc += w.color_count[k];
-        w.color_count[k] = 0;
+// Colour offsets (one wavefront, two colours per lane); resets the counts
+// and cursors for the next step.
+__global__ void __launch_bounds__(64) k_color_scan(DeviceWorld w) {
+    static_assert(kMaxColors == 128, "two colours per lane");
+    if (gcount(w) == 0) return;
+    int t = threadIdx.x;
+    uint32_t c0 = w.color_count[2 * t], c1 = w.color_count[2 * t + 1];
+    uint32_t a = c0 + c1;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t x = __shfl_up(a, off);
+        if (t >= off) a += x;
     }
-    w.color_start[kMaxColors] = acc;
+    uint32_t base = a - (c0 + c1);
+    w.color_start[2 * t] = base;
+    w.color_start[2 * t + 1] = base + c0;
+    w.color_count[2 * t] = 0;
+    w.color_count[2 * t + 1] = 0;
+    w.color_cursor[2 * t] = 0;
+    w.color_cursor[2 * t + 1] = 0;
+    if (t == 63) w.color_start[kMaxColors] = a;
 }
 
 // Bucket by colour: block-local ranks, one atomic per (block, colour).

@@ -152,16 +152,19 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
-    A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
+    A(&d.bounds, 8); A(&d.bounds_part, 7 * ((size_t)n / 256 + 1));
+    d.gt_cap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n));
+    A(&d.cell_key, n); A(&d.grid_key, n); A(&d.gt_key, d.gt_cap); A(&d.gt_val, d.gt_cap);
     uint32_t P = d.pair_cap;
-    A(&d.pair_keys, P); A(&d.counters, CTR_COUNT);
+    A(&d.pair_keys, P); A(&d.counters, (size_t)CTR_COUNT * kCtrStride);
     A(&d.pairs_cur, P); A(&d.pairs_old, P);
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
     A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
     A(&d.cons, P);
-    A(&d.uf_parent, n); A(&d.isl_root, n); A(&d.isl_ccount, n + 1); A(&d.isl_cstart, n + 1);
-    A(&d.isl_bcount, n + 1); A(&d.isl_bstart, n + 1); A(&d.isl_cons, P); A(&d.isl_bodies, n);
-    A(&d.body_local, n); A(&d.isl_tier, n); A(&d.isl_small, n); A(&d.isl_medium, n); A(&d.isl_large, n);
+    A(&d.uf_parent, n); A(&d.isl_root, n); A(&d.cons_root, P); A(&d.isl_count, n + 1); A(&d.isl_start, n + 1);
+    A(&d.isl_cons, P); A(&d.isl_bodies, n); A(&d.body_local, n); A(&d.isl_medium, n);
+    d.cls_cap = std::max<uint32_t>(1, std::min<uint32_t>(n, P));
+    A(&d.cls_list, 64 * (size_t)d.cls_cap);
     A(&d.gcons, P); A(&d.color, P); A(&d.prio, P);
     A(&d.inc_count, n + 1); A(&d.inc_offset, n + 1); A(&d.inc_list, 2 * (size_t)P);
     A(&d.color_count, kMaxColors); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
@@ -296,6 +299,12 @@ bepu_status bepu_set_timing(bepu_world w, int32_t enable) {
     return BEPU_OK;
 }
 
+bepu_status bepu_set_broadphase(bepu_world w, int32_t kind) {
+    if (!w || (kind != BEPU_BROADPHASE_GRID && kind != BEPU_BROADPHASE_TREE)) return BEPU_E_INVALID_ARG;
+    w->d.bp_kind = (uint32_t)kind;
+    return BEPU_OK;
+}
+
 bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (!w || n_steps < 0) return BEPU_E_INVALID_ARG;
     bepu_status st = upload(w);
@@ -315,8 +324,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     for (int k = 0; k < n_steps; ++k) {
         int slot = (int)timed - (n_steps - k);  // >= 0 for the timed steps
         hipEvent_t* e = slot >= 0 ? &w->ev[(size_t)slot * (ST_COUNT + 1)] : nullptr;
-        launch_fill_u32(d.counters, 0u, CTR_COUNT, s);
-        launch_fill_u64(d.hash_keys_cur, ~0ull, d.hash_cap, s);
+        launch_step_reset(d, s);
         if (e) (void)hipEventRecord(e[0], s);
         launch_body_pre(d, s);
         if (e) (void)hipEventRecord(e[1], s);
@@ -364,8 +372,10 @@ bepu_status bepu_get_stage_times(bepu_world w, float* ms, uint32_t cap, uint32_t
 static bepu_status read_counters(bepu_world w, uint32_t* c) {
     if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
     if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
-    if (hipMemcpy(c, w->d.counters, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+    std::vector<uint32_t> all((size_t)CTR_COUNT * kCtrStride);
+    if (hipMemcpy(all.data(), w->d.counters, all.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
         return BEPU_E_DEVICE;
+    for (uint32_t i = 0; i < CTR_COUNT; ++i) c[i] = all[ctr(i)];
     return BEPU_OK;
 }
 
@@ -489,10 +499,13 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     if (!w || !out) return BEPU_E_INVALID_ARG;
     uint32_t c[CTR_COUNT];
     if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
+    bool any = w->steps > 0;
+    uint32_t npairs = any ? std::min(c[CTR_PAIRS], w->d.pair_cap) : 0;
+    launch_stats(w->d, npairs, w->stream);
+    if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
     std::memset(out, 0, sizeof(*out));
     out->bodies = w->d.nbodies;
-    bool any = w->steps > 0;
-    out->pairs = any ? std::min(c[CTR_PAIRS], w->d.pair_cap) : 0;
+    out->pairs = npairs;
     out->manifolds = any ? c[CTR_CONS] : 0;
     out->contacts = any ? c[CTR_CONTACTS] : 0;
     out->colors = any ? c[CTR_COLORS] : 0;
@@ -500,6 +513,7 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     float mp;
     std::memcpy(&mp, &c[CTR_MAXPEN], 4);
     out->max_penetration = any ? mp : 0.0f;
+    out->global_manifolds = any ? std::min(c[CTR_GCONS], w->d.pair_cap) : 0;
     return c[CTR_OVERFLOW] ? BEPU_E_CAPACITY : BEPU_OK;
 }
 

@@ -69,7 +69,7 @@ struct DeviceWorld {
     float4* mat;               // static, kinetic, bounce
     uint32_t* bflags;          // bit0 dynamic
     // broadphase
-    uint32_t* morton;
+    uint32_t* morton;          // sort keys: Morton code (tree) or cell hash (grid)
     uint32_t* morton_sorted;
     uint32_t* leaf_ids;        // body id per sorted leaf
     uint32_t* leaf_ids_tmp;
@@ -77,8 +77,15 @@ struct DeviceWorld {
     uint32_t tree_levels;
     uint32_t tree_off[32];
     uint32_t tree_cnt[32];
-    uint32_t* bounds;          // ordered-int encoded scene bounds (6)
-    uint32_t* bounds_part;     // per-block partial bounds (6 per block of 256 bodies)
+    uint32_t* bounds;          // [0..5] ordered-int scene bounds of AABB centres, [6] grid base cell edge (float bits)
+    uint32_t* bounds_part;     // per-block partials (7 per block of 256 bodies: bounds, sum of largest sides)
+    uint32_t bp_kind;          // BEPU_BROADPHASE_*
+    // hash grid
+    uint64_t* cell_key;        // per body: level and cell
+    uint64_t* grid_key;        // cell keys in sorted order
+    uint64_t* gt_key;          // cell table: key -> hash run (start << 32 | count)
+    uint64_t* gt_val;
+    uint32_t gt_cap;           // power of two
     // pairs
     uint64_t* pair_keys;       // this step's overlap list
     uint32_t* counters;        // see CTR_*
@@ -92,17 +99,15 @@ struct DeviceWorld {
     uint32_t* cons;            // constraint -> pair slot (unordered)
     uint32_t* uf_parent;       // union-find forest over bodies
     uint32_t* isl_root;        // island (smallest dynamic body id) of every dynamic body
-    uint32_t* isl_ccount;      // per root: constraints (body_cap + 1, zero between steps)
-    uint32_t* isl_cstart;
-    uint32_t* isl_bcount;      // per root: dynamic bodies
-    uint32_t* isl_bstart;
+    uint32_t* cons_root;       // island of every constraint
+    uint64_t* isl_count;       // per root: constraints << 32 | dynamic bodies (zero between steps)
+    uint64_t* isl_start;       // exclusive scan of isl_count (n + 1)
     uint32_t* isl_cons;        // island-contiguous constraint list (pair slots)
     uint32_t* isl_bodies;      // island-contiguous body list
     uint32_t* body_local;      // index of a body inside its island's body list
-    uint32_t* isl_tier;        // per root: 1 small, 2 medium, 3 large
-    uint32_t* isl_small;       // island roots per tier
-    uint32_t* isl_medium;
-    uint32_t* isl_large;
+    uint32_t* isl_medium;      // roots of islands solved by one workgroup each
+    uint32_t* cls_list;        // small island roots, class k at [k * cls_cap, k * cls_cap + count)
+    uint32_t cls_cap;          // per-class capacity (>= number of islands)
     // global path (large islands)
     uint32_t* gcons;           // pair slots of constraints in large islands
     int* color;                // per gcons entry
@@ -125,24 +130,31 @@ constexpr uint32_t kMaxTreeLevels = 32;
 // counters[] indices
 enum {
     CTR_PAIRS = 0,
-    CTR_CONS = 1,
+    CTR_CONS = 1,         // CONS..MAXPEN: last step's statistics, filled on request
     CTR_COLORS = 2,
     CTR_CONTACTS = 3,
     CTR_MAXPEN = 4,       // float bits (non-negative), max via atomicMax on uint
     CTR_OVERFLOW = 5,     // pair/hash capacity overflow flag
     CTR_ROUND = 6,        // colouring rounds
+    CTR_LEVELS = 7,       // grid levels holding at least one body (bit mask)
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
     CTR_GCONS = 11,       // constraints on the global path
-    CTR_ISL_SMALL = 12,
     CTR_ISL_MEDIUM = 13,
-    CTR_ISL_LARGE = 14,
-    CTR_COUNT = 16
+    CTR_CLS0 = 16,        // 16..79: small islands per packing class
+    CTR_COUNT = 80
 };
+
+// Every counter sits on its own 256-byte line: device atomics on one line
+// serialise (about 10 ns each on MI355X), atomics on different lines do not.
+constexpr uint32_t kCtrStride = 64;
+__host__ __device__ constexpr uint32_t ctr(uint32_t i) { return i * kCtrStride; }
 
 // Host-side launchers (one per stage), defined in the kernel sources.
 void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
 void launch_narrowphase(const DeviceWorld& w, hipStream_t st);
+// Last step's constraint, contact, colour and penetration statistics into the counters.
+void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);
 void launch_islands(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
 void launch_island_solve(const DeviceWorld& w, hipStream_t st);
 void launch_global_solver(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
@@ -154,5 +166,6 @@ size_t broadphase_sort_temp_bytes(uint32_t n);
 void tree_layout(uint32_t n, uint32_t* levels, uint32_t* off, uint32_t* cnt);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+void launch_step_reset(const DeviceWorld& w, hipStream_t st);
 
 }  // namespace bepu

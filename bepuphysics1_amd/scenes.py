@@ -164,3 +164,23 @@ def stress(n=32768, edge=None, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE)):
             dims = (rng.uniform(0.25, 1.0), 0.0, 0.0)
         out[i] = _row(s, dims, 1.0, pos[i], quat=q[i])
     return out
+
+
+def mixed_sizes(n=2048, seed=0, edge=200.0, offset=(0.0, 0.0, 0.0), static_fraction=0.1, max_half=10.0):
+    """Boxes and spheres whose sizes span more than two orders of magnitude
+    (half extents / radii log-uniform in [0.05, max_half]) at random positions and
+    orientations, a tenth of them static: exercises every level of the grid
+    broadphase and the kinematic-kinematic rule.  `offset` moves the scene
+    away from the origin."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros(n, BODY_DTYPE)
+    q = random_quaternions(rng, n)
+    pos = (rng.uniform(0.0, edge, (n, 3)) + np.asarray(offset, np.float64)).astype(np.float32)
+    for i in range(n):
+        size = np.exp(rng.uniform(np.log(0.05), np.log(max_half), 3))
+        mass = 0.0 if rng.uniform() < static_fraction else 1.0
+        if i % 2 == 0:
+            out[i] = _row(SHAPE_BOX, 2 * size, mass, pos[i], quat=q[i])
+        else:
+            out[i] = _row(SHAPE_SPHERE, (size[0], 0.0, 0.0), mass, pos[i], quat=q[i])
+    return out

@@ -321,6 +321,12 @@ class Space:
         check(self._lib.bepu_get_stats(self._handle(), ctypes.byref(s)), "get_stats")
         return s
 
+    BROADPHASES = {"grid": 0, "tree": 1}
+
+    def set_broadphase(self, kind):
+        """Space.BroadPhase setter (Space.cs:110-134): "grid" (default) or "tree"."""
+        check(self._lib.bepu_set_broadphase(self._handle(), self.BROADPHASES[kind]), "set_broadphase")
+
     def set_timing(self, enable=True):
         check(self._lib.bepu_set_timing(self._handle(), 1 if enable else 0), "set_timing")
 

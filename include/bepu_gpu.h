@@ -153,7 +153,7 @@ typedef struct bepu_step_stats {
     uint32_t colors;
     uint32_t steps;           /* steps run since creation */
     float max_penetration;    /* deepest contact of the last step */
-    float pad;
+    uint32_t global_manifolds; /* manifolds in islands too large for one workgroup */
 } bepu_step_stats;
 
 typedef struct bepu_world_s* bepu_world;
@@ -205,6 +205,14 @@ bepu_status bepu_get_stats(bepu_world world, bepu_step_stats* out);
  * counterpart: it plays the role of the reference's PROFILE stage timers
  * (ProcessingStage.cs:26-59, Space.cs:486-520). */
 bepu_status bepu_set_timing(bepu_world world, int32_t enable);
+
+/* Broadphase algorithm of a world; replaces the Space.BroadPhase setter
+ * (Space.cs:110-134), which swaps DynamicHierarchy for SortAndSweep1D or
+ * BruteForce.  Both kinds produce the same pair set.  Takes effect at the
+ * next step. */
+#define BEPU_BROADPHASE_GRID 0  /* hierarchical hash grid (default) */
+#define BEPU_BROADPHASE_TREE 1  /* implicit tree over Morton order */
+bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
 bepu_status bepu_get_stage_times(bepu_world world, float* ms, uint32_t capacity,
                                  uint32_t* count_out, const char** names);
 

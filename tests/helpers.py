@@ -7,9 +7,11 @@ def f32bits(a):
     return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
 
 
-def run_gpu(scene, steps, settings=None, capacity=None):
+def run_gpu(scene, steps, settings=None, capacity=None, broadphase=None):
     from bepuphysics1_amd import Space
     sp = Space(capacity=capacity or max(len(scene), 2), settings=settings)
+    if broadphase:
+        sp.set_broadphase(broadphase)
     sp.add_scene(scene)
     if steps:
         sp.Update(steps)

@@ -91,13 +91,38 @@ def test_box_sphere_pile_steps():
     _compare_impulses(g, o)
 
 
-def test_stress_broadphase_exact():
+@pytest.mark.parametrize("broadphase", ["grid", "tree"])
+def test_stress_broadphase_exact(broadphase):
     sc = scenes.stress(4096, seed=11)
-    g, o = run_gpu(sc, 1), run_oracle(sc, 1)
+    g, o = run_gpu(sc, 1, broadphase=broadphase), run_oracle(sc, 1)
     pg, po = g.pairs(), o.pairs()
     assert len(po) > 1000
     assert np.array_equal(pg, po)
     _compare_manifolds(g, o)
+
+
+@pytest.mark.parametrize("broadphase", ["grid", "tree"])
+@pytest.mark.parametrize("offset", [(0.0, 0.0, 0.0), (-3.0e4, 1.0e3, 5.0e4)])
+def test_mixed_sizes_broadphase_exact(broadphase, offset):
+    """Sizes over three orders of magnitude (several grid levels), static
+    bodies, and a scene far from the origin: pair set equals the oracle's
+    brute force exactly."""
+    sc = scenes.mixed_sizes(3000, seed=5, offset=offset)
+    g = run_gpu(sc, 1, broadphase=broadphase)
+    o = run_oracle(sc, 0)
+    o.set_bruteforce_broadphase(True)
+    o.step(1)
+    pg, po = g.pairs(), o.pairs()
+    assert len(po) > 500
+    assert np.array_equal(pg, po)
+
+
+def test_pyramid_tree_broadphase():
+    sc = scenes.pyramid()
+    g, o = run_gpu(sc, 30, broadphase="tree"), run_oracle(sc, 30)
+    assert np.array_equal(g.pairs(), o.pairs())
+    _compare_impulses(g, o)
+    assert np.array_equal(f32bits(g.motion_states()), f32bits(o.motion_states()))
 
 
 def test_towers_small():
