@@ -119,14 +119,22 @@ def cpu_baseline(scene, steps):
 
 
 def calibrate_copy(nbytes=1 << 30):
-    """Plain device copy of a known size, for scaling FETCH_SIZE / WRITE_SIZE."""
-    import torch
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
+    """Plain device-to-device copy of a known size (the HIP runtime's copy
+    kernel), run in the same process as the workload so that one profiler
+    pass records both: the counter values it reads for `nbytes` read and
+    `nbytes` written scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"calibration_copy_bytes": nbytes}))
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
 
 
 def main():
@@ -149,9 +157,6 @@ def main():
               "(independent worlds shard across GPUs without communication; see DESIGN.md)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if a.calibrate_copy:
-        calibrate_copy()
-        return
 
     from bepuphysics1_amd import Space, scenes
 
@@ -212,6 +217,7 @@ def main():
                          "towers scene, %d steps after 1 untimed step, %.1f s" % (a.cpu_steps, dt_cpu)}
 
     parity = None if a.no_parity else parity_check()
+    copy_bytes = calibrate_copy() if a.calibrate_copy else None
 
     line = {
         "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
@@ -241,6 +247,7 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "parity": parity,
+        "calibration_copy_bytes": copy_bytes,
         "towers_standing": standing,
         "max_penetration": round(float(st.max_penetration), 5),
     }

@@ -167,7 +167,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.cls_list, 64 * (size_t)d.cls_cap);
     A(&d.gcons, P); A(&d.color, P); A(&d.prio, P);
     A(&d.inc_count, n + 1); A(&d.inc_offset, n + 1); A(&d.inc_list, 2 * (size_t)P);
-    A(&d.color_count, kMaxColors); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
+    A(&d.color_count, kMaxColors); A(&d.scan_part, 4096); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
     A(&d.ordered, P);
     A(&d.grow_c, P); A(&d.grow_v, P); A(&d.grow_pair, P);
     if (st != BEPU_OK) {
@@ -179,7 +179,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
         return BEPU_E_DEVICE;
     }
     w->sort_bytes = broadphase_sort_temp_bytes(n);
-    w->scan_bytes = std::max(solver_scan_temp_bytes(n + 1), island_scan_temp_bytes(n + 1));
+    w->scan_bytes = island_scan_temp_bytes(n + 1);
     if (hipMalloc(&w->sort_temp, std::max<size_t>(w->sort_bytes, 16)) != hipSuccess ||
         hipMalloc(&w->scan_temp, std::max<size_t>(w->scan_bytes, 16)) != hipSuccess) {
         bepu_world_destroy(w);
@@ -336,7 +336,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[4], s);
         launch_island_solve(d, s);
         if (e) (void)hipEventRecord(e[5], s);
-        launch_global_solver(d, w->scan_temp, w->scan_bytes, s);
+        launch_global_solver(d, s);
         if (e) (void)hipEventRecord(e[6], s);
         launch_integrate(d, s);
         if (e) (void)hipEventRecord(e[7], s);

@@ -118,6 +118,7 @@ struct DeviceWorld {
     uint32_t* color_count;     // kMaxColors
     uint32_t* color_start;     // kMaxColors + 1
     uint32_t* color_cursor;
+    uint32_t* scan_part;       // per-workgroup partial sums of the global path's scan (4096)
     uint32_t* ordered;         // solve order -> gcons index
     RowConst* grow_c;          // rows in solve order
     RowVar* grow_v;
@@ -157,8 +158,7 @@ void launch_narrowphase(const DeviceWorld& w, hipStream_t st);
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);
 void launch_islands(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
 void launch_island_solve(const DeviceWorld& w, hipStream_t st);
-void launch_global_solver(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
-size_t solver_scan_temp_bytes(uint32_t n);
+void launch_global_solver(const DeviceWorld& w, hipStream_t st);
 size_t island_scan_temp_bytes(uint32_t n);
 void launch_integrate(const DeviceWorld& w, hipStream_t st);
 size_t broadphase_sort_temp_bytes(uint32_t n);

@@ -135,3 +135,37 @@ def test_towers_small():
     _compare_impulses(g, o)
     ms = g.motion_states()
     assert np.abs(ms[1:, 0] - sc["state"][1:, 0]).max() < 0.1
+
+
+def test_large_island_global_path():
+    """A 28-row pyramid (406 boxes) is one island of ~800 constraints: it is
+    solved by the cooperative global path and must match the oracle exactly."""
+    sc = scenes.pyramid(rows=28)
+    g, o = run_gpu(sc, 0), run_oracle(sc, 0)
+    for _ in range(3):
+        g.Update(10)
+        o.step(10)
+        assert np.array_equal(g.pairs(), o.pairs())
+        _compare_impulses(g, o)
+    st = g.stats()
+    assert st.global_manifolds == st.manifolds and st.manifolds > 256
+    assert np.array_equal(f32bits(g.motion_states()), f32bits(o.motion_states()))
+
+
+def test_mixed_tiers_one_world():
+    """Small (towers), medium (a 10-row pyramid) and large (a 28-row pyramid)
+    islands in one world, side by side."""
+    t = scenes.towers(nx=4, nz=4, height=6, seed=2)
+    p1 = scenes.pyramid(rows=10)[1:]
+    p2 = scenes.pyramid(rows=28)[1:]
+    p1["state"][:, 2] += 20.0
+    p2["state"][:, 2] -= 20.0
+    sc = np.concatenate([scenes.make([scenes.ground(width=200.0, length=200.0)]), t[1:], p1, p2])
+    g, o = run_gpu(sc, 0), run_oracle(sc, 0)
+    g.Update(15)
+    o.step(15)
+    assert np.array_equal(g.pairs(), o.pairs())
+    _compare_impulses(g, o)
+    st = g.stats()
+    assert 0 < st.global_manifolds < st.manifolds
+    assert np.array_equal(f32bits(g.motion_states()), f32bits(o.motion_states()))
