@@ -6,6 +6,21 @@
 
 namespace bepu {
 
+// Half-octave bin of a non-negative length: bin 2 * (e + 16) + h for
+// lengths in [2^e (1 + h/2), 2^e (1.5 + h/2)), clamped to [0, 63].
+__device__ __forceinline__ int side_bin(float side) {
+    if (!(side > 0.0f)) return 0;
+    uint32_t u = __float_as_uint(side);
+    int e = (int)((u >> 23) & 0xFFu) - 127;
+    int b = 2 * (e + 16) + (int)((u >> 22) & 1u);
+    return min(max(b, 0), 63);
+}
+
+// Upper end of a half-octave bin.
+__device__ __forceinline__ float side_bin_edge(int b) {
+    return ldexpf((b & 1) ? 2.0f : 1.5f, b / 2 - 16);
+}
+
 __device__ __forceinline__ uint32_t f2ord(float f) {
     uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -108,73 +123,69 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
         side = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
         if (!(side >= 0.0f && side < 3.0e38f)) side = 0.0f;  // a non-finite box does not steer the grid size
     }
-    // Scene bounds of AABB centres and the sum of largest sides: wave then
-    // block reduction; one partial per block, folded by k_bounds_reduce (no
-    // global atomics, fixed summation order).
+    // Scene bounds of AABB centres (wave then block reduction; one partial
+    // per block, folded by k_bounds_reduce) and the histogram of largest
+    // sides (one atomic per block and non-empty bin, bins on separate lines).
     __shared__ uint32_t part[4][6];
-    __shared__ float pside[4];
+    __shared__ uint32_t hist[64];
+    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
+    __syncthreads();
+    if (valid) atomicAdd(&hist[side_bin(side)], 1u);
     uint32_t v6[6] = {valid ? f2ord(cx) : 0xffffffffu, valid ? f2ord(cy) : 0xffffffffu,
                       valid ? f2ord(cz) : 0xffffffffu, valid ? f2ord(cx) : 0u, valid ? f2ord(cy) : 0u,
                       valid ? f2ord(cz) : 0u};
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = 32; off > 0; off >>= 1)
         for (int k = 0; k < 3; ++k) {
             v6[k] = min(v6[k], (uint32_t)__shfl_xor((int)v6[k], off));
             v6[3 + k] = max(v6[3 + k], (uint32_t)__shfl_xor((int)v6[3 + k], off));
         }
-        side += __shfl_xor(side, off);
-    }
     int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 6; ++k) part[wv][k] = v6[k];
-        pside[wv] = side;
-    }
     __syncthreads();
     if (threadIdx.x < 6) {
         int k = threadIdx.x;
         uint32_t r = part[0][k];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) r = k < 3 ? min(r, part[q][k]) : max(r, part[q][k]);
-        w.bounds_part[blockIdx.x * 7 + k] = r;
-    } else if (threadIdx.x == 6) {
-        float t = pside[0];
-        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) t += pside[q];
-        w.bounds_part[blockIdx.x * 7 + 6] = __float_as_uint(t);
+        w.bounds_part[blockIdx.x * 6 + k] = r;
     }
+    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&w.counters[ctr(CTR_SIDE0 + threadIdx.x)], hist[threadIdx.x]);
 }
 
 __global__ void __launch_bounds__(256) k_bounds_reduce(DeviceWorld w, uint32_t nparts) {
     __shared__ uint32_t red[6][256];
-    __shared__ float rs[256];
+    __shared__ uint32_t hist[64];
     uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0, 0, 0};
-    float sum = 0;
-    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x) {
+    for (uint32_t p = threadIdx.x; p < nparts; p += blockDim.x)
         for (int k = 0; k < 3; ++k) {
-            lo[k] = min(lo[k], w.bounds_part[p * 7 + k]);
-            hi[k] = max(hi[k], w.bounds_part[p * 7 + 3 + k]);
+            lo[k] = min(lo[k], w.bounds_part[p * 6 + k]);
+            hi[k] = max(hi[k], w.bounds_part[p * 6 + 3 + k]);
         }
-        sum += __uint_as_float(w.bounds_part[p * 7 + 6]);
-    }
     for (int k = 0; k < 3; ++k) {
         red[k][threadIdx.x] = lo[k];
         red[3 + k][threadIdx.x] = hi[k];
     }
-    rs[threadIdx.x] = sum;
+    if (threadIdx.x < 64) hist[threadIdx.x] = w.counters[ctr(CTR_SIDE0 + threadIdx.x)];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
+        if ((int)threadIdx.x < s)
             for (int k = 0; k < 3; ++k) {
                 red[k][threadIdx.x] = min(red[k][threadIdx.x], red[k][threadIdx.x + s]);
                 red[3 + k][threadIdx.x] = max(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
             }
-            rs[threadIdx.x] += rs[threadIdx.x + s];
-        }
         __syncthreads();
     }
     if (threadIdx.x < 6) w.bounds[threadIdx.x] = red[threadIdx.x][0];
     if (threadIdx.x == 6) {
-        // Grid base cell edge: twice the mean largest side (1 when degenerate).
-        float e = 2.0f * (rs[0] / (float)w.nbodies);
-        if (!(e > 1e-6f && e < 1e30f)) e = 1.0f;
-        w.bounds[6] = __float_as_uint(e);
+        // Grid base cell edge: the largest side of 90% of the bodies fits the
+        // finest level (bodies at level 0 have largest side <= 0.99 * edge).
+        uint32_t need = w.nbodies - w.nbodies / 10, acc = 0;
+        int b = 0;
+        for (; b < 63; ++b) {
+            acc += hist[b];
+            if (acc >= need) break;
+        }
+        w.bounds[6] = __float_as_uint(side_bin_edge(b) * (1.0f / 0.99f) * 1.0001f);
     }
 }
 
@@ -212,7 +223,10 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
 __global__ void __launch_bounds__(256) k_step_reset(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < w.hash_cap) w.hash_keys_cur[i] = ~0ull;
-    if (i < w.gt_cap) w.gt_key[i] = ~0ull;
+    if (i < w.gt_cap) {
+        w.gt_key[i] = ~0ull;
+        w.gt_head[i] = 0xffffffffu;
+    }
     if (i < CTR_COUNT) w.counters[ctr(i)] = 0;
 }
 

@@ -1,5 +1,4 @@
-// k_broadphase.hip -- broadphase stage: the radix sort shared by the two
-// broadphases and the dispatch between them (the reference swaps its
+// k_broadphase.hip -- broadphase stage: dispatch between the two broadphases (the reference swaps its
 // broadphase through the Space.BroadPhase setter, Space.cs:110-134).
 //   BEPU_BROADPHASE_GRID  hierarchical hash grid (k_grid.hip), the default;
 //   BEPU_BROADPHASE_TREE  implicit tree over Morton order (k_bvh.hip).
@@ -12,13 +11,12 @@ namespace bepu {
 
 void launch_morton(const DeviceWorld& w, hipStream_t st);
 void launch_bvh_after_sort(const DeviceWorld& w, hipStream_t st);
-void launch_grid_cells(const DeviceWorld& w, hipStream_t st);
-void launch_grid_after_sort(const DeviceWorld& w, hipStream_t st);
+void launch_grid(const DeviceWorld& w, hipStream_t st);
 
 size_t broadphase_sort_temp_bytes(uint32_t n) {
     size_t bytes = 0;
     (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
+                                             (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
     return bytes;
 }
 
@@ -32,10 +30,7 @@ void launch_broadphase(const DeviceWorld& w, void* temp, size_t temp_bytes, hipS
                                                  (int)n, 0, 30, st);
         launch_bvh_after_sort(w, st);
     } else {
-        launch_grid_cells(w, st);
-        (void)hipcub::DeviceRadixSort::SortPairs(temp, bytes, w.morton, w.morton_sorted, w.leaf_ids_tmp, w.leaf_ids,
-                                                 (int)n, 0, 32, st);
-        launch_grid_after_sort(w, st);
+        launch_grid(w, st);
     }
 }
 

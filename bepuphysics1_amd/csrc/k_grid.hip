@@ -5,8 +5,8 @@
 // overlap (BoundingBox.cs:58-69) of two bodies, at least one dynamic
 // (kinematic-kinematic is NoBroadPhase, CollisionRules.cs:175-178).
 //
-// Levels l = 0..14 have cubic cells of edge e_l = s0 * 4^l, s0 = twice the
-// mean largest AABB side of the step.  A body lives at the finest level with
+// Levels l = 0..14 have cubic cells of edge e_l = s0 * 4^l, s0 chosen each
+// step so that the largest AABB side of 90% of the bodies fits level 0.  A body lives at the finest level with
 // largest side <= 0.99 * e_l, in the cell holding its AABB min corner.  If two
 // boxes overlap, their min corners differ by at most the larger of their
 // sides per axis, so:
@@ -22,12 +22,10 @@
 // computed in double precision, so the one-cell bound holds for coordinates
 // up to ~1e13 cell edges from the origin.
 //
-// Bodies are sorted by a 32-bit hash of their cell key (radix sort); a hash
-// table maps each cell key to its hash run in the sorted order.  Candidates
-// whose own cell key differs from the probed one (hash collisions, or cells
-// that alias after the 20-bit wrap of the packed coordinates) are skipped, so
-// no pair is tested twice.  Every probe is independent of the others, which
-// keeps the chain of dependent loads per body at three (table, key, box).
+// No sort: a hash table maps each packed cell key to the head of a linked
+// list of the bodies in that cell (one atomic exchange per body).  Probes are
+// independent of each other; the dependent chain per probe is the table
+// slot, the list head and the (short) list.
 #include "pairbuf.h"
 
 namespace bepu {
@@ -60,89 +58,60 @@ __device__ __forceinline__ uint64_t cell_key(int l, uint64_t x, uint64_t y, uint
     return ((uint64_t)l << 60) | ((x & 0xFFFFFull) << 40) | ((y & 0xFFFFFull) << 20) | (z & 0xFFFFFull);
 }
 
-__device__ __forceinline__ uint32_t cell_hash(uint64_t key) { return (uint32_t)(mix64(key) >> 32); }
-
 __device__ __forceinline__ float max_side(float4 mn, float4 mx) {
     return fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
 }
 
-// Level and cell of every body; the sort key is the cell hash.
-__global__ void __launch_bounds__(256) k_grid_cells(DeviceWorld w) {
+constexpr uint32_t kEndOfList = 0xffffffffu;
+
+// Slot of `key` in the cell table, inserting it if absent (~0 when full).
+__device__ __forceinline__ uint32_t table_slot(DeviceWorld& w, uint64_t key) {
+    uint32_t mask = w.gt_cap - 1, s = (uint32_t)mix64(key) & mask;
+    for (uint32_t p = 0; p < w.gt_cap; ++p) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&w.gt_key[s], kEmptyCell, key);
+        if (prev == kEmptyCell || prev == key) return s;
+        s = (s + 1) & mask;
+    }
+    return ~0u;
+}
+
+// Level and cell of every body; the body joins its cell's list (the table
+// maps the packed cell key to the list head; heads start empty every step).
+__global__ void __launch_bounds__(256) k_grid_insert(DeviceWorld w) {
+    __shared__ uint32_t s_mask;
+    if (threadIdx.x == 0) s_mask = 0;
+    __syncthreads();
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    bool alive = b < w.nbodies;
     uint32_t lmask = 0;
-    if (alive) {
+    if (b < w.nbodies) {
         float4 mn = w.bmin[b], mx = w.bmax[b];
         float s0 = __uint_as_float(w.bounds[6]);
         int l = grid_level(max_side(mn, mx), s0);
         double e = level_edge(s0, l);
         uint64_t key = cell_key(l, (uint64_t)cell_of(mn.x, e), (uint64_t)cell_of(mn.y, e), (uint64_t)cell_of(mn.z, e));
         w.cell_key[b] = key;
-        w.morton[b] = cell_hash(key);
-        w.leaf_ids_tmp[b] = b;
+        uint32_t slot = table_slot(w, key);
+        if (slot != ~0u) w.gnext[b] = atomicExch(&w.gt_head[slot], b);
+        else w.counters[ctr(CTR_OVERFLOW)] = 6;
         lmask = 1u << l;
     }
-    __shared__ uint32_t s_mask;
-    if (threadIdx.x == 0) s_mask = 0;
-    __syncthreads();
     for (int off = 32; off > 0; off >>= 1) lmask |= __shfl_xor(lmask, off);
     if (__lane_id() == 0 && lmask) atomicOr(&s_mask, lmask);
     __syncthreads();
     if (threadIdx.x == 0 && s_mask) atomicOr(&w.counters[ctr(CTR_LEVELS)], s_mask);
 }
 
-__device__ __forceinline__ void table_insert(DeviceWorld& w, uint64_t key, uint64_t val) {
-    uint32_t mask = w.gt_cap - 1, s = (uint32_t)mix64(key) & mask;
-    for (uint32_t p = 0; p < w.gt_cap; ++p) {
-        unsigned long long prev = atomicCAS((unsigned long long*)&w.gt_key[s], kEmptyCell, key);
-        if (prev == kEmptyCell) {
-            w.gt_val[s] = val;
-            return;
-        }
-        if (prev == key) return;
-        s = (s + 1) & mask;
-    }
-    w.counters[ctr(CTR_OVERFLOW)] = 6;
-}
-
-// Sorted leaves (box, body id and dynamic flag in max.w, cell key) and the
-// cell table: the first index of every hash run inserts the run's keys.
-__global__ void __launch_bounds__(256) k_grid_gather(DeviceWorld w) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = w.nbodies;
-    if (j >= n) return;
-    uint32_t b = w.leaf_ids[j];
-    float4 mn = w.bmin[b], mx = w.bmax[b];
-    bool dyn = (w.bflags[b] & 1u) != 0;
-    mx.w = __uint_as_float(b | (dyn ? 0x80000000u : 0u));
-    w.node_min[j] = mn;
-    w.node_max[j] = mx;
-    w.grid_key[j] = w.cell_key[b];
-    uint32_t h = w.morton_sorted[j];
-    if (j == 0 || w.morton_sorted[j - 1] != h) {
-        uint32_t e = j + 1;
-        while (e < n && w.morton_sorted[e] == h) ++e;
-        uint64_t val = ((uint64_t)j << 32) | (uint64_t)(e - j);
-        for (uint32_t t = j; t < e; ++t) table_insert(w, w.cell_key[w.leaf_ids[t]], val);
-    }
-}
-
-__device__ __forceinline__ bool table_find(const DeviceWorld& w, uint64_t key, uint64_t first, uint32_t slot,
-                                           uint32_t& start, uint32_t& count) {
+// List head of `key`, or kEndOfList when no body sits in that cell.
+__device__ __forceinline__ uint32_t table_head(const DeviceWorld& w, uint64_t key, uint64_t first, uint32_t slot) {
     uint32_t mask = w.gt_cap - 1, s = slot;
     uint64_t k = first;
     for (uint32_t p = 0; p < w.gt_cap; ++p) {
-        if (k == key) {
-            uint64_t v = w.gt_val[s];
-            start = (uint32_t)(v >> 32);
-            count = (uint32_t)v;
-            return true;
-        }
-        if (k == kEmptyCell) return false;
+        if (k == key) return w.gt_head[s];
+        if (k == kEmptyCell) return kEndOfList;
         s = (s + 1) & mask;
         k = w.gt_key[s];
     }
-    return false;
+    return kEndOfList;
 }
 
 // Own cell first, then the 13 neighbours that are "forward" in (z, y, x) order.
@@ -152,23 +121,21 @@ __constant__ int8_t c_fwd[14][3] = {{0, 0, 0},  {1, 0, 0},  {-1, 1, 0}, {0, 1, 0
 
 struct GridQuery {
     float4 qmin, qmax;
-    uint32_t j, body;
+    uint32_t body;
     bool dyn;
 };
 
-// Tests the candidates of one probed cell; `own` restricts to larger sorted
-// indices (the query's own cell at its own level).
+// Tests the bodies of one probed cell; `own` restricts to larger body ids
+// (the query's own cell at its own level).  The packed key identifies the
+// cell exactly up to the 20-bit coordinate wrap, and cells that alias are a
+// million cells apart, so their bodies fail the box test.
 __device__ __forceinline__ void probe_cell(DeviceWorld& w, const GridQuery& q, uint64_t key, uint64_t first,
                                            uint32_t slot, bool own, uint64_t* buf, uint32_t* buf_n) {
-    uint32_t start = 0, count = 0;
-    if (!table_find(w, key, first, slot, start, count)) return;
-    for (uint32_t t = start; t < start + count; ++t) {
-        if (own && t <= q.j) continue;
-        if (w.grid_key[t] != key) continue;
-        float4 cmin = w.node_min[t], cmax = w.node_max[t];
-        uint32_t cb = __float_as_uint(cmax.w);
-        bool hit = (q.dyn || (cb >> 31)) && overlap(q.qmin, q.qmax, cmin, cmax);
-        if (hit) buf_push(w, buf, buf_n, kQueryBuf, true, pair_key(q.body, cb & 0x7fffffffu));
+    for (uint32_t c = table_head(w, key, first, slot); c != kEndOfList; c = w.gnext[c]) {
+        if (own && c <= q.body) continue;
+        float4 cmin = w.bmin[c], cmax = w.bmax[c];
+        bool hit = (q.dyn || (w.bflags[c] & 1u)) && overlap(q.qmin, q.qmax, cmin, cmax);
+        if (hit) buf_push(w, buf, buf_n, kQueryBuf, true, pair_key(q.body, c));
     }
 }
 
@@ -177,16 +144,14 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
     __shared__ uint32_t buf_n, base_slot;
     if (threadIdx.x == 0) buf_n = 0;
     __syncthreads();
-    const uint32_t j = blockIdx.x * kQueryBlock + threadIdx.x;
-    if (j < w.nbodies) {
+    const uint32_t b = blockIdx.x * kQueryBlock + threadIdx.x;
+    if (b < w.nbodies) {
         GridQuery q;
-        q.j = j;
-        q.qmin = w.node_min[j];
-        q.qmax = w.node_max[j];
-        uint32_t qb = __float_as_uint(q.qmax.w);
-        q.body = qb & 0x7fffffffu;
-        q.dyn = (qb >> 31) != 0;
-        const uint64_t qkey = w.grid_key[j];
+        q.body = b;
+        q.qmin = w.bmin[b];
+        q.qmax = w.bmax[b];
+        q.dyn = (w.bflags[b] & 1u) != 0;
+        const uint64_t qkey = w.cell_key[b];
         const int l = (int)(qkey >> 60);
         const uint32_t mask = w.gt_cap - 1;
         // Own level: own cell and forward neighbours (coordinates wrap with the key).
@@ -232,14 +197,9 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
     buf_flush<kQueryBlock>(w, buf, &buf_n, kQueryBuf, &base_slot);
 }
 
-void launch_grid_cells(const DeviceWorld& w, hipStream_t st) {
+void launch_grid(const DeviceWorld& w, hipStream_t st) {
     uint32_t n = w.nbodies;
-    hipLaunchKernelGGL(k_grid_cells, dim3((n + 255) / 256), dim3(256), 0, st, w);
-}
-
-void launch_grid_after_sort(const DeviceWorld& w, hipStream_t st) {
-    uint32_t n = w.nbodies;
-    hipLaunchKernelGGL(k_grid_gather, dim3((n + 255) / 256), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_grid_insert, dim3((n + 255) / 256), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_grid_query, dim3((n + kQueryBlock - 1) / kQueryBlock), dim3(kQueryBlock), 0, st, w);
 }
 

@@ -152,9 +152,9 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
-    A(&d.bounds, 8); A(&d.bounds_part, 7 * ((size_t)n / 256 + 1));
+    A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
     d.gt_cap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)n));
-    A(&d.cell_key, n); A(&d.grid_key, n); A(&d.gt_key, d.gt_cap); A(&d.gt_val, d.gt_cap);
+    A(&d.cell_key, n); A(&d.gnext, n); A(&d.gt_key, d.gt_cap); A(&d.gt_head, d.gt_cap);
     uint32_t P = d.pair_cap;
     A(&d.pair_keys, P); A(&d.counters, (size_t)CTR_COUNT * kCtrStride);
     A(&d.pairs_cur, P); A(&d.pairs_old, P);

@@ -69,7 +69,7 @@ struct DeviceWorld {
     float4* mat;               // static, kinetic, bounce
     uint32_t* bflags;          // bit0 dynamic
     // broadphase
-    uint32_t* morton;          // sort keys: Morton code (tree) or cell hash (grid)
+    uint32_t* morton;          // Morton codes (tree broadphase)
     uint32_t* morton_sorted;
     uint32_t* leaf_ids;        // body id per sorted leaf
     uint32_t* leaf_ids_tmp;
@@ -78,13 +78,13 @@ struct DeviceWorld {
     uint32_t tree_off[32];
     uint32_t tree_cnt[32];
     uint32_t* bounds;          // [0..5] ordered-int scene bounds of AABB centres, [6] grid base cell edge (float bits)
-    uint32_t* bounds_part;     // per-block partials (7 per block of 256 bodies: bounds, sum of largest sides)
+    uint32_t* bounds_part;     // per-block partial bounds (6 per block of 256 bodies)
     uint32_t bp_kind;          // BEPU_BROADPHASE_*
     // hash grid
     uint64_t* cell_key;        // per body: level and cell
-    uint64_t* grid_key;        // cell keys in sorted order
-    uint64_t* gt_key;          // cell table: key -> hash run (start << 32 | count)
-    uint64_t* gt_val;
+    uint32_t* gnext;           // per body: next body of its cell's list
+    uint64_t* gt_key;          // cell table: packed cell key -> list head
+    uint32_t* gt_head;
     uint32_t gt_cap;           // power of two
     // pairs
     uint64_t* pair_keys;       // this step's overlap list
@@ -142,7 +142,8 @@ enum {
     CTR_GCONS = 11,       // constraints on the global path
     CTR_ISL_MEDIUM = 13,
     CTR_CLS0 = 16,        // 16..79: small islands per packing class
-    CTR_COUNT = 80
+    CTR_SIDE0 = 80,       // 80..143: bodies per half-octave bin of the largest AABB side
+    CTR_COUNT = 144
 };
 
 // Every counter sits on its own 256-byte line: device atomics on one line
