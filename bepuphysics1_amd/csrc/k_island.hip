@@ -160,30 +160,61 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
     int color = -1;
     const int la = have ? L.da[tid] : -1, lb = have ? L.db[tid] : -1;
     const uint64_t prio = have ? L.prio[tid] : 0;
-    for (int round = 0; round <= BLOCK; ++round) {
-        bool need = have && color < 0;
-        if (need) {
-            uint64_t m0 = 0, m1 = 0;
-            bool ready = true;
+    if constexpr (BLOCK == 64) {
+        // One wavefront: each lane's higher-priority neighbours as a lane mask,
+        // found in one pass; a round is then one ballot.
+        uint64_t hp = 0;
+        if (have)
             for (int k = clo; k < chi; ++k) {
                 if (k == tid) continue;
                 int ka = L.da[k], kb = L.db[k];
                 bool nb_ = (ka >= 0 && (ka == la || ka == lb)) || (kb >= 0 && (kb == la || kb == lb));
-                if (!nb_ || L.prio[k] < prio) continue;
-                int ck = __hip_atomic_load(&L.color[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (ck < 0) { ready = false; break; }
-                if (ck < 64) m0 |= 1ull << ck;
-                else m1 |= 1ull << (ck - 64);
+                if (nb_ && L.prio[k] >= prio) hp |= 1ull << k;
             }
-            if (ready) {
+        while (true) {
+            const bool need = have && color < 0;
+            const uint64_t pending = __ballot(need);
+            if (!pending) break;
+            if (need && (hp & pending) == 0) {
+                uint64_t m0 = 0, m1 = 0;
+                for (uint64_t m = hp; m; m &= m - 1) {
+                    int ck = L.color[__ffsll((unsigned long long)m) - 1];
+                    if (ck < 64) m0 |= 1ull << ck;
+                    else m1 |= 1ull << (ck - 64);
+                }
                 if (~m0) color = __ffsll((unsigned long long)~m0) - 1;
                 else if (~m1) color = 64 + __ffsll((unsigned long long)~m1) - 1;
                 else { color = kMaxColors - 1; w.counters[ctr(CTR_OVERFLOW)] = 4; }
+                L.color[tid] = color;
             }
+            __syncthreads();
         }
-        __syncthreads();
-        if (need && color >= 0) __hip_atomic_store(&L.color[tid], color, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (!__syncthreads_or(have && color < 0)) break;
+    } else {
+        for (int round = 0; round <= BLOCK; ++round) {
+            bool need = have && color < 0;
+            if (need) {
+                uint64_t m0 = 0, m1 = 0;
+                bool ready = true;
+                for (int k = clo; k < chi; ++k) {
+                    if (k == tid) continue;
+                    int ka = L.da[k], kb = L.db[k];
+                    bool nb_ = (ka >= 0 && (ka == la || ka == lb)) || (kb >= 0 && (kb == la || kb == lb));
+                    if (!nb_ || L.prio[k] < prio) continue;
+                    int ck = __hip_atomic_load(&L.color[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (ck < 0) { ready = false; break; }
+                    if (ck < 64) m0 |= 1ull << ck;
+                    else m1 |= 1ull << (ck - 64);
+                }
+                if (ready) {
+                    if (~m0) color = __ffsll((unsigned long long)~m0) - 1;
+                    else if (~m1) color = 64 + __ffsll((unsigned long long)~m1) - 1;
+                    else { color = kMaxColors - 1; w.counters[ctr(CTR_OVERFLOW)] = 4; }
+                }
+            }
+            __syncthreads();
+            if (need && color >= 0) __hip_atomic_store(&L.color[tid], color, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (!__syncthreads_or(have && color < 0)) break;
+        }
     }
     // Colours used = max colour + 1 over the workgroup.
     int ncol = have ? color + 1 : 0;
@@ -276,6 +307,8 @@ __device__ __forceinline__ void setup_lane(const DeviceWorld& w, IslandLds<BLOCK
 
 // Small islands: one bundle of same-class islands per wavefront, persistent
 // over the bundles.
+// Two wavefronts per SIMD (a few registers spill) beat one: every bundle of a
+// large scene is resident at once and the two hide each other's latency.
 __global__ void __launch_bounds__(kPackLanes) __attribute__((amdgpu_waves_per_eu(2))) k_island_pack(DeviceWorld w) {
     __shared__ IslandLds<kPackLanes, kPackSlots> L;
     __shared__ uint32_t s_cnt[64], s_bund[65];
