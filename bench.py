@@ -70,6 +70,26 @@ def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global
     return 0
 
 
+# Kernels of each timed stage, for the measured HBM traffic of the dominant one
+# (profiles/traffic.json: FETCH_SIZE / WRITE_SIZE per launch from rocprofv3
+# counter passes, scaled by a plain copy of known size).
+STAGE_KERNELS = {
+    "island_solve": ["bepu::k_island_pack(bepu::DeviceWorld)", "bepu::k_island_medium(bepu::DeviceWorld)"],
+    "narrowphase": ["bepu::k_narrowphase(bepu::DeviceWorld)"],
+    "broadphase": ["bepu::k_grid_insert(bepu::DeviceWorld)", "bepu::k_grid_query(bepu::DeviceWorld)"],
+    "global_solve": ["bepu::k_global(bepu::DeviceWorld)"],
+}
+
+
+def measured_traffic(stage):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if stage not in STAGE_KERNELS or not os.path.exists(path):
+        return None
+    t = json.load(open(path))["bytes_per_launch"]
+    ks = [k for k in STAGE_KERNELS[stage] if k in t]
+    return sum(t[k] for k in ks) if ks else None
+
+
 def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0):
     """Whole-step algorithmic bytes: B_body*N + B_pair*P + sum over manifolds."""
     return 528 * n_body + 312 * n_pair + (332 + 190) * n_cons + (72 + 70) * n_contacts + \
@@ -153,8 +173,8 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(independent worlds shard across GPUs without communication; see DESIGN.md)" % a.gpus,
+        print("bench.py: runs on one GPU only; --gpus %d is not supported "
+              "(independent worlds shard across GPUs without communication)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
 
@@ -237,7 +257,8 @@ def main():
                    "bodies": n_dyn, "pairs": st.pairs, "manifolds": st.manifolds, "contacts": st.contacts,
                    "colors": st.colors},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant),
+                     "kernel": dominant,
                      "kernel_ms": round(stages[dominant], 5), "alg_bytes_per_launch": int(dom_bytes)},
         "step_roofline": {"alg_bytes_per_step": int(step_alg),
                           "achieved_GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 2),

@@ -205,6 +205,8 @@ bepu_status bepu_get_stats(bepu_world world, bepu_step_stats* out);
  * counterpart: it plays the role of the reference's PROFILE stage timers
  * (ProcessingStage.cs:26-59, Space.cs:486-520). */
 bepu_status bepu_set_timing(bepu_world world, int32_t enable);
+bepu_status bepu_get_stage_times(bepu_world world, float* ms, uint32_t capacity,
+                                 uint32_t* count_out, const char** names);
 
 /* Broadphase algorithm of a world; replaces the Space.BroadPhase setter
  * (Space.cs:110-134), which swaps DynamicHierarchy for SortAndSweep1D or
@@ -213,8 +215,6 @@ bepu_status bepu_set_timing(bepu_world world, int32_t enable);
 #define BEPU_BROADPHASE_GRID 0  /* hierarchical hash grid (default) */
 #define BEPU_BROADPHASE_TREE 1  /* implicit tree over Morton order */
 bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
-bepu_status bepu_get_stage_times(bepu_world world, float* ms, uint32_t capacity,
-                                 uint32_t* count_out, const char** names);
 
 /* Library-wide information: 1 if a HIP device is usable. */
 int32_t bepu_device_available(void);

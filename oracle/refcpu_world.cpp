@@ -10,8 +10,8 @@
 //  * Solve order.  The reference visits solver updateables in the permuted
 //    order of PermutationMapper over the Solver.Add history (Solver.cs:239-263).
 //    Both the oracle and the GPU library use the deterministic colour order
-//    defined in colour_constraints() below (the north-star comparison is made
-//    "once the reference's constraint order is permuted to the colour order").
+//    defined in colour_constraints() below; impulses are compared once the
+//    reference's constraint order is permuted to that colour order.
 //  * Pair orientation.  The reference takes A/B from DynamicHierarchy traversal
 //    order; here A is the lower body id, except box-sphere where A is the box
 //    as BoxSpherePairHandler.Initialize enforces (BoxSpherePairHandler.cs:78-97).
