@@ -138,6 +138,57 @@ def cpu_baseline(scene, steps):
     return dyn * steps / dt, dt
 
 
+def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
+    """4096 independent piles (N ~ logU[8,256], boxes and spheres, own grounds,
+    8 iterations) stepped as sub-worlds of one batch, against the plain path:
+    the same worlds stepped one at a time, each in its own world handle
+    (sampled, then scaled to all worlds)."""
+    from bepuphysics1_amd import Space, scenes
+    from bepuphysics1_amd._lib import default_settings
+    st = default_settings()
+    st.iteration_limit = 8
+    worlds = scenes.batched_worlds(n_worlds, seed=0)
+    n_dyn = int(sum((w["mass"] > 0).sum() for w in worlds))
+    sp = Space(capacity=sum(len(w) for w in worlds), settings=st)
+    for w in worlds:
+        sp.add_world(w)
+    sp.Update(warmup)
+    sp.stats()
+    sp.set_timing(True)
+    t0 = time.perf_counter()
+    sp.Update(steps)
+    stats = sp.stats()
+    dt = time.perf_counter() - t0
+    stages = sp.stage_times()
+    sp.close()
+    batched = n_dyn * steps / dt
+    # Plain path: every world alone, one step after another.
+    rng = np.random.default_rng(1)
+    pick = rng.choice(n_worlds, size=min(sample_worlds, n_worlds), replace=False)
+    t_sum, b_sum = 0.0, 0
+    for k in pick:
+        one = Space(capacity=len(worlds[k]), settings=st)
+        one.add_scene(worlds[k])
+        one.Update(2)
+        one.stats()
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            one.Update(1)
+        one.stats()
+        t_sum += (time.perf_counter() - t0) / sample_steps
+        b_sum += int((worlds[k]["mass"] > 0).sum())
+        one.close()
+    plain = b_sum / t_sum  # body-steps/s when worlds are stepped one at a time
+    return {"workload": "%d worlds, N ~ logU[8,256] boxes + spheres on own grounds, dt 1/60, 8 iterations"
+                        % n_worlds,
+            "bodies": n_dyn, "steps": steps, "value": round(batched, 1), "ms_per_step": round(1e3 * dt / steps, 4),
+            "pairs": stats.pairs, "manifolds": stats.manifolds,
+            "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+            "plain_value": round(plain, 1), "plain_sample": "%d worlds x %d single steps, one world per handle"
+                                                            % (len(pick), sample_steps),
+            "ratio_vs_plain": round(batched / plain, 1)}
+
+
 def calibrate_copy(nbytes=1 << 30):
     """Plain device-to-device copy of a known size (the HIP runtime's copy
     kernel), run in the same process as the workload so that one profiler
@@ -170,6 +221,9 @@ def main():
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--latency-steps", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--batched-worlds", type=int, default=4096)
+    ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--no-batched", action="store_true")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -238,6 +292,7 @@ def main():
 
     parity = None if a.no_parity else parity_check()
     copy_bytes = calibrate_copy() if a.calibrate_copy else None
+    batched = None if a.no_batched else batched_leg(a.batched_worlds, a.batched_steps, 5, 48, 10)
 
     line = {
         "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
@@ -268,6 +323,7 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "parity": parity,
+        "batched": batched,
         "calibration_copy_bytes": copy_bytes,
         "towers_standing": standing,
         "max_penetration": round(float(st.max_penetration), 5),

@@ -107,6 +107,9 @@ EXPORTS = {
                                             ctypes.POINTER(ctypes.c_char_p)]),
     "bepu_set_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_set_broadphase": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "bepu_begin_world": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
+    "bepu_get_world_range": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_uint32)]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -130,11 +130,12 @@ constexpr int kPairBuf = 2 * kFanout * kTravBlock;
 // rooted at level k <= 30 needs at most 7 * 10 + 1 entries.
 constexpr int kStack = 72;
 
-__device__ __forceinline__ bool leaf_hit(float4 qmin, float4 qmax, bool qdyn, uint32_t j, uint32_t ci, float4 cmin,
-                                         float4 cmax) {
+// Leaf test: ranked above the query, collision rule, box overlap, same sub-world.
+__device__ __forceinline__ bool leaf_hit(const DeviceWorld& w, float4 qmin, float4 qmax, bool qdyn, uint32_t qworld,
+                                         uint32_t j, uint32_t ci, float4 cmin, float4 cmax) {
     uint32_t cb = __float_as_uint(cmax.w);
     bool mine = cmin.w > qmin.w || (cmin.w == qmin.w && ci > j);
-    return mine && (qdyn || (cb >> 31)) && overlap(qmin, qmax, cmin, cmax);
+    return mine && (qdyn || (cb >> 31)) && overlap(qmin, qmax, cmin, cmax) && w.wid[cb & 0x7fffffffu] == qworld;
 }
 
 __device__ __forceinline__ bool node_hit(float4 qmin, float4 qmax, bool qdyn, uint32_t j, uint32_t n, int level,
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(DeviceWorld w) {
     const uint32_t qbits = __float_as_uint(qmax.w);
     const uint32_t body = qbits & 0x7fffffffu;
     const bool qdyn = (qbits >> 31) != 0;
+    const uint32_t qworld = alive ? w.wid[body] : 0;
 
     // Ancestor siblings: bit k of `todo` = the sibling subtree at level k qualifies.
     uint32_t todo = 0;
@@ -181,7 +183,7 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(DeviceWorld w) {
             if (sib >= s_cnt[k]) continue;
             float4 cmin = w.node_min[s_off[k] + sib], cmax = w.node_max[s_off[k] + sib];
             if (k == 0) {
-                if (leaf_hit(qmin, qmax, qdyn, j, sib, cmin, cmax)) {
+                if (leaf_hit(w, qmin, qmax, qdyn, qworld, j, sib, cmin, cmax)) {
                     found = true;
                     key = pair_key(body, __float_as_uint(cmax.w) & 0x7fffffffu);
                 }
@@ -223,7 +225,7 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(DeviceWorld w) {
                 const uint32_t ci = cfirst + (uint32_t)c;
                 const float4 cmin = w.node_min[base + c], cmax = w.node_max[base + c];
                 if (cl == 0) {
-                    if (leaf_hit(qmin, qmax, qdyn, j, ci, cmin, cmax)) {
+                    if (leaf_hit(w, qmin, qmax, qdyn, qworld, j, ci, cmin, cmax)) {
                         fmask |= 1u << c;
                         keys[c] = pair_key(body, __float_as_uint(cmax.w) & 0x7fffffffu);
                     }

@@ -18,9 +18,11 @@
 // "forward" neighbour cells at its level (a same-level pair is reported once:
 // through the forward neighbour, or by the smaller sorted index inside one
 // cell) and the 27 cells around its min corner at every coarser non-empty
-// level (a cross-level pair is reported by the finer body).  Cell indices are
+// level of its world (a cross-level pair is reported by the finer body).  Cell indices are
 // computed in double precision, so the one-cell bound holds for coordinates
 // up to ~1e13 cell edges from the origin.
+//
+// Every cell key carries the body's sub-world, so batched worlds never pair.
 //
 // No sort: a hash table maps each packed cell key to the head of a linked
 // list of the bodies in that cell (one atomic exchange per body).  Probes are
@@ -54,8 +56,13 @@ __device__ __forceinline__ int64_t cell_of(float m, double edge) {
     return (int64_t)q;
 }
 
-__device__ __forceinline__ uint64_t cell_key(int l, uint64_t x, uint64_t y, uint64_t z) {
-    return ((uint64_t)l << 60) | ((x & 0xFFFFFull) << 40) | ((y & 0xFFFFFull) << 20) | (z & 0xFFFFFull);
+// Packed cell key: level (4 bits), sub-world (20 bits), cell coordinates
+// wrapped to 13 bits each.  Cells of one world that alias under the wrap are
+// 8192 cells apart, so their bodies fail the box test; bodies of different
+// worlds never share a key.
+__device__ __forceinline__ uint64_t cell_key(int l, uint32_t world, uint64_t x, uint64_t y, uint64_t z) {
+    return ((uint64_t)l << 60) | ((uint64_t)(world & 0xFFFFFu) << 39) | ((x & 0x1FFFull) << 26) |
+           ((y & 0x1FFFull) << 13) | (z & 0x1FFFull);
 }
 
 __device__ __forceinline__ float max_side(float4 mn, float4 mx) {
@@ -78,27 +85,35 @@ __device__ __forceinline__ uint32_t table_slot(DeviceWorld& w, uint64_t key) {
 // Level and cell of every body; the body joins its cell's list (the table
 // maps the packed cell key to the list head; heads start empty every step).
 __global__ void __launch_bounds__(256) k_grid_insert(DeviceWorld w) {
-    __shared__ uint32_t s_mask;
-    if (threadIdx.x == 0) s_mask = 0;
-    __syncthreads();
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t lmask = 0;
-    if (b < w.nbodies) {
+    bool alive = b < w.nbodies;
+    uint32_t lmask = 0, world = 0;
+    if (alive) {
         float4 mn = w.bmin[b], mx = w.bmax[b];
         float s0 = __uint_as_float(w.bounds[6]);
         int l = grid_level(max_side(mn, mx), s0);
         double e = level_edge(s0, l);
-        uint64_t key = cell_key(l, (uint64_t)cell_of(mn.x, e), (uint64_t)cell_of(mn.y, e), (uint64_t)cell_of(mn.z, e));
+        world = w.wid[b];
+        uint64_t key = cell_key(l, world, (uint64_t)cell_of(mn.x, e), (uint64_t)cell_of(mn.y, e),
+                                (uint64_t)cell_of(mn.z, e));
         w.cell_key[b] = key;
         uint32_t slot = table_slot(w, key);
         if (slot != ~0u) w.gnext[b] = atomicExch(&w.gt_head[slot], b);
         else w.counters[ctr(CTR_OVERFLOW)] = 6;
         lmask = 1u << l;
     }
-    for (int off = 32; off > 0; off >>= 1) lmask |= __shfl_xor(lmask, off);
-    if (__lane_id() == 0 && lmask) atomicOr(&s_mask, lmask);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_mask) atomicOr(&w.counters[ctr(CTR_LEVELS)], s_mask);
+    // Levels used per world: lanes of the first live lane's world combine
+    // their masks into one atomic (a world's bodies have consecutive ids).
+    uint64_t live = __ballot(alive);
+    if (live) {
+        int lead = __ffsll((unsigned long long)live) - 1;
+        uint32_t w0 = __shfl(world, lead);
+        bool same = alive && world == w0;
+        uint32_t m = same ? lmask : 0;
+        for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
+        if (__lane_id() == lead) atomicOr(&w.wlevels[w0], m);
+        if (alive && !same) atomicOr(&w.wlevels[world], lmask);
+    }
 }
 
 // List head of `key`, or kEndOfList when no body sits in that cell.
@@ -127,8 +142,8 @@ struct GridQuery {
 
 // Tests the bodies of one probed cell; `own` restricts to larger body ids
 // (the query's own cell at its own level).  The packed key identifies the
-// cell exactly up to the 20-bit coordinate wrap, and cells that alias are a
-// million cells apart, so their bodies fail the box test.
+// cell exactly up to the 13-bit coordinate wrap, and cells that alias are
+// 8192 cells apart, so their bodies fail the box test.
 __device__ __forceinline__ void probe_cell(DeviceWorld& w, const GridQuery& q, uint64_t key, uint64_t first,
                                            uint32_t slot, bool own, uint64_t* buf, uint32_t* buf_n) {
     for (uint32_t c = table_head(w, key, first, slot); c != kEndOfList; c = w.gnext[c]) {
@@ -153,15 +168,16 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
         q.dyn = (w.bflags[b] & 1u) != 0;
         const uint64_t qkey = w.cell_key[b];
         const int l = (int)(qkey >> 60);
+        const uint32_t world = w.wid[b];
         const uint32_t mask = w.gt_cap - 1;
         // Own level: own cell and forward neighbours (coordinates wrap with the key).
         {
-            const uint64_t cx = (qkey >> 40) & 0xFFFFFull, cy = (qkey >> 20) & 0xFFFFFull, cz = qkey & 0xFFFFFull;
+            const uint64_t cx = (qkey >> 26) & 0x1FFFull, cy = (qkey >> 13) & 0x1FFFull, cz = qkey & 0x1FFFull;
             uint64_t pk[14], first[14];
             uint32_t ps[14];
 #pragma unroll
             for (int o = 0; o < 14; ++o) {
-                pk[o] = cell_key(l, cx + (uint64_t)(int64_t)c_fwd[o][0], cy + (uint64_t)(int64_t)c_fwd[o][1],
+                pk[o] = cell_key(l, world, cx + (uint64_t)(int64_t)c_fwd[o][0], cy + (uint64_t)(int64_t)c_fwd[o][1],
                                  cz + (uint64_t)(int64_t)c_fwd[o][2]);
                 ps[o] = (uint32_t)mix64(pk[o]) & mask;
                 first[o] = w.gt_key[ps[o]];
@@ -170,7 +186,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
             for (int o = 0; o < 14; ++o) probe_cell(w, q, pk[o], first[o], ps[o], o == 0, buf, &buf_n);
         }
         // Coarser non-empty levels: the 27 cells around the min corner.
-        uint32_t coarse = w.counters[ctr(CTR_LEVELS)] & ~((2u << l) - 1u);
+        uint32_t coarse = w.wlevels[world] & ~((2u << l) - 1u);
         const float s0 = __uint_as_float(w.bounds[6]);
         while (coarse) {
             const int l2 = __ffs(coarse) - 1;
@@ -184,7 +200,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
                     uint32_t ps[3];
 #pragma unroll
                     for (int dx = -1; dx <= 1; ++dx) {
-                        pk[dx + 1] = cell_key(l2, cx + (uint64_t)(int64_t)dx, cy + (uint64_t)(int64_t)dy,
+                        pk[dx + 1] = cell_key(l2, world, cx + (uint64_t)(int64_t)dx, cy + (uint64_t)(int64_t)dy,
                                               cz + (uint64_t)(int64_t)dz);
                         ps[dx + 1] = (uint32_t)mix64(pk[dx + 1]) & mask;
                         first[dx + 1] = w.gt_key[ps[dx + 1]];

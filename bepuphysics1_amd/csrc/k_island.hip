@@ -293,7 +293,7 @@ __device__ __forceinline__ void setup_lane(const DeviceWorld& w, IslandLds<BLOCK
         gb = s.b;
         if (w.bflags[ga] & 1u) la = dbase + (int)w.body_local[ga];
         if (w.bflags[gb] & 1u) lb = dbase + (int)w.body_local[gb];
-        L.prio[tid] = mix64(s.key);
+        L.prio[tid] = mix64(local_pair_key(w, s.key, s.a));
         // A constraint has at most one kinematic body; it gets slot DYN + lane.
         int ext = DYN + tid;
         sa = la >= 0 ? la : ext;

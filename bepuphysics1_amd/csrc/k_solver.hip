@@ -3,7 +3,7 @@
 //
 // One cooperative kernel, phases separated by grid barriers:
 //   1. body -> constraint incidence counts (dynamic bodies only) and
-//      priorities mix64(key);
+//      priorities mix64(world-local key);
 //   2. exclusive scan of the counts (per-workgroup partials, then every
 //      workgroup adds the prefix of the partials before it);
 //   3. incidence lists;
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         const PairState& s = w.pairs_cur[w.gcons[c]];
         if (w.bflags[s.a] & 1u) atomicAdd(&w.inc_count[s.a], 1u);
         if (w.bflags[s.b] & 1u) atomicAdd(&w.inc_count[s.b], 1u);
-        w.prio[c] = mix64(s.key);
+        w.prio[c] = mix64(local_pair_key(w, s.key, s.a));
         w.color[c] = -1;
     }
     grid.sync();

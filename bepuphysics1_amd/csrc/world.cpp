@@ -28,6 +28,7 @@ struct HostBody {
     float4 pos, quat, linv, angv, ii0, ii1, ii2, li0, li1, li2, shape, stab, mat;
     float2 damp;
     uint32_t flags;
+    uint32_t wid, wbase;   // sub-world and its first body id
 };
 
 template <typename T>
@@ -60,6 +61,8 @@ struct bepu_world_s {
     uint32_t timed_steps = 0;
     uint32_t steps = 0;
     std::vector<void*> allocations;
+    // Sub-worlds: world k holds bodies [world_first[k], world_first[k] + world_count[k]).
+    std::vector<uint32_t> world_first{0}, world_count{0};
 };
 
 static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
@@ -150,6 +153,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.pos, n); A(&d.quat, n); A(&d.linv, n); A(&d.angv, n);
     A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
+    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, n);
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
     A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
@@ -243,7 +247,10 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape,
     b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
     b.flags = dyn ? 1u : 0u;
+    b.wid = (uint32_t)w->world_first.size() - 1;
+    b.wbase = w->world_first.back();
     w->staged.push_back(b);
+    w->world_count.back()++;
     if (id) *id = n;
     return BEPU_OK;
 }
@@ -267,11 +274,15 @@ static bepu_status upload(bepu_world w) {
               put4(d.mat, &HostBody::mat);
     for (size_t i = 0; i < k; ++i) tmp2[i] = w->staged[i].damp;
     ok = ok && hipMemcpy(d.damp + o, tmp2.data(), k * sizeof(float2), hipMemcpyHostToDevice) == hipSuccess;
-    for (size_t i = 0; i < k; ++i) tmpu[i] = w->staged[i].flags;
-    ok = ok && hipMemcpy(d.bflags + o, tmpu.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
+    auto putu = [&](uint32_t* dst, uint32_t HostBody::*f) -> bool {
+        for (size_t i = 0; i < k; ++i) tmpu[i] = w->staged[i].*f;
+        return hipMemcpy(dst + o, tmpu.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    ok = ok && putu(d.bflags, &HostBody::flags) && putu(d.wid, &HostBody::wid) && putu(d.wbase, &HostBody::wbase);
     if (!ok) return BEPU_E_DEVICE;
     w->uploaded += (uint32_t)k;
     d.nbodies = w->uploaded;
+    d.nworlds = (uint32_t)w->world_first.size();
     tree_layout(d.nbodies, &d.tree_levels, d.tree_off, d.tree_cnt);
     w->staged.clear();
     return BEPU_OK;
@@ -291,6 +302,24 @@ bepu_status bepu_add_sphere(bepu_world w, const bepu_motion_state* s, float radi
                             const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!(radius > 0)) return BEPU_E_INVALID_ARG;
     return add_body(w, s, BEPU_SHAPE_SPHERE, f3{radius, 0, 0}, sphere_volume_distribution(radius), mass, m, flags, id);
+}
+
+bepu_status bepu_begin_world(bepu_world w, uint32_t* world_id) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (w->world_count.back() > 0) {
+        if (w->world_first.size() >= (1u << 20)) return BEPU_E_CAPACITY;
+        w->world_first.push_back(w->uploaded + (uint32_t)w->staged.size());
+        w->world_count.push_back(0);
+    }
+    if (world_id) *world_id = (uint32_t)w->world_first.size() - 1;
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_world_range(bepu_world w, uint32_t world_id, uint32_t* first, uint32_t* count) {
+    if (!w || world_id >= w->world_first.size()) return BEPU_E_INVALID_ARG;
+    if (first) *first = w->world_first[world_id];
+    if (count) *count = w->world_count[world_id];
+    return BEPU_OK;
 }
 
 bepu_status bepu_set_timing(bepu_world w, int32_t enable) {

@@ -68,6 +68,10 @@ struct DeviceWorld {
     float2* damp;              // linear, angular damping
     float4* mat;               // static, kinetic, bounce
     uint32_t* bflags;          // bit0 dynamic
+    uint32_t* wid;             // sub-world of every body (bodies of different sub-worlds never interact)
+    uint32_t* wbase;           // first body id of the body's sub-world
+    uint32_t* wlevels;         // per sub-world: grid levels holding at least one of its bodies
+    uint32_t nworlds;
     // broadphase
     uint32_t* morton;          // Morton codes (tree broadphase)
     uint32_t* morton_sorted;
@@ -137,7 +141,7 @@ enum {
     CTR_MAXPEN = 4,       // float bits (non-negative), max via atomicMax on uint
     CTR_OVERFLOW = 5,     // pair/hash capacity overflow flag
     CTR_ROUND = 6,        // colouring rounds
-    CTR_LEVELS = 7,       // grid levels holding at least one body (bit mask)
+    CTR_FREE7 = 7,
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
     CTR_GCONS = 11,       // constraints on the global path
     CTR_ISL_MEDIUM = 13,
@@ -150,6 +154,14 @@ enum {
 // serialise (about 10 ns each on MI355X), atomics on different lines do not.
 constexpr uint32_t kCtrStride = 64;
 __host__ __device__ constexpr uint32_t ctr(uint32_t i) { return i * kCtrStride; }
+
+// Pair key with world-local body ids: the colouring priority of a constraint
+// depends only on its own world, so a world solves identically alone or
+// batched with others.
+__device__ __forceinline__ uint64_t local_pair_key(const DeviceWorld& w, uint64_t key, uint32_t a) {
+    uint64_t base = w.wbase[a];
+    return key - ((base << 32) | base);
+}
 
 // Host-side launchers (one per stage), defined in the kernel sources.
 void launch_body_pre(const DeviceWorld& w, hipStream_t st);

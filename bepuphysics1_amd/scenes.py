@@ -184,3 +184,13 @@ def mixed_sizes(n=2048, seed=0, edge=200.0, offset=(0.0, 0.0, 0.0), static_fract
         else:
             out[i] = _row(SHAPE_SPHERE, (size[0], 0.0, 0.0), mass, pos[i], quat=q[i])
     return out
+
+
+def batched_worlds(n_worlds=4096, seed=0, nmin=8, nmax=256, shapes=(SHAPE_BOX, SHAPE_SPHERE), drop=(1.0, 10.0)):
+    """Independent worlds for the batched workload: world k is a random pile
+    (its own ground) of N_k bodies, N_k log-uniform in [nmin, nmax], dropped
+    from heights uniform in `drop`.  Returns a list of scene arrays."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_worlds)).astype(np.int64)
+    seeds = rng.integers(0, 2**31 - 1, n_worlds)
+    return [random_pile(int(n), seed=int(s), shapes=shapes, drop=drop) for n, s in zip(sizes, seeds)]

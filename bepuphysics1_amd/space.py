@@ -247,6 +247,25 @@ class Space:
     def _uploaded(self, idx):
         return idx is not None and idx < self._n_uploaded
 
+    def begin_world(self):
+        """Starts a new independent sub-world (the analogue of another Space):
+        bodies added afterwards belong to it and never pair with other worlds."""
+        out = ctypes.c_uint32()
+        check(self._lib.bepu_begin_world(self._handle(), ctypes.byref(out)), "begin_world")
+        return out.value
+
+    def add_world(self, scene):
+        """Adds `scene` as a new sub-world; returns (world id, body ids)."""
+        wid = self.begin_world()
+        return wid, self.add_scene(scene)
+
+    def world_range(self, wid):
+        """(first body id, body count) of a sub-world."""
+        first, count = ctypes.c_uint32(), ctypes.c_uint32()
+        check(self._lib.bepu_get_world_range(self._handle(), wid, ctypes.byref(first), ctypes.byref(count)),
+              "world_range")
+        return first.value, count.value
+
     def add_scene(self, scene):
         """Adds every row of a bepuphysics1_amd.scenes array; returns the ids."""
         lib, h = self._lib, self._handle()

@@ -179,6 +179,18 @@ bepu_status bepu_add_sphere(bepu_world world, const bepu_motion_state* state, fl
                             float mass, const bepu_material* material, uint32_t flags,
                             uint32_t* id_out);
 
+/* Batched worlds.  A world handle may hold many independent sub-worlds, each
+ * the analogue of one Space: bodies of different sub-worlds never form pairs,
+ * and every sub-world's result is bit-identical to stepping it alone (its
+ * arithmetic, including the constraint colouring, depends on world-local ids
+ * only).  bepu_begin_world starts a new sub-world whose bodies are those added
+ * after the call (a sub-world without bodies is reused); sub-world 0 exists
+ * from creation.  One bepu_step steps every sub-world.  No reference
+ * counterpart: the reference steps one Space per Update call. */
+bepu_status bepu_begin_world(bepu_world world, uint32_t* world_id_out);
+bepu_status bepu_get_world_range(bepu_world world, uint32_t world_id, uint32_t* first_body_out,
+                                 uint32_t* count_out);
+
 /* Space.Update() x n_steps (Space.cs:527-531). */
 bepu_status bepu_step(bepu_world world, int32_t n_steps);
 

@@ -169,3 +169,41 @@ def test_mixed_tiers_one_world():
     st = g.stats()
     assert 0 < st.global_manifolds < st.manifolds
     assert np.array_equal(f32bits(g.motion_states()), f32bits(o.motion_states()))
+
+
+def test_batched_worlds_match_alone():
+    """24 independent piles stepped as sub-worlds of one batch: each world's
+    state and impulses are bit-identical to the same world stepped alone on
+    the GPU, its pairs never cross worlds, and a sample matches the oracle."""
+    from bepuphysics1_amd import Space
+    worlds = scenes.batched_worlds(24, seed=7, nmin=8, nmax=64)
+    steps = 40
+    sp = Space(capacity=sum(len(w) for w in worlds))
+    for sc in worlds:
+        sp.add_world(sc)
+    sp.Update(steps)
+    ms_b, pairs_b = sp.motion_states(), sp.pairs()
+    imp_b = impulse_table(sp.impulses())
+    lo, hi = (pairs_b >> 32).astype(np.int64), (pairs_b & 0xffffffff).astype(np.int64)
+    for k, sc in enumerate(worlds):
+        first, count = sp.world_range(k)
+        assert count == len(sc)
+        inside_lo = (lo >= first) & (lo < first + count)
+        inside_hi = (hi >= first) & (hi < first + count)
+        assert np.array_equal(inside_lo, inside_lo & inside_hi) or not inside_lo.any()
+        mine = pairs_b[inside_lo] - np.uint64((first << 32) | first)
+        g = run_gpu(sc, steps)
+        assert np.array_equal(np.sort(mine), g.pairs()), "world %d pairs" % k
+        assert np.array_equal(f32bits(ms_b[first:first + count]), f32bits(g.motion_states())), "world %d" % k
+        ia = impulse_table(g.impulses())
+        shift = (first << 32) | first
+        for key, v in ia.items():
+            assert imp_b[key + shift] == v, "world %d impulses" % k
+        g.close()
+        if k < 4:
+            o = run_oracle(sc, steps)
+            assert np.array_equal(f32bits(ms_b[first:first + count]), f32bits(o.motion_states())), "oracle %d" % k
+    # no pair joins two worlds
+    wid = np.searchsorted(np.array([sp.world_range(k)[0] for k in range(len(worlds))]), lo, side="right")
+    wid2 = np.searchsorted(np.array([sp.world_range(k)[0] for k in range(len(worlds))]), hi, side="right")
+    assert np.array_equal(wid, wid2)
