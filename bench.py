@@ -189,6 +189,30 @@ def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
             "ratio_vs_plain": round(batched / plain, 1)}
 
 
+def served_leg(rate, duration, max_wall):
+    """Served rollouts: Poisson arrivals at `rate`/s for `duration` s of wall
+    time, each a random pile of N ~ logU[16,2048] bodies (boxes and spheres,
+    own ground) stepped S ~ U[60,600] times at 8 iterations; continuous
+    batching on one world handle (admission against capacity and a 200 GB
+    budget), then the queue drains."""
+    from bepuphysics1_amd import scenes
+    from bepuphysics1_amd._lib import default_settings
+    from bepuphysics1_amd.serve import Server, run_poisson
+    st = default_settings()
+    st.iteration_limit = 8
+    srv = Server(capacity=1_000_000, settings=st, hbm_budget=200e9)
+
+    def make_request(rng):
+        n = int(np.exp(rng.uniform(np.log(16), np.log(2048))))
+        return scenes.random_pile_fast(n, rng), int(rng.integers(60, 601))
+
+    out = run_poisson(srv, rate, duration, make_request, seed=0, drain=True, max_wall=max_wall)
+    srv.close()
+    out["workload"] = ("Poisson %g requests/s for %g s, N ~ logU[16,2048] boxes + spheres, S ~ U[60,600], "
+                       "8 iterations, continuous batching" % (rate, duration))
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30):
     """Plain device-to-device copy of a known size (the HIP runtime's copy
     kernel), run in the same process as the workload so that one profiler
@@ -224,6 +248,9 @@ def main():
     ap.add_argument("--batched-worlds", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--served-rate", type=float, default=2000.0)
+    ap.add_argument("--served-seconds", type=float, default=3.0)
+    ap.add_argument("--no-served", action="store_true")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -293,6 +320,7 @@ def main():
     parity = None if a.no_parity else parity_check()
     copy_bytes = calibrate_copy() if a.calibrate_copy else None
     batched = None if a.no_batched else batched_leg(a.batched_worlds, a.batched_steps, 5, 48, 10)
+    served = None if a.no_served else served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0)
 
     line = {
         "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
@@ -324,6 +352,7 @@ def main():
         "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "parity": parity,
         "batched": batched,
+        "served": served,
         "calibration_copy_bytes": copy_bytes,
         "towers_standing": standing,
         "max_penetration": round(float(st.max_penetration), 5),

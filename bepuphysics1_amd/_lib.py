@@ -86,6 +86,11 @@ class StepStats(ctypes.Structure):
 
 
 # Every entry point of include/bepu_gpu.h: name -> (restype, argtypes)
+class BodyDesc(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32), ("dims", ctypes.c_float * 3), ("mass", ctypes.c_float),
+                ("state", MotionState), ("material", Material), ("flags", ctypes.c_uint32)]
+
+
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 EXPORTS = {
@@ -108,6 +113,12 @@ EXPORTS = {
     "bepu_set_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_set_broadphase": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_begin_world": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
+    "bepu_insert_world": (ctypes.c_int, [_vp, ctypes.POINTER(BodyDesc), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "bepu_remove_world": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "bepu_synchronize": (ctypes.c_int, [_vp]),
+    "bepu_get_motion_states_range": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32,
+                                                    ctypes.POINTER(MotionState)]),
     "bepu_get_world_range": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(ctypes.c_uint32)]),
     "bepu_device_available": (ctypes.c_int32, []),

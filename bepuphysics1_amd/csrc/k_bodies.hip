@@ -37,10 +37,17 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bepu_settings& s = w.s;
     float cx = 0, cy = 0, cz = 0, side = 0;
-    bool valid = i < w.nbodies;
-    if (valid) {
+    bool valid = false;
+    if (i < w.nbodies) {
         w.uf_parent[i] = i;  // union-find forest of this step's islands
-        bool dyn = (w.bflags[i] & 1u) != 0;
+        valid = (w.bflags[i] & BF_PRESENT) != 0;
+        if (!valid) {  // a removed world's slot: an empty box overlaps nothing
+            w.bmin[i] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+            w.bmax[i] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        }
+    }
+    if (valid) {
+        bool dyn = (w.bflags[i] & BF_DYNAMIC) != 0;
         float4 P = w.pos[i];
         float4 Q = w.quat[i];
         float4 V = w.linv[i];
@@ -192,7 +199,7 @@ __global__ void __launch_bounds__(256) k_bounds_reduce(DeviceWorld w, uint32_t n
 // Entity.PreUpdatePosition (EntityBase.cs:1124-1155), discrete mode.
 __global__ void __launch_bounds__(256) k_integrate(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w.nbodies) return;
+    if (i >= w.nbodies || !(w.bflags[i] & BF_PRESENT)) return;
     float dt = w.s.dt;
     float4 Q = w.quat[i];
     float4 P = w.pos[i];

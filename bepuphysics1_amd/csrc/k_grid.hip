@@ -86,7 +86,7 @@ __device__ __forceinline__ uint32_t table_slot(DeviceWorld& w, uint64_t key) {
 // maps the packed cell key to the list head; heads start empty every step).
 __global__ void __launch_bounds__(256) k_grid_insert(DeviceWorld w) {
     uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    bool alive = b < w.nbodies;
+    bool alive = b < w.nbodies && (w.bflags[b] & BF_PRESENT);
     uint32_t lmask = 0, world = 0;
     if (alive) {
         float4 mn = w.bmin[b], mx = w.bmax[b];
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
     if (threadIdx.x == 0) buf_n = 0;
     __syncthreads();
     const uint32_t b = blockIdx.x * kQueryBlock + threadIdx.x;
-    if (b < w.nbodies) {
+    if (b < w.nbodies && (w.bflags[b] & BF_PRESENT)) {
         GridQuery q;
         q.body = b;
         q.qmin = w.bmin[b];

@@ -682,7 +682,11 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= npairs) return;
     uint64_t key = w.pair_keys[t];
-    int old = hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
+    // A body inserted for this step (possibly into ids a removed world used)
+    // has no history: never pick up the previous step's state for its pairs.
+    uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
+    bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
+    int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
     PairState s;
     if (old >= 0) s = w.pairs_old[old];
     else init_pair(w, key, s);

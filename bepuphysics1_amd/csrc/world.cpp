@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <map>
 #include <vector>
 
 #include "../../include/bepu_gpu.h"
@@ -29,6 +30,7 @@ struct HostBody {
     float2 damp;
     uint32_t flags;
     uint32_t wid, wbase;   // sub-world and its first body id
+    uint32_t born;         // index of the first step the body takes part in
 };
 
 template <typename T>
@@ -63,6 +65,10 @@ struct bepu_world_s {
     std::vector<void*> allocations;
     // Sub-worlds: world k holds bodies [world_first[k], world_first[k] + world_count[k]).
     std::vector<uint32_t> world_first{0}, world_count{0};
+    std::vector<uint8_t> world_live{1};
+    uint32_t cur_world = 0;                  // world the add calls append to
+    std::vector<uint32_t> free_world_ids;
+    std::map<uint32_t, uint32_t> free_ranges;  // body ranges of removed worlds: first -> count
 };
 
 static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
@@ -153,7 +159,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.pos, n); A(&d.quat, n); A(&d.linv, n); A(&d.angv, n);
     A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
-    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, n);
+    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, n); A(&d.born, n);
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
     A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
@@ -212,16 +218,14 @@ static bool finite_n(const float* v, int n) {
 // Entity construction: Entity.Initialize(collidable, mass) -> BecomeDynamic /
 // BecomeKinematic (EntityBase.cs:632-645, 867-905), Orientation setter
 // normalisation (EntityBase.cs:68-84).
-static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape, f3 dims, const m3& vd, float mass,
-                            const bepu_material* m, uint32_t flags, uint32_t* id) {
+static bepu_status make_body(bepu_world w, const bepu_motion_state* s, int shape, f3 dims, const m3& vd, float mass,
+                             const bepu_material* m, uint32_t flags, HostBody& b) {
     if (!w || !s) return BEPU_E_INVALID_ARG;
     if (std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
     if (!finite_n(s->position, 3) || !finite_n(s->orientation, 4) || !finite_n(s->linear_velocity, 3) ||
         !finite_n(s->angular_velocity, 3))
         return BEPU_E_INVALID_ARG;
-    uint32_t n = w->uploaded + (uint32_t)w->staged.size();
-    if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
-    HostBody b{};
+    b = HostBody{};
     bool dyn = mass > 0;
     m3 li{0, 0, 0, 0, 0, 0, 0, 0, 0};
     float im = 0;
@@ -246,25 +250,39 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape,
     b.stab = make_float4(0, 0, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
     b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
-    b.flags = dyn ? 1u : 0u;
-    b.wid = (uint32_t)w->world_first.size() - 1;
-    b.wbase = w->world_first.back();
+    b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT;
+    b.born = w->steps;
+    return BEPU_OK;
+}
+
+static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape, f3 dims, const m3& vd, float mass,
+                            const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    uint32_t n = w->uploaded + (uint32_t)w->staged.size();
+    if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
+    HostBody b;
+    bepu_status st = make_body(w, s, shape, dims, vd, mass, m, flags, b);
+    if (st != BEPU_OK) return st;
+    b.wid = w->cur_world;
+    b.wbase = w->world_first[w->cur_world];
     w->staged.push_back(b);
-    w->world_count.back()++;
+    w->world_count[w->cur_world]++;
     if (id) *id = n;
     return BEPU_OK;
 }
 
-static bepu_status upload(bepu_world w) {
-    if (w->staged.empty()) return BEPU_OK;
-    size_t k = w->staged.size();
-    uint32_t o = w->uploaded;
+// Copies body records to device ids [o, o + k) (after the work already queued
+// on the world's stream, which may read them).
+static bepu_status upload_range(bepu_world w, uint32_t o, const std::vector<HostBody>& src) {
+    size_t k = src.size();
+    if (k == 0) return BEPU_OK;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
     std::vector<float4> tmp(k);
     std::vector<float2> tmp2(k);
     std::vector<uint32_t> tmpu(k);
     DeviceWorld& d = w->d;
     auto put4 = [&](float4* dst, float4 HostBody::*f) -> bool {
-        for (size_t i = 0; i < k; ++i) tmp[i] = w->staged[i].*f;
+        for (size_t i = 0; i < k; ++i) tmp[i] = src[i].*f;
         return hipMemcpy(dst + o, tmp.data(), k * sizeof(float4), hipMemcpyHostToDevice) == hipSuccess;
     };
     bool ok = put4(d.pos, &HostBody::pos) && put4(d.quat, &HostBody::quat) && put4(d.linv, &HostBody::linv) &&
@@ -272,20 +290,40 @@ static bepu_status upload(bepu_world w) {
               put4(d.ii2, &HostBody::ii2) && put4(d.li0, &HostBody::li0) && put4(d.li1, &HostBody::li1) &&
               put4(d.li2, &HostBody::li2) && put4(d.shape, &HostBody::shape) && put4(d.stab, &HostBody::stab) &&
               put4(d.mat, &HostBody::mat);
-    for (size_t i = 0; i < k; ++i) tmp2[i] = w->staged[i].damp;
+    for (size_t i = 0; i < k; ++i) tmp2[i] = src[i].damp;
     ok = ok && hipMemcpy(d.damp + o, tmp2.data(), k * sizeof(float2), hipMemcpyHostToDevice) == hipSuccess;
     auto putu = [&](uint32_t* dst, uint32_t HostBody::*f) -> bool {
-        for (size_t i = 0; i < k; ++i) tmpu[i] = w->staged[i].*f;
+        for (size_t i = 0; i < k; ++i) tmpu[i] = src[i].*f;
         return hipMemcpy(dst + o, tmpu.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
     };
-    ok = ok && putu(d.bflags, &HostBody::flags) && putu(d.wid, &HostBody::wid) && putu(d.wbase, &HostBody::wbase);
-    if (!ok) return BEPU_E_DEVICE;
-    w->uploaded += (uint32_t)k;
+    ok = ok && putu(d.bflags, &HostBody::flags) && putu(d.wid, &HostBody::wid) && putu(d.wbase, &HostBody::wbase) &&
+         putu(d.born, &HostBody::born);
+    return ok ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+static bepu_status upload(bepu_world w) {
+    if (w->staged.empty()) return BEPU_OK;
+    if (upload_range(w, w->uploaded, w->staged) != BEPU_OK) return BEPU_E_DEVICE;
+    DeviceWorld& d = w->d;
+    w->uploaded += (uint32_t)w->staged.size();
     d.nbodies = w->uploaded;
     d.nworlds = (uint32_t)w->world_first.size();
     tree_layout(d.nbodies, &d.tree_levels, d.tree_off, d.tree_cnt);
     w->staged.clear();
     return BEPU_OK;
+}
+
+static uint32_t new_world_id(bepu_world w) {
+    if (!w->free_world_ids.empty()) {
+        uint32_t id = w->free_world_ids.back();
+        w->free_world_ids.pop_back();
+        return id;
+    }
+    if (w->world_first.size() >= (1u << 20)) return ~0u;
+    w->world_first.push_back(0);
+    w->world_count.push_back(0);
+    w->world_live.push_back(0);
+    return (uint32_t)w->world_first.size() - 1;
 }
 
 extern "C" {
@@ -306,19 +344,149 @@ bepu_status bepu_add_sphere(bepu_world w, const bepu_motion_state* s, float radi
 
 bepu_status bepu_begin_world(bepu_world w, uint32_t* world_id) {
     if (!w) return BEPU_E_INVALID_ARG;
-    if (w->world_count.back() > 0) {
-        if (w->world_first.size() >= (1u << 20)) return BEPU_E_CAPACITY;
-        w->world_first.push_back(w->uploaded + (uint32_t)w->staged.size());
-        w->world_count.push_back(0);
+    if (w->world_count[w->cur_world] > 0) {
+        uint32_t id = new_world_id(w);
+        if (id == ~0u) return BEPU_E_CAPACITY;
+        w->world_first[id] = w->uploaded + (uint32_t)w->staged.size();
+        w->world_count[id] = 0;
+        w->world_live[id] = 1;
+        w->cur_world = id;
     }
-    if (world_id) *world_id = (uint32_t)w->world_first.size() - 1;
+    if (world_id) *world_id = w->cur_world;
     return BEPU_OK;
 }
 
 bepu_status bepu_get_world_range(bepu_world w, uint32_t world_id, uint32_t* first, uint32_t* count) {
-    if (!w || world_id >= w->world_first.size()) return BEPU_E_INVALID_ARG;
+    if (!w || world_id >= w->world_first.size() || !w->world_live[world_id]) return BEPU_E_INVALID_ARG;
     if (first) *first = w->world_first[world_id];
     if (count) *count = w->world_count[world_id];
+    return BEPU_OK;
+}
+
+bepu_status bepu_insert_world(bepu_world w, const bepu_body_desc* bodies, uint32_t n, uint32_t* world_id,
+                              uint32_t* first_body) {
+    if (!w || !bodies || n == 0) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    std::vector<HostBody> recs(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const bepu_body_desc& b = bodies[i];
+        bepu_status st;
+        if (b.shape == BEPU_SHAPE_BOX) {
+            if (!(b.dims[0] > 0) || !(b.dims[1] > 0) || !(b.dims[2] > 0)) return BEPU_E_INVALID_ARG;
+            st = make_body(w, &b.state, BEPU_SHAPE_BOX, f3{b.dims[0] * 0.5f, b.dims[1] * 0.5f, b.dims[2] * 0.5f},
+                           box_volume_distribution(b.dims[0], b.dims[1], b.dims[2]), b.mass, &b.material, b.flags,
+                           recs[i]);
+        } else if (b.shape == BEPU_SHAPE_SPHERE) {
+            if (!(b.dims[0] > 0)) return BEPU_E_INVALID_ARG;
+            st = make_body(w, &b.state, BEPU_SHAPE_SPHERE, f3{b.dims[0], 0, 0}, sphere_volume_distribution(b.dims[0]),
+                           b.mass, &b.material, b.flags, recs[i]);
+        } else {
+            return BEPU_E_UNSUPPORTED;
+        }
+        if (st != BEPU_OK) return st;
+    }
+    // First fit among the ranges of removed worlds, else the end of the array.
+    uint32_t first = ~0u;
+    for (auto it = w->free_ranges.begin(); it != w->free_ranges.end(); ++it)
+        if (it->second >= n) {
+            first = it->first;
+            uint32_t rest = it->second - n;
+            w->free_ranges.erase(it);
+            if (rest) w->free_ranges[first + n] = rest;
+            break;
+        }
+    bool append = first == ~0u;
+    if (append) {
+        if ((uint64_t)w->uploaded + n > w->d.body_cap) return BEPU_E_CAPACITY;
+        first = w->uploaded;
+    }
+    uint32_t id = new_world_id(w);
+    if (id == ~0u) {
+        if (!append) w->free_ranges[first] = n;
+        return BEPU_E_CAPACITY;
+    }
+    for (auto& r : recs) {
+        r.wid = id;
+        r.wbase = first;
+    }
+    if (upload_range(w, first, recs) != BEPU_OK) return BEPU_E_DEVICE;
+    w->world_first[id] = first;
+    w->world_count[id] = n;
+    w->world_live[id] = 1;
+    if (append) {
+        w->uploaded += n;
+        w->d.nbodies = w->uploaded;
+        tree_layout(w->d.nbodies, &w->d.tree_levels, w->d.tree_off, w->d.tree_cnt);
+        // Later add calls open a new sub-world after this one.
+        w->cur_world = id;
+    }
+    w->d.nworlds = (uint32_t)w->world_first.size();
+    if (world_id) *world_id = id;
+    if (first_body) *first_body = first;
+    return BEPU_OK;
+}
+
+bepu_status bepu_remove_world(bepu_world w, uint32_t world_id) {
+    if (!w || world_id >= w->world_first.size() || !w->world_live[world_id]) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    uint32_t first = w->world_first[world_id], n = w->world_count[world_id];
+    if (n) launch_fill_u32(w->d.bflags + first, 0u, n, w->stream);  // absent: no box, no pairs, not dynamic
+    w->world_live[world_id] = 0;
+    w->world_count[world_id] = 0;
+    if (n) {
+        // Merge with free neighbours.
+        auto next = w->free_ranges.lower_bound(first);
+        if (next != w->free_ranges.end() && next->first == first + n) {
+            n += next->second;
+            w->free_ranges.erase(next);
+        }
+        auto prev = w->free_ranges.lower_bound(first);
+        if (prev != w->free_ranges.begin()) {
+            --prev;
+            if (prev->first + prev->second == first) {
+                first = prev->first;
+                n += prev->second;
+                w->free_ranges.erase(prev);
+            }
+        }
+        w->free_ranges[first] = n;
+    }
+    if (world_id == w->cur_world) {
+        // Appends continue in a fresh sub-world.
+        uint32_t id = new_world_id(w);
+        w->world_first[id] = w->uploaded + (uint32_t)w->staged.size();
+        w->world_count[id] = 0;
+        w->world_live[id] = 1;
+        w->cur_world = id;
+    }
+    w->free_world_ids.push_back(world_id);
+    return BEPU_OK;
+}
+
+bepu_status bepu_synchronize(bepu_world w) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    return check(hipStreamSynchronize(w->stream));
+}
+
+bepu_status bepu_get_motion_states_range(bepu_world w, uint32_t first, uint32_t count, bepu_motion_state* out) {
+    if (!w || (!out && count)) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    if ((uint64_t)first + count > w->d.nbodies) return BEPU_E_INVALID_ARG;
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    std::vector<float4> p(count), q(count), v(count), a(count);
+    bool ok = count == 0 ||
+              (hipMemcpy(p.data(), w->d.pos + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
+               hipMemcpy(q.data(), w->d.quat + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
+               hipMemcpy(v.data(), w->d.linv + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
+               hipMemcpy(a.data(), w->d.angv + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess);
+    if (!ok) return BEPU_E_DEVICE;
+    for (uint32_t i = 0; i < count; ++i) {
+        bepu_motion_state& m = out[i];
+        m.position[0] = p[i].x; m.position[1] = p[i].y; m.position[2] = p[i].z;
+        m.orientation[0] = q[i].x; m.orientation[1] = q[i].y; m.orientation[2] = q[i].z; m.orientation[3] = q[i].w;
+        m.linear_velocity[0] = v[i].x; m.linear_velocity[1] = v[i].y; m.linear_velocity[2] = v[i].z;
+        m.angular_velocity[0] = a[i].x; m.angular_velocity[1] = a[i].y; m.angular_velocity[2] = a[i].z;
+    }
     return BEPU_OK;
 }
 
@@ -353,6 +521,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     for (int k = 0; k < n_steps; ++k) {
         int slot = (int)timed - (n_steps - k);  // >= 0 for the timed steps
         hipEvent_t* e = slot >= 0 ? &w->ev[(size_t)slot * (ST_COUNT + 1)] : nullptr;
+        d.step_no = w->steps;
         launch_step_reset(d, s);
         if (e) (void)hipEventRecord(e[0], s);
         launch_body_pre(d, s);

@@ -48,6 +48,7 @@ struct PairState {
 };
 
 enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u };
+enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u };
 
 struct RowConst;
 struct RowVar;
@@ -67,11 +68,13 @@ struct DeviceWorld {
     float4* stab;              // prev velocity, time below, slowing, allow stabilization
     float2* damp;              // linear, angular damping
     float4* mat;               // static, kinetic, bounce
-    uint32_t* bflags;          // bit0 dynamic
+    uint32_t* bflags;          // BF_* bits (0: absent -- a removed world's slot)
     uint32_t* wid;             // sub-world of every body (bodies of different sub-worlds never interact)
     uint32_t* wbase;           // first body id of the body's sub-world
     uint32_t* wlevels;         // per sub-world: grid levels holding at least one of its bodies
     uint32_t nworlds;
+    uint32_t* born;            // index of the first step a body takes part in
+    uint32_t step_no;          // index of the step being run
     // broadphase
     uint32_t* morton;          // Morton codes (tree broadphase)
     uint32_t* morton_sorted;

@@ -194,3 +194,31 @@ def batched_worlds(n_worlds=4096, seed=0, nmin=8, nmax=256, shapes=(SHAPE_BOX, S
     sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), n_worlds)).astype(np.int64)
     seeds = rng.integers(0, 2**31 - 1, n_worlds)
     return [random_pile(int(n), seed=int(s), shapes=shapes, drop=drop) for n, s in zip(sizes, seeds)]
+
+
+def random_pile_fast(n, rng, drop=(1.0, 10.0), sphere_fraction=0.5):
+    """Vectorised random pile (same distribution family as random_pile):
+    boxes with half extents in [0.25, 1] and spheres with radius in
+    [0.25, 1], uniform random orientations, over a sqrt(n) footprint,
+    dropped from heights uniform in `drop`, plus the world's own ground."""
+    foot = 2.0 * np.sqrt(max(n, 1)) + 2.0
+    out = np.zeros(n + 1, BODY_DTYPE)
+    out[0] = ground(foot + 20.0, 1.0, foot + 20.0)
+    body = out[1:]
+    sph = rng.uniform(size=n) < sphere_fraction
+    body["shape"] = np.where(sph, SHAPE_SPHERE, SHAPE_BOX)
+    dims = 2 * rng.uniform(0.25, 1.0, (n, 3))
+    rad = rng.uniform(0.25, 1.0, n)
+    dims[sph, 0] = rad[sph]
+    dims[sph, 1:] = 0.0
+    body["dims"] = dims
+    body["mass"] = 1.0
+    st = np.zeros((n, 13), np.float32)
+    st[:, 0] = rng.uniform(-foot / 2, foot / 2, n)
+    st[:, 1] = rng.uniform(drop[0], drop[1], n)
+    st[:, 2] = rng.uniform(-foot / 2, foot / 2, n)
+    st[:, 3:7] = random_quaternions(rng, n)
+    body["state"] = st
+    body["material"] = (1.0, 0.8, 0.0)
+    body["flags"] = FLAG_ALWAYS_ACTIVE
+    return out

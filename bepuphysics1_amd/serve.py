@@ -1,0 +1,148 @@
+"""Continuous batching of simulation rollouts on one GPU world handle.
+
+A request is a world (a scene array) to be stepped S times.  All resident
+worlds are sub-worlds of one device world and advance together, one
+`bepu_step` per tick.  Between ticks, finished worlds are read back and
+removed (their body ranges and ids are freed) and waiting requests are
+admitted while the body capacity and the HBM budget allow, so new work joins
+a batch that is already running on the device.  Every world's result is
+bit-identical to stepping it alone: sub-worlds never pair with each other,
+their colouring uses world-local ids, and an inserted world never inherits
+the contact history of a previous occupant of its ids.
+
+There is no reference counterpart: the reference steps one Space per
+Update call (Space.cs:527-531); each request here would be its own Space.
+"""
+import collections
+import time
+
+import numpy as np
+
+from .space import Space
+
+# Device bytes per body slot of a world handle: body arrays (~250 B), and the
+# pair, hash, island and solver arrays sized 4 pairs per body (~3 KB).
+BYTES_PER_BODY = 3400
+
+
+class Request:
+    __slots__ = ("rid", "scene", "steps", "arrival", "admitted", "finished", "done", "world", "first", "result",
+                 "tick_latencies", "n_dyn")
+
+    def __init__(self, rid, scene, steps, arrival):
+        self.rid, self.scene, self.steps, self.arrival = rid, scene, int(steps), arrival
+        self.n_dyn = int((scene["mass"] > 0).sum())
+        self.admitted = self.finished = None
+        self.done = 0
+        self.world = self.first = None
+        self.result = None
+        self.tick_latencies = []
+
+
+class Server:
+    """Admission queue + resident batch.  `capacity` bodies are allocated on
+    the device up front; `hbm_budget` (bytes) caps admission as well."""
+
+    def __init__(self, capacity, settings=None, hbm_budget=200e9, clock=time.perf_counter):
+        self.capacity = int(min(capacity, hbm_budget // BYTES_PER_BODY))
+        self.space = Space(capacity=self.capacity, settings=settings)
+        self.queue = collections.deque()
+        self.resident = {}
+        self.completed = []
+        self.resident_bodies = 0
+        self.clock = clock
+        self.ticks = 0
+        self.body_steps = 0
+        self._next = 0
+
+    def submit(self, scene, steps, arrival=None):
+        r = Request(self._next, scene, steps, self.clock() if arrival is None else arrival)
+        self._next += 1
+        self.queue.append(r)
+        return r
+
+    def admit(self):
+        """Admits queued requests in arrival order while they fit."""
+        while self.queue:
+            r = self.queue[0]
+            n = len(r.scene)
+            if self.resident_bodies + n > self.capacity:
+                break
+            try:
+                r.world, r.first = self.space.insert_world(r.scene)
+            except Exception as e:  # capacity exhausted by fragmentation: wait for retirements
+                if getattr(e, "status", None) == 7:
+                    break
+                raise
+            self.queue.popleft()
+            r.admitted = self.clock()
+            self.resident[r.rid] = r
+            self.resident_bodies += n
+
+    def tick(self):
+        """One step of every resident world; retires the finished ones."""
+        if not self.resident:
+            return 0.0
+        t0 = self.clock()
+        self.space.Update(1)
+        self.space.synchronize()
+        self.ticks += 1
+        done = []
+        for r in self.resident.values():
+            r.done += 1
+            self.body_steps += r.n_dyn
+            if r.done >= r.steps:
+                done.append(r)
+        for r in done:
+            r.result = self.space.motion_states_range(r.first, len(r.scene))
+            self.space.remove_world(r.world)
+            r.finished = self.clock()
+            del self.resident[r.rid]
+            self.resident_bodies -= len(r.scene)
+            self.completed.append(r)
+        dt = self.clock() - t0
+        for r in list(self.resident.values()) + done:
+            r.tick_latencies.append(dt)
+        return dt
+
+    def close(self):
+        self.space.close()
+
+
+def run_poisson(server, rate, duration, make_request, seed=0, drain=True, max_wall=None):
+    """Arrivals of a Poisson process (`rate` per second of wall time) for
+    `duration` seconds, served tick by tick.  `make_request(rng)` returns
+    (scene, steps).  Returns a summary dict."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    next_arrival = rng.exponential(1.0 / rate)
+    while True:
+        now = time.perf_counter() - t0
+        while next_arrival <= min(now, duration):
+            scene, steps = make_request(rng)
+            server.submit(scene, steps, arrival=t0 + next_arrival)
+            next_arrival += rng.exponential(1.0 / rate)
+        server.admit()
+        if server.resident:
+            server.tick()
+        if now > duration and (not drain or (not server.resident and not server.queue)):
+            break
+        if max_wall is not None and now > max_wall:
+            break
+    wall = time.perf_counter() - t0
+    step_lat = np.array([x for r in server.completed for x in r.tick_latencies]) if server.completed else np.zeros(1)
+    return {
+        "wall_s": round(wall, 3),
+        "requests_completed": len(server.completed),
+        "requests_waiting": len(server.queue),
+        "ticks": server.ticks,
+        "body_steps": server.body_steps,
+        "value": round(server.body_steps / wall, 1),
+        "step_latency_ms": {"p50": round(1e3 * float(np.percentile(step_lat, 50)), 4),
+                            "p99": round(1e3 * float(np.percentile(step_lat, 99)), 4)},
+        "request_latency_s": {
+            "p50": round(float(np.percentile([r.finished - r.arrival for r in server.completed], 50)), 4)
+            if server.completed else None,
+            "p99": round(float(np.percentile([r.finished - r.arrival for r in server.completed], 99)), 4)
+            if server.completed else None},
+    }

@@ -259,6 +259,31 @@ class Space:
         wid = self.begin_world()
         return wid, self.add_scene(scene)
 
+    def insert_world(self, scene):
+        """Inserts `scene` as a new sub-world between steps (continuous
+        batching); returns (world id, first body id).  The scene array's rows
+        have the layout of bepu_body_desc."""
+        scene = np.ascontiguousarray(scene)
+        assert scene.dtype.itemsize == ctypes.sizeof(_lib.BodyDesc)
+        wid, first = ctypes.c_uint32(), ctypes.c_uint32()
+        ptr = scene.ctypes.data_as(ctypes.POINTER(_lib.BodyDesc))
+        check(self._lib.bepu_insert_world(self._handle(), ptr, len(scene), ctypes.byref(wid), ctypes.byref(first)),
+              "insert_world")
+        self._ms_cache = None
+        return wid.value, first.value
+
+    def remove_world(self, wid):
+        check(self._lib.bepu_remove_world(self._handle(), wid), "remove_world")
+
+    def synchronize(self):
+        check(self._lib.bepu_synchronize(self._handle()), "synchronize")
+
+    def motion_states_range(self, first, count):
+        """(count, 13) motion states of bodies [first, first + count)."""
+        out = (_lib.MotionState * max(count, 1))()
+        check(self._lib.bepu_get_motion_states_range(self._handle(), first, count, out), "motion_states_range")
+        return np.frombuffer(out, dtype=np.float32, count=13 * count).reshape(count, 13).copy()
+
     def world_range(self, wid):
         """(first body id, body count) of a sub-world."""
         first, count = ctypes.c_uint32(), ctypes.c_uint32()

@@ -191,6 +191,33 @@ bepu_status bepu_begin_world(bepu_world world, uint32_t* world_id_out);
 bepu_status bepu_get_world_range(bepu_world world, uint32_t world_id, uint32_t* first_body_out,
                                  uint32_t* count_out);
 
+/* Continuous batching.  One body of a world inserted in a single call. */
+typedef struct bepu_body_desc {
+    int32_t shape;            /* BEPU_SHAPE_BOX or BEPU_SHAPE_SPHERE */
+    float dims[3];            /* box: width, height, length; sphere: radius, 0, 0 */
+    float mass;               /* <= 0: kinematic */
+    bepu_motion_state state;
+    bepu_material material;
+    uint32_t flags;           /* BEPU_FLAG_* */
+} bepu_body_desc;
+
+/* Inserts a whole new sub-world at any time between steps: its bodies take
+ * a free contiguous id range (one left by a removed world if large enough,
+ * else the end of the arrays) and join the next step.  A body's first step
+ * never picks up state left by a previous occupant of its id.  Returns the
+ * world id and its first body id.  bepu_remove_world retires a sub-world:
+ * its bodies leave every stage from the next step on and its id range and
+ * world id are reused.  bepu_get_motion_states_range reads one world's
+ * bodies.  No reference counterpart (each request would be its own Space). */
+bepu_status bepu_insert_world(bepu_world world, const bepu_body_desc* bodies, uint32_t count,
+                              uint32_t* world_id_out, uint32_t* first_body_out);
+bepu_status bepu_remove_world(bepu_world world, uint32_t world_id);
+bepu_status bepu_get_motion_states_range(bepu_world world, uint32_t first_body, uint32_t count,
+                                         bepu_motion_state* out);
+
+/* Waits until every queued step of the world has finished on the device. */
+bepu_status bepu_synchronize(bepu_world world);
+
 /* Space.Update() x n_steps (Space.cs:527-531). */
 bepu_status bepu_step(bepu_world world, int32_t n_steps);
 
