@@ -117,6 +117,13 @@ EXPORTS = {
                                          ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "bepu_remove_world": (ctypes.c_int, [_vp, ctypes.c_uint32]),
     "bepu_synchronize": (ctypes.c_int, [_vp]),
+    "bepu_insert_worlds": (ctypes.c_int, [_vp, ctypes.POINTER(BodyDesc), ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                                          ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "bepu_remove_worlds": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]),
+    "bepu_get_motion_states_ranges": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32),
+                                                     ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+                                                     ctypes.POINTER(MotionState)]),
     "bepu_get_motion_states_range": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32,
                                                     ctypes.POINTER(MotionState)]),
     "bepu_get_world_range": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),

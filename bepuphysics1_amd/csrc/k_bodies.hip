@@ -243,6 +243,49 @@ void launch_step_reset(const DeviceWorld& w, hipStream_t st) {
     hipLaunchKernelGGL(k_step_reset, dim3((n + 255) / 256), dim3(256), 0, st, w);
 }
 
+__global__ void __launch_bounds__(256) k_scatter_bodies(DeviceWorld w, const BodyRecord* recs, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BodyRecord& r = recs[i];
+    uint32_t b = r.target;
+    w.pos[b] = r.pos;
+    w.quat[b] = r.quat;
+    w.linv[b] = r.linv;
+    w.angv[b] = r.angv;
+    w.ii0[b] = r.ii0;
+    w.ii1[b] = r.ii1;
+    w.ii2[b] = r.ii2;
+    w.li0[b] = r.li0;
+    w.li1[b] = r.li1;
+    w.li2[b] = r.li2;
+    w.shape[b] = r.shape;
+    w.stab[b] = r.stab;
+    w.mat[b] = r.mat;
+    w.damp[b] = r.damp;
+    w.bflags[b] = r.flags;
+    w.wid[b] = r.wid;
+    w.wbase[b] = r.wbase;
+    w.born[b] = r.born;
+}
+
+__global__ void __launch_bounds__(256) k_gather_states(DeviceWorld w, const uint32_t* ids, uint32_t n, float4* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = ids[i];
+    out[4 * i + 0] = w.pos[b];
+    out[4 * i + 1] = w.quat[b];
+    out[4 * i + 2] = w.linv[b];
+    out[4 * i + 3] = w.angv[b];
+}
+
+void launch_scatter_bodies(const DeviceWorld& w, const BodyRecord* recs, uint32_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_scatter_bodies, dim3((n + 255) / 256), dim3(256), 0, st, w, recs, n);
+}
+
+void launch_gather_states(const DeviceWorld& w, const uint32_t* ids, uint32_t n, float4* out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_gather_states, dim3((n + 255) / 256), dim3(256), 0, st, w, ids, n, out);
+}
+
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st) {
     if (n == 0) return;
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);

@@ -25,13 +25,7 @@ namespace {
 enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE, ST_COUNT };
 const char* kStageNames = "body_pre,broadphase,narrowphase,islands,island_solve,global_solve,integrate";
 
-struct HostBody {
-    float4 pos, quat, linv, angv, ii0, ii1, ii2, li0, li1, li2, shape, stab, mat;
-    float2 damp;
-    uint32_t flags;
-    uint32_t wid, wbase;   // sub-world and its first body id
-    uint32_t born;         // index of the first step the body takes part in
-};
+using HostBody = BodyRecord;
 
 template <typename T>
 bepu_status dalloc(T** p, size_t n) {
@@ -69,6 +63,11 @@ struct bepu_world_s {
     uint32_t cur_world = 0;                  // world the add calls append to
     std::vector<uint32_t> free_world_ids;
     std::map<uint32_t, uint32_t> free_ranges;  // body ranges of removed worlds: first -> count
+    BodyRecord* stage = nullptr;            // device staging of body records
+    uint32_t stage_cap = 0;
+    uint32_t* id_stage = nullptr;           // device staging of body ids for readbacks
+    float4* state_stage = nullptr;          // 4 float4 per body read back
+    uint32_t rb_cap = 0;
 };
 
 static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
@@ -127,6 +126,9 @@ bepu_status bepu_world_destroy(bepu_world w) {
     if (!w) return BEPU_E_INVALID_ARG;
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     for (void* p : w->allocations) (void)hipFree(p);
+    if (w->stage) (void)hipFree(w->stage);
+    if (w->id_stage) (void)hipFree(w->id_stage);
+    if (w->state_stage) (void)hipFree(w->state_stage);
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -271,34 +273,30 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape,
     return BEPU_OK;
 }
 
-// Copies body records to device ids [o, o + k) (after the work already queued
-// on the world's stream, which may read them).
-static bepu_status upload_range(bepu_world w, uint32_t o, const std::vector<HostBody>& src) {
-    size_t k = src.size();
+// Copies body records to the device in one block and scatters them to their
+// target ids (after the work already queued on the world's stream, which may
+// read those ids).
+static bepu_status upload_records(bepu_world w, std::vector<BodyRecord>& recs) {
+    uint32_t k = (uint32_t)recs.size();
     if (k == 0) return BEPU_OK;
     if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
-    std::vector<float4> tmp(k);
-    std::vector<float2> tmp2(k);
-    std::vector<uint32_t> tmpu(k);
-    DeviceWorld& d = w->d;
-    auto put4 = [&](float4* dst, float4 HostBody::*f) -> bool {
-        for (size_t i = 0; i < k; ++i) tmp[i] = src[i].*f;
-        return hipMemcpy(dst + o, tmp.data(), k * sizeof(float4), hipMemcpyHostToDevice) == hipSuccess;
-    };
-    bool ok = put4(d.pos, &HostBody::pos) && put4(d.quat, &HostBody::quat) && put4(d.linv, &HostBody::linv) &&
-              put4(d.angv, &HostBody::angv) && put4(d.ii0, &HostBody::ii0) && put4(d.ii1, &HostBody::ii1) &&
-              put4(d.ii2, &HostBody::ii2) && put4(d.li0, &HostBody::li0) && put4(d.li1, &HostBody::li1) &&
-              put4(d.li2, &HostBody::li2) && put4(d.shape, &HostBody::shape) && put4(d.stab, &HostBody::stab) &&
-              put4(d.mat, &HostBody::mat);
-    for (size_t i = 0; i < k; ++i) tmp2[i] = src[i].damp;
-    ok = ok && hipMemcpy(d.damp + o, tmp2.data(), k * sizeof(float2), hipMemcpyHostToDevice) == hipSuccess;
-    auto putu = [&](uint32_t* dst, uint32_t HostBody::*f) -> bool {
-        for (size_t i = 0; i < k; ++i) tmpu[i] = src[i].*f;
-        return hipMemcpy(dst + o, tmpu.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice) == hipSuccess;
-    };
-    ok = ok && putu(d.bflags, &HostBody::flags) && putu(d.wid, &HostBody::wid) && putu(d.wbase, &HostBody::wbase) &&
-         putu(d.born, &HostBody::born);
-    return ok ? BEPU_OK : BEPU_E_DEVICE;
+    if (k > w->stage_cap) {
+        uint32_t cap = std::max<uint32_t>(k, 2 * w->stage_cap);
+        BodyRecord* p = nullptr;
+        if (hipMalloc(&p, (size_t)cap * sizeof(BodyRecord)) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
+        if (w->stage) (void)hipFree(w->stage);
+        w->stage = p;
+        w->stage_cap = cap;
+    }
+    if (hipMemcpy(w->stage, recs.data(), (size_t)k * sizeof(BodyRecord), hipMemcpyHostToDevice) != hipSuccess)
+        return BEPU_E_DEVICE;
+    launch_scatter_bodies(w->d, w->stage, k, w->stream);
+    return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+static bepu_status upload_range(bepu_world w, uint32_t o, std::vector<HostBody>& src) {
+    for (size_t i = 0; i < src.size(); ++i) src[i].target = o + (uint32_t)i;
+    return upload_records(w, src);
 }
 
 static bepu_status upload(bepu_world w) {
@@ -363,67 +361,89 @@ bepu_status bepu_get_world_range(bepu_world w, uint32_t world_id, uint32_t* firs
     return BEPU_OK;
 }
 
-bepu_status bepu_insert_world(bepu_world w, const bepu_body_desc* bodies, uint32_t n, uint32_t* world_id,
-                              uint32_t* first_body) {
-    if (!w || !bodies || n == 0) return BEPU_E_INVALID_ARG;
-    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
-    std::vector<HostBody> recs(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const bepu_body_desc& b = bodies[i];
-        bepu_status st;
-        if (b.shape == BEPU_SHAPE_BOX) {
-            if (!(b.dims[0] > 0) || !(b.dims[1] > 0) || !(b.dims[2] > 0)) return BEPU_E_INVALID_ARG;
-            st = make_body(w, &b.state, BEPU_SHAPE_BOX, f3{b.dims[0] * 0.5f, b.dims[1] * 0.5f, b.dims[2] * 0.5f},
-                           box_volume_distribution(b.dims[0], b.dims[1], b.dims[2]), b.mass, &b.material, b.flags,
-                           recs[i]);
-        } else if (b.shape == BEPU_SHAPE_SPHERE) {
-            if (!(b.dims[0] > 0)) return BEPU_E_INVALID_ARG;
-            st = make_body(w, &b.state, BEPU_SHAPE_SPHERE, f3{b.dims[0], 0, 0}, sphere_volume_distribution(b.dims[0]),
-                           b.mass, &b.material, b.flags, recs[i]);
-        } else {
-            return BEPU_E_UNSUPPORTED;
-        }
-        if (st != BEPU_OK) return st;
+static bepu_status body_from_desc(bepu_world w, const bepu_body_desc& b, HostBody& rec) {
+    if (b.shape == BEPU_SHAPE_BOX) {
+        if (!(b.dims[0] > 0) || !(b.dims[1] > 0) || !(b.dims[2] > 0)) return BEPU_E_INVALID_ARG;
+        return make_body(w, &b.state, BEPU_SHAPE_BOX, f3{b.dims[0] * 0.5f, b.dims[1] * 0.5f, b.dims[2] * 0.5f},
+                         box_volume_distribution(b.dims[0], b.dims[1], b.dims[2]), b.mass, &b.material, b.flags, rec);
     }
-    // First fit among the ranges of removed worlds, else the end of the array.
-    uint32_t first = ~0u;
-    for (auto it = w->free_ranges.begin(); it != w->free_ranges.end(); ++it)
-        if (it->second >= n) {
-            first = it->first;
-            uint32_t rest = it->second - n;
-            w->free_ranges.erase(it);
-            if (rest) w->free_ranges[first + n] = rest;
+    if (b.shape == BEPU_SHAPE_SPHERE) {
+        if (!(b.dims[0] > 0)) return BEPU_E_INVALID_ARG;
+        return make_body(w, &b.state, BEPU_SHAPE_SPHERE, f3{b.dims[0], 0, 0}, sphere_volume_distribution(b.dims[0]),
+                         b.mass, &b.material, b.flags, rec);
+    }
+    return BEPU_E_UNSUPPORTED;
+}
+
+bepu_status bepu_insert_worlds(bepu_world w, const bepu_body_desc* bodies, const uint32_t* counts, uint32_t nworlds,
+                               uint32_t* world_ids, uint32_t* firsts, uint32_t* inserted) {
+    if (inserted) *inserted = 0;
+    if (!w || (!bodies && nworlds) || (!counts && nworlds)) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    std::vector<HostBody> recs;
+    uint32_t done = 0;
+    size_t off = 0;
+    bepu_status result = BEPU_OK;
+    for (; done < nworlds; ++done) {
+        uint32_t n = counts[done];
+        if (n == 0) { result = BEPU_E_INVALID_ARG; break; }
+        size_t base = recs.size();
+        recs.resize(base + n);
+        bepu_status st = BEPU_OK;
+        for (uint32_t i = 0; i < n && st == BEPU_OK; ++i) st = body_from_desc(w, bodies[off + i], recs[base + i]);
+        if (st != BEPU_OK) { recs.resize(base); result = st; break; }
+        // First fit among the ranges of removed worlds, else the end of the array.
+        uint32_t first = ~0u;
+        for (auto it = w->free_ranges.begin(); it != w->free_ranges.end(); ++it)
+            if (it->second >= n) {
+                first = it->first;
+                uint32_t rest = it->second - n;
+                w->free_ranges.erase(it);
+                if (rest) w->free_ranges[first + n] = rest;
+                break;
+            }
+        bool append = first == ~0u;
+        if (append) {
+            if ((uint64_t)w->uploaded + n > w->d.body_cap) { recs.resize(base); result = BEPU_E_CAPACITY; break; }
+            first = w->uploaded;
+        }
+        uint32_t id = new_world_id(w);
+        if (id == ~0u) {
+            if (!append) w->free_ranges[first] = n;
+            recs.resize(base);
+            result = BEPU_E_CAPACITY;
             break;
         }
-    bool append = first == ~0u;
-    if (append) {
-        if ((uint64_t)w->uploaded + n > w->d.body_cap) return BEPU_E_CAPACITY;
-        first = w->uploaded;
+        for (uint32_t i = 0; i < n; ++i) {
+            recs[base + i].wid = id;
+            recs[base + i].wbase = first;
+            recs[base + i].target = first + i;
+        }
+        w->world_first[id] = first;
+        w->world_count[id] = n;
+        w->world_live[id] = 1;
+        if (append) {
+            w->uploaded += n;
+            w->cur_world = id;  // later add calls open a new sub-world after this one
+        }
+        if (world_ids) world_ids[done] = id;
+        if (firsts) firsts[done] = first;
+        off += n;
     }
-    uint32_t id = new_world_id(w);
-    if (id == ~0u) {
-        if (!append) w->free_ranges[first] = n;
-        return BEPU_E_CAPACITY;
-    }
-    for (auto& r : recs) {
-        r.wid = id;
-        r.wbase = first;
-    }
-    if (upload_range(w, first, recs) != BEPU_OK) return BEPU_E_DEVICE;
-    w->world_first[id] = first;
-    w->world_count[id] = n;
-    w->world_live[id] = 1;
-    if (append) {
-        w->uploaded += n;
+    if (!recs.empty()) {
         w->d.nbodies = w->uploaded;
+        w->d.nworlds = (uint32_t)w->world_first.size();
         tree_layout(w->d.nbodies, &w->d.tree_levels, w->d.tree_off, w->d.tree_cnt);
-        // Later add calls open a new sub-world after this one.
-        w->cur_world = id;
+        if (upload_records(w, recs) != BEPU_OK) return BEPU_E_DEVICE;
     }
-    w->d.nworlds = (uint32_t)w->world_first.size();
-    if (world_id) *world_id = id;
-    if (first_body) *first_body = first;
-    return BEPU_OK;
+    if (inserted) *inserted = done;
+    return done == nworlds ? BEPU_OK : result;
+}
+
+bepu_status bepu_insert_world(bepu_world w, const bepu_body_desc* bodies, uint32_t n, uint32_t* world_id,
+                              uint32_t* first_body) {
+    if (n == 0) return BEPU_E_INVALID_ARG;
+    return bepu_insert_worlds(w, bodies, &n, 1, world_id, first_body, nullptr);
 }
 
 bepu_status bepu_remove_world(bepu_world w, uint32_t world_id) {
@@ -468,24 +488,60 @@ bepu_status bepu_synchronize(bepu_world w) {
     return check(hipStreamSynchronize(w->stream));
 }
 
-bepu_status bepu_get_motion_states_range(bepu_world w, uint32_t first, uint32_t count, bepu_motion_state* out) {
-    if (!w || (!out && count)) return BEPU_E_INVALID_ARG;
+bepu_status bepu_get_motion_states_ranges(bepu_world w, const uint32_t* firsts, const uint32_t* counts,
+                                          uint32_t nranges, bepu_motion_state* out) {
+    if (!w || (nranges && (!firsts || !counts || !out))) return BEPU_E_INVALID_ARG;
     if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
-    if ((uint64_t)first + count > w->d.nbodies) return BEPU_E_INVALID_ARG;
+    std::vector<uint32_t> ids;
+    for (uint32_t r = 0; r < nranges; ++r) {
+        if ((uint64_t)firsts[r] + counts[r] > w->d.nbodies) return BEPU_E_INVALID_ARG;
+        for (uint32_t i = 0; i < counts[r]; ++i) ids.push_back(firsts[r] + i);
+    }
+    uint32_t k = (uint32_t)ids.size();
     if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
-    std::vector<float4> p(count), q(count), v(count), a(count);
-    bool ok = count == 0 ||
-              (hipMemcpy(p.data(), w->d.pos + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
-               hipMemcpy(q.data(), w->d.quat + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
-               hipMemcpy(v.data(), w->d.linv + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess &&
-               hipMemcpy(a.data(), w->d.angv + first, count * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess);
-    if (!ok) return BEPU_E_DEVICE;
-    for (uint32_t i = 0; i < count; ++i) {
+    if (k == 0) return BEPU_OK;
+    if (k > w->rb_cap) {
+        uint32_t cap = std::max<uint32_t>(k, 2 * w->rb_cap);
+        uint32_t* pi = nullptr;
+        float4* ps = nullptr;
+        if (hipMalloc(&pi, (size_t)cap * sizeof(uint32_t)) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
+        if (hipMalloc(&ps, (size_t)cap * 4 * sizeof(float4)) != hipSuccess) {
+            (void)hipFree(pi);
+            return BEPU_E_OUT_OF_MEMORY;
+        }
+        if (w->id_stage) (void)hipFree(w->id_stage);
+        if (w->state_stage) (void)hipFree(w->state_stage);
+        w->id_stage = pi;
+        w->state_stage = ps;
+        w->rb_cap = cap;
+    }
+    if (hipMemcpy(w->id_stage, ids.data(), (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return BEPU_E_DEVICE;
+    launch_gather_states(w->d, w->id_stage, k, w->state_stage, w->stream);
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    std::vector<float4> st((size_t)k * 4);
+    if (hipMemcpy(st.data(), w->state_stage, st.size() * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
+        return BEPU_E_DEVICE;
+    for (uint32_t i = 0; i < k; ++i) {
+        const float4 &p = st[4 * i], &q = st[4 * i + 1], &v = st[4 * i + 2], &a = st[4 * i + 3];
         bepu_motion_state& m = out[i];
-        m.position[0] = p[i].x; m.position[1] = p[i].y; m.position[2] = p[i].z;
-        m.orientation[0] = q[i].x; m.orientation[1] = q[i].y; m.orientation[2] = q[i].z; m.orientation[3] = q[i].w;
-        m.linear_velocity[0] = v[i].x; m.linear_velocity[1] = v[i].y; m.linear_velocity[2] = v[i].z;
-        m.angular_velocity[0] = a[i].x; m.angular_velocity[1] = a[i].y; m.angular_velocity[2] = a[i].z;
+        m.position[0] = p.x; m.position[1] = p.y; m.position[2] = p.z;
+        m.orientation[0] = q.x; m.orientation[1] = q.y; m.orientation[2] = q.z; m.orientation[3] = q.w;
+        m.linear_velocity[0] = v.x; m.linear_velocity[1] = v.y; m.linear_velocity[2] = v.z;
+        m.angular_velocity[0] = a.x; m.angular_velocity[1] = a.y; m.angular_velocity[2] = a.z;
+    }
+    return BEPU_OK;
+}
+
+bepu_status bepu_get_motion_states_range(bepu_world w, uint32_t first, uint32_t count, bepu_motion_state* out) {
+    return bepu_get_motion_states_ranges(w, &first, &count, 1, out);
+}
+
+bepu_status bepu_remove_worlds(bepu_world w, const uint32_t* world_ids, uint32_t n) {
+    if (!w || (n && !world_ids)) return BEPU_E_INVALID_ARG;
+    for (uint32_t i = 0; i < n; ++i) {
+        bepu_status st = bepu_remove_world(w, world_ids[i]);
+        if (st != BEPU_OK) return st;
     }
     return BEPU_OK;
 }

@@ -50,6 +50,20 @@ struct PairState {
 enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u };
 enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u };
 
+// One body as built on the host (Entity construction), copied to the device
+// in one block and scattered into the structure-of-arrays by k_scatter_bodies.
+struct BodyRecord {
+    float4 pos, quat, linv, angv, ii0, ii1, ii2, li0, li1, li2, shape, stab, mat;
+    float2 damp;
+    uint32_t flags;   // BF_*
+    uint32_t wid;     // sub-world
+    uint32_t wbase;   // its first body id
+    uint32_t born;    // index of the first step the body takes part in
+    uint32_t target;  // body id
+    uint32_t pad;
+};
+static_assert(sizeof(BodyRecord) == 240, "BodyRecord layout");
+
 struct RowConst;
 struct RowVar;
 
@@ -183,5 +197,8 @@ void tree_layout(uint32_t n, uint32_t* levels, uint32_t* off, uint32_t* cnt);
 void launch_fill_u64(uint64_t* p, uint64_t v, size_t n, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_step_reset(const DeviceWorld& w, hipStream_t st);
+// Body records -> structure-of-arrays; motion states of listed bodies -> 4 float4 each.
+void launch_scatter_bodies(const DeviceWorld& w, const BodyRecord* recs, uint32_t n, hipStream_t st);
+void launch_gather_states(const DeviceWorld& w, const uint32_t* ids, uint32_t n, float4* out, hipStream_t st);
 
 }  // namespace bepu

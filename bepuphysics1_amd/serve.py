@@ -62,25 +62,28 @@ class Server:
         return r
 
     def admit(self):
-        """Admits queued requests in arrival order while they fit."""
-        while self.queue:
-            r = self.queue[0]
-            n = len(r.scene)
-            if self.resident_bodies + n > self.capacity:
+        """Admits queued requests in arrival order while they fit (one
+        insertion call for all of them)."""
+        batch, room = [], self.capacity - self.resident_bodies
+        for r in self.queue:
+            if len(r.scene) > room:
                 break
-            try:
-                r.world, r.first = self.space.insert_world(r.scene)
-            except Exception as e:  # capacity exhausted by fragmentation: wait for retirements
-                if getattr(e, "status", None) == 7:
-                    break
-                raise
+            batch.append(r)
+            room -= len(r.scene)
+        if not batch:
+            return
+        placed = self.space.insert_worlds([r.scene for r in batch])
+        now = self.clock()
+        for r, (wid, first) in zip(batch, placed):
             self.queue.popleft()
-            r.admitted = self.clock()
+            r.world, r.first = wid, first
+            r.admitted = now
             self.resident[r.rid] = r
-            self.resident_bodies += n
+            self.resident_bodies += len(r.scene)
 
     def tick(self):
-        """One step of every resident world; retires the finished ones."""
+        """One step of every resident world; retires the finished ones (one
+        readback for all of them)."""
         if not self.resident:
             return 0.0
         t0 = self.clock()
@@ -93,13 +96,16 @@ class Server:
             self.body_steps += r.n_dyn
             if r.done >= r.steps:
                 done.append(r)
-        for r in done:
-            r.result = self.space.motion_states_range(r.first, len(r.scene))
-            self.space.remove_world(r.world)
-            r.finished = self.clock()
-            del self.resident[r.rid]
-            self.resident_bodies -= len(r.scene)
-            self.completed.append(r)
+        if done:
+            states = self.space.motion_states_ranges([r.first for r in done], [len(r.scene) for r in done])
+            self.space.remove_worlds([r.world for r in done])
+            now = self.clock()
+            for r, ms in zip(done, states):
+                r.result = ms
+                r.finished = now
+                del self.resident[r.rid]
+                self.resident_bodies -= len(r.scene)
+                self.completed.append(r)
         dt = self.clock() - t0
         for r in list(self.resident.values()) + done:
             r.tick_latencies.append(dt)

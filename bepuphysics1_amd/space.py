@@ -272,6 +272,41 @@ class Space:
         self._ms_cache = None
         return wid.value, first.value
 
+    def insert_worlds(self, scene_list):
+        """Inserts several scenes as new sub-worlds in one call; returns a list
+        of (world id, first body id) for those that fit (in order)."""
+        if not scene_list:
+            return []
+        packed = np.ascontiguousarray(np.concatenate(scene_list))
+        counts = np.array([len(x) for x in scene_list], np.uint32)
+        k = len(scene_list)
+        ids, firsts, done = np.zeros(k, np.uint32), np.zeros(k, np.uint32), ctypes.c_uint32()
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        st = self._lib.bepu_insert_worlds(self._handle(), packed.ctypes.data_as(ctypes.POINTER(_lib.BodyDesc)),
+                                          counts.ctypes.data_as(u32p), k, ids.ctypes.data_as(u32p),
+                                          firsts.ctypes.data_as(u32p), ctypes.byref(done))
+        if st not in (0, 7):
+            check(st, "insert_worlds")
+        self._ms_cache = None
+        return list(zip(ids[:done.value].tolist(), firsts[:done.value].tolist()))
+
+    def remove_worlds(self, wids):
+        a = np.ascontiguousarray(wids, np.uint32)
+        check(self._lib.bepu_remove_worlds(self._handle(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(a)),
+              "remove_worlds")
+
+    def motion_states_ranges(self, firsts, counts):
+        """Motion states of several body ranges in one readback: a list of (count_i, 13) arrays."""
+        f = np.ascontiguousarray(firsts, np.uint32)
+        c = np.ascontiguousarray(counts, np.uint32)
+        total = int(c.sum())
+        out = (_lib.MotionState * max(total, 1))()
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        check(self._lib.bepu_get_motion_states_ranges(self._handle(), f.ctypes.data_as(u32p), c.ctypes.data_as(u32p),
+                                                      len(f), out), "motion_states_ranges")
+        flat = np.frombuffer(out, dtype=np.float32, count=13 * total).reshape(total, 13).copy()
+        return np.split(flat, np.cumsum(c)[:-1].astype(np.int64)) if len(c) else []
+
     def remove_world(self, wid):
         check(self._lib.bepu_remove_world(self._handle(), wid), "remove_world")
 

@@ -214,6 +214,16 @@ bepu_status bepu_insert_world(bepu_world world, const bepu_body_desc* bodies, ui
 bepu_status bepu_remove_world(bepu_world world, uint32_t world_id);
 bepu_status bepu_get_motion_states_range(bepu_world world, uint32_t first_body, uint32_t count,
                                          bepu_motion_state* out);
+/* The same for many worlds per call (one device copy each way): worlds are
+ * inserted in order until one does not fit; `inserted_out` tells how many
+ * went in (BEPU_E_CAPACITY when not all).  Bodies of all worlds are packed
+ * back to back in `bodies`, counts[i] per world. */
+bepu_status bepu_insert_worlds(bepu_world world, const bepu_body_desc* bodies, const uint32_t* counts,
+                               uint32_t nworlds, uint32_t* world_ids_out, uint32_t* first_bodies_out,
+                               uint32_t* inserted_out);
+bepu_status bepu_remove_worlds(bepu_world world, const uint32_t* world_ids, uint32_t count);
+bepu_status bepu_get_motion_states_ranges(bepu_world world, const uint32_t* first_bodies,
+                                          const uint32_t* counts, uint32_t nranges, bepu_motion_state* out);
 
 /* Waits until every queued step of the world has finished on the device. */
 bepu_status bepu_synchronize(bepu_world world);
