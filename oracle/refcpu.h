@@ -52,4 +52,49 @@ bool sphere_sphere_colliding(float radiusA, float radiusB, const V3& posA, const
 bool box_sphere_colliding(const V3& half, const V3& boxPos, const Q4& boxQ, float radius,
                           const V3& spherePos, float max_contact_distance, ContactData* contact);
 
+// ---------------------------------------------------------------------------
+// General convex path (refcpu_convex.cpp)
+
+// RigidTransform (BEPUutilities/RigidTransform.cs)
+struct RT {
+    V3 pos;
+    Q4 q;
+};
+
+// What the support mapping of a ConvexShape needs: BEPU_SHAPE_* kind, box
+// half extents (margin included, as BoxShape stores them), capsule half
+// length, hull vertices (recentred surface points) and the collision margin.
+struct ConvexShapeRef {
+    int kind;
+    V3 half;
+    float half_length;
+    const V3* verts;
+    int nverts;
+    float margin;
+};
+
+// GeneralConvexPairTester state kept across steps (GeneralConvexPairTester.cs:15-50):
+// the state machine, the boolean GJK's warm-start axis and the deep-contact
+// search direction.  The cached simplex is always the initial one (a Point
+// simplex at the shapes' origins) because simplex caching is off by default.
+enum { CT_SEPARATED = 0, CT_SHALLOW = 1, CT_DEEP = 2 };
+struct ConvexTesterState {
+    int state;
+    V3 separating_axis;
+    V3 local_direction;
+};
+
+V3 support_local(const ConvexShapeRef& s, const V3& d);
+Q4 qconcat(const Q4& a, const Q4& b);
+V3 rt_transform(const V3& p, const RT& t);
+V3 rt_transform_by_inverse(const V3& p, const RT& t);
+RT local_transform(const RT& a, const RT& b);
+bool gjk_intersecting(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* axis);
+bool gjk_closest_points(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* ca, V3* cb);
+bool mpr_contact(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& axis,
+                 ContactData* c);
+// va / vb: the entities' linear velocities (used on entering deep contact from separation).
+bool convex_candidate(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& va,
+                      const V3& vb, ConvexTesterState* st, ContactData* c);
+
 }  // namespace ref

@@ -100,6 +100,18 @@ def lib():
         L.refcpu_add_sphere.argtypes = [vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
                                         ctypes.POINTER(Material), ctypes.c_uint32,
                                         ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_add_capsule.argtypes = [vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, ctypes.POINTER(Material), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_uint32)]
+        fpp = ctypes.POINTER(ctypes.c_float)
+        L.refcpu_add_convex_hull.argtypes = [vp, ctypes.POINTER(MotionState), fpp, ctypes.c_uint32, fpp,
+                                             ctypes.c_float, ctypes.POINTER(Material), ctypes.c_uint32,
+                                             ctypes.POINTER(ctypes.c_uint32)]
+        shape_args = [fpp, fpp, ctypes.c_int, fpp, fpp, fpp, ctypes.c_int, fpp]
+        L.refcpu_gjk_intersecting.argtypes = shape_args + [fpp]
+        L.refcpu_gjk_closest.argtypes = shape_args + [fpp]
+        L.refcpu_mpr_contact.argtypes = shape_args + [fpp, fpp]
+        L.refcpu_convex_candidate.argtypes = shape_args + [fpp, fpp, ctypes.POINTER(ctypes.c_int), fpp, fpp, fpp]
         L.refcpu_step.argtypes = [vp, ctypes.c_int]
         L.refcpu_broadphase_only.argtypes = [vp]
         for name, typ in (("refcpu_get_motion_states", MotionState), ("refcpu_get_manifolds", Manifold),
@@ -152,7 +164,9 @@ class World:
     def set_bruteforce_broadphase(self, on=True):
         lib().refcpu_set_broadphase(self.h, 1 if on else 0)
 
-    def add_body(self, shape, dims, mass, state, material, flags=0):
+    def add_body(self, shape, dims, mass, state, material, flags=0, hull=None):
+        """`hull` = (surface vertices [n,3], volume distribution [3,3]) for a
+        BEPU_SHAPE_CONVEX_HULL body (as bepu_compute_hull returns them)."""
         ms = MotionState()
         ms.position[:] = [float(x) for x in state[0:3]]
         ms.orientation[:] = [float(x) for x in state[3:7]]
@@ -166,16 +180,28 @@ class World:
         elif shape == 1:
             rc = lib().refcpu_add_sphere(self.h, ctypes.byref(ms), float(dims[0]), float(mass),
                                          ctypes.byref(mat), int(flags), ctypes.byref(bid))
+        elif shape == 3:
+            rc = lib().refcpu_add_capsule(self.h, ctypes.byref(ms), float(dims[0]), float(dims[1]), float(mass),
+                                          ctypes.byref(mat), int(flags), ctypes.byref(bid))
+        elif shape == 2:
+            verts = np.ascontiguousarray(hull[0], np.float32)
+            vd = np.ascontiguousarray(hull[1], np.float32)
+            rc = lib().refcpu_add_convex_hull(self.h, ctypes.byref(ms), _fptr(verts), len(verts), _fptr(vd),
+                                              float(mass), ctypes.byref(mat), int(flags), ctypes.byref(bid))
         else:
             raise NotImplementedError("shape %d" % shape)
         if rc != 0:
             raise ValueError("refcpu add failed with status %d" % rc)
         return bid.value
 
-    def add_scene(self, scene):
+    def add_scene(self, scene, hulls=None):
+        """`hulls[k]` = (vertices, volume distribution) of hull shape id k, which
+        a hull body names in dims[0]."""
         for i in range(len(scene)):
             r = scene[i]
-            self.add_body(int(r["shape"]), r["dims"], r["mass"], r["state"], r["material"], int(r["flags"]))
+            shape = int(r["shape"])
+            hull = hulls[int(r["dims"][0])] if shape == 2 else None
+            self.add_body(shape, r["dims"], r["mass"], r["state"], r["material"], int(r["flags"]), hull=hull)
 
     def step(self, n=1):
         lib().refcpu_step(self.h, int(n))
@@ -259,3 +285,76 @@ def box_sphere(half, bp, bq, r, sp):
 
 def mix64(k):
     return int(lib().refcpu_mix64(int(k)))
+
+
+# ---------------------------------------------------------------------------
+# General convex queries on two posed shapes.  A shape is a dict with
+# "kind" (0 box, 1 sphere, 2 hull, 3 capsule), "half" (box half extents,
+# margin included), "half_length" (capsule), "verts" (hull), "margin".
+
+def _shape_arrays(sh):
+    kind = int(sh["kind"])
+    d = np.zeros(5, np.float32)
+    d[0] = kind
+    if kind == 0:
+        d[1:4] = sh["half"]
+    elif kind == 3:
+        d[1] = sh["half_length"]
+    d[4] = sh["margin"]
+    v = np.ascontiguousarray(sh.get("verts", np.zeros((1, 3))), np.float32)
+    return d, v, (len(v) if kind == 2 else 0)
+
+
+def _pose(p):
+    return np.ascontiguousarray(p, np.float32)
+
+
+def gjk_intersecting(sa, pa, sb, pb, axis=(0, 0, 0)):
+    da, va, na = _shape_arrays(sa)
+    db, vb, nb = _shape_arrays(sb)
+    pa, pb = _pose(pa), _pose(pb)
+    ax = np.array(axis, np.float32)
+    hit = lib().refcpu_gjk_intersecting(_fptr(da), _fptr(va), na, _fptr(pa), _fptr(db), _fptr(vb), nb, _fptr(pb),
+                                        _fptr(ax))
+    return bool(hit), ax
+
+
+def gjk_closest(sa, pa, sb, pb):
+    da, va, na = _shape_arrays(sa)
+    db, vb, nb = _shape_arrays(sb)
+    pa, pb = _pose(pa), _pose(pb)
+    out = np.zeros(6, np.float32)
+    hit = lib().refcpu_gjk_closest(_fptr(da), _fptr(va), na, _fptr(pa), _fptr(db), _fptr(vb), nb, _fptr(pb),
+                                   _fptr(out))
+    return bool(hit), out[:3].copy(), out[3:].copy()
+
+
+def mpr_contact(sa, pa, sb, pb, axis):
+    da, va, na = _shape_arrays(sa)
+    db, vb, nb = _shape_arrays(sb)
+    pa, pb = _pose(pa), _pose(pb)
+    ax = np.array(axis, np.float32)
+    out = np.zeros(8, np.float32)
+    hit = lib().refcpu_mpr_contact(_fptr(da), _fptr(va), na, _fptr(pa), _fptr(db), _fptr(vb), nb, _fptr(pb),
+                                   _fptr(ax), _fptr(out))
+    return bool(hit), out
+
+
+class ConvexTester:
+    """GeneralConvexPairTester state carried across calls."""
+
+    def __init__(self):
+        self.state = ctypes.c_int(0)
+        self.sep_axis = np.zeros(3, np.float32)
+        self.local_dir = np.zeros(3, np.float32)
+
+    def candidate(self, sa, pa, sb, pb, va=(0, 0, 0), vb=(0, 0, 0)):
+        da, xa, na = _shape_arrays(sa)
+        db, xb, nb = _shape_arrays(sb)
+        pa, pb = _pose(pa), _pose(pb)
+        va, vb = np.array(va, np.float32), np.array(vb, np.float32)
+        out = np.zeros(8, np.float32)
+        hit = lib().refcpu_convex_candidate(_fptr(da), _fptr(xa), na, _fptr(pa), _fptr(db), _fptr(xb), nb, _fptr(pb),
+                                            _fptr(va), _fptr(vb), ctypes.byref(self.state), _fptr(self.sep_axis),
+                                            _fptr(self.local_dir), _fptr(out))
+        return bool(hit), out

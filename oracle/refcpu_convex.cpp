@@ -1,0 +1,1110 @@
+// refcpu_convex.cpp -- oracle restatement of the general convex pair path
+// (hulls and capsules, and any pair involving them): support mappings, the
+// Minkowski-difference helpers, boolean GJK, closest-point GJK, the MPR
+// contact with its surface casts and depth refinement, and the
+// GeneralConvexPairTester state machine.
+//
+// TEST INFRASTRUCTURE ONLY (see refcpu.h).  Each expression keeps the operand
+// order of the reference so that FP32 results are bit-identical to what the
+// C# code computes on an IEEE binary32 machine.
+#include <cfloat>
+#include <cstring>
+#include <vector>
+#include "../include/bepu_gpu.h"
+#include "refcpu.h"
+
+namespace ref {
+
+namespace {
+
+const float kEpsilon = 1e-7f;     // Toolbox.Epsilon (BEPUutilities/Toolbox.cs:25)
+const float kBigEpsilon = 1e-5f;  // Toolbox.BigEpsilon (Toolbox.cs:20)
+// GJKToolbox.MaximumGJKIterations / HighGJKIterations (GJK/GJKToolbox.cs:17,22)
+const int kMaxGJKIterations = 15;
+const int kHighGJKIterations = 8;
+// PairSimplex.ProgressionEpsilon / DistanceConvergenceEpsilon (GJK/PairSimplex.cs:97,102)
+const float kProgressionEpsilon = 1e-8f;
+const float kDistanceConvergenceEpsilon = 1e-7f;
+// MPRToolbox limits and epsilons (MPRToolbox.cs:24-99)
+const int kInnerIterationLimit = 15;
+const int kOuterIterationLimit = 15;
+const float kSurfaceEpsilon = 1e-7f;
+const float kDepthRefinementEpsilon = 1e-4f;
+const int kMaximumDepthRefinementIterations = 3;
+
+const V3 kZero{0, 0, 0};
+const V3 kUp{0, 1, 0};
+
+enum { S_EMPTY = 0, S_POINT, S_SEGMENT, S_TRIANGLE, S_TETRAHEDRON };  // SimplexState (PairSimplex.cs:12-19)
+
+// ---------------------------------------------------------------------------
+// Support mappings
+
+}  // namespace
+
+// GetLocalExtremePointWithoutMargin of each shape:
+// BoxShape.cs:182-185, SphereShape.cs:95-98, CapsuleShape.cs:146-154, ConvexHullShape.cs:289-306.
+V3 support_local(const ConvexShapeRef& s, const V3& d) {
+    switch (s.kind) {
+    case BEPU_SHAPE_BOX:
+        return V3{signf_int(d.x) * (s.half.x - s.margin), signf_int(d.y) * (s.half.y - s.margin),
+                  signf_int(d.z) * (s.half.z - s.margin)};
+    case BEPU_SHAPE_CAPSULE:
+        if (d.y > 0) return V3{0, s.half_length, 0};
+        if (d.y < 0) return V3{0, -s.half_length, 0};
+        return kZero;
+    case BEPU_SHAPE_CONVEX_HULL: {
+        float mx = dot(s.verts[0], d);
+        int mi = 0;
+        for (int i = 1; i < s.nverts; i++) {
+            float dd = dot(s.verts[i], d);
+            if (dd > mx) {
+                mx = dd;
+                mi = i;
+            }
+        }
+        return s.verts[mi];
+    }
+    default:  // sphere
+        return kZero;
+    }
+}
+
+// Quaternion.Concatenate (BEPUutilities/Quaternion.cs:101-118)
+Q4 qconcat(const Q4& a, const Q4& b) {
+    Q4 r;
+    r.x = a.w * b.x + a.x * b.w + a.z * b.y - a.y * b.z;
+    r.y = a.w * b.y + a.y * b.w + a.x * b.z - a.z * b.x;
+    r.z = a.w * b.z + a.z * b.w + a.y * b.x - a.x * b.y;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    return r;
+}
+
+// RigidTransform.Transform / TransformByInverse (BEPUutilities/RigidTransform.cs:135-156)
+V3 rt_transform(const V3& p, const RT& t) { return add(qtransform(p, t.q), t.pos); }
+V3 rt_transform_by_inverse(const V3& p, const RT& t) { return qtransform(sub(p, t.pos), qconj(t.q)); }
+
+// MinkowskiToolbox.GetLocalTransform (MinkowskiToolbox.cs:19-29): B in A's space.
+RT local_transform(const RT& a, const RT& b) {
+    Q4 ca = qconj(a.q);
+    RT r;
+    r.q = qconcat(b.q, ca);
+    r.pos = qtransform(sub(b.pos, a.pos), ca);
+    return r;
+}
+
+namespace {
+
+// ConvexShape.GetExtremePointWithoutMargin (ConvexShape.cs:71-80)
+V3 extreme_without_margin(const ConvexShapeRef& s, const V3& direction, const RT& t) {
+    V3 d = qtransform(direction, qconj(t.q));
+    V3 p = support_local(s, d);
+    p = qtransform(p, t.q);
+    return add(p, t.pos);
+}
+
+// MinkowskiToolbox.ExpandMinkowskiSum, contribution form (MinkowskiToolbox.cs:133-148)
+V3 expand_contribution(float ma, float mb, const V3& d) {
+    float ls = length_sq(d);
+    if (ls > kEpsilon) return mul(d, (ma + mb) / std::sqrt(ls));
+    return kZero;
+}
+
+// MinkowskiToolbox.GetLocalMinkowskiExtremePoint, first overload (MinkowskiToolbox.cs:38-50)
+V3 mk_extreme(const ConvexShapeRef& a, const ConvexShapeRef& b, const V3& d, const RT& lb) {
+    V3 e = support_local(a, d);
+    V3 v = extreme_without_margin(b, neg(d), lb);
+    e = sub(e, v);
+    v = expand_contribution(a.margin, b.margin, d);
+    return add(e, v);
+}
+
+// MinkowskiToolbox.GetLocalMinkowskiExtremePoint with the A-side point
+// (MinkowskiToolbox.cs:63-80) and ExpandMinkowskiSum's per-shape form (:160-176).
+V3 mk_extreme_a(const ConvexShapeRef& a, const ConvexShapeRef& b, const V3& d, const RT& lb, V3* ea) {
+    V3 pa = support_local(a, d);
+    V3 pb = extreme_without_margin(b, neg(d), lb);
+    float ls = length_sq(d);
+    if (ls > kEpsilon) {
+        ls = 1 / std::sqrt(ls);
+        V3 c = mul(d, a.margin * ls);
+        pa = add(pa, c);
+        c = mul(d, b.margin * ls);
+        pb = sub(pb, c);
+    }
+    *ea = pa;
+    return sub(pa, pb);
+}
+
+// ---------------------------------------------------------------------------
+// SimpleSimplex (GJK/SimpleSimplex.cs), used by the boolean test.
+
+struct SSimplex {
+    V3 A, B, C, D;
+    int state;
+};
+
+// SimpleSimplex.GetPointOnSegmentClosestToOrigin (SimpleSimplex.cs:73-83)
+V3 ss_segment(const SSimplex& s) {
+    V3 seg = sub(s.B, s.A);
+    float dotA = dot(seg, s.A);
+    float V = -dotA / length_sq(seg);
+    return add(mul(seg, V), s.A);
+}
+
+// SimpleSimplex.GetPointOnTriangleClosestToOrigin (SimpleSimplex.cs:108-199)
+V3 ss_triangle(SSimplex& s) {
+    V3 ab = sub(s.B, s.A), ac = sub(s.C, s.A);
+    float d5 = -dot(ab, s.C), d6 = -dot(ac, s.C);
+    if (d6 >= 0.f && d5 <= d6) {
+        s.state = S_POINT;
+        s.A = s.C;
+        return s.A;
+    }
+    float d1 = -dot(ab, s.A), d2 = -dot(ac, s.A);
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.f && d2 > 0.f && d6 < 0.f) {
+        s.state = S_SEGMENT;
+        s.B = s.C;
+        float V = d2 / (d2 - d6);
+        return add(mul(ac, V), s.A);
+    }
+    float d3 = -dot(ab, s.B), d4 = -dot(ac, s.B);
+    float va = d3 * d6 - d5 * d4;
+    float d3d4, d6d5;
+    if (va <= 0.f && (d3d4 = d4 - d3) > 0.f && (d6d5 = d5 - d6) > 0.f) {
+        s.state = S_SEGMENT;
+        V3 B = s.B, C = s.C;
+        s.A = C;
+        float U = d3d4 / (d3d4 + d6d5);
+        return add(mul(sub(C, B), U), B);
+    }
+    float vc = d1 * d4 - d3 * d2;
+    float denom = 1.f / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    V3 p = mul(ab, v);
+    V3 acw = mul(ac, w);
+    p = add(s.A, p);
+    return add(p, acw);
+}
+
+// SimpleSimplex.TryTetrahedronTriangle (SimpleSimplex.cs:254-370)
+bool ss_try_face(const V3& A, const V3& B, const V3& C, const V3& other, SSimplex* out, V3* point) {
+    std::memset(out, 0, sizeof(*out));
+    *point = kZero;
+    V3 ab = sub(B, A), ac = sub(C, A);
+    V3 normal = cross(ab, ac);
+    V3 AD = sub(other, A);
+    float AdotN = dot(A, normal), ADdotN = dot(AD, normal);
+    if (!(AdotN * ADdotN > 0)) return false;
+    float CdotAB = -dot(ab, C), CdotAC = -dot(ac, C);
+    if (CdotAC >= 0.f && CdotAB <= CdotAC) {
+        out->state = S_POINT;
+        out->A = C;
+        *point = C;
+        return true;
+    }
+    float AdotAB = -dot(ab, A), AdotAC = -dot(ac, A);
+    float vb = CdotAB * AdotAC - AdotAB * CdotAC;
+    if (vb <= 0.f && AdotAC > 0.f && CdotAC < 0.f) {
+        out->state = S_SEGMENT;
+        out->A = A;
+        out->B = C;
+        float V = AdotAC / (AdotAC - CdotAC);
+        *point = add(mul(ac, V), A);
+        return true;
+    }
+    float BdotAB = -dot(ab, B), BdotAC = -dot(ac, B);
+    float va = BdotAB * CdotAC - CdotAB * BdotAC;
+    float d3d4, d6d5;
+    if (va <= 0.f && (d3d4 = BdotAC - BdotAB) > 0.f && (d6d5 = CdotAB - CdotAC) > 0.f) {
+        out->state = S_SEGMENT;
+        out->A = B;
+        out->B = C;
+        float V = d3d4 / (d3d4 + d6d5);
+        *point = add(mul(sub(C, B), V), B);
+        return true;
+    }
+    float vc = AdotAB * BdotAC - BdotAB * AdotAC;
+    out->A = A;
+    out->B = B;
+    out->C = C;
+    out->state = S_TRIANGLE;
+    float denom = 1.f / (va + vb + vc);
+    float w = vc * denom, v = vb * denom;
+    V3 p = mul(ab, v);
+    V3 acw = mul(ac, w);
+    p = add(A, p);
+    *point = add(p, acw);
+    return true;
+}
+
+// SimpleSimplex.GetPointOnTetrahedronClosestToOrigin (SimpleSimplex.cs:204-250)
+bool ss_tetrahedron(SSimplex& s, V3* point) {
+    SSimplex best;
+    std::memset(&best, 0, sizeof(best));
+    *point = kZero;
+    float minimum = FLT_MAX;
+    SSimplex cand;
+    V3 cp;
+    float cd;
+    if (ss_try_face(s.A, s.C, s.D, s.B, &cand, &cp)) {
+        *point = cp;
+        best = cand;
+        minimum = length_sq(cp);
+    }
+    if (ss_try_face(s.C, s.B, s.D, s.A, &cand, &cp) && (cd = length_sq(cp)) < minimum) {
+        *point = cp;
+        best = cand;
+        minimum = cd;
+    }
+    if (ss_try_face(s.B, s.A, s.D, s.C, &cand, &cp) && (cd = length_sq(cp)) < minimum) {
+        *point = cp;
+        best = cand;
+        minimum = cd;
+    }
+    if (minimum < FLT_MAX) {
+        s = best;
+        return false;
+    }
+    return true;
+}
+
+// SimpleSimplex.GetPointClosestToOrigin (SimpleSimplex.cs:39-67)
+bool ss_closest(SSimplex& s, V3* point) {
+    switch (s.state) {
+    case S_POINT: *point = s.A; break;
+    case S_SEGMENT: *point = ss_segment(s); break;
+    case S_TRIANGLE: *point = ss_triangle(s); break;
+    case S_TETRAHEDRON: return ss_tetrahedron(s, point);
+    default: *point = kZero; break;
+    }
+    return false;
+}
+
+// SimpleSimplex.AddNewSimplexPoint (SimpleSimplex.cs:376-397)
+void ss_add(SSimplex& s, const V3& p) {
+    switch (s.state) {
+    case S_EMPTY: s.state = S_POINT; s.A = p; break;
+    case S_POINT: s.state = S_SEGMENT; s.B = p; break;
+    case S_SEGMENT: s.state = S_TRIANGLE; s.C = p; break;
+    case S_TRIANGLE: s.state = S_TETRAHEDRON; s.D = p; break;
+    default: break;
+    }
+}
+
+// SimpleSimplex.GetErrorTolerance (SimpleSimplex.cs:403-420)
+float ss_error_tolerance(const SSimplex& s) {
+    switch (s.state) {
+    case S_POINT: return length_sq(s.A);
+    case S_SEGMENT: return maxf(length_sq(s.A), length_sq(s.B));
+    case S_TRIANGLE: return maxf(length_sq(s.A), maxf(length_sq(s.B), length_sq(s.C)));
+    case S_TETRAHEDRON:
+        return maxf(length_sq(s.A), maxf(length_sq(s.B), maxf(length_sq(s.C), length_sq(s.D))));
+    default: return 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PairSimplex (GJK/PairSimplex.cs), used by the closest-point test.
+
+struct PSimplex {
+    V3 sa[4], sb[4];  // SimplexA / SimplexB: A, B, C, D
+    V3 A, B, C, D;
+    int state;
+    float U, V, W;
+    RT lb;
+    float error_tolerance;
+    float prev_distance;
+};
+
+// PairSimplex(ref CachedSimplex, ref RigidTransform) for the cached simplex the
+// pair tester hands in: State = Point with zero local points
+// (GeneralConvexPairTester.cs:329, PairSimplex.cs:163-193).
+void ps_init_point(PSimplex& s, const RT& lb) {
+    std::memset(&s, 0, sizeof(s));
+    s.prev_distance = FLT_MAX;
+    s.error_tolerance = 0;
+    s.lb = lb;
+    s.state = S_POINT;
+    s.sb[0] = qtransform(kZero, lb.q);
+    s.sb[0] = add(s.sb[0], lb.pos);
+    s.A = sub(s.sa[0], s.sb[0]);
+}
+
+// PairSimplex.GetPointOnSegmentClosestToOrigin (PairSimplex.cs:377-420)
+V3 ps_segment(PSimplex& s) {
+    V3 seg = sub(s.B, s.A);
+    float dotA = dot(seg, s.A);
+    if (dotA > 0) {
+        s.state = S_POINT;
+        s.U = 1;
+        return s.A;
+    }
+    float dotB = dot(seg, s.B);
+    if (dotB > 0) {
+        s.U = dotB / length_sq(seg);
+        s.V = 1 - s.U;
+        return add(mul(seg, s.V), s.A);
+    }
+    s.A = s.B;
+    s.sa[0] = s.sa[1];
+    s.sb[0] = s.sb[1];
+    s.state = S_POINT;
+    s.U = 1;
+    return s.A;
+}
+
+// PairSimplex.GetPointOnTriangleClosestToOrigin (PairSimplex.cs:426-565)
+V3 ps_triangle(PSimplex& s) {
+    V3 ab = sub(s.B, s.A), ac = sub(s.C, s.A);
+    float AdotAB = -dot(ab, s.A), AdotAC = -dot(ac, s.A);
+    if (AdotAC <= 0.f && AdotAB <= 0) {
+        s.state = S_POINT;
+        s.U = 1;
+        return s.A;
+    }
+    float BdotAB = -dot(ab, s.B), BdotAC = -dot(ac, s.B);
+    if (BdotAB >= 0.f && BdotAC <= BdotAB) {
+        s.state = S_POINT;
+        s.A = s.B;
+        s.U = 1;
+        s.sa[0] = s.sa[1];
+        s.sb[0] = s.sb[1];
+        return s.B;
+    }
+    float vc = AdotAB * BdotAC - BdotAB * AdotAC;
+    if (vc <= 0 && AdotAB > 0 && BdotAB < 0) {
+        s.state = S_SEGMENT;
+        s.V = AdotAB / (AdotAB - BdotAB);
+        s.U = 1 - s.V;
+        return add(mul(ab, s.V), s.A);
+    }
+    float CdotAB = -dot(ab, s.C), CdotAC = -dot(ac, s.C);
+    if (CdotAC >= 0.f && CdotAB <= CdotAC) {
+        s.state = S_POINT;
+        s.A = s.C;
+        s.sa[0] = s.sa[2];
+        s.sb[0] = s.sb[2];
+        s.U = 1;
+        return s.A;
+    }
+    float vb = CdotAB * AdotAC - AdotAB * CdotAC;
+    if (vb <= 0.f && AdotAC > 0.f && CdotAC < 0.f) {
+        s.state = S_SEGMENT;
+        s.B = s.C;
+        s.sa[1] = s.sa[2];
+        s.sb[1] = s.sb[2];
+        s.V = AdotAC / (AdotAC - CdotAC);
+        s.U = 1 - s.V;
+        return add(mul(ac, s.V), s.A);
+    }
+    float va = BdotAB * CdotAC - CdotAB * BdotAC;
+    float d3d4, d6d5;
+    if (va <= 0.f && (d3d4 = BdotAC - BdotAB) > 0.f && (d6d5 = CdotAB - CdotAC) > 0.f) {
+        s.state = S_SEGMENT;
+        s.A = s.C;
+        s.sa[0] = s.sa[2];
+        s.sb[0] = s.sb[2];
+        s.U = d3d4 / (d3d4 + d6d5);
+        s.V = 1 - s.U;
+        return add(mul(sub(s.C, s.B), s.U), s.B);
+    }
+    float denom = 1.f / (va + vb + vc);
+    s.V = vb * denom;
+    s.W = vc * denom;
+    s.U = 1 - s.V - s.W;
+    V3 p = mul(ab, s.V);
+    V3 acw = mul(ac, s.W);
+    p = add(s.A, p);
+    return add(p, acw);
+}
+
+// PairSimplex.TryTetrahedronTriangle (PairSimplex.cs:655-845)
+bool ps_try_face(const V3& A, const V3& B, const V3& C, const V3& A1, const V3& B1, const V3& C1, const V3& A2,
+                 const V3& B2, const V3& C2, float tol, const V3& other, PSimplex* out, V3* point) {
+    std::memset(out, 0, sizeof(*out));
+    *point = kZero;
+    V3 ab = sub(B, A), ac = sub(C, A);
+    V3 normal = cross(ab, ac);
+    V3 AD = sub(other, A);
+    float AdotN = dot(A, normal), ADdotN = dot(AD, normal);
+    if (!(AdotN * ADdotN >= -kEpsilon * tol)) return false;
+    float AdotAB = -dot(ab, A), AdotAC = -dot(ac, A);
+    if (AdotAC <= 0.f && AdotAB <= 0) {
+        out->state = S_POINT;
+        out->A = A;
+        out->U = 1;
+        out->sa[0] = A1;
+        out->sb[0] = A2;
+        *point = A;
+        return true;
+    }
+    float BdotAB = -dot(ab, B), BdotAC = -dot(ac, B);
+    if (BdotAB >= 0.f && BdotAC <= BdotAB) {
+        out->state = S_POINT;
+        out->A = B;
+        out->U = 1;
+        out->sa[0] = B1;
+        out->sb[0] = B2;
+        *point = B;
+        return true;
+    }
+    float vc = AdotAB * BdotAC - BdotAB * AdotAC;
+    if (vc <= 0 && AdotAB > 0 && BdotAB < 0) {
+        out->state = S_SEGMENT;
+        out->V = AdotAB / (AdotAB - BdotAB);
+        out->U = 1 - out->V;
+        out->A = A;
+        out->B = B;
+        out->sa[0] = A1;
+        out->sb[0] = A2;
+        out->sa[1] = B1;
+        out->sb[1] = B2;
+        *point = add(mul(ab, out->V), A);
+        return true;
+    }
+    float CdotAB = -dot(ab, C), CdotAC = -dot(ac, C);
+    if (CdotAC >= 0.f && CdotAB <= CdotAC) {
+        out->state = S_POINT;
+        out->A = C;
+        out->U = 1;
+        out->sa[0] = C1;
+        out->sb[0] = C2;
+        *point = C;
+        return true;
+    }
+    float vb = CdotAB * AdotAC - AdotAB * CdotAC;
+    if (vb <= 0.f && AdotAC > 0.f && CdotAC < 0.f) {
+        out->state = S_SEGMENT;
+        out->A = A;
+        out->B = C;
+        out->sa[0] = A1;
+        out->sa[1] = C1;
+        out->sb[0] = A2;
+        out->sb[1] = C2;
+        out->V = AdotAC / (AdotAC - CdotAC);
+        out->U = 1 - out->V;
+        *point = add(mul(ac, out->V), A);
+        return true;
+    }
+    float va = BdotAB * CdotAC - CdotAB * BdotAC;
+    float d3d4, d6d5;
+    if (va <= 0.f && (d3d4 = BdotAC - BdotAB) > 0.f && (d6d5 = CdotAB - CdotAC) > 0.f) {
+        out->state = S_SEGMENT;
+        out->A = B;
+        out->B = C;
+        out->sa[0] = B1;
+        out->sa[1] = C1;
+        out->sb[0] = B2;
+        out->sb[1] = C2;
+        out->V = d3d4 / (d3d4 + d6d5);
+        out->U = 1 - out->V;
+        *point = add(mul(sub(C, B), out->V), B);
+        return true;
+    }
+    out->A = A;
+    out->B = B;
+    out->C = C;
+    out->sa[0] = A1;
+    out->sa[1] = B1;
+    out->sa[2] = C1;
+    out->sb[0] = A2;
+    out->sb[1] = B2;
+    out->sb[2] = C2;
+    out->state = S_TRIANGLE;
+    float denom = 1.f / (va + vb + vc);
+    out->W = vc * denom;
+    out->V = vb * denom;
+    out->U = 1 - out->V - out->W;
+    V3 p = mul(ab, out->V);
+    V3 acw = mul(ac, out->W);
+    p = add(A, p);
+    *point = add(p, acw);
+    return true;
+}
+
+// PairSimplex.GetPointOnTetrahedronClosestToOrigin (PairSimplex.cs:572-650)
+bool ps_tetrahedron(PSimplex& s, V3* point) {
+    PSimplex best;
+    std::memset(&best, 0, sizeof(best));
+    *point = kZero;
+    float minimum = FLT_MAX;
+    PSimplex cand;
+    V3 cp;
+    float cd;
+    if (ps_try_face(s.A, s.C, s.D, s.sa[0], s.sa[2], s.sa[3], s.sb[0], s.sb[2], s.sb[3], s.error_tolerance, s.B, &cand,
+                    &cp)) {
+        *point = cp;
+        best = cand;
+        minimum = length_sq(cp);
+    }
+    if (ps_try_face(s.B, s.D, s.C, s.sa[1], s.sa[3], s.sa[2], s.sb[1], s.sb[3], s.sb[2], s.error_tolerance, s.A, &cand,
+                    &cp) &&
+        (cd = length_sq(cp)) < minimum) {
+        *point = cp;
+        best = cand;
+        minimum = cd;
+    }
+    if (ps_try_face(s.A, s.D, s.B, s.sa[0], s.sa[3], s.sa[1], s.sb[0], s.sb[3], s.sb[1], s.error_tolerance, s.C, &cand,
+                    &cp) &&
+        (cd = length_sq(cp)) < minimum) {
+        *point = cp;
+        best = cand;
+        minimum = cd;
+    }
+    if (ps_try_face(s.A, s.B, s.C, s.sa[0], s.sa[1], s.sa[2], s.sb[0], s.sb[1], s.sb[2], s.error_tolerance, s.D, &cand,
+                    &cp) &&
+        (cd = length_sq(cp)) < minimum) {
+        *point = cp;
+        best = cand;
+        minimum = cd;
+    }
+    if (minimum < FLT_MAX) {
+        best.lb = s.lb;
+        best.prev_distance = s.prev_distance;
+        best.error_tolerance = s.error_tolerance;
+        s = best;
+        return false;
+    }
+    return true;
+}
+
+// PairSimplex.GetPointClosestToOrigin (PairSimplex.cs:341-370)
+bool ps_closest(PSimplex& s, V3* point) {
+    switch (s.state) {
+    case S_POINT:
+        *point = s.A;
+        s.U = 1;
+        break;
+    case S_SEGMENT: *point = ps_segment(s); break;
+    case S_TRIANGLE: *point = ps_triangle(s); break;
+    case S_TETRAHEDRON: return ps_tetrahedron(s, point);
+    default: *point = kZero; break;
+    }
+    return false;
+}
+
+// PairSimplex.GetNewSimplexPoint (PairSimplex.cs:861-920)
+bool ps_new_point(PSimplex& s, const ConvexShapeRef& a, const ConvexShapeRef& b, int iteration, const V3& cp) {
+    V3 nd = neg(cp);
+    V3 pa = support_local(a, nd);
+    V3 pb = extreme_without_margin(b, cp, s.lb);
+    V3 S = sub(pa, pb);
+    float dotS = dot(S, nd);
+    float dist = length_sq(cp);
+    float progression = dotS + dist;
+    if (iteration > kHighGJKIterations && dist - s.prev_distance < kDistanceConvergenceEpsilon * s.error_tolerance)
+        return true;
+    if (dist < s.prev_distance) s.prev_distance = dist;
+    switch (s.state) {
+    case S_POINT:
+        s.error_tolerance = maxf(length_sq(s.A), length_sq(S));
+        if (progression <= s.error_tolerance * kProgressionEpsilon) return true;
+        s.state = S_SEGMENT;
+        s.B = S;
+        s.sa[1] = pa;
+        s.sb[1] = pb;
+        return false;
+    case S_SEGMENT:
+        s.error_tolerance = maxf(maxf(length_sq(s.A), length_sq(s.B)), length_sq(S));
+        if (progression <= s.error_tolerance * kProgressionEpsilon) return true;
+        s.state = S_TRIANGLE;
+        s.C = S;
+        s.sa[2] = pa;
+        s.sb[2] = pb;
+        return false;
+    case S_TRIANGLE:
+        s.error_tolerance = maxf(maxf(length_sq(s.A), length_sq(s.B)), maxf(length_sq(s.C), length_sq(S)));
+        if (progression <= s.error_tolerance * kProgressionEpsilon) return true;
+        s.state = S_TETRAHEDRON;
+        s.D = S;
+        s.sa[3] = pa;
+        s.sb[3] = pb;
+        return false;
+    default: return false;
+    }
+}
+
+// PairSimplex.GetClosestPoints (PairSimplex.cs:925-965)
+void ps_closest_points(const PSimplex& s, V3* ca, V3* cb) {
+    switch (s.state) {
+    case S_POINT:
+        *ca = s.sa[0];
+        *cb = s.sb[0];
+        return;
+    case S_SEGMENT:
+        *ca = add(mul(s.sa[0], s.U), mul(s.sa[1], s.V));
+        *cb = add(mul(s.sb[0], s.U), mul(s.sb[1], s.V));
+        return;
+    case S_TRIANGLE: {
+        V3 p = add(mul(s.sa[0], s.U), mul(s.sa[1], s.V));
+        *ca = add(p, mul(s.sa[2], s.W));
+        p = add(mul(s.sb[0], s.U), mul(s.sb[1], s.V));
+        *cb = add(p, mul(s.sb[2], s.W));
+        return;
+    }
+    default:
+        *ca = kZero;
+        *cb = kZero;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MPR (MPRToolbox.cs)
+
+// MPRToolbox.AreLocalShapesOverlapping (MPRToolbox.cs:413-598, the internal overload at :426), origin ray = B's local position.
+bool mpr_overlapping(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& lb) {
+    const V3& ray = lb.pos;
+    if (length_sq(ray) < kEpsilon) return true;
+    V3 v0 = neg(ray);
+    V3 n = ray;
+    V3 v1 = mk_extreme(a, b, n, lb);
+    n = cross(v1, v0);
+    if (length_sq(n) < kEpsilon) return !(dot(v1, ray) < 0);
+    V3 v2 = mk_extreme(a, b, n, lb);
+    n = cross(sub(v1, v0), sub(v2, v0));
+    V3 v3;
+    int count = 0;
+    while (true) {
+        v3 = mk_extreme(a, b, n, lb);
+        if (count > kOuterIterationLimit) break;
+        count++;
+        if (dot(cross(v1, v3), v0) < 0) {
+            v2 = v3;
+            n = cross(sub(v1, v0), sub(v3, v0));
+            continue;
+        }
+        if (dot(cross(v3, v2), v0) < 0) {
+            v1 = v3;
+            n = cross(sub(v2, v0), sub(v3, v0));
+            continue;
+        }
+        break;
+    }
+    while (true) {
+        n = cross(sub(v3, v2), sub(v1, v2));
+        float d = dot(n, v1);
+        if (d >= 0) return true;
+        V3 v4 = mk_extreme(a, b, n, lb);
+        float d2 = dot(v4, n);
+        if (d2 < 0) return false;
+        if (d2 - d < kSurfaceEpsilon || count > kInnerIterationLimit) return false;
+        count++;
+        V3 t = cross(v4, v0);
+        if (dot(v1, t) >= 0) {
+            if (dot(v2, t) >= 0) v1 = v4;
+            else v3 = v4;
+        } else {
+            if (dot(v3, t) >= 0) v2 = v4;
+            else v1 = v4;
+        }
+    }
+}
+
+// Toolbox.GetBarycentricCoordinates (BEPUutilities/Toolbox.cs:1665-1737)
+void barycentric(const V3& p, const V3& a, const V3& b, const V3& c, float* wa, float* wb, float* wc) {
+    V3 ab = sub(b, a), ac = sub(c, a);
+    V3 n = cross(ab, ac);
+    float x = n.x < 0 ? -n.x : n.x;
+    float y = n.y < 0 ? -n.y : n.y;
+    float z = n.z < 0 ? -n.z : n.z;
+    float nu, nv, den;
+    if (x >= y && x >= z) {
+        nu = (p.y - b.y) * (b.z - c.z) - (b.y - c.y) * (p.z - b.z);
+        nv = (p.y - c.y) * (c.z - a.z) - (c.y - a.y) * (p.z - c.z);
+        den = n.x;
+    } else if (y >= z) {
+        nu = (p.x - b.x) * (b.z - c.z) - (b.x - c.x) * (p.z - b.z);
+        nv = (p.x - c.x) * (c.z - a.z) - (c.x - a.x) * (p.z - c.z);
+        den = -n.y;
+    } else {
+        nu = (p.x - b.x) * (b.y - c.y) - (b.x - c.x) * (p.y - b.y);
+        nv = (p.x - c.x) * (c.y - a.y) - (c.x - a.x) * (p.y - c.y);
+        den = n.z;
+    }
+    // The thresholds are double literals: the comparison happens in double.
+    if ((double)den < -1e-9 || (double)den > 1e-9) {
+        den = 1 / den;
+        *wa = nu * den;
+        *wb = nv * den;
+        *wc = 1 - *wa - *wb;
+    } else {
+        float d1 = distance_sq(p, a), d2 = distance_sq(p, b), d3 = distance_sq(p, c);
+        if (d1 < d2 && d1 < d3) { *wa = 1; *wb = 0; *wc = 0; }
+        else if (d2 < d3) { *wa = 0; *wb = 1; *wc = 0; }
+        else { *wa = 0; *wb = 0; *wc = 1; }
+    }
+}
+
+// MPRToolbox.LocalSurfaceCast, both overloads (MPRToolbox.cs:610-880 without the
+// position, 886-1172 with it).  The overloads differ in how the Minkowski
+// support point is formed (difference plus margin expansion versus expanding
+// each shape's point) and so in their rounding; `want_position` selects.
+void surface_cast(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& lb, const V3& dir, bool want_position,
+                  float* t, V3* normal, V3* position) {
+    V3 v1A{}, v2A{}, v3A{}, v4A{};
+    auto support = [&](const V3& d, V3* ea) {
+        return want_position ? mk_extreme_a(a, b, d, lb, ea) : mk_extreme(a, b, d, lb);
+    };
+    V3 n = dir;
+    V3 v1 = support(n, &v1A);
+    n = cross(dir, v1);
+    if (length_sq(n) < kEpsilon) {
+        float rl = length_sq(dir);
+        if (rl > kEpsilon * .01f) *normal = divide(dir, std::sqrt(rl));
+        else *normal = kZero;
+        float rate = dot(*normal, dir);
+        float distance = dot(*normal, v1);
+        *t = rate > 0 ? distance / rate : 0;
+        if (want_position) *position = v1A;
+        return;
+    }
+    V3 v2 = support(n, &v2A);
+    n = cross(v1, v2);
+    if (dot(n, dir) > 0) {
+        n = neg(n);
+        V3 tmp = v1;
+        v1 = v2;
+        v2 = tmp;
+        tmp = v1A;
+        v1A = v2A;
+        v2A = tmp;
+    }
+    V3 v3;
+    int count = 0;
+    while (true) {
+        v3 = support(n, &v3A);
+        if (count > kOuterIterationLimit) {
+            *t = FLT_MAX;
+            *normal = kUp;
+            if (want_position) *position = kZero;
+            return;
+        }
+        count++;
+        if (dot(cross(v1, v3), dir) < 0) {
+            v2 = v3;
+            v2A = v3A;
+            n = cross(v1, v3);
+            continue;
+        }
+        if (dot(cross(v3, v2), dir) < 0) {
+            v1 = v3;
+            v1A = v3A;
+            n = cross(v2, v3);
+            continue;
+        }
+        break;
+    }
+    count = 0;
+    while (true) {
+        n = cross(sub(v1, v2), sub(v3, v2));
+        V3 v4 = support(n, &v4A);
+        float d = dot(n, v1);
+        float sd = dot(v4, n);
+        if (sd - d < kSurfaceEpsilon || count > kInnerIterationLimit) {
+            float ls = length_sq(n);
+            if (ls > kEpsilon * .01f) {
+                *normal = divide(n, std::sqrt(ls));
+                d = dot(*normal, dir);
+                sd = dot(*normal, v1);
+                *t = d > 0 ? sd / d : 0;
+            } else {
+                *normal = kUp;
+                *t = 0;
+            }
+            if (want_position) {
+                float w1, w2, w3;
+                V3 p = mul(dir, *t);
+                barycentric(p, v1, v2, v3, &w1, &w2, &w3);
+                p = mul(v1A, w1);
+                p = add(mul(v2A, w2), p);
+                p = add(mul(v3A, w3), p);
+                *position = p;
+            }
+            return;
+        }
+        V3 tt = cross(v4, dir);
+        if (dot(v1, tt) >= 0) {
+            if (dot(v2, tt) >= 0) {
+                v1 = v4;
+                v1A = v4A;
+            } else {
+                v3 = v4;
+                v3A = v4A;
+            }
+        } else {
+            if (dot(v3, tt) >= 0) {
+                v2 = v4;
+                v2A = v4A;
+            } else {
+                v1 = v4;
+                v1A = v4A;
+            }
+        }
+        count++;
+    }
+}
+
+// MPRToolbox.RefinePenetration (MPRToolbox.cs:1282-1322)
+void refine_penetration(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& lb, float initial_depth,
+                        const V3& initial_normal, float* depth, V3* normal, V3* position) {
+    int count = 0;
+    *normal = initial_normal;
+    *depth = initial_depth;
+    float cd;
+    V3 cn;
+    while (true) {
+        surface_cast(a, b, lb, *normal, true, &cd, &cn, position);
+        if (*depth - cd <= kDepthRefinementEpsilon || ++count >= kMaximumDepthRefinementIterations) {
+            *depth = cd;
+            break;
+        }
+        *depth = cd;
+        *normal = cn;
+    }
+}
+
+}  // namespace
+
+// GJKToolbox.AreShapesIntersecting with a warm-started axis (GJKToolbox.cs:47-90)
+bool gjk_intersecting(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* axis) {
+    RT lb = local_transform(ta, tb);
+    SSimplex s;
+    std::memset(&s, 0, sizeof(s));
+    s.state = S_EMPTY;
+    ss_add(s, mk_extreme(a, b, *axis, lb));
+    V3 cp;
+    int count = 0;
+    while (count++ < kMaxGJKIterations) {
+        if (ss_closest(s, &cp) || length_sq(cp) <= ss_error_tolerance(s) * kBigEpsilon) return true;
+        V3 d = neg(cp);
+        V3 e = mk_extreme(a, b, d, lb);
+        if (dot(e, cp) > 0) {
+            *axis = d;
+            return false;
+        }
+        ss_add(s, e);
+    }
+    return false;
+}
+
+// GJKToolbox.GetClosestPoints with the pair tester's cached simplex (GJKToolbox.cs:131-182):
+// closest points of the shapes' cores, returned in world space.
+bool gjk_closest_points(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* ca, V3* cb) {
+    RT lb = local_transform(ta, tb);
+    PSimplex s;
+    ps_init_point(s, lb);
+    V3 cp;
+    int count = 0;
+    bool intersecting = false;
+    while (true) {
+        if (ps_closest(s, &cp) || length_sq(cp) <= kEpsilon * s.error_tolerance) {
+            *ca = kZero;
+            *cb = kZero;
+            intersecting = true;
+            break;
+        }
+        if (++count > kMaxGJKIterations) break;
+        if (ps_new_point(s, a, b, count, cp)) break;
+    }
+    if (!intersecting) ps_closest_points(s, ca, cb);
+    *ca = rt_transform(*ca, ta);
+    *cb = rt_transform(*cb, ta);
+    return intersecting;
+}
+
+// MPRToolbox.GetContact (MPRToolbox.cs:1201-1280)
+bool mpr_contact(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& axis,
+                 ContactData* c) {
+    RT lb = local_transform(ta, tb);
+    if (!mpr_overlapping(a, b, lb)) {
+        std::memset(c, 0, sizeof(*c));
+        return false;
+    }
+    float depth;
+    V3 normal, dir;
+    float ls = length_sq(axis);
+    if (ls > kEpsilon) {
+        dir = divide(axis, std::sqrt(ls));
+        surface_cast(a, b, lb, dir, false, &depth, &normal, nullptr);
+    } else {
+        depth = FLT_MAX;
+        normal = kUp;
+    }
+    ls = length_sq(lb.pos);
+    if (ls > kEpsilon) {
+        dir = divide(lb.pos, std::sqrt(ls));
+        float dc;
+        V3 nc;
+        surface_cast(a, b, lb, dir, false, &dc, &nc, nullptr);
+        if (dc < depth) {
+            normal = nc;
+            depth = dc;
+        }
+    }
+    V3 pos;
+    refine_penetration(a, b, lb, depth, normal, &depth, &normal, &pos);
+    c->id = -1;
+    M3 o = m3_from_quat(ta.q);
+    c->normal = m3_transform(normal, o);
+    c->pos = add(m3_transform(pos, o), ta.pos);
+    c->depth = depth;
+    return true;
+}
+
+namespace {
+
+// GeneralConvexPairTester.DoShallowContact / DoDeepContact (GeneralConvexPairTester.cs:106-318)
+bool deep_contact(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& va,
+                  const V3& vb, int previous, ConvexTesterState* st, ContactData* c) {
+    if (previous == CT_SEPARATED) {
+        // Both collidables belong to entities: the relative linear velocity.
+        st->local_direction = sub(va, vb);
+        if (length_sq(st->local_direction) < kEpsilon) st->local_direction = kUp;
+    }
+    if (mpr_contact(a, b, ta, tb, st->local_direction, c)) {
+        if (c->depth < a.margin + b.margin) st->state = CT_SHALLOW;
+        return true;
+    }
+    st->state = CT_SEPARATED;
+    return false;
+}
+
+bool shallow_contact(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& va,
+                     const V3& vb, int previous, ConvexTesterState* st, ContactData* c) {
+    V3 ca, cb;
+    gjk_closest_points(a, b, ta, tb, &ca, &cb);
+    V3 disp = sub(cb, ca);
+    float d2 = length_sq(disp);
+    if (d2 < kEpsilon) {
+        st->state = CT_DEEP;
+        return deep_contact(a, b, ta, tb, va, vb, previous, st, c);
+    }
+    st->local_direction = disp;
+    float margin = a.margin + b.margin;
+    if (d2 < margin * margin) {
+        std::memset(c, 0, sizeof(*c));
+        if (margin > kEpsilon) c->pos = mul(disp, a.margin / margin);
+        else c->pos = kZero;
+        c->pos = add(ca, c->pos);
+        float distance = std::sqrt(d2);
+        c->normal = divide(disp, distance);
+        c->depth = margin - distance;
+        return true;
+    }
+    st->state = CT_SEPARATED;
+    std::memset(c, 0, sizeof(*c));
+    return false;
+}
+
+}  // namespace
+
+// GeneralConvexPairTester.GenerateContactCandidate (GeneralConvexPairTester.cs:60-104)
+bool convex_candidate(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& va,
+                      const V3& vb, ConvexTesterState* st, ContactData* c) {
+    int previous = st->state;
+    switch (st->state) {
+    case CT_SEPARATED:
+        if (gjk_intersecting(a, b, ta, tb, &st->separating_axis)) {
+            st->state = CT_SHALLOW;
+            return shallow_contact(a, b, ta, tb, va, vb, previous, st, c);
+        }
+        std::memset(c, 0, sizeof(*c));
+        return false;
+    case CT_SHALLOW: return shallow_contact(a, b, ta, tb, va, vb, previous, st, c);
+    case CT_DEEP: return deep_contact(a, b, ta, tb, va, vb, previous, st, c);
+    default: std::memset(c, 0, sizeof(*c)); return false;
+    }
+}
+
+}  // namespace ref
+
+// ---------------------------------------------------------------------------
+// C entry points for the tests: single queries on two posed shapes.
+// A shape is {kind, d0, d1, d2, margin}: box half extents (margin included),
+// capsule half length in d0, hull vertices in `verts` (n of them, xyz).
+// A pose is {px, py, pz, qx, qy, qz, qw}.
+
+namespace {
+
+ref::ConvexShapeRef make_shape(const float* s, const float* verts, int n, std::vector<ref::V3>* store) {
+    ref::ConvexShapeRef r;
+    std::memset(&r, 0, sizeof(r));
+    r.kind = (int)s[0];
+    r.half = ref::V3{s[1], s[2], s[3]};
+    r.half_length = s[1];
+    r.margin = s[4];
+    if (r.kind == BEPU_SHAPE_CONVEX_HULL) {
+        store->resize(n);
+        for (int i = 0; i < n; ++i) (*store)[i] = ref::V3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+        r.verts = store->data();
+        r.nverts = n;
+    }
+    return r;
+}
+ref::RT make_pose(const float* p) { return ref::RT{ref::V3{p[0], p[1], p[2]}, ref::Q4{p[3], p[4], p[5], p[6]}}; }
+void put_contact(const ref::ContactData& c, float* out) {
+    out[0] = c.pos.x; out[1] = c.pos.y; out[2] = c.pos.z;
+    out[3] = c.normal.x; out[4] = c.normal.y; out[5] = c.normal.z;
+    out[6] = c.depth;
+    out[7] = (float)c.id;
+}
+
+}  // namespace
+
+extern "C" {
+
+int refcpu_gjk_intersecting(const float* sa, const float* va, int na, const float* pa, const float* sb,
+                            const float* vb, int nb, const float* pb, float* axis_io) {
+    std::vector<ref::V3> ha, hb;
+    ref::V3 axis{axis_io[0], axis_io[1], axis_io[2]};
+    bool r = ref::gjk_intersecting(make_shape(sa, va, na, &ha), make_shape(sb, vb, nb, &hb), make_pose(pa),
+                                   make_pose(pb), &axis);
+    axis_io[0] = axis.x; axis_io[1] = axis.y; axis_io[2] = axis.z;
+    return r ? 1 : 0;
+}
+
+int refcpu_gjk_closest(const float* sa, const float* va, int na, const float* pa, const float* sb, const float* vb,
+                       int nb, const float* pb, float* out6) {
+    std::vector<ref::V3> ha, hb;
+    ref::V3 ca, cb;
+    bool r = ref::gjk_closest_points(make_shape(sa, va, na, &ha), make_shape(sb, vb, nb, &hb), make_pose(pa),
+                                     make_pose(pb), &ca, &cb);
+    out6[0] = ca.x; out6[1] = ca.y; out6[2] = ca.z;
+    out6[3] = cb.x; out6[4] = cb.y; out6[5] = cb.z;
+    return r ? 1 : 0;
+}
+
+int refcpu_mpr_contact(const float* sa, const float* va, int na, const float* pa, const float* sb, const float* vb,
+                       int nb, const float* pb, const float* axis, float* out8) {
+    std::vector<ref::V3> ha, hb;
+    ref::ContactData c;
+    bool r = ref::mpr_contact(make_shape(sa, va, na, &ha), make_shape(sb, vb, nb, &hb), make_pose(pa), make_pose(pb),
+                              ref::V3{axis[0], axis[1], axis[2]}, &c);
+    put_contact(c, out8);
+    return r ? 1 : 0;
+}
+
+// One GenerateContactCandidate call; state_io / sep_axis_io / local_dir_io carry
+// the tester state across calls.
+int refcpu_convex_candidate(const float* sa, const float* va, int na, const float* pa, const float* sb,
+                            const float* vb, int nb, const float* pb, const float* vel_a, const float* vel_b,
+                            int* state_io, float* sep_axis_io, float* local_dir_io, float* out8) {
+    std::vector<ref::V3> ha, hb;
+    ref::ConvexTesterState st;
+    st.state = *state_io;
+    st.separating_axis = ref::V3{sep_axis_io[0], sep_axis_io[1], sep_axis_io[2]};
+    st.local_direction = ref::V3{local_dir_io[0], local_dir_io[1], local_dir_io[2]};
+    ref::ContactData c;
+    bool r = ref::convex_candidate(make_shape(sa, va, na, &ha), make_shape(sb, vb, nb, &hb), make_pose(pa),
+                                   make_pose(pb), ref::V3{vel_a[0], vel_a[1], vel_a[2]},
+                                   ref::V3{vel_b[0], vel_b[1], vel_b[2]}, &st, &c);
+    *state_io = st.state;
+    sep_axis_io[0] = st.separating_axis.x; sep_axis_io[1] = st.separating_axis.y; sep_axis_io[2] = st.separating_axis.z;
+    local_dir_io[0] = st.local_direction.x; local_dir_io[1] = st.local_direction.y; local_dir_io[2] = st.local_direction.z;
+    put_contact(c, out8);
+    return r ? 1 : 0;
+}
+
+}  // extern "C"

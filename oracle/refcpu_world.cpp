@@ -19,6 +19,7 @@
 //    IsAlwaysActive; the per-body stabilization state is kept
 //    (SimulationIslandMember.cs:53-70, EntityBase.cs:927-936).
 #include <algorithm>
+#include <cfloat>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -35,7 +36,10 @@ namespace ref {
 struct Body {
     int shape;       // BEPU_SHAPE_*
     V3 half;         // box half extents
-    float radius;    // sphere radius (= collision margin)
+    float radius;    // sphere / capsule radius (= collision margin)
+    float half_length;  // capsule
+    float margin;    // collision margin (box and hull: the default margin)
+    int hull_off, hull_n;  // hull vertices in refcpu_world::hull_verts
     bool dynamic;
     float mass, inv_mass;
     M3 local_inertia_inv;
@@ -113,6 +117,11 @@ struct Pair {
     Sliding sl;
     Twist tw;
     bool prev_colliding;
+    // GeneralConvexContactManifold: the pair tester and the per-contact
+    // supplement data (local offsets, base depth) indexed by slot.
+    ConvexTesterState gc;
+    V3 off_a[4], off_b[4];
+    float base_depth[4];
     bool group_ais;
     int group_cur, group_zero;
     int color;
@@ -129,6 +138,7 @@ using namespace ref;
 struct refcpu_world {
     bepu_settings s;
     std::vector<Body> bodies;
+    std::vector<V3> hull_verts;
     std::map<uint64_t, Pair> pairs;
     std::vector<uint64_t> overlaps;
     std::vector<uint64_t> order;  // solve order of the last step (pair keys)
@@ -237,6 +247,47 @@ void bounding_box(refcpu_world* w, Body& b) {
         off.z = back.x * o.m13 + back.y * o.m23 + back.z * o.m33;
         b.bmax = add(b.pos, off);
         b.bmin = sub(b.pos, off);
+    } else if (b.shape == BEPU_SHAPE_CAPSULE) {
+        // CapsuleShape.GetBoundingBox (CapsuleShape.cs:95-137); Quaternion.TransformY (Quaternion.cs:586-608)
+        const Q4& r = b.q;
+        float x2 = r.x + r.x, y2 = r.y + r.y, z2 = r.z + r.z;
+        float xx2 = r.x * x2, xy2 = r.x * y2, yz2 = r.y * z2, zz2 = r.z * z2;
+        float wx2 = r.w * x2, wz2 = r.w * z2;
+        float hl = b.half_length, m = b.radius;
+        V3 up{hl * (xy2 - wz2), hl * (1.f - xx2 - zz2), hl * (yz2 + wx2)};
+        V3 mn, mx;
+        if (up.x > 0) { mx.x = up.x + m; mn.x = -up.x - m; } else { mx.x = -up.x + m; mn.x = up.x - m; }
+        if (up.y > 0) { mx.y = up.y + m; mn.y = -up.y - m; } else { mx.y = -up.y + m; mn.y = up.y - m; }
+        if (up.z > 0) { mx.z = up.z + m; mn.z = -up.z - m; } else { mx.z = -up.z + m; mn.z = up.z - m; }
+        b.bmin = add(b.pos, mn);
+        b.bmax = add(b.pos, mx);
+    } else if (b.shape == BEPU_SHAPE_CONVEX_HULL) {
+        // ConvexHullShape.GetBoundingBox (ConvexHullShape.cs:206-286)
+        M3 o = m3_from_quat(b.q);
+        const V3* v = &w->hull_verts[b.hull_off];
+        V3 right{o.m11, o.m21, o.m31}, up{o.m12, o.m22, o.m32}, back{o.m13, o.m23, o.m33};
+        float maxX = dot(v[0], right), minX = maxX;
+        float maxY = dot(v[0], up), minY = maxY;
+        float maxZ = dot(v[0], back), minZ = maxZ;
+        int iminX = 0, imaxX = 0, iminY = 0, imaxY = 0, iminZ = 0, imaxZ = 0;
+        for (int i = 1; i < b.hull_n; ++i) {
+            float d = dot(v[i], right);
+            if (d < minX) { minX = d; iminX = i; } else if (d > maxX) { maxX = d; imaxX = i; }
+            d = dot(v[i], up);
+            if (d < minY) { minY = d; iminY = i; } else if (d > maxY) { maxY = d; imaxY = i; }
+            d = dot(v[i], back);
+            if (d < minZ) { minZ = d; iminZ = i; } else if (d > maxZ) { maxZ = d; imaxZ = i; }
+        }
+        // EntityShape.TransformLocalExtremePoints (EntityShape.cs:58-63)
+        V3 pos{v[imaxX].x * o.m11 + v[imaxX].y * o.m21 + v[imaxX].z * o.m31,
+               v[imaxY].x * o.m12 + v[imaxY].y * o.m22 + v[imaxY].z * o.m32,
+               v[imaxZ].x * o.m13 + v[imaxZ].y * o.m23 + v[imaxZ].z * o.m33};
+        V3 neg_{v[iminX].x * o.m11 + v[iminX].y * o.m21 + v[iminX].z * o.m31,
+                v[iminY].x * o.m12 + v[iminY].y * o.m22 + v[iminY].z * o.m32,
+                v[iminZ].x * o.m13 + v[iminZ].y * o.m23 + v[iminZ].z * o.m33};
+        float m = b.margin;
+        b.bmax = V3{b.pos.x + pos.x + m, b.pos.y + pos.y + m, b.pos.z + pos.z + m};
+        b.bmin = V3{b.pos.x + neg_.x - m, b.pos.y + neg_.y - m, b.pos.z + neg_.z - m};
     } else {
         float r = b.radius;
         b.bmin = V3{b.pos.x - r, b.pos.y - r, b.pos.z - r};
@@ -449,6 +500,139 @@ void update_single_contact(refcpu_world* w, Pair& p) {
         manifold_remove(p, 0);
     }
     p.prev_colliding = colliding;
+}
+
+ConvexShapeRef shape_ref(const refcpu_world* w, const Body& b) {
+    ConvexShapeRef s;
+    std::memset(&s, 0, sizeof(s));
+    s.kind = b.shape;
+    s.half = b.half;
+    s.half_length = b.half_length;
+    s.margin = b.margin;
+    if (b.shape == BEPU_SHAPE_CONVEX_HULL) {
+        s.verts = &w->hull_verts[b.hull_off];
+        s.nverts = b.hull_n;
+    }
+    return s;
+}
+
+// GeneralConvexContactManifold.Add: the contact plus its supplement data
+// (GeneralConvexContactManifold.cs:128-137).
+void manifold_add_general(Pair& p, const ContactData& c, const RT& ta, const RT& tb) {
+    int slot = 0;
+    while (p.used[slot]) ++slot;
+    p.base_depth[slot] = c.depth;
+    p.off_a[slot] = rt_transform_by_inverse(c.pos, ta);
+    p.off_b[slot] = rt_transform_by_inverse(c.pos, tb);
+    manifold_add(p, c);
+}
+
+// ContactReducer.ReduceContacts for four contacts and one candidate
+// (CollisionTests/ContactReducer.cs:197-318).  Returns the list index to
+// remove (or -1) and whether the candidate is added.
+int reduce_contacts(const Pair& p, const ContactData& cand, bool* add_candidate) {
+    const ContactObj* c[4];
+    for (int i = 0; i < 4; ++i) c[i] = &p.slot[p.clist[i]];
+    float maxDepth = -FLT_MAX;
+    int deepest = -1;
+    for (int i = 0; i < 4; i++)
+        if (c[i]->depth > maxDepth) { deepest = i; maxDepth = c[i]->depth; }
+    if (cand.depth > maxDepth) deepest = 4;
+    V3 deepPos = deepest < 4 ? c[deepest]->pos : cand.pos;
+    float furthestDist = 0;
+    int furthest = -1;
+    for (int i = 0; i < 4; i++) {
+        float d = distance_sq(c[i]->pos, deepPos);
+        if (d > furthestDist) { furthestDist = d; furthest = i; }
+    }
+    if (distance_sq(cand.pos, deepPos) > furthestDist) furthest = 4;
+    V3 farPos = furthest < 4 ? c[furthest]->pos : cand.pos;
+    V3 xAxis = sub(deepPos, farPos);
+    V3 yAxis = cross(xAxis, c[0]->normal);
+    float minY = FLT_MAX, maxY = -FLT_MAX;
+    int iminY = -1, imaxY = -1;
+    for (int i = 0; i < 4; i++) {
+        float d = dot(c[i]->pos, yAxis);
+        if (d < minY) { iminY = i; minY = d; }
+        if (d > maxY) { imaxY = i; maxY = d; }
+    }
+    float d = dot(cand.pos, yAxis);
+    if (d < minY) iminY = 4;
+    if (d > maxY) imaxY = 4;
+    if (deepest == 4 || furthest == 4 || iminY == 4 || imaxY == 4) {
+        *add_candidate = true;
+        for (int i = 0; i < 4; i++)
+            if (!(i == deepest || i == furthest || i == iminY || i == imaxY)) return i;
+        return -1;
+    }
+    *add_candidate = false;
+    return -1;
+}
+
+// GeneralConvexContactManifold.Update (GeneralConvexContactManifold.cs:71-126) with
+// ContactRefresher.ContactRefresh (CollisionTests/ContactRefresher.cs:18-69) and
+// IsContactUnique (GeneralConvexContactManifold.cs:145-166).
+void update_general_convex(refcpu_world* w, Pair& p) {
+    const Body& A = w->bodies[p.a];
+    const Body& B = w->bodies[p.b];
+    RT ta{A.pos, A.q}, tb{B.pos, B.q};
+    int rem[4], nrem = 0;
+    for (int k = 0; k < p.ccount; k++) {
+        int slot = p.clist[k];
+        ContactObj& c = p.slot[slot];
+        V3 na = rt_transform(p.off_a[slot], ta);
+        V3 nb = rt_transform(p.off_b[slot], tb);
+        V3 ab = sub(nb, na);
+        float d = dot(ab, c.normal);
+        V3 t = mul(c.normal, d);
+        t = sub(ab, t);
+        d = length_sq(t);
+        if (d > w->s.contact_invalidation_length_sq) {
+            rem[nrem++] = k;
+        } else {
+            d = dot(ab, c.normal);
+            c.depth = p.base_depth[slot] - d;
+            if (c.depth < -w->s.max_contact_distance) rem[nrem++] = k;
+            else c.pos = mul(add(nb, na), .5f);
+        }
+    }
+    for (int i = nrem - 1; i >= 0; i--) manifold_remove(p, rem[i]);
+    ContactData cand;
+    if (convex_candidate(shape_ref(w, A), shape_ref(w, B), ta, tb, A.v, B.v, &p.gc, &cand)) {
+        for (int i = 0; i < p.ccount; ++i) {
+            if (dot(p.slot[p.clist[i]].normal, cand.normal) < 0) {
+                manifold_remove(p, i);
+                break;
+            }
+        }
+        bool unique = true;
+        for (int i = 0; i < p.ccount; i++) {
+            int slot = p.clist[i];
+            ContactObj& c = p.slot[slot];
+            if (distance_sq(c.pos, cand.pos) < w->s.contact_min_separation_sq) {
+                c.normal = cand.normal;
+                c.pos = cand.pos;
+                c.depth = cand.depth;
+                p.base_depth[slot] = cand.depth;
+                p.off_a[slot] = rt_transform_by_inverse(cand.pos, ta);
+                p.off_b[slot] = rt_transform_by_inverse(cand.pos, tb);
+                unique = false;
+                break;
+            }
+        }
+        if (unique) {
+            if (p.ccount == 4) {
+                bool add_cand;
+                int r = reduce_contacts(p, cand, &add_cand);
+                if (r >= 0) manifold_remove(p, r);
+                if (add_cand) manifold_add_general(p, cand, ta, tb);
+            } else {
+                manifold_add_general(p, cand, ta, tb);
+            }
+        }
+    } else {
+        for (int i = p.ccount - 1; i >= 0; i--) manifold_remove(p, i);
+    }
 }
 
 Pair make_pair_state(refcpu_world* w, uint32_t i, uint32_t j) {
@@ -1084,6 +1268,7 @@ void step_once(refcpu_world* w) {
         if (it != w->pairs.end()) p = it->second;
         else p = make_pair_state(w, (uint32_t)(key >> 32), (uint32_t)(key & 0xffffffffu));
         if (p.kind == BEPU_PAIR_BOX_BOX) update_box_box(w, p);
+        else if (p.kind == BEPU_PAIR_CONVEX) update_general_convex(w, p);
         else update_single_contact(w, p);
         next.emplace(key, p);
     }
@@ -1166,6 +1351,7 @@ int refcpu_add_box(refcpu_world* w, const bepu_motion_state* st, float width, fl
     Body b;
     std::memset(&b, 0, sizeof(b));
     b.shape = BEPU_SHAPE_BOX;
+    b.margin = w->s.default_margin;
     // BoxShape(width, height, length) (BoxShape.cs:56-62) and ComputeDescription (:118-138)
     b.half = V3{width * 0.5f, height * 0.5f, length * 0.5f};
     float w2 = width * width, h2 = height * height, l2 = length * length;
@@ -1188,9 +1374,55 @@ int refcpu_add_sphere(refcpu_world* w, const bepu_motion_state* st, float radius
     std::memset(&b, 0, sizeof(b));
     b.shape = BEPU_SHAPE_SPHERE;
     b.radius = radius;
+    b.margin = radius;
     // SphereShape.ComputeDescription (SphereShape.cs:52-67)
     float diag = ((2.0f / 5.0f) * radius * radius);
     M3 vd{diag, 0, 0, 0, diag, 0, 0, 0, diag};
+    init_body(w, &b, st, mass, m, flags, vd);
+    if (id) *id = (uint32_t)w->bodies.size();
+    w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+int refcpu_add_capsule(refcpu_world* w, const bepu_motion_state* st, float length, float radius, float mass,
+                       const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!st || !(length >= 0) || !(radius > 0) || std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
+    if (!finite3(st->position, 3) || !finite3(st->orientation, 4)) return BEPU_E_INVALID_ARG;
+    Body b;
+    std::memset(&b, 0, sizeof(b));
+    b.shape = BEPU_SHAPE_CAPSULE;
+    // CapsuleShape(length, radius) and ComputeDescription (CapsuleShape.cs:40-44, 76-92)
+    b.half_length = length * 0.5f;
+    b.radius = radius;
+    b.margin = radius;
+    float effectiveLength = length + radius / 2;
+    float diag = (.0833333333f * effectiveLength * effectiveLength + .25f * radius * radius);
+    M3 vd{diag, 0, 0, 0, .5f * radius * radius, 0, 0, 0, diag};
+    init_body(w, &b, st, mass, m, flags, vd);
+    if (id) *id = (uint32_t)w->bodies.size();
+    w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+// ConvexHullShape(localSurfaceVertices, description) (ConvexHullShape.cs:114-129): the
+// hull is given as its recentred surface vertices and its volume distribution
+// (row-major 3x3), as ConvexHullShape.ComputeDescription produces them.
+int refcpu_add_convex_hull(refcpu_world* w, const bepu_motion_state* st, const float* verts, uint32_t nverts,
+                           const float* volume_distribution, float mass, const bepu_material* m, uint32_t flags,
+                           uint32_t* id) {
+    if (!st || !verts || nverts == 0 || !volume_distribution || std::isnan(mass) || std::isinf(mass))
+        return BEPU_E_INVALID_ARG;
+    if (!finite3(st->position, 3) || !finite3(st->orientation, 4) || !finite3(verts, 3 * nverts))
+        return BEPU_E_INVALID_ARG;
+    Body b;
+    std::memset(&b, 0, sizeof(b));
+    b.shape = BEPU_SHAPE_CONVEX_HULL;
+    b.margin = w->s.default_margin;
+    b.hull_off = (int)w->hull_verts.size();
+    b.hull_n = (int)nverts;
+    for (uint32_t i = 0; i < nverts; ++i) w->hull_verts.push_back(V3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]});
+    const float* v = volume_distribution;
+    M3 vd{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
     init_body(w, &b, st, mass, m, flags, vd);
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
