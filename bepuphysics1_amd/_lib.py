@@ -102,6 +102,16 @@ EXPORTS = {
                                     _u32p]),
     "bepu_add_sphere": (ctypes.c_int, [_vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
                                        ctypes.POINTER(Material), ctypes.c_uint32, _u32p]),
+    "bepu_add_capsule": (ctypes.c_int, [_vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, ctypes.POINTER(Material), ctypes.c_uint32, _u32p]),
+    "bepu_compute_hull": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, _u32p,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float)]),
+    "bepu_create_hull_shape": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, _u32p,
+                                              ctypes.POINTER(ctypes.c_float)]),
+    "bepu_add_convex_hull": (ctypes.c_int, [_vp, ctypes.POINTER(MotionState), ctypes.c_uint32, ctypes.c_float,
+                                            ctypes.POINTER(Material), ctypes.c_uint32, _u32p]),
     "bepu_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_get_motion_states": (ctypes.c_int, [_vp, ctypes.POINTER(MotionState), ctypes.c_uint32, _u32p]),
     "bepu_get_pairs": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32, _u32p]),

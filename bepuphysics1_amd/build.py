@@ -20,8 +20,8 @@ OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libbepu_gpu.so")
 ARCH = os.environ.get("BEPU_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_bvh.hip", "k_grid.hip", "k_narrowphase.hip", "k_island.hip",
-           "k_solver.hip", "world.cpp"]
+SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_bvh.hip", "k_grid.hip", "k_narrowphase.hip", "k_convex.hip",
+           "k_island.hip", "k_solver.hip", "hull.cpp", "world.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC",

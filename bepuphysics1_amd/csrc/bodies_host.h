@@ -3,7 +3,9 @@
 // Entity/ConvexShape construction computes them in the reference.
 #pragma once
 #include <cmath>
+#include <cstring>
 
+#include "../../include/bepu_gpu.h"
 #include "dmath.h"
 
 namespace bepu {
@@ -58,6 +60,26 @@ inline m3 adaptive_invert(const m3& m) {
     default: break;
     }
     return r;
+}
+
+// CapsuleShape.ComputeDescription (CapsuleShape.cs:76-92): a cylindrical
+// approximation of the capsule's volume distribution.
+inline m3 capsule_volume_distribution(float length, float radius) {
+    float effective = length + radius / 2;
+    float diag = (.0833333333f * effective * effective + .25f * radius * radius);
+    return m3{diag, 0, 0, 0, .5f * radius * radius, 0, 0, 0, diag};
+}
+
+// Device shape records: capsule (radius, half length, 0, kind); hull (margin,
+// first vertex, vertex count as uint bits, kind).
+inline float4 capsule_shape(float length, float radius) {
+    return make_float4(radius, length * 0.5f, 0, (float)BEPU_SHAPE_CAPSULE);
+}
+inline float4 hull_shape(float margin, uint32_t first, uint32_t count) {
+    float f, c;
+    std::memcpy(&f, &first, 4);
+    std::memcpy(&c, &count, 4);
+    return make_float4(margin, f, c, (float)BEPU_SHAPE_CONVEX_HULL);
 }
 
 inline m3 scale3(const m3& m, float s) {

@@ -1,0 +1,206 @@
+// k_convex.hip -- narrowphase of general convex pairs (any pair with a hull
+// or a capsule): GeneralConvexContactManifold.Update
+// (CollisionTests/Manifolds/GeneralConvexContactManifold.cs:71-126) on the
+// pairs the main narrowphase kernel queued.  One thread per pair; the pair
+// tester's iterative GJK / MPR (convex.h) diverges per pair, so it runs in a
+// kernel of its own rather than inflating the closed-form pairs' kernel.
+#include "convex.h"
+#include "manifold.h"
+#include "wave.h"
+
+namespace bepu {
+
+namespace {
+
+__device__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i) {
+    float4 sh = w.shape[i];
+    cvx::Shape s;
+    s.kind = (int)sh.w;
+    s.half = f3{0, 0, 0};
+    s.half_length = 0;
+    s.verts = nullptr;
+    s.nverts = 0;
+    if (s.kind == BEPU_SHAPE_BOX) {
+        s.half = f3_of(sh);
+        s.margin = w.s.default_margin;  // BoxShape's margin: CollisionDetectionSettings.DefaultMargin
+    } else if (s.kind == BEPU_SHAPE_CAPSULE) {
+        s.margin = sh.x;
+        s.half_length = sh.y;
+    } else if (s.kind == BEPU_SHAPE_CONVEX_HULL) {
+        s.margin = sh.x;
+        s.verts = w.hull_verts + __float_as_uint(sh.y);
+        s.nverts = (int)__float_as_uint(sh.z);
+    } else {
+        s.margin = sh.x;  // sphere: the radius
+    }
+    return s;
+}
+
+// ContactReducer.ReduceContacts for four contacts and one candidate
+// (CollisionTests/ContactReducer.cs:197-318): the list index to remove (or -1).
+__device__ int reduce_contacts(const PairState& s, f3 cpos, float cdepth, bool* add) {
+    f3 P[4];
+    float D[4];
+    for (int i = 0; i < 4; ++i) {
+        float4 c = s.cpos[s.clist[i]];
+        P[i] = f3_of(c);
+        D[i] = c.w;
+    }
+    float maxDepth = -FLT_MAX;
+    int deepest = -1;
+    for (int i = 0; i < 4; i++)
+        if (D[i] > maxDepth) {
+            deepest = i;
+            maxDepth = D[i];
+        }
+    if (cdepth > maxDepth) deepest = 4;
+    f3 deepPos = deepest < 4 ? P[deepest] : cpos;
+    float furthestDist = 0;
+    int furthest = -1;
+    for (int i = 0; i < 4; i++) {
+        float d = distsq3(P[i], deepPos);
+        if (d > furthestDist) {
+            furthestDist = d;
+            furthest = i;
+        }
+    }
+    if (distsq3(cpos, deepPos) > furthestDist) furthest = 4;
+    f3 farPos = furthest < 4 ? P[furthest] : cpos;
+    f3 yAxis = cross3(deepPos - farPos, f3_of(s.cnrm[s.clist[0]]));
+    float minY = FLT_MAX, maxY = -FLT_MAX;
+    int iminY = -1, imaxY = -1;
+    for (int i = 0; i < 4; i++) {
+        float d = dot3(P[i], yAxis);
+        if (d < minY) {
+            iminY = i;
+            minY = d;
+        }
+        if (d > maxY) {
+            imaxY = i;
+            maxY = d;
+        }
+    }
+    float d = dot3(cpos, yAxis);
+    if (d < minY) iminY = 4;
+    if (d > maxY) imaxY = 4;
+    if (deepest == 4 || furthest == 4 || iminY == 4 || imaxY == 4) {
+        *add = true;
+        for (int i = 0; i < 4; i++)
+            if (!(i == deepest || i == furthest || i == iminY || i == imaxY)) return i;
+        return -1;
+    }
+    *add = false;
+    return -1;
+}
+
+// GeneralConvexContactManifold.Add: the contact and its supplement data
+// (GeneralConvexContactManifold.cs:128-137).
+__device__ void add_general(PairState& s, ConvexState& e, f3 pos, f3 nrm, float depth, int id, const cvx::RT& ta,
+                            const cvx::RT& tb) {
+    int slot = 0;
+    while (s.flags & (16u << slot)) ++slot;
+    e.offa[slot] = f4_of(cvx::rt_transform_by_inverse(pos, ta), depth);
+    e.offb[slot] = f4_of(cvx::rt_transform_by_inverse(pos, tb), 0.0f);
+    add_contact(s, pos, nrm, depth, id);
+}
+
+// GeneralConvexContactManifold.Update (GeneralConvexContactManifold.cs:71-126) with
+// ContactRefresher.ContactRefresh (CollisionTests/ContactRefresher.cs:18-69) and
+// IsContactUnique (GeneralConvexContactManifold.cs:145-166).
+__device__ void update_general(const DeviceWorld& w, PairState& s, ConvexState& e) {
+    float4 PA = w.pos[s.a], QA = w.quat[s.a], PB = w.pos[s.b], QB = w.quat[s.b];
+    cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
+    cvx::RT tb{f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w}};
+    int rem[4], nrem = 0;
+    for (int k = 0; k < s.ccount; k++) {
+        int slot = s.clist[k];
+        f3 na = cvx::rt_transform(f3_of(e.offa[slot]), ta);
+        f3 nb = cvx::rt_transform(f3_of(e.offb[slot]), tb);
+        f3 ab = nb - na;
+        f3 n = f3_of(s.cnrm[slot]);
+        float d = dot3(ab, n);
+        f3 t = ab - n * d;
+        d = lensq3(t);
+        if (d > w.s.contact_invalidation_length_sq) {
+            rem[nrem++] = k;
+        } else {
+            d = dot3(ab, n);
+            float depth = e.offa[slot].w - d;
+            s.cpos[slot].w = depth;
+            if (depth < -w.s.max_contact_distance) rem[nrem++] = k;
+            else s.cpos[slot] = f4_of((nb + na) * .5f, depth);
+        }
+    }
+    for (int i = nrem - 1; i >= 0; i--) remove_contact(s, rem[i]);
+    cvx::TesterState st{__float_as_int(e.sep.w), f3_of(e.sep), f3_of(e.dir)};
+    f3 pos, nrm;
+    float depth;
+    int id;
+    if (cvx::candidate(shape_of(w, s.a), shape_of(w, s.b), ta, tb, f3_of(w.linv[s.a]), f3_of(w.linv[s.b]), st, &pos,
+                       &nrm, &depth, &id)) {
+        for (int i = 0; i < s.ccount; ++i)
+            if (dot3(f3_of(s.cnrm[s.clist[i]]), nrm) < 0) {
+                remove_contact(s, i);
+                break;
+            }
+        bool unique = true;
+        for (int i = 0; i < s.ccount; i++) {
+            int slot = s.clist[i];
+            if (distsq3(f3_of(s.cpos[slot]), pos) < w.s.contact_min_separation_sq) {
+                s.cnrm[slot] = f4_of(nrm, s.cnrm[slot].w);
+                s.cpos[slot] = f4_of(pos, depth);
+                e.offa[slot] = f4_of(cvx::rt_transform_by_inverse(pos, ta), depth);
+                e.offb[slot] = f4_of(cvx::rt_transform_by_inverse(pos, tb), 0.0f);
+                unique = false;
+                break;
+            }
+        }
+        if (unique) {
+            if (s.ccount == 4) {
+                bool add;
+                int r = reduce_contacts(s, pos, depth, &add);
+                if (r >= 0) remove_contact(s, r);
+                if (add) add_general(s, e, pos, nrm, depth, id, ta, tb);
+            } else {
+                add_general(s, e, pos, nrm, depth, id, ta, tb);
+            }
+        }
+    } else {
+        for (int i = s.ccount - 1; i >= 0; i--) remove_contact(s, i);
+    }
+    e.sep = f4_of(st.separating_axis, __int_as_float(st.state));
+    e.dir = f4_of(st.local_direction, 0.0f);
+}
+
+constexpr int kConvexBlock = 64;
+constexpr uint32_t kConvexMaxGrid = 2048;
+
+}  // namespace
+
+__global__ void __launch_bounds__(kConvexBlock) k_convex(DeviceWorld w) {
+    uint32_t n = w.counters[ctr(CTR_CONVEX)];
+    for (uint32_t i = blockIdx.x * kConvexBlock + threadIdx.x; i < n; i += gridDim.x * kConvexBlock) {
+        uint2 wk = w.conv_work[i];
+        PairState s = w.pairs_cur[wk.x];
+        ConvexState e;
+        if (wk.y != ~0u) {
+            e = w.conv_old[wk.y];
+        } else {
+            e.sep = e.dir = make_float4(0, 0, 0, 0);
+            for (int k = 0; k < 4; ++k) e.offa[k] = e.offb[k] = make_float4(0, 0, 0, 0);
+        }
+        update_general(w, s, e);
+        w.pairs_cur[wk.x] = s;
+        w.conv_cur[wk.x] = e;
+        // A constraint joins the islands of its two bodies when both are dynamic.
+        if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
+    }
+}
+
+void launch_convex(const DeviceWorld& w, hipStream_t st) {
+    if (!w.conv_work || w.pair_cap == 0) return;
+    uint32_t grid = std::min<uint32_t>(kConvexMaxGrid, (w.pair_cap + kConvexBlock - 1) / kConvexBlock);
+    hipLaunchKernelGGL(k_convex, dim3(grid), dim3(kConvexBlock), 0, st, w);
+}
+
+}  // namespace bepu

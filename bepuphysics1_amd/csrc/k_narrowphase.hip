@@ -12,7 +12,7 @@
 //   sphere      SphereTester.AreSpheresColliding (SphereTester.cs:23-56)
 // Manifolds: BoxContactManifold.Update (BoxContactManifold.cs:125-206),
 // SphereContactManifold / BoxSphereContactManifold.Update (…Manifold.cs:55-81).
-#include "world.h"
+#include "manifold.h"
 #include "wave.h"
 
 namespace bepu {
@@ -502,57 +502,6 @@ __device__ bool box_sphere(f3 h, f3 bp, q4 bq, float r, f3 sp, float maxd, f3* p
     return true;
 }
 
-// ---------------------------------------------------------------------------
-// Manifold / contact-constraint bookkeeping on a PairState held in registers.
-
-__device__ __forceinline__ bool convex_kind(int kind) { return kind == BEPU_PAIR_BOX_BOX || kind == BEPU_PAIR_CONVEX; }
-
-// ConvexContactManifoldConstraint / NonConvexContactManifoldConstraint.AddContact
-__device__ void add_contact(PairState& s, f3 pos, f3 nrm, float depth, int id) {
-    int slot = 0;
-    while (s.flags & (16u << slot)) ++slot;
-    s.flags |= 16u << slot;
-    s.cpos[slot] = f4_of(pos, depth);
-    s.cnrm[slot] = f4_of(nrm, __int_as_float(id));
-    s.clist[s.ccount++] = (int8_t)slot;
-    int k = s.npen++;
-    s.pslot[k] = (int8_t)slot;
-    s.penacc[k] = 0;
-    if (convex_kind(s.kind)) {
-        if (!(s.flags & PF_FRICTION_ACTIVE)) {
-            s.flags |= PF_FRICTION_ACTIVE;
-            for (int q = 0; q < 6; ++q) s.sllin[q] = 0;
-        }
-    } else {
-        s.fracc[k] = 0;
-        s.frlin[k] = make_float4(0, 0, 0, 0);
-    }
-}
-
-// ContactManifold.Remove (FastRemoveAt) -> RemoveContact (ordered RemoveAt of the rows)
-__device__ void remove_contact(PairState& s, int index) {
-    int slot = s.clist[index];
-    s.ccount--;
-    if (index < s.ccount) s.clist[index] = s.clist[s.ccount];
-    s.flags &= ~(16u << slot);
-    int idx = -1;
-    for (int i = 0; i < s.npen; ++i)
-        if (s.pslot[i] == slot) { idx = i; break; }
-    if (idx < 0) return;
-    for (int k = idx; k < s.npen - 1; ++k) {
-        s.pslot[k] = s.pslot[k + 1];
-        s.penacc[k] = s.penacc[k + 1];
-        s.fracc[k] = s.fracc[k + 1];
-        s.frlin[k] = s.frlin[k + 1];
-    }
-    s.npen--;
-    if (convex_kind(s.kind) && s.npen == 0) {
-        s.flags &= ~PF_FRICTION_ACTIVE;
-        s.twacc = 0;
-        s.slacc[0] = s.slacc[1] = 0;
-    }
-}
-
 __device__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m, LdsList& tmp) {
     float4 PA = w.pos[s.a], PB = w.pos[s.b], QA = w.quat[s.a], QB = w.quat[s.b];
     float4 SA = w.shape[s.a], SB = w.shape[s.b];
@@ -690,6 +639,22 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     PairState s;
     if (old >= 0) s = w.pairs_old[old];
     else init_pair(w, key, s);
+    if (s.kind == BEPU_PAIR_CONVEX) {
+        // General convex pairs run their iterative GJK / MPR in k_convex, which
+        // keeps this kernel's register budget for the closed-form pairs.
+        w.pairs_cur[t] = s;
+        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
+        // Append (t, old) to the work list: one atomic per wavefront's convex lanes.
+        uint64_t active = __ballot(1);
+        int lane = __lane_id();
+        int leader = __ffsll((unsigned long long)active) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&w.counters[ctr(CTR_CONVEX)], (uint32_t)__popcll(active));
+        base = __shfl(base, leader);
+        w.conv_work[base + (uint32_t)__popcll(active & ((1ull << lane) - 1ull))] =
+            make_uint2(t, old >= 0 ? (uint32_t)old : ~0u);
+        return;
+    }
     if (s.kind == BEPU_PAIR_BOX_BOX) {
         LdsList m{&lds_pd[0][threadIdx.x], &lds_id[0][threadIdx.x], 0};
         LdsList tmp{&lds_pd[1][threadIdx.x], &lds_id[1][threadIdx.x], 0};

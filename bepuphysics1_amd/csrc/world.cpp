@@ -16,6 +16,7 @@
 
 #include "../../include/bepu_gpu.h"
 #include "bodies_host.h"
+#include "hull.h"
 #include "solver_rows.h"
 
 using namespace bepu;
@@ -68,6 +69,15 @@ struct bepu_world_s {
     uint32_t* id_stage = nullptr;           // device staging of body ids for readbacks
     float4* state_stage = nullptr;          // 4 float4 per body read back
     uint32_t rb_cap = 0;
+    // Hull shapes: vertices back to back in hull_host (mirrored in d.hull_verts).
+    struct HullShape {
+        uint32_t first, count;
+        m3 volume_distribution;
+    };
+    std::vector<HullShape> hull_shapes;
+    bool convex_bodies = false;             // a capsule or hull was ever added: run k_convex
+    std::vector<float4> hull_host;
+    uint32_t hull_dev_cap = 0;
 };
 
 static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
@@ -129,6 +139,7 @@ bepu_status bepu_world_destroy(bepu_world w) {
     if (w->stage) (void)hipFree(w->stage);
     if (w->id_stage) (void)hipFree(w->id_stage);
     if (w->state_stage) (void)hipFree(w->state_stage);
+    if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -170,6 +181,8 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     uint32_t P = d.pair_cap;
     A(&d.pair_keys, P); A(&d.counters, (size_t)CTR_COUNT * kCtrStride);
     A(&d.pairs_cur, P); A(&d.pairs_old, P);
+    // General convex pair state: 2 x 160 B + 8 B per pair slot.
+    A(&d.conv_cur, P); A(&d.conv_old, P); A(&d.conv_work, P);
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
     A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
     A(&d.cons, P);
@@ -220,7 +233,7 @@ static bool finite_n(const float* v, int n) {
 // Entity construction: Entity.Initialize(collidable, mass) -> BecomeDynamic /
 // BecomeKinematic (EntityBase.cs:632-645, 867-905), Orientation setter
 // normalisation (EntityBase.cs:68-84).
-static bepu_status make_body(bepu_world w, const bepu_motion_state* s, int shape, f3 dims, const m3& vd, float mass,
+static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 shape, const m3& vd, float mass,
                              const bepu_material* m, uint32_t flags, HostBody& b) {
     if (!w || !s) return BEPU_E_INVALID_ARG;
     if (std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
@@ -248,7 +261,8 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, int shape
     b.li0 = make_float4(li.m11, li.m12, li.m13, 0);
     b.li1 = make_float4(li.m21, li.m22, li.m23, 0);
     b.li2 = make_float4(li.m31, li.m32, li.m33, 0);
-    b.shape = make_float4(dims.x, dims.y, dims.z, (float)shape);
+    b.shape = shape;
+    if (shape.w == (float)BEPU_SHAPE_CAPSULE || shape.w == (float)BEPU_SHAPE_CONVEX_HULL) w->convex_bodies = true;
     b.stab = make_float4(0, 0, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
     b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
@@ -257,13 +271,13 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, int shape
     return BEPU_OK;
 }
 
-static bepu_status add_body(bepu_world w, const bepu_motion_state* s, int shape, f3 dims, const m3& vd, float mass,
+static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 shape, const m3& vd, float mass,
                             const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!w) return BEPU_E_INVALID_ARG;
     uint32_t n = w->uploaded + (uint32_t)w->staged.size();
     if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
     HostBody b;
-    bepu_status st = make_body(w, s, shape, dims, vd, mass, m, flags, b);
+    bepu_status st = make_body(w, s, shape, vd, mass, m, flags, b);
     if (st != BEPU_OK) return st;
     b.wid = w->cur_world;
     b.wbase = w->world_first[w->cur_world];
@@ -330,14 +344,97 @@ bepu_status bepu_add_box(bepu_world w, const bepu_motion_state* s, float width, 
                          float mass, const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!(width > 0) || !(height > 0) || !(length > 0)) return BEPU_E_INVALID_ARG;
     // BoxShape(width, height, length): half extents (BoxShape.cs:56-62)
-    return add_body(w, s, BEPU_SHAPE_BOX, f3{width * 0.5f, height * 0.5f, length * 0.5f},
+    return add_body(w, s, make_float4(width * 0.5f, height * 0.5f, length * 0.5f, (float)BEPU_SHAPE_BOX),
                     box_volume_distribution(width, height, length), mass, m, flags, id);
 }
 
 bepu_status bepu_add_sphere(bepu_world w, const bepu_motion_state* s, float radius, float mass,
                             const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!(radius > 0)) return BEPU_E_INVALID_ARG;
-    return add_body(w, s, BEPU_SHAPE_SPHERE, f3{radius, 0, 0}, sphere_volume_distribution(radius), mass, m, flags, id);
+    return add_body(w, s, make_float4(radius, 0, 0, (float)BEPU_SHAPE_SPHERE), sphere_volume_distribution(radius), mass,
+                    m, flags, id);
+}
+
+bepu_status bepu_add_capsule(bepu_world w, const bepu_motion_state* s, float length, float radius, float mass,
+                             const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!(length >= 0) || !std::isfinite(length) || !(radius > 0)) return BEPU_E_INVALID_ARG;
+    // CapsuleShape(length, radius): half length, the radius as the collision margin (CapsuleShape.cs:40-44)
+    return add_body(w, s, capsule_shape(length, radius), capsule_volume_distribution(length, radius), mass, m, flags,
+                    id);
+}
+
+bepu_status bepu_compute_hull(const float* points, uint32_t npoints, float* vertices, uint32_t capacity,
+                              uint32_t* nvertices, float* volume_distribution, float* center, float* volume) {
+    if (!points || npoints == 0) return BEPU_E_INVALID_ARG;
+    HullDescription h;
+    if (!compute_hull_description(reinterpret_cast<const f3*>(points), npoints, &h)) return BEPU_E_INVALID_ARG;
+    uint32_t n = (uint32_t)h.vertices.size();
+    if (nvertices) *nvertices = n;
+    if (vertices)
+        for (uint32_t i = 0; i < n && i < capacity; ++i) {
+            vertices[3 * i] = h.vertices[i].x;
+            vertices[3 * i + 1] = h.vertices[i].y;
+            vertices[3 * i + 2] = h.vertices[i].z;
+        }
+    if (volume_distribution) {
+        const m3& v = h.volume_distribution;
+        float t[9] = {v.m11, v.m12, v.m13, v.m21, v.m22, v.m23, v.m31, v.m32, v.m33};
+        std::memcpy(volume_distribution, t, sizeof(t));
+    }
+    if (center) {
+        center[0] = h.center.x;
+        center[1] = h.center.y;
+        center[2] = h.center.z;
+    }
+    if (volume) *volume = h.volume;
+    return BEPU_OK;
+}
+
+bepu_status bepu_create_hull_shape(bepu_world w, const float* points, uint32_t npoints, uint32_t* shape_id,
+                                   float* center) {
+    if (!w || !points || npoints == 0) return BEPU_E_INVALID_ARG;
+    HullDescription h;
+    if (!compute_hull_description(reinterpret_cast<const f3*>(points), npoints, &h)) return BEPU_E_INVALID_ARG;
+    bepu_world_s::HullShape hs;
+    hs.first = (uint32_t)w->hull_host.size();
+    hs.count = (uint32_t)h.vertices.size();
+    hs.volume_distribution = h.volume_distribution;
+    for (const f3& v : h.vertices) w->hull_host.push_back(make_float4(v.x, v.y, v.z, 0));
+    // The device copy grows by doubling; queued steps may read it, so wait for them.
+    uint32_t need = (uint32_t)w->hull_host.size();
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    if (need > w->hull_dev_cap) {
+        uint32_t cap = std::max<uint32_t>(need, std::max<uint32_t>(256, 2 * w->hull_dev_cap));
+        float4* p = nullptr;
+        if (hipMalloc(&p, (size_t)cap * sizeof(float4)) != hipSuccess) {
+            w->hull_host.resize(hs.first);
+            return BEPU_E_OUT_OF_MEMORY;
+        }
+        if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
+        w->d.hull_verts = p;
+        w->hull_dev_cap = cap;
+        if (hipMemcpy(p, w->hull_host.data(), (size_t)need * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess)
+            return BEPU_E_DEVICE;
+    } else if (hipMemcpy(w->d.hull_verts + hs.first, w->hull_host.data() + hs.first, (size_t)hs.count * sizeof(float4),
+                         hipMemcpyHostToDevice) != hipSuccess) {
+        return BEPU_E_DEVICE;
+    }
+    w->hull_shapes.push_back(hs);
+    if (shape_id) *shape_id = (uint32_t)w->hull_shapes.size() - 1;
+    if (center) {
+        center[0] = h.center.x;
+        center[1] = h.center.y;
+        center[2] = h.center.z;
+    }
+    return BEPU_OK;
+}
+
+bepu_status bepu_add_convex_hull(bepu_world w, const bepu_motion_state* s, uint32_t shape_id, float mass,
+                                 const bepu_material* m, uint32_t flags, uint32_t* id) {
+    if (!w || shape_id >= w->hull_shapes.size()) return BEPU_E_INVALID_ARG;
+    const auto& hs = w->hull_shapes[shape_id];
+    return add_body(w, s, hull_shape(w->d.s.default_margin, hs.first, hs.count), hs.volume_distribution, mass, m,
+                    flags, id);
 }
 
 bepu_status bepu_begin_world(bepu_world w, uint32_t* world_id) {
@@ -364,12 +461,25 @@ bepu_status bepu_get_world_range(bepu_world w, uint32_t world_id, uint32_t* firs
 static bepu_status body_from_desc(bepu_world w, const bepu_body_desc& b, HostBody& rec) {
     if (b.shape == BEPU_SHAPE_BOX) {
         if (!(b.dims[0] > 0) || !(b.dims[1] > 0) || !(b.dims[2] > 0)) return BEPU_E_INVALID_ARG;
-        return make_body(w, &b.state, BEPU_SHAPE_BOX, f3{b.dims[0] * 0.5f, b.dims[1] * 0.5f, b.dims[2] * 0.5f},
+        return make_body(w, &b.state,
+                         make_float4(b.dims[0] * 0.5f, b.dims[1] * 0.5f, b.dims[2] * 0.5f, (float)BEPU_SHAPE_BOX),
                          box_volume_distribution(b.dims[0], b.dims[1], b.dims[2]), b.mass, &b.material, b.flags, rec);
     }
     if (b.shape == BEPU_SHAPE_SPHERE) {
         if (!(b.dims[0] > 0)) return BEPU_E_INVALID_ARG;
-        return make_body(w, &b.state, BEPU_SHAPE_SPHERE, f3{b.dims[0], 0, 0}, sphere_volume_distribution(b.dims[0]),
+        return make_body(w, &b.state, make_float4(b.dims[0], 0, 0, (float)BEPU_SHAPE_SPHERE),
+                         sphere_volume_distribution(b.dims[0]), b.mass, &b.material, b.flags, rec);
+    }
+    if (b.shape == BEPU_SHAPE_CAPSULE) {
+        if (!(b.dims[0] >= 0) || !std::isfinite(b.dims[0]) || !(b.dims[1] > 0)) return BEPU_E_INVALID_ARG;
+        return make_body(w, &b.state, capsule_shape(b.dims[0], b.dims[1]),
+                         capsule_volume_distribution(b.dims[0], b.dims[1]), b.mass, &b.material, b.flags, rec);
+    }
+    if (b.shape == BEPU_SHAPE_CONVEX_HULL) {
+        float k = b.dims[0];
+        if (!(k >= 0) || k != std::floor(k) || k >= (float)w->hull_shapes.size()) return BEPU_E_INVALID_ARG;
+        const auto& hs = w->hull_shapes[(uint32_t)k];
+        return make_body(w, &b.state, hull_shape(w->d.s.default_margin, hs.first, hs.count), hs.volume_distribution,
                          b.mass, &b.material, b.flags, rec);
     }
     return BEPU_E_UNSUPPORTED;
@@ -585,6 +695,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
         if (e) (void)hipEventRecord(e[2], s);
         launch_narrowphase(d, s);
+        if (w->convex_bodies) launch_convex(d, s);
         if (e) (void)hipEventRecord(e[3], s);
         launch_islands(d, w->scan_temp, w->scan_bytes, s);
         if (e) (void)hipEventRecord(e[4], s);
@@ -596,6 +707,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[7], s);
         // This step's pair states and table become next step's history.
         std::swap(d.pairs_cur, d.pairs_old);
+        std::swap(d.conv_cur, d.conv_old);
         std::swap(d.hash_keys_cur, d.hash_keys_old);
         std::swap(d.hash_vals_cur, d.hash_vals_old);
         w->steps++;

@@ -48,6 +48,18 @@ struct PairState {
 };
 
 enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u };
+
+// State of a general convex pair (a pair with a hull or a capsule) that the
+// PairState does not hold: the GeneralConvexPairTester's state machine
+// (sep.w: state as int bits), its boolean-GJK warm-start axis (sep.xyz) and
+// deep-contact search direction (dir.xyz), and the manifold's supplement
+// data per contact slot (local offsets on A and B; offa.w: base depth).
+struct ConvexState {
+    float4 sep;
+    float4 dir;
+    float4 offa[4];
+    float4 offb[4];
+};
 enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u };
 
 // One body as built on the host (Entity construction), copied to the device
@@ -77,7 +89,9 @@ struct DeviceWorld {
     float4 *pos, *quat, *linv, *angv;
     float4 *ii0, *ii1, *ii2;   // world inverse inertia rows; ii0.w = inverse mass
     float4 *li0, *li1, *li2;   // local inverse inertia rows
-    float4* shape;             // half extents (box) or (radius,0,0) ; w = shape kind
+    float4* shape;             // w = shape kind; box: half extents; sphere: (radius, 0, 0);
+                               // capsule: (radius, half length, 0); hull: (margin, first vertex, vertex count as uint bits)
+    float4* hull_verts;        // vertices of every hull shape, back to back (xyz)
     float4 *bmin, *bmax;
     float4* stab;              // prev velocity, time below, slowing, allow stabilization
     float2* damp;              // linear, angular damping
@@ -112,6 +126,9 @@ struct DeviceWorld {
     uint32_t* counters;        // see CTR_*
     PairState* pairs_cur;      // this step's pair states (index = overlap index)
     PairState* pairs_old;      // last step's
+    ConvexState* conv_cur;     // general convex pairs: this step's extra state (index = overlap index)
+    ConvexState* conv_old;     // last step's
+    uint2* conv_work;          // this step's general convex pairs: (overlap index, last step's index or ~0)
     uint64_t* hash_keys_cur;
     uint32_t* hash_vals_cur;
     uint64_t* hash_keys_old;
@@ -158,7 +175,7 @@ enum {
     CTR_MAXPEN = 4,       // float bits (non-negative), max via atomicMax on uint
     CTR_OVERFLOW = 5,     // pair/hash capacity overflow flag
     CTR_ROUND = 6,        // colouring rounds
-    CTR_FREE7 = 7,
+    CTR_CONVEX = 7,       // general convex pairs queued for k_convex this step
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
     CTR_GCONS = 11,       // constraints on the global path
     CTR_ISL_MEDIUM = 13,
@@ -184,6 +201,8 @@ __device__ __forceinline__ uint64_t local_pair_key(const DeviceWorld& w, uint64_
 void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
 void launch_narrowphase(const DeviceWorld& w, hipStream_t st);
+// General convex pairs queued by the narrowphase (GJK / MPR).
+void launch_convex(const DeviceWorld& w, hipStream_t st);
 // Last step's constraint, contact, colour and penetration statistics into the counters.
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);
 void launch_islands(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);

@@ -3,14 +3,17 @@
 A scene is a numpy structured array, one row per body, in add order:
 
     shape     int32    BEPU_SHAPE_BOX (0) / SPHERE (1) / CONVEX_HULL (2) / CAPSULE (3)
-    dims      f32[3]   box (width, height, length); sphere (radius, 0, 0)
+    dims      f32[3]   box (width, height, length); sphere (radius, 0, 0);
+                       capsule (length, radius, 0); hull (index into hull_library(), 0, 0)
     mass      f32      <= 0 makes a kinematic body (Entity(shape, mass), EntityBase.cs:632-645)
     state     f32[13]  MotionState: position xyz, orientation xyzw, linear v, angular v
     material  f32[3]   static friction, kinetic friction, bounciness
     flags     uint32   BEPU_FLAG_*
 
 The same array feeds the GPU library (through ``Space``) and the oracle, so
-both run on identical inputs.  Scenes follow the reference's demo scenes
+both run on identical inputs.  Hull bodies share the shapes of one seeded
+library (``hull_library()``): register it with ``Space.register_hulls``
+before adding a scene that has hulls.  Scenes follow the reference's demo scenes
 (BEPUphysicsDemos/Demos/PyramidDemo.cs:17-37, BoxBoxTestDemo.cs:18-51) at the
 sizes of the benchmark workloads.
 """
@@ -33,6 +36,40 @@ BODY_DTYPE = np.dtype([
 ])
 
 IDENTITY = (0.0, 0.0, 0.0, 1.0)
+
+HULL_LIBRARY_SIZE = 256
+_HULL_LIBRARY = None
+
+
+def hull_points(rng, n=16, scale=(0.5, 1.5)):
+    """n random points on the unit sphere scaled by a factor uniform in `scale`."""
+    p = rng.normal(size=(n, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    return (p * rng.uniform(*scale)).astype(np.float32)
+
+
+def hull_library():
+    """The seeded hull shapes scenes refer to: 16 random points on the unit
+    sphere scaled by [0.5, 1.5] each (the hulls of the stress workload)."""
+    global _HULL_LIBRARY
+    if _HULL_LIBRARY is None:
+        rng = np.random.default_rng(0x4855)
+        _HULL_LIBRARY = [hull_points(rng) for _ in range(HULL_LIBRARY_SIZE)]
+    return _HULL_LIBRARY
+
+
+def has_hulls(scene):
+    return bool((np.asarray(scene["shape"]) == SHAPE_CONVEX_HULL).any())
+
+
+def _shape_dims(rng, s):
+    if s == SHAPE_BOX:
+        return 2 * rng.uniform(0.25, 1.0, 3)
+    if s == SHAPE_SPHERE:
+        return (rng.uniform(0.25, 1.0), 0.0, 0.0)
+    if s == SHAPE_CAPSULE:
+        return (rng.uniform(0.5, 2.0), rng.uniform(0.2, 0.6), 0.0)
+    return (float(rng.integers(0, HULL_LIBRARY_SIZE)), 0.0, 0.0)
 
 
 def _row(shape, dims, mass, pos, quat=IDENTITY, vel=(0, 0, 0), angvel=(0, 0, 0),
@@ -135,19 +172,21 @@ def random_pile(n, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE), drop=(1.0, 10.0), w
     q = random_quaternions(rng, n)
     for i in range(n):
         s = int(kinds[i])
-        if s == SHAPE_BOX:
-            dims = 2 * rng.uniform(0.25, 1.0, 3)
-        else:
-            dims = (rng.uniform(0.25, 1.0), 0.0, 0.0)
+        dims = _shape_dims(rng, s)
         pos = (rng.uniform(-foot / 2, foot / 2), rng.uniform(*drop), rng.uniform(-foot / 2, foot / 2))
         out[k + i] = _row(s, dims, 1.0, pos, quat=q[i])
     return out
 
 
+STRESS_MIX = (SHAPE_BOX, SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL)
+
+
 def stress(n=32768, edge=None, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE)):
     """Bodies placed uniformly at random in a cube of side `edge` with random
-    orientations, no ground.  Box half extents and sphere radii in [0.25, 1].
-    With the default edge the mean AABB degree is about 4 for this mix."""
+    orientations, no ground, shapes in equal shares of `shapes` (STRESS_MIX:
+    50% boxes, 25% spheres, 25% hulls).  Box half extents and sphere radii in
+    [0.25, 1]; hulls from hull_library().  With the default edge the mean AABB
+    degree is about 4 for the box/sphere mix."""
     rng = np.random.default_rng(seed)
     if edge is None:
         edge = 100.0 * (n / 32768.0) ** (1.0 / 3.0) * 0.74
@@ -158,11 +197,7 @@ def stress(n=32768, edge=None, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE)):
     pos = rng.uniform(0.0, edge, (n, 3)).astype(np.float32)
     for i in range(n):
         s = int(kinds[i])
-        if s == SHAPE_BOX:
-            dims = 2 * rng.uniform(0.25, 1.0, 3)
-        else:
-            dims = (rng.uniform(0.25, 1.0), 0.0, 0.0)
-        out[i] = _row(s, dims, 1.0, pos[i], quat=q[i])
+        out[i] = _row(s, _shape_dims(rng, s), 1.0, pos[i], quat=q[i])
     return out
 
 
@@ -196,21 +231,27 @@ def batched_worlds(n_worlds=4096, seed=0, nmin=8, nmax=256, shapes=(SHAPE_BOX, S
     return [random_pile(int(n), seed=int(s), shapes=shapes, drop=drop) for n, s in zip(sizes, seeds)]
 
 
-def random_pile_fast(n, rng, drop=(1.0, 10.0), sphere_fraction=0.5):
+def random_pile_fast(n, rng, drop=(1.0, 10.0), sphere_fraction=0.5, hull_fraction=0.0):
     """Vectorised random pile (same distribution family as random_pile):
-    boxes with half extents in [0.25, 1] and spheres with radius in
-    [0.25, 1], uniform random orientations, over a sqrt(n) footprint,
-    dropped from heights uniform in `drop`, plus the world's own ground."""
+    boxes with half extents in [0.25, 1], spheres with radius in [0.25, 1]
+    and hulls from hull_library(), uniform random orientations, over a
+    sqrt(n) footprint, dropped from heights uniform in `drop`, plus the
+    world's own ground."""
     foot = 2.0 * np.sqrt(max(n, 1)) + 2.0
     out = np.zeros(n + 1, BODY_DTYPE)
     out[0] = ground(foot + 20.0, 1.0, foot + 20.0)
     body = out[1:]
-    sph = rng.uniform(size=n) < sphere_fraction
-    body["shape"] = np.where(sph, SHAPE_SPHERE, SHAPE_BOX)
+    u = rng.uniform(size=n)
+    sph = u < sphere_fraction
+    hul = (u >= sphere_fraction) & (u < sphere_fraction + hull_fraction)
+    body["shape"] = np.where(sph, SHAPE_SPHERE, np.where(hul, SHAPE_CONVEX_HULL, SHAPE_BOX))
     dims = 2 * rng.uniform(0.25, 1.0, (n, 3))
     rad = rng.uniform(0.25, 1.0, n)
     dims[sph, 0] = rad[sph]
     dims[sph, 1:] = 0.0
+    hid = rng.integers(0, HULL_LIBRARY_SIZE, n)
+    dims[hul, 0] = hid[hul]
+    dims[hul, 1:] = 0.0
     body["dims"] = dims
     body["mass"] = 1.0
     st = np.zeros((n, 13), np.float32)

@@ -21,8 +21,9 @@ import numpy as np
 from .space import Space
 
 # Device bytes per body slot of a world handle: body arrays (~250 B), and the
-# pair, hash, island and solver arrays sized 4 pairs per body (~3 KB).
-BYTES_PER_BODY = 3400
+# pair, hash, island, solver and general-convex arrays sized 4 pairs per body
+# (~4.4 KB).
+BYTES_PER_BODY = 4700
 
 
 class Request:
@@ -41,11 +42,15 @@ class Request:
 
 class Server:
     """Admission queue + resident batch.  `capacity` bodies are allocated on
-    the device up front; `hbm_budget` (bytes) caps admission as well."""
+    the device up front; `hbm_budget` (bytes) caps admission as well.
+    `hull_shapes`: point clouds of the hull shapes requests refer to."""
 
-    def __init__(self, capacity, settings=None, hbm_budget=200e9, clock=time.perf_counter):
+    def __init__(self, capacity, settings=None, hbm_budget=200e9, clock=time.perf_counter, hull_shapes=None):
         self.capacity = int(min(capacity, hbm_budget // BYTES_PER_BODY))
         self.space = Space(capacity=self.capacity, settings=settings)
+        if hull_shapes is not None:
+            # Hull bodies of the requests name these shapes by index.
+            self.space.register_hulls(hull_shapes)
         self.queue = collections.deque()
         self.resident = {}
         self.completed = []

@@ -178,6 +178,35 @@ bepu_status bepu_add_box(bepu_world world, const bepu_motion_state* state, float
 bepu_status bepu_add_sphere(bepu_world world, const bepu_motion_state* state, float radius,
                             float mass, const bepu_material* material, uint32_t flags,
                             uint32_t* id_out);
+/* Space.Add(new Capsule(motionState, length, radius, mass)) -- Entities/Prefabs/Capsule.cs:149-153,
+ * CapsuleShape.cs:40-92: a segment of `length` along the local y axis, swept by `radius`
+ * (the collision margin). */
+bepu_status bepu_add_capsule(bepu_world world, const bepu_motion_state* state, float length,
+                             float radius, float mass, const bepu_material* material,
+                             uint32_t flags, uint32_t* id_out);
+
+/* Convex hulls.  bepu_compute_hull is ConvexHullShape.ComputeDescription
+ * (ConvexHullShape.cs:154-175): the hull of `points` (xyz triples;
+ * ConvexHullHelper.GetConvexHull), its volume distribution and volume
+ * (InertiaHelper.ComputeShapeDistribution) and its unique surface vertices
+ * recentred on the volume centroid `center_out`.  Pure host computation; any
+ * output pointer may be null; at most `capacity` vertices are written and
+ * `nvertices_out` receives their number.  A point set without volume returns
+ * BEPU_E_INVALID_ARG ("Point set is degenerate", ConvexHullHelper.cs:302-367). */
+bepu_status bepu_compute_hull(const float* points, uint32_t npoints, float* vertices_out,
+                              uint32_t capacity, uint32_t* nvertices_out,
+                              float* volume_distribution_out /* 9, row-major */,
+                              float* center_out /* 3 */, float* volume_out);
+/* new ConvexHullShape(points, out center) registered with the world: many bodies
+ * may share one hull shape, as entities share a ConvexHullShape instance. */
+bepu_status bepu_create_hull_shape(bepu_world world, const float* points, uint32_t npoints,
+                                   uint32_t* shape_id_out, float* center_out /* 3, may be null */);
+/* Space.Add(new ConvexHull(motionState, vertices, mass)) -- Entities/Prefabs/ConvexHull.cs:83-87:
+ * the body's position is the motion state's (the shape's local origin is its centroid),
+ * collision margin CollisionDetectionSettings.DefaultMargin. */
+bepu_status bepu_add_convex_hull(bepu_world world, const bepu_motion_state* state,
+                                 uint32_t hull_shape_id, float mass, const bepu_material* material,
+                                 uint32_t flags, uint32_t* id_out);
 
 /* Batched worlds.  A world handle may hold many independent sub-worlds, each
  * the analogue of one Space: bodies of different sub-worlds never form pairs,
@@ -193,8 +222,9 @@ bepu_status bepu_get_world_range(bepu_world world, uint32_t world_id, uint32_t* 
 
 /* Continuous batching.  One body of a world inserted in a single call. */
 typedef struct bepu_body_desc {
-    int32_t shape;            /* BEPU_SHAPE_BOX or BEPU_SHAPE_SPHERE */
-    float dims[3];            /* box: width, height, length; sphere: radius, 0, 0 */
+    int32_t shape;            /* BEPU_SHAPE_* */
+    float dims[3];            /* box: width, height, length; sphere: radius, 0, 0;
+                                 capsule: length, radius, 0; hull: shape id (an integer), 0, 0 */
     float mass;               /* <= 0: kinematic */
     bepu_motion_state state;
     bepu_material material;

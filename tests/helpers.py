@@ -7,11 +7,26 @@ def f32bits(a):
     return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
 
 
+def oracle_hulls():
+    """The scene hull library as the oracle takes it: (surface vertices, volume
+    distribution) per shape, computed by the library's host hull builder."""
+    global _ORACLE_HULLS
+    if _ORACLE_HULLS is None:
+        from bepuphysics1_amd import compute_hull, scenes
+        _ORACLE_HULLS = [compute_hull(p)[:2] for p in scenes.hull_library()]
+    return _ORACLE_HULLS
+
+
+_ORACLE_HULLS = None
+
+
 def run_gpu(scene, steps, settings=None, capacity=None, broadphase=None):
-    from bepuphysics1_amd import Space
+    from bepuphysics1_amd import Space, scenes
     sp = Space(capacity=capacity or max(len(scene), 2), settings=settings)
     if broadphase:
         sp.set_broadphase(broadphase)
+    if scenes.has_hulls(scene):
+        sp.register_hulls(scenes.hull_library())
     sp.add_scene(scene)
     if steps:
         sp.Update(steps)
@@ -20,8 +35,9 @@ def run_gpu(scene, steps, settings=None, capacity=None, broadphase=None):
 
 def run_oracle(scene, steps, settings=None):
     from oracle import refcpu
+    from bepuphysics1_amd import scenes
     w = refcpu.World(settings)
-    w.add_scene(scene)
+    w.add_scene(scene, oracle_hulls() if scenes.has_hulls(scene) else None)
     if steps:
         w.step(steps)
     return w
