@@ -138,18 +138,22 @@ def cpu_baseline(scene, steps):
     return dyn * steps / dt, dt
 
 
+MIX = None  # boxes, spheres and 16-point hulls in equal shares (set in main)
+
+
 def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
-    """4096 independent piles (N ~ logU[8,256], boxes and spheres, own grounds,
-    8 iterations) stepped as sub-worlds of one batch, against the plain path:
-    the same worlds stepped one at a time, each in its own world handle
-    (sampled, then scaled to all worlds)."""
+    """4096 independent piles (N ~ logU[8,256]; boxes, spheres and 16-point
+    hulls; own grounds; 8 iterations) stepped as sub-worlds of one batch,
+    against the plain path: the same worlds stepped one at a time, each in its
+    own world handle (sampled, then scaled to all worlds)."""
     from bepuphysics1_amd import Space, scenes
     from bepuphysics1_amd._lib import default_settings
     st = default_settings()
     st.iteration_limit = 8
-    worlds = scenes.batched_worlds(n_worlds, seed=0)
+    worlds = scenes.batched_worlds(n_worlds, seed=0, shapes=MIX)
     n_dyn = int(sum((w["mass"] > 0).sum() for w in worlds))
     sp = Space(capacity=sum(len(w) for w in worlds), settings=st)
+    sp.register_hulls(scenes.hull_library())
     for w in worlds:
         sp.add_world(w)
     sp.Update(warmup)
@@ -168,6 +172,7 @@ def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
     t_sum, b_sum = 0.0, 0
     for k in pick:
         one = Space(capacity=len(worlds[k]), settings=st)
+        one.register_hulls(scenes.hull_library())
         one.add_scene(worlds[k])
         one.Update(2)
         one.stats()
@@ -179,8 +184,8 @@ def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
         b_sum += int((worlds[k]["mass"] > 0).sum())
         one.close()
     plain = b_sum / t_sum  # body-steps/s when worlds are stepped one at a time
-    return {"workload": "%d worlds, N ~ logU[8,256] boxes + spheres on own grounds, dt 1/60, 8 iterations"
-                        % n_worlds,
+    return {"workload": "%d worlds, N ~ logU[8,256] boxes + spheres + 16-point hulls on own grounds, dt 1/60, "
+                        "8 iterations" % n_worlds,
             "bodies": n_dyn, "steps": steps, "value": round(batched, 1), "ms_per_step": round(1e3 * dt / steps, 4),
             "pairs": stats.pairs, "manifolds": stats.manifolds,
             "stages_ms": {k: round(v, 4) for k, v in stages.items()},
@@ -191,25 +196,62 @@ def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
 
 def served_leg(rate, duration, max_wall):
     """Served rollouts: Poisson arrivals at `rate`/s for `duration` s of wall
-    time, each a random pile of N ~ logU[16,2048] bodies (boxes and spheres,
-    own ground) stepped S ~ U[60,600] times at 8 iterations; continuous
-    batching on one world handle (admission against capacity and a 200 GB
-    budget), then the queue drains."""
+    time, each a random pile of N ~ logU[16,2048] bodies (boxes, spheres and
+    16-point hulls in equal shares, own ground) stepped S ~ U[60,600] times at
+    8 iterations; continuous batching on one world handle (admission against
+    capacity and a 200 GB budget), then the queue drains."""
     from bepuphysics1_amd import scenes
     from bepuphysics1_amd._lib import default_settings
     from bepuphysics1_amd.serve import Server, run_poisson
     st = default_settings()
     st.iteration_limit = 8
-    srv = Server(capacity=1_000_000, settings=st, hbm_budget=200e9)
+    srv = Server(capacity=1_000_000, settings=st, hbm_budget=200e9, hull_shapes=scenes.hull_library())
 
     def make_request(rng):
         n = int(np.exp(rng.uniform(np.log(16), np.log(2048))))
-        return scenes.random_pile_fast(n, rng), int(rng.integers(60, 601))
+        return scenes.random_pile_fast(n, rng, sphere_fraction=1 / 3, hull_fraction=1 / 3), int(rng.integers(60, 601))
 
     out = run_poisson(srv, rate, duration, make_request, seed=0, drain=True, max_wall=max_wall)
     srv.close()
-    out["workload"] = ("Poisson %g requests/s for %g s, N ~ logU[16,2048] boxes + spheres, S ~ U[60,600], "
-                       "8 iterations, continuous batching" % (rate, duration))
+    out["workload"] = ("Poisson %g requests/s for %g s, N ~ logU[16,2048] boxes + spheres + 16-point hulls, "
+                       "S ~ U[60,600], 8 iterations, continuous batching" % (rate, duration))
+    return out
+
+
+def stress_leg(n, reps, cpu):
+    """Broadphase + narrowphase stress: n bodies (50% boxes, 25% spheres, 25%
+    16-point hulls) at random in a cube sized for a mean AABB degree of 4, one
+    step from a fresh world (every pair new), timed after upload; the oracle's
+    step on the same scene beside it."""
+    from bepuphysics1_amd import Space, scenes
+    sc = scenes.stress(n, seed=0, shapes=scenes.STRESS_MIX)
+    best, stages, stats = None, None, None
+    for _ in range(reps):
+        sp = Space(capacity=n)
+        sp.register_hulls(scenes.hull_library())
+        sp.add_scene(sc)
+        sp.motion_states()  # upload and synchronise
+        sp.set_timing(True)
+        t0 = time.perf_counter()
+        sp.Update(1)
+        sp.synchronize()
+        dt = time.perf_counter() - t0
+        if best is None or dt < best:
+            best, stages, stats = dt, sp.stage_times(), sp.stats()
+        sp.close()
+    out = {"workload": "stress, %d bodies (50%% boxes, 25%% spheres, 25%% hulls), 1 step from a fresh world" % n,
+           "ms_per_step": round(1e3 * best, 4), "pairs": stats.pairs, "manifolds": stats.manifolds,
+           "contacts": stats.contacts, "pairs_per_s": round(stats.pairs / best, 1),
+           "stages_ms": {k: round(v, 4) for k, v in stages.items()}}
+    if cpu:
+        from oracle import refcpu
+        from bepuphysics1_amd import compute_hull
+        w = refcpu.World()
+        w.add_scene(sc, [compute_hull(p)[:2] for p in scenes.hull_library()])
+        t0 = time.perf_counter()
+        w.step(1)
+        out["cpu_oracle_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
+        w.close()
     return out
 
 
@@ -251,6 +293,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=2000.0)
     ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--stress-bodies", type=int, default=32768)
+    ap.add_argument("--no-stress", action="store_true")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -260,6 +304,8 @@ def main():
         sys.exit(2)
 
     from bepuphysics1_amd import Space, scenes
+    global MIX
+    MIX = (scenes.SHAPE_BOX, scenes.SHAPE_SPHERE, scenes.SHAPE_CONVEX_HULL)
 
     scene = scenes.towers(nx=a.nx, nz=a.nz, height=a.height, seed=0)
     n_dyn = int((scene["mass"] > 0).sum())
@@ -321,6 +367,7 @@ def main():
     copy_bytes = calibrate_copy() if a.calibrate_copy else None
     batched = None if a.no_batched else batched_leg(a.batched_worlds, a.batched_steps, 5, 48, 10)
     served = None if a.no_served else served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0)
+    stress = None if a.no_stress else stress_leg(a.stress_bodies, 3, not a.no_cpu_baseline)
 
     line = {
         "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
@@ -353,6 +400,7 @@ def main():
         "parity": parity,
         "batched": batched,
         "served": served,
+        "stress": stress,
         "calibration_copy_bytes": copy_bytes,
         "towers_standing": standing,
         "max_penetration": round(float(st.max_penetration), 5),

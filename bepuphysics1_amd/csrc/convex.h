@@ -412,137 +412,127 @@ BEPU_D f3 ps_triangle(PSimplex& s) {
     return p + acw;
 }
 
-// PairSimplex.TryTetrahedronTriangle (PairSimplex.cs:655-845).  The vertex
-// arguments index the simplex: face (i, j, k) with `o` the opposite vertex.
-BEPU_D bool ps_try_face(const PSimplex& s, const f3* P, int i, int j, int k, int o, float tol, PSimplex* out,
-                        f3* point) {
-    f3 A = P[i], B = P[j], C = P[k];
-    ps_clear(*out);
-    *point = zero3();
+// Result of one face test of the tetrahedron case: the sub-simplex kept
+// (state and up to three vertex indices of the tetrahedron), its barycentric
+// weights and its closest point.  The winner is applied to the simplex once,
+// instead of carrying a candidate copy of the whole simplex per face.
+struct FaceResult {
+    int state;
+    int v0, v1, v2;
+    float U, V, W;
+    f3 point;
+};
+
+template <int I>
+BEPU_D f3 vtx(const PSimplex& s) {
+    return I == 0 ? s.A : (I == 1 ? s.B : (I == 2 ? s.C : s.D));
+}
+BEPU_D f3 sel4(int i, f3 a, f3 b, f3 c, f3 d) { return i == 0 ? a : (i == 1 ? b : (i == 2 ? c : d)); }
+
+// PairSimplex.TryTetrahedronTriangle (PairSimplex.cs:655-845): face (i, j, k) of
+// the tetrahedron with `o` the opposite vertex.
+template <int i, int j, int k, int o>
+BEPU_D bool ps_try_face(const PSimplex& s, float tol, FaceResult* r) {
+    f3 A = vtx<i>(s), B = vtx<j>(s), C = vtx<k>(s);
     f3 ab = B - A, ac = C - A;
     f3 normal = cross3(ab, ac);
-    f3 AD = P[o] - A;
+    f3 AD = vtx<o>(s) - A;
     float AdotN = dot3(A, normal), ADdotN = dot3(AD, normal);
     if (!(AdotN * ADdotN >= -kEpsilon * tol)) return false;
+    r->V = r->W = 0;
     float AdotAB = -dot3(ab, A), AdotAC = -dot3(ac, A);
     if (AdotAC <= 0.f && AdotAB <= 0) {
-        out->state = S_POINT;
-        out->A = A;
-        out->U = 1;
-        out->sa[0] = s.sa[i];
-        out->sb[0] = s.sb[i];
-        *point = A;
+        *r = FaceResult{S_POINT, i, 0, 0, 1, 0, 0, A};
         return true;
     }
     float BdotAB = -dot3(ab, B), BdotAC = -dot3(ac, B);
     if (BdotAB >= 0.f && BdotAC <= BdotAB) {
-        out->state = S_POINT;
-        out->A = B;
-        out->U = 1;
-        out->sa[0] = s.sa[j];
-        out->sb[0] = s.sb[j];
-        *point = B;
+        *r = FaceResult{S_POINT, j, 0, 0, 1, 0, 0, B};
         return true;
     }
     float vc = AdotAB * BdotAC - BdotAB * AdotAC;
     if (vc <= 0 && AdotAB > 0 && BdotAB < 0) {
-        out->state = S_SEGMENT;
-        out->V = AdotAB / (AdotAB - BdotAB);
-        out->U = 1 - out->V;
-        out->A = A;
-        out->B = B;
-        out->sa[0] = s.sa[i];
-        out->sb[0] = s.sb[i];
-        out->sa[1] = s.sa[j];
-        out->sb[1] = s.sb[j];
-        *point = ab * out->V + A;
+        float V = AdotAB / (AdotAB - BdotAB);
+        *r = FaceResult{S_SEGMENT, i, j, 0, 1 - V, V, 0, ab * V + A};
         return true;
     }
     float CdotAB = -dot3(ab, C), CdotAC = -dot3(ac, C);
     if (CdotAC >= 0.f && CdotAB <= CdotAC) {
-        out->state = S_POINT;
-        out->A = C;
-        out->U = 1;
-        out->sa[0] = s.sa[k];
-        out->sb[0] = s.sb[k];
-        *point = C;
+        *r = FaceResult{S_POINT, k, 0, 0, 1, 0, 0, C};
         return true;
     }
     float vb = CdotAB * AdotAC - AdotAB * CdotAC;
     if (vb <= 0.f && AdotAC > 0.f && CdotAC < 0.f) {
-        out->state = S_SEGMENT;
-        out->A = A;
-        out->B = C;
-        out->sa[0] = s.sa[i];
-        out->sa[1] = s.sa[k];
-        out->sb[0] = s.sb[i];
-        out->sb[1] = s.sb[k];
-        out->V = AdotAC / (AdotAC - CdotAC);
-        out->U = 1 - out->V;
-        *point = ac * out->V + A;
+        float V = AdotAC / (AdotAC - CdotAC);
+        *r = FaceResult{S_SEGMENT, i, k, 0, 1 - V, V, 0, ac * V + A};
         return true;
     }
     float va = BdotAB * CdotAC - CdotAB * BdotAC;
     float d3d4, d6d5;
     if (va <= 0.f && (d3d4 = BdotAC - BdotAB) > 0.f && (d6d5 = CdotAB - CdotAC) > 0.f) {
-        out->state = S_SEGMENT;
-        out->A = B;
-        out->B = C;
-        out->sa[0] = s.sa[j];
-        out->sa[1] = s.sa[k];
-        out->sb[0] = s.sb[j];
-        out->sb[1] = s.sb[k];
-        out->V = d3d4 / (d3d4 + d6d5);
-        out->U = 1 - out->V;
-        *point = (C - B) * out->V + B;
+        float V = d3d4 / (d3d4 + d6d5);
+        *r = FaceResult{S_SEGMENT, j, k, 0, 1 - V, V, 0, (C - B) * V + B};
         return true;
     }
-    out->A = A;
-    out->B = B;
-    out->C = C;
-    out->sa[0] = s.sa[i];
-    out->sa[1] = s.sa[j];
-    out->sa[2] = s.sa[k];
-    out->sb[0] = s.sb[i];
-    out->sb[1] = s.sb[j];
-    out->sb[2] = s.sb[k];
-    out->state = S_TRIANGLE;
     float denom = 1.f / (va + vb + vc);
-    out->W = vc * denom;
-    out->V = vb * denom;
-    out->U = 1 - out->V - out->W;
-    f3 p = ab * out->V;
-    f3 acw = ac * out->W;
+    float W = vc * denom;
+    float V = vb * denom;
+    f3 p = ab * V;
+    f3 acw = ac * W;
     p = A + p;
-    *point = p + acw;
+    *r = FaceResult{S_TRIANGLE, i, j, k, 1 - V - W, V, W, p + acw};
     return true;
 }
 
-// PairSimplex.GetPointOnTetrahedronClosestToOrigin (PairSimplex.cs:572-650)
+// PairSimplex.GetPointOnTetrahedronClosestToOrigin (PairSimplex.cs:572-650).  The
+// simplex becomes the winning face's sub-simplex; fields beyond its state are
+// never read before they are written again.
 BEPU_D bool ps_tetrahedron(PSimplex& s, float tol, f3* point) {
-    f3 P[4] = {s.A, s.B, s.C, s.D};
-    PSimplex best;
-    ps_clear(best);
+    FaceResult best, cand;
+    best.state = S_EMPTY;
     *point = zero3();
     float minimum = FLT_MAX;
-    PSimplex cand;
-    f3 cp;
     float cd;
-    // Faces ACD|B, BDC|A, ADB|C, ABC|D.
-    const int faces[4][4] = {{0, 2, 3, 1}, {1, 3, 2, 0}, {0, 3, 1, 2}, {0, 1, 2, 3}};
-    // The first accepted face is taken without a comparison (as the reference
-    // does), which matters only for a non-finite candidate.
-    for (int f = 0; f < 4; ++f) {
-        if (ps_try_face(s, P, faces[f][0], faces[f][1], faces[f][2], faces[f][3], tol, &cand, &cp) &&
-            (f == 0 || (cd = lensq3(cp)) < minimum)) {
-            if (f == 0) cd = lensq3(cp);
-            *point = cp;
-            best = cand;
-            minimum = cd;
-        }
+    // Faces ACD|B, BDC|A, ADB|C, ABC|D.  The first accepted face is taken
+    // without a comparison (as the reference does), which matters only for a
+    // non-finite candidate.
+    if (ps_try_face<0, 2, 3, 1>(s, tol, &cand)) {
+        best = cand;
+        minimum = lensq3(cand.point);
+    }
+    if (ps_try_face<1, 3, 2, 0>(s, tol, &cand) && (cd = lensq3(cand.point)) < minimum) {
+        best = cand;
+        minimum = cd;
+    }
+    if (ps_try_face<0, 3, 1, 2>(s, tol, &cand) && (cd = lensq3(cand.point)) < minimum) {
+        best = cand;
+        minimum = cd;
+    }
+    if (ps_try_face<0, 1, 2, 3>(s, tol, &cand) && (cd = lensq3(cand.point)) < minimum) {
+        best = cand;
+        minimum = cd;
     }
     if (minimum < FLT_MAX) {
-        s = best;
+        *point = best.point;
+        f3 A = sel4(best.v0, s.A, s.B, s.C, s.D), B = sel4(best.v1, s.A, s.B, s.C, s.D),
+           C = sel4(best.v2, s.A, s.B, s.C, s.D);
+        f3 a0 = sel4(best.v0, s.sa[0], s.sa[1], s.sa[2], s.sa[3]), a1 = sel4(best.v1, s.sa[0], s.sa[1], s.sa[2], s.sa[3]),
+           a2 = sel4(best.v2, s.sa[0], s.sa[1], s.sa[2], s.sa[3]);
+        f3 b0 = sel4(best.v0, s.sb[0], s.sb[1], s.sb[2], s.sb[3]), b1 = sel4(best.v1, s.sb[0], s.sb[1], s.sb[2], s.sb[3]),
+           b2 = sel4(best.v2, s.sb[0], s.sb[1], s.sb[2], s.sb[3]);
+        s.state = best.state;
+        s.A = A;
+        s.B = B;
+        s.C = C;
+        s.sa[0] = a0;
+        s.sa[1] = a1;
+        s.sa[2] = a2;
+        s.sb[0] = b0;
+        s.sb[1] = b1;
+        s.sb[2] = b2;
+        s.U = best.U;
+        s.V = best.V;
+        s.W = best.W;
         return false;
     }
     return true;
@@ -722,8 +712,8 @@ BEPU_D void barycentric(f3 p, f3 a, f3 b, f3 c, float* wa, float* wb, float* wc)
 // MPRToolbox.LocalSurfaceCast, both overloads (MPRToolbox.cs:610-880 and
 // 886-1172): `with_position` selects the overload that expands each shape's
 // support point separately and interpolates the contact position.
-BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, bool with_position, float* t,
-                         f3* normal, f3* position) {
+template <bool with_position>
+BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, float* t, f3* normal, f3* position) {
     f3 v1A = zero3(), v2A = zero3(), v3A = zero3(), v4A = zero3();
     f3 n = dir;
     f3 v1 = with_position ? mk_extreme_a(a, b, n, lb, &v1A) : mk_extreme(a, b, n, lb);
@@ -832,7 +822,7 @@ BEPU_D bool mpr_contact(const Shape& a, const Shape& b, const RT& ta, const RT& 
     f3 normal;
     float ls = lensq3(axis);
     if (ls > kEpsilon) {
-        surface_cast(a, b, lb, divide3(axis, sqrtf(ls)), false, &dep, &normal, nullptr);
+        surface_cast<false>(a, b, lb, divide3(axis, sqrtf(ls)), &dep, &normal, nullptr);
     } else {
         dep = FLT_MAX;
         normal = f3{0, 1, 0};
@@ -841,7 +831,7 @@ BEPU_D bool mpr_contact(const Shape& a, const Shape& b, const RT& ta, const RT& 
     if (ls > kEpsilon) {
         float dc;
         f3 nc;
-        surface_cast(a, b, lb, divide3(lb.pos, sqrtf(ls)), false, &dc, &nc, nullptr);
+        surface_cast<false>(a, b, lb, divide3(lb.pos, sqrtf(ls)), &dc, &nc, nullptr);
         if (dc < dep) {
             normal = nc;
             dep = dc;
@@ -852,7 +842,7 @@ BEPU_D bool mpr_contact(const Shape& a, const Shape& b, const RT& ta, const RT& 
     while (true) {
         float cd;
         f3 cn;
-        surface_cast(a, b, lb, normal, true, &cd, &cn, &p);
+        surface_cast<true>(a, b, lb, normal, &cd, &cn, &p);
         if (dep - cd <= kDepthRefinementEpsilon || ++count >= kMaxDepthRefinementIterations) {
             dep = cd;
             break;

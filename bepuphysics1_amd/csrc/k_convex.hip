@@ -12,7 +12,7 @@ namespace bepu {
 
 namespace {
 
-__device__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i) {
+__device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i) {
     float4 sh = w.shape[i];
     cvx::Shape s;
     s.kind = (int)sh.w;
@@ -38,7 +38,7 @@ __device__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i) {
 
 // ContactReducer.ReduceContacts for four contacts and one candidate
 // (CollisionTests/ContactReducer.cs:197-318): the list index to remove (or -1).
-__device__ int reduce_contacts(const PairState& s, f3 cpos, float cdepth, bool* add) {
+__device__ __forceinline__ int reduce_contacts(const PairState& s, f3 cpos, float cdepth, bool* add) {
     f3 P[4];
     float D[4];
     for (int i = 0; i < 4; ++i) {
@@ -95,7 +95,7 @@ __device__ int reduce_contacts(const PairState& s, f3 cpos, float cdepth, bool* 
 
 // GeneralConvexContactManifold.Add: the contact and its supplement data
 // (GeneralConvexContactManifold.cs:128-137).
-__device__ void add_general(PairState& s, ConvexState& e, f3 pos, f3 nrm, float depth, int id, const cvx::RT& ta,
+__device__ __forceinline__ void add_general(PairState& s, ConvexState& e, f3 pos, f3 nrm, float depth, int id, const cvx::RT& ta,
                             const cvx::RT& tb) {
     int slot = 0;
     while (s.flags & (16u << slot)) ++slot;
@@ -107,7 +107,7 @@ __device__ void add_general(PairState& s, ConvexState& e, f3 pos, f3 nrm, float 
 // GeneralConvexContactManifold.Update (GeneralConvexContactManifold.cs:71-126) with
 // ContactRefresher.ContactRefresh (CollisionTests/ContactRefresher.cs:18-69) and
 // IsContactUnique (GeneralConvexContactManifold.cs:145-166).
-__device__ void update_general(const DeviceWorld& w, PairState& s, ConvexState& e) {
+__device__ __forceinline__ void update_general(const DeviceWorld& w, PairState& s, ConvexState& e) {
     float4 PA = w.pos[s.a], QA = w.quat[s.a], PB = w.pos[s.b], QB = w.quat[s.b];
     cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
     cvx::RT tb{f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w}};
@@ -133,6 +133,10 @@ __device__ void update_general(const DeviceWorld& w, PairState& s, ConvexState& 
     }
     for (int i = nrem - 1; i >= 0; i--) remove_contact(s, rem[i]);
     cvx::TesterState st{__float_as_int(e.sep.w), f3_of(e.sep), f3_of(e.dir)};
+    // Keep the state opaque to the optimiser: a state known on one path (a new
+    // pair starts Separated) would otherwise get its own clone of the whole
+    // pair test, doubling the code and its register pressure.
+    asm volatile("" : "+v"(st.state));
     f3 pos, nrm;
     float depth;
     int id;
@@ -181,17 +185,30 @@ __global__ void __launch_bounds__(kConvexBlock) k_convex(DeviceWorld w) {
     uint32_t n = w.counters[ctr(CTR_CONVEX)];
     for (uint32_t i = blockIdx.x * kConvexBlock + threadIdx.x; i < n; i += gridDim.x * kConvexBlock) {
         uint2 wk = w.conv_work[i];
-        PairState s = w.pairs_cur[wk.x];
-        ConvexState e;
+        // The pair's state is updated in place in global memory: its contact
+        // slots are indexed at run time, which would put a private copy in
+        // scratch memory.
+        PairState& s = w.pairs_cur[wk.x];
+        ConvexState& e = w.conv_cur[wk.x];
+        // Carry last step's state over field by field (a whole-struct copy
+        // between global arrays is lowered through scratch memory).
         if (wk.y != ~0u) {
-            e = w.conv_old[wk.y];
+            const ConvexState& o = w.conv_old[wk.y];
+            float4 t0 = o.sep, t1 = o.dir;
+            float4 a0 = o.offa[0], a1 = o.offa[1], a2 = o.offa[2], a3 = o.offa[3];
+            float4 b0 = o.offb[0], b1 = o.offb[1], b2 = o.offb[2], b3 = o.offb[3];
+            e.sep = t0;
+            e.dir = t1;
+            e.offa[0] = a0; e.offa[1] = a1; e.offa[2] = a2; e.offa[3] = a3;
+            e.offb[0] = b0; e.offb[1] = b1; e.offb[2] = b2; e.offb[3] = b3;
         } else {
-            e.sep = e.dir = make_float4(0, 0, 0, 0);
-            for (int k = 0; k < 4; ++k) e.offa[k] = e.offb[k] = make_float4(0, 0, 0, 0);
+            float4 z = make_float4(0, 0, 0, 0);
+            e.sep = z;
+            e.dir = z;
+            e.offa[0] = z; e.offa[1] = z; e.offa[2] = z; e.offa[3] = z;
+            e.offb[0] = z; e.offb[1] = z; e.offb[2] = z; e.offb[3] = z;
         }
         update_general(w, s, e);
-        w.pairs_cur[wk.x] = s;
-        w.conv_cur[wk.x] = e;
         // A constraint joins the islands of its two bodies when both are dynamic.
         if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
     }

@@ -185,11 +185,14 @@ def stress(n=32768, edge=None, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE)):
     """Bodies placed uniformly at random in a cube of side `edge` with random
     orientations, no ground, shapes in equal shares of `shapes` (STRESS_MIX:
     50% boxes, 25% spheres, 25% hulls).  Box half extents and sphere radii in
-    [0.25, 1]; hulls from hull_library().  With the default edge the mean AABB
-    degree is about 4 for the box/sphere mix."""
+    [0.25, 1]; hulls from hull_library().  The default edge gives a mean AABB
+    degree of about 4 (74 for boxes and spheres, 71.2 with hulls, at 32,768
+    bodies, scaled with the cube root of n)."""
     rng = np.random.default_rng(seed)
     if edge is None:
-        edge = 100.0 * (n / 32768.0) ** (1.0 / 3.0) * 0.74
+        # Mean AABB degree about 4 (about 2 pairs per body) for either mix.
+        base = 71.2 if SHAPE_CONVEX_HULL in shapes else 74.0
+        edge = base * (n / 32768.0) ** (1.0 / 3.0)
     out = np.zeros(n, BODY_DTYPE)
     kinds = np.array(shapes)[np.arange(n) % len(shapes)]
     rng.shuffle(kinds)
