@@ -124,11 +124,15 @@ def parity_check():
             "max_impulse_rel_err": err, "max_motion_abs_diff": motion}
 
 
-def cpu_baseline(scene, steps):
-    """Oracle (single-threaded C++ restatement) on the same scene; one warm-up step excluded."""
+def cpu_baseline(scene, steps, threads=1):
+    """Oracle (C++ restatement of the reference step) on the same scene; one
+    warm-up step excluded.  With threads > 1 the body stages, broadphase cells
+    and pair updates run in parallel and the solver colour by colour (same
+    results as one thread)."""
     from oracle import refcpu
     w = refcpu.World()
     w.add_scene(scene)
+    w.set_threads(threads)
     w.step(1)
     t0 = time.perf_counter()
     w.step(steps)
@@ -358,10 +362,15 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        v_cpu, dt_cpu = cpu_baseline(scene, a.cpu_steps)
-        cpu = {"value": round(v_cpu, 1), "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": "oracle (single-threaded C++ restatement of the reference path), same 81,920-box "
-                         "towers scene, %d steps after 1 untimed step, %.1f s" % (a.cpu_steps, dt_cpu)}
+        threads = max(1, min(16, os.cpu_count() or 1))
+        v_cpu, dt_cpu = cpu_baseline(scene, a.cpu_steps, threads)
+        v_1t, dt_1t = cpu_baseline(scene, max(2, a.cpu_steps // 4), 1)
+        cpu = {"value": round(v_cpu, 1), "unit": "body-steps/s", "cores": threads, "kind": "port",
+               "sample": "oracle (C++ restatement of the reference step; %d threads: parallel body stages, "
+                         "broadphase and pair updates, colour-parallel solver), same 81,920-box towers scene, "
+                         "%d steps after 1 untimed step, %.1f s" % (threads, a.cpu_steps, dt_cpu),
+               "single_thread_value": round(v_1t, 1),
+               "single_thread_sample": "%d steps, %.1f s" % (max(2, a.cpu_steps // 4), dt_1t)}
 
     parity = None if a.no_parity else parity_check()
     copy_bytes = calibrate_copy() if a.calibrate_copy else None

@@ -93,6 +93,7 @@ def lib():
         L.refcpu_create.argtypes = [ctypes.POINTER(Settings)]
         L.refcpu_destroy.argtypes = [vp]
         L.refcpu_set_broadphase.argtypes = [vp, ctypes.c_int]
+        L.refcpu_set_threads.argtypes = [vp, ctypes.c_int]
         L.refcpu_settings_default.argtypes = [ctypes.POINTER(Settings)]
         L.refcpu_add_box.argtypes = [vp, ctypes.POINTER(MotionState), ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, ctypes.c_float, ctypes.POINTER(Material),
@@ -160,6 +161,14 @@ class World:
             self.close()
         except Exception:
             pass
+
+    def set_threads(self, n):
+        """Threads for step(); the results are identical for any count."""
+        lib().refcpu_set_threads(self.h, int(n))
+
+    def set_broadphase_mode(self, mode):
+        """0: uniform grid (default), 1: brute force, 2: sort and sweep -- one pair set."""
+        lib().refcpu_set_broadphase(self.h, int(mode))
 
     def set_bruteforce_broadphase(self, on=True):
         lib().refcpu_set_broadphase(self.h, 1 if on else 0)

@@ -23,9 +23,14 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../include/bepu_gpu.h"
@@ -135,21 +140,115 @@ struct Settings {
 
 using namespace ref;
 
+namespace ref {
+
+// A persistent pool for the multithreaded step: run(n, fn) calls fn(begin, end)
+// over [0, n) in chunks on every worker and the caller, and returns when all
+// chunks are done.
+class Pool {
+public:
+    explicit Pool(int nthreads) {
+        for (int i = 1; i < nthreads; ++i) workers_.emplace_back([this] { loop(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    int size() const { return (int)workers_.size() + 1; }
+    void run(size_t n, size_t chunk, const std::function<void(size_t, size_t)>& fn) {
+        if (n == 0) return;
+        if (workers_.empty() || n <= chunk) {
+            fn(0, n);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> g(m_);
+            fn_ = &fn;
+            n_ = n;
+            chunk_ = chunk;
+            next_.store(0);
+            pending_ = (int)workers_.size();
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> l(m_);
+        done_cv_.wait(l, [this] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+
+private:
+    void work() {
+        for (;;) {
+            size_t b = next_.fetch_add(chunk_);
+            if (b >= n_) break;
+            (*fn_)(b, std::min(n_, b + chunk_));
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+            }
+            work();
+            std::lock_guard<std::mutex> g(m_);
+            if (--pending_ == 0) done_cv_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+    int pending_ = 0;
+    const std::function<void(size_t, size_t)>* fn_ = nullptr;
+    size_t n_ = 0, chunk_ = 1;
+    std::atomic<size_t> next_{0};
+};
+
+// The pair states of the last step in key order.
+struct PairTable {
+    std::vector<uint64_t> keys;
+    std::vector<Pair> vals;
+    const Pair* find(uint64_t k) const {
+        auto it = std::lower_bound(keys.begin(), keys.end(), k);
+        return (it != keys.end() && *it == k) ? &vals[it - keys.begin()] : nullptr;
+    }
+};
+
+}  // namespace ref
+
 struct refcpu_world {
     bepu_settings s;
     std::vector<Body> bodies;
     std::vector<V3> hull_verts;
-    std::map<uint64_t, Pair> pairs;
+    PairTable pairs;
     std::vector<uint64_t> overlaps;
     std::vector<uint64_t> order;  // solve order of the last step (pair keys)
     int ncolors = 0;
     uint32_t steps = 0;
     float max_pen = 0;
     int threads = 1;
-    int broadphase_mode = 0;  // 0: sort and sweep, 1: brute force
+    std::unique_ptr<Pool> pool;  // threads > 1
+    int broadphase_mode = 0;  // 0: uniform grid, 1: brute force, 2: sort and sweep
 };
 
 namespace {
+
+// Runs fn(begin, end) over [0, n): on the pool when the world is multithreaded.
+void par_for(refcpu_world* w, size_t n, size_t chunk, const std::function<void(size_t, size_t)>& fn) {
+    if (w->pool) w->pool->run(n, chunk, fn);
+    else fn(0, n);
+}
 
 // ---------------------------------------------------------------------------
 // Entity construction (host side, once per body)
@@ -353,15 +452,100 @@ void broadphase_sweep(refcpu_world* w, std::vector<uint64_t>* out) {
         float fa = w->bodies[a].bmin.x, fb = w->bodies[b].bmin.x;
         return fa < fb || (fa == fb && a < b);
     });
-    for (size_t i = 0; i < n; ++i) {
-        const Body& a = w->bodies[idx[i]];
-        for (size_t j = i + 1; j < n; ++j) {
-            const Body& b = w->bodies[idx[j]];
-            if (b.bmin.x > a.bmax.x) break;
-            if (!a.dynamic && !b.dynamic) continue;
-            if (intersects(a, b)) out->push_back(pair_key(idx[i], idx[j]));
+    // The sweep is split over ranges of the sorted order; the pair set is
+    // sorted afterwards, so the split does not change the result.
+    const size_t chunk = 2048;
+    std::vector<std::vector<uint64_t>> parts((n + chunk - 1) / chunk);
+    par_for(w, parts.size(), 1, [&](size_t p0, size_t p1) {
+        for (size_t p = p0; p < p1; ++p) {
+            std::vector<uint64_t>& o = parts[p];
+            for (size_t i = p * chunk; i < std::min(n, (p + 1) * chunk); ++i) {
+                const Body& a = w->bodies[idx[i]];
+                for (size_t j = i + 1; j < n; ++j) {
+                    const Body& b = w->bodies[idx[j]];
+                    if (b.bmin.x > a.bmax.x) break;
+                    if (!a.dynamic && !b.dynamic) continue;
+                    if (intersects(a, b)) o.push_back(pair_key(idx[i], idx[j]));
+                }
+            }
         }
+    });
+    for (auto& o : parts) out->insert(out->end(), o.begin(), o.end());
+    std::sort(out->begin(), out->end());
+}
+
+// The same pair set from a uniform grid: cell edge twice the median largest
+// AABB side; a body spanning more than 8 cells per axis (or with a non-finite
+// box) is tested against every body instead.  A pair is emitted only from
+// the cell that holds the maximum of the two boxes' minimum corners, a point
+// of their intersection, so each overlapping pair is found exactly once.
+void broadphase_grid(refcpu_world* w, std::vector<uint64_t>* out) {
+    out->clear();
+    size_t n = w->bodies.size();
+    if (n < 2) return;
+    std::vector<float> side;
+    side.reserve(n);
+    for (const Body& b : w->bodies) {
+        float e = std::max(std::max(b.bmax.x - b.bmin.x, b.bmax.y - b.bmin.y), b.bmax.z - b.bmin.z);
+        if (std::isfinite(e)) side.push_back(e);
     }
+    if (side.empty()) return;
+    std::nth_element(side.begin(), side.begin() + side.size() / 2, side.end());
+    double h = std::max(2.0 * (double)side[side.size() / 2], 1e-3);
+    auto cell = [&](float v) { return (int64_t)std::floor((double)v / h); };
+    auto pack = [](int64_t x, int64_t y, int64_t z) {
+        return ((uint64_t)(x & 0x1fffff) << 42) | ((uint64_t)(y & 0x1fffff) << 21) | (uint64_t)(z & 0x1fffff);
+    };
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    std::vector<uint32_t> large;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Body& b = w->bodies[i];
+        bool finite = std::isfinite(b.bmin.x) && std::isfinite(b.bmin.y) && std::isfinite(b.bmin.z) &&
+                      std::isfinite(b.bmax.x) && std::isfinite(b.bmax.y) && std::isfinite(b.bmax.z);
+        if (!finite) { large.push_back(i); continue; }
+        int64_t x0 = cell(b.bmin.x), x1 = cell(b.bmax.x), y0 = cell(b.bmin.y), y1 = cell(b.bmax.y);
+        int64_t z0 = cell(b.bmin.z), z1 = cell(b.bmax.z);
+        if (x1 - x0 > 8 || y1 - y0 > 8 || z1 - z0 > 8 || std::llabs(x0) > (1 << 19) || std::llabs(y0) > (1 << 19) ||
+            std::llabs(z0) > (1 << 19)) {
+            large.push_back(i);
+            continue;
+        }
+        for (int64_t x = x0; x <= x1; ++x)
+            for (int64_t y = y0; y <= y1; ++y)
+                for (int64_t z = z0; z <= z1; ++z) grid[pack(x, y, z)].push_back(i);
+    }
+    std::vector<std::pair<uint64_t, const std::vector<uint32_t>*>> cells;
+    cells.reserve(grid.size());
+    for (auto& kv : grid) cells.push_back({kv.first, &kv.second});
+    std::vector<std::vector<uint64_t>> parts(cells.size() / 256 + 1);
+    par_for(w, parts.size(), 1, [&](size_t p0, size_t p1) {
+        for (size_t p = p0; p < p1; ++p) {
+            for (size_t c = p * 256; c < std::min(cells.size(), (p + 1) * 256); ++c) {
+                const std::vector<uint32_t>& l = *cells[c].second;
+                for (size_t i = 0; i < l.size(); ++i)
+                    for (size_t j = i + 1; j < l.size(); ++j) {
+                        const Body& a = w->bodies[l[i]];
+                        const Body& b = w->bodies[l[j]];
+                        if (!a.dynamic && !b.dynamic) continue;
+                        if (!intersects(a, b)) continue;
+                        uint64_t own = pack(cell(std::max(a.bmin.x, b.bmin.x)), cell(std::max(a.bmin.y, b.bmin.y)),
+                                            cell(std::max(a.bmin.z, b.bmin.z)));
+                        if (own == cells[c].first) parts[p].push_back(pair_key(l[i], l[j]));
+                    }
+            }
+        }
+    });
+    for (auto& o : parts) out->insert(out->end(), o.begin(), o.end());
+    std::vector<char> is_large(n, 0);
+    for (uint32_t i : large) is_large[i] = 1;
+    for (uint32_t L : large)
+        for (uint32_t j = 0; j < n; ++j) {
+            if (j == L || (is_large[j] && j < L)) continue;
+            const Body& a = w->bodies[L];
+            const Body& b = w->bodies[j];
+            if (!a.dynamic && !b.dynamic) continue;
+            if (intersects(a, b)) out->push_back(pair_key(L, j));
+        }
     std::sort(out->begin(), out->end());
 }
 
@@ -675,27 +859,39 @@ void colour_constraints(refcpu_world* w, std::vector<Pair*>& cons) {
     pri.reserve(cons.size());
     for (Pair* p : cons) pri.push_back({mix64(pair_key(p->a, p->b)), p});
     std::sort(pri.begin(), pri.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-    // Per dynamic body: bitmask (and overflow list) of colours already taken.
-    std::vector<std::vector<int>> used(w->bodies.size());
+    // Per dynamic body: the colours already taken, as a bitmask for the first
+    // 64 and a list beyond.
+    std::vector<uint64_t> mask(w->bodies.size(), 0);
+    std::map<uint32_t, std::vector<int>> more;
     int ncol = 0;
     for (auto& e : pri) {
         Pair* p = e.second;
-        std::vector<char> taken(64, 0);
-        auto mark = [&](uint32_t bi) {
-            if (!w->bodies[bi].dynamic) return;
-            for (int c : used[bi]) {
-                if ((size_t)c >= taken.size()) taken.resize(c + 1, 0);
-                taken[c] = 1;
+        bool da = w->bodies[p->a].dynamic, db = w->bodies[p->b].dynamic;
+        uint64_t m = (da ? mask[p->a] : 0) | (db ? mask[p->b] : 0);
+        int c;
+        if (~m) {
+            c = __builtin_ctzll(~m);
+        } else {
+            std::vector<char> taken;
+            for (uint32_t bi : {p->a, p->b}) {
+                if (!w->bodies[bi].dynamic) continue;
+                auto it = more.find(bi);
+                if (it == more.end()) continue;
+                for (int k : it->second) {
+                    if ((size_t)(k - 64) >= taken.size()) taken.resize(k - 63, 0);
+                    taken[k - 64] = 1;
+                }
             }
-        };
-        mark(p->a);
-        mark(p->b);
-        int c = 0;
-        while ((size_t)c < taken.size() && taken[c]) ++c;
+            c = 64;
+            while ((size_t)(c - 64) < taken.size() && taken[c - 64]) ++c;
+        }
         p->color = c;
         if (c + 1 > ncol) ncol = c + 1;
-        if (w->bodies[p->a].dynamic) used[p->a].push_back(c);
-        if (w->bodies[p->b].dynamic) used[p->b].push_back(c);
+        for (uint32_t bi : {p->a, p->b}) {
+            if (!w->bodies[bi].dynamic) continue;
+            if (c < 64) mask[bi] |= 1ull << c;
+            else more[bi].push_back(c);
+        }
     }
     std::sort(cons.begin(), cons.end(), [](Pair* x, Pair* y) {
         if (x->color != y->color) return x->color < y->color;
@@ -1253,31 +1449,42 @@ void solve_iteration(refcpu_world* w, Pair& p) {
 // One time step (Space.DoTimeStep, Space.cs:496-521)
 
 void step_once(refcpu_world* w) {
-    for (Body& b : w->bodies) candidacy(w, b);
-    for (Body& b : w->bodies) force_update(w, b);
-    for (Body& b : w->bodies) bounding_box(w, b);
+    // The body stages are per-body independent, the pair updates per-pair
+    // independent, and the solver runs colour by colour (no two constraints of
+    // a colour share a dynamic body), so the multithreaded step computes the
+    // same bits as the single-threaded one.
+    par_for(w, w->bodies.size(), 4096, [&](size_t b0, size_t b1) {
+        for (size_t i = b0; i < b1; ++i) {
+            candidacy(w, w->bodies[i]);
+            force_update(w, w->bodies[i]);
+            bounding_box(w, w->bodies[i]);
+        }
+    });
     if (w->broadphase_mode == 1) broadphase_bruteforce(w, &w->overlaps);
-    else broadphase_sweep(w, &w->overlaps);
+    else if (w->broadphase_mode == 2) broadphase_sweep(w, &w->overlaps);
+    else broadphase_grid(w, &w->overlaps);
 
     // NarrowPhase.UpdateSingleThreaded: update (or create) one pair per overlap,
     // then drop pairs whose overlap vanished (NarrowPhase.cs:254-340).
-    std::map<uint64_t, Pair> next;
-    for (uint64_t key : w->overlaps) {
-        auto it = w->pairs.find(key);
-        Pair p;
-        if (it != w->pairs.end()) p = it->second;
-        else p = make_pair_state(w, (uint32_t)(key >> 32), (uint32_t)(key & 0xffffffffu));
-        if (p.kind == BEPU_PAIR_BOX_BOX) update_box_box(w, p);
-        else if (p.kind == BEPU_PAIR_CONVEX) update_general_convex(w, p);
-        else update_single_contact(w, p);
-        next.emplace(key, p);
-    }
-    w->pairs.swap(next);
+    std::vector<Pair> updated(w->overlaps.size());
+    par_for(w, w->overlaps.size(), 256, [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) {
+            uint64_t key = w->overlaps[k];
+            const Pair* old = w->pairs.find(key);
+            Pair& p = updated[k];
+            if (old) p = *old;
+            else p = make_pair_state(w, (uint32_t)(key >> 32), (uint32_t)(key & 0xffffffffu));
+            if (p.kind == BEPU_PAIR_BOX_BOX) update_box_box(w, p);
+            else if (p.kind == BEPU_PAIR_CONVEX) update_general_convex(w, p);
+            else update_single_contact(w, p);
+        }
+    });
+    w->pairs.keys = w->overlaps;
+    w->pairs.vals.swap(updated);
 
     std::vector<Pair*> cons;
     float maxpen = 0;
-    for (auto& kv : w->pairs) {
-        Pair& p = kv.second;
+    for (Pair& p : w->pairs.vals) {
         if (p.ccount > 0) cons.push_back(&p);
         for (int i = 0; i < p.ccount; ++i) maxpen = std::max(maxpen, p.slot[p.clist[i]].depth);
     }
@@ -1287,13 +1494,28 @@ void step_once(refcpu_world* w) {
     for (Pair* p : cons) w->order.push_back(pair_key(p->a, p->b));
 
     // Solver.UpdateSingleThreaded (Solver.cs:239-263) in colour order.
-    for (Pair* p : cons) prestep(w, *p);
-    for (Pair* p : cons) warm_start(w, *p);
-    for (int it = 0; it < w->s.iteration_limit; ++it)
-        for (Pair* p : cons) solve_iteration(w, *p);
+    par_for(w, cons.size(), 512, [&](size_t c0, size_t c1) {
+        for (size_t c = c0; c < c1; ++c) prestep(w, *cons[c]);
+    });
+    std::vector<size_t> cstart(1, 0);
+    for (size_t c = 1; c < cons.size(); ++c)
+        if (cons[c]->color != cons[c - 1]->color) cstart.push_back(c);
+    cstart.push_back(cons.size());
+    auto by_colour = [&](auto&& fn) {
+        for (size_t g = 0; g + 1 < cstart.size(); ++g) {
+            size_t base = cstart[g];
+            par_for(w, cstart[g + 1] - base, 512, [&](size_t c0, size_t c1) {
+                for (size_t c = c0; c < c1; ++c) fn(*cons[base + c]);
+            });
+        }
+    };
+    by_colour([&](Pair& p) { warm_start(w, p); });
+    for (int it = 0; it < w->s.iteration_limit; ++it) by_colour([&](Pair& p) { solve_iteration(w, p); });
 
     // ContinuousPositionUpdater.UpdateSingleThreaded, discrete path (ContinuousPositionUpdater.cs:134-154)
-    for (Body& b : w->bodies) integrate(w, b);
+    par_for(w, w->bodies.size(), 4096, [&](size_t b0, size_t b1) {
+        for (size_t i = b0; i < b1; ++i) integrate(w, w->bodies[i]);
+    });
     w->steps++;
 }
 
@@ -1335,6 +1557,12 @@ refcpu_world* refcpu_create(const bepu_settings* s) {
     return w;
 }
 void refcpu_destroy(refcpu_world* w) { delete w; }
+// Threads used by refcpu_step (1: single-threaded).  Results do not depend on it.
+void refcpu_set_threads(refcpu_world* w, int n) {
+    w->threads = n < 1 ? 1 : n;
+    if (w->threads > 1) w->pool.reset(new Pool(w->threads));
+    else w->pool.reset();
+}
 void refcpu_set_broadphase(refcpu_world* w, int mode) { w->broadphase_mode = mode; }
 
 static int finite3(const float* v, int n) {
@@ -1441,7 +1669,8 @@ int refcpu_broadphase_only(refcpu_world* w) {
     for (Body& b : w->bodies) force_update(w, b);
     for (Body& b : w->bodies) bounding_box(w, b);
     if (w->broadphase_mode == 1) broadphase_bruteforce(w, &w->overlaps);
-    else broadphase_sweep(w, &w->overlaps);
+    else if (w->broadphase_mode == 2) broadphase_sweep(w, &w->overlaps);
+    else broadphase_grid(w, &w->overlaps);
     return BEPU_OK;
 }
 
@@ -1480,13 +1709,13 @@ int refcpu_get_pairs(refcpu_world* w, uint64_t* keys, uint32_t cap, uint32_t* n)
 
 int refcpu_get_manifolds(refcpu_world* w, bepu_manifold* out, uint32_t cap, uint32_t* n) {
     uint32_t cnt = 0;
-    for (auto& kv : w->pairs) {
-        const Pair& p = kv.second;
+    for (size_t k = 0; k < w->pairs.keys.size(); ++k) {
+        const Pair& p = w->pairs.vals[k];
         if (p.ccount == 0) continue;
         if (cnt < cap) {
             bepu_manifold& m = out[cnt];
             std::memset(&m, 0, sizeof(m));
-            m.key = kv.first;
+            m.key = w->pairs.keys[k];
             m.body_a = p.a;
             m.body_b = p.b;
             m.count = p.ccount;
@@ -1510,7 +1739,7 @@ int refcpu_get_impulses(refcpu_world* w, bepu_impulses* out, uint32_t cap, uint3
     uint32_t cnt = (uint32_t)w->order.size();
     if (n) *n = cnt;
     for (uint32_t i = 0; i < cnt && i < cap; ++i) {
-        const Pair& p = w->pairs.at(w->order[i]);
+        const Pair& p = *w->pairs.find(w->order[i]);
         bepu_impulses& o = out[i];
         std::memset(&o, 0, sizeof(o));
         o.key = w->order[i];
@@ -1530,8 +1759,8 @@ int refcpu_get_stats(refcpu_world* w, bepu_step_stats* s) {
     s->bodies = (uint32_t)w->bodies.size();
     s->pairs = (uint32_t)w->overlaps.size();
     uint32_t m = 0, c = 0;
-    for (auto& kv : w->pairs)
-        if (kv.second.ccount > 0) { m++; c += kv.second.ccount; }
+    for (const Pair& p : w->pairs.vals)
+        if (p.ccount > 0) { m++; c += p.ccount; }
     s->manifolds = m;
     s->contacts = c;
     s->colors = (uint32_t)w->ncolors;

@@ -156,8 +156,12 @@ def test_box_sphere_closed_form(oracle):
 # ---------------------------------------------------------------- broadphase
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
-def test_sweep_pairs_equal_brute_force(oracle, seed):
+def test_grid_and_sweep_pairs_equal_brute_force(oracle, seed):
+    """The oracle's grid broadphase (default) and its sort-and-sweep give the
+    brute-force pair set, including bodies far larger than the grid cell."""
     sc = scenes.stress(1500, seed=seed)
+    if seed == 2:
+        sc = np.concatenate([scenes.mixed_sizes(400, seed=5, edge=60.0), sc])
     a = oracle.World()
     a.add_scene(sc)
     a.broadphase_only()
@@ -165,11 +169,19 @@ def test_sweep_pairs_equal_brute_force(oracle, seed):
     b.add_scene(sc)
     b.set_bruteforce_broadphase(True)
     b.broadphase_only()
+    c = oracle.World()
+    c.add_scene(sc)
+    c.set_broadphase_mode(2)
+    c.broadphase_only()
     assert np.array_equal(a.pairs(), b.pairs())
-    # and both equal an independent numpy brute force on the oracle's AABBs
+    assert np.array_equal(c.pairs(), b.pairs())
+    # and all equal an independent numpy brute force on the oracle's AABBs
+    # (kinematic-kinematic pairs excluded)
     ab = a.aabbs()
     lo, hi = ab[:, :3], ab[:, 3:]
     ov = np.all((lo[:, None, :] <= hi[None, :, :]) & (lo[None, :, :] <= hi[:, None, :]), axis=2)
+    dyn = sc["mass"] > 0
+    ov &= dyn[:, None] | dyn[None, :]
     i, j = np.nonzero(np.triu(ov, 1))
     keys = np.sort((i.astype(np.uint64) << np.uint64(32)) | j.astype(np.uint64))
     assert np.array_equal(a.pairs(), keys)
@@ -336,3 +348,22 @@ def test_mix64_is_a_bijection_sample(oracle):
     keys = np.arange(20000, dtype=np.uint64) * np.uint64(0x100000001)
     vals = {oracle.mix64(int(k)) for k in keys}
     assert len(vals) == len(keys)
+
+
+@pytest.mark.parametrize("threads", [2, 5])
+def test_multithreaded_oracle_is_bit_identical(oracle, threads):
+    """The multithreaded step (body stages and pair updates in parallel, the
+    solver colour by colour) computes the same bits as the single-threaded one."""
+    from helpers import oracle_hulls
+    sc = scenes.random_pile(150, seed=4, shapes=(0, 1, 2, 3))
+    a, b = oracle.World(), oracle.World()
+    a.add_scene(sc, oracle_hulls())
+    b.add_scene(sc, oracle_hulls())
+    b.set_threads(threads)
+    a.step(40)
+    b.step(40)
+    assert np.array_equal(a.motion_states().view(np.uint32), b.motion_states().view(np.uint32))
+    assert np.array_equal(a.pairs(), b.pairs())
+    ia = {int(r.key): bytes(r) for r in a.impulses()}
+    ib = {int(r.key): bytes(r) for r in b.impulses()}
+    assert ia == ib
