@@ -182,9 +182,12 @@ constexpr uint32_t kConvexMaxGrid = 2048;
 }  // namespace
 
 __global__ void __launch_bounds__(kConvexBlock) k_convex(DeviceWorld w) {
-    uint32_t n = w.counters[ctr(CTR_CONVEX)];
+    // The three state regions of the work list, back to back.
+    uint32_t n0 = w.counters[ctr(CTR_CONVEX)], n1 = w.counters[ctr(CTR_CONVEX_SHALLOW)];
+    uint32_t n = n0 + n1 + w.counters[ctr(CTR_CONVEX_DEEP)];
     for (uint32_t i = blockIdx.x * kConvexBlock + threadIdx.x; i < n; i += gridDim.x * kConvexBlock) {
-        uint2 wk = w.conv_work[i];
+        size_t at = i < n0 ? i : (i < n0 + n1 ? (size_t)w.pair_cap + (i - n0) : 2 * (size_t)w.pair_cap + (i - n0 - n1));
+        uint2 wk = w.conv_work[at];
         // The pair's state is updated in place in global memory: its contact
         // slots are indexed at run time, which would put a private copy in
         // scratch memory.

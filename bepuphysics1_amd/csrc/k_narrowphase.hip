@@ -644,15 +644,24 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
         // keeps this kernel's register budget for the closed-form pairs.
         w.pairs_cur[t] = s;
         if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
-        // Append (t, old) to the work list: one atomic per wavefront's convex lanes.
-        uint64_t active = __ballot(1);
+        // Append (t, old) to the work list of the pair tester's state (new
+        // pairs start Separated), so that k_convex's wavefronts mostly run one
+        // branch of the state machine: one atomic per wavefront and state.
+        int cstate = old >= 0 ? __float_as_int(w.conv_old[old].sep.w) : 0;
+        if (cstate != 1 && cstate != 2) cstate = 0;
         int lane = __lane_id();
-        int leader = __ffsll((unsigned long long)active) - 1;
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&w.counters[ctr(CTR_CONVEX)], (uint32_t)__popcll(active));
-        base = __shfl(base, leader);
-        w.conv_work[base + (uint32_t)__popcll(active & ((1ull << lane) - 1ull))] =
-            make_uint2(t, old >= 0 ? (uint32_t)old : ~0u);
+        for (int q = 0; q < 3; ++q) {
+            uint64_t mine = __ballot(cstate == q);
+            if (!mine) continue;
+            int leader = __ffsll((unsigned long long)mine) - 1;
+            uint32_t base = 0;
+            uint32_t c = q == 0 ? CTR_CONVEX : (q == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP);
+            if (lane == leader) base = atomicAdd(&w.counters[ctr(c)], (uint32_t)__popcll(mine));
+            base = __shfl(base, leader);
+            if (cstate == q)
+                w.conv_work[(size_t)q * w.pair_cap + base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull))] =
+                    make_uint2(t, old >= 0 ? (uint32_t)old : ~0u);
+        }
         return;
     }
     if (s.kind == BEPU_PAIR_BOX_BOX) {

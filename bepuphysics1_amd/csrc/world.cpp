@@ -181,8 +181,8 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     uint32_t P = d.pair_cap;
     A(&d.pair_keys, P); A(&d.counters, (size_t)CTR_COUNT * kCtrStride);
     A(&d.pairs_cur, P); A(&d.pairs_old, P);
-    // General convex pair state: 2 x 160 B + 8 B per pair slot.
-    A(&d.conv_cur, P); A(&d.conv_old, P); A(&d.conv_work, P);
+    // General convex pair state: 2 x 160 B + 3 x 8 B per pair slot.
+    A(&d.conv_cur, P); A(&d.conv_old, P); A(&d.conv_work, 3 * (size_t)P);
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
     A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
     A(&d.cons, P);

@@ -128,7 +128,8 @@ struct DeviceWorld {
     PairState* pairs_old;      // last step's
     ConvexState* conv_cur;     // general convex pairs: this step's extra state (index = overlap index)
     ConvexState* conv_old;     // last step's
-    uint2* conv_work;          // this step's general convex pairs: (overlap index, last step's index or ~0)
+    uint2* conv_work;          // this step's general convex pairs: (overlap index, last step's index or ~0),
+                               // in three regions of pair_cap entries by the pair tester's state
     uint64_t* hash_keys_cur;
     uint32_t* hash_vals_cur;
     uint64_t* hash_keys_old;
@@ -175,7 +176,9 @@ enum {
     CTR_MAXPEN = 4,       // float bits (non-negative), max via atomicMax on uint
     CTR_OVERFLOW = 5,     // pair/hash capacity overflow flag
     CTR_ROUND = 6,        // colouring rounds
-    CTR_CONVEX = 7,       // general convex pairs queued for k_convex this step
+    CTR_CONVEX = 7,       // general convex pairs queued for k_convex this step: Separated,
+    CTR_CONVEX_SHALLOW = 12,  // Shallow and
+    CTR_CONVEX_DEEP = 14,     // Deep tester state
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
     CTR_GCONS = 11,       // constraints on the global path
     CTR_ISL_MEDIUM = 13,
