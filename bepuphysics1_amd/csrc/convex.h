@@ -118,11 +118,18 @@ BEPU_D f3 mk_extreme(const Shape& a, const Shape& b, f3 d, const RT& lb) {
 }
 
 // GetLocalMinkowskiExtremePoint with the A-side point and the per-shape
-// ExpandMinkowskiSum (MinkowskiToolbox.cs:63-80, 160-176).
-BEPU_D f3 mk_extreme_a(const Shape& a, const Shape& b, f3 d, const RT& lb, f3* ea) {
+// ExpandMinkowskiSum (MinkowskiToolbox.cs:63-80, 160-176) when `per_shape`,
+// else the first overload (as mk_extreme).  The two round differently; one
+// function keeps a single copy of the support scans in the surface cast.
+BEPU_D f3 mk_extreme_sel(const Shape& a, const Shape& b, f3 d, const RT& lb, bool per_shape, f3* ea) {
     f3 pa = support_local(a, d);
     f3 pb = extreme_without_margin(b, -d, lb);
     float ls = lensq3(d);
+    if (!per_shape) {
+        f3 e = pa - pb;
+        f3 v = ls > kEpsilon ? d * ((a.margin + b.margin) / sqrtf(ls)) : zero3();
+        return e + v;
+    }
     if (ls > kEpsilon) {
         ls = 1 / sqrtf(ls);
         pa = pa + d * (a.margin * ls);
@@ -712,11 +719,11 @@ BEPU_D void barycentric(f3 p, f3 a, f3 b, f3 c, float* wa, float* wb, float* wc)
 // MPRToolbox.LocalSurfaceCast, both overloads (MPRToolbox.cs:610-880 and
 // 886-1172): `with_position` selects the overload that expands each shape's
 // support point separately and interpolates the contact position.
-template <bool with_position>
-BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, float* t, f3* normal, f3* position) {
+BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, bool with_position, float* t, f3* normal,
+                         f3* position) {
     f3 v1A = zero3(), v2A = zero3(), v3A = zero3(), v4A = zero3();
     f3 n = dir;
-    f3 v1 = with_position ? mk_extreme_a(a, b, n, lb, &v1A) : mk_extreme(a, b, n, lb);
+    f3 v1 = mk_extreme_sel(a, b, n, lb, with_position, &v1A);
     n = cross3(dir, v1);
     if (lensq3(n) < kEpsilon) {
         float rl = lensq3(dir);
@@ -727,7 +734,7 @@ BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, f
         if (with_position) *position = v1A;
         return;
     }
-    f3 v2 = with_position ? mk_extreme_a(a, b, n, lb, &v2A) : mk_extreme(a, b, n, lb);
+    f3 v2 = mk_extreme_sel(a, b, n, lb, with_position, &v2A);
     n = cross3(v1, v2);
     if (dot3(n, dir) > 0) {
         n = -n;
@@ -741,7 +748,7 @@ BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, f
     f3 v3;
     int count = 0;
     while (true) {
-        v3 = with_position ? mk_extreme_a(a, b, n, lb, &v3A) : mk_extreme(a, b, n, lb);
+        v3 = mk_extreme_sel(a, b, n, lb, with_position, &v3A);
         if (count > kOuterIterationLimit) {
             *t = FLT_MAX;
             *normal = f3{0, 1, 0};
@@ -766,7 +773,7 @@ BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, f
     count = 0;
     while (true) {
         n = cross3(v1 - v2, v3 - v2);
-        f3 v4 = with_position ? mk_extreme_a(a, b, n, lb, &v4A) : mk_extreme(a, b, n, lb);
+        f3 v4 = mk_extreme_sel(a, b, n, lb, with_position, &v4A);
         float d = dot3(n, v1);
         float sd = dot3(v4, n);
         if (sd - d < kSurfaceEpsilon || count > kInnerIterationLimit) {
@@ -818,37 +825,49 @@ BEPU_D bool mpr_contact(const Shape& a, const Shape& b, const RT& ta, const RT& 
                         float* depth) {
     RT lb = local_transform(ta, tb);
     if (!mpr_overlapping(a, b, lb)) return false;
-    float dep;
-    f3 normal;
-    float ls = lensq3(axis);
-    if (ls > kEpsilon) {
-        surface_cast<false>(a, b, lb, divide3(axis, sqrtf(ls)), &dep, &normal, nullptr);
-    } else {
-        dep = FLT_MAX;
-        normal = f3{0, 1, 0};
-    }
-    ls = lensq3(lb.pos);
-    if (ls > kEpsilon) {
-        float dc;
-        f3 nc;
-        surface_cast<false>(a, b, lb, divide3(lb.pos, sqrtf(ls)), &dc, &nc, nullptr);
-        if (dc < dep) {
-            normal = nc;
-            dep = dc;
-        }
-    }
-    f3 p;
-    int count = 0;
+    // The casts of GetContact run through one call site: phase 0 casts along
+    // the given axis, phase 1 along the origin offset (keeping the shallower
+    // result), then RefinePenetration casts along the current normal with the
+    // contact position until the depth settles or three refinements ran.
+    float dep = FLT_MAX;
+    f3 normal = f3{0, 1, 0};
+    f3 p = zero3();
+    int phase = 0, count = 0;
     while (true) {
-        float cd;
-        f3 cn;
-        surface_cast<true>(a, b, lb, normal, &cd, &cn, &p);
-        if (dep - cd <= kDepthRefinementEpsilon || ++count >= kMaxDepthRefinementIterations) {
-            dep = cd;
-            break;
+        f3 dir;
+        if (phase < 2) {
+            f3 src = phase == 0 ? axis : lb.pos;
+            float ls = lensq3(src);
+            if (!(ls > kEpsilon)) {
+                ++phase;
+                continue;
+            }
+            dir = divide3(src, sqrtf(ls));
+        } else {
+            dir = normal;
         }
-        dep = cd;
-        normal = cn;
+        float cd;
+        f3 cn, cp;
+        surface_cast(a, b, lb, dir, phase == 2, &cd, &cn, &cp);
+        if (phase == 0) {
+            dep = cd;
+            normal = cn;
+            phase = 1;
+        } else if (phase == 1) {
+            if (cd < dep) {
+                normal = cn;
+                dep = cd;
+            }
+            phase = 2;
+        } else {
+            p = cp;
+            if (dep - cd <= kDepthRefinementEpsilon || ++count >= kMaxDepthRefinementIterations) {
+                dep = cd;
+                break;
+            }
+            dep = cd;
+            normal = cn;
+        }
     }
     m3 o = mat_from_quat(ta.q);
     *nrm = mat_transform(normal, o);
@@ -885,32 +904,39 @@ BEPU_D bool deep_contact(const Shape& a, const Shape& b, const RT& ta, const RT&
 BEPU_D bool candidate(const Shape& a, const Shape& b, const RT& ta, const RT& tb, f3 va, f3 vb, TesterState& st,
                       f3* pos, f3* nrm, float* depth, int* id) {
     int previous = st.state;
-    if (st.state == CT_DEEP) return deep_contact(a, b, ta, tb, va, vb, previous, st, pos, nrm, depth, id);
-    if (st.state == CT_SEPARATED) {
-        if (!gjk_intersecting(a, b, ta, tb, &st.separating_axis)) return false;
-        st.state = CT_SHALLOW;
+    // The deep test has one call site (entered from the Deep state or when the
+    // closest-point test finds the cores intersecting), which keeps one copy
+    // of the MPR code in the kernel.
+    bool deep = st.state == CT_DEEP;
+    if (!deep) {
+        if (st.state == CT_SEPARATED) {
+            if (!gjk_intersecting(a, b, ta, tb, &st.separating_axis)) return false;
+            st.state = CT_SHALLOW;
+        }
+        f3 ca, cb;
+        gjk_closest_points(a, b, ta, tb, &ca, &cb);
+        f3 disp = cb - ca;
+        float d2 = lensq3(disp);
+        if (d2 < kEpsilon) {
+            st.state = CT_DEEP;
+            deep = true;
+        } else {
+            st.local_direction = disp;
+            float margin = a.margin + b.margin;
+            if (d2 < margin * margin) {
+                f3 p = margin > kEpsilon ? disp * (a.margin / margin) : zero3();
+                *pos = ca + p;
+                float distance = sqrtf(d2);
+                *nrm = divide3(disp, distance);
+                *depth = margin - distance;
+                *id = 0;
+                return true;
+            }
+            st.state = CT_SEPARATED;
+            return false;
+        }
     }
-    f3 ca, cb;
-    gjk_closest_points(a, b, ta, tb, &ca, &cb);
-    f3 disp = cb - ca;
-    float d2 = lensq3(disp);
-    if (d2 < kEpsilon) {
-        st.state = CT_DEEP;
-        return deep_contact(a, b, ta, tb, va, vb, previous, st, pos, nrm, depth, id);
-    }
-    st.local_direction = disp;
-    float margin = a.margin + b.margin;
-    if (d2 < margin * margin) {
-        f3 p = margin > kEpsilon ? disp * (a.margin / margin) : zero3();
-        *pos = ca + p;
-        float distance = sqrtf(d2);
-        *nrm = divide3(disp, distance);
-        *depth = margin - distance;
-        *id = 0;
-        return true;
-    }
-    st.state = CT_SEPARATED;
-    return false;
+    return deep_contact(a, b, ta, tb, va, vb, previous, st, pos, nrm, depth, id);
 }
 
 }  // namespace cvx
