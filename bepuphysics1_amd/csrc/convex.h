@@ -59,18 +59,25 @@ BEPU_D f3 support_local(const Shape& s, f3 d) {
         return zero3();
     }
     if (s.kind == BEPU_SHAPE_CONVEX_HULL) {
-        float4 v = s.verts[0];
-        float mx = v.x * d.x + v.y * d.y + v.z * d.z;
-        int mi = 0;
-        for (int i = 1; i < s.nverts; i++) {
-            float4 u = s.verts[i];
-            float dd = u.x * d.x + u.y * d.y + u.z * d.z;
-            if (dd > mx) {
-                mx = dd;
-                mi = i;
+        // The first vertex of largest dot product (ties keep the earlier one).
+        // Vertices are loaded eight at a time so their latencies overlap.
+        constexpr int kChunk = 8;
+        float mx = 0;
+        f3 best = zero3();
+        for (int i0 = 0; i0 < s.nverts; i0 += kChunk) {
+            float4 v[kChunk];
+#pragma unroll
+            for (int k = 0; k < kChunk; ++k) v[k] = s.verts[min(i0 + k, s.nverts - 1)];
+#pragma unroll
+            for (int k = 0; k < kChunk; ++k) {
+                float dd = v[k].x * d.x + v[k].y * d.y + v[k].z * d.z;
+                if (i0 + k < s.nverts && (i0 + k == 0 || dd > mx)) {
+                    mx = dd;
+                    best = f3_of(v[k]);
+                }
             }
         }
-        return f3_of(s.verts[mi]);
+        return best;
     }
     return zero3();
 }
