@@ -284,10 +284,11 @@ __global__ void __launch_bounds__(256) k_step_reset(DeviceWorld w) {
     }
     if (i < CTR_COUNT) w.counters[ctr(i)] = 0;
     if (i < w.nworlds) w.wlevels[i] = 0;
+    if (i < 16 * w.nworlds) w.wlev_count[i] = 0;
 }
 
 void launch_step_reset(const DeviceWorld& w, hipStream_t st) {
-    uint32_t n = std::max<uint32_t>(std::max<uint32_t>(w.hash_cap, w.gt_cap), std::max<uint32_t>(CTR_COUNT, w.nworlds));
+    uint32_t n = std::max<uint32_t>(std::max<uint32_t>(w.hash_cap, w.gt_cap), std::max<uint32_t>(CTR_COUNT, 16 * w.nworlds));
     hipLaunchKernelGGL(k_step_reset, dim3((n + 255) / 256), dim3(256), 0, st, w);
 }
 

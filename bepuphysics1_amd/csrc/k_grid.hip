@@ -17,8 +17,9 @@
 // A body therefore finds all its pairs by probing its own cell and the 13
 // "forward" neighbour cells at its level (a same-level pair is reported once:
 // through the forward neighbour, or by the smaller sorted index inside one
-// cell) and the 27 cells around its min corner at every coarser non-empty
-// level of its world (a cross-level pair is reported by the finer body).  Cell indices are
+// cell) and, at every coarser non-empty level of its world, the 8 to 27
+// cells from the one below its min corner to the one holding its max corner
+// (a cross-level pair is reported by the finer body).  Cell indices are
 // computed in double precision, so the one-cell bound holds for coordinates
 // up to ~1e13 cell edges from the origin.
 //
@@ -101,6 +102,27 @@ __global__ void __launch_bounds__(256) k_grid_insert(DeviceWorld w) {
         if (slot != ~0u) w.gnext[b] = atomicExch(&w.gt_head[slot], b);
         else w.counters[ctr(CTR_OVERFLOW)] = 6;
         lmask = 1u << l;
+    }
+    // Bodies per world and level, and the first kLevelList of them: lanes with
+    // one (world, level) combine into one atomic (a world's bodies have
+    // consecutive ids, so a wavefront holds few distinct keys).
+    {
+        const int lane = __lane_id();
+        uint32_t key = alive ? world * 16u + (uint32_t)(31 - __clz(lmask)) : ~0u;
+        uint64_t pending = __ballot(alive);
+        while (pending) {
+            int lead = __ffsll((unsigned long long)pending) - 1;
+            uint32_t k0 = __shfl(key, lead);
+            uint64_t same = __ballot(key == k0) & pending;
+            uint32_t base = 0;
+            if (lane == lead) base = atomicAdd(&w.wlev_count[k0], (uint32_t)__popcll(same));
+            base = __shfl(base, lead);
+            if ((same >> lane) & 1ull) {
+                uint32_t idx = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+                if (idx < kLevelList) w.wlev_list[k0 * kLevelList + idx] = b;
+            }
+            pending &= ~same;
+        }
     }
     // Levels used per world: lanes of the first live lane's world combine
     // their masks into one atomic (a world's bodies have consecutive ids).
@@ -185,28 +207,49 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
 #pragma unroll
             for (int o = 0; o < 14; ++o) probe_cell(w, q, pk[o], first[o], ps[o], o == 0, buf, &buf_n);
         }
-        // Coarser non-empty levels: the 27 cells around the min corner.
+        // Coarser non-empty levels: the bodies themselves when the level holds
+        // at most kLevelList of the world's bodies (a pile's ground), else the
+        // cells a coarser body overlapping this one can sit in.
         uint32_t coarse = w.wlevels[world] & ~((2u << l) - 1u);
         const float s0 = __uint_as_float(w.bounds[6]);
         while (coarse) {
             const int l2 = __ffs(coarse) - 1;
             coarse &= coarse - 1;
+            const uint32_t wl = world * 16u + (uint32_t)l2;
+            const uint32_t cnt = w.wlev_count[wl];
+            if (cnt <= kLevelList) {
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    uint32_t c = w.wlev_list[wl * kLevelList + k];
+                    float4 cmin = w.bmin[c], cmax = w.bmax[c];
+                    bool hit = (q.dyn || (w.bflags[c] & 1u)) && overlap(q.qmin, q.qmax, cmin, cmax);
+                    if (hit) buf_push(w, buf, buf_n, kQueryBuf, true, pair_key(q.body, c));
+                }
+                continue;
+            }
+            // A coarser body overlapping this one has its min corner in
+            // [qmin - edge, qmax] per axis, so its cell lies between the cell
+            // below the query's min corner and the cell of its max corner: 2
+            // cells per axis, or 3 when the query straddles a cell boundary.
             const double e = level_edge(s0, l2);
-            const uint64_t cx = (uint64_t)cell_of(q.qmin.x, e), cy = (uint64_t)cell_of(q.qmin.y, e),
-                           cz = (uint64_t)cell_of(q.qmin.z, e);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy) {
+            const int64_t x0 = cell_of(q.qmin.x, e), y0 = cell_of(q.qmin.y, e), z0 = cell_of(q.qmin.z, e);
+            // (Clamped: a non-finite box must not drive the loop bounds.)
+            const int nx = (int)min(max(cell_of(q.qmax.x, e) - x0, (int64_t)0), (int64_t)1) + 2,
+                      ny = (int)min(max(cell_of(q.qmax.y, e) - y0, (int64_t)0), (int64_t)1) + 2,
+                      nz = (int)min(max(cell_of(q.qmax.z, e) - z0, (int64_t)0), (int64_t)1) + 2;
+            for (int dz = 0; dz < nz; ++dz)
+                for (int dy = 0; dy < ny; ++dy) {
                     uint64_t pk[3], first[3];
                     uint32_t ps[3];
 #pragma unroll
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        pk[dx + 1] = cell_key(l2, world, cx + (uint64_t)(int64_t)dx, cy + (uint64_t)(int64_t)dy,
-                                              cz + (uint64_t)(int64_t)dz);
-                        ps[dx + 1] = (uint32_t)mix64(pk[dx + 1]) & mask;
-                        first[dx + 1] = w.gt_key[ps[dx + 1]];
+                    for (int dx = 0; dx < 3; ++dx) {
+                        pk[dx] = cell_key(l2, world, (uint64_t)(x0 - 1 + dx), (uint64_t)(y0 - 1 + dy),
+                                          (uint64_t)(z0 - 1 + dz));
+                        ps[dx] = (uint32_t)mix64(pk[dx]) & mask;
+                        first[dx] = dx < nx ? w.gt_key[ps[dx]] : kEmptyCell;
                     }
 #pragma unroll
-                    for (int o = 0; o < 3; ++o) probe_cell(w, q, pk[o], first[o], ps[o], false, buf, &buf_n);
+                    for (int o = 0; o < 3; ++o)
+                        if (o < nx) probe_cell(w, q, pk[o], first[o], ps[o], false, buf, &buf_n);
                 }
         }
     }

@@ -172,7 +172,9 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.pos, n); A(&d.quat, n); A(&d.linv, n); A(&d.angv, n);
     A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
-    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, n); A(&d.born, n);
+    // World ids stay below the number of worlds alive at once (ids are reused), at most n + 1.
+    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, n + 1); A(&d.born, n);
+    A(&d.wlev_count, 16 * ((size_t)n + 1)); A(&d.wlev_list, 16 * kLevelList * ((size_t)n + 1));
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
     A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
@@ -674,8 +676,8 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (st != BEPU_OK) return st;
     DeviceWorld& d = w->d;
     hipStream_t s = w->stream;
-    // Stage timers: every step of this call when timing is on, else the last step only.
-    uint32_t timed = w->timing ? (uint32_t)n_steps : (n_steps > 0 ? 1u : 0u);
+    // Stage timers: every step of this call when timing is on, none otherwise.
+    uint32_t timed = w->timing ? (uint32_t)n_steps : 0u;
     size_t need = (size_t)timed * (ST_COUNT + 1);
     if (need > 0 && hipStreamSynchronize(s) != hipSuccess) return BEPU_E_DEVICE;
     while (w->ev.size() < need) {

@@ -100,6 +100,8 @@ struct DeviceWorld {
     uint32_t* wid;             // sub-world of every body (bodies of different sub-worlds never interact)
     uint32_t* wbase;           // first body id of the body's sub-world
     uint32_t* wlevels;         // per sub-world: grid levels holding at least one of its bodies
+    uint32_t* wlev_count;      // per sub-world and grid level (16 each): bodies at that level
+    uint32_t* wlev_list;       // the first kLevelList of them (a sparse coarse level is tested directly)
     uint32_t nworlds;
     uint32_t* born;            // index of the first step a body takes part in
     uint32_t step_no;          // index of the step being run
@@ -165,6 +167,7 @@ struct DeviceWorld {
 };
 
 constexpr int kMaxColors = 128;
+constexpr uint32_t kLevelList = 4;
 constexpr uint32_t kMaxTreeLevels = 32;
 
 // counters[] indices

@@ -277,8 +277,9 @@ bepu_status bepu_get_impulses(bepu_world world, bepu_impulses* out, uint32_t cap
 bepu_status bepu_get_stats(bepu_world world, bepu_step_stats* out);
 
 /* Stage timers.  With timing enabled every step of a bepu_step call records
- * HIP events between its stages on the world's stream; otherwise only the
- * last step does.  bepu_get_stage_times returns the average device time per
+ * HIP events between its stages on the world's stream (the events cost device
+ * time of their own); with timing off no events are recorded and
+ * bepu_get_stage_times reports the last timed call.  It returns the average device time per
  * stage (milliseconds) over those steps; `names` receives a pointer to a
  * static, comma separated list of stage names matching `ms`.  No reference
  * counterpart: it plays the role of the reference's PROFILE stage timers
