@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
                     uint32_t c = w.wlev_list[wl * kLevelList + k];
                     float4 cmin = w.bmin[c], cmax = w.bmax[c];
                     bool hit = (q.dyn || (w.bflags[c] & 1u)) && overlap(q.qmin, q.qmax, cmin, cmax);
-                    if (hit) buf_push(w, buf, buf_n, kQueryBuf, true, pair_key(q.body, c));
+                    if (hit) buf_push(w, buf, &buf_n, kQueryBuf, true, pair_key(q.body, c));
                 }
                 continue;
             }

@@ -1,7 +1,7 @@
 // k_solver.hip -- global solver path for islands too large for one
 // workgroup's LDS (the island kernels in k_island.hip take all others).
 //
-// One cooperative kernel, phases separated by grid barriers:
+// One persistent kernel, phases separated by grid barriers:
 //   1. body -> constraint incidence counts (dynamic bodies only) and
 //      priorities mix64(world-local key);
 //   2. exclusive scan of the counts (per-workgroup partials, then every
@@ -18,17 +18,64 @@
 //      written back to the persistent pair state.
 // With no constraint on this path the kernel returns before its first
 // barrier (every workgroup reads the same count), so an empty path costs one
-// launch.
-#include <hip/hip_cooperative_groups.h>
-
+// ordinary launch.
+//
+// The grid barrier is a counter, not a cooperative launch: a cooperative
+// launch costs about 20 us on MI355X whatever the grid, paid every step even
+// when the path is empty.  The grid is sized from the kernel's occupancy so
+// that all its workgroups are resident together on an idle device; the wait
+// at a barrier is bounded in wall-clock time, so a grid that is not resident
+// (another stream holding compute units) ends the kernel with an error flag
+// instead of hanging.
 #include "solver_rows.h"
-
-namespace cg = cooperative_groups;
 
 namespace bepu {
 
 constexpr int kSolveBlock = 256;
-constexpr int kMaxGrid = 4096;   // workgroups of the cooperative grid (scan partials)
+constexpr int kMaxGrid = 4096;   // workgroups of the solver grid (scan partials)
+
+// Grid barrier over a resident grid.  Arrivals count up in CTR_GBAR (cleared
+// by the step reset; the kernel runs once per step); barrier k is passed when
+// (k + 1) * gridDim.x workgroups have arrived.  Every wave fences at agent
+// scope on both sides, so all writes before the barrier are seen after it.
+// Returns false when the kernel must stop: this wait, or another workgroup's,
+// ran past `limit` wall-clock ticks.
+struct GridBarrier {
+    uint32_t* count;
+    uint32_t* abort;
+    uint32_t* overflow;
+    uint64_t limit;
+    uint32_t phase;
+};
+
+__device__ __forceinline__ bool grid_barrier(GridBarrier& gb, uint32_t* s_ok) {
+    __threadfence();
+    __syncthreads();
+    gb.phase += 1;
+    if (threadIdx.x == 0) {
+        const uint32_t target = gb.phase * gridDim.x;
+        atomicAdd(gb.count, 1u);
+        const uint64_t t0 = (uint64_t)wall_clock64();
+        uint32_t ok = 1;
+        while ((int32_t)(__hip_atomic_load(gb.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+            if (__hip_atomic_load(gb.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if ((uint64_t)wall_clock64() - t0 > gb.limit) {
+                atomicExch(gb.abort, 1u);
+                atomicExch(gb.overflow, 8u);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    __threadfence();
+    return *s_ok != 0;
+}
 
 __device__ __forceinline__ uint32_t gcount(const DeviceWorld& w) {
     return min(w.counters[ctr(CTR_GCONS)], w.pair_cap);
@@ -92,13 +139,14 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* s) {
     return r;
 }
 
-__global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
+__global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w, uint64_t wait_limit) {
     __shared__ uint32_t s_scan[kSolveBlock];
     __shared__ uint32_t s_hist[kMaxColors], s_base[kMaxColors], s_start[kMaxColors + 1];
-    __shared__ uint32_t s_carry;
-    cg::grid_group grid = cg::this_grid();
+    __shared__ uint32_t s_carry, s_ok;
     const uint32_t m = gcount(w);
     if (m == 0) return;  // uniform across the grid: no barrier is skipped by a subset
+    GridBarrier gb{&w.counters[ctr(CTR_GBAR)], &w.counters[ctr(CTR_GBAR_ABORT)], &w.counters[ctr(CTR_OVERFLOW)],
+                   wait_limit, 0};
     const uint32_t G = gridDim.x, gsize = G * blockDim.x;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = w.nbodies;
@@ -111,7 +159,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         w.prio[c] = mix64(local_pair_key(w, s.key, s.a));
         w.color[c] = -1;
     }
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
 
     // 2. exclusive scan of inc_count[0..n] (inc_count[n] is zero) into inc_offset
     const uint32_t N = n + 1, chunk = (N + G - 1) / G;
@@ -122,7 +170,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         uint32_t tot = block_scan_incl(part, s_scan);
         if (threadIdx.x == kSolveBlock - 1) w.scan_part[blockIdx.x] = tot;
     }
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
     {
         uint32_t before = 0;
         for (uint32_t b = threadIdx.x; b < blockIdx.x; b += blockDim.x) before += w.scan_part[b];
@@ -139,7 +187,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
             __syncthreads();
         }
     }
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
 
     // 3. incidence lists (counting down leaves inc_count at zero for the next step)
     for (uint32_t c = tid; c < m; c += gsize) {
@@ -147,7 +195,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         if (w.bflags[s.a] & 1u) w.inc_list[w.inc_offset[s.a] + atomicSub(&w.inc_count[s.a], 1u) - 1] = c;
         if (w.bflags[s.b] & 1u) w.inc_list[w.inc_offset[s.b] + atomicSub(&w.inc_count[s.b], 1u) - 1] = c;
     }
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
 
     // 4. colouring rounds.  Each round colours at least the highest-priority
     // uncoloured constraint, so m + 1 rounds always suffice.  Three rotating
@@ -168,7 +216,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         }
         for (int off = 32; off > 0; off >>= 1) left += __shfl_xor(left, off);
         if (__lane_id() == 0 && left) atomicAdd(remaining, left);
-        grid.sync();
+        if (!grid_barrier(gb, &s_ok)) return;
         if (__hip_atomic_load(remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
     }
 
@@ -179,7 +227,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
     __syncthreads();
     for (int k = threadIdx.x; k < kMaxColors; k += blockDim.x)
         if (s_hist[k]) atomicAdd(&w.color_count[k], s_hist[k]);
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
     if (threadIdx.x == 0) {
         uint32_t acc = 0;
         for (int k = 0; k < kMaxColors; ++k) {
@@ -204,7 +252,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         if (have) w.ordered[s_start[col] + s_base[col] + rank] = c;
         __syncthreads();
     }
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
     // Counts and cursors are no longer read: clear them for the next step.
     if (blockIdx.x == 0)
         for (int k = threadIdx.x; k < kMaxColors; k += blockDim.x) {
@@ -226,7 +274,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         w.grow_v[p] = v;
         w.grow_pair[p] = slot;
     }
-    grid.sync();
+    if (!grid_barrier(gb, &s_ok)) return;
 
     // 7. warm start and iterations, colour by colour
     int ncol = 0;
@@ -234,19 +282,20 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w) {
         if (s_start[k + 1] > s_start[k]) ncol = k + 1;
     for (int col = 0; col < ncol; ++col) {
         for (uint32_t p = s_start[col] + tid; p < s_start[col + 1]; p += gsize) global_phase(w, p, true);
-        grid.sync();
+        if (!grid_barrier(gb, &s_ok)) return;
     }
     for (int it = 0; it < w.s.iteration_limit; ++it)
         for (int col = 0; col < ncol; ++col) {
             for (uint32_t p = s_start[col] + tid; p < s_start[col + 1]; p += gsize) global_phase(w, p, false);
-            grid.sync();
+            if (!grid_barrier(gb, &s_ok)) return;
         }
     for (uint32_t p = tid; p < m; p += gsize) writeback_row(w.pairs_cur[w.grow_pair[p]], w.grow_v[p]);
 }
 
 // ---------------------------------------------------------------------------
 
-static int cooperative_grid(const void* kernel, int block) {
+// Workgroups that fit on the device at once (at most two per compute unit).
+static int resident_grid(const void* kernel, int block) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     hipDeviceProp_t prop;
@@ -260,10 +309,15 @@ static int cooperative_grid(const void* kernel, int block) {
 
 void launch_global_solver(const DeviceWorld& w, hipStream_t st) {
     static int g = 0;
-    if (!g) g = cooperative_grid((const void*)k_global, kSolveBlock);
-    DeviceWorld wc = w;
-    void* args[] = {&wc};
-    (void)hipLaunchCooperativeKernel((const void*)k_global, dim3(g), dim3(kSolveBlock), args, 0, st);
+    static uint64_t limit = 0;
+    if (!g) {
+        g = resident_grid((const void*)k_global, kSolveBlock);
+        int dev = 0, khz = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+        limit = (uint64_t)khz * 1000u;  // one second per barrier wait
+    }
+    hipLaunchKernelGGL(k_global, dim3(g), dim3(kSolveBlock), 0, st, w, limit);
 }
 
 }  // namespace bepu

@@ -882,6 +882,7 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     std::memcpy(&mp, &c[CTR_MAXPEN], 4);
     out->max_penetration = any ? mp : 0.0f;
     out->global_manifolds = any ? std::min(c[CTR_GCONS], w->d.pair_cap) : 0;
+    if (c[CTR_GBAR_ABORT]) return BEPU_E_DEVICE;  // the global solver's grid was not resident
     return c[CTR_OVERFLOW] ? BEPU_E_CAPACITY : BEPU_OK;
 }
 

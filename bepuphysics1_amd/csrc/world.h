@@ -185,9 +185,11 @@ enum {
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
     CTR_GCONS = 11,       // constraints on the global path
     CTR_ISL_MEDIUM = 13,
+    CTR_GBAR = 15,        // arrivals at the global solver's grid barriers
     CTR_CLS0 = 16,        // 16..79: small islands per packing class
     CTR_SIDE0 = 80,       // 80..143: bodies per half-octave bin of the largest AABB side
-    CTR_COUNT = 144
+    CTR_GBAR_ABORT = 144, // set when a grid barrier wait ran out of time
+    CTR_COUNT = 145
 };
 
 // Every counter sits on its own 256-byte line: device atomics on one line
