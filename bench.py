@@ -11,13 +11,16 @@ The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 average duration (HIP events on the world's stream), against
                 the MI355X HBM peak of 8 TB/s;
-  cpu_baseline  the oracle (this repository's single-threaded C++
-                restatement of the reference path) timed on the host on a
-                bounded sample of the same scene;
+  cpu_baseline  the oracle (this repository's C++ restatement of the
+                reference path) timed on up to 16 host threads, and on one,
+                on a bounded sample of the same scene;
   parity        a small scene stepped on both sides: pair sets and impulses;
-  latency       p50 / p99 of single synchronised steps.
+  latency       p50 / p99 of single synchronised steps;
+  batched, served, stress
+                the many-world batch, continuous batching of rollouts and the
+                broadphase/narrowphase stress scene.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs towers,batched,served,stress]
 """
 import argparse
 import json
@@ -77,7 +80,7 @@ STAGE_KERNELS = {
     "island_solve": ["bepu::k_island_pack(bepu::DeviceWorld)", "bepu::k_island_medium(bepu::DeviceWorld)"],
     "narrowphase": ["bepu::k_narrowphase(bepu::DeviceWorld)"],
     "broadphase": ["bepu::k_grid_insert(bepu::DeviceWorld)", "bepu::k_grid_query(bepu::DeviceWorld)"],
-    "global_solve": ["bepu::k_global(bepu::DeviceWorld)"],
+    "global_solve": ["bepu::k_global(bepu::DeviceWorld, unsigned long)"],
 }
 
 
@@ -172,6 +175,13 @@ def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
     stages = sp.stage_times()
     sp.close()
     batched = n_dyn * steps / dt
+    out = {"workload": "%d worlds, N ~ logU[8,256] boxes + spheres + 16-point hulls on own grounds, dt 1/60, "
+                       "8 iterations" % n_worlds,
+           "bodies": n_dyn, "steps": steps, "value": round(batched, 1), "ms_per_step": round(1e3 * dt / steps, 4),
+           "pairs": stats.pairs, "manifolds": stats.manifolds,
+           "stages_ms": {k: round(v, 4) for k, v in stages.items()}}
+    if sample_worlds <= 0:
+        return out
     # Plain path: every world alone, one step after another.
     rng = np.random.default_rng(1)
     pick = rng.choice(n_worlds, size=min(sample_worlds, n_worlds), replace=False)
@@ -190,14 +200,10 @@ def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
         b_sum += int((worlds[k]["mass"] > 0).sum())
         one.close()
     plain = b_sum / t_sum  # body-steps/s when worlds are stepped one at a time
-    return {"workload": "%d worlds, N ~ logU[8,256] boxes + spheres + 16-point hulls on own grounds, dt 1/60, "
-                        "8 iterations" % n_worlds,
-            "bodies": n_dyn, "steps": steps, "value": round(batched, 1), "ms_per_step": round(1e3 * dt / steps, 4),
-            "pairs": stats.pairs, "manifolds": stats.manifolds,
-            "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-            "plain_value": round(plain, 1), "plain_sample": "%d worlds x %d single steps, one world per handle"
-                                                            % (len(pick), sample_steps),
-            "ratio_vs_plain": round(batched / plain, 1)}
+    out.update({"plain_value": round(plain, 1),
+                "plain_sample": "%d worlds x %d single steps, one world per handle" % (len(pick), sample_steps),
+                "ratio_vs_plain": round(batched / plain, 1)})
+    return out
 
 
 def served_leg(rate, duration, max_wall):
@@ -296,12 +302,20 @@ def main():
     ap.add_argument("--batched-worlds", type=int, default=4096)
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=48,
+                    help="worlds of the batch also stepped alone, for the batched/plain ratio (0: skip)")
     ap.add_argument("--served-rate", type=float, default=2000.0)
     ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--stress-bodies", type=int, default=32768)
     ap.add_argument("--no-stress", action="store_true")
+    ap.add_argument("--legs", default="towers,batched,served,stress",
+                    help="comma list of workloads to run; without towers only the other legs' results are printed")
     a = ap.parse_args()
+    legs = set(x.strip() for x in a.legs.split(",") if x.strip())
+    for name, off in (("batched", a.no_batched), ("served", a.no_served), ("stress", a.no_stress)):
+        if off:
+            legs.discard(name)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; --gpus %d is not supported "
@@ -312,6 +326,18 @@ def main():
     from bepuphysics1_amd import Space, scenes
     global MIX
     MIX = (scenes.SHAPE_BOX, scenes.SHAPE_SPHERE, scenes.SHAPE_CONVEX_HULL)
+
+    if "towers" not in legs:
+        # Secondary workloads alone (kernel profiles of one workload at a time).
+        out = {"legs": sorted(legs)}
+        if "batched" in legs:
+            out["batched"] = batched_leg(a.batched_worlds, a.batched_steps, 5, a.plain_sample, 10)
+        if "served" in legs:
+            out["served"] = served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0)
+        if "stress" in legs:
+            out["stress"] = stress_leg(a.stress_bodies, 3, False)
+        print(json.dumps(out))
+        return
 
     scene = scenes.towers(nx=a.nx, nz=a.nz, height=a.height, seed=0)
     n_dyn = int((scene["mass"] > 0).sum())
@@ -379,9 +405,9 @@ def main():
 
     parity = None if a.no_parity else parity_check()
     copy_bytes = calibrate_copy() if a.calibrate_copy else None
-    batched = None if a.no_batched else batched_leg(a.batched_worlds, a.batched_steps, 5, 48, 10)
-    served = None if a.no_served else served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0)
-    stress = None if a.no_stress else stress_leg(a.stress_bodies, 3, not a.no_cpu_baseline)
+    batched = batched_leg(a.batched_worlds, a.batched_steps, 5, a.plain_sample, 10) if "batched" in legs else None
+    served = served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0) if "served" in legs else None
+    stress = stress_leg(a.stress_bodies, 3, not a.no_cpu_baseline) if "stress" in legs else None
 
     line = {
         "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
