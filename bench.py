@@ -1,26 +1,36 @@
 """Benchmark of the gfx950 step path: body-steps/s on one MI355X.
 
-Workload (N=1): the large stacking scene -- a 64 x 64 grid of 20-high box
-towers (81,920 dynamic boxes, half extent 0.5, 0.01 seeded jitter) on a
-static ground, dt 1/60, 10 solver iterations, FP32, warm starting on.
-Inputs are resident on the device before the timed region; the timed region
-is K full steps (force, AABB, broadphase, narrowphase, colouring, prestep,
-warm start + iterations, integration) bracketed by device synchronisation.
+Headline workload (N=1): 4,096 independent worlds stepped together (the
+batched many-world form): each world a random pile of N ~ logU[8,256] bodies
+(boxes, spheres and 16-point convex hulls in equal shares) dropped onto its
+own ground, dt 1/60, 8 solver iterations, FP32, warm starting on.  One
+bepu_step advances every world; each world's result is bit-identical to the
+same world stepped alone.  Inputs are resident on the device before the timed
+region; the timed region is K full steps (force, AABB, broadphase,
+narrowphase, general convex pair tests, islands and colouring, prestep, warm
+start + iterations, integration) bracketed by device synchronisation.
 
 The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 average duration (HIP events on the world's stream), against
-                the MI355X HBM peak of 8 TB/s;
+                the MI355X HBM peak of 8 TB/s, with the measured HBM traffic
+                of that kernel (rocprofv3 counter passes, profiles/);
+  latency_ms    p50 / p99 of single synchronised batched steps;
+  plain_path    the same worlds stepped one at a time, each in its own world
+                handle (a sample), and the batched/plain ratio;
   cpu_baseline  the oracle (this repository's C++ restatement of the
-                reference path) timed on up to 16 host threads, and on one,
-                on a bounded sample of the same scene;
-  parity        a small scene stepped on both sides: pair sets and impulses;
-  latency       p50 / p99 of single synchronised steps;
-  batched, served, stress
-                the many-world batch, continuous batching of rollouts and the
+                reference step) on a sample of the same worlds, one world per
+                host thread on up to 16 threads;
+  parity        batched worlds against the oracle, each world alone: motion
+                states bit for bit and the largest relative impulse error;
+  towers        the plain single-world path on 64 x 64 towers of 20 boxes
+                (81,920 bodies, 10 iterations) with its own roofline and
+                oracle baseline;
+  served, stress
+                continuous batching of rollouts, and one step of the
                 broadphase/narrowphase stress scene.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs towers,batched,served,stress]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs batched,towers,served,stress]
 """
 import argparse
 import json
@@ -34,12 +44,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BATCHED_ITERATIONS = 8
 
 # Algorithmic bytes per unit of work, per stage (FP32, each element counted
 # once per stage and once per solver iteration, no cache-reuse credit):
 #   body_pre      force/inertia 160 + AABB 80 per body
 #   broadphase    Morton 32 + radix sort 64 + tree build 88 + traversal leaf read 24 per body
 #   narrowphase   312 per broadphase pair
+#   convex        1,120 per general convex pair (a pair with a hull or a
+#                 capsule): pair state 336 and tester/supplement state 160,
+#                 each read and written, + pose, velocity and shape of both
+#                 bodies 128 (hull vertices are read from an LDS copy)
 #   islands       union-find and island bucketing: 40 per body (parent init 4,
 #                 compress 12, count 4, body list 8, local index 4, tier list 8)
 #                 + 32 per manifold (union reads 16, count 8, constraint list 8)
@@ -49,11 +64,14 @@ HBM_PEAK_GBS = 8000.0
 #                 colouring (pair read 24, incidence 16, colour/priority 16, order 8)
 #   integrate     80 per body
 # c = contacts of the manifold (the scene's mean is used per tier).
+B_CONVEX_PAIR = 1120
+
+
 def manifold_bytes(m, contacts, iters):
     return (332 + 190) * m + (72 + 70) * contacts + iters * (316 * m + 44 * contacts)
 
 
-def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global=0):
+def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0):
     c_mean = n_contacts / n_cons if n_cons else 0.0
     m_isl = n_cons - n_global
     if stage == "body_pre":
@@ -62,6 +80,8 @@ def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global
         return 208 * n_body
     if stage == "narrowphase":
         return 312 * n_pair
+    if stage == "convex":
+        return B_CONVEX_PAIR * n_convex
     if stage == "islands":
         return 40 * n_body + 32 * n_cons
     if stage == "island_solve":
@@ -73,65 +93,220 @@ def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global
     return 0
 
 
-# Kernels of each timed stage, for the measured HBM traffic of the dominant one
-# (profiles/traffic.json: FETCH_SIZE / WRITE_SIZE per launch from rocprofv3
-# counter passes, scaled by a plain copy of known size).
+def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0):
+    """Whole-step algorithmic bytes: B_body*N + B_pair*P + B_convex*P_convex + sum over manifolds."""
+    return 528 * n_body + 312 * n_pair + B_CONVEX_PAIR * n_convex + (332 + 190) * n_cons + \
+        (72 + 70) * n_contacts + iters * (316 * n_cons + 44 * n_contacts)
+
+
+# Kernels of each timed stage (name substrings), for the measured HBM traffic
+# of the dominant one: profiles/traffic_<workload>.json holds FETCH_SIZE /
+# WRITE_SIZE per launch from rocprofv3 counter passes, scaled by a plain copy
+# of known size.
 STAGE_KERNELS = {
-    "island_solve": ["bepu::k_island_pack(bepu::DeviceWorld)", "bepu::k_island_medium(bepu::DeviceWorld)"],
-    "narrowphase": ["bepu::k_narrowphase(bepu::DeviceWorld)"],
-    "broadphase": ["bepu::k_grid_insert(bepu::DeviceWorld)", "bepu::k_grid_query(bepu::DeviceWorld)"],
-    "global_solve": ["bepu::k_global(bepu::DeviceWorld, unsigned long)"],
+    "island_solve": ["k_island_pack", "k_island_medium"],
+    "narrowphase": ["k_narrowphase"],
+    "convex": ["k_convex"],
+    "broadphase": ["k_grid_insert", "k_grid_query"],
+    "global_solve": ["k_global"],
+    "body_pre": ["k_body_pre"],
+    "integrate": ["k_integrate"],
 }
 
 
-def measured_traffic(stage):
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+def measured_traffic(stage, workload):
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if stage not in STAGE_KERNELS or not os.path.exists(path):
         return None
     t = json.load(open(path))["bytes_per_launch"]
-    ks = [k for k in STAGE_KERNELS[stage] if k in t]
+    ks = [k for k in t if any(s in k for s in STAGE_KERNELS[stage])]
     return sum(t[k] for k in ks) if ks else None
 
 
-def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0):
-    """Whole-step algorithmic bytes: B_body*N + B_pair*P + sum over manifolds."""
-    return 528 * n_body + 312 * n_pair + (332 + 190) * n_cons + (72 + 70) * n_contacts + \
-        iters * (316 * n_cons + 44 * n_contacts)
+def stage_table(stages, counts):
+    out = {}
+    for name, ms in stages.items():
+        b = algorithmic_bytes(name, **counts)
+        out[name] = {"ms": round(ms, 5), "alg_bytes": int(b),
+                     "GBps": round(b / (ms * 1e-3) / 1e9, 2) if ms > 0 else None}
+    return out
 
 
-def parity_check():
-    """Small towers scene stepped on the GPU and on the oracle."""
+def roofline(stages, counts, workload):
+    dominant = max(stages, key=lambda k: stages[k])
+    dom_bytes = algorithmic_bytes(dominant, **counts)
+    achieved = dom_bytes / (stages[dominant] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant, workload),
+            "kernel": dominant, "kernel_ms": round(stages[dominant], 5), "alg_bytes_per_launch": int(dom_bytes)}
+
+
+def counts_of(st, iters):
+    return dict(n_body=st.bodies, n_pair=st.pairs, n_cons=st.manifolds, n_contacts=st.contacts, iters=iters,
+                n_global=st.global_manifolds, n_convex=int(sum(st.convex_pairs)))
+
+
+def latency(sp, steps):
+    lat = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        sp.Update(1)
+        sp.stats()
+        lat.append(1e3 * (time.perf_counter() - t))
+    return {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)}
+
+
+MIX = None  # boxes, spheres and 16-point hulls in equal shares (set in main)
+
+
+def batched_settings():
+    from bepuphysics1_amd._lib import default_settings
+    st = default_settings()
+    st.iteration_limit = BATCHED_ITERATIONS
+    return st
+
+
+def batched_run(a):
+    """The headline: every world of the batch in one world handle, one
+    bepu_step per step for all of them."""
+    from bepuphysics1_amd import Space, scenes
+    worlds = scenes.batched_worlds(a.worlds, seed=0, shapes=MIX)
+    n_dyn = int(sum((w["mass"] > 0).sum() for w in worlds))
+    sp = Space(capacity=sum(len(w) for w in worlds), settings=batched_settings())
+    sp.register_hulls(scenes.hull_library())
+    for w in worlds:
+        sp.add_world(w)
+    if a.warmup:
+        sp.Update(a.warmup)
+    sp.stats()  # synchronise
+    t0 = time.perf_counter()
+    sp.Update(a.steps)
+    st = sp.stats()  # synchronises the world's stream
+    elapsed = time.perf_counter() - t0
+    counts = counts_of(st, BATCHED_ITERATIONS)
+    # Stage times (HIP events between the stages) in a separate run, so that
+    # the events' own cost stays out of `value`.
+    sp.set_timing(True)
+    sp.Update(min(a.steps, 20))
+    stages = sp.stage_times()
+    sp.set_timing(False)
+    lat = latency(sp, a.latency_steps)
+    sp.close()
+    return worlds, {
+        "value": n_dyn * a.steps / elapsed, "ms_per_step": 1e3 * elapsed / a.steps, "bodies": n_dyn,
+        "stats": st, "counts": counts, "stages": stages, "latency": lat}
+
+
+def plain_path(worlds, sample_worlds, sample_steps):
+    """The same worlds stepped one at a time, each alone in its own world
+    handle (a sample of them, scaled to body-steps/s)."""
+    from bepuphysics1_amd import Space, scenes
+    rng = np.random.default_rng(1)
+    pick = rng.choice(len(worlds), size=min(sample_worlds, len(worlds)), replace=False)
+    t_sum, b_sum = 0.0, 0
+    for k in pick:
+        one = Space(capacity=len(worlds[k]), settings=batched_settings())
+        one.register_hulls(scenes.hull_library())
+        one.add_scene(worlds[k])
+        one.Update(2)
+        one.stats()
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            one.Update(1)
+        one.stats()
+        t_sum += (time.perf_counter() - t0) / sample_steps
+        b_sum += int((worlds[k]["mass"] > 0).sum())
+        one.close()
+    return b_sum / t_sum, "%d worlds x %d single steps, one world per handle" % (len(pick), sample_steps)
+
+
+def oracle_hull_table():
+    from bepuphysics1_amd import compute_hull, scenes
+    return [compute_hull(p)[:2] for p in scenes.hull_library()]
+
+
+def cpu_baseline_batched(worlds, sample_worlds, warmup, steps, threads):
+    """The oracle on a sample of the batch's worlds: every world its own oracle
+    world, the worlds spread over `threads` host threads (the oracle releases
+    the interpreter lock while it steps).  `warmup` untimed steps per world (as
+    on the GPU), then `steps` steps of every sampled world timed together."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import refcpu
+    hulls = oracle_hull_table()
+    rng = np.random.default_rng(2)
+    pick = rng.choice(len(worlds), size=min(sample_worlds, len(worlds)), replace=False)
+    st = refcpu.default_settings()
+    st.iteration_limit = BATCHED_ITERATIONS
+    ows = []
+    for k in pick:
+        w = refcpu.World(st)
+        w.add_scene(worlds[k], hulls)
+        ows.append(w)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda w: w.step(warmup), ows))
+        t0 = time.perf_counter()
+        list(ex.map(lambda w: w.step(steps), ows))
+        dt = time.perf_counter() - t0
+    n = int(sum((worlds[k]["mass"] > 0).sum() for k in pick))
+    for w in ows:
+        w.close()
+    return {"value": round(n * steps / dt, 1), "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": "oracle (C++ restatement of the reference step), %d of the %d worlds (%d bodies), one "
+                      "oracle world per task on %d host threads, %d steps after %d untimed steps, %.1f s"
+                      % (len(pick), len(worlds), n, threads, steps, warmup, dt)}
+
+
+def f32bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def impulse_rel_err(rg, ro):
+    x = np.array(list(rg.normal) + list(rg.friction) + list(rg.sliding) + [rg.twist], np.float64)
+    y = np.array(list(ro.normal) + list(ro.friction) + list(ro.sliding) + [ro.twist], np.float64)
+    return float((np.abs(x - y) / np.maximum(np.maximum(np.abs(x), np.abs(y)), 1e-3)).max())
+
+
+def parity_batched(n_worlds, steps):
+    """Worlds of the batched workload stepped together on the GPU and each
+    alone on the oracle: motion states bit for bit, impulses by relative error."""
     from bepuphysics1_amd import Space, scenes
     from oracle import refcpu
-    sc = scenes.towers(nx=8, nz=8, height=20, seed=1)
-    sp = Space(capacity=len(sc))
-    sp.add_scene(sc)
-    sp.Update(60)
-    w = refcpu.World()
-    w.add_scene(sc)
-    w.step(60)
-    pairs_exact = bool(np.array_equal(sp.pairs(), w.pairs()))
-    ig = {int(r.key): r for r in sp.impulses()}
-    io = {int(r.key): r for r in w.impulses()}
-    err = 0.0
-    same_keys = set(ig) == set(io)
-    if same_keys:
-        for k, ro in io.items():
-            rg = ig[k]
-            a = np.array(list(rg.normal) + list(rg.friction) + list(rg.sliding) + [rg.twist], np.float64)
-            b = np.array(list(ro.normal) + list(ro.friction) + list(ro.sliding) + [ro.twist], np.float64)
-            err = max(err, float((np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-3)).max()))
-    motion = float(np.abs(sp.motion_states() - w.motion_states()).max())
+    worlds = scenes.batched_worlds(n_worlds, seed=7, shapes=MIX)
+    sp = Space(capacity=sum(len(w) for w in worlds), settings=batched_settings())
+    sp.register_hulls(scenes.hull_library())
+    for w in worlds:
+        sp.add_world(w)
+    sp.Update(steps)
+    ms = sp.motion_states()
+    imp = {int(r.key): r for r in sp.impulses()}
+    hulls = oracle_hull_table()
+    st = refcpu.default_settings()
+    st.iteration_limit = BATCHED_ITERATIONS
+    exact, err, keys_equal = True, 0.0, True
+    for k, sc in enumerate(worlds):
+        first, count = sp.world_range(k)
+        o = refcpu.World(st)
+        o.add_scene(sc, hulls)
+        o.step(steps)
+        exact &= bool(np.array_equal(f32bits(ms[first:first + count]), f32bits(o.motion_states())))
+        shift = (first << 32) | first
+        io = {int(r.key) + shift: r for r in o.impulses()}
+        mine = {key: r for key, r in imp.items() if first <= (key & 0xffffffff) < first + count}
+        keys_equal &= set(io) == set(mine)
+        for key, ro in io.items():
+            if key in mine:
+                err = max(err, impulse_rel_err(mine[key], ro))
+        o.close()
     sp.close()
-    return {"scene": "towers 8x8x20, 60 steps", "pairs_bit_exact": pairs_exact, "constraint_sets_equal": same_keys,
-            "max_impulse_rel_err": err, "max_motion_abs_diff": motion}
+    return {"scene": "%d worlds of the batched workload (seed 7), %d steps, batched on the GPU vs each alone on "
+                     "the oracle" % (n_worlds, steps),
+            "motion_states_bit_exact": exact, "constraint_sets_equal": keys_equal, "max_impulse_rel_err": err}
 
 
-def cpu_baseline(scene, steps, threads=1):
-    """Oracle (C++ restatement of the reference step) on the same scene; one
-    warm-up step excluded.  With threads > 1 the body stages, broadphase cells
-    and pair updates run in parallel and the solver colour by colour (same
-    results as one thread)."""
+def cpu_baseline_towers(scene, steps, threads=1):
+    """Oracle on the towers scene; one warm-up step excluded.  With threads > 1
+    the body stages, broadphase cells and pair updates run in parallel and the
+    solver colour by colour (same results as one thread)."""
     from oracle import refcpu
     w = refcpu.World()
     w.add_scene(scene)
@@ -145,64 +320,79 @@ def cpu_baseline(scene, steps, threads=1):
     return dyn * steps / dt, dt
 
 
-MIX = None  # boxes, spheres and 16-point hulls in equal shares (set in main)
-
-
-def batched_leg(n_worlds, steps, warmup, sample_worlds, sample_steps):
-    """4096 independent piles (N ~ logU[8,256]; boxes, spheres and 16-point
-    hulls; own grounds; 8 iterations) stepped as sub-worlds of one batch,
-    against the plain path: the same worlds stepped one at a time, each in its
-    own world handle (sampled, then scaled to all worlds)."""
+def parity_towers():
+    """Small towers scene stepped on the GPU and on the oracle."""
     from bepuphysics1_amd import Space, scenes
-    from bepuphysics1_amd._lib import default_settings
-    st = default_settings()
-    st.iteration_limit = 8
-    worlds = scenes.batched_worlds(n_worlds, seed=0, shapes=MIX)
-    n_dyn = int(sum((w["mass"] > 0).sum() for w in worlds))
-    sp = Space(capacity=sum(len(w) for w in worlds), settings=st)
-    sp.register_hulls(scenes.hull_library())
-    for w in worlds:
-        sp.add_world(w)
-    sp.Update(warmup)
+    from oracle import refcpu
+    sc = scenes.towers(nx=8, nz=8, height=20, seed=1)
+    sp = Space(capacity=len(sc))
+    sp.add_scene(sc)
+    sp.Update(60)
+    w = refcpu.World()
+    w.add_scene(sc)
+    w.step(60)
+    pairs_exact = bool(np.array_equal(sp.pairs(), w.pairs()))
+    ig = {int(r.key): r for r in sp.impulses()}
+    io = {int(r.key): r for r in w.impulses()}
+    same_keys = set(ig) == set(io)
+    err = max((impulse_rel_err(ig[k], ro) for k, ro in io.items()), default=0.0) if same_keys else None
+    motion_exact = bool(np.array_equal(f32bits(sp.motion_states()), f32bits(w.motion_states())))
+    sp.close()
+    return {"scene": "towers 8x8x20, 60 steps", "pairs_bit_exact": pairs_exact, "constraint_sets_equal": same_keys,
+            "max_impulse_rel_err": err, "motion_states_bit_exact": motion_exact}
+
+
+def towers_leg(a, cpu, parity):
+    """The plain single-world path on the large stacking scene."""
+    from bepuphysics1_amd import Space, scenes
+    scene = scenes.towers(nx=a.nx, nz=a.nz, height=a.height, seed=0)
+    n_dyn = int((scene["mass"] > 0).sum())
+    sp = Space(capacity=len(scene))
+    sp.add_scene(scene)
+    iters = sp.Solver.IterationLimit
+    sp.Update(10)
     sp.stats()
     t0 = time.perf_counter()
-    sp.Update(steps)
-    stats = sp.stats()
-    dt = time.perf_counter() - t0
-    # Stage times in a separate run: per-stage events cost time of their own.
+    sp.Update(a.tower_steps)
+    st = sp.stats()
+    elapsed = time.perf_counter() - t0
     sp.set_timing(True)
-    sp.Update(min(steps, 20))
+    sp.Update(min(a.tower_steps, 50))
     stages = sp.stage_times()
+    sp.set_timing(False)
+    lat = latency(sp, a.latency_steps)
+    ms_state = sp.motion_states()
+    top = ms_state[1:].reshape(a.nx * a.nz, a.height, 13)[:, -1, :]
+    start_top = scene["state"][1:].reshape(a.nx * a.nz, a.height, 13)[:, -1, :]
+    # A tower stands while its top box has drifted less than half a box sideways
+    # and sits within one box height of where it started (contacts are soft:
+    # 20 stacked contacts compress by about 20 x allowed penetration).
+    standing = int(((np.abs(top[:, 0:3:2] - start_top[:, 0:3:2]).max(axis=1) < 0.5) &
+                    (np.abs(top[:, 1] - start_top[:, 1]) < 1.0)).sum())
     sp.close()
-    batched = n_dyn * steps / dt
-    out = {"workload": "%d worlds, N ~ logU[8,256] boxes + spheres + 16-point hulls on own grounds, dt 1/60, "
-                       "8 iterations" % n_worlds,
-           "bodies": n_dyn, "steps": steps, "value": round(batched, 1), "ms_per_step": round(1e3 * dt / steps, 4),
-           "pairs": stats.pairs, "manifolds": stats.manifolds,
-           "stages_ms": {k: round(v, 4) for k, v in stages.items()}}
-    if sample_worlds <= 0:
-        return out
-    # Plain path: every world alone, one step after another.
-    rng = np.random.default_rng(1)
-    pick = rng.choice(n_worlds, size=min(sample_worlds, n_worlds), replace=False)
-    t_sum, b_sum = 0.0, 0
-    for k in pick:
-        one = Space(capacity=len(worlds[k]), settings=st)
-        one.register_hulls(scenes.hull_library())
-        one.add_scene(worlds[k])
-        one.Update(2)
-        one.stats()
-        t0 = time.perf_counter()
-        for _ in range(sample_steps):
-            one.Update(1)
-        one.stats()
-        t_sum += (time.perf_counter() - t0) / sample_steps
-        b_sum += int((worlds[k]["mass"] > 0).sum())
-        one.close()
-    plain = b_sum / t_sum  # body-steps/s when worlds are stepped one at a time
-    out.update({"plain_value": round(plain, 1),
-                "plain_sample": "%d worlds x %d single steps, one world per handle" % (len(pick), sample_steps),
-                "ratio_vs_plain": round(batched / plain, 1)})
+    counts = counts_of(st, iters)
+    value = n_dyn * a.tower_steps / elapsed
+    out = {"workload": "towers %dx%dx%d boxes (plain single-world GPU path), dt 1/60, %d iterations"
+                       % (a.nx, a.nz, a.height, iters),
+           "bodies": n_dyn, "steps": a.tower_steps, "value": round(value, 1),
+           "ms_per_step": round(1e3 * elapsed / a.tower_steps, 4),
+           "pairs": st.pairs, "manifolds": st.manifolds, "contacts": st.contacts, "colors": st.colors,
+           "roofline": roofline(stages, counts, "towers"), "stages": stage_table(stages, counts),
+           "latency_ms": lat, "towers_standing": standing, "max_penetration": round(float(st.max_penetration), 5)}
+    if cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        v_cpu, dt_cpu = cpu_baseline_towers(scene, a.cpu_steps, threads)
+        v_1t, dt_1t = cpu_baseline_towers(scene, max(2, a.cpu_steps // 4), 1)
+        out["cpu_baseline"] = {
+            "value": round(v_cpu, 1), "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": "oracle (C++ restatement of the reference step; %d threads: parallel body stages, broadphase "
+                      "and pair updates, colour-parallel solver), same 81,920-box towers scene, %d steps after 1 "
+                      "untimed step, %.1f s" % (threads, a.cpu_steps, dt_cpu),
+            "single_thread_value": round(v_1t, 1),
+            "single_thread_sample": "%d steps, %.1f s" % (max(2, a.cpu_steps // 4), dt_1t)}
+        out["speedup_vs_cpu"] = round(value / v_cpu, 1)
+    if parity:
+        out["parity"] = parity_towers()
     return out
 
 
@@ -213,11 +403,8 @@ def served_leg(rate, duration, max_wall):
     8 iterations; continuous batching on one world handle (admission against
     capacity and a 200 GB budget), then the queue drains."""
     from bepuphysics1_amd import scenes
-    from bepuphysics1_amd._lib import default_settings
     from bepuphysics1_amd.serve import Server, run_poisson
-    st = default_settings()
-    st.iteration_limit = 8
-    srv = Server(capacity=1_000_000, settings=st, hbm_budget=200e9, hull_shapes=scenes.hull_library())
+    srv = Server(capacity=1_000_000, settings=batched_settings(), hbm_budget=200e9, hull_shapes=scenes.hull_library())
 
     def make_request(rng):
         n = int(np.exp(rng.uniform(np.log(16), np.log(2048))))
@@ -253,13 +440,13 @@ def stress_leg(n, reps, cpu):
         sp.close()
     out = {"workload": "stress, %d bodies (50%% boxes, 25%% spheres, 25%% hulls), 1 step from a fresh world" % n,
            "ms_per_step": round(1e3 * best, 4), "pairs": stats.pairs, "manifolds": stats.manifolds,
-           "contacts": stats.contacts, "pairs_per_s": round(stats.pairs / best, 1),
+           "contacts": stats.contacts, "convex_pairs_by_state": list(stats.convex_pairs),
+           "pairs_per_s": round(stats.pairs / best, 1),
            "stages_ms": {k: round(v, 4) for k, v in stages.items()}}
     if cpu:
         from oracle import refcpu
-        from bepuphysics1_amd import compute_hull
         w = refcpu.World()
-        w.add_scene(sc, [compute_hull(p)[:2] for p in scenes.hull_library()])
+        w.add_scene(sc, oracle_hull_table())
         t0 = time.perf_counter()
         w.step(1)
         out["cpu_oracle_ms"] = round(1e3 * (time.perf_counter() - t0), 2)
@@ -289,33 +476,28 @@ def calibrate_copy(nbytes=1 << 30):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=300, help="timed batched steps")
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--latency-steps", type=int, default=50)
+    ap.add_argument("--plain-sample", type=int, default=48,
+                    help="worlds of the batch also stepped alone, for the batched/plain ratio (0: skip)")
+    ap.add_argument("--cpu-worlds", type=int, default=4096, help="worlds in the oracle's sample")
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--height", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--latency-steps", type=int, default=50)
-    ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--batched-worlds", type=int, default=4096)
-    ap.add_argument("--batched-steps", type=int, default=50)
-    ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--plain-sample", type=int, default=48,
-                    help="worlds of the batch also stepped alone, for the batched/plain ratio (0: skip)")
+    ap.add_argument("--tower-steps", type=int, default=200)
     ap.add_argument("--served-rate", type=float, default=2000.0)
     ap.add_argument("--served-seconds", type=float, default=3.0)
-    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--stress-bodies", type=int, default=32768)
-    ap.add_argument("--no-stress", action="store_true")
-    ap.add_argument("--legs", default="towers,batched,served,stress",
-                    help="comma list of workloads to run; without towers only the other legs' results are printed")
+    ap.add_argument("--legs", default="batched,towers,served,stress",
+                    help="comma list of workloads; without batched only the other legs' results are printed")
     a = ap.parse_args()
     legs = set(x.strip() for x in a.legs.split(",") if x.strip())
-    for name, off in (("batched", a.no_batched), ("served", a.no_served), ("stress", a.no_stress)):
-        if off:
-            legs.discard(name)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; --gpus %d is not supported "
@@ -323,127 +505,68 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from bepuphysics1_amd import Space, scenes
+    from bepuphysics1_amd import scenes
     global MIX
     MIX = (scenes.SHAPE_BOX, scenes.SHAPE_SPHERE, scenes.SHAPE_CONVEX_HULL)
+    cpu = not a.no_cpu_baseline
+    copy_bytes = calibrate_copy() if a.calibrate_copy else None
 
-    if "towers" not in legs:
+    if "batched" not in legs:
         # Secondary workloads alone (kernel profiles of one workload at a time).
-        out = {"legs": sorted(legs)}
-        if "batched" in legs:
-            out["batched"] = batched_leg(a.batched_worlds, a.batched_steps, 5, a.plain_sample, 10)
+        out = {"legs": sorted(legs), "calibration_copy_bytes": copy_bytes}
+        if "towers" in legs:
+            out["towers"] = towers_leg(a, cpu, not a.no_parity)
         if "served" in legs:
             out["served"] = served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0)
         if "stress" in legs:
-            out["stress"] = stress_leg(a.stress_bodies, 3, False)
+            out["stress"] = stress_leg(a.stress_bodies, 3, cpu)
         print(json.dumps(out))
         return
 
-    scene = scenes.towers(nx=a.nx, nz=a.nz, height=a.height, seed=0)
-    n_dyn = int((scene["mass"] > 0).sum())
-    sp = Space(capacity=len(scene))
-    sp.add_scene(scene)
-    iters = sp.Solver.IterationLimit
-    if a.warmup:
-        sp.Update(a.warmup)
-    sp.stats()  # synchronise
-
-    t0 = time.perf_counter()
-    sp.Update(a.steps)
-    st = sp.stats()  # synchronises the world's stream
-    elapsed = time.perf_counter() - t0
-    # Stage times (HIP events between the stages) in a separate run, so that
-    # the events' own cost stays out of `value`.
-    sp.set_timing(True)
-    sp.Update(min(a.steps, 50))
-    stages = sp.stage_times()
-    sp.set_timing(False)
-
-    value = n_dyn * a.steps / elapsed
-    ms_per_step = 1e3 * elapsed / a.steps
-
-    counts = dict(n_body=st.bodies, n_pair=st.pairs, n_cons=st.manifolds, n_contacts=st.contacts, iters=iters,
-                  n_global=st.global_manifolds)
-    per_stage = {}
-    for name, ms in stages.items():
-        b = algorithmic_bytes(name, **counts)
-        per_stage[name] = {"ms": round(ms, 5), "alg_bytes": int(b),
-                           "GBps": round(b / (ms * 1e-3) / 1e9, 2) if ms > 0 else None}
-    dominant = max(stages, key=lambda k: stages[k])
-    dom_bytes = algorithmic_bytes(dominant, **counts)
-    achieved = dom_bytes / (stages[dominant] * 1e-3) / 1e9
-    step_alg = step_bytes(**counts)
-
-    # latency of single synchronised steps
-    lat = []
-    for _ in range(a.latency_steps):
-        t = time.perf_counter()
-        sp.Update(1)
-        sp.stats()
-        lat.append(1e3 * (time.perf_counter() - t))
-    ms_state = sp.motion_states()
-    top = ms_state[1:].reshape(a.nx * a.nz, a.height, 13)[:, -1, :]
-    start_top = scene["state"][1:].reshape(a.nx * a.nz, a.height, 13)[:, -1, :]
-    # A tower stands while its top box has drifted less than half a box sideways
-    # and sits within one box height of where it started (contacts are soft:
-    # 20 stacked contacts compress by about 20 x allowed penetration).
-    standing = int(((np.abs(top[:, 0:3:2] - start_top[:, 0:3:2]).max(axis=1) < 0.5) &
-                    (np.abs(top[:, 1] - start_top[:, 1]) < 1.0)).sum())
-    sp.close()
-
-    cpu = None
-    if not a.no_cpu_baseline:
+    worlds, b = batched_run(a)
+    st, counts, stages = b["stats"], b["counts"], b["stages"]
+    plain = None
+    if a.plain_sample > 0:
+        pv, ps = plain_path(worlds, a.plain_sample, 10)
+        plain = {"value": round(pv, 1), "sample": ps, "ratio_batched_vs_plain": round(b["value"] / pv, 1)}
+    cpu_line = None
+    if cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
-        v_cpu, dt_cpu = cpu_baseline(scene, a.cpu_steps, threads)
-        v_1t, dt_1t = cpu_baseline(scene, max(2, a.cpu_steps // 4), 1)
-        cpu = {"value": round(v_cpu, 1), "unit": "body-steps/s", "cores": threads, "kind": "port",
-               "sample": "oracle (C++ restatement of the reference step; %d threads: parallel body stages, "
-                         "broadphase and pair updates, colour-parallel solver), same 81,920-box towers scene, "
-                         "%d steps after 1 untimed step, %.1f s" % (threads, a.cpu_steps, dt_cpu),
-               "single_thread_value": round(v_1t, 1),
-               "single_thread_sample": "%d steps, %.1f s" % (max(2, a.cpu_steps // 4), dt_1t)}
-
-    parity = None if a.no_parity else parity_check()
-    copy_bytes = calibrate_copy() if a.calibrate_copy else None
-    batched = batched_leg(a.batched_worlds, a.batched_steps, 5, a.plain_sample, 10) if "batched" in legs else None
-    served = served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0) if "served" in legs else None
-    stress = stress_leg(a.stress_bodies, 3, not a.no_cpu_baseline) if "stress" in legs else None
-
+        cpu_line = cpu_baseline_batched(worlds, a.cpu_worlds, a.warmup, a.cpu_steps, threads)
+    parity = None if a.no_parity else parity_batched(16, 60)
+    step_alg = step_bytes(**counts)
     line = {
-        "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency",
-        "value": round(value, 1),
+        "metric": "body-steps/sec @ <=1e-4 impulse err; p50/p99 step latency, batched worlds",
+        "value": round(b["value"], 1),
         "unit": "body-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(b["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "towers %dx%dx%d boxes (plain single-world GPU path), dt 1/60, %d iterations"
-                   % (a.nx, a.nz, a.height, iters),
-                   "bodies": n_dyn, "pairs": st.pairs, "manifolds": st.manifolds, "contacts": st.contacts,
-                   "colors": st.colors},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant),
-                     "kernel": dominant,
-                     "kernel_ms": round(stages[dominant], 5), "alg_bytes_per_launch": int(dom_bytes)},
+        "config": {"workload": "batched many-world: %d worlds, N ~ logU[8,256] boxes + spheres + 16-point hulls "
+                               "dropped onto own grounds, dt 1/60, %d iterations, one step of every world per "
+                               "bepu_step" % (a.worlds, BATCHED_ITERATIONS),
+                   "worlds": a.worlds, "bodies": b["bodies"], "pairs": st.pairs, "manifolds": st.manifolds,
+                   "contacts": st.contacts, "convex_pairs_by_state": list(st.convex_pairs)},
+        "roofline": roofline(stages, counts, "batched"),
         "step_roofline": {"alg_bytes_per_step": int(step_alg),
-                          "achieved_GBps": round(step_alg / (ms_per_step * 1e-3) / 1e9, 2),
-                          "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
-        "stages": per_stage,
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4)},
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+                          "achieved_GBps": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9, 2),
+                          "frac": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},
+        "stages": stage_table(stages, counts),
+        "latency_ms": b["latency"],
+        "cpu_baseline": cpu_line,
+        "speedup_vs_cpu": round(b["value"] / cpu_line["value"], 1) if cpu_line else None,
+        "plain_path": plain,
         "parity": parity,
-        "batched": batched,
-        "served": served,
-        "stress": stress,
+        "towers": towers_leg(a, cpu, not a.no_parity) if "towers" in legs else None,
+        "served": served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0) if "served" in legs else None,
+        "stress": stress_leg(a.stress_bodies, 3, cpu) if "stress" in legs else None,
         "calibration_copy_bytes": copy_bytes,
-        "towers_standing": standing,
-        "max_penetration": round(float(st.max_penetration), 5),
     }
     print(json.dumps(line))
 

@@ -12,7 +12,8 @@ namespace bepu {
 
 namespace {
 
-__device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i) {
+// `pool` is the hull vertex pool: the device array, or its copy in LDS.
+__device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, const float4* pool, uint32_t i) {
     float4 sh = w.shape[i];
     cvx::Shape s;
     s.kind = (int)sh.w;
@@ -28,7 +29,7 @@ __device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i)
         s.half_length = sh.y;
     } else if (s.kind == BEPU_SHAPE_CONVEX_HULL) {
         s.margin = sh.x;
-        s.verts = w.hull_verts + __float_as_uint(sh.y);
+        s.verts = pool + __float_as_uint(sh.y);
         s.nverts = (int)__float_as_uint(sh.z);
     } else {
         s.margin = sh.x;  // sphere: the radius
@@ -107,7 +108,8 @@ __device__ __forceinline__ void add_general(PairState& s, ConvexState& e, f3 pos
 // GeneralConvexContactManifold.Update (GeneralConvexContactManifold.cs:71-126) with
 // ContactRefresher.ContactRefresh (CollisionTests/ContactRefresher.cs:18-69) and
 // IsContactUnique (GeneralConvexContactManifold.cs:145-166).
-__device__ __forceinline__ void update_general(const DeviceWorld& w, PairState& s, ConvexState& e) {
+__device__ __forceinline__ void update_general(const DeviceWorld& w, const float4* pool, PairState& s,
+                                               ConvexState& e) {
     float4 PA = w.pos[s.a], QA = w.quat[s.a], PB = w.pos[s.b], QB = w.quat[s.b];
     cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
     cvx::RT tb{f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w}};
@@ -140,7 +142,7 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, PairState& 
     f3 pos, nrm;
     float depth;
     int id;
-    if (cvx::candidate(shape_of(w, s.a), shape_of(w, s.b), ta, tb, f3_of(w.linv[s.a]), f3_of(w.linv[s.b]), st, &pos,
+    if (cvx::candidate(shape_of(w, pool, s.a), shape_of(w, pool, s.b), ta, tb, f3_of(w.linv[s.a]), f3_of(w.linv[s.b]), st, &pos,
                        &nrm, &depth, &id)) {
         for (int i = 0; i < s.ccount; ++i)
             if (dot3(f3_of(s.cnrm[s.clist[i]]), nrm) < 0) {
@@ -177,16 +179,34 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, PairState& 
 }
 
 constexpr int kConvexBlock = 64;
-constexpr uint32_t kConvexMaxGrid = 2048;
+constexpr uint32_t kConvexMaxGrid = 8192;  // one pair per thread up to 524,288 queued pairs
+// With at most this many hull vertices (64 KB) every workgroup first copies
+// the whole vertex pool to LDS: a support scan's loads then take LDS latency
+// instead of a round trip to L2, which bounds a pair test's dependent chain.
+constexpr uint32_t kLdsHullVerts = 4096;
+constexpr int kConvexLdsBlock = 256;
 
 }  // namespace
 
-__global__ void __launch_bounds__(kConvexBlock) k_convex(DeviceWorld w) {
-    // The three state regions of the work list, back to back.
-    uint32_t n0 = w.counters[ctr(CTR_CONVEX)], n1 = w.counters[ctr(CTR_CONVEX_SHALLOW)];
-    uint32_t n = n0 + n1 + w.counters[ctr(CTR_CONVEX_DEEP)];
-    for (uint32_t i = blockIdx.x * kConvexBlock + threadIdx.x; i < n; i += gridDim.x * kConvexBlock) {
-        size_t at = i < n0 ? i : (i < n0 + n1 ? (size_t)w.pair_cap + (i - n0) : 2 * (size_t)w.pair_cap + (i - n0 - n1));
+template <bool kLds, int kBlock>
+__global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
+    extern __shared__ float4 lds_hull[];
+    // The three state regions of the work list, costliest first: Deep pairs
+    // (MPR with surface casts and refinement), then Shallow (closest-point
+    // GJK), then Separated (boolean GJK).  Workgroups start in index order, so
+    // the long ones start first and the short ones fill in behind them.
+    uint32_t n2 = w.counters[ctr(CTR_CONVEX_DEEP)], n1 = w.counters[ctr(CTR_CONVEX_SHALLOW)];
+    uint32_t n = n2 + n1 + w.counters[ctr(CTR_CONVEX)];
+    if (blockIdx.x * kBlock >= n) return;  // uniform per workgroup
+    const float4* pool = w.hull_verts;
+    if (kLds) {
+        for (uint32_t v = threadIdx.x; v < w.hull_nverts; v += kBlock) lds_hull[v] = w.hull_verts[v];
+        __syncthreads();
+        pool = lds_hull;
+    }
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        size_t at = i < n2 ? 2 * (size_t)w.pair_cap + i
+                           : (i < n2 + n1 ? (size_t)w.pair_cap + (i - n2) : (size_t)(i - n2 - n1));
         uint2 wk = w.conv_work[at];
         // The pair's state is updated in place in global memory: its contact
         // slots are indexed at run time, which would put a private copy in
@@ -211,7 +231,7 @@ __global__ void __launch_bounds__(kConvexBlock) k_convex(DeviceWorld w) {
             e.offa[0] = z; e.offa[1] = z; e.offa[2] = z; e.offa[3] = z;
             e.offb[0] = z; e.offb[1] = z; e.offb[2] = z; e.offb[3] = z;
         }
-        update_general(w, s, e);
+        update_general(w, pool, s, e);
         // A constraint joins the islands of its two bodies when both are dynamic.
         if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
     }
@@ -219,8 +239,21 @@ __global__ void __launch_bounds__(kConvexBlock) k_convex(DeviceWorld w) {
 
 void launch_convex(const DeviceWorld& w, hipStream_t st) {
     if (!w.conv_work || w.pair_cap == 0) return;
-    uint32_t grid = std::min<uint32_t>(kConvexMaxGrid, (w.pair_cap + kConvexBlock - 1) / kConvexBlock);
-    hipLaunchKernelGGL(k_convex, dim3(grid), dim3(kConvexBlock), 0, st, w);
+    if (w.hull_nverts <= kLdsHullVerts) {
+        static bool attr = false;
+        if (!attr) {
+            (void)hipFuncSetAttribute((const void*)k_convex<true, kConvexLdsBlock>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsHullVerts * sizeof(float4)));
+            attr = true;
+        }
+        uint32_t grid = std::min<uint32_t>(kConvexMaxGrid * kConvexBlock / kConvexLdsBlock,
+                                           (w.pair_cap + kConvexLdsBlock - 1) / kConvexLdsBlock);
+        hipLaunchKernelGGL((k_convex<true, kConvexLdsBlock>), dim3(grid), dim3(kConvexLdsBlock),
+                           w.hull_nverts * sizeof(float4), st, w);
+    } else {
+        uint32_t grid = std::min<uint32_t>(kConvexMaxGrid, (w.pair_cap + kConvexBlock - 1) / kConvexBlock);
+        hipLaunchKernelGGL((k_convex<false, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w);
+    }
 }
 
 }  // namespace bepu

@@ -190,21 +190,35 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
             __syncthreads();
         }
     } else {
+        // The higher-priority neighbours of each lane as a lane mask over the
+        // workgroup, found in one pass; a round then visits only them.
+        constexpr int kWords = BLOCK / 64;
+        uint64_t hp[kWords];
+#pragma unroll
+        for (int q = 0; q < kWords; ++q) hp[q] = 0;
+        if (have)
+#pragma unroll
+            for (int q = 0; q < kWords; ++q)
+                for (int k = max(clo, q * 64); k < min(chi, q * 64 + 64); ++k) {
+                    if (k == tid) continue;
+                    int ka = L.da[k], kb = L.db[k];
+                    bool nb_ = (ka >= 0 && (ka == la || ka == lb)) || (kb >= 0 && (kb == la || kb == lb));
+                    if (nb_ && L.prio[k] >= prio) hp[q] |= 1ull << (k - q * 64);
+                }
         for (int round = 0; round <= BLOCK; ++round) {
             bool need = have && color < 0;
             if (need) {
                 uint64_t m0 = 0, m1 = 0;
                 bool ready = true;
-                for (int k = clo; k < chi; ++k) {
-                    if (k == tid) continue;
-                    int ka = L.da[k], kb = L.db[k];
-                    bool nb_ = (ka >= 0 && (ka == la || ka == lb)) || (kb >= 0 && (kb == la || kb == lb));
-                    if (!nb_ || L.prio[k] < prio) continue;
-                    int ck = __hip_atomic_load(&L.color[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (ck < 0) { ready = false; break; }
-                    if (ck < 64) m0 |= 1ull << ck;
-                    else m1 |= 1ull << (ck - 64);
-                }
+#pragma unroll
+                for (int q = 0; q < kWords; ++q)
+                    for (uint64_t m = hp[q]; ready && m; m &= m - 1) {
+                        int k = q * 64 + __ffsll((unsigned long long)m) - 1;
+                        int ck = __hip_atomic_load(&L.color[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (ck < 0) { ready = false; break; }
+                        if (ck < 64) m0 |= 1ull << ck;
+                        else m1 |= 1ull << (ck - 64);
+                    }
                 if (ready) {
                     if (~m0) color = __ffsll((unsigned long long)~m0) - 1;
                     else if (~m1) color = 64 + __ffsll((unsigned long long)~m1) - 1;

@@ -598,6 +598,23 @@ __device__ void init_pair(const DeviceWorld& w, uint64_t key, PairState& s) {
     s.static_f = ma.x * mb.x;
 }
 
+// Copies a pair state between global arrays, seven float4 loads in flight
+// before their stores (a struct assignment would be lowered through scratch
+// memory, and a plain loop waits for each load before its store, since the
+// arrays might overlap).  The memory clobber keeps later field accesses of
+// the destination after the stores.
+__device__ __forceinline__ void copy_pair_state(PairState* dst, const PairState* src) {
+    static_assert(sizeof(PairState) == 21 * 16 && alignof(PairState) == 16, "PairState is copied as 21 float4s");
+    const float4* a = reinterpret_cast<const float4*>(src);
+    float4* b = reinterpret_cast<float4*>(dst);
+#pragma unroll
+    for (int c = 0; c < 21; c += 7) {
+        float4 r0 = a[c], r1 = a[c + 1], r2 = a[c + 2], r3 = a[c + 3], r4 = a[c + 4], r5 = a[c + 5], r6 = a[c + 6];
+        b[c] = r0; b[c + 1] = r1; b[c + 2] = r2; b[c + 3] = r3; b[c + 4] = r4; b[c + 5] = r5; b[c + 6] = r6;
+    }
+    asm volatile("" ::: "memory");
+}
+
 __device__ __forceinline__ uint32_t hash_slot(uint64_t key, uint32_t cap) { return (uint32_t)mix64(key) & (cap - 1); }
 
 __device__ int hash_find(const uint64_t* keys, const uint32_t* vals, uint32_t cap, uint64_t key) {
@@ -636,13 +653,14 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
     bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
     int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
-    PairState s;
-    if (old >= 0) s = w.pairs_old[old];
+    // The state is updated in place in this step's slot: its contact slots are
+    // indexed at run time, which would put a private copy in scratch memory.
+    PairState& s = w.pairs_cur[t];
+    if (old >= 0) copy_pair_state(&s, &w.pairs_old[old]);
     else init_pair(w, key, s);
     if (s.kind == BEPU_PAIR_CONVEX) {
         // General convex pairs run their iterative GJK / MPR in k_convex, which
         // keeps this kernel's register budget for the closed-form pairs.
-        w.pairs_cur[t] = s;
         if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
         // Append (t, old) to the work list of the pair tester's state (new
         // pairs start Separated), so that k_convex's wavefronts mostly run one
@@ -671,7 +689,6 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     } else {
         update_single_pair(w, s);
     }
-    w.pairs_cur[t] = s;
     if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
     // A constraint (a pair with contacts) joins the islands of its two bodies
     // when both are dynamic; the union-find forest was reset by this step's

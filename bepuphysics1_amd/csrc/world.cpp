@@ -23,8 +23,8 @@ using namespace bepu;
 
 namespace {
 
-enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE, ST_COUNT };
-const char* kStageNames = "body_pre,broadphase,narrowphase,islands,island_solve,global_solve,integrate";
+enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_CONVEX, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE, ST_COUNT };
+const char* kStageNames = "body_pre,broadphase,narrowphase,convex,islands,island_solve,global_solve,integrate";
 
 using HostBody = BodyRecord;
 
@@ -422,6 +422,7 @@ bepu_status bepu_create_hull_shape(bepu_world w, const float* points, uint32_t n
         return BEPU_E_DEVICE;
     }
     w->hull_shapes.push_back(hs);
+    w->d.hull_nverts = need;
     if (shape_id) *shape_id = (uint32_t)w->hull_shapes.size() - 1;
     if (center) {
         center[0] = h.center.x;
@@ -697,16 +698,17 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
         if (e) (void)hipEventRecord(e[2], s);
         launch_narrowphase(d, s);
-        if (w->convex_bodies) launch_convex(d, s);
         if (e) (void)hipEventRecord(e[3], s);
-        launch_islands(d, w->scan_temp, w->scan_bytes, s);
+        if (w->convex_bodies) launch_convex(d, s);
         if (e) (void)hipEventRecord(e[4], s);
-        launch_island_solve(d, s);
+        launch_islands(d, w->scan_temp, w->scan_bytes, s);
         if (e) (void)hipEventRecord(e[5], s);
-        launch_global_solver(d, s);
+        launch_island_solve(d, s);
         if (e) (void)hipEventRecord(e[6], s);
-        launch_integrate(d, s);
+        launch_global_solver(d, s);
         if (e) (void)hipEventRecord(e[7], s);
+        launch_integrate(d, s);
+        if (e) (void)hipEventRecord(e[8], s);
         // This step's pair states and table become next step's history.
         std::swap(d.pairs_cur, d.pairs_old);
         std::swap(d.conv_cur, d.conv_old);
@@ -882,6 +884,9 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     std::memcpy(&mp, &c[CTR_MAXPEN], 4);
     out->max_penetration = any ? mp : 0.0f;
     out->global_manifolds = any ? std::min(c[CTR_GCONS], w->d.pair_cap) : 0;
+    out->convex_pairs[0] = any ? c[CTR_CONVEX] : 0;
+    out->convex_pairs[1] = any ? c[CTR_CONVEX_SHALLOW] : 0;
+    out->convex_pairs[2] = any ? c[CTR_CONVEX_DEEP] : 0;
     if (c[CTR_GBAR_ABORT]) return BEPU_E_DEVICE;  // the global solver's grid was not resident
     return c[CTR_OVERFLOW] ? BEPU_E_CAPACITY : BEPU_OK;
 }

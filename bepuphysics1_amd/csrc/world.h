@@ -92,6 +92,7 @@ struct DeviceWorld {
     float4* shape;             // w = shape kind; box: half extents; sphere: (radius, 0, 0);
                                // capsule: (radius, half length, 0); hull: (margin, first vertex, vertex count as uint bits)
     float4* hull_verts;        // vertices of every hull shape, back to back (xyz)
+    uint32_t hull_nverts;      // vertices in hull_verts
     float4 *bmin, *bmax;
     float4* stab;              // prev velocity, time below, slowing, allow stabilization
     float2* damp;              // linear, angular damping

@@ -154,6 +154,8 @@ typedef struct bepu_step_stats {
     uint32_t steps;           /* steps run since creation */
     float max_penetration;    /* deepest contact of the last step */
     uint32_t global_manifolds; /* manifolds in islands too large for one workgroup */
+    uint32_t convex_pairs[3]; /* general convex pairs (any hull or capsule) tested in the last step,
+                                 by pair tester state: separated, shallow, deep */
 } bepu_step_stats;
 
 typedef struct bepu_world_s* bepu_world;
