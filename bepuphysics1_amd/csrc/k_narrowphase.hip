@@ -37,10 +37,6 @@ struct LdsList {
         for (int k = i; k < n - 1; ++k) { P(k) = P(k + 1); I(k) = I(k + 1); }
         --n;
     }
-    __device__ void copy_from(LdsList& o) {
-        for (int k = 0; k < o.n; ++k) { P(k) = o.P(k); I(k) = o.I(k); }
-        n = o.n;
-    }
 };
 
 // ---------------------------------------------------------------------------
@@ -236,18 +232,25 @@ __device__ bool intersect_edge(f3 a1, f3 a2, const ClipEdge& ce, f3* out) {
 
 // Keeps entries [from, n) of `l` whose depth against the clip face is <= 0,
 // recomputing their depth (the three "Compute depths" passes of ClipFacesDirect).
-__device__ void refilter_depths(LdsList& l, LdsList& tmp, int from, f3 mtd, float clipFaceDot) {
-    tmp.copy_from(l);
-    l.n = from;
-    for (int i = from; i < tmp.n; i++) {
-        float4 it = tmp.P(i);
+// Compacts in place: an entry moves only to a lower index, so the kept entries
+// keep their order, as a rebuilt list would.
+__device__ void refilter_depths(LdsList& l, int from, f3 mtd, float clipFaceDot) {
+    int j = from;
+    for (int i = from; i < l.n; i++) {
+        float4 it = l.P(i);
+        int id = l.I(i);
         float depth = dot3(f3_of(it), mtd) - clipFaceDot;
-        if (depth <= 0) l.add(make_float4(it.x, it.y, it.z, depth), tmp.I(i));
+        if (depth <= 0) {
+            l.P(j) = make_float4(it.x, it.y, it.z, depth);
+            l.I(j) = id;
+            ++j;
+        }
     }
+    l.n = j;
 }
 
 // ClipFacesDirect, non-ALLOWUNSAFE (BoxBoxCollider.cs:4517-5278)
-__device__ void clip_faces(const Face& cf, const Face& face, f3 mtd, LdsList& out, LdsList& tmp) {
+__device__ void clip_faces(const Face& cf, const Face& face, f3 mtd, LdsList& out) {
     out.n = 0;
     float icw = 1 / cf.width, ich = 1 / cf.height;
     float icw2 = icw * icw, ich2 = ich * ich;
@@ -281,7 +284,7 @@ __device__ void clip_faces(const Face& cf, const Face& face, f3 mtd, LdsList& ou
     for (int i = 0; i < 4; ++i)
         if (fin & (1u << i)) out.add(f4_of(face.v[i], 0.0f), face.id[i]);
     float clipFaceDot = dot3(cf.v[0], mtd);
-    refilter_depths(out, tmp, 0, mtd, clipFaceDot);
+    refilter_depths(out, 0, mtd, clipFaceDot);
     int prev = out.n;
     if (prev >= 4) return;
     float b = dot3(face.v[0], face.n);
@@ -290,7 +293,7 @@ __device__ void clip_faces(const Face& cf, const Face& face, f3 mtd, LdsList& ou
             float a = dot3(cf.v[i], face.n);
             out.add(f4_of(cf.v[i] - face.n * (a - b), 0.0f), cf.id[i] + 8);
         }
-    refilter_depths(out, tmp, prev, mtd, clipFaceDot);
+    refilter_depths(out, prev, mtd, clipFaceDot);
     prev = out.n;
     if (prev >= 4) return;
     // Clip edge k is bounded by: 0 -> maxY, 1 -> minX, 2 -> minY, 3 -> maxX.
@@ -315,11 +318,12 @@ __device__ void clip_faces(const Face& cf, const Face& face, f3 mtd, LdsList& ou
             }
         }
     }
-    refilter_depths(out, tmp, prev, mtd, clipFaceDot);
+    refilter_depths(out, prev, mtd, clipFaceDot);
 }
 
-// PruneContactsMaxDistance, non-ALLOWUNSAFE (BoxBoxCollider.cs:3612-3741)
-__device__ void prune4(f3 mtd, LdsList& in, LdsList& out) {
+// PruneContactsMaxDistance, non-ALLOWUNSAFE (BoxBoxCollider.cs:3612-3741), in
+// place: the four picks are read before the list is rewritten with them.
+__device__ void prune4(f3 mtd, LdsList& in) {
     int n = in.n;
     int deepest = 0;
     for (int i = 1; i < n; i++)
@@ -339,9 +343,13 @@ __device__ void prune4(f3 mtd, LdsList& in, LdsList& out) {
         if (d < minY) { minY = d; mn = i; }
         else if (d > maxY) { maxY = d; mx = i; }
     }
-    int pick[4] = {deepest, furthest, mn, mx};
-    out.n = 0;
-    for (int k = 0; k < 4; ++k) out.add(in.P(pick[k]), in.I(pick[k]));
+    float4 p0 = in.P(deepest), p1 = in.P(furthest), p2 = in.P(mn), p3 = in.P(mx);
+    int i0 = in.I(deepest), i1 = in.I(furthest), i2 = in.I(mn), i3 = in.I(mx);
+    in.P(0) = p0; in.I(0) = i0;
+    in.P(1) = p1; in.I(1) = i1;
+    in.P(2) = p2; in.I(2) = i2;
+    in.P(3) = p3; in.I(3) = i3;
+    in.n = 4;
 }
 
 // Separating-axis tables for the nine edge axes A.i x B.j (BoxBoxCollider.cs:1795-2352):
@@ -373,8 +381,8 @@ __device__ __forceinline__ float m3_at(const m3& m, int k) {
 }
 
 // BoxBoxCollider.AreBoxesColliding (BoxBoxCollider.cs:1494-2386).  Fills `out`
-// (and uses `tmp`) with up to 4 contacts; returns whether the boxes collide.
-__device__ bool box_box(f3 ha, f3 hb, f3 pA, q4 qA, f3 pB, q4 qB, f3* axis_out, LdsList& out, LdsList& tmp) {
+// with up to 4 contacts; returns whether the boxes collide.
+__device__ bool box_box(f3 ha, f3 hb, f3 pA, q4 qA, f3 pB, q4 qB, f3* axis_out, LdsList& out) {
     m3 aO = mat_from_quat(qA);
     m3 bO = mat_from_quat(qB);
     f3 t = pB - pA;
@@ -453,12 +461,9 @@ __device__ bool box_box(f3 ha, f3 hb, f3 pA, q4 qA, f3 pB, q4 qB, f3* axis_out, 
         Face fa, fb;
         nearest_face(pA, aO, -minAxis, ha, fa);
         nearest_face(pB, bO, minAxis, hb, fb);
-        if (feature == 0) clip_faces(fa, fb, -minAxis, out, tmp);
-        else clip_faces(fb, fa, minAxis, out, tmp);
-        if (out.n > 4) {
-            tmp.copy_from(out);
-            prune4(minAxis, tmp, out);
-        }
+        if (feature == 0) clip_faces(fa, fb, -minAxis, out);
+        else clip_faces(fb, fa, minAxis, out);
+        if (out.n > 4) prune4(minAxis, out);
     }
     *axis_out = minAxis;
     return true;
@@ -502,13 +507,13 @@ __device__ bool box_sphere(f3 h, f3 bp, q4 bq, float r, f3 sp, float maxd, f3* p
     return true;
 }
 
-__device__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m, LdsList& tmp) {
+__device__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m) {
     float4 PA = w.pos[s.a], PB = w.pos[s.b], QA = w.quat[s.a], QB = w.quat[s.b];
     float4 SA = w.shape[s.a], SB = w.shape[s.b];
     f3 axis;
     m.n = 0;
     if (box_box(f3_of(SA), f3_of(SB), f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}, f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w},
-                &axis, m, tmp)) {
+                &axis, m)) {
         axis = -axis;
         int rem[4], nrem = 0;
         for (int i = 0; i < s.ccount; i++) {
@@ -642,8 +647,8 @@ __device__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64
 }
 
 __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
-    __shared__ float4 lds_pd[2][8 * kNpBlock];
-    __shared__ int lds_id[2][8 * kNpBlock];
+    __shared__ float4 lds_pd[8 * kNpBlock];
+    __shared__ int lds_id[8 * kNpBlock];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= npairs) return;
@@ -683,9 +688,8 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
         return;
     }
     if (s.kind == BEPU_PAIR_BOX_BOX) {
-        LdsList m{&lds_pd[0][threadIdx.x], &lds_id[0][threadIdx.x], 0};
-        LdsList tmp{&lds_pd[1][threadIdx.x], &lds_id[1][threadIdx.x], 0};
-        update_box_pair(w, s, m, tmp);
+        LdsList m{&lds_pd[threadIdx.x], &lds_id[threadIdx.x], 0};
+        update_box_pair(w, s, m);
     } else {
         update_single_pair(w, s);
     }
