@@ -646,6 +646,65 @@ __device__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64
     return false;
 }
 
+// Pairs with a hull or a capsule take the general convex path
+// (NarrowPhaseHelper.cs:360-367); boxes and spheres have closed-form testers.
+__device__ __forceinline__ bool general_convex_pair(const DeviceWorld& w, uint32_t i, uint32_t j) {
+    int si = (int)w.shape[i].w, sj = (int)w.shape[j].w;
+    return !((si == BEPU_SHAPE_BOX || si == BEPU_SHAPE_SPHERE) && (sj == BEPU_SHAPE_BOX || sj == BEPU_SHAPE_SPHERE));
+}
+
+// This step's state of a pair in slot t: last step's (found in last step's
+// key -> slot table), or a fresh one.  The state is updated in place in the
+// slot: its contact slots are indexed at run time, which would put a private
+// copy in scratch memory.  Returns last step's slot (-1 for a new pair).
+__device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t) {
+    // A body inserted for this step (possibly into ids a removed world used)
+    // has no history: never pick up the previous step's state for its pairs.
+    uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
+    bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
+    int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
+    PairState& s = w.pairs_cur[t];
+    if (old >= 0) copy_pair_state(&s, &w.pairs_old[old]);
+    else init_pair(w, key, s);
+    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
+    return old;
+}
+
+// General convex pairs: their state carried into this step's slot, and
+// (slot, last slot) appended to the work list of the pair tester's state (new
+// pairs start Separated), so that k_convex's wavefronts mostly run one branch
+// of the state machine: one atomic per wavefront and state.
+__global__ void __launch_bounds__(kNpBlock) k_convex_prep(DeviceWorld w) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
+    int cstate = -1;
+    uint32_t old_slot = ~0u;
+    if (t < npairs) {
+        uint64_t key = w.pair_keys[t];
+        if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) {
+            int old = begin_pair(w, key, t);
+            cstate = old >= 0 ? __float_as_int(w.conv_old[old].sep.w) : 0;
+            if (cstate != 1 && cstate != 2) cstate = 0;
+            if (old >= 0) old_slot = (uint32_t)old;
+        }
+    }
+    const int lane = __lane_id();
+    for (int q = 0; q < 3; ++q) {
+        uint64_t mine = __ballot(cstate == q);
+        if (!mine) continue;
+        int leader = __ffsll((unsigned long long)mine) - 1;
+        uint32_t base = 0;
+        uint32_t c = q == 0 ? CTR_CONVEX : (q == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP);
+        if (lane == leader) base = atomicAdd(&w.counters[ctr(c)], (uint32_t)__popcll(mine));
+        base = __shfl(base, leader);
+        if (cstate == q)
+            w.conv_work[(size_t)q * w.pair_cap + base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull))] =
+                make_uint2(t, old_slot);
+    }
+}
+
+// Closed-form pairs (box-box, sphere-sphere, box-sphere); general convex pairs
+// are left to k_convex_prep and k_convex.
 __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     __shared__ float4 lds_pd[8 * kNpBlock];
     __shared__ int lds_id[8 * kNpBlock];
@@ -653,47 +712,15 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= npairs) return;
     uint64_t key = w.pair_keys[t];
-    // A body inserted for this step (possibly into ids a removed world used)
-    // has no history: never pick up the previous step's state for its pairs.
-    uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
-    bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
-    int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
-    // The state is updated in place in this step's slot: its contact slots are
-    // indexed at run time, which would put a private copy in scratch memory.
+    if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) return;
+    begin_pair(w, key, t);
     PairState& s = w.pairs_cur[t];
-    if (old >= 0) copy_pair_state(&s, &w.pairs_old[old]);
-    else init_pair(w, key, s);
-    if (s.kind == BEPU_PAIR_CONVEX) {
-        // General convex pairs run their iterative GJK / MPR in k_convex, which
-        // keeps this kernel's register budget for the closed-form pairs.
-        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
-        // Append (t, old) to the work list of the pair tester's state (new
-        // pairs start Separated), so that k_convex's wavefronts mostly run one
-        // branch of the state machine: one atomic per wavefront and state.
-        int cstate = old >= 0 ? __float_as_int(w.conv_old[old].sep.w) : 0;
-        if (cstate != 1 && cstate != 2) cstate = 0;
-        int lane = __lane_id();
-        for (int q = 0; q < 3; ++q) {
-            uint64_t mine = __ballot(cstate == q);
-            if (!mine) continue;
-            int leader = __ffsll((unsigned long long)mine) - 1;
-            uint32_t base = 0;
-            uint32_t c = q == 0 ? CTR_CONVEX : (q == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP);
-            if (lane == leader) base = atomicAdd(&w.counters[ctr(c)], (uint32_t)__popcll(mine));
-            base = __shfl(base, leader);
-            if (cstate == q)
-                w.conv_work[(size_t)q * w.pair_cap + base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull))] =
-                    make_uint2(t, old >= 0 ? (uint32_t)old : ~0u);
-        }
-        return;
-    }
     if (s.kind == BEPU_PAIR_BOX_BOX) {
         LdsList m{&lds_pd[threadIdx.x], &lds_id[threadIdx.x], 0};
         update_box_pair(w, s, m);
     } else {
         update_single_pair(w, s);
     }
-    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
     // A constraint (a pair with contacts) joins the islands of its two bodies
     // when both are dynamic; the union-find forest was reset by this step's
     // body pass.
@@ -751,6 +778,11 @@ void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st) {
     launch_fill_u32(w.counters + ctr(CTR_COLORS), 0u, 1, st);
     launch_fill_u32(w.counters + ctr(CTR_MAXPEN), 0u, 1, st);
     if (npairs) hipLaunchKernelGGL(k_stats, dim3((npairs + 255) / 256), dim3(256), 0, st, w, npairs);
+}
+
+void launch_convex_prep(const DeviceWorld& w, hipStream_t st) {
+    if (w.pair_cap == 0) return;
+    hipLaunchKernelGGL(k_convex_prep, dim3((w.pair_cap + kNpBlock - 1) / kNpBlock), dim3(kNpBlock), 0, st, w);
 }
 
 void launch_narrowphase(const DeviceWorld& w, hipStream_t st) {

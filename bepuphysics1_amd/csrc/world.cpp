@@ -47,6 +47,11 @@ uint32_t next_pow2(uint64_t v) {
 struct bepu_world_s {
     DeviceWorld d{};
     hipStream_t stream = nullptr;
+    // General convex pairs are tested on a second stream while the closed-form
+    // pairs run on the first (both kernels are bound by memory latency and
+    // leave most of the device idle alone); fork and join are events.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<HostBody> staged;   // bodies not yet uploaded
     uint32_t uploaded = 0;
     void* sort_temp = nullptr;
@@ -142,6 +147,10 @@ bepu_status bepu_world_destroy(bepu_world w) {
     if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
+    if (w->side) (void)hipStreamSynchronize(w->side);
+    if (w->fork) (void)hipEventDestroy(w->fork);
+    if (w->join) (void)hipEventDestroy(w->join);
+    if (w->side) (void)hipStreamDestroy(w->side);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
     return BEPU_OK;
@@ -201,7 +210,10 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
         bepu_world_destroy(w);
         return st;
     }
-    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&w->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&w->join, hipEventDisableTiming) != hipSuccess) {
         bepu_world_destroy(w);
         return BEPU_E_DEVICE;
     }
@@ -697,9 +709,18 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[1], s);
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
         if (e) (void)hipEventRecord(e[2], s);
+        if (w->convex_bodies) {
+            // Queue the general convex pairs, then test them on the side
+            // stream while the closed-form pairs run here.
+            launch_convex_prep(d, s);
+            (void)hipEventRecord(w->fork, s);
+            (void)hipStreamWaitEvent(w->side, w->fork, 0);
+            launch_convex(d, w->side);
+            (void)hipEventRecord(w->join, w->side);
+        }
         launch_narrowphase(d, s);
         if (e) (void)hipEventRecord(e[3], s);
-        if (w->convex_bodies) launch_convex(d, s);
+        if (w->convex_bodies) (void)hipStreamWaitEvent(s, w->join, 0);
         if (e) (void)hipEventRecord(e[4], s);
         launch_islands(d, w->scan_temp, w->scan_bytes, s);
         if (e) (void)hipEventRecord(e[5], s);

@@ -211,6 +211,7 @@ void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
 void launch_narrowphase(const DeviceWorld& w, hipStream_t st);
 // General convex pairs queued by the narrowphase (GJK / MPR).
+void launch_convex_prep(const DeviceWorld& w, hipStream_t st);
 void launch_convex(const DeviceWorld& w, hipStream_t st);
 // Last step's constraint, contact, colour and penetration statistics into the counters.
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);
