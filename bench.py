@@ -176,6 +176,9 @@ def batched_run(a):
     sp.register_hulls(scenes.hull_library())
     for w in worlds:
         sp.add_world(w)
+    # The autotuner picks the launch configuration on the batch's own first
+    # steps (untimed; every configuration gives bit-identical results).
+    tune = sp.autotune(a.autotune_steps) if a.autotune_steps > 0 else dict(sp.tuning(), steps=0)
     if a.warmup:
         sp.Update(a.warmup)
     sp.stats()  # synchronise
@@ -194,7 +197,7 @@ def batched_run(a):
     sp.close()
     return worlds, {
         "value": n_dyn * a.steps / elapsed, "ms_per_step": 1e3 * elapsed / a.steps, "bodies": n_dyn,
-        "stats": st, "counts": counts, "stages": stages, "latency": lat}
+        "stats": st, "counts": counts, "stages": stages, "latency": lat, "tuning": tune}
 
 
 def plain_path(worlds, sample_worlds, sample_steps):
@@ -479,6 +482,8 @@ def main():
     ap.add_argument("--steps", type=int, default=300, help="timed batched steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--autotune-steps", type=int, default=1,
+                    help="steps per candidate and round of the launch autotuner before the warm-up (0: defaults)")
     ap.add_argument("--latency-steps", type=int, default=50)
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="worlds of the batch also stepped alone, for the batched/plain ratio (0: skip)")
@@ -534,7 +539,8 @@ def main():
         threads = max(1, min(16, os.cpu_count() or 1))
         # The oracle's sample sits in the middle of the GPU's timed window (the
         # piles settle over the run and later steps carry more contacts).
-        cpu_line = cpu_baseline_batched(worlds, a.cpu_worlds, a.warmup + a.steps // 2, a.cpu_steps, threads)
+        cpu_line = cpu_baseline_batched(worlds, a.cpu_worlds, b["tuning"]["steps"] + a.warmup + a.steps // 2,
+                                        a.cpu_steps, threads)
     parity = None if a.no_parity else parity_batched(16, 60)
     step_alg = step_bytes(**counts)
     line = {
@@ -554,7 +560,8 @@ def main():
                                "dropped onto own grounds, dt 1/60, %d iterations, one step of every world per "
                                "bepu_step" % (a.worlds, BATCHED_ITERATIONS),
                    "worlds": a.worlds, "bodies": b["bodies"], "pairs": st.pairs, "manifolds": st.manifolds,
-                   "contacts": st.contacts, "convex_pairs_by_state": list(st.convex_pairs)},
+                   "contacts": st.contacts, "convex_pairs_by_state": list(st.convex_pairs),
+                   "autotune": b["tuning"]},
         "roofline": roofline(stages, counts, "batched"),
         "step_roofline": {"alg_bytes_per_step": int(step_alg),
                           "achieved_GBps": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9, 2),

@@ -79,6 +79,12 @@ class Impulses(ctypes.Structure):
                 ("sliding", ctypes.c_float * 2), ("twist", ctypes.c_float), ("pad", ctypes.c_float)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("narrowphase_block", ctypes.c_uint32), ("convex_lds", ctypes.c_uint32),
+                ("convex_side_stream", ctypes.c_uint32), ("small_island_max", ctypes.c_uint32),
+                ("ms_per_step", ctypes.c_float)]
+
+
 class StepStats(ctypes.Structure):
     _fields_ = [("bodies", ctypes.c_uint32), ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32),
                 ("contacts", ctypes.c_uint32), ("colors", ctypes.c_uint32), ("steps", ctypes.c_uint32),
@@ -123,6 +129,9 @@ EXPORTS = {
                                             ctypes.POINTER(ctypes.c_char_p)]),
     "bepu_set_timing": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_set_broadphase": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "bepu_get_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(Tuning)]),
+    "bepu_set_tuning": (ctypes.c_int, [_vp, ctypes.POINTER(Tuning)]),
+    "bepu_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(Tuning), _u32p]),
     "bepu_begin_world": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
     "bepu_insert_world": (ctypes.c_int, [_vp, ctypes.POINTER(BodyDesc), ctypes.c_uint32,
                                          ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),

@@ -183,7 +183,7 @@ constexpr uint32_t kConvexMaxGrid = 8192;  // one pair per thread up to 524,288 
 // With at most this many hull vertices (64 KB) every workgroup first copies
 // the whole vertex pool to LDS: a support scan's loads then take LDS latency
 // instead of a round trip to L2, which bounds a pair test's dependent chain.
-constexpr uint32_t kLdsHullVerts = 4096;
+constexpr uint32_t kLdsHullVerts = kConvexLdsMaxVerts;
 constexpr int kConvexLdsBlock = 256;
 
 }  // namespace
@@ -237,9 +237,9 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
     }
 }
 
-void launch_convex(const DeviceWorld& w, hipStream_t st) {
+void launch_convex(const DeviceWorld& w, bool lds, hipStream_t st) {
     if (!w.conv_work || w.pair_cap == 0) return;
-    if (w.hull_nverts <= kLdsHullVerts) {
+    if (lds && w.hull_nverts <= kLdsHullVerts) {
         static bool attr = false;
         if (!attr) {
             (void)hipFuncSetAttribute((const void*)k_convex<true, kConvexLdsBlock>,

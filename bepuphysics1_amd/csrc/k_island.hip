@@ -4,7 +4,8 @@
 // Constraints only couple through dynamic bodies, so the connected components
 // of the constraint graph over dynamic bodies (simulation islands,
 // DeactivationManagement/SimulationIsland.cs) can be solved independently.
-//   * small islands (<= 64 constraints) are packed several to a wavefront:
+//   * small islands (<= 64 constraints, or a lower bound the launch
+//     configuration sets) are packed several to a wavefront:
 //     an island of nc constraints belongs to class k = min(64, 64 / nc) and
 //     a bundle holds k islands of one class, 64 / k lanes each;
 //   * medium islands (<= 256 constraints) take one 256-thread workgroup;
@@ -35,10 +36,10 @@ __device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
 
 // Tier of an island with nc constraints and nb dynamic bodies.  A connected
 // island has nb <= nc + 1; the check keeps the slot layout safe regardless.
-__device__ __forceinline__ int island_tier(uint32_t nc, uint32_t nb) {
+__device__ __forceinline__ int island_tier(uint32_t nc, uint32_t nb, uint32_t small_max) {
     if (nc == 0) return 0;
     if (nb > nc + 1) return 3;
-    if (nc <= (uint32_t)kPackLanes) return 1;
+    if (nc <= min(small_max, (uint32_t)kPackLanes)) return 1;
     if (nc <= (uint32_t)kMediumBlock) return 2;
     return 3;
 }
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(256) k_isl_fill(DeviceWorld w) {
     if (cons) {
         uint64_t s0 = w.isl_start[rc], s1 = w.isl_start[rc + 1];
         w.isl_cons[hi32(s0) + hi32(oc) - 1] = i;
-        global = island_tier(hi32(s1) - hi32(s0), lo32(s1) - lo32(s0)) == 3;
+        global = island_tier(hi32(s1) - hi32(s0), lo32(s1) - lo32(s0), w.small_island_max) == 3;
     }
     block_push<256>(w.gcons, &w.counters[ctr(CTR_GCONS)], global, i, s_tmp);
 
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(256) k_isl_fill(DeviceWorld w) {
         w.body_local[i] = pos;
         if (rb == i) {
             uint32_t nc = hi32(s1) - hi32(s0);
-            int tier = island_tier(nc, lo32(s1) - lo32(s0));
+            int tier = island_tier(nc, lo32(s1) - lo32(s0), w.small_island_max);
             small = tier == 1;
             medium = tier == 2;
             if (small) {

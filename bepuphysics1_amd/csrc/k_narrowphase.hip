@@ -22,13 +22,15 @@ constexpr float kEps = 1e-7f;      // Toolbox.Epsilon
 constexpr float kBigEps = 1e-5f;   // Toolbox.BigEpsilon
 
 // Candidate contact list of one thread, stored in LDS columns:
-// entry k of thread t is pd[k * kNpBlock + t] (position xyz, depth) and id[...].
+// entry k of thread t is pd[k * stride + t] (position xyz, depth) and id[...],
+// stride = the workgroup size.
 struct LdsList {
     float4* pd;
     int* id;
     int n;
-    __device__ float4& P(int k) { return pd[k * kNpBlock]; }
-    __device__ int& I(int k) { return id[k * kNpBlock]; }
+    int stride;
+    __device__ float4& P(int k) { return pd[k * stride]; }
+    __device__ int& I(int k) { return id[k * stride]; }
     // TinyStructList semantics: Add beyond 8 entries is ignored, RemoveAt shifts.
     __device__ void add(float4 p, int i) {
         if (n < 8) { P(n) = p; I(n) = i; ++n; }
@@ -705,9 +707,10 @@ __global__ void __launch_bounds__(kNpBlock) k_convex_prep(DeviceWorld w) {
 
 // Closed-form pairs (box-box, sphere-sphere, box-sphere); general convex pairs
 // are left to k_convex_prep and k_convex.
-__global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
-    __shared__ float4 lds_pd[8 * kNpBlock];
-    __shared__ int lds_id[8 * kNpBlock];
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
+    __shared__ float4 lds_pd[8 * BLOCK];
+    __shared__ int lds_id[8 * BLOCK];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= npairs) return;
@@ -716,7 +719,7 @@ __global__ void __launch_bounds__(kNpBlock) k_narrowphase(DeviceWorld w) {
     begin_pair(w, key, t);
     PairState& s = w.pairs_cur[t];
     if (s.kind == BEPU_PAIR_BOX_BOX) {
-        LdsList m{&lds_pd[threadIdx.x], &lds_id[threadIdx.x], 0};
+        LdsList m{&lds_pd[threadIdx.x], &lds_id[threadIdx.x], 0, BLOCK};
         update_box_pair(w, s, m);
     } else {
         update_single_pair(w, s);
@@ -785,10 +788,12 @@ void launch_convex_prep(const DeviceWorld& w, hipStream_t st) {
     hipLaunchKernelGGL(k_convex_prep, dim3((w.pair_cap + kNpBlock - 1) / kNpBlock), dim3(kNpBlock), 0, st, w);
 }
 
-void launch_narrowphase(const DeviceWorld& w, hipStream_t st) {
+void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st) {
     if (w.pair_cap == 0) return;
     // The grid covers the pair capacity; threads past this step's pair count exit.
-    hipLaunchKernelGGL(k_narrowphase, dim3((w.pair_cap + kNpBlock - 1) / kNpBlock), dim3(kNpBlock), 0, st, w);
+    if (block == 64) hipLaunchKernelGGL(k_narrowphase<64>, dim3((w.pair_cap + 63) / 64), dim3(64), 0, st, w);
+    else if (block == 256) hipLaunchKernelGGL(k_narrowphase<256>, dim3((w.pair_cap + 255) / 256), dim3(256), 0, st, w);
+    else hipLaunchKernelGGL(k_narrowphase<128>, dim3((w.pair_cap + 127) / 128), dim3(128), 0, st, w);
 }
 
 }  // namespace bepu

@@ -52,6 +52,10 @@ struct bepu_world_s {
     // leave most of the device idle alone); fork and join are events.
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // Launch configuration (bepu_tuning); small_island_max lives in d.
+    uint32_t np_block = 128;
+    bool convex_lds = true;
+    bool convex_side = true;
     std::vector<HostBody> staged;   // bodies not yet uploaded
     uint32_t uploaded = 0;
     void* sort_temp = nullptr;
@@ -171,6 +175,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     uint32_t n = capacity_bodies;
     d.body_cap = n;
     d.pair_cap = std::max<uint32_t>(1024, 4 * n);
+    d.small_island_max = 64;
     d.hash_cap = next_pow2(2ull * d.pair_cap);
     bepu_status st = BEPU_OK;
     auto A = [&](auto** p, size_t cnt) {
@@ -677,6 +682,93 @@ bepu_status bepu_set_timing(bepu_world w, int32_t enable) {
     return BEPU_OK;
 }
 
+bepu_status bepu_get_tuning(bepu_world w, bepu_tuning* out) {
+    if (!w || !out) return BEPU_E_INVALID_ARG;
+    out->narrowphase_block = w->np_block;
+    out->convex_lds = w->convex_lds ? 1u : 0u;
+    out->convex_side_stream = w->convex_side ? 1u : 0u;
+    out->small_island_max = w->d.small_island_max;
+    out->ms_per_step = 0;
+    return BEPU_OK;
+}
+
+bepu_status bepu_set_tuning(bepu_world w, const bepu_tuning* t) {
+    if (!w || !t) return BEPU_E_INVALID_ARG;
+    if ((t->narrowphase_block != 64 && t->narrowphase_block != 128 && t->narrowphase_block != 256) ||
+        t->convex_lds > 1 || t->convex_side_stream > 1 || (t->small_island_max != 32 && t->small_island_max != 64))
+        return BEPU_E_INVALID_ARG;
+    w->np_block = t->narrowphase_block;
+    w->convex_lds = t->convex_lds != 0;
+    w->convex_side = t->convex_side_stream != 0;
+    w->d.small_island_max = t->small_island_max;
+    return BEPU_OK;
+}
+
+// Coordinate descent over the launch configuration on the world's own steps:
+// every choice gives bit-identical results, so the trial steps are ordinary
+// steps.  Each candidate value of a choice runs `trial_steps` steps in each of
+// two rounds (the rounds interleave the candidates, so a workload that changes
+// over the steps biases none of them), timed by events on the world's stream.
+bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, uint32_t* steps_run) {
+    if (!w || trial_steps == 0 || trial_steps > 1000) return BEPU_E_INVALID_ARG;
+    bepu_status st = upload(w);
+    if (st != BEPU_OK) return st;
+    struct Choice {
+        uint32_t bepu_tuning::*field;
+        std::vector<uint32_t> values;
+    };
+    std::vector<Choice> choices = {{&bepu_tuning::narrowphase_block, {64, 128, 256}},
+                                   {&bepu_tuning::small_island_max, {32, 64}}};
+    if (w->convex_bodies) {
+        choices.push_back({&bepu_tuning::convex_side_stream, {0, 1}});
+        if (w->d.hull_nverts <= kConvexLdsMaxVerts) choices.push_back({&bepu_tuning::convex_lds, {0, 1}});
+    }
+    bepu_tuning cur;
+    (void)bepu_get_tuning(w, &cur);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return BEPU_E_DEVICE;
+    }
+    const bool timing = w->timing;
+    w->timing = false;
+    uint32_t ran = 0;
+    float best_ms = 0;
+    st = BEPU_OK;
+    for (const Choice& c : choices) {
+        std::vector<double> total(c.values.size(), 0.0);
+        for (int round = 0; round < 2 && st == BEPU_OK; ++round)
+            for (size_t i = 0; i < c.values.size() && st == BEPU_OK; ++i) {
+                bepu_tuning t = cur;
+                t.*(c.field) = c.values[i];
+                (void)bepu_set_tuning(w, &t);
+                float ms = 0;
+                if (hipEventRecord(e0, w->stream) != hipSuccess) st = BEPU_E_DEVICE;
+                if (st == BEPU_OK) st = bepu_step(w, (int32_t)trial_steps);
+                if (st == BEPU_OK && (hipEventRecord(e1, w->stream) != hipSuccess ||
+                                      hipEventSynchronize(e1) != hipSuccess ||
+                                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    st = BEPU_E_DEVICE;
+                total[i] += ms;
+                ran += trial_steps;
+            }
+        if (st != BEPU_OK) break;
+        size_t best = 0;
+        for (size_t i = 1; i < total.size(); ++i)
+            if (total[i] < total[best]) best = i;
+        cur.*(c.field) = c.values[best];
+        best_ms = (float)(total[best] / (2.0 * trial_steps));
+    }
+    (void)bepu_set_tuning(w, &cur);
+    w->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cur.ms_per_step = best_ms;
+    if (out) *out = cur;
+    if (steps_run) *steps_run = ran;
+    return st;
+}
+
 bepu_status bepu_set_broadphase(bepu_world w, int32_t kind) {
     if (!w || (kind != BEPU_BROADPHASE_GRID && kind != BEPU_BROADPHASE_TREE)) return BEPU_E_INVALID_ARG;
     w->d.bp_kind = (uint32_t)kind;
@@ -709,18 +801,20 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[1], s);
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
         if (e) (void)hipEventRecord(e[2], s);
-        if (w->convex_bodies) {
-            // Queue the general convex pairs, then test them on the side
-            // stream while the closed-form pairs run here.
-            launch_convex_prep(d, s);
+        const bool side = w->convex_bodies && w->convex_side;
+        if (w->convex_bodies) launch_convex_prep(d, s);
+        if (side) {
+            // Test the general convex pairs on the side stream while the
+            // closed-form pairs run here.
             (void)hipEventRecord(w->fork, s);
             (void)hipStreamWaitEvent(w->side, w->fork, 0);
-            launch_convex(d, w->side);
+            launch_convex(d, w->convex_lds, w->side);
             (void)hipEventRecord(w->join, w->side);
         }
-        launch_narrowphase(d, s);
+        launch_narrowphase(d, w->np_block, s);
         if (e) (void)hipEventRecord(e[3], s);
-        if (w->convex_bodies) (void)hipStreamWaitEvent(s, w->join, 0);
+        if (side) (void)hipStreamWaitEvent(s, w->join, 0);
+        else if (w->convex_bodies) launch_convex(d, w->convex_lds, s);
         if (e) (void)hipEventRecord(e[4], s);
         launch_islands(d, w->scan_temp, w->scan_bytes, s);
         if (e) (void)hipEventRecord(e[5], s);

@@ -93,6 +93,7 @@ struct DeviceWorld {
                                // capsule: (radius, half length, 0); hull: (margin, first vertex, vertex count as uint bits)
     float4* hull_verts;        // vertices of every hull shape, back to back (xyz)
     uint32_t hull_nverts;      // vertices in hull_verts
+    uint32_t small_island_max; // islands of at most this many constraints are packed per wavefront
     float4 *bmin, *bmax;
     float4* stab;              // prev velocity, time below, slowing, allow stabilization
     float2* damp;              // linear, angular damping
@@ -209,10 +210,11 @@ __device__ __forceinline__ uint64_t local_pair_key(const DeviceWorld& w, uint64_
 // Host-side launchers (one per stage), defined in the kernel sources.
 void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
-void launch_narrowphase(const DeviceWorld& w, hipStream_t st);
+void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st);
 // General convex pairs queued by the narrowphase (GJK / MPR).
 void launch_convex_prep(const DeviceWorld& w, hipStream_t st);
-void launch_convex(const DeviceWorld& w, hipStream_t st);
+void launch_convex(const DeviceWorld& w, bool lds, hipStream_t st);
+constexpr uint32_t kConvexLdsMaxVerts = 4096;  // hull vertex pools up to 64 KB can be copied to LDS
 // Last step's constraint, contact, colour and penetration statistics into the counters.
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);
 void launch_islands(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);

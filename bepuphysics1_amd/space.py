@@ -480,6 +480,36 @@ class Space:
         """Space.BroadPhase setter (Space.cs:110-134): "grid" (default) or "tree"."""
         check(self._lib.bepu_set_broadphase(self._handle(), self.BROADPHASES[kind]), "set_broadphase")
 
+    TUNING_FIELDS = ("narrowphase_block", "convex_lds", "convex_side_stream", "small_island_max")
+
+    def tuning(self):
+        """The launch configuration (bepu_tuning) as a dict."""
+        t = _lib.Tuning()
+        check(self._lib.bepu_get_tuning(self._handle(), ctypes.byref(t)), "get_tuning")
+        return {f: int(getattr(t, f)) for f in self.TUNING_FIELDS}
+
+    def set_tuning(self, **values):
+        """Sets some fields of the launch configuration; results do not depend on it."""
+        t = _lib.Tuning()
+        check(self._lib.bepu_get_tuning(self._handle(), ctypes.byref(t)), "get_tuning")
+        for f, v in values.items():
+            if f not in self.TUNING_FIELDS:
+                raise KeyError(f)
+            setattr(t, f, int(v))
+        check(self._lib.bepu_set_tuning(self._handle(), ctypes.byref(t)), "set_tuning")
+
+    def autotune(self, trial_steps=2):
+        """Searches the launch configuration on this world's own steps (they
+        advance the simulation) and keeps the fastest; returns it with the
+        chosen configuration's time per step and the steps run."""
+        t = _lib.Tuning()
+        n = ctypes.c_uint32()
+        check(self._lib.bepu_autotune(self._handle(), int(trial_steps), ctypes.byref(t), ctypes.byref(n)), "autotune")
+        out = {f: int(getattr(t, f)) for f in self.TUNING_FIELDS}
+        out["ms_per_step"] = float(t.ms_per_step)
+        out["steps"] = int(n.value)
+        return out
+
     def set_timing(self, enable=True):
         check(self._lib.bepu_set_timing(self._handle(), 1 if enable else 0), "set_timing")
 

@@ -298,6 +298,41 @@ bepu_status bepu_get_stage_times(bepu_world world, float* ms, uint32_t capacity,
 #define BEPU_BROADPHASE_TREE 1  /* implicit tree over Morton order */
 bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
 
+/* Launch configuration of a world's step and its autotuner.  None of these
+ * choices changes any arithmetic: every choice gives bit-identical results,
+ * so the tuner times real steps of the world itself.  No reference
+ * counterpart (the reference's stages have no launch shapes); it is the
+ * autotuner over block size and colouring granularity of the batched form.
+ *   narrowphase_block  workgroup size of the closed-form narrowphase: 64, 128 or 256
+ *   convex_lds         1: each general convex workgroup copies the hull vertex pool to
+ *                      LDS (only when the pool holds at most 4,096 vertices); 0: support
+ *                      scans read global memory
+ *   convex_side_stream 1: general convex pairs are tested on a second stream beside the
+ *                      closed-form narrowphase; 0: after it on the world's stream
+ *   small_island_max   islands of at most this many constraints (32 or 64) are packed
+ *                      several to a wavefront; larger ones up to 256 take a workgroup each
+ *   ms_per_step        (filled by bepu_autotune) the chosen configuration's time */
+typedef struct bepu_tuning {
+    uint32_t narrowphase_block;
+    uint32_t convex_lds;
+    uint32_t convex_side_stream;
+    uint32_t small_island_max;
+    float ms_per_step;
+} bepu_tuning;
+
+bepu_status bepu_get_tuning(bepu_world world, bepu_tuning* out);
+/* Takes effect at the next step; BEPU_E_INVALID_ARG for a value outside the sets above. */
+bepu_status bepu_set_tuning(bepu_world world, const bepu_tuning* tuning);
+/* Steps the world while it searches the configuration, one choice at a time
+ * (coordinate descent from the current configuration): every candidate value
+ * runs `trial_steps` steps in each of two interleaved rounds, timed by HIP
+ * events, and the fastest is kept.  Runs 2 * trial_steps steps of the world
+ * per candidate value (18 * trial_steps with every choice open; the convex
+ * choices are open only in worlds with hulls or capsules), leaves the best
+ * configuration set, reports it in `out` and the steps run in `steps_run`
+ * (either may be NULL). */
+bepu_status bepu_autotune(bepu_world world, uint32_t trial_steps, bepu_tuning* out, uint32_t* steps_run);
+
 /* Library-wide information: 1 if a HIP device is usable. */
 int32_t bepu_device_available(void);
 const char* bepu_status_string(bepu_status status);

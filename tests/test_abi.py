@@ -48,14 +48,15 @@ def test_struct_sizes_match_header_layout(tmp_path):
         pytest.skip("no C compiler")
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "%s/include/bepu_gpu.h"\n'
-                   'int main(){printf("%%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu", sizeof(bepu_settings),'
+                   'int main(){printf("%%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu", sizeof(bepu_settings),'
                    'sizeof(bepu_motion_state), sizeof(bepu_material), sizeof(bepu_contact), sizeof(bepu_manifold),'
-                   'sizeof(bepu_impulses), sizeof(bepu_step_stats), sizeof(bepu_body_desc));}' % ROOT)
+                   'sizeof(bepu_impulses), sizeof(bepu_step_stats), sizeof(bepu_body_desc), sizeof(bepu_tuning));}'
+                   % ROOT)
     exe = tmp_path / "sz"
     subprocess.run([cc, str(src), "-o", str(exe)], check=True)
     sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     mine = [ctypes.sizeof(t) for t in (_lib.Settings, _lib.MotionState, _lib.Material, _lib.Contact,
-                                       _lib.Manifold, _lib.Impulses, _lib.StepStats, _lib.BodyDesc)]
+                                       _lib.Manifold, _lib.Impulses, _lib.StepStats, _lib.BodyDesc, _lib.Tuning)]
     assert mine == sizes
 
 

@@ -117,3 +117,39 @@ def test_served_hull_worlds_match_alone():
         assert np.array_equal(f32bits(r.result), f32bits(g.motion_states())), "request %d" % r.rid
         g.close()
     srv.close()
+
+
+def test_launch_configurations_give_identical_results():
+    """Every launch configuration the autotuner chooses from steps a world to
+    the same bits, and the autotuner leaves a valid one set."""
+    from bepuphysics1_amd import Space
+    worlds = scenes.batched_worlds(24, seed=31, nmin=8, nmax=120, shapes=ALL_SHAPES)
+    configs = [dict(narrowphase_block=64, convex_lds=0, convex_side_stream=0, small_island_max=32),
+               dict(narrowphase_block=256, convex_lds=1, convex_side_stream=1, small_island_max=64),
+               dict(narrowphase_block=128, convex_lds=1, convex_side_stream=0, small_island_max=32)]
+    results = []
+    for cfg in configs:
+        sp = Space(capacity=sum(len(w) for w in worlds))
+        sp.register_hulls(scenes.hull_library())
+        for w in worlds:
+            sp.add_world(w)
+        sp.set_tuning(**cfg)
+        sp.Update(40)
+        assert sp.tuning() == cfg
+        results.append((f32bits(sp.motion_states()), impulse_table(sp.impulses())))
+        sp.close()
+    for ms, imp in results[1:]:
+        assert np.array_equal(ms, results[0][0])
+        assert imp == results[0][1]
+    # The tuner steps the world with the candidates: same bits as plain steps.
+    sp = Space(capacity=sum(len(w) for w in worlds))
+    sp.register_hulls(scenes.hull_library())
+    for w in worlds:
+        sp.add_world(w)
+    sp.Update(10)
+    chosen = sp.autotune(trial_steps=1)
+    assert chosen["steps"] == 18 and chosen["ms_per_step"] > 0
+    assert chosen["narrowphase_block"] in (64, 128, 256) and chosen["small_island_max"] in (32, 64)
+    sp.Update(40 - 10 - chosen["steps"])
+    assert np.array_equal(f32bits(sp.motion_states()), results[0][0])
+    sp.close()
