@@ -191,12 +191,23 @@ constexpr int kConvexLdsBlock = 256;
 template <bool kLds, int kBlock>
 __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
     extern __shared__ float4 lds_hull[];
-    // The three state regions of the work list, costliest first: Deep pairs
-    // (MPR with surface casts and refinement), then Shallow (closest-point
-    // GJK), then Separated (boolean GJK).  Workgroups start in index order, so
-    // the long ones start first and the short ones fill in behind them.
-    uint32_t n2 = w.counters[ctr(CTR_CONVEX_DEEP)], n1 = w.counters[ctr(CTR_CONVEX_SHALLOW)];
-    uint32_t n = n2 + n1 + w.counters[ctr(CTR_CONVEX)];
+    // The state regions of the work list, costliest first: Deep pairs (MPR
+    // with surface casts and refinement), then Shallow (closest-point GJK),
+    // then Separated (boolean GJK).  Workgroups start in index order, so the
+    // long ones start first and the short ones fill in behind them.
+    // Six lists: per state, the pairs of two hulls (from the region's front)
+    // then the others (from its back).
+    uint32_t cnt[6], beg[7];
+    cnt[0] = w.counters[ctr(CTR_CONVEX_DEEP)];
+    cnt[1] = w.counters[ctr(CTR_CONVEX_OTHER0 + 2)];
+    cnt[2] = w.counters[ctr(CTR_CONVEX_SHALLOW)];
+    cnt[3] = w.counters[ctr(CTR_CONVEX_OTHER0 + 1)];
+    cnt[4] = w.counters[ctr(CTR_CONVEX)];
+    cnt[5] = w.counters[ctr(CTR_CONVEX_OTHER0)];
+    beg[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) beg[q + 1] = beg[q] + cnt[q];
+    const uint32_t n = beg[6];
     if (blockIdx.x * kBlock >= n) return;  // uniform per workgroup
     const float4* pool = w.hull_verts;
     if (kLds) {
@@ -205,9 +216,12 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
         pool = lds_hull;
     }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        size_t at = i < n2 ? 2 * (size_t)w.pair_cap + i
-                           : (i < n2 + n1 ? (size_t)w.pair_cap + (i - n2) : (size_t)(i - n2 - n1));
-        uint2 wk = w.conv_work[at];
+        int q = 0;
+#pragma unroll
+        for (int k = 1; k < 6; ++k) q += i >= beg[k];
+        const uint32_t state = 2 - (uint32_t)(q >> 1), k = i - beg[q];
+        const uint32_t at = (q & 1) ? w.pair_cap - 1 - k : k;
+        uint2 wk = w.conv_work[(size_t)state * w.pair_cap + at];
         // The pair's state is updated in place in global memory: its contact
         // slots are indexed at run time, which would put a private copy in
         // scratch memory.

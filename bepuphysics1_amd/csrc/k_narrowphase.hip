@@ -680,28 +680,41 @@ __global__ void __launch_bounds__(kNpBlock) k_convex_prep(DeviceWorld w) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     int cstate = -1;
+    bool two_hulls = false;
     uint32_t old_slot = ~0u;
     if (t < npairs) {
         uint64_t key = w.pair_keys[t];
-        if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) {
+        uint32_t i = (uint32_t)(key >> 32), j = (uint32_t)key;
+        if (general_convex_pair(w, i, j)) {
             int old = begin_pair(w, key, t);
             cstate = old >= 0 ? __float_as_int(w.conv_old[old].sep.w) : 0;
             if (cstate != 1 && cstate != 2) cstate = 0;
             if (old >= 0) old_slot = (uint32_t)old;
+            two_hulls = (int)w.shape[i].w == BEPU_SHAPE_CONVEX_HULL && (int)w.shape[j].w == BEPU_SHAPE_CONVEX_HULL;
         }
     }
+    // Six lists (state x two hulls or not), so that a wavefront of k_convex
+    // mostly runs one branch of the state machine with one pair of support
+    // mappings: pairs of two hulls fill a state's region from its front, the
+    // others from its back.
     const int lane = __lane_id();
-    for (int q = 0; q < 3; ++q) {
-        uint64_t mine = __ballot(cstate == q);
+    for (int q = 0; q < 6; ++q) {
+        const int state = q >> 1;
+        const bool front = (q & 1) == 0;
+        const bool me = cstate == state && two_hulls == front;
+        uint64_t mine = __ballot(me);
         if (!mine) continue;
         int leader = __ffsll((unsigned long long)mine) - 1;
         uint32_t base = 0;
-        uint32_t c = q == 0 ? CTR_CONVEX : (q == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP);
+        uint32_t c = front ? (state == 0 ? CTR_CONVEX : (state == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP))
+                           : CTR_CONVEX_OTHER0 + (uint32_t)state;
         if (lane == leader) base = atomicAdd(&w.counters[ctr(c)], (uint32_t)__popcll(mine));
         base = __shfl(base, leader);
-        if (cstate == q)
-            w.conv_work[(size_t)q * w.pair_cap + base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull))] =
-                make_uint2(t, old_slot);
+        if (me) {
+            uint32_t k = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
+            uint32_t at = front ? k : w.pair_cap - 1 - k;
+            w.conv_work[(size_t)state * w.pair_cap + at] = make_uint2(t, old_slot);
+        }
     }
 }
 

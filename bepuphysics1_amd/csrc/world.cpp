@@ -999,9 +999,9 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     std::memcpy(&mp, &c[CTR_MAXPEN], 4);
     out->max_penetration = any ? mp : 0.0f;
     out->global_manifolds = any ? std::min(c[CTR_GCONS], w->d.pair_cap) : 0;
-    out->convex_pairs[0] = any ? c[CTR_CONVEX] : 0;
-    out->convex_pairs[1] = any ? c[CTR_CONVEX_SHALLOW] : 0;
-    out->convex_pairs[2] = any ? c[CTR_CONVEX_DEEP] : 0;
+    out->convex_pairs[0] = any ? c[CTR_CONVEX] + c[CTR_CONVEX_OTHER0] : 0;
+    out->convex_pairs[1] = any ? c[CTR_CONVEX_SHALLOW] + c[CTR_CONVEX_OTHER0 + 1] : 0;
+    out->convex_pairs[2] = any ? c[CTR_CONVEX_DEEP] + c[CTR_CONVEX_OTHER0 + 2] : 0;
     if (c[CTR_GBAR_ABORT]) return BEPU_E_DEVICE;  // the global solver's grid was not resident
     return c[CTR_OVERFLOW] ? BEPU_E_CAPACITY : BEPU_OK;
 }

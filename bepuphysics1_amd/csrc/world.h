@@ -133,7 +133,8 @@ struct DeviceWorld {
     ConvexState* conv_cur;     // general convex pairs: this step's extra state (index = overlap index)
     ConvexState* conv_old;     // last step's
     uint2* conv_work;          // this step's general convex pairs: (overlap index, last step's index or ~0),
-                               // in three regions of pair_cap entries by the pair tester's state
+                               // in three regions of pair_cap entries by the pair tester's state; a
+                               // region holds pairs of two hulls from its front, the others from its back
     uint64_t* hash_keys_cur;
     uint32_t* hash_vals_cur;
     uint64_t* hash_keys_old;
@@ -181,9 +182,9 @@ enum {
     CTR_MAXPEN = 4,       // float bits (non-negative), max via atomicMax on uint
     CTR_OVERFLOW = 5,     // pair/hash capacity overflow flag
     CTR_ROUND = 6,        // colouring rounds
-    CTR_CONVEX = 7,       // general convex pairs queued for k_convex this step: Separated,
+    CTR_CONVEX = 7,       // general convex pairs of two hulls queued for k_convex this step: Separated,
     CTR_CONVEX_SHALLOW = 12,  // Shallow and
-    CTR_CONVEX_DEEP = 14,     // Deep tester state
+    CTR_CONVEX_DEEP = 14,     // Deep tester state (the other general convex pairs: CTR_CONVEX_OTHER0 + state)
     CTR_UNCOLORED0 = 8,   // 8..10: uncoloured counts of rotating colouring rounds
     CTR_GCONS = 11,       // constraints on the global path
     CTR_ISL_MEDIUM = 13,
@@ -191,7 +192,8 @@ enum {
     CTR_CLS0 = 16,        // 16..79: small islands per packing class
     CTR_SIDE0 = 80,       // 80..143: bodies per half-octave bin of the largest AABB side
     CTR_GBAR_ABORT = 144, // set when a grid barrier wait ran out of time
-    CTR_COUNT = 145
+    CTR_CONVEX_OTHER0 = 145,  // 145..147: general convex pairs with at most one hull, per tester state
+    CTR_COUNT = 148
 };
 
 // Every counter sits on its own 256-byte line: device atomics on one line
