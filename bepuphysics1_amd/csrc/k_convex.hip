@@ -102,7 +102,7 @@ __device__ __forceinline__ void add_general(PairState& s, ConvexState& e, f3 pos
     while (s.flags & (16u << slot)) ++slot;
     e.offa[slot] = f4_of(cvx::rt_transform_by_inverse(pos, ta), depth);
     e.offb[slot] = f4_of(cvx::rt_transform_by_inverse(pos, tb), 0.0f);
-    add_contact(s, pos, nrm, depth, id);
+    add_contact_inplace(s, pos, nrm, depth, id);
 }
 
 // GeneralConvexContactManifold.Update (GeneralConvexContactManifold.cs:71-126) with
@@ -133,7 +133,7 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, const float
             else s.cpos[slot] = f4_of((nb + na) * .5f, depth);
         }
     }
-    for (int i = nrem - 1; i >= 0; i--) remove_contact(s, rem[i]);
+    for (int i = nrem - 1; i >= 0; i--) remove_contact_inplace(s, rem[i]);
     cvx::TesterState st{__float_as_int(e.sep.w), f3_of(e.sep), f3_of(e.dir)};
     // Keep the state opaque to the optimiser: a state known on one path (a new
     // pair starts Separated) would otherwise get its own clone of the whole
@@ -146,7 +146,7 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, const float
                        &nrm, &depth, &id)) {
         for (int i = 0; i < s.ccount; ++i)
             if (dot3(f3_of(s.cnrm[s.clist[i]]), nrm) < 0) {
-                remove_contact(s, i);
+                remove_contact_inplace(s, i);
                 break;
             }
         bool unique = true;
@@ -165,14 +165,14 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, const float
             if (s.ccount == 4) {
                 bool add;
                 int r = reduce_contacts(s, pos, depth, &add);
-                if (r >= 0) remove_contact(s, r);
+                if (r >= 0) remove_contact_inplace(s, r);
                 if (add) add_general(s, e, pos, nrm, depth, id, ta, tb);
             } else {
                 add_general(s, e, pos, nrm, depth, id, ta, tb);
             }
         }
     } else {
-        for (int i = s.ccount - 1; i >= 0; i--) remove_contact(s, i);
+        for (int i = s.ccount - 1; i >= 0; i--) remove_contact_inplace(s, i);
     }
     e.sep = f4_of(st.separating_axis, __int_as_float(st.state));
     e.dir = f4_of(st.local_direction, 0.0f);

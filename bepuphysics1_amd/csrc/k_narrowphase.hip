@@ -509,7 +509,7 @@ __device__ bool box_sphere(f3 h, f3 bp, q4 bq, float r, f3 sp, float maxd, f3* p
     return true;
 }
 
-__device__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m) {
+__device__ __forceinline__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m) {
     float4 PA = w.pos[s.a], PB = w.pos[s.b], QA = w.quat[s.a], QB = w.quat[s.b];
     float4 SA = w.shape[s.a], SB = w.shape[s.b];
     f3 axis;
@@ -517,34 +517,42 @@ __device__ void update_box_pair(const DeviceWorld& w, PairState& s, LdsList& m) 
     if (box_box(f3_of(SA), f3_of(SB), f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}, f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w},
                 &axis, m)) {
         axis = -axis;
-        int rem[4], nrem = 0;
-        for (int i = 0; i < s.ccount; i++) {
-            int slot = s.clist[i];
-            int id = __float_as_int(s.cnrm[slot].w);
+        // Contacts whose id is no longer found, removed afterwards in
+        // descending list order (indices fixed, so the state stays in registers).
+        uint32_t rem = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (i >= s.ccount) break;
+            const int slot = s.clist[i];
+            const int id = __float_as_int(get4(s.cnrm, slot).w);
             bool found = false;
             for (int j = m.n - 1; j >= 0; j--) {
                 if (m.I(j) == id) {
                     found = true;
                     float4 c = m.P(j);
-                    s.cpos[slot] = make_float4(c.x, c.y, c.z, -c.w);
-                    s.cnrm[slot] = f4_of(axis, __int_as_float(id));
+                    set4(s.cpos, slot, make_float4(c.x, c.y, c.z, -c.w));
+                    set4(s.cnrm, slot, f4_of(axis, __int_as_float(id)));
                     m.remove_at(j);
                     break;
                 }
             }
-            if (!found) rem[nrem++] = i;
+            if (!found) rem |= 1u << i;
         }
-        for (int i = nrem - 1; i >= 0; i--) remove_contact(s, rem[i]);
+#pragma unroll
+        for (int i = 3; i >= 0; i--)
+            if (rem & (1u << i)) remove_contact(s, i);
         for (int i = 0; i < m.n; i++) {
             float4 c = m.P(i);
             add_contact(s, f3_of(c), axis, -c.w, m.I(i));
         }
     } else {
-        for (int i = s.ccount - 1; i >= 0; i--) remove_contact(s, i);
+#pragma unroll
+        for (int i = 3; i >= 0; i--)
+            if (i < s.ccount) remove_contact(s, i);
     }
 }
 
-__device__ void update_single_pair(const DeviceWorld& w, PairState& s) {
+__device__ __forceinline__ void update_single_pair(const DeviceWorld& w, PairState& s) {
     float4 PA = w.pos[s.a], PB = w.pos[s.b];
     float4 SA = w.shape[s.a], SB = w.shape[s.b];
     f3 pos, nrm;
@@ -564,9 +572,9 @@ __device__ void update_single_pair(const DeviceWorld& w, PairState& s) {
             add_contact(s, pos, nrm, depth, 0);
             colliding = true;
         } else if (prev) {
-            int slot = s.clist[0];
-            s.cpos[slot] = f4_of(pos, depth);
-            s.cnrm[slot] = f4_of(nrm, s.cnrm[slot].w);
+            const int slot = s.clist[0];
+            set4(s.cpos, slot, f4_of(pos, depth));
+            set4(s.cnrm, slot, f4_of(nrm, get4(s.cnrm, slot).w));
             colliding = true;
         }
     } else if (prev) {
@@ -575,7 +583,7 @@ __device__ void update_single_pair(const DeviceWorld& w, PairState& s) {
     s.flags = colliding ? (s.flags | PF_PREV_COLLIDING) : (s.flags & ~PF_PREV_COLLIDING);
 }
 
-__device__ void init_pair(const DeviceWorld& w, uint64_t key, PairState& s) {
+__device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, PairState& s) {
     uint32_t i = (uint32_t)(key >> 32), j = (uint32_t)(key & 0xffffffffu);
     int si = (int)w.shape[i].w, sj = (int)w.shape[j].w;
     s.key = key;
@@ -729,14 +737,24 @@ __global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
     if (t >= npairs) return;
     uint64_t key = w.pair_keys[t];
     if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) return;
-    begin_pair(w, key, t);
-    PairState& s = w.pairs_cur[t];
+    // A body inserted for this step (possibly into ids a removed world used)
+    // has no history: never pick up the previous step's state for its pairs.
+    uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
+    bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
+    int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
+    // The state lives in registers from the load to the store (every contact
+    // slot index is resolved through selects).
+    PairState s;
+    if (old >= 0) s = w.pairs_old[old];
+    else init_pair(w, key, s);
     if (s.kind == BEPU_PAIR_BOX_BOX) {
         LdsList m{&lds_pd[threadIdx.x], &lds_id[threadIdx.x], 0, BLOCK};
         update_box_pair(w, s, m);
     } else {
         update_single_pair(w, s);
     }
+    w.pairs_cur[t] = s;
+    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
     // A constraint (a pair with contacts) joins the islands of its two bodies
     // when both are dynamic; the union-find forest was reset by this step's
     // body pass.
