@@ -560,7 +560,7 @@ def main():
                                "dropped onto own grounds, dt 1/60, %d iterations, one step of every world per "
                                "bepu_step" % (a.worlds, BATCHED_ITERATIONS),
                    "worlds": a.worlds, "bodies": b["bodies"], "pairs": st.pairs, "manifolds": st.manifolds,
-                   "contacts": st.contacts, "convex_pairs_by_state": list(st.convex_pairs),
+                   "contacts": st.contacts, "colors": st.colors, "convex_pairs_by_state": list(st.convex_pairs),
                    "autotune": b["tuning"]},
         "roofline": roofline(stages, counts, "batched"),
         "step_roofline": {"alg_bytes_per_step": int(step_alg),

@@ -260,11 +260,19 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
     };
     // Prestep reads velocities only; no body changes before every lane is done.
     if (have) {
-        PairState& s = w.pairs_cur[slot];
+        // On a register copy of the pair state (one load of all of it); the
+        // fields the prestep updates (remembered friction directions) and the
+        // colour go back.
+        PairState s = w.pairs_cur[slot];
         load(sa, A);
         load(sb, B);
         prestep_row(st, s, f3_of(w.pos[ga]), f3_of(w.pos[gb]), A, B, row, var);
-        s.color = color;
+        PairState& g = w.pairs_cur[slot];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) g.frlin[k] = s.frlin[k];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) g.sllin[q] = s.sllin[q];
+        g.color = color;
     }
     __syncthreads();
     for (int col = 0; col < ncol; ++col) {

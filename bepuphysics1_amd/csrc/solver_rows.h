@@ -10,6 +10,7 @@
 // (ContactFrictionConstraint.cs:131-287) with the reference's operation order;
 // the early-out follows SolverGroup.SolveUpdateable (SolverGroup.cs:117-150).
 #pragma once
+#include "manifold.h"
 #include "world.h"
 
 namespace bepu {
@@ -129,7 +130,7 @@ BEPU_D void prestep_row(const bepu_settings& st, PairState& s, f3 pA, f3 pB, con
         v.facc[k] = s.fracc[k];
         if (k >= npen) continue;
         int cs = s.pslot[k];
-        float4 cp = s.cpos[cs], cn = s.cnrm[cs];
+        float4 cp = get4(s.cpos, cs), cn = get4(s.cnrm, cs);
         f3 lin{-cn.x, -cn.y, -cn.z};
         f3 ra = f3_of(cp) - pA;
         f3 aA{(ra.y * lin.z) - (ra.z * lin.y), (ra.z * lin.x) - (ra.x * lin.z), (ra.x * lin.y) - (ra.y * lin.x)};
@@ -207,17 +208,18 @@ BEPU_D void prestep_row(const bepu_settings& st, PairState& s, f3 pA, f3 pB, con
     if (convex && group && (s.flags & PF_FRICTION_ACTIVE)) {
         // SlidingFrictionTwoAxis.Update (SlidingFrictionTwoAxis.cs:242-440)
         f3 center;
-        f3 c0 = f3_of(s.cpos[s.pslot[0]]);
+        const f3 c0 = f3_of(get4(s.cpos, s.pslot[0])), c1 = f3_of(get4(s.cpos, s.pslot[1])),
+                 c2 = f3_of(get4(s.cpos, s.pslot[2])), c3 = f3_of(get4(s.cpos, s.pslot[3]));
         if (npen == 1) center = c0;
-        else if (npen == 2) center = (c0 + f3_of(s.cpos[s.pslot[1]])) * .5f;
-        else if (npen == 3) center = (f3_of(s.cpos[s.pslot[2]]) + (c0 + f3_of(s.cpos[s.pslot[1]]))) * .333333333f;
-        else center = (f3_of(s.cpos[s.pslot[3]]) + (f3_of(s.cpos[s.pslot[2]]) + (c0 + f3_of(s.cpos[s.pslot[1]])))) * .25f;
+        else if (npen == 2) center = (c0 + c1) * .5f;
+        else if (npen == 3) center = (c2 + (c0 + c1)) * .333333333f;
+        else center = (c3 + (c2 + (c0 + c1))) * .25f;
         f3 ra = center - pA;
         f3 vA = cross3(A.w, ra) + A.v;
         f3 rb = center - pB;
         f3 vB = cross3(B.w, rb) + B.v;
         f3 rv = vA - vB;
-        f3 n = f3_of(s.cnrm[s.pslot[0]]);
+        f3 n = f3_of(get4(s.cnrm, s.pslot[0]));
         float nvs = n.x * rv.x + n.y * rv.y + n.z * rv.z;
         rv.x -= nvs * n.x;
         rv.y -= nvs * n.y;
@@ -297,7 +299,10 @@ BEPU_D void prestep_row(const bepu_settings& st, PairState& s, f3 pA, f3 pB, con
         r.c.tfric = tf * st.twist_friction_factor;
         r.c.tang = n;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) r.c.lever[k] = k < npen ? len3(f3_of(s.cpos[s.pslot[k]]) - center) : 0.0f;
+        for (int k = 0; k < 4; ++k) {
+            const f3 ck = k == 0 ? c0 : (k == 1 ? c1 : (k == 2 ? c2 : c3));
+            r.c.lever[k] = k < npen ? len3(ck - center) : 0.0f;
+        }
         act |= 1u << 5;
     }
     v.act = act;
