@@ -753,9 +753,13 @@ bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, 
                 ran += trial_steps;
             }
         if (st != BEPU_OK) break;
+        // The current value is kept unless another is faster by more than 3%:
+        // the trial steps see a workload that changes from step to step.
         size_t best = 0;
-        for (size_t i = 1; i < total.size(); ++i)
-            if (total[i] < total[best]) best = i;
+        for (size_t i = 0; i < total.size(); ++i)
+            if (c.values[i] == cur.*(c.field)) best = i;
+        for (size_t i = 0; i < total.size(); ++i)
+            if (total[i] < 0.97 * total[best]) best = i;
         cur.*(c.field) = c.values[best];
         best_ms = (float)(total[best] / (2.0 * trial_steps));
     }
@@ -801,7 +805,9 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[1], s);
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
         if (e) (void)hipEventRecord(e[2], s);
-        const bool side = w->convex_bodies && w->convex_side;
+        // Steps with stage events run the general convex tests after the
+        // closed-form narrowphase, so that each stage's time is its own.
+        const bool side = w->convex_bodies && w->convex_side && !e;
         if (w->convex_bodies) launch_convex_prep(d, s);
         if (side) {
             // Test the general convex pairs on the side stream while the

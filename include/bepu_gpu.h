@@ -280,7 +280,9 @@ bepu_status bepu_get_stats(bepu_world world, bepu_step_stats* out);
 
 /* Stage timers.  With timing enabled every step of a bepu_step call records
  * HIP events between its stages on the world's stream (the events cost device
- * time of their own); with timing off no events are recorded and
+ * time of their own, and the general convex tests then run after the
+ * closed-form narrowphase instead of beside it, so that each stage's time is
+ * its own); with timing off no events are recorded and
  * bepu_get_stage_times reports the last timed call.  It returns the average device time per
  * stage (milliseconds) over those steps; `names` receives a pointer to a
  * static, comma separated list of stage names matching `ms`.  No reference
@@ -326,7 +328,8 @@ bepu_status bepu_set_tuning(bepu_world world, const bepu_tuning* tuning);
 /* Steps the world while it searches the configuration, one choice at a time
  * (coordinate descent from the current configuration): every candidate value
  * runs `trial_steps` steps in each of two interleaved rounds, timed by HIP
- * events, and the fastest is kept.  Runs 2 * trial_steps steps of the world
+ * events, and the fastest is kept if it beats the current value by more than
+ * 3%.  Runs 2 * trial_steps steps of the world
  * per candidate value (18 * trial_steps with every choice open; the convex
  * choices are open only in worlds with hulls or capsules), leaves the best
  * configuration set, reports it in `out` and the steps run in `steps_run`
