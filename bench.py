@@ -176,9 +176,21 @@ def batched_run(a):
     sp.register_hulls(scenes.hull_library())
     for w in worlds:
         sp.add_world(w)
-    # The autotuner picks the launch configuration on the batch's own first
-    # steps (untimed; every configuration gives bit-identical results).
-    tune = sp.autotune(a.autotune_steps) if a.autotune_steps > 0 else dict(sp.tuning(), steps=0)
+    tune = dict(sp.tuning(), steps=0)
+    if a.autotune_steps > 0:
+        # The autotuner picks the launch configuration on a copy of the batch
+        # stepped into its settled state (most of the timed window is settled
+        # piles, whose costs differ from the falling ones), untimed; every
+        # configuration gives bit-identical results.
+        tsp = Space(capacity=sum(len(w) for w in worlds), settings=batched_settings())
+        tsp.register_hulls(scenes.hull_library())
+        for w in worlds:
+            tsp.add_world(w)
+        tsp.Update(a.autotune_after)
+        tune = tsp.autotune(a.autotune_steps)
+        tsp.close()
+        sp.set_tuning(**{k: tune[k] for k in Space.TUNING_FIELDS})
+        tune["tuned_after_steps"] = a.autotune_after
     if a.warmup:
         sp.Update(a.warmup)
     sp.stats()  # synchronise
@@ -482,8 +494,10 @@ def main():
     ap.add_argument("--steps", type=int, default=300, help="timed batched steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--autotune-steps", type=int, default=1,
-                    help="steps per candidate and round of the launch autotuner before the warm-up (0: defaults)")
+    ap.add_argument("--autotune-steps", type=int, default=3,
+                    help="steps per candidate and round of the launch autotuner (0: default configuration)")
+    ap.add_argument("--autotune-after", type=int, default=160,
+                    help="steps a copy of the batch runs before the autotuner times candidates on it")
     ap.add_argument("--latency-steps", type=int, default=50)
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="worlds of the batch also stepped alone, for the batched/plain ratio (0: skip)")
@@ -539,8 +553,7 @@ def main():
         threads = max(1, min(16, os.cpu_count() or 1))
         # The oracle's sample sits in the middle of the GPU's timed window (the
         # piles settle over the run and later steps carry more contacts).
-        cpu_line = cpu_baseline_batched(worlds, a.cpu_worlds, b["tuning"]["steps"] + a.warmup + a.steps // 2,
-                                        a.cpu_steps, threads)
+        cpu_line = cpu_baseline_batched(worlds, a.cpu_worlds, a.warmup + a.steps // 2, a.cpu_steps, threads)
     parity = None if a.no_parity else parity_batched(16, 60)
     step_alg = step_bytes(**counts)
     line = {

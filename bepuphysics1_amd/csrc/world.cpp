@@ -717,12 +717,14 @@ bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, 
         uint32_t bepu_tuning::*field;
         std::vector<uint32_t> values;
     };
-    std::vector<Choice> choices = {{&bepu_tuning::narrowphase_block, {64, 128, 256}},
-                                   {&bepu_tuning::small_island_max, {32, 64}}};
+    // The general convex choices first: they move the most time.
+    std::vector<Choice> choices;
     if (w->convex_bodies) {
-        choices.push_back({&bepu_tuning::convex_side_stream, {0, 1}});
         if (w->d.hull_nverts <= kConvexLdsMaxVerts) choices.push_back({&bepu_tuning::convex_lds, {0, 1}});
+        choices.push_back({&bepu_tuning::convex_side_stream, {0, 1}});
     }
+    choices.push_back({&bepu_tuning::narrowphase_block, {64, 128, 256}});
+    choices.push_back({&bepu_tuning::small_island_max, {32, 64}});
     bepu_tuning cur;
     (void)bepu_get_tuning(w, &cur);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -743,23 +745,26 @@ bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, 
                 t.*(c.field) = c.values[i];
                 (void)bepu_set_tuning(w, &t);
                 float ms = 0;
-                if (hipEventRecord(e0, w->stream) != hipSuccess) st = BEPU_E_DEVICE;
+                // One untimed step first, then the timed ones queued behind it,
+                // so that the device never waits for the host between them.
+                st = bepu_step(w, 1);
+                if (st == BEPU_OK && hipEventRecord(e0, w->stream) != hipSuccess) st = BEPU_E_DEVICE;
                 if (st == BEPU_OK) st = bepu_step(w, (int32_t)trial_steps);
                 if (st == BEPU_OK && (hipEventRecord(e1, w->stream) != hipSuccess ||
                                       hipEventSynchronize(e1) != hipSuccess ||
                                       hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
                     st = BEPU_E_DEVICE;
                 total[i] += ms;
-                ran += trial_steps;
+                ran += trial_steps + 1;
             }
         if (st != BEPU_OK) break;
-        // The current value is kept unless another is faster by more than 3%:
+        // The current value is kept unless another is faster by more than 2%:
         // the trial steps see a workload that changes from step to step.
         size_t best = 0;
         for (size_t i = 0; i < total.size(); ++i)
             if (c.values[i] == cur.*(c.field)) best = i;
         for (size_t i = 0; i < total.size(); ++i)
-            if (total[i] < 0.97 * total[best]) best = i;
+            if (total[i] < 0.98 * total[best]) best = i;
         cur.*(c.field) = c.values[best];
         best_ms = (float)(total[best] / (2.0 * trial_steps));
     }

@@ -327,10 +327,11 @@ bepu_status bepu_get_tuning(bepu_world world, bepu_tuning* out);
 bepu_status bepu_set_tuning(bepu_world world, const bepu_tuning* tuning);
 /* Steps the world while it searches the configuration, one choice at a time
  * (coordinate descent from the current configuration): every candidate value
- * runs `trial_steps` steps in each of two interleaved rounds, timed by HIP
+ * runs one untimed step and then `trial_steps` timed steps in each of two
+ * interleaved rounds, timed by HIP
  * events, and the fastest is kept if it beats the current value by more than
- * 3%.  Runs 2 * trial_steps steps of the world
- * per candidate value (18 * trial_steps with every choice open; the convex
+ * 2%.  Runs 2 * (trial_steps + 1) steps of the world
+ * per candidate value (18 * (trial_steps + 1) with every choice open; the convex
  * choices are open only in worlds with hulls or capsules), leaves the best
  * configuration set, reports it in `out` and the steps run in `steps_run`
  * (either may be NULL). */

@@ -134,7 +134,7 @@ def test_launch_configurations_give_identical_results():
         for w in worlds:
             sp.add_world(w)
         sp.set_tuning(**cfg)
-        sp.Update(40)
+        sp.Update(60)
         assert sp.tuning() == cfg
         results.append((f32bits(sp.motion_states()), impulse_table(sp.impulses())))
         sp.close()
@@ -148,8 +148,8 @@ def test_launch_configurations_give_identical_results():
         sp.add_world(w)
     sp.Update(10)
     chosen = sp.autotune(trial_steps=1)
-    assert chosen["steps"] == 18 and chosen["ms_per_step"] > 0
+    assert chosen["steps"] == 36 and chosen["ms_per_step"] > 0
     assert chosen["narrowphase_block"] in (64, 128, 256) and chosen["small_island_max"] in (32, 64)
-    sp.Update(40 - 10 - chosen["steps"])
+    sp.Update(60 - 10 - chosen["steps"])
     assert np.array_equal(f32bits(sp.motion_states()), results[0][0])
     sp.close()
