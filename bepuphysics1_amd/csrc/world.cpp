@@ -54,7 +54,7 @@ struct bepu_world_s {
     hipEvent_t fork = nullptr, join = nullptr;
     // Launch configuration (bepu_tuning); small_island_max lives in d.
     uint32_t np_block = 128;
-    bool convex_lds = true;
+    bool convex_lds = false;
     bool convex_side = true;
     std::vector<HostBody> staged;   // bodies not yet uploaded
     uint32_t uploaded = 0;

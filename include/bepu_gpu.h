@@ -305,14 +305,14 @@ bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
  * so the tuner times real steps of the world itself.  No reference
  * counterpart (the reference's stages have no launch shapes); it is the
  * autotuner over block size and colouring granularity of the batched form.
- *   narrowphase_block  workgroup size of the closed-form narrowphase: 64, 128 or 256
+ *   narrowphase_block  workgroup size of the closed-form narrowphase: 64, 128 (default) or 256
  *   convex_lds         1: each general convex workgroup copies the hull vertex pool to
- *                      LDS (only when the pool holds at most 4,096 vertices); 0: support
- *                      scans read global memory
- *   convex_side_stream 1: general convex pairs are tested on a second stream beside the
- *                      closed-form narrowphase; 0: after it on the world's stream
- *   small_island_max   islands of at most this many constraints (32 or 64) are packed
- *                      several to a wavefront; larger ones up to 256 take a workgroup each
+ *                      LDS (only when the pool holds at most 4,096 vertices); 0 (default):
+ *                      support scans read global memory
+ *   convex_side_stream 1 (default): general convex pairs are tested on a second stream beside
+ *                      the closed-form narrowphase; 0: after it on the world's stream
+ *   small_island_max   islands of at most this many constraints (32 or 64, default 64) are
+ *                      packed several to a wavefront; larger ones up to 256 take a workgroup each
  *   ms_per_step        (filled by bepu_autotune) the chosen configuration's time */
 typedef struct bepu_tuning {
     uint32_t narrowphase_block;
