@@ -104,7 +104,7 @@ def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0
 # WRITE_SIZE per launch from rocprofv3 counter passes, scaled by a plain copy
 # of known size.
 STAGE_KERNELS = {
-    "island_solve": ["k_island_pack", "k_island_medium"],
+    "island_solve": ["k_island_solve"],
     "narrowphase": ["k_narrowphase"],
     "convex": ["k_convex"],
     "broadphase": ["k_grid_insert", "k_grid_query"],

@@ -136,6 +136,23 @@ struct IslandLds {
     int red[BLOCK / 64];
 };
 
+// Barrier of the lanes solving one island group.  A 64-lane group is one
+// wavefront, several of which may share a workgroup while working on
+// different bundles, so it must not wait for the other wavefronts: the LDS
+// accesses of one wavefront complete in issue order, and the fences keep the
+// compiler from moving them across this point.  Larger groups are the whole
+// workgroup.
+template <int BLOCK>
+__device__ __forceinline__ void group_sync() {
+    if constexpr (BLOCK == 64) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <int BLOCK, int NSLOT>
 __device__ __forceinline__ void lds_body_load(IslandLds<BLOCK, NSLOT>& L, const DeviceWorld& w, int l, uint32_t b) {
     L.v[l] = w.linv[b];
@@ -153,7 +170,7 @@ __device__ __forceinline__ void lds_body_load(IslandLds<BLOCK, NSLOT>& L, const 
 template <int BLOCK, int DYN, int NSLOT>
 __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSLOT>& L, bool have, uint32_t slot,
                                             uint32_t ga, uint32_t gb, int sa, int sb, int clo, int chi) {
-    const int tid = threadIdx.x;
+    const int tid = (int)threadIdx.x % BLOCK;
     const bepu_settings& st = w.s;
     // Colouring: smallest colour unused by higher-priority neighbours; rounds
     // until every constraint is set (each round sets the highest-priority
@@ -188,7 +205,7 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
                 else { color = kMaxColors - 1; w.counters[ctr(CTR_OVERFLOW)] = 4; }
                 L.color[tid] = color;
             }
-            __syncthreads();
+            group_sync<BLOCK>();
         }
     } else {
         // The higher-priority neighbours of each lane as a lane mask over the
@@ -274,7 +291,7 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
         for (int q = 0; q < 6; ++q) g.sllin[q] = s.sllin[q];
         g.color = color;
     }
-    __syncthreads();
+    group_sync<BLOCK>();
     for (int col = 0; col < ncol; ++col) {
         if (have && color == col) {
             load(sa, A);
@@ -283,7 +300,7 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
             store(sa, A);
             store(sb, B);
         }
-        __syncthreads();
+        group_sync<BLOCK>();
     }
     const float mi = st.min_impulse;
     const int mic = st.min_iteration_count;
@@ -296,7 +313,7 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
                 store(sa, A);
                 store(sb, B);
             }
-            __syncthreads();
+            group_sync<BLOCK>();
         }
     if (have) writeback_row(w.pairs_cur[slot], var);
 }
@@ -306,7 +323,7 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
 template <int BLOCK, int DYN, int NSLOT>
 __device__ __forceinline__ void setup_lane(const DeviceWorld& w, IslandLds<BLOCK, NSLOT>& L, bool have, uint32_t slot,
                                            int dbase, uint32_t& ga, uint32_t& gb, int& sa, int& sb) {
-    const int tid = threadIdx.x;
+    const int tid = (int)threadIdx.x % BLOCK;
     int la = -1, lb = -1;
     ga = gb = 0;
     sa = sb = 0;
@@ -328,15 +345,86 @@ __device__ __forceinline__ void setup_lane(const DeviceWorld& w, IslandLds<BLOCK
     L.color[tid] = -1;
 }
 
-// Small islands: one bundle of same-class islands per wavefront, persistent
-// over the bundles.
-// Two wavefronts per SIMD (a few registers spill) beat one: every bundle of a
-// large scene is resident at once and the two hide each other's latency.
-__global__ void __launch_bounds__(kPackLanes) __attribute__((amdgpu_waves_per_eu(2))) k_island_pack(DeviceWorld w) {
-    __shared__ IslandLds<kPackLanes, kPackSlots> L;
+using PackLds = IslandLds<kPackLanes, kPackSlots>;
+using MediumLds = IslandLds<kMediumBlock, kMediumSlots>;
+constexpr int kSolveBlock = kMediumBlock;                 // one medium island or four bundles
+constexpr int kSolveWaves = kSolveBlock / kPackLanes;
+constexpr size_t kSolveLds = sizeof(MediumLds) > kSolveWaves * sizeof(PackLds) ? sizeof(MediumLds)
+                                                                              : kSolveWaves * sizeof(PackLds);
+
+// One bundle of same-class small islands on one wavefront.  s_cnt[t] islands
+// are in class t (t + 1 islands per bundle); s_bund is the prefix of the
+// classes' bundle counts.
+__device__ __forceinline__ void pack_bundle(DeviceWorld& w, PackLds& L, const uint32_t* s_cnt, const uint32_t* s_bund,
+                                            uint32_t g) {
+    const uint32_t lane = threadIdx.x % kPackLanes;
+    // Class of bundle g: the last class whose first bundle is <= g.
+    int lo = 0, hi = 64;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (s_bund[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t per = (uint32_t)lo + 1;
+    const uint32_t lanes = 64u / per;               // >= nc of every island of the class
+    const uint32_t first = (g - s_bund[lo]) * per;
+    const uint32_t p = lane / lanes, q = lane % lanes;
+    const bool isl = p < per && first + p < s_cnt[lo];
+    uint32_t r = isl ? w.cls_list[(size_t)lo * w.cls_cap + first + p] : 0;
+    uint64_t s0 = isl ? w.isl_start[r] : 0, s1 = isl ? w.isl_start[r + 1] : 0;
+    const uint32_t cs = hi32(s0), nc = hi32(s1) - cs, bs = lo32(s0), nb = lo32(s1) - bs;
+    const int dbase = (int)(p * (lanes + 1));       // dynamic slots [dbase, dbase + nb), nb <= lanes + 1
+    for (uint32_t t = q; isl && t < nb; t += lanes) lds_body_load(L, w, dbase + (int)t, w.isl_bodies[bs + t]);
+    const bool have = isl && q < nc;
+    const uint32_t slot = have ? w.isl_cons[cs + q] : 0;
+    uint32_t ga, gb;
+    int sa, sb;
+    setup_lane<kPackLanes, kPackDyn, kPackSlots>(w, L, have, slot, dbase, ga, gb, sa, sb);
+    group_sync<kPackLanes>();
+    const int clo = (int)(p * lanes), chi = clo + (int)(isl ? nc : 0);
+    solve_lanes<kPackLanes, kPackDyn, kPackSlots>(w, L, have, slot, ga, gb, sa, sb, clo, chi);
+    for (uint32_t t = q; isl && t < nb; t += lanes) {
+        uint32_t b = w.isl_bodies[bs + t];
+        w.linv[b] = L.v[dbase + (int)t];
+        w.angv[b] = L.w[dbase + (int)t];
+    }
+    group_sync<kPackLanes>();
+}
+
+// One medium island on the whole workgroup.
+__device__ __forceinline__ void medium_island(DeviceWorld& w, MediumLds& L, uint32_t r) {
+    const int tid = threadIdx.x;
+    uint64_t s0 = w.isl_start[r], s1 = w.isl_start[r + 1];
+    const uint32_t cs = hi32(s0), nc = hi32(s1) - cs, bs = lo32(s0), nb = lo32(s1) - bs;
+    for (uint32_t t = tid; t < nb; t += kMediumBlock) lds_body_load(L, w, (int)t, w.isl_bodies[bs + t]);
+    const bool have = (uint32_t)tid < nc;
+    const uint32_t slot = have ? w.isl_cons[cs + tid] : 0;
+    uint32_t ga, gb;
+    int sa, sb;
+    setup_lane<kMediumBlock, kMediumDyn, kMediumSlots>(w, L, have, slot, 0, ga, gb, sa, sb);
+    __syncthreads();
+    solve_lanes<kMediumBlock, kMediumDyn, kMediumSlots>(w, L, have, slot, ga, gb, sa, sb, 0, (int)nc);
+    for (uint32_t t = tid; t < nb; t += kMediumBlock) {
+        uint32_t b = w.isl_bodies[bs + t];
+        w.linv[b] = L.v[t];
+        w.angv[b] = L.w[t];
+    }
+    __syncthreads();
+}
+
+// Small and medium islands in one persistent launch.  A medium island is a
+// long chain of dependent colour steps on one workgroup, so the first
+// workgroups (at most half of them) take the medium islands while the
+// wavefronts of the others take the bundles of small islands: the two kinds
+// overlap instead of the medium islands running alone on an idle device after
+// the bundles.  Two wavefronts per SIMD (a few registers spill) beat one:
+// every bundle of a large scene is resident at once and they hide each
+// other's latency.
+__global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_eu(2))) k_island_solve(DeviceWorld w) {
+    __shared__ __align__(16) unsigned char s_lds[kSolveLds];
     __shared__ uint32_t s_cnt[64], s_bund[65];
     const int tid = threadIdx.x;
-    {
+    if (tid < 64) {
         // Bundles per class (class t packs t + 1 islands) and their prefix.
         uint32_t c = w.counters[ctr(CTR_CLS0 + tid)];
         uint32_t nb = (c + (uint32_t)tid) / ((uint32_t)tid + 1);
@@ -350,66 +438,20 @@ __global__ void __launch_bounds__(kPackLanes) __attribute__((amdgpu_waves_per_eu
         if (tid == 63) s_bund[64] = a;
     }
     __syncthreads();
+    const uint32_t nmed = w.counters[ctr(CTR_ISL_MEDIUM)];
+    const uint32_t mgroups = min(nmed, gridDim.x / 2);
+    if (blockIdx.x < mgroups) {
+        MediumLds& L = *reinterpret_cast<MediumLds*>(s_lds);
+        for (uint32_t it = blockIdx.x; it < nmed; it += mgroups) medium_island(w, L, w.isl_medium[it]);
+        return;
+    }
+    // Bundle order runs across workgroups first, so consecutive bundles (of
+    // the same class, hence similar cost) go to different compute units.
+    const uint32_t groups = gridDim.x - mgroups, wave = (uint32_t)tid / kPackLanes;
+    PackLds& L = reinterpret_cast<PackLds*>(s_lds)[wave];
     const uint32_t total = s_bund[64];
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        // Class of bundle g: the last class whose first bundle is <= g.
-        int lo = 0, hi = 64;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (s_bund[mid] <= g) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t per = (uint32_t)lo + 1;
-        const uint32_t lanes = 64u / per;               // >= nc of every island of the class
-        const uint32_t first = (g - s_bund[lo]) * per;
-        const uint32_t p = (uint32_t)tid / lanes, q = (uint32_t)tid % lanes;
-        const bool isl = p < per && first + p < s_cnt[lo];
-        uint32_t r = isl ? w.cls_list[(size_t)lo * w.cls_cap + first + p] : 0;
-        uint64_t s0 = isl ? w.isl_start[r] : 0, s1 = isl ? w.isl_start[r + 1] : 0;
-        const uint32_t cs = hi32(s0), nc = hi32(s1) - cs, bs = lo32(s0), nb = lo32(s1) - bs;
-        const int dbase = (int)(p * (lanes + 1));       // dynamic slots [dbase, dbase + nb), nb <= lanes + 1
-        for (uint32_t t = q; isl && t < nb; t += lanes) lds_body_load(L, w, dbase + (int)t, w.isl_bodies[bs + t]);
-        const bool have = isl && q < nc;
-        const uint32_t slot = have ? w.isl_cons[cs + q] : 0;
-        uint32_t ga, gb;
-        int sa, sb;
-        setup_lane<kPackLanes, kPackDyn, kPackSlots>(w, L, have, slot, dbase, ga, gb, sa, sb);
-        __syncthreads();
-        const int clo = (int)(p * lanes), chi = clo + (int)(isl ? nc : 0);
-        solve_lanes<kPackLanes, kPackDyn, kPackSlots>(w, L, have, slot, ga, gb, sa, sb, clo, chi);
-        for (uint32_t t = q; isl && t < nb; t += lanes) {
-            uint32_t b = w.isl_bodies[bs + t];
-            w.linv[b] = L.v[dbase + (int)t];
-            w.angv[b] = L.w[dbase + (int)t];
-        }
-        __syncthreads();
-    }
-}
-
-// Medium islands: one island per workgroup, persistent over the list.
-__global__ void __launch_bounds__(kMediumBlock) k_island_medium(DeviceWorld w) {
-    __shared__ IslandLds<kMediumBlock, kMediumSlots> L;
-    const int tid = threadIdx.x;
-    const uint32_t count = w.counters[ctr(CTR_ISL_MEDIUM)];
-    for (uint32_t it = blockIdx.x; it < count; it += gridDim.x) {
-        uint32_t r = w.isl_medium[it];
-        uint64_t s0 = w.isl_start[r], s1 = w.isl_start[r + 1];
-        const uint32_t cs = hi32(s0), nc = hi32(s1) - cs, bs = lo32(s0), nb = lo32(s1) - bs;
-        for (uint32_t t = tid; t < nb; t += kMediumBlock) lds_body_load(L, w, (int)t, w.isl_bodies[bs + t]);
-        const bool have = (uint32_t)tid < nc;
-        const uint32_t slot = have ? w.isl_cons[cs + tid] : 0;
-        uint32_t ga, gb;
-        int sa, sb;
-        setup_lane<kMediumBlock, kMediumDyn, kMediumSlots>(w, L, have, slot, 0, ga, gb, sa, sb);
-        __syncthreads();
-        solve_lanes<kMediumBlock, kMediumDyn, kMediumSlots>(w, L, have, slot, ga, gb, sa, sb, 0, (int)nc);
-        for (uint32_t t = tid; t < nb; t += kMediumBlock) {
-            uint32_t b = w.isl_bodies[bs + t];
-            w.linv[b] = L.v[t];
-            w.angv[b] = L.w[t];
-        }
-        __syncthreads();
-    }
+    for (uint32_t g = wave * groups + (blockIdx.x - mgroups); g < total; g += groups * kSolveWaves)
+        pack_bundle(w, L, s_cnt, s_bund, g);
 }
 
 size_t island_scan_temp_bytes(uint32_t n) {
@@ -441,11 +483,9 @@ void launch_islands(const DeviceWorld& w, void* temp, size_t temp_bytes, hipStre
 }
 
 void launch_island_solve(const DeviceWorld& w, hipStream_t st) {
-    static int g_pack = 0, g_medium = 0;
-    if (!g_pack) g_pack = persistent_grid((const void*)k_island_pack, kPackLanes);
-    if (!g_medium) g_medium = persistent_grid((const void*)k_island_medium, kMediumBlock);
-    hipLaunchKernelGGL(k_island_pack, dim3(g_pack), dim3(kPackLanes), 0, st, w);
-    hipLaunchKernelGGL(k_island_medium, dim3(g_medium), dim3(kMediumBlock), 0, st, w);
+    static int g_solve = 0;
+    if (!g_solve) g_solve = persistent_grid((const void*)k_island_solve, kSolveBlock);
+    hipLaunchKernelGGL(k_island_solve, dim3(g_solve), dim3(kSolveBlock), 0, st, w);
 }
 
 }  // namespace bepu
