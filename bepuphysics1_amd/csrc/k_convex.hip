@@ -176,6 +176,7 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, const float
     }
     e.sep = f4_of(st.separating_axis, __int_as_float(st.state));
     e.dir = f4_of(st.local_direction, 0.0f);
+    s.tester = st.state;
 }
 
 constexpr int kConvexBlock = 64;

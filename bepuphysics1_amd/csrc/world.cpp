@@ -285,7 +285,9 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
     b.stab = make_float4(0, 0, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
     b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
-    b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT;
+    const bool hull = shape.w == (float)BEPU_SHAPE_CONVEX_HULL;
+    const bool general = hull || shape.w == (float)BEPU_SHAPE_CAPSULE;
+    b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT | (general ? BF_GENERAL : 0u) | (hull ? BF_HULL : 0u);
     b.born = w->steps;
     return BEPU_OK;
 }

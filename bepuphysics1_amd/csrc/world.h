@@ -44,7 +44,8 @@ struct PairState {
     float twacc;          // TwistFrictionConstraint accumulated impulse
     int32_t color;        // colour of the constraint in the last solve (-1 before)
     float sllin[6];       // SlidingFrictionTwoAxis linearA (remembered across steps)
-    float pad1[2];
+    int32_t tester;       // general convex pairs: the pair tester's state (ConvexState::sep.w) after the last test
+    float pad1;
 };
 
 enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u };
@@ -60,7 +61,9 @@ struct ConvexState {
     float4 offa[4];
     float4 offb[4];
 };
-enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u };
+// BF_GENERAL: a shape whose pairs take the general convex tester (hull or
+// capsule); BF_HULL: a convex hull.
+enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u };
 
 // One body as built on the host (Entity construction), copied to the device
 // in one block and scattered into the structure-of-arrays by k_scatter_bodies.
