@@ -598,7 +598,8 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
     s.twacc = 0;
     s.color = -1;
     for (int k = 0; k < 6; ++k) s.sllin[k] = 0;
-    s.pad1[0] = s.pad1[1] = 0;
+    s.tester = 0;
+    s.pad1 = 0;
     s.a = i;
     s.b = j;
     if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) s.kind = BEPU_PAIR_BOX_BOX;
@@ -618,16 +619,21 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
 // memory, and a plain loop waits for each load before its store, since the
 // arrays might overlap).  The memory clobber keeps later field accesses of
 // the destination after the stores.
-__device__ __forceinline__ void copy_pair_state(PairState* dst, const PairState* src) {
+// Copies a pair state; returns its `tester` field (read with the copy).
+__device__ __forceinline__ int copy_pair_state(PairState* dst, const PairState* src) {
     static_assert(sizeof(PairState) == 21 * 16 && alignof(PairState) == 16, "PairState is copied as 21 float4s");
+    static_assert(offsetof(PairState, tester) == 20 * 16 + 8, "tester is the third word of the last float4");
     const float4* a = reinterpret_cast<const float4*>(src);
     float4* b = reinterpret_cast<float4*>(dst);
+    float4 last = make_float4(0, 0, 0, 0);
 #pragma unroll
     for (int c = 0; c < 21; c += 7) {
         float4 r0 = a[c], r1 = a[c + 1], r2 = a[c + 2], r3 = a[c + 3], r4 = a[c + 4], r5 = a[c + 5], r6 = a[c + 6];
         b[c] = r0; b[c + 1] = r1; b[c + 2] = r2; b[c + 3] = r3; b[c + 4] = r4; b[c + 5] = r5; b[c + 6] = r6;
+        last = r6;
     }
     asm volatile("" ::: "memory");
+    return __float_as_int(last.z);
 }
 
 __device__ __forceinline__ uint32_t hash_slot(uint64_t key, uint32_t cap) { return (uint32_t)mix64(key) & (cap - 1); }
@@ -659,32 +665,39 @@ __device__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64
 // Pairs with a hull or a capsule take the general convex path
 // (NarrowPhaseHelper.cs:360-367); boxes and spheres have closed-form testers.
 __device__ __forceinline__ bool general_convex_pair(const DeviceWorld& w, uint32_t i, uint32_t j) {
-    int si = (int)w.shape[i].w, sj = (int)w.shape[j].w;
-    return !((si == BEPU_SHAPE_BOX || si == BEPU_SHAPE_SPHERE) && (sj == BEPU_SHAPE_BOX || sj == BEPU_SHAPE_SPHERE));
+    return ((w.bflags[i] | w.bflags[j]) & BF_GENERAL) != 0;
 }
 
 // This step's state of a pair in slot t: last step's (found in last step's
 // key -> slot table), or a fresh one.  The state is updated in place in the
 // slot: its contact slots are indexed at run time, which would put a private
-// copy in scratch memory.  Returns last step's slot (-1 for a new pair).
-__device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t) {
+// copy in scratch memory.  Returns last step's slot (-1 for a new pair) and
+// sets `tester` to the state it carries (0 for a new pair); the pair enters
+// this step's table in k_narrowphase.
+__device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t, int& tester) {
     // A body inserted for this step (possibly into ids a removed world used)
     // has no history: never pick up the previous step's state for its pairs.
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
     bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
     int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
     PairState& s = w.pairs_cur[t];
-    if (old >= 0) copy_pair_state(&s, &w.pairs_old[old]);
+    tester = 0;
+    if (old >= 0) tester = copy_pair_state(&s, &w.pairs_old[old]);
     else init_pair(w, key, s);
-    if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
     return old;
 }
 
 // General convex pairs: their state carried into this step's slot, and
 // (slot, last slot) appended to the work list of the pair tester's state (new
 // pairs start Separated), so that k_convex's wavefronts mostly run one branch
-// of the state machine: one atomic per wavefront and state.
-__global__ void __launch_bounds__(kNpBlock) k_convex_prep(DeviceWorld w) {
+// of the state machine.  Appends are gathered per workgroup: one global
+// atomic per workgroup and list (the list counters are shared by the whole
+// device, so per-wavefront atomics would queue on them).
+constexpr int kPrepBlock = 512;
+__global__ void __launch_bounds__(kPrepBlock) k_convex_prep(DeviceWorld w) {
+    __shared__ uint32_t s_cnt[6], s_base[6];
+    if (threadIdx.x < 6) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     int cstate = -1;
@@ -694,35 +707,41 @@ __global__ void __launch_bounds__(kNpBlock) k_convex_prep(DeviceWorld w) {
         uint64_t key = w.pair_keys[t];
         uint32_t i = (uint32_t)(key >> 32), j = (uint32_t)key;
         if (general_convex_pair(w, i, j)) {
-            int old = begin_pair(w, key, t);
-            cstate = old >= 0 ? __float_as_int(w.conv_old[old].sep.w) : 0;
+            int old = begin_pair(w, key, t, cstate);
             if (cstate != 1 && cstate != 2) cstate = 0;
             if (old >= 0) old_slot = (uint32_t)old;
-            two_hulls = (int)w.shape[i].w == BEPU_SHAPE_CONVEX_HULL && (int)w.shape[j].w == BEPU_SHAPE_CONVEX_HULL;
+            two_hulls = (w.bflags[i] & w.bflags[j] & BF_HULL) != 0;
         }
     }
     // Six lists (state x two hulls or not), so that a wavefront of k_convex
     // mostly runs one branch of the state machine with one pair of support
     // mappings: pairs of two hulls fill a state's region from its front, the
     // others from its back.
+    // List q = 2 * state + (two hulls ? 0 : 1).
     const int lane = __lane_id();
+    const int my_q = cstate < 0 ? -1 : 2 * cstate + (two_hulls ? 0 : 1);
+    uint32_t my_off = 0;
     for (int q = 0; q < 6; ++q) {
-        const int state = q >> 1;
-        const bool front = (q & 1) == 0;
-        const bool me = cstate == state && two_hulls == front;
-        uint64_t mine = __ballot(me);
+        const uint64_t mine = __ballot(my_q == q);
         if (!mine) continue;
-        int leader = __ffsll((unsigned long long)mine) - 1;
+        const int leader = __ffsll((unsigned long long)mine) - 1;
         uint32_t base = 0;
-        uint32_t c = front ? (state == 0 ? CTR_CONVEX : (state == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP))
-                           : CTR_CONVEX_OTHER0 + (uint32_t)state;
-        if (lane == leader) base = atomicAdd(&w.counters[ctr(c)], (uint32_t)__popcll(mine));
+        if (lane == leader) base = atomicAdd(&s_cnt[q], (uint32_t)__popcll(mine));
         base = __shfl(base, leader);
-        if (me) {
-            uint32_t k = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
-            uint32_t at = front ? k : w.pair_cap - 1 - k;
-            w.conv_work[(size_t)state * w.pair_cap + at] = make_uint2(t, old_slot);
-        }
+        if (my_q == q) my_off = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
+    }
+    __syncthreads();
+    if (threadIdx.x < 6 && s_cnt[threadIdx.x]) {
+        const uint32_t q = threadIdx.x, state = q >> 1;
+        const uint32_t c = (q & 1) == 0 ? (state == 0 ? CTR_CONVEX : (state == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP))
+                                        : CTR_CONVEX_OTHER0 + state;
+        s_base[q] = atomicAdd(&w.counters[ctr(c)], s_cnt[q]);
+    }
+    __syncthreads();
+    if (my_q >= 0) {
+        const uint32_t k = s_base[my_q] + my_off;
+        const uint32_t at = (my_q & 1) == 0 ? k : w.pair_cap - 1 - k;
+        w.conv_work[(size_t)(my_q >> 1) * w.pair_cap + at] = make_uint2(t, old_slot);
     }
 }
 
@@ -736,7 +755,11 @@ __global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= npairs) return;
     uint64_t key = w.pair_keys[t];
-    if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) return;
+    if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) {
+        // Tested by k_convex; only its entry in this step's table is made here.
+        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
+        return;
+    }
     // A body inserted for this step (possibly into ids a removed world used)
     // has no history: never pick up the previous step's state for its pairs.
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
@@ -816,7 +839,7 @@ void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st) {
 
 void launch_convex_prep(const DeviceWorld& w, hipStream_t st) {
     if (w.pair_cap == 0) return;
-    hipLaunchKernelGGL(k_convex_prep, dim3((w.pair_cap + kNpBlock - 1) / kNpBlock), dim3(kNpBlock), 0, st, w);
+    hipLaunchKernelGGL(k_convex_prep, dim3((w.pair_cap + kPrepBlock - 1) / kPrepBlock), dim3(kPrepBlock), 0, st, w);
 }
 
 void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st) {
