@@ -73,9 +73,11 @@ __global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
 // leaves the packed counts at zero for the next step), classifies islands
 // into tiers, lists the global path's constraints and the medium islands,
 // and appends small islands to their packing class's list (one global
-// atomic per workgroup and class).
-__global__ void __launch_bounds__(256) k_isl_fill(DeviceWorld w) {
-    __shared__ uint32_t s_tmp[256 / 64 + 1];
+// atomic per workgroup and class; the workgroups are large because these
+// counters are shared by the whole device).
+constexpr int kFillBlock = 1024;
+__global__ void __launch_bounds__(kFillBlock) k_isl_fill(DeviceWorld w) {
+    __shared__ uint32_t s_tmp[kFillBlock / 64 + 1];
     __shared__ uint32_t s_cls[64], s_base[64];
     if (threadIdx.x < 64) s_cls[threadIdx.x] = 0;
     __syncthreads();
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(256) k_isl_fill(DeviceWorld w) {
         w.isl_cons[hi32(s0) + hi32(oc) - 1] = i;
         global = island_tier(hi32(s1) - hi32(s0), lo32(s1) - lo32(s0), w.small_island_max) == 3;
     }
-    block_push<256>(w.gcons, &w.counters[ctr(CTR_GCONS)], global, i, s_tmp);
+    block_push<kFillBlock>(w.gcons, &w.counters[ctr(CTR_GCONS)], global, i, s_tmp);
 
     bool body = i < w.nbodies && (w.bflags[i] & 1u);
     uint32_t rb = body ? w.isl_root[i] : 0;
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(256) k_isl_fill(DeviceWorld w) {
             }
         }
     }
-    block_push<256>(w.isl_medium, &w.counters[ctr(CTR_ISL_MEDIUM)], medium, i, s_tmp);
+    block_push<kFillBlock>(w.isl_medium, &w.counters[ctr(CTR_ISL_MEDIUM)], medium, i, s_tmp);
     if (threadIdx.x < 64) {
         uint32_t c = s_cls[threadIdx.x];
         s_base[threadIdx.x] = c ? atomicAdd(&w.counters[ctr(CTR_CLS0 + threadIdx.x)], c) : 0;
@@ -479,7 +481,7 @@ void launch_islands(const DeviceWorld& w, void* temp, size_t temp_bytes, hipStre
     size_t bytes = temp_bytes;
     (void)hipcub::DeviceScan::ExclusiveSum(temp, bytes, (unsigned long long*)w.isl_count,
                                            (unsigned long long*)w.isl_start, (int)(n + 1), st);
-    hipLaunchKernelGGL(k_isl_fill, gm, b, 0, st, w);
+    hipLaunchKernelGGL(k_isl_fill, dim3((std::max(n, cap) + kFillBlock - 1) / kFillBlock), dim3(kFillBlock), 0, st, w);
 }
 
 void launch_island_solve(const DeviceWorld& w, hipStream_t st) {
