@@ -105,8 +105,8 @@ def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0
 # of known size.
 STAGE_KERNELS = {
     "island_solve": ["k_island_solve"],
-    "narrowphase": ["k_narrowphase"],
-    "convex": ["k_convex"],
+    "narrowphase": ["k_pair_prep", "k_narrowphase"],
+    "convex": ["k_convex<"],
     "broadphase": ["k_grid_insert", "k_grid_query"],
     "global_solve": ["k_global"],
     "body_pre": ["k_body_pre"],

@@ -662,10 +662,11 @@ __device__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64
     return false;
 }
 
-// Pairs with a hull or a capsule take the general convex path
-// (NarrowPhaseHelper.cs:360-367); boxes and spheres have closed-form testers.
-__device__ __forceinline__ bool general_convex_pair(const DeviceWorld& w, uint32_t i, uint32_t j) {
-    return ((w.bflags[i] | w.bflags[j]) & BF_GENERAL) != 0;
+// Counter of general convex work list q = 2 * tester state + (two hulls ? 0 : 1).
+__device__ __forceinline__ uint32_t convex_list_counter(uint32_t q) {
+    const uint32_t state = q >> 1;
+    if (q & 1) return CTR_CONVEX_OTHER0 + state;
+    return state == 0 ? CTR_CONVEX : (state == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP);
 }
 
 // This step's state of a pair in slot t: last step's (found in last step's
@@ -673,7 +674,7 @@ __device__ __forceinline__ bool general_convex_pair(const DeviceWorld& w, uint32
 // slot: its contact slots are indexed at run time, which would put a private
 // copy in scratch memory.  Returns last step's slot (-1 for a new pair) and
 // sets `tester` to the state it carries (0 for a new pair); the pair enters
-// this step's table in k_narrowphase.
+// this step's key -> slot table in k_narrowphase.
 __device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t, int& tester) {
     // A body inserted for this step (possibly into ids a removed world used)
     // has no history: never pick up the previous step's state for its pairs.
@@ -687,41 +688,45 @@ __device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, ui
     return old;
 }
 
-// General convex pairs: their state carried into this step's slot, and
-// (slot, last slot) appended to the work list of the pair tester's state (new
-// pairs start Separated), so that k_convex's wavefronts mostly run one branch
-// of the state machine.  Appends are gathered per workgroup: one global
-// atomic per workgroup and list (the list counters are shared by the whole
-// device, so per-wavefront atomics would queue on them).
+// Every pair of this step goes to a work list.  Pairs with a hull or a
+// capsule take the general convex path (NarrowPhaseHelper.cs:360-367); boxes
+// and spheres have closed-form testers.  A general convex pair gets
+// its state carried into this step's slot and (slot, last slot) appended to
+// the list of its pair tester's state (new pairs start Separated), so that a
+// wavefront of k_convex mostly runs one branch of the state machine; pairs of
+// two hulls fill a state's region from its front, the others from its back.
+// A closed-form pair is listed by kind (box-box from the front of np_work, the
+// others from its back), so that a wavefront of k_narrowphase runs one
+// contact routine.  Appends are gathered per workgroup: one global atomic per
+// workgroup and list (the list counters are shared by the whole device, so
+// per-wavefront atomics would queue on them).
 constexpr int kPrepBlock = 512;
-__global__ void __launch_bounds__(kPrepBlock) k_convex_prep(DeviceWorld w) {
-    __shared__ uint32_t s_cnt[6], s_base[6];
-    if (threadIdx.x < 6) s_cnt[threadIdx.x] = 0;
+__global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
+    __shared__ uint32_t s_cnt[8], s_base[8];
+    if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
-    int cstate = -1;
-    bool two_hulls = false;
+    // List: 2 * state + (two hulls ? 0 : 1) for general convex pairs, 6 for
+    // box-box, 7 for the other closed-form pairs.
+    int my_q = -1;
     uint32_t old_slot = ~0u;
     if (t < npairs) {
         uint64_t key = w.pair_keys[t];
-        uint32_t i = (uint32_t)(key >> 32), j = (uint32_t)key;
-        if (general_convex_pair(w, i, j)) {
+        const uint32_t fi = w.bflags[(uint32_t)(key >> 32)], fj = w.bflags[(uint32_t)key];
+        if ((fi | fj) & BF_GENERAL) {
+            int cstate;
             int old = begin_pair(w, key, t, cstate);
             if (cstate != 1 && cstate != 2) cstate = 0;
             if (old >= 0) old_slot = (uint32_t)old;
-            two_hulls = (w.bflags[i] & w.bflags[j] & BF_HULL) != 0;
+            my_q = 2 * cstate + ((fi & fj & BF_HULL) ? 0 : 1);
+        } else {
+            my_q = ((fi | fj) & BF_SPHERE) ? 7 : 6;
         }
     }
-    // Six lists (state x two hulls or not), so that a wavefront of k_convex
-    // mostly runs one branch of the state machine with one pair of support
-    // mappings: pairs of two hulls fill a state's region from its front, the
-    // others from its back.
-    // List q = 2 * state + (two hulls ? 0 : 1).
     const int lane = __lane_id();
-    const int my_q = cstate < 0 ? -1 : 2 * cstate + (two_hulls ? 0 : 1);
     uint32_t my_off = 0;
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < 8; ++q) {
         const uint64_t mine = __ballot(my_q == q);
         if (!mine) continue;
         const int leader = __ffsll((unsigned long long)mine) - 1;
@@ -731,35 +736,49 @@ __global__ void __launch_bounds__(kPrepBlock) k_convex_prep(DeviceWorld w) {
         if (my_q == q) my_off = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
     }
     __syncthreads();
-    if (threadIdx.x < 6 && s_cnt[threadIdx.x]) {
-        const uint32_t q = threadIdx.x, state = q >> 1;
-        const uint32_t c = (q & 1) == 0 ? (state == 0 ? CTR_CONVEX : (state == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP))
-                                        : CTR_CONVEX_OTHER0 + state;
+    if (threadIdx.x < 8 && s_cnt[threadIdx.x]) {
+        const uint32_t q = threadIdx.x;
+        uint32_t c;
+        if (q == 6) c = CTR_NP_BOX;
+        else if (q == 7) c = CTR_NP_SINGLE;
+        else c = convex_list_counter(q);
         s_base[q] = atomicAdd(&w.counters[ctr(c)], s_cnt[q]);
     }
     __syncthreads();
     if (my_q >= 0) {
         const uint32_t k = s_base[my_q] + my_off;
         const uint32_t at = (my_q & 1) == 0 ? k : w.pair_cap - 1 - k;
-        w.conv_work[(size_t)(my_q >> 1) * w.pair_cap + at] = make_uint2(t, old_slot);
+        if (my_q < 6) w.conv_work[(size_t)(my_q >> 1) * w.pair_cap + at] = make_uint2(t, old_slot);
+        else w.np_work[at] = t;
     }
 }
 
-// Closed-form pairs (box-box, sphere-sphere, box-sphere); general convex pairs
-// are left to k_convex_prep and k_convex.
+// Closed-form pairs (box-box, sphere-sphere, box-sphere) from k_pair_prep's
+// lists.  General convex pairs are tested by k_convex; the threads past the
+// closed-form pairs enter them into this step's key -> slot table, off the
+// path that k_convex waits on.
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
     __shared__ float4 lds_pd[8 * BLOCK];
     __shared__ int lds_id[8 * BLOCK];
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
-    if (t >= npairs) return;
-    uint64_t key = w.pair_keys[t];
-    if (general_convex_pair(w, (uint32_t)(key >> 32), (uint32_t)key)) {
-        // Tested by k_convex; only its entry in this step's table is made here.
-        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nbox = w.counters[ctr(CTR_NP_BOX)], nsingle = w.counters[ctr(CTR_NP_SINGLE)];
+    if (k >= nbox + nsingle) {
+        uint32_t c = k - (nbox + nsingle);
+        for (uint32_t q = 0; q < 6; ++q) {
+            const uint32_t n = w.counters[ctr(convex_list_counter(q))];
+            if (c < n) {
+                const uint32_t t = w.conv_work[(size_t)(q >> 1) * w.pair_cap + ((q & 1) == 0 ? c : w.pair_cap - 1 - c)].x;
+                if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[t], t))
+                    w.counters[ctr(CTR_OVERFLOW)] = 3;
+                return;
+            }
+            c -= n;
+        }
         return;
     }
+    const uint32_t t = w.np_work[k < nbox ? k : w.pair_cap - 1 - (k - nbox)];
+    uint64_t key = w.pair_keys[t];
     // A body inserted for this step (possibly into ids a removed world used)
     // has no history: never pick up the previous step's state for its pairs.
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
@@ -837,9 +856,9 @@ void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st) {
     if (npairs) hipLaunchKernelGGL(k_stats, dim3((npairs + 255) / 256), dim3(256), 0, st, w, npairs);
 }
 
-void launch_convex_prep(const DeviceWorld& w, hipStream_t st) {
+void launch_pair_prep(const DeviceWorld& w, hipStream_t st) {
     if (w.pair_cap == 0) return;
-    hipLaunchKernelGGL(k_convex_prep, dim3((w.pair_cap + kPrepBlock - 1) / kPrepBlock), dim3(kPrepBlock), 0, st, w);
+    hipLaunchKernelGGL(k_pair_prep, dim3((w.pair_cap + kPrepBlock - 1) / kPrepBlock), dim3(kPrepBlock), 0, st, w);
 }
 
 void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st) {

@@ -198,7 +198,7 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.pair_keys, P); A(&d.counters, (size_t)CTR_COUNT * kCtrStride);
     A(&d.pairs_cur, P); A(&d.pairs_old, P);
     // General convex pair state: 2 x 160 B + 3 x 8 B per pair slot.
-    A(&d.conv_cur, P); A(&d.conv_old, P); A(&d.conv_work, 3 * (size_t)P);
+    A(&d.conv_cur, P); A(&d.conv_old, P); A(&d.conv_work, 3 * (size_t)P); A(&d.np_work, P);
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
     A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
     A(&d.cons, P);
@@ -287,7 +287,8 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
     const bool hull = shape.w == (float)BEPU_SHAPE_CONVEX_HULL;
     const bool general = hull || shape.w == (float)BEPU_SHAPE_CAPSULE;
-    b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT | (general ? BF_GENERAL : 0u) | (hull ? BF_HULL : 0u);
+    b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT | (general ? BF_GENERAL : 0u) | (hull ? BF_HULL : 0u) |
+              (shape.w == (float)BEPU_SHAPE_SPHERE ? BF_SPHERE : 0u);
     b.born = w->steps;
     return BEPU_OK;
 }
@@ -815,7 +816,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         // Steps with stage events run the general convex tests after the
         // closed-form narrowphase, so that each stage's time is its own.
         const bool side = w->convex_bodies && w->convex_side && !e;
-        if (w->convex_bodies) launch_convex_prep(d, s);
+        launch_pair_prep(d, s);
         if (side) {
             // Test the general convex pairs on the side stream while the
             // closed-form pairs run here.

@@ -62,8 +62,8 @@ struct ConvexState {
     float4 offb[4];
 };
 // BF_GENERAL: a shape whose pairs take the general convex tester (hull or
-// capsule); BF_HULL: a convex hull.
-enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u };
+// capsule); BF_HULL: a convex hull; BF_SPHERE: a sphere.
+enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u };
 
 // One body as built on the host (Entity construction), copied to the device
 // in one block and scattered into the structure-of-arrays by k_scatter_bodies.
@@ -138,6 +138,8 @@ struct DeviceWorld {
     uint2* conv_work;          // this step's general convex pairs: (overlap index, last step's index or ~0),
                                // in three regions of pair_cap entries by the pair tester's state; a
                                // region holds pairs of two hulls from its front, the others from its back
+    uint32_t* np_work;         // this step's closed-form pairs (overlap index): box-box from the front,
+                               // the others (a sphere with a box or a sphere) from the back
     uint64_t* hash_keys_cur;
     uint32_t* hash_vals_cur;
     uint64_t* hash_keys_old;
@@ -196,7 +198,9 @@ enum {
     CTR_SIDE0 = 80,       // 80..143: bodies per half-octave bin of the largest AABB side
     CTR_GBAR_ABORT = 144, // set when a grid barrier wait ran out of time
     CTR_CONVEX_OTHER0 = 145,  // 145..147: general convex pairs with at most one hull, per tester state
-    CTR_COUNT = 148
+    CTR_NP_BOX = 148,     // closed-form pairs listed for k_narrowphase: box-box
+    CTR_NP_SINGLE = 149,  // and the others
+    CTR_COUNT = 150
 };
 
 // Every counter sits on its own 256-byte line: device atomics on one line
@@ -217,7 +221,7 @@ void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
 void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st);
 // General convex pairs queued by the narrowphase (GJK / MPR).
-void launch_convex_prep(const DeviceWorld& w, hipStream_t st);
+void launch_pair_prep(const DeviceWorld& w, hipStream_t st);
 void launch_convex(const DeviceWorld& w, bool lds, hipStream_t st);
 constexpr uint32_t kConvexLdsMaxVerts = 4096;  // hull vertex pools up to 64 KB can be copied to LDS
 // Last step's constraint, contact, colour and penetration statistics into the counters.
