@@ -47,9 +47,9 @@ uint32_t next_pow2(uint64_t v) {
 struct bepu_world_s {
     DeviceWorld d{};
     hipStream_t stream = nullptr;
-    // General convex pairs are tested on a second stream while the closed-form
-    // pairs run on the first (both kernels are bound by memory latency and
-    // leave most of the device idle alone); fork and join are events.
+    // The closed-form pairs are tested on a second stream while the general
+    // convex pairs run on the first (both kernels are bound by memory latency
+    // and leave most of the device idle alone); fork and join are events.
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     // Launch configuration (bepu_tuning); small_island_max lives in d.
@@ -818,17 +818,20 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         const bool side = w->convex_bodies && w->convex_side && !e;
         launch_pair_prep(d, s);
         if (side) {
-            // Test the general convex pairs on the side stream while the
-            // closed-form pairs run here.
+            // Test the closed-form pairs on the side stream while the general
+            // convex pairs (the longer of the two) run here: a launch behind
+            // an event starts a few microseconds later than one in order.
             (void)hipEventRecord(w->fork, s);
             (void)hipStreamWaitEvent(w->side, w->fork, 0);
-            launch_convex(d, w->convex_lds, w->side);
+            launch_narrowphase(d, w->np_block, w->side);
             (void)hipEventRecord(w->join, w->side);
+            launch_convex(d, w->convex_lds, s);
+            (void)hipStreamWaitEvent(s, w->join, 0);
+        } else {
+            launch_narrowphase(d, w->np_block, s);
         }
-        launch_narrowphase(d, w->np_block, s);
         if (e) (void)hipEventRecord(e[3], s);
-        if (side) (void)hipStreamWaitEvent(s, w->join, 0);
-        else if (w->convex_bodies) launch_convex(d, w->convex_lds, s);
+        if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
         if (e) (void)hipEventRecord(e[4], s);
         launch_islands(d, w->scan_temp, w->scan_bytes, s);
         if (e) (void)hipEventRecord(e[5], s);

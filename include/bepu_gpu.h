@@ -309,8 +309,8 @@ bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
  *   convex_lds         1: each general convex workgroup copies the hull vertex pool to
  *                      LDS (only when the pool holds at most 4,096 vertices); 0 (default):
  *                      support scans read global memory
- *   convex_side_stream 1 (default): general convex pairs are tested on a second stream beside
- *                      the closed-form narrowphase; 0: after it on the world's stream
+ *   convex_side_stream 1 (default): the closed-form narrowphase runs on a second stream beside
+ *                      the general convex tests; 0: both in turn on the world's stream
  *   small_island_max   islands of at most this many constraints (32 or 64, default 64) are
  *                      packed several to a wavefront; larger ones up to 256 take a workgroup each
  *   ms_per_step        (filled by bepu_autotune) the chosen configuration's time */
