@@ -614,28 +614,6 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
     s.static_f = ma.x * mb.x;
 }
 
-// Copies a pair state between global arrays, seven float4 loads in flight
-// before their stores (a struct assignment would be lowered through scratch
-// memory, and a plain loop waits for each load before its store, since the
-// arrays might overlap).  The memory clobber keeps later field accesses of
-// the destination after the stores.
-// Copies a pair state; returns its `tester` field (read with the copy).
-__device__ __forceinline__ int copy_pair_state(PairState* dst, const PairState* src) {
-    static_assert(sizeof(PairState) == 21 * 16 && alignof(PairState) == 16, "PairState is copied as 21 float4s");
-    static_assert(offsetof(PairState, tester) == 20 * 16 + 8, "tester is the third word of the last float4");
-    const float4* a = reinterpret_cast<const float4*>(src);
-    float4* b = reinterpret_cast<float4*>(dst);
-    float4 last = make_float4(0, 0, 0, 0);
-#pragma unroll
-    for (int c = 0; c < 21; c += 7) {
-        float4 r0 = a[c], r1 = a[c + 1], r2 = a[c + 2], r3 = a[c + 3], r4 = a[c + 4], r5 = a[c + 5], r6 = a[c + 6];
-        b[c] = r0; b[c + 1] = r1; b[c + 2] = r2; b[c + 3] = r3; b[c + 4] = r4; b[c + 5] = r5; b[c + 6] = r6;
-        last = r6;
-    }
-    asm volatile("" ::: "memory");
-    return __float_as_int(last.z);
-}
-
 __device__ __forceinline__ uint32_t hash_slot(uint64_t key, uint32_t cap) { return (uint32_t)mix64(key) & (cap - 1); }
 
 __device__ int hash_find(const uint64_t* keys, const uint32_t* vals, uint32_t cap, uint64_t key) {
@@ -669,22 +647,20 @@ __device__ __forceinline__ uint32_t convex_list_counter(uint32_t q) {
     return state == 0 ? CTR_CONVEX : (state == 1 ? CTR_CONVEX_SHALLOW : CTR_CONVEX_DEEP);
 }
 
-// This step's state of a pair in slot t: last step's (found in last step's
-// key -> slot table), or a fresh one.  The state is updated in place in the
-// slot: its contact slots are indexed at run time, which would put a private
-// copy in scratch memory.  Returns last step's slot (-1 for a new pair) and
-// sets `tester` to the state it carries (0 for a new pair); the pair enters
-// this step's key -> slot table in k_narrowphase.
+// Last step's slot of a general convex pair (found in last step's key -> slot
+// table; -1 for a new pair) and the pair tester's state it ended that step in
+// (0 for a new pair).  A new pair's state is set up in slot t here; k_pair_prep
+// copies a carried-over one.  The pair enters this step's table in
+// k_narrowphase.
 __device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t, int& tester) {
     // A body inserted for this step (possibly into ids a removed world used)
     // has no history: never pick up the previous step's state for its pairs.
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
     bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
     int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
-    PairState& s = w.pairs_cur[t];
     tester = 0;
-    if (old >= 0) tester = copy_pair_state(&s, &w.pairs_old[old]);
-    else init_pair(w, key, s);
+    if (old >= 0) tester = w.pairs_old[old].tester;
+    else init_pair(w, key, w.pairs_cur[t]);
     return old;
 }
 
@@ -725,6 +701,29 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
         }
     }
     const int lane = __lane_id();
+    {
+        // Carried-over states: the wavefront copies them together, three pair
+        // states (21 float4 each) per pass with consecutive lanes on
+        // consecutive 16-byte chunks, instead of each lane reading its own
+        // pair's 336 bytes 16 bytes at a time.
+        static_assert(sizeof(PairState) == 21 * 16 && alignof(PairState) == 16, "PairState is copied as 21 float4s");
+        const int part = lane / 21, chunk = lane % 21;  // lane 63 (part 3) idles
+        uint64_t todo = __ballot(old_slot != ~0u);
+        while (todo) {
+            const int l0 = __ffsll((unsigned long long)todo) - 1;
+            todo &= todo - 1;
+            const int l1 = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+            if (todo) todo &= todo - 1;
+            const int l2 = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+            if (todo) todo &= todo - 1;
+            const int src_lane = part == 0 ? l0 : (part == 1 ? l1 : (part == 2 ? l2 : -1));
+            const uint32_t src = __shfl(old_slot, src_lane < 0 ? 0 : src_lane);
+            const uint32_t dst = __shfl(t, src_lane < 0 ? 0 : src_lane);
+            if (src_lane >= 0)
+                reinterpret_cast<float4*>(&w.pairs_cur[dst])[chunk] =
+                    reinterpret_cast<const float4*>(&w.pairs_old[src])[chunk];
+        }
+    }
     uint32_t my_off = 0;
     for (int q = 0; q < 8; ++q) {
         const uint64_t mine = __ballot(my_q == q);
