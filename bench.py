@@ -494,7 +494,7 @@ def main():
     ap.add_argument("--steps", type=int, default=300, help="timed batched steps")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--autotune-steps", type=int, default=3,
+    ap.add_argument("--autotune-steps", type=int, default=10,
                     help="steps per candidate and round of the launch autotuner (0: default configuration)")
     ap.add_argument("--autotune-after", type=int, default=160,
                     help="steps a copy of the batch runs before the autotuner times candidates on it")

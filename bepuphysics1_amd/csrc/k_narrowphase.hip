@@ -752,31 +752,10 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
     }
 }
 
-// Closed-form pairs (box-box, sphere-sphere, box-sphere) from k_pair_prep's
-// lists.  General convex pairs are tested by k_convex; the threads past the
-// closed-form pairs enter them into this step's key -> slot table, off the
-// path that k_convex waits on.
+// One closed-form pair in slot t: last step's state (or a new one) updated by
+// its contact routine, stored to this step's slot and entered in the table.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
-    __shared__ float4 lds_pd[8 * BLOCK];
-    __shared__ int lds_id[8 * BLOCK];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nbox = w.counters[ctr(CTR_NP_BOX)], nsingle = w.counters[ctr(CTR_NP_SINGLE)];
-    if (k >= nbox + nsingle) {
-        uint32_t c = k - (nbox + nsingle);
-        for (uint32_t q = 0; q < 6; ++q) {
-            const uint32_t n = w.counters[ctr(convex_list_counter(q))];
-            if (c < n) {
-                const uint32_t t = w.conv_work[(size_t)(q >> 1) * w.pair_cap + ((q & 1) == 0 ? c : w.pair_cap - 1 - c)].x;
-                if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[t], t))
-                    w.counters[ctr(CTR_OVERFLOW)] = 3;
-                return;
-            }
-            c -= n;
-        }
-        return;
-    }
-    const uint32_t t = w.np_work[k < nbox ? k : w.pair_cap - 1 - (k - nbox)];
+__device__ __forceinline__ void narrowphase_pair(DeviceWorld& w, uint32_t t, float4* lds_pd, int* lds_id) {
     uint64_t key = w.pair_keys[t];
     // A body inserted for this step (possibly into ids a removed world used)
     // has no history: never pick up the previous step's state for its pairs.
@@ -800,6 +779,40 @@ __global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
     // when both are dynamic; the union-find forest was reset by this step's
     // body pass.
     if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
+}
+
+// Closed-form pairs (box-box, sphere-sphere, box-sphere) from k_pair_prep's
+// lists, or in order when every pair is of one kind.  General convex pairs
+// are tested by k_convex; the threads past the closed-form pairs enter them
+// into this step's key -> slot table, off the path that k_convex waits on.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
+    __shared__ float4 lds_pd[8 * BLOCK];
+    __shared__ int lds_id[8 * BLOCK];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!w.pair_lists) {
+        // Every pair is a closed-form pair of one kind: taken in order.
+        if (k >= min(w.counters[ctr(CTR_PAIRS)], w.pair_cap)) return;
+        narrowphase_pair<BLOCK>(w, k, lds_pd, lds_id);
+        return;
+    }
+    const uint32_t nbox = w.counters[ctr(CTR_NP_BOX)], nsingle = w.counters[ctr(CTR_NP_SINGLE)];
+    if (k >= nbox + nsingle) {
+        uint32_t c = k - (nbox + nsingle);
+        for (uint32_t q = 0; q < 6; ++q) {
+            const uint32_t n = w.counters[ctr(convex_list_counter(q))];
+            if (c < n) {
+                const uint32_t t = w.conv_work[(size_t)(q >> 1) * w.pair_cap + ((q & 1) == 0 ? c : w.pair_cap - 1 - c)].x;
+                if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[t], t))
+                    w.counters[ctr(CTR_OVERFLOW)] = 3;
+                return;
+            }
+            c -= n;
+        }
+        return;
+    }
+    const uint32_t t = w.np_work[k < nbox ? k : w.pair_cap - 1 - (k - nbox)];
+    narrowphase_pair<BLOCK>(w, t, lds_pd, lds_id);
 }
 
 // Statistics of the last step, computed on request from its pair states

@@ -85,6 +85,7 @@ struct bepu_world_s {
     };
     std::vector<HullShape> hull_shapes;
     bool convex_bodies = false;             // a capsule or hull was ever added: run k_convex
+    bool box_bodies = false, sphere_bodies = false;  // a box / a sphere was ever added
     std::vector<float4> hull_host;
     uint32_t hull_dev_cap = 0;
 };
@@ -282,6 +283,8 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
     b.li2 = make_float4(li.m31, li.m32, li.m33, 0);
     b.shape = shape;
     if (shape.w == (float)BEPU_SHAPE_CAPSULE || shape.w == (float)BEPU_SHAPE_CONVEX_HULL) w->convex_bodies = true;
+    if (shape.w == (float)BEPU_SHAPE_BOX) w->box_bodies = true;
+    if (shape.w == (float)BEPU_SHAPE_SPHERE) w->sphere_bodies = true;
     b.stab = make_float4(0, 0, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
     b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
@@ -816,7 +819,10 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         // Steps with stage events run the general convex tests after the
         // closed-form narrowphase, so that each stage's time is its own.
         const bool side = w->convex_bodies && w->convex_side && !e;
-        launch_pair_prep(d, s);
+        // Pairs of more than one tester are listed by tester first; a world of
+        // boxes only (or spheres only) skips that pass.
+        d.pair_lists = (w->convex_bodies || (w->box_bodies && w->sphere_bodies)) ? 1u : 0u;
+        if (d.pair_lists) launch_pair_prep(d, s);
         if (side) {
             // Test the closed-form pairs on the side stream while the general
             // convex pairs (the longer of the two) run here: a launch behind

@@ -97,6 +97,8 @@ struct DeviceWorld {
     float4* hull_verts;        // vertices of every hull shape, back to back (xyz)
     uint32_t hull_nverts;      // vertices in hull_verts
     uint32_t small_island_max; // islands of at most this many constraints are packed per wavefront
+    uint32_t pair_lists;       // 1: k_pair_prep lists this step's pairs by tester; 0: every pair is a
+                               // closed-form pair of one kind and k_narrowphase takes them in order
     float4 *bmin, *bmax;
     float4* stab;              // prev velocity, time below, slowing, allow stabilization
     float2* damp;              // linear, angular damping
