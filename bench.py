@@ -54,7 +54,8 @@ BATCHED_ITERATIONS = 8
 #   convex        1,120 per general convex pair (a pair with a hull or a
 #                 capsule): pair state 336 and tester/supplement state 160,
 #                 each read and written, + pose, velocity and shape of both
-#                 bodies 128 (hull vertices are read from an LDS copy)
+#                 bodies 128 (the hull vertex pool, a few KB that stays in
+#                 cache, is not counted)
 #   islands       union-find and island bucketing: 40 per body (parent init 4,
 #                 compress 12, count 4, body list 8, local index 4, tier list 8)
 #                 + 32 per manifold (union reads 16, count 8, constraint list 8)
