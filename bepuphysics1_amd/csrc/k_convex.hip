@@ -228,18 +228,9 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
         // scratch memory.
         PairState& s = w.pairs_cur[wk.x];
         ConvexState& e = w.conv_cur[wk.x];
-        // Carry last step's state over field by field (a whole-struct copy
-        // between global arrays is lowered through scratch memory).
-        if (wk.y != ~0u) {
-            const ConvexState& o = w.conv_old[wk.y];
-            float4 t0 = o.sep, t1 = o.dir;
-            float4 a0 = o.offa[0], a1 = o.offa[1], a2 = o.offa[2], a3 = o.offa[3];
-            float4 b0 = o.offb[0], b1 = o.offb[1], b2 = o.offb[2], b3 = o.offb[3];
-            e.sep = t0;
-            e.dir = t1;
-            e.offa[0] = a0; e.offa[1] = a1; e.offa[2] = a2; e.offa[3] = a3;
-            e.offb[0] = b0; e.offb[1] = b1; e.offb[2] = b2; e.offb[3] = b3;
-        } else {
+        // k_pair_prep carried last step's state over; a new pair starts from
+        // zero.
+        if (wk.y == ~0u) {
             float4 z = make_float4(0, 0, 0, 0);
             e.sep = z;
             e.dir = z;

@@ -666,8 +666,9 @@ __device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, ui
 
 // Every pair of this step goes to a work list.  Pairs with a hull or a
 // capsule take the general convex path (NarrowPhaseHelper.cs:360-367); boxes
-// and spheres have closed-form testers.  A general convex pair gets
-// its state carried into this step's slot and (slot, last slot) appended to
+// and spheres have closed-form testers.  A general convex pair gets its pair
+// state and its tester state carried into this step's slot and (slot, last
+// slot) appended to
 // the list of its pair tester's state (new pairs start Separated), so that a
 // wavefront of k_convex mostly runs one branch of the state machine; pairs of
 // two hulls fill a state's region from its front, the others from its back.
@@ -722,6 +723,25 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
             if (src_lane >= 0)
                 reinterpret_cast<float4*>(&w.pairs_cur[dst])[chunk] =
                     reinterpret_cast<const float4*>(&w.pairs_old[src])[chunk];
+        }
+        // Their tester and supplement states the same way, six (10 float4
+        // each) per pass.
+        static_assert(sizeof(ConvexState) == 10 * 16 && alignof(ConvexState) == 16, "ConvexState is copied as 10 float4s");
+        const int cpart = lane / 10, cchunk = lane % 10;  // lanes 60..63 idle
+        todo = __ballot(old_slot != ~0u);
+        while (todo) {
+            int src_lane = -1;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int lk = todo ? __ffsll((unsigned long long)todo) - 1 : -1;
+                if (todo) todo &= todo - 1;
+                if (cpart == k) src_lane = lk;
+            }
+            const uint32_t src = __shfl(old_slot, src_lane < 0 ? 0 : src_lane);
+            const uint32_t dst = __shfl(t, src_lane < 0 ? 0 : src_lane);
+            if (src_lane >= 0)
+                reinterpret_cast<float4*>(&w.conv_cur[dst])[cchunk] =
+                    reinterpret_cast<const float4*>(&w.conv_old[src])[cchunk];
         }
     }
     uint32_t my_off = 0;
