@@ -486,7 +486,9 @@ void launch_islands(const DeviceWorld& w, void* temp, size_t temp_bytes, hipStre
 
 void launch_island_solve(const DeviceWorld& w, hipStream_t st) {
     static int g_solve = 0;
-    if (!g_solve) g_solve = persistent_grid((const void*)k_island_solve, kSolveBlock);
+    // At least two workgroups: with medium islands present, at least one takes
+    // them and at least one takes the bundles.
+    if (!g_solve) g_solve = std::max(2, persistent_grid((const void*)k_island_solve, kSolveBlock));
     hipLaunchKernelGGL(k_island_solve, dim3(g_solve), dim3(kSolveBlock), 0, st, w);
 }
 
