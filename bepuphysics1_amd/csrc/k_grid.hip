@@ -195,17 +195,20 @@ __global__ void __launch_bounds__(kQueryBlock) k_grid_query(DeviceWorld w) {
         // Own level: own cell and forward neighbours (coordinates wrap with the key).
         {
             const uint64_t cx = (qkey >> 26) & 0x1FFFull, cy = (qkey >> 13) & 0x1FFFull, cz = qkey & 0x1FFFull;
-            uint64_t pk[14], first[14];
-            uint32_t ps[14];
+            // The 14 table reads are issued together; the keys and slots are
+            // recomputed for the probes rather than held in registers.
+            auto key_of = [&](int o) {
+                return cell_key(l, world, cx + (uint64_t)(int64_t)c_fwd[o][0], cy + (uint64_t)(int64_t)c_fwd[o][1],
+                                cz + (uint64_t)(int64_t)c_fwd[o][2]);
+            };
+            uint64_t first[14];
+#pragma unroll
+            for (int o = 0; o < 14; ++o) first[o] = w.gt_key[(uint32_t)mix64(key_of(o)) & mask];
 #pragma unroll
             for (int o = 0; o < 14; ++o) {
-                pk[o] = cell_key(l, world, cx + (uint64_t)(int64_t)c_fwd[o][0], cy + (uint64_t)(int64_t)c_fwd[o][1],
-                                 cz + (uint64_t)(int64_t)c_fwd[o][2]);
-                ps[o] = (uint32_t)mix64(pk[o]) & mask;
-                first[o] = w.gt_key[ps[o]];
+                const uint64_t pk = key_of(o);
+                probe_cell(w, q, pk, first[o], (uint32_t)mix64(pk) & mask, o == 0, buf, &buf_n);
             }
-#pragma unroll
-            for (int o = 0; o < 14; ++o) probe_cell(w, q, pk[o], first[o], ps[o], o == 0, buf, &buf_n);
         }
         // Coarser non-empty levels: the bodies themselves when the level holds
         // at most kLevelList of the world's bodies (a pile's ground), else the
