@@ -17,67 +17,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle_refcpu.so")
 
 
-class Settings(ctypes.Structure):
-    _fields_ = [
-        ("gravity", ctypes.c_float * 3),
-        ("dt", ctypes.c_float),
-        ("iteration_limit", ctypes.c_int32),
-        ("min_iteration_count", ctypes.c_int32),
-        ("min_impulse", ctypes.c_float),
-        ("allowed_penetration", ctypes.c_float),
-        ("default_margin", ctypes.c_float),
-        ("max_contact_distance", ctypes.c_float),
-        ("contact_invalidation_length_sq", ctypes.c_float),
-        ("contact_min_separation_sq", ctypes.c_float),
-        ("bounce_velocity_threshold", ctypes.c_float),
-        ("max_penetration_recovery_speed", ctypes.c_float),
-        ("penetration_recovery_stiffness", ctypes.c_float),
-        ("static_friction_velocity_threshold", ctypes.c_float),
-        ("twist_friction_factor", ctypes.c_float),
-        ("softness", ctypes.c_float),
-        ("use_stabilization", ctypes.c_int32),
-        ("velocity_lower_limit", ctypes.c_float),
-        ("low_velocity_time_minimum", ctypes.c_float),
-        ("linear_damping", ctypes.c_float),
-        ("angular_damping", ctypes.c_float),
-    ]
-
-
-class MotionState(ctypes.Structure):
-    _fields_ = [
-        ("position", ctypes.c_float * 3),
-        ("orientation", ctypes.c_float * 4),
-        ("linear_velocity", ctypes.c_float * 3),
-        ("angular_velocity", ctypes.c_float * 3),
-    ]
-
-
-class Material(ctypes.Structure):
-    _fields_ = [("static_friction", ctypes.c_float), ("kinetic_friction", ctypes.c_float),
-                ("bounciness", ctypes.c_float)]
-
-
-class Contact(ctypes.Structure):
-    _fields_ = [("position", ctypes.c_float * 3), ("normal", ctypes.c_float * 3),
-                ("penetration_depth", ctypes.c_float), ("id", ctypes.c_int32)]
-
-
-class Manifold(ctypes.Structure):
-    _fields_ = [("key", ctypes.c_uint64), ("body_a", ctypes.c_uint32), ("body_b", ctypes.c_uint32),
-                ("count", ctypes.c_int32), ("kind", ctypes.c_int32), ("contacts", Contact * 4)]
-
-
-class Impulses(ctypes.Structure):
-    _fields_ = [("key", ctypes.c_uint64), ("color", ctypes.c_int32), ("count", ctypes.c_int32),
-                ("normal", ctypes.c_float * 4), ("friction", ctypes.c_float * 4),
-                ("sliding", ctypes.c_float * 2), ("twist", ctypes.c_float), ("pad", ctypes.c_float)]
-
-
-class StepStats(ctypes.Structure):
-    _fields_ = [("bodies", ctypes.c_uint32), ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32),
-                ("contacts", ctypes.c_uint32), ("colors", ctypes.c_uint32), ("steps", ctypes.c_uint32),
-                ("max_penetration", ctypes.c_float), ("pad", ctypes.c_float)]
-
+# The structs are include/bepu_gpu.h's: one Python definition of that layout
+# serves the product binding and the oracle (refcpu_world.cpp fills the same
+# header structs), and tests/test_abi.py checks it against both compiled
+# libraries' sizeof.
+from bepuphysics1_amd._lib import (Contact, Impulses, Manifold, Material,  # noqa: E402,F401
+                                   MotionState, Settings, StepStats)
 
 _lib = None
 
@@ -130,6 +75,8 @@ def lib():
         L.refcpu_box_sphere.argtypes = [fp, fp, fp, ctypes.c_float, fp, fp]
         L.refcpu_mix64.restype = ctypes.c_uint64
         L.refcpu_mix64.argtypes = [ctypes.c_uint64]
+        L.refcpu_sizeof.restype = ctypes.c_uint32
+        L.refcpu_sizeof.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 

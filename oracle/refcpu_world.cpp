@@ -1557,6 +1557,21 @@ refcpu_world* refcpu_create(const bepu_settings* s) {
     return w;
 }
 void refcpu_destroy(refcpu_world* w) { delete w; }
+// sizeof of the header structs as this library was compiled, so that a
+// binding can check its layout against them: 0 settings, 1 motion state,
+// 2 material, 3 contact, 4 manifold, 5 impulses, 6 step stats.
+uint32_t refcpu_sizeof(int which) {
+    switch (which) {
+    case 0: return sizeof(bepu_settings);
+    case 1: return sizeof(bepu_motion_state);
+    case 2: return sizeof(bepu_material);
+    case 3: return sizeof(bepu_contact);
+    case 4: return sizeof(bepu_manifold);
+    case 5: return sizeof(bepu_impulses);
+    case 6: return sizeof(bepu_step_stats);
+    }
+    return 0;
+}
 // Threads used by refcpu_step (1: single-threaded).  Results do not depend on it.
 void refcpu_set_threads(refcpu_world* w, int n) {
     w->threads = n < 1 ? 1 : n;

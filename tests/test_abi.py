@@ -38,26 +38,80 @@ def test_binding_covers_header():
     assert set(declared_functions()) <= set(_lib.EXPORTS)
 
 
-def test_struct_sizes_match_header_layout(tmp_path):
-    """ctypes layouts equal the C compiler's layout of the header structs."""
+HEADER_STRUCTS = ("bepu_settings", "bepu_motion_state", "bepu_material", "bepu_contact", "bepu_manifold",
+                  "bepu_impulses", "bepu_step_stats", "bepu_body_desc", "bepu_tuning")
+
+
+@pytest.fixture(scope="module")
+def header_sizes(tmp_path_factory):
+    """sizeof of every header struct as gcc lays it out."""
     import shutil
     import subprocess
-    from bepuphysics1_amd import _lib
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
-    src = tmp_path / "sz.c"
-    src.write_text('#include <stdio.h>\n#include "%s/include/bepu_gpu.h"\n'
-                   'int main(){printf("%%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu %%zu", sizeof(bepu_settings),'
-                   'sizeof(bepu_motion_state), sizeof(bepu_material), sizeof(bepu_contact), sizeof(bepu_manifold),'
-                   'sizeof(bepu_impulses), sizeof(bepu_step_stats), sizeof(bepu_body_desc), sizeof(bepu_tuning));}'
-                   % ROOT)
-    exe = tmp_path / "sz"
+    d = tmp_path_factory.mktemp("sz")
+    src = d / "sz.c"
+    body = " ".join('printf("%%zu ", sizeof(%s));' % n for n in HEADER_STRUCTS)
+    src.write_text('#include <stdio.h>\n#include "%s/include/bepu_gpu.h"\nint main(){%s return 0;}' % (ROOT, body))
+    exe = d / "sz"
     subprocess.run([cc, str(src), "-o", str(exe)], check=True)
-    sizes = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    mine = [ctypes.sizeof(t) for t in (_lib.Settings, _lib.MotionState, _lib.Material, _lib.Contact,
-                                       _lib.Manifold, _lib.Impulses, _lib.StepStats, _lib.BodyDesc, _lib.Tuning)]
-    assert mine == sizes
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    return dict(zip(HEADER_STRUCTS, (int(x) for x in out)))
+
+
+def test_struct_sizes_match_header_layout(header_sizes):
+    """ctypes layouts equal the C compiler's layout of the header structs."""
+    from bepuphysics1_amd import _lib
+    mine = dict(zip(HEADER_STRUCTS, (ctypes.sizeof(t) for t in (
+        _lib.Settings, _lib.MotionState, _lib.Material, _lib.Contact, _lib.Manifold, _lib.Impulses,
+        _lib.StepStats, _lib.BodyDesc, _lib.Tuning))))
+    assert mine == header_sizes
+
+
+def test_oracle_binding_layout_matches_its_library(header_sizes):
+    """The oracle binding's structs have the sizes the oracle library was
+    compiled with (it fills the header's structs: a shorter ctypes struct
+    would be written past its end)."""
+    from oracle import refcpu
+    L = refcpu.lib()
+    types = (refcpu.Settings, refcpu.MotionState, refcpu.Material, refcpu.Contact, refcpu.Manifold,
+             refcpu.Impulses, refcpu.StepStats)
+    for i, t in enumerate(types):
+        assert ctypes.sizeof(t) == L.refcpu_sizeof(i) == header_sizes[HEADER_STRUCTS[i]], t.__name__
+
+
+_CS_SIZES = {"float": 4, "int": 4, "uint": 4, "ulong": 8, "long": 8, "IntPtr": 8}
+
+
+def _csharp_struct_sizes(text):
+    """Sequential-layout sizes of the structs of a C# source (scalar fields
+    and nested structs declared earlier in it)."""
+    sizes, aligns = {}, {}
+    for name, body in re.findall(r"public struct (\w+)\s*(?://[^\n]*)?\s*\{(.*?)\}", text, flags=re.S):
+        off, al = 0, 1
+        for typ, names in re.findall(r"public\s+(\w+)\s+([\w\s,]+);", body):
+            sz = _CS_SIZES.get(typ, sizes.get(typ))
+            a = min(_CS_SIZES.get(typ, aligns.get(typ, 4)), 8)
+            assert sz is not None, (name, typ)
+            for _ in names.split(","):
+                off = (off + a - 1) // a * a + sz
+                al = max(al, a)
+        sizes[name] = (off + al - 1) // al * al
+        aligns[name] = al
+    return sizes
+
+
+def test_csharp_stub_layout_matches_header(header_sizes):
+    """The P/Invoke stub's structs in INTEGRATION.md have the header's sizes,
+    so bepu_get_stats and the others write exactly into them."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    cs = _csharp_struct_sizes(text)
+    pairs = {"BepuSettings": "bepu_settings", "BepuMotionState": "bepu_motion_state",
+             "BepuMaterial": "bepu_material", "BepuStepStats": "bepu_step_stats", "BepuTuning": "bepu_tuning",
+             "BepuBodyDesc": "bepu_body_desc"}
+    for c, h in pairs.items():
+        assert cs.get(c) == header_sizes[h], (c, cs.get(c), header_sizes[h])
 
 
 def test_defaults_match_reference_constants():
