@@ -75,6 +75,7 @@ def lib():
         L.refcpu_box_sphere.argtypes = [fp, fp, fp, ctypes.c_float, fp, fp]
         L.refcpu_mix64.restype = ctypes.c_uint64
         L.refcpu_mix64.argtypes = [ctypes.c_uint64]
+        L.refcpu_reduce_contacts.argtypes = [fp, fp, fp, fp, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
         L.refcpu_sizeof.restype = ctypes.c_uint32
         L.refcpu_sizeof.argtypes = [ctypes.c_int]
         _lib = L
@@ -241,6 +242,16 @@ def box_sphere(half, bp, bq, r, sp):
 
 def mix64(k):
     return int(lib().refcpu_mix64(int(k)))
+
+
+def reduce_contacts(positions, depths, normal, cand_pos, cand_depth):
+    """ContactReducer.ReduceContacts on four contacts (list order) and a
+    candidate: returns (list index removed or -1, candidate added)."""
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    p, d, n, c = f(positions), f(depths), f(normal), f(cand_pos)
+    add = ctypes.c_int()
+    r = lib().refcpu_reduce_contacts(_fptr(p), _fptr(d), _fptr(n), _fptr(c), float(cand_depth), ctypes.byref(add))
+    return int(r), bool(add.value)
 
 
 # ---------------------------------------------------------------------------

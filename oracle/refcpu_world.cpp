@@ -1784,6 +1784,33 @@ int refcpu_get_stats(refcpu_world* w, bepu_step_stats* s) {
     return BEPU_OK;
 }
 
+// ContactReducer.ReduceContacts on a hand-built manifold: four contacts
+// (positions xyz and depths, list order) with a shared normal and one
+// candidate.  Returns the list index removed (-1: none); *add_candidate
+// tells whether the candidate joins.
+int refcpu_reduce_contacts(const float* pos12, const float* depth4, const float* normal3, const float* cand_pos3,
+                           float cand_depth, int* add_candidate) {
+    Pair p;
+    std::memset(&p, 0, sizeof(p));
+    for (int i = 0; i < 4; ++i) {
+        p.slot[i].pos = V3{pos12[3 * i], pos12[3 * i + 1], pos12[3 * i + 2]};
+        p.slot[i].normal = V3{normal3[0], normal3[1], normal3[2]};
+        p.slot[i].depth = depth4[i];
+        p.used[i] = true;
+        p.clist[i] = i;
+    }
+    p.ccount = 4;
+    ContactData cand;
+    cand.pos = V3{cand_pos3[0], cand_pos3[1], cand_pos3[2]};
+    cand.normal = V3{normal3[0], normal3[1], normal3[2]};
+    cand.depth = cand_depth;
+    cand.id = 0;
+    bool add = false;
+    int r = reduce_contacts(p, cand, &add);
+    *add_candidate = add ? 1 : 0;
+    return r;
+}
+
 // Stand-alone collision tests for closed-form checks.
 int refcpu_box_box(const float* halfA, const float* posA, const float* qA, const float* halfB, const float* posB,
                    const float* qB, float* axis_out, float* distance_out, float* contacts_out /* 8 x (x,y,z,depth) */,

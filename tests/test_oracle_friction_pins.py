@@ -1,0 +1,548 @@
+"""Closed-form pins of the oracle's friction rows, stabilization, box manifold
+persistence and contact reducer.  CPU only.
+
+Every expectation here comes from rigid-body mechanics or from a hand
+derivation on a hand-built input, not from the oracle's own formulas, and is
+chosen so that a wrong Jacobian entry, a wrong 2x2 inverse, a swapped axis or
+a wrong friction coefficient fails it:
+
+- ContactFrictionConstraint (ContactFrictionConstraint.cs:131-287): a sphere
+  on a static ground ends rolling at v0 * m r^2 / (m r^2 + I), from an impulse
+  (static friction) or after sliding (kinetic friction); angular momentum about
+  the contact point is conserved.
+- SlidingFrictionTwoAxis (SlidingFrictionTwoAxis.cs:155-482): a cube sliding
+  on a static ground decelerates by mu_k * sum(normal impulses) / m per step
+  along a straight line, its front contacts carry mu_k * N more than its back
+  ones (torque balance), and it stops after the discrete Coulomb distance.
+- TwistFrictionConstraint (TwistFrictionConstraint.cs:65-204): one iteration
+  of the static row leaves w0 * (1/m) / (n.I^-1.n + 1/m) (the reference adds the
+  inverse mass to this angular row's effective mass, TwistFrictionConstraint.cs:133-147);
+  while spinning fast, I_n dw = -twist impulse and |twist impulse| =
+  mu_k * sum(lever_i * normal_i).
+- Stabilization (EntityBase.cs:927-936, SimulationIslandMember.cs:53-70): a
+  slow free body's damping is boosted to 1 - (1 - d)(1 - boost) with
+  boost = 1 - |v| / (2 * 0.26), from the step after it starts slowing down,
+  or once it has been slow for more than LowVelocityTimeMinimum.
+- Box manifold ids (BoxContactManifold.cs:125-206): a resting box keeps the
+  same contact ids in the same list order step after step.
+- ContactReducer (ContactReducer.cs:197-318): keep sets derived by hand.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from bepuphysics1_amd import scenes
+
+G = 9.81
+DT = 1.0 / 60.0
+
+
+def _settings(oracle, damping=False, stabilization=False, iterations=None, gravity=(0.0, -G, 0.0)):
+    s = oracle.default_settings()
+    if not damping:
+        s.linear_damping = 0.0
+        s.angular_damping = 0.0
+    s.use_stabilization = 1 if stabilization else 0
+    if iterations is not None:
+        s.iteration_limit = iterations
+    s.gravity[:] = list(gravity)
+    return s
+
+
+def _yaw(theta):
+    return (0.0, math.sin(theta / 2), 0.0, math.cos(theta / 2))
+
+
+def _world(oracle, rows, **kw):
+    w = oracle.World(_settings(oracle, **kw))
+    w.add_scene(scenes.make(rows))
+    return w
+
+
+# Ground material (static 1, kinetic 0.5) times body material (1, 1): the pair
+# has static friction 1 and kinetic friction 0.5 (product blend, MaterialManager.cs:77-85).
+GROUND = dict(material=(1.0, 0.5, 0.0))
+MU_S, MU_K = 1.0, 0.5
+
+
+# ---------------------------------------------------------------- ContactFrictionConstraint
+
+def _sphere_inertia(m, r):
+    # SphereShape volume distribution 0.4 r^2 (solid sphere) times
+    # InertiaHelper.InertiaTensorScale 2.5: I = m r^2.
+    return 2.5 * m * 0.4 * r * r
+
+
+@pytest.mark.parametrize("v0", [(0.12, 0.0, -0.05), (-0.03, 0.0, 0.09)])
+def test_sphere_static_friction_impulse_gives_rolling(oracle, v0):
+    """Static regime (|v| < 0.2, impulse inside the cone): one step takes the
+    sphere from sliding to rolling: v = v0 m r^2 / (m r^2 + I), w = n x v / r
+    with the contact below the centre; later steps keep it rolling."""
+    m, r = 1.3, 0.5
+    w = _world(oracle, [scenes.ground(**GROUND),
+                        scenes._row(scenes.SHAPE_SPHERE, (r, 0, 0), m, (0, r, 0), vel=v0, material=(1, 1, 0))])
+    I = _sphere_inertia(m, r)
+    k = m * r * r / (m * r * r + I)
+    v0 = np.asarray(v0)
+    x_prev = np.array([0.0, r, 0.0])
+    for step in range(4):
+        w.step(1)
+        ms = w.motion_states()[1]
+        v, om = ms[7:10], ms[10:13]
+        # the first step starts touching (lever r exactly); later the soft
+        # contact sinks by about 1.6e-4 per step and the lever shortens by as much
+        tol = 2e-6 if step == 0 else 1e-3 * np.abs(v0).max()
+        assert np.allclose(v[[0, 2]], k * v0[[0, 2]], atol=tol), (step, v)
+        # rolling without slipping: the material point at the contact is at
+        # rest horizontally, v + w x (c - x) = 0, with the contact and the
+        # centre where the step's solver saw them (before the position update)
+        (man,) = w.manifolds()
+        lever = np.asarray(man.contacts[0].position) - x_prev
+        slip = v + np.cross(om, lever)
+        assert np.abs(slip[[0, 2]]).max() < 2e-6
+        assert lever[1] == pytest.approx(-r, abs=5e-4)
+        assert abs(om[1]) < 1e-6
+        x_prev = ms[0:3].astype(float)
+
+
+def test_sphere_kinetic_friction_ends_rolling(oracle):
+    """Kinetic regime (a bowling ball): while it slips, each step's friction
+    impulse is mu_k times the normal impulse, opposite to the slip, and the
+    sphere ends rolling at v0 m r^2 / (m r^2 + I) (angular momentum about the
+    contact point is conserved)."""
+    m, r = 2.0, 0.4
+    v0 = np.array([1.6, 0.0, 1.2])
+    w = _world(oracle, [scenes.ground(**GROUND),
+                        scenes._row(scenes.SHAPE_SPHERE, (r, 0, 0), m, (0, r, 0), vel=v0, material=(1, 1, 0))])
+    I = _sphere_inertia(m, r)
+    vprev = v0.copy()
+    slipping_steps = 0
+    rolling = None
+    for _ in range(40):
+        w.step(1)
+        ms = w.motion_states()[1]
+        v = ms[7:10].astype(float)
+        slip = v[[0, 2]] + np.array([ms[12], -ms[10]]) * r  # v + w x (0,-r,0), x and z
+        (imp,) = w.impulses()
+        # still slipping above the static threshold after the step: the
+        # whole step was kinetic and the friction saturated
+        if np.linalg.norm(slip) > 0.25:
+            slipping_steps += 1
+            # friction impulse = mu_k * normal impulse, and the velocity change is that impulse over m
+            assert abs(imp.friction[0]) == pytest.approx(MU_K * imp.normal[0], rel=1e-4)
+            dv = np.linalg.norm((v - vprev)[[0, 2]])
+            assert m * dv == pytest.approx(abs(imp.friction[0]), rel=1e-3)
+            # straight line: the friction acts along the initial slip direction
+            assert v[0] * v0[2] - v[2] * v0[0] == pytest.approx(0.0, abs=1e-5)
+        elif rolling is None and np.linalg.norm(slip) < 0.01:
+            rolling = v
+        vprev = v
+    assert slipping_steps >= 8
+    # The soft contact sinks 1.6e-4 per step while it slides, so the lever is
+    # 0.4% short of r when it starts rolling: within 0.5% of the closed form.
+    assert np.allclose(rolling[[0, 2]], v0[[0, 2]] * m * r * r / (m * r * r + I), rtol=5e-3)
+
+
+# ---------------------------------------------------------------- SlidingFrictionTwoAxis
+
+def _cube_slide(oracle, theta, v0, steps):
+    """Unit cube, mass 1, yawed by theta, launched along its local +x axis."""
+    d = np.array([math.cos(theta), 0.0, -math.sin(theta)])  # local +x in world
+    w = _world(oracle, [scenes.ground(**GROUND),
+                        scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 0.498, 0), quat=_yaw(theta),
+                                    vel=tuple(v0 * d), material=(1, 1, 0))])
+    out = []
+    for _ in range(steps):
+        w.step(1)
+        ms = w.motion_states()[1].astype(float)
+        (imp,) = w.impulses()
+        (man,) = w.manifolds()
+        out.append((ms, imp, man))
+    return d, out
+
+
+def test_cube_sliding_decelerates_at_mu_k_n(oracle):
+    """While sliding (|v| > 0.2): |sliding impulse| = mu_k * sum(normal impulses),
+    the speed drops by exactly that over m each step, and the box keeps its
+    straight line (the friction is antiparallel to the slip)."""
+    theta, v0 = 0.4, 3.0
+    d, run = _cube_slide(oracle, theta, v0, 30)
+    vprev = math.hypot(run[1][0][7], run[1][0][9])
+    for ms, imp, _ in run[2:]:
+        v = ms[7:10]
+        speed = math.hypot(v[0], v[2])
+        ntot = sum(imp.normal[:imp.count])
+        assert imp.count == 4
+        # the clamp uses the normal impulses of the row's last active
+        # iteration; rows stop at |d lambda| < 0.001 (SolverGroup early-out)
+        assert math.hypot(*imp.sliding) == pytest.approx(MU_K * ntot, abs=MU_K * 2e-3)
+        assert vprev - speed == pytest.approx(math.hypot(*imp.sliding) / 1.0, rel=2e-3)
+        assert math.atan2(-v[2], v[0]) == pytest.approx(theta, abs=2e-4)
+        assert ntot == pytest.approx(G * DT, rel=0.05)
+        vprev = speed
+    # about mu_k g dt per step in all
+    assert v0 - vprev == pytest.approx(len(run) * MU_K * G * DT, rel=0.06)
+
+
+def test_cube_sliding_front_contacts_carry_the_friction_torque(oracle):
+    """Torque balance of a sliding block: friction mu_k N acts at the base, h
+    below the centre of mass; with no tipping the normal impulses at the
+    leading face (half-width a ahead) exceed the trailing ones by
+    mu_k N h / a (here h = a, so by mu_k N)."""
+    theta = 0.4
+    d, run = _cube_slide(oracle, theta, 3.0, 24)
+    for ms, imp, man in run[8:]:
+        pos = np.array([man.contacts[i].position for i in range(man.count)])
+        ids = [man.contacts[i].id for i in range(man.count)]
+        assert len(set(ids)) == 4
+        # the impulse list follows the penetration constraint order, which for
+        # this manifold is its contact order (no contact was removed)
+        normals = np.array(imp.normal[:imp.count])
+        ahead = (pos - ms[0:3]) @ d > 0
+        n_front, n_back = normals[ahead].sum(), normals[~ahead].sum()
+        ntot = n_front + n_back
+        h = ms[1] - pos[:, 1].mean()
+        assert n_front - n_back == pytest.approx(MU_K * ntot * h / 0.5, rel=0.03)
+
+
+def test_cube_stopping_distance_discrete_coulomb(oracle):
+    """The cube stops where discrete Coulomb friction puts it: v_{k+1} =
+    max(0, v_k - mu(v_k) g dt), mu = mu_k above the static threshold 0.2 and
+    mu_s below, x_{k+1} = x_k + v_{k+1} dt (the position update uses the
+    solved velocity); then static friction holds it."""
+    theta, v0 = -0.7, 2.5
+    d, run = _cube_slide(oracle, theta, v0, 70)
+    v, x = v0, 0.0
+    for _ in range(70):
+        mu = MU_K if v > 0.2 else MU_S
+        v = max(0.0, v - mu * G * DT)
+        x += v * DT
+    final = run[-1][0]
+    travelled = (final[0:3] - np.array([0, 0.498, 0])) @ d
+    assert travelled == pytest.approx(x, rel=0.02)
+    # stopped horizontally (vertically a resting contact keeps the small sink
+    # speed that its softness term balances against the penetration bias)
+    assert np.abs(final[[7, 9]]).max() < 1e-3
+    # perpendicular drift stays negligible
+    perp = np.cross(final[0:3] - np.array([0, 0.498, 0]), d)
+    assert abs(perp[1]) < 1e-3
+
+
+def _qmul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return (aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+            aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz)
+
+
+@pytest.mark.parametrize("tilt,yaw,vel", [(0.0, 0.8, (0.04, 0.0, 0.03)), (0.0, -0.5, (-0.02, 0.0, 0.05)),
+                                          (0.25, 0.7, (0.03, 0.0, -0.04)), (-0.2, -1.1, (-0.05, 0.0, 0.01))])
+def test_static_sliding_friction_zeroes_centre_slip_in_one_iteration(oracle, tilt, yaw, vel):
+    """A flat 2 x 0.5 x 1 box, yawed so that its world inverse inertia couples
+    its two tangent directions: the 2x2 effective mass of the tangent rows has
+    off-diagonal terms.  The ground is flat or tilted about the axis (1, 0, 1),
+    so that the normal and both tangent axes have x, y and z parts.  In the
+    static regime the row is one exact 2x2 solve, so after a single iteration
+    (penetration rows, then sliding, then twist, which does not move the
+    centre) the material point at the manifold centre has no tangential
+    velocity left: v + w x (c - x) lies along the normal."""
+    qt = _quat_axis((1, 0, 1), tilt) if tilt else (0.0, 0.0, 0.0, 1.0)
+    n = _rot(qt)[:, 1]
+    ground = scenes._row(scenes.SHAPE_BOX, (40.0, 1.0, 40.0), 0.0, tuple(-0.5 * n), quat=qt,
+                         material=GROUND["material"], flags=0)
+    q = _qmul(qt, _yaw(yaw))
+    x0 = (0.25 - 0.002) * n
+    w = _world(oracle, [ground, scenes._row(scenes.SHAPE_BOX, (2.0, 0.5, 1.0), 1.5, tuple(x0), quat=q,
+                                            vel=vel, angvel=(0.03, 0.0, -0.02), material=(1, 1, 0))],
+               iterations=1)
+    x_prev = x0
+    for _ in range(3):
+        w.step(1)
+        ms = w.motion_states()[1].astype(float)
+        (man,) = w.manifolds()
+        assert man.count == 4
+        centre = np.mean([man.contacts[i].position for i in range(4)], axis=0)
+        slip = ms[7:10] + np.cross(ms[10:13], centre - x_prev)
+        assert np.linalg.norm(slip - (slip @ n) * n) < 3e-6
+        x_prev = ms[0:3]
+
+
+def _quat_axis(axis, angle):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    return tuple(a * math.sin(angle / 2)) + (math.cos(angle / 2),)
+
+
+def _rot(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def _incline(oracle, theta, material, steps):
+    """A static slab tilted by theta about the horizontal axis (1, 0, 1):
+    its normal and fall line have x, y and z parts.  A unit cube rests on it,
+    aligned with it."""
+    q = _quat_axis((1, 0, 1), theta)
+    R = _rot(q)
+    n = R[:, 1]
+    centre = -0.5 * n  # slab 40 x 1 x 40 whose top face passes through the origin
+    ground = scenes._row(scenes.SHAPE_BOX, (40.0, 1.0, 40.0), 0.0, tuple(centre), quat=q, material=material,
+                         flags=0)
+    box = scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, tuple((0.5 - 0.002) * n), quat=q, material=(1, 1, 0))
+    w = _world(oracle, [ground, box])
+    g = np.array([0.0, -G, 0.0])
+    fall = g - (g @ n) * n
+    fall /= np.linalg.norm(fall)
+    out = []
+    for _ in range(steps):
+        w.step(1)
+        out.append(w.motion_states()[1].astype(float))
+    return n, fall, out
+
+
+def test_incline_static_friction_holds(oracle):
+    """tan(theta) below mu_s: static friction holds the cube on the incline."""
+    theta = 0.3  # tan = 0.31 < mu_s = 0.5
+    n, fall, run = _incline(oracle, theta, (0.5, 0.3, 0.0), 120)
+    start = (0.5 - 0.002) * n
+    moved = run[-1][0:3] - start
+    # it sinks along the normal to the soft contact's resting depth; along
+    # the incline the sequential solver creeps by at most about 2 mm in 2 s
+    assert np.linalg.norm(moved - (moved @ n) * n) < 3e-3
+    # (kinetic friction 0.3 < tan(theta) would slide it at 0.09 m/s^2: 0.17 m/s after 2 s)
+    assert np.linalg.norm(run[-1][7:10] - (run[-1][7:10] @ n) * n) < 5e-3
+
+
+def test_incline_slides_at_coulomb_acceleration(oracle):
+    """tan(theta) above mu_s: the cube slides down the fall line with the
+    acceleration g (sin theta - mu cos theta), mu = mu_s while the slip is
+    below 0.2 (the row is saturated) and mu_k above it, without leaving the
+    fall line.  The row picks mu from the slip it sees at prestep time, after
+    this step's gravity: v + g dt sin(theta) along the fall line."""
+    theta = 0.5  # tan = 0.546 > mu_s = 0.5
+    mu_s, mu_k = 0.5, 0.3
+    n, fall, run = _incline(oracle, theta, (mu_s, mu_k, 0.0), 50)
+    speeds = [float(ms[7:10] @ fall) for ms in run]
+    for k in range(6, len(run)):
+        v = run[k][7:10]
+        tangential = v - (v @ n) * n
+        # along the fall line
+        assert np.linalg.norm(tangential - (tangential @ fall) * fall) < 1e-3 * max(1.0, np.linalg.norm(tangential))
+        seen = speeds[k - 1] + G * DT * math.sin(theta)
+        mu = mu_k if seen > 0.2 else mu_s
+        expect = G * DT * (math.sin(theta) - mu * math.cos(theta))
+        if abs(seen - 0.2) > 0.02:  # away from the switch
+            assert speeds[k] - speeds[k - 1] == pytest.approx(expect, rel=0.02), k
+    assert speeds[-1] > 1.0
+
+
+# ---------------------------------------------------------------- TwistFrictionConstraint
+
+def _cube_spin(oracle, w0, iterations=None, m=1.0, size=1.0, theta=0.3):
+    return _world(oracle, [scenes.ground(**GROUND),
+                           scenes._row(scenes.SHAPE_BOX, (size, size, size), m, (0, size / 2 - 0.002, 0),
+                                       quat=_yaw(theta), angvel=(0, w0, 0), material=(1, 1, 0))],
+                  iterations=iterations)
+
+
+@pytest.mark.parametrize("m,size", [(1.0, 1.0), (3.0, 0.6)])
+def test_twist_static_one_iteration_effective_mass(oracle, m, size):
+    """One iteration of the static twist row removes lambda = -w / (e + 1/m)
+    with e = n.I^-1.n: the spin left is w0 (1/m) / (e + 1/m).  (Without the
+    inverse mass term the row would stop the spin outright.)"""
+    w0 = 0.15
+    w = _cube_spin(oracle, w0, iterations=1, m=m, size=size)
+    w.step(1)
+    ms = w.motion_states()[1]
+    # cube: volume distribution (s^2 + s^2) / 12 about y, times 2.5
+    I = 2.5 * m * (2 * size * size) / 12
+    e = 1 / I
+    assert ms[11] == pytest.approx(w0 * (1 / m) / (e + 1 / m), rel=1e-4)
+
+
+def test_twist_kinetic_decelerates_at_mu_k_lever_n(oracle):
+    """Spinning fast (|w| > 0.2): |twist impulse| = mu_k * sum_i |c_i - centre|
+    * normal_i over the contacts, and it is the cube's whole change of angular
+    momentum about the normal.  The pair's A is the ground (the lower id), so
+    the cube, its B, receives -impulse * n: I dw_y = -impulse * n_y."""
+    m, size, w0 = 1.0, 1.0, 4.0
+    w = _cube_spin(oracle, w0, m=m, size=size)
+    I = 2.5 * m * (2 * size * size) / 12
+    wprev = w0
+    checked = 0
+    for _ in range(45):
+        w.step(1)
+        ms = w.motion_states()[1]
+        (imp,) = w.impulses()
+        (man,) = w.manifolds()
+        om = float(ms[11])
+        if wprev > 0.3 and om > 0.3:
+            pos = np.array([man.contacts[i].position for i in range(man.count)])
+            lever = np.linalg.norm(pos - pos.mean(axis=0), axis=1)
+            bound = MU_K * float(lever @ np.array(imp.normal[:imp.count]))
+            ny = man.contacts[0].normal[1]
+            assert abs(ny) == pytest.approx(1.0, abs=1e-5)
+            assert abs(imp.twist) == pytest.approx(bound, abs=MU_K * 2e-3)
+            assert I * (om - wprev) == pytest.approx(-imp.twist * ny, rel=1e-3)
+            assert om < wprev
+            checked += 1
+        wprev = om
+    assert checked >= 20
+    # it stops after about w0 I / (mu_k a sqrt(2) m g dt) steps (a = half size)
+    assert abs(wprev) < 1e-3
+
+
+# ---------------------------------------------------------------- stabilization
+
+def test_stabilization_boost_closed_form(oracle):
+    """A free, slow body (no gravity, default damping 0.03 / 0.15): the first
+    step damps by (1 - d)^dt; once it is slowing, each step damps linear and
+    angular velocity by ((1 - d) * (1 - boost))^dt with boost = 1 - sqrt(E) /
+    (2 * 0.26), E = |v|^2 + |w|^2 before the damping."""
+    w = _world(oracle, [scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 5, 0), vel=(0.1, 0.02, 0.0),
+                                    angvel=(0.0, 0.05, 0.03))],
+               damping=True, stabilization=True, gravity=(0, 0, 0))
+    v = np.array([0.1, 0.02, 0.0])
+    om = np.array([0.0, 0.05, 0.03])
+    for step in range(40):
+        if step == 0:
+            fl, fa = 0.97 ** DT, 0.85 ** DT
+        else:
+            E = v @ v + om @ om
+            boost = 1 - math.sqrt(E) / (2 * 0.26)
+            fl = (0.97 * (1 - boost)) ** DT
+            fa = (0.85 * (1 - boost)) ** DT
+        v, om = v * fl, om * fa
+        w.step(1)
+        ms = w.motion_states()[0]
+        assert np.allclose(ms[7:10], v, rtol=2e-5, atol=1e-9), step
+        assert np.allclose(ms[10:13], om, rtol=2e-5, atol=1e-9), step
+
+
+def test_stabilization_no_boost_when_fast(oracle):
+    """Above the velocity limit (E >= 0.26^2) only the plain damping applies."""
+    w = _world(oracle, [scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 5, 0), vel=(0.3, 0.0, 0.0))],
+               damping=True, stabilization=True, gravity=(0, 0, 0))
+    w.step(10)
+    assert w.motion_states()[0][7] == pytest.approx(0.3 * 0.97 ** (10 * DT), rel=1e-5)
+
+
+@pytest.mark.parametrize("dt,expect_first", [(1.0 / 60.0, 61), (1.0 / 64.0, 65)])
+def test_stabilization_after_low_velocity_time(oracle, dt, expect_first):
+    """A slow body that speeds up (so it is never 'slowing') gets the boost
+    only once its time below the limit exceeds LowVelocityTimeMinimum (1 s),
+    strictly: from the first step whose accumulated float32 time is > 1.  With
+    dt = 1/64 the time after 64 steps is exactly 1, which is not enough."""
+    # It starts moving (a body at rest counts as slowing: 0 <= 0) and speeds up
+    # under a weak gravity, so it is never slowing.
+    g, v0 = -0.01, -0.001
+    s = _settings(oracle, stabilization=True, gravity=(0, g, 0))
+    s.dt = dt
+    w = oracle.World(s)
+    w.add_scene(scenes.make([scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 5, 0), vel=(0, v0, 0))]))
+    t = np.float32(0)
+    first = None
+    for k in range(1, 90):
+        t = np.float32(t + np.float32(dt))
+        if first is None and t > 1:
+            first = k  # the candidacy of step k sees t > 1; the boost applies in that step
+    # without damping and before the boost, v_n = v0 + n g dt (to float rounding)
+    for n in range(1, 90):
+        w.step(1)
+        vy = float(w.motion_states()[0][8])
+        if n < first:
+            assert vy == pytest.approx(v0 + n * g * dt, rel=1e-4), n
+        else:
+            # boosted: linear damping 0 + boost (1 - 0), E after the gravity update
+            pre = v0 + n * g * dt
+            boost = 1 - abs(pre) / (2 * 0.26)
+            assert vy == pytest.approx(pre * (1 - boost) ** dt, rel=1e-4)
+            break
+    assert n == first == expect_first
+
+
+# ---------------------------------------------------------------- box manifold persistence
+
+def test_resting_box_keeps_contact_ids_and_order(oracle):
+    """A cube resting on the ground: four contacts whose ids (feature ids)
+    and list order stay the same from step to step, and whose normal
+    impulses add up to the weight m g dt."""
+    w = _world(oracle, [scenes.ground(**GROUND),
+                        scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 2.0, (0.3, 0.497, -0.2), quat=_yaw(0.9),
+                                    material=(1, 1, 0))], damping=True, stabilization=True)
+    w.step(30)
+    ref = None
+    for _ in range(20):
+        w.step(1)
+        (man,) = w.manifolds()
+        ids = [man.contacts[i].id for i in range(man.count)]
+        assert man.count == 4 and len(set(ids)) == 4
+        if ref is None:
+            ref = ids
+        assert ids == ref
+        (imp,) = w.impulses()
+        assert sum(imp.normal[:4]) == pytest.approx(2.0 * G * DT, rel=0.02)
+
+
+def test_persistent_contacts_carry_warm_start(oracle):
+    """With one solver iteration per step a resting cube only settles level
+    if its matched contacts keep their accumulated impulses from step to step
+    (warm start): the total normal impulse then equals m g dt and the cube
+    does not rock.  Contacts re-created every step would start from zero and
+    a single sequential pass over four corners leaves a net torque."""
+    w = _world(oracle, [scenes.ground(**GROUND),
+                        scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 0.497, 0), quat=_yaw(0.3),
+                                    material=(1, 1, 0))], iterations=1)
+    w.step(120)
+    worst = 0.0
+    for _ in range(30):
+        w.step(1)
+        ms = w.motion_states()[1]
+        (imp,) = w.impulses()
+        assert sum(imp.normal[:imp.count]) == pytest.approx(G * DT, rel=0.01)
+        worst = max(worst, float(np.abs(ms[[10, 12]]).max()))
+    assert worst < 1e-3
+    # still level: the body's y axis stays vertical
+    x, y, z, wq = ms[3:7]
+    assert 1 - 2 * (x * x + z * z) == pytest.approx(1.0, abs=1e-5)
+
+
+# ---------------------------------------------------------------- ContactReducer
+
+SQUARE = [(1, 0, 1), (-1, 0, 1), (-1, 0, -1), (1, 0, -1)]
+UP = (0, 1, 0)
+
+
+def test_reducer_keeps_manifold_for_interior_shallow_candidate(oracle):
+    """A shallow candidate at the centre of the square is none of the four
+    extremes (deepest, furthest from it, two extremes across): nothing changes."""
+    r, add = oracle.reduce_contacts(SQUARE, [0.01] * 4, UP, (0, 0, 0), 0.005)
+    assert (r, add) == (-1, False)
+
+
+def test_reducer_deep_centre_candidate_drops_the_far_corner(oracle):
+    """A candidate deeper than every contact at the centre: it is the deepest;
+    the furthest from it is corner 0 (all corners tie; the first strict
+    maximum wins); corners 1 and 3 are the extremes across the 0-2 diagonal;
+    so corner 2 goes and the candidate comes in."""
+    r, add = oracle.reduce_contacts(SQUARE, [0.01] * 4, UP, (0, 0, 0), 0.05)
+    assert (r, add) == (2, True)
+
+
+def test_reducer_far_candidate_replaces_the_nearest_corner(oracle):
+    """Corner 0 deepest; a candidate far out at (-4, 0, 0) is the furthest
+    from it; across the 0-candidate line corners 3 and 1 are the extremes;
+    corner 2 (next to the candidate) is dropped."""
+    r, add = oracle.reduce_contacts(SQUARE, [0.02, 0.01, 0.01, 0.01], UP, (-4, 0, 0), 0.0)
+    assert (r, add) == (2, True)
+
+
+def test_reducer_candidate_extreme_across_only(oracle):
+    """Corner 1 deepest, corner 3 furthest from it; a shallow candidate far out
+    along the 2-0 diagonal beyond corner 0 is the extreme across the 1-3 line:
+    it joins, and the first corner that is not an extreme (corner 0 is no
+    longer one; corner 2 still is) goes."""
+    r, add = oracle.reduce_contacts(SQUARE, [0.01, 0.03, 0.01, 0.01], UP, (1.5, 0, 1.5), 0.0)
+    assert (r, add) == (0, True)
