@@ -115,13 +115,42 @@ STAGE_KERNELS = {
 }
 
 
-def measured_traffic(stage, workload):
+def measured_traffic(stage, workload, window=None):
+    """HBM bytes per launch of the stage's kernels from the counter passes in
+    profiles/traffic_<workload>.json, when that file was measured over the
+    same steps as this run (`window` = (steps, warmup)); else None."""
     path = os.path.join(ROOT, "profiles", "traffic_%s.json" % workload)
     if stage not in STAGE_KERNELS or not os.path.exists(path):
         return None
-    t = json.load(open(path))["bytes_per_launch"]
+    doc = json.load(open(path))
+    if window is not None and tuple(doc.get("window", ())) != tuple(window):
+        return None
+    t = doc["bytes_per_launch"]
     ks = [k for k in t if any(s in k for s in STAGE_KERNELS[stage])]
     return sum(t[k] for k in ks) if ks else None
+
+
+def host_cpu():
+    """nproc, the threads this process may run on, and the CPU model."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": affinity, "model": model}
+
+
+def baseline_threads():
+    """Host threads for the oracle baseline: every core this process may use,
+    at most 16 (one GPU's share of the host on the benchmark machines)."""
+    return max(1, min(16, host_cpu()["affinity"] or 1))
 
 
 def stage_table(stages, counts):
@@ -133,12 +162,13 @@ def stage_table(stages, counts):
     return out
 
 
-def roofline(stages, counts, workload):
+def roofline(stages, counts, workload, window=None):
     dominant = max(stages, key=lambda k: stages[k])
     dom_bytes = algorithmic_bytes(dominant, **counts)
     achieved = dom_bytes / (stages[dominant] * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant, workload),
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(dominant, workload, window),
+            "traffic_window": "steps %d-%d after the upload" % (window[1] + 1, window[1] + window[0]) if window else None,
             "kernel": dominant, "kernel_ms": round(stages[dominant], 5), "alg_bytes_per_launch": int(dom_bytes)}
 
 
@@ -189,9 +219,15 @@ def batched_run(a):
             tsp.add_world(w)
         tsp.Update(a.autotune_after)
         tune = tsp.autotune(a.autotune_steps)
+        tune["tuned_after_steps"] = a.autotune_after
+        # The settled phase, which most of a long run is: the tuned copy is
+        # now more than 300 steps in; time a fixed number of its steps.
+        settled = timed_steps(tsp, a.settled_steps, n_dyn)
+        settled["after_steps"] = a.autotune_after + tune["steps"]
         tsp.close()
         sp.set_tuning(**{k: tune[k] for k in Space.TUNING_FIELDS})
-        tune["tuned_after_steps"] = a.autotune_after
+    else:
+        settled = None
     if a.warmup:
         sp.Update(a.warmup)
     sp.stats()  # synchronise
@@ -200,17 +236,38 @@ def batched_run(a):
     st = sp.stats()  # synchronises the world's stream
     elapsed = time.perf_counter() - t0
     counts = counts_of(st, BATCHED_ITERATIONS)
-    # Stage times (HIP events between the stages) in a separate run, so that
-    # the events' own cost stays out of `value`.
-    sp.set_timing(True)
-    sp.Update(min(a.steps, 20))
-    stages = sp.stage_times()
-    sp.set_timing(False)
     lat = latency(sp, a.latency_steps)
     sp.close()
+    # Stage times: HIP events between the stages of the same steps as the
+    # timed window (W + 1 .. W + K), on a replay of the batch (every step is
+    # deterministic, so the replay's steps are the timed steps bit for bit),
+    # so that the events' own cost stays out of `value`.
+    rsp = Space(capacity=sum(len(w) for w in worlds), settings=batched_settings())
+    rsp.register_hulls(scenes.hull_library())
+    for w in worlds:
+        rsp.add_world(w)
+    rsp.set_tuning(**{k: tune[k] for k in Space.TUNING_FIELDS})
+    if a.warmup:
+        rsp.Update(a.warmup)
+    rsp.set_timing(True)
+    rsp.Update(a.steps)
+    stages = rsp.stage_times()
+    rsp.close()
     return worlds, {
         "value": n_dyn * a.steps / elapsed, "ms_per_step": 1e3 * elapsed / a.steps, "bodies": n_dyn,
-        "stats": st, "counts": counts, "stages": stages, "latency": lat, "tuning": tune}
+        "stats": st, "counts": counts, "stages": stages, "latency": lat, "tuning": tune, "settled": settled}
+
+
+def timed_steps(sp, steps, n_dyn):
+    """`steps` synchronised steps of a world handle as a body-steps/s line."""
+    sp.stats()
+    t0 = time.perf_counter()
+    sp.Update(steps)
+    st = sp.stats()
+    dt = time.perf_counter() - t0
+    return {"value": round(n_dyn * steps / dt, 1), "ms_per_step": round(1e3 * dt / steps, 4), "steps": steps,
+            "pairs": st.pairs, "manifolds": st.manifolds, "contacts": st.contacts,
+            "convex_pairs_by_state": list(st.convex_pairs)}
 
 
 def plain_path(worlds, sample_worlds, sample_steps):
@@ -269,7 +326,8 @@ def cpu_baseline_batched(worlds, sample_worlds, warmup, steps, threads):
     return {"value": round(n * steps / dt, 1), "unit": "body-steps/s", "cores": threads, "kind": "port",
             "sample": "oracle (C++ restatement of the reference step), %d of the %d worlds (%d bodies), one "
                       "oracle world per task on %d host threads, %d steps after %d untimed steps, %.1f s"
-                      % (len(pick), len(worlds), n, threads, steps, warmup, dt)}
+                      % (len(pick), len(worlds), n, threads, steps, warmup, dt),
+            "host": host_cpu()}
 
 
 def f32bits(a):
@@ -393,10 +451,10 @@ def towers_leg(a, cpu, parity):
            "bodies": n_dyn, "steps": a.tower_steps, "value": round(value, 1),
            "ms_per_step": round(1e3 * elapsed / a.tower_steps, 4),
            "pairs": st.pairs, "manifolds": st.manifolds, "contacts": st.contacts, "colors": st.colors,
-           "roofline": roofline(stages, counts, "towers"), "stages": stage_table(stages, counts),
+           "roofline": roofline(stages, counts, "towers", (a.tower_steps, 10)), "stages": stage_table(stages, counts),
            "latency_ms": lat, "towers_standing": standing, "max_penetration": round(float(st.max_penetration), 5)}
     if cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = baseline_threads()
         v_cpu, dt_cpu = cpu_baseline_towers(scene, a.cpu_steps, threads)
         v_1t, dt_1t = cpu_baseline_towers(scene, max(2, a.cpu_steps // 4), 1)
         out["cpu_baseline"] = {
@@ -405,7 +463,7 @@ def towers_leg(a, cpu, parity):
                       "and pair updates, colour-parallel solver), same 81,920-box towers scene, %d steps after 1 "
                       "untimed step, %.1f s" % (threads, a.cpu_steps, dt_cpu),
             "single_thread_value": round(v_1t, 1),
-            "single_thread_sample": "%d steps, %.1f s" % (max(2, a.cpu_steps // 4), dt_1t)}
+            "single_thread_sample": "%d steps, %.1f s" % (max(2, a.cpu_steps // 4), dt_1t), "host": host_cpu()}
         out["speedup_vs_cpu"] = round(value / v_cpu, 1)
     if parity:
         out["parity"] = parity_towers()
@@ -489,6 +547,73 @@ def calibrate_copy(nbytes=1 << 30):
     return nbytes
 
 
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _window_index(rows, steps, warmup):
+    """Per launch (rows in dispatch order): whether it belongs to a timed
+    step.  A step starts at its k_step_reset launch; the profiled command runs
+    the batch for warmup + steps steps, then its replay for as many (run with
+    --latency-steps 0 --autotune-steps 0 --plain-sample 0 --no-parity
+    --no-cpu-baseline --settled-steps 0), so the run's step j of each run is
+    timed when j > warmup."""
+    run_len = warmup + steps
+    g = 0
+    out = []
+    for r in rows:
+        if "k_step_reset" in r["Kernel_Name"]:
+            g += 1
+        j = (g - 1) % run_len + 1 if g else 0
+        out.append(g > 0 and j > warmup)
+    return out
+
+
+def summarize_trace(directory, steps, warmup, fetch_scale, write_scale):
+    """Per-kernel launch times (kernel trace) and HBM bytes (FETCH_SIZE /
+    WRITE_SIZE counter passes, scaled) of a rocprofv3 --output-format csv run
+    of the batched leg, over all launches and over the timed window's."""
+    import glob
+    out = {"window": [steps, warmup], "kernels": {}}
+    traces = glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True)
+    counters = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)
+    for path in traces:
+        rows = sorted(_csv_rows(path), key=lambda r: int(r["Dispatch_Id"]))
+        win = _window_index(rows, steps, warmup)
+        for r, w in zip(rows, win):
+            k = out["kernels"].setdefault(r["Kernel_Name"], {"calls": 0, "us": 0.0, "win_calls": 0, "win_us": 0.0})
+            us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            k["calls"] += 1
+            k["us"] += us
+            if w:
+                k["win_calls"] += 1
+                k["win_us"] += us
+    for path in counters:
+        rows = _csv_rows(path)
+        by_dispatch = {}
+        for r in rows:
+            d = by_dispatch.setdefault(int(r["Dispatch_Id"]), {"Kernel_Name": r["Kernel_Name"]})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        disp = [by_dispatch[i] for i in sorted(by_dispatch)]
+        win = _window_index(disp, steps, warmup)
+        for d, w in zip(disp, win):
+            if not w:
+                continue
+            k = out["kernels"].setdefault(d["Kernel_Name"], {"calls": 0, "us": 0.0, "win_calls": 0, "win_us": 0.0})
+            for name, scale in (("FETCH_SIZE", fetch_scale), ("WRITE_SIZE", write_scale)):
+                if name in d:
+                    k.setdefault(name, []).append(d[name] * 1024.0 * scale)
+    for name, k in out["kernels"].items():
+        k["avg_us"] = k["us"] / k["calls"] if k["calls"] else None
+        k["win_avg_us"] = k["win_us"] / k["win_calls"] if k["win_calls"] else None
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            if c in k:
+                k[c + "_bytes_per_launch"] = float(np.mean(k.pop(c)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,6 +625,8 @@ def main():
     ap.add_argument("--autotune-after", type=int, default=160,
                     help="steps a copy of the batch runs before the autotuner times candidates on it")
     ap.add_argument("--latency-steps", type=int, default=50)
+    ap.add_argument("--settled-steps", type=int, default=50,
+                    help="steps timed on the autotuner's copy of the batch once it has settled")
     ap.add_argument("--plain-sample", type=int, default=48,
                     help="worlds of the batch also stepped alone, for the batched/plain ratio (0: skip)")
     ap.add_argument("--cpu-worlds", type=int, default=4096, help="worlds in the oracle's sample")
@@ -516,8 +643,17 @@ def main():
     ap.add_argument("--stress-bodies", type=int, default=32768)
     ap.add_argument("--legs", default="batched,towers,served,stress",
                     help="comma list of workloads; without batched only the other legs' results are printed")
+    ap.add_argument("--summarize-trace", metavar="DIR",
+                    help="print per-kernel averages over the timed window of a rocprofv3 csv run in DIR and exit")
+    ap.add_argument("--fetch-scale", type=float, default=2.0,
+                    help="bytes per FETCH_SIZE KiB unit / 1024, from --calibrate-copy")
+    ap.add_argument("--write-scale", type=float, default=1.0)
     a = ap.parse_args()
     legs = set(x.strip() for x in a.legs.split(",") if x.strip())
+    if a.summarize_trace:
+        print(json.dumps(summarize_trace(a.summarize_trace, a.steps, a.warmup, a.fetch_scale, a.write_scale),
+                         indent=1))
+        return
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; --gpus %d is not supported "
@@ -551,7 +687,7 @@ def main():
         plain = {"value": round(pv, 1), "sample": ps, "ratio_batched_vs_plain": round(b["value"] / pv, 1)}
     cpu_line = None
     if cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = baseline_threads()
         # The oracle's sample sits in the middle of the GPU's timed window (the
         # piles settle over the run and later steps carry more contacts).
         cpu_line = cpu_baseline_batched(worlds, a.cpu_worlds, a.warmup + a.steps // 2, a.cpu_steps, threads)
@@ -576,7 +712,8 @@ def main():
                    "worlds": a.worlds, "bodies": b["bodies"], "pairs": st.pairs, "manifolds": st.manifolds,
                    "contacts": st.contacts, "colors": st.colors, "convex_pairs_by_state": list(st.convex_pairs),
                    "autotune": b["tuning"]},
-        "roofline": roofline(stages, counts, "batched"),
+        "roofline": roofline(stages, counts, "batched", (a.steps, a.warmup)),
+        "settled": b["settled"],
         "step_roofline": {"alg_bytes_per_step": int(step_alg),
                           "achieved_GBps": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9, 2),
                           "frac": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},

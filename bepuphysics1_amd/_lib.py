@@ -148,6 +148,14 @@ EXPORTS = {
                                                     ctypes.POINTER(MotionState)]),
     "bepu_get_world_range": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(ctypes.c_uint32)]),
+    "bepu_world_create_ex": (ctypes.c_int, [ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.POINTER(_vp)]),
+    "bepu_clear_error": (ctypes.c_int, [_vp]),
+    "bepu_remove": (ctypes.c_int, [_vp, ctypes.c_uint32]),
+    "bepu_set_motion_states": (ctypes.c_int, [_vp, _u32p, ctypes.POINTER(MotionState), ctypes.c_uint32,
+                                              ctypes.c_uint32]),
+    "bepu_get_settings": (ctypes.c_int, [_vp, ctypes.POINTER(Settings)]),
+    "bepu_set_settings": (ctypes.c_int, [_vp, ctypes.POINTER(Settings)]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -282,7 +282,14 @@ __global__ void __launch_bounds__(256) k_step_reset(DeviceWorld w) {
         w.gt_key[i] = ~0ull;
         w.gt_head[i] = 0xffffffffu;
     }
-    if (i < CTR_COUNT) w.counters[ctr(i)] = 0;
+    if (i < CTR_COUNT && i != CTR_STICKY) {
+        // The previous step's overflow and barrier abort stay visible in the
+        // sticky word; the thread that clears a flag folds it in.
+        uint32_t v = w.counters[ctr(i)];
+        if (v && i == CTR_OVERFLOW) atomicOr(&w.counters[ctr(CTR_STICKY)], STICKY_OVERFLOW);
+        if (v && i == CTR_GBAR_ABORT) atomicOr(&w.counters[ctr(CTR_STICKY)], STICKY_GBAR_ABORT);
+        w.counters[ctr(i)] = 0;
+    }
     if (i < w.nworlds) w.wlevels[i] = 0;
     if (i < 16 * w.nworlds) w.wlev_count[i] = 0;
 }
@@ -325,6 +332,30 @@ __global__ void __launch_bounds__(256) k_gather_states(DeviceWorld w, const uint
     out[4 * i + 1] = w.quat[b];
     out[4 * i + 2] = w.linv[b];
     out[4 * i + 3] = w.angv[b];
+}
+
+__global__ void __launch_bounds__(256) k_set_states(DeviceWorld w, const StateRecord* recs, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const StateRecord& r = recs[i];
+    uint32_t b = r.target;
+    if (r.fields & BEPU_STATE_POSITION) w.pos[b] = r.pos;
+    if (r.fields & BEPU_STATE_ORIENTATION) w.quat[b] = r.quat;
+    if (r.fields & BEPU_STATE_LINEAR_VELOCITY) w.linv[b] = r.linv;
+    if (r.fields & BEPU_STATE_ANGULAR_VELOCITY) w.angv[b] = r.angv;
+}
+
+__global__ void __launch_bounds__(256) k_remove_bodies(DeviceWorld w, const uint32_t* ids, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w.bflags[ids[i]] = 0u;
+}
+
+void launch_set_states(const DeviceWorld& w, const StateRecord* recs, uint32_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_set_states, dim3((n + 255) / 256), dim3(256), 0, st, w, recs, n);
+}
+
+void launch_remove_bodies(const DeviceWorld& w, const uint32_t* ids, uint32_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_remove_bodies, dim3((n + 255) / 256), dim3(256), 0, st, w, ids, n);
 }
 
 void launch_scatter_bodies(const DeviceWorld& w, const BodyRecord* recs, uint32_t n, hipStream_t st) {

@@ -18,6 +18,7 @@
 #include "bodies_host.h"
 #include "hull.h"
 #include "solver_rows.h"
+#include "world_host.h"
 
 using namespace bepu;
 
@@ -25,8 +26,6 @@ namespace {
 
 enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_CONVEX, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE, ST_COUNT };
 const char* kStageNames = "body_pre,broadphase,narrowphase,convex,islands,island_solve,global_solve,integrate";
-
-using HostBody = BodyRecord;
 
 template <typename T>
 bepu_status dalloc(T** p, size_t n) {
@@ -44,53 +43,31 @@ uint32_t next_pow2(uint64_t v) {
 
 }  // namespace
 
-struct bepu_world_s {
-    DeviceWorld d{};
-    hipStream_t stream = nullptr;
-    // The closed-form pairs are tested on a second stream while the general
-    // convex pairs run on the first (both kernels are bound by memory latency
-    // and leave most of the device idle alone); fork and join are events.
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    // Launch configuration (bepu_tuning); small_island_max lives in d.
-    uint32_t np_block = 128;
-    bool convex_lds = false;
-    bool convex_side = true;
-    std::vector<HostBody> staged;   // bodies not yet uploaded
-    uint32_t uploaded = 0;
-    void* sort_temp = nullptr;
-    size_t sort_bytes = 0;
-    void* scan_temp = nullptr;
-    size_t scan_bytes = 0;
-    bool timing = false;
-    std::vector<hipEvent_t> ev;     // (ST_COUNT + 1) per timed step of the last bepu_step call
-    uint32_t timed_steps = 0;
-    uint32_t steps = 0;
-    std::vector<void*> allocations;
-    // Sub-worlds: world k holds bodies [world_first[k], world_first[k] + world_count[k]).
-    std::vector<uint32_t> world_first{0}, world_count{0};
-    std::vector<uint8_t> world_live{1};
-    uint32_t cur_world = 0;                  // world the add calls append to
-    std::vector<uint32_t> free_world_ids;
-    std::map<uint32_t, uint32_t> free_ranges;  // body ranges of removed worlds: first -> count
-    BodyRecord* stage = nullptr;            // device staging of body records
-    uint32_t stage_cap = 0;
-    uint32_t* id_stage = nullptr;           // device staging of body ids for readbacks
-    float4* state_stage = nullptr;          // 4 float4 per body read back
-    uint32_t rb_cap = 0;
-    // Hull shapes: vertices back to back in hull_host (mirrored in d.hull_verts).
-    struct HullShape {
-        uint32_t first, count;
-        m3 volume_distribution;
-    };
-    std::vector<HullShape> hull_shapes;
-    bool convex_bodies = false;             // a capsule or hull was ever added: run k_convex
-    bool box_bodies = false, sphere_bodies = false;  // a box / a sphere was ever added
-    std::vector<float4> hull_host;
-    uint32_t hull_dev_cap = 0;
-};
 
 static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
+
+// Overflow (pair, hash, colour or list capacity) and grid barrier aborts of
+// any step since the last bepu_clear_error, as a status.
+bepu_status bepu::sticky_status(uint32_t bits) {
+    if (bits & STICKY_GBAR_ABORT) return BEPU_E_DEVICE;
+    if (bits & STICKY_OVERFLOW) return BEPU_E_CAPACITY;
+    return BEPU_OK;
+}
+
+// Waits for the world's stream and folds the device's error flags (the
+// sticky word and the last step's own flags) into w->sticky.  Allocates nothing.
+bepu_status bepu::poll_errors(bepu_world w) {
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    uint32_t e[3] = {0, 0, 0};
+    const uint32_t idx[3] = {CTR_STICKY, CTR_OVERFLOW, CTR_GBAR_ABORT};
+    for (int i = 0; i < 3; ++i)
+        if (hipMemcpyAsync(&e[i], w->d.counters + ctr(idx[i]), sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream) !=
+            hipSuccess)
+            return BEPU_E_DEVICE;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    w->sticky |= e[0] | (e[1] ? (uint32_t)STICKY_OVERFLOW : 0u) | (e[2] ? (uint32_t)STICKY_GBAR_ABORT : 0u);
+    return sticky_status(w->sticky);
+}
 
 extern "C" {
 
@@ -149,6 +126,7 @@ bepu_status bepu_world_destroy(bepu_world w) {
     if (w->stage) (void)hipFree(w->stage);
     if (w->id_stage) (void)hipFree(w->id_stage);
     if (w->state_stage) (void)hipFree(w->state_stage);
+    if (w->sstage) (void)hipFree(w->sstage);
     if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
@@ -161,21 +139,36 @@ bepu_status bepu_world_destroy(bepu_world w) {
     return BEPU_OK;
 }
 
+static bool settings_valid(const bepu_settings& s) {
+    return s.min_iteration_count <= 6 && s.iteration_limit >= 0 && s.dt > 0 && std::isfinite(s.dt) &&
+           std::isfinite(s.gravity[0]) && std::isfinite(s.gravity[1]) && std::isfinite(s.gravity[2]);
+}
+
 bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_bodies, bepu_world* out) {
-    if (!out || capacity_bodies == 0) return BEPU_E_INVALID_ARG;
+    return bepu_world_create_ex(settings, capacity_bodies, 0, out);
+}
+
+bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacity_bodies, uint32_t capacity_pairs,
+                                 bepu_world* out) {
+    if (!out || capacity_bodies == 0 || capacity_bodies > (1u << 28) || capacity_pairs > (1u << 29))
+        return BEPU_E_INVALID_ARG;
     *out = nullptr;
     if (!bepu_device_available()) return BEPU_E_DEVICE;
     bepu_world w = new bepu_world_s();
     if (settings) w->d.s = *settings;
     else bepu_settings_default(&w->d.s);
-    if (w->d.s.min_iteration_count > 6 || w->d.s.iteration_limit < 0 || !(w->d.s.dt > 0)) {
+    if (!settings_valid(w->d.s)) {
         delete w;
         return BEPU_E_INVALID_ARG;
+    }
+    if (hipGetDevice(&w->device) != hipSuccess) {
+        delete w;
+        return BEPU_E_DEVICE;
     }
     DeviceWorld& d = w->d;
     uint32_t n = capacity_bodies;
     d.body_cap = n;
-    d.pair_cap = std::max<uint32_t>(1024, 4 * n);
+    d.pair_cap = capacity_pairs ? std::max<uint32_t>(16, capacity_pairs) : std::max<uint32_t>(1024, 4 * n);
     d.small_island_max = 64;
     d.hash_cap = next_pow2(2ull * d.pair_cap);
     bepu_status st = BEPU_OK;
@@ -187,9 +180,11 @@ bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_b
     A(&d.pos, n); A(&d.quat, n); A(&d.linv, n); A(&d.angv, n);
     A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
     A(&d.shape, n); A(&d.bmin, n); A(&d.bmax, n); A(&d.stab, n); A(&d.damp, n); A(&d.mat, n); A(&d.bflags, n);
-    // World ids stay below the number of worlds alive at once (ids are reused), at most n + 1.
-    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, n + 1); A(&d.born, n);
-    A(&d.wlev_count, 16 * ((size_t)n + 1)); A(&d.wlev_list, 16 * kLevelList * ((size_t)n + 1));
+    // World ids are reused, so they stay below the number of worlds alive at
+    // once: at most n worlds with bodies, world 0 and the (possibly empty)
+    // world the add calls append to, n + 2 (new_world_id enforces it).
+    A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, (size_t)n + 2); A(&d.born, n);
+    A(&d.wlev_count, 16 * ((size_t)n + 2)); A(&d.wlev_list, 16 * kLevelList * ((size_t)n + 2));
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
     A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
@@ -308,6 +303,8 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 sha
     b.wbase = w->world_first[w->cur_world];
     w->staged.push_back(b);
     w->world_count[w->cur_world]++;
+    if (w->absent.size() < (size_t)n + 1) w->absent.resize((size_t)n + 1, 0);
+    w->absent[n] = 0;
     if (id) *id = n;
     return BEPU_OK;
 }
@@ -356,7 +353,8 @@ static uint32_t new_world_id(bepu_world w) {
         w->free_world_ids.pop_back();
         return id;
     }
-    if (w->world_first.size() >= (1u << 20)) return ~0u;
+    // The per-world device arrays hold body_cap + 2 worlds; the grid key holds 20 bits.
+    if (w->world_first.size() >= std::min<uint64_t>(1u << 20, (uint64_t)w->d.body_cap + 2)) return ~0u;
     w->world_first.push_back(0);
     w->world_count.push_back(0);
     w->world_live.push_back(0);
@@ -558,6 +556,8 @@ bepu_status bepu_insert_worlds(bepu_world w, const bepu_body_desc* bodies, const
         w->world_first[id] = first;
         w->world_count[id] = n;
         w->world_live[id] = 1;
+        if (w->absent.size() < (size_t)first + n) w->absent.resize((size_t)first + n, 0);
+        std::fill(w->absent.begin() + first, w->absent.begin() + first + n, 0);
         if (append) {
             w->uploaded += n;
             w->cur_world = id;  // later add calls open a new sub-world after this one
@@ -607,6 +607,10 @@ bepu_status bepu_remove_world(bepu_world w, uint32_t world_id) {
         }
         w->free_ranges[first] = n;
     }
+    if (n) std::fill(w->absent.begin() + first, w->absent.begin() + first + n, 1);
+    // Free the id first, so that a fresh sub-world for the appends below takes
+    // it back: the number of ids never exceeds the worlds alive at once.
+    w->free_world_ids.push_back(world_id);
     if (world_id == w->cur_world) {
         // Appends continue in a fresh sub-world.
         uint32_t id = new_world_id(w);
@@ -615,13 +619,22 @@ bepu_status bepu_remove_world(bepu_world w, uint32_t world_id) {
         w->world_live[id] = 1;
         w->cur_world = id;
     }
-    w->free_world_ids.push_back(world_id);
     return BEPU_OK;
 }
 
 bepu_status bepu_synchronize(bepu_world w) {
     if (!w) return BEPU_E_INVALID_ARG;
-    return check(hipStreamSynchronize(w->stream));
+    return poll_errors(w);
+}
+
+bepu_status bepu_clear_error(bepu_world w) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    for (uint32_t i : {(uint32_t)CTR_STICKY, (uint32_t)CTR_OVERFLOW, (uint32_t)CTR_GBAR_ABORT})
+        launch_fill_u32(w->d.counters + ctr(i), 0u, 1, w->stream);
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    w->sticky = 0;
+    return BEPU_OK;
 }
 
 bepu_status bepu_get_motion_states_ranges(bepu_world w, const uint32_t* firsts, const uint32_t* counts,
@@ -654,7 +667,8 @@ bepu_status bepu_get_motion_states_ranges(bepu_world w, const uint32_t* firsts, 
     if (hipMemcpy(w->id_stage, ids.data(), (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
         return BEPU_E_DEVICE;
     launch_gather_states(w->d, w->id_stage, k, w->state_stage, w->stream);
-    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    bepu_status err = poll_errors(w);
+    if (err == BEPU_E_DEVICE && !(w->sticky & STICKY_GBAR_ABORT)) return err;
     std::vector<float4> st((size_t)k * 4);
     if (hipMemcpy(st.data(), w->state_stage, st.size() * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
         return BEPU_E_DEVICE;
@@ -666,7 +680,7 @@ bepu_status bepu_get_motion_states_ranges(bepu_world w, const uint32_t* firsts, 
         m.linear_velocity[0] = v.x; m.linear_velocity[1] = v.y; m.linear_velocity[2] = v.z;
         m.angular_velocity[0] = a.x; m.angular_velocity[1] = a.y; m.angular_velocity[2] = a.z;
     }
-    return BEPU_OK;
+    return err;
 }
 
 bepu_status bepu_get_motion_states_range(bepu_world w, uint32_t first, uint32_t count, bepu_motion_state* out) {
@@ -792,6 +806,9 @@ bepu_status bepu_set_broadphase(bepu_world w, int32_t kind) {
 
 bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (!w || n_steps < 0) return BEPU_E_INVALID_ARG;
+    // A world whose steps overflowed a capacity or aborted a grid barrier
+    // takes no more steps until the caller acknowledges it (bepu_clear_error).
+    if (w->sticky) return sticky_status(w->sticky);
     bepu_status st = upload(w);
     if (st != BEPU_OK) return st;
     DeviceWorld& d = w->d;
@@ -884,13 +901,16 @@ static bepu_status read_counters(bepu_world w, uint32_t* c) {
     if (hipMemcpy(all.data(), w->d.counters, all.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
         return BEPU_E_DEVICE;
     for (uint32_t i = 0; i < CTR_COUNT; ++i) c[i] = all[ctr(i)];
+    w->sticky |= c[CTR_STICKY] | (c[CTR_OVERFLOW] ? (uint32_t)STICKY_OVERFLOW : 0u) |
+                 (c[CTR_GBAR_ABORT] ? (uint32_t)STICKY_GBAR_ABORT : 0u);
     return BEPU_OK;
 }
 
 bepu_status bepu_get_motion_states(bepu_world w, bepu_motion_state* out, uint32_t cap, uint32_t* count) {
     if (!w) return BEPU_E_INVALID_ARG;
     if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
-    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    bepu_status err = poll_errors(w);
+    if (err == BEPU_E_DEVICE && !(w->sticky & STICKY_GBAR_ABORT)) return err;
     uint32_t n = w->d.nbodies;
     if (count) *count = n;
     if (!out || cap == 0) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
@@ -908,14 +928,15 @@ bepu_status bepu_get_motion_states(bepu_world w, bepu_motion_state* out, uint32_
         m.linear_velocity[0] = v[i].x; m.linear_velocity[1] = v[i].y; m.linear_velocity[2] = v[i].z;
         m.angular_velocity[0] = a[i].x; m.angular_velocity[1] = a[i].y; m.angular_velocity[2] = a[i].z;
     }
-    return n <= cap ? BEPU_OK : BEPU_E_CAPACITY;
+    if (n > cap) return BEPU_E_CAPACITY;
+    return err;
 }
 
 bepu_status bepu_get_pairs(bepu_world w, uint64_t* keys, uint32_t cap, uint32_t* count) {
     if (!w) return BEPU_E_INVALID_ARG;
     uint32_t c[CTR_COUNT];
     if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
-    if (c[CTR_OVERFLOW]) return BEPU_E_CAPACITY;
+    if (w->sticky) return sticky_status(w->sticky);
     uint32_t n = std::min(c[CTR_PAIRS], w->d.pair_cap);
     if (count) *count = n;
     if (!keys || cap < n) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
@@ -946,7 +967,7 @@ bepu_status bepu_get_manifolds(bepu_world w, bepu_manifold* out, uint32_t cap, u
     std::sort(live.begin(), live.end(), [](const PairState* a, const PairState* b) { return a->key < b->key; });
     uint32_t n = (uint32_t)live.size();
     if (count) *count = n;
-    if (!out || cap < n) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
+    if (!out || cap < n) return n == 0 ? sticky_status(w->sticky) : BEPU_E_CAPACITY;
     for (uint32_t i = 0; i < n; ++i) {
         const PairState& s = *live[i];
         bepu_manifold& m = out[i];
@@ -966,7 +987,7 @@ bepu_status bepu_get_manifolds(bepu_world w, bepu_manifold* out, uint32_t cap, u
             std::memcpy(&o.id, &idf, 4);
         }
     }
-    return BEPU_OK;
+    return sticky_status(w->sticky);
 }
 
 bepu_status bepu_get_impulses(bepu_world w, bepu_impulses* out, uint32_t cap, uint32_t* count) {
@@ -998,9 +1019,9 @@ bepu_status bepu_get_impulses(bepu_world w, bepu_impulses* out, uint32_t cap, ui
     });
     uint32_t m = (uint32_t)v.size();
     if (count) *count = m;
-    if (!out || cap < m) return m == 0 ? BEPU_OK : BEPU_E_CAPACITY;
+    if (!out || cap < m) return m == 0 ? sticky_status(w->sticky) : BEPU_E_CAPACITY;
     std::copy(v.begin(), v.end(), out);
-    return BEPU_OK;
+    return sticky_status(w->sticky);
 }
 
 bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
@@ -1025,8 +1046,112 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     out->convex_pairs[0] = any ? c[CTR_CONVEX] + c[CTR_CONVEX_OTHER0] : 0;
     out->convex_pairs[1] = any ? c[CTR_CONVEX_SHALLOW] + c[CTR_CONVEX_OTHER0 + 1] : 0;
     out->convex_pairs[2] = any ? c[CTR_CONVEX_DEEP] + c[CTR_CONVEX_OTHER0 + 2] : 0;
-    if (c[CTR_GBAR_ABORT]) return BEPU_E_DEVICE;  // the global solver's grid was not resident
-    return c[CTR_OVERFLOW] ? BEPU_E_CAPACITY : BEPU_OK;
+    // Any step since the last bepu_clear_error that overflowed a capacity
+    // (BEPU_E_CAPACITY) or whose global solver grid was not resident (BEPU_E_DEVICE).
+    return sticky_status(w->sticky);
+}
+
+// Space.Remove(entity) (Space.cs:364-482).  The body leaves every stage from
+// the next step on; its pairs end with it.  Its id is not reused.
+bepu_status bepu_remove(bepu_world w, uint32_t id) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    uint32_t total = w->uploaded + (uint32_t)w->staged.size();
+    if (id >= total || w->absent[id]) return BEPU_E_NOT_FOUND;
+    w->absent[id] = 1;
+    if (id >= w->uploaded) {
+        w->staged[id - w->uploaded].flags = 0u;  // uploaded absent
+        return BEPU_OK;
+    }
+    launch_fill_u32(w->d.bflags + id, 0u, 1, w->stream);
+    return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+// Entity state setters for each listed body, the parts `fields` selects:
+// Position, Orientation (normalised as the Orientation setter does,
+// EntityBase.cs:68-84), LinearVelocity, AngularVelocity; all four is the
+// MotionState setter (Entities/EntityBase.cs:218-235).  Takes effect at the
+// next step; pair caches, warm-start impulses and stabilization state stay.
+bepu_status bepu_set_motion_states(bepu_world w, const uint32_t* ids, const bepu_motion_state* states, uint32_t n,
+                                   uint32_t fields) {
+    if (!w || (n && (!ids || !states)) || fields == 0 || (fields & ~BEPU_STATE_ALL)) return BEPU_E_INVALID_ARG;
+    uint32_t total = w->uploaded + (uint32_t)w->staged.size();
+    uint32_t dev = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const bepu_motion_state& m = states[k];
+        if (ids[k] >= total || w->absent[ids[k]]) return BEPU_E_NOT_FOUND;
+        if (((fields & BEPU_STATE_POSITION) && !finite_n(m.position, 3)) ||
+            ((fields & BEPU_STATE_ORIENTATION) && !finite_n(m.orientation, 4)) ||
+            ((fields & BEPU_STATE_LINEAR_VELOCITY) && !finite_n(m.linear_velocity, 3)) ||
+            ((fields & BEPU_STATE_ANGULAR_VELOCITY) && !finite_n(m.angular_velocity, 3)))
+            return BEPU_E_INVALID_ARG;
+        if (ids[k] < w->uploaded) dev++;
+    }
+    std::vector<StateRecord> recs;
+    recs.reserve(dev);
+    for (uint32_t k = 0; k < n; ++k) {
+        const bepu_motion_state& m = states[k];
+        q4 q = qnormalize(q4{m.orientation[0], m.orientation[1], m.orientation[2], m.orientation[3]});
+        float4 pos = make_float4(m.position[0], m.position[1], m.position[2], 0);
+        float4 quat = make_float4(q.x, q.y, q.z, q.w);
+        float4 linv = make_float4(m.linear_velocity[0], m.linear_velocity[1], m.linear_velocity[2], 0);
+        float4 angv = make_float4(m.angular_velocity[0], m.angular_velocity[1], m.angular_velocity[2], 0);
+        if (ids[k] >= w->uploaded) {
+            HostBody& b = w->staged[ids[k] - w->uploaded];
+            if (fields & BEPU_STATE_POSITION) b.pos = pos;
+            if (fields & BEPU_STATE_LINEAR_VELOCITY) b.linv = linv;
+            if (fields & BEPU_STATE_ANGULAR_VELOCITY) b.angv = angv;
+            if (fields & BEPU_STATE_ORIENTATION) {
+                b.quat = quat;
+                m3 li{b.li0.x, b.li0.y, b.li0.z, b.li1.x, b.li1.y, b.li1.z, b.li2.x, b.li2.y, b.li2.z};
+                m3 I = rotate_inertia(mat_from_quat(q), li);
+                b.ii0 = make_float4(I.m11, I.m12, I.m13, b.ii0.w);
+                b.ii1 = make_float4(I.m21, I.m22, I.m23, 0);
+                b.ii2 = make_float4(I.m31, I.m32, I.m33, 0);
+            }
+            continue;
+        }
+        StateRecord r{};
+        r.pos = pos;
+        r.quat = quat;
+        r.linv = linv;
+        r.angv = angv;
+        r.target = ids[k];
+        r.fields = fields;
+        recs.push_back(r);
+    }
+    if (recs.empty()) return BEPU_OK;
+    // Queued steps may still read the staging buffer of an earlier call.
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    uint32_t k = (uint32_t)recs.size();
+    if (k > w->sstage_cap) {
+        uint32_t cap = std::max<uint32_t>(k, 2 * w->sstage_cap);
+        StateRecord* p = nullptr;
+        if (hipMalloc(&p, (size_t)cap * sizeof(StateRecord)) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
+        if (w->sstage) (void)hipFree(w->sstage);
+        w->sstage = p;
+        w->sstage_cap = cap;
+    }
+    if (hipMemcpy(w->sstage, recs.data(), (size_t)k * sizeof(StateRecord), hipMemcpyHostToDevice) != hipSuccess)
+        return BEPU_E_DEVICE;
+    launch_set_states(w->d, w->sstage, k, w->stream);
+    return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+bepu_status bepu_get_settings(bepu_world w, bepu_settings* out) {
+    if (!w || !out) return BEPU_E_INVALID_ARG;
+    *out = w->d.s;
+    return BEPU_OK;
+}
+
+// Solver.IterationLimit (Constraints/Solver.cs:19), ForceUpdater.Gravity
+// (OtherSpaceStages/ForceUpdater.cs:19-29), TimeStepSettings.TimeStepDuration
+// and the other settings, changed between steps: every later step uses them.
+// Values an entity takes at creation (its damping, a hull's margin) keep
+// theirs, as the reference's entities do.
+bepu_status bepu_set_settings(bepu_world w, const bepu_settings* settings) {
+    if (!w || !settings || !settings_valid(*settings)) return BEPU_E_INVALID_ARG;
+    w->d.s = *settings;
+    return BEPU_OK;
 }
 
 }  // extern "C"

@@ -202,8 +202,13 @@ enum {
     CTR_CONVEX_OTHER0 = 145,  // 145..147: general convex pairs with at most one hull, per tester state
     CTR_NP_BOX = 148,     // closed-form pairs listed for k_narrowphase: box-box
     CTR_NP_SINGLE = 149,  // and the others
-    CTR_COUNT = 150
+    CTR_STICKY = 150,     // STICKY_* bits of every step since the last bepu_clear_error (never reset by a step)
+    CTR_COUNT = 151
 };
+
+// Sticky error bits: the start of each step folds the previous step's
+// overflow flag and grid barrier abort into CTR_STICKY before clearing them.
+enum { STICKY_OVERFLOW = 1u, STICKY_GBAR_ABORT = 2u };
 
 // Every counter sits on its own 256-byte line: device atomics on one line
 // serialise (about 10 ns each on MI355X), atomics on different lines do not.
@@ -242,5 +247,16 @@ void launch_step_reset(const DeviceWorld& w, hipStream_t st);
 // Body records -> structure-of-arrays; motion states of listed bodies -> 4 float4 each.
 void launch_scatter_bodies(const DeviceWorld& w, const BodyRecord* recs, uint32_t n, hipStream_t st);
 void launch_gather_states(const DeviceWorld& w, const uint32_t* ids, uint32_t n, float4* out, hipStream_t st);
+// Motion states written between steps (Entity.MotionState setter): one record per listed body.
+struct StateRecord {
+    float4 pos, quat, linv, angv;
+    uint32_t target;
+    uint32_t fields;  // BEPU_STATE_* parts to write
+    uint32_t pad[2];
+};
+static_assert(sizeof(StateRecord) == 80, "StateRecord layout");
+void launch_set_states(const DeviceWorld& w, const StateRecord* recs, uint32_t n, hipStream_t st);
+// Marks bodies absent (Space.Remove): no AABB, no pairs, not dynamic from the next step on.
+void launch_remove_bodies(const DeviceWorld& w, const uint32_t* ids, uint32_t n, hipStream_t st);
 
 }  // namespace bepu

@@ -1,0 +1,73 @@
+// world_host.h -- host-side state of a world handle (bepu_world), shared by
+// the C ABI sources.  Internal: not part of the installed interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "../../include/bepu_gpu.h"
+#include "bodies_host.h"
+#include "world.h"
+
+using HostBody = bepu::BodyRecord;
+
+struct bepu_world_s {
+    bepu::DeviceWorld d{};
+    hipStream_t stream = nullptr;
+    // The closed-form pairs are tested on a second stream while the general
+    // convex pairs run on the first (both kernels are bound by memory latency
+    // and leave most of the device idle alone); fork and join are events.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    // Launch configuration (bepu_tuning); small_island_max lives in d.
+    uint32_t np_block = 128;
+    bool convex_lds = false;
+    bool convex_side = true;
+    std::vector<HostBody> staged;   // bodies not yet uploaded
+    uint32_t uploaded = 0;
+    void* sort_temp = nullptr;
+    size_t sort_bytes = 0;
+    void* scan_temp = nullptr;
+    size_t scan_bytes = 0;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;     // (ST_COUNT + 1) per timed step of the last bepu_step call
+    uint32_t timed_steps = 0;
+    uint32_t steps = 0;
+    std::vector<void*> allocations;
+    // Sub-worlds: world k holds bodies [world_first[k], world_first[k] + world_count[k]).
+    std::vector<uint32_t> world_first{0}, world_count{0};
+    std::vector<uint8_t> world_live{1};
+    uint32_t cur_world = 0;                  // world the add calls append to
+    std::vector<uint32_t> free_world_ids;
+    std::map<uint32_t, uint32_t> free_ranges;  // body ranges of removed worlds: first -> count
+    bepu::BodyRecord* stage = nullptr;      // device staging of body records
+    uint32_t stage_cap = 0;
+    uint32_t* id_stage = nullptr;           // device staging of body ids for readbacks
+    float4* state_stage = nullptr;          // 4 float4 per body read back
+    uint32_t rb_cap = 0;
+    // Hull shapes: vertices back to back in hull_host (mirrored in d.hull_verts).
+    struct HullShape {
+        uint32_t first, count;
+        m3 volume_distribution;
+    };
+    std::vector<HullShape> hull_shapes;
+    bool convex_bodies = false;             // a capsule or hull was ever added: run k_convex
+    bool box_bodies = false, sphere_bodies = false;  // a box / a sphere was ever added
+    std::vector<float4> hull_host;
+    uint32_t hull_dev_cap = 0;
+    int device = 0;                         // the HIP device the world lives on
+    uint32_t sticky = 0;                    // STICKY_* bits seen by the host since the last bepu_clear_error
+    std::vector<uint8_t> absent;            // per body id: removed (bepu_remove, or its world was removed)
+    bepu::StateRecord* sstage = nullptr;    // device staging of motion states written between steps
+    uint32_t sstage_cap = 0;
+};
+
+namespace bepu {
+// The status of the STICKY_* error bits: BEPU_E_DEVICE for a grid barrier
+// abort, BEPU_E_CAPACITY for an overflow, else BEPU_OK.
+bepu_status sticky_status(uint32_t bits);
+// Waits for the world's stream and folds the device's error flags into
+// w->sticky; returns sticky_status of the result.  Allocates nothing.
+bepu_status poll_errors(bepu_world w);
+}  // namespace bepu

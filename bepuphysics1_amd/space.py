@@ -5,18 +5,23 @@ Names and argument meaning follow BEPUphysics v1 (paths relative to the
 reference repository root):
   Space()                      BEPUphysics/Space.cs:199-233
   Space.Add(entity)            Space.cs:241-358  (ArgumentException on a double add -> ValueError)
+  Space.Remove(entity)         Space.cs:364-482  (ArgumentException on a foreign remove -> ValueError)
   Space.Update()               Space.cs:527-531
   Box(position, w, h, l, mass) Entities/Prefabs/Box.cs:33-37 (mass omitted: kinematic)
   Sphere(position, r, mass)    Entities/Prefabs/Sphere.cs:47-51
   Capsule(position, length, r, mass)    Entities/Prefabs/Capsule.cs:124-128
   ConvexHull(position, points, mass)    Entities/Prefabs/ConvexHull.cs:83-87 (body at `position`,
                                         the shape recentred on its volume centroid)
-  Entity.Position/Orientation/LinearVelocity/AngularVelocity, Material, IsAlwaysActive
+  Entity.Position/Orientation/LinearVelocity/AngularVelocity/MotionState (EntityBase.cs:51-235;
+                               after the body is on the device each setter writes its field there)
+  Material, IsAlwaysActive
   Space.Solver.IterationLimit  Constraints/Solver.cs:15
   Space.ForceUpdater.Gravity   OtherSpaceStages/ForceUpdater.cs
   Space.TimeStepSettings.TimeStepDuration  TimeStepSettings.cs:19
 Entities added before the first Update are uploaded to the device in one
 batch; state lives on the device afterwards and entity properties read it back.
+Settings (Solver.IterationLimit, ForceUpdater.Gravity, TimeStepDuration) can
+be changed at any time; later steps use them.
 """
 import ctypes
 
@@ -26,6 +31,11 @@ from . import _lib
 from ._lib import BepuError, check
 
 SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE = 0, 1, 2, 3
+STATE_POSITION, STATE_ORIENTATION, STATE_LINEAR_VELOCITY, STATE_ANGULAR_VELOCITY = 1, 2, 4, 8
+STATE_ALL = 15
+_STATE_FIELD = {"position": (STATE_POSITION, slice(0, 3)), "orientation": (STATE_ORIENTATION, slice(3, 7)),
+                "linear_velocity": (STATE_LINEAR_VELOCITY, slice(7, 10)),
+                "angular_velocity": (STATE_ANGULAR_VELOCITY, slice(10, 13))}
 
 
 def compute_hull(points):
@@ -98,9 +108,14 @@ class Entity:
         return tuple(float(x) for x in st[sl]) if st is not None else self._pending[key]
 
     def _set(self, key, value):
-        if self._space is not None and self._space._uploaded(self._id):
-            raise NotImplementedError("changing the state of an entity already on the device is not supported")
-        self._pending[key] = tuple(float(x) for x in value)
+        value = tuple(float(x) for x in value)
+        if self._space is not None and self._id is not None:
+            # The entity's setter on the world (staged or on the device).
+            mask, sl = _STATE_FIELD[key]
+            row = np.zeros(13, np.float32)
+            row[sl] = value
+            self._space.set_motion_states([self._id], row[None, :], mask)
+        self._pending[key] = value
 
     Position = property(lambda s: s._get("position", slice(0, 3)), lambda s, v: s._set("position", v))
     Orientation = property(lambda s: s._get("orientation", slice(3, 7)), lambda s, v: s._set("orientation", v))
@@ -108,6 +123,20 @@ class Entity:
                               lambda s, v: s._set("linear_velocity", v))
     AngularVelocity = property(lambda s: s._get("angular_velocity", slice(10, 13)),
                                lambda s, v: s._set("angular_velocity", v))
+
+    @property
+    def MotionState(self):
+        """(position, orientation xyzw, linear velocity, angular velocity)."""
+        return self.Position + self.Orientation + self.LinearVelocity + self.AngularVelocity
+
+    @MotionState.setter
+    def MotionState(self, st):
+        st = [float(x) for x in st]
+        assert len(st) == 13
+        if self._space is not None and self._id is not None:
+            self._space.set_motion_states([self._id], np.asarray(st, np.float32)[None, :], STATE_ALL)
+        for key, (_, sl) in _STATE_FIELD.items():
+            self._pending[key] = tuple(st[sl])
 
     def _motion_state(self):
         ms = _lib.MotionState()
@@ -228,10 +257,12 @@ class _TimeStepSettings:
 class Space:
     """A simulation world whose step runs on the GPU."""
 
-    def __init__(self, capacity=1024, settings=None):
+    def __init__(self, capacity=1024, settings=None, pair_capacity=0):
+        """`capacity` bodies; `pair_capacity` broadphase pairs (0: max(1024, 4 * capacity))."""
         self._lib = _lib.load()
         self._settings = settings if settings is not None else _lib.default_settings()
         self._capacity = int(capacity)
+        self._pair_capacity = int(pair_capacity)
         self._h = None
         self._entities = []
         self._n_uploaded = 0
@@ -244,15 +275,21 @@ class Space:
     def _handle(self):
         if self._h is None:
             h = ctypes.c_void_p()
-            check(self._lib.bepu_world_create(ctypes.byref(self._settings), self._capacity, ctypes.byref(h)),
-                  "bepu_world_create")
+            check(self._lib.bepu_world_create_ex(ctypes.byref(self._settings), self._capacity, self._pair_capacity,
+                                                 ctypes.byref(h)), "bepu_world_create")
             self._h = h
         return self._h
 
     def _configure(self, field, value):
-        if self._h is not None:
-            raise NotImplementedError("settings are fixed once the world exists on the device")
         setattr(self._settings, field, value)
+        if self._h is not None:
+            check(self._lib.bepu_set_settings(self._h, ctypes.byref(self._settings)), "set_settings")
+
+    def set_settings(self, settings):
+        """Replaces the whole settings block; later steps use it."""
+        self._settings = settings
+        if self._h is not None:
+            check(self._lib.bepu_set_settings(self._h, ctypes.byref(settings)), "set_settings")
 
     def close(self):
         if self._h is not None:
@@ -285,12 +322,43 @@ class Space:
         self._ms_cache = None
         return entity
 
+    def Remove(self, entity):
+        """Space.Remove(entity): an Entity of this space, or a body id."""
+        if isinstance(entity, Entity):
+            if entity._space is not self:
+                raise ValueError("The object does not belong to this Space; cannot remove it.")
+            bid = entity._id
+        else:
+            bid = int(entity)
+        st = self._lib.bepu_remove(self._handle(), bid)
+        if st == 3:
+            raise ValueError("Body %d does not belong to this Space; cannot remove it." % bid)
+        check(st, "Space.Remove")
+        if isinstance(entity, Entity):
+            entity._space = None
+            entity._id = None
+        self._ms_cache = None
+
+    def set_motion_states(self, ids, states, fields=STATE_ALL):
+        """Entity state setters for the listed bodies ([n, 13] states; `fields`
+        a STATE_* mask selecting the parts written)."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        st = np.ascontiguousarray(states, np.float32).reshape(-1, 13)
+        check(self._lib.bepu_set_motion_states(self._handle(), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                               st.ctypes.data_as(ctypes.POINTER(_lib.MotionState)), len(ids),
+                                               int(fields)), "set_motion_states")
+        self._ms_cache = None
+
+    def clear_error(self):
+        """Acknowledges a sticky overflow / barrier error so stepping resumes."""
+        check(self._lib.bepu_clear_error(self._handle()), "clear_error")
+
     def Update(self, steps=1):
         check(self._lib.bepu_step(self._handle(), int(steps)), "Space.Update")
         self._n_uploaded = len(self._entities)
         self._ms_cache = None
 
-    add, update = Add, Update
+    add, update, remove = Add, Update, Remove
 
     @property
     def Entities(self):

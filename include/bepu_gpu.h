@@ -168,6 +168,14 @@ void bepu_settings_default(bepu_settings* out);
  * the device arrays; adding more bodies than that returns BEPU_E_CAPACITY. */
 bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_bodies,
                               bepu_world* out);
+/* The same with the capacity of broadphase pairs (manifold slots) given:
+ * 0 takes the default max(1024, 4 * capacity_bodies).  Dense stacks and
+ * close-packed piles hold up to about 11 pairs per body; a step that finds
+ * more pairs than the capacity drops the excess and every later readback
+ * returns BEPU_E_CAPACITY (see bepu_clear_error).  No reference counterpart
+ * (the reference's pair lists grow on the managed heap). */
+bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacity_bodies,
+                                 uint32_t capacity_pairs, bepu_world* out);
 bepu_status bepu_world_destroy(bepu_world world);
 
 /* Space.Add(new Box(motionState, width, height, length, mass)) -- Space.cs:241-358,
@@ -257,8 +265,46 @@ bepu_status bepu_remove_worlds(bepu_world world, const uint32_t* world_ids, uint
 bepu_status bepu_get_motion_states_ranges(bepu_world world, const uint32_t* first_bodies,
                                           const uint32_t* counts, uint32_t nranges, bepu_motion_state* out);
 
-/* Waits until every queued step of the world has finished on the device. */
+/* Waits until every queued step of the world has finished on the device.
+ * Returns BEPU_E_CAPACITY when any step since the last bepu_clear_error
+ * overflowed a capacity (pairs, pair table, grid table, colours), and
+ * BEPU_E_DEVICE when the global solver's grid barrier gave up waiting (its
+ * workgroups were not all resident).  The same status comes back from every
+ * readback (whose outputs are still filled), and bepu_step refuses to run
+ * while it is set: the flags are sticky until bepu_clear_error. */
 bepu_status bepu_synchronize(bepu_world world);
+/* Acknowledges and clears the sticky error of the world (no reference
+ * counterpart: the reference's lists grow instead of overflowing). */
+bepu_status bepu_clear_error(bepu_world world);
+
+/* Space.Remove(entity) -- Space.cs:364-482: the body leaves every stage from
+ * the next step on, and its pairs and constraints end with it.  An id that is
+ * unknown or already removed returns BEPU_E_NOT_FOUND (the reference's
+ * ArgumentException, Space.cs:366-367).  Ids are not reused. */
+bepu_status bepu_remove(bepu_world world, uint32_t id);
+/* Entity state setters for each of `n` listed bodies; `fields` selects which
+ * parts of states[i] are written, as the reference's setters do one at a time:
+ * Position (Entities/EntityBase.cs:51-62), Orientation (normalised,
+ * EntityBase.cs:68-84), LinearVelocity (:181-195), AngularVelocity (:125-140);
+ * BEPU_STATE_ALL is the MotionState setter (:218-235).  Takes effect at the
+ * next step; pair caches, warm-start impulses and the stabilization state are
+ * kept.  BEPU_E_NOT_FOUND for a removed or unknown id, BEPU_E_INVALID_ARG for
+ * a non-finite written value or no field (nothing is written then). */
+#define BEPU_STATE_POSITION 1u
+#define BEPU_STATE_ORIENTATION 2u
+#define BEPU_STATE_LINEAR_VELOCITY 4u
+#define BEPU_STATE_ANGULAR_VELOCITY 8u
+#define BEPU_STATE_ALL 15u
+bepu_status bepu_set_motion_states(bepu_world world, const uint32_t* ids, const bepu_motion_state* states,
+                                   uint32_t n, uint32_t fields);
+/* The settings block of a world, readable and writable between steps:
+ * Solver.IterationLimit (Constraints/Solver.cs:19), ForceUpdater.Gravity
+ * (OtherSpaceStages/ForceUpdater.cs:19-29), TimeStepSettings.TimeStepDuration
+ * (TimeStepSettings.cs:19) and the rest.  Every later step uses the new
+ * values; what an entity took at creation (its damping, a hull's margin)
+ * keeps its value, as the reference's entities do. */
+bepu_status bepu_get_settings(bepu_world world, bepu_settings* out);
+bepu_status bepu_set_settings(bepu_world world, const bepu_settings* settings);
 
 /* Space.Update() x n_steps (Space.cs:527-531). */
 bepu_status bepu_step(bepu_world world, int32_t n_steps);

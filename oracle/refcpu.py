@@ -75,6 +75,10 @@ def lib():
         L.refcpu_box_sphere.argtypes = [fp, fp, fp, ctypes.c_float, fp, fp]
         L.refcpu_mix64.restype = ctypes.c_uint64
         L.refcpu_mix64.argtypes = [ctypes.c_uint64]
+        L.refcpu_remove.argtypes = [vp, ctypes.c_uint32]
+        L.refcpu_set_motion_states.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(MotionState),
+                                               ctypes.c_uint32, ctypes.c_uint32]
+        L.refcpu_set_settings.argtypes = [vp, ctypes.POINTER(Settings)]
         L.refcpu_reduce_contacts.argtypes = [fp, fp, fp, fp, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
         L.refcpu_sizeof.restype = ctypes.c_uint32
         L.refcpu_sizeof.argtypes = [ctypes.c_int]
@@ -162,6 +166,23 @@ class World:
 
     def step(self, n=1):
         lib().refcpu_step(self.h, int(n))
+
+    def remove(self, body_id):
+        """Space.Remove(entity); returns the status code."""
+        return lib().refcpu_remove(self.h, int(body_id))
+
+    def set_motion_states(self, ids, states, fields=15):
+        """Entity state setters for each listed body; `states` is [n, 13];
+        `fields` is a BEPU_STATE_* mask (15: the MotionState setter)."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        st = np.ascontiguousarray(states, np.float32).reshape(-1, 13)
+        return lib().refcpu_set_motion_states(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                              st.ctypes.data_as(ctypes.POINTER(MotionState)), len(ids), int(fields))
+
+    def set_settings(self, settings):
+        """Settings changed between steps (iteration limit, gravity, dt, ...)."""
+        self._s = settings
+        return lib().refcpu_set_settings(self.h, ctypes.byref(settings))
 
     def broadphase_only(self):
         lib().refcpu_broadphase_only(self.h)

@@ -60,6 +60,9 @@ struct Body {
     bool allow_stabilization;
     bepu_material mat;
     V3 bmin, bmax;
+    // Space.Remove(entity) (Space.cs:364-482): the body left every stage; its
+    // broadphase entry is gone, so none of its pairs survive.
+    bool removed;
 };
 
 struct ContactObj {
@@ -293,7 +296,7 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
 // SimulationIslandMember.UpdateDeactivationCandidacy, dynamic and active branch
 // (SimulationIslandMember.cs:53-70)
 void candidacy(refcpu_world* w, Body& b) {
-    if (!b.dynamic) return;
+    if (!b.dynamic || b.removed) return;
     float velocity = length_sq(b.v) + length_sq(b.w);
     b.is_slowing = velocity <= b.prev_velocity;
     float lim = w->s.velocity_lower_limit;
@@ -304,7 +307,7 @@ void candidacy(refcpu_world* w, Body& b) {
 
 // Entity.UpdateForForces (EntityBase.cs:911-977); ForceUpdater.gravityDt (ForceUpdater.cs:85-92)
 void force_update(refcpu_world* w, Body& b) {
-    if (!b.dynamic) return;
+    if (!b.dynamic || b.removed) return;
     const bepu_settings& s = w->s;
     V3 gravityDt = mul(V3{s.gravity[0], s.gravity[1], s.gravity[2]}, s.dt);
     b.v = add(gravityDt, b.v);
@@ -335,6 +338,7 @@ void force_update(refcpu_world* w, Body& b) {
 // EntityCollidable.UpdateBoundingBox -> Shape.GetBoundingBox -> ExpandBoundingBox
 // (EntityCollidable.cs:140-148,195-257; BoxShape.cs:149-174; SphereShape.cs:75-86)
 void bounding_box(refcpu_world* w, Body& b) {
+    if (b.removed) return;
     if (b.shape == BEPU_SHAPE_BOX) {
         M3 o = m3_from_quat(b.q);
         V3 right{signf_int(o.m11) * b.half.x, signf_int(o.m21) * b.half.y, signf_int(o.m31) * b.half.z};
@@ -402,6 +406,7 @@ void bounding_box(refcpu_world* w, Body& b) {
 
 // Entity.PreUpdatePosition (EntityBase.cs:1124-1155)
 void integrate(refcpu_world* w, Body& b) {
+    if (b.removed) return;
     float dt = w->s.dt;
     V3 inc = mul(b.w, dt * .5f);
     Q4 m{inc.x, inc.y, inc.z, 0};
@@ -420,6 +425,7 @@ void integrate(refcpu_world* w, Body& b) {
 // (CollisionRules.cs:175-178) and one entry must be active (BroadPhase.cs:97-102).
 
 inline bool intersects(const Body& a, const Body& b) {
+    if (a.removed || b.removed) return false;
     if (b.bmin.x > a.bmax.x || b.bmin.y > a.bmax.y || b.bmin.z > a.bmax.z) return false;
     if (a.bmin.x > b.bmax.x || a.bmin.y > b.bmax.y || a.bmin.z > b.bmax.z) return false;
     return true;
@@ -486,6 +492,7 @@ void broadphase_grid(refcpu_world* w, std::vector<uint64_t>* out) {
     std::vector<float> side;
     side.reserve(n);
     for (const Body& b : w->bodies) {
+        if (b.removed) continue;
         float e = std::max(std::max(b.bmax.x - b.bmin.x, b.bmax.y - b.bmin.y), b.bmax.z - b.bmin.z);
         if (std::isfinite(e)) side.push_back(e);
     }
@@ -500,6 +507,7 @@ void broadphase_grid(refcpu_world* w, std::vector<uint64_t>* out) {
     std::vector<uint32_t> large;
     for (uint32_t i = 0; i < n; ++i) {
         const Body& b = w->bodies[i];
+        if (b.removed) continue;
         bool finite = std::isfinite(b.bmin.x) && std::isfinite(b.bmin.y) && std::isfinite(b.bmin.z) &&
                       std::isfinite(b.bmax.x) && std::isfinite(b.bmax.y) && std::isfinite(b.bmax.z);
         if (!finite) { large.push_back(i); continue; }
@@ -1591,7 +1599,7 @@ int refcpu_add_box(refcpu_world* w, const bepu_motion_state* st, float width, fl
     if (!st || !(width > 0) || !(height > 0) || !(length > 0) || std::isnan(mass) || std::isinf(mass))
         return BEPU_E_INVALID_ARG;
     if (!finite3(st->position, 3) || !finite3(st->orientation, 4)) return BEPU_E_INVALID_ARG;
-    Body b;
+    Body b{};
     std::memset(&b, 0, sizeof(b));
     b.shape = BEPU_SHAPE_BOX;
     b.margin = w->s.default_margin;
@@ -1613,7 +1621,7 @@ int refcpu_add_sphere(refcpu_world* w, const bepu_motion_state* st, float radius
                       const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!st || !(radius > 0) || std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
     if (!finite3(st->position, 3) || !finite3(st->orientation, 4)) return BEPU_E_INVALID_ARG;
-    Body b;
+    Body b{};
     std::memset(&b, 0, sizeof(b));
     b.shape = BEPU_SHAPE_SPHERE;
     b.radius = radius;
@@ -1631,7 +1639,7 @@ int refcpu_add_capsule(refcpu_world* w, const bepu_motion_state* st, float lengt
                        const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!st || !(length >= 0) || !(radius > 0) || std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
     if (!finite3(st->position, 3) || !finite3(st->orientation, 4)) return BEPU_E_INVALID_ARG;
-    Body b;
+    Body b{};
     std::memset(&b, 0, sizeof(b));
     b.shape = BEPU_SHAPE_CAPSULE;
     // CapsuleShape(length, radius) and ComputeDescription (CapsuleShape.cs:40-44, 76-92)
@@ -1657,7 +1665,7 @@ int refcpu_add_convex_hull(refcpu_world* w, const bepu_motion_state* st, const f
         return BEPU_E_INVALID_ARG;
     if (!finite3(st->position, 3) || !finite3(st->orientation, 4) || !finite3(verts, 3 * nverts))
         return BEPU_E_INVALID_ARG;
-    Body b;
+    Body b{};
     std::memset(&b, 0, sizeof(b));
     b.shape = BEPU_SHAPE_CONVEX_HULL;
     b.margin = w->s.default_margin;
@@ -1781,6 +1789,57 @@ int refcpu_get_stats(refcpu_world* w, bepu_step_stats* s) {
     s->colors = (uint32_t)w->ncolors;
     s->steps = w->steps;
     s->max_penetration = w->max_pen;
+    return BEPU_OK;
+}
+
+// Space.Remove(entity) (Space.cs:364-482).  An unknown or already removed id
+// is the reference's ArgumentException ("not a member of this space").
+int refcpu_remove(refcpu_world* w, uint32_t id) {
+    if (id >= w->bodies.size() || w->bodies[id].removed) return BEPU_E_NOT_FOUND;
+    w->bodies[id].removed = true;
+    return BEPU_OK;
+}
+
+// Entity state setters, for each listed body the parts `fields` selects:
+// Position (EntityBase.cs:51-62), Orientation (normalised; orientation matrix
+// and world inertia refreshed, EntityBase.cs:68-84), LinearVelocity
+// (:181-195), AngularVelocity (:125-140); all four: MotionState (:218-235).
+int refcpu_set_motion_states(refcpu_world* w, const uint32_t* ids, const bepu_motion_state* st, uint32_t n,
+                             uint32_t fields) {
+    if (fields == 0 || (fields & ~BEPU_STATE_ALL)) return BEPU_E_INVALID_ARG;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
+        const bepu_motion_state& m = st[k];
+        if (((fields & BEPU_STATE_POSITION) && !finite3(m.position, 3)) ||
+            ((fields & BEPU_STATE_ORIENTATION) && !finite3(m.orientation, 4)) ||
+            ((fields & BEPU_STATE_LINEAR_VELOCITY) && !finite3(m.linear_velocity, 3)) ||
+            ((fields & BEPU_STATE_ANGULAR_VELOCITY) && !finite3(m.angular_velocity, 3)))
+            return BEPU_E_INVALID_ARG;
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        Body& b = w->bodies[ids[k]];
+        const bepu_motion_state& m = st[k];
+        if (fields & BEPU_STATE_POSITION) b.pos = V3{m.position[0], m.position[1], m.position[2]};
+        if (fields & BEPU_STATE_ORIENTATION) {
+            b.q = qnormalize(Q4{m.orientation[0], m.orientation[1], m.orientation[2], m.orientation[3]});
+            b.R = m3_from_quat(b.q);
+            b.inertia_inv = m3_mul(m3_mul_transposed(b.R, b.local_inertia_inv), b.R);
+        }
+        if (fields & BEPU_STATE_LINEAR_VELOCITY)
+            b.v = V3{m.linear_velocity[0], m.linear_velocity[1], m.linear_velocity[2]};
+        if (fields & BEPU_STATE_ANGULAR_VELOCITY)
+            b.w = V3{m.angular_velocity[0], m.angular_velocity[1], m.angular_velocity[2]};
+    }
+    return BEPU_OK;
+}
+
+// Solver.IterationLimit, ForceUpdater.Gravity, TimeStepSettings.TimeStepDuration
+// and the other settings changed between steps (Constraints/Solver.cs:19,
+// OtherSpaceStages/ForceUpdater.cs:19-29, TimeStepSettings.cs:19).  Per-entity
+// values taken at creation (damping) keep their values, as the reference's do.
+int refcpu_set_settings(refcpu_world* w, const bepu_settings* s) {
+    if (!s || s->min_iteration_count > 6 || s->iteration_limit < 0 || !(s->dt > 0)) return BEPU_E_INVALID_ARG;
+    w->s = *s;
     return BEPU_OK;
 }
 

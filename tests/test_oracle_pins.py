@@ -367,3 +367,95 @@ def test_multithreaded_oracle_is_bit_identical(oracle, threads):
     ia = {int(r.key): bytes(r) for r in a.impulses()}
     ib = {int(r.key): bytes(r) for r in b.impulses()}
     assert ia == ib
+
+
+# ---------------------------------------------------------------- host API between steps
+
+def test_removed_body_leaves_every_stage(oracle):
+    """Space.Remove: the body has no pairs from the next step on (the pair set
+    equals brute force over the other bodies) and no longer moves; a second
+    removal is a foreign remove (not found)."""
+    sc = scenes.random_pile(60, seed=9)
+    w = oracle.World()
+    w.add_scene(sc)
+    w.step(20)
+    gone = [4, 11, 30]
+    for b in gone:
+        assert w.remove(b) == 0
+    assert w.remove(11) == 3
+    assert w.remove(len(sc)) == 3
+    frozen = w.motion_states()[gone].copy()
+    w.step(5)
+    keys = w.pairs()
+    ends = set((keys >> np.uint64(32)).tolist()) | set((keys & np.uint64(0xffffffff)).tolist())
+    assert not ends & set(gone)
+    assert np.array_equal(w.motion_states()[gone], frozen)
+    # the rest: brute force over the AABBs of the bodies still present
+    ab = w.aabbs()
+    keep = np.ones(len(sc), bool)
+    keep[gone] = False
+    lo, hi = ab[:, :3], ab[:, 3:]
+    ov = np.all((lo[:, None, :] <= hi[None, :, :]) & (lo[None, :, :] <= hi[:, None, :]), axis=2)
+    dyn = sc["mass"] > 0
+    ov &= (dyn[:, None] | dyn[None, :]) & keep[:, None] & keep[None, :]
+    i, j = np.nonzero(np.triu(ov, 1))
+    assert np.array_equal(np.sort((i.astype(np.uint64) << np.uint64(32)) | j.astype(np.uint64)), keys)
+
+
+def test_single_field_setters_touch_only_their_field(oracle):
+    """Entity.Position (1), Orientation (2), LinearVelocity (4) and
+    AngularVelocity (8) setters write their own part only; the orientation is
+    normalised as the reference's setter does (a unit quaternion stays put)."""
+    s = oracle.default_settings()
+    s.gravity[:] = [0, 0, 0]
+    w = oracle.World(s)
+    st0 = np.zeros(13, np.float32)
+    st0[6] = 1.0
+    st0[0:3] = [1, 2, 3]
+    w.add_body(0, (1, 1, 1), 1.0, st0, (1, 0.8, 0))
+    new = np.arange(13, dtype=np.float32)[None, :] + 1
+    for mask, sl in ((1, slice(0, 3)), (2, slice(3, 7)), (4, slice(7, 10)), (8, slice(10, 13))):
+        before = w.motion_states()[0].copy()
+        assert w.set_motion_states([0], new, mask) == 0
+        after = w.motion_states()[0]
+        others = np.ones(13, bool)
+        others[sl] = False
+        assert np.array_equal(after[others], before[others])
+        if mask == 2:
+            q = new[0, 3:7] / np.linalg.norm(new[0, 3:7])
+            assert np.allclose(after[sl], q, atol=1e-7)
+        else:
+            assert np.array_equal(after[sl], new[0, sl])
+    assert w.set_motion_states([0], new, 0) == 1
+    bad = new.copy()
+    bad[0, 8] = np.nan
+    assert w.set_motion_states([0], bad, 4) == 1
+    assert w.set_motion_states([0], bad, 1) == 0  # the NaN is not in the written part
+
+
+def test_settings_change_takes_effect_next_step(oracle):
+    """ForceUpdater.Gravity and the time step changed between steps: a free
+    body's velocity follows v += g dt with the new values from the next step."""
+    s = oracle.default_settings()
+    s.linear_damping = s.angular_damping = 0.0
+    s.use_stabilization = 0
+    w = oracle.World(s)
+    st0 = np.zeros(13, np.float32)
+    st0[6] = 1.0
+    w.add_body(0, (1, 1, 1), 1.0, st0, (1, 0.8, 0))
+    w.step(3)
+    v3 = w.motion_states()[0][8]
+    assert v3 == pytest.approx(3 * -9.81 / 60, rel=1e-6)
+    s2 = oracle.default_settings()
+    s2.linear_damping = s2.angular_damping = 0.0
+    s2.use_stabilization = 0
+    s2.gravity[:] = [2.0, 0.0, 0.0]
+    s2.dt = 0.01
+    assert w.set_settings(s2) == 0
+    w.step(4)
+    ms = w.motion_states()[0]
+    assert ms[7] == pytest.approx(4 * 2.0 * 0.01, rel=1e-6)
+    assert ms[8] == pytest.approx(v3, rel=1e-6)
+    bad = oracle.default_settings()
+    bad.dt = 0.0
+    assert w.set_settings(bad) == 1
