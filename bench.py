@@ -178,6 +178,8 @@ def counts_of(st, iters):
 
 
 def latency(sp, steps):
+    if steps <= 0:
+        return None
     lat = []
     for _ in range(steps):
         t = time.perf_counter()
@@ -554,20 +556,21 @@ def _csv_rows(path):
 
 
 def _window_index(rows, steps, warmup):
-    """Per launch (rows in dispatch order): whether it belongs to a timed
-    step.  A step starts at its k_step_reset launch; the profiled command runs
-    the batch for warmup + steps steps, then its replay for as many (run with
-    --latency-steps 0 --autotune-steps 0 --plain-sample 0 --no-parity
-    --no-cpu-baseline --settled-steps 0), so the run's step j of each run is
-    timed when j > warmup."""
+    """Per launch (rows in dispatch order): whether it belongs to a step the
+    stage events time.  A step starts at its k_step_reset launch.  The
+    profiled command (--legs batched --latency-steps 0 --autotune-steps 0
+    --plain-sample 0 --no-parity --no-cpu-baseline --settled-steps 0) runs the
+    batch for warmup + steps steps, then its replay for as many; the events
+    time the replay's steps warmup + 1 .. warmup + steps."""
     run_len = warmup + steps
+    total = sum(1 for r in rows if "k_step_reset" in r["Kernel_Name"])
+    first = total - run_len + warmup  # steps before the replay's timed ones
     g = 0
     out = []
     for r in rows:
         if "k_step_reset" in r["Kernel_Name"]:
             g += 1
-        j = (g - 1) % run_len + 1 if g else 0
-        out.append(g > 0 and j > warmup)
+        out.append(g > first)
     return out
 
 
@@ -599,18 +602,21 @@ def summarize_trace(directory, steps, warmup, fetch_scale, write_scale):
         disp = [by_dispatch[i] for i in sorted(by_dispatch)]
         win = _window_index(disp, steps, warmup)
         for d, w in zip(disp, win):
-            if not w:
-                continue
             k = out["kernels"].setdefault(d["Kernel_Name"], {"calls": 0, "us": 0.0, "win_calls": 0, "win_us": 0.0})
-            for name, scale in (("FETCH_SIZE", fetch_scale), ("WRITE_SIZE", write_scale)):
+            for name in ("FETCH_SIZE", "WRITE_SIZE"):
                 if name in d:
-                    k.setdefault(name, []).append(d[name] * 1024.0 * scale)
+                    k.setdefault(name + "_kib_all", []).append(d[name])
+                    if w:
+                        k.setdefault(name + "_kib_win", []).append(d[name])
     for name, k in out["kernels"].items():
         k["avg_us"] = k["us"] / k["calls"] if k["calls"] else None
         k["win_avg_us"] = k["win_us"] / k["win_calls"] if k["win_calls"] else None
-        for c in ("FETCH_SIZE", "WRITE_SIZE"):
-            if c in k:
-                k[c + "_bytes_per_launch"] = float(np.mean(k.pop(c)))
+        for c, scale in (("FETCH_SIZE", fetch_scale), ("WRITE_SIZE", write_scale)):
+            for part in ("all", "win"):
+                vals = k.pop("%s_kib_%s" % (c, part), None)
+                if vals:
+                    k["%s_kib_%s" % (c, part)] = float(np.mean(vals))
+                    k["%s_bytes_%s" % (c, part)] = float(np.mean(vals)) * 1024.0 * scale
     return out
 
 

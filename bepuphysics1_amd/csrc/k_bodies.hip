@@ -28,7 +28,7 @@ __device__ __forceinline__ uint32_t f2ord(float f) {
 
 // One pass per body, in the order Space.DoTimeStep runs the stages
 // (Space.cs:505-508): SimulationIslandMember.UpdateDeactivationCandidacy
-// (SimulationIslandMember.cs:53-70), Entity.UpdateForForces
+// (SimulationIslandMember.cs:53-95), Entity.UpdateForForces
 // (EntityBase.cs:911-977), EntityCollidable.UpdateBoundingBox
 // (EntityCollidable.cs:140-148,195-257 with BoxShape.cs:149-174 /
 // SphereShape.cs:75-86).  Also folds the AABB centres into the scene bounds
@@ -57,12 +57,28 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
         if (dyn) {
             float4 st = w.stab[i];
             float lim = s.velocity_lower_limit;
-            // candidacy
+            // candidacy; clearing the candidate flag also zeroes the time
+            // below the limit (the IsDeactivationCandidate setter), which an
+            // always-active body does every step
             float velocity = lensq3(v) + lensq3(av);
             float slowing = velocity <= st.x ? 1.0f : 0.0f;
             float tb = st.y;
             if (velocity < lim * lim) tb += s.dt;
             else tb = 0;
+            const uint32_t bf = w.bflags[i];
+            uint32_t cand = bf & BF_CANDIDATE;
+            if (!(bf & BF_ALWAYS_ACTIVE)) {
+                if (!cand) {
+                    if (tb > s.low_velocity_time_minimum && slowing != 0.0f) cand = BF_CANDIDATE;
+                } else if (tb <= s.low_velocity_time_minimum) {
+                    cand = 0;
+                    tb = 0;
+                }
+            } else {
+                cand = 0;
+                tb = 0;
+            }
+            if (cand != (bf & BF_CANDIDATE)) w.bflags[i] = (bf & ~BF_CANDIDATE) | cand;
             // force update: gravity, stabilization boost, damping, world inertia
             f3 gdt = f3{s.gravity[0], s.gravity[1], s.gravity[2]} * s.dt;
             v = gdt + v;

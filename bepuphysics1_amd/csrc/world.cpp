@@ -286,7 +286,8 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
     const bool hull = shape.w == (float)BEPU_SHAPE_CONVEX_HULL;
     const bool general = hull || shape.w == (float)BEPU_SHAPE_CAPSULE;
     b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT | (general ? BF_GENERAL : 0u) | (hull ? BF_HULL : 0u) |
-              (shape.w == (float)BEPU_SHAPE_SPHERE ? BF_SPHERE : 0u);
+              (shape.w == (float)BEPU_SHAPE_SPHERE ? BF_SPHERE : 0u) |
+              ((flags & BEPU_FLAG_ALWAYS_ACTIVE) ? BF_ALWAYS_ACTIVE : 0u);
     b.born = w->steps;
     return BEPU_OK;
 }

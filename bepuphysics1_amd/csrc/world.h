@@ -63,7 +63,10 @@ struct ConvexState {
 };
 // BF_GENERAL: a shape whose pairs take the general convex tester (hull or
 // capsule); BF_HULL: a convex hull; BF_SPHERE: a sphere.
-enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u };
+// BF_ALWAYS_ACTIVE: SimulationIslandMember.IsAlwaysActive; BF_CANDIDATE: its
+// IsDeactivationCandidate.
+enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u, BF_ALWAYS_ACTIVE = 32u,
+       BF_CANDIDATE = 64u };
 
 // One body as built on the host (Entity construction), copied to the device
 // in one block and scattered into the structure-of-arrays by k_scatter_bodies.

@@ -58,6 +58,8 @@ struct Body {
     bool is_slowing;
     float time_below;
     bool allow_stabilization;
+    bool always_active;  // SimulationIslandMember.IsAlwaysActive
+    bool candidate;      // SimulationIslandMember.IsDeactivationCandidate
     bepu_material mat;
     V3 bmin, bmax;
     // Space.Remove(entity) (Space.cs:364-482): the body left every stage; its
@@ -287,6 +289,8 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
     b->is_slowing = false;
     b->time_below = 0;
     b->allow_stabilization = (flags & BEPU_FLAG_NO_STABILIZATION) == 0;
+    b->always_active = (flags & BEPU_FLAG_ALWAYS_ACTIVE) != 0;
+    b->candidate = false;
     b->mat = m ? *m : bepu_material{1.0f, 0.8f, 0.0f};
 }
 
@@ -294,7 +298,10 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
 // Body stages
 
 // SimulationIslandMember.UpdateDeactivationCandidacy, dynamic and active branch
-// (SimulationIslandMember.cs:53-70)
+// (SimulationIslandMember.cs:53-95).  Clearing the candidate flag goes
+// through the IsDeactivationCandidate setter, which also zeroes the time
+// below the limit (SimulationIslandMember.cs:171-190): an always-active body
+// clears it every step, so its time never accumulates.
 void candidacy(refcpu_world* w, Body& b) {
     if (!b.dynamic || b.removed) return;
     float velocity = length_sq(b.v) + length_sq(b.w);
@@ -302,6 +309,17 @@ void candidacy(refcpu_world* w, Body& b) {
     float lim = w->s.velocity_lower_limit;
     if (velocity < lim * lim) b.time_below += w->s.dt;
     else b.time_below = 0;
+    if (!b.always_active) {
+        if (!b.candidate) {
+            if (b.time_below > w->s.low_velocity_time_minimum && b.is_slowing) b.candidate = true;
+        } else if (b.time_below <= w->s.low_velocity_time_minimum) {
+            b.candidate = false;
+            b.time_below = 0;
+        }
+    } else {
+        b.candidate = false;
+        b.time_below = 0;
+    }
     b.prev_velocity = velocity;
 }
 

@@ -434,14 +434,18 @@ def test_stabilization_after_low_velocity_time(oracle, dt, expect_first):
     """A slow body that speeds up (so it is never 'slowing') gets the boost
     only once its time below the limit exceeds LowVelocityTimeMinimum (1 s),
     strictly: from the first step whose accumulated float32 time is > 1.  With
-    dt = 1/64 the time after 64 steps is exactly 1, which is not enough."""
+    dt = 1/64 the time after 64 steps is exactly 1, which is not enough.  An
+    always-active body never gets there: clearing its candidate flag every
+    step zeroes its time (SimulationIslandMember.cs:93-95,186-189)."""
     # It starts moving (a body at rest counts as slowing: 0 <= 0) and speeds up
     # under a weak gravity, so it is never slowing.
     g, v0 = -0.01, -0.001
     s = _settings(oracle, stabilization=True, gravity=(0, g, 0))
     s.dt = dt
     w = oracle.World(s)
-    w.add_scene(scenes.make([scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 5, 0), vel=(0, v0, 0))]))
+    w.add_scene(scenes.make([scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (0, 5, 0), vel=(0, v0, 0), flags=0),
+                             scenes._row(scenes.SHAPE_BOX, (1, 1, 1), 1.0, (9, 5, 0), vel=(0, v0, 0),
+                                         flags=scenes.FLAG_ALWAYS_ACTIVE)]))
     t = np.float32(0)
     first = None
     for k in range(1, 90):
@@ -451,16 +455,16 @@ def test_stabilization_after_low_velocity_time(oracle, dt, expect_first):
     # without damping and before the boost, v_n = v0 + n g dt (to float rounding)
     for n in range(1, 90):
         w.step(1)
-        vy = float(w.motion_states()[0][8])
+        vy, vy_always = (float(x) for x in w.motion_states()[:, 8])
+        assert vy_always == pytest.approx(v0 + n * g * dt, rel=1e-4), n
         if n < first:
             assert vy == pytest.approx(v0 + n * g * dt, rel=1e-4), n
-        else:
+        elif n == first:
             # boosted: linear damping 0 + boost (1 - 0), E after the gravity update
             pre = v0 + n * g * dt
             boost = 1 - abs(pre) / (2 * 0.26)
             assert vy == pytest.approx(pre * (1 - boost) ** dt, rel=1e-4)
-            break
-    assert n == first == expect_first
+    assert first == expect_first
 
 
 # ---------------------------------------------------------------- box manifold persistence
