@@ -383,6 +383,23 @@ bepu_status bepu_set_tuning(bepu_world world, const bepu_tuning* tuning);
  * (either may be NULL). */
 bepu_status bepu_autotune(bepu_world world, uint32_t trial_steps, bepu_tuning* out, uint32_t* steps_run);
 
+/* Readback of body ranges that allocates nothing, for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation).  No reference counterpart.  The same result as
+ * bepu_get_motion_states_ranges, staged through a fixed device array of the
+ * library (one per GPU, held until the process ends) instead of buffers it
+ * allocates: at most BEPU_NOALLOC_MAX_RANGES ranges and
+ * BEPU_NOALLOC_MAX_BODIES bodies per call (else BEPU_E_CAPACITY), and no
+ * bodies added since the last step (their upload could allocate:
+ * BEPU_E_INVALID_ARG).  Every world on one GPU shares that array, which keeps
+ * the last call's contents; calls from one process are serialised, and each
+ * waits for its copies before it returns. */
+#define BEPU_NOALLOC_MAX_RANGES 16384
+#define BEPU_NOALLOC_MAX_BODIES 262144
+bepu_status bepu_get_motion_states_ranges_noalloc(bepu_world world, const uint32_t* first_bodies,
+                                                  const uint32_t* counts, uint32_t nranges,
+                                                  bepu_motion_state* out);
+
 /* Library-wide information: 1 if a HIP device is usable. */
 int32_t bepu_device_available(void);
 const char* bepu_status_string(bepu_status status);
