@@ -210,7 +210,14 @@ def batched_run(a):
     for w in worlds:
         sp.add_world(w)
     tune = dict(sp.tuning(), steps=0)
-    if a.autotune_steps > 0:
+    forced = {}
+    for kv in filter(None, (a.tuning or "").split(",")):
+        k, v = kv.split("=")
+        forced[k.strip()] = int(v)
+    if forced:
+        sp.set_tuning(**forced)
+        tune = dict(sp.tuning(), steps=0, forced=True)
+    if a.autotune_steps > 0 or (forced and a.settled_steps > 0):
         # The autotuner picks the launch configuration on a copy of the batch
         # stepped into its settled state (most of the timed window is settled
         # piles, whose costs differ from the falling ones), untimed; every
@@ -220,7 +227,12 @@ def batched_run(a):
         for w in worlds:
             tsp.add_world(w)
         tsp.Update(a.autotune_after)
-        tune = tsp.autotune(a.autotune_steps)
+        if forced:
+            tsp.set_tuning(**forced)
+            tsp.Update(a.autotune_steps * 18)
+            tune["steps"] = a.autotune_steps * 18
+        else:
+            tune = tsp.autotune(a.autotune_steps)
         tune["tuned_after_steps"] = a.autotune_after
         # The settled phase, which most of a long run is: the tuned copy is
         # now more than 300 steps in; time a fixed number of its steps.
@@ -631,6 +643,8 @@ def main():
     ap.add_argument("--autotune-after", type=int, default=160,
                     help="steps a copy of the batch runs before the autotuner times candidates on it")
     ap.add_argument("--latency-steps", type=int, default=50)
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch configuration instead of the autotuner's, e.g. convex_lds=0,narrowphase_block=64")
     ap.add_argument("--settled-steps", type=int, default=50,
                     help="steps timed on the autotuner's copy of the batch once it has settled")
     ap.add_argument("--plain-sample", type=int, default=48,

@@ -40,8 +40,9 @@ struct Shape {
     int kind;
     f3 half;
     float half_length;
-    const float4* verts;
+    const float4* verts;  // vertex k at verts[k * vstride]
     int nverts;
+    int vstride;
     float margin;
 };
 
@@ -67,7 +68,7 @@ BEPU_D f3 support_local(const Shape& s, f3 d) {
         for (int i0 = 0; i0 < s.nverts; i0 += kChunk) {
             float4 v[kChunk];
 #pragma unroll
-            for (int k = 0; k < kChunk; ++k) v[k] = s.verts[min(i0 + k, s.nverts - 1)];
+            for (int k = 0; k < kChunk; ++k) v[k] = s.verts[min(i0 + k, s.nverts - 1) * s.vstride];
 #pragma unroll
             for (int k = 0; k < kChunk; ++k) {
                 float dd = v[k].x * d.x + v[k].y * d.y + v[k].z * d.z;

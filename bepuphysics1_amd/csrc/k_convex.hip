@@ -12,8 +12,9 @@ namespace bepu {
 
 namespace {
 
-// `pool` is the hull vertex pool: the device array, or its copy in LDS.
-__device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, const float4* pool, uint32_t i) {
+// The shape of body i.  A hull's vertices are read from `verts` (the device
+// pool, or this pair's copy in LDS) at stride `vstride`.
+__device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i, const float4* verts, int vstride) {
     float4 sh = w.shape[i];
     cvx::Shape s;
     s.kind = (int)sh.w;
@@ -21,6 +22,7 @@ __device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, const float
     s.half_length = 0;
     s.verts = nullptr;
     s.nverts = 0;
+    s.vstride = vstride;
     if (s.kind == BEPU_SHAPE_BOX) {
         s.half = f3_of(sh);
         s.margin = w.s.default_margin;  // BoxShape's margin: CollisionDetectionSettings.DefaultMargin
@@ -29,7 +31,7 @@ __device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, const float
         s.half_length = sh.y;
     } else if (s.kind == BEPU_SHAPE_CONVEX_HULL) {
         s.margin = sh.x;
-        s.verts = pool + __float_as_uint(sh.y);
+        s.verts = verts;
         s.nverts = (int)__float_as_uint(sh.z);
     } else {
         s.margin = sh.x;  // sphere: the radius
@@ -108,8 +110,8 @@ __device__ __forceinline__ void add_general(PairState& s, ConvexState& e, f3 pos
 // GeneralConvexContactManifold.Update (GeneralConvexContactManifold.cs:71-126) with
 // ContactRefresher.ContactRefresh (CollisionTests/ContactRefresher.cs:18-69) and
 // IsContactUnique (GeneralConvexContactManifold.cs:145-166).
-__device__ __forceinline__ void update_general(const DeviceWorld& w, const float4* pool, PairState& s,
-                                               ConvexState& e) {
+__device__ __forceinline__ void update_general(const DeviceWorld& w, const cvx::Shape& sa, const cvx::Shape& sb,
+                                               PairState& s, ConvexState& e) {
     float4 PA = w.pos[s.a], QA = w.quat[s.a], PB = w.pos[s.b], QB = w.quat[s.b];
     cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
     cvx::RT tb{f3_of(PB), q4{QB.x, QB.y, QB.z, QB.w}};
@@ -142,8 +144,7 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, const float
     f3 pos, nrm;
     float depth;
     int id;
-    if (cvx::candidate(shape_of(w, pool, s.a), shape_of(w, pool, s.b), ta, tb, f3_of(w.linv[s.a]), f3_of(w.linv[s.b]), st, &pos,
-                       &nrm, &depth, &id)) {
+    if (cvx::candidate(sa, sb, ta, tb, f3_of(w.linv[s.a]), f3_of(w.linv[s.b]), st, &pos, &nrm, &depth, &id)) {
         for (int i = 0; i < s.ccount; ++i)
             if (dot3(f3_of(s.cnrm[s.clist[i]]), nrm) < 0) {
                 remove_contact_inplace(s, i);
@@ -181,17 +182,36 @@ __device__ __forceinline__ void update_general(const DeviceWorld& w, const float
 
 constexpr int kConvexBlock = 64;
 constexpr uint32_t kConvexMaxGrid = 8192;  // one pair per thread up to 524,288 queued pairs
-// With at most this many hull vertices (64 KB) every workgroup first copies
-// the whole vertex pool to LDS: a support scan's loads then take LDS latency
-// instead of a round trip to L2, which bounds a pair test's dependent chain.
-constexpr uint32_t kLdsHullVerts = kConvexLdsMaxVerts;
-constexpr int kConvexLdsBlock = 256;
+
+// Copies body i's hull vertices (if it is a hull) to dst[k * kBlock]: a
+// thread's vertices lie one lane apart, so a wavefront's LDS reads of vertex
+// k are consecutive 16-byte words (no bank conflicts).
+template <int kBlock>
+__device__ __forceinline__ void stage_hull(const DeviceWorld& w, uint32_t i, float4* dst) {
+    float4 sh = w.shape[i];
+    if (sh.w != (float)BEPU_SHAPE_CONVEX_HULL) return;
+    const float4* src = w.hull_verts + __float_as_uint(sh.y);
+    const int n = (int)__float_as_uint(sh.z);
+    constexpr int kChunk = 8;
+    for (int k0 = 0; k0 < n; k0 += kChunk) {
+        float4 v[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k) v[k] = src[min(k0 + k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < kChunk; ++k)
+            if (k0 + k < n) dst[(k0 + k) * kBlock] = v[k];
+    }
+}
 
 }  // namespace
 
-template <bool kLds, int kBlock>
+// kStage: each pair's hull vertices are first copied to this thread's columns
+// of an LDS block, so that every support scan of its GJK / MPR iterations
+// reads LDS instead of making two dependent round trips to L2 (at most
+// kConvexStageVerts vertices per hull; the host checks).
+template <bool kStage, int kBlock>
 __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
-    extern __shared__ float4 lds_hull[];
+    __shared__ float4 stage[kStage ? 2 * kConvexStageVerts * kBlock : 1];
     // The state regions of the work list, costliest first: Deep pairs (MPR
     // with surface casts and refinement), then Shallow (closest-point GJK),
     // then Separated (boolean GJK).  Workgroups start in index order, so the
@@ -210,12 +230,6 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
     for (int q = 0; q < 6; ++q) beg[q + 1] = beg[q] + cnt[q];
     const uint32_t n = beg[6];
     if (blockIdx.x * kBlock >= n) return;  // uniform per workgroup
-    const float4* pool = w.hull_verts;
-    if (kLds) {
-        for (uint32_t v = threadIdx.x; v < w.hull_nverts; v += kBlock) lds_hull[v] = w.hull_verts[v];
-        __syncthreads();
-        pool = lds_hull;
-    }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         int q = 0;
 #pragma unroll
@@ -237,29 +251,32 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
             e.offa[0] = z; e.offa[1] = z; e.offa[2] = z; e.offa[3] = z;
             e.offb[0] = z; e.offb[1] = z; e.offb[2] = z; e.offb[3] = z;
         }
-        update_general(w, pool, s, e);
+        cvx::Shape sa, sb;
+        if (kStage) {
+            float4* ca = stage + threadIdx.x;
+            float4* cb = stage + kConvexStageVerts * kBlock + threadIdx.x;
+            stage_hull<kBlock>(w, s.a, ca);
+            stage_hull<kBlock>(w, s.b, cb);
+            sa = shape_of(w, s.a, ca, kBlock);
+            sb = shape_of(w, s.b, cb, kBlock);
+        } else {
+            float4 ha = w.shape[s.a], hb = w.shape[s.b];
+            sa = shape_of(w, s.a, w.hull_verts + __float_as_uint(ha.y), 1);
+            sb = shape_of(w, s.b, w.hull_verts + __float_as_uint(hb.y), 1);
+        }
+        update_general(w, sa, sb, s, e);
         // A constraint joins the islands of its two bodies when both are dynamic.
         if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
     }
 }
 
-void launch_convex(const DeviceWorld& w, bool lds, hipStream_t st) {
+void launch_convex(const DeviceWorld& w, bool stage, hipStream_t st) {
     if (!w.conv_work || w.pair_cap == 0) return;
-    if (lds && w.hull_nverts <= kLdsHullVerts) {
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute((const void*)k_convex<true, kConvexLdsBlock>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsHullVerts * sizeof(float4)));
-            attr = true;
-        }
-        uint32_t grid = std::min<uint32_t>(kConvexMaxGrid * kConvexBlock / kConvexLdsBlock,
-                                           (w.pair_cap + kConvexLdsBlock - 1) / kConvexLdsBlock);
-        hipLaunchKernelGGL((k_convex<true, kConvexLdsBlock>), dim3(grid), dim3(kConvexLdsBlock),
-                           w.hull_nverts * sizeof(float4), st, w);
-    } else {
-        uint32_t grid = std::min<uint32_t>(kConvexMaxGrid, (w.pair_cap + kConvexBlock - 1) / kConvexBlock);
+    uint32_t grid = std::min<uint32_t>(kConvexMaxGrid, (w.pair_cap + kConvexBlock - 1) / kConvexBlock);
+    if (stage && w.hull_max_verts <= kConvexStageVerts)
+        hipLaunchKernelGGL((k_convex<true, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w);
+    else
         hipLaunchKernelGGL((k_convex<false, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w);
-    }
 }
 
 }  // namespace bepu

@@ -445,6 +445,7 @@ bepu_status bepu_create_hull_shape(bepu_world w, const float* points, uint32_t n
     }
     w->hull_shapes.push_back(hs);
     w->d.hull_nverts = need;
+    w->d.hull_max_verts = std::max(w->d.hull_max_verts, hs.count);
     if (shape_id) *shape_id = (uint32_t)w->hull_shapes.size() - 1;
     if (center) {
         center[0] = h.center.x;
@@ -741,7 +742,7 @@ bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, 
     // The general convex choices first: they move the most time.
     std::vector<Choice> choices;
     if (w->convex_bodies) {
-        if (w->d.hull_nverts <= kConvexLdsMaxVerts) choices.push_back({&bepu_tuning::convex_lds, {0, 1}});
+        if (w->d.hull_max_verts <= kConvexStageVerts) choices.push_back({&bepu_tuning::convex_lds, {0, 1}});
         choices.push_back({&bepu_tuning::convex_side_stream, {0, 1}});
     }
     choices.push_back({&bepu_tuning::narrowphase_block, {64, 128, 256}});

@@ -99,6 +99,7 @@ struct DeviceWorld {
                                // capsule: (radius, half length, 0); hull: (margin, first vertex, vertex count as uint bits)
     float4* hull_verts;        // vertices of every hull shape, back to back (xyz)
     uint32_t hull_nverts;      // vertices in hull_verts
+    uint32_t hull_max_verts;   // vertices of the largest hull shape
     uint32_t small_island_max; // islands of at most this many constraints are packed per wavefront
     uint32_t pair_lists;       // 1: k_pair_prep lists this step's pairs by tester; 0: every pair is a
                                // closed-form pair of one kind and k_narrowphase takes them in order
@@ -232,8 +233,10 @@ void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_b
 void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st);
 // General convex pairs queued by the narrowphase (GJK / MPR).
 void launch_pair_prep(const DeviceWorld& w, hipStream_t st);
-void launch_convex(const DeviceWorld& w, bool lds, hipStream_t st);
-constexpr uint32_t kConvexLdsMaxVerts = 4096;  // hull vertex pools up to 64 KB can be copied to LDS
+// `stage`: each pair's hull vertices are copied to LDS before its test (when
+// no hull has more than kConvexStageVerts vertices), else read from HBM.
+void launch_convex(const DeviceWorld& w, bool stage, hipStream_t st);
+constexpr uint32_t kConvexStageVerts = 16;
 // Last step's constraint, contact, colour and penetration statistics into the counters.
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);
 void launch_islands(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
