@@ -492,7 +492,7 @@ def served_leg(rate, duration, max_wall):
     capacity and a 200 GB budget), then the queue drains."""
     from bepuphysics1_amd import scenes
     from bepuphysics1_amd.serve import Server, run_poisson
-    srv = Server(capacity=1_000_000, settings=batched_settings(), hbm_budget=200e9, hull_shapes=scenes.hull_library())
+    srv = Server(capacity=4_000_000, settings=batched_settings(), hbm_budget=200e9, hull_shapes=scenes.hull_library())
 
     def make_request(rng):
         n = int(np.exp(rng.uniform(np.log(16), np.log(2048))))
@@ -659,7 +659,7 @@ def main():
     ap.add_argument("--height", type=int, default=20)
     ap.add_argument("--tower-steps", type=int, default=200)
     ap.add_argument("--served-rate", type=float, default=2000.0)
-    ap.add_argument("--served-seconds", type=float, default=3.0)
+    ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--stress-bodies", type=int, default=32768)
     ap.add_argument("--legs", default="batched,towers,served,stress",
                     help="comma list of workloads; without batched only the other legs' results are printed")

@@ -158,6 +158,8 @@ EXPORTS = {
     "bepu_set_settings": (ctypes.c_int, [_vp, ctypes.POINTER(Settings)]),
     "bepu_get_motion_states_ranges_noalloc": (ctypes.c_int, [_vp, _u32p, _u32p, ctypes.c_uint32,
                                                              ctypes.POINTER(MotionState)]),
+    "bepu_request_motion_states_ranges": (ctypes.c_int, [_vp, _u32p, _u32p, ctypes.c_uint32, _u32p]),
+    "bepu_collect_motion_states": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(MotionState)]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -127,6 +127,18 @@ bepu_status bepu_world_destroy(bepu_world w) {
     if (w->id_stage) (void)hipFree(w->id_stage);
     if (w->state_stage) (void)hipFree(w->state_stage);
     if (w->sstage) (void)hipFree(w->sstage);
+    for (auto& slot : w->insert_ring) {
+        if (slot.host) (void)hipHostFree(slot.host);
+        if (slot.done) (void)hipEventDestroy(slot.done);
+    }
+    for (auto& rb : w->rb_ring) {
+        if (rb.ids_host) (void)hipHostFree(rb.ids_host);
+        if (rb.st_host) (void)hipHostFree(rb.st_host);
+        if (rb.err_host) (void)hipHostFree(rb.err_host);
+        if (rb.ids_dev) (void)hipFree(rb.ids_dev);
+        if (rb.st_dev) (void)hipFree(rb.st_dev);
+        if (rb.done) (void)hipEventDestroy(rb.done);
+    }
     if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
@@ -311,13 +323,16 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 sha
 }
 
 // Copies body records to the device in one block and scatters them to their
-// target ids (after the work already queued on the world's stream, which may
-// read those ids).
+// target ids, in stream order after the work already queued (which may read
+// those ids).  The records pass through a pinned host buffer of the insert
+// ring and an asynchronous copy; only growing the device staging waits for
+// the queued work.
 static bepu_status upload_records(bepu_world w, std::vector<BodyRecord>& recs) {
     uint32_t k = (uint32_t)recs.size();
     if (k == 0) return BEPU_OK;
-    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    const size_t bytes = (size_t)k * sizeof(BodyRecord);
     if (k > w->stage_cap) {
+        if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
         uint32_t cap = std::max<uint32_t>(k, 2 * w->stage_cap);
         BodyRecord* p = nullptr;
         if (hipMalloc(&p, (size_t)cap * sizeof(BodyRecord)) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
@@ -325,8 +340,24 @@ static bepu_status upload_records(bepu_world w, std::vector<BodyRecord>& recs) {
         w->stage = p;
         w->stage_cap = cap;
     }
-    if (hipMemcpy(w->stage, recs.data(), (size_t)k * sizeof(BodyRecord), hipMemcpyHostToDevice) != hipSuccess)
+    auto& slot = w->insert_ring[w->insert_next];
+    w->insert_next = (w->insert_next + 1) % bepu_world_s::kInsertRing;
+    if (!slot.done && hipEventCreateWithFlags(&slot.done, hipEventDisableTiming) != hipSuccess) return BEPU_E_DEVICE;
+    if (slot.pending && hipEventSynchronize(slot.done) != hipSuccess) return BEPU_E_DEVICE;  // its last copy finished
+    slot.pending = false;
+    if (slot.cap < bytes) {
+        size_t cap = std::max(bytes, 2 * slot.cap);
+        if (slot.host) (void)hipHostFree(slot.host);
+        slot.host = nullptr;
+        slot.cap = 0;
+        if (hipHostMalloc(&slot.host, cap, hipHostMallocDefault) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
+        slot.cap = cap;
+    }
+    std::memcpy(slot.host, recs.data(), bytes);
+    if (hipMemcpyAsync(w->stage, slot.host, bytes, hipMemcpyHostToDevice, w->stream) != hipSuccess ||
+        hipEventRecord(slot.done, w->stream) != hipSuccess)
         return BEPU_E_DEVICE;
+    slot.pending = true;
     launch_scatter_bodies(w->d, w->stage, k, w->stream);
     return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
 }
@@ -1154,6 +1185,96 @@ bepu_status bepu_set_settings(bepu_world w, const bepu_settings* settings) {
     if (!w || !settings || !settings_valid(*settings)) return BEPU_E_INVALID_ARG;
     w->d.s = *settings;
     return BEPU_OK;
+}
+
+// Asynchronous form of bepu_get_motion_states_ranges: queues the gather and
+// the copy to pinned host memory behind the steps already queued and returns
+// a ticket at once; bepu_collect_motion_states waits for that copy alone.
+bepu_status bepu_request_motion_states_ranges(bepu_world w, const uint32_t* firsts, const uint32_t* counts,
+                                              uint32_t nranges, uint32_t* ticket) {
+    if (!w || !ticket || (nranges && (!firsts || !counts))) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    uint64_t k64 = 0;
+    for (uint32_t r = 0; r < nranges; ++r) {
+        if ((uint64_t)firsts[r] + counts[r] > w->d.nbodies) return BEPU_E_INVALID_ARG;
+        k64 += counts[r];
+    }
+    if (k64 > (1u << 30)) return BEPU_E_INVALID_ARG;
+    const uint32_t k = (uint32_t)k64;
+    const uint32_t t = w->rb_next_ticket;
+    auto& rb = w->rb_ring[t % bepu_world_s::kReadbackRing];
+    if (rb.busy) return BEPU_E_CAPACITY;  // the ring holds this many uncollected readbacks
+    if (!rb.done && hipEventCreateWithFlags(&rb.done, hipEventDisableTiming) != hipSuccess) return BEPU_E_DEVICE;
+    if (!rb.err_host && hipHostMalloc((void**)&rb.err_host, 3 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
+        return BEPU_E_OUT_OF_MEMORY;
+    if (k > rb.cap) {
+        // The slot's previous copies completed when it was collected.
+        if (rb.ids_host) (void)hipHostFree(rb.ids_host);
+        if (rb.st_host) (void)hipHostFree(rb.st_host);
+        if (rb.ids_dev) (void)hipFree(rb.ids_dev);
+        if (rb.st_dev) (void)hipFree(rb.st_dev);
+        rb.ids_host = nullptr;
+        rb.st_host = nullptr;
+        rb.ids_dev = nullptr;
+        rb.st_dev = nullptr;
+        rb.cap = 0;
+        uint32_t cap = std::max<uint32_t>(k, 1024);
+        if (hipHostMalloc((void**)&rb.ids_host, (size_t)cap * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&rb.st_host, (size_t)cap * 4 * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&rb.ids_dev, (size_t)cap * sizeof(uint32_t)) != hipSuccess ||
+            hipMalloc(&rb.st_dev, (size_t)cap * 4 * sizeof(float4)) != hipSuccess)
+            return BEPU_E_OUT_OF_MEMORY;
+        rb.cap = cap;
+    }
+    uint32_t o = 0;
+    for (uint32_t r = 0; r < nranges; ++r)
+        for (uint32_t i = 0; i < counts[r]; ++i) rb.ids_host[o++] = firsts[r] + i;
+    hipStream_t s = w->stream;
+    if (k) {
+        if (hipMemcpyAsync(rb.ids_dev, rb.ids_host, (size_t)k * sizeof(uint32_t), hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+            return BEPU_E_DEVICE;
+        launch_gather_states(w->d, rb.ids_dev, k, rb.st_dev, s);
+        if (hipMemcpyAsync(rb.st_host, rb.st_dev, (size_t)k * 4 * sizeof(float4), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+            return BEPU_E_DEVICE;
+    }
+    const uint32_t idx[3] = {CTR_STICKY, CTR_OVERFLOW, CTR_GBAR_ABORT};
+    for (int i = 0; i < 3; ++i)
+        if (hipMemcpyAsync(rb.err_host + i, w->d.counters + ctr(idx[i]), sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           s) != hipSuccess)
+            return BEPU_E_DEVICE;
+    if (hipEventRecord(rb.done, s) != hipSuccess) return BEPU_E_DEVICE;
+    rb.n = k;
+    rb.ticket = t;
+    rb.busy = true;
+    w->rb_next_ticket = t + 1 ? t + 1 : 1;
+    *ticket = t;
+    return BEPU_OK;
+}
+
+// Waits for a requested readback and writes its motion states (in request
+// order) to `out`; the status carries the world's error flags as they stood
+// when the readback ran (BEPU_E_CAPACITY / BEPU_E_DEVICE, outputs filled).
+bepu_status bepu_collect_motion_states(bepu_world w, uint32_t ticket, bepu_motion_state* out) {
+    if (!w || ticket == 0) return BEPU_E_INVALID_ARG;
+    auto& rb = w->rb_ring[ticket % bepu_world_s::kReadbackRing];
+    if (!rb.busy || rb.ticket != ticket) return BEPU_E_NOT_FOUND;
+    if (rb.n && !out) return BEPU_E_INVALID_ARG;
+    if (hipEventSynchronize(rb.done) != hipSuccess) return BEPU_E_DEVICE;
+    for (uint32_t i = 0; i < rb.n; ++i) {
+        const float4 &p = rb.st_host[4 * i], &q = rb.st_host[4 * i + 1], &v = rb.st_host[4 * i + 2],
+                     &a = rb.st_host[4 * i + 3];
+        bepu_motion_state& m = out[i];
+        m.position[0] = p.x; m.position[1] = p.y; m.position[2] = p.z;
+        m.orientation[0] = q.x; m.orientation[1] = q.y; m.orientation[2] = q.z; m.orientation[3] = q.w;
+        m.linear_velocity[0] = v.x; m.linear_velocity[1] = v.y; m.linear_velocity[2] = v.z;
+        m.angular_velocity[0] = a.x; m.angular_velocity[1] = a.y; m.angular_velocity[2] = a.z;
+    }
+    rb.busy = false;
+    w->sticky |= rb.err_host[0] | (rb.err_host[1] ? (uint32_t)STICKY_OVERFLOW : 0u) |
+                 (rb.err_host[2] ? (uint32_t)STICKY_GBAR_ABORT : 0u);
+    return sticky_status(w->sticky);
 }
 
 }  // extern "C"

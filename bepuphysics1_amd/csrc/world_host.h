@@ -61,6 +61,36 @@ struct bepu_world_s {
     std::vector<uint8_t> absent;            // per body id: removed (bepu_remove, or its world was removed)
     bepu::StateRecord* sstage = nullptr;    // device staging of motion states written between steps
     uint32_t sstage_cap = 0;
+    // Inserted body records go through a ring of pinned host buffers whose
+    // asynchronous copies to the device staging are fenced by events, so an
+    // insertion never waits for the steps queued before it.
+    struct PinnedSlot {
+        void* host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kInsertRing = 3;
+    PinnedSlot insert_ring[kInsertRing];
+    int insert_next = 0;
+    // Asynchronous readbacks of body ranges (bepu_request_motion_states_ranges):
+    // a ring of slots, each with its own device and pinned host buffers, the
+    // error words copied beside the states, and an event.
+    struct Readback {
+        uint32_t* ids_host = nullptr;
+        uint32_t* ids_dev = nullptr;
+        float4* st_dev = nullptr;
+        float4* st_host = nullptr;
+        uint32_t* err_host = nullptr;  // sticky, overflow and abort words at the readback's place in the stream
+        uint32_t cap = 0;
+        uint32_t n = 0;
+        uint32_t ticket = 0;
+        hipEvent_t done = nullptr;
+        bool busy = false;
+    };
+    static constexpr int kReadbackRing = 8;
+    Readback rb_ring[kReadbackRing];
+    uint32_t rb_next_ticket = 1;
 };
 
 namespace bepu {

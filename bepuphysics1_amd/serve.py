@@ -10,6 +10,13 @@ bit-identical to stepping it alone: sub-worlds never pair with each other,
 their colouring uses world-local ids, and an inserted world never inherits
 the contact history of a previous occupant of its ids.
 
+The loop is pipelined: a tick queues its step, queues the readback of the
+worlds that finish with it (and frees their ranges, in stream order after
+that readback), and then collects the readback queued one tick earlier, so
+the device always has the next step queued while the host reads results and
+admits new requests.  Inserts copy through pinned memory without waiting
+for the device either.
+
 There is no reference counterpart: the reference steps one Space per
 Update call (Space.cs:527-531); each request here would be its own Space.
 """
@@ -52,6 +59,7 @@ class Server:
             # Hull bodies of the requests name these shapes by index.
             self.space.register_hulls(hull_shapes)
         self.queue = collections.deque()
+        self.pending = collections.deque()  # (readback request, finished requests) not yet collected
         self.resident = {}
         self.completed = []
         self.resident_bodies = 0
@@ -87,32 +95,40 @@ class Server:
             self.resident_bodies += len(r.scene)
 
     def tick(self):
-        """One step of every resident world; retires the finished ones (one
-        readback for all of them)."""
-        if not self.resident:
+        """Queues one step of every resident world and the readback of those
+        it finishes (one readback for all of them, also queued when none
+        finishes: it marks the step's end), frees their ranges, then collects
+        the readbacks of earlier ticks: all of them once nothing is resident."""
+        if not self.resident and not self.pending:
             return 0.0
         t0 = self.clock()
-        self.space.Update(1)
-        self.space.synchronize()
-        self.ticks += 1
-        done = []
-        for r in self.resident.values():
-            r.done += 1
-            self.body_steps += r.n_dyn
-            if r.done >= r.steps:
-                done.append(r)
-        if done:
-            states = self.space.motion_states_ranges([r.first for r in done], [len(r.scene) for r in done])
-            self.space.remove_worlds([r.world for r in done])
+        stepped = list(self.resident.values())
+        if stepped:
+            self.space.Update(1)
+            self.ticks += 1
+            done = []
+            for r in stepped:
+                r.done += 1
+                self.body_steps += r.n_dyn
+                if r.done >= r.steps:
+                    done.append(r)
+            req = self.space.request_motion_states_ranges([r.first for r in done], [len(r.scene) for r in done])
+            if done:
+                self.space.remove_worlds([r.world for r in done])
+                for r in done:
+                    del self.resident[r.rid]
+                    self.resident_bodies -= len(r.scene)
+            self.pending.append((req, done))
+        while len(self.pending) > (1 if self.resident else 0):
+            req, done = self.pending.popleft()
+            states = self.space.collect_motion_states(req)
             now = self.clock()
             for r, ms in zip(done, states):
                 r.result = ms
                 r.finished = now
-                del self.resident[r.rid]
-                self.resident_bodies -= len(r.scene)
                 self.completed.append(r)
         dt = self.clock() - t0
-        for r in list(self.resident.values()) + done:
+        for r in stepped:
             r.tick_latencies.append(dt)
         return dt
 
@@ -123,32 +139,53 @@ class Server:
 def run_poisson(server, rate, duration, make_request, seed=0, drain=True, max_wall=None):
     """Arrivals of a Poisson process (`rate` per second of wall time) for
     `duration` seconds, served tick by tick.  `make_request(rng)` returns
-    (scene, steps).  Returns a summary dict."""
+    (scene, steps).  The clients build their requests before the clock
+    starts (a request arrives with its scene); each is submitted at its
+    arrival time.  Returns a summary dict."""
     rng = np.random.default_rng(seed)
+    arrivals = []
+    t = rng.exponential(1.0 / rate)
+    while t <= duration:
+        scene, steps = make_request(rng)
+        arrivals.append((t, scene, steps))
+        t += rng.exponential(1.0 / rate)
+    offered = sum(int((sc["mass"] > 0).sum()) * st for _, sc, st in arrivals)
     t0 = time.perf_counter()
-    next_arrival = rng.exponential(1.0 / rate)
+    k = 0
+    waiting_max = 0
+    trace = []  # (seconds since start, body-steps so far) after every tick
     while True:
         now = time.perf_counter() - t0
-        while next_arrival <= min(now, duration):
-            scene, steps = make_request(rng)
-            server.submit(scene, steps, arrival=t0 + next_arrival)
-            next_arrival += rng.exponential(1.0 / rate)
+        while k < len(arrivals) and arrivals[k][0] <= now:
+            at, scene, steps = arrivals[k]
+            server.submit(scene, steps, arrival=t0 + at)
+            k += 1
+        waiting_max = max(waiting_max, len(server.queue))
         server.admit()
-        if server.resident:
+        if server.resident or server.pending:
             server.tick()
-        if now > duration and (not drain or (not server.resident and not server.queue)):
+            trace.append((time.perf_counter() - t0, server.body_steps))
+        if now > duration and (not drain or (not server.resident and not server.queue and not server.pending)):
             break
         if max_wall is not None and now > max_wall:
             break
     wall = time.perf_counter() - t0
+    # The rate over the second half of the arrival window: the resident set
+    # has built up by then (requests run up to 600 steps), and arrivals go on.
+    half = [(t, b) for t, b in trace if duration / 2 <= t <= duration]
+    steady = (half[-1][1] - half[0][1]) / (half[-1][0] - half[0][0]) if len(half) > 1 else None
     step_lat = np.array([x for r in server.completed for x in r.tick_latencies]) if server.completed else np.zeros(1)
     return {
         "wall_s": round(wall, 3),
         "requests_completed": len(server.completed),
         "requests_waiting": len(server.queue),
+        "requests_waiting_max": waiting_max,
+        "requests_offered": len(arrivals),
         "ticks": server.ticks,
         "body_steps": server.body_steps,
         "value": round(server.body_steps / wall, 1),
+        "steady_value": round(steady, 1) if steady else None,
+        "offered_body_steps_per_s": round(offered / duration, 1),
         "step_latency_ms": {"p50": round(1e3 * float(np.percentile(step_lat, 50)), 4),
                             "p99": round(1e3 * float(np.percentile(step_lat, 99)), 4)},
         "request_latency_s": {

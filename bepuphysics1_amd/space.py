@@ -427,6 +427,27 @@ class Space:
         flat = np.frombuffer(out, dtype=np.float32, count=13 * total).reshape(total, 13).copy()
         return np.split(flat, np.cumsum(c)[:-1].astype(np.int64)) if len(c) else []
 
+    def request_motion_states_ranges(self, firsts, counts):
+        """Queues a readback of several body ranges behind the queued steps;
+        returns a ticket for collect_motion_states (which waits for it)."""
+        f = np.ascontiguousarray(firsts, np.uint32)
+        c = np.ascontiguousarray(counts, np.uint32)
+        t = ctypes.c_uint32()
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        check(self._lib.bepu_request_motion_states_ranges(self._handle(), f.ctypes.data_as(u32p),
+                                                          c.ctypes.data_as(u32p), len(f), ctypes.byref(t)),
+              "request_motion_states_ranges")
+        return t.value, c
+
+    def collect_motion_states(self, request):
+        """The states of a requested readback: a list of (count_i, 13) arrays."""
+        ticket, c = request
+        total = int(c.sum())
+        out = (_lib.MotionState * max(total, 1))()
+        check(self._lib.bepu_collect_motion_states(self._handle(), ticket, out), "collect_motion_states")
+        flat = np.frombuffer(out, dtype=np.float32, count=13 * total).reshape(total, 13).copy()
+        return np.split(flat, np.cumsum(c)[:-1].astype(np.int64)) if len(c) else []
+
     def remove_world(self, wid):
         check(self._lib.bepu_remove_world(self._handle(), wid), "remove_world")
 

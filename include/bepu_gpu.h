@@ -265,6 +265,20 @@ bepu_status bepu_remove_worlds(bepu_world world, const uint32_t* world_ids, uint
 bepu_status bepu_get_motion_states_ranges(bepu_world world, const uint32_t* first_bodies,
                                           const uint32_t* counts, uint32_t nranges, bepu_motion_state* out);
 
+/* The same readback, asynchronous: bepu_request_motion_states_ranges queues
+ * it behind the steps already queued and returns a ticket at once (the ids
+ * are taken from the arguments before it returns);
+ * bepu_collect_motion_states waits for that readback alone and writes its
+ * states to `out` in request order (the sum of `counts` entries).  Up to 8
+ * readbacks may be outstanding (BEPU_E_CAPACITY beyond); a collected ticket
+ * is gone (BEPU_E_NOT_FOUND).  The collect status carries the error flags as
+ * they stood when the readback ran.  With inserts that never wait for the
+ * device either, a serving loop keeps its next step queued while it reads
+ * results (no reference counterpart). */
+bepu_status bepu_request_motion_states_ranges(bepu_world world, const uint32_t* first_bodies,
+                                              const uint32_t* counts, uint32_t nranges, uint32_t* ticket_out);
+bepu_status bepu_collect_motion_states(bepu_world world, uint32_t ticket, bepu_motion_state* out);
+
 /* Waits until every queued step of the world has finished on the device.
  * Returns BEPU_E_CAPACITY when any step since the last bepu_clear_error
  * overflowed a capacity (pairs, pair table, grid table, colours), and

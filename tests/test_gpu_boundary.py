@@ -213,3 +213,40 @@ def test_world_ids_stay_within_bounds():
     assert np.array_equal(f32bits(a), f32bits(b))
     with pytest.raises(Exception):
         sp.insert_world(desc)  # both slots taken
+
+
+def test_async_readback_matches_the_synchronous_one():
+    """bepu_request_motion_states_ranges / bepu_collect_motion_states give the
+    states bepu_get_motion_states_ranges gives at the same point of the
+    stream, while later steps are already queued; the ring holds 8
+    outstanding readbacks and a collected ticket is gone."""
+    worlds = scenes.batched_worlds(40, seed=2, shapes=(0, 1, 2))
+    sp = Space(capacity=sum(len(w) for w in worlds))
+    sp.register_hulls(scenes.hull_library())
+    spans = [(int(sp.add_world(w)[1][0]), len(w)) for w in worlds]
+    sp.Update(20)
+    f = [s[0] for s in spans[::3]]
+    c = [s[1] for s in spans[::3]]
+    req = sp.request_motion_states_ranges(f, c)
+    sp.Update(5)  # queued behind the readback
+    got = np.concatenate(sp.collect_motion_states(req))
+    sp2 = Space(capacity=sum(len(w) for w in worlds))
+    sp2.register_hulls(scenes.hull_library())
+    for w in worlds:
+        sp2.add_world(w)
+    sp2.Update(20)
+    want = np.concatenate(sp2.motion_states_ranges(f, c))
+    assert np.array_equal(f32bits(got), f32bits(want))
+    with pytest.raises(Exception):
+        sp.collect_motion_states(req)  # already collected
+    reqs = [sp.request_motion_states_ranges(f[:1], c[:1]) for _ in range(8)]
+    L, h = sp._lib, sp._handle()
+    t = ctypes.c_uint32()
+    one = np.array(f[:1], np.uint32)
+    cnt = np.array(c[:1], np.uint32)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    assert L.bepu_request_motion_states_ranges(h, one.ctypes.data_as(u32p), cnt.ctypes.data_as(u32p), 1,
+                                               ctypes.byref(t)) == E_CAPACITY
+    for r in reqs:
+        sp.collect_motion_states(r)
+    sp.synchronize()
