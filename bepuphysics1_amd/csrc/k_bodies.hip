@@ -41,6 +41,12 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
     if (i < w.nbodies) {
         w.uf_parent[i] = i;  // union-find forest of this step's islands
         valid = (w.bflags[i] & BF_PRESENT) != 0;
+        if (valid && w.wfault[w.wid[i]]) {
+            // Its world's state became non-finite: every body of that world
+            // leaves the simulation, so no other world sees it.
+            w.bflags[i] = 0u;
+            valid = false;
+        }
         if (!valid) {  // a removed world's slot: an empty box overlaps nothing
             w.bmin[i] = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
             w.bmax[i] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
@@ -279,6 +285,16 @@ __global__ void __launch_bounds__(256) k_integrate(DeviceWorld w) {
     f3 p = f3_of(P) + inc;
     w.quat[i] = make_float4(q.x, q.y, q.z, q.w);
     w.pos[i] = f4_of(p, 0.0f);
+    // The non-finite guard (the reference validates states with MathChecker in
+    // its debug builds, BEPUutilities/MathChecker.cs:23-50): a body whose state
+    // is no longer finite faults its world, which leaves from the next step on.
+    const bool finite = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(q.x) && isfinite(q.y) &&
+                        isfinite(q.z) && isfinite(q.w) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z) &&
+                        isfinite(av.x) && isfinite(av.y) && isfinite(av.z);
+    if (!finite) {
+        w.wfault[w.wid[i]] = 1u;
+        atomicOr(&w.counters[ctr(CTR_STICKY)], STICKY_NONFINITE);
+    }
 }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t v, size_t n) {

@@ -46,6 +46,32 @@ uint32_t next_pow2(uint64_t v) {
 
 static bepu_status check(hipError_t e) { return e == hipSuccess ? BEPU_OK : BEPU_E_DEVICE; }
 
+// The live sub-worlds whose state became non-finite, sorted (reads the
+// per-world flags only once a fault was seen: STICKY_NONFINITE).
+static bepu_status faulted_worlds(bepu_world w, std::vector<uint32_t>* out) {
+    out->clear();
+    if (!(w->sticky & STICKY_NONFINITE)) return BEPU_OK;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    std::vector<uint32_t> f(w->world_first.size(), 0);
+    if (!f.empty() && hipMemcpy(f.data(), w->d.wfault, f.size() * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return BEPU_E_DEVICE;
+    for (uint32_t k = 0; k < f.size(); ++k)
+        if (f[k] && w->world_live[k]) out->push_back(k);
+    return BEPU_OK;
+}
+
+// Whether any of the body ranges holds a body of a faulted world.
+static bool ranges_faulted(bepu_world w, const uint32_t* firsts, const uint32_t* counts, uint32_t n) {
+    std::vector<uint32_t> bad;
+    if (faulted_worlds(w, &bad) != BEPU_OK) return true;
+    for (uint32_t k : bad) {
+        uint64_t a0 = w->world_first[k], a1 = a0 + w->world_count[k];
+        for (uint32_t r = 0; r < n; ++r)
+            if ((uint64_t)firsts[r] < a1 && a0 < (uint64_t)firsts[r] + counts[r]) return true;
+    }
+    return false;
+}
+
 // Overflow (pair, hash, colour or list capacity) and grid barrier aborts of
 // any step since the last bepu_clear_error, as a status.
 bepu_status bepu::sticky_status(uint32_t bits) {
@@ -197,6 +223,7 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     // world the add calls append to, n + 2 (new_world_id enforces it).
     A(&d.wid, n); A(&d.wbase, n); A(&d.wlevels, (size_t)n + 2); A(&d.born, n);
     A(&d.wlev_count, 16 * ((size_t)n + 2)); A(&d.wlev_list, 16 * kLevelList * ((size_t)n + 2));
+    A(&d.wfault, (size_t)n + 2);
     A(&d.morton, n); A(&d.morton_sorted, n); A(&d.leaf_ids, n); A(&d.leaf_ids_tmp, n);
     A(&d.node_min, 2 * (size_t)n + kMaxTreeLevels); A(&d.node_max, 2 * (size_t)n + kMaxTreeLevels);
     A(&d.bounds, 8); A(&d.bounds_part, 6 * ((size_t)n / 256 + 1));
@@ -503,6 +530,7 @@ bepu_status bepu_begin_world(bepu_world w, uint32_t* world_id) {
         w->world_count[id] = 0;
         w->world_live[id] = 1;
         w->cur_world = id;
+        launch_fill_u32(w->d.wfault + id, 0u, 1, w->stream);
     }
     if (world_id) *world_id = w->cur_world;
     return BEPU_OK;
@@ -589,6 +617,7 @@ bepu_status bepu_insert_worlds(bepu_world w, const bepu_body_desc* bodies, const
         w->world_first[id] = first;
         w->world_count[id] = n;
         w->world_live[id] = 1;
+        launch_fill_u32(w->d.wfault + id, 0u, 1, w->stream);  // a reused id starts unfaulted
         if (w->absent.size() < (size_t)first + n) w->absent.resize((size_t)first + n, 0);
         std::fill(w->absent.begin() + first, w->absent.begin() + first + n, 0);
         if (append) {
@@ -662,11 +691,16 @@ bepu_status bepu_synchronize(bepu_world w) {
 
 bepu_status bepu_clear_error(bepu_world w) {
     if (!w) return BEPU_E_INVALID_ARG;
+    poll_errors(w);
     if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
-    for (uint32_t i : {(uint32_t)CTR_STICKY, (uint32_t)CTR_OVERFLOW, (uint32_t)CTR_GBAR_ABORT})
+    // An evicted sub-world stays evicted until it is removed: the bit that
+    // says some sub-world may be is kept.
+    const uint32_t keep = w->sticky & STICKY_NONFINITE;
+    launch_fill_u32(w->d.counters + ctr(CTR_STICKY), keep, 1, w->stream);
+    for (uint32_t i : {(uint32_t)CTR_OVERFLOW, (uint32_t)CTR_GBAR_ABORT})
         launch_fill_u32(w->d.counters + ctr(i), 0u, 1, w->stream);
     if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
-    w->sticky = 0;
+    w->sticky = keep;
     return BEPU_OK;
 }
 
@@ -713,6 +747,7 @@ bepu_status bepu_get_motion_states_ranges(bepu_world w, const uint32_t* firsts, 
         m.linear_velocity[0] = v.x; m.linear_velocity[1] = v.y; m.linear_velocity[2] = v.z;
         m.angular_velocity[0] = a.x; m.angular_velocity[1] = a.y; m.angular_velocity[2] = a.z;
     }
+    if (err == BEPU_OK && ranges_faulted(w, firsts, counts, nranges)) return BEPU_E_NONFINITE;
     return err;
 }
 
@@ -841,7 +876,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (!w || n_steps < 0) return BEPU_E_INVALID_ARG;
     // A world whose steps overflowed a capacity or aborted a grid barrier
     // takes no more steps until the caller acknowledges it (bepu_clear_error).
-    if (w->sticky) return sticky_status(w->sticky);
+    if (w->sticky & (STICKY_OVERFLOW | STICKY_GBAR_ABORT)) return sticky_status(w->sticky);
     bepu_status st = upload(w);
     if (st != BEPU_OK) return st;
     DeviceWorld& d = w->d;
@@ -962,6 +997,11 @@ bepu_status bepu_get_motion_states(bepu_world w, bepu_motion_state* out, uint32_
         m.angular_velocity[0] = a[i].x; m.angular_velocity[1] = a[i].y; m.angular_velocity[2] = a[i].z;
     }
     if (n > cap) return BEPU_E_CAPACITY;
+    if (err == BEPU_OK) {
+        std::vector<uint32_t> bad;
+        if (faulted_worlds(w, &bad) != BEPU_OK) return BEPU_E_DEVICE;
+        if (!bad.empty()) return BEPU_E_NONFINITE;
+    }
     return err;
 }
 
@@ -1275,6 +1315,22 @@ bepu_status bepu_collect_motion_states(bepu_world w, uint32_t ticket, bepu_motio
     w->sticky |= rb.err_host[0] | (rb.err_host[1] ? (uint32_t)STICKY_OVERFLOW : 0u) |
                  (rb.err_host[2] ? (uint32_t)STICKY_GBAR_ABORT : 0u);
     return sticky_status(w->sticky);
+}
+
+// Sub-worlds evicted by the non-finite guard (a body's position, orientation
+// or velocity no longer finite after a step): their bodies left every stage
+// from the next step on.
+bepu_status bepu_get_faulted_worlds(bepu_world w, uint32_t* ids, uint32_t cap, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    bepu_status st = poll_errors(w);
+    if (st == BEPU_E_DEVICE && !(w->sticky & STICKY_GBAR_ABORT)) return st;
+    std::vector<uint32_t> bad;
+    if (faulted_worlds(w, &bad) != BEPU_OK) return BEPU_E_DEVICE;
+    if (count) *count = (uint32_t)bad.size();
+    if (!ids) return BEPU_OK;  // count query
+    if (cap < bad.size()) return BEPU_E_CAPACITY;
+    std::copy(bad.begin(), bad.end(), ids);
+    return BEPU_OK;
 }
 
 }  // extern "C"

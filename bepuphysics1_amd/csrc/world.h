@@ -113,6 +113,7 @@ struct DeviceWorld {
     uint32_t* wlevels;         // per sub-world: grid levels holding at least one of its bodies
     uint32_t* wlev_count;      // per sub-world and grid level (16 each): bodies at that level
     uint32_t* wlev_list;       // the first kLevelList of them (a sparse coarse level is tested directly)
+    uint32_t* wfault;          // per sub-world: 1 once a body's state became non-finite (the world is evicted)
     uint32_t nworlds;
     uint32_t* born;            // index of the first step a body takes part in
     uint32_t step_no;          // index of the step being run
@@ -212,7 +213,9 @@ enum {
 
 // Sticky error bits: the start of each step folds the previous step's
 // overflow flag and grid barrier abort into CTR_STICKY before clearing them.
-enum { STICKY_OVERFLOW = 1u, STICKY_GBAR_ABORT = 2u };
+// STICKY_NONFINITE is not fatal: it tells that some sub-world was evicted
+// (its flag in wfault) while the others go on.
+enum { STICKY_OVERFLOW = 1u, STICKY_GBAR_ABORT = 2u, STICKY_NONFINITE = 4u };
 
 // Every counter sits on its own 256-byte line: device atomics on one line
 // serialise (about 10 ns each on MI355X), atomics on different lines do not.

@@ -35,7 +35,7 @@ BYTES_PER_BODY = 4700
 
 class Request:
     __slots__ = ("rid", "scene", "steps", "arrival", "admitted", "finished", "done", "world", "first", "result",
-                 "tick_latencies", "n_dyn")
+                 "tick_latencies", "n_dyn", "failed")
 
     def __init__(self, rid, scene, steps, arrival):
         self.rid, self.scene, self.steps, self.arrival = rid, scene, int(steps), arrival
@@ -44,6 +44,7 @@ class Request:
         self.done = 0
         self.world = self.first = None
         self.result = None
+        self.failed = False
         self.tick_latencies = []
 
 
@@ -126,6 +127,9 @@ class Server:
             for r, ms in zip(done, states):
                 r.result = ms
                 r.finished = now
+                # A world evicted by the non-finite guard is quarantined:
+                # it completes with its error instead of a result.
+                r.failed = not bool(np.isfinite(ms).all())
                 self.completed.append(r)
         dt = self.clock() - t0
         for r in stepped:

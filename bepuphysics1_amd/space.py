@@ -349,6 +349,16 @@ class Space:
                                                int(fields)), "set_motion_states")
         self._ms_cache = None
 
+    def faulted_worlds(self):
+        """Sub-worlds evicted by the non-finite guard (still live), sorted."""
+        n = ctypes.c_uint32()
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        check(self._lib.bepu_get_faulted_worlds(self._handle(), None, 0, ctypes.byref(n)), "get_faulted_worlds")
+        out = np.zeros(max(n.value, 1), np.uint32)
+        check(self._lib.bepu_get_faulted_worlds(self._handle(), out.ctypes.data_as(u32p), n.value, ctypes.byref(n)),
+              "get_faulted_worlds")
+        return out[:n.value].tolist()
+
     def clear_error(self):
         """Acknowledges a sticky overflow / barrier error so stepping resumes."""
         check(self._lib.bepu_clear_error(self._handle()), "clear_error")

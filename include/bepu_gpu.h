@@ -291,6 +291,22 @@ bepu_status bepu_synchronize(bepu_world world);
  * counterpart: the reference's lists grow instead of overflowing). */
 bepu_status bepu_clear_error(bepu_world world);
 
+/* The non-finite guard.  The reference validates positions, orientations and
+ * velocities with MathChecker in its debug builds (BEPUutilities/MathChecker.cs:23-50,
+ * NotFiniteNumberException).  Here a sub-world whose body state is no longer
+ * finite after a step is evicted: from the next step on every body of that
+ * sub-world leaves the simulation (as if removed), so the other sub-worlds of
+ * a batch go on bit-identical.  bepu_get_faulted_worlds lists the evicted
+ * sub-worlds still live (sorted); synchronous readbacks whose bodies include
+ * one of them return BEPU_E_NONFINITE (outputs still filled: an evicted body
+ * keeps the non-finite state it reached); an asynchronous collect reports the
+ * states as they are and no status of its own for this, so that it stays
+ * free of device waits.  Stepping continues, and bepu_clear_error keeps the
+ * evictions.  Bodies appended to an evicted sub-world are evicted with it; a
+ * sub-world id reused by an insertion starts unfaulted. */
+bepu_status bepu_get_faulted_worlds(bepu_world world, uint32_t* world_ids, uint32_t capacity,
+                                    uint32_t* count_out);
+
 /* Space.Remove(entity) -- Space.cs:364-482: the body leaves every stage from
  * the next step on, and its pairs and constraints end with it.  An id that is
  * unknown or already removed returns BEPU_E_NOT_FOUND (the reference's

@@ -250,3 +250,62 @@ def test_async_readback_matches_the_synchronous_one():
     for r in reqs:
         sp.collect_motion_states(r)
     sp.synchronize()
+
+
+def test_non_finite_world_is_evicted_and_the_others_go_on():
+    """One sub-world of a batch is driven to a non-finite state (position x
+    3.4e38 with velocity x 3e38 overflows at the next integration): it is
+    listed by bepu_get_faulted_worlds, synchronous readbacks touching it
+    return BEPU_E_NONFINITE, and every other world steps on bit-identical to
+    the same batch without the fault.  The fault survives bepu_clear_error;
+    removing the world and inserting a new one into its id starts clean."""
+    E_NONFINITE = 6
+    worlds = scenes.batched_worlds(24, seed=3, shapes=(0, 1, 2))
+    cap = sum(len(w) for w in worlds) + 64
+
+    def make():
+        sp = Space(capacity=cap)
+        sp.register_hulls(scenes.hull_library())
+        spans = [sp.add_world(w) for w in worlds]
+        return sp, [(int(wid), int(ids[0]), len(ids)) for wid, ids in spans]
+
+    a, spans = make()
+    b, _ = make()
+    a.Update(10)
+    b.Update(10)
+    bad = 5
+    wid, first, n = spans[bad]
+    dyn = np.flatnonzero(worlds[bad]["mass"] > 0)
+    body = first + int(dyn[0])
+    st = a.motion_states_range(body, 1).copy()
+    st[0, 0] = 3.4e38
+    st[0, 7] = 3e38
+    a.set_motion_states([body], st)
+    assert a.faulted_worlds() == []
+    a.Update(30)
+    b.Update(30)
+    assert a.faulted_worlds() == [wid]
+    assert b.faulted_worlds() == []
+    others = [s for k, s in enumerate(spans) if k != bad]
+    ga = a.motion_states_ranges([s[1] for s in others], [s[2] for s in others])
+    gb = b.motion_states_ranges([s[1] for s in others], [s[2] for s in others])
+    for x, y in zip(ga, gb):
+        assert np.array_equal(f32bits(x), f32bits(y))
+    L, h = a._lib, a._handle()
+    out = (_lib.MotionState * n)()
+    assert L.bepu_get_motion_states_range(h, first, n, out) == E_NONFINITE
+    got = np.frombuffer(out, dtype=np.float32).reshape(n, 13)
+    assert not np.isfinite(got[int(dyn[0])]).all()
+    cnt = ctypes.c_uint32()
+    whole = (_lib.MotionState * cap)()
+    assert L.bepu_get_motion_states(h, whole, cap, ctypes.byref(cnt)) == E_NONFINITE
+    assert L.bepu_clear_error(h) == 0
+    assert a.faulted_worlds() == [wid]
+    a.remove_world(wid)
+    assert a.faulted_worlds() == []
+    wid2, first2 = a.insert_world(worlds[bad])
+    a.Update(5)
+    assert a.faulted_worlds() == []
+    assert np.isfinite(a.motion_states_range(first2, n)).all()
+    a.close()
+    b.close()
