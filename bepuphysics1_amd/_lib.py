@@ -46,6 +46,7 @@ class Settings(ctypes.Structure):
         ("low_velocity_time_minimum", ctypes.c_float),
         ("linear_damping", ctypes.c_float),
         ("angular_damping", ctypes.c_float),
+        ("max_deactivation_attempts", ctypes.c_int32),
     ]
 
 

@@ -122,6 +122,7 @@ void bepu_settings_default(bepu_settings* s) {
     s->low_velocity_time_minimum = 1;
     s->linear_damping = .03f;
     s->angular_damping = .15f;
+    s->max_deactivation_attempts = 100;
 }
 
 int32_t bepu_device_available(void) {

@@ -78,6 +78,9 @@ typedef struct bepu_settings {
     float low_velocity_time_minimum;   /* DeactivationManager.LowVelocityTimeMinimum (1) */
     float linear_damping;              /* Entity default LinearDamping (0.03) */
     float angular_damping;             /* Entity default AngularDamping (0.15) */
+    int32_t max_deactivation_attempts; /* DeactivationManager.MaximumDeactivationAttemptsPerFrame (100):
+                                          island members looked at per step and sub-world when
+                                          deciding which islands go to sleep */
 } bepu_settings;
 
 /* BEPUphysics/EntityStateManagement/MotionState.cs:10-55 */

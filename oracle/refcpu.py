@@ -76,6 +76,9 @@ def lib():
         L.refcpu_mix64.restype = ctypes.c_uint64
         L.refcpu_mix64.argtypes = [ctypes.c_uint64]
         L.refcpu_remove.argtypes = [vp, ctypes.c_uint32]
+        L.refcpu_activate.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]
+        L.refcpu_get_activity.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_uint32)]
         L.refcpu_set_motion_states.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(MotionState),
                                                ctypes.c_uint32, ctypes.c_uint32]
         L.refcpu_set_settings.argtypes = [vp, ctypes.POINTER(Settings)]
@@ -178,6 +181,20 @@ class World:
         st = np.ascontiguousarray(states, np.float32).reshape(-1, 13)
         return lib().refcpu_set_motion_states(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                               st.ctypes.data_as(ctypes.POINTER(MotionState)), len(ids), int(fields))
+
+    def activate(self, ids):
+        """Entity.ActivityInformation.Activate() for each listed body."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        return lib().refcpu_activate(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(ids))
+
+    def activity(self):
+        """Entity.ActivityInformation.IsActive of every body (bool array)."""
+        n = ctypes.c_uint32()
+        lib().refcpu_get_activity(self.h, None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), np.uint8)
+        lib().refcpu_get_activity(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), n.value,
+                                  ctypes.byref(n))
+        return out[:n.value].astype(bool)
 
     def set_settings(self, settings):
         """Settings changed between steps (iteration limit, gravity, dt, ...)."""

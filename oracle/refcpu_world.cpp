@@ -60,6 +60,14 @@ struct Body {
     bool allow_stabilization;
     bool always_active;  // SimulationIslandMember.IsAlwaysActive
     bool candidate;      // SimulationIslandMember.IsDeactivationCandidate
+    // Activity (DeactivationManagement/SimulationIsland.cs, SimulationIslandMember.cs).
+    // A dynamic body sleeps with its island; a kinematic body's activity is
+    // its time below the limit used as a flag (SimulationIslandMember.cs:
+    // 227-250): <= 0 active, 1 inactive.
+    bool asleep;         // its island is inactive
+    bool wake;           // Activate() since the last step: the island is active at the next one
+    bool nodeact;        // its island may not deactivate at its next check (SimulationIsland.allowDeactivation)
+    uint32_t label;      // its island at the end of the last step (kNoLabel: its own)
     bepu_material mat;
     V3 bmin, bmax;
     // Space.Remove(entity) (Space.cs:364-482): the body left every stage; its
@@ -135,6 +143,9 @@ struct Pair {
     bool group_ais;
     int group_cur, group_zero;
     int color;
+    // Not updated this step: both bodies were inactive when the broad phase
+    // ran (BroadPhase.cs:97-102; NarrowPhase.cs:300-333 keeps such pairs).
+    bool frozen;
 };
 
 struct Settings {
@@ -245,6 +256,7 @@ struct refcpu_world {
     int threads = 1;
     std::unique_ptr<Pool> pool;  // threads > 1
     int broadphase_mode = 0;  // 0: uniform grid, 1: brute force, 2: sort and sweep
+    uint32_t deact_cursor = 0xffffffffu;  // last island looked at by the deactivation rotation (none yet)
 };
 
 namespace {
@@ -291,6 +303,11 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
     b->allow_stabilization = (flags & BEPU_FLAG_NO_STABILIZATION) == 0;
     b->always_active = (flags & BEPU_FLAG_ALWAYS_ACTIVE) != 0;
     b->candidate = false;
+    // DeactivationManager.Add activates the new member (DeactivationManager.cs:164):
+    // a kinematic body's flag becomes -1 (active until two quiet steps pass).
+    if (!b->dynamic) b->time_below = -1;
+    b->asleep = b->wake = b->nodeact = false;
+    b->label = 0xffffffffu;
     b->mat = m ? *m : bepu_material{1.0f, 0.8f, 0.0f};
 }
 
@@ -323,9 +340,170 @@ void candidacy(refcpu_world* w, Body& b) {
     b.prev_velocity = velocity;
 }
 
+constexpr uint32_t kNoLabel = 0xffffffffu;
+
+uint32_t label_of(const refcpu_world* w, uint32_t i) {
+    const uint32_t l = w->bodies[i].label;
+    return l == kNoLabel ? i : l;
+}
+
+// SimulationIslandMember.UpdateDeactivationCandidacy for a kinematic member
+// (SimulationIslandMember.cs:96-139): an inactive one (flag 1) stays so until
+// activated; an active one becomes a candidate at zero velocity (flag -1 ->
+// 0 -> 1: inactive after a full quiet step) and is active again (-1) when it
+// moves.  Returns whether it is active after the update.
+bool kinematic_candidacy(refcpu_world* w, Body& b) {
+    (void)w;
+    if (b.time_below > 0) return false;
+    float velocity = length_sq(b.v) + length_sq(b.w);
+    if (velocity == 0 && !b.always_active) {
+        if (b.time_below == 0) b.time_below = 1;
+        else if (b.time_below < 0) b.time_below = 0;
+    } else {
+        b.time_below = -1;
+    }
+    return b.time_below <= 0;
+}
+
+inline bool kinematic_active(const Body& b) { return !b.dynamic && b.time_below <= 0; }
+
+// Whether a body takes part in this step's stages (IsActive): a dynamic body
+// when its island is active, a kinematic one by its flag.
+inline bool body_active(const Body& b) { return !b.removed && (b.dynamic ? !b.asleep : b.time_below <= 0); }
+
+// SimulationIslandMember.Activate (SimulationIslandMember.cs:256-277): the
+// candidate flag is cleared (which zeroes the time below the limit) and the
+// island becomes active; a kinematic member's flag becomes -1.
+void activate(Body& b) {
+    if (b.removed) return;
+    if (b.dynamic) {
+        b.candidate = false;
+        b.time_below = 0;
+        b.wake = true;
+    } else {
+        b.time_below = -1;
+    }
+}
+
+struct UnionFind {
+    std::vector<uint32_t> p;
+    explicit UnionFind(size_t n) : p(n) {
+        for (size_t i = 0; i < n; ++i) p[i] = (uint32_t)i;
+    }
+    uint32_t find(uint32_t x) {
+        while (p[x] != x) {
+            p[x] = p[p[x]];
+            x = p[x];
+        }
+        return x;
+    }
+    void unite(uint32_t a, uint32_t b) {
+        a = find(a);
+        b = find(b);
+        if (a != b) p[std::max(a, b)] = std::min(a, b);
+    }
+};
+
+// DeactivationManager.Update (DeactivationManager.cs:196-205) at the start of
+// the step, over the islands the last step ended with (the connected
+// components of its constraints; split attempts are flushed at once):
+//   * kinematic members update their flag; an active one keeps the islands
+//     it touches through constraints active and forbids their next
+//     deactivation (SimulationIslandMember.cs:105-139);
+//   * an island with a member activated since the last step is active;
+//   * members of active islands update their candidacy (the dynamic branch,
+//     SimulationIslandMember.cs:53-95), the others only remember their speed;
+//   * DeactivateObjects (DeactivationManager.cs:291-320): islands are looked
+//     at in turn, continuing after the last one looked at in the previous
+//     step, until the members looked at reach MaximumDeactivationAttempts-
+//     PerFrame or every island was looked at once; an active island whose
+//     members are all candidates goes to sleep (SimulationIsland.TryToDeactivate,
+//     SimulationIsland.cs:92-114).  The turn order is by smallest member id
+//     (the reference's follows its island list).  The rotation runs in a step
+//     only if the world holds a dynamic body that is not always active.
+void deactivation_stage(refcpu_world* w) {
+    const uint32_t n = (uint32_t)w->bodies.size();
+    for (Body& b : w->bodies)
+        if (!b.dynamic && !b.removed) kinematic_candidacy(w, b);
+    std::vector<uint8_t> act(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const Body& b = w->bodies[i];
+        if (b.dynamic && !b.removed && b.wake) act[label_of(w, i)] = 1;
+    }
+    for (const Pair& p : w->pairs.vals) {
+        if (p.ccount <= 0) continue;
+        Body& A = w->bodies[p.a];
+        Body& B = w->bodies[p.b];
+        if (A.removed || B.removed) continue;
+        if (kinematic_active(A) && B.dynamic) {
+            act[label_of(w, p.b)] = 1;
+            B.nodeact = true;
+        }
+        if (kinematic_active(B) && A.dynamic) {
+            act[label_of(w, p.a)] = 1;
+            A.nodeact = true;
+        }
+    }
+    struct Agg {
+        uint32_t members = 0, cands = 0, minid = kNoLabel;
+        bool active = false, nodeact = false;
+    };
+    std::vector<Agg> agg(n);
+    bool sleepy = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        Body& b = w->bodies[i];
+        if (!b.dynamic || b.removed) continue;
+        const uint32_t L = label_of(w, i);
+        const bool active = !b.asleep || act[L];
+        if (active) {
+            candidacy(w, b);
+            b.asleep = false;
+        } else {
+            b.prev_velocity = length_sq(b.v) + length_sq(b.w);
+        }
+        b.wake = false;
+        Agg& g = agg[L];
+        g.members++;
+        g.cands += b.candidate ? 1u : 0u;
+        g.active = g.active || active;
+        g.nodeact = g.nodeact || b.nodeact;
+        g.minid = std::min(g.minid, i);
+        if (!b.always_active) sleepy = true;
+    }
+    if (!sleepy) return;
+    std::vector<uint32_t> reps;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Body& b = w->bodies[i];
+        if (b.dynamic && !b.removed && agg[label_of(w, i)].minid == i) reps.push_back(i);
+    }
+    if (reps.empty()) return;
+    size_t start = 0;
+    if (w->deact_cursor != kNoLabel)
+        start = (size_t)(std::upper_bound(reps.begin(), reps.end(), w->deact_cursor) - reps.begin());
+    std::vector<uint8_t> checked(n, 0), sleep(n, 0);
+    int64_t budget = 0;
+    for (size_t k = 0; k < reps.size(); ++k) {
+        if (budget >= (int64_t)w->s.max_deactivation_attempts) break;
+        const uint32_t r = reps[(start + k) % reps.size()];
+        const uint32_t L = label_of(w, r);
+        const Agg& g = agg[L];
+        checked[L] = 1;
+        if (!g.nodeact && g.active && g.cands == g.members) sleep[L] = 1;
+        budget += g.members;
+        w->deact_cursor = r;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        Body& b = w->bodies[i];
+        if (!b.dynamic || b.removed) continue;
+        const uint32_t L = label_of(w, i);
+        if (checked[L]) b.nodeact = false;
+        if (sleep[L]) b.asleep = true;
+    }
+}
+
 // Entity.UpdateForForces (EntityBase.cs:911-977); ForceUpdater.gravityDt (ForceUpdater.cs:85-92)
 void force_update(refcpu_world* w, Body& b) {
-    if (!b.dynamic || b.removed) return;
+    if (!b.dynamic || b.removed || b.asleep) return;  // ForceUpdater.cs:75: active entities only
     const bepu_settings& s = w->s;
     V3 gravityDt = mul(V3{s.gravity[0], s.gravity[1], s.gravity[2]}, s.dt);
     b.v = add(gravityDt, b.v);
@@ -356,7 +534,9 @@ void force_update(refcpu_world* w, Body& b) {
 // EntityCollidable.UpdateBoundingBox -> Shape.GetBoundingBox -> ExpandBoundingBox
 // (EntityCollidable.cs:140-148,195-257; BoxShape.cs:149-174; SphereShape.cs:75-86)
 void bounding_box(refcpu_world* w, Body& b) {
-    if (b.removed) return;
+    // BoundingBoxUpdater.cs:48: active entries only (a sleeping body keeps its
+    // box; recomputing an inactive kinematic body's gives the same box).
+    if (b.removed || (b.dynamic && b.asleep)) return;
     if (b.shape == BEPU_SHAPE_BOX) {
         M3 o = m3_from_quat(b.q);
         V3 right{signf_int(o.m11) * b.half.x, signf_int(o.m21) * b.half.y, signf_int(o.m31) * b.half.z};
@@ -424,7 +604,7 @@ void bounding_box(refcpu_world* w, Body& b) {
 
 // Entity.PreUpdatePosition (EntityBase.cs:1124-1155)
 void integrate(refcpu_world* w, Body& b) {
-    if (b.removed) return;
+    if (!body_active(b)) return;  // ContinuousPositionUpdater.cs:71: active entities only
     float dt = w->s.dt;
     V3 inc = mul(b.w, dt * .5f);
     Q4 m{inc.x, inc.y, inc.z, 0};
@@ -1472,6 +1652,83 @@ void solve_iteration(refcpu_world* w, Pair& p) {
 }
 
 // ---------------------------------------------------------------------------
+// Activity after the narrow phase
+
+// A pair that gained contacts adds a constraint to the solver, and adding its
+// island connection merges the islands and activates the result
+// (StandardPairHandler.cs:158-168 -> Solver.Add -> DeactivationManager.Add /
+// Merge, DeactivationManager.cs:323-388); a new constraint with an active
+// kinematic body activates the island too (a merge with no island).  So an
+// island of this step's constraints (frozen ones included) is active when a
+// member is awake or a constraint holds an active kinematic body.  Frozen
+// pairs whose island became active are stale and removed (RemoveStaleOverlaps,
+// NarrowPhase.cs:300-333: not updated, one entry active); the next step
+// creates them again.  The islands the step ends with, the labels the next
+// deactivation stage uses, are the components of the constraints left.
+void activity_pass(refcpu_world* w, PairTable& t) {
+    const uint32_t n = (uint32_t)w->bodies.size();
+    UnionFind uf(n);
+    for (const Pair& p : t.vals)
+        if (p.ccount > 0 && w->bodies[p.a].dynamic && w->bodies[p.b].dynamic) uf.unite(p.a, p.b);
+    std::vector<uint8_t> act(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const Body& b = w->bodies[i];
+        if (b.dynamic && !b.removed && !b.asleep) act[uf.find(i)] = 1;
+    }
+    for (const Pair& p : t.vals) {
+        if (p.ccount <= 0) continue;
+        const Body& A = w->bodies[p.a];
+        const Body& B = w->bodies[p.b];
+        if (kinematic_active(A) && B.dynamic) act[uf.find(p.b)] = 1;
+        if (kinematic_active(B) && A.dynamic) act[uf.find(p.a)] = 1;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        Body& b = w->bodies[i];
+        if (b.dynamic && !b.removed && act[uf.find(i)]) b.asleep = false;
+    }
+    size_t m = 0;
+    for (size_t k = 0; k < t.keys.size(); ++k) {
+        const Pair& p = t.vals[k];
+        if (p.frozen && (body_active(w->bodies[p.a]) || body_active(w->bodies[p.b]))) continue;
+        if (m != k) {
+            t.keys[m] = t.keys[k];
+            t.vals[m] = t.vals[k];
+        }
+        ++m;
+    }
+    t.keys.resize(m);
+    t.vals.resize(m);
+    UnionFind live(n);
+    for (const Pair& p : t.vals)
+        if (p.ccount > 0 && w->bodies[p.a].dynamic && w->bodies[p.b].dynamic) live.unite(p.a, p.b);
+    for (uint32_t i = 0; i < n; ++i) {
+        Body& b = w->bodies[i];
+        if (b.dynamic && !b.removed) b.label = live.find(i);
+    }
+}
+
+// A constraint removed this step queues a split attempt, flushed at the next
+// deactivation update (DeactivationManager.Remove / FlushSplits,
+// DeactivationManager.cs:232-283, 401-417): when its two dynamic bodies are
+// no longer connected, both are activated (TryToSplit, DeactivationManager.cs:
+// 432-555), which restarts their time below the limit.  Every attempt is
+// flushed at once (the reference flushes max(3, 1%) of them per step).
+void split_activation(refcpu_world* w, const PairTable& old, const PairTable& now) {
+    for (const Pair& p : old.vals) {
+        if (p.ccount <= 0) continue;
+        Body& A = w->bodies[p.a];
+        Body& B = w->bodies[p.b];
+        if (!A.dynamic || !B.dynamic || A.removed || B.removed) continue;
+        const Pair* q = now.find(pair_key(p.a, p.b));
+        if (q && q->ccount > 0) continue;
+        if (label_of(w, p.a) != label_of(w, p.b)) {
+            activate(A);
+            activate(B);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // One time step (Space.DoTimeStep, Space.cs:496-521)
 
 void step_once(refcpu_world* w) {
@@ -1479,9 +1736,9 @@ void step_once(refcpu_world* w) {
     // independent, and the solver runs colour by colour (no two constraints of
     // a colour share a dynamic body), so the multithreaded step computes the
     // same bits as the single-threaded one.
+    deactivation_stage(w);
     par_for(w, w->bodies.size(), 4096, [&](size_t b0, size_t b1) {
         for (size_t i = b0; i < b1; ++i) {
-            candidacy(w, w->bodies[i]);
             force_update(w, w->bodies[i]);
             bounding_box(w, w->bodies[i]);
         }
@@ -1491,27 +1748,55 @@ void step_once(refcpu_world* w) {
     else broadphase_grid(w, &w->overlaps);
 
     // NarrowPhase.UpdateSingleThreaded: update (or create) one pair per overlap,
-    // then drop pairs whose overlap vanished (NarrowPhase.cs:254-340).
+    // then drop pairs whose overlap vanished (NarrowPhase.cs:254-340).  The
+    // broad phase reports no overlap of two inactive entries (BroadPhase.cs:
+    // 97-102): such a pair is kept as it was, not updated, and none is created.
     std::vector<Pair> updated(w->overlaps.size());
+    std::vector<uint8_t> keep(w->overlaps.size(), 1);
     par_for(w, w->overlaps.size(), 256, [&](size_t k0, size_t k1) {
         for (size_t k = k0; k < k1; ++k) {
             uint64_t key = w->overlaps[k];
+            const uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)(key & 0xffffffffu);
             const Pair* old = w->pairs.find(key);
             Pair& p = updated[k];
+            if (!body_active(w->bodies[ka]) && !body_active(w->bodies[kb])) {
+                if (old) {
+                    p = *old;
+                    p.frozen = true;
+                } else {
+                    keep[k] = 0;
+                }
+                continue;
+            }
             if (old) p = *old;
-            else p = make_pair_state(w, (uint32_t)(key >> 32), (uint32_t)(key & 0xffffffffu));
+            else p = make_pair_state(w, ka, kb);
+            p.frozen = false;
             if (p.kind == BEPU_PAIR_BOX_BOX) update_box_box(w, p);
             else if (p.kind == BEPU_PAIR_CONVEX) update_general_convex(w, p);
             else update_single_contact(w, p);
         }
     });
-    w->pairs.keys = w->overlaps;
-    w->pairs.vals.swap(updated);
+    PairTable next;
+    next.keys.reserve(w->overlaps.size());
+    next.vals.reserve(w->overlaps.size());
+    for (size_t k = 0; k < w->overlaps.size(); ++k)
+        if (keep[k]) {
+            next.keys.push_back(w->overlaps[k]);
+            next.vals.push_back(std::move(updated[k]));
+        }
+    activity_pass(w, next);
+    split_activation(w, w->pairs, next);
+    w->pairs = std::move(next);
+    w->overlaps = w->pairs.keys;
 
+    // Solver.UpdateSolverActivity (SolverUpdateable.cs:202-218): the
+    // constraints of inactive islands are not solved.
     std::vector<Pair*> cons;
     float maxpen = 0;
     for (Pair& p : w->pairs.vals) {
-        if (p.ccount > 0) cons.push_back(&p);
+        if (p.ccount > 0 && !(w->bodies[p.a].dynamic && w->bodies[p.a].asleep) &&
+            !(w->bodies[p.b].dynamic && w->bodies[p.b].asleep))
+            cons.push_back(&p);
         for (int i = 0; i < p.ccount; ++i) maxpen = std::max(maxpen, p.slot[p.clist[i]].depth);
     }
     w->max_pen = maxpen;
@@ -1574,6 +1859,7 @@ void refcpu_settings_default(bepu_settings* s) {
     s->low_velocity_time_minimum = 1;
     s->linear_damping = .03f;
     s->angular_damping = .15f;
+    s->max_deactivation_attempts = 100;
 }
 
 refcpu_world* refcpu_create(const bepu_settings* s) {
@@ -1776,14 +2062,22 @@ int refcpu_get_manifolds(refcpu_world* w, bepu_manifold* out, uint32_t cap, uint
     return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
 }
 
+// Accumulated impulses of every constraint (a sleeping island's keep the
+// values they had when it went to sleep), by colour of the last solve, then key.
 int refcpu_get_impulses(refcpu_world* w, bepu_impulses* out, uint32_t cap, uint32_t* n) {
-    uint32_t cnt = (uint32_t)w->order.size();
+    std::vector<const Pair*> live;
+    for (const Pair& p : w->pairs.vals)
+        if (p.ccount > 0) live.push_back(&p);
+    std::sort(live.begin(), live.end(), [](const Pair* a, const Pair* b) {
+        return a->color != b->color ? a->color < b->color : pair_key(a->a, a->b) < pair_key(b->a, b->b);
+    });
+    uint32_t cnt = (uint32_t)live.size();
     if (n) *n = cnt;
     for (uint32_t i = 0; i < cnt && i < cap; ++i) {
-        const Pair& p = *w->pairs.find(w->order[i]);
+        const Pair& p = *live[i];
         bepu_impulses& o = out[i];
         std::memset(&o, 0, sizeof(o));
-        o.key = w->order[i];
+        o.key = pair_key(p.a, p.b);
         o.color = p.color;
         o.count = p.npen;
         for (int k = 0; k < p.npen; ++k) o.normal[k] = p.pen[k].acc;
@@ -1812,10 +2106,34 @@ int refcpu_get_stats(refcpu_world* w, bepu_step_stats* s) {
 
 // Space.Remove(entity) (Space.cs:364-482).  An unknown or already removed id
 // is the reference's ArgumentException ("not a member of this space").
+// The removed member's island loses it, and the dynamic bodies it had
+// constraints with are activated (Space.Remove -> DeactivationManager.Remove
+// -> RemoveSimulationIslandFromMember, DeactivationManager.cs:565-656).
 int refcpu_remove(refcpu_world* w, uint32_t id) {
     if (id >= w->bodies.size() || w->bodies[id].removed) return BEPU_E_NOT_FOUND;
     w->bodies[id].removed = true;
+    for (const Pair& p : w->pairs.vals) {
+        if (p.ccount <= 0 || (p.a != id && p.b != id)) continue;
+        Body& other = w->bodies[p.a == id ? p.b : p.a];
+        if (other.dynamic) activate(other);
+    }
     return BEPU_OK;
+}
+
+// Entity.ActivityInformation.Activate() for the listed bodies.
+int refcpu_activate(refcpu_world* w, const uint32_t* ids, uint32_t n) {
+    for (uint32_t k = 0; k < n; ++k)
+        if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
+    for (uint32_t k = 0; k < n; ++k) activate(w->bodies[ids[k]]);
+    return BEPU_OK;
+}
+
+// Entity.ActivityInformation.IsActive of every body (0 for a removed one).
+int refcpu_get_activity(refcpu_world* w, uint8_t* out, uint32_t cap, uint32_t* n) {
+    uint32_t cnt = (uint32_t)w->bodies.size();
+    if (n) *n = cnt;
+    for (uint32_t i = 0; i < cnt && i < cap; ++i) out[i] = body_active(w->bodies[i]) ? 1 : 0;
+    return cnt <= cap ? BEPU_OK : BEPU_E_CAPACITY;
 }
 
 // Entity state setters, for each listed body the parts `fields` selects:
@@ -1847,6 +2165,7 @@ int refcpu_set_motion_states(refcpu_world* w, const uint32_t* ids, const bepu_mo
             b.v = V3{m.linear_velocity[0], m.linear_velocity[1], m.linear_velocity[2]};
         if (fields & BEPU_STATE_ANGULAR_VELOCITY)
             b.w = V3{m.angular_velocity[0], m.angular_velocity[1], m.angular_velocity[2]};
+        activate(b);  // every one of these setters activates the entity
     }
     return BEPU_OK;
 }
