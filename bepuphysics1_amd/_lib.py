@@ -162,6 +162,8 @@ EXPORTS = {
     "bepu_request_motion_states_ranges": (ctypes.c_int, [_vp, _u32p, _u32p, ctypes.c_uint32, _u32p]),
     "bepu_collect_motion_states": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(MotionState)]),
     "bepu_get_faulted_worlds": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, _u32p]),
+    "bepu_get_activity": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, _u32p]),
+    "bepu_activate": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
     bool valid = false;
     if (i < w.nbodies) {
         w.uf_parent[i] = i;  // union-find forest of this step's islands
+        if (w.sleeping) w.act2[i] = 0u;
         valid = (w.bflags[i] & BF_PRESENT) != 0;
         if (valid && w.wfault[w.wid[i]]) {
             // Its world's state became non-finite: every body of that world
@@ -52,7 +53,29 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
             w.bmax[i] = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
         }
     }
-    if (valid) {
+    bool asleep = false;
+    if (valid && w.sleeping) {
+        // The island decisions of this step's deactivation stage.
+        const uint32_t f = w.bflags[i];
+        if (f & BF_DYNAMIC) {
+            const uint32_t fl = w.lab_flags[w.comp_root[i]];
+            uint32_t nf = f;
+            if (fl & LF_CHECKED) nf &= ~(uint32_t)BF_NODEACT;
+            if (fl & LF_SLEEP) nf |= BF_ASLEEP;
+            if (nf != f) w.bflags[i] = nf;
+            asleep = (nf & BF_ASLEEP) != 0;
+        }
+    }
+    if (valid && asleep) {
+        // ForceUpdater and BoundingBoxUpdater skip inactive entities
+        // (ForceUpdater.cs:75, BoundingBoxUpdater.cs:48): the box stays.
+        const float4 mn = w.bmin[i], mx = w.bmax[i];
+        cx = 0.5f * (mn.x + mx.x);
+        cy = 0.5f * (mn.y + mx.y);
+        cz = 0.5f * (mn.z + mx.z);
+        side = fmaxf(fmaxf(mx.x - mn.x, mx.y - mn.y), mx.z - mn.z);
+        if (!(side >= 0.0f && side < 3.0e38f)) side = 0.0f;
+    } else if (valid) {
         bool dyn = (w.bflags[i] & BF_DYNAMIC) != 0;
         float4 P = w.pos[i];
         float4 Q = w.quat[i];
@@ -60,31 +83,50 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
         float4 W = w.angv[i];
         f3 v = f3_of(V), av = f3_of(W);
         q4 q{Q.x, Q.y, Q.z, Q.w};
+        if (!dyn && !w.sleeping) {
+            // a kinematic member's candidacy (the deactivation stage does it
+            // when it runs)
+            const uint32_t bf = w.bflags[i];
+            const float fl = w.stab[i].y;
+            const float next = kinematic_flag_next(fl, lensq3(v) + lensq3(av), (bf & BF_ALWAYS_ACTIVE) != 0);
+            if (next != fl) w.stab[i].y = next;
+            const uint32_t nf = next <= 0.0f ? (bf | BF_KIN_ACTIVE) : (bf & ~(uint32_t)BF_KIN_ACTIVE);
+            if (nf != bf) w.bflags[i] = nf;
+        }
         if (dyn) {
             float4 st = w.stab[i];
             float lim = s.velocity_lower_limit;
-            // candidacy; clearing the candidate flag also zeroes the time
-            // below the limit (the IsDeactivationCandidate setter), which an
-            // always-active body does every step
-            float velocity = lensq3(v) + lensq3(av);
-            float slowing = velocity <= st.x ? 1.0f : 0.0f;
-            float tb = st.y;
-            if (velocity < lim * lim) tb += s.dt;
-            else tb = 0;
-            const uint32_t bf = w.bflags[i];
-            uint32_t cand = bf & BF_CANDIDATE;
-            if (!(bf & BF_ALWAYS_ACTIVE)) {
-                if (!cand) {
-                    if (tb > s.low_velocity_time_minimum && slowing != 0.0f) cand = BF_CANDIDATE;
-                } else if (tb <= s.low_velocity_time_minimum) {
+            float velocity, slowing, tb;
+            if (w.sleeping) {
+                // candidacy ran in the deactivation stage
+                velocity = st.x;
+                tb = st.y;
+                slowing = st.z;
+            } else {
+                // candidacy; clearing the candidate flag also zeroes the time
+                // below the limit (the IsDeactivationCandidate setter), which an
+                // always-active body does every step
+                velocity = lensq3(v) + lensq3(av);
+                slowing = velocity <= st.x ? 1.0f : 0.0f;
+                tb = st.y;
+                if (velocity < lim * lim) tb += s.dt;
+                else tb = 0;
+                const uint32_t bf = w.bflags[i];
+                uint32_t cand = bf & BF_CANDIDATE;
+                if (!(bf & BF_ALWAYS_ACTIVE)) {
+                    if (!cand) {
+                        if (tb > s.low_velocity_time_minimum && slowing != 0.0f) cand = BF_CANDIDATE;
+                    } else if (tb <= s.low_velocity_time_minimum) {
+                        cand = 0;
+                        tb = 0;
+                    }
+                } else {
                     cand = 0;
                     tb = 0;
                 }
-            } else {
-                cand = 0;
-                tb = 0;
+                const uint32_t nf = ((bf & ~(uint32_t)(BF_CANDIDATE | BF_WAKE)) | cand);
+                if (nf != bf) w.bflags[i] = nf;
             }
-            if (cand != (bf & BF_CANDIDATE)) w.bflags[i] = (bf & ~BF_CANDIDATE) | cand;
             // force update: gravity, stabilization boost, damping, world inertia
             f3 gdt = f3{s.gravity[0], s.gravity[1], s.gravity[2]} * s.dt;
             v = gdt + v;
@@ -267,9 +309,13 @@ __global__ void __launch_bounds__(256) k_bounds_reduce(DeviceWorld w, uint32_t n
 }
 
 // Entity.PreUpdatePosition (EntityBase.cs:1124-1155), discrete mode.
+// PositionUpdater: active entities only (ContinuousPositionUpdater.cs:71).
 __global__ void __launch_bounds__(256) k_integrate(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w.nbodies || !(w.bflags[i] & BF_PRESENT)) return;
+    if (i >= w.nbodies) return;
+    const uint32_t bf = w.bflags[i];
+    if (!(bf & BF_PRESENT) || (bf & BF_ASLEEP)) return;
+    if (!(bf & BF_DYNAMIC) && w.stab[i].y > 0.0f) return;
     float dt = w.s.dt;
     float4 Q = w.quat[i];
     float4 P = w.pos[i];
@@ -314,20 +360,33 @@ __global__ void __launch_bounds__(256) k_step_reset(DeviceWorld w) {
         w.gt_key[i] = ~0ull;
         w.gt_head[i] = 0xffffffffu;
     }
-    if (i < CTR_COUNT && i != CTR_STICKY) {
+    if (i < CTR_COUNT && i != CTR_STICKY && i != CTR_PAIRS_PREV) {
         // The previous step's overflow and barrier abort stay visible in the
-        // sticky word; the thread that clears a flag folds it in.
+        // sticky word; the thread that clears a flag folds it in.  The pair
+        // count is kept as the last step's.
         uint32_t v = w.counters[ctr(i)];
         if (v && i == CTR_OVERFLOW) atomicOr(&w.counters[ctr(CTR_STICKY)], STICKY_OVERFLOW);
         if (v && i == CTR_GBAR_ABORT) atomicOr(&w.counters[ctr(CTR_STICKY)], STICKY_GBAR_ABORT);
+        if (i == CTR_PAIRS) w.counters[ctr(CTR_PAIRS_PREV)] = v;
         w.counters[ctr(i)] = 0;
     }
     if (i < w.nworlds) w.wlevels[i] = 0;
     if (i < 16 * w.nworlds) w.wlev_count[i] = 0;
+    if (w.sleeping) {
+        // per-island sums of the deactivation stage
+        if (i < w.nbodies) {
+            w.lab_act[i] = 0u;
+            w.lab_count[i] = 0ull;
+            w.lab_min[i] = 0xffffffffu;
+            w.lab_flags[i] = 0u;
+        }
+        if (i < w.nworlds) w.wsleepy[i] = 0u;
+    }
 }
 
 void launch_step_reset(const DeviceWorld& w, hipStream_t st) {
     uint32_t n = std::max<uint32_t>(std::max<uint32_t>(w.hash_cap, w.gt_cap), std::max<uint32_t>(CTR_COUNT, 16 * w.nworlds));
+    n = std::max<uint32_t>(n, w.nbodies);
     hipLaunchKernelGGL(k_step_reset, dim3((n + 255) / 256), dim3(256), 0, st, w);
 }
 
@@ -351,6 +410,7 @@ __global__ void __launch_bounds__(256) k_scatter_bodies(DeviceWorld w, const Bod
     w.mat[b] = r.mat;
     w.damp[b] = r.damp;
     w.bflags[b] = r.flags;
+    w.comp_root[b] = b;  // its own island until its first step
     w.wid[b] = r.wid;
     w.wbase[b] = r.wbase;
     w.born[b] = r.born;
@@ -375,6 +435,18 @@ __global__ void __launch_bounds__(256) k_set_states(DeviceWorld w, const StateRe
     if (r.fields & BEPU_STATE_ORIENTATION) w.quat[b] = r.quat;
     if (r.fields & BEPU_STATE_LINEAR_VELOCITY) w.linv[b] = r.linv;
     if (r.fields & BEPU_STATE_ANGULAR_VELOCITY) w.angv[b] = r.angv;
+    // Every setter activates the entity (SimulationIslandMember.Activate,
+    // SimulationIslandMember.cs:256-277): a dynamic body's candidate flag and
+    // time below the limit clear and its island is active at the next step;
+    // a kinematic body's flag becomes -1.
+    const uint32_t f = w.bflags[b];
+    if (f & BF_DYNAMIC) {
+        w.bflags[b] = (f & ~(uint32_t)BF_CANDIDATE) | BF_WAKE;
+        w.stab[b].y = 0.0f;
+    } else if (f & BF_PRESENT) {
+        w.stab[b].y = -1.0f;
+        w.bflags[b] = f | BF_KIN_ACTIVE;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_remove_bodies(DeviceWorld w, const uint32_t* ids, uint32_t n) {

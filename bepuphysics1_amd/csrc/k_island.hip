@@ -47,20 +47,32 @@ __device__ __forceinline__ int island_tier(uint32_t nc, uint32_t nb, uint32_t sm
 // Islands per bundle for a small island (class index = result - 1).
 __device__ __forceinline__ uint32_t pack_class(uint32_t nc) { return min(64u, 64u / nc); }
 
+// The solver's bodies and constraints: dynamic bodies of active islands, and
+// pairs with contacts that were not removed, of active islands
+// (Solver.UpdateSolverActivity, SolverUpdateable.cs:202-218).
+__device__ __forceinline__ bool solver_body(uint32_t f) { return (f & BF_DYNAMIC) && !(f & BF_ASLEEP); }
+__device__ __forceinline__ bool solver_constraint(const DeviceWorld& w, const PairState& s) {
+    if (s.ccount <= 0 || (s.flags & (PF_DEAD | PF_VOID))) return false;
+    return !((w.bflags[s.a] | w.bflags[s.b]) & BF_ASLEEP);
+}
+
 // Island of every dynamic body and every constraint (a pair with contacts);
 // counts both per island in one packed word (constraints high, bodies low).
+// Without deactivation the islands are also the labels of the next step's
+// deactivation stage.
 __global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
-    bool body = i < w.nbodies && (w.bflags[i] & 1u);
+    bool body = i < w.nbodies && solver_body(w.bflags[i]);
     uint32_t rb = body ? uf_find(w.uf_parent, i) : 0;
     if (body) w.isl_root[i] = rb;
+    if (body && !w.sleeping) w.comp_root[i] = rb;
     (void)wave_atomic_add<unsigned long long>((unsigned long long*)w.isl_count, rb, body, 1ull);
     bool cons = false;
     uint32_t rc = 0;
     if (i < npairs) {
         const PairState& s = w.pairs_cur[i];
-        cons = s.ccount > 0;
+        cons = solver_constraint(w, s);
         if (cons) {
             rc = uf_find(w.uf_parent, (w.bflags[s.a] & 1u) ? s.a : s.b);
             w.cons_root[i] = rc;
@@ -85,7 +97,7 @@ __global__ void __launch_bounds__(kFillBlock) k_isl_fill(DeviceWorld w) {
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     unsigned long long* cnt = (unsigned long long*)w.isl_count;
 
-    bool cons = i < npairs && w.pairs_cur[i].ccount > 0;
+    bool cons = i < npairs && solver_constraint(w, w.pairs_cur[i]);
     uint32_t rc = cons ? w.cons_root[i] : 0;
     unsigned long long oc = wave_atomic_add<unsigned long long>(cnt, rc, cons, 0ull - kConsOne);
     bool global = false;
@@ -96,7 +108,7 @@ __global__ void __launch_bounds__(kFillBlock) k_isl_fill(DeviceWorld w) {
     }
     block_push<kFillBlock>(w.gcons, &w.counters[ctr(CTR_GCONS)], global, i, s_tmp);
 
-    bool body = i < w.nbodies && (w.bflags[i] & 1u);
+    bool body = i < w.nbodies && solver_body(w.bflags[i]);
     uint32_t rb = body ? w.isl_root[i] : 0;
     unsigned long long ob = wave_atomic_add<unsigned long long>(cnt, rb, body, ~0ull);
     bool medium = false, small = false;

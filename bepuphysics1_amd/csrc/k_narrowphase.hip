@@ -614,31 +614,6 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
     s.static_f = ma.x * mb.x;
 }
 
-__device__ __forceinline__ uint32_t hash_slot(uint64_t key, uint32_t cap) { return (uint32_t)mix64(key) & (cap - 1); }
-
-__device__ int hash_find(const uint64_t* keys, const uint32_t* vals, uint32_t cap, uint64_t key) {
-    uint32_t h = hash_slot(key, cap);
-    for (uint32_t probe = 0; probe < cap; ++probe) {
-        uint64_t k = keys[h];
-        if (k == key) return (int)vals[h];
-        if (k == ~0ull) return -1;
-        h = (h + 1) & (cap - 1);
-    }
-    return -1;
-}
-
-__device__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64_t key, uint32_t val) {
-    uint32_t h = hash_slot(key, cap);
-    for (uint32_t probe = 0; probe < cap; ++probe) {
-        unsigned long long prev = atomicCAS((unsigned long long*)&keys[h], ~0ull, (unsigned long long)key);
-        if (prev == ~0ull || prev == key) {
-            vals[h] = val;
-            return true;
-        }
-        h = (h + 1) & (cap - 1);
-    }
-    return false;
-}
 
 // Counter of general convex work list q = 2 * tester state + (two hulls ? 0 : 1).
 __device__ __forceinline__ uint32_t convex_list_counter(uint32_t q) {
@@ -652,12 +627,27 @@ __device__ __forceinline__ uint32_t convex_list_counter(uint32_t q) {
 // (0 for a new pair).  A new pair's state is set up in slot t here; k_pair_prep
 // copies a carried-over one.  The pair enters this step's table in
 // k_narrowphase.
-__device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t, int& tester) {
-    // A body inserted for this step (possibly into ids a removed world used)
-    // has no history: never pick up the previous step's state for its pairs.
+// Last step's slot of the pair with this key, or -1: none, a body inserted
+// for this step (possibly into ids a removed world used) has no history, and
+// a slot that was no pair (stale or void) is no history either.
+__device__ __forceinline__ int prior_pair(const DeviceWorld& w, uint64_t key) {
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
     bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
     int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
+    if (old >= 0 && (w.pairs_old[old].flags & (PF_DEAD | PF_VOID))) old = -1;
+    return old;
+}
+
+// An overlap of two inactive bodies with no pair to keep: a slot with no state.
+__device__ __forceinline__ void void_pair(const DeviceWorld& w, uint64_t key, uint32_t t) {
+    PairState s;
+    init_pair(w, key, s);
+    s.flags |= PF_VOID;
+    w.pairs_cur[t] = s;
+}
+
+__device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t, int& tester) {
+    int old = prior_pair(w, key);
     tester = 0;
     if (old >= 0) tester = w.pairs_old[old].tester;
     else init_pair(w, key, w.pairs_cur[t]);
@@ -688,10 +678,20 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
     // box-box, 7 for the other closed-form pairs.
     int my_q = -1;
     uint32_t old_slot = ~0u;
+    bool frozen = false;
+    uint64_t key = 0;
     if (t < npairs) {
-        uint64_t key = w.pair_keys[t];
+        key = w.pair_keys[t];
         const uint32_t fi = w.bflags[(uint32_t)(key >> 32)], fj = w.bflags[(uint32_t)key];
-        if ((fi | fj) & BF_GENERAL) {
+        if (body_inactive(fi) && body_inactive(fj)) {
+            // The broad phase reports no overlap of two inactive entries
+            // (BroadPhase.cs:97-102): last step's pair is kept as it was (not
+            // updated, NarrowPhase.cs:300-333), and none is created.
+            frozen = true;
+            const int old = prior_pair(w, key);
+            if (old >= 0) old_slot = (uint32_t)old;
+            else void_pair(w, key, t);
+        } else if ((fi | fj) & BF_GENERAL) {
             int cstate;
             int old = begin_pair(w, key, t, cstate);
             if (cstate != 1 && cstate != 2) cstate = 0;
@@ -708,6 +708,8 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
         // consecutive 16-byte chunks, instead of each lane reading its own
         // pair's 336 bytes 16 bytes at a time.
         static_assert(sizeof(PairState) == 21 * 16 && alignof(PairState) == 16, "PairState is copied as 21 float4s");
+        constexpr int kFlagsChunk = 1;  // PairState::flags is the w of the second float4
+        static_assert(offsetof(PairState, flags) == 16 * kFlagsChunk + 12, "PairState::flags position");
         const int part = lane / 21, chunk = lane % 21;  // lane 63 (part 3) idles
         uint64_t todo = __ballot(old_slot != ~0u);
         while (todo) {
@@ -720,9 +722,16 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
             const int src_lane = part == 0 ? l0 : (part == 1 ? l1 : (part == 2 ? l2 : -1));
             const uint32_t src = __shfl(old_slot, src_lane < 0 ? 0 : src_lane);
             const uint32_t dst = __shfl(t, src_lane < 0 ? 0 : src_lane);
-            if (src_lane >= 0)
-                reinterpret_cast<float4*>(&w.pairs_cur[dst])[chunk] =
-                    reinterpret_cast<const float4*>(&w.pairs_old[src])[chunk];
+            const bool src_frozen = __shfl((int)frozen, src_lane < 0 ? 0 : src_lane) != 0;
+            if (src_lane >= 0) {
+                float4 v = reinterpret_cast<const float4*>(&w.pairs_old[src])[chunk];
+                if (chunk == kFlagsChunk) {
+                    // this step's activity bits of the pair
+                    uint32_t fl = __float_as_uint(v.w) & ~(uint32_t)(PF_FROZEN | PF_DEAD | PF_VOID);
+                    v.w = __uint_as_float(fl | (src_frozen ? (uint32_t)PF_FROZEN : 0u));
+                }
+                reinterpret_cast<float4*>(&w.pairs_cur[dst])[chunk] = v;
+            }
         }
         // Their tester and supplement states the same way, six (10 float4
         // each) per pass.
@@ -743,6 +752,12 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
                 reinterpret_cast<float4*>(&w.conv_cur[dst])[cchunk] =
                     reinterpret_cast<const float4*>(&w.conv_old[src])[cchunk];
         }
+    }
+    if (frozen && old_slot != ~0u) {
+        // A kept pair enters this step's table and, with contacts, its island.
+        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
+        const PairState& o = w.pairs_old[old_slot];
+        if (o.ccount > 0 && (w.bflags[o.a] & 1u) && (w.bflags[o.b] & 1u)) uf_union(w.uf_parent, o.a, o.b);
     }
     uint32_t my_off = 0;
     for (int q = 0; q < 8; ++q) {
@@ -777,17 +792,21 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
 template <int BLOCK>
 __device__ __forceinline__ void narrowphase_pair(DeviceWorld& w, uint32_t t, float4* lds_pd, int* lds_id) {
     uint64_t key = w.pair_keys[t];
-    // A body inserted for this step (possibly into ids a removed world used)
-    // has no history: never pick up the previous step's state for its pairs.
-    uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
-    bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
-    int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
+    int old = prior_pair(w, key);
+    const bool frozen = body_inactive(w.bflags[(uint32_t)(key >> 32)]) && body_inactive(w.bflags[(uint32_t)key]);
+    if (frozen && old < 0) {
+        void_pair(w, key, t);
+        return;
+    }
     // The state lives in registers from the load to the store (every contact
     // slot index is resolved through selects).
     PairState s;
     if (old >= 0) s = w.pairs_old[old];
     else init_pair(w, key, s);
-    if (s.kind == BEPU_PAIR_BOX_BOX) {
+    s.flags &= ~(uint32_t)(PF_FROZEN | PF_DEAD | PF_VOID);
+    if (frozen) {
+        s.flags |= PF_FROZEN;  // kept as it was: both bodies inactive
+    } else if (s.kind == BEPU_PAIR_BOX_BOX) {
         LdsList m{&lds_pd[threadIdx.x], &lds_id[threadIdx.x], 0, BLOCK};
         update_box_pair(w, s, m);
     } else {
@@ -845,7 +864,7 @@ __global__ void __launch_bounds__(256) k_stats(DeviceWorld w, uint32_t npairs) {
     float mp = 0;
     if (t < npairs) {
         const PairState& s = w.pairs_old[t];
-        if (s.ccount > 0) {
+        if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID))) {
             cons = 1;
             contacts = (uint32_t)s.ccount;
             for (int i = 0; i < s.ccount; ++i) mp = fmaxf(mp, s.cpos[s.clist[i]].w);

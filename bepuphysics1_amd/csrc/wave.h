@@ -48,7 +48,8 @@ __device__ __forceinline__ void block_push(uint32_t* list, uint32_t* counter, bo
 // Adds `inc` to counter[key] for every valid lane and returns the value each
 // lane would have seen from its own atomicAdd.  Lanes sharing the first valid
 // lane's key are combined into a single atomic (a large island's bodies and
-// constraints all hit one counter).  Call from converged control flow.
+// constraints all hit one counter), so every valid lane must pass the same
+// `inc`.  Call from converged control flow.
 template <typename T>
 __device__ __forceinline__ T wave_atomic_add(T* counter, uint32_t key, bool valid, T inc) {
     uint64_t vmask = __ballot(valid);

@@ -154,6 +154,8 @@ bepu_status bepu_world_destroy(bepu_world w) {
     if (w->id_stage) (void)hipFree(w->id_stage);
     if (w->state_stage) (void)hipFree(w->state_stage);
     if (w->sstage) (void)hipFree(w->sstage);
+    if (w->ranges_host) (void)hipHostFree(w->ranges_host);
+    if (w->ranges_done) (void)hipEventDestroy(w->ranges_done);
     for (auto& slot : w->insert_ring) {
         if (slot.host) (void)hipHostFree(slot.host);
         if (slot.done) (void)hipEventDestroy(slot.done);
@@ -247,6 +249,11 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     A(&d.color_count, kMaxColors); A(&d.scan_part, 4096); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
     A(&d.ordered, P);
     A(&d.grow_c, P); A(&d.grow_v, P); A(&d.grow_pair, P);
+    // deactivation: per body (labels are body ids), per sub-world
+    A(&d.comp_root, n); A(&d.lab_act, n); A(&d.lab_count, n); A(&d.lab_min, n); A(&d.lab_flags, n);
+    A(&d.act2, n); A(&d.uf2, n); A(&d.deact_m, (size_t)n + 1); A(&d.deact_e, (size_t)n + 1);
+    A(&d.wrange, 3 * ((size_t)n + 2)); A(&d.wcursor, (size_t)n + 2); A(&d.wcurgen, (size_t)n + 2);
+    A(&d.wkey, (size_t)n + 2); A(&d.wsleepy, (size_t)n + 2);
     if (st != BEPU_OK) {
         bepu_world_destroy(w);
         return st;
@@ -259,7 +266,7 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
         return BEPU_E_DEVICE;
     }
     w->sort_bytes = broadphase_sort_temp_bytes(n);
-    w->scan_bytes = island_scan_temp_bytes(n + 1);
+    w->scan_bytes = std::max(island_scan_temp_bytes(n + 1), deactivation_scan_temp_bytes(n + 1));
     if (hipMalloc(&w->sort_temp, std::max<size_t>(w->sort_bytes, 16)) != hipSuccess ||
         hipMalloc(&w->scan_temp, std::max<size_t>(w->scan_bytes, 16)) != hipSuccess) {
         bepu_world_destroy(w);
@@ -269,6 +276,16 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     w->allocations.push_back(w->scan_temp);
     launch_fill_u64(d.hash_keys_old, ~0ull, d.hash_cap, w->stream);
     launch_fill_u64(d.hash_keys_cur, ~0ull, d.hash_cap, w->stream);
+    launch_fill_u32(d.wkey, 0u, (size_t)n + 2, w->stream);
+    launch_fill_u32(d.wcurgen, 0u, (size_t)n + 2, w->stream);
+    launch_fill_u32(d.wcursor, kNoCursor, (size_t)n + 2, w->stream);
+    launch_fill_u32(d.counters, 0u, (size_t)CTR_COUNT * kCtrStride, w->stream);
+    if (hipHostMalloc((void**)&w->ranges_host, 3 * ((size_t)n + 2) * sizeof(uint32_t), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&w->ranges_done, hipEventDisableTiming) != hipSuccess) {
+        bepu_world_destroy(w);
+        return BEPU_E_OUT_OF_MEMORY;
+    }
     if (hipStreamSynchronize(w->stream) != hipSuccess) {
         bepu_world_destroy(w);
         return BEPU_E_DEVICE;
@@ -320,14 +337,18 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
     if (shape.w == (float)BEPU_SHAPE_CAPSULE || shape.w == (float)BEPU_SHAPE_CONVEX_HULL) w->convex_bodies = true;
     if (shape.w == (float)BEPU_SHAPE_BOX) w->box_bodies = true;
     if (shape.w == (float)BEPU_SHAPE_SPHERE) w->sphere_bodies = true;
-    b.stab = make_float4(0, 0, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
+    // A kinematic body's time below the limit is its activity flag, -1 when
+    // added (DeactivationManager.Add activates the member).
+    b.stab = make_float4(0, dyn ? 0.0f : -1.0f, 0, (flags & BEPU_FLAG_NO_STABILIZATION) ? 0.0f : 1.0f);
     b.mat = m ? make_float4(m->static_friction, m->kinetic_friction, m->bounciness, 0) : make_float4(1.0f, 0.8f, 0.0f, 0);
     b.damp = make_float2(w->d.s.linear_damping, w->d.s.angular_damping);
     const bool hull = shape.w == (float)BEPU_SHAPE_CONVEX_HULL;
     const bool general = hull || shape.w == (float)BEPU_SHAPE_CAPSULE;
     b.flags = (dyn ? BF_DYNAMIC : 0u) | BF_PRESENT | (general ? BF_GENERAL : 0u) | (hull ? BF_HULL : 0u) |
               (shape.w == (float)BEPU_SHAPE_SPHERE ? BF_SPHERE : 0u) |
-              ((flags & BEPU_FLAG_ALWAYS_ACTIVE) ? BF_ALWAYS_ACTIVE : 0u);
+              ((flags & BEPU_FLAG_ALWAYS_ACTIVE) ? BF_ALWAYS_ACTIVE : 0u) | (dyn ? 0u : (uint32_t)BF_KIN_ACTIVE);
+    // A dynamic body that may sleep turns the deactivation stages on.
+    if (dyn && !(flags & BEPU_FLAG_ALWAYS_ACTIVE)) w->d.sleeping = 1;
     b.born = w->steps;
     return BEPU_OK;
 }
@@ -344,6 +365,7 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 sha
     b.wbase = w->world_first[w->cur_world];
     w->staged.push_back(b);
     w->world_count[w->cur_world]++;
+    w->ranges_dirty = true;
     if (w->absent.size() < (size_t)n + 1) w->absent.resize((size_t)n + 1, 0);
     w->absent[n] = 0;
     if (id) *id = n;
@@ -408,17 +430,45 @@ static bepu_status upload(bepu_world w) {
 }
 
 static uint32_t new_world_id(bepu_world w) {
+    uint32_t id;
     if (!w->free_world_ids.empty()) {
-        uint32_t id = w->free_world_ids.back();
+        id = w->free_world_ids.back();
         w->free_world_ids.pop_back();
-        return id;
+    } else {
+        // The per-world device arrays hold body_cap + 2 worlds; the grid key holds 20 bits.
+        if (w->world_first.size() >= std::min<uint64_t>(1u << 20, (uint64_t)w->d.body_cap + 2)) return ~0u;
+        w->world_first.push_back(0);
+        w->world_count.push_back(0);
+        w->world_live.push_back(0);
+        w->world_gen.push_back(0);
+        id = (uint32_t)w->world_first.size() - 1;
     }
-    // The per-world device arrays hold body_cap + 2 worlds; the grid key holds 20 bits.
-    if (w->world_first.size() >= std::min<uint64_t>(1u << 20, (uint64_t)w->d.body_cap + 2)) return ~0u;
-    w->world_first.push_back(0);
-    w->world_count.push_back(0);
-    w->world_live.push_back(0);
-    return (uint32_t)w->world_first.size() - 1;
+    // A new sub-world in this id: its deactivation turn starts afresh.
+    w->world_gen[id] = w->next_gen++;
+    w->ranges_dirty = true;
+    return id;
+}
+
+// Copies (first, count, generation) of every sub-world to the device when a
+// range changed since the last copy (through a pinned buffer; the previous
+// copy from it has finished by then, or is waited for).
+static bepu_status push_world_ranges(bepu_world w) {
+    if (!w->ranges_dirty) return BEPU_OK;
+    if (w->ranges_pending && hipEventSynchronize(w->ranges_done) != hipSuccess) return BEPU_E_DEVICE;
+    w->ranges_pending = false;
+    const size_t nw = w->world_first.size();
+    for (size_t k = 0; k < nw; ++k) {
+        w->ranges_host[3 * k] = w->world_first[k];
+        w->ranges_host[3 * k + 1] = w->world_live[k] ? w->world_count[k] : 0u;
+        w->ranges_host[3 * k + 2] = w->world_gen[k];
+    }
+    if (hipMemcpyAsync(w->d.wrange, w->ranges_host, 3 * nw * sizeof(uint32_t), hipMemcpyHostToDevice, w->stream) !=
+            hipSuccess ||
+        hipEventRecord(w->ranges_done, w->stream) != hipSuccess)
+        return BEPU_E_DEVICE;
+    w->ranges_pending = true;
+    w->ranges_dirty = false;
+    return BEPU_OK;
 }
 
 extern "C" {
@@ -652,6 +702,7 @@ bepu_status bepu_remove_world(bepu_world w, uint32_t world_id) {
     if (n) launch_fill_u32(w->d.bflags + first, 0u, n, w->stream);  // absent: no box, no pairs, not dynamic
     w->world_live[world_id] = 0;
     w->world_count[world_id] = 0;
+    w->ranges_dirty = true;
     if (n) {
         // Merge with free neighbours.
         auto next = w->free_ranges.lower_bound(first);
@@ -880,6 +931,8 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (w->sticky & (STICKY_OVERFLOW | STICKY_GBAR_ABORT)) return sticky_status(w->sticky);
     bepu_status st = upload(w);
     if (st != BEPU_OK) return st;
+    st = push_world_ranges(w);
+    if (st != BEPU_OK) return st;
     DeviceWorld& d = w->d;
     hipStream_t s = w->stream;
     // Stage timers: every step of this call when timing is on, none otherwise.
@@ -898,6 +951,8 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         d.step_no = w->steps;
         launch_step_reset(d, s);
         if (e) (void)hipEventRecord(e[0], s);
+        // DeactivationManager.Update, then the force update and bounding boxes.
+        launch_deactivation(d, w->scan_temp, w->scan_bytes, s);
         launch_body_pre(d, s);
         if (e) (void)hipEventRecord(e[1], s);
         launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
@@ -925,6 +980,9 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[3], s);
         if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
         if (e) (void)hipEventRecord(e[4], s);
+        // Islands woken by this step's new constraints, stale frozen pairs,
+        // split activation.
+        launch_activity(d, s);
         launch_islands(d, w->scan_temp, w->scan_bytes, s);
         if (e) (void)hipEventRecord(e[5], s);
         launch_island_solve(d, s);
@@ -1012,6 +1070,21 @@ bepu_status bepu_get_pairs(bepu_world w, uint64_t* keys, uint32_t cap, uint32_t*
     if (read_counters(w, c) != BEPU_OK) return BEPU_E_DEVICE;
     if (w->sticky) return sticky_status(w->sticky);
     uint32_t n = std::min(c[CTR_PAIRS], w->d.pair_cap);
+    if (w->d.sleeping && w->steps) {
+        // Overlaps of two inactive bodies with no pair, and pairs removed as
+        // stale, are not pairs.
+        std::vector<PairState> ps(n);
+        if (n && hipMemcpy(ps.data(), w->d.pairs_old, n * sizeof(PairState), hipMemcpyDeviceToHost) != hipSuccess)
+            return BEPU_E_DEVICE;
+        std::vector<uint64_t> live;
+        for (const PairState& s : ps)
+            if (!(s.flags & (PF_DEAD | PF_VOID))) live.push_back(s.key);
+        std::sort(live.begin(), live.end());
+        if (count) *count = (uint32_t)live.size();
+        if (!keys || cap < live.size()) return live.empty() ? BEPU_OK : BEPU_E_CAPACITY;
+        std::copy(live.begin(), live.end(), keys);
+        return BEPU_OK;
+    }
     if (count) *count = n;
     if (!keys || cap < n) return n == 0 ? BEPU_OK : BEPU_E_CAPACITY;
     if (n && hipMemcpy(keys, w->d.pair_keys, n * sizeof(uint64_t), hipMemcpyDeviceToHost) != hipSuccess)
@@ -1027,6 +1100,9 @@ static bepu_status read_pairs(bepu_world w, std::vector<PairState>& ps, uint32_t
     ps.resize(n);
     if (n && hipMemcpy(ps.data(), w->d.pairs_old, n * sizeof(PairState), hipMemcpyDeviceToHost) != hipSuccess)
         return BEPU_E_DEVICE;
+    // stale and void slots are not pairs
+    ps.erase(std::remove_if(ps.begin(), ps.end(), [](const PairState& s) { return (s.flags & (PF_DEAD | PF_VOID)) != 0; }),
+             ps.end());
     return BEPU_OK;
 }
 
@@ -1136,6 +1212,9 @@ bepu_status bepu_remove(bepu_world w, uint32_t id) {
         w->staged[id - w->uploaded].flags = 0u;  // uploaded absent
         return BEPU_OK;
     }
+    // The dynamic bodies it had constraints with are activated
+    // (DeactivationManager.RemoveSimulationIslandFromMember).
+    launch_removal_wake(w->d, id, w->stream);
     launch_fill_u32(w->d.bflags + id, 0u, 1, w->stream);
     return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
 }
@@ -1145,6 +1224,8 @@ bepu_status bepu_remove(bepu_world w, uint32_t id) {
 // EntityBase.cs:68-84), LinearVelocity, AngularVelocity; all four is the
 // MotionState setter (Entities/EntityBase.cs:218-235).  Takes effect at the
 // next step; pair caches, warm-start impulses and stabilization state stay.
+static bepu_status upload_state_records(bepu_world w, std::vector<StateRecord>& recs);
+
 bepu_status bepu_set_motion_states(bepu_world w, const uint32_t* ids, const bepu_motion_state* states, uint32_t n,
                                    uint32_t fields) {
     if (!w || (n && (!ids || !states)) || fields == 0 || (fields & ~BEPU_STATE_ALL)) return BEPU_E_INVALID_ARG;
@@ -1193,6 +1274,12 @@ bepu_status bepu_set_motion_states(bepu_world w, const uint32_t* ids, const bepu
         r.fields = fields;
         recs.push_back(r);
     }
+    return upload_state_records(w, recs);
+}
+
+// State records to the device, applied in stream order (the records activate
+// their bodies; fields 0 writes no state).
+static bepu_status upload_state_records(bepu_world w, std::vector<StateRecord>& recs) {
     if (recs.empty()) return BEPU_OK;
     // Queued steps may still read the staging buffer of an earlier call.
     if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
@@ -1209,6 +1296,46 @@ bepu_status bepu_set_motion_states(bepu_world w, const uint32_t* ids, const bepu
         return BEPU_E_DEVICE;
     launch_set_states(w->d, w->sstage, k, w->stream);
     return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+// Entity.ActivityInformation.Activate() (SimulationIslandMember.cs:256-277).
+bepu_status bepu_activate(bepu_world w, const uint32_t* ids, uint32_t n) {
+    if (!w || (n && !ids)) return BEPU_E_INVALID_ARG;
+    uint32_t total = w->uploaded + (uint32_t)w->staged.size();
+    for (uint32_t k = 0; k < n; ++k)
+        if (ids[k] >= total || w->absent[ids[k]]) return BEPU_E_NOT_FOUND;
+    std::vector<StateRecord> recs;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ids[k] >= w->uploaded) continue;  // not yet stepped: active already
+        StateRecord r{};
+        r.target = ids[k];
+        r.fields = 0;
+        recs.push_back(r);
+    }
+    return upload_state_records(w, recs);
+}
+
+// Entity.ActivityInformation.IsActive of every body: a dynamic body whose
+// island is awake, a kinematic body whose flag is <= 0; 0 for a removed body.
+bepu_status bepu_get_activity(bepu_world w, uint8_t* out, uint32_t cap, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
+    bepu_status err = poll_errors(w);
+    if (err == BEPU_E_DEVICE && !(w->sticky & STICKY_GBAR_ABORT)) return err;
+    const uint32_t n = w->d.nbodies;
+    if (count) *count = n;
+    if (!out) return BEPU_OK;  // count query
+    if (cap < n) return BEPU_E_CAPACITY;
+    std::vector<uint32_t> f(n);
+    std::vector<float4> st(n);
+    if (n && (hipMemcpy(f.data(), w->d.bflags, n * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(st.data(), w->d.stab, n * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess))
+        return BEPU_E_DEVICE;
+    for (uint32_t i = 0; i < n; ++i) {
+        const bool present = (f[i] & BF_PRESENT) != 0;
+        out[i] = !present ? 0 : ((f[i] & BF_DYNAMIC) ? !(f[i] & BF_ASLEEP) : st[i].y <= 0.0f);
+    }
+    return err;
 }
 
 bepu_status bepu_get_settings(bepu_world w, bepu_settings* out) {

@@ -48,7 +48,12 @@ struct PairState {
     float pad1;
 };
 
-enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u };
+// PF_FROZEN: not updated this step (both bodies inactive when the broad phase
+// ran; the state is last step's); PF_DEAD: a frozen pair whose island woke in
+// this step, stale and removed (the next step creates it again); PF_VOID: an
+// overlap of two inactive bodies with no pair to keep (no state).  Dead and
+// void slots are not pairs: no constraint, no readback, no history.
+enum { PF_PREV_COLLIDING = 1u, PF_FRICTION_ACTIVE = 2u, PF_FROZEN = 256u, PF_DEAD = 512u, PF_VOID = 1024u };
 
 // State of a general convex pair (a pair with a hull or a capsule) that the
 // PairState does not hold: the GeneralConvexPairTester's state machine
@@ -64,9 +69,13 @@ struct ConvexState {
 // BF_GENERAL: a shape whose pairs take the general convex tester (hull or
 // capsule); BF_HULL: a convex hull; BF_SPHERE: a sphere.
 // BF_ALWAYS_ACTIVE: SimulationIslandMember.IsAlwaysActive; BF_CANDIDATE: its
-// IsDeactivationCandidate.
+// IsDeactivationCandidate.  Activity: BF_ASLEEP, a dynamic body whose island
+// sleeps; BF_KIN_ACTIVE, a kinematic body active this step (its flag, the
+// time below the limit in stab.y, is <= 0); BF_NODEACT, the island may not
+// deactivate at its next check (an active kinematic body touched it);
+// BF_WAKE, activated since the last step (its island is active at the next).
 enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u, BF_ALWAYS_ACTIVE = 32u,
-       BF_CANDIDATE = 64u };
+       BF_CANDIDATE = 64u, BF_ASLEEP = 128u, BF_KIN_ACTIVE = 256u, BF_NODEACT = 512u, BF_WAKE = 1024u };
 
 // One body as built on the host (Entity construction), copied to the device
 // in one block and scattered into the structure-of-arrays by k_scatter_bodies.
@@ -179,7 +188,70 @@ struct DeviceWorld {
     RowConst* grow_c;          // rows in solve order
     RowVar* grow_v;
     uint32_t* grow_pair;
+    // deactivation (sleeping); the stages run only when `sleeping` (a dynamic
+    // body that is not always active was added)
+    uint32_t sleeping;
+    uint32_t* comp_root;       // per dynamic body: its island at the end of the last step (label = a body id)
+    uint32_t* lab_act;         // per label: an island member was activated (zeroed each step)
+    uint64_t* lab_count;       // per label: candidates << 32 | members
+    uint32_t* lab_min;         // per label: smallest present member (the island's turn position)
+    uint32_t* lab_flags;       // per label: LF_* bits
+    uint32_t* act2;            // per root of this step's union-find: the island is active
+    uint32_t* uf2;             // union-find of the constraints left after stale pairs are removed
+    uint32_t* deact_m;         // per body: its island's members if it is the island's smallest member (n + 1)
+    uint32_t* deact_e;         // exclusive scan of deact_m (n + 1)
+    uint32_t* wrange;          // per sub-world: first body, body count, generation (host-written)
+    uint32_t* wcursor;         // per sub-world: last island looked at by the rotation
+    uint32_t* wcurgen;         // per sub-world: generation the cursor belongs to
+    uint32_t* wkey;            // per sub-world: furthest island looked at this step (turn position + 1)
+    uint32_t* wsleepy;         // per sub-world: holds a dynamic body that is not always active (this step)
 };
+
+// Key -> pair slot table (open addressing, linear probing; empty key ~0).
+__device__ __forceinline__ uint32_t hash_slot(uint64_t key, uint32_t cap) { return (uint32_t)mix64(key) & (cap - 1); }
+
+__device__ __forceinline__ int hash_find(const uint64_t* keys, const uint32_t* vals, uint32_t cap, uint64_t key) {
+    uint32_t h = hash_slot(key, cap);
+    for (uint32_t probe = 0; probe < cap; ++probe) {
+        uint64_t k = keys[h];
+        if (k == key) return (int)vals[h];
+        if (k == ~0ull) return -1;
+        h = (h + 1) & (cap - 1);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ bool hash_insert(uint64_t* keys, uint32_t* vals, uint32_t cap, uint64_t key, uint32_t val) {
+    uint32_t h = hash_slot(key, cap);
+    for (uint32_t probe = 0; probe < cap; ++probe) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&keys[h], ~0ull, (unsigned long long)key);
+        if (prev == ~0ull || prev == key) {
+            vals[h] = val;
+            return true;
+        }
+        h = (h + 1) & (cap - 1);
+    }
+    return false;
+}
+
+// Label flags of the deactivation stage.
+enum { LF_ACTIVE = 1u, LF_NODEACT = 2u, LF_CHECKED = 4u, LF_SLEEP = 8u };
+constexpr uint32_t kNoCursor = 0xffffffffu;
+
+// Whether a body is inactive for the broad and narrow phases (IsActive false):
+// a sleeping dynamic body or a kinematic body whose flag is above 0.
+__host__ __device__ __forceinline__ bool body_inactive(uint32_t f) {
+    return (f & BF_DYNAMIC) ? (f & BF_ASLEEP) != 0 : (f & BF_KIN_ACTIVE) == 0;
+}
+
+// A kinematic member's flag after its candidacy update
+// (SimulationIslandMember.cs:96-139): inactive (> 0) stays; at zero velocity
+// -1 -> 0 -> 1; moving (or always active) -1.
+__host__ __device__ __forceinline__ float kinematic_flag_next(float flag, float velocity, bool always_active) {
+    if (flag > 0.0f) return flag;
+    if (velocity == 0.0f && !always_active) return flag == 0.0f ? 1.0f : 0.0f;
+    return -1.0f;
+}
 
 constexpr int kMaxColors = 128;
 constexpr uint32_t kLevelList = 4;
@@ -208,7 +280,9 @@ enum {
     CTR_NP_BOX = 148,     // closed-form pairs listed for k_narrowphase: box-box
     CTR_NP_SINGLE = 149,  // and the others
     CTR_STICKY = 150,     // STICKY_* bits of every step since the last bepu_clear_error (never reset by a step)
-    CTR_COUNT = 151
+    CTR_PAIRS_PREV = 151, // the last step's pair count (copied by the step reset)
+    CTR_DEAD = 152,       // frozen pairs removed as stale this step
+    CTR_COUNT = 153
 };
 
 // Sticky error bits: the start of each step folds the previous step's
@@ -267,5 +341,13 @@ static_assert(sizeof(StateRecord) == 80, "StateRecord layout");
 void launch_set_states(const DeviceWorld& w, const StateRecord* recs, uint32_t n, hipStream_t st);
 // Marks bodies absent (Space.Remove): no AABB, no pairs, not dynamic from the next step on.
 void launch_remove_bodies(const DeviceWorld& w, const uint32_t* ids, uint32_t n, hipStream_t st);
+// Deactivation: the stage at the start of a step (candidacy, island turn),
+// the activity pass after the narrow phase (merges wake islands, stale frozen
+// pairs, split activation), and the activation of the bodies a removed body
+// had constraints with.
+void launch_deactivation(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st);
+void launch_activity(const DeviceWorld& w, hipStream_t st);
+void launch_removal_wake(const DeviceWorld& w, uint32_t id, hipStream_t st);
+size_t deactivation_scan_temp_bytes(uint32_t n);
 
 }  // namespace bepu

@@ -38,6 +38,15 @@ struct bepu_world_s {
     // Sub-worlds: world k holds bodies [world_first[k], world_first[k] + world_count[k]).
     std::vector<uint32_t> world_first{0}, world_count{0};
     std::vector<uint8_t> world_live{1};
+    // Generation of each world id (a new one at every (re)use), and the copy
+    // of (first, count, generation) per world the deactivation stage reads:
+    // refreshed through a pinned buffer before a step when a range changed.
+    std::vector<uint32_t> world_gen{1};
+    uint32_t next_gen = 2;
+    bool ranges_dirty = true;
+    uint32_t* ranges_host = nullptr;
+    hipEvent_t ranges_done = nullptr;
+    bool ranges_pending = false;
     uint32_t cur_world = 0;                  // world the add calls append to
     std::vector<uint32_t> free_world_ids;
     std::map<uint32_t, uint32_t> free_ranges;  // body ranges of removed worlds: first -> count

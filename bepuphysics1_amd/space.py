@@ -349,6 +349,21 @@ class Space:
                                                int(fields)), "set_motion_states")
         self._ms_cache = None
 
+    def activity(self):
+        """Entity.ActivityInformation.IsActive of every body (bool array)."""
+        n = ctypes.c_uint32()
+        check(self._lib.bepu_get_activity(self._handle(), None, 0, ctypes.byref(n)), "get_activity")
+        out = np.zeros(max(n.value, 1), np.uint8)
+        check(self._lib.bepu_get_activity(self._handle(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                          n.value, ctypes.byref(n)), "get_activity")
+        return out[:n.value].astype(bool)
+
+    def activate(self, ids):
+        """Entity.ActivityInformation.Activate() for each listed body."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        check(self._lib.bepu_activate(self._handle(), ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(ids)),
+              "activate")
+
     def faulted_worlds(self):
         """Sub-worlds evicted by the non-finite guard (still live), sorted."""
         n = ctypes.c_uint32()

@@ -310,6 +310,23 @@ bepu_status bepu_clear_error(bepu_world world);
 bepu_status bepu_get_faulted_worlds(bepu_world world, uint32_t* world_ids, uint32_t capacity,
                                     uint32_t* count_out);
 
+/* Deactivation (sleeping), DeactivationManagement/DeactivationManager.cs:
+ * a body that is not BEPU_FLAG_ALWAYS_ACTIVE becomes a deactivation
+ * candidate after low_velocity_time_minimum below velocity_lower_limit, and
+ * its simulation island (the bodies its constraints connect) sleeps once
+ * every member is a candidate, checked for up to max_deactivation_attempts
+ * island members per step and sub-world, in turn.  A sleeping island keeps
+ * its state and its pairs and is not stepped until it is activated: by a new
+ * constraint with an awake body, by an active kinematic body touching it, by
+ * the removal of a body it touches, by a state setter or by bepu_activate.
+ *
+ * EntityBase.ActivityInformation.IsActive (SimulationIslandMember.cs:227-250)
+ * of every body: 1 active, 0 asleep, an inactive kinematic body or removed. */
+bepu_status bepu_get_activity(bepu_world world, uint8_t* active_out, uint32_t capacity, uint32_t* count_out);
+/* EntityBase.ActivityInformation.Activate() (SimulationIslandMember.cs:256-277)
+ * for the listed bodies: their islands are active at the next step. */
+bepu_status bepu_activate(bepu_world world, const uint32_t* ids, uint32_t n);
+
 /* Space.Remove(entity) -- Space.cs:364-482: the body leaves every stage from
  * the next step on, and its pairs and constraints end with it.  An id that is
  * unknown or already removed returns BEPU_E_NOT_FOUND (the reference's
