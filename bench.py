@@ -252,6 +252,7 @@ def batched_run(a):
     counts = counts_of(st, BATCHED_ITERATIONS)
     lat = latency(sp, a.latency_steps)
     sp.close()
+    ref_defaults = reference_defaults_run(a, worlds, n_dyn, tune)
     # Stage times: HIP events between the stages of the same steps as the
     # timed window (W + 1 .. W + K), on a replay of the batch (every step is
     # deterministic, so the replay's steps are the timed steps bit for bit),
@@ -269,7 +270,36 @@ def batched_run(a):
     rsp.close()
     return worlds, {
         "value": n_dyn * a.steps / elapsed, "ms_per_step": 1e3 * elapsed / a.steps, "bodies": n_dyn,
-        "stats": st, "counts": counts, "stages": stages, "latency": lat, "tuning": tune, "settled": settled}
+        "stats": st, "counts": counts, "stages": stages, "latency": lat, "tuning": tune, "settled": settled,
+        "reference_defaults": ref_defaults}
+
+
+def reference_defaults_run(a, worlds, n_dyn, tune):
+    """The same batch under the reference's defaults: no body is always
+    active, so a settled island sleeps (DeactivationManager) and is not
+    stepped until something wakes it.  Timed over the same window as the
+    headline (steps W + 1 .. W + K); a body-step counts every body of the
+    batch each step, asleep or not, as a Space.Update over the world does."""
+    from bepuphysics1_amd import Space, scenes
+    sp = Space(capacity=sum(len(w) for w in worlds), settings=batched_settings())
+    sp.register_hulls(scenes.hull_library())
+    for w in worlds:
+        w2 = w.copy()
+        w2["flags"] = 0
+        sp.add_world(w2)
+    sp.set_tuning(**{k: tune[k] for k in Space.TUNING_FIELDS})
+    if a.warmup:
+        sp.Update(a.warmup)
+    sp.stats()
+    t0 = time.perf_counter()
+    sp.Update(a.steps)
+    st = sp.stats()
+    dt = time.perf_counter() - t0
+    act = sp.activity()
+    dyn = np.concatenate([w["mass"] > 0 for w in worlds])
+    sp.close()
+    return {"value": round(n_dyn * a.steps / dt, 1), "ms_per_step": round(1e3 * dt / a.steps, 4),
+            "asleep_at_end": round(float((~act[dyn]).mean()), 4), "manifolds_at_end": st.manifolds}
 
 
 def timed_steps(sp, steps, n_dyn):
@@ -734,6 +764,7 @@ def main():
                    "autotune": b["tuning"]},
         "roofline": roofline(stages, counts, "batched", (a.steps, a.warmup)),
         "settled": b["settled"],
+        "reference_defaults": b["reference_defaults"],
         "step_roofline": {"alg_bytes_per_step": int(step_alg),
                           "achieved_GBps": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9, 2),
                           "frac": round(step_alg / (b["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)},

@@ -59,12 +59,15 @@ __device__ __forceinline__ bool solver_constraint(const DeviceWorld& w, const Pa
 // Island of every dynamic body and every constraint (a pair with contacts);
 // counts both per island in one packed word (constraints high, bodies low).
 // Without deactivation the islands are also the labels of the next step's
-// deactivation stage.
+// deactivation stage; with it, the labels already are the components of the
+// constraints the solver takes (the union-find forest also joined the pairs
+// removed as stale, which would leave islands the solver's tiers take for
+// unconnected).
 __global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     bool body = i < w.nbodies && solver_body(w.bflags[i]);
-    uint32_t rb = body ? uf_find(w.uf_parent, i) : 0;
+    uint32_t rb = body ? (w.sleeping ? w.comp_root[i] : uf_find(w.uf_parent, i)) : 0;
     if (body) w.isl_root[i] = rb;
     if (body && !w.sleeping) w.comp_root[i] = rb;
     (void)wave_atomic_add<unsigned long long>((unsigned long long*)w.isl_count, rb, body, 1ull);
@@ -74,7 +77,8 @@ __global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
         const PairState& s = w.pairs_cur[i];
         cons = solver_constraint(w, s);
         if (cons) {
-            rc = uf_find(w.uf_parent, (w.bflags[s.a] & 1u) ? s.a : s.b);
+            const uint32_t d = (w.bflags[s.a] & 1u) ? s.a : s.b;
+            rc = w.sleeping ? w.comp_root[d] : uf_find(w.uf_parent, d);
             w.cons_root[i] = rc;
         }
     }

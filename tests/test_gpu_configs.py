@@ -8,7 +8,8 @@ against the oracle (run on an MI355X):
       impulses and motion states exact;
   C3  64 x 64 towers of 20 boxes (81,920 bodies), 10 iterations: the first
       steps exact at full size, then all 600 steps: the deepest contact
-      bounded and the towers standing counted.
+      bounded and the towers standing counted; and under the reference's
+      defaults (sleeping) all 600 steps exact, standing and sleeping reported.
 """
 import concurrent.futures
 import ctypes
@@ -158,3 +159,49 @@ def test_towers_c3_full_size_600_steps():
           % (standing, drift.max(), np.median(drift), deepest))
     assert deepest < 0.1
     assert np.isfinite(ms).all()
+
+
+@pytest.mark.timeout(600)
+def test_towers_c3_full_size_with_sleeping_exact():
+    """C3 under the reference's defaults (no body always active, so towers
+    go to sleep once settled): all 81,920 boxes for 600 steps, motion states
+    and activity bit for bit against the oracle at steps 100 and 600; the
+    deepest contact, the towers standing and the bodies asleep reported.
+    The deactivation budget looks at 100 island members per step, 5 towers,
+    so a full turn over the 4,096 towers takes 820 steps: a tower not yet put
+    to sleep leans as an always-active one does (and one that fell sleeps
+    once its boxes rest), so standing is reported, not required."""
+    sc = scenes.towers()
+    sc["flags"] = 0
+    g = run_gpu(sc, 0)
+    o = run_oracle(sc, 0)
+    o.set_threads(_threads())
+    deepest = 0.0
+    done = 0
+    for c in (100, 300, 600):
+        while done < c:
+            k = min(100, c - done)
+            g.Update(k)
+            deepest = max(deepest, g.stats().max_penetration)
+            done += k
+        o.step(c - o.stats().steps)
+        if c in (100, 600):
+            assert np.array_equal(f32bits(g.motion_states()), f32bits(o.motion_states())), "step %d" % c
+            assert np.array_equal(g.activity(), o.activity()), "step %d activity" % c
+    o.close()
+    ms = g.motion_states()
+    act = g.activity()
+    dyn = sc["mass"] > 0
+    start = sc["state"]
+    ys = start[:, 1]
+    top = np.zeros(len(sc), bool)
+    top[dyn] = ys[dyn] == ys[dyn].max()
+    drift = np.hypot(ms[top, 0] - start[top, 0], ms[top, 2] - start[top, 2])
+    standing = int(((drift < 0.5) & (np.abs(ms[top, 1] - start[top, 1]) < 1.0)).sum())
+    asleep = int((~act[dyn]).sum())
+    up = (drift < 0.5) & (np.abs(ms[top, 1] - start[top, 1]) < 1.0)
+    top_asleep = ~act[top]
+    print("C3 with sleeping, 600 steps: %d of 4096 towers standing, %d of them asleep, %d of %d boxes asleep, "
+          "deepest %.4f" % (standing, int(top_asleep.sum()), asleep, int(dyn.sum()), deepest))
+    assert deepest < 0.1
+    assert top_asleep.sum() > 1000
