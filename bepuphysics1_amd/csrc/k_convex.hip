@@ -265,6 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
             sb = shape_of(w, s.b, w.hull_verts + __float_as_uint(hb.y), 1);
         }
         update_general(w, sa, sb, s, e);
+        w.hot_cur[wk.x] = hot_of(s);
         // A constraint joins the islands of its two bodies when both are dynamic.
         if (s.ccount > 0 && (w.bflags[s.a] & 1u) && (w.bflags[s.b] & 1u)) uf_union(w.uf_parent, s.a, s.b);
     }

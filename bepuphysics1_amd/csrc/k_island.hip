@@ -51,7 +51,7 @@ __device__ __forceinline__ uint32_t pack_class(uint32_t nc) { return min(64u, 64
 // pairs with contacts that were not removed, of active islands
 // (Solver.UpdateSolverActivity, SolverUpdateable.cs:202-218).
 __device__ __forceinline__ bool solver_body(uint32_t f) { return (f & BF_DYNAMIC) && !(f & BF_ASLEEP); }
-__device__ __forceinline__ bool solver_constraint(const DeviceWorld& w, const PairState& s) {
+__device__ __forceinline__ bool solver_constraint(const DeviceWorld& w, const PairHot& s) {
     if (s.ccount <= 0 || (s.flags & (PF_DEAD | PF_VOID))) return false;
     return !((w.bflags[s.a] | w.bflags[s.b]) & BF_ASLEEP);
 }
@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
     bool cons = false;
     uint32_t rc = 0;
     if (i < npairs) {
-        const PairState& s = w.pairs_cur[i];
+        const PairHot s = w.hot_cur[i];
         cons = solver_constraint(w, s);
         if (cons) {
             const uint32_t d = (w.bflags[s.a] & 1u) ? s.a : s.b;
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(kFillBlock) k_isl_fill(DeviceWorld w) {
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     unsigned long long* cnt = (unsigned long long*)w.isl_count;
 
-    bool cons = i < npairs && solver_constraint(w, w.pairs_cur[i]);
+    bool cons = i < npairs && solver_constraint(w, w.hot_cur[i]);
     uint32_t rc = cons ? w.cons_root[i] : 0;
     unsigned long long oc = wave_atomic_add<unsigned long long>(cnt, rc, cons, 0ull - kConsOne);
     bool global = false;

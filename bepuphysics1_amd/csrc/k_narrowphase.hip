@@ -634,7 +634,7 @@ __device__ __forceinline__ int prior_pair(const DeviceWorld& w, uint64_t key) {
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
     bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
     int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
-    if (old >= 0 && (w.pairs_old[old].flags & (PF_DEAD | PF_VOID))) old = -1;
+    if (old >= 0 && (w.hot_old[old].flags & (PF_DEAD | PF_VOID))) old = -1;
     return old;
 }
 
@@ -644,6 +644,7 @@ __device__ __forceinline__ void void_pair(const DeviceWorld& w, uint64_t key, ui
     init_pair(w, key, s);
     s.flags |= PF_VOID;
     w.pairs_cur[t] = s;
+    w.hot_cur[t] = hot_of(s);
 }
 
 __device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, uint32_t t, int& tester) {
@@ -756,8 +757,10 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
     if (frozen && old_slot != ~0u) {
         // A kept pair enters this step's table and, with contacts, its island.
         if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
-        const PairState& o = w.pairs_old[old_slot];
+        PairHot o = w.hot_old[old_slot];
         if (o.ccount > 0 && (w.bflags[o.a] & 1u) && (w.bflags[o.b] & 1u)) uf_union(w.uf_parent, o.a, o.b);
+        o.flags = (o.flags & ~(uint32_t)(PF_DEAD | PF_VOID)) | PF_FROZEN;
+        w.hot_cur[t] = o;
     }
     uint32_t my_off = 0;
     for (int q = 0; q < 8; ++q) {
@@ -813,6 +816,7 @@ __device__ __forceinline__ void narrowphase_pair(DeviceWorld& w, uint32_t t, flo
         update_single_pair(w, s);
     }
     w.pairs_cur[t] = s;
+    w.hot_cur[t] = hot_of(s);
     if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
     // A constraint (a pair with contacts) joins the islands of its two bodies
     // when both are dynamic; the union-find forest was reset by this step's

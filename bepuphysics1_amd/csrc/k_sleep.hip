@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_deact_wake(DeviceWorld w) {
     }
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS_PREV)], w.pair_cap);
     if (i < np) {
-        const PairState& s = w.pairs_old[i];
+        const PairHot s = w.hot_old[i];
         if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID)) && carried(w, s.a) && carried(w, s.b)) {
             for (int side = 0; side < 2; ++side) {
                 const uint32_t k = side ? s.b : s.a, d = side ? s.a : s.b;
@@ -215,12 +215,16 @@ __global__ void __launch_bounds__(256) k_deact_turn(DeviceWorld w) {
         before = (uint64_t)(E[f + cnt] - E[c + 1]) + (uint64_t)(E[i] - E[f]);
         cyc = i + cnt - c;
     }
-    if ((int64_t)before >= (int64_t)w.s.max_deactivation_attempts) return;
+    const int64_t limit = w.s.max_deactivation_attempts;
+    if ((int64_t)before >= limit) return;
     const uint64_t cm = w.lab_count[L];
     uint32_t fl = w.lab_flags[L] | LF_CHECKED;
     if (!(fl & LF_NODEACT) && (fl & LF_ACTIVE) && (uint32_t)(cm >> 32) == (uint32_t)cm) fl |= LF_SLEEP;
     w.lab_flags[L] = fl;
-    atomicMax(&w.wkey[k], cyc + 1);
+    // The last island looked at (the next one is over the budget, or this is
+    // the last of the turn) moves the cursor: one writer per sub-world.
+    const uint64_t after = before + (uint32_t)cm;
+    if ((int64_t)after >= limit || after == (uint64_t)(E[f + cnt] - E[f])) w.wkey[k] = cyc + 1;
 }
 
 // After the narrow phase: an island of this step's constraints (frozen ones
@@ -248,7 +252,7 @@ __global__ void __launch_bounds__(256) k_wake_mark(DeviceWorld w) {
     }
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (i < np) {
-        const PairState& s = w.pairs_cur[i];
+        const PairHot s = w.hot_cur[i];
         if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID))) {
             const uint32_t fa = w.bflags[s.a], fb = w.bflags[s.b];
             if (!(fa & BF_DYNAMIC) && (fa & BF_KIN_ACTIVE) && (fb & BF_DYNAMIC)) w.act2[uf_find(w.uf_parent, s.b)] = 1u;
@@ -274,13 +278,14 @@ __global__ void __launch_bounds__(256) k_wake_apply(DeviceWorld w) {
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     bool dead = false;
     if (i < np) {
-        const PairState& s = w.pairs_cur[i];
+        const PairHot s = w.hot_cur[i];
         if ((s.flags & PF_FROZEN) && !(s.flags & PF_VOID)) {
             const uint32_t fa = w.bflags[s.a], fb = w.bflags[s.b];
             const bool aa = (fa & BF_DYNAMIC) ? w.act2[uf_find(w.uf_parent, s.a)] != 0u : (fa & BF_KIN_ACTIVE) != 0;
             const bool ab = (fb & BF_DYNAMIC) ? w.act2[uf_find(w.uf_parent, s.b)] != 0u : (fb & BF_KIN_ACTIVE) != 0;
             if (aa || ab) {
-                w.pairs_cur[i].flags = s.flags | PF_DEAD;
+                w.pairs_cur[i].flags |= PF_DEAD;
+                w.hot_cur[i].flags = s.flags | PF_DEAD;
                 dead = true;
             }
         }
@@ -298,7 +303,7 @@ __global__ void __launch_bounds__(256) k_live_union(DeviceWorld w) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= np) return;
-    const PairState& s = w.pairs_cur[t];
+    const PairHot s = w.hot_cur[t];
     if (s.ccount <= 0 || (s.flags & (PF_DEAD | PF_VOID))) return;
     const uint32_t fa = w.bflags[s.a], fb = w.bflags[s.b];
     if ((fa & BF_DYNAMIC) && (fb & BF_DYNAMIC) && !(fa & BF_ASLEEP)) uf_union(w.uf2, s.a, s.b);
@@ -327,16 +332,17 @@ __global__ void __launch_bounds__(256) k_split_wake(DeviceWorld w) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS_PREV)], w.pair_cap);
     if (t >= np) return;
-    const PairState& s = w.pairs_old[t];
+    const PairHot s = w.hot_old[t];
     if (s.ccount <= 0 || (s.flags & (PF_DEAD | PF_VOID))) return;
     if (!carried(w, s.a) || !carried(w, s.b)) return;
     if (!(w.bflags[s.a] & BF_DYNAMIC) || !(w.bflags[s.b] & BF_DYNAMIC)) return;
-    const int j = hash_find(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, s.key);
+    if (w.comp_root[s.a] == w.comp_root[s.b]) return;  // still connected
+    const uint64_t key = s.a < s.b ? ((uint64_t)s.a << 32 | s.b) : ((uint64_t)s.b << 32 | s.a);
+    const int j = hash_find(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key);
     if (j >= 0) {
-        const PairState& q = w.pairs_cur[j];
+        const PairHot q = w.hot_cur[j];
         if (q.ccount > 0 && !(q.flags & (PF_DEAD | PF_VOID))) return;
     }
-    if (w.comp_root[s.a] == w.comp_root[s.b]) return;
     activate_dynamic(w, s.a, 0u);
     activate_dynamic(w, s.b, 0u);
 }
@@ -348,7 +354,7 @@ __global__ void __launch_bounds__(256) k_removal_wake(DeviceWorld w, uint32_t id
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= np) return;
-    const PairState& s = w.pairs_old[t];
+    const PairHot s = w.hot_old[t];
     if (s.ccount <= 0 || (s.flags & (PF_DEAD | PF_VOID)) || (s.a != id && s.b != id)) return;
     const uint32_t o = s.a == id ? s.b : s.a;
     if (w.bflags[o] & BF_DYNAMIC) activate_dynamic(w, o, BF_WAKE);

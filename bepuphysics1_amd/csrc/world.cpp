@@ -234,7 +234,7 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     A(&d.cell_key, n); A(&d.gnext, n); A(&d.gt_key, d.gt_cap); A(&d.gt_head, d.gt_cap);
     uint32_t P = d.pair_cap;
     A(&d.pair_keys, P); A(&d.counters, (size_t)CTR_COUNT * kCtrStride);
-    A(&d.pairs_cur, P); A(&d.pairs_old, P);
+    A(&d.pairs_cur, P); A(&d.pairs_old, P); A(&d.hot_cur, P); A(&d.hot_old, P);
     // General convex pair state: 2 x 160 B + 3 x 8 B per pair slot.
     A(&d.conv_cur, P); A(&d.conv_old, P); A(&d.conv_work, 3 * (size_t)P); A(&d.np_work, P);
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
@@ -993,6 +993,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         if (e) (void)hipEventRecord(e[8], s);
         // This step's pair states and table become next step's history.
         std::swap(d.pairs_cur, d.pairs_old);
+        std::swap(d.hot_cur, d.hot_old);
         std::swap(d.conv_cur, d.conv_old);
         std::swap(d.hash_keys_cur, d.hash_keys_old);
         std::swap(d.hash_vals_cur, d.hash_vals_old);
