@@ -77,6 +77,9 @@ def lib():
         L.refcpu_mix64.argtypes = [ctypes.c_uint64]
         L.refcpu_remove.argtypes = [vp, ctypes.c_uint32]
         L.refcpu_activate.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]
+        L.refcpu_set_solve_order.argtypes = [vp, ctypes.c_int]
+        L.refcpu_permuted_order.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]
         L.refcpu_get_activity.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_uint32)]
         L.refcpu_set_motion_states.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(MotionState),
@@ -87,6 +90,13 @@ def lib():
         L.refcpu_sizeof.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
+
+
+def permuted_order(n, k, count):
+    """The permuted visit order of step k over n constraints (first `count` visits)."""
+    out = np.zeros(max(count, 1), np.int64)
+    lib().refcpu_permuted_order(int(n), int(k), int(count), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return out[:count]
 
 
 def default_settings():
@@ -181,6 +191,11 @@ class World:
         st = np.ascontiguousarray(states, np.float32).reshape(-1, 13)
         return lib().refcpu_set_motion_states(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                               st.ctypes.data_as(ctypes.POINTER(MotionState)), len(ids), int(fields))
+
+    def set_solve_order(self, mode):
+        """0: colour by colour (the GPU's order); 1: the reference's permuted
+        single-threaded order (Solver.cs:237-263, PermutationMapper.cs)."""
+        return lib().refcpu_set_solve_order(self.h, int(mode))
 
     def activate(self, ids):
         """Entity.ActivityInformation.Activate() for each listed body."""

@@ -257,6 +257,12 @@ struct refcpu_world {
     std::unique_ptr<Pool> pool;  // threads > 1
     int broadphase_mode = 0;  // 0: uniform grid, 1: brute force, 2: sort and sweep
     uint32_t deact_cursor = 0xffffffffu;  // last island looked at by the deactivation rotation (none yet)
+    // Solve order: 0 colour by colour (the GPU's); 1 the reference's
+    // single-threaded order (prestep and warm start in list order, then every
+    // iteration through one permutation of the list per step); 2 the same
+    // with the list in descending key order (another order of addition).
+    int solve_order = 0;
+    int64_t perm_index = 0;
 };
 
 namespace {
@@ -1652,6 +1658,57 @@ void solve_iteration(refcpu_world* w, Pair& p) {
 }
 
 // ---------------------------------------------------------------------------
+// The reference's single-threaded solve order (Solver.UpdateSingleThreaded,
+// Solver.cs:237-263; BEPUutilities/PermutationMapper.cs:27-127): prestep and
+// exclusive update (warm start) in list order, then iterationLimit x n visits
+// in the order (i * prime + offset) mod n, where each step advances the
+// permutation index k, prime = primes[k mod |primes|] and offset = prime * k.
+// A prime above n makes i -> i * prime mod n a bijection, so every block of n
+// visits is one fixed permutation of the list.  Two things cannot follow the
+// reference exactly: its list order (the order constraints were added, which
+// follows its broad phase's overlap order) is taken here as pair-key order,
+// and its prime table is replaced by primes of the same range generated here
+// (the smallest prime at or above 350,000,000 + 4,999,999 k, k = 0..127).
+
+int64_t perm_prime(int64_t k) {
+    static std::vector<int64_t> table;
+    static std::mutex m;
+    std::lock_guard<std::mutex> g(m);
+    if (table.empty()) {
+        for (int64_t j = 0; j < 128; ++j) {
+            int64_t c = 350000000 + 4999999 * j;
+            for (;; ++c) {
+                bool prime = c > 1 && (c % 2 != 0 || c == 2);
+                for (int64_t d = 3; prime && d * d <= c; d += 2)
+                    if (c % d == 0) prime = false;
+                if (prime) break;
+            }
+            table.push_back(c);
+        }
+    }
+    return table[(size_t)(k % (int64_t)table.size())];
+}
+
+// The visit order of one step: entry i of the iterations is list[order(i)].
+inline int64_t permuted_index(int64_t i, int64_t n, int64_t k) {
+    const int64_t prime = perm_prime(k);
+    return (i * prime + prime * k) % n;
+}
+
+void permuted_solve(refcpu_world* w, const std::vector<Pair*>& cons, bool descending) {
+    std::vector<Pair*> list(cons);
+    std::sort(list.begin(), list.end(), [&](const Pair* a, const Pair* b) {
+        const uint64_t ka = pair_key(a->a, a->b), kb = pair_key(b->a, b->b);
+        return descending ? ka > kb : ka < kb;
+    });
+    for (Pair* p : list) warm_start(w, *p);
+    ++w->perm_index;
+    const int64_t n = (int64_t)list.size();
+    const int64_t total = (int64_t)w->s.iteration_limit * n;
+    for (int64_t i = 0; i < total; ++i) solve_iteration(w, *list[(size_t)permuted_index(i, n, w->perm_index)]);
+}
+
+// ---------------------------------------------------------------------------
 // Activity after the narrow phase
 
 // A pair that gained contacts adds a constraint to the solver, and adding its
@@ -1820,8 +1877,12 @@ void step_once(refcpu_world* w) {
             });
         }
     };
-    by_colour([&](Pair& p) { warm_start(w, p); });
-    for (int it = 0; it < w->s.iteration_limit; ++it) by_colour([&](Pair& p) { solve_iteration(w, p); });
+    if (w->solve_order != 0) {
+        permuted_solve(w, cons, w->solve_order == 2);
+    } else {
+        by_colour([&](Pair& p) { warm_start(w, p); });
+        for (int it = 0; it < w->s.iteration_limit; ++it) by_colour([&](Pair& p) { solve_iteration(w, p); });
+    }
 
     // ContinuousPositionUpdater.UpdateSingleThreaded, discrete path (ContinuousPositionUpdater.cs:134-154)
     par_for(w, w->bodies.size(), 4096, [&](size_t b0, size_t b1) {
@@ -2117,6 +2178,24 @@ int refcpu_remove(refcpu_world* w, uint32_t id) {
         Body& other = w->bodies[p.a == id ? p.b : p.a];
         if (other.dynamic) activate(other);
     }
+    return BEPU_OK;
+}
+
+// Solve order of the following steps: 0 colour by colour, 1 the reference's
+// permuted single-threaded order, 2 the same over the list in descending key
+// order (test instrument: how far the colour order moves impulses and stacks
+// from the reference's kind of order, beside how far the reference's own
+// order moves with the order constraints were added).
+int refcpu_set_solve_order(refcpu_world* w, int mode) {
+    if (mode < 0 || mode > 2) return BEPU_E_INVALID_ARG;
+    w->solve_order = mode;
+    return BEPU_OK;
+}
+
+// The permuted visit order of step k over n constraints: out[i] for i < count.
+int refcpu_permuted_order(int64_t n, int64_t k, int64_t count, int64_t* out) {
+    if (n <= 0 || k < 0 || count < 0) return BEPU_E_INVALID_ARG;
+    for (int64_t i = 0; i < count; ++i) out[i] = permuted_index(i, n, k);
     return BEPU_OK;
 }
 
