@@ -252,7 +252,7 @@ def batched_run(a):
     counts = counts_of(st, BATCHED_ITERATIONS)
     lat = latency(sp, a.latency_steps)
     sp.close()
-    ref_defaults = reference_defaults_run(a, worlds, n_dyn, tune)
+    ref_defaults = None if a.no_reference_defaults else reference_defaults_run(a, worlds, n_dyn, tune)
     # Stage times: HIP events between the stages of the same steps as the
     # timed window (W + 1 .. W + K), on a replay of the batch (every step is
     # deterministic, so the replay's steps are the timed steps bit for bit),
@@ -616,12 +616,16 @@ def _window_index(rows, steps, warmup):
     return out
 
 
-def summarize_trace(directory, steps, warmup, fetch_scale, write_scale):
+def summarize_trace(directory, steps, warmup, fetch_scale, write_scale, workload="batched"):
     """Per-kernel launch times (kernel trace) and HBM bytes (FETCH_SIZE /
     WRITE_SIZE counter passes, scaled) of a rocprofv3 --output-format csv run
-    of the batched leg, over all launches and over the timed window's."""
+    of one leg, over all launches and over the timed window's, and the bytes
+    per launch `roofline` reads (profiles/traffic_<workload>.json): the
+    window's for the batched leg (its replay's steps warmup + 1 .. warmup +
+    steps), every launch's for the towers leg (a steady scene)."""
     import glob
-    out = {"window": [steps, warmup], "kernels": {}}
+    out = {"workload": workload, "window": [steps, warmup], "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "kernels": {}}
     traces = glob.glob(os.path.join(directory, "**", "*kernel_trace.csv"), recursive=True)
     counters = glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True)
     for path in traces:
@@ -659,6 +663,11 @@ def summarize_trace(directory, steps, warmup, fetch_scale, write_scale):
                 if vals:
                     k["%s_kib_%s" % (c, part)] = float(np.mean(vals))
                     k["%s_bytes_%s" % (c, part)] = float(np.mean(vals)) * 1024.0 * scale
+    part = "win" if workload == "batched" else "all"
+    out["bytes_per_launch"] = {
+        name: int(k["FETCH_SIZE_bytes_" + part] + k["WRITE_SIZE_bytes_" + part])
+        for name, k in out["kernels"].items()
+        if "FETCH_SIZE_bytes_" + part in k and "WRITE_SIZE_bytes_" + part in k}
     return out
 
 
@@ -683,6 +692,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-reference-defaults", action="store_true",
+                    help="skip the batch under the reference's defaults (sleeping)")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--nz", type=int, default=64)
@@ -701,7 +712,9 @@ def main():
     a = ap.parse_args()
     legs = set(x.strip() for x in a.legs.split(",") if x.strip())
     if a.summarize_trace:
-        print(json.dumps(summarize_trace(a.summarize_trace, a.steps, a.warmup, a.fetch_scale, a.write_scale),
+        workload = "batched" if "batched" in legs else "towers"
+        steps, warmup = (a.steps, a.warmup) if workload == "batched" else (a.tower_steps, 10)
+        print(json.dumps(summarize_trace(a.summarize_trace, steps, warmup, a.fetch_scale, a.write_scale, workload),
                          indent=1))
         return
 
