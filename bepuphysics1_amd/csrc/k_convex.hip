@@ -209,8 +209,15 @@ __device__ __forceinline__ void stage_hull(const DeviceWorld& w, uint32_t i, flo
 // of an LDS block, so that every support scan of its GJK / MPR iterations
 // reads LDS instead of making two dependent round trips to L2 (at most
 // kConvexStageVerts vertices per hull; the host checks).
+//
+// `choose` 0: run; 1 (the staging variant) or 2 (the global one): run only
+// when this step's pairs favour it.  Copying the hulls pays off when few of
+// the pairs are Shallow (resting contacts, a short closest-point test each):
+// in a settling batch the Separated and Deep pairs of the falling phase gain
+// from it, the settled piles' Shallow pairs lose.  Both variants compute the
+// same bits; both launches read the same counters, so exactly one runs.
 template <bool kStage, int kBlock>
-__global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
+__global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w, uint32_t choose) {
     __shared__ float4 stage[kStage ? 2 * kConvexStageVerts * kBlock : 1];
     // The state regions of the work list, costliest first: Deep pairs (MPR
     // with surface casts and refinement), then Shallow (closest-point GJK),
@@ -230,6 +237,11 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
     for (int q = 0; q < 6; ++q) beg[q + 1] = beg[q] + cnt[q];
     const uint32_t n = beg[6];
     if (blockIdx.x * kBlock >= n) return;  // uniform per workgroup
+    if (choose) {
+        const uint32_t shallow = cnt[2] + cnt[3];
+        const bool staging_pays = shallow < n - shallow;
+        if ((choose == 1) != staging_pays) return;  // uniform per launch
+    }
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         int q = 0;
 #pragma unroll
@@ -271,13 +283,18 @@ __global__ void __launch_bounds__(kBlock) k_convex(DeviceWorld w) {
     }
 }
 
-void launch_convex(const DeviceWorld& w, bool stage, hipStream_t st) {
+void launch_convex(const DeviceWorld& w, uint32_t lds_mode, hipStream_t st) {
     if (!w.conv_work || w.pair_cap == 0) return;
     uint32_t grid = std::min<uint32_t>(kConvexMaxGrid, (w.pair_cap + kConvexBlock - 1) / kConvexBlock);
-    if (stage && w.hull_max_verts <= kConvexStageVerts)
-        hipLaunchKernelGGL((k_convex<true, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w);
-    else
-        hipLaunchKernelGGL((k_convex<false, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w);
+    const bool can_stage = w.hull_max_verts <= kConvexStageVerts;
+    if (can_stage && lds_mode == 2) {
+        hipLaunchKernelGGL((k_convex<true, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w, 1u);
+        hipLaunchKernelGGL((k_convex<false, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w, 2u);
+    } else if (can_stage && lds_mode == 1) {
+        hipLaunchKernelGGL((k_convex<true, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w, 0u);
+    } else {
+        hipLaunchKernelGGL((k_convex<false, kConvexBlock>), dim3(grid), dim3(kConvexBlock), 0, st, w, 0u);
+    }
 }
 
 }  // namespace bepu

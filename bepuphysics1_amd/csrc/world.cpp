@@ -825,7 +825,7 @@ bepu_status bepu_set_timing(bepu_world w, int32_t enable) {
 bepu_status bepu_get_tuning(bepu_world w, bepu_tuning* out) {
     if (!w || !out) return BEPU_E_INVALID_ARG;
     out->narrowphase_block = w->np_block;
-    out->convex_lds = w->convex_lds ? 1u : 0u;
+    out->convex_lds = w->convex_lds;
     out->convex_side_stream = w->convex_side ? 1u : 0u;
     out->small_island_max = w->d.small_island_max;
     out->ms_per_step = 0;
@@ -835,10 +835,10 @@ bepu_status bepu_get_tuning(bepu_world w, bepu_tuning* out) {
 bepu_status bepu_set_tuning(bepu_world w, const bepu_tuning* t) {
     if (!w || !t) return BEPU_E_INVALID_ARG;
     if ((t->narrowphase_block != 64 && t->narrowphase_block != 128 && t->narrowphase_block != 256) ||
-        t->convex_lds > 1 || t->convex_side_stream > 1 || (t->small_island_max != 32 && t->small_island_max != 64))
+        t->convex_lds > 2 || t->convex_side_stream > 1 || (t->small_island_max != 32 && t->small_island_max != 64))
         return BEPU_E_INVALID_ARG;
     w->np_block = t->narrowphase_block;
-    w->convex_lds = t->convex_lds != 0;
+    w->convex_lds = t->convex_lds;
     w->convex_side = t->convex_side_stream != 0;
     w->d.small_island_max = t->small_island_max;
     return BEPU_OK;
@@ -860,7 +860,7 @@ bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, 
     // The general convex choices first: they move the most time.
     std::vector<Choice> choices;
     if (w->convex_bodies) {
-        if (w->d.hull_max_verts <= kConvexStageVerts) choices.push_back({&bepu_tuning::convex_lds, {0, 1}});
+        if (w->d.hull_max_verts <= kConvexStageVerts) choices.push_back({&bepu_tuning::convex_lds, {2, 0, 1}});
         choices.push_back({&bepu_tuning::convex_side_stream, {0, 1}});
     }
     choices.push_back({&bepu_tuning::narrowphase_block, {64, 128, 256}});

@@ -323,9 +323,10 @@ void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_b
 void launch_narrowphase(const DeviceWorld& w, uint32_t block, hipStream_t st);
 // General convex pairs queued by the narrowphase (GJK / MPR).
 void launch_pair_prep(const DeviceWorld& w, hipStream_t st);
-// `stage`: each pair's hull vertices are copied to LDS before its test (when
-// no hull has more than kConvexStageVerts vertices), else read from HBM.
-void launch_convex(const DeviceWorld& w, bool stage, hipStream_t st);
+// `lds_mode` 1: each pair's hull vertices are copied to LDS before its test
+// (when no hull has more than kConvexStageVerts vertices); 0: read from HBM;
+// 2: each step picks one of the two on the device from its pair counts.
+void launch_convex(const DeviceWorld& w, uint32_t lds_mode, hipStream_t st);
 constexpr uint32_t kConvexStageVerts = 16;
 // Last step's constraint, contact, colour and penetration statistics into the counters.
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);

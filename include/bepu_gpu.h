@@ -403,8 +403,9 @@ bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
  * autotuner over block size and colouring granularity of the batched form.
  *   narrowphase_block  workgroup size of the closed-form narrowphase: 64, 128 (default) or 256
  *   convex_lds         1: each general convex pair's hull vertices are copied to LDS before
- *                      its test (only when no hull has more than 16 vertices); 0 (default):
- *                      support scans read them from HBM
+ *                      its test (only when no hull has more than 16 vertices); 0: support
+ *                      scans read them from HBM; 2 (default): each step picks one of the two
+ *                      on the device (staging while Shallow pairs are the minority)
  *   convex_side_stream 1 (default): the closed-form narrowphase runs on a second stream beside
  *                      the general convex tests; 0: both in turn on the world's stream
  *   small_island_max   islands of at most this many constraints (32 or 64, default 64) are

@@ -126,7 +126,8 @@ def test_launch_configurations_give_identical_results():
     worlds = scenes.batched_worlds(24, seed=31, nmin=8, nmax=120, shapes=ALL_SHAPES)
     configs = [dict(narrowphase_block=64, convex_lds=0, convex_side_stream=0, small_island_max=32),
                dict(narrowphase_block=256, convex_lds=1, convex_side_stream=1, small_island_max=64),
-               dict(narrowphase_block=128, convex_lds=1, convex_side_stream=0, small_island_max=32)]
+               dict(narrowphase_block=128, convex_lds=1, convex_side_stream=0, small_island_max=32),
+               dict(narrowphase_block=128, convex_lds=2, convex_side_stream=1, small_island_max=64)]
     results = []
     for cfg in configs:
         sp = Space(capacity=sum(len(w) for w in worlds))
@@ -148,7 +149,9 @@ def test_launch_configurations_give_identical_results():
         sp.add_world(w)
     sp.Update(10)
     chosen = sp.autotune(trial_steps=1)
-    assert chosen["steps"] == 36 and chosen["ms_per_step"] > 0
+    # 2 rounds x (1 untimed + 1 timed step) per candidate: convex_lds 3,
+    # convex_side_stream 2, narrowphase_block 3, small_island_max 2
+    assert chosen["steps"] == 2 * 2 * (3 + 2 + 3 + 2) and chosen["ms_per_step"] > 0
     assert chosen["narrowphase_block"] in (64, 128, 256) and chosen["small_island_max"] in (32, 64)
     sp.Update(60 - 10 - chosen["steps"])
     assert np.array_equal(f32bits(sp.motion_states()), results[0][0])
