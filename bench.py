@@ -229,8 +229,9 @@ def batched_run(a):
         tsp.Update(a.autotune_after)
         if forced:
             tsp.set_tuning(**forced)
-            tsp.Update(a.autotune_steps * 18)
-            tune["steps"] = a.autotune_steps * 18
+            # as many steps as the autotuner runs with every choice open
+            tsp.Update(24 * (a.autotune_steps + 1))
+            tune["steps"] = 24 * (a.autotune_steps + 1)
         else:
             tune = tsp.autotune(a.autotune_steps)
         tune["tuned_after_steps"] = a.autotune_after

@@ -83,7 +83,7 @@ class Impulses(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("narrowphase_block", ctypes.c_uint32), ("convex_lds", ctypes.c_uint32),
                 ("convex_side_stream", ctypes.c_uint32), ("small_island_max", ctypes.c_uint32),
-                ("ms_per_step", ctypes.c_float)]
+                ("step_graph", ctypes.c_uint32), ("ms_per_step", ctypes.c_float)]
 
 
 class StepStats(ctypes.Structure):

@@ -360,7 +360,8 @@ __global__ void __launch_bounds__(256) k_step_reset(DeviceWorld w) {
         w.gt_key[i] = ~0ull;
         w.gt_head[i] = 0xffffffffu;
     }
-    if (i < CTR_COUNT && i != CTR_STICKY && i != CTR_PAIRS_PREV) {
+    if (i == CTR_STEPS) w.counters[ctr(i)] += 1u;
+    if (i < CTR_COUNT && i != CTR_STICKY && i != CTR_PAIRS_PREV && i != CTR_STEPS) {
         // The previous step's overflow and barrier abort stay visible in the
         // sticky word; the thread that clears a flag folds it in.  The pair
         // count is kept as the last step's.

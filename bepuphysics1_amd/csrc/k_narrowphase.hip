@@ -632,7 +632,8 @@ __device__ __forceinline__ uint32_t convex_list_counter(uint32_t q) {
 // a slot that was no pair (stale or void) is no history either.
 __device__ __forceinline__ int prior_pair(const DeviceWorld& w, uint64_t key) {
     uint32_t ka = (uint32_t)(key >> 32), kb = (uint32_t)key;
-    bool fresh = w.born[ka] == w.step_no || w.born[kb] == w.step_no;
+    const uint32_t step = step_index(w);
+    bool fresh = w.born[ka] == step || w.born[kb] == step;
     int old = fresh ? -1 : hash_find(w.hash_keys_old, w.hash_vals_old, w.hash_cap, key);
     if (old >= 0 && (w.hot_old[old].flags & (PF_DEAD | PF_VOID))) old = -1;
     return old;

@@ -74,7 +74,7 @@ __device__ __forceinline__ float body_velocity_sq(const DeviceWorld& w, uint32_t
 
 // A body of the last step (not one inserted for this step into a reused id).
 __device__ __forceinline__ bool carried(const DeviceWorld& w, uint32_t i) {
-    return (w.bflags[i] & BF_PRESENT) && w.born[i] != w.step_no;
+    return (w.bflags[i] & BF_PRESENT) && w.born[i] != step_index(w);
 }
 
 // Islands activated since the last step, and islands an active kinematic

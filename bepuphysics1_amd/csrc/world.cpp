@@ -169,6 +169,8 @@ bepu_status bepu_world_destroy(bepu_world w) {
         if (rb.done) (void)hipEventDestroy(rb.done);
     }
     if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
+    for (auto& g : w->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto& e : w->ev)
         if (e) (void)hipEventDestroy(e);
     if (w->side) (void)hipStreamSynchronize(w->side);
@@ -828,6 +830,7 @@ bepu_status bepu_get_tuning(bepu_world w, bepu_tuning* out) {
     out->convex_lds = w->convex_lds;
     out->convex_side_stream = w->convex_side ? 1u : 0u;
     out->small_island_max = w->d.small_island_max;
+    out->step_graph = w->step_graph ? 1u : 0u;
     out->ms_per_step = 0;
     return BEPU_OK;
 }
@@ -835,8 +838,10 @@ bepu_status bepu_get_tuning(bepu_world w, bepu_tuning* out) {
 bepu_status bepu_set_tuning(bepu_world w, const bepu_tuning* t) {
     if (!w || !t) return BEPU_E_INVALID_ARG;
     if ((t->narrowphase_block != 64 && t->narrowphase_block != 128 && t->narrowphase_block != 256) ||
-        t->convex_lds > 2 || t->convex_side_stream > 1 || (t->small_island_max != 32 && t->small_island_max != 64))
+        t->convex_lds > 2 || t->convex_side_stream > 1 || (t->small_island_max != 32 && t->small_island_max != 64) ||
+        t->step_graph > 1)
         return BEPU_E_INVALID_ARG;
+    w->step_graph = t->step_graph != 0;
     w->np_block = t->narrowphase_block;
     w->convex_lds = t->convex_lds;
     w->convex_side = t->convex_side_stream != 0;
@@ -865,6 +870,10 @@ bepu_status bepu_autotune(bepu_world w, uint32_t trial_steps, bepu_tuning* out, 
     }
     choices.push_back({&bepu_tuning::narrowphase_block, {64, 128, 256}});
     choices.push_back({&bepu_tuning::small_island_max, {32, 64}});
+    // Replaying a captured step saves the host its launches; whether the
+    // device runs the graph's launches closer together than the stream's
+    // depends on the world (on MI355X a 70-body world runs them 5% apart).
+    if (w->d.bp_kind == BEPU_BROADPHASE_GRID) choices.push_back({&bepu_tuning::step_graph, {0, 1}});
     bepu_tuning cur;
     (void)bepu_get_tuning(w, &cur);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -924,6 +933,105 @@ bepu_status bepu_set_broadphase(bepu_world w, int32_t kind) {
     return BEPU_OK;
 }
 
+// The launches of one step on the world's stream (and its side stream), with
+// stage events when `e` is given.
+static bepu_status enqueue_step(bepu_world w, hipEvent_t* e) {
+    DeviceWorld& d = w->d;
+    hipStream_t s = w->stream;
+    launch_step_reset(d, s);
+    if (e) (void)hipEventRecord(e[0], s);
+    // DeactivationManager.Update, then the force update and bounding boxes.
+    launch_deactivation(d, w->scan_temp, w->scan_bytes, s);
+    launch_body_pre(d, s);
+    if (e) (void)hipEventRecord(e[1], s);
+    launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
+    if (e) (void)hipEventRecord(e[2], s);
+    // Steps with stage events run the general convex tests after the
+    // closed-form narrowphase, so that each stage's time is its own.
+    const bool side = w->convex_bodies && w->convex_side && !e;
+    if (d.pair_lists) launch_pair_prep(d, s);
+    if (side) {
+        // Test the closed-form pairs on the side stream while the general
+        // convex pairs (the longer of the two) run here: a launch behind
+        // an event starts a few microseconds later than one in order.
+        (void)hipEventRecord(w->fork, s);
+        (void)hipStreamWaitEvent(w->side, w->fork, 0);
+        launch_narrowphase(d, w->np_block, w->side);
+        (void)hipEventRecord(w->join, w->side);
+        launch_convex(d, w->convex_lds, s);
+        (void)hipStreamWaitEvent(s, w->join, 0);
+    } else {
+        launch_narrowphase(d, w->np_block, s);
+    }
+    if (e) (void)hipEventRecord(e[3], s);
+    if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
+    if (e) (void)hipEventRecord(e[4], s);
+    // Islands woken by this step's new constraints, stale frozen pairs,
+    // split activation.
+    launch_activity(d, s);
+    launch_islands(d, w->scan_temp, w->scan_bytes, s);
+    if (e) (void)hipEventRecord(e[5], s);
+    launch_island_solve(d, s);
+    if (e) (void)hipEventRecord(e[6], s);
+    launch_global_solver(d, s);
+    if (e) (void)hipEventRecord(e[7], s);
+    launch_integrate(d, s);
+    if (e) (void)hipEventRecord(e[8], s);
+    return BEPU_OK;
+}
+
+// The host state a captured step depends on besides DeviceWorld (which the
+// launches take by value).
+static uint64_t graph_config(bepu_world w) {
+    return (uint64_t)w->np_block | ((uint64_t)w->convex_lds << 16) | ((uint64_t)w->convex_side << 24) |
+           ((uint64_t)w->convex_bodies << 25);
+}
+
+// One step through a HIP graph of its launches.  There is one graph per
+// parity of the step count (the double-buffered pair arrays swap every step,
+// and the launches take their pointers by value); a graph is captured on
+// first use and again whenever the world's device description or launch
+// configuration changed (bodies added or removed, settings, tuning), the
+// executable graph being updated in place where the launches kept their
+// shape.  The index of the step is read from device memory, so one graph
+// replays every later step of its parity.  The tree broadphase (its radix
+// sort), and a failed capture, take ordinary launches.
+static bepu_status replay_step(bepu_world w) {
+    if (!w->step_graph || w->d.bp_kind != BEPU_BROADPHASE_GRID) return enqueue_step(w, nullptr);
+    auto& g = w->graphs[w->steps & 1u];
+    const uint64_t cfg = graph_config(w);
+    if (!g.exec || g.config != cfg || std::memcmp(&g.key, &w->d, sizeof(DeviceWorld)) != 0) {
+        hipGraph_t graph = nullptr;
+        bool ok = hipStreamBeginCapture(w->stream, hipStreamCaptureModeRelaxed) == hipSuccess;
+        if (ok) {
+            const bepu_status est = enqueue_step(w, nullptr);
+            ok = hipStreamEndCapture(w->stream, &graph) == hipSuccess && graph && est == BEPU_OK;
+        }
+        if (ok && g.exec) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult why;
+            if (hipGraphExecUpdate(g.exec, graph, &bad, &why) != hipSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(g.exec);
+                g.exec = nullptr;
+            }
+        }
+        if (ok && !g.exec) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+            // Steps of this world are launched one by one from now on.
+            (void)hipGetLastError();
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            w->step_graph = false;
+            return enqueue_step(w, nullptr);
+        }
+        std::memcpy((void*)&g.key, &w->d, sizeof(DeviceWorld));
+        g.config = cfg;
+    }
+    return hipGraphLaunch(g.exec, w->stream) == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
 bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (!w || n_steps < 0) return BEPU_E_INVALID_ARG;
     // A world whose steps overflowed a capacity or aborted a grid barrier
@@ -945,52 +1053,14 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         w->ev.push_back(e);
     }
     if (timed) w->timed_steps = timed;
+    // Pairs of more than one tester are listed by tester first; a world of
+    // boxes only (or spheres only) skips that pass.
+    d.pair_lists = (w->convex_bodies || (w->box_bodies && w->sphere_bodies)) ? 1u : 0u;
     for (int k = 0; k < n_steps; ++k) {
         int slot = (int)timed - (n_steps - k);  // >= 0 for the timed steps
         hipEvent_t* e = slot >= 0 ? &w->ev[(size_t)slot * (ST_COUNT + 1)] : nullptr;
-        d.step_no = w->steps;
-        launch_step_reset(d, s);
-        if (e) (void)hipEventRecord(e[0], s);
-        // DeactivationManager.Update, then the force update and bounding boxes.
-        launch_deactivation(d, w->scan_temp, w->scan_bytes, s);
-        launch_body_pre(d, s);
-        if (e) (void)hipEventRecord(e[1], s);
-        launch_broadphase(d, w->sort_temp, w->sort_bytes, s);
-        if (e) (void)hipEventRecord(e[2], s);
-        // Steps with stage events run the general convex tests after the
-        // closed-form narrowphase, so that each stage's time is its own.
-        const bool side = w->convex_bodies && w->convex_side && !e;
-        // Pairs of more than one tester are listed by tester first; a world of
-        // boxes only (or spheres only) skips that pass.
-        d.pair_lists = (w->convex_bodies || (w->box_bodies && w->sphere_bodies)) ? 1u : 0u;
-        if (d.pair_lists) launch_pair_prep(d, s);
-        if (side) {
-            // Test the closed-form pairs on the side stream while the general
-            // convex pairs (the longer of the two) run here: a launch behind
-            // an event starts a few microseconds later than one in order.
-            (void)hipEventRecord(w->fork, s);
-            (void)hipStreamWaitEvent(w->side, w->fork, 0);
-            launch_narrowphase(d, w->np_block, w->side);
-            (void)hipEventRecord(w->join, w->side);
-            launch_convex(d, w->convex_lds, s);
-            (void)hipStreamWaitEvent(s, w->join, 0);
-        } else {
-            launch_narrowphase(d, w->np_block, s);
-        }
-        if (e) (void)hipEventRecord(e[3], s);
-        if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
-        if (e) (void)hipEventRecord(e[4], s);
-        // Islands woken by this step's new constraints, stale frozen pairs,
-        // split activation.
-        launch_activity(d, s);
-        launch_islands(d, w->scan_temp, w->scan_bytes, s);
-        if (e) (void)hipEventRecord(e[5], s);
-        launch_island_solve(d, s);
-        if (e) (void)hipEventRecord(e[6], s);
-        launch_global_solver(d, s);
-        if (e) (void)hipEventRecord(e[7], s);
-        launch_integrate(d, s);
-        if (e) (void)hipEventRecord(e[8], s);
+        st = e ? enqueue_step(w, e) : replay_step(w);
+        if (st != BEPU_OK) return st;
         // This step's pair states and table become next step's history.
         std::swap(d.pairs_cur, d.pairs_old);
         std::swap(d.hot_cur, d.hot_old);

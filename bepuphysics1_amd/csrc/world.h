@@ -136,7 +136,6 @@ struct DeviceWorld {
     uint32_t* wfault;          // per sub-world: 1 once a body's state became non-finite (the world is evicted)
     uint32_t nworlds;
     uint32_t* born;            // index of the first step a body takes part in
-    uint32_t step_no;          // index of the step being run
     // broadphase
     uint32_t* morton;          // Morton codes (tree broadphase)
     uint32_t* morton_sorted;
@@ -295,7 +294,8 @@ enum {
     CTR_STICKY = 150,     // STICKY_* bits of every step since the last bepu_clear_error (never reset by a step)
     CTR_PAIRS_PREV = 151, // the last step's pair count (copied by the step reset)
     CTR_DEAD = 152,       // frozen pairs removed as stale this step
-    CTR_COUNT = 153
+    CTR_STEPS = 153,      // steps started (the step reset adds one; never cleared)
+    CTR_COUNT = 154
 };
 
 // Sticky error bits: the start of each step folds the previous step's
@@ -308,6 +308,11 @@ enum { STICKY_OVERFLOW = 1u, STICKY_GBAR_ABORT = 2u, STICKY_NONFINITE = 4u };
 // serialise (about 10 ns each on MI355X), atomics on different lines do not.
 constexpr uint32_t kCtrStride = 64;
 __host__ __device__ constexpr uint32_t ctr(uint32_t i) { return i * kCtrStride; }
+
+// Index of the step being run (the world's step count when it started).  It
+// is kept in device memory, not passed by the host, so that the launches of
+// one step replay unchanged for any later step.
+__device__ __forceinline__ uint32_t step_index(const DeviceWorld& w) { return w.counters[ctr(CTR_STEPS)] - 1u; }
 
 // Pair key with world-local body ids: the colouring priority of a constraint
 // depends only on its own world, so a world solves identically alone or

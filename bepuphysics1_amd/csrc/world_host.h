@@ -24,6 +24,16 @@ struct bepu_world_s {
     uint32_t np_block = 128;
     uint32_t convex_lds = 2;        // 0 global hull reads, 1 LDS staging, 2 chosen per step on the device
     bool convex_side = true;
+    bool step_graph = false;        // steps replay a captured HIP graph (bepu_tuning.step_graph)
+    // One captured step per parity of the step count (the pair arrays swap
+    // every step and the launches hold their pointers), with the device
+    // description and launch configuration it was captured with.
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        bepu::DeviceWorld key{};
+        uint64_t config = 0;
+    };
+    StepGraph graphs[2];
     std::vector<HostBody> staged;   // bodies not yet uploaded
     uint32_t uploaded = 0;
     void* sort_temp = nullptr;

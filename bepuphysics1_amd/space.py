@@ -594,7 +594,7 @@ class Space:
         """Space.BroadPhase setter (Space.cs:110-134): "grid" (default) or "tree"."""
         check(self._lib.bepu_set_broadphase(self._handle(), self.BROADPHASES[kind]), "set_broadphase")
 
-    TUNING_FIELDS = ("narrowphase_block", "convex_lds", "convex_side_stream", "small_island_max")
+    TUNING_FIELDS = ("narrowphase_block", "convex_lds", "convex_side_stream", "small_island_max", "step_graph")
 
     def tuning(self):
         """The launch configuration (bepu_tuning) as a dict."""

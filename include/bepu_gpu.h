@@ -410,12 +410,18 @@ bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
  *                      the general convex tests; 0: both in turn on the world's stream
  *   small_island_max   islands of at most this many constraints (32 or 64, default 64) are
  *                      packed several to a wavefront; larger ones up to 256 take a workgroup each
+ *   step_graph         1: the launches of a step are captured once into a HIP graph (one per
+ *                      parity of the step count, captured again after the world or this
+ *                      configuration changes) and replayed, which saves the host its launches;
+ *                      0 (default): launched one by one.  Steps with stage timing and the tree
+ *                      broadphase are always launched one by one
  *   ms_per_step        (filled by bepu_autotune) the chosen configuration's time */
 typedef struct bepu_tuning {
     uint32_t narrowphase_block;
     uint32_t convex_lds;
     uint32_t convex_side_stream;
     uint32_t small_island_max;
+    uint32_t step_graph;
     float ms_per_step;
 } bepu_tuning;
 
@@ -428,7 +434,7 @@ bepu_status bepu_set_tuning(bepu_world world, const bepu_tuning* tuning);
  * interleaved rounds, timed by HIP
  * events, and the fastest is kept if it beats the current value by more than
  * 2%.  Runs 2 * (trial_steps + 1) steps of the world
- * per candidate value (18 * (trial_steps + 1) with every choice open; the convex
+ * per candidate value (24 * (trial_steps + 1) with every choice open; the convex
  * choices are open only in worlds with hulls or capsules), leaves the best
  * configuration set, reports it in `out` and the steps run in `steps_run`
  * (either may be NULL). */

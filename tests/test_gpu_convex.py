@@ -124,10 +124,10 @@ def test_launch_configurations_give_identical_results():
     the same bits, and the autotuner leaves a valid one set."""
     from bepuphysics1_amd import Space
     worlds = scenes.batched_worlds(24, seed=31, nmin=8, nmax=120, shapes=ALL_SHAPES)
-    configs = [dict(narrowphase_block=64, convex_lds=0, convex_side_stream=0, small_island_max=32),
-               dict(narrowphase_block=256, convex_lds=1, convex_side_stream=1, small_island_max=64),
-               dict(narrowphase_block=128, convex_lds=1, convex_side_stream=0, small_island_max=32),
-               dict(narrowphase_block=128, convex_lds=2, convex_side_stream=1, small_island_max=64)]
+    configs = [dict(narrowphase_block=64, convex_lds=0, convex_side_stream=0, small_island_max=32, step_graph=0),
+               dict(narrowphase_block=256, convex_lds=1, convex_side_stream=1, small_island_max=64, step_graph=1),
+               dict(narrowphase_block=128, convex_lds=1, convex_side_stream=0, small_island_max=32, step_graph=1),
+               dict(narrowphase_block=128, convex_lds=2, convex_side_stream=1, small_island_max=64, step_graph=0)]
     results = []
     for cfg in configs:
         sp = Space(capacity=sum(len(w) for w in worlds))
@@ -150,8 +150,8 @@ def test_launch_configurations_give_identical_results():
     sp.Update(10)
     chosen = sp.autotune(trial_steps=1)
     # 2 rounds x (1 untimed + 1 timed step) per candidate: convex_lds 3,
-    # convex_side_stream 2, narrowphase_block 3, small_island_max 2
-    assert chosen["steps"] == 2 * 2 * (3 + 2 + 3 + 2) and chosen["ms_per_step"] > 0
+    # convex_side_stream 2, narrowphase_block 3, small_island_max 2, step_graph 2
+    assert chosen["steps"] == 2 * 2 * (3 + 2 + 3 + 2 + 2) and chosen["ms_per_step"] > 0
     assert chosen["narrowphase_block"] in (64, 128, 256) and chosen["small_island_max"] in (32, 64)
     sp.Update(60 - 10 - chosen["steps"])
     assert np.array_equal(f32bits(sp.motion_states()), results[0][0])
