@@ -339,8 +339,8 @@ def plain_path(worlds, sample_worlds, sample_steps):
 
 
 def oracle_hull_table():
-    from bepuphysics1_amd import compute_hull, scenes
-    return [compute_hull(p)[:2] for p in scenes.hull_library()]
+    from bepuphysics1_amd import compute_hull, compute_hull_minimum_radius, scenes
+    return [compute_hull(p)[:2] + (compute_hull_minimum_radius(p),) for p in scenes.hull_library()]
 
 
 def cpu_baseline_batched(worlds, sample_worlds, warmup, steps, threads):

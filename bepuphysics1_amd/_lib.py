@@ -46,7 +46,7 @@ class Settings(ctypes.Structure):
         ("low_velocity_time_minimum", ctypes.c_float),
         ("linear_damping", ctypes.c_float),
         ("angular_damping", ctypes.c_float),
-        ("max_deactivation_attempts", ctypes.c_int32),
+        ("max_deactivation_attempts", ctypes.c_int32), ("core_shape_scaling", ctypes.c_float),
     ]
 
 
@@ -164,6 +164,9 @@ EXPORTS = {
     "bepu_get_faulted_worlds": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, _u32p]),
     "bepu_get_activity": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, _u32p]),
     "bepu_activate": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32]),
+    "bepu_set_position_update_mode": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, ctypes.c_int32]),
+    "bepu_compute_hull_minimum_radius": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
+                                                        ctypes.POINTER(ctypes.c_float)]),
     "bepu_device_available": (ctypes.c_int32, []),
     "bepu_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

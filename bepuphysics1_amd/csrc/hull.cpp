@@ -271,6 +271,25 @@ void shape_distribution(const std::vector<f3>& v, const std::vector<int>& tri, f
 
 }  // namespace
 
+// InertiaHelper.ComputeMinimumRadius (InertiaHelper.cs:529-570): the
+// smallest distance from `center` to the plane of a hull triangle (the input
+// points, the hull's triangle list), skipping degenerate triangles.  The
+// reference throws for a negative distance (a centre outside the hull), which
+// the hull of the points around its own centroid does not produce.
+static float minimum_radius(const std::vector<f3>& p, const std::vector<int>& tri, f3 center) {
+    float best = FLT_MAX;
+    for (size_t i = 0; i + 2 < tri.size(); i += 3) {
+        const f3 a = p[tri[i]], b = p[tri[i + 1]], c = p[tri[i + 2]];
+        f3 n = cross3(c - a, b - a);
+        const float ls = lensq3(n);
+        if (!(ls > 1e-10f)) continue;
+        n = divide3(n, sqrtf(ls));
+        const float d = dot3(n, a - center);
+        if (d < best) best = d;
+    }
+    return best;
+}
+
 bool compute_hull_description(const f3* points, uint32_t n, HullDescription* out) {
     std::vector<f3> p(points, points + n);
     for (const f3& q : p)
@@ -286,6 +305,7 @@ bool compute_hull_description(const f3* points, uint32_t n, HullDescription* out
     shape_distribution(p, tri, &out->center, &out->volume, &out->volume_distribution);
     if (!(out->volume > 0)) return false;
     for (f3& v : out->vertices) v = v - out->center;
+    out->minimum_radius = minimum_radius(p, tri, out->center);
     return true;
 }
 

@@ -11,6 +11,7 @@ struct HullDescription {
     m3 volume_distribution;    // about the centroid
     f3 center;                 // volume centroid of the input points' hull
     float volume;
+    float minimum_radius;      // InertiaHelper.ComputeMinimumRadius about the centroid (no margin)
 };
 
 // ConvexHullShape.ComputeDescription (ConvexHullShape.cs:154-175): the hull

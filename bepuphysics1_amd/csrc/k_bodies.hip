@@ -327,10 +327,18 @@ __global__ void __launch_bounds__(256) k_integrate(DeviceWorld w) {
     m = qmul(m, q);
     q = q4{q.x + m.x, q.y + m.y, q.z + m.z, q.w + m.w};
     q = qnormalize(q);
-    inc = v * dt;
-    f3 p = f3_of(P) + inc;
     w.quat[i] = make_float4(q.x, q.y, q.z, q.w);
-    w.pos[i] = f4_of(p, 0.0f);
+    f3 p = f3_of(P);
+    if (bf & BF_CCD_MODE) {
+        // Passive and continuous bodies move once their pairs' times of
+        // impact are known (k_ccd); until then their collidables keep the
+        // orientation of the step's start.
+        w.qold[i] = Q;
+    } else {
+        inc = v * dt;
+        p = p + inc;
+        w.pos[i] = f4_of(p, 0.0f);
+    }
     // The non-finite guard (the reference validates states with MathChecker in
     // its debug builds, BEPUutilities/MathChecker.cs:23-50): a body whose state
     // is no longer finite faults its world, which leaves from the next step on.
@@ -415,6 +423,8 @@ __global__ void __launch_bounds__(256) k_scatter_bodies(DeviceWorld w, const Bod
     w.wid[b] = r.wid;
     w.wbase[b] = r.wbase;
     w.born[b] = r.born;
+    w.minrad[b] = r.minrad;
+    w.toi[b] = 0x3f800000u;  // 1.0f
 }
 
 __global__ void __launch_bounds__(256) k_gather_states(DeviceWorld w, const uint32_t* ids, uint32_t n, float4* out) {
@@ -432,6 +442,10 @@ __global__ void __launch_bounds__(256) k_set_states(DeviceWorld w, const StateRe
     if (i >= n) return;
     const StateRecord& r = recs[i];
     uint32_t b = r.target;
+    if (r.fields & kStateUpdateMode) {
+        w.bflags[b] = (w.bflags[b] & ~BF_CCD_MODE) | r.pad[0];
+        return;
+    }
     if (r.fields & BEPU_STATE_POSITION) w.pos[b] = r.pos;
     if (r.fields & BEPU_STATE_ORIENTATION) w.quat[b] = r.quat;
     if (r.fields & BEPU_STATE_LINEAR_VELOCITY) w.linv[b] = r.linv;

@@ -599,7 +599,7 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
     s.color = -1;
     for (int k = 0; k < 6; ++k) s.sllin[k] = 0;
     s.tester = 0;
-    s.pad1 = 0;
+    s.toi = 1.0f;
     s.a = i;
     s.b = j;
     if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) s.kind = BEPU_PAIR_BOX_BOX;

@@ -123,6 +123,7 @@ void bepu_settings_default(bepu_settings* s) {
     s->linear_damping = .03f;
     s->angular_damping = .15f;
     s->max_deactivation_attempts = 100;
+    s->core_shape_scaling = .8f;
 }
 
 int32_t bepu_device_available(void) {
@@ -184,7 +185,13 @@ bepu_status bepu_world_destroy(bepu_world w) {
 
 static bool settings_valid(const bepu_settings& s) {
     return s.min_iteration_count <= 6 && s.iteration_limit >= 0 && s.dt > 0 && std::isfinite(s.dt) &&
-           std::isfinite(s.gravity[0]) && std::isfinite(s.gravity[1]) && std::isfinite(s.gravity[2]);
+           std::isfinite(s.gravity[0]) && std::isfinite(s.gravity[1]) && std::isfinite(s.gravity[2]) &&
+           !std::isnan(s.core_shape_scaling);
+}
+
+// MotionSettings.CoreShapeScaling's setter clamps to [0, 0.99] (MotionSettings.cs:22-34).
+static void clamp_settings(bepu_settings& s) {
+    s.core_shape_scaling = s.core_shape_scaling < 0.0f ? 0.0f : (s.core_shape_scaling > .99f ? .99f : s.core_shape_scaling);
 }
 
 bepu_status bepu_world_create(const bepu_settings* settings, uint32_t capacity_bodies, bepu_world* out) {
@@ -204,6 +211,7 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
         delete w;
         return BEPU_E_INVALID_ARG;
     }
+    clamp_settings(w->d.s);
     if (hipGetDevice(&w->device) != hipSuccess) {
         delete w;
         return BEPU_E_DEVICE;
@@ -256,6 +264,8 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     A(&d.act2, n); A(&d.uf2, n); A(&d.deact_m, (size_t)n + 1); A(&d.deact_e, (size_t)n + 1);
     A(&d.wrange, 3 * ((size_t)n + 2)); A(&d.wcursor, (size_t)n + 2); A(&d.wcurgen, (size_t)n + 2);
     A(&d.wkey, (size_t)n + 2); A(&d.wsleepy, (size_t)n + 2);
+    // continuous collision detection
+    A(&d.minrad, n); A(&d.toi, n); A(&d.qold, n);
     if (st != BEPU_OK) {
         bepu_world_destroy(w);
         return st;
@@ -282,6 +292,7 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     launch_fill_u32(d.wcurgen, 0u, (size_t)n + 2, w->stream);
     launch_fill_u32(d.wcursor, kNoCursor, (size_t)n + 2, w->stream);
     launch_fill_u32(d.counters, 0u, (size_t)CTR_COUNT * kCtrStride, w->stream);
+    launch_fill_u32(d.toi, 0x3f800000u, n, w->stream);  // 1.0f: no time of impact yet
     if (hipHostMalloc((void**)&w->ranges_host, 3 * ((size_t)n + 2) * sizeof(uint32_t), hipHostMallocDefault) !=
             hipSuccess ||
         hipEventCreateWithFlags(&w->ranges_done, hipEventDisableTiming) != hipSuccess) {
@@ -308,7 +319,7 @@ static bool finite_n(const float* v, int n) {
 // BecomeKinematic (EntityBase.cs:632-645, 867-905), Orientation setter
 // normalisation (EntityBase.cs:68-84).
 static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 shape, const m3& vd, float mass,
-                             const bepu_material* m, uint32_t flags, HostBody& b) {
+                             const bepu_material* m, uint32_t flags, HostBody& b, float hull_minimum_radius = 0) {
     if (!w || !s) return BEPU_E_INVALID_ARG;
     if (std::isnan(mass) || std::isinf(mass)) return BEPU_E_INVALID_ARG;
     if (!finite_n(s->position, 3) || !finite_n(s->orientation, 4) || !finite_n(s->linear_velocity, 3) ||
@@ -351,17 +362,28 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
               ((flags & BEPU_FLAG_ALWAYS_ACTIVE) ? BF_ALWAYS_ACTIVE : 0u) | (dyn ? 0u : (uint32_t)BF_KIN_ACTIVE);
     // A dynamic body that may sleep turns the deactivation stages on.
     if (dyn && !(flags & BEPU_FLAG_ALWAYS_ACTIVE)) w->d.sleeping = 1;
+    // Entity.PositionUpdateMode (EntityBase.cs:1046); Continuous wins when both are given.
+    if (flags & BEPU_FLAG_CCD_CONTINUOUS) b.flags |= BF_CONTINUOUS;
+    else if (flags & BEPU_FLAG_CCD_PASSIVE) b.flags |= BF_PASSIVE;
+    if (b.flags & BF_CCD_MODE) w->d.ccd = 1;
+    // ConvexShape.MinimumRadius: BoxShape.cs:134 (half the smallest width:
+    // the smallest half extent, exactly), SphereShape.cs:62 and
+    // CapsuleShape.cs:89 (the radius), ConvexHullShape.cs:170 (the hull's
+    // own plus the margin).
+    if (shape.w == (float)BEPU_SHAPE_BOX) b.minrad = std::min(shape.x, std::min(shape.y, shape.z));
+    else if (shape.w == (float)BEPU_SHAPE_CONVEX_HULL) b.minrad = hull_minimum_radius + shape.x;
+    else b.minrad = shape.x;
     b.born = w->steps;
     return BEPU_OK;
 }
 
 static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 shape, const m3& vd, float mass,
-                            const bepu_material* m, uint32_t flags, uint32_t* id) {
+                            const bepu_material* m, uint32_t flags, uint32_t* id, float hull_minimum_radius = 0) {
     if (!w) return BEPU_E_INVALID_ARG;
     uint32_t n = w->uploaded + (uint32_t)w->staged.size();
     if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
     HostBody b;
-    bepu_status st = make_body(w, s, shape, vd, mass, m, flags, b);
+    bepu_status st = make_body(w, s, shape, vd, mass, m, flags, b, hull_minimum_radius);
     if (st != BEPU_OK) return st;
     b.wid = w->cur_world;
     b.wbase = w->world_first[w->cur_world];
@@ -525,6 +547,14 @@ bepu_status bepu_compute_hull(const float* points, uint32_t npoints, float* vert
     return BEPU_OK;
 }
 
+bepu_status bepu_compute_hull_minimum_radius(const float* points, uint32_t npoints, float* radius) {
+    if (!points || npoints == 0 || !radius) return BEPU_E_INVALID_ARG;
+    HullDescription h;
+    if (!compute_hull_description(reinterpret_cast<const f3*>(points), npoints, &h)) return BEPU_E_INVALID_ARG;
+    *radius = h.minimum_radius;
+    return BEPU_OK;
+}
+
 bepu_status bepu_create_hull_shape(bepu_world w, const float* points, uint32_t npoints, uint32_t* shape_id,
                                    float* center) {
     if (!w || !points || npoints == 0) return BEPU_E_INVALID_ARG;
@@ -534,6 +564,7 @@ bepu_status bepu_create_hull_shape(bepu_world w, const float* points, uint32_t n
     hs.first = (uint32_t)w->hull_host.size();
     hs.count = (uint32_t)h.vertices.size();
     hs.volume_distribution = h.volume_distribution;
+    hs.minimum_radius = h.minimum_radius;
     for (const f3& v : h.vertices) w->hull_host.push_back(make_float4(v.x, v.y, v.z, 0));
     // The device copy grows by doubling; queued steps may read it, so wait for them.
     uint32_t need = (uint32_t)w->hull_host.size();
@@ -571,7 +602,7 @@ bepu_status bepu_add_convex_hull(bepu_world w, const bepu_motion_state* s, uint3
     if (!w || shape_id >= w->hull_shapes.size()) return BEPU_E_INVALID_ARG;
     const auto& hs = w->hull_shapes[shape_id];
     return add_body(w, s, hull_shape(w->d.s.default_margin, hs.first, hs.count), hs.volume_distribution, mass, m,
-                    flags, id);
+                    flags, id, hs.minimum_radius);
 }
 
 bepu_status bepu_begin_world(bepu_world w, uint32_t* world_id) {
@@ -618,7 +649,7 @@ static bepu_status body_from_desc(bepu_world w, const bepu_body_desc& b, HostBod
         if (!(k >= 0) || k != std::floor(k) || k >= (float)w->hull_shapes.size()) return BEPU_E_INVALID_ARG;
         const auto& hs = w->hull_shapes[(uint32_t)k];
         return make_body(w, &b.state, hull_shape(w->d.s.default_margin, hs.first, hs.count), hs.volume_distribution,
-                         b.mass, &b.material, b.flags, rec);
+                         b.mass, &b.material, b.flags, rec, hs.minimum_radius);
     }
     return BEPU_E_UNSUPPORTED;
 }
@@ -976,6 +1007,7 @@ static bepu_status enqueue_step(bepu_world w, hipEvent_t* e) {
     launch_global_solver(d, s);
     if (e) (void)hipEventRecord(e[7], s);
     launch_integrate(d, s);
+    if (d.ccd) launch_ccd(d, s);
     if (e) (void)hipEventRecord(e[8], s);
     return BEPU_OK;
 }
@@ -1386,6 +1418,34 @@ bepu_status bepu_activate(bepu_world w, const uint32_t* ids, uint32_t n) {
     return upload_state_records(w, recs);
 }
 
+// Entity.PositionUpdateMode setter (EntityBase.cs:1050-1068): the mode bits of
+// each listed body, from the next step on; the body is not activated.
+bepu_status bepu_set_position_update_mode(bepu_world w, const uint32_t* ids, uint32_t n, int32_t mode) {
+    if (!w || (n && !ids) || mode < BEPU_POSITION_UPDATE_DISCRETE || mode > BEPU_POSITION_UPDATE_CONTINUOUS)
+        return BEPU_E_INVALID_ARG;
+    uint32_t total = w->uploaded + (uint32_t)w->staged.size();
+    for (uint32_t k = 0; k < n; ++k)
+        if (ids[k] >= total || w->absent[ids[k]]) return BEPU_E_NOT_FOUND;
+    const uint32_t bits = mode == BEPU_POSITION_UPDATE_CONTINUOUS
+                              ? (uint32_t)BF_CONTINUOUS
+                              : (mode == BEPU_POSITION_UPDATE_PASSIVE ? (uint32_t)BF_PASSIVE : 0u);
+    if (bits && n) w->d.ccd = 1;
+    std::vector<StateRecord> recs;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ids[k] >= w->uploaded) {
+            HostBody& b = w->staged[ids[k] - w->uploaded];
+            b.flags = (b.flags & ~BF_CCD_MODE) | bits;
+            continue;
+        }
+        StateRecord r{};
+        r.target = ids[k];
+        r.fields = kStateUpdateMode;
+        r.pad[0] = bits;
+        recs.push_back(r);
+    }
+    return upload_state_records(w, recs);
+}
+
 // Entity.ActivityInformation.IsActive of every body: a dynamic body whose
 // island is awake, a kinematic body whose flag is <= 0; 0 for a removed body.
 bepu_status bepu_get_activity(bepu_world w, uint8_t* out, uint32_t cap, uint32_t* count) {
@@ -1423,6 +1483,7 @@ bepu_status bepu_get_settings(bepu_world w, bepu_settings* out) {
 bepu_status bepu_set_settings(bepu_world w, const bepu_settings* settings) {
     if (!w || !settings || !settings_valid(*settings)) return BEPU_E_INVALID_ARG;
     w->d.s = *settings;
+    clamp_settings(w->d.s);
     return BEPU_OK;
 }
 

@@ -45,7 +45,8 @@ struct PairState {
     int32_t color;        // colour of the constraint in the last solve (-1 before)
     float sllin[6];       // SlidingFrictionTwoAxis linearA (remembered across steps)
     int32_t tester;       // general convex pairs: the pair tester's state (ConvexState::sep.w) after the last test
-    float pad1;
+    float toi;            // CollidablePairHandler.timeOfImpact: 1 for a new pair, recomputed in the steps
+                          // it has a continuous body and an active one, kept otherwise
 };
 
 // PF_FROZEN: not updated this step (both bodies inactive when the broad phase
@@ -85,8 +86,12 @@ struct ConvexState {
 // time below the limit in stab.y, is <= 0); BF_NODEACT, the island may not
 // deactivate at its next check (an active kinematic body touched it);
 // BF_WAKE, activated since the last step (its island is active at the next).
+// BF_PASSIVE / BF_CONTINUOUS: PositionUpdateMode Passive / Continuous (none:
+// Discrete).
 enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u, BF_ALWAYS_ACTIVE = 32u,
-       BF_CANDIDATE = 64u, BF_ASLEEP = 128u, BF_KIN_ACTIVE = 256u, BF_NODEACT = 512u, BF_WAKE = 1024u };
+       BF_CANDIDATE = 64u, BF_ASLEEP = 128u, BF_KIN_ACTIVE = 256u, BF_NODEACT = 512u, BF_WAKE = 1024u,
+       BF_PASSIVE = 2048u, BF_CONTINUOUS = 4096u };
+constexpr uint32_t BF_CCD_MODE = BF_PASSIVE | BF_CONTINUOUS;
 
 // One body as built on the host (Entity construction), copied to the device
 // in one block and scattered into the structure-of-arrays by k_scatter_bodies.
@@ -98,7 +103,7 @@ struct BodyRecord {
     uint32_t wbase;   // its first body id
     uint32_t born;    // index of the first step the body takes part in
     uint32_t target;  // body id
-    uint32_t pad;
+    float minrad;     // the shape's MinimumRadius
 };
 static_assert(sizeof(BodyRecord) == 240, "BodyRecord layout");
 
@@ -136,6 +141,12 @@ struct DeviceWorld {
     uint32_t* wfault;          // per sub-world: 1 once a body's state became non-finite (the world is evicted)
     uint32_t nworlds;
     uint32_t* born;            // index of the first step a body takes part in
+    // continuous collision detection (run only when `ccd`: a body was ever
+    // made passive or continuous)
+    uint32_t ccd;
+    float* minrad;             // per body: the shape's MinimumRadius
+    uint32_t* toi;             // per body: smallest time of impact of its pairs this step (float bits)
+    float4* qold;              // per passive / continuous body: its orientation before this step's update
     // broadphase
     uint32_t* morton;          // Morton codes (tree broadphase)
     uint32_t* morton_sorted;
@@ -340,6 +351,10 @@ void launch_island_solve(const DeviceWorld& w, hipStream_t st);
 void launch_global_solver(const DeviceWorld& w, hipStream_t st);
 size_t island_scan_temp_bytes(uint32_t n);
 void launch_integrate(const DeviceWorld& w, hipStream_t st);
+// Continuous position update (after launch_integrate, when w.ccd): times of
+// impact of the pairs with a continuous body, then the linear motion of the
+// passive and continuous bodies clamped to their smallest time of impact.
+void launch_ccd(const DeviceWorld& w, hipStream_t st);
 size_t broadphase_sort_temp_bytes(uint32_t n);
 // Level offsets/counts of the implicit tree over n leaves (level 0 = leaves).
 void tree_layout(uint32_t n, uint32_t* levels, uint32_t* off, uint32_t* cnt);
@@ -353,9 +368,11 @@ void launch_gather_states(const DeviceWorld& w, const uint32_t* ids, uint32_t n,
 struct StateRecord {
     float4 pos, quat, linv, angv;
     uint32_t target;
-    uint32_t fields;  // BEPU_STATE_* parts to write
-    uint32_t pad[2];
+    uint32_t fields;  // BEPU_STATE_* parts to write, or kStateUpdateMode
+    uint32_t pad[2];  // kStateUpdateMode: pad[0] = the BF_CCD_MODE bits to set
 };
+// A StateRecord that sets the body's position update mode (and activates nothing).
+constexpr uint32_t kStateUpdateMode = 1u << 16;
 static_assert(sizeof(StateRecord) == 80, "StateRecord layout");
 void launch_set_states(const DeviceWorld& w, const StateRecord* recs, uint32_t n, hipStream_t st);
 // Marks bodies absent (Space.Remove): no AABB, no pairs, not dynamic from the next step on.

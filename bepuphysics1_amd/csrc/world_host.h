@@ -69,6 +69,7 @@ struct bepu_world_s {
     struct HullShape {
         uint32_t first, count;
         m3 volume_distribution;
+        float minimum_radius;  // without the margin
     };
     std::vector<HullShape> hull_shapes;
     bool convex_bodies = false;             // a capsule or hull was ever added: run k_convex

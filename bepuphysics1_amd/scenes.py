@@ -25,6 +25,8 @@ SHAPE_CONVEX_HULL = 2
 SHAPE_CAPSULE = 3
 FLAG_ALWAYS_ACTIVE = 1
 FLAG_NO_STABILIZATION = 2
+FLAG_CCD_PASSIVE = 4      # PositionUpdateMode.Passive
+FLAG_CCD_CONTINUOUS = 8   # PositionUpdateMode.Continuous
 
 BODY_DTYPE = np.dtype([
     ("shape", np.int32),
@@ -266,3 +268,40 @@ def random_pile_fast(n, rng, drop=(1.0, 10.0), sphere_fraction=0.5, hull_fractio
     body["material"] = (1.0, 0.8, 0.0)
     body["flags"] = FLAG_ALWAYS_ACTIVE
     return out
+
+
+def shooting_gallery(n=64, seed=0, speed=(120.0, 480.0), shapes=(SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL,
+                                                                  SHAPE_CAPSULE), modes=(0.6, 0.2)):
+    """Fast projectiles fired at thin walls (continuous collision detection):
+    a ground, a 0.2-thick kinematic wall across x = 0 and another at x = 12,
+    and n projectiles between them with speeds uniform in `speed` (2-8 m per
+    step at 60 Hz) along -x or +x with small sideways components, half of
+    them flying at each wall.  Projectile sizes are small next to the travel
+    per step (box half extents and radii in [0.08, 0.25]; hulls from
+    hull_library()); a fraction modes[0] is continuous, modes[1] passive and
+    the rest discrete.  Continuous projectiles stop at the walls; the
+    discrete ones tunnel."""
+    rng = np.random.default_rng(seed)
+    rows = [ground(60.0, 1.0, 60.0),
+            _row(SHAPE_BOX, (0.2, 8.0, 16.0), 0.0, (0.0, 4.0, 0.0), flags=0),
+            _row(SHAPE_BOX, (0.2, 8.0, 16.0), 0.0, (12.0, 4.0, 0.0), flags=0)]
+    q = random_quaternions(rng, n)
+    for i in range(n):
+        s = int(rng.choice(np.array(shapes)))
+        if s == SHAPE_SPHERE:
+            dims = (rng.uniform(0.08, 0.25), 0.0, 0.0)
+        elif s == SHAPE_BOX:
+            dims = tuple(2 * rng.uniform(0.08, 0.25, 3))
+        elif s == SHAPE_CAPSULE:
+            dims = (rng.uniform(0.1, 0.4), rng.uniform(0.08, 0.2), 0.0)
+        else:
+            dims = (float(rng.integers(0, HULL_LIBRARY_SIZE)), 0.0, 0.0)
+        left = i % 2 == 0
+        x = rng.uniform(2.0, 10.0)
+        v = rng.uniform(*speed) * (-1.0 if left else 1.0)
+        vel = (v, rng.uniform(-10.0, 10.0), rng.uniform(-10.0, 10.0))
+        u = rng.uniform()
+        mode = FLAG_CCD_CONTINUOUS if u < modes[0] else (FLAG_CCD_PASSIVE if u < modes[0] + modes[1] else 0)
+        rows.append(_row(s, dims, 1.0, (x, rng.uniform(1.0, 7.0), rng.uniform(-6.0, 6.0)), quat=q[i], vel=vel,
+                         angvel=tuple(rng.uniform(-5, 5, 3)), flags=FLAG_ALWAYS_ACTIVE | mode))
+    return make(rows)

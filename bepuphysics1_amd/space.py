@@ -55,6 +55,16 @@ def compute_hull(points):
     return verts[:n.value].copy(), vd, center, vol.value
 
 
+def compute_hull_minimum_radius(points):
+    """InertiaHelper.ComputeMinimumRadius of the hull of `points` about its
+    centroid (bepu_compute_hull_minimum_radius), without the margin."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    r = ctypes.c_float()
+    check(_lib.load().bepu_compute_hull_minimum_radius(p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), len(p),
+                                                       ctypes.byref(r)), "compute_hull_minimum_radius")
+    return r.value
+
+
 class Material:
     """BEPUphysics/Materials/Material.cs (defaults 1.0 / 0.8 / 0.0)."""
 
@@ -357,6 +367,16 @@ class Space:
         check(self._lib.bepu_get_activity(self._handle(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                           n.value, ctypes.byref(n)), "get_activity")
         return out[:n.value].astype(bool)
+
+    POSITION_UPDATE_MODES = {"discrete": 0, "passive": 1, "continuous": 2}
+
+    def set_position_update_mode(self, ids, mode):
+        """Entity.PositionUpdateMode for each listed body ("discrete",
+        "passive", "continuous" or 0..2); from the next step on."""
+        m = self.POSITION_UPDATE_MODES.get(mode, mode)
+        a = np.ascontiguousarray(ids, np.uint32)
+        check(self._lib.bepu_set_position_update_mode(self._handle(), a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                      len(a), int(m)), "set_position_update_mode")
 
     def activate(self, ids):
         """Entity.ActivityInformation.Activate() for each listed body."""

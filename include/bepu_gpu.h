@@ -81,6 +81,10 @@ typedef struct bepu_settings {
     int32_t max_deactivation_attempts; /* DeactivationManager.MaximumDeactivationAttemptsPerFrame (100):
                                           island members looked at per step and sub-world when
                                           deciding which islands go to sleep */
+    float core_shape_scaling;          /* MotionSettings.CoreShapeScaling (0.8; clamped to [0, 0.99]
+                                          as its setter does, Settings/MotionSettings.cs:22-34):
+                                          the continuous collision test sweeps spheres of this
+                                          fraction of each shape's minimum radius */
 } bepu_settings;
 
 /* BEPUphysics/EntityStateManagement/MotionState.cs:10-55 */
@@ -102,6 +106,8 @@ typedef struct bepu_material {
 /* Per-body flags. */
 #define BEPU_FLAG_ALWAYS_ACTIVE 1u      /* SimulationIslandMember.IsAlwaysActive */
 #define BEPU_FLAG_NO_STABILIZATION 2u   /* SimulationIslandMember.AllowStabilization = false */
+#define BEPU_FLAG_CCD_PASSIVE 4u        /* Entity.PositionUpdateMode = Passive (EntityBase.cs:1046-1068) */
+#define BEPU_FLAG_CCD_CONTINUOUS 8u     /* Entity.PositionUpdateMode = Continuous */
 
 /* Shape kinds. */
 #define BEPU_SHAPE_BOX 0
@@ -210,6 +216,12 @@ bepu_status bepu_compute_hull(const float* points, uint32_t npoints, float* vert
                               uint32_t capacity, uint32_t* nvertices_out,
                               float* volume_distribution_out /* 9, row-major */,
                               float* center_out /* 3 */, float* volume_out);
+/* InertiaHelper.ComputeMinimumRadius (CollisionShapes/ConvexShapes/InertiaHelper.cs:529-570)
+ * of the hull of `points` about its volume centroid: the distance from the centroid to the
+ * nearest hull face plane.  A hull shape's MinimumRadius is this plus its collision margin
+ * (ConvexHullShape.cs:170); continuous collision detection sweeps a fraction of it.
+ * BEPU_E_INVALID_ARG for a point set without volume. */
+bepu_status bepu_compute_hull_minimum_radius(const float* points, uint32_t npoints, float* radius_out);
 /* new ConvexHullShape(points, out center) registered with the world: many bodies
  * may share one hull shape, as entities share a ConvexHullShape instance. */
 bepu_status bepu_create_hull_shape(bepu_world world, const float* points, uint32_t npoints,
@@ -347,6 +359,25 @@ bepu_status bepu_remove(bepu_world world, uint32_t id);
 #define BEPU_STATE_ALL 15u
 bepu_status bepu_set_motion_states(bepu_world world, const uint32_t* ids, const bepu_motion_state* states,
                                    uint32_t n, uint32_t fields);
+/* Continuous collision detection, PositionUpdating/ContinuousPositionUpdater.cs:76-160.
+ * Entity.PositionUpdateMode setter (EntityBase.cs:1050-1068) for each listed body:
+ *   BEPU_POSITION_UPDATE_DISCRETE    (the default) the position moves by v dt;
+ *   BEPU_POSITION_UPDATE_PASSIVE     the position moves by v dt toi, toi the smallest time
+ *                                    of impact of the body's pairs (1 when none);
+ *   BEPU_POSITION_UPDATE_CONTINUOUS  as passive, and its pairs compute times of impact
+ *                                    (ConvexPairHandler.UpdateTimeOfImpact,
+ *                                    ConvexPairHandler.cs:37-111): a sphere of
+ *                                    core_shape_scaling x the shape's minimum radius is cast
+ *                                    along the relative motion against the other shape
+ *                                    (GJKToolbox.CCDSphereCast, GJKToolbox.cs:470-495).
+ * Orientation always integrates fully.  Takes effect at the next step and does not
+ * activate the body.  BEPU_E_NOT_FOUND for a removed or unknown id, BEPU_E_INVALID_ARG
+ * for another mode (nothing is written then). */
+#define BEPU_POSITION_UPDATE_DISCRETE 0
+#define BEPU_POSITION_UPDATE_PASSIVE 1
+#define BEPU_POSITION_UPDATE_CONTINUOUS 2
+bepu_status bepu_set_position_update_mode(bepu_world world, const uint32_t* ids, uint32_t n, int32_t mode);
+
 /* The settings block of a world, readable and writable between steps:
  * Solver.IterationLimit (Constraints/Solver.cs:19), ForceUpdater.Gravity
  * (OtherSpaceStages/ForceUpdater.cs:19-29), TimeStepSettings.TimeStepDuration

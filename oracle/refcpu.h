@@ -96,5 +96,11 @@ bool mpr_contact(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta,
 // va / vb: the entities' linear velocities (used on entering deep contact from separation).
 bool convex_candidate(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& va,
                       const V3& vb, ConvexTesterState* st, ContactData* c);
+// Casts of continuous collision detection (GJKToolbox.cs:195-495); *t is the
+// hit parameter in units of `dir`.
+bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t);
+bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t);
+bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
+                     float max_length, float scaling, float* t);
 
 }  // namespace ref

@@ -51,8 +51,12 @@ def lib():
                                          ctypes.POINTER(ctypes.c_uint32)]
         fpp = ctypes.POINTER(ctypes.c_float)
         L.refcpu_add_convex_hull.argtypes = [vp, ctypes.POINTER(MotionState), fpp, ctypes.c_uint32, fpp,
-                                             ctypes.c_float, ctypes.POINTER(Material), ctypes.c_uint32,
-                                             ctypes.POINTER(ctypes.c_uint32)]
+                                             ctypes.c_float, ctypes.c_float, ctypes.POINTER(Material),
+                                             ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_set_position_update_mode.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+                                                      ctypes.c_int]
+        L.refcpu_cast.argtypes = [ctypes.c_int, fpp, fpp, ctypes.c_int, fpp, fpp, fpp, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, fpp]
         shape_args = [fpp, fpp, ctypes.c_int, fpp, fpp, fpp, ctypes.c_int, fpp]
         L.refcpu_gjk_intersecting.argtypes = shape_args + [fpp]
         L.refcpu_gjk_closest.argtypes = shape_args + [fpp]
@@ -139,8 +143,10 @@ class World:
         lib().refcpu_set_broadphase(self.h, 1 if on else 0)
 
     def add_body(self, shape, dims, mass, state, material, flags=0, hull=None):
-        """`hull` = (surface vertices [n,3], volume distribution [3,3]) for a
-        BEPU_SHAPE_CONVEX_HULL body (as bepu_compute_hull returns them)."""
+        """`hull` = (surface vertices [n,3], volume distribution [3,3][, minimum
+        radius without the margin]) for a BEPU_SHAPE_CONVEX_HULL body (as
+        bepu_compute_hull and bepu_compute_hull_minimum_radius return them;
+        only continuous collision detection reads the radius, 0 if absent)."""
         ms = MotionState()
         ms.position[:] = [float(x) for x in state[0:3]]
         ms.orientation[:] = [float(x) for x in state[3:7]]
@@ -161,7 +167,9 @@ class World:
             verts = np.ascontiguousarray(hull[0], np.float32)
             vd = np.ascontiguousarray(hull[1], np.float32)
             rc = lib().refcpu_add_convex_hull(self.h, ctypes.byref(ms), _fptr(verts), len(verts), _fptr(vd),
-                                              float(mass), ctypes.byref(mat), int(flags), ctypes.byref(bid))
+                                              float(hull[2] if len(hull) > 2 else 0.0), float(mass),
+                                              ctypes.byref(mat), int(flags),
+                                              ctypes.byref(bid))
         else:
             raise NotImplementedError("shape %d" % shape)
         if rc != 0:
@@ -196,6 +204,13 @@ class World:
         """0: colour by colour (the GPU's order); 1: the reference's permuted
         single-threaded order (Solver.cs:237-263, PermutationMapper.cs)."""
         return lib().refcpu_set_solve_order(self.h, int(mode))
+
+    def set_position_update_mode(self, ids, mode):
+        """Entity.PositionUpdateMode for each listed body: 0 Discrete, 1
+        Passive, 2 Continuous.  Returns the status code."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        return lib().refcpu_set_position_update_mode(self.h, ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                     len(ids), int(mode))
 
     def activate(self, ids):
         """Entity.ActivityInformation.Activate() for each listed body."""
@@ -378,3 +393,17 @@ class ConvexTester:
                                             _fptr(va), _fptr(vb), ctypes.byref(self.state), _fptr(self.sep_axis),
                                             _fptr(self.local_dir), _fptr(out))
         return bool(hit), out
+
+
+def cast(kind, shape, pose, origin, direction, radius=0.0, max_length=1.0, scaling=0.8):
+    """GJKToolbox casts on one posed shape: kind "sphere" (SphereCast), "ray"
+    (RayCast) or "ccd" (CCDSphereCast); returns (hit, t)."""
+    k = {"sphere": 0, "ray": 1, "ccd": 2}[kind]
+    d, v, n = _shape_arrays(shape)
+    p = _pose(pose)
+    o = np.ascontiguousarray(origin, np.float32)
+    dr = np.ascontiguousarray(direction, np.float32)
+    t = np.zeros(1, np.float32)
+    hit = lib().refcpu_cast(k, _fptr(d), _fptr(v), n, _fptr(p), _fptr(o), _fptr(dr), float(radius),
+                            float(max_length), float(scaling), _fptr(t))
+    return bool(hit), float(t[0])

@@ -1018,6 +1018,386 @@ bool convex_candidate(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT
     }
 }
 
+// ---------------------------------------------------------------------------
+// Casts for continuous collision detection (GJK/GJKToolbox.cs:195-495) and
+// their simplex, RaySimplex (GJK/RaySimplex.cs).  The reduction functions run
+// on the simplex shifted to the current hit location (`s`, the reference's
+// `this`) and reduce the unshifted simplex (`out`) in place.
+
+namespace {
+
+struct RSimplex {
+    V3 A, B, C, D;
+    int state;
+};
+
+// RaySimplex.GetPointOnSegmentClosestToOrigin (RaySimplex.cs:76-115).  Past
+// B the reduced simplex is B, and the point returned is the shifted A, as
+// the reference returns it.
+V3 rs_segment(const RSimplex& s, RSimplex& out) {
+    V3 seg = sub(s.B, s.A);
+    float dotA = dot(seg, s.A);
+    if (dotA > 0) {
+        out.state = S_POINT;
+        return s.A;
+    }
+    float dotB = dot(seg, s.B);
+    if (dotB > 0) {
+        float V = -dotA / length_sq(seg);
+        return add(mul(seg, V), s.A);
+    }
+    out.A = out.B;
+    out.state = S_POINT;
+    return s.A;
+}
+
+// RaySimplex.GetPointOnTriangleClosestToOrigin (RaySimplex.cs:122-240).  In
+// the region of C the reduced simplex is C and the point returned is the
+// shifted A, as the reference returns it.
+V3 rs_triangle(const RSimplex& s, RSimplex& out) {
+    V3 ab = sub(s.B, s.A), ac = sub(s.C, s.A);
+    float AdotAB = -dot(ab, s.A), AdotAC = -dot(ac, s.A);
+    if (AdotAC <= 0.f && AdotAB <= 0) {
+        out.state = S_POINT;
+        return s.A;
+    }
+    float BdotAB = -dot(ab, s.B), BdotAC = -dot(ac, s.B);
+    if (BdotAB >= 0.f && BdotAC <= BdotAB) {
+        out.state = S_POINT;
+        out.A = out.B;
+        return s.B;
+    }
+    float vc = AdotAB * BdotAC - BdotAB * AdotAC;
+    if (vc <= 0 && AdotAB > 0 && BdotAB < 0) {
+        out.state = S_SEGMENT;
+        float V = AdotAB / (AdotAB - BdotAB);
+        return add(mul(ab, V), s.A);
+    }
+    float CdotAB = -dot(ab, s.C), CdotAC = -dot(ac, s.C);
+    if (CdotAC >= 0.f && CdotAB <= CdotAC) {
+        out.state = S_POINT;
+        out.A = out.C;
+        return s.A;
+    }
+    float vb = CdotAB * AdotAC - AdotAB * CdotAC;
+    if (vb <= 0.f && AdotAC > 0.f && CdotAC < 0.f) {
+        out.state = S_SEGMENT;
+        out.B = out.C;
+        float V = AdotAC / (AdotAC - CdotAC);
+        return add(mul(ac, V), s.A);
+    }
+    float va = BdotAB * CdotAC - CdotAB * BdotAC;
+    float d3d4, d6d5;
+    if (va <= 0.f && (d3d4 = BdotAC - BdotAB) > 0.f && (d6d5 = CdotAB - CdotAC) > 0.f) {
+        out.state = S_SEGMENT;
+        out.A = out.C;
+        float U = d3d4 / (d3d4 + d6d5);
+        return add(mul(sub(s.C, s.B), U), s.B);
+    }
+    float denom = 1.f / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    V3 point = mul(ab, v);
+    V3 acw = mul(ac, w);
+    point = add(s.A, point);
+    return add(point, acw);
+}
+
+// RaySimplex.TryTetrahedronTriangle (RaySimplex.cs:320-470): the face ABC
+// (shifted), its unshifted vertices sA..sC, and the fourth vertex.
+bool rs_try_face(const V3& A, const V3& B, const V3& C, const V3& sA, const V3& sB, const V3& sC, const V3& other,
+                 RSimplex* cand, V3* point) {
+    std::memset(cand, 0, sizeof(*cand));
+    *point = kZero;
+    V3 ab = sub(B, A), ac = sub(C, A);
+    V3 normal = cross(ab, ac);
+    V3 AD = sub(other, A);
+    float AdotN = dot(A, normal), ADdotN = dot(AD, normal);
+    if (!(AdotN * ADdotN >= 0)) return false;
+    float AdotAB = -dot(ab, A), AdotAC = -dot(ac, A);
+    if (AdotAC <= 0.f && AdotAB <= 0) {
+        cand->state = S_POINT;
+        cand->A = sA;
+        *point = A;
+        return true;
+    }
+    float BdotAB = -dot(ab, B), BdotAC = -dot(ac, B);
+    if (BdotAB >= 0.f && BdotAC <= BdotAB) {
+        cand->state = S_POINT;
+        cand->A = sB;
+        *point = B;
+        return true;
+    }
+    float vc = AdotAB * BdotAC - BdotAB * AdotAC;
+    if (vc <= 0 && AdotAB > 0 && BdotAB < 0) {
+        cand->state = S_SEGMENT;
+        cand->A = sA;
+        cand->B = sB;
+        float V = AdotAB / (AdotAB - BdotAB);
+        *point = add(mul(ab, V), A);
+        return true;
+    }
+    float CdotAB = -dot(ab, C), CdotAC = -dot(ac, C);
+    if (CdotAC >= 0.f && CdotAB <= CdotAC) {
+        cand->state = S_POINT;
+        cand->A = sC;
+        *point = C;
+        return true;
+    }
+    float vb = CdotAB * AdotAC - AdotAB * CdotAC;
+    if (vb <= 0.f && AdotAC > 0.f && CdotAC < 0.f) {
+        cand->state = S_SEGMENT;
+        cand->A = sA;
+        cand->B = sC;
+        float V = AdotAC / (AdotAC - CdotAC);
+        *point = add(mul(ac, V), A);
+        return true;
+    }
+    float va = BdotAB * CdotAC - CdotAB * BdotAC;
+    float d3d4, d6d5;
+    if (va <= 0.f && (d3d4 = BdotAC - BdotAB) > 0.f && (d6d5 = CdotAB - CdotAC) > 0.f) {
+        cand->state = S_SEGMENT;
+        cand->A = sB;
+        cand->B = sC;
+        float V = d3d4 / (d3d4 + d6d5);
+        *point = add(mul(sub(C, B), V), B);
+        return true;
+    }
+    cand->state = S_TRIANGLE;
+    cand->A = sA;
+    cand->B = sB;
+    cand->C = sC;
+    float denom = 1.f / (va + vb + vc);
+    float w = vc * denom, v = vb * denom;
+    V3 p = mul(ab, v);
+    V3 acw = mul(ac, w);
+    p = add(A, p);
+    *point = add(p, acw);
+    return true;
+}
+
+// RaySimplex.GetPointOnTetrahedronClosestToOrigin (RaySimplex.cs:247-317):
+// the closest of the faces DAC, DCB, DBA (their origin-side tests) and ABC;
+// true when none qualifies (the origin is enclosed; the point is zero).
+bool rs_tetrahedron(const RSimplex& s, RSimplex& out, V3* point) {
+    RSimplex best;
+    std::memset(&best, 0, sizeof(best));
+    *point = kZero;
+    float best_d = FLT_MAX;
+    RSimplex cand;
+    V3 cp;
+    float d;
+    if (rs_try_face(s.A, s.C, s.D, out.A, out.C, out.D, s.B, &cand, &cp)) {
+        *point = cp;
+        best = cand;
+        best_d = length_sq(cp);
+    }
+    if (rs_try_face(s.C, s.B, s.D, out.C, out.B, out.D, s.A, &cand, &cp) && (d = length_sq(cp)) < best_d) {
+        *point = cp;
+        best = cand;
+        best_d = d;
+    }
+    if (rs_try_face(s.B, s.A, s.D, out.B, out.A, out.D, s.C, &cand, &cp) && (d = length_sq(cp)) < best_d) {
+        *point = cp;
+        best = cand;
+        best_d = d;
+    }
+    if (rs_try_face(s.A, s.B, s.C, out.A, out.B, out.C, s.D, &cand, &cp) && (d = length_sq(cp)) < best_d) {
+        *point = cp;
+        best = cand;
+        best_d = d;
+    }
+    if (best_d < FLT_MAX) {
+        out = best;
+        return false;
+    }
+    return true;
+}
+
+// RaySimplex.GetPointClosestToOrigin (RaySimplex.cs:40-69), called on the
+// shifted simplex with the unshifted one to reduce.
+V3 rs_closest(const RSimplex& s, RSimplex& out) {
+    V3 p;
+    switch (s.state) {
+    case S_POINT: return s.A;
+    case S_SEGMENT: return rs_segment(s, out);
+    case S_TRIANGLE: return rs_triangle(s, out);
+    case S_TETRAHEDRON: rs_tetrahedron(s, out, &p); return p;
+    default: return kZero;
+    }
+}
+
+// RaySimplex.AddNewSimplexPoint (RaySimplex.cs:478-520): the simplex grows
+// by `p`; returns it shifted to the hit location (hit - vertex).
+RSimplex rs_add(RSimplex& s, const V3& p, const V3& hit) {
+    RSimplex sh;
+    std::memset(&sh, 0, sizeof(sh));
+    switch (s.state) {
+    case S_EMPTY:
+        s.state = S_POINT;
+        s.A = p;
+        sh.A = sub(hit, s.A);
+        break;
+    case S_POINT:
+        s.state = S_SEGMENT;
+        s.B = p;
+        sh.A = sub(hit, s.A);
+        sh.B = sub(hit, s.B);
+        break;
+    case S_SEGMENT:
+        s.state = S_TRIANGLE;
+        s.C = p;
+        sh.A = sub(hit, s.A);
+        sh.B = sub(hit, s.B);
+        sh.C = sub(hit, s.C);
+        break;
+    case S_TRIANGLE:
+        s.state = S_TETRAHEDRON;
+        s.D = p;
+        sh.A = sub(hit, s.A);
+        sh.B = sub(hit, s.B);
+        sh.C = sub(hit, s.C);
+        sh.D = sub(hit, s.D);
+        break;
+    default: break;
+    }
+    sh.state = s.state;
+    return sh;
+}
+
+// RaySimplex.GetErrorTolerance (RaySimplex.cs:527-560)
+float rs_error_tolerance(const RSimplex& s, const V3& o) {
+    switch (s.state) {
+    case S_POINT: return distance_sq(s.A, o);
+    case S_SEGMENT: return maxf(distance_sq(s.A, o), distance_sq(s.B, o));
+    case S_TRIANGLE: return maxf(distance_sq(s.A, o), maxf(distance_sq(s.B, o), distance_sq(s.C, o)));
+    case S_TETRAHEDRON:
+        return maxf(distance_sq(s.A, o), maxf(distance_sq(s.B, o), maxf(distance_sq(s.C, o), distance_sq(s.D, o))));
+    default: return 0;
+    }
+}
+
+// ConvexShape.GetLocalExtremePoint (ConvexShape.cs:104-113): with the margin.
+V3 local_extreme_with_margin(const ConvexShapeRef& s, V3 d) {
+    V3 p = support_local(s, d);
+    float ls = length_sq(d);
+    if (ls > kEpsilon) {
+        d = mul(d, s.margin / std::sqrt(ls));
+        p = add(p, d);
+    }
+    return p;
+}
+
+}  // namespace
+
+// GJKToolbox.SphereCast (GJKToolbox.cs:387-459): a ray from `origin` along
+// `dir` (lengths in units of dir) swept by a sphere of `radius` against the
+// posed shape; *t receives the hit parameter, also where the cast fails (0
+// after too many iterations, as the reference's fresh RayHit).
+bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, const RT& xf, float max_length,
+                 float* t) {
+    origin = sub(origin, xf.pos);
+    Q4 cj = qconj(xf.q);
+    origin = qtransform(origin, cj);
+    dir = qtransform(dir, cj);
+    float T = 0;
+    V3 loc = origin;
+    V3 v = loc;
+    RSimplex simplex;
+    std::memset(&simplex, 0, sizeof(simplex));
+    int count = 0;
+    while (length_sq(v) >= kEpsilon * rs_error_tolerance(simplex, origin)) {
+        if (++count > kMaxGJKIterations) {
+            *t = 0;
+            return false;
+        }
+        V3 p = support_local(shape, v);
+        V3 contribution = expand_contribution(shape.margin, radius, v);
+        p = add(p, contribution);
+        V3 w = sub(loc, p);
+        float vw = dot(v, w);
+        if (vw > 0) {
+            float vdir = dot(v, dir);
+            T = T - vw / vdir;
+            if (vdir >= 0 || T > max_length) {
+                *t = T;
+                return false;
+            }
+            loc = add(mul(dir, T), origin);
+        }
+        RSimplex shifted = rs_add(simplex, p, loc);
+        v = rs_closest(shifted, simplex);
+    }
+    *t = T;
+    return true;
+}
+
+// GJKToolbox.RayCast (GJKToolbox.cs:195-265): the ray against the shape with
+// its margin.
+bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t) {
+    origin = sub(origin, xf.pos);
+    Q4 cj = qconj(xf.q);
+    origin = qtransform(origin, cj);
+    dir = qtransform(dir, cj);
+    float T = 0;
+    V3 loc = origin;
+    V3 off = loc;
+    RSimplex simplex;
+    std::memset(&simplex, 0, sizeof(simplex));
+    int count = 0;
+    while (length_sq(off) >= kEpsilon * rs_error_tolerance(simplex, origin)) {
+        if (++count > kMaxGJKIterations) {
+            *t = 0;
+            return false;
+        }
+        V3 e = local_extreme_with_margin(shape, off);
+        V3 eo = sub(loc, e);
+        float vw = dot(off, eo);
+        if (vw > 0) {
+            float cd = dot(off, dir);
+            if (cd >= 0) {
+                *t = 0;
+                return false;
+            }
+            T = T - vw / cd;
+            if (T > max_length) {
+                *t = 0;
+                return false;
+            }
+            loc = add(mul(dir, T), origin);
+        }
+        RSimplex shifted = rs_add(simplex, e, loc);
+        off = rs_closest(shifted, simplex);
+    }
+    *t = T;
+    return true;
+}
+
+// GJKToolbox.CCDSphereCast (GJKToolbox.cs:470-495): the sphere cast, retried
+// with the radius scaled by `scaling` while it starts embedded (a hit at 0),
+// up to three times, then a plain ray cast.
+bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
+                     float max_length, float scaling, float* t) {
+    int iterations = 0;
+    for (;;) {
+        float T;
+        if (sphere_cast(origin, dir, radius, shape, xf, max_length, &T) && T > 0) {
+            *t = T;
+            return true;
+        }
+        if (T > max_length || T < 0) {
+            *t = T;
+            return false;
+        }
+        radius *= scaling;
+        iterations++;
+        if (iterations > 3) {
+            bool hit = ray_cast(origin, dir, shape, xf, max_length, &T);
+            *t = T;
+            return hit && T > 0;
+        }
+    }
+}
+
 }  // namespace ref
 
 // ---------------------------------------------------------------------------
@@ -1104,6 +1484,21 @@ int refcpu_convex_candidate(const float* sa, const float* va, int na, const floa
     sep_axis_io[0] = st.separating_axis.x; sep_axis_io[1] = st.separating_axis.y; sep_axis_io[2] = st.separating_axis.z;
     local_dir_io[0] = st.local_direction.x; local_dir_io[1] = st.local_direction.y; local_dir_io[2] = st.local_direction.z;
     put_contact(c, out8);
+    return r ? 1 : 0;
+}
+
+// The casts on one posed shape: `kind` 0 SphereCast, 1 RayCast, 2
+// CCDSphereCast (with `scaling`); ray origin o3 and direction d3; returns
+// hit (1) or not (0), *t the hit parameter either way.
+int refcpu_cast(int kind, const float* s, const float* verts, int n, const float* pose, const float* o3,
+                const float* d3, float radius, float max_length, float scaling, float* t) {
+    std::vector<ref::V3> h;
+    ref::ConvexShapeRef shape = make_shape(s, verts, n, &h);
+    ref::V3 o{o3[0], o3[1], o3[2]}, d{d3[0], d3[1], d3[2]};
+    bool r;
+    if (kind == 0) r = ref::sphere_cast(o, d, radius, shape, make_pose(pose), max_length, t);
+    else if (kind == 1) r = ref::ray_cast(o, d, shape, make_pose(pose), max_length, t);
+    else r = ref::ccd_sphere_cast(o, d, radius, shape, make_pose(pose), max_length, scaling, t);
     return r ? 1 : 0;
 }
 

@@ -73,6 +73,14 @@ struct Body {
     // Space.Remove(entity) (Space.cs:364-482): the body left every stage; its
     // broadphase entry is gone, so none of its pairs survive.
     bool removed;
+    // PositionUpdateMode (PositionUpdating/ICCDPositionUpdateable.cs:6-21):
+    // 0 Discrete, 1 Passive, 2 Continuous; the shape's MinimumRadius; and
+    // the orientation of the collidable's world transform while the linear
+    // motion waits for the times of impact (PreUpdatePosition turns the
+    // entity but refreshes that transform only for discrete bodies).
+    int pmode;
+    float min_radius;
+    Q4 q_world;
 };
 
 struct ContactObj {
@@ -146,6 +154,10 @@ struct Pair {
     // Not updated this step: both bodies were inactive when the broad phase
     // ran (BroadPhase.cs:97-102; NarrowPhase.cs:300-333 keeps such pairs).
     bool frozen;
+    // CollidablePairHandler.timeOfImpact (CollidablePairHandler.cs:59): 1 for
+    // a new pair, recomputed by ConvexPairHandler.UpdateTimeOfImpact in the
+    // steps it has a continuous body and an active one, kept otherwise.
+    float toi;
 };
 
 struct Settings {
@@ -263,6 +275,7 @@ struct refcpu_world {
     // with the list in descending key order (another order of addition).
     int solve_order = 0;
     int64_t perm_index = 0;
+    bool ccd = false;  // a body was ever passive or continuous
 };
 
 namespace {
@@ -315,6 +328,9 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
     b->asleep = b->wake = b->nodeact = false;
     b->label = 0xffffffffu;
     b->mat = m ? *m : bepu_material{1.0f, 0.8f, 0.0f};
+    b->pmode = (flags & BEPU_FLAG_CCD_CONTINUOUS) ? 2 : ((flags & BEPU_FLAG_CCD_PASSIVE) ? 1 : 0);
+    if (b->pmode) w->ccd = true;
+    b->q_world = b->q;
 }
 
 // ---------------------------------------------------------------------------
@@ -608,18 +624,82 @@ void bounding_box(refcpu_world* w, Body& b) {
     }
 }
 
-// Entity.PreUpdatePosition (EntityBase.cs:1124-1155)
+// Entity.PreUpdatePosition (EntityBase.cs:1124-1155): the orientation of
+// every active body; the position of the discrete ones.
 void integrate(refcpu_world* w, Body& b) {
     if (!body_active(b)) return;  // ContinuousPositionUpdater.cs:71: active entities only
     float dt = w->s.dt;
+    b.q_world = b.q;
     V3 inc = mul(b.w, dt * .5f);
     Q4 m{inc.x, inc.y, inc.z, 0};
     m = qmul(m, b.q);
     b.q = qadd(b.q, m);
     b.q = qnormalize(b.q);
     b.R = m3_from_quat(b.q);
-    inc = mul(b.v, dt);
-    b.pos = add(b.pos, inc);
+    if (b.pmode == 0) {
+        inc = mul(b.v, dt);
+        b.pos = add(b.pos, inc);
+    }
+}
+
+ConvexShapeRef shape_ref(const refcpu_world* w, const Body& b);
+
+// The collidable's world transform while the times of impact are computed:
+// a moving passive or continuous body still has the pose of the step's
+// start (its orientation was turned, its transform not refreshed), the
+// others their current pose.
+RT ccd_transform(const Body& b) {
+    return RT{b.pos, (b.pmode != 0 && body_active(b)) ? b.q_world : b.q};
+}
+
+// ConvexPairHandler.UpdateTimeOfImpact (ConvexPairHandler.cs:37-111), as the
+// pair's continuous bodies request it (EntityBase.cs:1071-1082: every
+// continuous body asks each of its pairs, active or not; with two continuous
+// bodies only one request computes).  A pair that no request computes keeps
+// its time of impact.
+void pair_time_of_impact(refcpu_world* w, Pair& p) {
+    const Body& A = w->bodies[p.a];
+    const Body& B = w->bodies[p.b];
+    if (A.pmode != 2 && B.pmode != 2) return;
+    if (!body_active(A) && !body_active(B)) return;
+    // Discrete bodies already moved: the sweep is the other body's motion.
+    V3 vel;
+    if (A.pmode == 0) vel = B.v;
+    else if (B.pmode == 0) vel = neg(A.v);
+    else vel = sub(B.v, A.v);
+    vel = mul(vel, w->s.dt);
+    const float vsq = length_sq(vel);
+    const float scaling = w->s.core_shape_scaling;
+    const RT ta = ccd_transform(A), tb = ccd_transform(B);
+    float t;
+    const float ra = A.min_radius * scaling;
+    p.toi = 1;
+    if (ra * ra < vsq && ccd_sphere_cast(ta.pos, neg(vel), ra, shape_ref(w, B), tb, p.toi, scaling, &t)) p.toi = t;
+    const float rb = B.min_radius * scaling;
+    if (rb * rb < vsq && ccd_sphere_cast(tb.pos, vel, rb, shape_ref(w, A), ta, p.toi, scaling, &t)) p.toi = t;
+    // Embedded from the start: no clamping (the contacts handle it).
+    if (p.toi == 0) p.toi = 1;
+}
+
+// ContinuousPositionUpdater.UpdateSingleThreaded after the pre-update
+// (ContinuousPositionUpdater.cs:140-154): times of impact of the pairs with a
+// continuous body, then Entity.UpdatePositionContinuously (EntityBase.cs:
+// 1093-1121) for every active passive or continuous body: the linear motion
+// scaled by the smallest time of impact of its pairs.
+void continuous_update(refcpu_world* w) {
+    std::vector<float> least(w->bodies.size(), 1.0f);
+    for (Pair& p : w->pairs.vals) {
+        pair_time_of_impact(w, p);
+        if (p.toi < least[p.a]) least[p.a] = p.toi;
+        if (p.toi < least[p.b]) least[p.b] = p.toi;
+    }
+    const float dt = w->s.dt;
+    for (size_t i = 0; i < w->bodies.size(); ++i) {
+        Body& b = w->bodies[i];
+        if (b.pmode == 0 || !body_active(b)) continue;
+        V3 inc = mul(b.v, dt * least[i]);
+        b.pos = add(b.pos, inc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1048,6 +1128,7 @@ Pair make_pair_state(refcpu_world* w, uint32_t i, uint32_t j) {
     p.bounce = ma.bounciness * mb.bounciness;
     p.kinetic = ma.kinetic_friction * mb.kinetic_friction;
     p.static_f = ma.static_friction * mb.static_friction;
+    p.toi = 1;
     return p;
 }
 
@@ -1884,10 +1965,11 @@ void step_once(refcpu_world* w) {
         for (int it = 0; it < w->s.iteration_limit; ++it) by_colour([&](Pair& p) { solve_iteration(w, p); });
     }
 
-    // ContinuousPositionUpdater.UpdateSingleThreaded, discrete path (ContinuousPositionUpdater.cs:134-154)
+    // ContinuousPositionUpdater.UpdateSingleThreaded (ContinuousPositionUpdater.cs:134-154)
     par_for(w, w->bodies.size(), 4096, [&](size_t b0, size_t b1) {
         for (size_t i = b0; i < b1; ++i) integrate(w, w->bodies[i]);
     });
+    if (w->ccd) continuous_update(w);
     w->steps++;
 }
 
@@ -1921,12 +2003,14 @@ void refcpu_settings_default(bepu_settings* s) {
     s->linear_damping = .03f;
     s->angular_damping = .15f;
     s->max_deactivation_attempts = 100;
+    s->core_shape_scaling = .8f;
 }
 
 refcpu_world* refcpu_create(const bepu_settings* s) {
     refcpu_world* w = new refcpu_world();
     if (s) w->s = *s;
     else refcpu_settings_default(&w->s);
+    w->s.core_shape_scaling = clampf(w->s.core_shape_scaling, 0, .99f);  // MotionSettings.cs:22-34
     return w;
 }
 void refcpu_destroy(refcpu_world* w) { delete w; }
@@ -1977,6 +2061,7 @@ int refcpu_add_box(refcpu_world* w, const bepu_motion_state* st, float width, fl
     vd.m22 = (w2 + l2) * inv12;
     vd.m33 = (w2 + h2) * inv12;
     init_body(w, &b, st, mass, m, flags, vd);
+    b.min_radius = 0.5f * std::min(width, std::min(height, length));  // BoxShape.cs:134
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
     return BEPU_OK;
@@ -1995,6 +2080,7 @@ int refcpu_add_sphere(refcpu_world* w, const bepu_motion_state* st, float radius
     float diag = ((2.0f / 5.0f) * radius * radius);
     M3 vd{diag, 0, 0, 0, diag, 0, 0, 0, diag};
     init_body(w, &b, st, mass, m, flags, vd);
+    b.min_radius = radius;  // SphereShape.cs:62
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
     return BEPU_OK;
@@ -2015,17 +2101,19 @@ int refcpu_add_capsule(refcpu_world* w, const bepu_motion_state* st, float lengt
     float diag = (.0833333333f * effectiveLength * effectiveLength + .25f * radius * radius);
     M3 vd{diag, 0, 0, 0, .5f * radius * radius, 0, 0, 0, diag};
     init_body(w, &b, st, mass, m, flags, vd);
+    b.min_radius = radius;  // CapsuleShape.cs:89
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
     return BEPU_OK;
 }
 
 // ConvexHullShape(localSurfaceVertices, description) (ConvexHullShape.cs:114-129): the
-// hull is given as its recentred surface vertices and its volume distribution
-// (row-major 3x3), as ConvexHullShape.ComputeDescription produces them.
+// hull is given as its recentred surface vertices, its volume distribution
+// (row-major 3x3) and its minimum radius without the margin, as
+// ConvexHullShape.ComputeDescription produces them.
 int refcpu_add_convex_hull(refcpu_world* w, const bepu_motion_state* st, const float* verts, uint32_t nverts,
-                           const float* volume_distribution, float mass, const bepu_material* m, uint32_t flags,
-                           uint32_t* id) {
+                           const float* volume_distribution, float minimum_radius, float mass,
+                           const bepu_material* m, uint32_t flags, uint32_t* id) {
     if (!st || !verts || nverts == 0 || !volume_distribution || std::isnan(mass) || std::isinf(mass))
         return BEPU_E_INVALID_ARG;
     if (!finite3(st->position, 3) || !finite3(st->orientation, 4) || !finite3(verts, 3 * nverts))
@@ -2040,6 +2128,7 @@ int refcpu_add_convex_hull(refcpu_world* w, const bepu_motion_state* st, const f
     const float* v = volume_distribution;
     M3 vd{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]};
     init_body(w, &b, st, mass, m, flags, vd);
+    b.min_radius = minimum_radius + b.margin;  // ConvexHullShape.cs:170
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
     return BEPU_OK;
@@ -2254,8 +2343,20 @@ int refcpu_set_motion_states(refcpu_world* w, const uint32_t* ids, const bepu_mo
 // OtherSpaceStages/ForceUpdater.cs:19-29, TimeStepSettings.cs:19).  Per-entity
 // values taken at creation (damping) keep their values, as the reference's do.
 int refcpu_set_settings(refcpu_world* w, const bepu_settings* s) {
-    if (!s || s->min_iteration_count > 6 || s->iteration_limit < 0 || !(s->dt > 0)) return BEPU_E_INVALID_ARG;
+    if (!s || s->min_iteration_count > 6 || s->iteration_limit < 0 || !(s->dt > 0) || std::isnan(s->core_shape_scaling))
+        return BEPU_E_INVALID_ARG;
     w->s = *s;
+    w->s.core_shape_scaling = clampf(w->s.core_shape_scaling, 0, .99f);  // MotionSettings.cs:22-34
+    return BEPU_OK;
+}
+
+// Entity.PositionUpdateMode setter (EntityBase.cs:1050-1068) for each listed body.
+int refcpu_set_position_update_mode(refcpu_world* w, const uint32_t* ids, uint32_t n, int mode) {
+    if ((n && !ids) || mode < 0 || mode > 2) return BEPU_E_INVALID_ARG;
+    for (uint32_t k = 0; k < n; ++k)
+        if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
+    for (uint32_t k = 0; k < n; ++k) w->bodies[ids[k]].pmode = mode;
+    if (mode && n) w->ccd = true;
     return BEPU_OK;
 }
 

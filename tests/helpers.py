@@ -12,8 +12,8 @@ def oracle_hulls():
     distribution) per shape, computed by the library's host hull builder."""
     global _ORACLE_HULLS
     if _ORACLE_HULLS is None:
-        from bepuphysics1_amd import compute_hull, scenes
-        _ORACLE_HULLS = [compute_hull(p)[:2] for p in scenes.hull_library()]
+        from bepuphysics1_amd import compute_hull, compute_hull_minimum_radius, scenes
+        _ORACLE_HULLS = [compute_hull(p)[:2] + (compute_hull_minimum_radius(p),) for p in scenes.hull_library()]
     return _ORACLE_HULLS
 
 
