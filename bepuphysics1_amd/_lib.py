@@ -165,6 +165,10 @@ EXPORTS = {
     "bepu_get_activity": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_uint32, _u32p]),
     "bepu_activate": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32]),
     "bepu_set_position_update_mode": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, ctypes.c_int32]),
+    "bepu_add_ball_socket": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                                            _u32p]),
+    "bepu_set_joint_settings": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "bepu_get_joint_impulses": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, _u32p]),
     "bepu_compute_hull_minimum_radius": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
                                                         ctypes.POINTER(ctypes.c_float)]),
     "bepu_device_available": (ctypes.c_int32, []),

@@ -83,6 +83,19 @@ __global__ void __launch_bounds__(256) k_uf_roots(DeviceWorld w) {
         }
     }
     (void)wave_atomic_add<unsigned long long>((unsigned long long*)w.isl_count, rc, cons, kConsOne);
+    // Joints are constraints of the island of their dynamic connection(s).
+    bool joint = false;
+    uint32_t rj = 0;
+    if (i < w.njoints) {
+        const JointState& j = w.joints[i];
+        joint = solver_joint(w, j);
+        if (joint) {
+            const uint32_t d = (w.bflags[j.a] & 1u) ? j.a : j.b;
+            rj = w.sleeping ? w.comp_root[d] : uf_find(w.uf_parent, d);
+            w.cons_root[w.pair_cap + i] = rj;
+        }
+    }
+    (void)wave_atomic_add<unsigned long long>((unsigned long long*)w.isl_count, rj, joint, kConsOne);
 }
 
 // Scatters constraints and bodies into island-contiguous lists (counting down
@@ -111,6 +124,18 @@ __global__ void __launch_bounds__(kFillBlock) k_isl_fill(DeviceWorld w) {
         global = island_tier(hi32(s1) - hi32(s0), lo32(s1) - lo32(s0), w.small_island_max) == 3;
     }
     block_push<kFillBlock>(w.gcons, &w.counters[ctr(CTR_GCONS)], global, i, s_tmp);
+    // joints: constraint id pair_cap + j
+    const bool joint = i < w.njoints && solver_joint(w, w.joints[i]);
+    const uint32_t jid = w.pair_cap + i;
+    const uint32_t rj = joint ? w.cons_root[jid] : 0;
+    unsigned long long oj = wave_atomic_add<unsigned long long>(cnt, rj, joint, 0ull - kConsOne);
+    bool jglobal = false;
+    if (joint) {
+        uint64_t s0 = w.isl_start[rj], s1 = w.isl_start[rj + 1];
+        w.isl_cons[hi32(s0) + hi32(oj) - 1] = jid;
+        jglobal = island_tier(hi32(s1) - hi32(s0), lo32(s1) - lo32(s0), w.small_island_max) == 3;
+    }
+    block_push<kFillBlock>(w.gcons, &w.counters[ctr(CTR_GCONS)], jglobal, jid, s_tmp);
 
     bool body = i < w.nbodies && solver_body(w.bflags[i]);
     uint32_t rb = body ? w.isl_root[i] : 0;
@@ -294,7 +319,16 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
         }
     };
     // Prestep reads velocities only; no body changes before every lane is done.
-    if (have) {
+    const bool joint = have && slot >= w.pair_cap;
+    if (joint) {
+        JointState& J = w.joints[slot - w.pair_cap];
+        load(sa, A);
+        load(sb, B);
+        const float4 qa = w.quat[ga], qb = w.quat[gb];
+        prestep_joint(st, J, f3_of(w.pos[ga]), q4{qa.x, qa.y, qa.z, qa.w}, f3_of(w.pos[gb]), q4{qb.x, qb.y, qb.z, qb.w},
+                      A, B, row, var);
+        J.color = color;
+    } else if (have) {
         // On a register copy of the pair state (one load of all of it); the
         // fields the prestep updates (remembered friction directions) and the
         // colour go back.
@@ -333,7 +367,8 @@ __device__ __forceinline__ void solve_lanes(DeviceWorld& w, IslandLds<BLOCK, NSL
             }
             group_sync<BLOCK>();
         }
-    if (have) writeback_row(w.pairs_cur[slot], var);
+    if (joint) writeback_joint(w.joints[slot - w.pair_cap], var);
+    else if (have) writeback_row(w.pairs_cur[slot], var);
 }
 
 // Per-lane constraint set-up: global bodies, LDS slots (dynamic: base +
@@ -346,12 +381,10 @@ __device__ __forceinline__ void setup_lane(const DeviceWorld& w, IslandLds<BLOCK
     ga = gb = 0;
     sa = sb = 0;
     if (have) {
-        const PairState& s = w.pairs_cur[slot];
-        ga = s.a;
-        gb = s.b;
+        cons_ends(w, slot, ga, gb);
         if (w.bflags[ga] & 1u) la = dbase + (int)w.body_local[ga];
         if (w.bflags[gb] & 1u) lb = dbase + (int)w.body_local[gb];
-        L.prio[tid] = mix64(local_pair_key(w, s.key, s.a));
+        L.prio[tid] = cons_priority(w, slot);
         // A constraint has at most one kinematic body; it gets slot DYN + lane.
         int ext = DYN + tid;
         sa = la >= 0 ? la : ext;
@@ -491,7 +524,7 @@ static int persistent_grid(const void* kernel, int block) {
 }
 
 void launch_islands(const DeviceWorld& w, void* temp, size_t temp_bytes, hipStream_t st) {
-    uint32_t n = w.nbodies, cap = w.pair_cap;
+    uint32_t n = w.nbodies, cap = std::max(w.pair_cap, w.njoints);
     dim3 b(256), gm((std::max(n, cap) + 255) / 256);
     hipLaunchKernelGGL(k_uf_roots, gm, b, 0, st, w);
     size_t bytes = temp_bytes;

@@ -87,21 +87,31 @@ __global__ void __launch_bounds__(256) k_deact_wake(DeviceWorld w) {
         const uint32_t f = w.bflags[i];
         if ((f & BF_DYNAMIC) && (f & BF_WAKE)) w.lab_act[w.comp_root[i]] = 1u;
     }
+    // A kinematic connection whose flag becomes active (the candidacy update
+    // below) forbids the next deactivation of the island it touches and
+    // activates it: through a constraint of the last step, or a joint.
+    auto kinematic_rule = [&](uint32_t a, uint32_t b) {
+        for (int side = 0; side < 2; ++side) {
+            const uint32_t k = side ? b : a, d = side ? a : b;
+            const uint32_t fk = w.bflags[k], fd = w.bflags[d];
+            if ((fk & BF_DYNAMIC) || !(fd & BF_DYNAMIC)) continue;
+            const float next = kinematic_flag_next(w.stab[k].y, body_velocity_sq(w, k), (fk & BF_ALWAYS_ACTIVE) != 0);
+            if (next <= 0.0f) {
+                w.lab_act[w.comp_root[d]] = 1u;
+                atomicOr(&w.bflags[d], (uint32_t)BF_NODEACT);
+            }
+        }
+    };
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS_PREV)], w.pair_cap);
     if (i < np) {
         const PairHot s = w.hot_old[i];
-        if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID)) && carried(w, s.a) && carried(w, s.b)) {
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t k = side ? s.b : s.a, d = side ? s.a : s.b;
-                const uint32_t fk = w.bflags[k], fd = w.bflags[d];
-                if ((fk & BF_DYNAMIC) || !(fd & BF_DYNAMIC)) continue;
-                const float next = kinematic_flag_next(w.stab[k].y, body_velocity_sq(w, k), (fk & BF_ALWAYS_ACTIVE) != 0);
-                if (next <= 0.0f) {
-                    w.lab_act[w.comp_root[d]] = 1u;
-                    atomicOr(&w.bflags[d], (uint32_t)BF_NODEACT);
-                }
-            }
-        }
+        if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID)) && carried(w, s.a) && carried(w, s.b))
+            kinematic_rule(s.a, s.b);
+    }
+    if (i < w.njoints) {
+        const JointState& j = w.joints[i];
+        if (!(j.flags & JF_DEAD) && (w.bflags[j.a] & BF_PRESENT) && (w.bflags[j.b] & BF_PRESENT))
+            kinematic_rule(j.a, j.b);
     }
 }
 
@@ -250,15 +260,17 @@ __global__ void __launch_bounds__(256) k_wake_mark(DeviceWorld w) {
         const uint32_t f = w.bflags[i];
         if ((f & BF_DYNAMIC) && !(f & BF_ASLEEP)) w.act2[uf_find(w.uf_parent, i)] = 1u;
     }
+    auto kinematic_wake = [&](uint32_t a, uint32_t b) {
+        const uint32_t fa = w.bflags[a], fb = w.bflags[b];
+        if (!(fa & BF_DYNAMIC) && (fa & BF_KIN_ACTIVE) && (fb & BF_DYNAMIC)) w.act2[uf_find(w.uf_parent, b)] = 1u;
+        if (!(fb & BF_DYNAMIC) && (fb & BF_KIN_ACTIVE) && (fa & BF_DYNAMIC)) w.act2[uf_find(w.uf_parent, a)] = 1u;
+    };
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (i < np) {
         const PairHot s = w.hot_cur[i];
-        if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID))) {
-            const uint32_t fa = w.bflags[s.a], fb = w.bflags[s.b];
-            if (!(fa & BF_DYNAMIC) && (fa & BF_KIN_ACTIVE) && (fb & BF_DYNAMIC)) w.act2[uf_find(w.uf_parent, s.b)] = 1u;
-            if (!(fb & BF_DYNAMIC) && (fb & BF_KIN_ACTIVE) && (fa & BF_DYNAMIC)) w.act2[uf_find(w.uf_parent, s.a)] = 1u;
-        }
+        if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID))) kinematic_wake(s.a, s.b);
     }
+    if (i < w.njoints && !(w.joints[i].flags & JF_DEAD)) kinematic_wake(w.joints[i].a, w.joints[i].b);
 }
 
 // Wakes the members of active islands; a frozen pair with an active body is
@@ -301,6 +313,10 @@ __global__ void __launch_bounds__(256) k_wake_apply(DeviceWorld w) {
 __global__ void __launch_bounds__(256) k_live_union(DeviceWorld w) {
     if (w.counters[ctr(CTR_DEAD)] == 0) return;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < w.njoints) {
+        const JointState& j = w.joints[t];
+        if (joint_edge(w, j) && !(w.bflags[j.a] & BF_ASLEEP)) uf_union(w.uf2, j.a, j.b);
+    }
     const uint32_t np = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     if (t >= np) return;
     const PairHot s = w.hot_cur[t];
@@ -369,7 +385,7 @@ size_t deactivation_scan_temp_bytes(uint32_t n) {
 void launch_deactivation(const DeviceWorld& w, void* scan_temp, size_t scan_temp_bytes, hipStream_t st) {
     if (!w.sleeping || w.nbodies == 0) return;
     const uint32_t nb = (w.nbodies + 255) / 256;
-    const uint32_t nwake = (std::max<uint32_t>(w.nbodies, w.pair_cap) + 255) / 256;
+    const uint32_t nwake = (std::max<uint32_t>(std::max<uint32_t>(w.nbodies, w.pair_cap), w.njoints) + 255) / 256;
     hipLaunchKernelGGL(k_deact_wake, dim3(nwake), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_deact_members, dim3(nb), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_deact_turn_m, dim3((w.nbodies + 1 + 255) / 256), dim3(256), 0, st, w);
@@ -382,11 +398,13 @@ void launch_activity(const DeviceWorld& w, hipStream_t st) {
     if (!w.sleeping || w.nbodies == 0) return;
     const uint32_t nb = (w.nbodies + 255) / 256;
     const uint32_t np = (w.pair_cap + 255) / 256;
+    const uint32_t npj = (std::max<uint32_t>(w.pair_cap, w.njoints) + 255) / 256;
     const uint32_t nboth = (std::max<uint32_t>(w.nbodies, w.pair_cap) + 255) / 256;
-    const uint32_t nmark = (std::max<uint32_t>(std::max<uint32_t>(w.nbodies, w.pair_cap), w.nworlds) + 255) / 256;
+    const uint32_t nmark =
+        (std::max<uint32_t>(std::max<uint32_t>(w.nbodies, w.pair_cap), std::max<uint32_t>(w.nworlds, w.njoints)) + 255) / 256;
     hipLaunchKernelGGL(k_wake_mark, dim3(nmark), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_wake_apply, dim3(nboth), dim3(256), 0, st, w);
-    hipLaunchKernelGGL(k_live_union, dim3(np), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_live_union, dim3(npj), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_live_roots, dim3(nb), dim3(256), 0, st, w);
     hipLaunchKernelGGL(k_split_wake, dim3(np), dim3(256), 0, st, w);
 }

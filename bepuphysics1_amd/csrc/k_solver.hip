@@ -78,7 +78,7 @@ __device__ __forceinline__ bool grid_barrier(GridBarrier& gb, uint32_t* s_ok) {
 }
 
 __device__ __forceinline__ uint32_t gcount(const DeviceWorld& w) {
-    return min(w.counters[ctr(CTR_GCONS)], w.pair_cap);
+    return min(w.counters[ctr(CTR_GCONS)], w.pair_cap + w.joint_cap);
 }
 
 __device__ __forceinline__ int load_color(const int* p) {
@@ -87,10 +87,10 @@ __device__ __forceinline__ int load_color(const int* p) {
 
 // Returns the colour, or -1 while a higher-priority neighbour is uncoloured.
 __device__ int try_color(const DeviceWorld& w, uint32_t c) {
-    const PairState& s = w.pairs_cur[w.gcons[c]];
+    uint32_t bodies[2];
+    cons_ends(w, w.gcons[c], bodies[0], bodies[1]);
     uint64_t p = w.prio[c];
     uint64_t mask0 = 0, mask1 = 0;
-    uint32_t bodies[2] = {s.a, s.b};
     for (int k = 0; k < 2; ++k) {
         uint32_t b = bodies[k];
         if (!(w.bflags[b] & 1u)) continue;
@@ -111,8 +111,8 @@ __device__ int try_color(const DeviceWorld& w, uint32_t c) {
 
 __device__ __forceinline__ void global_phase(const DeviceWorld& w, uint32_t p, bool warm) {
     uint32_t slot = w.grow_pair[p];
-    const PairState& s = w.pairs_cur[slot];
-    uint32_t a = s.a, b = s.b;
+    uint32_t a, b;
+    cons_ends(w, slot, a, b);
     BodyV A = load_body_global(w, a), B = load_body_global(w, b);
     const RowConst& r = w.grow_c[p];
     RowVar v = w.grow_v[p];
@@ -153,10 +153,11 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w, uint64_t 
 
     // 1. incidence counts, priorities
     for (uint32_t c = tid; c < m; c += gsize) {
-        const PairState& s = w.pairs_cur[w.gcons[c]];
-        if (w.bflags[s.a] & 1u) atomicAdd(&w.inc_count[s.a], 1u);
-        if (w.bflags[s.b] & 1u) atomicAdd(&w.inc_count[s.b], 1u);
-        w.prio[c] = mix64(local_pair_key(w, s.key, s.a));
+        uint32_t a, b;
+        cons_ends(w, w.gcons[c], a, b);
+        if (w.bflags[a] & 1u) atomicAdd(&w.inc_count[a], 1u);
+        if (w.bflags[b] & 1u) atomicAdd(&w.inc_count[b], 1u);
+        w.prio[c] = cons_priority(w, w.gcons[c]);
         w.color[c] = -1;
     }
     if (!grid_barrier(gb, &s_ok)) return;
@@ -191,9 +192,10 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w, uint64_t 
 
     // 3. incidence lists (counting down leaves inc_count at zero for the next step)
     for (uint32_t c = tid; c < m; c += gsize) {
-        const PairState& s = w.pairs_cur[w.gcons[c]];
-        if (w.bflags[s.a] & 1u) w.inc_list[w.inc_offset[s.a] + atomicSub(&w.inc_count[s.a], 1u) - 1] = c;
-        if (w.bflags[s.b] & 1u) w.inc_list[w.inc_offset[s.b] + atomicSub(&w.inc_count[s.b], 1u) - 1] = c;
+        uint32_t a, b;
+        cons_ends(w, w.gcons[c], a, b);
+        if (w.bflags[a] & 1u) w.inc_list[w.inc_offset[a] + atomicSub(&w.inc_count[a], 1u) - 1] = c;
+        if (w.bflags[b] & 1u) w.inc_list[w.inc_offset[b] + atomicSub(&w.inc_count[b], 1u) - 1] = c;
     }
     if (!grid_barrier(gb, &s_ok)) return;
 
@@ -264,12 +266,21 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w, uint64_t 
     for (uint32_t p = tid; p < m; p += gsize) {
         uint32_t c = w.ordered[p];
         uint32_t slot = w.gcons[c];
-        PairState& s = w.pairs_cur[slot];
-        BodyV A = load_body_global(w, s.a), B = load_body_global(w, s.b);
         RowConst r;
         RowVar v;
-        prestep_row(w.s, s, f3_of(w.pos[s.a]), f3_of(w.pos[s.b]), A, B, r, v);
-        s.color = w.color[c];
+        if (slot >= w.pair_cap) {
+            JointState& J = w.joints[slot - w.pair_cap];
+            BodyV A = load_body_global(w, J.a), B = load_body_global(w, J.b);
+            const float4 qa = w.quat[J.a], qb = w.quat[J.b];
+            prestep_joint(w.s, J, f3_of(w.pos[J.a]), q4{qa.x, qa.y, qa.z, qa.w}, f3_of(w.pos[J.b]),
+                          q4{qb.x, qb.y, qb.z, qb.w}, A, B, r, v);
+            J.color = w.color[c];
+        } else {
+            PairState& s = w.pairs_cur[slot];
+            BodyV A = load_body_global(w, s.a), B = load_body_global(w, s.b);
+            prestep_row(w.s, s, f3_of(w.pos[s.a]), f3_of(w.pos[s.b]), A, B, r, v);
+            s.color = w.color[c];
+        }
         w.grow_c[p] = r;
         w.grow_v[p] = v;
         w.grow_pair[p] = slot;
@@ -289,7 +300,11 @@ __global__ void __launch_bounds__(kSolveBlock) k_global(DeviceWorld w, uint64_t 
             for (uint32_t p = s_start[col] + tid; p < s_start[col + 1]; p += gsize) global_phase(w, p, false);
             if (!grid_barrier(gb, &s_ok)) return;
         }
-    for (uint32_t p = tid; p < m; p += gsize) writeback_row(w.pairs_cur[w.grow_pair[p]], w.grow_v[p]);
+    for (uint32_t p = tid; p < m; p += gsize) {
+        const uint32_t slot = w.grow_pair[p];
+        if (slot >= w.pair_cap) writeback_joint(w.joints[slot - w.pair_cap], w.grow_v[p]);
+        else writeback_row(w.pairs_cur[slot], w.grow_v[p]);
+    }
 }
 
 // ---------------------------------------------------------------------------

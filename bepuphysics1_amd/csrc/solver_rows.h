@@ -46,8 +46,19 @@ struct RowConst {
             f3 flin[4], fA[4], fB[4];
             float ffric[4], fv2i[4];
         } n;
+        struct {  // joints: the point constraint of a BallSocketJoint
+            m3 M;           // massMatrix
+            f3 ra, rb;      // worldOffsetA / worldOffsetB
+            f3 bias;        // biasVelocity
+            float soft;     // softness
+        } j;
     };
 };
+
+// RowConst.kind of a joint (pair kinds are BEPU_PAIR_*).  A joint's RowVar
+// holds its accumulated impulse in pacc[0..2], isActiveInSolver in kGroupBit
+// and iterationsAtZeroImpulse in `zero`.
+constexpr int kJointKind = 16;
 
 // Values the iterations update.
 struct RowVar {
@@ -309,9 +320,141 @@ BEPU_D void prestep_row(const bepu_settings& st, PairState& s, f3 pA, f3 pB, con
     v.zero = 0;
 }
 
+// Matrix3x3.CreateCrossProduct (BEPUutilities/Matrix3x3.cs:378-389)
+BEPU_D m3 cross_matrix(f3 v) { return m3{0, -v.z, v.y, v.z, 0, -v.x, -v.y, v.x, 0}; }
+// Matrix3x3.Multiply (Matrix3x3.cs:766-791)
+BEPU_D m3 mat_mul(const m3& a, const m3& b) {
+    m3 r;
+    r.m11 = a.m11 * b.m11 + a.m12 * b.m21 + a.m13 * b.m31;
+    r.m12 = a.m11 * b.m12 + a.m12 * b.m22 + a.m13 * b.m32;
+    r.m13 = a.m11 * b.m13 + a.m12 * b.m23 + a.m13 * b.m33;
+    r.m21 = a.m21 * b.m11 + a.m22 * b.m21 + a.m23 * b.m31;
+    r.m22 = a.m21 * b.m12 + a.m22 * b.m22 + a.m23 * b.m32;
+    r.m23 = a.m21 * b.m13 + a.m22 * b.m23 + a.m23 * b.m33;
+    r.m31 = a.m31 * b.m11 + a.m32 * b.m21 + a.m33 * b.m31;
+    r.m32 = a.m31 * b.m12 + a.m32 * b.m22 + a.m33 * b.m32;
+    r.m33 = a.m31 * b.m13 + a.m32 * b.m23 + a.m33 * b.m33;
+    return r;
+}
+// Matrix3x3.Subtract (Matrix3x3.cs:973-997)
+BEPU_D m3 mat_sub(const m3& a, const m3& b) {
+    return m3{a.m11 - b.m11, a.m12 - b.m12, a.m13 - b.m13, a.m21 - b.m21, a.m22 - b.m22,
+              a.m23 - b.m23, a.m31 - b.m31, a.m32 - b.m32, a.m33 - b.m33};
+}
+// Matrix3x3.Invert (Matrix3x3.cs:505-530) with Determinant (:1231-1235)
+BEPU_D m3 mat_invert(const m3& m) {
+    const float det = m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 -
+                      m.m31 * m.m22 * m.m13 - m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
+    const float d = 1 / det;
+    m3 r;
+    r.m11 = (m.m22 * m.m33 - m.m23 * m.m32) * d;
+    r.m12 = (m.m13 * m.m32 - m.m33 * m.m12) * d;
+    r.m13 = (m.m12 * m.m23 - m.m22 * m.m13) * d;
+    r.m21 = (m.m23 * m.m31 - m.m21 * m.m33) * d;
+    r.m22 = (m.m11 * m.m33 - m.m13 * m.m31) * d;
+    r.m23 = (m.m13 * m.m21 - m.m11 * m.m23) * d;
+    r.m31 = (m.m21 * m.m32 - m.m22 * m.m31) * d;
+    r.m32 = (m.m12 * m.m31 - m.m11 * m.m32) * d;
+    r.m33 = (m.m11 * m.m22 - m.m12 * m.m21) * d;
+    return r;
+}
+
+// Solver.UnsafePrestep -> BallSocketJoint.Update (Solver.cs:265-275,
+// BallSocketJoint.cs:229-300): world offsets of the anchors, the softened
+// effective mass of the point constraint and the bias toward closing the gap
+// between the anchors.  qA / qB give the bodies' orientation matrices
+// (Matrix3x3.CreateFromQuaternion, as the entities keep them).
+BEPU_D void prestep_joint(const bepu_settings& st, const JointState& J, f3 pA, q4 qA, f3 pB, q4 qB, const BodyV& A,
+                          const BodyV& B, RowConst& r, RowVar& v) {
+    r.kind = kJointKind;
+    r.npen = 0;
+    r.j.ra = mat_transform(f3_of(J.la), mat_from_quat(qA));
+    r.j.rb = mat_transform(f3_of(J.lb), mat_from_quat(qB));
+    // SpringSettings.ComputeErrorReductionAndSoftness (SpringSettings.cs:109-124), update rate 1/dt
+    const float dt = st.dt;
+    const float rate = 1 / dt;
+    const float multiplier = 1 / (dt * J.la.w + J.lb.w);
+    const float erp = J.la.w * multiplier;
+    r.j.soft = rate * multiplier;
+    const m3 rA = cross_matrix(r.j.ra), rB = cross_matrix(r.j.rb);
+    m3 k;
+    if (A.dyn && B.dyn) {
+        const m3 lin{A.im + B.im, 0, 0, 0, A.im + B.im, 0, 0, 0, A.im + B.im};
+        const m3 angA = mat_mul(mat_mul(rA, A.I), rA), angB = mat_mul(mat_mul(rB, B.I), rB);
+        k = mat_sub(mat_sub(lin, angA), angB);
+    } else if (A.dyn) {
+        const m3 lin{A.im, 0, 0, 0, A.im, 0, 0, 0, A.im};
+        k = mat_sub(lin, mat_mul(mat_mul(rA, A.I), rA));
+    } else {
+        const m3 lin{B.im, 0, 0, 0, B.im, 0, 0, 0, B.im};
+        k = mat_sub(lin, mat_mul(mat_mul(rB, B.I), rB));
+    }
+    k.m11 += r.j.soft;
+    k.m22 += r.j.soft;
+    k.m33 += r.j.soft;
+    r.j.M = mat_invert(k);
+    f3 err = pB + r.j.rb;
+    err = err - pA;
+    err = err - r.j.ra;
+    r.j.bias = err * -erp;
+    const float len = lensq3(r.j.bias);
+    if (len > J.max_corr_sq) {
+        const float m = J.acc.w / sqrtf(len);
+        r.j.bias.x *= m;
+        r.j.bias.y *= m;
+        r.j.bias.z *= m;
+    }
+    v.pacc[0] = J.acc.x;
+    v.pacc[1] = J.acc.y;
+    v.pacc[2] = J.acc.z;
+    v.pacc[3] = 0;
+    v.act = kGroupBit;
+    v.zero = 0;
+}
+
+// The joint's impulse on its connections (BallSocketJoint.cs:308-328, 364-383).
+BEPU_D void joint_apply(const RowConst& r, f3 impulse, BodyV& A, BodyV& B) {
+    if (A.dyn) {
+        const f3 l = -impulse;
+        apply_lin(A, l);
+        apply_ang(A, cross3(r.j.ra, l));
+    }
+    if (B.dyn) {
+        apply_lin(B, impulse);
+        apply_ang(B, cross3(r.j.rb, impulse));
+    }
+}
+
+// Solver.UnsafeSolveIteration (Solver.cs:285-314) -> BallSocketJoint.SolveIteration
+// (BallSocketJoint.cs:335-388).
+BEPU_D void solve_joint(const RowConst& r, RowVar& v, BodyV& A, BodyV& B, float mi, int mic) {
+    const f3 aVel = A.v + cross3(A.w, r.j.ra);
+    const f3 bVel = B.v + cross3(B.w, r.j.rb);
+    f3 lambda;
+    lambda.x = aVel.x - bVel.x + r.j.bias.x - r.j.soft * v.pacc[0];
+    lambda.y = aVel.y - bVel.y + r.j.bias.y - r.j.soft * v.pacc[1];
+    lambda.z = aVel.z - bVel.z + r.j.bias.z - r.j.soft * v.pacc[2];
+    lambda = mat_transform(lambda, r.j.M);
+    v.pacc[0] += lambda.x;
+    v.pacc[1] += lambda.y;
+    v.pacc[2] += lambda.z;
+    joint_apply(r, lambda, A, B);
+    const float impulse = fabsf(lambda.x) + fabsf(lambda.y) + fabsf(lambda.z);
+    if (impulse < mi) {
+        v.zero++;
+        if ((int)v.zero > mic) v.act &= ~kGroupBit;
+    } else {
+        v.zero = 0;
+    }
+}
+
 // Solver.UnsafeExclusiveUpdate -> group ExclusiveUpdate (Solver.cs:277-283)
 BEPU_D void warm_row(const RowConst& r, const RowVar& v, BodyV& A, BodyV& B) {
     if (!(v.act & kGroupBit)) return;
+    if (r.kind == kJointKind) {
+        joint_apply(r, f3{v.pacc[0], v.pacc[1], v.pacc[2]}, A, B);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         if (k < r.npen && (v.act & (1u << k))) row_apply(r.plin[k], r.pA[k], r.pB[k], v.pacc[k], A, B);
@@ -343,6 +486,10 @@ BEPU_D void account(RowVar& v, int k, float impulse, float min_impulse, int min_
 // NonConvexContactManifoldConstraint.cs:201-212).
 BEPU_D void solve_row(const RowConst& r, RowVar& v, BodyV& A, BodyV& B, float mi, int mic) {
     if (!(v.act & kGroupBit)) return;
+    if (r.kind == kJointKind) {
+        solve_joint(r, v, A, B, mi, mic);
+        return;
+    }
     int active = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -435,6 +582,13 @@ BEPU_D void writeback_row(PairState& s, const RowVar& v) {
         s.slacc[1] = v.sacc[1];
         s.twacc = v.tacc;
     }
+}
+
+// The joint's accumulated impulse back into its state (warm start of the next step).
+BEPU_D void writeback_joint(JointState& J, const RowVar& v) {
+    J.acc.x = v.pacc[0];
+    J.acc.y = v.pacc[1];
+    J.acc.z = v.pacc[2];
 }
 
 BEPU_D BodyV load_body_global(const DeviceWorld& w, uint32_t b) {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -250,15 +251,19 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     A(&d.hash_keys_cur, d.hash_cap); A(&d.hash_vals_cur, d.hash_cap);
     A(&d.hash_keys_old, d.hash_cap); A(&d.hash_vals_old, d.hash_cap);
     A(&d.cons, P);
-    A(&d.uf_parent, n); A(&d.isl_root, n); A(&d.cons_root, P); A(&d.isl_count, n + 1); A(&d.isl_start, n + 1);
-    A(&d.isl_cons, P); A(&d.isl_bodies, n); A(&d.body_local, n); A(&d.isl_medium, n);
-    d.cls_cap = std::max<uint32_t>(1, std::min<uint32_t>(n, P));
+    // Solver constraints: the pair slots, then the joints (ids pair_cap + j).
+    d.joint_cap = n;
+    const size_t C = (size_t)P + d.joint_cap;
+    A(&d.joints, d.joint_cap);
+    A(&d.uf_parent, n); A(&d.isl_root, n); A(&d.cons_root, C); A(&d.isl_count, n + 1); A(&d.isl_start, n + 1);
+    A(&d.isl_cons, C); A(&d.isl_bodies, n); A(&d.body_local, n); A(&d.isl_medium, n);
+    d.cls_cap = (uint32_t)std::max<size_t>(1, std::min<size_t>(n, C));
     A(&d.cls_list, 64 * (size_t)d.cls_cap);
-    A(&d.gcons, P); A(&d.color, P); A(&d.prio, P);
-    A(&d.inc_count, n + 1); A(&d.inc_offset, n + 1); A(&d.inc_list, 2 * (size_t)P);
+    A(&d.gcons, C); A(&d.color, C); A(&d.prio, C);
+    A(&d.inc_count, n + 1); A(&d.inc_offset, n + 1); A(&d.inc_list, 2 * C);
     A(&d.color_count, kMaxColors); A(&d.scan_part, 4096); A(&d.color_start, kMaxColors + 1); A(&d.color_cursor, kMaxColors);
-    A(&d.ordered, P);
-    A(&d.grow_c, P); A(&d.grow_v, P); A(&d.grow_pair, P);
+    A(&d.ordered, C);
+    A(&d.grow_c, C); A(&d.grow_v, C); A(&d.grow_pair, C);
     // deactivation: per body (labels are body ids), per sub-world
     A(&d.comp_root, n); A(&d.lab_act, n); A(&d.lab_count, n); A(&d.lab_min, n); A(&d.lab_flags, n);
     A(&d.act2, n); A(&d.uf2, n); A(&d.deact_m, (size_t)n + 1); A(&d.deact_e, (size_t)n + 1);
@@ -390,8 +395,14 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 sha
     w->staged.push_back(b);
     w->world_count[w->cur_world]++;
     w->ranges_dirty = true;
-    if (w->absent.size() < (size_t)n + 1) w->absent.resize((size_t)n + 1, 0);
+    if (w->absent.size() < (size_t)n + 1) {
+        w->absent.resize((size_t)n + 1, 0);
+        w->body_dynamic.resize((size_t)n + 1, 0);
+        w->body_world.resize((size_t)n + 1, 0);
+    }
     w->absent[n] = 0;
+    w->body_dynamic[n] = (b.flags & BF_DYNAMIC) ? 1 : 0;
+    w->body_world[n] = b.wid;
     if (id) *id = n;
     return BEPU_OK;
 }
@@ -702,8 +713,18 @@ bepu_status bepu_insert_worlds(bepu_world w, const bepu_body_desc* bodies, const
         w->world_count[id] = n;
         w->world_live[id] = 1;
         launch_fill_u32(w->d.wfault + id, 0u, 1, w->stream);  // a reused id starts unfaulted
-        if (w->absent.size() < (size_t)first + n) w->absent.resize((size_t)first + n, 0);
+        if (w->absent.size() < (size_t)first + n) {
+            w->absent.resize((size_t)first + n, 0);
+            w->body_dynamic.resize((size_t)first + n, 0);
+            w->body_world.resize((size_t)first + n, 0);
+        }
         std::fill(w->absent.begin() + first, w->absent.begin() + first + n, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            w->body_dynamic[first + i] = (recs[base + i].flags & BF_DYNAMIC) ? 1 : 0;
+            w->body_world[first + i] = id;
+        }
+        if (w->world_joints.size() <= id) w->world_joints.resize((size_t)id + 1, 0);
+        w->world_joints[id] = 0;  // a reused id starts its joint ordinals again
         if (append) {
             w->uploaded += n;
             w->cur_world = id;  // later add calls open a new sub-world after this one
@@ -733,6 +754,7 @@ bepu_status bepu_remove_world(bepu_world w, uint32_t world_id) {
     if (upload(w) != BEPU_OK) return BEPU_E_DEVICE;
     uint32_t first = w->world_first[world_id], n = w->world_count[world_id];
     if (n) launch_fill_u32(w->d.bflags + first, 0u, n, w->stream);  // absent: no box, no pairs, not dynamic
+    if (n) launch_joint_sweep(w->d, w->stream);  // its joints end
     w->world_live[world_id] = 0;
     w->world_count[world_id] = 0;
     w->ranges_dirty = true;
@@ -997,8 +1019,9 @@ static bepu_status enqueue_step(bepu_world w, hipEvent_t* e) {
     if (e) (void)hipEventRecord(e[3], s);
     if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
     if (e) (void)hipEventRecord(e[4], s);
-    // Islands woken by this step's new constraints, stale frozen pairs,
-    // split activation.
+    // Joints join their bodies' islands; then the islands woken by this
+    // step's new constraints, stale frozen pairs, split activation.
+    launch_joint_edges(d, s);
     launch_activity(d, s);
     launch_islands(d, w->scan_temp, w->scan_bytes, s);
     if (e) (void)hipEventRecord(e[5], s);
@@ -1295,7 +1318,7 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     float mp;
     std::memcpy(&mp, &c[CTR_MAXPEN], 4);
     out->max_penetration = any ? mp : 0.0f;
-    out->global_manifolds = any ? std::min(c[CTR_GCONS], w->d.pair_cap) : 0;
+    out->global_manifolds = any ? std::min(c[CTR_GCONS], w->d.pair_cap + w->d.joint_cap) : 0;
     out->convex_pairs[0] = any ? c[CTR_CONVEX] + c[CTR_CONVEX_OTHER0] : 0;
     out->convex_pairs[1] = any ? c[CTR_CONVEX_SHALLOW] + c[CTR_CONVEX_OTHER0 + 1] : 0;
     out->convex_pairs[2] = any ? c[CTR_CONVEX_DEEP] + c[CTR_CONVEX_OTHER0 + 2] : 0;
@@ -1319,7 +1342,74 @@ bepu_status bepu_remove(bepu_world w, uint32_t id) {
     // (DeactivationManager.RemoveSimulationIslandFromMember).
     launch_removal_wake(w->d, id, w->stream);
     launch_fill_u32(w->d.bflags + id, 0u, 1, w->stream);
+    // Its joints end; their other connections are activated.
+    launch_joint_sweep(w->d, w->stream);
     return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+// new BallSocketJoint(connectionA, connectionB, anchor) added to the space
+// (BallSocketJoint.cs:57-63): the bodies' offsets to the anchor in their
+// local frames, from their poses after the steps queued so far.  Both bodies
+// belong to one sub-world; at least one is dynamic ("Cannot constrain two
+// kinematic bodies", BallSocketJoint.cs:281).
+bepu_status bepu_add_ball_socket(bepu_world w, uint32_t a, uint32_t b, const float* anchor, uint32_t* id) {
+    if (!w || !anchor || a == b || !std::isfinite(anchor[0]) || !std::isfinite(anchor[1]) ||
+        !std::isfinite(anchor[2]))
+        return BEPU_E_INVALID_ARG;
+    const uint32_t total = w->uploaded + (uint32_t)w->staged.size();
+    if (a >= total || b >= total || w->absent[a] || w->absent[b]) return BEPU_E_NOT_FOUND;
+    if (!w->body_dynamic[a] && !w->body_dynamic[b]) return BEPU_E_INVALID_ARG;
+    if (w->body_world[a] != w->body_world[b]) return BEPU_E_INVALID_ARG;
+    if (w->d.njoints >= w->d.joint_cap) return BEPU_E_CAPACITY;
+    // The joint reads the bodies' poses on the device: upload staged bodies first.
+    bepu_status st = upload(w);
+    if (st != BEPU_OK) return st;
+    const uint32_t wid = w->body_world[a];
+    if (w->world_joints.size() <= wid) w->world_joints.resize((size_t)wid + 1, 0);
+    const uint32_t jid = w->d.njoints;
+    launch_joint_init(w->d, jid, a, b, w->world_joints[wid]++, make_float4(anchor[0], anchor[1], anchor[2], 0),
+                      w->stream);
+    w->d.njoints++;
+    if (id) *id = jid;
+    return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+// SpringSettings.Stiffness / Damping (clamped at zero, SpringSettings.cs:61-80;
+// both zero is refused, as ComputeErrorReductionAndSoftness throws) and
+// Joint.MaxCorrectiveVelocity (Joint.cs:24-38).
+bepu_status bepu_set_joint_settings(bepu_world w, uint32_t joint, float stiffness, float damping,
+                                    float max_corrective_velocity) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    if (joint >= w->d.njoints) return BEPU_E_NOT_FOUND;
+    if (std::isnan(stiffness) || std::isnan(damping) || std::isnan(max_corrective_velocity)) return BEPU_E_INVALID_ARG;
+    stiffness = std::max(0.0f, stiffness);
+    damping = std::max(0.0f, damping);
+    if (stiffness == 0 && damping == 0) return BEPU_E_INVALID_ARG;
+    const float mc = std::max(0.0f, max_corrective_velocity);
+    const float mc2 = mc >= FLT_MAX ? FLT_MAX : mc * mc;
+    launch_joint_settings(w->d, joint, stiffness, damping, mc, mc2, w->stream);
+    return hipGetLastError() == hipSuccess ? BEPU_OK : BEPU_E_DEVICE;
+}
+
+// BallSocketJoint.TotalImpulse of every joint, three floats each.
+bepu_status bepu_get_joint_impulses(bepu_world w, float* out, uint32_t capacity, uint32_t* count) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    const uint32_t n = w->d.njoints;
+    if (count) *count = n;
+    if (!out) return BEPU_OK;
+    if (capacity < n) return BEPU_E_CAPACITY;
+    bepu_status err = poll_errors(w);
+    if (err == BEPU_E_DEVICE && !(w->sticky & STICKY_GBAR_ABORT)) return err;
+    std::vector<JointState> js(n);
+    if (n && hipMemcpyAsync(js.data(), w->d.joints, n * sizeof(JointState), hipMemcpyDeviceToHost, w->stream) != hipSuccess)
+        return BEPU_E_DEVICE;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    for (uint32_t k = 0; k < n; ++k) {
+        out[3 * k] = js[k].acc.x;
+        out[3 * k + 1] = js[k].acc.y;
+        out[3 * k + 2] = js[k].acc.z;
+    }
+    return err;
 }
 
 // Entity state setters for each listed body, the parts `fields` selects:

@@ -107,6 +107,26 @@ struct BodyRecord {
 };
 static_assert(sizeof(BodyRecord) == 240, "BodyRecord layout");
 
+// A BallSocketJoint (Constraints/TwoEntity/Joints/BallSocketJoint.cs): its
+// connections, local anchors, spring settings and accumulated impulse.  In
+// the solver's constraint ids, those at and above pair_cap name joints (id -
+// pair_cap); those below are pair slots.
+struct JointState {
+    uint32_t a, b;      // ConnectionA / ConnectionB
+    uint32_t local;     // ordinal among its sub-world's joints: colouring key kJointKey | local
+    uint32_t flags;     // JF_DEAD: a connection was removed, the joint no longer acts
+    float4 la;          // localAnchorA; w: SpringSettings.Stiffness
+    float4 lb;          // localAnchorB; w: SpringSettings.Damping
+    float4 acc;         // accumulatedImpulse; w: MaxCorrectiveVelocity
+    float max_corr_sq;  // its square (float max when unlimited)
+    int32_t color;      // colour in the last solve (-1 before)
+    uint32_t pad[2];
+};
+static_assert(sizeof(JointState) == 80, "JointState layout");
+enum { JF_DEAD = 1u };
+// The joints' colouring keys lie above every world-local pair key.
+constexpr uint64_t kJointKey = 1ull << 63;
+
 struct RowConst;
 struct RowVar;
 
@@ -147,6 +167,10 @@ struct DeviceWorld {
     float* minrad;             // per body: the shape's MinimumRadius
     uint32_t* toi;             // per body: smallest time of impact of its pairs this step (float bits)
     float4* qold;              // per passive / continuous body: its orientation before this step's update
+    // joints
+    JointState* joints;
+    uint32_t njoints;
+    uint32_t joint_cap;
     // broadphase
     uint32_t* morton;          // Morton codes (tree broadphase)
     uint32_t* morton_sorted;
@@ -333,6 +357,36 @@ __device__ __forceinline__ uint64_t local_pair_key(const DeviceWorld& w, uint64_
     return key - ((base << 32) | base);
 }
 
+// A joint in the solver: not dead, and the island of a dynamic connection
+// active (SolverUpdateable.UpdateSolverActivity, SolverUpdateable.cs:202-218).
+__device__ __forceinline__ bool solver_joint(const DeviceWorld& w, const JointState& j) {
+    if (j.flags & JF_DEAD) return false;
+    const uint32_t fa = w.bflags[j.a], fb = w.bflags[j.b];
+    return ((fa & BF_DYNAMIC) && !(fa & BF_ASLEEP)) || ((fb & BF_DYNAMIC) && !(fb & BF_ASLEEP));
+}
+// A joint joins the islands of its connections when both are dynamic.
+__device__ __forceinline__ bool joint_edge(const DeviceWorld& w, const JointState& j) {
+    return !(j.flags & JF_DEAD) && (w.bflags[j.a] & BF_DYNAMIC) && (w.bflags[j.b] & BF_DYNAMIC);
+}
+// The two bodies of solver constraint `id` (a pair slot or pair_cap + joint).
+__device__ __forceinline__ void cons_ends(const DeviceWorld& w, uint32_t id, uint32_t& a, uint32_t& b) {
+    if (id >= w.pair_cap) {
+        const JointState& j = w.joints[id - w.pair_cap];
+        a = j.a;
+        b = j.b;
+    } else {
+        const PairHot h = w.hot_cur[id];
+        a = h.a;
+        b = h.b;
+    }
+}
+// Its colouring priority: mix64 of its world-local key.
+__device__ __forceinline__ uint64_t cons_priority(const DeviceWorld& w, uint32_t id) {
+    if (id >= w.pair_cap) return mix64(kJointKey | w.joints[id - w.pair_cap].local);
+    const PairState& s = w.pairs_cur[id];
+    return mix64(local_pair_key(w, s.key, s.a));
+}
+
 // Host-side launchers (one per stage), defined in the kernel sources.
 void launch_body_pre(const DeviceWorld& w, hipStream_t st);
 void launch_broadphase(const DeviceWorld& w, void* sort_temp, size_t sort_temp_bytes, hipStream_t st);
@@ -355,6 +409,16 @@ void launch_integrate(const DeviceWorld& w, hipStream_t st);
 // impact of the pairs with a continuous body, then the linear motion of the
 // passive and continuous bodies clamped to their smallest time of impact.
 void launch_ccd(const DeviceWorld& w, hipStream_t st);
+// Joints: their island edges in this step's union-find (after the narrow
+// phase, before the activity pass and the islands); set-up of joint `id`
+// from the bodies' current poses; its spring settings; the joints of removed
+// bodies end (their present dynamic connections are activated).
+void launch_joint_edges(const DeviceWorld& w, hipStream_t st);
+void launch_joint_init(const DeviceWorld& w, uint32_t id, uint32_t a, uint32_t b, uint32_t local, float4 anchor,
+                       hipStream_t st);
+void launch_joint_settings(const DeviceWorld& w, uint32_t id, float stiffness, float damping, float max_corr,
+                           float max_corr_sq, hipStream_t st);
+void launch_joint_sweep(const DeviceWorld& w, hipStream_t st);
 size_t broadphase_sort_temp_bytes(uint32_t n);
 // Level offsets/counts of the implicit tree over n leaves (level 0 = leaves).
 void tree_layout(uint32_t n, uint32_t* levels, uint32_t* off, uint32_t* cnt);

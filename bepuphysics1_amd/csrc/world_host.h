@@ -79,6 +79,9 @@ struct bepu_world_s {
     int device = 0;                         // the HIP device the world lives on
     uint32_t sticky = 0;                    // STICKY_* bits seen by the host since the last bepu_clear_error
     std::vector<uint8_t> absent;            // per body id: removed (bepu_remove, or its world was removed)
+    std::vector<uint8_t> body_dynamic;      // per body id: dynamic (mass > 0) when it was made
+    std::vector<uint32_t> body_world;       // per body id: its sub-world
+    std::vector<uint32_t> world_joints;     // per world id: joints made in it (the next joint's ordinal)
     bepu::StateRecord* sstage = nullptr;    // device staging of motion states written between steps
     uint32_t sstage_cap = 0;
     // Inserted body records go through a ring of pinned host buffers whose

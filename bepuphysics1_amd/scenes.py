@@ -305,3 +305,40 @@ def shooting_gallery(n=64, seed=0, speed=(120.0, 480.0), shapes=(SHAPE_BOX, SHAP
         rows.append(_row(s, dims, 1.0, (x, rng.uniform(1.0, 7.0), rng.uniform(-6.0, 6.0)), quat=q[i], vel=vel,
                          angvel=tuple(rng.uniform(-5, 5, 3)), flags=FLAG_ALWAYS_ACTIVE | mode))
     return make(rows)
+
+
+def chains(n_chains=6, links=8, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE),
+           flags=FLAG_ALWAYS_ACTIVE, spacing=3.0):
+    """Chains of `links` bodies hung from kinematic anchors by ball-socket
+    joints (BallSocketJoint), started sideways so that they swing down onto a
+    ground and a few loose boxes.  Returns (scene, joints) where joints lists
+    (body a, body b, anchor xyz) in the order to add them."""
+    rng = np.random.default_rng(seed)
+    rows = [ground(80.0, 1.0, 80.0)]
+    joints = []
+    half = 0.75  # half the distance between joints along a chain (hulls are up to 1.5 across)
+    top = links * 2 * half + 2.0
+    for c in range(n_chains):
+        x0 = (c - n_chains / 2) * spacing
+        anchor_id = len(rows)
+        rows.append(_row(SHAPE_BOX, (0.3, 0.3, 0.3), 0.0, (x0, top, 0.0), flags=0))
+        prev, prev_end = anchor_id, np.array([x0, top, 0.0])
+        for k in range(links):
+            s = int(rng.choice(np.array(shapes)))
+            # lay the chain out sideways from the anchor
+            centre = prev_end + np.array([half, 0.0, 0.0])
+            if s == SHAPE_SPHERE:
+                dims = (0.25, 0.0, 0.0)
+            elif s == SHAPE_BOX:
+                dims = (0.5, 0.3, 0.3)
+            elif s == SHAPE_CAPSULE:
+                dims = (0.3, 0.15, 0.0)
+            else:
+                dims = (float(rng.integers(0, HULL_LIBRARY_SIZE)), 0.0, 0.0)
+            bid = len(rows)
+            rows.append(_row(s, dims, float(rng.uniform(0.5, 2.0)), tuple(centre), flags=flags))
+            joints.append((prev, bid, tuple(float(v) for v in prev_end)))
+            prev, prev_end = bid, centre + (centre - prev_end)
+    for k in range(4):
+        rows.append(_row(SHAPE_BOX, (0.8, 0.8, 0.8), 1.0, ((k - 2) * 2.0, 0.4, 3.0), flags=flags))
+    return make(rows), joints

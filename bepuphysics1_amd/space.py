@@ -368,6 +368,31 @@ class Space:
                                           n.value, ctypes.byref(n)), "get_activity")
         return out[:n.value].astype(bool)
 
+    def add_ball_socket(self, a, b, anchor):
+        """Space.Add(new BallSocketJoint(a, b, anchor)): joins bodies a and b
+        (ids, or entities of this space) at the world point `anchor`; returns
+        the joint id."""
+        a = a.Id if isinstance(a, Entity) else int(a)
+        b = b.Id if isinstance(b, Entity) else int(b)
+        p = (ctypes.c_float * 3)(*[float(x) for x in anchor])
+        jid = ctypes.c_uint32()
+        check(self._lib.bepu_add_ball_socket(self._handle(), a, b, p, ctypes.byref(jid)), "add_ball_socket")
+        return jid.value
+
+    def set_joint_settings(self, joint, stiffness=600000.0, damping=90000.0, max_corrective=3.4028234663852886e38):
+        """SpringSettings (stiffness, damping) and MaxCorrectiveVelocity of a joint."""
+        check(self._lib.bepu_set_joint_settings(self._handle(), int(joint), float(stiffness), float(damping),
+                                                float(max_corrective)), "set_joint_settings")
+
+    def joint_impulses(self):
+        """BallSocketJoint.TotalImpulse of every joint, [n, 3]."""
+        n = ctypes.c_uint32()
+        check(self._lib.bepu_get_joint_impulses(self._handle(), None, 0, ctypes.byref(n)), "get_joint_impulses")
+        out = np.zeros((max(n.value, 1), 3), np.float32)
+        check(self._lib.bepu_get_joint_impulses(self._handle(), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                n.value, ctypes.byref(n)), "get_joint_impulses")
+        return out[:n.value]
+
     POSITION_UPDATE_MODES = {"discrete": 0, "passive": 1, "continuous": 2}
 
     def set_position_update_mode(self, ids, mode):

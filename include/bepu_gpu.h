@@ -162,7 +162,7 @@ typedef struct bepu_step_stats {
     uint32_t colors;
     uint32_t steps;           /* steps run since creation */
     float max_penetration;    /* deepest contact of the last step */
-    uint32_t global_manifolds; /* manifolds in islands too large for one workgroup */
+    uint32_t global_manifolds; /* constraints (manifolds and joints) in islands too large for one workgroup */
     uint32_t convex_pairs[3]; /* general convex pairs (any hull or capsule) tested in the last step,
                                  by pair tester state: separated, shallow, deep */
 } bepu_step_stats;
@@ -377,6 +377,33 @@ bepu_status bepu_set_motion_states(bepu_world world, const uint32_t* ids, const 
 #define BEPU_POSITION_UPDATE_PASSIVE 1
 #define BEPU_POSITION_UPDATE_CONTINUOUS 2
 bepu_status bepu_set_position_update_mode(bepu_world world, const uint32_t* ids, uint32_t n, int32_t mode);
+
+/* Two-body joints in the solver (Constraints/TwoEntity/Joints/).  A joint is a
+ * constraint of the islands, the colouring and the solve like a contact
+ * manifold constraint; joined bodies are one island (they sleep and wake
+ * together).
+ * new BallSocketJoint(connectionA, connectionB, anchor) added to the space
+ * (BallSocketJoint.cs:57-63): keeps the world point `anchor` (xyz) of both
+ * bodies together, from their poses after the steps already queued.  Both
+ * bodies belong to one sub-world, at least one is dynamic (the reference
+ * throws "Cannot constrain two kinematic bodies"; a null connection, the
+ * reference's WorldEntity, is given as a kinematic body).  SpringSettings
+ * default to stiffness 600000 and damping 90000, no corrective velocity limit.
+ * BEPU_E_NOT_FOUND for an unknown or removed body, BEPU_E_INVALID_ARG for two
+ * kinematic bodies, one body twice, different sub-worlds or a non-finite
+ * anchor, BEPU_E_CAPACITY beyond the world's body capacity in joints.  A
+ * joint of a removed body ends (its other connection is activated). */
+bepu_status bepu_add_ball_socket(bepu_world world, uint32_t body_a, uint32_t body_b, const float* anchor,
+                                 uint32_t* joint_id_out);
+/* SpringSettings.Stiffness / Damping (clamped at 0; both 0 is BEPU_E_INVALID_ARG,
+ * SpringSettings.cs:61-124) and Joint.MaxCorrectiveVelocity (Joint.cs:24-38;
+ * FLT_MAX or more: no limit) of a joint, from the next step on. */
+bepu_status bepu_set_joint_settings(bepu_world world, uint32_t joint_id, float stiffness, float damping,
+                                    float max_corrective_velocity);
+/* BallSocketJoint.TotalImpulse (the accumulated impulse, BallSocketJoint.cs:146-149)
+ * of every joint in id order, three floats each; `count_out` gets the number of
+ * joints (a NULL `impulses` only counts). */
+bepu_status bepu_get_joint_impulses(bepu_world world, float* impulses, uint32_t capacity, uint32_t* count_out);
 
 /* The settings block of a world, readable and writable between steps:
  * Solver.IterationLimit (Constraints/Solver.cs:19), ForceUpdater.Gravity

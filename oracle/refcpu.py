@@ -55,6 +55,10 @@ def lib():
                                              ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         L.refcpu_set_position_update_mode.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
                                                       ctypes.c_int]
+        L.refcpu_add_ball_socket.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, fpp,
+                                             ctypes.POINTER(ctypes.c_uint32)]
+        L.refcpu_set_joint_settings.argtypes = [vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.refcpu_get_joint_impulses.argtypes = [vp, fpp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         L.refcpu_cast.argtypes = [ctypes.c_int, fpp, fpp, ctypes.c_int, fpp, fpp, fpp, ctypes.c_float,
                                   ctypes.c_float, ctypes.c_float, fpp]
         shape_args = [fpp, fpp, ctypes.c_int, fpp, fpp, fpp, ctypes.c_int, fpp]
@@ -204,6 +208,29 @@ class World:
         """0: colour by colour (the GPU's order); 1: the reference's permuted
         single-threaded order (Solver.cs:237-263, PermutationMapper.cs)."""
         return lib().refcpu_set_solve_order(self.h, int(mode))
+
+    def add_ball_socket(self, a, b, anchor):
+        """new BallSocketJoint(a, b, anchor) added to the space; returns the
+        joint id (raises on a bad argument)."""
+        p = np.ascontiguousarray(anchor, np.float32)
+        jid = ctypes.c_uint32()
+        rc = lib().refcpu_add_ball_socket(self.h, int(a), int(b), _fptr(p), ctypes.byref(jid))
+        if rc != 0:
+            raise ValueError("refcpu_add_ball_socket failed with status %d" % rc)
+        return jid.value
+
+    def set_joint_settings(self, joint, stiffness=600000.0, damping=90000.0, max_corrective=3.4028234663852886e38):
+        """SpringSettings and MaxCorrectiveVelocity of a joint; returns the status code."""
+        return lib().refcpu_set_joint_settings(self.h, int(joint), float(stiffness), float(damping),
+                                               float(max_corrective))
+
+    def joint_impulses(self):
+        """BallSocketJoint.TotalImpulse of every joint, [n, 3]."""
+        n = ctypes.c_uint32()
+        lib().refcpu_get_joint_impulses(self.h, None, 0, ctypes.byref(n))
+        out = np.zeros((max(n.value, 1), 3), np.float32)
+        lib().refcpu_get_joint_impulses(self.h, _fptr(out), n.value, ctypes.byref(n))
+        return out[:n.value]
 
     def set_position_update_mode(self, ids, mode):
         """Entity.PositionUpdateMode for each listed body: 0 Discrete, 1

@@ -160,6 +160,29 @@ struct Pair {
     float toi;
 };
 
+// BallSocketJoint (Constraints/TwoEntity/Joints/BallSocketJoint.cs) and its
+// SolverUpdateable bookkeeping (SolverSettings: currentIterations,
+// iterationsAtZeroImpulse; isActiveInSolver).
+struct JointC {
+    uint32_t a, b;       // ConnectionA / ConnectionB
+    uint32_t local;      // its ordinal among the joints of its world (the colouring key)
+    bool dead;           // a connection was removed: the joint no longer acts
+    V3 localA, localB;   // localAnchorA / localAnchorB
+    float stiffness, damping;        // SpringSettings (600000, 90000)
+    float max_corr, max_corr_sq;     // Joint.MaxCorrectiveVelocity (float.MaxValue)
+    V3 acc;              // accumulatedImpulse
+    // prestep
+    V3 offA, offB, bias;
+    M3 mass;
+    float softness;
+    bool ais;
+    int cur, zero;
+    int color;
+};
+
+// The joints' colouring keys lie above every pair key.
+constexpr uint64_t kJointKey = 1ull << 63;
+
 struct Settings {
     bepu_settings s;
 };
@@ -276,6 +299,7 @@ struct refcpu_world {
     int solve_order = 0;
     int64_t perm_index = 0;
     bool ccd = false;  // a body was ever passive or continuous
+    std::vector<JointC> joints;
 };
 
 namespace {
@@ -452,20 +476,25 @@ void deactivation_stage(refcpu_world* w) {
         const Body& b = w->bodies[i];
         if (b.dynamic && !b.removed && b.wake) act[label_of(w, i)] = 1;
     }
-    for (const Pair& p : w->pairs.vals) {
-        if (p.ccount <= 0) continue;
-        Body& A = w->bodies[p.a];
-        Body& B = w->bodies[p.b];
-        if (A.removed || B.removed) continue;
+    // A moving kinematic member keeps its connected islands awake: its
+    // connections are its contact constraints and its joints.
+    auto kinematic_rule = [&](uint32_t a, uint32_t b) {
+        Body& A = w->bodies[a];
+        Body& B = w->bodies[b];
+        if (A.removed || B.removed) return;
         if (kinematic_active(A) && B.dynamic) {
-            act[label_of(w, p.b)] = 1;
+            act[label_of(w, b)] = 1;
             B.nodeact = true;
         }
         if (kinematic_active(B) && A.dynamic) {
-            act[label_of(w, p.a)] = 1;
+            act[label_of(w, a)] = 1;
             A.nodeact = true;
         }
-    }
+    };
+    for (const Pair& p : w->pairs.vals)
+        if (p.ccount > 0) kinematic_rule(p.a, p.b);
+    for (const JointC& j : w->joints)
+        if (!j.dead) kinematic_rule(j.a, j.b);
     struct Agg {
         uint32_t members = 0, cands = 0, minid = kNoLabel;
         bool active = false, nodeact = false;
@@ -1147,10 +1176,21 @@ uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-void colour_constraints(refcpu_world* w, std::vector<Pair*>& cons) {
-    std::vector<std::pair<uint64_t, Pair*>> pri;
+// A constraint of the solver: a contact manifold constraint (a pair with
+// contacts) or a joint.
+struct CRef {
+    Pair* p;
+    JointC* j;
+    uint32_t a() const { return p ? p->a : j->a; }
+    uint32_t b() const { return p ? p->b : j->b; }
+    uint64_t key() const { return p ? pair_key(p->a, p->b) : (kJointKey | j->local); }
+    int& color() const { return p ? p->color : j->color; }
+};
+
+void colour_constraints(refcpu_world* w, std::vector<CRef>& cons) {
+    std::vector<std::pair<uint64_t, CRef>> pri;
     pri.reserve(cons.size());
-    for (Pair* p : cons) pri.push_back({mix64(pair_key(p->a, p->b)), p});
+    for (const CRef& c : cons) pri.push_back({mix64(c.key()), c});
     std::sort(pri.begin(), pri.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
     // Per dynamic body: the colours already taken, as a bitmask for the first
     // 64 and a list beyond.
@@ -1158,15 +1198,16 @@ void colour_constraints(refcpu_world* w, std::vector<Pair*>& cons) {
     std::map<uint32_t, std::vector<int>> more;
     int ncol = 0;
     for (auto& e : pri) {
-        Pair* p = e.second;
-        bool da = w->bodies[p->a].dynamic, db = w->bodies[p->b].dynamic;
-        uint64_t m = (da ? mask[p->a] : 0) | (db ? mask[p->b] : 0);
+        const CRef& cr = e.second;
+        const uint32_t ia = cr.a(), ib = cr.b();
+        bool da = w->bodies[ia].dynamic, db = w->bodies[ib].dynamic;
+        uint64_t m = (da ? mask[ia] : 0) | (db ? mask[ib] : 0);
         int c;
         if (~m) {
             c = __builtin_ctzll(~m);
         } else {
             std::vector<char> taken;
-            for (uint32_t bi : {p->a, p->b}) {
+            for (uint32_t bi : {ia, ib}) {
                 if (!w->bodies[bi].dynamic) continue;
                 auto it = more.find(bi);
                 if (it == more.end()) continue;
@@ -1178,17 +1219,17 @@ void colour_constraints(refcpu_world* w, std::vector<Pair*>& cons) {
             c = 64;
             while ((size_t)(c - 64) < taken.size() && taken[c - 64]) ++c;
         }
-        p->color = c;
+        cr.color() = c;
         if (c + 1 > ncol) ncol = c + 1;
-        for (uint32_t bi : {p->a, p->b}) {
+        for (uint32_t bi : {ia, ib}) {
             if (!w->bodies[bi].dynamic) continue;
             if (c < 64) mask[bi] |= 1ull << c;
             else more[bi].push_back(c);
         }
     }
-    std::sort(cons.begin(), cons.end(), [](Pair* x, Pair* y) {
-        if (x->color != y->color) return x->color < y->color;
-        return pair_key(x->a, x->b) < pair_key(y->a, y->b);
+    std::sort(cons.begin(), cons.end(), [](const CRef& x, const CRef& y) {
+        if (x.color() != y.color()) return x.color() < y.color();
+        return x.key() < y.key();
     });
     w->ncolors = ncol;
 }
@@ -1739,6 +1780,168 @@ void solve_iteration(refcpu_world* w, Pair& p) {
 }
 
 // ---------------------------------------------------------------------------
+// BallSocketJoint (Constraints/TwoEntity/Joints/BallSocketJoint.cs)
+
+// SolverUpdateable.UpdateSolverActivity (SolverUpdateable.cs:202-218): a
+// joint is solved while the island of one of its dynamic connections is
+// active (a kinematic connection has no island).
+bool joint_active(const refcpu_world* w, const JointC& j) {
+    if (j.dead) return false;
+    const Body& A = w->bodies[j.a];
+    const Body& B = w->bodies[j.b];
+    return (A.dynamic && !A.asleep) || (B.dynamic && !B.asleep);
+}
+
+// Matrix3x3.CreateCrossProduct (BEPUutilities/Matrix3x3.cs:378-389)
+inline M3 m3_cross(const V3& v) { return M3{0, -v.z, v.y, v.z, 0, -v.x, -v.y, v.x, 0}; }
+// Matrix3x3.Subtract (Matrix3x3.cs:973-997)
+inline M3 m3_sub(const M3& a, const M3& b) {
+    return M3{a.m11 - b.m11, a.m12 - b.m12, a.m13 - b.m13, a.m21 - b.m21, a.m22 - b.m22,
+              a.m23 - b.m23, a.m31 - b.m31, a.m32 - b.m32, a.m33 - b.m33};
+}
+// Matrix3x3.CreateScale (Matrix3x3.cs:437-440)
+inline M3 m3_diag(float s) { return M3{s, 0, 0, 0, s, 0, 0, 0, s}; }
+// Matrix3x3.Invert (Matrix3x3.cs:505-530) with Determinant (:1231-1235)
+inline M3 m3_invert(const M3& m) {
+    const float det = m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 -
+                      m.m31 * m.m22 * m.m13 - m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
+    const float d = 1 / det;
+    M3 r;
+    r.m11 = (m.m22 * m.m33 - m.m23 * m.m32) * d;
+    r.m12 = (m.m13 * m.m32 - m.m33 * m.m12) * d;
+    r.m13 = (m.m12 * m.m23 - m.m22 * m.m13) * d;
+    r.m21 = (m.m23 * m.m31 - m.m21 * m.m33) * d;
+    r.m22 = (m.m11 * m.m33 - m.m13 * m.m31) * d;
+    r.m23 = (m.m13 * m.m21 - m.m11 * m.m23) * d;
+    r.m31 = (m.m21 * m.m32 - m.m22 * m.m31) * d;
+    r.m32 = (m.m12 * m.m31 - m.m11 * m.m32) * d;
+    r.m33 = (m.m11 * m.m22 - m.m12 * m.m21) * d;
+    return r;
+}
+
+// Solver.UnsafePrestep -> BallSocketJoint.Update (Solver.cs:265-275,
+// BallSocketJoint.cs:229-300): world offsets, the softened effective mass
+// of the point constraint, and the bias velocity toward closing the gap.
+void joint_prestep(refcpu_world* w, JointC& j) {
+    j.ais = joint_active(w, j);
+    if (!j.ais) return;
+    j.cur = 0;
+    j.zero = 0;
+    const Body& A = w->bodies[j.a];
+    const Body& B = w->bodies[j.b];
+    const float dt = w->s.dt;
+    j.offA = m3_transform(j.localA, A.R);
+    j.offB = m3_transform(j.localB, B.R);
+    // SpringSettings.ComputeErrorReductionAndSoftness (SpringSettings.cs:109-124), update rate 1/dt
+    const float rate = 1 / dt;
+    const float multiplier = 1 / (dt * j.stiffness + j.damping);
+    const float erp = j.stiffness * multiplier;
+    j.softness = rate * multiplier;
+    const M3 rA = m3_cross(j.offA), rB = m3_cross(j.offB);
+    M3 k;
+    if (A.dynamic && B.dynamic) {
+        const M3 lin = m3_diag(A.inv_mass + B.inv_mass);
+        M3 angA = m3_mul(rA, A.inertia_inv), angB = m3_mul(rB, B.inertia_inv);
+        angA = m3_mul(angA, rA);
+        angB = m3_mul(angB, rB);
+        k = m3_sub(lin, angA);
+        k = m3_sub(k, angB);
+    } else if (A.dynamic) {
+        const M3 lin = m3_diag(A.inv_mass);
+        M3 angA = m3_mul(rA, A.inertia_inv);
+        angA = m3_mul(angA, rA);
+        k = m3_sub(lin, angA);
+    } else {
+        const M3 lin = m3_diag(B.inv_mass);
+        M3 angB = m3_mul(rB, B.inertia_inv);
+        angB = m3_mul(angB, rB);
+        k = m3_sub(lin, angB);
+    }
+    k.m11 += j.softness;
+    k.m22 += j.softness;
+    k.m33 += j.softness;
+    j.mass = m3_invert(k);
+    V3 err = add(B.pos, j.offB);
+    err = sub(err, A.pos);
+    err = sub(err, j.offA);
+    j.bias = mul(err, -erp);
+    const float len = length_sq(j.bias);
+    if (len > j.max_corr_sq) {
+        const float m = j.max_corr / std::sqrt(len);
+        j.bias.x *= m;
+        j.bias.y *= m;
+        j.bias.z *= m;
+    }
+}
+
+// The joint's impulse on its connections (BallSocketJoint.cs:308-328, 364-383).
+inline void joint_apply(const JointC& j, const V3& impulse, Body& A, Body& B) {
+    if (A.dynamic) {
+        const V3 l = neg(impulse);
+        const float lv[3] = {l.x, l.y, l.z};
+        apply_linear(A, lv);
+        const V3 t = cross(j.offA, l);
+        const float tv[3] = {t.x, t.y, t.z};
+        apply_angular(A, tv);
+    }
+    if (B.dynamic) {
+        const float lv[3] = {impulse.x, impulse.y, impulse.z};
+        apply_linear(B, lv);
+        const V3 t = cross(j.offB, impulse);
+        const float tv[3] = {t.x, t.y, t.z};
+        apply_angular(B, tv);
+    }
+}
+
+// Solver.UnsafeExclusiveUpdate -> BallSocketJoint.ExclusiveUpdate (warm start)
+void joint_warm(refcpu_world* w, JointC& j) {
+    if (!j.ais) return;
+    joint_apply(j, j.acc, w->bodies[j.a], w->bodies[j.b]);
+}
+
+// Solver.UnsafeSolveIteration (Solver.cs:285-314) -> BallSocketJoint.SolveIteration
+// (BallSocketJoint.cs:335-388)
+void joint_solve(refcpu_world* w, JointC& j) {
+    if (!j.ais) return;
+    j.cur++;
+    if (j.cur > w->s.iteration_limit) {
+        j.ais = false;
+        return;
+    }
+    Body& A = w->bodies[j.a];
+    Body& B = w->bodies[j.b];
+    const V3 aVel = add(A.v, cross(A.w, j.offA));
+    const V3 bVel = add(B.v, cross(B.w, j.offB));
+    V3 lambda;
+    lambda.x = aVel.x - bVel.x + j.bias.x - j.softness * j.acc.x;
+    lambda.y = aVel.y - bVel.y + j.bias.y - j.softness * j.acc.y;
+    lambda.z = aVel.z - bVel.z + j.bias.z - j.softness * j.acc.z;
+    lambda = m3_transform(lambda, j.mass);
+    j.acc = add(j.acc, lambda);
+    joint_apply(j, lambda, A, B);
+    const float impulse = std::fabs(lambda.x) + std::fabs(lambda.y) + std::fabs(lambda.z);
+    if (impulse < w->s.min_impulse) {
+        j.zero++;
+        if (j.zero > w->s.min_iteration_count) j.ais = false;
+    } else {
+        j.zero = 0;
+    }
+}
+
+void con_prestep(refcpu_world* w, const CRef& c) {
+    if (c.p) prestep(w, *c.p);
+    else joint_prestep(w, *c.j);
+}
+void con_warm(refcpu_world* w, const CRef& c) {
+    if (c.p) warm_start(w, *c.p);
+    else joint_warm(w, *c.j);
+}
+void con_solve(refcpu_world* w, const CRef& c) {
+    if (c.p) solve_iteration(w, *c.p);
+    else joint_solve(w, *c.j);
+}
+
+// ---------------------------------------------------------------------------
 // The reference's single-threaded solve order (Solver.UpdateSingleThreaded,
 // Solver.cs:237-263; BEPUutilities/PermutationMapper.cs:27-127): prestep and
 // exclusive update (warm start) in list order, then iterationLimit x n visits
@@ -1776,17 +1979,17 @@ inline int64_t permuted_index(int64_t i, int64_t n, int64_t k) {
     return (i * prime + prime * k) % n;
 }
 
-void permuted_solve(refcpu_world* w, const std::vector<Pair*>& cons, bool descending) {
-    std::vector<Pair*> list(cons);
-    std::sort(list.begin(), list.end(), [&](const Pair* a, const Pair* b) {
-        const uint64_t ka = pair_key(a->a, a->b), kb = pair_key(b->a, b->b);
+void permuted_solve(refcpu_world* w, const std::vector<CRef>& cons, bool descending) {
+    std::vector<CRef> list(cons);
+    std::sort(list.begin(), list.end(), [&](const CRef& a, const CRef& b) {
+        const uint64_t ka = a.key(), kb = b.key();
         return descending ? ka > kb : ka < kb;
     });
-    for (Pair* p : list) warm_start(w, *p);
+    for (const CRef& c : list) con_warm(w, c);
     ++w->perm_index;
     const int64_t n = (int64_t)list.size();
     const int64_t total = (int64_t)w->s.iteration_limit * n;
-    for (int64_t i = 0; i < total; ++i) solve_iteration(w, *list[(size_t)permuted_index(i, n, w->perm_index)]);
+    for (int64_t i = 0; i < total; ++i) con_solve(w, list[(size_t)permuted_index(i, n, w->perm_index)]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1803,23 +2006,34 @@ void permuted_solve(refcpu_world* w, const std::vector<Pair*>& cons, bool descen
 // NarrowPhase.cs:300-333: not updated, one entry active); the next step
 // creates them again.  The islands the step ends with, the labels the next
 // deactivation stage uses, are the components of the constraints left.
+// A joint between two dynamic bodies joins their islands for as long as it
+// lives (its SimulationIslandConnection).
+inline bool joint_edge(const refcpu_world* w, const JointC& j) {
+    return !j.dead && w->bodies[j.a].dynamic && w->bodies[j.b].dynamic;
+}
+
 void activity_pass(refcpu_world* w, PairTable& t) {
     const uint32_t n = (uint32_t)w->bodies.size();
     UnionFind uf(n);
     for (const Pair& p : t.vals)
         if (p.ccount > 0 && w->bodies[p.a].dynamic && w->bodies[p.b].dynamic) uf.unite(p.a, p.b);
+    for (const JointC& j : w->joints)
+        if (joint_edge(w, j)) uf.unite(j.a, j.b);
     std::vector<uint8_t> act(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         const Body& b = w->bodies[i];
         if (b.dynamic && !b.removed && !b.asleep) act[uf.find(i)] = 1;
     }
-    for (const Pair& p : t.vals) {
-        if (p.ccount <= 0) continue;
-        const Body& A = w->bodies[p.a];
-        const Body& B = w->bodies[p.b];
-        if (kinematic_active(A) && B.dynamic) act[uf.find(p.b)] = 1;
-        if (kinematic_active(B) && A.dynamic) act[uf.find(p.a)] = 1;
-    }
+    auto kinematic_wake = [&](uint32_t a, uint32_t b) {
+        const Body& A = w->bodies[a];
+        const Body& B = w->bodies[b];
+        if (kinematic_active(A) && B.dynamic) act[uf.find(b)] = 1;
+        if (kinematic_active(B) && A.dynamic) act[uf.find(a)] = 1;
+    };
+    for (const Pair& p : t.vals)
+        if (p.ccount > 0) kinematic_wake(p.a, p.b);
+    for (const JointC& j : w->joints)
+        if (!j.dead) kinematic_wake(j.a, j.b);
     for (uint32_t i = 0; i < n; ++i) {
         Body& b = w->bodies[i];
         if (b.dynamic && !b.removed && act[uf.find(i)]) b.asleep = false;
@@ -1839,6 +2053,8 @@ void activity_pass(refcpu_world* w, PairTable& t) {
     UnionFind live(n);
     for (const Pair& p : t.vals)
         if (p.ccount > 0 && w->bodies[p.a].dynamic && w->bodies[p.b].dynamic) live.unite(p.a, p.b);
+    for (const JointC& j : w->joints)
+        if (joint_edge(w, j)) live.unite(j.a, j.b);
     for (uint32_t i = 0; i < n; ++i) {
         Body& b = w->bodies[i];
         if (b.dynamic && !b.removed) b.label = live.find(i);
@@ -1929,40 +2145,43 @@ void step_once(refcpu_world* w) {
 
     // Solver.UpdateSolverActivity (SolverUpdateable.cs:202-218): the
     // constraints of inactive islands are not solved.
-    std::vector<Pair*> cons;
+    std::vector<CRef> cons;
     float maxpen = 0;
     for (Pair& p : w->pairs.vals) {
         if (p.ccount > 0 && !(w->bodies[p.a].dynamic && w->bodies[p.a].asleep) &&
             !(w->bodies[p.b].dynamic && w->bodies[p.b].asleep))
-            cons.push_back(&p);
+            cons.push_back(CRef{&p, nullptr});
         for (int i = 0; i < p.ccount; ++i) maxpen = std::max(maxpen, p.slot[p.clist[i]].depth);
     }
+    for (JointC& j : w->joints)
+        if (joint_active(w, j)) cons.push_back(CRef{nullptr, &j});
     w->max_pen = maxpen;
     colour_constraints(w, cons);
     w->order.clear();
-    for (Pair* p : cons) w->order.push_back(pair_key(p->a, p->b));
+    for (const CRef& c : cons)
+        if (c.p) w->order.push_back(c.key());
 
     // Solver.UpdateSingleThreaded (Solver.cs:239-263) in colour order.
     par_for(w, cons.size(), 512, [&](size_t c0, size_t c1) {
-        for (size_t c = c0; c < c1; ++c) prestep(w, *cons[c]);
+        for (size_t c = c0; c < c1; ++c) con_prestep(w, cons[c]);
     });
     std::vector<size_t> cstart(1, 0);
     for (size_t c = 1; c < cons.size(); ++c)
-        if (cons[c]->color != cons[c - 1]->color) cstart.push_back(c);
+        if (cons[c].color() != cons[c - 1].color()) cstart.push_back(c);
     cstart.push_back(cons.size());
     auto by_colour = [&](auto&& fn) {
         for (size_t g = 0; g + 1 < cstart.size(); ++g) {
             size_t base = cstart[g];
             par_for(w, cstart[g + 1] - base, 512, [&](size_t c0, size_t c1) {
-                for (size_t c = c0; c < c1; ++c) fn(*cons[base + c]);
+                for (size_t c = c0; c < c1; ++c) fn(cons[base + c]);
             });
         }
     };
     if (w->solve_order != 0) {
         permuted_solve(w, cons, w->solve_order == 2);
     } else {
-        by_colour([&](Pair& p) { warm_start(w, p); });
-        for (int it = 0; it < w->s.iteration_limit; ++it) by_colour([&](Pair& p) { solve_iteration(w, p); });
+        by_colour([&](const CRef& c) { con_warm(w, c); });
+        for (int it = 0; it < w->s.iteration_limit; ++it) by_colour([&](const CRef& c) { con_solve(w, c); });
     }
 
     // ContinuousPositionUpdater.UpdateSingleThreaded (ContinuousPositionUpdater.cs:134-154)
@@ -2266,6 +2485,72 @@ int refcpu_remove(refcpu_world* w, uint32_t id) {
         if (p.ccount <= 0 || (p.a != id && p.b != id)) continue;
         Body& other = w->bodies[p.a == id ? p.b : p.a];
         if (other.dynamic) activate(other);
+    }
+    // A joint of the removed body stops acting; its other connection wakes.
+    for (JointC& j : w->joints) {
+        if (j.dead || (j.a != id && j.b != id)) continue;
+        j.dead = true;
+        Body& other = w->bodies[j.a == id ? j.b : j.a];
+        if (other.dynamic && !other.removed) activate(other);
+    }
+    return BEPU_OK;
+}
+
+// new BallSocketJoint(connectionA, connectionB, anchor) (BallSocketJoint.cs:57-63)
+// added to the space: the offsets from each body's position to the anchor,
+// stored in each body's local frame (the OffsetA / OffsetB setters,
+// BallSocketJoint.cs:108-130); SpringSettings defaults (SpringSettings.cs:
+// 34-35), no corrective velocity limit (Joint.cs:14-15).
+int refcpu_add_ball_socket(refcpu_world* w, uint32_t a, uint32_t b, const float* anchor, uint32_t* id) {
+    if (!anchor || a == b) return BEPU_E_INVALID_ARG;
+    if (a >= w->bodies.size() || b >= w->bodies.size() || w->bodies[a].removed || w->bodies[b].removed)
+        return BEPU_E_NOT_FOUND;
+    const Body& A = w->bodies[a];
+    const Body& B = w->bodies[b];
+    if (!A.dynamic && !B.dynamic) return BEPU_E_INVALID_ARG;  // "Cannot constrain two kinematic bodies."
+    JointC j;
+    std::memset(&j, 0, sizeof(j));
+    j.a = a;
+    j.b = b;
+    j.local = (uint32_t)w->joints.size();
+    const V3 p{anchor[0], anchor[1], anchor[2]};
+    j.localA = m3_transform_transpose(sub(p, A.pos), A.R);
+    j.localB = m3_transform_transpose(sub(p, B.pos), B.R);
+    j.stiffness = 600000;
+    j.damping = 90000;
+    j.max_corr = FLT_MAX;
+    j.max_corr_sq = FLT_MAX;
+    j.color = -1;
+    if (id) *id = (uint32_t)w->joints.size();
+    w->joints.push_back(j);
+    return BEPU_OK;
+}
+
+// SpringSettings.Stiffness / Damping (clamped at 0, SpringSettings.cs:61-80)
+// and Joint.MaxCorrectiveVelocity (Joint.cs:24-38).
+int refcpu_set_joint_settings(refcpu_world* w, uint32_t id, float stiffness, float damping, float max_corrective) {
+    if (id >= w->joints.size()) return BEPU_E_NOT_FOUND;
+    stiffness = std::max(0.0f, stiffness);
+    damping = std::max(0.0f, damping);
+    // "Constraints cannot have both 0 stiffness and 0 damping." (SpringSettings.cs:119)
+    if (stiffness == 0 && damping == 0) return BEPU_E_INVALID_ARG;
+    JointC& j = w->joints[id];
+    j.stiffness = stiffness;
+    j.damping = damping;
+    j.max_corr = std::max(0.0f, max_corrective);
+    j.max_corr_sq = j.max_corr >= FLT_MAX ? FLT_MAX : j.max_corr * j.max_corr;
+    return BEPU_OK;
+}
+
+// BallSocketJoint.TotalImpulse of every joint (3 floats each).
+int refcpu_get_joint_impulses(refcpu_world* w, float* out, uint32_t cap, uint32_t* n) {
+    if (n) *n = (uint32_t)w->joints.size();
+    if (!out) return BEPU_OK;
+    if (cap < w->joints.size()) return BEPU_E_CAPACITY;
+    for (size_t k = 0; k < w->joints.size(); ++k) {
+        out[3 * k] = w->joints[k].acc.x;
+        out[3 * k + 1] = w->joints[k].acc.y;
+        out[3 * k + 2] = w->joints[k].acc.z;
     }
     return BEPU_OK;
 }
