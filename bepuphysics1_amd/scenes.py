@@ -4,7 +4,9 @@ A scene is a numpy structured array, one row per body, in add order:
 
     shape     int32    BEPU_SHAPE_BOX (0) / SPHERE (1) / CONVEX_HULL (2) / CAPSULE (3)
     dims      f32[3]   box (width, height, length); sphere (radius, 0, 0);
-                       capsule (length, radius, 0); hull (index into hull_library(), 0, 0)
+                       capsule (length, radius, 0); hull (index into hull_library(), 0, 0);
+                       static mesh (index into the scene's mesh list, sidedness,
+                       1 to improve boundary behaviour)
     mass      f32      <= 0 makes a kinematic body (Entity(shape, mass), EntityBase.cs:632-645)
     state     f32[13]  MotionState: position xyz, orientation xyzw, linear v, angular v
     material  f32[3]   static friction, kinetic friction, bounciness
@@ -23,6 +25,7 @@ SHAPE_BOX = 0
 SHAPE_SPHERE = 1
 SHAPE_CONVEX_HULL = 2
 SHAPE_CAPSULE = 3
+SHAPE_STATIC_MESH = 4
 FLAG_ALWAYS_ACTIVE = 1
 FLAG_NO_STABILIZATION = 2
 FLAG_CCD_PASSIVE = 4      # PositionUpdateMode.Passive
@@ -58,6 +61,58 @@ def hull_library():
         rng = np.random.default_rng(0x4855)
         _HULL_LIBRARY = [hull_points(rng) for _ in range(HULL_LIBRARY_SIZE)]
     return _HULL_LIBRARY
+
+
+def has_meshes(scene):
+    return bool((np.asarray(scene["shape"]) == SHAPE_STATIC_MESH).any())
+
+
+def static_mesh(mesh, sidedness=0, improve_boundary=True, material=(1.0, 1.0, 0.0)):
+    """A StaticMesh row: `mesh` indexes the mesh list that goes with the scene
+    (each entry (vertices [n, 3], indices [3k], transform or None), as
+    heightfield_mesh returns them)."""
+    return _row(SHAPE_STATIC_MESH, (float(mesh), float(sidedness), 1.0 if improve_boundary else 0.0), 0.0,
+                (0.0, 0.0, 0.0), material=material, flags=0)
+
+
+def heightfield_mesh(cells=16, size=32.0, amplitude=0.0, seed=0, center=(0.0, 0.0), y=0.0, transform=None):
+    """A cells x cells grid of quads over a size x size square, two triangles
+    per quad (counterclockwise seen from above), heights y plus a seeded
+    smooth bump field of the given amplitude; as (vertices, indices, transform)."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(-size / 2, size / 2, cells + 1)
+    gx, gz = np.meshgrid(xs + center[0], xs + center[1])
+    h = np.full_like(gx, y)
+    if amplitude:
+        for _ in range(4):
+            k = rng.uniform(0.5, 2.0, 2) * 2 * np.pi / size
+            ph = rng.uniform(0, 2 * np.pi, 2)
+            h += amplitude / 4 * np.sin(k[0] * gx + ph[0]) * np.cos(k[1] * gz + ph[1])
+    verts = np.stack([gx.ravel(), h.ravel(), gz.ravel()], 1).astype(np.float32)
+    idx = []
+    n = cells + 1
+    for j in range(cells):
+        for i in range(cells):
+            a = j * n + i
+            b, c, d = a + 1, a + n, a + n + 1
+            idx += [a, c, b, b, c, d]
+    return verts, np.asarray(idx, np.int32), transform
+
+
+def mesh_pile(n=64, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE), drop=(1.0, 6.0),
+              cells=16, amplitude=1.0, sidedness=0, improve_boundary=True):
+    """A random pile of `shapes` dropped onto a static heightfield mesh (the
+    reference's StaticMeshDemo / TerrainDemo ground).  Returns (scene, meshes)."""
+    rng = np.random.default_rng(seed)
+    foot = 2.0 * np.sqrt(max(n, 1)) + 2.0
+    size = foot + 8.0
+    meshes = [heightfield_mesh(cells, size, amplitude, seed=seed)]
+    rows = [static_mesh(0, sidedness, improve_boundary)]
+    for _ in range(n):
+        s = int(rng.choice(shapes))
+        pos = (rng.uniform(-foot / 2, foot / 2), amplitude + rng.uniform(*drop), rng.uniform(-foot / 2, foot / 2))
+        rows.append(_row(s, _shape_dims(rng, s), 1.0, pos, quat=tuple(random_quaternions(rng, 1)[0])))
+    return make(rows), meshes
 
 
 def has_hulls(scene):

@@ -114,6 +114,9 @@ typedef struct bepu_material {
 #define BEPU_SHAPE_SPHERE 1
 #define BEPU_SHAPE_CONVEX_HULL 2
 #define BEPU_SHAPE_CAPSULE 3
+/* A static triangle mesh (BroadPhaseEntries/StaticMesh.cs): a static
+ * collidable, not an entity, that never moves and is never active. */
+#define BEPU_SHAPE_STATIC_MESH 4
 
 /* One contact of a manifold, in the manifold's contact-list order. */
 typedef struct bepu_contact {
@@ -136,12 +139,23 @@ typedef struct bepu_manifold {
 #define BEPU_PAIR_BOX_SPHERE 1
 #define BEPU_PAIR_SPHERE_SPHERE 2
 #define BEPU_PAIR_CONVEX 3
+/* A convex (box, capsule, hull) with a static mesh (StaticMeshConvexPairHandler)
+ * and a sphere with a static mesh (StaticMeshSpherePairHandler): body_a is the
+ * convex, body_b the mesh, the manifold holds up to four contacts from the
+ * mesh's triangles with normals from the convex towards the mesh. */
+#define BEPU_PAIR_MESH_CONVEX 4
+#define BEPU_PAIR_MESH_SPHERE 5
+
+/* TriangleSidedness (BEPUutilities/TriangleSidedness.cs) of a static mesh's triangles. */
+#define BEPU_SIDED_DOUBLE 0
+#define BEPU_SIDED_CLOCKWISE 1
+#define BEPU_SIDED_COUNTERCLOCKWISE 2
 
 /* Accumulated impulses of one contact constraint after a step, listed in
  * solve order (colour, then key).  `normal[i]` is the i-th penetration
  * constraint (ContactPenetrationConstraint.accumulatedImpulse);
  * convex pairs fill `sliding` (SlidingFrictionTwoAxis) and `twist`
- * (TwistFrictionConstraint); non-convex pairs (sphere-sphere, box-sphere)
+ * (TwistFrictionConstraint); non-convex pairs (sphere-sphere, box-sphere, mesh)
  * fill `friction[i]` (ContactFrictionConstraint). */
 typedef struct bepu_impulses {
     uint64_t key;

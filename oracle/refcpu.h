@@ -91,6 +91,9 @@ V3 rt_transform_by_inverse(const V3& p, const RT& t);
 RT local_transform(const RT& a, const RT& b);
 bool gjk_intersecting(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* axis);
 bool gjk_closest_points(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* ca, V3* cb);
+// The same from the cached Point simplex whose B-side local point is `simplex_b`.
+bool gjk_closest_points_from(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb,
+                             const V3& simplex_b, V3* ca, V3* cb);
 bool mpr_contact(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, const V3& axis,
                  ContactData* c);
 // va / vb: the entities' linear velocities (used on entering deep contact from separation).
@@ -102,5 +105,75 @@ bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, c
 bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t);
 bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
                      float max_length, float scaling, float* t);
+
+// ---------------------------------------------------------------------------
+// Triangles of static meshes (refcpu_convex.cpp, refcpu_mesh.cpp)
+
+// A TriangleShape (CollisionShapes/ConvexShapes/TriangleShape.cs) as a
+// ConvexShapeRef: kind kShapeTriangle, verts -> its three vertices.
+constexpr int kShapeTriangle = 16;
+// TriangleSidedness (BEPUutilities/TriangleSidedness.cs)
+enum { SIDED_DOUBLE = 0, SIDED_CLOCKWISE = 1, SIDED_COUNTERCLOCKWISE = 2 };
+// VoronoiRegion (BEPUutilities/VoronoiRegion.cs)
+enum { VR_A = 0, VR_B, VR_C, VR_AB, VR_AC, VR_BC, VR_ABC };
+// A mesh triangle in the local space of the convex it is tested against.
+struct TriangleRef {
+    V3 v[3];
+    int sidedness;
+    float margin;
+};
+// TriangleConvexPairTester's state kept across steps (TriangleConvexPairTester.cs:14-20):
+// the state machine, the escape counter and the boolean GJK's warm-start axis.
+enum { TT_PLANE = 0, TT_SEPARATED = 1, TT_NEAR = 2, TT_DEEP = 3 };
+struct TriTesterState {
+    int state;
+    int escapes;
+    V3 axis;
+};
+ConvexShapeRef triangle_shape(const TriangleRef& t);
+// Toolbox.GetClosestPointOnTriangleToPoint (BEPUutilities/Toolbox.cs:338-419).
+int closest_point_on_triangle(const V3& a, const V3& b, const V3& c, const V3& p, V3* out);
+// TriangleConvexPairTester.GenerateContactCandidates (TriangleConvexPairTester.cs:23-46):
+// up to two candidates in the convex's local space; `min_radius` is the
+// convex shape's MinimumRadius, `core_scaling` MotionSettings.CoreShapeScaling.
+int triangle_convex_candidates(const ConvexShapeRef& convex, float min_radius, float core_scaling,
+                               const TriangleRef& t, TriTesterState* st, ContactData out[2]);
+// TriangleConvexPairTester.GetRegion (TriangleConvexPairTester.cs:708-897).
+int triangle_convex_region(const TriangleRef& t, const ContactData& c);
+// TriangleSpherePairTester.GenerateContactCandidates (TriangleSpherePairTester.cs:20-94):
+// at most one candidate; *region is the tester's lastRegion.
+int triangle_sphere_candidates(float radius, const TriangleRef& t, ContactData* out, int* region);
+
+// AffineTransform (BEPUutilities/AffineTransform.cs): linear part and translation.
+struct Affine {
+    M3 m;
+    V3 t;
+};
+// A static mesh shape (StaticMeshShape.cs, TransformableMeshData.cs): local
+// vertices, index triples, the world transform and its MeshBoundingBoxTree
+// (DataStructures/MeshBoundingBoxTree.cs) flattened: node k has a box and,
+// for an internal node, children a and b; a leaf has a = -1 and b = its
+// triangle (the position of its first index).
+struct MeshNode {
+    V3 bmin, bmax;
+    int a, b;
+};
+struct MeshShape {
+    std::vector<V3> verts;
+    std::vector<int> idx;
+    Affine xf;
+    std::vector<MeshNode> nodes;  // nodes[0] is the root (empty mesh: no nodes)
+};
+// MeshBoundingBoxTree.Reconstruct (MeshBoundingBoxTree.cs:60-69).
+void mesh_build_tree(MeshShape* m);
+// MeshBoundingBoxTree.GetOverlaps(BoundingBox) (MeshBoundingBoxTree.cs:112-122):
+// triangles (first-index positions) in the tree's depth-first order.
+void mesh_overlaps(const MeshShape& m, const V3& bmin, const V3& bmax, std::vector<int>* out);
+// TransformableMeshData.GetTriangle (TransformableMeshData.cs:67-72): the world-space triangle.
+void mesh_world_triangle(const MeshShape& m, int tri, V3 out[3]);
+V3 affine_transform(const V3& p, const Affine& a);
+// Toolbox.GetTriangleBoundingBox (BEPUutilities/Toolbox.cs:1516-1580)
+void triangle_bounding_box(const V3& a, const V3& b, const V3& c, V3* mn, V3* mx);
+Affine affine_multiply(const Affine& a, const Affine& b);
 
 }  // namespace ref

@@ -94,6 +94,19 @@ def lib():
                                                ctypes.c_uint32, ctypes.c_uint32]
         L.refcpu_set_settings.argtypes = [vp, ctypes.POINTER(Settings)]
         L.refcpu_reduce_contacts.argtypes = [fp, fp, fp, fp, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
+        ip = ctypes.POINTER(ctypes.c_int32)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.refcpu_create_mesh_shape.argtypes = [vp, fpp, ctypes.c_uint32, ip, ctypes.c_uint32, fpp, u32p]
+        L.refcpu_add_static_mesh.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(Material), u32p]
+        L.refcpu_get_mesh_tree.argtypes = [vp, ctypes.c_uint32, fpp, ip, ctypes.c_uint32, u32p]
+        L.refcpu_mesh_overlaps.argtypes = [vp, ctypes.c_uint32, fpp, fpp, ip, ctypes.c_uint32, u32p]
+        L.refcpu_triangle_convex.argtypes = [fpp, fpp, ctypes.c_int, ctypes.c_float, ctypes.c_float, fpp,
+                                             ctypes.c_int, ctypes.c_float, ip, fpp, fpp]
+        L.refcpu_triangle_region.argtypes = [fpp, fpp]
+        L.refcpu_triangle_sphere.argtypes = [ctypes.c_float, fpp, ctypes.c_int, ctypes.c_float, fpp,
+                                             ctypes.POINTER(ctypes.c_int)]
+        L.refcpu_closest_point_triangle.argtypes = [fpp, fpp, fpp]
         L.refcpu_sizeof.restype = ctypes.c_uint32
         L.refcpu_sizeof.argtypes = [ctypes.c_int]
         _lib = L
@@ -180,14 +193,70 @@ class World:
             raise ValueError("refcpu add failed with status %d" % rc)
         return bid.value
 
-    def add_scene(self, scene, hulls=None):
+    def add_scene(self, scene, hulls=None, meshes=None):
         """`hulls[k]` = (vertices, volume distribution) of hull shape id k, which
-        a hull body names in dims[0]."""
+        a hull body names in dims[0]; `meshes[k]` = (vertices, indices,
+        transform) of the mesh a static mesh row names in dims[0]."""
+        mesh_ids = {}
         for i in range(len(scene)):
             r = scene[i]
             shape = int(r["shape"])
+            if shape == 4:
+                k = int(r["dims"][0])
+                if k not in mesh_ids:
+                    mesh_ids[k] = self.create_mesh_shape(*meshes[k])
+                self.add_static_mesh(mesh_ids[k], int(r["dims"][1]), bool(r["dims"][2]), r["material"])
+                continue
             hull = hulls[int(r["dims"][0])] if shape == 2 else None
             self.add_body(shape, r["dims"], r["mass"], r["state"], r["material"], int(r["flags"]), hull=hull)
+
+    def create_mesh_shape(self, vertices, indices, transform=None):
+        """new StaticMeshShape(vertices, indices[, worldTransform]); `transform`
+        is (3x3 linear part as Matrix3x3 rows, translation) or None.  Returns
+        the mesh shape id."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        ix = np.ascontiguousarray(indices, np.int32).reshape(-1)
+        t = None
+        if transform is not None:
+            t = np.concatenate([np.asarray(transform[0], np.float32).reshape(9),
+                                np.asarray(transform[1], np.float32).reshape(3)]).astype(np.float32)
+        sid = ctypes.c_uint32()
+        rc = lib().refcpu_create_mesh_shape(self.h, _fptr(v), len(v), ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            len(ix), _fptr(t) if t is not None else None, ctypes.byref(sid))
+        if rc != 0:
+            raise ValueError("refcpu_create_mesh_shape failed with status %d" % rc)
+        return sid.value
+
+    def add_static_mesh(self, shape_id, sidedness=0, improve_boundary=True, material=(1.0, 0.8, 0.0)):
+        """A StaticMesh of a mesh shape added to the space; returns its body id."""
+        mat = Material(*[float(x) for x in material])
+        bid = ctypes.c_uint32()
+        rc = lib().refcpu_add_static_mesh(self.h, int(shape_id), int(sidedness), 1 if improve_boundary else 0,
+                                          ctypes.byref(mat), ctypes.byref(bid))
+        if rc != 0:
+            raise ValueError("refcpu_add_static_mesh failed with status %d" % rc)
+        return bid.value
+
+    def mesh_tree(self, shape_id):
+        """The flattened MeshBoundingBoxTree: (boxes [n, 6], links [n, 2])."""
+        n = ctypes.c_uint32()
+        lib().refcpu_get_mesh_tree(self.h, int(shape_id), None, None, 0, ctypes.byref(n))
+        boxes = np.zeros((max(n.value, 1), 6), np.float32)
+        links = np.zeros((max(n.value, 1), 2), np.int32)
+        lib().refcpu_get_mesh_tree(self.h, int(shape_id), _fptr(boxes),
+                                   links.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value, ctypes.byref(n))
+        return boxes[:n.value], links[:n.value]
+
+    def mesh_overlaps(self, shape_id, bmin, bmax):
+        """MeshBoundingBoxTree.GetOverlaps(box): triangles (first-index positions) in traversal order."""
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        n = ctypes.c_uint32()
+        lib().refcpu_mesh_overlaps(self.h, int(shape_id), _fptr(lo), _fptr(hi), None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), np.int32)
+        lib().refcpu_mesh_overlaps(self.h, int(shape_id), _fptr(lo), _fptr(hi),
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value, ctypes.byref(n))
+        return out[:n.value]
 
     def step(self, n=1):
         lib().refcpu_step(self.h, int(n))
@@ -434,3 +503,53 @@ def cast(kind, shape, pose, origin, direction, radius=0.0, max_length=1.0, scali
     hit = lib().refcpu_cast(k, _fptr(d), _fptr(v), n, _fptr(p), _fptr(o), _fptr(dr), float(radius),
                             float(max_length), float(scaling), _fptr(t))
     return bool(hit), float(t[0])
+
+
+# ---------------------------------------------------------------------------
+# Triangle pair testers on one triangle in the convex's local space.
+
+class TriangleTester:
+    """TriangleConvexPairTester state (state, escape attempts, separating axis)
+    carried across calls."""
+
+    def __init__(self):
+        self.st = np.zeros(2, np.int32)
+        self.axis = np.zeros(3, np.float32)
+
+    @property
+    def state(self):
+        return int(self.st[0])
+
+    def candidates(self, shape, tri, sidedness=0, min_radius=0.0, core_scaling=0.8, tri_margin=0.0):
+        d, v, n = _shape_arrays(shape)
+        t = np.ascontiguousarray(tri, np.float32).reshape(9)
+        out = np.zeros((2, 8), np.float32)
+        k = lib().refcpu_triangle_convex(_fptr(d), _fptr(v), n, float(min_radius), float(core_scaling), _fptr(t),
+                                         int(sidedness), float(tri_margin),
+                                         self.st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _fptr(self.axis),
+                                         _fptr(out))
+        return out[:k].copy()
+
+
+def triangle_region(tri, normal):
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    nn = np.ascontiguousarray(normal, np.float32)
+    return int(lib().refcpu_triangle_region(_fptr(t), _fptr(nn)))
+
+
+def triangle_sphere(radius, tri, sidedness=0, tri_margin=0.0):
+    """(candidates [k, 8], Voronoi region)."""
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    out = np.zeros(8, np.float32)
+    reg = ctypes.c_int(-1)
+    k = lib().refcpu_triangle_sphere(float(radius), _fptr(t), int(sidedness), float(tri_margin), _fptr(out),
+                                     ctypes.byref(reg))
+    return out[None, :].copy()[:k], reg.value
+
+
+def closest_point_triangle(tri, p):
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    pp = np.ascontiguousarray(p, np.float32)
+    out = np.zeros(3, np.float32)
+    r = lib().refcpu_closest_point_triangle(_fptr(t), _fptr(pp), _fptr(out))
+    return r, out

@@ -65,6 +65,13 @@ V3 support_local(const ConvexShapeRef& s, const V3& d) {
         }
         return s.verts[mi];
     }
+    case kShapeTriangle: {
+        // TriangleShape.GetLocalExtremePointWithoutMargin (TriangleShape.cs:166-185)
+        float dA = dot(d, s.verts[0]), dB = dot(d, s.verts[1]), dC = dot(d, s.verts[2]);
+        if (dA > dB && dA > dC) return s.verts[0];
+        if (dB > dC) return s.verts[1];
+        return s.verts[2];
+    }
     default:  // sphere
         return kZero;
     }
@@ -319,15 +326,16 @@ struct PSimplex {
 };
 
 // PairSimplex(ref CachedSimplex, ref RigidTransform) for the cached simplex the
-// pair tester hands in: State = Point with zero local points
-// (GeneralConvexPairTester.cs:329, PairSimplex.cs:163-193).
-void ps_init_point(PSimplex& s, const RT& lb) {
+// pair tester hands in: State = Point, LocalSimplexA.A zero and
+// LocalSimplexB.A = `pb` (zero for GeneralConvexPairTester.cs:329, the
+// triangle's centroid for TriangleConvexPairTester.cs:241-242; PairSimplex.cs:163-193).
+void ps_init_point(PSimplex& s, const RT& lb, const V3& pb = kZero) {
     std::memset(&s, 0, sizeof(s));
     s.prev_distance = FLT_MAX;
     s.error_tolerance = 0;
     s.lb = lb;
     s.state = S_POINT;
-    s.sb[0] = qtransform(kZero, lb.q);
+    s.sb[0] = qtransform(pb, lb.q);
     s.sb[0] = add(s.sb[0], lb.pos);
     s.A = sub(s.sa[0], s.sb[0]);
 }
@@ -653,9 +661,9 @@ void ps_closest_points(const PSimplex& s, V3* ca, V3* cb) {
 // ---------------------------------------------------------------------------
 // MPR (MPRToolbox.cs)
 
-// MPRToolbox.AreLocalShapesOverlapping (MPRToolbox.cs:413-598, the internal overload at :426), origin ray = B's local position.
-bool mpr_overlapping(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& lb) {
-    const V3& ray = lb.pos;
+// MPRToolbox.AreLocalShapesOverlapping (MPRToolbox.cs:413-598): the internal
+// overload at :426 with its origin ray; the public one at :413 passes B's local position.
+bool mpr_overlapping_ray(const ConvexShapeRef& a, const ConvexShapeRef& b, const V3& ray, const RT& lb) {
     if (length_sq(ray) < kEpsilon) return true;
     V3 v0 = neg(ray);
     V3 n = ray;
@@ -700,6 +708,10 @@ bool mpr_overlapping(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT&
             else v1 = v4;
         }
     }
+}
+
+bool mpr_overlapping(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& lb) {
+    return mpr_overlapping_ray(a, b, lb.pos, lb);
 }
 
 // Toolbox.GetBarycentricCoordinates (BEPUutilities/Toolbox.cs:1665-1737)
@@ -892,9 +904,13 @@ bool gjk_intersecting(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT
 // GJKToolbox.GetClosestPoints with the pair tester's cached simplex (GJKToolbox.cs:131-182):
 // closest points of the shapes' cores, returned in world space.
 bool gjk_closest_points(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb, V3* ca, V3* cb) {
+    return gjk_closest_points_from(a, b, ta, tb, kZero, ca, cb);
+}
+bool gjk_closest_points_from(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT& ta, const RT& tb,
+                             const V3& simplex_b, V3* ca, V3* cb) {
     RT lb = local_transform(ta, tb);
     PSimplex s;
-    ps_init_point(s, lb);
+    ps_init_point(s, lb, simplex_b);
     V3 cp;
     int count = 0;
     bool intersecting = false;
@@ -1398,6 +1414,421 @@ bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const Convex
     }
 }
 
+// ---------------------------------------------------------------------------
+// Triangles of static meshes: the TriangleShape support (above, in
+// support_local), Toolbox.GetClosestPointOnTriangleToPoint and the two
+// TrianglePairTesters.  The convex is shape A at the identity, the triangle
+// (already in the convex's local space) shape B at the identity.
+
+ConvexShapeRef triangle_shape(const TriangleRef& t) {
+    ConvexShapeRef r;
+    std::memset(&r, 0, sizeof(r));
+    r.kind = kShapeTriangle;
+    r.verts = t.v;
+    r.nverts = 3;
+    r.margin = t.margin;
+    return r;
+}
+
+int closest_point_on_triangle(const V3& a, const V3& b, const V3& c, const V3& p, V3* out) {
+    V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 < 0) {
+        *out = a;
+        return VR_A;
+    }
+    V3 bp = sub(p, b);
+    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) {
+        *out = b;
+        return VR_B;
+    }
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        float v = d1 / (d1 - d3);
+        *out = add(mul(ab, v), a);
+        return VR_AB;
+    }
+    V3 cp = sub(p, c);
+    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) {
+        *out = c;
+        return VR_C;
+    }
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) {
+        float w = d2 / (d2 - d6);
+        *out = add(mul(ac, w), a);
+        return VR_AC;
+    }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        *out = add(mul(sub(c, b), w), b);
+        return VR_BC;
+    }
+    float denom = 1 / (va + vb + vc);
+    float v = vb * denom, w = vc * denom;
+    *out = add(add(a, mul(ab, v)), mul(ac, w));
+    return VR_ABC;
+}
+
+namespace {
+
+const RT kIdentityRT{V3{0, 0, 0}, Q4{0, 0, 0, 1}};
+constexpr int kEscapeAttemptPeriod = 10;  // TriangleConvexPairTester.cs:16
+
+// TriangleConvexPairTester.GetVoronoiRegion (TriangleConvexPairTester.cs:619-686)
+int tri_voronoi(const TriangleRef& t, const V3& p) {
+    V3 ab = sub(t.v[1], t.v[0]), ac = sub(t.v[2], t.v[0]), ap = sub(p, t.v[0]);
+    float APdotAB = dot(ap, ab), APdotAC = dot(ap, ac);
+    if (APdotAC <= 0.f && APdotAB <= 0) return VR_A;
+    V3 bp = sub(p, t.v[1]);
+    float BPdotAB = dot(ab, bp), BPdotAC = dot(ac, bp);
+    if (BPdotAB >= 0.f && BPdotAC <= BPdotAB) return VR_B;
+    float vc = APdotAB * BPdotAC - BPdotAB * APdotAC;
+    if (vc <= 0 && APdotAB > 0 && BPdotAB < 0) return VR_AB;
+    V3 cp = sub(p, t.v[2]);
+    float CPdotAB = dot(ab, cp), CPdotAC = dot(ac, cp);
+    if (CPdotAC >= 0.f && CPdotAB <= CPdotAC) return VR_C;
+    float vb = CPdotAB * APdotAC - APdotAB * CPdotAC;
+    if (vb <= 0.f && APdotAC > 0.f && CPdotAC < 0.f) return VR_AC;
+    float va = BPdotAB * CPdotAC - CPdotAB * BPdotAC;
+    if (va <= 0.f && (BPdotAC - BPdotAB) > 0.f && (CPdotAB - CPdotAC) > 0.f) return VR_BC;
+    return VR_ABC;
+}
+
+// TryToEscape, both overloads (TriangleConvexPairTester.cs:529-545)
+void tri_escape(TriTesterState* st) {
+    if (++st->escapes == kEscapeAttemptPeriod) {
+        st->escapes = 0;
+        st->state = TT_PLANE;
+    }
+}
+void tri_escape_at(TriTesterState* st, const TriangleRef& t, const V3& position) {
+    if (++st->escapes == kEscapeAttemptPeriod && tri_voronoi(t, position) == VR_ABC) {
+        st->escapes = 0;
+        st->state = TT_PLANE;
+    }
+}
+
+// TryInnerSphereContact (TriangleConvexPairTester.cs:548-613)
+bool tri_inner_sphere(float min_radius, float core_scaling, const TriangleRef& t, ContactData* c) {
+    std::memset(c, 0, sizeof(*c));
+    V3 closest;
+    closest_point_on_triangle(t.v[0], t.v[1], t.v[2], kZero, &closest);
+    float length = length_sq(closest);
+    float minimumRadius = min_radius * (core_scaling + .01f);
+    if (length < minimumRadius * minimumRadius) {
+        V3 n = cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]));
+        float d = dot(closest, n);
+        if ((t.sidedness == SIDED_CLOCKWISE && d > 0) || (t.sidedness == SIDED_COUNTERCLOCKWISE && d < 0)) return false;
+        length = std::sqrt(length);
+        c->pos = closest;
+        if (length > kEpsilon) {
+            c->normal = divide(closest, length);
+        } else {
+            float nl = length_sq(n);
+            if (length_sq(n) > kEpsilon) {
+                n = divide(n, std::sqrt(nl));
+                c->normal = t.sidedness == SIDED_CLOCKWISE ? n : neg(n);
+            } else {
+                std::memset(c, 0, sizeof(*c));
+                return false;
+            }
+        }
+        c->depth = min_radius - length;
+        c->id = -1;
+        return true;
+    }
+    return false;
+}
+
+// The surface casts of the deep test: a normal facing against a one-sided
+// triangle loses its component along the triangle normal and the depth is
+// scaled to the corrected direction (TriangleConvexPairTester.cs:372-459).
+// A degenerate correction sets the *contact's* depth and normal, not the
+// candidate's, as the reference does.
+void tri_cast_corrected(const ConvexShapeRef& cvx, const ConvexShapeRef& tri, int sided, const V3& tn, const V3& dir,
+                        float* cd, V3* cn, float* depth, V3* normal) {
+    surface_cast(cvx, tri, kIdentityRT, dir, false, cd, cn, nullptr);
+    float d = dot(tn, *cn);
+    if ((sided == SIDED_CLOCKWISE && d > 0) || (sided == SIDED_COUNTERCLOCKWISE && d < 0)) {
+        V3 previous = *cn;
+        d = dot(*cn, tn);
+        V3 p = mul(*cn, d);
+        *cn = sub(*cn, p);
+        float length = length_sq(*cn);
+        if (length > kEpsilon) {
+            *cn = divide(*cn, std::sqrt(length));
+            d = dot(*cn, previous);
+            *cd *= d;
+        } else {
+            *depth = FLT_MAX;
+            *normal = kZero;
+        }
+    }
+}
+
+int tri_deep(const ConvexShapeRef& cvx, float min_radius, float core_scaling, const TriangleRef& t,
+             TriTesterState* st, ContactData out[2]);
+
+// DoExternalNear (TriangleConvexPairTester.cs:230-298)
+int tri_near(const ConvexShapeRef& cvx, float min_radius, float core_scaling, const TriangleRef& t,
+             TriTesterState* st, ContactData out[2]) {
+    const ConvexShapeRef tri = triangle_shape(t);
+    V3 centroid = add(add(t.v[0], t.v[1]), t.v[2]);
+    centroid = mul(centroid, .33333333f);
+    V3 ca, cb;
+    if (gjk_closest_points_from(cvx, tri, kIdentityRT, kIdentityRT, centroid, &ca, &cb)) {
+        st->state = TT_DEEP;
+        return tri_deep(cvx, min_radius, core_scaling, t, st, out);
+    }
+    V3 disp = sub(cb, ca);
+    float d2 = length_sq(disp);
+    float margin = cvx.margin + t.margin;
+    if (d2 < margin * margin) {
+        ContactData c;
+        std::memset(&c, 0, sizeof(c));
+        if (t.sidedness != SIDED_DOUBLE) {
+            V3 n = cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]));
+            float d = dot(n, disp);
+            if (t.sidedness == SIDED_CLOCKWISE && d > 0) return 0;
+            if (t.sidedness == SIDED_COUNTERCLOCKWISE && d < 0) return 0;
+        }
+        if (margin > kEpsilon) c.pos = mul(disp, cvx.margin / margin);
+        else c.pos = kZero;
+        c.pos = add(ca, c.pos);
+        c.normal = disp;
+        float distance = std::sqrt(d2);
+        c.normal = divide(c.normal, distance);
+        c.depth = margin - distance;
+        out[0] = c;
+        tri_escape_at(st, t, c.pos);
+        return 1;
+    }
+    st->state = TT_SEPARATED;
+    return 0;
+}
+
+// DoExternalSeparated (TriangleConvexPairTester.cs:215-228)
+int tri_separated(const ConvexShapeRef& cvx, float min_radius, float core_scaling, const TriangleRef& t,
+                  TriTesterState* st, ContactData out[2]) {
+    if (gjk_intersecting(cvx, triangle_shape(t), kIdentityRT, kIdentityRT, &st->axis)) {
+        st->state = TT_NEAR;
+        return tri_near(cvx, min_radius, core_scaling, t, st, out);
+    }
+    tri_escape(st);
+    return 0;
+}
+
+// DoDeepContact (TriangleConvexPairTester.cs:300-527)
+int tri_deep(const ConvexShapeRef& cvx, float min_radius, float core_scaling, const TriangleRef& t,
+             TriTesterState* st, ContactData out[2]) {
+    const ConvexShapeRef tri = triangle_shape(t);
+    const int sided = t.sidedness;
+    V3 center = add(add(t.v[0], t.v[1]), t.v[2]);
+    center = mul(center, 1.f / 3.f);
+    int n_out = 0;
+    ContactData c;
+    std::memset(&c, 0, sizeof(c));
+    if (mpr_overlapping_ray(cvx, tri, center, kIdentityRT)) {
+        V3 tn = cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]));
+        float ls = length_sq(tn);
+        if (ls < kEpsilon * .01f) {
+            surface_cast(cvx, tri, kIdentityRT, center, true, &c.depth, &c.normal, &c.pos);
+        } else {
+            tn = divide(tn, std::sqrt(ls));
+            V3 AO = sub(center, t.v[0]), BO = sub(center, t.v[1]), CO = sub(center, t.v[2]);
+            V3 AB = sub(t.v[1], t.v[0]), BC = sub(t.v[2], t.v[1]), CA = sub(t.v[0], t.v[2]);
+            float d = dot(AO, AB);
+            V3 ABn = mul(AB, d / length_sq(AB));
+            ABn = normalize(sub(AO, ABn));
+            d = dot(BO, BC);
+            V3 BCn = mul(BC, d / length_sq(BC));
+            BCn = normalize(sub(BO, BCn));
+            d = dot(CO, CA);
+            V3 CAn = mul(CA, d / length_sq(CA));
+            CAn = normalize(sub(CO, CAn));
+            float cd;
+            V3 cn;
+            tri_cast_corrected(cvx, tri, sided, tn, ABn, &c.depth, &c.normal, &c.depth, &c.normal);
+            tri_cast_corrected(cvx, tri, sided, tn, BCn, &cd, &cn, &c.depth, &c.normal);
+            if (cd < c.depth) {
+                c.normal = cn;
+                c.depth = cd;
+            }
+            tri_cast_corrected(cvx, tri, sided, tn, CAn, &cd, &cn, &c.depth, &c.normal);
+            if (cd < c.depth) {
+                c.normal = cn;
+                c.depth = cd;
+            }
+            if (sided != SIDED_CLOCKWISE) {
+                surface_cast(cvx, tri, kIdentityRT, tn, false, &cd, &cn, nullptr);
+                if (cd < c.depth) {
+                    c.normal = cn;
+                    c.depth = cd;
+                }
+            }
+            if (sided != SIDED_COUNTERCLOCKWISE) {
+                tn = neg(tn);
+                surface_cast(cvx, tri, kIdentityRT, tn, false, &cd, &cn, nullptr);
+                if (cd < c.depth) {
+                    c.normal = cn;
+                    c.depth = cd;
+                }
+            }
+        }
+        refine_penetration(cvx, tri, kIdentityRT, c.depth, c.normal, &c.depth, &c.normal, &c.pos);
+        bool skip = false;
+        if (sided != SIDED_DOUBLE && dot(tn, c.normal) < 0) skip = true;
+        if (!skip) {
+            c.id = -1;
+            if (c.depth < cvx.margin + t.margin) st->state = TT_NEAR;
+            out[n_out++] = c;
+        }
+    }
+    if (tri_inner_sphere(min_radius, core_scaling, t, &c)) out[n_out++] = c;
+    if (n_out > 0) return n_out;
+    st->state = TT_SEPARATED;
+    return 0;
+}
+
+// DoPlaneTest (TriangleConvexPairTester.cs:48-211)
+int tri_plane(const ConvexShapeRef& cvx, float min_radius, float core_scaling, const TriangleRef& t,
+              TriTesterState* st, ContactData out[2]) {
+    V3 ab = sub(t.v[1], t.v[0]), ac = sub(t.v[2], t.v[0]);
+    V3 rn = cross(ac, ab);
+    float dotA = dot(t.v[0], rn);
+    if (t.sidedness == SIDED_DOUBLE) {
+        if (dotA < 0) {
+            rn = neg(rn);
+            dotA = -dotA;
+        }
+    } else if (t.sidedness == SIDED_COUNTERCLOCKWISE) {
+        rn = neg(rn);
+        dotA = -dotA;
+    }
+    V3 extreme = support_local(cvx, rn);
+    if (tri_voronoi(t, extreme) != VR_ABC) {
+        st->state = TT_SEPARATED;
+        return tri_separated(cvx, min_radius, core_scaling, t, st, out);
+    }
+    float dotE = dot(extreme, rn);
+    float tt = (dotA - dotE) / length_sq(rn);
+    V3 offset = mul(rn, tt);
+    float dsq = length_sq(offset);
+    float marginSum = t.margin + cvx.margin;
+    if (tt <= 0 || dsq < marginSum * marginSum) {
+        ContactData c;
+        std::memset(&c, 0, sizeof(c));
+        if (marginSum > kEpsilon) c.pos = mul(offset, cvx.margin / marginSum);
+        else c.pos = kZero;
+        c.pos = add(extreme, c.pos);
+        float nl = length(rn);
+        c.normal = divide(rn, nl);
+        float distance = nl * tt;
+        c.depth = marginSum - distance;
+        int n_out = 0;
+        if (c.depth > marginSum) {
+            ContactData alt;
+            if (tri_inner_sphere(min_radius, core_scaling, t, &alt)) out[n_out++] = alt;
+            int previous = st->state;
+            st->state = TT_NEAR;
+            ContactData alts[2];
+            if (tri_near(cvx, min_radius, core_scaling, t, st, alts) > 0) {
+                alt = alts[0];
+                if (alt.depth + .01f < c.depth) {
+                    out[n_out++] = alt;
+                } else {
+                    out[n_out++] = c;
+                    st->state = previous;
+                }
+            } else {
+                st->state = previous;
+                return 0;
+            }
+        } else {
+            out[n_out++] = c;
+        }
+        return n_out;
+    }
+    return 0;
+}
+
+}  // namespace
+
+int triangle_convex_candidates(const ConvexShapeRef& convex, float min_radius, float core_scaling,
+                               const TriangleRef& t, TriTesterState* st, ContactData out[2]) {
+    switch (st->state) {
+    case TT_PLANE: return tri_plane(convex, min_radius, core_scaling, t, st, out);
+    case TT_SEPARATED: return tri_separated(convex, min_radius, core_scaling, t, st, out);
+    case TT_NEAR: return tri_near(convex, min_radius, core_scaling, t, st, out);
+    case TT_DEEP: return tri_deep(convex, min_radius, core_scaling, t, st, out);
+    default: return 0;
+    }
+}
+
+int triangle_convex_region(const TriangleRef& t, const ContactData& c) {
+    float dotA = -dot(t.v[0], c.normal), dotB = -dot(t.v[1], c.normal), dotC = -dot(t.v[2], c.normal);
+    const float faceEpsilon = .01f, edgeEpsilon = .01f;
+    // Extreme vertex e, second vertex s, third vertex o: a face normal when
+    // the extreme and third dots nearly tie, else the edge (e, s) when the
+    // normal is nearly perpendicular to it, else the vertex e.
+    auto edge_or_vertex = [&](float de, float dthird, const V3& ve, const V3& vs, int edge_region, int vertex_region) {
+        if (std::fabs(de - dthird) < faceEpsilon) return (int)VR_ABC;
+        V3 e = sub(vs, ve);
+        float ed = dot(e, c.normal);
+        if (ed * ed < length_sq(e) * edgeEpsilon) return edge_region;
+        return vertex_region;
+    };
+    if (dotA > dotB && dotA > dotC) {
+        if (dotB > dotC) return edge_or_vertex(dotA, dotC, t.v[0], t.v[1], VR_AB, VR_A);
+        return edge_or_vertex(dotA, dotB, t.v[0], t.v[2], VR_AC, VR_A);
+    } else if (dotB > dotC) {
+        if (dotC > dotA) return edge_or_vertex(dotB, dotA, t.v[1], t.v[2], VR_BC, VR_B);
+        return edge_or_vertex(dotB, dotC, t.v[1], t.v[0], VR_AB, VR_B);
+    } else {
+        if (dotA > dotB) return edge_or_vertex(dotC, dotB, t.v[2], t.v[0], VR_AC, VR_C);
+        return edge_or_vertex(dotC, dotA, t.v[2], t.v[1], VR_BC, VR_C);
+    }
+}
+
+int triangle_sphere_candidates(float radius, const TriangleRef& t, ContactData* out, int* region) {
+    std::memset(out, 0, sizeof(*out));
+    V3 n = cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]));
+    if (length_sq(n) < kEpsilon * .01f) {
+        n = add(add(t.v[0], t.v[1]), t.v[2]);
+        n = mul(n, 1 / 3.f);
+        if (length_sq(n) < kEpsilon * .01f) n = kUp;
+    }
+    float d = dot(n, t.v[0]);
+    if (t.sidedness == SIDED_DOUBLE) {
+        if (d < 0) n = neg(n);
+    } else if (t.sidedness == SIDED_CLOCKWISE) {
+        if (d > 0) return 0;
+    } else {
+        if (d < 0) return 0;
+    }
+    V3 closest;
+    *region = closest_point_on_triangle(t.v[0], t.v[1], t.v[2], kZero, &closest);
+    float lsq = length_sq(closest);
+    float marginSum = t.margin + radius;
+    if (lsq <= marginSum * marginSum) {
+        if (lsq < kEpsilon) {
+            out->normal = normalize(neg(n));
+            out->depth = marginSum;
+            return 1;
+        }
+        lsq = std::sqrt(lsq);
+        out->normal = divide(closest, lsq);
+        out->depth = marginSum - lsq;
+        out->pos = closest;
+        return 1;
+    }
+    return 0;
+}
+
 }  // namespace ref
 
 // ---------------------------------------------------------------------------
@@ -1500,6 +1931,65 @@ int refcpu_cast(int kind, const float* s, const float* verts, int n, const float
     else if (kind == 1) r = ref::ray_cast(o, d, shape, make_pose(pose), max_length, t);
     else r = ref::ccd_sphere_cast(o, d, radius, shape, make_pose(pose), max_length, scaling, t);
     return r ? 1 : 0;
+}
+
+// TriangleConvexPairTester.GenerateContactCandidates on a triangle already in
+// the convex's local space (tri9: three vertices); state_io = {state,
+// escapes} and axis_io carry the tester across calls; out16: two contacts of
+// 8 floats.  Returns the number of candidates.
+int refcpu_triangle_convex(const float* s, const float* verts, int n, float min_radius, float core_scaling,
+                           const float* tri9, int sidedness, float tri_margin, int* state_io, float* axis_io,
+                           float* out16) {
+    std::vector<ref::V3> h;
+    ref::TriangleRef t;
+    for (int k = 0; k < 3; ++k) t.v[k] = ref::V3{tri9[3 * k], tri9[3 * k + 1], tri9[3 * k + 2]};
+    t.sidedness = sidedness;
+    t.margin = tri_margin;
+    ref::TriTesterState st{state_io[0], state_io[1], ref::V3{axis_io[0], axis_io[1], axis_io[2]}};
+    ref::ContactData c[2];
+    std::memset(c, 0, sizeof(c));
+    int r = ref::triangle_convex_candidates(make_shape(s, verts, n, &h), min_radius, core_scaling, t, &st, c);
+    state_io[0] = st.state;
+    state_io[1] = st.escapes;
+    axis_io[0] = st.axis.x; axis_io[1] = st.axis.y; axis_io[2] = st.axis.z;
+    put_contact(c[0], out16);
+    put_contact(c[1], out16 + 8);
+    return r;
+}
+
+// TriangleConvexPairTester.GetRegion of a contact normal (normal3).
+int refcpu_triangle_region(const float* tri9, const float* normal3) {
+    ref::TriangleRef t;
+    for (int k = 0; k < 3; ++k) t.v[k] = ref::V3{tri9[3 * k], tri9[3 * k + 1], tri9[3 * k + 2]};
+    t.sidedness = 0;
+    t.margin = 0;
+    ref::ContactData c;
+    std::memset(&c, 0, sizeof(c));
+    c.normal = ref::V3{normal3[0], normal3[1], normal3[2]};
+    return ref::triangle_convex_region(t, c);
+}
+
+// TriangleSpherePairTester.GenerateContactCandidates; *region the Voronoi region.
+int refcpu_triangle_sphere(float radius, const float* tri9, int sidedness, float tri_margin, float* out8,
+                           int* region) {
+    ref::TriangleRef t;
+    for (int k = 0; k < 3; ++k) t.v[k] = ref::V3{tri9[3 * k], tri9[3 * k + 1], tri9[3 * k + 2]};
+    t.sidedness = sidedness;
+    t.margin = tri_margin;
+    ref::ContactData c;
+    int r = ref::triangle_sphere_candidates(radius, t, &c, region);
+    put_contact(c, out8);
+    return r;
+}
+
+// Toolbox.GetClosestPointOnTriangleToPoint: returns the region, out3 the point.
+int refcpu_closest_point_triangle(const float* tri9, const float* p3, float* out3) {
+    ref::V3 v[3];
+    for (int k = 0; k < 3; ++k) v[k] = ref::V3{tri9[3 * k], tri9[3 * k + 1], tri9[3 * k + 2]};
+    ref::V3 o;
+    int r = ref::closest_point_on_triangle(v[0], v[1], v[2], ref::V3{p3[0], p3[1], p3[2]}, &o);
+    out3[0] = o.x; out3[1] = o.y; out3[2] = o.z;
+    return r;
 }
 
 }  // extern "C"

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -81,6 +82,11 @@ struct Body {
     int pmode;
     float min_radius;
     Q4 q_world;
+    // A static mesh (StaticMesh.cs): its shape (refcpu_world::mesh_shapes),
+    // TriangleSidedness and ImproveBoundaryBehavior; -1 for an entity.
+    int mesh;
+    int sidedness;
+    bool improve_boundary;
 };
 
 struct ContactObj {
@@ -158,6 +164,18 @@ struct Pair {
     // a new pair, recomputed by ConvexPairHandler.UpdateTimeOfImpact in the
     // steps it has a continuous body and an active one, kept otherwise.
     float toi;
+    // Mesh pairs with a convex (not a sphere): the manifold's table of
+    // TriangleConvexPairTesters, refcpu_world::mesh_testers[mesh_slot].
+    int mesh_slot;
+};
+
+// One entry of TriangleMeshConvexContactManifold.activePairTesters
+// (TriangleMeshConvexContactManifold.cs:19-20): the triangle's vertex
+// indices (TriangleIndices) and its tester.
+struct MeshTester {
+    int ia, ib, ic;
+    TriTesterState st;
+    bool updated;
 };
 
 // BallSocketJoint (Constraints/TwoEntity/Joints/BallSocketJoint.cs) and its
@@ -300,6 +318,13 @@ struct refcpu_world {
     int64_t perm_index = 0;
     bool ccd = false;  // a body was ever passive or continuous
     std::vector<JointC> joints;
+    std::vector<MeshShape> mesh_shapes;
+    // Tester tables of the mesh pairs (Pair::mesh_slot), allocated as pairs
+    // are created (under the lock: pairs are updated on the pool) and freed
+    // when they end.
+    std::deque<std::vector<MeshTester>> mesh_testers;
+    std::vector<int> mesh_free;
+    std::mutex mesh_mu;
 };
 
 namespace {
@@ -355,6 +380,9 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
     b->pmode = (flags & BEPU_FLAG_CCD_CONTINUOUS) ? 2 : ((flags & BEPU_FLAG_CCD_PASSIVE) ? 1 : 0);
     if (b->pmode) w->ccd = true;
     b->q_world = b->q;
+    b->mesh = -1;
+    b->sidedness = SIDED_DOUBLE;
+    b->improve_boundary = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -421,7 +449,7 @@ inline bool body_active(const Body& b) { return !b.removed && (b.dynamic ? !b.as
 // candidate flag is cleared (which zeroes the time below the limit) and the
 // island becomes active; a kinematic member's flag becomes -1.
 void activate(Body& b) {
-    if (b.removed) return;
+    if (b.removed || b.mesh >= 0) return;  // a static collidable has no activity
     if (b.dynamic) {
         b.candidate = false;
         b.time_below = 0;
@@ -582,14 +610,11 @@ void force_update(refcpu_world* w, Body& b) {
     b.inertia_inv = m3_mul(m3_mul_transposed(b.R, b.local_inertia_inv), b.R);
 }
 
-// EntityCollidable.UpdateBoundingBox -> Shape.GetBoundingBox -> ExpandBoundingBox
-// (EntityCollidable.cs:140-148,195-257; BoxShape.cs:149-174; SphereShape.cs:75-86)
-void bounding_box(refcpu_world* w, Body& b) {
-    // BoundingBoxUpdater.cs:48: active entries only (a sleeping body keeps its
-    // box; recomputing an inactive kinematic body's gives the same box).
-    if (b.removed || (b.dynamic && b.asleep)) return;
+// Shape.GetBoundingBox at a pose (BoxShape.cs:149-174; SphereShape.cs:75-86;
+// CapsuleShape.cs:95-137; ConvexHullShape.cs:206-286).
+void shape_bounding_box(const refcpu_world* w, const Body& b, const V3& bpos, const Q4& bq, V3* bmin, V3* bmax) {
     if (b.shape == BEPU_SHAPE_BOX) {
-        M3 o = m3_from_quat(b.q);
+        M3 o = m3_from_quat(bq);
         V3 right{signf_int(o.m11) * b.half.x, signf_int(o.m21) * b.half.y, signf_int(o.m31) * b.half.z};
         V3 up{signf_int(o.m12) * b.half.x, signf_int(o.m22) * b.half.y, signf_int(o.m32) * b.half.z};
         V3 back{signf_int(o.m13) * b.half.x, signf_int(o.m23) * b.half.y, signf_int(o.m33) * b.half.z};
@@ -597,11 +622,11 @@ void bounding_box(refcpu_world* w, Body& b) {
         off.x = right.x * o.m11 + right.y * o.m21 + right.z * o.m31;
         off.y = up.x * o.m12 + up.y * o.m22 + up.z * o.m32;
         off.z = back.x * o.m13 + back.y * o.m23 + back.z * o.m33;
-        b.bmax = add(b.pos, off);
-        b.bmin = sub(b.pos, off);
+        *bmax = add(bpos, off);
+        *bmin = sub(bpos, off);
     } else if (b.shape == BEPU_SHAPE_CAPSULE) {
-        // CapsuleShape.GetBoundingBox (CapsuleShape.cs:95-137); Quaternion.TransformY (Quaternion.cs:586-608)
-        const Q4& r = b.q;
+        // Quaternion.TransformY (Quaternion.cs:586-608)
+        const Q4& r = bq;
         float x2 = r.x + r.x, y2 = r.y + r.y, z2 = r.z + r.z;
         float xx2 = r.x * x2, xy2 = r.x * y2, yz2 = r.y * z2, zz2 = r.z * z2;
         float wx2 = r.w * x2, wz2 = r.w * z2;
@@ -611,11 +636,10 @@ void bounding_box(refcpu_world* w, Body& b) {
         if (up.x > 0) { mx.x = up.x + m; mn.x = -up.x - m; } else { mx.x = -up.x + m; mn.x = up.x - m; }
         if (up.y > 0) { mx.y = up.y + m; mn.y = -up.y - m; } else { mx.y = -up.y + m; mn.y = up.y - m; }
         if (up.z > 0) { mx.z = up.z + m; mn.z = -up.z - m; } else { mx.z = -up.z + m; mn.z = up.z - m; }
-        b.bmin = add(b.pos, mn);
-        b.bmax = add(b.pos, mx);
+        *bmin = add(bpos, mn);
+        *bmax = add(bpos, mx);
     } else if (b.shape == BEPU_SHAPE_CONVEX_HULL) {
-        // ConvexHullShape.GetBoundingBox (ConvexHullShape.cs:206-286)
-        M3 o = m3_from_quat(b.q);
+        M3 o = m3_from_quat(bq);
         const V3* v = &w->hull_verts[b.hull_off];
         V3 right{o.m11, o.m21, o.m31}, up{o.m12, o.m22, o.m32}, back{o.m13, o.m23, o.m33};
         float maxX = dot(v[0], right), minX = maxX;
@@ -638,13 +662,23 @@ void bounding_box(refcpu_world* w, Body& b) {
                 v[iminY].x * o.m12 + v[iminY].y * o.m22 + v[iminY].z * o.m32,
                 v[iminZ].x * o.m13 + v[iminZ].y * o.m23 + v[iminZ].z * o.m33};
         float m = b.margin;
-        b.bmax = V3{b.pos.x + pos.x + m, b.pos.y + pos.y + m, b.pos.z + pos.z + m};
-        b.bmin = V3{b.pos.x + neg_.x - m, b.pos.y + neg_.y - m, b.pos.z + neg_.z - m};
+        *bmax = V3{bpos.x + pos.x + m, bpos.y + pos.y + m, bpos.z + pos.z + m};
+        *bmin = V3{bpos.x + neg_.x - m, bpos.y + neg_.y - m, bpos.z + neg_.z - m};
     } else {
         float r = b.radius;
-        b.bmin = V3{b.pos.x - r, b.pos.y - r, b.pos.z - r};
-        b.bmax = V3{b.pos.x + r, b.pos.y + r, b.pos.z + r};
+        *bmin = V3{bpos.x - r, bpos.y - r, bpos.z - r};
+        *bmax = V3{bpos.x + r, bpos.y + r, bpos.z + r};
     }
+}
+
+// EntityCollidable.UpdateBoundingBox -> Shape.GetBoundingBox -> ExpandBoundingBox
+// (EntityCollidable.cs:140-148,195-257)
+void bounding_box(refcpu_world* w, Body& b) {
+    // BoundingBoxUpdater.cs:48: active entries only (a sleeping body keeps its
+    // box; recomputing an inactive kinematic body's gives the same box).
+    if (b.removed || (b.dynamic && b.asleep)) return;
+    if (b.mesh >= 0) return;  // StaticMesh.UpdateBoundingBox: the tree's root box, fixed
+    shape_bounding_box(w, b, b.pos, b.q, &b.bmin, &b.bmax);
     float dt = w->s.dt;
     if (dt > 0) {
         if (b.v.x > 0) b.bmax.x += b.v.x * dt * 1.0f; else b.bmin.x += b.v.x * dt * 1.0f;
@@ -1140,13 +1174,409 @@ void update_general_convex(refcpu_world* w, Pair& p) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Static meshes: StaticMeshContactManifold over TriangleMeshConvexContactManifold
+// (CollisionTests/Manifolds/TriangleMeshConvexContactManifold.cs:101-434,
+// StaticMeshContactManifold.cs:33-71).  The mesh transform of the manifold is
+// the identity (MeshTransform, TriangleMeshConvexContactManifold.cs:76-82):
+// the mesh's vertices reach world space through its data's affine transform.
+
+const RT kMeshTransform{V3{0, 0, 0}, Q4{0, 0, 0, 1}};
+// CollisionDetectionSettings.nonconvexNormalDotMinimum (CollisionDetectionSettings.cs:52)
+const float kNonconvexNormalDotMinimum = .99f;
+
+// ContactReducer.ReduceContacts, general form (CollisionTests/ContactReducer.cs:19-195):
+// the manifold's contacts (list order) and the candidates; fills the contact
+// list indices to remove (ascending) and the candidates kept.
+void reduce_contacts_general(const Pair& p, const std::vector<ContactData>& cand, std::vector<int>* remove,
+                             std::vector<ContactData>* keep) {
+    const int nc = p.ccount;
+    auto cpos = [&](int i) { return p.slot[p.clist[i]].pos; };
+    float maxDepth = -FLT_MAX;
+    int deepest = -1;
+    V3 normal{0, 0, 0};
+    for (int i = 0; i < nc; i++) {
+        const ContactObj& c = p.slot[p.clist[i]];
+        normal = add(normal, c.normal);
+        if (c.depth > maxDepth) { deepest = i; maxDepth = c.depth; }
+    }
+    for (size_t i = 0; i < cand.size(); i++) {
+        normal = add(normal, cand[i].normal);
+        if (cand[i].depth > maxDepth) { deepest = nc + (int)i; maxDepth = cand[i].depth; }
+    }
+    if (length_sq(normal) < 1e-7f) {
+        if (nc > 0) normal = p.slot[p.clist[0]].normal;
+        else normal = cand[0].normal;
+    }
+    V3 deepPos = deepest < nc ? cpos(deepest) : cand[deepest - nc].pos;
+    float furthestDist = 0;
+    int furthest = -1;
+    for (int i = 0; i < nc; i++) {
+        float d = distance_sq(cpos(i), deepPos);
+        if (d > furthestDist) { furthestDist = d; furthest = i; }
+    }
+    for (size_t i = 0; i < cand.size(); i++) {
+        float d = distance_sq(cand[i].pos, deepPos);
+        if (d > furthestDist) { furthestDist = d; furthest = nc + (int)i; }
+    }
+    if (furthest == -1) {
+        if (nc > 0) {
+            for (int i = 1; i < nc; i++) remove->push_back(i);
+            return;
+        }
+        keep->push_back(cand[0]);
+        return;
+    }
+    V3 farPos = furthest < nc ? cpos(furthest) : cand[furthest - nc].pos;
+    V3 xAxis = sub(deepPos, farPos);
+    V3 yAxis = cross(xAxis, normal);
+    float minY = FLT_MAX, maxY = -FLT_MAX;
+    int iminY = -1, imaxY = -1;
+    for (int i = 0; i < nc; i++) {
+        float d = dot(cpos(i), yAxis);
+        if (d < minY) { iminY = i; minY = d; }
+        if (d > maxY) { imaxY = i; maxY = d; }
+    }
+    for (size_t i = 0; i < cand.size(); i++) {
+        float d = dot(cand[i].pos, yAxis);
+        if (d < minY) { iminY = nc + (int)i; minY = d; }
+        if (d > maxY) { imaxY = nc + (int)i; maxY = d; }
+    }
+    for (size_t i = 0; i < cand.size(); i++) {
+        int t = (int)i + nc;
+        if (t == deepest || t == furthest || t == iminY || t == imaxY) keep->push_back(cand[i]);
+    }
+    for (int i = 0; i < nc; i++)
+        if (!(i == deepest || i == furthest || i == iminY || i == imaxY)) remove->push_back(i);
+}
+
+// The boundary sets of the improved boundary handling
+// (TriangleMeshConvexContactManifold.cs:27-49, 773-787): edges are unordered
+// vertex index pairs.
+struct BoundaryContact {
+    bool should_correct;
+    V3 corrected;
+    int e0, e1;  // edge (vertex contacts: e0 is the vertex)
+    ContactData c;
+};
+struct BoundarySets {
+    std::vector<int> blocked_vertices;
+    std::vector<std::pair<int, int>> blocked_edges;
+    std::vector<BoundaryContact> edges, vertices;
+    void block_vertex(int v) {
+        if (!has_vertex(v)) blocked_vertices.push_back(v);
+    }
+    void block_edge(int a, int b) {
+        if (!has_edge(a, b)) blocked_edges.push_back({a, b});
+    }
+    bool has_vertex(int v) const {
+        for (int x : blocked_vertices)
+            if (x == v) return true;
+        return false;
+    }
+    bool has_edge(int a, int b) const {
+        for (const auto& e : blocked_edges)
+            if ((e.first == a && e.second == b) || (e.first == b && e.second == a)) return true;
+        return false;
+    }
+};
+
+// GetNormal (TriangleMeshConvexContactManifold.cs:450-481)
+V3 mesh_corrected_normal(const V3& uncorrected, const TriangleRef& t) {
+    V3 ab = sub(t.v[1], t.v[0]), ac = sub(t.v[2], t.v[0]);
+    V3 n;
+    if (t.sidedness == SIDED_DOUBLE) {
+        n = cross(ab, ac);
+        if (dot(n, uncorrected) < 0) n = neg(n);
+    } else if (t.sidedness == SIDED_CLOCKWISE) {
+        n = cross(ac, ab);
+    } else {
+        n = cross(ab, ac);
+    }
+    if (length_sq(n) < 1e-7f) n = uncorrected;
+    return n;
+}
+
+// AnalyzeCandidate (TriangleMeshConvexContactManifold.cs:483-596): true for a
+// face contact (always kept); edge and vertex contacts are queued and every
+// other region of the triangle is blocked.
+bool mesh_analyze(int region, const int ix[3], const TriangleRef& t, bool should_correct, const ContactData& c,
+                  BoundarySets& sets) {
+    const int A = ix[0], B = ix[1], C = ix[2];
+    BoundaryContact bc;
+    bc.should_correct = should_correct;
+    bc.c = c;
+    switch (region) {
+    case VR_A:
+    case VR_B:
+    case VR_C: {
+        const int v = region == VR_A ? A : (region == VR_B ? B : C);
+        bc.corrected = mesh_corrected_normal(c.normal, t);
+        bc.e0 = v;
+        bc.e1 = -1;
+        sets.vertices.push_back(bc);
+        sets.block_edge(A, B);
+        sets.block_edge(B, C);
+        sets.block_edge(A, C);
+        if (region != VR_A) sets.block_vertex(A);
+        if (region != VR_B) sets.block_vertex(B);
+        if (region != VR_C) sets.block_vertex(C);
+        return false;
+    }
+    case VR_AB:
+    case VR_AC:
+    case VR_BC: {
+        bc.corrected = mesh_corrected_normal(c.normal, t);
+        bc.e0 = region == VR_BC ? B : A;
+        bc.e1 = region == VR_AB ? B : C;
+        sets.edges.push_back(bc);
+        if (region != VR_AB) sets.block_edge(A, B);
+        if (region != VR_BC) sets.block_edge(B, C);
+        if (region != VR_AC) sets.block_edge(A, C);
+        sets.block_vertex(A);
+        sets.block_vertex(B);
+        sets.block_vertex(C);
+        return false;
+    }
+    default:
+        sets.block_edge(A, B);
+        sets.block_edge(B, C);
+        sets.block_edge(A, C);
+        sets.block_vertex(A);
+        sets.block_vertex(B);
+        sets.block_vertex(C);
+        return true;
+    }
+}
+
+// TriangleMeshConvexContactManifold.Update (TriangleMeshConvexContactManifold.cs:101-434)
+void update_mesh(refcpu_world* w, Pair& p) {
+    const Body& A = w->bodies[p.a];
+    const Body& M = w->bodies[p.b];
+    const MeshShape& shape = w->mesh_shapes[M.mesh];
+    const RT ta{A.pos, A.q};
+    // ContactRefresher.ContactRefresh (ContactRefresher.cs:18-69)
+    int rem[4], nrem = 0;
+    for (int k = 0; k < p.ccount; k++) {
+        int slot = p.clist[k];
+        ContactObj& c = p.slot[slot];
+        V3 na = rt_transform(p.off_a[slot], ta);
+        V3 nb = rt_transform(p.off_b[slot], kMeshTransform);
+        V3 ab = sub(nb, na);
+        float d = dot(ab, c.normal);
+        V3 t = mul(c.normal, d);
+        t = sub(ab, t);
+        d = length_sq(t);
+        if (d > w->s.contact_invalidation_length_sq) {
+            rem[nrem++] = k;
+        } else {
+            d = dot(ab, c.normal);
+            c.depth = p.base_depth[slot] - d;
+            if (c.depth < -w->s.max_contact_distance) rem[nrem++] = k;
+            else c.pos = mul(add(nb, na), .5f);
+        }
+    }
+    for (int i = nrem - 1; i >= 0; i--) manifold_remove(p, rem[i]);
+
+    std::vector<int> tris;
+    mesh_overlaps(shape, A.bmin, A.bmax, &tris);
+    const bool improve = M.improve_boundary;
+    BoundarySets sets;
+    std::vector<ContactData> cand;
+
+    // RigidTransform.Invert (RigidTransform.cs:95-100), AffineTransform.CreateFromRigidTransform
+    // (AffineTransform.cs:232-236), PrecomputeTriangleTransform (StaticMeshContactManifold.cs:44-49).
+    RT inv;
+    inv.q = qconj(A.q);
+    inv.pos = neg(qtransform(A.pos, inv.q));
+    Affine convex_inverse{m3_from_quat(inv.q), inv.pos};
+    const Affine to_local = affine_multiply(shape.xf, convex_inverse);
+    V3 lmin, lmax;
+    shape_bounding_box(w, A, V3{0, 0, 0}, Q4{0, 0, 0, 1}, &lmin, &lmax);
+    const M3 orientation = m3_from_quat(A.q);
+    const ConvexShapeRef cvx = shape_ref(w, A);
+    const bool sphere = p.kind == BEPU_PAIR_MESH_SPHERE;
+    std::vector<MeshTester>* testers = sphere ? nullptr : &w->mesh_testers[p.mesh_slot];
+    int guaranteed = 0;
+
+    // AddLocalContact and IsContactUnique (TriangleMeshConvexContactManifold.cs:436-447, 617-674)
+    auto add_local = [&](ContactData c) {
+        c.pos = add(m3_transform(c.pos, orientation), A.pos);
+        c.normal = m3_transform(c.normal, orientation);
+        for (int i = 0; i < p.ccount; i++) {
+            const int slot = p.clist[i];
+            ContactObj& e = p.slot[slot];
+            if (distance_sq(e.pos, c.pos) < w->s.contact_min_separation_sq) {
+                float d = dot(e.normal, c.normal);
+                if (std::fabs(d) >= kNonconvexNormalDotMinimum) {
+                    e.normal = c.normal;
+                    e.pos = c.pos;
+                    e.depth = c.depth;
+                    p.base_depth[slot] = c.depth;
+                    p.off_a[slot] = rt_transform_by_inverse(c.pos, ta);
+                    p.off_b[slot] = rt_transform_by_inverse(c.pos, kMeshTransform);
+                    return;
+                }
+            }
+        }
+        for (const ContactData& e : cand) {
+            if (distance_sq(e.pos, c.pos) < w->s.contact_min_separation_sq) {
+                float d = dot(e.normal, c.normal);
+                if (std::fabs(d) >= kNonconvexNormalDotMinimum) return;
+            }
+        }
+        cand.push_back(c);
+    };
+
+    for (int tri : tris) {
+        // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66)
+        const int ix[3] = {shape.idx[tri], shape.idx[tri + 1], shape.idx[tri + 2]};
+        TriangleRef t;
+        for (int k = 0; k < 3; ++k) t.v[k] = affine_transform(shape.verts[ix[k]], to_local);
+        t.sidedness = M.sidedness;
+        t.margin = 0;
+        // Toolbox.GetTriangleBoundingBox (BEPUutilities/Toolbox.cs:1516-1580) against the local box.
+        V3 tmin, tmax;
+        triangle_bounding_box(t.v[0], t.v[1], t.v[2], &tmin, &tmax);
+        if (tmin.x > lmax.x || tmin.y > lmax.y || tmin.z > lmax.z) continue;
+        if (lmin.x > tmax.x || lmin.y > tmax.y || lmin.z > tmax.z) continue;
+        ContactData out[2];
+        int n = 0, region_sphere = VR_ABC;
+        bool should_correct = false;
+        MeshTester* tester = nullptr;
+        if (sphere) {
+            n = triangle_sphere_candidates(cvx.margin, t, &out[0], &region_sphere);
+        } else {
+            for (MeshTester& m : *testers)
+                if (m.ia == ix[0] && m.ib == ix[1] && m.ic == ix[2]) { tester = &m; break; }
+            if (!tester) {
+                MeshTester m;
+                m.ia = ix[0];
+                m.ib = ix[1];
+                m.ic = ix[2];
+                m.st.state = TT_PLANE;
+                m.st.escapes = 0;
+                m.st.axis = V3{0, 0, 0};
+                testers->push_back(m);
+                tester = &testers->back();
+            }
+            tester->updated = true;
+            n = triangle_convex_candidates(cvx, A.min_radius, w->s.core_shape_scaling, t, &tester->st, out);
+            should_correct = tester->st.state == TT_DEEP;
+        }
+        for (int j = 0; j < n; j++) {
+            if (improve) {
+                const int region = sphere ? region_sphere : triangle_convex_region(t, out[j]);
+                if (mesh_analyze(region, ix, t, should_correct, out[j], sets)) {
+                    guaranteed++;
+                    add_local(out[j]);
+                }
+            } else {
+                add_local(out[j]);
+            }
+        }
+    }
+
+    if (improve) {
+        // Edge contacts fighting each other in one plane get the triangle normal
+        // (TriangleMeshConvexContactManifold.cs:241-316).
+        if (sets.vertices.empty() && guaranteed == 0 && sets.edges.size() > 1) {
+            bool same_plane = true, against = false;
+            V3 first = sets.edges[0].c.normal;
+            sets.edges[0].corrected = normalize(sets.edges[0].corrected);
+            float d = dot(first, sets.edges[0].corrected);
+            if (std::fabs(d) > .01f) {
+                same_plane = false;
+            } else {
+                for (size_t i = 1; i < sets.edges.size(); i++) {
+                    d = dot(sets.edges[i].c.normal, first);
+                    if (d < 0) against = true;
+                    d = dot(sets.edges[i].c.normal, sets.edges[0].corrected);
+                    if (std::fabs(d) > .01f) {
+                        same_plane = false;
+                        break;
+                    }
+                }
+            }
+            if (same_plane && against) {
+                sets.edges[0].c.normal = sets.edges[0].corrected;
+                sets.edges[0].should_correct = true;
+                for (size_t i = 1; i < sets.edges.size(); i++) {
+                    sets.edges[i].corrected = normalize(sets.edges[i].corrected);
+                    d = dot(sets.edges[i].corrected, sets.edges[i].c.normal);
+                    if (d < .01) {
+                        sets.edges[i].c.normal = sets.edges[i].corrected;
+                        sets.edges[i].should_correct = true;
+                    }
+                }
+            }
+        }
+        // Unblocked boundary contacts as they are; blocked ones with their
+        // normal corrected when allowed (TriangleMeshConvexContactManifold.cs:322-376).
+        auto emit = [&](BoundaryContact& bc, bool blocked) {
+            if (!blocked) {
+                add_local(bc.c);
+            } else if (bc.should_correct || guaranteed == 0) {
+                bc.corrected = normalize(bc.corrected);
+                float d = dot(bc.corrected, bc.c.normal);
+                bc.c.normal = bc.corrected;
+                bc.c.depth *= maxf(0, d);
+                add_local(bc.c);
+            }
+        };
+        for (BoundaryContact& bc : sets.edges) emit(bc, sets.has_edge(bc.e0, bc.e1));
+        for (BoundaryContact& bc : sets.vertices) emit(bc, sets.has_vertex(bc.e0));
+    }
+
+    // Stale pair testers (TriangleMeshConvexContactManifold.cs:387-401).
+    if (testers) {
+        for (int i = (int)testers->size() - 1; i >= 0; --i) {
+            if (!(*testers)[i].updated) {
+                (*testers)[i] = testers->back();
+                testers->pop_back();
+            } else {
+                (*testers)[i].updated = false;
+            }
+        }
+    }
+
+    if (p.ccount + (int)cand.size() > 4) {
+        std::vector<int> remove;
+        std::vector<ContactData> keep;
+        reduce_contacts_general(p, cand, &remove, &keep);
+        for (int i = (int)remove.size() - 1; i >= 0; i--) manifold_remove(p, remove[i]);
+        for (int i = (int)keep.size() - 1; i >= 0; i--) manifold_add_general(p, keep[i], ta, kMeshTransform);
+    } else {
+        for (const ContactData& c : cand) manifold_add_general(p, c, ta, kMeshTransform);
+    }
+}
+
 Pair make_pair_state(refcpu_world* w, uint32_t i, uint32_t j) {
     Pair p;
     std::memset(&p, 0, sizeof(p));
     const Body& bi = w->bodies[i];
     const Body& bj = w->bodies[j];
     int si = bi.shape, sj = bj.shape;
-    if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) { p.kind = BEPU_PAIR_BOX_BOX; p.a = i; p.b = j; }
+    p.mesh_slot = -1;
+    if (si == BEPU_SHAPE_STATIC_MESH || sj == BEPU_SHAPE_STATIC_MESH) {
+        // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79): A is the convex.
+        p.a = si == BEPU_SHAPE_STATIC_MESH ? j : i;
+        p.b = si == BEPU_SHAPE_STATIC_MESH ? i : j;
+        if (w->bodies[p.a].shape == BEPU_SHAPE_SPHERE) {
+            p.kind = BEPU_PAIR_MESH_SPHERE;
+        } else {
+            p.kind = BEPU_PAIR_MESH_CONVEX;
+            std::lock_guard<std::mutex> g(w->mesh_mu);
+            if (!w->mesh_free.empty()) {
+                p.mesh_slot = w->mesh_free.back();
+                w->mesh_free.pop_back();
+            } else {
+                p.mesh_slot = (int)w->mesh_testers.size();
+                w->mesh_testers.emplace_back();
+            }
+            w->mesh_testers[p.mesh_slot].clear();
+        }
+    } else if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) { p.kind = BEPU_PAIR_BOX_BOX; p.a = i; p.b = j; }
     else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_SPHERE) { p.kind = BEPU_PAIR_SPHERE_SPHERE; p.a = i; p.b = j; }
     else if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_SPHERE) { p.kind = BEPU_PAIR_BOX_SPHERE; p.a = i; p.b = j; }
     else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_BOX) { p.kind = BEPU_PAIR_BOX_SPHERE; p.a = j; p.b = i; }
@@ -2127,6 +2557,7 @@ void step_once(refcpu_world* w) {
             p.frozen = false;
             if (p.kind == BEPU_PAIR_BOX_BOX) update_box_box(w, p);
             else if (p.kind == BEPU_PAIR_CONVEX) update_general_convex(w, p);
+            else if (p.kind == BEPU_PAIR_MESH_CONVEX || p.kind == BEPU_PAIR_MESH_SPHERE) update_mesh(w, p);
             else update_single_contact(w, p);
         }
     });
@@ -2140,6 +2571,15 @@ void step_once(refcpu_world* w) {
         }
     activity_pass(w, next);
     split_activation(w, w->pairs, next);
+    if (!w->mesh_testers.empty()) {
+        // The tester tables of mesh pairs that ended are free again.
+        std::vector<char> used(w->mesh_testers.size(), 0);
+        for (const Pair& p : next.vals)
+            if (p.mesh_slot >= 0) used[p.mesh_slot] = 1;
+        w->mesh_free.clear();
+        for (int i = (int)used.size() - 1; i >= 0; --i)
+            if (!used[i]) w->mesh_free.push_back(i);
+    }
     w->pairs = std::move(next);
     w->overlaps = w->pairs.keys;
 
@@ -2350,6 +2790,83 @@ int refcpu_add_convex_hull(refcpu_world* w, const bepu_motion_state* st, const f
     b.min_radius = minimum_radius + b.margin;  // ConvexHullShape.cs:170
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+// new StaticMeshShape(vertices, indices, worldTransform) (StaticMeshShape.cs:36-47,
+// TransformableMeshData.cs:40-47): `transform` is the AffineTransform's linear
+// part (Matrix3x3 M11..M33, row-major) and translation, or null for the
+// identity; its MeshBoundingBoxTree is built here (TriangleMesh.cs:46-50).
+int refcpu_create_mesh_shape(refcpu_world* w, const float* verts, uint32_t nverts, const int32_t* idx, uint32_t nidx,
+                             const float* transform, uint32_t* id) {
+    if (!verts || !idx || nverts == 0 || nidx == 0 || nidx % 3 != 0) return BEPU_E_INVALID_ARG;
+    if (!finite3(verts, 3 * nverts) || (transform && !finite3(transform, 12))) return BEPU_E_INVALID_ARG;
+    for (uint32_t i = 0; i < nidx; ++i)
+        if (idx[i] < 0 || (uint32_t)idx[i] >= nverts) return BEPU_E_INVALID_ARG;
+    MeshShape m;
+    for (uint32_t i = 0; i < nverts; ++i) m.verts.push_back(V3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]});
+    m.idx.assign(idx, idx + nidx);
+    if (transform) {
+        const float* t = transform;
+        m.xf.m = M3{t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7], t[8]};
+        m.xf.t = V3{t[9], t[10], t[11]};
+    } else {
+        m.xf.m = M3{1, 0, 0, 0, 1, 0, 0, 0, 1};
+        m.xf.t = V3{0, 0, 0};
+    }
+    mesh_build_tree(&m);
+    if (id) *id = (uint32_t)w->mesh_shapes.size();
+    w->mesh_shapes.push_back(std::move(m));
+    return BEPU_OK;
+}
+
+// new StaticMesh(...) added to the space (StaticMesh.cs): a static collidable
+// (kinematic collision group, never active) whose box is the tree's root box.
+int refcpu_add_static_mesh(refcpu_world* w, uint32_t shape, int sidedness, int improve_boundary,
+                           const bepu_material* m, uint32_t* id) {
+    if (shape >= w->mesh_shapes.size() || sidedness < 0 || sidedness > 2) return BEPU_E_INVALID_ARG;
+    Body b{};
+    std::memset(&b, 0, sizeof(b));
+    b.shape = BEPU_SHAPE_STATIC_MESH;
+    bepu_motion_state st{};
+    st.orientation[3] = 1;
+    init_body(w, &b, &st, 0, m, 0, M3{0, 0, 0, 0, 0, 0, 0, 0, 0});
+    b.time_below = 1;  // never active (StaticCollidable.IsActive, StaticCollidable.cs:72-75)
+    b.mesh = (int)shape;
+    b.sidedness = sidedness;
+    b.improve_boundary = improve_boundary != 0;
+    const MeshShape& ms = w->mesh_shapes[shape];
+    b.bmin = ms.nodes[0].bmin;
+    b.bmax = ms.nodes[0].bmax;
+    if (id) *id = (uint32_t)w->bodies.size();
+    w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+// The flattened tree of a mesh shape (tests): per node bmin[3], bmax[3], a, b.
+int refcpu_get_mesh_tree(refcpu_world* w, uint32_t shape, float* boxes, int32_t* links, uint32_t cap, uint32_t* n) {
+    if (shape >= w->mesh_shapes.size()) return BEPU_E_INVALID_ARG;
+    const MeshShape& m = w->mesh_shapes[shape];
+    *n = (uint32_t)m.nodes.size();
+    for (uint32_t k = 0; k < m.nodes.size() && k < cap; ++k) {
+        const MeshNode& nd = m.nodes[k];
+        float* o = boxes + 6 * k;
+        o[0] = nd.bmin.x; o[1] = nd.bmin.y; o[2] = nd.bmin.z;
+        o[3] = nd.bmax.x; o[4] = nd.bmax.y; o[5] = nd.bmax.z;
+        links[2 * k] = nd.a;
+        links[2 * k + 1] = nd.b;
+    }
+    return BEPU_OK;
+}
+
+// MeshBoundingBoxTree.GetOverlaps for a box (tests): triangles in traversal order.
+int refcpu_mesh_overlaps(refcpu_world* w, uint32_t shape, const float* bmin, const float* bmax, int32_t* out,
+                         uint32_t cap, uint32_t* n) {
+    if (shape >= w->mesh_shapes.size()) return BEPU_E_INVALID_ARG;
+    std::vector<int> t;
+    mesh_overlaps(w->mesh_shapes[shape], V3{bmin[0], bmin[1], bmin[2]}, V3{bmax[0], bmax[1], bmax[2]}, &t);
+    *n = (uint32_t)t.size();
+    for (uint32_t k = 0; k < t.size() && k < cap; ++k) out[k] = t[k];
     return BEPU_OK;
 }
 
