@@ -21,8 +21,8 @@ LIB = os.path.join(OUT_DIR, "libbepu_gpu.so")
 ARCH = os.environ.get("BEPU_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["k_bodies.hip", "k_broadphase.hip", "k_bvh.hip", "k_grid.hip", "k_narrowphase.hip", "k_convex.hip",
-           "k_island.hip", "k_solver.hip", "k_sleep.hip", "k_ccd.hip", "k_joint.hip", "hull.cpp", "world.cpp",
-           "noalloc.cpp"]
+           "k_island.hip", "k_solver.hip", "k_sleep.hip", "k_ccd.hip", "k_joint.hip", "k_mesh.hip", "hull.cpp",
+           "mesh_tree.cpp", "world.cpp", "noalloc.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC",

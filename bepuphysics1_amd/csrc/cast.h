@@ -259,7 +259,8 @@ BEPU_D float rs_error_tolerance(const RSimplex& s, f3 o) {
 }
 
 // ConvexShape.GetLocalExtremePoint (ConvexShape.cs:104-113): with the margin.
-BEPU_D f3 local_extreme_with_margin(const Shape& s, f3 d) {
+template <class S>
+BEPU_D f3 local_extreme_with_margin(const S& s, f3 d) {
     f3 p = support_local(s, d);
     float ls = lensq3(d);
     if (ls > kEpsilon) {
@@ -272,8 +273,12 @@ BEPU_D f3 local_extreme_with_margin(const Shape& s, f3 d) {
 // GJKToolbox.SphereCast (GJKToolbox.cs:387-459): a ray from `origin` along
 // `dir` (lengths in units of dir) swept by a sphere of `radius` against the
 // posed shape; t receives the hit parameter, also where the cast fails (0
-// after too many iterations).
-BEPU_D bool sphere_cast(f3 origin, f3 dir, float radius, const Shape& shape, const RT& xf, float max_length, float& t) {
+// after too many iterations); `normal` (optional) RayHit.Normal, the search
+// direction of the last advance in world space.
+template <class S>
+BEPU_D bool sphere_cast(f3 origin, f3 dir, float radius, const S& shape, const RT& xf, float max_length, float& t,
+                        f3* normal = nullptr) {
+    f3 hn = zero3();
     origin = origin - xf.pos;
     const q4 cj = qconj(xf.q);
     origin = qrotate(origin, cj);
@@ -302,17 +307,22 @@ BEPU_D bool sphere_cast(f3 origin, f3 dir, float radius, const Shape& shape, con
                 return false;
             }
             loc = dir * T + origin;
+            hn = v;
         }
         const RSimplex shifted = rs_add(simplex, p, loc);
         v = rs_closest(shifted, simplex);
     }
     t = T;
+    if (normal) *normal = qrotate(hn, xf.q);
     return true;
 }
 
 // GJKToolbox.RayCast (GJKToolbox.cs:195-265): the ray against the shape with
 // its margin.
-BEPU_D bool ray_cast(f3 origin, f3 dir, const Shape& shape, const RT& xf, float max_length, float& t) {
+template <class S>
+BEPU_D bool ray_cast(f3 origin, f3 dir, const S& shape, const RT& xf, float max_length, float& t,
+                     f3* normal = nullptr) {
+    f3 hn = zero3();
     origin = origin - xf.pos;
     const q4 cj = qconj(xf.q);
     origin = qrotate(origin, cj);
@@ -342,22 +352,25 @@ BEPU_D bool ray_cast(f3 origin, f3 dir, const Shape& shape, const RT& xf, float 
                 return false;
             }
             loc = dir * T + origin;
+            hn = off;
         }
         const RSimplex shifted = rs_add(simplex, e, loc);
         off = rs_closest(shifted, simplex);
     }
     t = T;
+    if (normal) *normal = qrotate(hn, xf.q);
     return true;
 }
 
 // GJKToolbox.CCDSphereCast (GJKToolbox.cs:470-495): the sphere cast, retried
 // with the radius scaled by `scaling` while it starts embedded (a hit at 0),
 // up to three times, then a plain ray cast.
-BEPU_D bool ccd_sphere_cast(f3 origin, f3 dir, float radius, const Shape& shape, const RT& xf, float max_length,
-                            float scaling, float& t) {
+template <class S>
+BEPU_D bool ccd_sphere_cast(f3 origin, f3 dir, float radius, const S& shape, const RT& xf, float max_length,
+                            float scaling, float& t, f3* normal = nullptr) {
     for (int iterations = 1;; ++iterations) {
         float T;
-        if (sphere_cast(origin, dir, radius, shape, xf, max_length, T) && T > 0) {
+        if (sphere_cast(origin, dir, radius, shape, xf, max_length, T, normal) && T > 0) {
             t = T;
             return true;
         }
@@ -367,7 +380,7 @@ BEPU_D bool ccd_sphere_cast(f3 origin, f3 dir, float radius, const Shape& shape,
         }
         radius *= scaling;
         if (iterations > 3) {
-            const bool hit = ray_cast(origin, dir, shape, xf, max_length, T);
+            const bool hit = ray_cast(origin, dir, shape, xf, max_length, T, normal);
             t = T;
             return hit && T > 0;
         }

@@ -83,6 +83,23 @@ BEPU_D f3 support_local(const Shape& s, f3 d) {
     return zero3();
 }
 
+// A mesh triangle in the local space of the convex it is tested against
+// (TriangleShape, CollisionShapes/ConvexShapes/TriangleShape.cs), as shape B of
+// the GJK and MPR routines below (which take B's type as a template
+// parameter, so the closed set of entity shapes above keeps its own code).
+struct TriShape {
+    f3 v0, v1, v2;
+    float margin;
+};
+
+// TriangleShape.GetLocalExtremePointWithoutMargin (TriangleShape.cs:166-185)
+BEPU_D f3 support_local(const TriShape& s, f3 d) {
+    float dA = dot3(d, s.v0), dB = dot3(d, s.v1), dC = dot3(d, s.v2);
+    if (dA > dB && dA > dC) return s.v0;
+    if (dB > dC) return s.v1;
+    return s.v2;
+}
+
 // Quaternion.Concatenate (BEPUutilities/Quaternion.cs:101-118)
 BEPU_D q4 qconcat(q4 a, q4 b) {
     q4 r;
@@ -107,7 +124,8 @@ BEPU_D RT local_transform(const RT& a, const RT& b) {
 }
 
 // ConvexShape.GetExtremePointWithoutMargin (ConvexShape.cs:71-80)
-BEPU_D f3 extreme_without_margin(const Shape& s, f3 direction, const RT& t) {
+template <class SB>
+BEPU_D f3 extreme_without_margin(const SB& s, f3 direction, const RT& t) {
     f3 d = qrotate(direction, qconj(t.q));
     f3 p = support_local(s, d);
     p = qrotate(p, t.q);
@@ -116,7 +134,8 @@ BEPU_D f3 extreme_without_margin(const Shape& s, f3 direction, const RT& t) {
 
 // GetLocalMinkowskiExtremePoint, first overload, with the contribution form of
 // ExpandMinkowskiSum (MinkowskiToolbox.cs:38-50, 133-148).
-BEPU_D f3 mk_extreme(const Shape& a, const Shape& b, f3 d, const RT& lb) {
+template <class SB>
+BEPU_D f3 mk_extreme(const Shape& a, const SB& b, f3 d, const RT& lb) {
     f3 e = support_local(a, d);
     f3 v = extreme_without_margin(b, -d, lb);
     e = e - v;
@@ -129,7 +148,8 @@ BEPU_D f3 mk_extreme(const Shape& a, const Shape& b, f3 d, const RT& lb) {
 // ExpandMinkowskiSum (MinkowskiToolbox.cs:63-80, 160-176) when `per_shape`,
 // else the first overload (as mk_extreme).  The two round differently; one
 // function keeps a single copy of the support scans in the surface cast.
-BEPU_D f3 mk_extreme_sel(const Shape& a, const Shape& b, f3 d, const RT& lb, bool per_shape, f3* ea) {
+template <class SB>
+BEPU_D f3 mk_extreme_sel(const Shape& a, const SB& b, f3 d, const RT& lb, bool per_shape, f3* ea) {
     f3 pa = support_local(a, d);
     f3 pb = extreme_without_margin(b, -d, lb);
     float ls = lensq3(d);
@@ -298,7 +318,8 @@ BEPU_D float ss_error_tolerance(const SSimplex& s) {
 }
 
 // GJKToolbox.AreShapesIntersecting with a warm-started axis (GJKToolbox.cs:47-90)
-BEPU_D bool gjk_intersecting(const Shape& a, const Shape& b, const RT& ta, const RT& tb, f3* axis) {
+template <class SB>
+BEPU_D bool gjk_intersecting(const Shape& a, const SB& b, const RT& ta, const RT& tb, f3* axis) {
     RT lb = local_transform(ta, tb);
     SSimplex s{mk_extreme(a, b, *axis, lb), zero3(), zero3(), zero3(), S_POINT};  // AddNewSimplexPoint on Empty
     f3 cp;
@@ -554,16 +575,21 @@ BEPU_D bool ps_tetrahedron(PSimplex& s, float tol, f3* point) {
 }
 
 // GJKToolbox.GetClosestPoints with the pair tester's cached simplex: a Point
-// simplex at the shapes' origins (GJKToolbox.cs:131-182, PairSimplex.cs:163-193,
-// :341-370, :861-965).  Closest points of the cores, in world space.
-BEPU_D bool gjk_closest_points(const Shape& a, const Shape& b, const RT& ta, const RT& tb, f3* ca, f3* cb) {
+// simplex whose B-side local point is `simplex_b` (the shapes' origins for
+// GeneralConvexPairTester, the triangle's centroid for
+// TriangleConvexPairTester.cs:241-242; GJKToolbox.cs:131-182,
+// PairSimplex.cs:163-193, :341-370, :861-965).  Closest points of the cores,
+// in world space.
+template <class SB>
+BEPU_D bool gjk_closest_points(const Shape& a, const SB& b, const RT& ta, const RT& tb, f3* ca, f3* cb,
+                               f3 simplex_b = f3{0, 0, 0}) {
     RT lb = local_transform(ta, tb);
     PSimplex s;
     ps_clear(s);
     float prev_distance = FLT_MAX;
     float tol = 0;
     s.state = S_POINT;
-    s.sb[0] = qrotate(zero3(), lb.q) + lb.pos;
+    s.sb[0] = qrotate(simplex_b, lb.q) + lb.pos;
     s.A = s.sa[0] - s.sb[0];
     f3 cp;
     int count = 0;
@@ -640,9 +666,11 @@ BEPU_D bool gjk_closest_points(const Shape& a, const Shape& b, const RT& ta, con
 // ---------------------------------------------------------------------------
 // MPR (MPRToolbox.cs)
 
-// MPRToolbox.AreLocalShapesOverlapping (MPRToolbox.cs:413-598)
-BEPU_D bool mpr_overlapping(const Shape& a, const Shape& b, const RT& lb) {
-    f3 ray = lb.pos;
+// MPRToolbox.AreLocalShapesOverlapping (MPRToolbox.cs:413-598): the internal
+// overload at :426 with its origin ray; the public one at :413 passes B's
+// local position.
+template <class SB>
+BEPU_D bool mpr_overlapping_ray(const Shape& a, const SB& b, f3 ray, const RT& lb) {
     if (lensq3(ray) < kEpsilon) return true;
     f3 v0 = -ray;
     f3 n = ray;
@@ -689,6 +717,8 @@ BEPU_D bool mpr_overlapping(const Shape& a, const Shape& b, const RT& lb) {
     }
 }
 
+BEPU_D bool mpr_overlapping(const Shape& a, const Shape& b, const RT& lb) { return mpr_overlapping_ray(a, b, lb.pos, lb); }
+
 // Toolbox.GetBarycentricCoordinates (BEPUutilities/Toolbox.cs:1665-1737)
 BEPU_D void barycentric(f3 p, f3 a, f3 b, f3 c, float* wa, float* wb, float* wc) {
     f3 ab = b - a, ac = c - a;
@@ -727,7 +757,8 @@ BEPU_D void barycentric(f3 p, f3 a, f3 b, f3 c, float* wa, float* wb, float* wc)
 // MPRToolbox.LocalSurfaceCast, both overloads (MPRToolbox.cs:610-880 and
 // 886-1172): `with_position` selects the overload that expands each shape's
 // support point separately and interpolates the contact position.
-BEPU_D void surface_cast(const Shape& a, const Shape& b, const RT& lb, f3 dir, bool with_position, float* t, f3* normal,
+template <class SB>
+BEPU_D void surface_cast(const Shape& a, const SB& b, const RT& lb, f3 dir, bool with_position, float* t, f3* normal,
                          f3* position) {
     f3 v1A = zero3(), v2A = zero3(), v3A = zero3(), v4A = zero3();
     f3 n = dir;

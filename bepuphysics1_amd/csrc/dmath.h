@@ -116,6 +116,20 @@ BEPU_HD f3 mat_transform(f3 v, const m3& m) {
     return f3{v.x * m.m11 + v.y * m.m21 + v.z * m.m31, v.x * m.m12 + v.y * m.m22 + v.z * m.m32,
               v.x * m.m13 + v.y * m.m23 + v.z * m.m33};
 }
+// Matrix3x3.Multiply (Matrix3x3.cs:766-791)
+BEPU_HD m3 mat_mul(const m3& a, const m3& b) {
+    m3 r;
+    r.m11 = a.m11 * b.m11 + a.m12 * b.m21 + a.m13 * b.m31;
+    r.m12 = a.m11 * b.m12 + a.m12 * b.m22 + a.m13 * b.m32;
+    r.m13 = a.m11 * b.m13 + a.m12 * b.m23 + a.m13 * b.m33;
+    r.m21 = a.m21 * b.m11 + a.m22 * b.m21 + a.m23 * b.m31;
+    r.m22 = a.m21 * b.m12 + a.m22 * b.m22 + a.m23 * b.m32;
+    r.m23 = a.m21 * b.m13 + a.m22 * b.m23 + a.m23 * b.m33;
+    r.m31 = a.m31 * b.m11 + a.m32 * b.m21 + a.m33 * b.m31;
+    r.m32 = a.m31 * b.m12 + a.m32 * b.m22 + a.m33 * b.m32;
+    r.m33 = a.m31 * b.m13 + a.m32 * b.m23 + a.m33 * b.m33;
+    return r;
+}
 // Matrix3x3.TransformTranspose (Matrix3x3.cs:1113-1125)
 BEPU_HD f3 mat_transform_t(f3 v, const m3& m) {
     return f3{v.x * m.m11 + v.y * m.m12 + v.z * m.m13, v.x * m.m21 + v.y * m.m22 + v.z * m.m23,

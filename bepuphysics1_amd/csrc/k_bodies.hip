@@ -2,7 +2,7 @@
 // force update, bounding box (one fused pass), and position integration.
 //
 // HBM-bound elementwise kernels: one thread per body, float4 SoA loads.
-#include "world.h"
+#include "body_shape.h"
 
 namespace bepu {
 
@@ -161,73 +161,8 @@ __global__ void __launch_bounds__(256) k_body_pre(DeviceWorld w) {
             w.stab[i] = make_float4(velocity, tb, slowing, st.w);
         }
         // bounding box
-        float4 sh = w.shape[i];
         f3 mn, mx;
-        if (sh.w == (float)BEPU_SHAPE_BOX) {
-            m3 o = mat_from_quat(q);
-            f3 right{sign_ref(o.m11) * sh.x, sign_ref(o.m21) * sh.y, sign_ref(o.m31) * sh.z};
-            f3 up{sign_ref(o.m12) * sh.x, sign_ref(o.m22) * sh.y, sign_ref(o.m32) * sh.z};
-            f3 back{sign_ref(o.m13) * sh.x, sign_ref(o.m23) * sh.y, sign_ref(o.m33) * sh.z};
-            f3 off;
-            off.x = right.x * o.m11 + right.y * o.m21 + right.z * o.m31;
-            off.y = up.x * o.m12 + up.y * o.m22 + up.z * o.m32;
-            off.z = back.x * o.m13 + back.y * o.m23 + back.z * o.m33;
-            f3 p = f3_of(P);
-            mx = p + off;
-            mn = p - off;
-        } else if (sh.w == (float)BEPU_SHAPE_CAPSULE) {
-            // CapsuleShape.GetBoundingBox (CapsuleShape.cs:95-137) with Quaternion.TransformY
-            // (Quaternion.cs:586-608): the segment's upper end, grown by the radius.
-            float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
-            float xx2 = q.x * x2, xy2 = q.x * y2, yz2 = q.y * z2, zz2 = q.z * z2;
-            float wx2 = q.w * x2, wz2 = q.w * z2;
-            float hl = sh.y, r = sh.x;
-            f3 up{hl * (xy2 - wz2), hl * (1.f - xx2 - zz2), hl * (yz2 + wx2)};
-            f3 lo, hi;
-            if (up.x > 0) { hi.x = up.x + r; lo.x = -up.x - r; } else { hi.x = -up.x + r; lo.x = up.x - r; }
-            if (up.y > 0) { hi.y = up.y + r; lo.y = -up.y - r; } else { hi.y = -up.y + r; lo.y = up.y - r; }
-            if (up.z > 0) { hi.z = up.z + r; lo.z = -up.z - r; } else { hi.z = -up.z + r; lo.z = up.z - r; }
-            f3 p = f3_of(P);
-            mn = p + lo;
-            mx = p + hi;
-        } else if (sh.w == (float)BEPU_SHAPE_CONVEX_HULL) {
-            // ConvexHullShape.GetBoundingBox (ConvexHullShape.cs:206-286): extreme
-            // vertices along the rotated axes (first extreme kept on ties), then
-            // EntityShape.TransformLocalExtremePoints (EntityShape.cs:58-63).
-            m3 o = mat_from_quat(q);
-            const float4* hv = w.hull_verts + __float_as_uint(sh.y);
-            uint32_t nv = __float_as_uint(sh.z);
-            f3 right{o.m11, o.m21, o.m31}, upv{o.m12, o.m22, o.m32}, back{o.m13, o.m23, o.m33};
-            f3 v0 = f3_of(hv[0]);
-            float maxX = dot3(v0, right), minX = maxX;
-            float maxY = dot3(v0, upv), minY = maxY;
-            float maxZ = dot3(v0, back), minZ = maxZ;
-            uint32_t iminX = 0, imaxX = 0, iminY = 0, imaxY = 0, iminZ = 0, imaxZ = 0;
-            for (uint32_t k = 1; k < nv; ++k) {
-                f3 v = f3_of(hv[k]);
-                float d = dot3(v, right);
-                if (d < minX) { minX = d; iminX = k; } else if (d > maxX) { maxX = d; imaxX = k; }
-                d = dot3(v, upv);
-                if (d < minY) { minY = d; iminY = k; } else if (d > maxY) { maxY = d; imaxY = k; }
-                d = dot3(v, back);
-                if (d < minZ) { minZ = d; iminZ = k; } else if (d > maxZ) { maxZ = d; imaxZ = k; }
-            }
-            f3 a = f3_of(hv[imaxX]), b = f3_of(hv[imaxY]), c = f3_of(hv[imaxZ]);
-            f3 pos{a.x * o.m11 + a.y * o.m21 + a.z * o.m31, b.x * o.m12 + b.y * o.m22 + b.z * o.m32,
-                   c.x * o.m13 + c.y * o.m23 + c.z * o.m33};
-            a = f3_of(hv[iminX]);
-            b = f3_of(hv[iminY]);
-            c = f3_of(hv[iminZ]);
-            f3 neg{a.x * o.m11 + a.y * o.m21 + a.z * o.m31, b.x * o.m12 + b.y * o.m22 + b.z * o.m32,
-                   c.x * o.m13 + c.y * o.m23 + c.z * o.m33};
-            float m = sh.x;
-            mx = f3{P.x + pos.x + m, P.y + pos.y + m, P.z + pos.z + m};
-            mn = f3{P.x + neg.x - m, P.y + neg.y - m, P.z + neg.z - m};
-        } else {
-            float r = sh.x;
-            mn = f3{P.x - r, P.y - r, P.z - r};
-            mx = f3{P.x + r, P.y + r, P.z + r};
-        }
+        shape_aabb(w, w.shape[i], f3_of(P), q, &mn, &mx);
         float dt = s.dt;
         if (dt > 0) {
             if (v.x > 0) mx.x += v.x * dt * 1.0f; else mn.x += v.x * dt * 1.0f;
@@ -458,7 +393,7 @@ __global__ void __launch_bounds__(256) k_set_states(DeviceWorld w, const StateRe
     if (f & BF_DYNAMIC) {
         w.bflags[b] = (f & ~(uint32_t)BF_CANDIDATE) | BF_WAKE;
         w.stab[b].y = 0.0f;
-    } else if (f & BF_PRESENT) {
+    } else if ((f & BF_PRESENT) && !(f & BF_MESH)) {
         w.stab[b].y = -1.0f;
         w.bflags[b] = f | BF_KIN_ACTIVE;
     }

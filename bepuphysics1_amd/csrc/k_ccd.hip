@@ -16,6 +16,7 @@
 // Launched only in worlds where a body was made passive or continuous.
 #include "body_shape.h"
 #include "cast.h"
+#include "mesh.h"
 
 namespace bepu {
 
@@ -71,6 +72,64 @@ __device__ __forceinline__ float pair_toi(const DeviceWorld& w, uint32_t a, uint
     return toi;
 }
 
+// StaticMeshPairHandler.UpdateTimeOfImpact (StaticMeshPairHandler.cs:95-159)
+// for an active continuous convex a (the mesh m never moves): its core
+// sphere swept along its motion against each triangle its box overlaps (the
+// manifold's overlapped triangles of this step), each triangle moved by the
+// convex's position; a hit further than Toolbox.BigEpsilon (on a one-sided
+// mesh only from the side the mesh keeps) shortens the time of impact.
+__device__ float mesh_toi(const DeviceWorld& w, uint32_t a, uint32_t m, float toi) {
+    const f3 velocity = f3_of(w.linv[a]) * w.s.dt;
+    const float vsq = lensq3(velocity);
+    const float minimumRadius = w.minrad[a] * w.s.core_shape_scaling;
+    toi = 1;
+    if (!(minimumRadius * minimumRadius < vsq)) return toi;
+    const float4 msh = w.shape[m];
+    const MeshInfo mi = mesh_info(w, __float_as_uint(msh.x));
+    const int sided = (int)msh.y;
+    const f3 pa = f3_of(w.pos[a]);
+    const f3 qmin = f3_of(w.bmin[a]), qmax = f3_of(w.bmax[a]);
+    const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
+    auto overlaps = [&](int node) {
+        const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
+        if (qmin.x > hi.x || qmin.y > hi.y || qmin.z > hi.z) return false;
+        if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
+        return true;
+    };
+    const cvx::RT identity{f3{0, 0, 0}, q4{0, 0, 0, 1}};
+    int stack[kMeshStack];
+    int top = 0;
+    if (mi.ntri > 0 && overlaps(0)) stack[top++] = 0;
+    while (top > 0) {
+        const int node = stack[--top];
+        const int ca = __float_as_int(nodes[2 * node].w), cb = __float_as_int(nodes[2 * node + 1].w);
+        if (ca >= 0) {
+            if (overlaps(cb) && top < kMeshStack) stack[top++] = cb;
+            if (overlaps(ca) && top < kMeshStack) stack[top++] = ca;
+            continue;
+        }
+        cvx::TriShape t;
+        const int32_t* ix = w.mesh_idx + mi.ibase + cb;
+        t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), mi.m, mi.t) - pa;
+        t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), mi.m, mi.t) - pa;
+        t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), mi.m, mi.t) - pa;
+        t.margin = 0;
+        float hitT;
+        f3 hitN;
+        if (cvx::ccd_sphere_cast(f3{0, 0, 0}, velocity, minimumRadius, t, identity, toi, w.s.core_shape_scaling, hitT,
+                                 &hitN) &&
+            hitT > cvx::kBigEpsilon) {
+            if (sided != SIDED_DOUBLE) {
+                const float d = dot3(cross3(t.v1 - t.v0, t.v2 - t.v0), hitN);
+                if ((sided == SIDED_COUNTERCLOCKWISE && d < 0) || (sided == SIDED_CLOCKWISE && d > 0)) toi = hitT;
+            } else {
+                toi = hitT;
+            }
+        }
+    }
+    return toi;
+}
+
 __global__ void __launch_bounds__(256) k_toi(DeviceWorld w) {
     const uint32_t n = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -80,6 +139,17 @@ __global__ void __launch_bounds__(256) k_toi(DeviceWorld w) {
         if (!((fa | fb) & BF_CCD_MODE)) continue;  // no passive or continuous body reads it
         const bool ma = moves(w, h.a, fa), mb = moves(w, h.b, fb);
         float toi;
+        if (fb & BF_MESH) {
+            // A mesh pair (the convex is A): requested by an active continuous convex.
+            toi = w.pairs_cur[k].toi;
+            if ((fa & BF_CONTINUOUS) && ma) {
+                toi = mesh_toi(w, h.a, h.b, toi);
+                w.pairs_cur[k].toi = toi;
+            }
+            const uint32_t bits = __float_as_uint(toi);
+            if ((fa & BF_CCD_MODE) && ma && bits < kOneBits) atomicMin(&w.toi[h.a], bits);
+            continue;
+        }
         // Requested by the pair's continuous bodies (EntityBase.cs:1071-1082)
         // whenever one of the two is active; otherwise the pair keeps the
         // time of impact it had.

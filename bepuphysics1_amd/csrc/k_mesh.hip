@@ -1,0 +1,478 @@
+// k_mesh.hip -- narrowphase of static mesh pairs: StaticMeshContactManifold
+// over TriangleMeshConvexContactManifold.Update
+// (CollisionTests/Manifolds/TriangleMeshConvexContactManifold.cs:101-434,
+// StaticMeshContactManifold.cs:33-71) for the pairs k_pair_prep listed.
+//
+// One thread per pair: the contact refresh, the depth-first walk of the
+// mesh's MeshBoundingBoxTree over the convex's box (child A's subtree before
+// child B's, MeshBoundingBoxTree.cs:112-122, 214-226), each overlapped
+// triangle carried into the convex's local space, pruned by the convex's
+// local box and tested by its TriangleConvexPairTester (kept per triangle in
+// the pair's tester table) or the TriangleSpherePairTester, the improved
+// boundary handling, the removal of stale testers and the contact reduction.
+// The manifold's own transform is the identity: the mesh's triangles reach
+// world space through its shape's affine transform.
+#include "body_shape.h"
+#include "manifold.h"
+#include "mesh.h"
+
+namespace bepu {
+
+namespace {
+
+const cvx::RT kMeshTransform{f3{0, 0, 0}, q4{0, 0, 0, 1}};
+// CollisionDetectionSettings.nonconvexNormalDotMinimum (CollisionDetectionSettings.cs:52)
+constexpr float kNonconvexNormalDotMinimum = .99f;
+
+// An edge or vertex contact queued by AnalyzeCandidate
+// (TriangleMeshConvexContactManifold.cs:773-787): the contact in the
+// convex's local space, its corrected normal, the edge's (or vertex's) mesh
+// vertex indices and whether the tester allows the correction.
+struct Boundary {
+    MeshContact c;
+    f3 corrected;
+    int e0, e1;
+    bool should_correct;
+};
+
+// The per-step scratch of one pair's update.
+struct MeshScratch {
+    MeshContact cand[kMeshCandidates];
+    Boundary edges[kMeshCandidates], verts[kMeshCandidates];
+    int bverts[kMeshBlocked];
+    int2 bedges[kMeshBlocked];
+    int ncand, nedges, nverts, nbv, nbe;
+    bool overflow;
+};
+
+__device__ __forceinline__ bool has_vertex(const MeshScratch& m, int v) {
+    for (int k = 0; k < m.nbv; ++k)
+        if (m.bverts[k] == v) return true;
+    return false;
+}
+__device__ __forceinline__ bool has_edge(const MeshScratch& m, int a, int b) {
+    for (int k = 0; k < m.nbe; ++k)
+        if ((m.bedges[k].x == a && m.bedges[k].y == b) || (m.bedges[k].x == b && m.bedges[k].y == a)) return true;
+    return false;
+}
+__device__ __forceinline__ void block_vertex(MeshScratch& m, int v) {
+    if (has_vertex(m, v)) return;
+    if (m.nbv < kMeshBlocked) m.bverts[m.nbv++] = v;
+    else m.overflow = true;
+}
+__device__ __forceinline__ void block_edge(MeshScratch& m, int a, int b) {
+    if (has_edge(m, a, b)) return;
+    if (m.nbe < kMeshBlocked) m.bedges[m.nbe++] = make_int2(a, b);
+    else m.overflow = true;
+}
+
+// GetNormal (TriangleMeshConvexContactManifold.cs:450-481)
+__device__ __forceinline__ f3 corrected_normal(f3 uncorrected, const cvx::TriShape& t, int sided) {
+    f3 ab = t.v1 - t.v0, ac = t.v2 - t.v0;
+    f3 n;
+    if (sided == SIDED_DOUBLE) {
+        n = cross3(ab, ac);
+        if (dot3(n, uncorrected) < 0) n = -n;
+    } else if (sided == SIDED_CLOCKWISE) {
+        n = cross3(ac, ab);
+    } else {
+        n = cross3(ab, ac);
+    }
+    if (lensq3(n) < 1e-7f) n = uncorrected;
+    return n;
+}
+
+// AnalyzeCandidate (TriangleMeshConvexContactManifold.cs:483-596): true for a
+// face contact, which is always kept; an edge or vertex contact is queued and
+// every other region of the triangle blocked.
+__device__ bool analyze(int region, const int ix[3], const cvx::TriShape& t, int sided, bool should_correct,
+                        const MeshContact& c, MeshScratch& m) {
+    const int A = ix[0], B = ix[1], C = ix[2];
+    if (region <= VR_C) {
+        Boundary b{c, corrected_normal(c.normal, t, sided), region == VR_A ? A : (region == VR_B ? B : C), -1,
+                   should_correct};
+        if (m.nverts < kMeshCandidates) m.verts[m.nverts++] = b;
+        else m.overflow = true;
+        block_edge(m, A, B);
+        block_edge(m, B, C);
+        block_edge(m, A, C);
+        if (region != VR_A) block_vertex(m, A);
+        if (region != VR_B) block_vertex(m, B);
+        if (region != VR_C) block_vertex(m, C);
+        return false;
+    }
+    if (region <= VR_BC) {
+        Boundary b{c, corrected_normal(c.normal, t, sided), region == VR_BC ? B : A, region == VR_AB ? B : C,
+                   should_correct};
+        if (m.nedges < kMeshCandidates) m.edges[m.nedges++] = b;
+        else m.overflow = true;
+        if (region != VR_AB) block_edge(m, A, B);
+        if (region != VR_BC) block_edge(m, B, C);
+        if (region != VR_AC) block_edge(m, A, C);
+        block_vertex(m, A);
+        block_vertex(m, B);
+        block_vertex(m, C);
+        return false;
+    }
+    block_edge(m, A, B);
+    block_edge(m, B, C);
+    block_edge(m, A, C);
+    block_vertex(m, A);
+    block_vertex(m, B);
+    block_vertex(m, C);
+    return true;
+}
+
+// StaticMeshContactManifold's Add (TriangleMeshConvexContactManifold.cs:598-608):
+// the contact and its supplement data (local offsets on the convex and the
+// mesh, base depth).
+__device__ __forceinline__ void add_mesh_contact(PairState& s, ConvexState& e, const MeshContact& c, const cvx::RT& ta) {
+    int slot = 0;
+    while (s.flags & (16u << slot)) ++slot;
+    e.offa[slot] = f4_of(cvx::rt_transform_by_inverse(c.pos, ta), c.depth);
+    e.offb[slot] = f4_of(cvx::rt_transform_by_inverse(c.pos, kMeshTransform), 0.0f);
+    add_contact_inplace(s, c.pos, c.normal, c.depth, c.id);
+}
+
+// AddLocalContact and IsContactUnique (TriangleMeshConvexContactManifold.cs:
+// 436-447, 617-674): the candidate goes to world space; a contact of the
+// manifold at the same place with a nearly parallel normal takes its data
+// instead, a candidate already queued there drops it.
+__device__ void add_local(const DeviceWorld& w, PairState& s, ConvexState& e, MeshScratch& m, MeshContact c,
+                          const m3& orientation, const cvx::RT& ta) {
+    c.pos = mat_transform(c.pos, orientation) + ta.pos;
+    c.normal = mat_transform(c.normal, orientation);
+    for (int i = 0; i < s.ccount; i++) {
+        const int slot = s.clist[i];
+        if (distsq3(f3_of(s.cpos[slot]), c.pos) < w.s.contact_min_separation_sq) {
+            float d = dot3(f3_of(s.cnrm[slot]), c.normal);
+            if (fabsf(d) >= kNonconvexNormalDotMinimum) {
+                s.cnrm[slot] = f4_of(c.normal, s.cnrm[slot].w);
+                s.cpos[slot] = f4_of(c.pos, c.depth);
+                e.offa[slot] = f4_of(cvx::rt_transform_by_inverse(c.pos, ta), c.depth);
+                e.offb[slot] = f4_of(cvx::rt_transform_by_inverse(c.pos, kMeshTransform), 0.0f);
+                return;
+            }
+        }
+    }
+    for (int i = 0; i < m.ncand; i++) {
+        if (distsq3(m.cand[i].pos, c.pos) < w.s.contact_min_separation_sq) {
+            float d = dot3(m.cand[i].normal, c.normal);
+            if (fabsf(d) >= kNonconvexNormalDotMinimum) return;
+        }
+    }
+    if (m.ncand < kMeshCandidates) m.cand[m.ncand++] = c;
+    else m.overflow = true;
+}
+
+// ContactReducer.ReduceContacts, general form (CollisionTests/ContactReducer.cs:19-195):
+// the manifold's contacts (list order) and the candidates; the list indices
+// to remove (ascending, as a bit mask) and the candidates kept (bit mask).
+__device__ void reduce_general(const PairState& s, const MeshScratch& m, uint32_t* remove, uint32_t* keep) {
+    const int nc = s.ccount;
+    *remove = 0;
+    *keep = 0;
+    float maxDepth = -FLT_MAX;
+    int deepest = -1;
+    f3 normal{0, 0, 0};
+    for (int i = 0; i < nc; i++) {
+        const int slot = s.clist[i];
+        normal = normal + f3_of(s.cnrm[slot]);
+        if (s.cpos[slot].w > maxDepth) {
+            deepest = i;
+            maxDepth = s.cpos[slot].w;
+        }
+    }
+    for (int i = 0; i < m.ncand; i++) {
+        normal = normal + m.cand[i].normal;
+        if (m.cand[i].depth > maxDepth) {
+            deepest = nc + i;
+            maxDepth = m.cand[i].depth;
+        }
+    }
+    if (lensq3(normal) < 1e-7f) {
+        if (nc > 0) normal = f3_of(s.cnrm[s.clist[0]]);
+        else normal = m.cand[0].normal;
+    }
+    auto position = [&](int k) { return k < nc ? f3_of(s.cpos[s.clist[k]]) : m.cand[k - nc].pos; };
+    const f3 deepPos = position(deepest);
+    float furthestDist = 0;
+    int furthest = -1;
+    for (int k = 0; k < nc + m.ncand; k++) {
+        float d = distsq3(position(k), deepPos);
+        if (d > furthestDist) {
+            furthestDist = d;
+            furthest = k;
+        }
+    }
+    if (furthest == -1) {
+        if (nc > 0) {
+            for (int i = 1; i < nc; i++) *remove |= 1u << i;
+            return;
+        }
+        *keep = 1u;
+        return;
+    }
+    const f3 xAxis = deepPos - position(furthest);
+    const f3 yAxis = cross3(xAxis, normal);
+    float minY = FLT_MAX, maxY = -FLT_MAX;
+    int iminY = -1, imaxY = -1;
+    for (int k = 0; k < nc + m.ncand; k++) {
+        float d = dot3(position(k), yAxis);
+        if (d < minY) {
+            iminY = k;
+            minY = d;
+        }
+        if (d > maxY) {
+            imaxY = k;
+            maxY = d;
+        }
+    }
+    for (int k = 0; k < nc + m.ncand; k++) {
+        const bool extreme = k == deepest || k == furthest || k == iminY || k == imaxY;
+        if (k < nc && !extreme) *remove |= 1u << k;
+        if (k >= nc && extreme) *keep |= 1u << (k - nc);
+    }
+}
+
+__device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, MeshTester* tab) {
+    const uint32_t A = s.a, M = s.b;
+    const float4 PA = w.pos[A], QA = w.quat[A];
+    const cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
+    // ContactRefresher.ContactRefresh (ContactRefresher.cs:18-69)
+    int rem[4], nrem = 0;
+    for (int k = 0; k < s.ccount; k++) {
+        int slot = s.clist[k];
+        f3 na = cvx::rt_transform(f3_of(e.offa[slot]), ta);
+        f3 nb = cvx::rt_transform(f3_of(e.offb[slot]), kMeshTransform);
+        f3 ab = nb - na;
+        f3 n = f3_of(s.cnrm[slot]);
+        float d = dot3(ab, n);
+        f3 t = ab - n * d;
+        d = lensq3(t);
+        if (d > w.s.contact_invalidation_length_sq) {
+            rem[nrem++] = k;
+        } else {
+            d = dot3(ab, n);
+            float depth = e.offa[slot].w - d;
+            s.cpos[slot].w = depth;
+            if (depth < -w.s.max_contact_distance) rem[nrem++] = k;
+            else s.cpos[slot] = f4_of((nb + na) * .5f, depth);
+        }
+    }
+    for (int i = nrem - 1; i >= 0; i--) remove_contact_inplace(s, rem[i]);
+
+    const float4 msh = w.shape[M];
+    const MeshInfo mi = mesh_info(w, __float_as_uint(msh.x));
+    const int sided = (int)msh.y;
+    const bool improve = msh.z != 0.0f;
+    // RigidTransform.Invert (RigidTransform.cs:95-100), AffineTransform.CreateFromRigidTransform
+    // (AffineTransform.cs:232-236), AffineTransform.Multiply (AffineTransform.cs:186-194).
+    const q4 iq = qconj(ta.q);
+    const f3 ip = -qrotate(ta.pos, iq);
+    const m3 im = mat_from_quat(iq);
+    const m3 lm = mat_mul(mi.m, im);
+    const f3 lt = mat_transform(mi.t, im) + ip;
+    f3 lmin, lmax;
+    shape_aabb(w, w.shape[A], f3{0, 0, 0}, q4{0, 0, 0, 1}, &lmin, &lmax);
+    const m3 orientation = mat_from_quat(ta.q);
+    const float4 sha = w.shape[A];
+    TriConvex cv;
+    cv.shape = shape_of(w, A, w.hull_verts + __float_as_uint(sha.y), 1);
+    cv.min_radius = w.minrad[A];
+    cv.core_scaling = w.s.core_shape_scaling;
+    const bool sphere = s.kind == BEPU_PAIR_MESH_SPHERE;
+    int ntest = sphere ? 0 : s.tester;
+    uint32_t updated = 0;
+    int guaranteed = 0;
+    MeshScratch m;
+    m.ncand = m.nedges = m.nverts = m.nbv = m.nbe = 0;
+    m.overflow = false;
+
+    // MeshBoundingBoxTree.GetOverlaps over the convex's box, depth first.
+    const f3 qmin = f3_of(w.bmin[A]), qmax = f3_of(w.bmax[A]);
+    const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
+    auto overlaps = [&](int node) {
+        const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
+        if (qmin.x > hi.x || qmin.y > hi.y || qmin.z > hi.z) return false;
+        if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
+        return true;
+    };
+    int stack[kMeshStack];
+    int top = 0;
+    if (mi.ntri > 0 && overlaps(0)) stack[top++] = 0;
+    while (top > 0) {
+        const int node = stack[--top];
+        const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
+        if (a >= 0) {
+            if (overlaps(b)) {
+                if (top < kMeshStack) stack[top++] = b;
+                else m.overflow = true;
+            }
+            if (overlaps(a)) {
+                if (top < kMeshStack) stack[top++] = a;
+                else m.overflow = true;
+            }
+            continue;
+        }
+        // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66): leaf b
+        // is the triangle's first index position.
+        const int ix[3] = {w.mesh_idx[mi.ibase + b], w.mesh_idx[mi.ibase + b + 1], w.mesh_idx[mi.ibase + b + 2]};
+        cvx::TriShape t;
+        t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), lm, lt);
+        t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), lm, lt);
+        t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), lm, lt);
+        t.margin = 0;
+        f3 tmin, tmax;
+        tri_axis(t.v0.x, t.v1.x, t.v2.x, &tmin.x, &tmax.x);
+        tri_axis(t.v0.y, t.v1.y, t.v2.y, &tmin.y, &tmax.y);
+        tri_axis(t.v0.z, t.v1.z, t.v2.z, &tmin.z, &tmax.z);
+        if (tmin.x > lmax.x || tmin.y > lmax.y || tmin.z > lmax.z) continue;
+        if (lmin.x > tmax.x || lmin.y > tmax.y || lmin.z > tmax.z) continue;
+        MeshContact out[2];
+        int n = 0, region_sphere = VR_ABC;
+        bool should_correct = false;
+        if (sphere) {
+            n = triangle_sphere(cv.shape.margin, t, sided, &out[0], &region_sphere);
+        } else {
+            int k = 0;
+            while (k < ntest && !(tab[k].ia == ix[0] && tab[k].ib == ix[1] && tab[k].ic == ix[2])) ++k;
+            if (k == ntest) {
+                if (ntest == kMeshTesters) {
+                    m.overflow = true;
+                    continue;
+                }
+                tab[k] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
+                ++ntest;
+            }
+            updated |= 1u << k;
+            MeshTester st = tab[k];
+            n = triangle_convex(cv, t, sided, st, out);
+            tab[k] = st;
+            should_correct = st.state == TT_DEEP;
+        }
+        for (int j = 0; j < n; j++) {
+            if (improve) {
+                const int region = sphere ? region_sphere : triangle_convex_region(t, out[j].normal);
+                if (analyze(region, ix, t, sided, should_correct, out[j], m)) {
+                    guaranteed++;
+                    add_local(w, s, e, m, out[j], orientation, ta);
+                }
+            } else {
+                add_local(w, s, e, m, out[j], orientation, ta);
+            }
+        }
+    }
+
+    if (improve) {
+        // Edge contacts fighting each other in one plane get the triangle
+        // normal (TriangleMeshConvexContactManifold.cs:241-316).
+        if (m.nverts == 0 && guaranteed == 0 && m.nedges > 1) {
+            bool same_plane = true, against = false;
+            const f3 first = m.edges[0].c.normal;
+            m.edges[0].corrected = normalize3(m.edges[0].corrected);
+            float d = dot3(first, m.edges[0].corrected);
+            if (fabsf(d) > .01f) {
+                same_plane = false;
+            } else {
+                for (int i = 1; i < m.nedges; i++) {
+                    d = dot3(m.edges[i].c.normal, first);
+                    if (d < 0) against = true;
+                    d = dot3(m.edges[i].c.normal, m.edges[0].corrected);
+                    if (fabsf(d) > .01f) {
+                        same_plane = false;
+                        break;
+                    }
+                }
+            }
+            if (same_plane && against) {
+                m.edges[0].c.normal = m.edges[0].corrected;
+                m.edges[0].should_correct = true;
+                for (int i = 1; i < m.nedges; i++) {
+                    m.edges[i].corrected = normalize3(m.edges[i].corrected);
+                    d = dot3(m.edges[i].corrected, m.edges[i].c.normal);
+                    if ((double)d < .01) {
+                        m.edges[i].c.normal = m.edges[i].corrected;
+                        m.edges[i].should_correct = true;
+                    }
+                }
+            }
+        }
+        // Unblocked boundary contacts as they are; blocked ones with their
+        // normal corrected when allowed (TriangleMeshConvexContactManifold.cs:322-376).
+        for (int pass = 0; pass < 2; ++pass) {
+            const int nb = pass == 0 ? m.nedges : m.nverts;
+            for (int i = 0; i < nb; ++i) {
+                Boundary& bc = pass == 0 ? m.edges[i] : m.verts[i];
+                const bool blocked = pass == 0 ? has_edge(m, bc.e0, bc.e1) : has_vertex(m, bc.e0);
+                if (!blocked) {
+                    add_local(w, s, e, m, bc.c, orientation, ta);
+                } else if (bc.should_correct || guaranteed == 0) {
+                    bc.corrected = normalize3(bc.corrected);
+                    float d = dot3(bc.corrected, bc.c.normal);
+                    bc.c.normal = bc.corrected;
+                    bc.c.depth *= maxf_ref(0, d);
+                    add_local(w, s, e, m, bc.c, orientation, ta);
+                }
+            }
+        }
+    }
+
+    // Stale pair testers leave the table, the last entry taking a removed
+    // one's place (TriangleMeshConvexContactManifold.cs:387-401).
+    for (int i = ntest - 1; i >= 0; --i) {
+        if (!(updated & (1u << i))) {
+            tab[i] = tab[ntest - 1];
+            if (updated & (1u << (ntest - 1))) updated |= 1u << i;
+            --ntest;
+        }
+    }
+    if (!sphere) s.tester = ntest;
+
+    if (s.ccount + m.ncand > 4) {
+        uint32_t remove, keep;
+        reduce_general(s, m, &remove, &keep);
+        for (int i = 3; i >= 0; i--)
+            if (remove & (1u << i)) remove_contact_inplace(s, i);
+        for (int i = m.ncand - 1; i >= 0; i--)
+            if (keep & (1u << i)) add_mesh_contact(s, e, m.cand[i], ta);
+    } else {
+        for (int i = 0; i < m.ncand; i++) add_mesh_contact(s, e, m.cand[i], ta);
+    }
+    if (m.overflow) w.counters[ctr(CTR_OVERFLOW)] = 5;
+}
+
+constexpr int kMeshBlock = 64;
+
+__global__ void __launch_bounds__(kMeshBlock) k_mesh(DeviceWorld w) {
+    const uint32_t n = w.counters[ctr(CTR_MESH)];
+    for (uint32_t k = blockIdx.x * kMeshBlock + threadIdx.x; k < n; k += gridDim.x * kMeshBlock) {
+        const uint2 wk = w.mesh_work[k];
+        PairState& s = w.pairs_cur[wk.x];
+        ConvexState& e = w.conv_cur[wk.x];
+        // k_pair_prep carried last step's pair state, supplement data and
+        // tester table over; a new pair starts from zero.
+        if (wk.y == ~0u) {
+            const float4 z = make_float4(0, 0, 0, 0);
+            e.sep = z;
+            e.dir = z;
+            e.offa[0] = z; e.offa[1] = z; e.offa[2] = z; e.offa[3] = z;
+            e.offb[0] = z; e.offb[1] = z; e.offb[2] = z; e.offb[3] = z;
+        }
+        update_mesh(w, s, e, w.mtest_cur + (size_t)wk.x * kMeshTesters);
+        w.hot_cur[wk.x] = hot_of(s);
+        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[wk.x], wk.x))
+            w.counters[ctr(CTR_OVERFLOW)] = 3;
+        // The mesh is not dynamic: a mesh pair joins no islands.
+    }
+}
+
+}  // namespace
+
+void launch_mesh(const DeviceWorld& w, hipStream_t st) {
+    if (!w.meshes || w.pair_cap == 0) return;
+    const uint32_t grid = std::min<uint32_t>(4096, (w.pair_cap + kMeshBlock - 1) / kMeshBlock);
+    hipLaunchKernelGGL(k_mesh, dim3(grid), dim3(kMeshBlock), 0, st, w);
+}
+
+}  // namespace bepu

@@ -13,6 +13,7 @@
 // Manifolds: BoxContactManifold.Update (BoxContactManifold.cs:125-206),
 // SphereContactManifold / BoxSphereContactManifold.Update (…Manifold.cs:55-81).
 #include "manifold.h"
+#include "mesh.h"
 #include "wave.h"
 
 namespace bepu {
@@ -602,7 +603,14 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
     s.toi = 1.0f;
     s.a = i;
     s.b = j;
-    if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) s.kind = BEPU_PAIR_BOX_BOX;
+    if (si == BEPU_SHAPE_STATIC_MESH || sj == BEPU_SHAPE_STATIC_MESH) {
+        // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79): A is the convex.
+        if (si == BEPU_SHAPE_STATIC_MESH) {
+            s.a = j;
+            s.b = i;
+        }
+        s.kind = (int)w.shape[s.a].w == BEPU_SHAPE_SPHERE ? BEPU_PAIR_MESH_SPHERE : BEPU_PAIR_MESH_CONVEX;
+    } else if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_BOX) s.kind = BEPU_PAIR_BOX_BOX;
     else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_SPHERE) s.kind = BEPU_PAIR_SPHERE_SPHERE;
     else if (si == BEPU_SHAPE_BOX && sj == BEPU_SHAPE_SPHERE) s.kind = BEPU_PAIR_BOX_SPHERE;
     else if (si == BEPU_SHAPE_SPHERE && sj == BEPU_SHAPE_BOX) { s.kind = BEPU_PAIR_BOX_SPHERE; s.a = j; s.b = i; }
@@ -666,25 +674,27 @@ __device__ __forceinline__ int begin_pair(const DeviceWorld& w, uint64_t key, ui
 // two hulls fill a state's region from its front, the others from its back.
 // A closed-form pair is listed by kind (box-box from the front of np_work, the
 // others from its back), so that a wavefront of k_narrowphase runs one
-// contact routine.  Appends are gathered per workgroup: one global atomic per
+// contact routine.  A pair with a static mesh goes to k_mesh's list with its
+// tester table carried over (frozen mesh pairs keep theirs too).  Appends are gathered per workgroup: one global atomic per
 // workgroup and list (the list counters are shared by the whole device, so
 // per-wavefront atomics would queue on them).
 constexpr int kPrepBlock = 512;
 __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
-    __shared__ uint32_t s_cnt[8], s_base[8];
-    if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
+    __shared__ uint32_t s_cnt[9], s_base[9];
+    if (threadIdx.x < 9) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t npairs = min(w.counters[ctr(CTR_PAIRS)], w.pair_cap);
     // List: 2 * state + (two hulls ? 0 : 1) for general convex pairs, 6 for
-    // box-box, 7 for the other closed-form pairs.
+    // box-box, 7 for the other closed-form pairs, 8 for mesh pairs.
     int my_q = -1;
     uint32_t old_slot = ~0u;
-    bool frozen = false;
+    bool frozen = false, mesh = false;
     uint64_t key = 0;
     if (t < npairs) {
         key = w.pair_keys[t];
         const uint32_t fi = w.bflags[(uint32_t)(key >> 32)], fj = w.bflags[(uint32_t)key];
+        mesh = ((fi | fj) & BF_MESH) != 0;
         if (body_inactive(fi) && body_inactive(fj)) {
             // The broad phase reports no overlap of two inactive entries
             // (BroadPhase.cs:97-102): last step's pair is kept as it was (not
@@ -693,6 +703,11 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
             const int old = prior_pair(w, key);
             if (old >= 0) old_slot = (uint32_t)old;
             else void_pair(w, key, t);
+        } else if (mesh) {
+            int unused;
+            const int old = begin_pair(w, key, t, unused);
+            if (old >= 0) old_slot = (uint32_t)old;
+            my_q = 8;
         } else if ((fi | fj) & BF_GENERAL) {
             int cstate;
             int old = begin_pair(w, key, t, cstate);
@@ -755,6 +770,14 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
                     reinterpret_cast<const float4*>(&w.conv_old[src])[cchunk];
         }
     }
+    if (mesh && old_slot != ~0u) {
+        // A mesh pair's tester table (its live entries; the count travels in
+        // the pair state).
+        const int nt = w.pairs_old[old_slot].tester;
+        const MeshTester* src = w.mtest_old + (size_t)old_slot * kMeshTesters;
+        MeshTester* dst = w.mtest_cur + (size_t)t * kMeshTesters;
+        for (int k = 0; k < nt; ++k) dst[k] = src[k];
+    }
     if (frozen && old_slot != ~0u) {
         // A kept pair enters this step's table and, with contacts, its island.
         if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, key, t)) w.counters[ctr(CTR_OVERFLOW)] = 3;
@@ -764,7 +787,7 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
         w.hot_cur[t] = o;
     }
     uint32_t my_off = 0;
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < 9; ++q) {
         const uint64_t mine = __ballot(my_q == q);
         if (!mine) continue;
         const int leader = __ffsll((unsigned long long)mine) - 1;
@@ -774,16 +797,19 @@ __global__ void __launch_bounds__(kPrepBlock) k_pair_prep(DeviceWorld w) {
         if (my_q == q) my_off = base + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull));
     }
     __syncthreads();
-    if (threadIdx.x < 8 && s_cnt[threadIdx.x]) {
+    if (threadIdx.x < 9 && s_cnt[threadIdx.x]) {
         const uint32_t q = threadIdx.x;
         uint32_t c;
-        if (q == 6) c = CTR_NP_BOX;
+        if (q == 8) c = CTR_MESH;
+        else if (q == 6) c = CTR_NP_BOX;
         else if (q == 7) c = CTR_NP_SINGLE;
         else c = convex_list_counter(q);
         s_base[q] = atomicAdd(&w.counters[ctr(c)], s_cnt[q]);
     }
     __syncthreads();
-    if (my_q >= 0) {
+    if (my_q == 8) {
+        w.mesh_work[s_base[8] + my_off] = make_uint2(t, old_slot);
+    } else if (my_q >= 0) {
         const uint32_t k = s_base[my_q] + my_off;
         const uint32_t at = (my_q & 1) == 0 ? k : w.pair_cap - 1 - k;
         if (my_q < 6) w.conv_work[(size_t)(my_q >> 1) * w.pair_cap + at] = make_uint2(t, old_slot);

@@ -18,6 +18,8 @@
 #include "../../include/bepu_gpu.h"
 #include "bodies_host.h"
 #include "hull.h"
+#include "mesh.h"
+#include "mesh_tree.h"
 #include "solver_rows.h"
 #include "world_host.h"
 
@@ -171,6 +173,9 @@ bepu_status bepu_world_destroy(bepu_world w) {
         if (rb.done) (void)hipEventDestroy(rb.done);
     }
     if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
+    for (void* p : {(void*)w->d.mesh_info, (void*)w->d.mesh_verts, (void*)w->d.mesh_idx, (void*)w->d.mesh_nodes,
+                    (void*)w->d.mtest_cur, (void*)w->d.mtest_old, (void*)w->d.mesh_work})
+        if (p) (void)hipFree(p);
     for (auto& g : w->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (auto& e : w->ev)
@@ -382,6 +387,8 @@ static bepu_status make_body(bepu_world w, const bepu_motion_state* s, float4 sh
     return BEPU_OK;
 }
 
+static bepu_status stage_body(bepu_world w, HostBody& b, uint32_t* id);
+
 static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 shape, const m3& vd, float mass,
                             const bepu_material* m, uint32_t flags, uint32_t* id, float hull_minimum_radius = 0) {
     if (!w) return BEPU_E_INVALID_ARG;
@@ -390,6 +397,12 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 sha
     HostBody b;
     bepu_status st = make_body(w, s, shape, vd, mass, m, flags, b, hull_minimum_radius);
     if (st != BEPU_OK) return st;
+    return stage_body(w, b, id);
+}
+
+// Appends a made body to the world the add calls append to.
+static bepu_status stage_body(bepu_world w, HostBody& b, uint32_t* id) {
+    uint32_t n = w->uploaded + (uint32_t)w->staged.size();
     b.wid = w->cur_world;
     b.wbase = w->world_first[w->cur_world];
     w->staged.push_back(b);
@@ -398,10 +411,12 @@ static bepu_status add_body(bepu_world w, const bepu_motion_state* s, float4 sha
     if (w->absent.size() < (size_t)n + 1) {
         w->absent.resize((size_t)n + 1, 0);
         w->body_dynamic.resize((size_t)n + 1, 0);
+        w->body_mesh.resize((size_t)n + 1, 0);
         w->body_world.resize((size_t)n + 1, 0);
     }
     w->absent[n] = 0;
     w->body_dynamic[n] = (b.flags & BF_DYNAMIC) ? 1 : 0;
+    w->body_mesh[n] = (b.flags & BF_MESH) ? 1 : 0;
     w->body_world[n] = b.wid;
     if (id) *id = n;
     return BEPU_OK;
@@ -616,6 +631,114 @@ bepu_status bepu_add_convex_hull(bepu_world w, const bepu_motion_state* s, uint3
                     flags, id, hs.minimum_radius);
 }
 
+// Replaces the device copy of a host array with one holding all of `host`.
+extern "C++" template <typename T>
+static bepu_status mirror(const std::vector<T>& host, T** dev) {
+    T* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(host.size(), 1) * sizeof(T)) != hipSuccess) return BEPU_E_OUT_OF_MEMORY;
+    if (!host.empty() && hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return BEPU_E_DEVICE;
+    }
+    if (*dev) (void)hipFree(*dev);
+    *dev = p;
+    return BEPU_OK;
+}
+
+static float bits_float(uint32_t v) {
+    float f;
+    std::memcpy(&f, &v, sizeof(f));
+    return f;
+}
+
+bepu_status bepu_create_mesh_shape(bepu_world w, const float* vertices, uint32_t nvertices, const int32_t* indices,
+                                   uint32_t nindices, const float* transform, uint32_t* shape_id) {
+    if (!w || !vertices || !indices || nvertices == 0 || nindices == 0 || nindices % 3 != 0 || nvertices > (1u << 28) ||
+        nindices > (1u << 28))
+        return BEPU_E_INVALID_ARG;
+    if (!finite_n(vertices, 3 * (int)nvertices) || (transform && !finite_n(transform, 12))) return BEPU_E_INVALID_ARG;
+    for (uint32_t i = 0; i < nindices; ++i)
+        if (indices[i] < 0 || (uint32_t)indices[i] >= nvertices) return BEPU_E_INVALID_ARG;
+    m3 lin{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    f3 tr{0, 0, 0};
+    if (transform) {
+        lin = m3{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5], transform[6],
+                 transform[7], transform[8]};
+        tr = f3{transform[9], transform[10], transform[11]};
+    }
+    std::vector<f3> verts(nvertices);
+    for (uint32_t i = 0; i < nvertices; ++i) verts[i] = f3{vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+    std::vector<MeshTreeNode> nodes;
+    uint32_t depth = 0;
+    build_mesh_tree(verts.data(), indices, nindices, lin, tr, &nodes, &depth);
+    // The device walk keeps one pending node per level.
+    if (depth > (uint32_t)kMeshStack) return BEPU_E_CAPACITY;
+    // The first mesh shape brings the mesh pairs' tables (per pair slot,
+    // double-buffered) and their work list.
+    DeviceWorld& d = w->d;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    if (!d.meshes) {
+        const size_t P = std::max<size_t>(d.pair_cap, 1);
+        if (hipMalloc(&d.mtest_cur, P * kMeshTesters * sizeof(MeshTester)) != hipSuccess ||
+            hipMalloc(&d.mtest_old, P * kMeshTesters * sizeof(MeshTester)) != hipSuccess ||
+            hipMalloc(&d.mesh_work, P * sizeof(uint2)) != hipSuccess)
+            return BEPU_E_OUT_OF_MEMORY;
+        d.meshes = 1;
+    }
+    const uint32_t vbase = (uint32_t)w->mesh_verts_host.size(), ibase = (uint32_t)w->mesh_idx_host.size(),
+                   nbase = (uint32_t)(w->mesh_nodes_host.size() / 2);
+    for (uint32_t i = 0; i < nvertices; ++i)
+        w->mesh_verts_host.push_back(make_float4(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2], 0));
+    w->mesh_idx_host.insert(w->mesh_idx_host.end(), indices, indices + nindices);
+    for (const MeshTreeNode& n : nodes) {
+        w->mesh_nodes_host.push_back(f4_of(n.bmin, bits_float((uint32_t)n.a)));
+        w->mesh_nodes_host.push_back(f4_of(n.bmax, bits_float((uint32_t)n.b)));
+    }
+    w->mesh_info_host.push_back(make_float4(lin.m11, lin.m12, lin.m13, bits_float(vbase)));
+    w->mesh_info_host.push_back(make_float4(lin.m21, lin.m22, lin.m23, bits_float(ibase)));
+    w->mesh_info_host.push_back(make_float4(lin.m31, lin.m32, lin.m33, bits_float(nbase)));
+    w->mesh_info_host.push_back(f4_of(tr, bits_float(nindices / 3)));
+    bepu_status st = mirror(w->mesh_verts_host, &d.mesh_verts);
+    if (st == BEPU_OK) st = mirror(w->mesh_idx_host, &d.mesh_idx);
+    if (st == BEPU_OK) st = mirror(w->mesh_nodes_host, &d.mesh_nodes);
+    if (st == BEPU_OK) st = mirror(w->mesh_info_host, &d.mesh_info);
+    if (st != BEPU_OK) return st;
+    if (shape_id) *shape_id = w->mesh_shapes;
+    w->mesh_shapes++;
+    return BEPU_OK;
+}
+
+// The body record of a StaticMesh: a static collidable in the kinematic
+// collision group (StaticCollidable.cs:21), never active (its flag above 0),
+// at the identity (its triangles carry the shape's world transform).
+static bepu_status mesh_body(bepu_world w, uint32_t shape_id, int32_t sidedness, int32_t improve,
+                             const bepu_material* material, HostBody& rec) {
+    if (shape_id >= w->mesh_shapes || sidedness < BEPU_SIDED_DOUBLE || sidedness > BEPU_SIDED_COUNTERCLOCKWISE)
+        return BEPU_E_INVALID_ARG;
+    bepu_motion_state st{};
+    st.orientation[3] = 1;
+    const float4 shape =
+        make_float4(bits_float(shape_id), (float)sidedness, improve ? 1.0f : 0.0f, (float)BEPU_SHAPE_STATIC_MESH);
+    bepu_status s = make_body(w, &st, shape, m3{0, 0, 0, 0, 0, 0, 0, 0, 0}, 0, material, 0, rec);
+    if (s != BEPU_OK) return s;
+    rec.stab.y = 1.0f;
+    rec.flags = (rec.flags & ~(uint32_t)BF_KIN_ACTIVE) | BF_MESH;
+    rec.minrad = 0;
+    w->mesh_bodies = true;
+    return BEPU_OK;
+}
+
+bepu_status bepu_add_static_mesh(bepu_world w, uint32_t shape_id, int32_t sidedness, int32_t improve_boundary,
+                                 const bepu_material* material, uint32_t* id) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    uint32_t n = w->uploaded + (uint32_t)w->staged.size();
+    if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
+    HostBody b;
+    bepu_status st = mesh_body(w, shape_id, sidedness, improve_boundary, material, b);
+    if (st != BEPU_OK) return st;
+    return stage_body(w, b, id);
+}
+
 bepu_status bepu_begin_world(bepu_world w, uint32_t* world_id) {
     if (!w) return BEPU_E_INVALID_ARG;
     if (w->world_count[w->cur_world] > 0) {
@@ -661,6 +784,12 @@ static bepu_status body_from_desc(bepu_world w, const bepu_body_desc& b, HostBod
         const auto& hs = w->hull_shapes[(uint32_t)k];
         return make_body(w, &b.state, hull_shape(w->d.s.default_margin, hs.first, hs.count), hs.volume_distribution,
                          b.mass, &b.material, b.flags, rec, hs.minimum_radius);
+    }
+    if (b.shape == BEPU_SHAPE_STATIC_MESH) {
+        float k = b.dims[0], sided = b.dims[1];
+        if (!(k >= 0) || k != std::floor(k) || k >= (float)w->mesh_shapes || sided != std::floor(sided))
+            return BEPU_E_INVALID_ARG;
+        return mesh_body(w, (uint32_t)k, (int32_t)sided, b.dims[2] != 0.0f ? 1 : 0, &b.material, rec);
     }
     return BEPU_E_UNSUPPORTED;
 }
@@ -1018,6 +1147,7 @@ static bepu_status enqueue_step(bepu_world w, hipEvent_t* e) {
     }
     if (e) (void)hipEventRecord(e[3], s);
     if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
+    if (w->mesh_bodies) launch_mesh(d, s);
     if (e) (void)hipEventRecord(e[4], s);
     // Joints join their bodies' islands; then the islands woken by this
     // step's new constraints, stale frozen pairs, split activation.
@@ -1110,7 +1240,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
     if (timed) w->timed_steps = timed;
     // Pairs of more than one tester are listed by tester first; a world of
     // boxes only (or spheres only) skips that pass.
-    d.pair_lists = (w->convex_bodies || (w->box_bodies && w->sphere_bodies)) ? 1u : 0u;
+    d.pair_lists = (w->convex_bodies || w->mesh_bodies || (w->box_bodies && w->sphere_bodies)) ? 1u : 0u;
     for (int k = 0; k < n_steps; ++k) {
         int slot = (int)timed - (n_steps - k);  // >= 0 for the timed steps
         hipEvent_t* e = slot >= 0 ? &w->ev[(size_t)slot * (ST_COUNT + 1)] : nullptr;
@@ -1120,6 +1250,7 @@ bepu_status bepu_step(bepu_world w, int32_t n_steps) {
         std::swap(d.pairs_cur, d.pairs_old);
         std::swap(d.hot_cur, d.hot_old);
         std::swap(d.conv_cur, d.conv_old);
+        std::swap(d.mtest_cur, d.mtest_old);
         std::swap(d.hash_keys_cur, d.hash_keys_old);
         std::swap(d.hash_vals_cur, d.hash_vals_old);
         w->steps++;
@@ -1359,6 +1490,7 @@ bepu_status bepu_add_ball_socket(bepu_world w, uint32_t a, uint32_t b, const flo
     const uint32_t total = w->uploaded + (uint32_t)w->staged.size();
     if (a >= total || b >= total || w->absent[a] || w->absent[b]) return BEPU_E_NOT_FOUND;
     if (!w->body_dynamic[a] && !w->body_dynamic[b]) return BEPU_E_INVALID_ARG;
+    if (w->body_mesh[a] || w->body_mesh[b]) return BEPU_E_INVALID_ARG;  // joints connect entities
     if (w->body_world[a] != w->body_world[b]) return BEPU_E_INVALID_ARG;
     if (w->d.njoints >= w->d.joint_cap) return BEPU_E_CAPACITY;
     // The joint reads the bodies' poses on the device: upload staged bodies first.
@@ -1427,6 +1559,7 @@ bepu_status bepu_set_motion_states(bepu_world w, const uint32_t* ids, const bepu
     for (uint32_t k = 0; k < n; ++k) {
         const bepu_motion_state& m = states[k];
         if (ids[k] >= total || w->absent[ids[k]]) return BEPU_E_NOT_FOUND;
+        if (w->body_mesh[ids[k]]) return BEPU_E_INVALID_ARG;  // a static mesh is not an entity
         if (((fields & BEPU_STATE_POSITION) && !finite_n(m.position, 3)) ||
             ((fields & BEPU_STATE_ORIENTATION) && !finite_n(m.orientation, 4)) ||
             ((fields & BEPU_STATE_LINEAR_VELOCITY) && !finite_n(m.linear_velocity, 3)) ||
@@ -1495,8 +1628,10 @@ static bepu_status upload_state_records(bepu_world w, std::vector<StateRecord>& 
 bepu_status bepu_activate(bepu_world w, const uint32_t* ids, uint32_t n) {
     if (!w || (n && !ids)) return BEPU_E_INVALID_ARG;
     uint32_t total = w->uploaded + (uint32_t)w->staged.size();
-    for (uint32_t k = 0; k < n; ++k)
+    for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= total || w->absent[ids[k]]) return BEPU_E_NOT_FOUND;
+        if (w->body_mesh[ids[k]]) return BEPU_E_INVALID_ARG;
+    }
     std::vector<StateRecord> recs;
     for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->uploaded) continue;  // not yet stepped: active already
@@ -1514,8 +1649,10 @@ bepu_status bepu_set_position_update_mode(bepu_world w, const uint32_t* ids, uin
     if (!w || (n && !ids) || mode < BEPU_POSITION_UPDATE_DISCRETE || mode > BEPU_POSITION_UPDATE_CONTINUOUS)
         return BEPU_E_INVALID_ARG;
     uint32_t total = w->uploaded + (uint32_t)w->staged.size();
-    for (uint32_t k = 0; k < n; ++k)
+    for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= total || w->absent[ids[k]]) return BEPU_E_NOT_FOUND;
+        if (w->body_mesh[ids[k]]) return BEPU_E_INVALID_ARG;
+    }
     const uint32_t bits = mode == BEPU_POSITION_UPDATE_CONTINUOUS
                               ? (uint32_t)BF_CONTINUOUS
                               : (mode == BEPU_POSITION_UPDATE_PASSIVE ? (uint32_t)BF_PASSIVE : 0u);

@@ -87,10 +87,11 @@ struct ConvexState {
 // deactivate at its next check (an active kinematic body touched it);
 // BF_WAKE, activated since the last step (its island is active at the next).
 // BF_PASSIVE / BF_CONTINUOUS: PositionUpdateMode Passive / Continuous (none:
-// Discrete).
+// Discrete).  BF_MESH: a static mesh (a static collidable: never dynamic,
+// never active, its box fixed).
 enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u, BF_ALWAYS_ACTIVE = 32u,
        BF_CANDIDATE = 64u, BF_ASLEEP = 128u, BF_KIN_ACTIVE = 256u, BF_NODEACT = 512u, BF_WAKE = 1024u,
-       BF_PASSIVE = 2048u, BF_CONTINUOUS = 4096u };
+       BF_PASSIVE = 2048u, BF_CONTINUOUS = 4096u, BF_MESH = 8192u };
 constexpr uint32_t BF_CCD_MODE = BF_PASSIVE | BF_CONTINUOUS;
 
 // One body as built on the host (Entity construction), copied to the device
@@ -129,6 +130,7 @@ constexpr uint64_t kJointKey = 1ull << 63;
 
 struct RowConst;
 struct RowVar;
+struct MeshTester;
 
 struct DeviceWorld {
     bepu_settings s;
@@ -167,6 +169,23 @@ struct DeviceWorld {
     float* minrad;             // per body: the shape's MinimumRadius
     uint32_t* toi;             // per body: smallest time of impact of its pairs this step (float bits)
     float4* qold;              // per passive / continuous body: its orientation before this step's update
+    // static meshes (present once a mesh shape was created): per mesh shape
+    // 4 float4 in mesh_info (the rows of its world transform's linear part
+    // with its vertex, index and node bases in .w, its translation with its
+    // triangle count in .w); local vertices (xyz), vertex indices; tree nodes
+    // as 2 float4 each, (box min, child a or -1 for a leaf) and (box max,
+    // child b or the leaf's triangle), children relative to the shape's node
+    // base.  Mesh pairs keep their tester tables per pair slot (kMeshTesters
+    // entries, double-buffered like the pair states; the count is in
+    // PairState::tester) and are listed for k_mesh as (slot, last slot).
+    uint32_t meshes;
+    float4* mesh_info;
+    float4* mesh_verts;
+    int32_t* mesh_idx;
+    float4* mesh_nodes;
+    MeshTester* mtest_cur;
+    MeshTester* mtest_old;
+    uint2* mesh_work;
     // joints
     JointState* joints;
     uint32_t njoints;
@@ -330,7 +349,8 @@ enum {
     CTR_PAIRS_PREV = 151, // the last step's pair count (copied by the step reset)
     CTR_DEAD = 152,       // frozen pairs removed as stale this step
     CTR_STEPS = 153,      // steps started (the step reset adds one; never cleared)
-    CTR_COUNT = 154
+    CTR_MESH = 154,       // mesh pairs listed for k_mesh this step
+    CTR_COUNT = 155
 };
 
 // Sticky error bits: the start of each step folds the previous step's
@@ -397,6 +417,8 @@ void launch_pair_prep(const DeviceWorld& w, hipStream_t st);
 // (when no hull has more than kConvexStageVerts vertices); 0: read from HBM;
 // 2: each step picks one of the two on the device from its pair counts.
 void launch_convex(const DeviceWorld& w, uint32_t lds_mode, hipStream_t st);
+// Mesh pairs queued by k_pair_prep (StaticMeshContactManifold.Update).
+void launch_mesh(const DeviceWorld& w, hipStream_t st);
 constexpr uint32_t kConvexStageVerts = 16;
 // Last step's constraint, contact, colour and penetration statistics into the counters.
 void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st);

@@ -76,10 +76,18 @@ struct bepu_world_s {
     bool box_bodies = false, sphere_bodies = false;  // a box / a sphere was ever added
     std::vector<float4> hull_host;
     uint32_t hull_dev_cap = 0;
+    // Static mesh shapes: their vertices, indices, tree nodes and records
+    // back to back on the host (mirrored in d.mesh_verts, d.mesh_idx,
+    // d.mesh_nodes, d.mesh_info); the device copies are reallocated as they grow.
+    uint32_t mesh_shapes = 0;
+    std::vector<float4> mesh_verts_host, mesh_nodes_host, mesh_info_host;
+    std::vector<int32_t> mesh_idx_host;
+    bool mesh_bodies = false;               // a static mesh was ever added: run k_mesh
     int device = 0;                         // the HIP device the world lives on
     uint32_t sticky = 0;                    // STICKY_* bits seen by the host since the last bepu_clear_error
     std::vector<uint8_t> absent;            // per body id: removed (bepu_remove, or its world was removed)
     std::vector<uint8_t> body_dynamic;      // per body id: dynamic (mass > 0) when it was made
+    std::vector<uint8_t> body_mesh;         // per body id: a static mesh (no state to set, never active)
     std::vector<uint32_t> body_world;       // per body id: its sub-world
     std::vector<uint32_t> world_joints;     // per world id: joints made in it (the next joint's ordinal)
     bepu::StateRecord* sstage = nullptr;    // device staging of motion states written between steps

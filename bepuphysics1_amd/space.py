@@ -30,7 +30,7 @@ import numpy as np
 from . import _lib
 from ._lib import BepuError, check
 
-SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE = 0, 1, 2, 3
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE, SHAPE_STATIC_MESH = 0, 1, 2, 3, 4
 STATE_POSITION, STATE_ORIENTATION, STATE_LINEAR_VELOCITY, STATE_ANGULAR_VELOCITY = 1, 2, 4, 8
 STATE_ALL = 15
 _STATE_FIELD = {"position": (STATE_POSITION, slice(0, 3)), "orientation": (STATE_ORIENTATION, slice(3, 7)),
@@ -225,6 +225,51 @@ class ConvexHull(Entity):
                                         ctypes.byref(self._material()), self._flags(), ctypes.byref(out))
 
 
+class StaticMesh:
+    """BroadPhaseEntries/StaticMesh.cs: a static triangle mesh (not an entity).
+    `vertices` [n, 3], `indices` three per triangle, `transform` the
+    AffineTransform (3x3 linear part as Matrix3x3 rows, translation) or None;
+    Sidedness 0 DoubleSided (default), 1 Clockwise, 2 Counterclockwise."""
+
+    def __init__(self, vertices, indices, transform=None):
+        self.Vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.Indices = np.ascontiguousarray(indices, np.int32).reshape(-1)
+        self.Transform = transform
+        self.Sidedness = 0
+        self.ImproveBoundaryBehavior = True
+        self.Material = Material()
+        self._space = None
+        self._id = None
+
+    @property
+    def Id(self):
+        return self._id
+
+    def _add(self, lib, h, out):
+        sid = ctypes.c_uint32()
+        st = lib.bepu_create_mesh_shape(h, *_mesh_args(self.Vertices, self.Indices, self.Transform), ctypes.byref(sid))
+        if st != 0:
+            return st
+        m = self.Material
+        mat = _lib.Material(m.StaticFriction, m.KineticFriction, m.Bounciness)
+        return lib.bepu_add_static_mesh(h, sid.value, int(self.Sidedness), 1 if self.ImproveBoundaryBehavior else 0,
+                                        ctypes.byref(mat), ctypes.byref(out))
+
+
+def _mesh_args(vertices, indices, transform):
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(indices, np.int32).reshape(-1)
+    fp = ctypes.POINTER(ctypes.c_float)
+    t = None
+    if transform is not None:
+        t = np.concatenate([np.asarray(transform[0], np.float32).reshape(9),
+                            np.asarray(transform[1], np.float32).reshape(3)]).astype(np.float32)
+    # the arrays must outlive the call: keep them on the function object
+    _mesh_args.keep = (v, ix, t)
+    return (v.ctypes.data_as(fp), len(v), ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(ix),
+            t.ctypes.data_as(fp) if t is not None else None)
+
+
 class _Solver:
     def __init__(self, space):
         self._space = space
@@ -275,6 +320,7 @@ class Space:
         self._pair_capacity = int(pair_capacity)
         self._h = None
         self._entities = []
+        self._mesh_ids = []
         self._n_uploaded = 0
         self._ms_cache = None
         self.Solver = _Solver(self)
@@ -547,6 +593,19 @@ class Space:
                                                center.ctypes.data_as(fp)), "create_hull_shape")
         return sid.value, center
 
+    def create_mesh_shape(self, vertices, indices, transform=None):
+        """Registers a static mesh shape (bepu_create_mesh_shape); returns its id."""
+        sid = ctypes.c_uint32()
+        check(self._lib.bepu_create_mesh_shape(self._handle(), *_mesh_args(vertices, indices, transform),
+                                               ctypes.byref(sid)), "create_mesh_shape")
+        return sid.value
+
+    def register_meshes(self, meshes):
+        """Registers mesh shapes in order ((vertices, indices, transform) each);
+        scene rows of static meshes name them by index in dims[0]."""
+        self._mesh_ids = [self.create_mesh_shape(*m) for m in meshes]
+        return self._mesh_ids
+
     def register_hulls(self, hull_points):
         """Registers hull shapes in order; scene rows of hull bodies name them
         by index in dims[0], so register a scene's hull list before its bodies."""
@@ -581,6 +640,9 @@ class Space:
             elif int(r["shape"]) == SHAPE_CONVEX_HULL:
                 st = lib.bepu_add_convex_hull(h, ctypes.byref(ms), int(d[0]), float(r["mass"]), ctypes.byref(mat),
                                               int(r["flags"]), ctypes.byref(out))
+            elif int(r["shape"]) == SHAPE_STATIC_MESH:
+                st = lib.bepu_add_static_mesh(h, self._mesh_ids[int(d[0])], int(d[1]), 1 if d[2] else 0,
+                                              ctypes.byref(mat), ctypes.byref(out))
             else:
                 raise NotImplementedError("shape %d" % int(r["shape"]))
             if st != 0:

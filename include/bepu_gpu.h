@@ -247,6 +247,28 @@ bepu_status bepu_add_convex_hull(bepu_world world, const bepu_motion_state* stat
                                  uint32_t hull_shape_id, float mass, const bepu_material* material,
                                  uint32_t flags, uint32_t* id_out);
 
+/* Static meshes.  bepu_create_mesh_shape is new StaticMeshShape(vertices,
+ * indices, worldTransform) (CollisionShapes/StaticMeshShape.cs:36-47,
+ * DataStructures/TransformableMeshData.cs): `vertices` xyz triples, `indices`
+ * three per triangle, `transform` the AffineTransform as 12 floats (its
+ * Matrix3x3 M11..M33 row by row, then the translation; a vertex v maps to
+ * v * M + t) or null for the identity.  The shape's MeshBoundingBoxTree
+ * (DataStructures/MeshBoundingBoxTree.cs:60-69) is built here; a tree deeper
+ * than 64 levels returns BEPU_E_CAPACITY.  Many static meshes may share one
+ * shape.  bepu_add_static_mesh is Space.Add(new StaticMesh(...))
+ * (BroadPhaseEntries/StaticMesh.cs): a static collidable with the given
+ * TriangleSidedness (BEPU_SIDED_*; the reference's default DoubleSided),
+ * ImproveBoundaryBehavior (default 1) and material.  It is not an entity:
+ * state setters, activation, position update modes and joints refuse it
+ * (BEPU_E_INVALID_ARG); bepu_remove removes it.  A mesh pair keeps at most 32
+ * triangle testers and meets at most 32 contact candidates per step; a step
+ * that needs more sets the sticky capacity error. */
+bepu_status bepu_create_mesh_shape(bepu_world world, const float* vertices, uint32_t nvertices,
+                                   const int32_t* indices, uint32_t nindices, const float* transform,
+                                   uint32_t* shape_id_out);
+bepu_status bepu_add_static_mesh(bepu_world world, uint32_t mesh_shape_id, int32_t sidedness,
+                                 int32_t improve_boundary, const bepu_material* material, uint32_t* id_out);
+
 /* Batched worlds.  A world handle may hold many independent sub-worlds, each
  * the analogue of one Space: bodies of different sub-worlds never form pairs,
  * and every sub-world's result is bit-identical to stepping it alone (its
@@ -263,7 +285,9 @@ bepu_status bepu_get_world_range(bepu_world world, uint32_t world_id, uint32_t* 
 typedef struct bepu_body_desc {
     int32_t shape;            /* BEPU_SHAPE_* */
     float dims[3];            /* box: width, height, length; sphere: radius, 0, 0;
-                                 capsule: length, radius, 0; hull: shape id (an integer), 0, 0 */
+                                 capsule: length, radius, 0; hull: shape id (an integer), 0, 0;
+                                 static mesh: mesh shape id, sidedness, improve boundary (0/1);
+                                 a static mesh's mass, state and flags are ignored */
     float mass;               /* <= 0: kinematic */
     bepu_motion_state state;
     bepu_material material;

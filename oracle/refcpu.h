@@ -101,10 +101,16 @@ bool convex_candidate(const ConvexShapeRef& a, const ConvexShapeRef& b, const RT
                       const V3& vb, ConvexTesterState* st, ContactData* c);
 // Casts of continuous collision detection (GJKToolbox.cs:195-495); *t is the
 // hit parameter in units of `dir`.
-bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t);
-bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t);
+// `normal` (optional) receives RayHit.Normal: the search direction of the
+// last advance, in world space (not normalised).
+bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t,
+                 V3* normal = nullptr);
+bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t,
+              V3* normal = nullptr);
 bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
                      float max_length, float scaling, float* t);
+bool ccd_sphere_cast_normal(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
+                            float max_length, float scaling, float* t, V3* normal);
 
 // ---------------------------------------------------------------------------
 // Triangles of static meshes (refcpu_convex.cpp, refcpu_mesh.cpp)

@@ -1310,7 +1310,8 @@ V3 local_extreme_with_margin(const ConvexShapeRef& s, V3 d) {
 // posed shape; *t receives the hit parameter, also where the cast fails (0
 // after too many iterations, as the reference's fresh RayHit).
 bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, const RT& xf, float max_length,
-                 float* t) {
+                 float* t, V3* normal) {
+    V3 hn{0, 0, 0};
     origin = sub(origin, xf.pos);
     Q4 cj = qconj(xf.q);
     origin = qtransform(origin, cj);
@@ -1339,17 +1340,21 @@ bool sphere_cast(V3 origin, V3 dir, float radius, const ConvexShapeRef& shape, c
                 return false;
             }
             loc = add(mul(dir, T), origin);
+            hn = v;
         }
         RSimplex shifted = rs_add(simplex, p, loc);
         v = rs_closest(shifted, simplex);
     }
     *t = T;
+    if (normal) *normal = qtransform(hn, xf.q);
     return true;
 }
 
 // GJKToolbox.RayCast (GJKToolbox.cs:195-265): the ray against the shape with
 // its margin.
-bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t) {
+bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, float max_length, float* t,
+              V3* normal) {
+    V3 hn{0, 0, 0};
     origin = sub(origin, xf.pos);
     Q4 cj = qconj(xf.q);
     origin = qtransform(origin, cj);
@@ -1380,11 +1385,13 @@ bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, floa
                 return false;
             }
             loc = add(mul(dir, T), origin);
+            hn = off;
         }
         RSimplex shifted = rs_add(simplex, e, loc);
         off = rs_closest(shifted, simplex);
     }
     *t = T;
+    if (normal) *normal = qtransform(hn, xf.q);
     return true;
 }
 
@@ -1393,10 +1400,15 @@ bool ray_cast(V3 origin, V3 dir, const ConvexShapeRef& shape, const RT& xf, floa
 // up to three times, then a plain ray cast.
 bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
                      float max_length, float scaling, float* t) {
+    V3 n;
+    return ccd_sphere_cast_normal(origin, dir, radius, shape, xf, max_length, scaling, t, &n);
+}
+bool ccd_sphere_cast_normal(const V3& origin, const V3& dir, float radius, const ConvexShapeRef& shape, const RT& xf,
+                            float max_length, float scaling, float* t, V3* normal) {
     int iterations = 0;
     for (;;) {
         float T;
-        if (sphere_cast(origin, dir, radius, shape, xf, max_length, &T) && T > 0) {
+        if (sphere_cast(origin, dir, radius, shape, xf, max_length, &T, normal) && T > 0) {
             *t = T;
             return true;
         }
@@ -1407,7 +1419,7 @@ bool ccd_sphere_cast(const V3& origin, const V3& dir, float radius, const Convex
         radius *= scaling;
         iterations++;
         if (iterations > 3) {
-            bool hit = ray_cast(origin, dir, shape, xf, max_length, &T);
+            bool hit = ray_cast(origin, dir, shape, xf, max_length, &T, normal);
             *t = T;
             return hit && T > 0;
         }

@@ -715,12 +715,18 @@ RT ccd_transform(const Body& b) {
     return RT{b.pos, (b.pmode != 0 && body_active(b)) ? b.q_world : b.q};
 }
 
+void mesh_time_of_impact(refcpu_world* w, Pair& p);
+
 // ConvexPairHandler.UpdateTimeOfImpact (ConvexPairHandler.cs:37-111), as the
 // pair's continuous bodies request it (EntityBase.cs:1071-1082: every
 // continuous body asks each of its pairs, active or not; with two continuous
 // bodies only one request computes).  A pair that no request computes keeps
 // its time of impact.
 void pair_time_of_impact(refcpu_world* w, Pair& p) {
+    if (p.kind == BEPU_PAIR_MESH_CONVEX || p.kind == BEPU_PAIR_MESH_SPHERE) {
+        mesh_time_of_impact(w, p);
+        return;
+    }
     const Body& A = w->bodies[p.a];
     const Body& B = w->bodies[p.b];
     if (A.pmode != 2 && B.pmode != 2) return;
@@ -1548,6 +1554,47 @@ void update_mesh(refcpu_world* w, Pair& p) {
         for (int i = (int)keep.size() - 1; i >= 0; i--) manifold_add_general(p, keep[i], ta, kMeshTransform);
     } else {
         for (const ContactData& c : cand) manifold_add_general(p, c, ta, kMeshTransform);
+    }
+}
+
+// StaticMeshPairHandler.UpdateTimeOfImpact (StaticMeshPairHandler.cs:95-159):
+// for an active continuous convex, its core sphere swept along its motion
+// against each triangle the manifold overlapped this step; a hit further than
+// Toolbox.BigEpsilon (on a one-sided mesh, only from the side the mesh keeps)
+// shortens the time of impact.
+void mesh_time_of_impact(refcpu_world* w, Pair& p) {
+    const Body& A = w->bodies[p.a];
+    const Body& M = w->bodies[p.b];
+    if (!(body_active(A) && A.pmode == 2)) return;
+    V3 velocity = mul(A.v, w->s.dt);
+    float vsq = length_sq(velocity);
+    const float minimumRadius = A.min_radius * w->s.core_shape_scaling;
+    p.toi = 1;
+    if (!(minimumRadius * minimumRadius < vsq)) return;
+    const MeshShape& shape = w->mesh_shapes[M.mesh];
+    std::vector<int> tris;
+    mesh_overlaps(shape, A.bmin, A.bmax, &tris);
+    const RT ta = ccd_transform(A);
+    for (int tri : tris) {
+        TriangleRef t;
+        mesh_world_triangle(shape, tri, t.v);
+        for (int k = 0; k < 3; ++k) t.v[k] = sub(t.v[k], ta.pos);
+        t.sidedness = M.sidedness;
+        t.margin = 0;
+        float hitT;
+        V3 hitN;
+        if (ccd_sphere_cast_normal(V3{0, 0, 0}, velocity, minimumRadius, triangle_shape(t), kMeshTransform, p.toi,
+                                   w->s.core_shape_scaling, &hitT, &hitN) &&
+            hitT > 1e-5f) {
+            if (M.sidedness != SIDED_DOUBLE) {
+                V3 n = cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]));
+                float d = dot(n, hitN);
+                if ((M.sidedness == SIDED_COUNTERCLOCKWISE && d < 0) || (M.sidedness == SIDED_CLOCKWISE && d > 0))
+                    p.toi = hitT;
+            } else {
+                p.toi = hitT;
+            }
+        }
     }
 }
 
@@ -3025,6 +3072,7 @@ int refcpu_add_ball_socket(refcpu_world* w, uint32_t a, uint32_t b, const float*
     const Body& A = w->bodies[a];
     const Body& B = w->bodies[b];
     if (!A.dynamic && !B.dynamic) return BEPU_E_INVALID_ARG;  // "Cannot constrain two kinematic bodies."
+    if (A.mesh >= 0 || B.mesh >= 0) return BEPU_E_INVALID_ARG;  // joints connect entities
     JointC j;
     std::memset(&j, 0, sizeof(j));
     j.a = a;
@@ -3092,8 +3140,10 @@ int refcpu_permuted_order(int64_t n, int64_t k, int64_t count, int64_t* out) {
 
 // Entity.ActivityInformation.Activate() for the listed bodies.
 int refcpu_activate(refcpu_world* w, const uint32_t* ids, uint32_t n) {
-    for (uint32_t k = 0; k < n; ++k)
+    for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
+        if (w->bodies[ids[k]].mesh >= 0) return BEPU_E_INVALID_ARG;  // a static collidable has no activity
+    }
     for (uint32_t k = 0; k < n; ++k) activate(w->bodies[ids[k]]);
     return BEPU_OK;
 }
@@ -3115,6 +3165,7 @@ int refcpu_set_motion_states(refcpu_world* w, const uint32_t* ids, const bepu_mo
     if (fields == 0 || (fields & ~BEPU_STATE_ALL)) return BEPU_E_INVALID_ARG;
     for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
+        if (w->bodies[ids[k]].mesh >= 0) return BEPU_E_INVALID_ARG;  // a static mesh is not an entity
         const bepu_motion_state& m = st[k];
         if (((fields & BEPU_STATE_POSITION) && !finite3(m.position, 3)) ||
             ((fields & BEPU_STATE_ORIENTATION) && !finite3(m.orientation, 4)) ||
@@ -3155,8 +3206,10 @@ int refcpu_set_settings(refcpu_world* w, const bepu_settings* s) {
 // Entity.PositionUpdateMode setter (EntityBase.cs:1050-1068) for each listed body.
 int refcpu_set_position_update_mode(refcpu_world* w, const uint32_t* ids, uint32_t n, int mode) {
     if ((n && !ids) || mode < 0 || mode > 2) return BEPU_E_INVALID_ARG;
-    for (uint32_t k = 0; k < n; ++k)
+    for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
+        if (w->bodies[ids[k]].mesh >= 0) return BEPU_E_INVALID_ARG;
+    }
     for (uint32_t k = 0; k < n; ++k) w->bodies[ids[k]].pmode = mode;
     if (mode && n) w->ccd = true;
     return BEPU_OK;

@@ -20,24 +20,26 @@ def oracle_hulls():
 _ORACLE_HULLS = None
 
 
-def run_gpu(scene, steps, settings=None, capacity=None, broadphase=None):
+def run_gpu(scene, steps, settings=None, capacity=None, broadphase=None, meshes=None):
     from bepuphysics1_amd import Space, scenes
     sp = Space(capacity=capacity or max(len(scene), 2), settings=settings)
     if broadphase:
         sp.set_broadphase(broadphase)
     if scenes.has_hulls(scene):
         sp.register_hulls(scenes.hull_library())
+    if meshes:
+        sp.register_meshes(meshes)
     sp.add_scene(scene)
     if steps:
         sp.Update(steps)
     return sp
 
 
-def run_oracle(scene, steps, settings=None):
+def run_oracle(scene, steps, settings=None, meshes=None):
     from oracle import refcpu
     from bepuphysics1_amd import scenes
     w = refcpu.World(settings)
-    w.add_scene(scene, oracle_hulls() if scenes.has_hulls(scene) else None)
+    w.add_scene(scene, oracle_hulls() if scenes.has_hulls(scene) else None, meshes)
     if steps:
         w.step(steps)
     return w

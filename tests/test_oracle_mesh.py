@@ -325,3 +325,25 @@ def test_mesh_pile_manifolds(oracle):
             assert m.contacts[i].penetration_depth < 0.1
     st = w.motion_states()
     assert st[1:, 1].min() > -1.5
+
+
+@pytest.mark.parametrize("sidedness,stops", [(0, True), (1, True), (2, False)])
+def test_continuous_body_stops_at_the_mesh(oracle, sidedness, stops):
+    # A sphere (radius 0.25) falling at 180 m/s (3 m per step) from 1 m above
+    # a mesh: a discrete body tunnels through; a continuous one is clamped to
+    # the time of impact of its core sphere (0.8 of its radius) against the
+    # triangles, so it stops with its centre 0.2 above the mesh -- unless the
+    # mesh is one-sided and keeps only the other side.
+    from oracle import refcpu
+    V, I, _ = scenes.heightfield_mesh(cells=4, size=20.0)
+    for continuous in (False, True):
+        w = refcpu.World()
+        w.add_static_mesh(w.create_mesh_shape(V, I), sidedness)
+        w.add_body(1, [0.25, 0, 0], 1.0, [0.3, 1.0, 0.2, 0, 0, 0, 1, 0, -180, 0, 0, 0, 0], [1, 0.8, 0],
+                   flags=1 | (8 if continuous else 0))
+        w.step(1)
+        y = w.motion_states()[1][1]
+        if continuous and stops:
+            assert abs(y - 0.8 * 0.25) < 1e-3
+        else:
+            assert y < -1.9
