@@ -26,6 +26,7 @@ SHAPE_SPHERE = 1
 SHAPE_CONVEX_HULL = 2
 SHAPE_CAPSULE = 3
 SHAPE_STATIC_MESH = 4
+SHAPE_TERRAIN = 5
 FLAG_ALWAYS_ACTIVE = 1
 FLAG_NO_STABILIZATION = 2
 FLAG_CCD_PASSIVE = 4      # PositionUpdateMode.Passive
@@ -64,7 +65,65 @@ def hull_library():
 
 
 def has_meshes(scene):
-    return bool((np.asarray(scene["shape"]) == SHAPE_STATIC_MESH).any())
+    return bool(np.isin(np.asarray(scene["shape"]), (SHAPE_STATIC_MESH, SHAPE_TERRAIN)).any())
+
+
+class TerrainDesc(tuple):
+    """A terrain in the mesh list that goes with a scene: heights[width,
+    length] (column x along the first axis, row z along the second), the
+    QuadTriangleOrganization (0 BottomLeftUpperRight, 1 BottomRightUpperLeft)
+    and the affine transform (3x3 linear part as Matrix3x3 rows, translation)."""
+    __slots__ = ()
+
+    def __new__(cls, heights, organization, transform):
+        lin = np.asarray(transform[0], np.float32).reshape(3, 3)
+        tr = np.asarray(transform[1], np.float32).reshape(3)
+        return tuple.__new__(cls, (np.ascontiguousarray(heights, np.float32), int(organization), (lin, tr)))
+
+    heights = property(lambda self: self[0])
+    organization = property(lambda self: self[1])
+    transform = property(lambda self: self[2])
+
+
+def terrain(k, improve_boundary=True, material=(1.0, 1.0, 0.0)):
+    """A Terrain row: `k` indexes the mesh list entry (a TerrainDesc) it uses."""
+    return _row(SHAPE_TERRAIN, (float(k), 1.0 if improve_boundary else 0.0, 0.0), 0.0, (0.0, 0.0, 0.0),
+                material=material, flags=0)
+
+
+def heightfield_terrain(width=17, length=17, spacing=2.0, amplitude=0.0, seed=0, center=(0.0, 0.0), y=0.0,
+                        organization=0, linear=None):
+    """A width x length heightfield with `spacing` between samples, centred
+    on `center` (x, z), heights y plus a seeded smooth bump field; `linear`
+    (3x3) replaces the plain scaling.  As a TerrainDesc."""
+    rng = np.random.default_rng(seed)
+    gx, gz = np.meshgrid(np.arange(width) * spacing, np.arange(length) * spacing, indexing="ij")
+    h = np.full(gx.shape, 0.0)
+    if amplitude:
+        size = spacing * max(width, length)
+        for _ in range(4):
+            k = rng.uniform(0.5, 2.0, 2) * 2 * np.pi / size
+            ph = rng.uniform(0, 2 * np.pi, 2)
+            h += amplitude / 4 * np.sin(k[0] * gx + ph[0]) * np.cos(k[1] * gz + ph[1])
+    lin = np.diag([spacing, 1.0, spacing]) if linear is None else np.asarray(linear, np.float64)
+    tr = (center[0] - (width - 1) * spacing / 2, y, center[1] - (length - 1) * spacing / 2)
+    return TerrainDesc(h, organization, (lin, tr))
+
+
+def terrain_pile(n=64, seed=0, shapes=(SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE), drop=(1.0, 6.0),
+                 spacing=2.0, amplitude=1.0, organization=0, improve_boundary=True):
+    """A random pile of `shapes` dropped onto a terrain (the reference's
+    TerrainDemo).  Returns (scene, meshes)."""
+    rng = np.random.default_rng(seed)
+    foot = 2.0 * np.sqrt(max(n, 1)) + 2.0
+    samples = int(np.ceil((foot + 8.0) / spacing)) + 1
+    meshes = [heightfield_terrain(samples, samples, spacing, amplitude, seed=seed, organization=organization)]
+    rows = [terrain(0, improve_boundary)]
+    for _ in range(n):
+        s = int(rng.choice(shapes))
+        pos = (rng.uniform(-foot / 2, foot / 2), amplitude + rng.uniform(*drop), rng.uniform(-foot / 2, foot / 2))
+        rows.append(_row(s, _shape_dims(rng, s), 1.0, pos, quat=tuple(random_quaternions(rng, 1)[0])))
+    return make(rows), meshes
 
 
 def static_mesh(mesh, sidedness=0, improve_boundary=True, material=(1.0, 1.0, 0.0)):

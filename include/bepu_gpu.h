@@ -117,6 +117,8 @@ typedef struct bepu_material {
 /* A static triangle mesh (BroadPhaseEntries/StaticMesh.cs): a static
  * collidable, not an entity, that never moves and is never active. */
 #define BEPU_SHAPE_STATIC_MESH 4
+/* A terrain (BroadPhaseEntries/Terrain.cs): a static heightfield collidable. */
+#define BEPU_SHAPE_TERRAIN 5
 
 /* One contact of a manifold, in the manifold's contact-list order. */
 typedef struct bepu_contact {
@@ -139,10 +141,11 @@ typedef struct bepu_manifold {
 #define BEPU_PAIR_BOX_SPHERE 1
 #define BEPU_PAIR_SPHERE_SPHERE 2
 #define BEPU_PAIR_CONVEX 3
-/* A convex (box, capsule, hull) with a static mesh (StaticMeshConvexPairHandler)
- * and a sphere with a static mesh (StaticMeshSpherePairHandler): body_a is the
- * convex, body_b the mesh, the manifold holds up to four contacts from the
- * mesh's triangles with normals from the convex towards the mesh. */
+/* A convex (box, capsule, hull) with a static mesh or a terrain
+ * (StaticMeshConvexPairHandler, TerrainConvexPairHandler) and a sphere with
+ * either (StaticMeshSpherePairHandler, TerrainSpherePairHandler): body_a is
+ * the convex, body_b the mesh or terrain, the manifold holds up to four
+ * contacts from its triangles with normals from the convex towards it. */
 #define BEPU_PAIR_MESH_CONVEX 4
 #define BEPU_PAIR_MESH_SPHERE 5
 

@@ -137,6 +137,8 @@ struct TriTesterState {
     V3 axis;
 };
 ConvexShapeRef triangle_shape(const TriangleRef& t);
+// ConvexShape.GetLocalExtremePoint (ConvexShape.cs:104-113): with the margin.
+V3 local_extreme_point(const ConvexShapeRef& s, const V3& d);
 // Toolbox.GetClosestPointOnTriangleToPoint (BEPUutilities/Toolbox.cs:338-419).
 int closest_point_on_triangle(const V3& a, const V3& b, const V3& c, const V3& p, V3* out);
 // TriangleConvexPairTester.GenerateContactCandidates (TriangleConvexPairTester.cs:23-46):
@@ -170,6 +172,30 @@ struct MeshShape {
     Affine xf;
     std::vector<MeshNode> nodes;  // nodes[0] is the root (empty mesh: no nodes)
 };
+// A terrain (CollisionShapes/TerrainShape.cs, BroadPhaseEntries/Terrain.cs):
+// heights[column * length + row] for `width` columns (x) and `length` rows
+// (z), the quads' triangle organization (QuadTriangleOrganization:
+// 0 BottomLeftUpperRight, 1 BottomRightUpperLeft) and the world transform.
+struct TerrainShapeRef {
+    int width, length, organization;
+    std::vector<float> heights;
+    Affine xf;
+    float h(int column, int row) const { return heights[(size_t)column * length + row]; }
+};
+// Matrix3x3.Invert (BEPUutilities/Matrix3x3.cs:505-531), AffineTransform.Invert (AffineTransform.cs:173-178)
+M3 m3_invert(const M3& m);
+Affine affine_invert(const Affine& a);
+// TerrainShape.GetBoundingBox (TerrainShape.cs:50-120)
+void terrain_bounding_box(const TerrainShapeRef& t, V3* mn, V3* mx);
+// TerrainShape.GetOverlaps (TerrainShape.cs:406-453): triangle ids (2 per quad) over a local box.
+void terrain_overlaps(const TerrainShapeRef& t, const V3& lmin, const V3& lmax, std::vector<int>* out);
+// TerrainShape.GetLocalIndices (TerrainShape.cs:538-583): (column, row) of the three vertices.
+void terrain_local_indices(const TerrainShapeRef& t, int tri, int col[3], int row[3]);
+// TerrainShape.GetTriangle (TerrainShape.cs:516-533): the world-space triangle.
+void terrain_world_triangle(const TerrainShapeRef& t, int tri, V3 out[3]);
+// Terrain.WorldTransform setter (Terrain.cs:45-62): the sidedness its winding gets.
+int terrain_sidedness(const Affine& xf);
+
 // MeshBoundingBoxTree.Reconstruct (MeshBoundingBoxTree.cs:60-69).
 void mesh_build_tree(MeshShape* m);
 // MeshBoundingBoxTree.GetOverlaps(BoundingBox) (MeshBoundingBoxTree.cs:112-122):

@@ -101,6 +101,11 @@ def lib():
                                              ctypes.POINTER(Material), u32p]
         L.refcpu_get_mesh_tree.argtypes = [vp, ctypes.c_uint32, fpp, ip, ctypes.c_uint32, u32p]
         L.refcpu_mesh_overlaps.argtypes = [vp, ctypes.c_uint32, fpp, fpp, ip, ctypes.c_uint32, u32p]
+        L.refcpu_create_terrain_shape.argtypes = [vp, fpp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, u32p]
+        L.refcpu_add_terrain.argtypes = [vp, ctypes.c_uint32, fpp, ctypes.c_int, ctypes.POINTER(Material), u32p]
+        L.refcpu_terrain_overlaps.argtypes = [vp, ctypes.c_uint32, fpp, fpp, ip, ctypes.c_uint32, u32p]
+        L.refcpu_terrain_triangle.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, fpp, ip]
+        L.refcpu_terrain_query_box.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, fpp]
         L.refcpu_triangle_convex.argtypes = [fpp, fpp, ctypes.c_int, ctypes.c_float, ctypes.c_float, fpp,
                                              ctypes.c_int, ctypes.c_float, ip, fpp, fpp]
         L.refcpu_triangle_region.argtypes = [fpp, fpp]
@@ -207,6 +212,11 @@ class World:
                     mesh_ids[k] = self.create_mesh_shape(*meshes[k])
                 self.add_static_mesh(mesh_ids[k], int(r["dims"][1]), bool(r["dims"][2]), r["material"])
                 continue
+            if shape == 5:
+                t = meshes[int(r["dims"][0])]
+                sid = self.create_terrain_shape(t.heights, t.organization)
+                self.add_terrain(sid, t.transform, bool(r["dims"][1]), r["material"])
+                continue
             hull = hulls[int(r["dims"][0])] if shape == 2 else None
             self.add_body(shape, r["dims"], r["mass"], r["state"], r["material"], int(r["flags"]), hull=hull)
 
@@ -236,6 +246,57 @@ class World:
         if rc != 0:
             raise ValueError("refcpu_add_static_mesh failed with status %d" % rc)
         return bid.value
+
+    def create_terrain_shape(self, heights, organization=0):
+        """new TerrainShape(heights[width, length], organization); returns its id."""
+        h = np.ascontiguousarray(heights, np.float32)
+        sid = ctypes.c_uint32()
+        rc = lib().refcpu_create_terrain_shape(self.h, _fptr(h), h.shape[0], h.shape[1], int(organization),
+                                               ctypes.byref(sid))
+        if rc != 0:
+            raise ValueError("refcpu_create_terrain_shape failed with status %d" % rc)
+        return sid.value
+
+    def add_terrain(self, shape_id, transform, improve_boundary=True, material=(1.0, 0.8, 0.0)):
+        """A Terrain of a terrain shape under the affine `transform` (3x3 linear
+        part as Matrix3x3 rows, translation); returns its body id."""
+        t = np.concatenate([np.asarray(transform[0], np.float32).reshape(9),
+                            np.asarray(transform[1], np.float32).reshape(3)]).astype(np.float32)
+        mat = Material(*[float(x) for x in material])
+        bid = ctypes.c_uint32()
+        rc = lib().refcpu_add_terrain(self.h, int(shape_id), _fptr(t), 1 if improve_boundary else 0,
+                                      ctypes.byref(mat), ctypes.byref(bid))
+        if rc != 0:
+            raise ValueError("refcpu_add_terrain failed with status %d" % rc)
+        return bid.value
+
+    def terrain_overlaps(self, body, lmin, lmax):
+        """TerrainShape.GetOverlaps over a local box: triangle ids in scan order."""
+        lo = np.ascontiguousarray(lmin, np.float32)
+        hi = np.ascontiguousarray(lmax, np.float32)
+        n = ctypes.c_uint32()
+        lib().refcpu_terrain_overlaps(self.h, int(body), _fptr(lo), _fptr(hi), None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), np.int32)
+        lib().refcpu_terrain_overlaps(self.h, int(body), _fptr(lo), _fptr(hi),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value, ctypes.byref(n))
+        return out[:n.value]
+
+    def terrain_triangle(self, body, tri, world):
+        """(vertices [3, 3], sequential indices or None) of a terrain triangle."""
+        v = np.zeros(9, np.float32)
+        ix = np.zeros(3, np.int32)
+        rc = lib().refcpu_terrain_triangle(self.h, int(body), int(tri), 1 if world else 0, _fptr(v),
+                                           ix.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        if rc != 0:
+            raise ValueError("refcpu_terrain_triangle failed with status %d" % rc)
+        return v.reshape(3, 3), (None if world else ix)
+
+    def terrain_query_box(self, convex, terrain):
+        out = np.zeros(6, np.float32)
+        rc = lib().refcpu_terrain_query_box(self.h, int(convex), int(terrain), _fptr(out))
+        if rc != 0:
+            raise ValueError("refcpu_terrain_query_box failed with status %d" % rc)
+        return out[:3], out[3:]
 
     def mesh_tree(self, shape_id):
         """The flattened MeshBoundingBoxTree: (boxes [n, 6], links [n, 2])."""

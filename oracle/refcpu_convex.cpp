@@ -1432,6 +1432,8 @@ bool ccd_sphere_cast_normal(const V3& origin, const V3& dir, float radius, const
 // TrianglePairTesters.  The convex is shape A at the identity, the triangle
 // (already in the convex's local space) shape B at the identity.
 
+V3 local_extreme_point(const ConvexShapeRef& s, const V3& d) { return local_extreme_with_margin(s, d); }
+
 ConvexShapeRef triangle_shape(const TriangleRef& t) {
     ConvexShapeRef r;
     std::memset(&r, 0, sizeof(r));

@@ -5,6 +5,8 @@
 //
 // TEST INFRASTRUCTURE ONLY (see refcpu.h).  Each expression keeps the operand
 // order of the reference.
+#include <algorithm>
+#include <cfloat>
 #include <cstdint>
 #include "refcpu.h"
 
@@ -213,6 +215,129 @@ void mesh_overlaps(const MeshShape& m, const V3& bmin, const V3& bmax, std::vect
         }
         if (intersects(m.nodes[n.b], bmin, bmax)) stack.push_back(n.b);
         if (intersects(m.nodes[n.a], bmin, bmax)) stack.push_back(n.a);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Terrains
+
+// Matrix3x3.Determinant (Matrix3x3.cs:1231-1235) and Matrix3x3.Invert (:505-531)
+M3 m3_invert(const M3& m) {
+    float det = m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 - m.m31 * m.m22 * m.m13 -
+                m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
+    float di = 1 / det;
+    M3 r;
+    r.m11 = (m.m22 * m.m33 - m.m23 * m.m32) * di;
+    r.m12 = (m.m13 * m.m32 - m.m33 * m.m12) * di;
+    r.m13 = (m.m12 * m.m23 - m.m22 * m.m13) * di;
+    r.m21 = (m.m23 * m.m31 - m.m21 * m.m33) * di;
+    r.m22 = (m.m11 * m.m33 - m.m13 * m.m31) * di;
+    r.m23 = (m.m13 * m.m21 - m.m11 * m.m23) * di;
+    r.m31 = (m.m21 * m.m32 - m.m22 * m.m31) * di;
+    r.m32 = (m.m12 * m.m31 - m.m11 * m.m32) * di;
+    r.m33 = (m.m11 * m.m22 - m.m12 * m.m21) * di;
+    return r;
+}
+
+Affine affine_invert(const Affine& a) {
+    Affine r;
+    r.m = m3_invert(a.m);
+    r.t = neg(m3_transform(a.t, r.m));
+    return r;
+}
+
+int terrain_sidedness(const Affine& xf) {
+    // Matrix3x3.AdjugateTranspose (Matrix3x3.cs:673-705): its Up row is
+    // (m21, m22, m23) of the transposed adjugate, i.e. (m12, m22, m32) of the adjugate.
+    const M3& m = xf.m;
+    V3 up_n{m.m13 * m.m32 - m.m33 * m.m12, m.m11 * m.m33 - m.m13 * m.m31, m.m12 * m.m31 - m.m11 * m.m32};
+    V3 up{m.m21, m.m22, m.m23};
+    return dot(up_n, up) < 0 ? SIDED_CLOCKWISE : SIDED_COUNTERCLOCKWISE;
+}
+
+void terrain_bounding_box(const TerrainShapeRef& t, V3* mn, V3* mx) {
+    float minX = FLT_MAX, maxX = -FLT_MAX, minY = FLT_MAX, maxY = -FLT_MAX, minZ = FLT_MAX, maxZ = -FLT_MAX;
+    V3 minXv{0, 0, 0}, maxXv{0, 0, 0}, minYv{0, 0, 0}, maxYv{0, 0, 0}, minZv{0, 0, 0}, maxZv{0, 0, 0};
+    for (int i = 0; i < t.width; i++)
+        for (int j = 0; j < t.length; j++) {
+            V3 v = m3_transform(V3{(float)i, t.h(i, j), (float)j}, t.xf.m);
+            if (v.x < minX) { minX = v.x; minXv = v; } else if (v.x > maxX) { maxX = v.x; maxXv = v; }
+            if (v.y < minY) { minY = v.y; minYv = v; } else if (v.y > maxY) { maxY = v.y; maxYv = v; }
+            if (v.z < minZ) { minZ = v.z; minZv = v; } else if (v.z > maxZ) { maxZ = v.z; maxZv = v; }
+        }
+    mn->x = minXv.x + t.xf.t.x;
+    mn->y = minYv.y + t.xf.t.y;
+    mn->z = minZv.z + t.xf.t.z;
+    mx->x = maxXv.x + t.xf.t.x;
+    mx->y = maxYv.y + t.xf.t.y;
+    mx->z = maxZv.z + t.xf.t.z;
+}
+
+void terrain_overlaps(const TerrainShapeRef& t, const V3& lmin, const V3& lmax, std::vector<int>* out) {
+    out->clear();
+    const int width = t.width;
+    const int minX = std::max((int)lmin.x, 0), minY = std::max((int)lmin.z, 0);
+    const int maxX = std::min((int)lmax.x, width - 2), maxY = std::min((int)lmax.z, t.length - 2);
+    for (int i = minX; i <= maxX; i++)
+        for (int j = minY; j <= maxY; j++) {
+            float y1 = t.h(i, j), y2 = t.h(i + 1, j), y3 = t.h(i, j + 1), y4 = t.h(i + 1, j + 1);
+            float highest = y1, lowest = y1;
+            if (y2 > highest) highest = y2; else if (y2 < lowest) lowest = y2;
+            if (y3 > highest) highest = y3; else if (y3 < lowest) lowest = y3;
+            if (y4 > highest) highest = y4; else if (y4 < lowest) lowest = y4;
+            if (lmax.y < lowest || lmin.y > highest) continue;
+            int quad = (i + j * width) << 1;
+            out->push_back(quad);
+            out->push_back(quad | 1);
+        }
+}
+
+void terrain_local_indices(const TerrainShapeRef& t, int tri, int col[3], int row[3]) {
+    int quad = tri / 2;
+    int r = quad / t.width;
+    int c = quad - r * t.width;
+    const bool blur = t.organization == 0;
+    if ((tri & 1) == 0) {
+        col[0] = c; row[0] = r;
+        col[1] = c + 1; row[1] = r;
+        if (blur) { col[2] = c; row[2] = r + 1; } else { col[2] = c + 1; row[2] = r + 1; }
+    } else if (blur) {
+        col[0] = c; row[0] = r + 1;
+        col[2] = c + 1; row[2] = r + 1;
+        col[1] = c + 1; row[1] = r;
+    } else {
+        col[0] = c; row[0] = r;
+        col[1] = c + 1; row[1] = r + 1;
+        col[2] = c; row[2] = r + 1;
+    }
+}
+
+// TerrainShape.GetPosition (TerrainShape.cs:377-398): clamped indices, world space.
+static V3 terrain_position(const TerrainShapeRef& t, int c, int r) {
+    if (c <= 0) c = 0;
+    else if (c >= t.width) c = t.width - 1;
+    if (r <= 0) r = 0;
+    else if (r >= t.length) r = t.length - 1;
+    return affine_transform(V3{(float)c, t.h(c, r), (float)r}, t.xf);
+}
+
+void terrain_world_triangle(const TerrainShapeRef& t, int tri, V3 out[3]) {
+    int quad = tri / 2;
+    int r = quad / t.width;
+    int c = quad - r * t.width;
+    const bool blur = t.organization == 0;
+    if ((tri & 1) == 0) {
+        out[0] = terrain_position(t, c, r);
+        out[1] = terrain_position(t, c + 1, r);
+        out[2] = blur ? terrain_position(t, c, r + 1) : terrain_position(t, c + 1, r + 1);
+    } else if (blur) {
+        out[0] = terrain_position(t, c, r + 1);
+        out[1] = terrain_position(t, c + 1, r + 1);
+        out[2] = terrain_position(t, c + 1, r);
+    } else {
+        out[0] = terrain_position(t, c, r);
+        out[1] = terrain_position(t, c, r + 1);
+        out[2] = terrain_position(t, c + 1, r + 1);
     }
 }
 

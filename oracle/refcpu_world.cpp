@@ -87,7 +87,12 @@ struct Body {
     int mesh;
     int sidedness;
     bool improve_boundary;
+    // A terrain (Terrain.cs): its entry in refcpu_world::terrains; -1 otherwise.
+    int terrain;
 };
+
+// A static collidable (a static mesh or a terrain): not an entity.
+inline bool is_static(const Body& b) { return b.mesh >= 0 || b.terrain >= 0; }
 
 struct ContactObj {
     V3 pos, normal;
@@ -167,6 +172,17 @@ struct Pair {
     // Mesh pairs with a convex (not a sphere): the manifold's table of
     // TriangleConvexPairTesters, refcpu_world::mesh_testers[mesh_slot].
     int mesh_slot;
+    // Terrain pairs: the local box of this step's triangle query (the time of
+    // impact sweeps the triangles the manifold found).
+    V3 tq_min, tq_max;  // a terrain pair's last triangle query box (TerrainContactManifold)
+    bool tq_valid;
+};
+
+// A Terrain (BroadPhaseEntries/Terrain.cs): its shape's heights, its world
+// transform, the sidedness that transform gives its triangles.
+struct TerrainInst {
+    TerrainShapeRef shape;
+    int sidedness;
 };
 
 // One entry of TriangleMeshConvexContactManifold.activePairTesters
@@ -319,6 +335,8 @@ struct refcpu_world {
     bool ccd = false;  // a body was ever passive or continuous
     std::vector<JointC> joints;
     std::vector<MeshShape> mesh_shapes;
+    std::vector<TerrainShapeRef> terrain_shapes;  // heights and organization (transform per terrain)
+    std::vector<TerrainInst> terrains;
     // Tester tables of the mesh pairs (Pair::mesh_slot), allocated as pairs
     // are created (under the lock: pairs are updated on the pool) and freed
     // when they end.
@@ -381,6 +399,7 @@ void init_body(refcpu_world* w, Body* b, const bepu_motion_state* st, float mass
     if (b->pmode) w->ccd = true;
     b->q_world = b->q;
     b->mesh = -1;
+    b->terrain = -1;
     b->sidedness = SIDED_DOUBLE;
     b->improve_boundary = false;
 }
@@ -449,7 +468,7 @@ inline bool body_active(const Body& b) { return !b.removed && (b.dynamic ? !b.as
 // candidate flag is cleared (which zeroes the time below the limit) and the
 // island becomes active; a kinematic member's flag becomes -1.
 void activate(Body& b) {
-    if (b.removed || b.mesh >= 0) return;  // a static collidable has no activity
+    if (b.removed || is_static(b)) return;  // a static collidable has no activity
     if (b.dynamic) {
         b.candidate = false;
         b.time_below = 0;
@@ -677,7 +696,7 @@ void bounding_box(refcpu_world* w, Body& b) {
     // BoundingBoxUpdater.cs:48: active entries only (a sleeping body keeps its
     // box; recomputing an inactive kinematic body's gives the same box).
     if (b.removed || (b.dynamic && b.asleep)) return;
-    if (b.mesh >= 0) return;  // StaticMesh.UpdateBoundingBox: the tree's root box, fixed
+    if (is_static(b)) return;  // a static mesh's or terrain's box is fixed
     shape_bounding_box(w, b, b.pos, b.q, &b.bmin, &b.bmax);
     float dt = w->s.dt;
     if (dt > 0) {
@@ -1356,10 +1375,49 @@ bool mesh_analyze(int region, const int ix[3], const TriangleRef& t, bool should
 }
 
 // TriangleMeshConvexContactManifold.Update (TriangleMeshConvexContactManifold.cs:101-434)
+// ConvexShape.GetLocalBoundingBox (ConvexShapes/ConvexShape.cs:225-281): the
+// shape at `shape_xf` in the frame of the affine `space`, from the extreme
+// points (with margin) along that frame's axes.
+void convex_local_bounding_box(const ConvexShapeRef& cvx, const RT& shape_xf, const Affine& space, V3* mn, V3* mx) {
+    const Affine inv = affine_invert(space);
+    // AffineTransform.Multiply(RigidTransform, AffineTransform) (AffineTransform.cs:202-211)
+    Affine t;
+    t.m = m3_mul(m3_from_quat(shape_xf.q), inv.m);
+    t.t = add(m3_transform(shape_xf.pos, inv.m), inv.t);
+    const M3& m = t.m;
+    V3 right = local_extreme_point(cvx, V3{m.m11, m.m21, m.m31});
+    V3 left = local_extreme_point(cvx, V3{-m.m11, -m.m21, -m.m31});
+    V3 up = local_extreme_point(cvx, V3{m.m12, m.m22, m.m32});
+    V3 down = local_extreme_point(cvx, V3{-m.m12, -m.m22, -m.m32});
+    V3 backward = local_extreme_point(cvx, V3{m.m13, m.m23, m.m33});
+    V3 forward = local_extreme_point(cvx, V3{-m.m13, -m.m23, -m.m33});
+    // EntityShape.TransformLocalExtremePoints (EntityShape.cs:58-63)
+    V3 pos{right.x * m.m11 + right.y * m.m21 + right.z * m.m31, up.x * m.m12 + up.y * m.m22 + up.z * m.m32,
+           backward.x * m.m13 + backward.y * m.m23 + backward.z * m.m33};
+    V3 neg_{left.x * m.m11 + left.y * m.m21 + left.z * m.m31, down.x * m.m12 + down.y * m.m22 + down.z * m.m32,
+            forward.x * m.m13 + forward.y * m.m23 + forward.z * m.m33};
+    *mx = V3{t.t.x + pos.x, t.t.y + pos.y, t.t.z + pos.z};
+    *mn = V3{t.t.x + neg_.x, t.t.y + neg_.y, t.t.z + neg_.z};
+}
+
+// TerrainContactManifold.FindOverlappingTriangles (TerrainContactManifold.cs:35-64):
+// the convex's box in the terrain's local frame, grown by its motion over the step.
+void terrain_query_box(refcpu_world* w, const Body& A, const Affine& xf, V3* mn, V3* mx) {
+    convex_local_bounding_box(shape_ref(w, A), RT{A.pos, A.q}, xf, mn, mx);
+    M3 inverse = m3_invert(xf.m);
+    V3 tv = m3_transform(A.v, inverse);
+    tv = mul(tv, w->s.dt);
+    if (tv.x > 0) mx->x += tv.x; else mn->x += tv.x;
+    if (tv.y > 0) mx->y += tv.y; else mn->y += tv.y;
+    if (tv.z > 0) mx->z += tv.z; else mn->z += tv.z;
+}
+
 void update_mesh(refcpu_world* w, Pair& p) {
     const Body& A = w->bodies[p.a];
     const Body& M = w->bodies[p.b];
-    const MeshShape& shape = w->mesh_shapes[M.mesh];
+    const bool terrain = M.terrain >= 0;
+    const TerrainShapeRef* ts = terrain ? &w->terrains[M.terrain].shape : nullptr;
+    const MeshShape* shape = terrain ? nullptr : &w->mesh_shapes[M.mesh];
     const RT ta{A.pos, A.q};
     // ContactRefresher.ContactRefresh (ContactRefresher.cs:18-69)
     int rem[4], nrem = 0;
@@ -1385,7 +1443,13 @@ void update_mesh(refcpu_world* w, Pair& p) {
     for (int i = nrem - 1; i >= 0; i--) manifold_remove(p, rem[i]);
 
     std::vector<int> tris;
-    mesh_overlaps(shape, A.bmin, A.bmax, &tris);
+    if (terrain) {
+        terrain_query_box(w, A, ts->xf, &p.tq_min, &p.tq_max);
+        p.tq_valid = true;
+        terrain_overlaps(*ts, p.tq_min, p.tq_max, &tris);
+    } else {
+        mesh_overlaps(*shape, A.bmin, A.bmax, &tris);
+    }
     const bool improve = M.improve_boundary;
     BoundarySets sets;
     std::vector<ContactData> cand;
@@ -1396,7 +1460,7 @@ void update_mesh(refcpu_world* w, Pair& p) {
     inv.q = qconj(A.q);
     inv.pos = neg(qtransform(A.pos, inv.q));
     Affine convex_inverse{m3_from_quat(inv.q), inv.pos};
-    const Affine to_local = affine_multiply(shape.xf, convex_inverse);
+    const Affine to_local = affine_multiply(terrain ? ts->xf : shape->xf, convex_inverse);
     V3 lmin, lmax;
     shape_bounding_box(w, A, V3{0, 0, 0}, Q4{0, 0, 0, 1}, &lmin, &lmax);
     const M3 orientation = m3_from_quat(A.q);
@@ -1435,10 +1499,22 @@ void update_mesh(refcpu_world* w, Pair& p) {
     };
 
     for (int tri : tris) {
-        // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66)
-        const int ix[3] = {shape.idx[tri], shape.idx[tri + 1], shape.idx[tri + 2]};
+        // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66, TerrainContactManifold.cs:76-93)
+        int ix[3];
         TriangleRef t;
-        for (int k = 0; k < 3; ++k) t.v[k] = affine_transform(shape.verts[ix[k]], to_local);
+        if (terrain) {
+            int col[3], row[3];
+            terrain_local_indices(*ts, tri, col, row);
+            for (int k = 0; k < 3; ++k) {
+                ix[k] = row[k] * ts->width + col[k];  // TerrainVertexIndices.ToSequentialIndex
+                t.v[k] = affine_transform(V3{(float)col[k], ts->h(col[k], row[k]), (float)row[k]}, to_local);
+            }
+        } else {
+            for (int k = 0; k < 3; ++k) {
+                ix[k] = shape->idx[tri + k];
+                t.v[k] = affine_transform(shape->verts[ix[k]], to_local);
+            }
+        }
         t.sidedness = M.sidedness;
         t.margin = 0;
         // Toolbox.GetTriangleBoundingBox (BEPUutilities/Toolbox.cs:1516-1580) against the local box.
@@ -1571,13 +1647,22 @@ void mesh_time_of_impact(refcpu_world* w, Pair& p) {
     const float minimumRadius = A.min_radius * w->s.core_shape_scaling;
     p.toi = 1;
     if (!(minimumRadius * minimumRadius < vsq)) return;
-    const MeshShape& shape = w->mesh_shapes[M.mesh];
+    const bool terrain = M.terrain >= 0;
+    const TerrainShapeRef* ts = terrain ? &w->terrains[M.terrain].shape : nullptr;
     std::vector<int> tris;
-    mesh_overlaps(shape, A.bmin, A.bmax, &tris);
+    if (terrain) {
+        // The overlaps the manifold's last update found (none before its first).
+        if (p.tq_valid) terrain_overlaps(*ts, p.tq_min, p.tq_max, &tris);
+    }
+    else mesh_overlaps(w->mesh_shapes[M.mesh], A.bmin, A.bmax, &tris);
     const RT ta = ccd_transform(A);
+    // TerrainPairHandler.UpdateTimeOfImpact (TerrainPairHandler.cs:81-137):
+    // the terrain's up direction decides which side of a triangle counts.
+    const V3 terrain_up = terrain ? V3{ts->xf.m.m21, ts->xf.m.m22, ts->xf.m.m23} : V3{0, 0, 0};
     for (int tri : tris) {
         TriangleRef t;
-        mesh_world_triangle(shape, tri, t.v);
+        if (terrain) terrain_world_triangle(*ts, tri, t.v);
+        else mesh_world_triangle(w->mesh_shapes[M.mesh], tri, t.v);
         for (int k = 0; k < 3; ++k) t.v[k] = sub(t.v[k], ta.pos);
         t.sidedness = M.sidedness;
         t.margin = 0;
@@ -1586,7 +1671,13 @@ void mesh_time_of_impact(refcpu_world* w, Pair& p) {
         if (ccd_sphere_cast_normal(V3{0, 0, 0}, velocity, minimumRadius, triangle_shape(t), kMeshTransform, p.toi,
                                    w->s.core_shape_scaling, &hitT, &hitN) &&
             hitT > 1e-5f) {
-            if (M.sidedness != SIDED_DOUBLE) {
+            if (terrain) {
+                V3 n = cross(sub(t.v[2], t.v[0]), sub(t.v[1], t.v[0]));
+                float d = dot(n, terrain_up);
+                if (d < 0) d = dot(n, hitN);
+                else d = -dot(n, hitN);
+                if (d < 0) p.toi = hitT;
+            } else if (M.sidedness != SIDED_DOUBLE) {
                 V3 n = cross(sub(t.v[1], t.v[0]), sub(t.v[2], t.v[0]));
                 float d = dot(n, hitN);
                 if ((M.sidedness == SIDED_COUNTERCLOCKWISE && d < 0) || (M.sidedness == SIDED_CLOCKWISE && d > 0))
@@ -1605,10 +1696,11 @@ Pair make_pair_state(refcpu_world* w, uint32_t i, uint32_t j) {
     const Body& bj = w->bodies[j];
     int si = bi.shape, sj = bj.shape;
     p.mesh_slot = -1;
-    if (si == BEPU_SHAPE_STATIC_MESH || sj == BEPU_SHAPE_STATIC_MESH) {
-        // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79): A is the convex.
-        p.a = si == BEPU_SHAPE_STATIC_MESH ? j : i;
-        p.b = si == BEPU_SHAPE_STATIC_MESH ? i : j;
+    if (is_static(bi) || is_static(bj)) {
+        // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79),
+        // TerrainPairHandler.Initialize (TerrainPairHandler.cs:48-65): A is the convex.
+        p.a = is_static(bi) ? j : i;
+        p.b = is_static(bi) ? i : j;
         if (w->bodies[p.a].shape == BEPU_SHAPE_SPHERE) {
             p.kind = BEPU_PAIR_MESH_SPHERE;
         } else {
@@ -2278,23 +2370,6 @@ inline M3 m3_sub(const M3& a, const M3& b) {
 }
 // Matrix3x3.CreateScale (Matrix3x3.cs:437-440)
 inline M3 m3_diag(float s) { return M3{s, 0, 0, 0, s, 0, 0, 0, s}; }
-// Matrix3x3.Invert (Matrix3x3.cs:505-530) with Determinant (:1231-1235)
-inline M3 m3_invert(const M3& m) {
-    const float det = m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 -
-                      m.m31 * m.m22 * m.m13 - m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
-    const float d = 1 / det;
-    M3 r;
-    r.m11 = (m.m22 * m.m33 - m.m23 * m.m32) * d;
-    r.m12 = (m.m13 * m.m32 - m.m33 * m.m12) * d;
-    r.m13 = (m.m12 * m.m23 - m.m22 * m.m13) * d;
-    r.m21 = (m.m23 * m.m31 - m.m21 * m.m33) * d;
-    r.m22 = (m.m11 * m.m33 - m.m13 * m.m31) * d;
-    r.m23 = (m.m13 * m.m21 - m.m11 * m.m23) * d;
-    r.m31 = (m.m21 * m.m32 - m.m22 * m.m31) * d;
-    r.m32 = (m.m12 * m.m31 - m.m11 * m.m32) * d;
-    r.m33 = (m.m11 * m.m22 - m.m12 * m.m21) * d;
-    return r;
-}
 
 // Solver.UnsafePrestep -> BallSocketJoint.Update (Solver.cs:265-275,
 // BallSocketJoint.cs:229-300): world offsets, the softened effective mass
@@ -2890,6 +2965,106 @@ int refcpu_add_static_mesh(refcpu_world* w, uint32_t shape, int sidedness, int i
     return BEPU_OK;
 }
 
+// new TerrainShape(heights, triangleOrganization) (TerrainShape.cs:38-56):
+// heights[column * length + row] for `width` columns and `length` rows.
+int refcpu_create_terrain_shape(refcpu_world* w, const float* heights, uint32_t width, uint32_t length,
+                                int organization, uint32_t* id) {
+    if (!heights || width < 2 || length < 2 || organization < 0 || organization > 1) return BEPU_E_INVALID_ARG;
+    if (!finite3(heights, (int)(width * length))) return BEPU_E_INVALID_ARG;
+    TerrainShapeRef t;
+    t.width = (int)width;
+    t.length = (int)length;
+    t.organization = organization;
+    t.heights.assign(heights, heights + (size_t)width * length);
+    if (id) *id = (uint32_t)w->terrain_shapes.size();
+    w->terrain_shapes.push_back(std::move(t));
+    return BEPU_OK;
+}
+
+// new Terrain(shape, worldTransform) added to the space (Terrain.cs): its
+// sidedness follows the transform's winding; its box is the shape's under
+// the transform (Terrain.UpdateBoundingBox, thickness 0).
+int refcpu_add_terrain(refcpu_world* w, uint32_t shape, const float* transform, int improve_boundary,
+                       const bepu_material* m, uint32_t* id) {
+    if (shape >= w->terrain_shapes.size() || !transform || !finite3(transform, 12)) return BEPU_E_INVALID_ARG;
+    TerrainInst ti;
+    ti.shape = w->terrain_shapes[shape];
+    ti.shape.xf.m = M3{transform[0], transform[1], transform[2], transform[3], transform[4], transform[5],
+                       transform[6], transform[7], transform[8]};
+    ti.shape.xf.t = V3{transform[9], transform[10], transform[11]};
+    ti.sidedness = terrain_sidedness(ti.shape.xf);
+    Body b{};
+    std::memset(&b, 0, sizeof(b));
+    b.shape = BEPU_SHAPE_TERRAIN;
+    bepu_motion_state st{};
+    st.orientation[3] = 1;
+    init_body(w, &b, &st, 0, m, 0, M3{0, 0, 0, 0, 0, 0, 0, 0, 0});
+    b.time_below = 1;  // never active
+    b.terrain = (int)w->terrains.size();
+    b.sidedness = ti.sidedness;
+    b.improve_boundary = improve_boundary != 0;
+    terrain_bounding_box(ti.shape, &b.bmin, &b.bmax);
+    // The thickness offset (Terrain.UpdateBoundingBox): Normalize(Down) * 0.
+    V3 off = mul(normalize(V3{-ti.shape.xf.m.m21, -ti.shape.xf.m.m22, -ti.shape.xf.m.m23}), 0.0f);
+    if (off.x < 0) b.bmin.x += off.x; else b.bmax.x += off.x;
+    if (off.y < 0) b.bmin.y += off.y; else b.bmax.y += off.y;
+    if (off.z < 0) b.bmin.z += off.z; else b.bmax.z += off.z;
+    w->terrains.push_back(ti);
+    if (id) *id = (uint32_t)w->bodies.size();
+    w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+// TerrainShape.GetOverlaps of a terrain body over a box in its local space (tests).
+int refcpu_terrain_overlaps(refcpu_world* w, uint32_t body, const float* lmin, const float* lmax, int32_t* out,
+                            uint32_t cap, uint32_t* n) {
+    if (body >= w->bodies.size() || w->bodies[body].terrain < 0) return BEPU_E_INVALID_ARG;
+    std::vector<int> tris;
+    terrain_overlaps(w->terrains[w->bodies[body].terrain].shape, V3{lmin[0], lmin[1], lmin[2]},
+                     V3{lmax[0], lmax[1], lmax[2]}, &tris);
+    for (uint32_t k = 0; k < tris.size() && k < cap; ++k) out[k] = tris[k];
+    if (n) *n = (uint32_t)tris.size();
+    return BEPU_OK;
+}
+
+// One triangle of a terrain body (tests): world != 0 gives TerrainShape.GetTriangle
+// in world space; world == 0 the GetLocalIndices vertices in the shape's local
+// space with their sequential indices.
+int refcpu_terrain_triangle(refcpu_world* w, uint32_t body, int tri, int world, float* out9, int32_t* idx3) {
+    if (body >= w->bodies.size() || w->bodies[body].terrain < 0) return BEPU_E_INVALID_ARG;
+    const TerrainShapeRef& t = w->terrains[w->bodies[body].terrain].shape;
+    V3 v[3];
+    if (world) {
+        terrain_world_triangle(t, tri, v);
+    } else {
+        int col[3], row[3];
+        terrain_local_indices(t, tri, col, row);
+        for (int k = 0; k < 3; ++k) {
+            v[k] = V3{(float)col[k], t.h(col[k], row[k]), (float)row[k]};
+            if (idx3) idx3[k] = row[k] * t.width + col[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        out9[3 * k] = v[k].x;
+        out9[3 * k + 1] = v[k].y;
+        out9[3 * k + 2] = v[k].z;
+    }
+    return BEPU_OK;
+}
+
+// The box a terrain pair's manifold queries (tests): the convex body's shape
+// in the terrain's local space, grown by its velocity over one step.
+int refcpu_terrain_query_box(refcpu_world* w, uint32_t convex, uint32_t body, float* out6) {
+    if (body >= w->bodies.size() || w->bodies[body].terrain < 0 || convex >= w->bodies.size() ||
+        is_static(w->bodies[convex]))
+        return BEPU_E_INVALID_ARG;
+    V3 mn, mx;
+    terrain_query_box(w, w->bodies[convex], w->terrains[w->bodies[body].terrain].shape.xf, &mn, &mx);
+    const float o[6] = {mn.x, mn.y, mn.z, mx.x, mx.y, mx.z};
+    std::memcpy(out6, o, sizeof(o));
+    return BEPU_OK;
+}
+
 // The flattened tree of a mesh shape (tests): per node bmin[3], bmax[3], a, b.
 int refcpu_get_mesh_tree(refcpu_world* w, uint32_t shape, float* boxes, int32_t* links, uint32_t cap, uint32_t* n) {
     if (shape >= w->mesh_shapes.size()) return BEPU_E_INVALID_ARG;
@@ -3072,7 +3247,7 @@ int refcpu_add_ball_socket(refcpu_world* w, uint32_t a, uint32_t b, const float*
     const Body& A = w->bodies[a];
     const Body& B = w->bodies[b];
     if (!A.dynamic && !B.dynamic) return BEPU_E_INVALID_ARG;  // "Cannot constrain two kinematic bodies."
-    if (A.mesh >= 0 || B.mesh >= 0) return BEPU_E_INVALID_ARG;  // joints connect entities
+    if (is_static(A) || is_static(B)) return BEPU_E_INVALID_ARG;  // joints connect entities
     JointC j;
     std::memset(&j, 0, sizeof(j));
     j.a = a;
@@ -3142,7 +3317,7 @@ int refcpu_permuted_order(int64_t n, int64_t k, int64_t count, int64_t* out) {
 int refcpu_activate(refcpu_world* w, const uint32_t* ids, uint32_t n) {
     for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
-        if (w->bodies[ids[k]].mesh >= 0) return BEPU_E_INVALID_ARG;  // a static collidable has no activity
+        if (is_static(w->bodies[ids[k]])) return BEPU_E_INVALID_ARG;  // a static collidable has no activity
     }
     for (uint32_t k = 0; k < n; ++k) activate(w->bodies[ids[k]]);
     return BEPU_OK;
@@ -3165,7 +3340,7 @@ int refcpu_set_motion_states(refcpu_world* w, const uint32_t* ids, const bepu_mo
     if (fields == 0 || (fields & ~BEPU_STATE_ALL)) return BEPU_E_INVALID_ARG;
     for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
-        if (w->bodies[ids[k]].mesh >= 0) return BEPU_E_INVALID_ARG;  // a static mesh is not an entity
+        if (is_static(w->bodies[ids[k]])) return BEPU_E_INVALID_ARG;  // a static collidable is not an entity
         const bepu_motion_state& m = st[k];
         if (((fields & BEPU_STATE_POSITION) && !finite3(m.position, 3)) ||
             ((fields & BEPU_STATE_ORIENTATION) && !finite3(m.orientation, 4)) ||
@@ -3208,7 +3383,7 @@ int refcpu_set_position_update_mode(refcpu_world* w, const uint32_t* ids, uint32
     if ((n && !ids) || mode < 0 || mode > 2) return BEPU_E_INVALID_ARG;
     for (uint32_t k = 0; k < n; ++k) {
         if (ids[k] >= w->bodies.size() || w->bodies[ids[k]].removed) return BEPU_E_NOT_FOUND;
-        if (w->bodies[ids[k]].mesh >= 0) return BEPU_E_INVALID_ARG;
+        if (is_static(w->bodies[ids[k]])) return BEPU_E_INVALID_ARG;
     }
     for (uint32_t k = 0; k < n; ++k) w->bodies[ids[k]].pmode = mode;
     if (mode && n) w->ccd = true;

@@ -1,0 +1,312 @@
+"""Pins the oracle's terrain path (TerrainContactManifold over
+TriangleMeshConvexContactManifold, TerrainShape's grid queries and triangle
+layouts, TerrainPairHandler's time of impact) against what the geometry
+fixes: overlap sets against a brute-force scan of the quads, the hand-derived
+scan order, triangle vertices and windings of both quad organizations, the
+terrain's bounding box and a convex's query box in closed form, contacts of
+a sphere on a sloped terrain, resting heights against a box ground, the same
+bodies on a terrain and on the static mesh of its triangles, a mirrored
+terrain, and continuous bodies stopping at a terrain.  The reference ships no
+golden vectors for this path.  CPU only.
+"""
+import numpy as np
+import pytest
+
+from bepuphysics1_amd import scenes
+
+DT = 1.0 / 60.0
+
+
+def _terrain_world(oracle, heights, linear, translation, organization=0, improve=True):
+    w = oracle.World()
+    sid = w.create_terrain_shape(heights, organization)
+    tid = w.add_terrain(sid, (np.asarray(linear, np.float32), np.asarray(translation, np.float32)), improve)
+    return w, tid
+
+
+def _brute_quads(h, lo, hi):
+    """Quads (i, j) whose cell [i, i+1) x [j, j+1) meets the box in x and z and
+    whose four heights' range meets it in y, for boxes with max x, z >= 0."""
+    W, L = h.shape
+    out = set()
+    for i in range(W - 1):
+        for j in range(L - 1):
+            if not (lo[0] < i + 1 and hi[0] >= i and lo[2] < j + 1 and hi[2] >= j):
+                continue
+            q = (h[i, j], h[i + 1, j], h[i, j + 1], h[i + 1, j + 1])
+            if hi[1] < min(q) or lo[1] > max(q):
+                continue
+            out.add((i, j))
+    return out
+
+
+def test_terrain_overlaps_equal_brute_force(oracle):
+    rng = np.random.default_rng(0)
+    h = rng.uniform(-1, 1, (7, 5)).astype(np.float32)
+    w, tid = _terrain_world(oracle, h, np.eye(3), (0, 0, 0))
+    for _ in range(300):
+        c = rng.uniform([-1, -1.5, -1], [8, 1.5, 6])
+        e = rng.uniform(0, [3, 1, 3])
+        lo, hi = (c - e).astype(np.float32), (c + e).astype(np.float32)
+        if hi[0] < 0 or hi[2] < 0:
+            continue
+        got = w.terrain_overlaps(tid, lo, hi)
+        assert len(got) % 2 == 0 and np.all(got[1::2] == got[0::2] | 1)
+        quads = {((q >> 1) % 7, (q >> 1) // 7) for q in got[0::2]}
+        assert quads == _brute_quads(h.astype(np.float64), lo, hi)
+
+
+def test_terrain_overlap_order_and_truncation(oracle):
+    # Columns outer, rows inner; quad (i, j) of a width-4 terrain is
+    # 2 (i + 4 j), its two triangles 2q and 2q + 1.  The bounds are truncated
+    # towards zero (TerrainShape.cs:448-452): a box ending at x = -0.5 still
+    # reaches column 0.
+    h = np.zeros((4, 4), np.float32)
+    w, tid = _terrain_world(oracle, h, np.eye(3), (0, 0, 0))
+    got = w.terrain_overlaps(tid, [0.2, -1, 0.5], [1.5, 1, 1.9])
+    assert list(got) == [0, 1, 8, 9, 2, 3, 10, 11]
+    assert list(w.terrain_overlaps(tid, [-3, -1, 0.5], [-0.5, 1, 0.7])) == [0, 1]
+    assert list(w.terrain_overlaps(tid, [0.2, 0.5, 0.2], [0.4, 1, 0.4])) == []  # above the flat quad
+    assert list(w.terrain_overlaps(tid, [5, -1, 0], [9, 1, 1])) == []  # past the last column
+
+
+@pytest.mark.parametrize("organization", [0, 1])
+def test_terrain_triangles(oracle, organization):
+    # Each quad splits along one diagonal: (c+1, r)-(c, r+1) for
+    # BottomLeftUpperRight, (c, r)-(c+1, r+1) for BottomRightUpperLeft, the
+    # right angles at the corners the enum names.  Local triangles all wind
+    # with A B x A C pointing down (-y); world triangles are the same points
+    # under the terrain's transform.
+    rng = np.random.default_rng(1)
+    h = rng.uniform(-1, 1, (4, 3)).astype(np.float32)
+    lin = np.array([[2, 0, 0.3], [0, 1, 0], [0, 0, 1.5]], np.float32)
+    tr = np.array([1, -2, 3], np.float32)
+    w, tid = _terrain_world(oracle, h, lin, tr, organization)
+    right = {0: [(0, 0), (1, 1)], 1: [(1, 0), (0, 1)]}[organization]
+    for r in range(2):
+        for c in range(3):
+            q = 2 * (c + 4 * r)
+            corners = set()
+            for k in range(2):
+                v, ix = w.terrain_triangle(tid, q + k, world=False)
+                cols, rows = v[:, 0].astype(int), v[:, 2].astype(int)
+                assert np.array_equal(v[:, 0], cols) and np.array_equal(v[:, 2], rows)
+                assert np.all((cols - c) * (cols - c - 1) == 0) and np.all((rows - r) * (rows - r - 1) == 0)
+                np.testing.assert_array_equal(v[:, 1], h[cols, rows])
+                np.testing.assert_array_equal(ix, rows * 4 + cols)
+                assert np.cross(v[1] - v[0], v[2] - v[0])[1] < 0
+                ra = (right[k][0] + c, right[k][1] + r)
+                d = [(cols[m] - ra[0], rows[m] - ra[1]) for m in range(3)]
+                assert (0, 0) in d and sum(abs(a) + abs(b) for a, b in d) == 2  # right angle there
+                corners |= set(zip(cols, rows))
+                wv, _ = w.terrain_triangle(tid, q + k, world=True)
+                exp = v.astype(np.float64) @ lin.astype(np.float64) + tr
+                for p in wv:  # the same three points, in either order
+                    assert np.min(np.abs(exp - p).max(1)) < 1e-5
+            assert len(corners) == 4
+
+
+def test_terrain_bounding_box(oracle):
+    # The extremes of the transformed samples plus the translation; with the
+    # reference's else-if (TerrainShape.cs:80-101) a sample that sets a
+    # running minimum is never tried as a maximum, so a peak at the first
+    # sample stays outside the box.
+    h = np.zeros((5, 4), np.float32)
+    h[2, 1], h[3, 3] = 1.5, -0.7
+    w, tid = _terrain_world(oracle, h, np.diag([2, 1, 3]), (1, 2, -1))
+    np.testing.assert_allclose(w.aabbs()[tid], [1, 1.3, -1, 9, 3.5, 8], atol=1e-6)
+    h2 = np.zeros((5, 4), np.float32)
+    h2[0, 0] = 1.5
+    w, tid = _terrain_world(oracle, h2, np.eye(3), (0, 0, 0))
+    np.testing.assert_allclose(w.aabbs()[tid], [0, 0, 0, 4, 0, 3], atol=1e-6)
+
+
+def test_query_box_closed_form(oracle):
+    # TerrainContactManifold.FindOverlappingTriangles: the convex's extent in
+    # the terrain's local frame, then its velocity over one step in that frame
+    # added to the side it points to.
+    h = np.zeros((8, 8), np.float32)
+    lin, tr = np.diag([2.0, 1.0, 0.5]), np.array([1.0, -1.0, 2.0])
+    w, tid = _terrain_world(oracle, h, lin, tr)
+    p, v = np.array([3.2, 1.1, 4.7]), np.array([2.0, -6.0, 1.0])
+    s = w.add_body(1, [0.5, 0, 0], 1.0, list(p) + [0, 0, 0, 1] + list(v) + [0, 0, 0], [1, 1, 0], flags=1)
+    q = np.array([0, np.sin(np.pi / 4), 0, np.cos(np.pi / 4)])
+    vb = np.array([-3.0, 4.0, -2.0])
+    b = w.add_body(0, [1, 2, 4], 1.0, list(p) + list(q) + list(vb) + [0, 0, 0], [1, 1, 0], flags=1)
+    sc = np.array([2.0, 1.0, 0.5])
+    for body, ext, vel in ((s, np.array([0.5, 0.5, 0.5]), v), (b, np.array([2.0, 1.0, 0.5]), vb)):
+        lo, hi = w.terrain_query_box(body, tid)
+        lv = vel / sc * DT
+        elo, ehi = (p - ext - tr) / sc, (p + ext - tr) / sc
+        elo += np.minimum(lv, 0)
+        ehi += np.maximum(lv, 0)
+        np.testing.assert_allclose(lo, elo, atol=2e-5)
+        np.testing.assert_allclose(hi, ehi, atol=2e-5)
+
+
+@pytest.mark.parametrize("organization,spacing", [(0, 1.0), (1, 1.0), (0, 2.0)])
+def test_sphere_on_a_sloped_terrain(oracle, organization, spacing):
+    # heights[c, r] = 0.3 c - 0.2 r: a plane y = (0.3 x - 0.2 z) / spacing in
+    # the world.  A sphere sunk 0.05 into it gets one contact with the
+    # plane's normal (towards the terrain) and depth 0.05; a terrain read
+    # with rows and columns swapped would tilt the other way.
+    c, r = np.meshgrid(np.arange(6), np.arange(6), indexing="ij")
+    h = (0.3 * c - 0.2 * r).astype(np.float32)
+    w, tid = _terrain_world(oracle, h, np.diag([spacing, 1, spacing]), (0, 0, 0), organization)
+    n = np.array([-0.3 / spacing, 1.0, 0.2 / spacing])
+    n /= np.linalg.norm(n)
+    x, z = 2.3 * spacing, 3.6 * spacing
+    foot = np.array([x, (0.3 * x - 0.2 * z) / spacing, z])
+    center = foot + n * 0.45
+    w.add_body(1, [0.5, 0, 0], 1.0, list(center) + [0, 0, 0, 1, 0, 0, 0, 0, 0, 0], [1, 1, 0])
+    w.step(1)
+    ms = [m for m in w.manifolds() if m.count]
+    assert len(ms) == 1 and ms[0].kind == 5 and ms[0].count == 1
+    ct = ms[0].contacts[0]
+    np.testing.assert_allclose(ct.normal, -n, atol=2e-6)
+    assert abs(ct.penetration_depth - 0.05) < 2e-5
+    assert np.linalg.norm(np.asarray(ct.position) - foot) < 0.05
+
+
+def _rest(oracle, ground, shape, dims, pos, steps, linear=None, vel=(0, 0, 0)):
+    w = oracle.World()
+    if ground == "terrain":
+        lin = np.diag([2.0, 1.0, 2.0]) if linear is None else linear
+        sid = w.create_terrain_shape(np.zeros((11, 11), np.float32))
+        w.add_terrain(sid, (lin, np.array([-lin[0, 0] * 5, 0, -lin[2, 2] * 5], np.float32)), material=(1, 0.8, 0))
+    else:
+        w.add_body(0, [20, 1, 20], 0, [0, -0.5, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0], [1, 0.8, 0])
+    w.add_body(shape, dims, 1.0, list(pos) + [0, 0, 0, 1] + list(vel) + [0, 0, 0], [1, 0.8, 0])
+    w.step(steps)
+    return w
+
+
+@pytest.mark.parametrize("shape,dims,h", [(0, [1, 1, 1], 0.5), (1, [0.5, 0, 0], 0.5), (3, [1.0, 0.3, 0], 0.8)])
+def test_bodies_rest_on_a_terrain_as_on_a_box_ground(oracle, shape, dims, h):
+    a = _rest(oracle, "terrain", shape, dims, (0.3, h + 0.05, 0.2), 240).motion_states()[1]
+    b = _rest(oracle, "box", shape, dims, (0.3, h + 0.05, 0.2), 240).motion_states()[1]
+    assert abs(a[1] - b[1]) < 2e-3 and abs(a[1] - (h - 0.01)) < 3e-3
+    assert np.abs(a[7:13]).max() < 0.01
+
+
+def test_sliding_on_a_terrain_as_on_a_box_ground(oracle):
+    # A box sliding at 4 m/s stops after about v0^2 / (2 mu g), mu = 0.8 x 0.8.
+    a = _rest(oracle, "terrain", 0, [1, 1, 1], (0.3, 0.5, 0.2), 90, vel=(4, 0, 0)).motion_states()[1]
+    b = _rest(oracle, "box", 0, [1, 1, 1], (0.3, 0.5, 0.2), 90, vel=(4, 0, 0)).motion_states()[1]
+    stop = 0.3 + 16 / (2 * 0.64 * 9.81)
+    assert -0.07 < a[0] - stop < 0 and abs(a[0] - b[0]) < 0.01
+
+
+def test_mirrored_terrain_still_supports_from_above(oracle):
+    # A transform that mirrors x flips the world winding of every triangle;
+    # the terrain's sidedness follows it (Terrain.cs:45-62), so bodies still
+    # rest on the top side.  A transform turned upside down (y scaled by -1)
+    # keeps its up row pointing down: then the underside is the solid one.
+    for lin in (np.diag([-2.0, 1.0, 2.0]), np.diag([2.0, 1.0, -2.0])):
+        st = _rest(oracle, "terrain", 1, [0.5, 0, 0], (0.3, 0.6, 0.2), 120, linear=lin).motion_states()[1]
+        assert abs(st[1] - 0.49) < 3e-3
+    st = _rest(oracle, "terrain", 1, [0.5, 0, 0], (0.3, 0.6, 0.2), 120,
+               linear=np.diag([2.0, -1.0, 2.0])).motion_states()[1]
+    assert st[1] < -1.0
+
+
+def _terrain_as_mesh(t):
+    """The static mesh of a terrain's triangles: world vertices and the
+    terrain's local triangle order (A B x A C down), so Counterclockwise."""
+    h = t.heights
+    W, L = h.shape
+    lin, tr = t.transform
+    cols, rows = np.meshgrid(np.arange(W), np.arange(L), indexing="xy")
+    local = np.stack([cols.ravel(), h[cols.ravel(), rows.ravel()], rows.ravel()], 1).astype(np.float32)
+    idx = []
+    for r in range(L - 1):
+        for c in range(W - 1):
+            a, b, cc, d = r * W + c, r * W + c + 1, (r + 1) * W + c, (r + 1) * W + c + 1
+            if t.organization == 0:
+                idx += [a, b, cc, cc, b, d]
+            else:
+                idx += [a, b, d, a, d, cc]
+    return local, np.asarray(idx, np.int32), (lin, tr)
+
+
+def test_bodies_on_a_terrain_as_on_its_static_mesh(oracle):
+    # Bodies spread apart over a bumpy terrain and over the static mesh of
+    # the same triangles (Counterclockwise): the two manifolds see the same
+    # triangles in the convex's frame bit for bit, only in another order (the
+    # grid scan against the tree walk), so the bodies land alike; the order
+    # of the candidates then lets them drift apart slowly.
+    from tests.helpers import oracle_hulls
+    t = scenes.heightfield_terrain(12, 12, 2.0, amplitude=1.2, seed=5)
+    V, I, xf = _terrain_as_mesh(t)
+    rng = np.random.default_rng(6)
+    bodies = []
+    for k in range(16):
+        s = (0, 1, 2, 3)[k % 4]
+        pos = (-9 + 6 * (k % 4), 2.5, -9 + 6 * (k // 4))
+        bodies.append((s, scenes._shape_dims(rng, s), pos, scenes.random_quaternions(rng, 1)[0]))
+    out = []
+    for ground in ("terrain", "mesh"):
+        w = oracle.World()
+        if ground == "terrain":
+            w.add_terrain(w.create_terrain_shape(t.heights, t.organization), t.transform)
+        else:
+            w.add_static_mesh(w.create_mesh_shape(V, I, xf), sidedness=2)
+        for s, dims, pos, q in bodies:
+            hull = oracle_hulls()[int(dims[0])] if s == 2 else None
+            w.add_body(s, dims, 1.0, list(pos) + list(q) + [0] * 6, [1, 0.8, 0], hull=hull)
+        w.step(40)
+        out.append(w.motion_states()[1:])
+        w.step(110)
+        out.append(w.motion_states()[1:])
+    np.testing.assert_allclose(out[0][:, :3], out[2][:, :3], atol=0.02)
+    np.testing.assert_allclose(out[1][:, 1], out[3][:, 1], atol=0.15)
+
+
+def test_terrain_pile_manifolds(oracle):
+    from tests.helpers import oracle_hulls
+    sc, meshes = scenes.terrain_pile(24, seed=2)
+    w = oracle.World()
+    w.add_scene(sc, oracle_hulls(), meshes)
+    w.step(180)
+    tm = [m for m in w.manifolds() if m.kind in (4, 5)]
+    assert tm
+    for m in tm:
+        assert 1 <= m.count <= 4 and m.body_b == 0
+        for i in range(m.count):
+            assert abs(np.linalg.norm(m.contacts[i].normal) - 1) < 1e-3
+            assert m.contacts[i].penetration_depth < 0.1
+    assert w.motion_states()[1:, 1].min() > -1.5
+
+
+@pytest.mark.parametrize("linear", [np.diag([2.0, 1.0, 2.0]), np.diag([-2.0, 1.0, 2.0])])
+@pytest.mark.parametrize("organization", [0, 1])
+def test_continuous_body_stops_at_the_terrain(oracle, linear, organization):
+    # A sphere (radius 0.25) falling at 180 m/s from 1 m above a flat
+    # terrain: a discrete body tunnels through; a continuous one stops with
+    # its core sphere (0.8 of its radius) touching.  The triangle normals
+    # are turned by the terrain's up direction (TerrainPairHandler.cs:
+    # 148-160), so both windings of the triangles and a mirrored transform
+    # stop it.
+    for continuous in (False, True):
+        w = oracle.World()
+        sid = w.create_terrain_shape(np.zeros((6, 6), np.float32), organization)
+        w.add_terrain(sid, (linear.astype(np.float32), np.array([-5 * linear[0, 0] / 2, 0, -5], np.float32)))
+        for x, z in ((0.3, 0.2), (-2.2, 1.7), (1.1, -3.4)):
+            w.add_body(1, [0.25, 0, 0], 1.0, [x, 1.0, z, 0, 0, 0, 1, 0, -180, 0, 0, 0, 0], [1, 0.8, 0],
+                       flags=1 | (8 if continuous else 0))
+        w.step(1)
+        y = w.motion_states()[1:, 1]
+        if continuous:
+            np.testing.assert_allclose(y, 0.2, atol=1e-3)
+        else:
+            assert np.all(y < -1.9)
+
+
+def test_terrain_argument_errors(oracle):
+    w = oracle.World()
+    with pytest.raises(ValueError):
+        w.create_terrain_shape(np.zeros((1, 4), np.float32))
+    with pytest.raises(ValueError):
+        w.create_terrain_shape(np.full((3, 3), np.nan, np.float32))
+    with pytest.raises(ValueError):
+        w.add_terrain(3, (np.eye(3), np.zeros(3)))
