@@ -125,6 +125,10 @@ EXPORTS = {
                                               ctypes.POINTER(ctypes.c_float), _u32p]),
     "bepu_add_static_mesh": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.POINTER(Material), _u32p]),
+    "bepu_create_terrain_shape": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
+                                                 ctypes.c_uint32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                                 _u32p]),
+    "bepu_add_terrain": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int32, ctypes.POINTER(Material), _u32p]),
     "bepu_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_get_motion_states": (ctypes.c_int, [_vp, ctypes.POINTER(MotionState), ctypes.c_uint32, _u32p]),
     "bepu_get_pairs": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32, _u32p]),

@@ -36,7 +36,8 @@ __device__ __forceinline__ cvx::Shape shape_of(const DeviceWorld& w, uint32_t i,
 // Shape.GetBoundingBox of a body's shape at pose (P, q), without the
 // velocity expansion: BoxShape.cs:149-174, SphereShape.cs:75-86,
 // CapsuleShape.cs:95-137, ConvexHullShape.cs:206-286; a static mesh's box is
-// its tree's root box (StaticMesh.UpdateBoundingBox, StaticMesh.cs:189-192).
+// its tree's root box (StaticMesh.UpdateBoundingBox, StaticMesh.cs:189-192), a
+// terrain's the box its shape was made with.
 __device__ __forceinline__ void shape_aabb(const DeviceWorld& w, float4 sh, f3 P, q4 q, f3* mn, f3* mx) {
     if (sh.w == (float)BEPU_SHAPE_BOX) {
         m3 o = mat_from_quat(q);
@@ -102,6 +103,11 @@ __device__ __forceinline__ void shape_aabb(const DeviceWorld& w, float4 sh, f3 P
         const uint32_t nbase = __float_as_uint(w.mesh_info[4 * __float_as_uint(sh.x) + 2].w);
         *mn = f3_of(w.mesh_nodes[2 * (size_t)nbase]);
         *mx = f3_of(w.mesh_nodes[2 * (size_t)nbase + 1]);
+    } else if (sh.w == (float)BEPU_SHAPE_TERRAIN) {
+        // Terrain.UpdateBoundingBox (Terrain.cs:195-212): fixed, made with the shape.
+        const float4* r = w.terrain_info + 6 * (size_t)__float_as_uint(sh.x);
+        *mn = f3_of(r[4]);
+        *mx = f3_of(r[5]);
     } else {
         float r = sh.x;
         *mn = f3{P.x - r, P.y - r, P.z - r};

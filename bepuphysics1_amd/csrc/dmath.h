@@ -130,6 +130,23 @@ BEPU_HD m3 mat_mul(const m3& a, const m3& b) {
     r.m33 = a.m31 * b.m13 + a.m32 * b.m23 + a.m33 * b.m33;
     return r;
 }
+// Matrix3x3.Invert (Matrix3x3.cs:505-530) with Determinant (:1231-1235)
+BEPU_HD m3 mat_invert(const m3& m) {
+    const float det = m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 -
+                      m.m31 * m.m22 * m.m13 - m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
+    const float d = 1 / det;
+    m3 r;
+    r.m11 = (m.m22 * m.m33 - m.m23 * m.m32) * d;
+    r.m12 = (m.m13 * m.m32 - m.m33 * m.m12) * d;
+    r.m13 = (m.m12 * m.m23 - m.m22 * m.m13) * d;
+    r.m21 = (m.m23 * m.m31 - m.m21 * m.m33) * d;
+    r.m22 = (m.m11 * m.m33 - m.m13 * m.m31) * d;
+    r.m23 = (m.m13 * m.m21 - m.m11 * m.m23) * d;
+    r.m31 = (m.m21 * m.m32 - m.m22 * m.m31) * d;
+    r.m32 = (m.m12 * m.m31 - m.m11 * m.m32) * d;
+    r.m33 = (m.m11 * m.m22 - m.m12 * m.m21) * d;
+    return r;
+}
 // Matrix3x3.TransformTranspose (Matrix3x3.cs:1113-1125)
 BEPU_HD f3 mat_transform_t(f3 v, const m3& m) {
     return f3{v.x * m.m11 + v.y * m.m12 + v.z * m.m13, v.x * m.m21 + v.y * m.m22 + v.z * m.m23,

@@ -73,21 +73,70 @@ __device__ __forceinline__ float pair_toi(const DeviceWorld& w, uint32_t a, uint
 }
 
 // StaticMeshPairHandler.UpdateTimeOfImpact (StaticMeshPairHandler.cs:95-159)
-// for an active continuous convex a (the mesh m never moves): its core
-// sphere swept along its motion against each triangle its box overlaps (the
-// manifold's overlapped triangles of this step), each triangle moved by the
+// and TerrainPairHandler.UpdateTimeOfImpact (TerrainPairHandler.cs:117-176)
+// for an active continuous convex a (the mesh or terrain m never moves): its
+// core sphere swept along its motion against each triangle of the manifold's
+// overlapped triangles (a mesh: those its box overlaps; a terrain: those of
+// the query box of its manifold's last update), each triangle moved by the
 // convex's position; a hit further than Toolbox.BigEpsilon (on a one-sided
-// mesh only from the side the mesh keeps) shortens the time of impact.
-__device__ float mesh_toi(const DeviceWorld& w, uint32_t a, uint32_t m, float toi) {
+// mesh only from the side the mesh keeps; on a terrain only against its up
+// side) shortens the time of impact.
+__device__ float mesh_toi(const DeviceWorld& w, uint32_t a, uint32_t m, uint32_t slot, float toi) {
     const f3 velocity = f3_of(w.linv[a]) * w.s.dt;
     const float vsq = lensq3(velocity);
     const float minimumRadius = w.minrad[a] * w.s.core_shape_scaling;
     toi = 1;
     if (!(minimumRadius * minimumRadius < vsq)) return toi;
     const float4 msh = w.shape[m];
-    const MeshInfo mi = mesh_info(w, __float_as_uint(msh.x));
     const int sided = (int)msh.y;
     const f3 pa = f3_of(w.pos[a]);
+    const cvx::RT identity{f3{0, 0, 0}, q4{0, 0, 0, 1}};
+    // One triangle in world space, moved by the convex's position; `up` the
+    // terrain's up direction (a mesh: unused).
+    auto cast = [&](cvx::TriShape& t, bool terrain, f3 up) {
+        t.v0 = t.v0 - pa;
+        t.v1 = t.v1 - pa;
+        t.v2 = t.v2 - pa;
+        t.margin = 0;
+        float hitT;
+        f3 hitN;
+        if (cvx::ccd_sphere_cast(f3{0, 0, 0}, velocity, minimumRadius, t, identity, toi, w.s.core_shape_scaling, hitT,
+                                 &hitN) &&
+            hitT > cvx::kBigEpsilon) {
+            if (terrain) {
+                // TerrainPairHandler.UpdateTimeOfImpact (TerrainPairHandler.cs:148-170):
+                // the normal AC x AB turned to the terrain's up side.
+                const f3 n = cross3(t.v2 - t.v0, t.v1 - t.v0);
+                float d = dot3(n, up);
+                if (d < 0) d = dot3(n, hitN);
+                else d = -dot3(n, hitN);
+                if (d < 0) toi = hitT;
+            } else if (sided != SIDED_DOUBLE) {
+                const float d = dot3(cross3(t.v1 - t.v0, t.v2 - t.v0), hitN);
+                if ((sided == SIDED_COUNTERCLOCKWISE && d < 0) || (sided == SIDED_CLOCKWISE && d > 0)) toi = hitT;
+            } else {
+                toi = hitT;
+            }
+        }
+    };
+    if (msh.w == (float)BEPU_SHAPE_TERRAIN) {
+        // The triangles the manifold's last update found: its query box.
+        const float4 lo = w.conv_cur[slot].sep, hi = w.conv_cur[slot].dir;
+        if (lo.w == 0.0f) return toi;
+        const TerrainInfo ti = terrain_info(w, __float_as_uint(msh.x));
+        const f3 up{ti.m.m21, ti.m.m22, ti.m.m23};
+        terrain_overlaps(w, ti, f3_of(lo), f3_of(hi), [&](int tri) {
+            f3 v[3];
+            terrain_world_triangle(w, ti, tri, v);
+            cvx::TriShape t;
+            t.v0 = v[0];
+            t.v1 = v[1];
+            t.v2 = v[2];
+            cast(t, true, up);
+        });
+        return toi;
+    }
+    const MeshInfo mi = mesh_info(w, __float_as_uint(msh.x));
     const f3 qmin = f3_of(w.bmin[a]), qmax = f3_of(w.bmax[a]);
     const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
     auto overlaps = [&](int node) {
@@ -96,7 +145,6 @@ __device__ float mesh_toi(const DeviceWorld& w, uint32_t a, uint32_t m, float to
         if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
         return true;
     };
-    const cvx::RT identity{f3{0, 0, 0}, q4{0, 0, 0, 1}};
     int stack[kMeshStack];
     int top = 0;
     if (mi.ntri > 0 && overlaps(0)) stack[top++] = 0;
@@ -110,22 +158,10 @@ __device__ float mesh_toi(const DeviceWorld& w, uint32_t a, uint32_t m, float to
         }
         cvx::TriShape t;
         const int32_t* ix = w.mesh_idx + mi.ibase + cb;
-        t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), mi.m, mi.t) - pa;
-        t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), mi.m, mi.t) - pa;
-        t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), mi.m, mi.t) - pa;
-        t.margin = 0;
-        float hitT;
-        f3 hitN;
-        if (cvx::ccd_sphere_cast(f3{0, 0, 0}, velocity, minimumRadius, t, identity, toi, w.s.core_shape_scaling, hitT,
-                                 &hitN) &&
-            hitT > cvx::kBigEpsilon) {
-            if (sided != SIDED_DOUBLE) {
-                const float d = dot3(cross3(t.v1 - t.v0, t.v2 - t.v0), hitN);
-                if ((sided == SIDED_COUNTERCLOCKWISE && d < 0) || (sided == SIDED_CLOCKWISE && d > 0)) toi = hitT;
-            } else {
-                toi = hitT;
-            }
-        }
+        t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), mi.m, mi.t);
+        t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), mi.m, mi.t);
+        t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), mi.m, mi.t);
+        cast(t, false, f3{0, 0, 0});
     }
     return toi;
 }
@@ -143,7 +179,7 @@ __global__ void __launch_bounds__(256) k_toi(DeviceWorld w) {
             // A mesh pair (the convex is A): requested by an active continuous convex.
             toi = w.pairs_cur[k].toi;
             if ((fa & BF_CONTINUOUS) && ma) {
-                toi = mesh_toi(w, h.a, h.b, toi);
+                toi = mesh_toi(w, h.a, h.b, k, toi);
                 w.pairs_cur[k].toi = toi;
             }
             const uint32_t bits = __float_as_uint(toi);

@@ -1,18 +1,22 @@
-// k_mesh.hip -- narrowphase of static mesh pairs: StaticMeshContactManifold
-// over TriangleMeshConvexContactManifold.Update
+// k_mesh.hip -- narrowphase of static mesh and terrain pairs:
+// StaticMeshContactManifold and TerrainContactManifold over
+// TriangleMeshConvexContactManifold.Update
 // (CollisionTests/Manifolds/TriangleMeshConvexContactManifold.cs:101-434,
-// StaticMeshContactManifold.cs:33-71) for the pairs k_pair_prep listed.
+// StaticMeshContactManifold.cs:33-71, TerrainContactManifold.cs:30-100) for
+// the pairs k_pair_prep listed.
 //
 // One thread per pair: the contact refresh, the depth-first walk of the
 // mesh's MeshBoundingBoxTree over the convex's box (child A's subtree before
-// child B's, MeshBoundingBoxTree.cs:112-122, 214-226), each overlapped
+// child B's, MeshBoundingBoxTree.cs:112-122, 214-226) or the scan of the
+// terrain's grid under the convex's box in its frame, each overlapped
 // triangle carried into the convex's local space, pruned by the convex's
 // local box and tested by its TriangleConvexPairTester (kept per triangle in
 // the pair's tester table) or the TriangleSpherePairTester, the improved
 // boundary handling, the removal of stale testers and the contact reduction.
-// The manifold's own transform is the identity: the mesh's triangles reach
-// world space through its shape's affine transform.
+// The manifold's own transform is the identity: the triangles reach world
+// space through the mesh shape's (or the terrain's) affine transform.
 #include "body_shape.h"
+#include "cast.h"
 #include "manifold.h"
 #include "mesh.h"
 
@@ -263,16 +267,30 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
     for (int i = nrem - 1; i >= 0; i--) remove_contact_inplace(s, rem[i]);
 
     const float4 msh = w.shape[M];
-    const MeshInfo mi = mesh_info(w, __float_as_uint(msh.x));
+    const bool terrain = msh.w == (float)BEPU_SHAPE_TERRAIN;
     const int sided = (int)msh.y;
     const bool improve = msh.z != 0.0f;
+    // The shape's world transform: the mesh shape's, or the terrain's.
+    m3 sm;
+    f3 st;
+    MeshInfo mi;
+    TerrainInfo ti;
+    if (terrain) {
+        ti = terrain_info(w, __float_as_uint(msh.x));
+        sm = ti.m;
+        st = ti.t;
+    } else {
+        mi = mesh_info(w, __float_as_uint(msh.x));
+        sm = mi.m;
+        st = mi.t;
+    }
     // RigidTransform.Invert (RigidTransform.cs:95-100), AffineTransform.CreateFromRigidTransform
     // (AffineTransform.cs:232-236), AffineTransform.Multiply (AffineTransform.cs:186-194).
     const q4 iq = qconj(ta.q);
     const f3 ip = -qrotate(ta.pos, iq);
     const m3 im = mat_from_quat(iq);
-    const m3 lm = mat_mul(mi.m, im);
-    const f3 lt = mat_transform(mi.t, im) + ip;
+    const m3 lm = mat_mul(sm, im);
+    const f3 lt = mat_transform(st, im) + ip;
     f3 lmin, lmax;
     shape_aabb(w, w.shape[A], f3{0, 0, 0}, q4{0, 0, 0, 1}, &lmin, &lmax);
     const m3 orientation = mat_from_quat(ta.q);
@@ -289,46 +307,16 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
     m.ncand = m.nedges = m.nverts = m.nbv = m.nbe = 0;
     m.overflow = false;
 
-    // MeshBoundingBoxTree.GetOverlaps over the convex's box, depth first.
-    const f3 qmin = f3_of(w.bmin[A]), qmax = f3_of(w.bmax[A]);
-    const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
-    auto overlaps = [&](int node) {
-        const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
-        if (qmin.x > hi.x || qmin.y > hi.y || qmin.z > hi.z) return false;
-        if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
-        return true;
-    };
-    int stack[kMeshStack];
-    int top = 0;
-    if (mi.ntri > 0 && overlaps(0)) stack[top++] = 0;
-    while (top > 0) {
-        const int node = stack[--top];
-        const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
-        if (a >= 0) {
-            if (overlaps(b)) {
-                if (top < kMeshStack) stack[top++] = b;
-                else m.overflow = true;
-            }
-            if (overlaps(a)) {
-                if (top < kMeshStack) stack[top++] = a;
-                else m.overflow = true;
-            }
-            continue;
-        }
-        // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66): leaf b
-        // is the triangle's first index position.
-        const int ix[3] = {w.mesh_idx[mi.ibase + b], w.mesh_idx[mi.ibase + b + 1], w.mesh_idx[mi.ibase + b + 2]};
-        cvx::TriShape t;
-        t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), lm, lt);
-        t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), lm, lt);
-        t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), lm, lt);
-        t.margin = 0;
+    // One overlapped triangle, its vertices in the convex's local space and
+    // its TriangleIndices in ix: the local box prune, its pair tester and the
+    // boundary analysis (TriangleMeshConvexContactManifold.cs:135-236).
+    auto triangle = [&](const int ix[3], const cvx::TriShape& t) {
         f3 tmin, tmax;
         tri_axis(t.v0.x, t.v1.x, t.v2.x, &tmin.x, &tmax.x);
         tri_axis(t.v0.y, t.v1.y, t.v2.y, &tmin.y, &tmax.y);
         tri_axis(t.v0.z, t.v1.z, t.v2.z, &tmin.z, &tmax.z);
-        if (tmin.x > lmax.x || tmin.y > lmax.y || tmin.z > lmax.z) continue;
-        if (lmin.x > tmax.x || lmin.y > tmax.y || lmin.z > tmax.z) continue;
+        if (tmin.x > lmax.x || tmin.y > lmax.y || tmin.z > lmax.z) return;
+        if (lmin.x > tmax.x || lmin.y > tmax.y || lmin.z > tmax.z) return;
         MeshContact out[2];
         int n = 0, region_sphere = VR_ABC;
         bool should_correct = false;
@@ -340,16 +328,16 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
             if (k == ntest) {
                 if (ntest == kMeshTesters) {
                     m.overflow = true;
-                    continue;
+                    return;
                 }
                 tab[k] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
                 ++ntest;
             }
             updated |= 1u << k;
-            MeshTester st = tab[k];
-            n = triangle_convex(cv, t, sided, st, out);
-            tab[k] = st;
-            should_correct = st.state == TT_DEEP;
+            MeshTester tst = tab[k];
+            n = triangle_convex(cv, t, sided, tst, out);
+            tab[k] = tst;
+            should_correct = tst.state == TT_DEEP;
         }
         for (int j = 0; j < n; j++) {
             if (improve) {
@@ -361,6 +349,97 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
             } else {
                 add_local(w, s, e, m, out[j], orientation, ta);
             }
+        }
+    };
+
+    if (terrain) {
+        // TerrainContactManifold.FindOverlappingTriangles (TerrainContactManifold.cs:30-64):
+        // ConvexShape.GetLocalBoundingBox (ConvexShape.cs:225-281) of the
+        // convex in the terrain's frame (AffineTransform.Invert,
+        // AffineTransform.cs:173-178; Multiply(RigidTransform,
+        // AffineTransform), :202-211; EntityShape.TransformLocalExtremePoints,
+        // EntityShape.cs:58-63), grown by its velocity over the step.
+        const m3 inv = mat_invert(ti.m);
+        const f3 inv_t = -mat_transform(ti.t, inv);
+        const m3 bm = mat_mul(orientation, inv);
+        const f3 bt = mat_transform(ta.pos, inv) + inv_t;
+        const f3 right = cvx::local_extreme_with_margin(cv.shape, f3{bm.m11, bm.m21, bm.m31});
+        const f3 left = cvx::local_extreme_with_margin(cv.shape, f3{-bm.m11, -bm.m21, -bm.m31});
+        const f3 up = cvx::local_extreme_with_margin(cv.shape, f3{bm.m12, bm.m22, bm.m32});
+        const f3 down = cvx::local_extreme_with_margin(cv.shape, f3{-bm.m12, -bm.m22, -bm.m32});
+        const f3 backward = cvx::local_extreme_with_margin(cv.shape, f3{bm.m13, bm.m23, bm.m33});
+        const f3 forward = cvx::local_extreme_with_margin(cv.shape, f3{-bm.m13, -bm.m23, -bm.m33});
+        const f3 pos{right.x * bm.m11 + right.y * bm.m21 + right.z * bm.m31,
+                     up.x * bm.m12 + up.y * bm.m22 + up.z * bm.m32,
+                     backward.x * bm.m13 + backward.y * bm.m23 + backward.z * bm.m33};
+        const f3 neg{left.x * bm.m11 + left.y * bm.m21 + left.z * bm.m31,
+                     down.x * bm.m12 + down.y * bm.m22 + down.z * bm.m32,
+                     forward.x * bm.m13 + forward.y * bm.m23 + forward.z * bm.m33};
+        f3 qmax{bt.x + pos.x, bt.y + pos.y, bt.z + pos.z};
+        f3 qmin{bt.x + neg.x, bt.y + neg.y, bt.z + neg.z};
+        f3 tv = mat_transform(f3_of(w.linv[A]), mat_invert(ti.m));
+        tv = tv * w.s.dt;
+        if (tv.x > 0) qmax.x += tv.x; else qmin.x += tv.x;
+        if (tv.y > 0) qmax.y += tv.y; else qmin.y += tv.y;
+        if (tv.z > 0) qmax.z += tv.z; else qmin.z += tv.z;
+        // Kept for the time of impact, which walks the same triangles.
+        e.sep = f4_of(qmin, 1.0f);
+        e.dir = f4_of(qmax, 0.0f);
+        terrain_overlaps(w, ti, qmin, qmax, [&](int tri) {
+            // ConfigureLocalTriangle (TerrainContactManifold.cs:76-100):
+            // TerrainVertexIndices.ToSequentialIndex for the indices.
+            int col[3], row[3], ix[3];
+            terrain_local_indices(ti, tri, col, row);
+            f3 v[3];
+            for (int k = 0; k < 3; ++k) {
+                ix[k] = row[k] * (int)ti.width + col[k];
+                v[k] = affine_transform(f3{(float)col[k], terrain_height(w, ti, col[k], row[k]), (float)row[k]}, lm,
+                                        lt);
+            }
+            cvx::TriShape t;
+            t.v0 = v[0];
+            t.v1 = v[1];
+            t.v2 = v[2];
+            t.margin = 0;
+            triangle(ix, t);
+        });
+    } else {
+        // MeshBoundingBoxTree.GetOverlaps over the convex's box, depth first.
+        const f3 qmin = f3_of(w.bmin[A]), qmax = f3_of(w.bmax[A]);
+        const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
+        auto overlaps = [&](int node) {
+            const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
+            if (qmin.x > hi.x || qmin.y > hi.y || qmin.z > hi.z) return false;
+            if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
+            return true;
+        };
+        int stack[kMeshStack];
+        int top = 0;
+        if (mi.ntri > 0 && overlaps(0)) stack[top++] = 0;
+        while (top > 0) {
+            const int node = stack[--top];
+            const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
+            if (a >= 0) {
+                if (overlaps(b)) {
+                    if (top < kMeshStack) stack[top++] = b;
+                    else m.overflow = true;
+                }
+                if (overlaps(a)) {
+                    if (top < kMeshStack) stack[top++] = a;
+                    else m.overflow = true;
+                }
+                continue;
+            }
+            // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66): leaf b
+            // is the triangle's first index position.
+            const int ix[3] = {w.mesh_idx[mi.ibase + b], w.mesh_idx[mi.ibase + b + 1],
+                               w.mesh_idx[mi.ibase + b + 2]};
+            cvx::TriShape t;
+            t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), lm, lt);
+            t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), lm, lt);
+            t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), lm, lt);
+            t.margin = 0;
+            triangle(ix, t);
         }
     }
 

@@ -603,9 +603,11 @@ __device__ __forceinline__ void init_pair(const DeviceWorld& w, uint64_t key, Pa
     s.toi = 1.0f;
     s.a = i;
     s.b = j;
-    if (si == BEPU_SHAPE_STATIC_MESH || sj == BEPU_SHAPE_STATIC_MESH) {
-        // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79): A is the convex.
-        if (si == BEPU_SHAPE_STATIC_MESH) {
+    const bool mi = si == BEPU_SHAPE_STATIC_MESH || si == BEPU_SHAPE_TERRAIN;
+    if (mi || sj == BEPU_SHAPE_STATIC_MESH || sj == BEPU_SHAPE_TERRAIN) {
+        // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79),
+        // TerrainPairHandler.Initialize (TerrainPairHandler.cs:70-93): A is the convex.
+        if (mi) {
             s.a = j;
             s.b = i;
         }

@@ -59,6 +59,105 @@ BEPU_D MeshInfo mesh_info(const DeviceWorld& w, uint32_t shape) {
 // AffineTransform.Transform (BEPUutilities/AffineTransform.cs:148-152)
 BEPU_D f3 affine_transform(f3 p, const m3& m, f3 t) { return mat_transform(p, m) + t; }
 
+// A terrain shape's record (6 float4 in terrain_info).
+struct TerrainInfo {
+    m3 m;
+    f3 t;
+    uint32_t hbase, width, length, organization;
+};
+BEPU_D TerrainInfo terrain_info(const DeviceWorld& w, uint32_t shape) {
+    const float4* r = w.terrain_info + 6 * (size_t)shape;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    TerrainInfo i;
+    i.m = m3{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r2.x, r2.y, r2.z};
+    i.t = f3_of(r3);
+    i.hbase = __float_as_uint(r0.w);
+    i.width = __float_as_uint(r1.w);
+    i.length = __float_as_uint(r2.w);
+    i.organization = __float_as_uint(r3.w);
+    return i;
+}
+BEPU_D float terrain_height(const DeviceWorld& w, const TerrainInfo& ti, int column, int row) {
+    return w.terrain_heights[ti.hbase + (size_t)column * ti.length + row];
+}
+
+// TerrainShape.GetLocalIndices (TerrainShape.cs:626-663): the (column, row)
+// of the three vertices of triangle `tri` (2 per quad, quads row by row).
+BEPU_D void terrain_local_indices(const TerrainInfo& ti, int tri, int col[3], int row[3]) {
+    const int quad = tri / 2;
+    const int r = quad / (int)ti.width;
+    const int c = quad - r * (int)ti.width;
+    const bool blur = ti.organization == 0;  // BottomLeftUpperRight
+    if ((tri & 1) == 0) {
+        col[0] = c; row[0] = r;
+        col[1] = c + 1; row[1] = r;
+        col[2] = blur ? c : c + 1; row[2] = r + 1;
+    } else if (blur) {
+        col[0] = c; row[0] = r + 1;
+        col[1] = c + 1; row[1] = r;
+        col[2] = c + 1; row[2] = r + 1;
+    } else {
+        col[0] = c; row[0] = r;
+        col[1] = c + 1; row[1] = r + 1;
+        col[2] = c; row[2] = r + 1;
+    }
+}
+
+// TerrainShape.GetPosition (TerrainShape.cs:432-452): clamped indices, world space.
+BEPU_D f3 terrain_position(const DeviceWorld& w, const TerrainInfo& ti, int c, int r) {
+    if (c <= 0) c = 0;
+    else if (c >= (int)ti.width) c = (int)ti.width - 1;
+    if (r <= 0) r = 0;
+    else if (r >= (int)ti.length) r = (int)ti.length - 1;
+    return affine_transform(f3{(float)c, terrain_height(w, ti, c, r), (float)r}, ti.m, ti.t);
+}
+
+// TerrainShape.GetTriangle (TerrainShape.cs:607-622) with GetFirstTriangle /
+// GetSecondTriangle (:557-604): the world-space triangle.
+BEPU_D void terrain_world_triangle(const DeviceWorld& w, const TerrainInfo& ti, int tri, f3 v[3]) {
+    const int quad = tri / 2;
+    const int r = quad / (int)ti.width;
+    const int c = quad - r * (int)ti.width;
+    const bool blur = ti.organization == 0;
+    if ((tri & 1) == 0) {
+        v[0] = terrain_position(w, ti, c, r);
+        v[1] = terrain_position(w, ti, c + 1, r);
+        v[2] = blur ? terrain_position(w, ti, c, r + 1) : terrain_position(w, ti, c + 1, r + 1);
+    } else if (blur) {
+        v[0] = terrain_position(w, ti, c, r + 1);
+        v[1] = terrain_position(w, ti, c + 1, r + 1);
+        v[2] = terrain_position(w, ti, c + 1, r);
+    } else {
+        v[0] = terrain_position(w, ti, c, r);
+        v[1] = terrain_position(w, ti, c, r + 1);
+        v[2] = terrain_position(w, ti, c + 1, r + 1);
+    }
+}
+
+// TerrainShape.GetOverlaps (TerrainShape.cs:497-543): the quads whose cells
+// the local box covers (bounds truncated towards zero) and whose four
+// heights reach its vertical range, columns outer, rows inner; `visit(tri)`
+// receives both triangles of each (2q, 2q + 1 for q = 2 (i + j width)).
+template <class F>
+BEPU_D void terrain_overlaps(const DeviceWorld& w, const TerrainInfo& ti, f3 lmin, f3 lmax, F&& visit) {
+    const int width = (int)ti.width;
+    const int minX = max((int)lmin.x, 0), minY = max((int)lmin.z, 0);
+    const int maxX = min((int)lmax.x, width - 2), maxY = min((int)lmax.z, (int)ti.length - 2);
+    for (int i = minX; i <= maxX; i++)
+        for (int j = minY; j <= maxY; j++) {
+            const float y1 = terrain_height(w, ti, i, j), y2 = terrain_height(w, ti, i + 1, j),
+                        y3 = terrain_height(w, ti, i, j + 1), y4 = terrain_height(w, ti, i + 1, j + 1);
+            float highest = y1, lowest = y1;
+            if (y2 > highest) highest = y2; else if (y2 < lowest) lowest = y2;
+            if (y3 > highest) highest = y3; else if (y3 < lowest) lowest = y3;
+            if (y4 > highest) highest = y4; else if (y4 < lowest) lowest = y4;
+            if (lmax.y < lowest || lmin.y > highest) continue;
+            const int quad = (i + j * width) << 1;
+            visit(quad);
+            visit(quad | 1);
+        }
+}
+
 // Toolbox.GetTriangleBoundingBox (BEPUutilities/Toolbox.cs:1516-1580), one axis.
 BEPU_D void tri_axis(float a, float b, float c, float* mn, float* mx) {
     if (a > b && a > c) {

@@ -341,23 +341,6 @@ BEPU_D m3 mat_sub(const m3& a, const m3& b) {
     return m3{a.m11 - b.m11, a.m12 - b.m12, a.m13 - b.m13, a.m21 - b.m21, a.m22 - b.m22,
               a.m23 - b.m23, a.m31 - b.m31, a.m32 - b.m32, a.m33 - b.m33};
 }
-// Matrix3x3.Invert (Matrix3x3.cs:505-530) with Determinant (:1231-1235)
-BEPU_D m3 mat_invert(const m3& m) {
-    const float det = m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 -
-                      m.m31 * m.m22 * m.m13 - m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
-    const float d = 1 / det;
-    m3 r;
-    r.m11 = (m.m22 * m.m33 - m.m23 * m.m32) * d;
-    r.m12 = (m.m13 * m.m32 - m.m33 * m.m12) * d;
-    r.m13 = (m.m12 * m.m23 - m.m22 * m.m13) * d;
-    r.m21 = (m.m23 * m.m31 - m.m21 * m.m33) * d;
-    r.m22 = (m.m11 * m.m33 - m.m13 * m.m31) * d;
-    r.m23 = (m.m13 * m.m21 - m.m11 * m.m23) * d;
-    r.m31 = (m.m21 * m.m32 - m.m22 * m.m31) * d;
-    r.m32 = (m.m12 * m.m31 - m.m11 * m.m32) * d;
-    r.m33 = (m.m11 * m.m22 - m.m12 * m.m21) * d;
-    return r;
-}
 
 // Solver.UnsafePrestep -> BallSocketJoint.Update (Solver.cs:265-275,
 // BallSocketJoint.cs:229-300): world offsets of the anchors, the softened

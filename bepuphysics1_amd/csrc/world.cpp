@@ -174,7 +174,8 @@ bepu_status bepu_world_destroy(bepu_world w) {
     }
     if (w->d.hull_verts) (void)hipFree(w->d.hull_verts);
     for (void* p : {(void*)w->d.mesh_info, (void*)w->d.mesh_verts, (void*)w->d.mesh_idx, (void*)w->d.mesh_nodes,
-                    (void*)w->d.mtest_cur, (void*)w->d.mtest_old, (void*)w->d.mesh_work})
+                    (void*)w->d.mtest_cur, (void*)w->d.mtest_old, (void*)w->d.mesh_work, (void*)w->d.terrain_info,
+                    (void*)w->d.terrain_heights})
         if (p) (void)hipFree(p);
     for (auto& g : w->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -651,6 +652,20 @@ static float bits_float(uint32_t v) {
     return f;
 }
 
+// The first mesh or terrain shape brings the mesh pairs' tables (per pair
+// slot, double-buffered) and their work list.
+static bepu_status mesh_tables(bepu_world w) {
+    DeviceWorld& d = w->d;
+    if (d.meshes) return BEPU_OK;
+    const size_t P = std::max<size_t>(d.pair_cap, 1);
+    if (hipMalloc(&d.mtest_cur, P * kMeshTesters * sizeof(MeshTester)) != hipSuccess ||
+        hipMalloc(&d.mtest_old, P * kMeshTesters * sizeof(MeshTester)) != hipSuccess ||
+        hipMalloc(&d.mesh_work, P * sizeof(uint2)) != hipSuccess)
+        return BEPU_E_OUT_OF_MEMORY;
+    d.meshes = 1;
+    return BEPU_OK;
+}
+
 bepu_status bepu_create_mesh_shape(bepu_world w, const float* vertices, uint32_t nvertices, const int32_t* indices,
                                    uint32_t nindices, const float* transform, uint32_t* shape_id) {
     if (!w || !vertices || !indices || nvertices == 0 || nindices == 0 || nindices % 3 != 0 || nvertices > (1u << 28) ||
@@ -673,18 +688,10 @@ bepu_status bepu_create_mesh_shape(bepu_world w, const float* vertices, uint32_t
     build_mesh_tree(verts.data(), indices, nindices, lin, tr, &nodes, &depth);
     // The device walk keeps one pending node per level.
     if (depth > (uint32_t)kMeshStack) return BEPU_E_CAPACITY;
-    // The first mesh shape brings the mesh pairs' tables (per pair slot,
-    // double-buffered) and their work list.
     DeviceWorld& d = w->d;
     if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
-    if (!d.meshes) {
-        const size_t P = std::max<size_t>(d.pair_cap, 1);
-        if (hipMalloc(&d.mtest_cur, P * kMeshTesters * sizeof(MeshTester)) != hipSuccess ||
-            hipMalloc(&d.mtest_old, P * kMeshTesters * sizeof(MeshTester)) != hipSuccess ||
-            hipMalloc(&d.mesh_work, P * sizeof(uint2)) != hipSuccess)
-            return BEPU_E_OUT_OF_MEMORY;
-        d.meshes = 1;
-    }
+    bepu_status st = mesh_tables(w);
+    if (st != BEPU_OK) return st;
     const uint32_t vbase = (uint32_t)w->mesh_verts_host.size(), ibase = (uint32_t)w->mesh_idx_host.size(),
                    nbase = (uint32_t)(w->mesh_nodes_host.size() / 2);
     for (uint32_t i = 0; i < nvertices; ++i)
@@ -698,7 +705,7 @@ bepu_status bepu_create_mesh_shape(bepu_world w, const float* vertices, uint32_t
     w->mesh_info_host.push_back(make_float4(lin.m21, lin.m22, lin.m23, bits_float(ibase)));
     w->mesh_info_host.push_back(make_float4(lin.m31, lin.m32, lin.m33, bits_float(nbase)));
     w->mesh_info_host.push_back(f4_of(tr, bits_float(nindices / 3)));
-    bepu_status st = mirror(w->mesh_verts_host, &d.mesh_verts);
+    st = mirror(w->mesh_verts_host, &d.mesh_verts);
     if (st == BEPU_OK) st = mirror(w->mesh_idx_host, &d.mesh_idx);
     if (st == BEPU_OK) st = mirror(w->mesh_nodes_host, &d.mesh_nodes);
     if (st == BEPU_OK) st = mirror(w->mesh_info_host, &d.mesh_info);
@@ -735,6 +742,102 @@ bepu_status bepu_add_static_mesh(bepu_world w, uint32_t shape_id, int32_t sidedn
     if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
     HostBody b;
     bepu_status st = mesh_body(w, shape_id, sidedness, improve_boundary, material, b);
+    if (st != BEPU_OK) return st;
+    return stage_body(w, b, id);
+}
+
+// TerrainShape.GetBoundingBox (TerrainShape.cs:86-140) under the terrain's
+// transform, with the reference's else-if (a sample that lowers a running
+// minimum is not tried as a maximum), then Terrain.UpdateBoundingBox's
+// thickness offset (Terrain.cs:195-212) for a thickness of zero.
+static void terrain_box(const float* h, uint32_t width, uint32_t length, const m3& lin, f3 tr, f3* mn, f3* mx) {
+    float minX = FLT_MAX, maxX = -FLT_MAX, minY = FLT_MAX, maxY = -FLT_MAX, minZ = FLT_MAX, maxZ = -FLT_MAX;
+    f3 minXv{0, 0, 0}, maxXv{0, 0, 0}, minYv{0, 0, 0}, maxYv{0, 0, 0}, minZv{0, 0, 0}, maxZv{0, 0, 0};
+    for (uint32_t i = 0; i < width; i++)
+        for (uint32_t j = 0; j < length; j++) {
+            const f3 v = mat_transform(f3{(float)i, h[(size_t)i * length + j], (float)j}, lin);
+            if (v.x < minX) { minX = v.x; minXv = v; } else if (v.x > maxX) { maxX = v.x; maxXv = v; }
+            if (v.y < minY) { minY = v.y; minYv = v; } else if (v.y > maxY) { maxY = v.y; maxYv = v; }
+            if (v.z < minZ) { minZ = v.z; minZv = v; } else if (v.z > maxZ) { maxZ = v.z; maxZv = v; }
+        }
+    *mn = f3{minXv.x + tr.x, minYv.y + tr.y, minZv.z + tr.z};
+    *mx = f3{maxXv.x + tr.x, maxYv.y + tr.y, maxZv.z + tr.z};
+    // Vector3.Normalize(Down) * thickness (Vector3.Normalize takes its
+    // reciprocal square root in double).
+    const f3 down{-lin.m21, -lin.m22, -lin.m23};
+    const float inv = (float)(1.0 / std::sqrt((double)(down.x * down.x + down.y * down.y + down.z * down.z)));
+    const f3 off{down.x * inv * 0.0f, down.y * inv * 0.0f, down.z * inv * 0.0f};
+    if (off.x < 0) mn->x += off.x; else mx->x += off.x;
+    if (off.y < 0) mn->y += off.y; else mx->y += off.y;
+    if (off.z < 0) mn->z += off.z; else mx->z += off.z;
+}
+
+bepu_status bepu_create_terrain_shape(bepu_world w, const float* heights, uint32_t width, uint32_t length,
+                                      int32_t organization, const float* transform, uint32_t* shape_id) {
+    if (!w || !heights || !transform || width < 2 || length < 2 || organization < 0 || organization > 1)
+        return BEPU_E_INVALID_ARG;
+    if ((uint64_t)width * length > (1u << 28) || (uint64_t)width * length * 2 > (uint64_t)INT32_MAX)
+        return BEPU_E_INVALID_ARG;
+    if (!finite_n(heights, (int)(width * length)) || !finite_n(transform, 12)) return BEPU_E_INVALID_ARG;
+    const m3 lin{transform[0], transform[1], transform[2], transform[3], transform[4],
+                 transform[5], transform[6], transform[7], transform[8]};
+    const f3 tr{transform[9], transform[10], transform[11]};
+    DeviceWorld& d = w->d;
+    if (hipStreamSynchronize(w->stream) != hipSuccess) return BEPU_E_DEVICE;
+    bepu_status st = mesh_tables(w);
+    if (st != BEPU_OK) return st;
+    f3 mn, mx;
+    terrain_box(heights, width, length, lin, tr, &mn, &mx);
+    const uint32_t hbase = (uint32_t)w->terrain_heights_host.size();
+    w->terrain_heights_host.insert(w->terrain_heights_host.end(), heights, heights + (size_t)width * length);
+    w->terrain_info_host.push_back(make_float4(lin.m11, lin.m12, lin.m13, bits_float(hbase)));
+    w->terrain_info_host.push_back(make_float4(lin.m21, lin.m22, lin.m23, bits_float(width)));
+    w->terrain_info_host.push_back(make_float4(lin.m31, lin.m32, lin.m33, bits_float(length)));
+    w->terrain_info_host.push_back(f4_of(tr, bits_float((uint32_t)organization)));
+    w->terrain_info_host.push_back(f4_of(mn, 0.0f));
+    w->terrain_info_host.push_back(f4_of(mx, 0.0f));
+    st = mirror(w->terrain_heights_host, &d.terrain_heights);
+    if (st == BEPU_OK) st = mirror(w->terrain_info_host, &d.terrain_info);
+    if (st != BEPU_OK) return st;
+    if (shape_id) *shape_id = w->terrain_shapes;
+    w->terrain_shapes++;
+    return BEPU_OK;
+}
+
+// The body record of a Terrain: a static collidable like a static mesh, its
+// sidedness calibrated from its transform's winding (Terrain.WorldTransform,
+// Terrain.cs:41-75: Counterclockwise unless the adjugate transpose turns the
+// up row against itself).
+static bepu_status terrain_body(bepu_world w, uint32_t shape_id, int32_t improve, const bepu_material* material,
+                                HostBody& rec) {
+    if (shape_id >= w->terrain_shapes) return BEPU_E_INVALID_ARG;
+    const float4* r = w->terrain_info_host.data() + 6 * (size_t)shape_id;
+    const m3 m{r[0].x, r[0].y, r[0].z, r[1].x, r[1].y, r[1].z, r[2].x, r[2].y, r[2].z};
+    // Matrix3x3.AdjugateTranspose (Matrix3x3.cs:673-705): its up row.
+    const f3 up_n{m.m13 * m.m32 - m.m33 * m.m12, m.m11 * m.m33 - m.m13 * m.m31, m.m12 * m.m31 - m.m11 * m.m32};
+    const f3 up{m.m21, m.m22, m.m23};
+    const float dd = up_n.x * up.x + up_n.y * up.y + up_n.z * up.z;
+    const int32_t sided = dd < 0 ? BEPU_SIDED_CLOCKWISE : BEPU_SIDED_COUNTERCLOCKWISE;
+    bepu_motion_state st{};
+    st.orientation[3] = 1;
+    const float4 shape =
+        make_float4(bits_float(shape_id), (float)sided, improve ? 1.0f : 0.0f, (float)BEPU_SHAPE_TERRAIN);
+    bepu_status s = make_body(w, &st, shape, m3{0, 0, 0, 0, 0, 0, 0, 0, 0}, 0, material, 0, rec);
+    if (s != BEPU_OK) return s;
+    rec.stab.y = 1.0f;
+    rec.flags = (rec.flags & ~(uint32_t)BF_KIN_ACTIVE) | BF_MESH;
+    rec.minrad = 0;
+    w->mesh_bodies = true;
+    return BEPU_OK;
+}
+
+bepu_status bepu_add_terrain(bepu_world w, uint32_t shape_id, int32_t improve_boundary, const bepu_material* material,
+                             uint32_t* id) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    uint32_t n = w->uploaded + (uint32_t)w->staged.size();
+    if (n >= w->d.body_cap) return BEPU_E_CAPACITY;
+    HostBody b;
+    bepu_status st = terrain_body(w, shape_id, improve_boundary, material, b);
     if (st != BEPU_OK) return st;
     return stage_body(w, b, id);
 }
@@ -790,6 +893,11 @@ static bepu_status body_from_desc(bepu_world w, const bepu_body_desc& b, HostBod
         if (!(k >= 0) || k != std::floor(k) || k >= (float)w->mesh_shapes || sided != std::floor(sided))
             return BEPU_E_INVALID_ARG;
         return mesh_body(w, (uint32_t)k, (int32_t)sided, b.dims[2] != 0.0f ? 1 : 0, &b.material, rec);
+    }
+    if (b.shape == BEPU_SHAPE_TERRAIN) {
+        float k = b.dims[0];
+        if (!(k >= 0) || k != std::floor(k) || k >= (float)w->terrain_shapes) return BEPU_E_INVALID_ARG;
+        return terrain_body(w, (uint32_t)k, b.dims[1] != 0.0f ? 1 : 0, &b.material, rec);
     }
     return BEPU_E_UNSUPPORTED;
 }

@@ -87,8 +87,8 @@ struct ConvexState {
 // deactivate at its next check (an active kinematic body touched it);
 // BF_WAKE, activated since the last step (its island is active at the next).
 // BF_PASSIVE / BF_CONTINUOUS: PositionUpdateMode Passive / Continuous (none:
-// Discrete).  BF_MESH: a static mesh (a static collidable: never dynamic,
-// never active, its box fixed).
+// Discrete).  BF_MESH: a static mesh or a terrain (a static collidable:
+// never dynamic, never active, its box fixed).
 enum { BF_DYNAMIC = 1u, BF_PRESENT = 2u, BF_GENERAL = 4u, BF_HULL = 8u, BF_SPHERE = 16u, BF_ALWAYS_ACTIVE = 32u,
        BF_CANDIDATE = 64u, BF_ASLEEP = 128u, BF_KIN_ACTIVE = 256u, BF_NODEACT = 512u, BF_WAKE = 1024u,
        BF_PASSIVE = 2048u, BF_CONTINUOUS = 4096u, BF_MESH = 8192u };
@@ -186,6 +186,13 @@ struct DeviceWorld {
     MeshTester* mtest_cur;
     MeshTester* mtest_old;
     uint2* mesh_work;
+    // terrains (their pairs share the mesh pairs' tables and list): per
+    // terrain shape 6 float4 in terrain_info (the rows of its transform's
+    // linear part with its height base, width and length in .w, its
+    // translation with its QuadTriangleOrganization in .w, its world box min
+    // and max); heights column by column (heights[column * length + row]).
+    float4* terrain_info;
+    float* terrain_heights;
     // joints
     JointState* joints;
     uint32_t njoints;

@@ -82,7 +82,11 @@ struct bepu_world_s {
     uint32_t mesh_shapes = 0;
     std::vector<float4> mesh_verts_host, mesh_nodes_host, mesh_info_host;
     std::vector<int32_t> mesh_idx_host;
-    bool mesh_bodies = false;               // a static mesh was ever added: run k_mesh
+    // Terrain shapes: heights and records (mirrored in d.terrain_heights, d.terrain_info).
+    uint32_t terrain_shapes = 0;
+    std::vector<float> terrain_heights_host;
+    std::vector<float4> terrain_info_host;
+    bool mesh_bodies = false;               // a static mesh or terrain was ever added: run k_mesh
     int device = 0;                         // the HIP device the world lives on
     uint32_t sticky = 0;                    // STICKY_* bits seen by the host since the last bepu_clear_error
     std::vector<uint8_t> absent;            // per body id: removed (bepu_remove, or its world was removed)

@@ -30,7 +30,7 @@ import numpy as np
 from . import _lib
 from ._lib import BepuError, check
 
-SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE, SHAPE_STATIC_MESH = 0, 1, 2, 3, 4
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CONVEX_HULL, SHAPE_CAPSULE, SHAPE_STATIC_MESH, SHAPE_TERRAIN = 0, 1, 2, 3, 4, 5
 STATE_POSITION, STATE_ORIENTATION, STATE_LINEAR_VELOCITY, STATE_ANGULAR_VELOCITY = 1, 2, 4, 8
 STATE_ALL = 15
 _STATE_FIELD = {"position": (STATE_POSITION, slice(0, 3)), "orientation": (STATE_ORIENTATION, slice(3, 7)),
@@ -254,6 +254,50 @@ class StaticMesh:
         mat = _lib.Material(m.StaticFriction, m.KineticFriction, m.Bounciness)
         return lib.bepu_add_static_mesh(h, sid.value, int(self.Sidedness), 1 if self.ImproveBoundaryBehavior else 0,
                                         ctypes.byref(mat), ctypes.byref(out))
+
+
+class Terrain:
+    """BroadPhaseEntries/Terrain.cs: a static heightfield (not an entity).
+    `heights` [width, length] (heights[column, row]; columns along local x,
+    rows along local z), `transform` the AffineTransform (3x3 linear part as
+    Matrix3x3 rows, translation); QuadTriangleOrganization 0
+    BottomLeftUpperRight (default), 1 BottomRightUpperLeft.  Its sidedness
+    follows the transform's winding."""
+
+    def __init__(self, heights, transform, organization=0):
+        self.Heights = np.ascontiguousarray(heights, np.float32)
+        self.Transform = transform
+        self.QuadTriangleOrganization = int(organization)
+        self.ImproveBoundaryBehavior = True
+        self.Material = Material()
+        self._space = None
+        self._id = None
+
+    @property
+    def Id(self):
+        return self._id
+
+    def _add(self, lib, h, out):
+        sid = ctypes.c_uint32()
+        st = lib.bepu_create_terrain_shape(h, *_terrain_args(self.Heights, self.QuadTriangleOrganization,
+                                                             self.Transform), ctypes.byref(sid))
+        if st != 0:
+            return st
+        m = self.Material
+        mat = _lib.Material(m.StaticFriction, m.KineticFriction, m.Bounciness)
+        return lib.bepu_add_terrain(h, sid.value, 1 if self.ImproveBoundaryBehavior else 0, ctypes.byref(mat),
+                                    ctypes.byref(out))
+
+
+def _terrain_args(heights, organization, transform):
+    hh = np.ascontiguousarray(heights, np.float32)
+    if hh.ndim != 2:
+        raise ValueError("terrain heights must be [width, length]")
+    t = np.concatenate([np.asarray(transform[0], np.float32).reshape(9),
+                        np.asarray(transform[1], np.float32).reshape(3)]).astype(np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    _terrain_args.keep = (hh, t)
+    return hh.ctypes.data_as(fp), hh.shape[0], hh.shape[1], int(organization), t.ctypes.data_as(fp)
 
 
 def _mesh_args(vertices, indices, transform):
@@ -600,10 +644,20 @@ class Space:
                                                ctypes.byref(sid)), "create_mesh_shape")
         return sid.value
 
+    def create_terrain_shape(self, heights, organization, transform):
+        """Registers a terrain shape with its transform (bepu_create_terrain_shape); returns its id."""
+        sid = ctypes.c_uint32()
+        check(self._lib.bepu_create_terrain_shape(self._handle(), *_terrain_args(heights, organization, transform),
+                                                  ctypes.byref(sid)), "create_terrain_shape")
+        return sid.value
+
     def register_meshes(self, meshes):
-        """Registers mesh shapes in order ((vertices, indices, transform) each);
-        scene rows of static meshes name them by index in dims[0]."""
-        self._mesh_ids = [self.create_mesh_shape(*m) for m in meshes]
+        """Registers mesh and terrain shapes in order ((vertices, indices,
+        transform) or a scenes.TerrainDesc each); scene rows of static meshes
+        and terrains name them by index in dims[0]."""
+        from .scenes import TerrainDesc
+        self._mesh_ids = [self.create_terrain_shape(m.heights, m.organization, m.transform)
+                          if isinstance(m, TerrainDesc) else self.create_mesh_shape(*m) for m in meshes]
         return self._mesh_ids
 
     def register_hulls(self, hull_points):
@@ -643,6 +697,9 @@ class Space:
             elif int(r["shape"]) == SHAPE_STATIC_MESH:
                 st = lib.bepu_add_static_mesh(h, self._mesh_ids[int(d[0])], int(d[1]), 1 if d[2] else 0,
                                               ctypes.byref(mat), ctypes.byref(out))
+            elif int(r["shape"]) == SHAPE_TERRAIN:
+                st = lib.bepu_add_terrain(h, self._mesh_ids[int(d[0])], 1 if d[1] else 0, ctypes.byref(mat),
+                                          ctypes.byref(out))
             else:
                 raise NotImplementedError("shape %d" % int(r["shape"]))
             if st != 0:

@@ -272,6 +272,29 @@ bepu_status bepu_create_mesh_shape(bepu_world world, const float* vertices, uint
 bepu_status bepu_add_static_mesh(bepu_world world, uint32_t mesh_shape_id, int32_t sidedness,
                                  int32_t improve_boundary, const bepu_material* material, uint32_t* id_out);
 
+/* Terrains.  bepu_create_terrain_shape is new TerrainShape(heights,
+ * organization) (CollisionShapes/TerrainShape.cs:60-72) together with the
+ * Terrain's WorldTransform (BroadPhaseEntries/Terrain.cs:41-75): `heights`
+ * column by column (heights[column * length + row], the reference's
+ * heights[column, row]; `width` columns along local x, `length` rows along
+ * local z, both at least 2), `organization` the QuadTriangleOrganization
+ * (0 BottomLeftUpperRight, 1 BottomRightUpperLeft), `transform` the
+ * AffineTransform as 12 floats (Matrix3x3 M11..M33 row by row, then the
+ * translation; the sample (c, heights[c, r], r) maps to its v * M + t).  The
+ * terrain's sidedness follows the transform's winding and its box is fixed
+ * here (TerrainShape.GetBoundingBox).  bepu_add_terrain is Space.Add(new
+ * Terrain(shape, transform)) (Terrain.cs:172-184) with ImproveBoundaryBehavior
+ * (the reference's default 1) and material; its Thickness is 0 (the
+ * reference's default), the thickness raycast is not offered.  A terrain is a
+ * static collidable like a static mesh: it is not an entity, setters,
+ * activation, update modes and joints refuse it, bepu_remove removes it, and
+ * its pairs share the mesh pairs' limits. */
+bepu_status bepu_create_terrain_shape(bepu_world world, const float* heights, uint32_t width, uint32_t length,
+                                      int32_t organization, const float* transform /* 12 */,
+                                      uint32_t* shape_id_out);
+bepu_status bepu_add_terrain(bepu_world world, uint32_t terrain_shape_id, int32_t improve_boundary,
+                             const bepu_material* material, uint32_t* id_out);
+
 /* Batched worlds.  A world handle may hold many independent sub-worlds, each
  * the analogue of one Space: bodies of different sub-worlds never form pairs,
  * and every sub-world's result is bit-identical to stepping it alone (its
@@ -290,7 +313,8 @@ typedef struct bepu_body_desc {
     float dims[3];            /* box: width, height, length; sphere: radius, 0, 0;
                                  capsule: length, radius, 0; hull: shape id (an integer), 0, 0;
                                  static mesh: mesh shape id, sidedness, improve boundary (0/1);
-                                 a static mesh's mass, state and flags are ignored */
+                                 terrain: terrain shape id, improve boundary (0/1), 0;
+                                 a static mesh's or terrain's mass, state and flags are ignored */
     float mass;               /* <= 0: kinematic */
     bepu_motion_state state;
     bepu_material material;

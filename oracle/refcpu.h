@@ -185,15 +185,15 @@ struct TerrainShapeRef {
 // Matrix3x3.Invert (BEPUutilities/Matrix3x3.cs:505-531), AffineTransform.Invert (AffineTransform.cs:173-178)
 M3 m3_invert(const M3& m);
 Affine affine_invert(const Affine& a);
-// TerrainShape.GetBoundingBox (TerrainShape.cs:50-120)
+// TerrainShape.GetBoundingBox (TerrainShape.cs:86-140)
 void terrain_bounding_box(const TerrainShapeRef& t, V3* mn, V3* mx);
-// TerrainShape.GetOverlaps (TerrainShape.cs:406-453): triangle ids (2 per quad) over a local box.
+// TerrainShape.GetOverlaps (TerrainShape.cs:497-543): triangle ids (2 per quad) over a local box.
 void terrain_overlaps(const TerrainShapeRef& t, const V3& lmin, const V3& lmax, std::vector<int>* out);
-// TerrainShape.GetLocalIndices (TerrainShape.cs:538-583): (column, row) of the three vertices.
+// TerrainShape.GetLocalIndices (TerrainShape.cs:626-663): (column, row) of the three vertices.
 void terrain_local_indices(const TerrainShapeRef& t, int tri, int col[3], int row[3]);
-// TerrainShape.GetTriangle (TerrainShape.cs:516-533): the world-space triangle.
+// TerrainShape.GetTriangle (TerrainShape.cs:607-622): the world-space triangle.
 void terrain_world_triangle(const TerrainShapeRef& t, int tri, V3 out[3]);
-// Terrain.WorldTransform setter (Terrain.cs:45-62): the sidedness its winding gets.
+// Terrain.WorldTransform setter (Terrain.cs:41-75): the sidedness its winding gets.
 int terrain_sidedness(const Affine& xf);
 
 // MeshBoundingBoxTree.Reconstruct (MeshBoundingBoxTree.cs:60-69).

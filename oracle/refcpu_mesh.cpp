@@ -312,7 +312,7 @@ void terrain_local_indices(const TerrainShapeRef& t, int tri, int col[3], int ro
     }
 }
 
-// TerrainShape.GetPosition (TerrainShape.cs:377-398): clamped indices, world space.
+// TerrainShape.GetPosition (TerrainShape.cs:432-452): clamped indices, world space.
 static V3 terrain_position(const TerrainShapeRef& t, int c, int r) {
     if (c <= 0) c = 0;
     else if (c >= t.width) c = t.width - 1;

@@ -1400,7 +1400,7 @@ void convex_local_bounding_box(const ConvexShapeRef& cvx, const RT& shape_xf, co
     *mn = V3{t.t.x + neg_.x, t.t.y + neg_.y, t.t.z + neg_.z};
 }
 
-// TerrainContactManifold.FindOverlappingTriangles (TerrainContactManifold.cs:35-64):
+// TerrainContactManifold.FindOverlappingTriangles (TerrainContactManifold.cs:30-64):
 // the convex's box in the terrain's local frame, grown by its motion over the step.
 void terrain_query_box(refcpu_world* w, const Body& A, const Affine& xf, V3* mn, V3* mx) {
     convex_local_bounding_box(shape_ref(w, A), RT{A.pos, A.q}, xf, mn, mx);
@@ -1656,7 +1656,7 @@ void mesh_time_of_impact(refcpu_world* w, Pair& p) {
     }
     else mesh_overlaps(w->mesh_shapes[M.mesh], A.bmin, A.bmax, &tris);
     const RT ta = ccd_transform(A);
-    // TerrainPairHandler.UpdateTimeOfImpact (TerrainPairHandler.cs:81-137):
+    // TerrainPairHandler.UpdateTimeOfImpact (TerrainPairHandler.cs:117-176):
     // the terrain's up direction decides which side of a triangle counts.
     const V3 terrain_up = terrain ? V3{ts->xf.m.m21, ts->xf.m.m22, ts->xf.m.m23} : V3{0, 0, 0};
     for (int tri : tris) {
@@ -1698,7 +1698,7 @@ Pair make_pair_state(refcpu_world* w, uint32_t i, uint32_t j) {
     p.mesh_slot = -1;
     if (is_static(bi) || is_static(bj)) {
         // StaticMeshPairHandler.Initialize (StaticMeshPairHandler.cs:57-79),
-        // TerrainPairHandler.Initialize (TerrainPairHandler.cs:48-65): A is the convex.
+        // TerrainPairHandler.Initialize (TerrainPairHandler.cs:70-93): A is the convex.
         p.a = is_static(bi) ? j : i;
         p.b = is_static(bi) ? i : j;
         if (w->bodies[p.a].shape == BEPU_SHAPE_SPHERE) {
@@ -2965,7 +2965,7 @@ int refcpu_add_static_mesh(refcpu_world* w, uint32_t shape, int sidedness, int i
     return BEPU_OK;
 }
 
-// new TerrainShape(heights, triangleOrganization) (TerrainShape.cs:38-56):
+// new TerrainShape(heights, triangleOrganization) (TerrainShape.cs:60-72):
 // heights[column * length + row] for `width` columns and `length` rows.
 int refcpu_create_terrain_shape(refcpu_world* w, const float* heights, uint32_t width, uint32_t length,
                                 int organization, uint32_t* id) {

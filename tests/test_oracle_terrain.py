@@ -59,7 +59,7 @@ def test_terrain_overlaps_equal_brute_force(oracle):
 def test_terrain_overlap_order_and_truncation(oracle):
     # Columns outer, rows inner; quad (i, j) of a width-4 terrain is
     # 2 (i + 4 j), its two triangles 2q and 2q + 1.  The bounds are truncated
-    # towards zero (TerrainShape.cs:448-452): a box ending at x = -0.5 still
+    # towards zero (TerrainShape.cs:499-503): a box ending at x = -0.5 still
     # reaches column 0.
     h = np.zeros((4, 4), np.float32)
     w, tid = _terrain_world(oracle, h, np.eye(3), (0, 0, 0))
@@ -108,7 +108,7 @@ def test_terrain_triangles(oracle, organization):
 
 def test_terrain_bounding_box(oracle):
     # The extremes of the transformed samples plus the translation; with the
-    # reference's else-if (TerrainShape.cs:80-101) a sample that sets a
+    # reference's else-if (TerrainShape.cs:95-130) a sample that sets a
     # running minimum is never tried as a maximum, so a peak at the first
     # sample stays outside the box.
     h = np.zeros((5, 4), np.float32)
@@ -199,7 +199,7 @@ def test_sliding_on_a_terrain_as_on_a_box_ground(oracle):
 
 def test_mirrored_terrain_still_supports_from_above(oracle):
     # A transform that mirrors x flips the world winding of every triangle;
-    # the terrain's sidedness follows it (Terrain.cs:45-62), so bodies still
+    # the terrain's sidedness follows it (Terrain.cs:41-75), so bodies still
     # rest on the top side.  A transform turned upside down (y scaled by -1)
     # keeps its up row pointing down: then the underside is the solid one.
     for lin in (np.diag([-2.0, 1.0, 2.0]), np.diag([2.0, 1.0, -2.0])):
