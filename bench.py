@@ -28,9 +28,12 @@ The JSON line also carries
                 oracle baseline;
   served, stress
                 continuous batching of rollouts, and one step of the
-                broadphase/narrowphase stress scene.
+                broadphase/narrowphase stress scene;
+  mesh          16,384 bodies of all four shapes on a terrain and on a
+                static triangle mesh: body-steps/s, the mesh pair kernel's
+                time and bytes, the oracle's rate and bit parity.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs batched,towers,served,stress]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs batched,towers,served,stress,mesh]
 """
 import argparse
 import json
@@ -66,13 +69,22 @@ BATCHED_ITERATIONS = 8
 #   integrate     80 per body
 # c = contacts of the manifold (the scene's mean is used per tier).
 B_CONVEX_PAIR = 1120
+#   mesh          per static mesh or terrain pair: pair state 336 and tester /
+#                 supplement state 160, each read and written, + both bodies'
+#                 pose, velocity and shape 128 (1,120), + 64 per live
+#                 triangle tester (32 B read and written); the triangles'
+#                 vertices, indices and tree nodes (a few MB that stay in
+#                 cache) are not counted
+B_MESH_PAIR = 1120
+B_MESH_TESTER = 64
 
 
 def manifold_bytes(m, contacts, iters):
     return (332 + 190) * m + (72 + 70) * contacts + iters * (316 * m + 44 * contacts)
 
 
-def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0):
+def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0, n_mesh=0,
+                      n_testers=0):
     c_mean = n_contacts / n_cons if n_cons else 0.0
     m_isl = n_cons - n_global
     if stage == "body_pre":
@@ -83,6 +95,8 @@ def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global
         return 312 * n_pair
     if stage == "convex":
         return B_CONVEX_PAIR * n_convex
+    if stage == "mesh":
+        return B_MESH_PAIR * n_mesh + B_MESH_TESTER * n_testers
     if stage == "islands":
         return 40 * n_body + 32 * n_cons
     if stage == "island_solve":
@@ -94,10 +108,10 @@ def algorithmic_bytes(stage, n_body, n_pair, n_cons, n_contacts, iters, n_global
     return 0
 
 
-def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0):
-    """Whole-step algorithmic bytes: B_body*N + B_pair*P + B_convex*P_convex + sum over manifolds."""
-    return 528 * n_body + 312 * n_pair + B_CONVEX_PAIR * n_convex + (332 + 190) * n_cons + \
-        (72 + 70) * n_contacts + iters * (316 * n_cons + 44 * n_contacts)
+def step_bytes(n_body, n_pair, n_cons, n_contacts, iters, n_global=0, n_convex=0, n_mesh=0, n_testers=0):
+    """Whole-step algorithmic bytes: B_body*N + B_pair*P + B_convex*P_convex + B_mesh*P_mesh + sum over manifolds."""
+    return 528 * n_body + 312 * n_pair + B_CONVEX_PAIR * n_convex + B_MESH_PAIR * n_mesh + B_MESH_TESTER * n_testers + \
+        (332 + 190) * n_cons + (72 + 70) * n_contacts + iters * (316 * n_cons + 44 * n_contacts)
 
 
 # Kernels of each timed stage (name substrings), for the measured HBM traffic
@@ -108,6 +122,7 @@ STAGE_KERNELS = {
     "island_solve": ["k_island_solve"],
     "narrowphase": ["k_pair_prep", "k_narrowphase"],
     "convex": ["k_convex<"],
+    "mesh": ["k_mesh"],
     "broadphase": ["k_grid_insert", "k_grid_query"],
     "global_solve": ["k_global"],
     "body_pre": ["k_body_pre"],
@@ -174,7 +189,8 @@ def roofline(stages, counts, workload, window=None):
 
 def counts_of(st, iters):
     return dict(n_body=st.bodies, n_pair=st.pairs, n_cons=st.manifolds, n_contacts=st.contacts, iters=iters,
-                n_global=st.global_manifolds, n_convex=int(sum(st.convex_pairs)))
+                n_global=st.global_manifolds, n_convex=int(sum(st.convex_pairs)), n_mesh=st.mesh_pairs,
+                n_testers=st.mesh_testers)
 
 
 def latency(sp, steps):
@@ -573,6 +589,82 @@ def stress_leg(n, reps, cpu):
     return out
 
 
+def mesh_leg(n, warmup, steps, cpu, parity):
+    """Static meshes and terrains: n bodies of all four shapes dropped onto a
+    bumpy terrain, and onto a bumpy static mesh (128 x 128 quads, 32,768
+    triangles), stepped `warmup` steps (the bodies land) and then `steps`
+    timed device-resident steps; stage times over 20 more steps.  Parity:
+    the motion states after the warm-up against the oracle's, bit for bit.
+    cpu: the oracle's next 10 steps on host threads."""
+    from bepuphysics1_amd import Space, scenes
+    out = {}
+    threads = baseline_threads()
+    for kind in ("terrain", "static_mesh"):
+        if kind == "terrain":
+            sc, meshes = scenes.terrain_pile(n, seed=1)
+            t = meshes[0]
+            desc = "%d bodies (boxes, spheres, 16-point hulls, capsules) on a %dx%d terrain (spacing 2, " \
+                   "amplitude 1)" % (n, t.heights.shape[0], t.heights.shape[1])
+        else:
+            sc, meshes = scenes.mesh_pile(n, seed=1, cells=128)
+            desc = "%d bodies (boxes, spheres, 16-point hulls, capsules) on a 128x128-quad static mesh " \
+                   "(32,768 triangles, amplitude 1)" % n
+        sp = Space(capacity=n + 1)
+        sp.register_hulls(scenes.hull_library())
+        sp.register_meshes(meshes)
+        sp.add_scene(sc)
+        sp.Update(warmup)
+        sp.synchronize()
+        st0 = sp.motion_states()
+        sp.synchronize()
+        t0 = time.perf_counter()
+        sp.Update(steps)
+        sp.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        sp.set_timing(True)
+        sp.Update(20)
+        stages = sp.stage_times()
+        sp.set_timing(False)
+        st = sp.stats()
+        counts = counts_of(st, 10)
+        mesh_ms = stages.get("mesh", 0.0)
+        mesh_bytes = algorithmic_bytes("mesh", **counts)
+        leg = {"workload": desc + ", dt 1/60, 10 iterations",
+               "value": round(len(sc) / dt, 1), "unit": "body-steps/s", "ms_per_step": round(1e3 * dt, 4),
+               "steps": steps, "warmup": warmup, "pairs": st.pairs, "manifolds": st.manifolds,
+               "contacts": st.contacts, "mesh_pairs": st.mesh_pairs, "mesh_testers": st.mesh_testers,
+               "convex_pairs_by_state": list(st.convex_pairs),
+               "stages": stage_table(stages, counts),
+               "roofline": {"bound": "latency (one pair per lane: triangle GJK/MPR chains)", "kernel": "k_mesh",
+                            "kernel_ms": round(mesh_ms, 5), "alg_bytes_per_launch": int(mesh_bytes),
+                            "achieved": round(mesh_bytes / (mesh_ms * 1e-3) / 1e9, 2) if mesh_ms > 0 else None,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(mesh_bytes / (mesh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+                            if mesh_ms > 0 else None,
+                            "traffic": measured_traffic("mesh", "mesh_" + kind)}}
+        sp.close()
+        if parity or cpu:
+            from oracle import refcpu
+            w = refcpu.World()
+            w.set_threads(threads)
+            w.add_scene(sc, oracle_hull_table(), meshes)
+            w.step(warmup)
+            if parity:
+                leg["parity"] = {"steps": warmup, "bit_identical_states":
+                                 bool(np.array_equal(f32bits(st0), f32bits(w.motion_states())))}
+            if cpu:
+                t0 = time.perf_counter()
+                w.step(10)
+                ct = (time.perf_counter() - t0) / 10
+                leg["cpu_baseline"] = {"value": round(len(sc) / ct, 1), "unit": "body-steps/s", "cores": threads,
+                                       "kind": "port", "sample": "the oracle's steps %d-%d of the same scene" %
+                                       (warmup + 1, warmup + 10)}
+                leg["speedup_vs_cpu"] = round(leg["value"] / leg["cpu_baseline"]["value"], 1)
+            w.close()
+        out[kind] = leg
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30):
     """Plain device-to-device copy of a known size (the HIP runtime's copy
     kernel), run in the same process as the workload so that one profiler
@@ -703,7 +795,9 @@ def main():
     ap.add_argument("--served-rate", type=float, default=2000.0)
     ap.add_argument("--served-seconds", type=float, default=4.0)
     ap.add_argument("--stress-bodies", type=int, default=32768)
-    ap.add_argument("--legs", default="batched,towers,served,stress",
+    ap.add_argument("--mesh-bodies", type=int, default=16384)
+    ap.add_argument("--mesh-steps", type=int, default=120)
+    ap.add_argument("--legs", default="batched,towers,served,stress,mesh",
                     help="comma list of workloads; without batched only the other legs' results are printed")
     ap.add_argument("--summarize-trace", metavar="DIR",
                     help="print per-kernel averages over the timed window of a rocprofv3 csv run in DIR and exit")
@@ -740,6 +834,8 @@ def main():
             out["served"] = served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0)
         if "stress" in legs:
             out["stress"] = stress_leg(a.stress_bodies, 3, cpu)
+        if "mesh" in legs:
+            out["mesh"] = mesh_leg(a.mesh_bodies, 60, a.mesh_steps, cpu, not a.no_parity)
         print(json.dumps(out))
         return
 
@@ -791,6 +887,7 @@ def main():
         "towers": towers_leg(a, cpu, not a.no_parity) if "towers" in legs else None,
         "served": served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0) if "served" in legs else None,
         "stress": stress_leg(a.stress_bodies, 3, cpu) if "stress" in legs else None,
+        "mesh": mesh_leg(a.mesh_bodies, 60, a.mesh_steps, cpu, not a.no_parity) if "mesh" in legs else None,
         "calibration_copy_bytes": copy_bytes,
     }
     print(json.dumps(line))

@@ -90,7 +90,8 @@ class StepStats(ctypes.Structure):
     _fields_ = [("bodies", ctypes.c_uint32), ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32),
                 ("contacts", ctypes.c_uint32), ("colors", ctypes.c_uint32), ("steps", ctypes.c_uint32),
                 ("max_penetration", ctypes.c_float), ("global_manifolds", ctypes.c_uint32),
-                ("convex_pairs", ctypes.c_uint32 * 3)]
+                ("convex_pairs", ctypes.c_uint32 * 3), ("mesh_pairs", ctypes.c_uint32),
+                ("mesh_testers", ctypes.c_uint32)]
 
 
 # Every entry point of include/bepu_gpu.h: name -> (restype, argtypes)

@@ -893,10 +893,11 @@ __global__ void __launch_bounds__(BLOCK) k_narrowphase(DeviceWorld w) {
 __global__ void __launch_bounds__(256) k_stats(DeviceWorld w, uint32_t npairs) {
     __shared__ uint32_t s_red[4][4];
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t cons = 0, contacts = 0, colors = 0;
+    uint32_t cons = 0, contacts = 0, colors = 0, testers = 0;
     float mp = 0;
     if (t < npairs) {
         const PairState& s = w.pairs_old[t];
+        if (s.kind == BEPU_PAIR_MESH_CONVEX && !(s.flags & (PF_DEAD | PF_VOID))) testers = (uint32_t)s.tester;
         if (s.ccount > 0 && !(s.flags & (PF_DEAD | PF_VOID))) {
             cons = 1;
             contacts = (uint32_t)s.ccount;
@@ -909,8 +910,11 @@ __global__ void __launch_bounds__(256) k_stats(DeviceWorld w, uint32_t npairs) {
         contacts += __shfl_xor(contacts, off);
         colors = max(colors, (uint32_t)__shfl_xor((int)colors, off));
         mp = fmaxf(mp, __shfl_xor(mp, off));
+        testers += __shfl_xor(testers, off);
     }
+    __shared__ uint32_t s_testers[4];
     int wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) s_testers[wv] = testers;
     if (__lane_id() == 0) {
         s_red[wv][0] = cons;
         s_red[wv][1] = contacts;
@@ -924,7 +928,9 @@ __global__ void __launch_bounds__(256) k_stats(DeviceWorld w, uint32_t npairs) {
             contacts += s_red[k][1];
             colors = max(colors, s_red[k][2]);
             mp = fmaxf(mp, __uint_as_float(s_red[k][3]));
+            testers += s_testers[k];
         }
+        if (testers) atomicAdd(&w.counters[ctr(CTR_MESH_TESTERS)], testers);
         if (cons) atomicAdd(&w.counters[ctr(CTR_CONS)], cons);
         if (contacts) atomicAdd(&w.counters[ctr(CTR_CONTACTS)], contacts);
         if (colors) atomicMax(&w.counters[ctr(CTR_COLORS)], colors);
@@ -937,6 +943,7 @@ void launch_stats(const DeviceWorld& w, uint32_t npairs, hipStream_t st) {
     launch_fill_u32(w.counters + ctr(CTR_CONTACTS), 0u, 1, st);
     launch_fill_u32(w.counters + ctr(CTR_COLORS), 0u, 1, st);
     launch_fill_u32(w.counters + ctr(CTR_MAXPEN), 0u, 1, st);
+    launch_fill_u32(w.counters + ctr(CTR_MESH_TESTERS), 0u, 1, st);
     if (npairs) hipLaunchKernelGGL(k_stats, dim3((npairs + 255) / 256), dim3(256), 0, st, w, npairs);
 }
 

@@ -27,8 +27,9 @@ using namespace bepu;
 
 namespace {
 
-enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_CONVEX, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE, ST_COUNT };
-const char* kStageNames = "body_pre,broadphase,narrowphase,convex,islands,island_solve,global_solve,integrate";
+enum Stage { ST_BODY = 0, ST_BROAD, ST_NARROW, ST_CONVEX, ST_MESH, ST_ISLANDS, ST_ISLAND_SOLVE, ST_GLOBAL_SOLVE, ST_INTEGRATE,
+             ST_COUNT };
+const char* kStageNames = "body_pre,broadphase,narrowphase,convex,mesh,islands,island_solve,global_solve,integrate";
 
 template <typename T>
 bepu_status dalloc(T** p, size_t n) {
@@ -1255,21 +1256,22 @@ static bepu_status enqueue_step(bepu_world w, hipEvent_t* e) {
     }
     if (e) (void)hipEventRecord(e[3], s);
     if (!side && w->convex_bodies) launch_convex(d, w->convex_lds, s);
-    if (w->mesh_bodies) launch_mesh(d, s);
     if (e) (void)hipEventRecord(e[4], s);
+    if (w->mesh_bodies) launch_mesh(d, s);
+    if (e) (void)hipEventRecord(e[5], s);
     // Joints join their bodies' islands; then the islands woken by this
     // step's new constraints, stale frozen pairs, split activation.
     launch_joint_edges(d, s);
     launch_activity(d, s);
     launch_islands(d, w->scan_temp, w->scan_bytes, s);
-    if (e) (void)hipEventRecord(e[5], s);
-    launch_island_solve(d, s);
     if (e) (void)hipEventRecord(e[6], s);
-    launch_global_solver(d, s);
+    launch_island_solve(d, s);
     if (e) (void)hipEventRecord(e[7], s);
+    launch_global_solver(d, s);
+    if (e) (void)hipEventRecord(e[8], s);
     launch_integrate(d, s);
     if (d.ccd) launch_ccd(d, s);
-    if (e) (void)hipEventRecord(e[8], s);
+    if (e) (void)hipEventRecord(e[9], s);
     return BEPU_OK;
 }
 
@@ -1561,6 +1563,8 @@ bepu_status bepu_get_stats(bepu_world w, bepu_step_stats* out) {
     out->convex_pairs[0] = any ? c[CTR_CONVEX] + c[CTR_CONVEX_OTHER0] : 0;
     out->convex_pairs[1] = any ? c[CTR_CONVEX_SHALLOW] + c[CTR_CONVEX_OTHER0 + 1] : 0;
     out->convex_pairs[2] = any ? c[CTR_CONVEX_DEEP] + c[CTR_CONVEX_OTHER0 + 2] : 0;
+    out->mesh_pairs = any && w->mesh_bodies ? c[CTR_MESH] : 0;
+    out->mesh_testers = any && w->mesh_bodies ? c[CTR_MESH_TESTERS] : 0;
     // Any step since the last bepu_clear_error that overflowed a capacity
     // (BEPU_E_CAPACITY) or whose global solver grid was not resident (BEPU_E_DEVICE).
     return sticky_status(w->sticky);

@@ -357,7 +357,8 @@ enum {
     CTR_DEAD = 152,       // frozen pairs removed as stale this step
     CTR_STEPS = 153,      // steps started (the step reset adds one; never cleared)
     CTR_MESH = 154,       // mesh pairs listed for k_mesh this step
-    CTR_COUNT = 155
+    CTR_MESH_TESTERS = 155,  // statistics: live triangle testers of the last step's mesh pairs
+    CTR_COUNT = 156
 };
 
 // Sticky error bits: the start of each step folds the previous step's

@@ -182,6 +182,8 @@ typedef struct bepu_step_stats {
     uint32_t global_manifolds; /* constraints (manifolds and joints) in islands too large for one workgroup */
     uint32_t convex_pairs[3]; /* general convex pairs (any hull or capsule) tested in the last step,
                                  by pair tester state: separated, shallow, deep */
+    uint32_t mesh_pairs;      /* static mesh and terrain pairs updated in the last step */
+    uint32_t mesh_testers;    /* triangle pair testers those pairs kept (a sphere pair keeps none) */
 } bepu_step_stats;
 
 typedef struct bepu_world_s* bepu_world;
