@@ -5,10 +5,11 @@
 // StaticMeshContactManifold.cs:33-71, TerrainContactManifold.cs:30-100) for
 // the pairs k_pair_prep listed.
 //
-// One thread per pair: the contact refresh, the depth-first walk of the
-// mesh's MeshBoundingBoxTree over the convex's box (child A's subtree before
-// child B's, MeshBoundingBoxTree.cs:112-122, 214-226) or the scan of the
-// terrain's grid under the convex's box in its frame, each overlapped
+// One wavefront per pair, its state in LDS: the contact refresh, the
+// depth-first walk of the mesh's MeshBoundingBoxTree over the convex's box
+// (child A's subtree before child B's, MeshBoundingBoxTree.cs:112-122,
+// 214-226) or the scan of the terrain's grid under the convex's box in its
+// frame, each overlapped
 // triangle carried into the convex's local space, pruned by the convex's
 // local box and tested by its TriangleConvexPairTester (kept per triangle in
 // the pair's tester table) or the TriangleSpherePairTester, the improved
@@ -87,14 +88,14 @@ __device__ __forceinline__ f3 corrected_normal(f3 uncorrected, const cvx::TriSha
 }
 
 // AnalyzeCandidate (TriangleMeshConvexContactManifold.cs:483-596): true for a
-// face contact, which is always kept; an edge or vertex contact is queued and
-// every other region of the triangle blocked.
-__device__ bool analyze(int region, const int ix[3], const cvx::TriShape& t, int sided, bool should_correct,
-                        const MeshContact& c, MeshScratch& m) {
+// face contact, which is always kept; an edge or vertex contact is queued
+// (with `corrected`, GetNormal of its triangle) and every other region of the
+// triangle blocked.
+__device__ bool analyze(int region, const int ix[3], f3 corrected, bool should_correct, const MeshContact& c,
+                        MeshScratch& m) {
     const int A = ix[0], B = ix[1], C = ix[2];
     if (region <= VR_C) {
-        Boundary b{c, corrected_normal(c.normal, t, sided), region == VR_A ? A : (region == VR_B ? B : C), -1,
-                   should_correct};
+        Boundary b{c, corrected, region == VR_A ? A : (region == VR_B ? B : C), -1, should_correct};
         if (m.nverts < kMeshCandidates) m.verts[m.nverts++] = b;
         else m.overflow = true;
         block_edge(m, A, B);
@@ -106,8 +107,7 @@ __device__ bool analyze(int region, const int ix[3], const cvx::TriShape& t, int
         return false;
     }
     if (region <= VR_BC) {
-        Boundary b{c, corrected_normal(c.normal, t, sided), region == VR_BC ? B : A, region == VR_AB ? B : C,
-                   should_correct};
+        Boundary b{c, corrected, region == VR_BC ? B : A, region == VR_AB ? B : C, should_correct};
         if (m.nedges < kMeshCandidates) m.edges[m.nedges++] = b;
         else m.overflow = true;
         if (region != VR_AB) block_edge(m, A, B);
@@ -239,32 +239,81 @@ __device__ void reduce_general(const PairState& s, const MeshScratch& m, uint32_
     }
 }
 
-__device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, MeshTester* tab) {
+// One triangle's tester results, kept for the in-order merge.
+struct TriOut {
+    MeshContact c[2];
+    f3 corrected[2];
+    int region[2];
+    int ix[3];
+    int n;               // contacts; -1: not tested (outside the local box, or no tester left)
+    int should_correct;
+};
+
+constexpr int kMeshChunk = 64;  // triangles tested side by side, one per lane
+
+// A pair's update in LDS: its state, tester state and tester table (loaded
+// from and stored back to the pair slot), the manifold's scratch, the
+// chunk's triangles and their tester results, the tree walk's stack.
+struct MeshShared {
+    PairState s;
+    ConvexState e;
+    MeshTester tab[kMeshTesters];
+    MeshScratch m;
+    TriOut r[kMeshChunk];
+    int tri[kMeshChunk];
+    int stack[kMeshStack];
+    int count;
+};
+
+// OR of a per-lane word over the wavefront.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off);
+    return v;
+}
+
+// The pair's update by one wavefront (TriangleMeshConvexContactManifold.Update,
+// TriangleMeshConvexContactManifold.cs:101-434).  Overlapped triangles are
+// gathered in order, up to one per lane; each lane carries its triangle into
+// the convex's local space, prunes it by the convex's local box and runs its
+// pair tester (TriangleConvexPairTester state from the pair's table, or the
+// TriangleSpherePairTester); lane 0 then merges the chunk's candidates in the
+// order of the triangles (boundary analysis, AddLocalContact), so the result
+// is the one the reference's serial loop gives.
+__device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
+    PairState& s = sh.s;
+    ConvexState& e = sh.e;
+    MeshScratch& m = sh.m;
     const uint32_t A = s.a, M = s.b;
     const float4 PA = w.pos[A], QA = w.quat[A];
     const cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
-    // ContactRefresher.ContactRefresh (ContactRefresher.cs:18-69)
-    int rem[4], nrem = 0;
-    for (int k = 0; k < s.ccount; k++) {
-        int slot = s.clist[k];
-        f3 na = cvx::rt_transform(f3_of(e.offa[slot]), ta);
-        f3 nb = cvx::rt_transform(f3_of(e.offb[slot]), kMeshTransform);
-        f3 ab = nb - na;
-        f3 n = f3_of(s.cnrm[slot]);
-        float d = dot3(ab, n);
-        f3 t = ab - n * d;
-        d = lensq3(t);
-        if (d > w.s.contact_invalidation_length_sq) {
-            rem[nrem++] = k;
-        } else {
-            d = dot3(ab, n);
-            float depth = e.offa[slot].w - d;
-            s.cpos[slot].w = depth;
-            if (depth < -w.s.max_contact_distance) rem[nrem++] = k;
-            else s.cpos[slot] = f4_of((nb + na) * .5f, depth);
+    const bool sphere = s.kind == BEPU_PAIR_MESH_SPHERE;
+    int ntest = sphere ? 0 : s.tester;
+    if (lane == 0) {
+        // ContactRefresher.ContactRefresh (ContactRefresher.cs:18-69)
+        int rem[4], nrem = 0;
+        for (int k = 0; k < s.ccount; k++) {
+            int slot = s.clist[k];
+            f3 na = cvx::rt_transform(f3_of(e.offa[slot]), ta);
+            f3 nb = cvx::rt_transform(f3_of(e.offb[slot]), kMeshTransform);
+            f3 ab = nb - na;
+            f3 n = f3_of(s.cnrm[slot]);
+            float d = dot3(ab, n);
+            f3 t = ab - n * d;
+            d = lensq3(t);
+            if (d > w.s.contact_invalidation_length_sq) {
+                rem[nrem++] = k;
+            } else {
+                d = dot3(ab, n);
+                float depth = e.offa[slot].w - d;
+                s.cpos[slot].w = depth;
+                if (depth < -w.s.max_contact_distance) rem[nrem++] = k;
+                else s.cpos[slot] = f4_of((nb + na) * .5f, depth);
+            }
         }
+        for (int i = nrem - 1; i >= 0; i--) remove_contact_inplace(s, rem[i]);
+        m.ncand = m.nedges = m.nverts = m.nbv = m.nbe = 0;
+        m.overflow = false;
     }
-    for (int i = nrem - 1; i >= 0; i--) remove_contact_inplace(s, rem[i]);
 
     const float4 msh = w.shape[M];
     const bool terrain = msh.w == (float)BEPU_SHAPE_TERRAIN;
@@ -299,57 +348,162 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
     cv.shape = shape_of(w, A, w.hull_verts + __float_as_uint(sha.y), 1);
     cv.min_radius = w.minrad[A];
     cv.core_scaling = w.s.core_shape_scaling;
-    const bool sphere = s.kind == BEPU_PAIR_MESH_SPHERE;
-    int ntest = sphere ? 0 : s.tester;
     uint32_t updated = 0;
     int guaranteed = 0;
-    MeshScratch m;
-    m.ncand = m.nedges = m.nverts = m.nbv = m.nbe = 0;
-    m.overflow = false;
+    const uint64_t below = (1ull << lane) - 1;  // lanes before this one
 
-    // One overlapped triangle, its vertices in the convex's local space and
-    // its TriangleIndices in ix: the local box prune, its pair tester and the
-    // boundary analysis (TriangleMeshConvexContactManifold.cs:135-236).
-    auto triangle = [&](const int ix[3], const cvx::TriShape& t) {
-        f3 tmin, tmax;
-        tri_axis(t.v0.x, t.v1.x, t.v2.x, &tmin.x, &tmax.x);
-        tri_axis(t.v0.y, t.v1.y, t.v2.y, &tmin.y, &tmax.y);
-        tri_axis(t.v0.z, t.v1.z, t.v2.z, &tmin.z, &tmax.z);
-        if (tmin.x > lmax.x || tmin.y > lmax.y || tmin.z > lmax.z) return;
-        if (lmin.x > tmax.x || lmin.y > tmax.y || lmin.z > tmax.z) return;
+    // Tests the chunk's `ntri` triangles (sh.tri, in order), one per lane,
+    // then merges their candidates in order on lane 0.
+    auto run_chunk = [&](int ntri) {
+        TriOut& r = sh.r[lane];
+        cvx::TriShape t;
+        int ix[3] = {-1, -1, -1};
+        bool pass = false;
+        if (lane < ntri) {
+            const int tri = sh.tri[lane];
+            if (terrain) {
+                // ConfigureLocalTriangle (TerrainContactManifold.cs:76-100):
+                // TerrainVertexIndices.ToSequentialIndex for the indices.
+                int col[3], row[3];
+                terrain_local_indices(ti, tri, col, row);
+                f3 v[3];
+                for (int k = 0; k < 3; ++k) {
+                    ix[k] = row[k] * (int)ti.width + col[k];
+                    v[k] = affine_transform(f3{(float)col[k], terrain_height(w, ti, col[k], row[k]), (float)row[k]},
+                                            lm, lt);
+                }
+                t.v0 = v[0];
+                t.v1 = v[1];
+                t.v2 = v[2];
+            } else {
+                // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66): a
+                // leaf holds the triangle's first index position.
+                ix[0] = w.mesh_idx[mi.ibase + tri];
+                ix[1] = w.mesh_idx[mi.ibase + tri + 1];
+                ix[2] = w.mesh_idx[mi.ibase + tri + 2];
+                t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), lm, lt);
+                t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), lm, lt);
+                t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), lm, lt);
+            }
+            t.margin = 0;
+            // Toolbox.GetTriangleBoundingBox against the convex's local box
+            // (TriangleMeshConvexContactManifold.cs:143-150).
+            f3 tmin, tmax;
+            tri_axis(t.v0.x, t.v1.x, t.v2.x, &tmin.x, &tmax.x);
+            tri_axis(t.v0.y, t.v1.y, t.v2.y, &tmin.y, &tmax.y);
+            tri_axis(t.v0.z, t.v1.z, t.v2.z, &tmin.z, &tmax.z);
+            pass = !(tmin.x > lmax.x || tmin.y > lmax.y || tmin.z > lmax.z) &&
+                   !(lmin.x > tmax.x || lmin.y > tmax.y || lmin.z > tmax.z);
+            r.ix[0] = ix[0];
+            r.ix[1] = ix[1];
+            r.ix[2] = ix[2];
+        }
         MeshContact out[2];
-        int n = 0, region_sphere = VR_ABC;
+        int n = -1, region_sphere = VR_ABC;
         bool should_correct = false;
         if (sphere) {
-            n = triangle_sphere(cv.shape.margin, t, sided, &out[0], &region_sphere);
+            if (pass) n = triangle_sphere(cv.shape.margin, t, sided, &out[0], &region_sphere);
         } else {
-            int k = 0;
-            while (k < ntest && !(tab[k].ia == ix[0] && tab[k].ib == ix[1] && tab[k].ic == ix[2])) ++k;
-            if (k == ntest) {
-                if (ntest == kMeshTesters) {
-                    m.overflow = true;
-                    return;
+            // The pair tester of each triangle (TriangleIndices), found in the
+            // table or appended in triangle order.
+            int found = -1;
+            if (pass)
+                for (int q = 0; q < ntest; ++q)
+                    if (sh.tab[q].ia == ix[0] && sh.tab[q].ib == ix[1] && sh.tab[q].ic == ix[2]) {
+                        found = q;
+                        break;
+                    }
+            __syncthreads();  // the chunk's indices are in LDS
+            // A triangle repeated within the chunk (the same TriangleIndices)
+            // must meet the tester its earlier occurrence updated.
+            bool dup = false;
+            if (pass)
+                for (int q = 0; q < lane && !dup; ++q)
+                    dup = sh.r[q].ix[0] == ix[0] && sh.r[q].ix[1] == ix[1] && sh.r[q].ix[2] == ix[2];
+            if (__ballot(dup) == 0) {
+                const uint64_t fresh = __ballot(pass && found < 0);
+                int ts = found;
+                if (pass && found < 0) {
+                    const int rank = __popcll(fresh & below);
+                    if (ntest + rank < kMeshTesters) {
+                        ts = ntest + rank;
+                        sh.tab[ts] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
+                    } else {
+                        m.overflow = true;  // the table is full: this triangle is not tested
+                    }
                 }
-                tab[k] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
-                ++ntest;
-            }
-            updated |= 1u << k;
-            MeshTester tst = tab[k];
-            n = triangle_convex(cv, t, sided, tst, out);
-            tab[k] = tst;
-            should_correct = tst.state == TT_DEEP;
-        }
-        for (int j = 0; j < n; j++) {
-            if (improve) {
-                const int region = sphere ? region_sphere : triangle_convex_region(t, out[j].normal);
-                if (analyze(region, ix, t, sided, should_correct, out[j], m)) {
-                    guaranteed++;
-                    add_local(w, s, e, m, out[j], orientation, ta);
+                ntest = min(ntest + __popcll(fresh), kMeshTesters);
+                updated |= wave_or(ts >= 0 ? 1u << ts : 0u);
+                if (ts >= 0) {
+                    MeshTester tst = sh.tab[ts];
+                    n = triangle_convex(cv, t, sided, tst, out);
+                    sh.tab[ts] = tst;
+                    should_correct = tst.state == TT_DEEP;
                 }
             } else {
-                add_local(w, s, e, m, out[j], orientation, ta);
+                // With a repeat the chunk's triangles go one at a time, in order.
+                for (int q = 0; q < ntri; ++q) {
+                    int ts = -1, nt = ntest;
+                    if (lane == q && pass) {
+                        for (int z = 0; z < ntest; ++z)
+                            if (sh.tab[z].ia == ix[0] && sh.tab[z].ib == ix[1] && sh.tab[z].ic == ix[2]) {
+                                ts = z;
+                                break;
+                            }
+                        if (ts < 0) {
+                            if (ntest < kMeshTesters) {
+                                ts = ntest;
+                                nt = ntest + 1;
+                                sh.tab[ts] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
+                            } else {
+                                m.overflow = true;
+                            }
+                        }
+                        if (ts >= 0) {
+                            MeshTester tst = sh.tab[ts];
+                            n = triangle_convex(cv, t, sided, tst, out);
+                            sh.tab[ts] = tst;
+                            should_correct = tst.state == TT_DEEP;
+                        }
+                    }
+                    ntest = __shfl(nt, q);
+                    ts = __shfl(ts, q);
+                    if (ts >= 0) updated |= 1u << ts;
+                    __syncthreads();
+                }
             }
         }
+        if (lane < ntri) {
+            r.n = n;
+            r.should_correct = should_correct ? 1 : 0;
+            for (int j = 0; j < n; ++j) {
+                r.c[j] = out[j];
+                if (improve) {
+                    r.region[j] = sphere ? region_sphere : triangle_convex_region(t, out[j].normal);
+                    r.corrected[j] = corrected_normal(out[j].normal, t, sided);
+                }
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            for (int q = 0; q < ntri; ++q) {
+                const int nq = sh.r[q].n;
+                for (int j = 0; j < nq; j++) {
+                    const MeshContact c = sh.r[q].c[j];
+                    if (improve) {
+                        const int qix[3] = {sh.r[q].ix[0], sh.r[q].ix[1], sh.r[q].ix[2]};
+                        if (analyze(sh.r[q].region[j], qix, sh.r[q].corrected[j], sh.r[q].should_correct != 0, c,
+                                    m)) {
+                            guaranteed++;
+                            add_local(w, s, e, m, c, orientation, ta);
+                        }
+                    } else {
+                        add_local(w, s, e, m, c, orientation, ta);
+                    }
+                }
+            }
+        }
+        __syncthreads();
     };
 
     if (terrain) {
@@ -382,29 +536,48 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
         if (tv.x > 0) qmax.x += tv.x; else qmin.x += tv.x;
         if (tv.y > 0) qmax.y += tv.y; else qmin.y += tv.y;
         if (tv.z > 0) qmax.z += tv.z; else qmin.z += tv.z;
-        // Kept for the time of impact, which walks the same triangles.
-        e.sep = f4_of(qmin, 1.0f);
-        e.dir = f4_of(qmax, 0.0f);
-        terrain_overlaps(w, ti, qmin, qmax, [&](int tri) {
-            // ConfigureLocalTriangle (TerrainContactManifold.cs:76-100):
-            // TerrainVertexIndices.ToSequentialIndex for the indices.
-            int col[3], row[3], ix[3];
-            terrain_local_indices(ti, tri, col, row);
-            f3 v[3];
-            for (int k = 0; k < 3; ++k) {
-                ix[k] = row[k] * (int)ti.width + col[k];
-                v[k] = affine_transform(f3{(float)col[k], terrain_height(w, ti, col[k], row[k]), (float)row[k]}, lm,
-                                        lt);
+        if (lane == 0) {
+            // Kept for the time of impact, which walks the same triangles.
+            e.sep = f4_of(qmin, 1.0f);
+            e.dir = f4_of(qmax, 0.0f);
+        }
+        __syncthreads();
+        // TerrainShape.GetOverlaps (TerrainShape.cs:497-543): quads of the
+        // covered cells (columns outer, rows inner) whose heights reach the
+        // box, 32 quads (64 triangles) per chunk.
+        const int width = (int)ti.width;
+        const int minX = max((int)qmin.x, 0), minY = max((int)qmin.z, 0);
+        const int maxX = min((int)qmax.x, width - 2), maxY = min((int)qmax.z, (int)ti.length - 2);
+        const int ny = maxY - minY + 1;
+        const int nquads = (maxX >= minX && maxY >= minY) ? (maxX - minX + 1) * ny : 0;
+        for (int q0 = 0; q0 < nquads; q0 += kMeshChunk / 2) {
+            const int qq = q0 + lane;
+            bool hit = false;
+            int quad = 0;
+            if (lane < kMeshChunk / 2 && qq < nquads) {
+                const int i = minX + qq / ny, j = minY + qq % ny;
+                const float y1 = terrain_height(w, ti, i, j), y2 = terrain_height(w, ti, i + 1, j),
+                            y3 = terrain_height(w, ti, i, j + 1), y4 = terrain_height(w, ti, i + 1, j + 1);
+                float highest = y1, lowest = y1;
+                if (y2 > highest) highest = y2; else if (y2 < lowest) lowest = y2;
+                if (y3 > highest) highest = y3; else if (y3 < lowest) lowest = y3;
+                if (y4 > highest) highest = y4; else if (y4 < lowest) lowest = y4;
+                hit = !(qmax.y < lowest || qmin.y > highest);
+                quad = (i + j * width) << 1;
             }
-            cvx::TriShape t;
-            t.v0 = v[0];
-            t.v1 = v[1];
-            t.v2 = v[2];
-            t.margin = 0;
-            triangle(ix, t);
-        });
+            const uint64_t hits = __ballot(hit);
+            if (hit) {
+                const int rank = __popcll(hits & below);
+                sh.tri[2 * rank] = quad;
+                sh.tri[2 * rank + 1] = quad | 1;
+            }
+            __syncthreads();
+            if (hits) run_chunk(2 * __popcll(hits));
+        }
     } else {
-        // MeshBoundingBoxTree.GetOverlaps over the convex's box, depth first.
+        // MeshBoundingBoxTree.GetOverlaps over the convex's box, depth first
+        // (child A's subtree before child B's), walked by lane 0 up to a
+        // chunk of leaves at a time.
         const f3 qmin = f3_of(w.bmin[A]), qmax = f3_of(w.bmax[A]);
         const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
         auto overlaps = [&](int node) {
@@ -413,36 +586,38 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
             if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
             return true;
         };
-        int stack[kMeshStack];
         int top = 0;
-        if (mi.ntri > 0 && overlaps(0)) stack[top++] = 0;
-        while (top > 0) {
-            const int node = stack[--top];
-            const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
-            if (a >= 0) {
-                if (overlaps(b)) {
-                    if (top < kMeshStack) stack[top++] = b;
-                    else m.overflow = true;
+        if (lane == 0 && mi.ntri > 0 && overlaps(0)) sh.stack[top++] = 0;
+        while (true) {
+            if (lane == 0) {
+                int cnt = 0;
+                while (top > 0 && cnt < kMeshChunk) {
+                    const int node = sh.stack[--top];
+                    const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
+                    if (a >= 0) {
+                        if (overlaps(b)) {
+                            if (top < kMeshStack) sh.stack[top++] = b;
+                            else m.overflow = true;
+                        }
+                        if (overlaps(a)) {
+                            if (top < kMeshStack) sh.stack[top++] = a;
+                            else m.overflow = true;
+                        }
+                        continue;
+                    }
+                    sh.tri[cnt++] = b;
                 }
-                if (overlaps(a)) {
-                    if (top < kMeshStack) stack[top++] = a;
-                    else m.overflow = true;
-                }
-                continue;
+                sh.count = cnt;
             }
-            // ConfigureLocalTriangle (StaticMeshContactManifold.cs:51-66): leaf b
-            // is the triangle's first index position.
-            const int ix[3] = {w.mesh_idx[mi.ibase + b], w.mesh_idx[mi.ibase + b + 1],
-                               w.mesh_idx[mi.ibase + b + 2]};
-            cvx::TriShape t;
-            t.v0 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[0]]), lm, lt);
-            t.v1 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[1]]), lm, lt);
-            t.v2 = affine_transform(f3_of(w.mesh_verts[mi.vbase + ix[2]]), lm, lt);
-            t.margin = 0;
-            triangle(ix, t);
+            __syncthreads();
+            const int cnt = sh.count;
+            __syncthreads();
+            if (cnt == 0) break;
+            run_chunk(cnt);
         }
     }
 
+    if (lane != 0) return;
     if (improve) {
         // Edge contacts fighting each other in one plane get the triangle
         // normal (TriangleMeshConvexContactManifold.cs:241-316).
@@ -501,7 +676,7 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
     // one's place (TriangleMeshConvexContactManifold.cs:387-401).
     for (int i = ntest - 1; i >= 0; --i) {
         if (!(updated & (1u << i))) {
-            tab[i] = tab[ntest - 1];
+            sh.tab[i] = sh.tab[ntest - 1];
             if (updated & (1u << (ntest - 1))) updated |= 1u << i;
             --ntest;
         }
@@ -521,28 +696,47 @@ __device__ void update_mesh(const DeviceWorld& w, PairState& s, ConvexState& e, 
     if (m.overflow) w.counters[ctr(CTR_OVERFLOW)] = 5;
 }
 
-constexpr int kMeshBlock = 64;
+static_assert(sizeof(PairState) % 16 == 0 && sizeof(ConvexState) % 16 == 0 && sizeof(MeshTester) % 16 == 0,
+              "pair records are copied as float4s");
 
-__global__ void __launch_bounds__(kMeshBlock) k_mesh(DeviceWorld w) {
+// One wavefront per listed pair (grid-stride over the list).
+__global__ void __launch_bounds__(64, 2) k_mesh(DeviceWorld w) {
+    __shared__ MeshShared sh;
+    const int lane = threadIdx.x;
     const uint32_t n = w.counters[ctr(CTR_MESH)];
-    for (uint32_t k = blockIdx.x * kMeshBlock + threadIdx.x; k < n; k += gridDim.x * kMeshBlock) {
+    constexpr int kPair4 = sizeof(PairState) / 16, kConv4 = sizeof(ConvexState) / 16,
+                  kTester4 = sizeof(MeshTester) / 16;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
         const uint2 wk = w.mesh_work[k];
-        PairState& s = w.pairs_cur[wk.x];
-        ConvexState& e = w.conv_cur[wk.x];
         // k_pair_prep carried last step's pair state, supplement data and
         // tester table over; a new pair starts from zero.
-        if (wk.y == ~0u) {
-            const float4 z = make_float4(0, 0, 0, 0);
-            e.sep = z;
-            e.dir = z;
-            e.offa[0] = z; e.offa[1] = z; e.offa[2] = z; e.offa[3] = z;
-            e.offb[0] = z; e.offb[1] = z; e.offb[2] = z; e.offb[3] = z;
+        const float4* ps = reinterpret_cast<const float4*>(&w.pairs_cur[wk.x]);
+        float4* pd = reinterpret_cast<float4*>(&sh.s);
+        if (lane < kPair4) pd[lane] = ps[lane];
+        const float4* cs = reinterpret_cast<const float4*>(&w.conv_cur[wk.x]);
+        float4* cd = reinterpret_cast<float4*>(&sh.e);
+        if (lane < kConv4) cd[lane] = wk.y == ~0u ? make_float4(0, 0, 0, 0) : cs[lane];
+        const int nt = w.pairs_cur[wk.x].tester * kTester4;
+        const float4* ts = reinterpret_cast<const float4*>(w.mtest_cur + (size_t)wk.x * kMeshTesters);
+        float4* td = reinterpret_cast<float4*>(sh.tab);
+        for (int i = lane; i < nt; i += 64) td[i] = ts[i];
+        __syncthreads();
+        mesh_pair(w, sh, lane);
+        __syncthreads();
+        float4* pw = reinterpret_cast<float4*>(&w.pairs_cur[wk.x]);
+        if (lane < kPair4) pw[lane] = pd[lane];
+        float4* cw = reinterpret_cast<float4*>(&w.conv_cur[wk.x]);
+        if (lane < kConv4) cw[lane] = cd[lane];
+        const int nw = sh.s.tester * kTester4;
+        float4* tw = reinterpret_cast<float4*>(w.mtest_cur + (size_t)wk.x * kMeshTesters);
+        for (int i = lane; i < nw; i += 64) tw[i] = td[i];
+        if (lane == 0) {
+            w.hot_cur[wk.x] = hot_of(sh.s);
+            if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[wk.x], wk.x))
+                w.counters[ctr(CTR_OVERFLOW)] = 3;
         }
-        update_mesh(w, s, e, w.mtest_cur + (size_t)wk.x * kMeshTesters);
-        w.hot_cur[wk.x] = hot_of(s);
-        if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[wk.x], wk.x))
-            w.counters[ctr(CTR_OVERFLOW)] = 3;
         // The mesh is not dynamic: a mesh pair joins no islands.
+        __syncthreads();
     }
 }
 
@@ -550,8 +744,8 @@ __global__ void __launch_bounds__(kMeshBlock) k_mesh(DeviceWorld w) {
 
 void launch_mesh(const DeviceWorld& w, hipStream_t st) {
     if (!w.meshes || w.pair_cap == 0) return;
-    const uint32_t grid = std::min<uint32_t>(4096, (w.pair_cap + kMeshBlock - 1) / kMeshBlock);
-    hipLaunchKernelGGL(k_mesh, dim3(grid), dim3(kMeshBlock), 0, st, w);
+    const uint32_t grid = std::min<uint32_t>(32768, w.pair_cap);
+    hipLaunchKernelGGL(k_mesh, dim3(grid), dim3(64), 0, st, w);
 }
 
 }  // namespace bepu
