@@ -91,7 +91,7 @@ __device__ __forceinline__ f3 corrected_normal(f3 uncorrected, const cvx::TriSha
 // face contact, which is always kept; an edge or vertex contact is queued
 // (with `corrected`, GetNormal of its triangle) and every other region of the
 // triangle blocked.
-__device__ bool analyze(int region, const int ix[3], f3 corrected, bool should_correct, const MeshContact& c,
+__device__ __forceinline__ bool analyze(int region, const int ix[3], f3 corrected, bool should_correct, const MeshContact& c,
                         MeshScratch& m) {
     const int A = ix[0], B = ix[1], C = ix[2];
     if (region <= VR_C) {
@@ -142,7 +142,7 @@ __device__ __forceinline__ void add_mesh_contact(PairState& s, ConvexState& e, c
 // 436-447, 617-674): the candidate goes to world space; a contact of the
 // manifold at the same place with a nearly parallel normal takes its data
 // instead, a candidate already queued there drops it.
-__device__ void add_local(const DeviceWorld& w, PairState& s, ConvexState& e, MeshScratch& m, MeshContact c,
+__device__ __forceinline__ void add_local(const DeviceWorld& w, PairState& s, ConvexState& e, MeshScratch& m, MeshContact c,
                           const m3& orientation, const cvx::RT& ta) {
     c.pos = mat_transform(c.pos, orientation) + ta.pos;
     c.normal = mat_transform(c.normal, orientation);
@@ -172,7 +172,7 @@ __device__ void add_local(const DeviceWorld& w, PairState& s, ConvexState& e, Me
 // ContactReducer.ReduceContacts, general form (CollisionTests/ContactReducer.cs:19-195):
 // the manifold's contacts (list order) and the candidates; the list indices
 // to remove (ascending, as a bit mask) and the candidates kept (bit mask).
-__device__ void reduce_general(const PairState& s, const MeshScratch& m, uint32_t* remove, uint32_t* keep) {
+__device__ __forceinline__ void reduce_general(const PairState& s, const MeshScratch& m, uint32_t* remove, uint32_t* keep) {
     const int nc = s.ccount;
     *remove = 0;
     *keep = 0;
@@ -249,46 +249,52 @@ struct TriOut {
     int should_correct;
 };
 
-constexpr int kMeshChunk = 64;  // triangles tested side by side, one per lane
-
 // A pair's update in LDS: its state, tester state and tester table (loaded
 // from and stored back to the pair slot), the manifold's scratch, the
-// chunk's triangles and their tester results, the tree walk's stack.
+// chunk's G triangles and their tester results, the tree walk's stack.
+template <int G>
 struct MeshShared {
     PairState s;
     ConvexState e;
     MeshTester tab[kMeshTesters];
     MeshScratch m;
-    TriOut r[kMeshChunk];
-    int tri[kMeshChunk];
+    TriOut r[G];
+    int tri[G];
     int stack[kMeshStack];
     int count;
 };
 
-// OR of a per-lane word over the wavefront.
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off);
+// OR of a per-lane word over the lanes of a group of G.
+template <int G>
+__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+    for (int off = G / 2; off > 0; off >>= 1) v |= (uint32_t)__shfl_xor((int)v, off);
     return v;
 }
 
-// The pair's update by one wavefront (TriangleMeshConvexContactManifold.Update,
-// TriangleMeshConvexContactManifold.cs:101-434).  Overlapped triangles are
-// gathered in order, up to one per lane; each lane carries its triangle into
-// the convex's local space, prunes it by the convex's local box and runs its
-// pair tester (TriangleConvexPairTester state from the pair's table, or the
-// TriangleSpherePairTester); lane 0 then merges the chunk's candidates in the
-// order of the triangles (boundary analysis, AddLocalContact), so the result
-// is the one the reference's serial loop gives.
-__device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
+// The update of one pair (TriangleMeshConvexContactManifold.Update,
+// TriangleMeshConvexContactManifold.cs:101-434) by a group of G lanes of the
+// wavefront (`gl` the lane in the group, `gb` the group's first lane; an
+// inactive group has no pair and only keeps step with the wavefront's
+// barriers).  Overlapped triangles are gathered in order, up to one per
+// lane; each lane carries its triangle into the convex's local space, prunes
+// it by the convex's local box and runs its pair tester (the
+// TriangleConvexPairTester state from the pair's table, or the
+// TriangleSpherePairTester); the group's first lane then merges the chunk's
+// candidates in the order of the triangles (boundary analysis,
+// AddLocalContact), so the result is the one the reference's serial loop
+// gives.  Every barrier is reached by the whole wavefront.
+template <int G>
+__device__ __forceinline__ void mesh_pair(const DeviceWorld& w, MeshShared<G>& sh, int gl, int gb, bool active) {
     PairState& s = sh.s;
     ConvexState& e = sh.e;
     MeshScratch& m = sh.m;
-    const uint32_t A = s.a, M = s.b;
-    const float4 PA = w.pos[A], QA = w.quat[A];
+    const uint32_t A = active ? s.a : 0u, M = active ? s.b : 0u;
+    const float4 PA = active ? w.pos[A] : make_float4(0, 0, 0, 0);
+    const float4 QA = active ? w.quat[A] : make_float4(0, 0, 0, 1);
     const cvx::RT ta{f3_of(PA), q4{QA.x, QA.y, QA.z, QA.w}};
-    const bool sphere = s.kind == BEPU_PAIR_MESH_SPHERE;
-    int ntest = sphere ? 0 : s.tester;
-    if (lane == 0) {
+    const bool sphere = active && s.kind == BEPU_PAIR_MESH_SPHERE;
+    int ntest = active && !sphere ? s.tester : 0;
+    if (gl == 0 && active) {
         // ContactRefresher.ContactRefresh (ContactRefresher.cs:18-69)
         int rem[4], nrem = 0;
         for (int k = 0; k < s.ccount; k++) {
@@ -315,20 +321,20 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
         m.overflow = false;
     }
 
-    const float4 msh = w.shape[M];
-    const bool terrain = msh.w == (float)BEPU_SHAPE_TERRAIN;
+    const float4 msh = active ? w.shape[M] : make_float4(0, 0, 0, 0);
+    const bool terrain = active && msh.w == (float)BEPU_SHAPE_TERRAIN;
     const int sided = (int)msh.y;
     const bool improve = msh.z != 0.0f;
     // The shape's world transform: the mesh shape's, or the terrain's.
-    m3 sm;
-    f3 st;
-    MeshInfo mi;
-    TerrainInfo ti;
+    m3 sm{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    f3 st{0, 0, 0};
+    MeshInfo mi{};
+    TerrainInfo ti{};
     if (terrain) {
         ti = terrain_info(w, __float_as_uint(msh.x));
         sm = ti.m;
         st = ti.t;
-    } else {
+    } else if (active) {
         mi = mesh_info(w, __float_as_uint(msh.x));
         sm = mi.m;
         st = mi.t;
@@ -340,27 +346,28 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
     const m3 im = mat_from_quat(iq);
     const m3 lm = mat_mul(sm, im);
     const f3 lt = mat_transform(st, im) + ip;
-    f3 lmin, lmax;
-    shape_aabb(w, w.shape[A], f3{0, 0, 0}, q4{0, 0, 0, 1}, &lmin, &lmax);
+    f3 lmin{0, 0, 0}, lmax{0, 0, 0};
+    if (active) shape_aabb(w, w.shape[A], f3{0, 0, 0}, q4{0, 0, 0, 1}, &lmin, &lmax);
     const m3 orientation = mat_from_quat(ta.q);
-    const float4 sha = w.shape[A];
+    const float4 sha = active ? w.shape[A] : make_float4(0, 0, 0, (float)BEPU_SHAPE_SPHERE);
     TriConvex cv;
     cv.shape = shape_of(w, A, w.hull_verts + __float_as_uint(sha.y), 1);
-    cv.min_radius = w.minrad[A];
+    cv.min_radius = active ? w.minrad[A] : 0.0f;
     cv.core_scaling = w.s.core_shape_scaling;
     uint32_t updated = 0;
     int guaranteed = 0;
-    const uint64_t below = (1ull << lane) - 1;  // lanes before this one
+    const uint64_t below = ((1ull << gl) - 1) << gb;  // the group's lanes before this one
 
-    // Tests the chunk's `ntri` triangles (sh.tri, in order), one per lane,
-    // then merges their candidates in order on lane 0.
-    auto run_chunk = [&](int ntri) {
-        TriOut& r = sh.r[lane];
+    // Tests the group's chunk of `ntri` triangles (sh.tri, in order; 0 when
+    // the group has none this pass), one per lane, then merges their
+    // candidates in order on the group's first lane.
+    auto run_chunk = [&](int ntri) __attribute__((always_inline)) {
+        TriOut& r = sh.r[gl];
         cvx::TriShape t;
         int ix[3] = {-1, -1, -1};
         bool pass = false;
-        if (lane < ntri) {
-            const int tri = sh.tri[lane];
+        if (gl < ntri) {
+            const int tri = sh.tri[gl];
             if (terrain) {
                 // ConfigureLocalTriangle (TerrainContactManifold.cs:76-100):
                 // TerrainVertexIndices.ToSequentialIndex for the indices.
@@ -401,79 +408,79 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
         MeshContact out[2];
         int n = -1, region_sphere = VR_ABC;
         bool should_correct = false;
-        if (sphere) {
-            if (pass) n = triangle_sphere(cv.shape.margin, t, sided, &out[0], &region_sphere);
-        } else {
-            // The pair tester of each triangle (TriangleIndices), found in the
-            // table or appended in triangle order.
-            int found = -1;
-            if (pass)
-                for (int q = 0; q < ntest; ++q)
-                    if (sh.tab[q].ia == ix[0] && sh.tab[q].ib == ix[1] && sh.tab[q].ic == ix[2]) {
-                        found = q;
-                        break;
-                    }
-            __syncthreads();  // the chunk's indices are in LDS
-            // A triangle repeated within the chunk (the same TriangleIndices)
-            // must meet the tester its earlier occurrence updated.
-            bool dup = false;
-            if (pass)
-                for (int q = 0; q < lane && !dup; ++q)
-                    dup = sh.r[q].ix[0] == ix[0] && sh.r[q].ix[1] == ix[1] && sh.r[q].ix[2] == ix[2];
-            if (__ballot(dup) == 0) {
-                const uint64_t fresh = __ballot(pass && found < 0);
-                int ts = found;
-                if (pass && found < 0) {
-                    const int rank = __popcll(fresh & below);
-                    if (ntest + rank < kMeshTesters) {
-                        ts = ntest + rank;
-                        sh.tab[ts] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
-                    } else {
-                        m.overflow = true;  // the table is full: this triangle is not tested
-                    }
+        // The pair tester of each triangle (TriangleIndices), found in the
+        // table or appended in triangle order.
+        const bool tconv = pass && !sphere;
+        int found = -1;
+        if (tconv)
+            for (int q = 0; q < ntest; ++q)
+                if (sh.tab[q].ia == ix[0] && sh.tab[q].ib == ix[1] && sh.tab[q].ic == ix[2]) {
+                    found = q;
+                    break;
                 }
-                ntest = min(ntest + __popcll(fresh), kMeshTesters);
-                updated |= wave_or(ts >= 0 ? 1u << ts : 0u);
-                if (ts >= 0) {
-                    MeshTester tst = sh.tab[ts];
-                    n = triangle_convex(cv, t, sided, tst, out);
-                    sh.tab[ts] = tst;
-                    should_correct = tst.state == TT_DEEP;
-                }
-            } else {
-                // With a repeat the chunk's triangles go one at a time, in order.
-                for (int q = 0; q < ntri; ++q) {
-                    int ts = -1, nt = ntest;
-                    if (lane == q && pass) {
-                        for (int z = 0; z < ntest; ++z)
-                            if (sh.tab[z].ia == ix[0] && sh.tab[z].ib == ix[1] && sh.tab[z].ic == ix[2]) {
-                                ts = z;
-                                break;
-                            }
-                        if (ts < 0) {
-                            if (ntest < kMeshTesters) {
-                                ts = ntest;
-                                nt = ntest + 1;
-                                sh.tab[ts] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
-                            } else {
-                                m.overflow = true;
-                            }
-                        }
-                        if (ts >= 0) {
-                            MeshTester tst = sh.tab[ts];
-                            n = triangle_convex(cv, t, sided, tst, out);
-                            sh.tab[ts] = tst;
-                            should_correct = tst.state == TT_DEEP;
-                        }
-                    }
-                    ntest = __shfl(nt, q);
-                    ts = __shfl(ts, q);
-                    if (ts >= 0) updated |= 1u << ts;
-                    __syncthreads();
+        __syncthreads();  // the chunk's indices are in LDS
+        // A triangle repeated within the chunk (the same TriangleIndices)
+        // must meet the tester its earlier occurrence updated.
+        bool dup = false;
+        if (tconv)
+            for (int q = 0; q < gl && !dup; ++q)
+                dup = sh.r[q].ix[0] == ix[0] && sh.r[q].ix[1] == ix[1] && sh.r[q].ix[2] == ix[2];
+        if (__ballot(dup) == 0) {
+            const uint64_t fresh = __ballot(tconv && found < 0);
+            int ts = found;
+            if (tconv && found < 0) {
+                const int rank = __popcll(fresh & below);
+                if (ntest + rank < kMeshTesters) {
+                    ts = ntest + rank;
+                    sh.tab[ts] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
+                } else {
+                    m.overflow = true;  // the table is full: this triangle is not tested
                 }
             }
+            const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gb;
+            ntest = min(ntest + __popcll(fresh & gmask), kMeshTesters);
+            updated |= group_or<G>(ts >= 0 ? 1u << ts : 0u);
+            if (ts >= 0) {
+                MeshTester tst = sh.tab[ts];
+                n = triangle_convex(cv, t, sided, tst, out);
+                sh.tab[ts] = tst;
+                should_correct = tst.state == TT_DEEP;
+            }
+        } else {
+            // With a repeat in any group of the wavefront, every group takes
+            // its chunk's triangles one at a time, in order.
+            for (int q = 0; q < G; ++q) {
+                int ts = -1, nt = ntest;
+                if (gl == q && tconv) {
+                    for (int z = 0; z < ntest; ++z)
+                        if (sh.tab[z].ia == ix[0] && sh.tab[z].ib == ix[1] && sh.tab[z].ic == ix[2]) {
+                            ts = z;
+                            break;
+                        }
+                    if (ts < 0) {
+                        if (ntest < kMeshTesters) {
+                            ts = ntest;
+                            nt = ntest + 1;
+                            sh.tab[ts] = MeshTester{ix[0], ix[1], ix[2], TT_PLANE, 0, 0.0f, 0.0f, 0.0f};
+                        } else {
+                            m.overflow = true;
+                        }
+                    }
+                    if (ts >= 0) {
+                        MeshTester tst = sh.tab[ts];
+                        n = triangle_convex(cv, t, sided, tst, out);
+                        sh.tab[ts] = tst;
+                        should_correct = tst.state == TT_DEEP;
+                    }
+                }
+                ntest = __shfl(nt, gb + q);
+                ts = __shfl(ts, gb + q);
+                if (ts >= 0) updated |= 1u << ts;
+                __syncthreads();
+            }
         }
-        if (lane < ntri) {
+        if (sphere && pass) n = triangle_sphere(cv.shape.margin, t, sided, &out[0], &region_sphere);
+        if (gl < ntri) {
             r.n = n;
             r.should_correct = should_correct ? 1 : 0;
             for (int j = 0; j < n; ++j) {
@@ -485,7 +492,7 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
             }
         }
         __syncthreads();
-        if (lane == 0) {
+        if (gl == 0) {
             for (int q = 0; q < ntri; ++q) {
                 const int nq = sh.r[q].n;
                 for (int j = 0; j < nq; j++) {
@@ -506,8 +513,12 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
         __syncthreads();
     };
 
+    // Terrain pairs: TerrainContactManifold.FindOverlappingTriangles
+    // (TerrainContactManifold.cs:30-64) and TerrainShape.GetOverlaps
+    // (TerrainShape.cs:497-543).
+    int minX = 0, minY = 0, ny = 1, nquads = 0, width = 0;
+    f3 qmin{0, 0, 0}, qmax{0, 0, 0};
     if (terrain) {
-        // TerrainContactManifold.FindOverlappingTriangles (TerrainContactManifold.cs:30-64):
         // ConvexShape.GetLocalBoundingBox (ConvexShape.cs:225-281) of the
         // convex in the terrain's frame (AffineTransform.Invert,
         // AffineTransform.cs:173-178; Multiply(RigidTransform,
@@ -529,95 +540,93 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
         const f3 neg{left.x * bm.m11 + left.y * bm.m21 + left.z * bm.m31,
                      down.x * bm.m12 + down.y * bm.m22 + down.z * bm.m32,
                      forward.x * bm.m13 + forward.y * bm.m23 + forward.z * bm.m33};
-        f3 qmax{bt.x + pos.x, bt.y + pos.y, bt.z + pos.z};
-        f3 qmin{bt.x + neg.x, bt.y + neg.y, bt.z + neg.z};
+        qmax = f3{bt.x + pos.x, bt.y + pos.y, bt.z + pos.z};
+        qmin = f3{bt.x + neg.x, bt.y + neg.y, bt.z + neg.z};
         f3 tv = mat_transform(f3_of(w.linv[A]), mat_invert(ti.m));
         tv = tv * w.s.dt;
         if (tv.x > 0) qmax.x += tv.x; else qmin.x += tv.x;
         if (tv.y > 0) qmax.y += tv.y; else qmin.y += tv.y;
         if (tv.z > 0) qmax.z += tv.z; else qmin.z += tv.z;
-        if (lane == 0) {
+        if (gl == 0) {
             // Kept for the time of impact, which walks the same triangles.
             e.sep = f4_of(qmin, 1.0f);
             e.dir = f4_of(qmax, 0.0f);
         }
-        __syncthreads();
-        // TerrainShape.GetOverlaps (TerrainShape.cs:497-543): quads of the
-        // covered cells (columns outer, rows inner) whose heights reach the
-        // box, 32 quads (64 triangles) per chunk.
-        const int width = (int)ti.width;
-        const int minX = max((int)qmin.x, 0), minY = max((int)qmin.z, 0);
+        width = (int)ti.width;
+        minX = max((int)qmin.x, 0);
+        minY = max((int)qmin.z, 0);
         const int maxX = min((int)qmax.x, width - 2), maxY = min((int)qmax.z, (int)ti.length - 2);
-        const int ny = maxY - minY + 1;
-        const int nquads = (maxX >= minX && maxY >= minY) ? (maxX - minX + 1) * ny : 0;
-        for (int q0 = 0; q0 < nquads; q0 += kMeshChunk / 2) {
-            const int qq = q0 + lane;
-            bool hit = false;
-            int quad = 0;
-            if (lane < kMeshChunk / 2 && qq < nquads) {
-                const int i = minX + qq / ny, j = minY + qq % ny;
-                const float y1 = terrain_height(w, ti, i, j), y2 = terrain_height(w, ti, i + 1, j),
-                            y3 = terrain_height(w, ti, i, j + 1), y4 = terrain_height(w, ti, i + 1, j + 1);
-                float highest = y1, lowest = y1;
-                if (y2 > highest) highest = y2; else if (y2 < lowest) lowest = y2;
-                if (y3 > highest) highest = y3; else if (y3 < lowest) lowest = y3;
-                if (y4 > highest) highest = y4; else if (y4 < lowest) lowest = y4;
-                hit = !(qmax.y < lowest || qmin.y > highest);
-                quad = (i + j * width) << 1;
-            }
-            const uint64_t hits = __ballot(hit);
-            if (hit) {
-                const int rank = __popcll(hits & below);
-                sh.tri[2 * rank] = quad;
-                sh.tri[2 * rank + 1] = quad | 1;
-            }
-            __syncthreads();
-            if (hits) run_chunk(2 * __popcll(hits));
+        ny = maxY - minY + 1;
+        nquads = (maxX >= minX && maxY >= minY) ? (maxX - minX + 1) * ny : 0;
+    }
+    // Static mesh pairs: MeshBoundingBoxTree.GetOverlaps over the convex's
+    // box, depth first (child A's subtree before child B's), walked by the
+    // group's first lane up to a chunk of leaves at a time.
+    const bool mesh = active && !terrain;
+    const f3 bmin = mesh ? f3_of(w.bmin[A]) : f3{0, 0, 0}, bmax = mesh ? f3_of(w.bmax[A]) : f3{0, 0, 0};
+    const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
+    auto overlaps = [&](int node) {
+        const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
+        if (bmin.x > hi.x || bmin.y > hi.y || bmin.z > hi.z) return false;
+        if (lo.x > bmax.x || lo.y > bmax.y || lo.z > bmax.z) return false;
+        return true;
+    };
+    int top = 0;
+    if (gl == 0 && mesh && mi.ntri > 0 && overlaps(0)) sh.stack[top++] = 0;
+    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gb;
+    // One pass per chunk: a terrain's next G / 2 covered cells (columns
+    // outer, rows inner) give the triangles of those quads whose heights
+    // reach the box; a mesh's walk gives its next G leaves.
+    for (int q0 = 0;; q0 += G / 2) {
+        bool hit = false;
+        int quad = 0;
+        const int qq = q0 + gl;
+        if (terrain && gl < G / 2 && qq < nquads) {
+            const int i = minX + qq / ny, j = minY + qq % ny;
+            const float y1 = terrain_height(w, ti, i, j), y2 = terrain_height(w, ti, i + 1, j),
+                        y3 = terrain_height(w, ti, i, j + 1), y4 = terrain_height(w, ti, i + 1, j + 1);
+            float highest = y1, lowest = y1;
+            if (y2 > highest) highest = y2; else if (y2 < lowest) lowest = y2;
+            if (y3 > highest) highest = y3; else if (y3 < lowest) lowest = y3;
+            if (y4 > highest) highest = y4; else if (y4 < lowest) lowest = y4;
+            hit = !(qmax.y < lowest || qmin.y > highest);
+            quad = (i + j * width) << 1;
         }
-    } else {
-        // MeshBoundingBoxTree.GetOverlaps over the convex's box, depth first
-        // (child A's subtree before child B's), walked by lane 0 up to a
-        // chunk of leaves at a time.
-        const f3 qmin = f3_of(w.bmin[A]), qmax = f3_of(w.bmax[A]);
-        const float4* nodes = w.mesh_nodes + 2 * (size_t)mi.nbase;
-        auto overlaps = [&](int node) {
-            const float4 lo = nodes[2 * node], hi = nodes[2 * node + 1];
-            if (qmin.x > hi.x || qmin.y > hi.y || qmin.z > hi.z) return false;
-            if (lo.x > qmax.x || lo.y > qmax.y || lo.z > qmax.z) return false;
-            return true;
-        };
-        int top = 0;
-        if (lane == 0 && mi.ntri > 0 && overlaps(0)) sh.stack[top++] = 0;
-        while (true) {
-            if (lane == 0) {
-                int cnt = 0;
-                while (top > 0 && cnt < kMeshChunk) {
-                    const int node = sh.stack[--top];
-                    const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
-                    if (a >= 0) {
-                        if (overlaps(b)) {
-                            if (top < kMeshStack) sh.stack[top++] = b;
-                            else m.overflow = true;
-                        }
-                        if (overlaps(a)) {
-                            if (top < kMeshStack) sh.stack[top++] = a;
-                            else m.overflow = true;
-                        }
-                        continue;
+        const uint64_t hits = __ballot(hit);
+        if (hit) {
+            const int rank = __popcll(hits & below);
+            sh.tri[2 * rank] = quad;
+            sh.tri[2 * rank + 1] = quad | 1;
+        }
+        if (gl == 0) {
+            int cnt = 0;
+            while (top > 0 && cnt < G) {
+                const int node = sh.stack[--top];
+                const int a = __float_as_int(nodes[2 * node].w), b = __float_as_int(nodes[2 * node + 1].w);
+                if (a >= 0) {
+                    if (overlaps(b)) {
+                        if (top < kMeshStack) sh.stack[top++] = b;
+                        else m.overflow = true;
                     }
-                    sh.tri[cnt++] = b;
+                    if (overlaps(a)) {
+                        if (top < kMeshStack) sh.stack[top++] = a;
+                        else m.overflow = true;
+                    }
+                    continue;
                 }
-                sh.count = cnt;
+                sh.tri[cnt++] = b;
             }
-            __syncthreads();
-            const int cnt = sh.count;
-            __syncthreads();
-            if (cnt == 0) break;
-            run_chunk(cnt);
+            sh.count = terrain ? 2 * __popcll(hits & gmask) : cnt;
         }
+        const bool more = terrain ? q0 + G / 2 < nquads : __shfl(top, gb) > 0;
+        __syncthreads();
+        const int cnt = sh.count;
+        __syncthreads();
+        if (__ballot(cnt > 0 || more) == 0) break;
+        run_chunk(cnt);
     }
 
-    if (lane != 0) return;
+    if (gl != 0 || !active) return;
     if (improve) {
         // Edge contacts fighting each other in one plane get the triangle
         // normal (TriangleMeshConvexContactManifold.cs:241-316).
@@ -699,41 +708,52 @@ __device__ void mesh_pair(const DeviceWorld& w, MeshShared& sh, int lane) {
 static_assert(sizeof(PairState) % 16 == 0 && sizeof(ConvexState) % 16 == 0 && sizeof(MeshTester) % 16 == 0,
               "pair records are copied as float4s");
 
-// One wavefront per listed pair (grid-stride over the list).
-__global__ void __launch_bounds__(64, 2) k_mesh(DeviceWorld w) {
-    __shared__ MeshShared sh;
-    const int lane = threadIdx.x;
+// Lanes per pair: four pairs per wavefront.
+constexpr int kMeshGroup = 16;
+
+// One group of kMeshGroup lanes per listed pair, 64 / kMeshGroup pairs per
+// single-wavefront workgroup (grid-stride over the list).
+__global__ void __launch_bounds__(64) k_mesh(DeviceWorld w) {
+    constexpr int G = kMeshGroup, P = 64 / G;
+    __shared__ MeshShared<G> shared[P];
+    const int lane = threadIdx.x, g = lane / G, gl = lane % G, gb = g * G;
+    MeshShared<G>& sh = shared[g];
     const uint32_t n = w.counters[ctr(CTR_MESH)];
     constexpr int kPair4 = sizeof(PairState) / 16, kConv4 = sizeof(ConvexState) / 16,
                   kTester4 = sizeof(MeshTester) / 16;
-    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-        const uint2 wk = w.mesh_work[k];
-        // k_pair_prep carried last step's pair state, supplement data and
-        // tester table over; a new pair starts from zero.
-        const float4* ps = reinterpret_cast<const float4*>(&w.pairs_cur[wk.x]);
+    for (uint32_t base = blockIdx.x * P; base < n; base += gridDim.x * P) {
+        const bool active = base + g < n;
+        const uint2 wk = active ? w.mesh_work[base + g] : make_uint2(0, 0);
         float4* pd = reinterpret_cast<float4*>(&sh.s);
-        if (lane < kPair4) pd[lane] = ps[lane];
-        const float4* cs = reinterpret_cast<const float4*>(&w.conv_cur[wk.x]);
         float4* cd = reinterpret_cast<float4*>(&sh.e);
-        if (lane < kConv4) cd[lane] = wk.y == ~0u ? make_float4(0, 0, 0, 0) : cs[lane];
-        const int nt = w.pairs_cur[wk.x].tester * kTester4;
-        const float4* ts = reinterpret_cast<const float4*>(w.mtest_cur + (size_t)wk.x * kMeshTesters);
         float4* td = reinterpret_cast<float4*>(sh.tab);
-        for (int i = lane; i < nt; i += 64) td[i] = ts[i];
+        if (active) {
+            // k_pair_prep carried last step's pair state, supplement data and
+            // tester table over; a new pair starts from zero.
+            const float4* ps = reinterpret_cast<const float4*>(&w.pairs_cur[wk.x]);
+            for (int i = gl; i < kPair4; i += G) pd[i] = ps[i];
+            const float4* cs = reinterpret_cast<const float4*>(&w.conv_cur[wk.x]);
+            for (int i = gl; i < kConv4; i += G) cd[i] = wk.y == ~0u ? make_float4(0, 0, 0, 0) : cs[i];
+            const int nt = w.pairs_cur[wk.x].tester * kTester4;
+            const float4* ts = reinterpret_cast<const float4*>(w.mtest_cur + (size_t)wk.x * kMeshTesters);
+            for (int i = gl; i < nt; i += G) td[i] = ts[i];
+        }
         __syncthreads();
-        mesh_pair(w, sh, lane);
+        mesh_pair<G>(w, sh, gl, gb, active);
         __syncthreads();
-        float4* pw = reinterpret_cast<float4*>(&w.pairs_cur[wk.x]);
-        if (lane < kPair4) pw[lane] = pd[lane];
-        float4* cw = reinterpret_cast<float4*>(&w.conv_cur[wk.x]);
-        if (lane < kConv4) cw[lane] = cd[lane];
-        const int nw = sh.s.tester * kTester4;
-        float4* tw = reinterpret_cast<float4*>(w.mtest_cur + (size_t)wk.x * kMeshTesters);
-        for (int i = lane; i < nw; i += 64) tw[i] = td[i];
-        if (lane == 0) {
-            w.hot_cur[wk.x] = hot_of(sh.s);
-            if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[wk.x], wk.x))
-                w.counters[ctr(CTR_OVERFLOW)] = 3;
+        if (active) {
+            float4* pw = reinterpret_cast<float4*>(&w.pairs_cur[wk.x]);
+            for (int i = gl; i < kPair4; i += G) pw[i] = pd[i];
+            float4* cw = reinterpret_cast<float4*>(&w.conv_cur[wk.x]);
+            for (int i = gl; i < kConv4; i += G) cw[i] = cd[i];
+            const int nw = sh.s.tester * kTester4;
+            float4* tw = reinterpret_cast<float4*>(w.mtest_cur + (size_t)wk.x * kMeshTesters);
+            for (int i = gl; i < nw; i += G) tw[i] = td[i];
+            if (gl == 0) {
+                w.hot_cur[wk.x] = hot_of(sh.s);
+                if (!hash_insert(w.hash_keys_cur, w.hash_vals_cur, w.hash_cap, w.pair_keys[wk.x], wk.x))
+                    w.counters[ctr(CTR_OVERFLOW)] = 3;
+            }
         }
         // The mesh is not dynamic: a mesh pair joins no islands.
         __syncthreads();
@@ -744,7 +764,8 @@ __global__ void __launch_bounds__(64, 2) k_mesh(DeviceWorld w) {
 
 void launch_mesh(const DeviceWorld& w, hipStream_t st) {
     if (!w.meshes || w.pair_cap == 0) return;
-    const uint32_t grid = std::min<uint32_t>(32768, w.pair_cap);
+    constexpr uint32_t P = 64 / kMeshGroup;
+    const uint32_t grid = std::min<uint32_t>(16384, (w.pair_cap + P - 1) / P);
     hipLaunchKernelGGL(k_mesh, dim3(grid), dim3(64), 0, st, w);
 }
 

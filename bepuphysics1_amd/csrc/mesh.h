@@ -307,7 +307,7 @@ const cvx::RT kIdentityRT{f3{0, 0, 0}, q4{0, 0, 0, 1}};
 // (along the three edge normals, with a one-sided triangle's correction, then
 // along both face normals, then RefinePenetration's casts along the current
 // normal) run through one call site.
-__device__ __noinline__ int deep(const TriConvex& cv, const cvx::TriShape& t, int sided, MeshTester& st,
+__device__ __forceinline__ int deep(const TriConvex& cv, const cvx::TriShape& t, int sided, MeshTester& st,
                                  MeshContact out[2]) {
     f3 center = (t.v0 + t.v1) + t.v2;
     center = center * (1.f / 3.f);
@@ -410,7 +410,7 @@ __device__ __noinline__ int deep(const TriConvex& cv, const cvx::TriShape& t, in
 }
 
 // DoExternalNear (TriangleConvexPairTester.cs:230-298)
-__device__ __noinline__ int near(const TriConvex& cv, const cvx::TriShape& t, int sided, MeshTester& st,
+__device__ __forceinline__ int near(const TriConvex& cv, const cvx::TriShape& t, int sided, MeshTester& st,
                                  MeshContact out[2]) {
     f3 centroid = (t.v0 + t.v1) + t.v2;
     centroid = centroid * .33333333f;
@@ -446,7 +446,7 @@ __device__ __noinline__ int near(const TriConvex& cv, const cvx::TriShape& t, in
 }
 
 // DoExternalSeparated (TriangleConvexPairTester.cs:215-228)
-__device__ __noinline__ int separated(const TriConvex& cv, const cvx::TriShape& t, int sided, MeshTester& st,
+__device__ __forceinline__ int separated(const TriConvex& cv, const cvx::TriShape& t, int sided, MeshTester& st,
                                       MeshContact out[2]) {
     f3 axis{st.ax, st.ay, st.az};
     const bool hit = cvx::gjk_intersecting(cv.shape, t, kIdentityRT, kIdentityRT, &axis);

@@ -106,6 +106,7 @@ def lib():
         L.refcpu_terrain_overlaps.argtypes = [vp, ctypes.c_uint32, fpp, fpp, ip, ctypes.c_uint32, u32p]
         L.refcpu_terrain_triangle.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, fpp, ip]
         L.refcpu_terrain_query_box.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, fpp]
+        L.refcpu_mesh_peaks.argtypes = [vp, ip]
         L.refcpu_triangle_convex.argtypes = [fpp, fpp, ctypes.c_int, ctypes.c_float, ctypes.c_float, fpp,
                                              ctypes.c_int, ctypes.c_float, ip, fpp, fpp]
         L.refcpu_triangle_region.argtypes = [fpp, fpp]
@@ -290,6 +291,14 @@ class World:
         if rc != 0:
             raise ValueError("refcpu_terrain_triangle failed with status %d" % rc)
         return v.reshape(3, 3), (None if world else ix)
+
+    def mesh_peaks(self):
+        """Largest per-pair sizes met by the mesh manifold updates so far:
+        candidates, edge contacts, vertex contacts, blocked vertices, blocked
+        edges, pair testers (before stale ones left)."""
+        out = np.zeros(6, np.int32)
+        lib().refcpu_mesh_peaks(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return out
 
     def terrain_query_box(self, convex, terrain):
         out = np.zeros(6, np.float32)

@@ -343,6 +343,10 @@ struct refcpu_world {
     std::deque<std::vector<MeshTester>> mesh_testers;
     std::vector<int> mesh_free;
     std::mutex mesh_mu;
+    // Largest per-pair list sizes any mesh manifold update met (tests): contact
+    // candidates, edge contacts, vertex contacts, blocked vertices, blocked
+    // edges, pair testers.
+    int mesh_peak[6] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -1611,9 +1615,11 @@ void update_mesh(refcpu_world* w, Pair& p) {
     }
 
     // Stale pair testers (TriangleMeshConvexContactManifold.cs:387-401).
+    int stale = 0;
     if (testers) {
         for (int i = (int)testers->size() - 1; i >= 0; --i) {
             if (!(*testers)[i].updated) {
+                ++stale;
                 (*testers)[i] = testers->back();
                 testers->pop_back();
             } else {
@@ -1622,6 +1628,13 @@ void update_mesh(refcpu_world* w, Pair& p) {
         }
     }
 
+    {
+        const int sizes[6] = {(int)cand.size(), (int)sets.edges.size(), (int)sets.vertices.size(),
+                              (int)sets.blocked_vertices.size(), (int)sets.blocked_edges.size(),
+                              testers ? (int)testers->size() + stale : 0};
+        std::lock_guard<std::mutex> g(w->mesh_mu);
+        for (int k = 0; k < 6; ++k) w->mesh_peak[k] = std::max(w->mesh_peak[k], sizes[k]);
+    }
     if (p.ccount + (int)cand.size() > 4) {
         std::vector<int> remove;
         std::vector<ContactData> keep;
@@ -3012,6 +3025,12 @@ int refcpu_add_terrain(refcpu_world* w, uint32_t shape, const float* transform, 
     w->terrains.push_back(ti);
     if (id) *id = (uint32_t)w->bodies.size();
     w->bodies.push_back(b);
+    return BEPU_OK;
+}
+
+// The largest per-pair list sizes the mesh manifold updates met (tests).
+int refcpu_mesh_peaks(refcpu_world* w, int32_t* out6) {
+    for (int k = 0; k < 6; ++k) out6[k] = w->mesh_peak[k];
     return BEPU_OK;
 }
 

@@ -149,3 +149,35 @@ def test_fast_bodies_against_a_mesh():
     cont = (sc["flags"][1:] & scenes.FLAG_CCD_CONTINUOUS) != 0
     assert (st[1:][cont, 1] > -1.5).mean() > 0.9  # the continuous ones stayed above the mesh
     _compare_impulses(g, o)
+
+
+def test_repeated_triangles_and_large_spheres():
+    """A mesh listing every triangle twice (the second copy meets the tester
+    the first updated, in the same step) and large spheres over a fine mesh
+    and a fine terrain (more overlapped triangles than one pass tests side by
+    side): bit-identical to the oracle."""
+    V, I, _ = scenes.heightfield_mesh(cells=12, size=12.0, amplitude=0.4, seed=5, center=(-8, 0))
+    I2 = np.concatenate([I, I])
+    Vf, If, _ = scenes.heightfield_mesh(cells=40, size=14.0, amplitude=0.3, seed=6, center=(8, 0))
+    t = scenes.heightfield_terrain(29, 29, 0.5, amplitude=0.3, seed=7, center=(0, 14))
+    meshes = [(V, I2, None), (Vf, If, None), t]
+    rows = [scenes.static_mesh(0), scenes.static_mesh(1), scenes.terrain(2)]
+    rng = np.random.default_rng(21)
+    for k in range(30):
+        shp = int(rng.choice(ALL_SHAPES))
+        rows.append(scenes._row(shp, scenes._shape_dims(rng, shp), 1.0,
+                                (rng.uniform(-12, -4), rng.uniform(1.5, 4), rng.uniform(-4, 4)),
+                                quat=tuple(scenes.random_quaternions(rng, 1)[0])))
+    for cx, cz in ((8, 0), (0, 14)):
+        for k in range(6):
+            rows.append(scenes._row(scenes.SHAPE_SPHERE, (rng.uniform(1.0, 1.5), 0, 0), 1.0,
+                                    (cx + rng.uniform(-4, 4), rng.uniform(3, 6), cz + rng.uniform(-4, 4))))
+    sc = scenes.make(rows)
+    g, o = _run_both(sc, meshes, 3, 30)
+    _compare_manifolds(g, o)
+    _compare_impulses(g, o)
+    # within the device's per-pair lists (32 candidates, 32 edge and 32
+    # vertex contacts, 64 blocked vertices and edges, 32 testers), with more
+    # than one pass of triangles for the spheres
+    peaks = o.mesh_peaks()
+    assert np.all(peaks <= [32, 32, 32, 64, 64, 32]) and peaks[1] > 11, peaks
