@@ -589,7 +589,7 @@ def stress_leg(n, reps, cpu):
     return out
 
 
-def mesh_leg(n, warmup, steps, cpu, parity):
+def mesh_leg(n, warmup, steps, cpu, parity, kinds=("terrain", "static_mesh")):
     """Static meshes and terrains: n bodies of all four shapes dropped onto a
     bumpy terrain, and onto a bumpy static mesh (128 x 128 quads, 32,768
     triangles), stepped `warmup` steps (the bodies land) and then `steps`
@@ -599,7 +599,7 @@ def mesh_leg(n, warmup, steps, cpu, parity):
     from bepuphysics1_amd import Space, scenes
     out = {}
     threads = baseline_threads()
-    for kind in ("terrain", "static_mesh"):
+    for kind in kinds:
         if kind == "terrain":
             sc, meshes = scenes.terrain_pile(n, seed=1)
             t = meshes[0]
@@ -797,6 +797,7 @@ def main():
     ap.add_argument("--stress-bodies", type=int, default=32768)
     ap.add_argument("--mesh-bodies", type=int, default=16384)
     ap.add_argument("--mesh-steps", type=int, default=120)
+    ap.add_argument("--mesh-kinds", default="terrain,static_mesh", help="grounds of the mesh leg (for profiling one)")
     ap.add_argument("--legs", default="batched,towers,served,stress,mesh",
                     help="comma list of workloads; without batched only the other legs' results are printed")
     ap.add_argument("--summarize-trace", metavar="DIR",
@@ -835,7 +836,7 @@ def main():
         if "stress" in legs:
             out["stress"] = stress_leg(a.stress_bodies, 3, cpu)
         if "mesh" in legs:
-            out["mesh"] = mesh_leg(a.mesh_bodies, 60, a.mesh_steps, cpu, not a.no_parity)
+            out["mesh"] = mesh_leg(a.mesh_bodies, 60, a.mesh_steps, cpu, not a.no_parity, a.mesh_kinds.split(","))
         print(json.dumps(out))
         return
 
@@ -887,7 +888,7 @@ def main():
         "towers": towers_leg(a, cpu, not a.no_parity) if "towers" in legs else None,
         "served": served_leg(a.served_rate, a.served_seconds, a.served_seconds + 20.0) if "served" in legs else None,
         "stress": stress_leg(a.stress_bodies, 3, cpu) if "stress" in legs else None,
-        "mesh": mesh_leg(a.mesh_bodies, 60, a.mesh_steps, cpu, not a.no_parity) if "mesh" in legs else None,
+        "mesh": mesh_leg(a.mesh_bodies, 60, a.mesh_steps, cpu, not a.no_parity, a.mesh_kinds.split(",")) if "mesh" in legs else None,
         "calibration_copy_bytes": copy_bytes,
     }
     print(json.dumps(line))
