@@ -130,6 +130,9 @@ EXPORTS = {
                                                  ctypes.c_uint32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
                                                  _u32p]),
     "bepu_add_terrain": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int32, ctypes.POINTER(Material), _u32p]),
+    "bepu_snapshot_size": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "bepu_save_snapshot": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    "bepu_restore_snapshot": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64]),
     "bepu_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "bepu_get_motion_states": (ctypes.c_int, [_vp, ctypes.POINTER(MotionState), ctypes.c_uint32, _u32p]),
     "bepu_get_pairs": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32, _u32p]),

@@ -234,7 +234,10 @@ bepu_status bepu_world_create_ex(const bepu_settings* settings, uint32_t capacit
     auto A = [&](auto** p, size_t cnt) {
         if (st != BEPU_OK) return;
         st = dalloc(p, cnt);
-        if (st == BEPU_OK) w->allocations.push_back((void*)*p);
+        if (st == BEPU_OK) {
+            w->allocations.push_back((void*)*p);
+            w->state_arrays.push_back({(size_t)((char*)p - (char*)&d), cnt * sizeof(**p)});
+        }
     };
     A(&d.pos, n); A(&d.quat, n); A(&d.linv, n); A(&d.angv, n);
     A(&d.ii0, n); A(&d.ii1, n); A(&d.ii2, n); A(&d.li0, n); A(&d.li1, n); A(&d.li2, n);
@@ -664,6 +667,8 @@ static bepu_status mesh_tables(bepu_world w) {
         hipMalloc(&d.mesh_work, P * sizeof(uint2)) != hipSuccess)
         return BEPU_E_OUT_OF_MEMORY;
     d.meshes = 1;
+    w->state_arrays.push_back({(size_t)((char*)&d.mtest_cur - (char*)&d), P * kMeshTesters * sizeof(MeshTester)});
+    w->state_arrays.push_back({(size_t)((char*)&d.mtest_old - (char*)&d), P * kMeshTesters * sizeof(MeshTester)});
     return BEPU_OK;
 }
 
@@ -1929,6 +1934,123 @@ bepu_status bepu_get_faulted_worlds(bepu_world w, uint32_t* ids, uint32_t cap, u
     if (!ids) return BEPU_OK;  // count query
     if (cap < bad.size()) return BEPU_E_CAPACITY;
     std::copy(bad.begin(), bad.end(), ids);
+    return BEPU_OK;
+}
+
+// Whole-world snapshots.  The state is every device array of the world
+// (bodies, pair keys and states, tester tables, key tables, islands,
+// deactivation labels and turns, joints, counters), the settings and the
+// step count; the structure (bodies, removals, sub-worlds, joints, shapes)
+// must be the one the snapshot was taken with, and a fingerprint of it is
+// checked on restore.
+extern "C++" {
+namespace {
+constexpr char kSnapMagic[8] = {'B', 'E', 'P', 'U', 'S', 'N', 'A', 'P'};
+constexpr uint32_t kSnapVersion = 1;
+struct SnapHeader {
+    char magic[8];
+    uint32_t version, narrays;
+    uint64_t fingerprint, steps, total;
+    uint32_t sticky, device_world_bytes;
+    bepu_settings settings;
+};
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+template <class T>
+uint64_t fnv_vec(uint64_t h, const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    h = fnv(h, &n, sizeof(n));
+    return v.empty() ? h : fnv(h, v.data(), v.size() * sizeof(T));
+}
+
+// The world's structure: bodies uploaded and staged, removals, sub-world
+// ranges, joints, shapes and the arrays that hold the state.
+uint64_t structure_fingerprint(bepu_world w) {
+    uint64_t h = 1469598103934665603ull;
+    const uint64_t scalars[10] = {w->uploaded, w->staged.size(), w->d.njoints, w->d.nworlds, w->hull_host.size(),
+                                  w->mesh_shapes, w->terrain_shapes, w->d.body_cap, w->d.pair_cap, w->d.sleeping};
+    h = fnv(h, scalars, sizeof(scalars));
+    h = fnv_vec(h, w->absent);
+    h = fnv_vec(h, w->world_first);
+    h = fnv_vec(h, w->world_count);
+    h = fnv_vec(h, w->world_live);
+    h = fnv_vec(h, w->state_arrays);
+    return h;
+}
+
+uint64_t snapshot_bytes(bepu_world w) {
+    uint64_t total = sizeof(SnapHeader);
+    for (const auto& a : w->state_arrays) total += a.second;
+    return total;
+}
+}  // namespace
+}  // extern "C++"
+
+bepu_status bepu_snapshot_size(bepu_world w, uint64_t* bytes) {
+    if (!w || !bytes) return BEPU_E_INVALID_ARG;
+    *bytes = snapshot_bytes(w);
+    return BEPU_OK;
+}
+
+bepu_status bepu_save_snapshot(bepu_world w, void* buffer, uint64_t capacity, uint64_t* bytes) {
+    if (!w) return BEPU_E_INVALID_ARG;
+    const uint64_t total = snapshot_bytes(w);
+    if (bytes) *bytes = total;
+    if (!buffer || capacity < total) return BEPU_E_INVALID_ARG;
+    if (!w->staged.empty()) {
+        const bepu_status st = upload(w);  // bodies added since the last step become device state first
+        if (st != BEPU_OK) return st;
+    }
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    SnapHeader h{};
+    std::memcpy(h.magic, kSnapMagic, sizeof(kSnapMagic));
+    h.version = kSnapVersion;
+    h.narrays = (uint32_t)w->state_arrays.size();
+    h.fingerprint = structure_fingerprint(w);
+    h.steps = w->steps;
+    h.total = total;
+    h.sticky = w->sticky;
+    h.device_world_bytes = (uint32_t)sizeof(DeviceWorld);
+    h.settings = w->d.s;
+    char* out = static_cast<char*>(buffer);
+    std::memcpy(out, &h, sizeof(h));
+    size_t off = sizeof(h);
+    for (const auto& a : w->state_arrays) {
+        const void* dev = *reinterpret_cast<void* const*>(reinterpret_cast<const char*>(&w->d) + a.first);
+        if (hipMemcpy(out + off, dev, a.second, hipMemcpyDeviceToHost) != hipSuccess) return BEPU_E_DEVICE;
+        off += a.second;
+    }
+    return BEPU_OK;
+}
+
+bepu_status bepu_restore_snapshot(bepu_world w, const void* buffer, uint64_t size) {
+    if (!w || !buffer || size < sizeof(SnapHeader)) return BEPU_E_INVALID_ARG;
+    SnapHeader h;
+    std::memcpy(&h, buffer, sizeof(h));
+    if (std::memcmp(h.magic, kSnapMagic, sizeof(kSnapMagic)) != 0 || h.version != kSnapVersion ||
+        h.device_world_bytes != sizeof(DeviceWorld) || h.total != size || h.narrays != w->state_arrays.size())
+        return BEPU_E_INVALID_ARG;
+    if (!w->staged.empty()) {
+        const bepu_status st = upload(w);
+        if (st != BEPU_OK) return st;
+    }
+    if (h.fingerprint != structure_fingerprint(w) || size != snapshot_bytes(w)) return BEPU_E_INVALID_ARG;
+    if (check(hipStreamSynchronize(w->stream)) != BEPU_OK) return BEPU_E_DEVICE;
+    const char* in = static_cast<const char*>(buffer);
+    size_t off = sizeof(h);
+    for (const auto& a : w->state_arrays) {
+        void* dev = *reinterpret_cast<void**>(reinterpret_cast<char*>(&w->d) + a.first);
+        if (hipMemcpy(dev, in + off, a.second, hipMemcpyHostToDevice) != hipSuccess) return BEPU_E_DEVICE;
+        off += a.second;
+    }
+    w->steps = h.steps;
+    w->sticky = h.sticky;
+    w->d.s = h.settings;
     return BEPU_OK;
 }
 

@@ -45,6 +45,10 @@ struct bepu_world_s {
     uint32_t timed_steps = 0;
     uint32_t steps = 0;
     std::vector<void*> allocations;
+    // Every device array of the world's state, as (offset of its pointer in
+    // DeviceWorld, bytes): what a snapshot copies (double-buffered arrays by
+    // role, whichever buffer holds it now).
+    std::vector<std::pair<size_t, size_t>> state_arrays;
     // Sub-worlds: world k holds bodies [world_first[k], world_first[k] + world_count[k]).
     std::vector<uint32_t> world_first{0}, world_count{0};
     std::vector<uint8_t> world_live{1};

@@ -651,6 +651,21 @@ class Space:
                                                   ctypes.byref(sid)), "create_terrain_shape")
         return sid.value
 
+    def save_snapshot(self):
+        """The whole world's state (bepu_save_snapshot) as a uint8 array."""
+        n = ctypes.c_uint64()
+        check(self._lib.bepu_snapshot_size(self._handle(), ctypes.byref(n)), "snapshot_size")
+        buf = np.empty(n.value, np.uint8)
+        check(self._lib.bepu_save_snapshot(self._handle(), buf.ctypes.data, n.value, ctypes.byref(n)),
+              "save_snapshot")
+        return buf
+
+    def restore_snapshot(self, snapshot):
+        """Loads a snapshot of this world (same structure) back (bepu_restore_snapshot)."""
+        buf = np.ascontiguousarray(snapshot, np.uint8)
+        check(self._lib.bepu_restore_snapshot(self._handle(), buf.ctypes.data, buf.size), "restore_snapshot")
+        self._ms_cache = None
+
     def register_meshes(self, meshes):
         """Registers mesh and terrain shapes in order ((vertices, indices,
         transform) or a scenes.TerrainDesc each); scene rows of static meshes

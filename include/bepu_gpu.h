@@ -274,6 +274,22 @@ bepu_status bepu_create_mesh_shape(bepu_world world, const float* vertices, uint
 bepu_status bepu_add_static_mesh(bepu_world world, uint32_t mesh_shape_id, int32_t sidedness,
                                  int32_t improve_boundary, const bepu_material* material, uint32_t* id_out);
 
+/* Whole-world snapshots (no reference counterpart: the reference exposes
+ * per-entity MotionState only, EntityStateManagement/MotionState.cs:10-55).
+ * A snapshot holds every body's state, the pair cache with its manifolds,
+ * accumulated impulses and pair tester state, the triangle tester tables,
+ * islands, sleeping labels and turns, joints, counters, the settings and the
+ * step count; stepping on from a restored snapshot gives the same bits as
+ * stepping on from the moment it was taken.  bepu_snapshot_size gives the
+ * bytes; bepu_save_snapshot writes them to caller memory (bodies staged since
+ * the last step are uploaded first); bepu_restore_snapshot loads one into
+ * the world it came from, or any world built the same way, provided the
+ * structure (bodies and removals, sub-worlds, joints, shapes, capacities) is
+ * unchanged: otherwise BEPU_E_INVALID_ARG and nothing is written. */
+bepu_status bepu_snapshot_size(bepu_world world, uint64_t* bytes_out);
+bepu_status bepu_save_snapshot(bepu_world world, void* buffer, uint64_t capacity, uint64_t* bytes_out);
+bepu_status bepu_restore_snapshot(bepu_world world, const void* buffer, uint64_t bytes);
+
 /* Terrains.  bepu_create_terrain_shape is new TerrainShape(heights,
  * organization) (CollisionShapes/TerrainShape.cs:60-72) together with the
  * Terrain's WorldTransform (BroadPhaseEntries/Terrain.cs:41-75): `heights`
