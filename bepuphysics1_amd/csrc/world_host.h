@@ -22,7 +22,7 @@ struct bepu_world_s {
     hipEvent_t fork = nullptr, join = nullptr;
     // Launch configuration (bepu_tuning); small_island_max lives in d.
     uint32_t np_block = 128;
-    uint32_t convex_lds = 2;        // 0 global hull reads, 1 LDS staging, 2 chosen per step on the device
+    uint32_t convex_lds = 0;        // 0 global hull reads, 1 LDS staging, 2 chosen per step on the device
     bool convex_side = true;
     bool step_graph = false;        // steps replay a captured HIP graph (bepu_tuning.step_graph)
     // One captured step per parity of the step count (the pair arrays swap

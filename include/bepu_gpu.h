@@ -544,8 +544,8 @@ bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
  * autotuner over block size and colouring granularity of the batched form.
  *   narrowphase_block  workgroup size of the closed-form narrowphase: 64, 128 (default) or 256
  *   convex_lds         1: each general convex pair's hull vertices are copied to LDS before
- *                      its test (only when no hull has more than 16 vertices); 0: support
- *                      scans read them from HBM; 2 (default): each step picks one of the two
+ *                      its test (only when no hull has more than 16 vertices); 0 (default):
+ *                      support scans read them from HBM; 2: each step picks one of the two
  *                      on the device (staging while Shallow pairs are the minority)
  *   convex_side_stream 1 (default): the closed-form narrowphase runs on a second stream beside
  *                      the general convex tests; 0: both in turn on the world's stream
@@ -555,7 +555,8 @@ bepu_status bepu_set_broadphase(bepu_world world, int32_t kind);
  *                      parity of the step count, captured again after the world or this
  *                      configuration changes) and replayed, which saves the host its launches;
  *                      0 (default): launched one by one.  Steps with stage timing and the tree
- *                      broadphase are always launched one by one
+ *                      broadphase are always launched one by one; the tuner tries replay only
+ *                      on steps shorter than 0.25 ms
  *   ms_per_step        (filled by bepu_autotune) the chosen configuration's time */
 typedef struct bepu_tuning {
     uint32_t narrowphase_block;

@@ -150,8 +150,9 @@ def test_launch_configurations_give_identical_results():
     sp.Update(10)
     chosen = sp.autotune(trial_steps=1)
     # 2 rounds x (1 untimed + 1 timed step) per candidate: convex_lds 3,
-    # convex_side_stream 2, narrowphase_block 3, small_island_max 2, step_graph 2
-    assert chosen["steps"] == 2 * 2 * (3 + 2 + 3 + 2 + 2) and chosen["ms_per_step"] > 0
+    # convex_side_stream 2, narrowphase_block 3, small_island_max 2, and
+    # step_graph 2 when the steps are short enough for replay to be tried
+    assert chosen["steps"] in (2 * 2 * (3 + 2 + 3 + 2), 2 * 2 * (3 + 2 + 3 + 2 + 2)) and chosen["ms_per_step"] > 0
     assert chosen["narrowphase_block"] in (64, 128, 256) and chosen["small_island_max"] in (32, 64)
     sp.Update(60 - 10 - chosen["steps"])
     assert np.array_equal(f32bits(sp.motion_states()), results[0][0])
